@@ -1,0 +1,172 @@
+/*
+ * wig.h — C ABI of libwig.so, the MI355X (gfx950) liquid step.
+ *
+ * The liquid step of worldinagrain (FLIP/APIC on a MAC grid with an MG-PCG
+ * pressure solve), behind the boundary the reference's host encodes it at:
+ * `Simulation.encodeFluid(pass, offset)` (js/sim.js:2466-2612).  The world's
+ * device buffers keep the reference's byte layouts (the liquid's WGSL
+ * bindings, shaders/fluid.wgsl:108-181,2543; js/sim.js:741-786), so a
+ * snapshot part uploaded here and downloaded again is the same bytes a
+ * WebGPU build holds.
+ *
+ * Plain C types only.  Every call returns a status: 0 ok, > 0 a warning
+ * (WIG_WARN_*), < 0 an error (WIG_E_*); wig_last_error() has the message
+ * (kept per thread).  One world handle per host thread; calls on a handle
+ * are ordered on its HIP stream and return without waiting unless stated.
+ */
+#ifndef WIG_H
+#define WIG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WIG_OK 0
+#define WIG_WARN_OVERFLOW 1       /* more particles than capacity: those past it lost */
+#define WIG_E_INVALID (-1)        /* bad argument, size or unsupported control */
+#define WIG_E_NOMEM (-2)          /* device allocation failed */
+#define WIG_E_DEVICE (-3)         /* a HIP call failed (device lost, launch error) */
+#define WIG_E_NOT_BUILT (-4)      /* no gfx950 code object in this library */
+
+/* Per-step uniform, 80 bytes: `Params` (shaders/common.wgsl:5-42), as the
+ * host's writeSlot fills it (js/sim.js:1972-2001).  The liquid step reads
+ * size, frame, seed, dx and earthG. */
+typedef struct WigParams {
+    uint32_t size[2];
+    uint32_t frame;
+    uint32_t seed;
+    uint32_t offset[2];
+    float supportMax;
+    float dx;
+    float day;
+    float dayRate;
+    float holePull;
+    float massPull;
+    float earthG;
+    float dt;
+    float dayDt;
+    float clockSpan;
+    uint32_t heatStages;
+    float heatLimit;
+    uint32_t holding;
+    float pad;
+} WigParams;
+
+/* Stop criteria of the pressure solve. */
+#define WIG_TOL_ABS_MAX 0   /* max |r| < tol, the reference's LIQUID_TOLERANCE (fluid.wgsl:2221-2226) */
+#define WIG_TOL_REL_L2 1    /* ||r||_2 <= tol * ||b||_2 */
+
+/* What the host decides per step (js/sim.js:2466-2612), plus the solver
+ * settings the reference fixes as constants (js/sim.js:61-63,
+ * fluid.wgsl:2063,2392). */
+typedef struct WigStepCtl {
+    uint32_t full;           /* fluidStats.active || bodiesRun(); 0 takes the idle path (:2478-2485) */
+    uint32_t viscous;        /* viscosity sweeps: must be 0 (water has none; see DESIGN) */
+    uint32_t surface;        /* surface tension pass: must be 0 (off at cm cells) */
+    uint32_t bodies_run;     /* free bodies: must be 0 */
+    uint32_t fluid_cycles;   /* PCG iterations encoded (sim.js fluidCycles, <= WIG_MAX_CYCLES) */
+    uint32_t shifted_first;  /* tiling of the step's first relaxation (sim.js shiftedFirst after its toggle) */
+    uint32_t tol_mode;       /* WIG_TOL_* */
+    float tol;               /* 1e-2 with WIG_TOL_ABS_MAX reproduces the reference */
+    uint32_t pre_relax;      /* relaxation dispatches before the coarse levels (PRE_RELAX = 1) */
+    uint32_t post_relax;     /* ... and after (POST_RELAX = 1) */
+    uint32_t sweep_pairs;    /* red-black sweep pairs per relaxation dispatch (SWEEPS = 4) */
+    uint32_t coarse_sweeps;  /* SOR sweep pairs on the coarsest level (COARSE_SWEEPS = 40) */
+    uint32_t reserved[4];
+} WigStepCtl;
+
+#define WIG_MAX_CYCLES 256
+
+/* The world's buffers, by the reference's part names (js/passes.js:78-84). */
+#define WIG_PART_CELLS 0       /* cells[cur]: 16 B a cell, then the gas plane (8 B a cell) */
+#define WIG_PART_PARTICLES0 1  /* `sorted`: 16 B header, then 3 vec4 streams of cap particles */
+#define WIG_PART_PARTICLES1 2  /* `parts`, the step's */
+#define WIG_PART_COUNTERS 3    /* 11 u32 a cell, then 2 a face */
+#define WIG_PART_FIELD 4       /* vec4<f32>: 2 a cell, 1 a face, viscosity words, MG levels */
+#define WIG_PART_BLOCKS 5      /* per block marks, state, lists, FLUID_STATS, totals */
+#define WIG_PART_FLUIDCG 6     /* vec4<f32> per cell (x, r, p, Ap), per block, sums */
+#define WIG_PART_SOLVE 7       /* this library's FP64 solve state (no counterpart in the reference) */
+#define WIG_PART_COUNT 8
+
+/* FLUID_STATS words (js/shared.js:271-272). */
+#define WIG_STAT_ACTIVE 0
+#define WIG_STAT_RETIRED 1
+#define WIG_STAT_VISCOUS 2
+#define WIG_STAT_PARTICLES 3
+#define WIG_STAT_RESIDUAL 4
+#define WIG_STAT_CYCLES 7
+#define WIG_STAT_MOST_CYCLES 8
+#define WIG_STATS 9
+
+/* What the last step's solve did, from WIG_PART_SOLVE. */
+typedef struct WigSolveStats {
+    double rz, alpha, beta, pq;   /* the last iteration's r.z, alpha, beta, p.Ap */
+    double rr;                    /* ||r||^2 after the last iteration run */
+    double bb;                    /* ||b||^2 of the step's right-hand side */
+    uint32_t cycles;              /* iterations run */
+    uint32_t converged;
+    uint32_t pad[2];
+} WigSolveStats;
+
+/* A world of width x height cells (the reference's Simulation buffers,
+ * js/sim.js:741-786).  coarse_levels 0 takes the reference's rule (halve
+ * until <= 512 cells, at most 8: js/sim.js:763-770); otherwise that many
+ * coarse levels.  Replaces `new Simulation(device, pipes, w, h, ...)`'s
+ * liquid allocation (js/sim.js:664,741-786).  NULL on error. */
+typedef struct WigWorld WigWorld;
+WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, uint32_t flags);
+
+/* Frees the world's device buffers (js/sim.js:1502-1527, Simulation.destroy). */
+int wig_destroy(WigWorld *w);
+
+/* The element kinds the liquid step reads (elems[t].kind, js/elements.js:8:
+ * 1 solid, 2 powder, 3 liquid, 4 gas), for element ids 0..255
+ * (the `elems` binding, fluid.wgsl:117). */
+int wig_set_kinds(WigWorld *w, const uint8_t kinds[256]);
+
+/* The world's sizes: cells, faces, blocks, particle capacity, coarse levels. */
+typedef struct WigLayoutInfo {
+    uint32_t width, height, cells, faces, blocks, blocks_wide, capacity, levels;
+    uint32_t level_width[9], level_height[9];
+    uint64_t part_bytes[WIG_PART_COUNT];
+} WigLayoutInfo;
+int wig_layout(const WigWorld *w, WigLayoutInfo *out);
+
+/* Copies a part to or from the device (js/sim.js:1429-1493's snapshot
+ * parts; queue.writeBuffer / mapAsync).  Synchronous. */
+int wig_upload(WigWorld *w, int part, const void *src, size_t bytes);
+int wig_download(WigWorld *w, int part, void *dst, size_t bytes);
+
+/* The device address of a part, for callers that share memory with the
+ * world (bytes of it as wig_layout gives them). */
+int wig_device_ptr(WigWorld *w, int part, void **ptr);
+
+/* Runs later calls on this HIP stream (a hipStream_t; NULL: the world's own). */
+int wig_set_stream(WigWorld *w, void *stream);
+
+/* One liquid step, enqueued (Simulation.encodeFluid, js/sim.js:2466-2612). */
+int wig_fluid_step(WigWorld *w, const WigParams *params, const WigStepCtl *ctl);
+
+/* FLUID_STATS as the frame's report copies them (js/sim.js:1760-1780), then
+ * MOST_CYCLES cleared as a report on steps does.  Waits for the stream. */
+int wig_report(WigWorld *w, uint32_t stats[WIG_STATS]);
+
+/* The last solve's FP64 scalars.  Waits for the stream. */
+int wig_solve_stats(WigWorld *w, WigSolveStats *out);
+
+/* Waits for everything enqueued on the world's stream. */
+int wig_sync(WigWorld *w);
+
+/* The last error on this thread, "" if none. */
+const char *wig_last_error(void);
+
+/* 1 if the library holds gfx950 code and a device is visible. */
+int wig_device_ok(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,141 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke(), bench.py's
+cpu_baseline leg).  ctypes binding of libwig_oracle.so, the CPU restatement of
+the reference's liquid step (oracle/wig_oracle.c)."""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from worldinagrain_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PATH = os.path.join(_HERE, "libwig_oracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_PATH):
+            raise RuntimeError(f"oracle not built: {_PATH} (run `make -C oracle`)")
+        L = ct.CDLL(_PATH)
+        L.wor_create.restype = ct.c_void_p
+        L.wor_create.argtypes = [ct.c_uint32, ct.c_uint32, ct.c_uint32]
+        L.wor_destroy.argtypes = [ct.c_void_p]
+        L.wor_part.restype = ct.c_void_p
+        L.wor_part.argtypes = [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_uint64)]
+        L.wor_set_kinds.argtypes = [ct.c_void_p, ct.c_char_p]
+        L.wor_set_dot_mode.argtypes = [ct.c_void_p, ct.c_int]
+        L.wor_layout.argtypes = [ct.c_void_p, ct.POINTER(abi.WigLayoutInfo)]
+        L.wor_fluid_step.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
+        L.wor_capture.argtypes = [ct.c_void_p, ct.c_int]
+        L.wor_captured_b.restype = ct.POINTER(ct.c_float)
+        L.wor_captured_b.argtypes = [ct.c_void_p]
+        L.wor_cg64.argtypes = [ct.c_void_p, ct.POINTER(ct.c_float), ct.POINTER(ct.c_double), ct.c_double, ct.c_int,
+                               ct.POINTER(ct.c_double)]
+        L.wor_relres.restype = ct.c_double
+        L.wor_relres.argtypes = [ct.c_void_p, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]
+        L.wor_pcg.restype = ct.c_uint32
+        L.wor_pcg.argtypes = [ct.c_uint32]
+        L.wor_seed_id.restype = ct.c_uint32
+        L.wor_seed_id.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_uint32, ct.c_uint32]
+        L.wor_seed_pos.argtypes = [ct.c_int32, ct.c_int32, ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_uint32,
+                                   ct.POINTER(ct.c_float)]
+        L.wor_f16.restype = ct.c_uint16
+        L.wor_f16.argtypes = [ct.c_float]
+        L.wor_f32.restype = ct.c_float
+        L.wor_f32.argtypes = [ct.c_uint16]
+        L.wor_exact_parts.argtypes = [ct.c_float, ct.c_float, ct.POINTER(ct.c_uint32)]
+        L.wor_exact_value.restype = ct.c_float
+        L.wor_exact_value.argtypes = [ct.c_uint32, ct.c_uint32, ct.c_float]
+        L.wor_fine_beta.restype = ct.c_float
+        L.wor_fine_beta.argtypes = [ct.c_float] * 4
+        _lib = L
+    return _lib
+
+
+_DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8}
+
+
+class OracleWorld:
+    """A world in host memory, stepped by the CPU restatement."""
+
+    def __init__(self, width, height, coarse_levels=0, kinds=None, dot_mode=1):
+        L = lib()
+        self._w = L.wor_create(width, height, coarse_levels)
+        if not self._w:
+            raise ValueError("invalid world size")
+        self.layout = abi.WigLayoutInfo()
+        L.wor_layout(self._w, ct.byref(self.layout))
+        if kinds is not None:
+            self.set_kinds(kinds)
+        L.wor_set_dot_mode(self._w, dot_mode)
+
+    def close(self):
+        if self._w:
+            lib().wor_destroy(self._w)
+            self._w = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_kinds(self, kinds):
+        k = np.asarray(kinds, dtype=np.uint8)
+        assert k.shape == (256,)
+        lib().wor_set_kinds(self._w, k.tobytes())
+
+    def part(self, p):
+        """A numpy view (u32 words, f32 for field/fluidCg) of the part's memory."""
+        n = ct.c_uint64()
+        ptr = lib().wor_part(self._w, p, ct.byref(n))
+        dt = _DTYPES.get(p, np.uint32)
+        count = n.value // np.dtype(dt).itemsize
+        buf = (ct.c_char * n.value).from_address(ptr)
+        return np.frombuffer(buf, dtype=dt, count=count)
+
+    def upload(self, p, data):
+        dst = self.part(p)
+        src = np.frombuffer(np.ascontiguousarray(data).tobytes(), dtype=dst.dtype)
+        assert src.size == dst.size, (p, src.size, dst.size)
+        dst[:] = src
+
+    def download(self, p):
+        return self.part(p).copy()
+
+    def step(self, params, ctl):
+        return lib().wor_fluid_step(self._w, ct.byref(params), ct.byref(ctl))
+
+    def capture(self, on=True):
+        lib().wor_capture(self._w, 1 if on else 0)
+
+    def captured_b(self):
+        ptr = lib().wor_captured_b(self._w)
+        return np.ctypeslib.as_array(ptr, shape=(self.layout.cells,)).copy()
+
+    def cg64(self, b, tol=1e-12, maxit=100000):
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        x = np.zeros(self.layout.cells, dtype=np.float64)
+        rel = ct.c_double()
+        it = lib().wor_cg64(self._w, b.ctypes.data_as(ct.POINTER(ct.c_float)),
+                            x.ctypes.data_as(ct.POINTER(ct.c_double)), tol, maxit, ct.byref(rel))
+        return x, it, rel.value
+
+    def relres(self, b, x):
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        return lib().wor_relres(self._w, b.ctypes.data_as(ct.POINTER(ct.c_float)), x.ctypes.data_as(ct.POINTER(ct.c_float)))
+
+    def solve_stats(self):
+        raw = self.part(abi.PART_SOLVE)[:64].tobytes()
+        return abi.WigSolveStats.from_buffer_copy(raw)
+
+    def report(self):
+        """FLUID_STATS, then MOST_CYCLES cleared (js/sim.js:1760-1780)."""
+        B = self.layout.blocks
+        blocks = self.part(abi.PART_BLOCKS)
+        stats = blocks[4 * B:4 * B + abi.STATS].copy()
+        blocks[4 * B + abi.STAT_MOST_CYCLES] = 0
+        return stats

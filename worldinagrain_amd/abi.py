@@ -1,0 +1,61 @@
+"""ctypes mirror of include/wig.h: the structs and constants of the C ABI."""
+import ctypes as ct
+
+WIG_OK = 0
+WIG_WARN_OVERFLOW = 1
+WIG_E_INVALID = -1
+WIG_E_NOMEM = -2
+WIG_E_DEVICE = -3
+WIG_E_NOT_BUILT = -4
+
+WIG_TOL_ABS_MAX = 0
+WIG_TOL_REL_L2 = 1
+WIG_MAX_CYCLES = 256
+
+PART_CELLS, PART_PARTICLES0, PART_PARTICLES1, PART_COUNTERS, PART_FIELD, PART_BLOCKS, PART_FLUIDCG, PART_SOLVE = range(8)
+PART_COUNT = 8
+PART_NAMES = ("cells", "particles0", "particles1", "counters", "field", "blocks", "fluidCg", "solve")
+
+STAT_ACTIVE, STAT_RETIRED, STAT_VISCOUS, STAT_PARTICLES, STAT_RESIDUAL = 0, 1, 2, 3, 4
+STAT_CYCLES, STAT_MOST_CYCLES = 7, 8
+STATS = 9
+
+
+class WigParams(ct.Structure):
+    """`Params`, 80 bytes (shaders/common.wgsl:5-42; js/sim.js:1972-2001)."""
+    _fields_ = [
+        ("size", ct.c_uint32 * 2), ("frame", ct.c_uint32), ("seed", ct.c_uint32),
+        ("offset", ct.c_uint32 * 2), ("supportMax", ct.c_float), ("dx", ct.c_float),
+        ("day", ct.c_float), ("dayRate", ct.c_float), ("holePull", ct.c_float), ("massPull", ct.c_float),
+        ("earthG", ct.c_float), ("dt", ct.c_float), ("dayDt", ct.c_float), ("clockSpan", ct.c_float),
+        ("heatStages", ct.c_uint32), ("heatLimit", ct.c_float), ("holding", ct.c_uint32), ("pad", ct.c_float),
+    ]
+
+
+class WigStepCtl(ct.Structure):
+    _fields_ = [
+        ("full", ct.c_uint32), ("viscous", ct.c_uint32), ("surface", ct.c_uint32), ("bodies_run", ct.c_uint32),
+        ("fluid_cycles", ct.c_uint32), ("shifted_first", ct.c_uint32), ("tol_mode", ct.c_uint32), ("tol", ct.c_float),
+        ("pre_relax", ct.c_uint32), ("post_relax", ct.c_uint32), ("sweep_pairs", ct.c_uint32),
+        ("coarse_sweeps", ct.c_uint32), ("reserved", ct.c_uint32 * 4),
+    ]
+
+
+class WigLayoutInfo(ct.Structure):
+    _fields_ = [
+        ("width", ct.c_uint32), ("height", ct.c_uint32), ("cells", ct.c_uint32), ("faces", ct.c_uint32),
+        ("blocks", ct.c_uint32), ("blocks_wide", ct.c_uint32), ("capacity", ct.c_uint32), ("levels", ct.c_uint32),
+        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 8),
+    ]
+
+
+class WigSolveStats(ct.Structure):
+    _fields_ = [
+        ("rz", ct.c_double), ("alpha", ct.c_double), ("beta", ct.c_double), ("pq", ct.c_double),
+        ("rr", ct.c_double), ("bb", ct.c_double), ("cycles", ct.c_uint32), ("converged", ct.c_uint32),
+        ("pad", ct.c_uint32 * 2),
+    ]
+
+
+assert ct.sizeof(WigParams) == 80
+assert ct.sizeof(WigSolveStats) == 64

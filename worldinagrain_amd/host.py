@@ -1,0 +1,124 @@
+"""The host side of the liquid step: what `Simulation` in js/sim.js keeps and
+decides around `encodeFluid` (js/sim.js:1386-1397,1893-1901,1972-2001,
+2466-2612), over any world that takes (params, ctl) steps: the HIP world of
+`worldinagrain_amd.gpu` in the product.
+
+Per step it writes the Params slot (size, frame, a mulberry32 seed, dx,
+earthG, dt), flips the relaxation's starting tiling, and takes the iteration
+budget and the active/idle decision from the last report, as the reference
+host does from its readback (here the report is read right after each step:
+the reference's lands one or two frames late)."""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from .elements import GRAVITY
+
+FLUID_CYCLES = 16          # js/sim.js:63
+FLUID_CYCLES_SPARE = 2     # js/sim.js:70
+LIQUID_TOLERANCE = 1e-2    # fluid.wgsl:2221
+
+
+@dataclass
+class SolverSettings:
+    """The pressure solve's settings. `reference()` is the reference's own
+    (js/sim.js:61-70; fluid.wgsl:2063,2221,2392)."""
+    tol_mode: int = abi.WIG_TOL_ABS_MAX
+    tol: float = LIQUID_TOLERANCE
+    max_cycles: int = FLUID_CYCLES
+    adaptive_cycles: bool = True
+    pre_relax: int = 1
+    post_relax: int = 1
+    sweep_pairs: int = 4
+    coarse_sweeps: int = 40
+
+    @staticmethod
+    def reference():
+        return SolverSettings()
+
+    @staticmethod
+    def relative(tol, max_cycles=100, sweep_pairs=2):
+        """A solve to ||r|| <= tol ||b|| (the benchmark configurations)."""
+        return SolverSettings(tol_mode=abi.WIG_TOL_REL_L2, tol=tol, max_cycles=max_cycles, adaptive_cycles=False,
+                              sweep_pairs=sweep_pairs)
+
+
+def mulberry32_next(state):
+    """(new state, u32): js/sim.js:1390-1397 (Tommy Ettinger's mulberry32)."""
+    state = (state + 0x6D2B79F5) & 0xFFFFFFFF
+    t = state
+    t = ((t ^ (t >> 15)) * (t | 1)) & 0xFFFFFFFF
+    t ^= (t + (((t ^ (t >> 7)) * (t | 61)) & 0xFFFFFFFF)) & 0xFFFFFFFF
+    return state, (t ^ (t >> 14)) & 0xFFFFFFFF
+
+
+class Simulation:
+    """The liquid's part of js/sim.js's Simulation over a world backend."""
+
+    def __init__(self, world, width, height, *, seed=0x5EED, dx=0.01, earth_g=None, dt=1.0 / 60.0,
+                 solver=None, frame=1):
+        self.world = world
+        self.width, self.height = width, height
+        self.rng_state = seed & 0xFFFFFFFF
+        self.dx = dx
+        self.earth_g = float(GRAVITY) if earth_g is None else earth_g
+        self.dt = dt
+        self.solver = solver or SolverSettings.reference()
+        self.frame_no = frame
+        self.shifted_first = False
+        self.fluid_stats = {"active": 1, "viscous": 0}
+        self.fluid_cycles = self.solver.max_cycles
+        self.last_stats = None
+
+    # Simulation.seed / random (js/sim.js:1386-1397)
+    def seed(self, n):
+        self.rng_state = n & 0xFFFFFFFF
+
+    def random(self):
+        self.rng_state, r = mulberry32_next(self.rng_state)
+        return r
+
+    def params(self):
+        """writeSlot (js/sim.js:1972-2001), the words the liquid reads."""
+        p = abi.WigParams()
+        p.size[0], p.size[1] = self.width, self.height
+        p.frame = self.frame_no & 0xFFFFFFFF
+        p.seed = self.random()
+        p.dx = self.dx
+        p.earthG = self.earth_g
+        p.dt = self.dt
+        return p
+
+    def ctl(self):
+        s = self.solver
+        self.shifted_first = not self.shifted_first       # js/sim.js:2530
+        c = abi.WigStepCtl()
+        c.full = 1 if self.fluid_stats["active"] else 0
+        c.fluid_cycles = self.fluid_cycles if s.adaptive_cycles else s.max_cycles
+        c.shifted_first = 1 if self.shifted_first else 0
+        c.tol_mode = s.tol_mode
+        c.tol = s.tol
+        c.pre_relax, c.post_relax = s.pre_relax, s.post_relax
+        c.sweep_pairs, c.coarse_sweeps = s.sweep_pairs, s.coarse_sweeps
+        return c
+
+    def step(self, read_report=True):
+        """One step's liquid (encodeFluid), then its report taken in."""
+        status = self.world.step(self.params(), self.ctl())
+        if status < 0:
+            raise RuntimeError(f"fluid step failed ({status})")
+        self.frame_no += 1
+        if read_report:
+            self.take_report(self.world.report())
+        return status
+
+    def take_report(self, stats):
+        """readReport's liquid part (js/sim.js:1893-1901)."""
+        stats = np.asarray(stats, dtype=np.uint32)
+        self.last_stats = stats
+        self.fluid_stats = {"active": int(stats[abi.STAT_ACTIVE]) + int(stats[abi.STAT_PARTICLES]),
+                            "viscous": int(stats[abi.STAT_VISCOUS])}
+        most = int(stats[abi.STAT_MOST_CYCLES])
+        cap = self.solver.max_cycles
+        self.fluid_cycles = cap if most >= self.fluid_cycles else min(most + FLUID_CYCLES_SPARE, cap)

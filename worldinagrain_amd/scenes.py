@@ -1,0 +1,118 @@
+"""Initial worlds as the reference's cells buffer (js/shared.js:60-79):
+16 bytes a cell (x word, temperature bits, velocity as 2 x f16, w), then the
+gas plane, 8 bytes a cell (vapour and mist, 2 x f16; oxygen used up and amount
+of gas, 2 x f16).  Liquid is painted undrawn, so the step's syncSeed seeds it
+with LIQUID_PER_CELL particles a cell (fluid.wgsl:863-890), as after a brush
+stroke.  Worlds are y-down; the scene sizes are given y-up and flipped."""
+import numpy as np
+
+from . import elements as E
+
+
+def _f32bits(x):
+    return np.frombuffer(np.float32(x).tobytes(), dtype=np.uint32)[0]
+
+
+def empty_cells(width, height, temp=20.0):
+    """A world of air at `temp`, each cell holding a cell of air at 1 atm."""
+    C = width * height
+    words = np.zeros(4 * C + 4 * ((C + 1) // 2), dtype=np.uint32)
+    cells = words[:4 * C].reshape(C, 4)
+    cells[:, 0] = E.AIR
+    cells[:, 1] = _f32bits(temp)
+    gas = words[4 * C:].reshape(-1, 2)[:C]           # a (vapour|mist, lost|amount) pair a cell
+    gas[:, 1] = np.uint32(np.float16(1.0).view(np.uint16)) << 16
+    return words
+
+
+def paint(words, width, height, mask, elem, temp=20.0, vel=None):
+    """Cells where mask (H x W, y-down) is true become `elem` (undrawn)."""
+    C = width * height
+    cells = words[:4 * C].reshape(height, width, 4)
+    gas = words[4 * C:].reshape(-1, 2)[:C].reshape(height, width, 2)
+    cells[mask, 0] = elem
+    cells[mask, 1] = _f32bits(temp)
+    cells[mask, 3] = 0
+    if vel is None:
+        cells[mask, 2] = 0
+    else:
+        vx = np.asarray(vel[0], dtype=np.float16).view(np.uint16).astype(np.uint32)
+        vy = np.asarray(vel[1], dtype=np.float16).view(np.uint16).astype(np.uint32)
+        packed = vx | (vy << 16)
+        cells[mask, 2] = packed[mask] if packed.shape == mask.shape else packed
+    gas[mask] = 0
+    return words
+
+
+def border(width, height, thickness=1):
+    m = np.zeros((height, width), dtype=bool)
+    m[:thickness, :] = m[-thickness:, :] = True
+    m[:, :thickness] = m[:, -thickness:] = True
+    return m
+
+
+def dam_break(width=256, height=256, block=(128, 128), wall=1):
+    """A solid border `wall` cells thick and water filling the cells
+    [wall, block) x [wall, block), y-up from the bottom left."""
+    w = empty_cells(width, height)
+    paint(w, width, height, border(width, height, wall), E.WALL)
+    m = np.zeros((height, width), dtype=bool)
+    for yu in range(wall, block[1]):
+        m[height - 1 - yu, wall:block[0]] = True
+    paint(w, width, height, m, E.WATER)
+    return w
+
+
+def still_tank(width=32, height=32, depth=16, wall=1):
+    """A closed box of water `depth` rows deep at rest."""
+    w = empty_cells(width, height)
+    paint(w, width, height, border(width, height, wall), E.WALL)
+    m = np.zeros((height, width), dtype=bool)
+    m[height - wall - depth:height - wall, wall:width - wall] = True
+    paint(w, width, height, m, E.WATER)
+    return w
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = np.uint64(seed)
+
+    def next(self):
+        with np.errstate(over="ignore"):
+            self.s = np.uint64(self.s + np.uint64(0x9E3779B97F4A7C15))
+            z = self.s
+            z = np.uint64((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9))
+            z = np.uint64((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB))
+            return int(z ^ (z >> np.uint64(31)))
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def discs_dam(width=1024, height=1024, discs=200, rmin=4, rmax=24, solid_share=0.10, fluid_rows=0.40,
+              vel_sigma=0.5, dt=1.0 / 120.0, seed=2):
+    """Random solid discs, radius U[rmin, rmax], centres uniform (SplitMix64
+    `seed`), up to `discs` of them or until `solid_share` of the cells are
+    solid; water in the non-solid cells of the lowest `fluid_rows` of the
+    rows, each cell's velocity N(0, vel_sigma^2) cells per unit time per
+    component, times dt per step (all particles seeded in a cell share it)."""
+    rng = SplitMix64(seed)
+    solid = np.zeros((height, width), dtype=bool)
+    yy, xx = np.mgrid[0:height, 0:width]
+    for _ in range(discs):
+        if solid.mean() >= solid_share:
+            break
+        cx = rng.uniform() * width
+        cy = rng.uniform() * height
+        r = rmin + rng.uniform() * (rmax - rmin)
+        solid |= (xx + 0.5 - cx) ** 2 + (yy + 0.5 - cy) ** 2 <= r * r
+    w = empty_cells(width, height)
+    paint(w, width, height, solid, E.WALL)
+    fluid = np.zeros((height, width), dtype=bool)
+    rows = int(round(fluid_rows * height))
+    fluid[height - rows:, :] = True
+    fluid &= ~solid
+    g = np.random.default_rng(seed)
+    vel = (g.normal(0.0, vel_sigma, size=(height, width)) * dt, g.normal(0.0, vel_sigma, size=(height, width)) * dt)
+    paint(w, width, height, fluid, E.WATER, vel=vel)
+    return w, int(fluid.sum())
