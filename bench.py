@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""The liquid step's benchmark on one MI355X.
+
+A step is one liquid step (encodeFluid) of the plain FP32 world at scale:
+a 1024 x 1024 MAC grid with ~10% of its cells solid as 200 random discs
+(radius U[4, 24], seeded), water in the non-solid cells of the lowest 40% of
+the rows at 4 particles a cell (~1.47 M particles), each cell's velocity
+N(0, 0.5^2) cells/s, dt 1/120, gravity 9.81 m/s^2 at 1 m a cell, MG-PCG with
+7 levels to a relative residual of 1e-6.  The world's buffers are resident
+in HBM before the timed region starts.
+
+Prints one JSON line: particle-steps/s over the timed steps (and the solves/s
+and p50/p99 step latency beside it), the HBM roofline of the kernel that
+takes the most time (its algorithmic bytes per launch over its mean launch
+time, from HIP events on the world's stream over the timed region), and the
+oracle (the CPU restatement of the reference's liquid step) timed on a
+bounded sample of the same workload on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s & MG-PCG solves/s at 1e-6 resid; p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit of the kernels the roofline may name (the
+# per-step byte model's terms: P2G reads a particle's 48 bytes; G2P reads 48
+# and writes 32; the sort counts (reads 8) and scatters (reads and writes 48
+# each): 224 a particle in all).  Units: particles in the step.
+KERNEL_BYTES_PER_PARTICLE = {"gather": 48, "advect": 80, "sortCount": 8, "scatter": 96 - 8,
+                             "syncParticles": 96}
+
+
+def build_world(size, seed=2):
+    from worldinagrain_amd import scenes
+    from worldinagrain_amd import elements as E
+    cells, fluid_cells = scenes.discs_dam(size, size, seed=seed)
+    return cells, fluid_cells, E.kinds()
+
+
+def config_simulation(world, size, tol):
+    import numpy as np
+    from worldinagrain_amd.host import Simulation, SolverSettings
+    g = float(np.float32(9.81 * (1.0 / 120.0) ** 2 / 1.0))
+    return Simulation(world, size, size, dx=1.0, earth_g=g, dt=1.0 / 120.0, solver=SolverSettings.relative(tol))
+
+
+def step_bytes(totals_delta):
+    """The per-step algorithmic byte model summed over the timed steps:
+    N*224 + C_a*65 + C_a*16 + K*C_a*93 (FP32 V-cycle storage)."""
+    st, cyc, parts, ca = totals_delta
+    if st == 0:
+        return 0.0
+    ca_mean = ca / st
+    return parts * 224.0 + ca * (65.0 + 16.0) + cyc * ca_mean * 93.0
+
+
+def cpu_baseline(cells, kinds, size, tol, steps):
+    """The oracle on the host: the same world, its first (seeding) step
+    untimed, then `steps` steps timed."""
+    from oracle.oracle import OracleWorld
+    from worldinagrain_amd import abi
+    w = OracleWorld(size, size, kinds=kinds, dot_mode=1)
+    w.upload(abi.PART_CELLS, cells)
+    sim = config_simulation(w, size, tol)
+    sim.step()
+    n = int(sim.last_stats[abi.STAT_PARTICLES])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the same {size}x{size} world ({n} particles) after its seeding step, "
+                      f"single-threaded C restatement of fluid.wgsl (oracle/wig_oracle.c, -O2), "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--kernel", default=None, help="kernel for the roofline (default: the one taking most time)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(independent worlds would shard across GPUs with no exchange)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from worldinagrain_amd import abi
+    from worldinagrain_amd.gpu import GpuWorld, kernel_names
+
+    size = args.size
+    cells, fluid_cells, kinds = build_world(size)
+    world = GpuWorld(size, size, kinds=kinds)
+    world.upload(abi.PART_CELLS, cells)
+    stream = torch.cuda.current_stream()
+    world.set_stream(stream.cuda_stream)
+    sim = config_simulation(world, size, args.tol)
+    for _ in range(max(args.warmup, 1)):
+        sim.step()
+    torch.cuda.synchronize()
+
+    # which kernel takes the most time: each timed over one step, untimed
+    kname = args.kernel
+    per_kernel = {}
+    if kname is None:
+        for name in kernel_names():
+            world.time_kernel(name)
+            sim.step(read_report=False)
+            ms, n = world.kernel_time()
+            per_kernel[name] = (ms, n)
+        world.time_kernel(None)
+        kname = max(per_kernel, key=lambda k: per_kernel[k][0])
+    torch.cuda.synchronize()
+
+    K = args.steps
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    t0 = world.run_totals()
+    world.time_kernel(kname)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for k in range(K):
+        starts[k].record(stream)
+        sim.step(read_report=False)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    kms, klaunches = world.kernel_time()
+    world.time_kernel(None)
+    t1 = world.run_totals()
+    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+    total_ms = starts[0].elapsed_time(ends[-1])
+    d = (t1.steps - t0.steps, t1.cycles - t0.cycles, t1.particles - t0.particles, t1.active_cells - t0.active_cells)
+    assert d[0] == K, d
+    particles_per_step = d[2] / K
+    value = d[2] / (total_ms / 1e3)
+    solves = K / (total_ms / 1e3)
+    bytes_total = step_bytes(d)
+
+    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": kname,
+            "launches": klaunches, "mean_launch_ms": kms / max(klaunches, 1)}
+    if kname in KERNEL_BYTES_PER_PARTICLE and klaunches:
+        per_launch = KERNEL_BYTES_PER_PARTICLE[kname] * particles_per_step * (K / klaunches)
+        roof["algorithmic_bytes_per_launch"] = per_launch
+        roof["achieved"] = per_launch / (kms / klaunches / 1e3) / 1e9
+    else:
+        # a solve or grid kernel: the step's byte model over the step time
+        roof["algorithmic_bytes_per_launch"] = None
+        roof["achieved"] = bytes_total / (total_ms / 1e3) / 1e9
+        roof["note"] = "no per-launch byte model for this kernel: the whole step's model over the step time"
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+    step_roof = bytes_total / (total_ms / 1e3) / 1e9
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": total_ms / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain FP32 at scale: {size}x{size} MAC grid, ~10% solid discs, water in the lowest "
+                               f"40% of rows at 4 particles/cell, dt 1/120, MG-PCG {1 + world.layout.levels} levels "
+                               f"to {args.tol:g} relative residual",
+                   "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
+                   "tol": args.tol, "parallelism": "single world, one GPU"},
+        "solves_per_s": solves, "pcg_iterations_per_solve": d[1] / K,
+        "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+        "step_roofline_frac": step_roof / HBM_PEAK_GBS, "step_achieved_GBs": step_roof,
+        "wall_s": wall,
+        "roofline": roof,
+    }
+    if per_kernel:
+        out["kernel_ms_one_step"] = {k: round(v[0], 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:12]}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cells, kinds, size, args.tol, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -157,6 +157,24 @@ int wig_report(WigWorld *w, uint32_t stats[WIG_STATS]);
 /* The last solve's FP64 scalars.  Waits for the stream. */
 int wig_solve_stats(WigWorld *w, WigSolveStats *out);
 
+/* Running totals over every full step since the world was made (this
+ * library's own counters, no counterpart in the reference): the steps, the
+ * PCG iterations run, the particles stepped and the cells of the blocks
+ * worked on, summed on the device so a timed loop reads them only after. */
+typedef struct WigRunTotals {
+    uint64_t steps, cycles, particles, active_cells;
+} WigRunTotals;
+int wig_run_totals(WigWorld *w, WigRunTotals *out);
+
+/* Kernel timing (this library's, for measurement): with id >= 0, every later
+ * launch of that kernel is bracketed by HIP events on the world's stream;
+ * wig_kernel_time waits and gives the launches and their summed time, and
+ * clears them.  id < 0 stops.  Ids are WIG_K_*; wig_kernel_name names one. */
+int wig_time_kernel(WigWorld *w, int id);
+int wig_kernel_time(WigWorld *w, double *ms, uint32_t *launches);
+const char *wig_kernel_name(int id);
+int wig_kernel_count(void);
+
 /* Waits for everything enqueued on the world's stream. */
 int wig_sync(WigWorld *w);
 

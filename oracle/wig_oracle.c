@@ -108,13 +108,13 @@
 #define E_AIR 0u
 #define E_WATER 3u
 /* water (js/elements.js:25-30,449-457,1122-1128) */
-#define WATER_WEIGHT (9.81f / 3600.0f / 0.01f)
+#define WATER_WEIGHT ((float)(9.81 / 3600.0 / 0.01))
 #define WATER_HEATCAP 4.18f
 #define WATER_LOT 0.0f
 #define WATER_HIT 100.0f
-#define WATER_UPSPAN (2257.0f / 4.18f)
-#define WATER_DOWNSPAN (334.0f / 4.18f)
-#define WATER_DOWNSCALE (4.18f / 1.9f)
+#define WATER_UPSPAN ((float)(2257.0 / 4.18))
+#define WATER_DOWNSPAN ((float)(334.0 / 4.18))
+#define WATER_DOWNSCALE ((float)(4.18 / 1.9))
 #define WATER_EXPANSION 1.42e-4f
 #define WATER_EXPANSION2 4.45e-6f
 #define AMBIENT_T 20.0f

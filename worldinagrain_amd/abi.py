@@ -59,3 +59,8 @@ class WigSolveStats(ct.Structure):
 
 assert ct.sizeof(WigParams) == 80
 assert ct.sizeof(WigSolveStats) == 64
+
+
+class WigRunTotals(ct.Structure):
+    _fields_ = [("steps", ct.c_uint64), ("cycles", ct.c_uint64), ("particles", ct.c_uint64),
+                ("active_cells", ct.c_uint64)]

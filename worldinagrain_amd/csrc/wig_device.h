@@ -1,0 +1,286 @@
+// Device-side definitions of the liquid step for gfx950: the world's view
+// (sizes and buffer pointers, passed by value to every kernel), WGSL's
+// arithmetic (round to even, saturating conversions, mix, f16 packing), and
+// the reference's index math (shaders/fluid.wgsl:246-513, 2228-2248).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WIG_BLOCK 8
+#define LIQUID_CELL_WORDS 11u
+#define LIQUID_PER_CELL 4u
+#define FLUID_STATS 9u
+#define MG_COARSEST 512
+#define MG_LEVELS 8
+#define WNONE 0xffffffffu
+#define G_COUNT 0u
+#define G_TAKEN 1u
+#define G_REGION 1u
+#define G_CAME 1u
+#define G_SORT 2u
+#define G_START 3u
+#define G_DEST 4u
+#define G_TYPE 5u
+#define G_DENSITY 6u
+#define G_MASS 7u
+#define G_AIR_TO 8u
+#define G_RIM 9u
+#define FIXF 65536.0f
+#define MARK_LIQUID 1u
+#define MARK_REGION 2u
+#define ACTIVE 1u
+#define HAS_FLUID 2u
+#define HAS_VISCOUS 4u
+#define NEAR_LIQUID 8u
+#define COUNT_ACTIVE 0u
+#define COUNT_RETIRED 1u
+#define COUNT_VISCOUS 2u
+#define COUNT_PARTICLES 3u
+#define COUNT_RESIDUAL 4u
+#define COUNT_CYCLES 7u
+#define COUNT_MOST_CYCLES 8u
+#define SMOOTH_OMEGA 1.0f
+#define OMEGA 1.8f
+#define SPREAD 0.3f
+#define MAX_DRIFT 0.5f
+#define DRIFT_KEEP 0.9f
+#define PROJECTED 2.0f
+#define AFFINE_KEEP 0.95f
+#define LIQUID_MAX_SPEED 8.0f
+#define ADVECT_STEPS 17
+#define THETA_MIN 0.1f
+#define RIM 10
+#define PINF 1.0f
+#define REGION_OPEN 0x80000000u
+#define CAME_PRESSING 1u
+#define CAME_FALLING 2u
+#define APART (1.0f / 256.0f)
+#define APART_FIX 16777216.0f
+#define BUBBLE_GAS 0.25f
+#define COLUMN_MOST 512
+#define POWDER_PUSH 0.3f
+#define RASTER_KINDS 4u
+#define ESCAPE_REACH 16
+#define SLIP 1.0f
+#define EXACT_FINE 4294967296.0f
+#define DEST_KEEP 0xffffffffu
+#define DEST_ASIDE 0x80000000u
+#define DEST_CELL 0x40000000u
+#define DEST_SPAN 0x3fffffffu
+#define LIQUID_COUNT_MAX 255u
+#define FL_DRAWN 4u
+#define KIND_SOLID 1u
+#define KIND_POWDER 2u
+#define KIND_LIQUID 3u
+#define KIND_GAS 4u
+#define E_AIR 0u
+#define E_WATER 3u
+#define WATER_WEIGHT ((float)(9.81 / 3600.0 / 0.01))
+#define WATER_HEATCAP 4.18f
+#define WATER_LOT 0.0f
+#define WATER_HIT 100.0f
+#define WATER_UPSPAN ((float)(2257.0 / 4.18))
+#define WATER_DOWNSPAN ((float)(334.0 / 4.18))
+#define WATER_DOWNSCALE ((float)(4.18 / 1.9))
+#define WATER_EXPANSION 1.42e-4f
+#define WATER_EXPANSION2 4.45e-6f
+#define AMBIENT_T 20.0f
+#define KELVIN 273.15f
+#define VAPOUR_MAX 8.0f
+#define BOIL_K (100.0f + KELVIN)
+
+// The FP64 state of a solve: its header, then three doubles per block
+// (two running sums of a pass and the squared residual).
+struct SolveHead {
+    double rz, alpha, beta, pq, rr, bb;
+    uint32_t cycles, converged, pad[2];
+};
+
+// Everything a kernel needs to know of the world and the step.
+struct Dev {
+    int32_t W, H;
+    uint32_t C, F, B, bw, bh, cap, L;
+    uint32_t lw[MG_LEVELS + 1], lh[MG_LEVELS + 1];
+    uint64_t lstart[MG_LEVELS + 2];
+    uint32_t *cells;
+    uint32_t *sorted;
+    uint32_t *parts;
+    uint32_t *counters;
+    float *field;
+    uint32_t *blocks;
+    float *cg;
+    SolveHead *head;
+    double *bpart;
+    const uint8_t *kind;
+    uint32_t *scratch;      // per-block flags and offsets (compact)
+    unsigned long long *totals;  // steps, PCG iterations, particles, active cells (run totals)
+    uint32_t frame, seed;
+    float earthG;
+    uint32_t sweep_pairs, coarse_sweeps, tol_mode;
+    float tol;
+};
+
+struct i2 { int32_t x, y; };
+struct v2 { float x, y; };
+__device__ __forceinline__ i2 I2(int32_t x, int32_t y) { i2 r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ v2 V2(float x, float y) { v2 r; r.x = x; r.y = y; return r; }
+
+// ---------------------------------------------------------------- WGSL arithmetic
+__device__ __forceinline__ float wround(float x) { return __builtin_rintf(x); }
+__device__ __forceinline__ float wclamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float wlength(v2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+__device__ __forceinline__ int32_t wi32(float x) {
+    if (x != x) return 0;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    return (int32_t)x;
+}
+__device__ __forceinline__ uint32_t wu32(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)x;
+}
+__device__ __forceinline__ float fbits(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t ubits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ uint32_t pack2x16(v2 v) {
+    _Float16 a = (_Float16)v.x, b = (_Float16)v.y;
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+__device__ __forceinline__ v2 unpack2x16(uint32_t u) {
+    _Float16 a = __builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu));
+    _Float16 b = __builtin_bit_cast(_Float16, (uint16_t)(u >> 16));
+    return V2((float)a, (float)b);
+}
+
+// cell words (js/shared.js:79,499-531)
+__device__ __forceinline__ uint32_t wType(uint32_t x) { return x & 0xffu; }
+__device__ __forceinline__ uint32_t wAux(uint32_t x) { return (x >> 8) & 0xffu; }
+__device__ __forceinline__ uint32_t wLife(uint32_t x) { return (x >> 16) & 0xffu; }
+__device__ __forceinline__ uint32_t wShade(uint32_t x) { return (x >> 24) & 0xfu; }
+__device__ __forceinline__ uint32_t wFlags(uint32_t x) { return x >> 28; }
+__device__ __forceinline__ uint32_t packWord(uint32_t t, uint32_t aux, uint32_t life, uint32_t shade, uint32_t flags) {
+    return (t & 0xffu) | ((aux & 0xffu) << 8) | ((life & 0xffu) << 16) | ((shade & 0xfu) << 24) | ((flags & 0xfu) << 28);
+}
+__device__ __forceinline__ uint32_t wWith(uint32_t x, uint32_t f) { return x | (f << 28); }
+__device__ __forceinline__ uint32_t wWithout(uint32_t x, uint32_t f) { return x & ~(f << 28); }
+__device__ __forceinline__ bool wHas(uint32_t x, uint32_t f) { return (wFlags(x) & f) != 0u; }
+
+// the PCG hash (common.wgsl:1415-1432)
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    uint32_t s = v * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28) + 4u)) ^ s) * 277803737u;
+    return (w >> 22) ^ w;
+}
+struct Rng { uint32_t s; };
+__device__ __forceinline__ void seedRng(Rng &r, uint32_t a, uint32_t b, uint32_t c) { r.s = pcg(a ^ pcg(b ^ pcg(c))); }
+__device__ __forceinline__ uint32_t rndU(Rng &r) { r.s = pcg(r.s); return r.s; }
+__device__ __forceinline__ float rnd(Rng &r) { return (float)(rndU(r) >> 8) * (1.0f / 16777216.0f); }
+
+// exact sums (common.wgsl:1379-1411)
+struct u2 { uint32_t lo, hi; };
+__device__ __forceinline__ u2 exactParts(float x, float scale) {
+    float s = wround(wclamp(x * scale, -9.2e18f, 9.2e18f));
+    u2 r;
+    if (fabsf(s) < 2147483648.0f) {
+        int32_t v = wi32(s);
+        r.lo = (uint32_t)v;
+        r.hi = v < 0 ? 0xffffffffu : 0u;
+        return r;
+    }
+    float hi = floorf(s / 4294967296.0f);
+    r.lo = wu32(s - hi * 4294967296.0f);
+    r.hi = (uint32_t)wi32(hi);
+    return r;
+}
+__device__ __forceinline__ float exactValue(u2 w, float scale) {
+    int32_t l = (int32_t)w.lo, h = (int32_t)w.hi;
+    if ((h == 0 && l >= 0) || (h == -1 && l < 0)) return (float)l / scale;
+    return ((float)h * 4294967296.0f + (float)w.lo) / scale;
+}
+__device__ __forceinline__ u2 addExact(u2 w, float x) {
+    u2 p = exactParts(x, EXACT_FINE);
+    u2 r;
+    r.lo = w.lo + p.lo;
+    r.hi = w.hi + p.hi + (r.lo < p.lo ? 1u : 0u);
+    return r;
+}
+
+// the eight neighbour steps (common.wgsl:52-90)
+__device__ __forceinline__ uint32_t stepIndex(i2 d) {
+    if (d.y == 0) return d.x > 0 ? 0u : 4u;
+    if (d.x == 0) return d.y > 0 ? 2u : 6u;
+    if (d.y > 0) return d.x > 0 ? 1u : 3u;
+    return d.x > 0 ? 7u : 5u;
+}
+__device__ __forceinline__ int32_t stepX(uint32_t i) {
+    uint32_t j = (i + 1u) & 7u;
+    return j <= 2u ? 1 : ((j >= 4u && j <= 6u) ? -1 : 0);
+}
+__device__ __forceinline__ i2 stepAt(uint32_t i) { return I2(stepX(i), stepX((i + 6u) & 7u)); }
+__device__ __forceinline__ i2 turnStep(i2 d, int32_t k) { return stepAt((stepIndex(d) + (uint32_t)(k + 8)) & 7u); }
+
+// water's effTempIn at 1 atm, expandedAt, liquidDensity, liquidVolume
+// (common.wgsl:981-990,1501-1530,1560-1564; fluid.wgsl:1048-1052)
+__device__ __forceinline__ float effTemp(float stored) {
+    return wclamp(stored, WATER_LOT, WATER_HIT) + fmaxf(stored - WATER_HIT - WATER_UPSPAN, 0.0f)
+        - fmaxf(WATER_LOT - WATER_DOWNSPAN - stored, 0.0f) * WATER_DOWNSCALE;
+}
+__device__ __forceinline__ float expandedAt(float temp) {
+    float d = temp - fmaxf(AMBIENT_T, WATER_LOT);
+    return wclamp(1.0f - WATER_EXPANSION * d - WATER_EXPANSION2 * d * d, 0.8f, 1.05f);
+}
+__device__ __forceinline__ float liquidDensity(float temp) { return 100.0f / 100.0f * expandedAt(effTemp(temp)); }
+__device__ __forceinline__ float liquidVolume(float temp) { return 1.0f / expandedAt(effTemp(temp)); }
+
+// ---------------------------------------------------------------- addressing
+#define CNT(d, i, g) ((d).counters[LIQUID_CELL_WORDS * (uint64_t)(i) + (g)])
+#define SUM(d, f, k) ((d).counters[LIQUID_CELL_WORDS * (uint64_t)(d).C + 2ull * (f) + (k)])
+#define FLD(d, v, c) ((d).field[4ull * (uint64_t)(v) + (c)])
+#define CGV(d, i, c) ((d).cg[4ull * (uint64_t)(i) + (c)])
+#define MOTION(d, pb, k, c) ((pb)[4u + 4ull * (uint64_t)(k) + (c)])
+#define STATE(d, pb, k, c) ((pb)[4u + 4ull * ((uint64_t)(d).cap + (k)) + (c)])
+#define AFFINE(d, pb, k, c) ((pb)[4u + 4ull * (2ull * (d).cap + (k)) + (c)])
+#define STATE_SLOT(d, n) ((d).B + (n))
+#define ACTIVE_SLOT(d, n) (2u * (d).B + (n))
+#define RETIRED_SLOT(d, n) (3u * (d).B + (n))
+#define COUNT_SLOT(d, k) (4u * (d).B + (k))
+#define TOTAL_SLOT(d, n) (4u * (d).B + FLUID_STATS + (n))
+
+__device__ __forceinline__ uint32_t kindOf(const Dev &d, uint32_t t) { return d.kind[t & 0xffu]; }
+__device__ __forceinline__ bool inWorld(const Dev &d, i2 c) { return c.x >= 0 && c.y >= 0 && c.x < d.W && c.y < d.H; }
+__device__ __forceinline__ uint32_t cellIndex(const Dev &d, i2 c) { return (uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x; }
+__device__ __forceinline__ i2 cellOf(const Dev &d, v2 p) {
+    int32_t x = wi32(floorf(p.x)), y = wi32(floorf(p.y));
+    x = x < 0 ? 0 : (x > d.W - 1 ? d.W - 1 : x);
+    y = y < 0 ? 0 : (y > d.H - 1 ? d.H - 1 : y);
+    return I2(x, y);
+}
+__device__ __forceinline__ i2 beside(i2 c, int k) {
+    return k < 2 ? I2(c.x + (k & 1) * 2 - 1, c.y) : I2(c.x, c.y + (k & 1) * 2 - 1);
+}
+__device__ __forceinline__ uint32_t uFace(const Dev &d, int32_t x, int32_t y) { return (uint32_t)y * (uint32_t)(d.W + 1) + (uint32_t)x; }
+__device__ __forceinline__ uint32_t vFace(const Dev &d, int32_t x, int32_t y) {
+    return (uint32_t)(d.W + 1) * (uint32_t)d.H + (uint32_t)y * (uint32_t)d.W + (uint32_t)x;
+}
+__device__ __forceinline__ bool faceInWorld(const Dev &d, i2 at, bool isU) {
+    int32_t ex = d.W + (isU ? 1 : 0), ey = d.H + (isU ? 0 : 1);
+    return at.x >= 0 && at.y >= 0 && at.x < ex && at.y < ey;
+}
+__device__ __forceinline__ uint32_t faceIndex(const Dev &d, i2 at, bool isU) { return isU ? uFace(d, at.x, at.y) : vFace(d, at.x, at.y); }
+__device__ __forceinline__ uint64_t faceSlot(const Dev &d, uint32_t f) { return 2ull * d.C + f; }
+__device__ __forceinline__ uint64_t viscRhsSlot(const Dev &d, uint32_t f) { return 2ull * d.C + d.F + f / 4u; }
+__device__ __forceinline__ uint64_t viscLastSlot(const Dev &d, uint32_t f) { return 2ull * d.C + d.F + (d.F + 3u) / 4u + f / 4u; }
+
+__device__ __forceinline__ bool inBlocks(const Dev &d, i2 b) { return b.x >= 0 && b.y >= 0 && b.x < (int32_t)d.bw && b.y * (int32_t)d.bw < (int32_t)d.B; }
+__device__ __forceinline__ uint32_t blockIndex(const Dev &d, i2 b) { return (uint32_t)b.y * d.bw + (uint32_t)b.x; }
+__device__ __forceinline__ uint32_t activeCount(const Dev &d) { return d.blocks[COUNT_SLOT(d, COUNT_ACTIVE)]; }
+__device__ __forceinline__ bool isActiveBlock(const Dev &d, i2 b) { return inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) != 0u; }
+__device__ __forceinline__ i2 activeBlock(const Dev &d, uint32_t n) {
+    uint32_t b = d.blocks[ACTIVE_SLOT(d, n)];
+    return I2((int32_t)(b % d.bw), (int32_t)(b / d.bw));
+}
+__device__ __forceinline__ i2 idiv8(i2 c) { return I2(c.x / WIG_BLOCK, c.y / WIG_BLOCK); }
+
+// the gas plane (common.wgsl:412-465)
+__device__ __forceinline__ uint64_t gasWord(const Dev &d, uint32_t i) { return 4ull * (d.C + (i >> 1)) + ((i & 1u) ? 2u : 0u); }

@@ -1,0 +1,1654 @@
+// The liquid step's kernels for gfx950, in the order js/sim.js encodeFluid
+// dispatches the reference's passes (js/sim.js:2466-2612).  Each kernel
+// cites the pass of shaders/fluid.wgsl it computes.
+//
+// Launch shapes: a workgroup of 64 threads is one wavefront and one 8 x 8
+// block of cells (li = y * 8 + x), and the passes over the blocks worked on
+// run a fixed grid that strides over the device-side active list (its count
+// is never read back to the host).  The passes over particles stride over
+// `count` the same way, 256 threads a workgroup.  Sums that must come out
+// the same every run are integer (fixed point), or FP64 reduced in one
+// fixed tree: a wavefront's 64 cells by halving, then the blocks' partials
+// in block-index order by one workgroup.
+#include "wig_device.h"
+
+#define WAVE 64
+
+__device__ __forceinline__ bool converged(const Dev &d, uint32_t cycle) {
+    return cycle > 0u && __hip_atomic_load(&d.head->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+__device__ __forceinline__ uint32_t ld(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// ------------------------------------------------------------ cell tests
+__device__ __forceinline__ float densityAt(const Dev &d, i2 c) {           // fluid.wgsl:322-327
+    if (!inWorld(d, c)) return 0.0f;
+    return (float)CNT(d, cellIndex(d, c), G_DENSITY) / (FIXF * (float)LIQUID_PER_CELL);
+}
+__device__ __forceinline__ float gridZ(const Dev &d, i2 c) { return inWorld(d, c) ? FLD(d, 2u * cellIndex(d, c), 2) : -1.0f; }
+__device__ __forceinline__ float gridX(const Dev &d, i2 c) { return inWorld(d, c) ? FLD(d, 2u * cellIndex(d, c), 0) : 0.0f; }
+__device__ __forceinline__ float driftAt(const Dev &d, i2 c) { return inWorld(d, c) ? FLD(d, 2u * cellIndex(d, c) + 1u, 0) : 0.0f; }
+__device__ __forceinline__ float faceBeta(float a, float b) {              // :369-374
+    if (a < 0.0f || b < 0.0f || (a <= 0.0f && b <= 0.0f)) return 0.0f;
+    return (a > 0.0f && b > 0.0f) ? 2.0f / (a + b) : 1.0f / fmaxf(a, b);
+}
+__device__ __forceinline__ float surfaceShare(float da, float db) {       // :384-386
+    return wclamp((da - 0.5f) / fmaxf(da - db, 1e-3f), THETA_MIN, 1.0f);
+}
+__device__ __forceinline__ float fineBeta(float a, float b, float da, float db) {   // :396-405
+    float beta = faceBeta(a, b);
+    if (a > 0.0f && b == 0.0f) return beta / (da > 0.0f ? surfaceShare(da, db) : 0.5f);
+    if (b > 0.0f && a == 0.0f) return beta / (db > 0.0f ? surfaceShare(db, da) : 0.5f);
+    return beta;
+}
+__device__ __forceinline__ bool isLiquidAt(const Dev &d, i2 c) {
+    return inWorld(d, c) && kindOf(d, wType(d.cells[4ull * cellIndex(d, c)])) == KIND_LIQUID;
+}
+__device__ __forceinline__ uint32_t stepBit(const Dev &d, uint32_t back) { return 1u << ((d.frame + back) & 1u); }
+__device__ __forceinline__ uint32_t fallingBit(const Dev &d, uint32_t back) { return 4u << ((d.frame + back) & 1u); }
+__device__ __forceinline__ bool fallingAt(const Dev &d, i2 q) {
+    return inWorld(d, q) && (wu32(FLD(d, 2u * cellIndex(d, q) + 1u, 3)) & fallingBit(d, 1u)) != 0u;
+}
+__device__ __forceinline__ float gasAmountAt(const Dev &d, uint32_t i) {
+    uint64_t at = gasWord(d, i);
+    v2 m = unpack2x16(d.cells[at]);
+    v2 r = unpack2x16(d.cells[at + 1]);
+    return r.y + m.x * (AMBIENT_T + KELVIN) / (BOIL_K * VAPOUR_MAX);
+}
+__device__ bool isObstacleCell(const Dev &d, i2 c, uint32_t vx) {       // :1123-1147
+    uint32_t t = wType(vx), k = kindOf(d, t);
+    if (k == KIND_GAS) {
+        int wet = 0;
+        for (int j = 0; j < 4; j++) wet += isLiquidAt(d, beside(c, j)) ? 1 : 0;
+        if (wet < 2) return false;
+        if (t == E_AIR) {
+            i2 up = I2(c.x, c.y - 1), up2 = I2(c.x, c.y - 2);
+            if (!isLiquidAt(d, up) || !isLiquidAt(d, up2) || fallingAt(d, up)) return false;
+            return gasAmountAt(d, cellIndex(d, c)) >= BUBBLE_GAS;
+        }
+        return true;
+    }
+    return k == KIND_SOLID || k == KIND_POWDER;
+}
+__device__ int32_t countAt(const Dev &d, i2 c) {                           // :1150-1159
+    if (!inWorld(d, c)) return -1;
+    uint32_t i = cellIndex(d, c);
+    if (isObstacleCell(d, c, d.cells[4ull * i])) return -1;
+    return (int32_t)CNT(d, i, G_COUNT);
+}
+__device__ v2 liquidIn(const Dev &d, uint32_t i) {                         // :1165-1179
+    uint32_t vx = d.cells[4ull * i];
+    float temp = fbits(d.cells[4ull * i + 1]);
+    uint32_t k = kindOf(d, wType(vx));
+    float viscosity = 0.0f;
+    uint32_t n = CNT(d, i, G_COUNT);
+    if (n > 0u) return V2((float)CNT(d, i, G_MASS) / (FIXF * (float)n), viscosity);
+    if (k != KIND_LIQUID) return V2(1.0f, 0.0f);
+    return V2(liquidDensity(temp), viscosity);
+}
+__device__ __forceinline__ bool isObstaclePos(const Dev &d, v2 pos) {     // :424-431
+    i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
+    if (gridZ(d, c) < 0.0f) return true;
+    // (outside the world gridZ is -1: the cell word below is never read there)
+    uint32_t k = kindOf(d, wType(d.cells[4ull * cellIndex(d, c)]));
+    return k == KIND_SOLID || k == KIND_POWDER;
+}
+__device__ __forceinline__ bool walls(const Dev &d, i2 c) {               // :1056-1062
+    if (!inWorld(d, c)) return true;
+    uint32_t k = kindOf(d, wType(d.cells[4ull * cellIndex(d, c)]));
+    return k == KIND_SOLID || k == KIND_POWDER;
+}
+
+// union-find over G_REGION words (label.wgsl:25-55)
+__device__ __forceinline__ uint32_t ufLink(const Dev &d, uint32_t at) { return ld(&CNT(d, at, G_REGION)); }
+__device__ uint32_t ufFind(const Dev &d, uint32_t i) {
+    uint32_t x = i;
+    for (;;) {
+        uint32_t up = ufLink(d, x);
+        if (up >= x) break;
+        x = up;
+    }
+    return x;
+}
+__device__ void ufJoin(const Dev &d, uint32_t i, uint32_t j) {
+    uint32_t a = i, b = j;
+    for (;;) {
+        a = ufFind(d, a);
+        b = ufFind(d, b);
+        if (a == b) return;
+        uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        uint32_t old = atomicMin(&CNT(d, hi, G_REGION), lo);
+        if (old == hi) return;
+        a = old;
+        b = lo;
+    }
+}
+__device__ __forceinline__ bool pinnedAt(const Dev &d, uint32_t i) { return ufLink(d, i) == i; }
+
+// facesOf (:296-303)
+struct Face3 { int32_t x, y, isU; };
+__device__ __forceinline__ void facesOf(const Dev &d, i2 c, Face3 fs[4]) {
+    bool right = c.x == d.W - 1 || (c.x % WIG_BLOCK == WIG_BLOCK - 1 && !isActiveBlock(d, I2(c.x / WIG_BLOCK + 1, c.y / WIG_BLOCK)));
+    bool bottom = c.y == d.H - 1 || (c.y % WIG_BLOCK == WIG_BLOCK - 1 && !isActiveBlock(d, I2(c.x / WIG_BLOCK, c.y / WIG_BLOCK + 1)));
+    fs[0].x = c.x; fs[0].y = c.y; fs[0].isU = 1;
+    fs[1].x = c.x; fs[1].y = c.y; fs[1].isU = 0;
+    fs[2].x = c.x + 1; fs[2].y = c.y; fs[2].isU = right ? 1 : -1;
+    fs[3].x = c.x; fs[3].y = c.y + 1; fs[3].isU = bottom ? 0 : -1;
+}
+
+// the cell of thread li of the block at slot wid of the active list
+__device__ __forceinline__ i2 blockCell(i2 b, uint32_t li) { return I2(b.x * WIG_BLOCK + (int32_t)(li % 8u), b.y * WIG_BLOCK + (int32_t)(li / 8u)); }
+
+#define FOR_ACTIVE_BLOCKS(d, wid) for (uint32_t wid = blockIdx.x, n_ = activeCount(d); wid < n_; wid += gridDim.x)
+
+// -------------------------------------------------- wavefront reductions
+__device__ __forceinline__ double wave_tree_sum(double v) {
+    // the 64 values summed by halving: lane i takes lane i + h, h = 32 .. 1
+    for (int h = 32; h > 0; h >>= 1) v += __shfl_down(v, h, WAVE);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int h = 32; h > 0; h >>= 1) { uint32_t o = __shfl_down(v, h, WAVE); v = o > v ? o : v; }
+    return v;
+}
+// inclusive prefix sum over the wavefront
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    uint32_t lane = lane_id();
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, WAVE);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
+// ================================================================== kernels
+
+// Zeroes n words (device memory set without hipMemset, so a captured graph
+// holds only kernels).
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// sync / syncMarks (:608-659), a workgroup per block of the world
+__global__ void __launch_bounds__(64) k_sync(Dev d, int claims) {
+    uint32_t li = threadIdx.x;
+    i2 c = I2((int32_t)blockIdx.x * WIG_BLOCK + (int32_t)(li % 8u), (int32_t)blockIdx.y * WIG_BLOCK + (int32_t)(li / 8u));
+    bool liquid = false;
+    if (inWorld(d, c)) {
+        uint32_t i = cellIndex(d, c);
+        if (i == 0u) {
+            d.parts[0] = 0u;
+            d.blocks[COUNT_SLOT(d, COUNT_ACTIVE)] = 0u;
+            d.blocks[COUNT_SLOT(d, COUNT_RETIRED)] = 0u;
+            d.blocks[COUNT_SLOT(d, COUNT_VISCOUS)] = 0u;
+        }
+        uint4 v = *(const uint4 *)&d.cells[4ull * i];
+        if (kindOf(d, wType(v.x)) == KIND_LIQUID) {
+            uint32_t home = v.w >> 8;
+            if (claims && wHas(v.x, FL_DRAWN) && home < d.C) atomicMax(&CNT(d, home, G_DEST), DEST_CELL | (DEST_SPAN - i));
+            liquid = true;
+        }
+    }
+    if (__ballot(liquid) && li == 0) atomicOr(&d.blocks[blockIdx.y * d.bw + blockIdx.x], MARK_LIQUID);
+}
+
+__device__ __forceinline__ void markBlockOf(const Dev &d, v2 pos) {
+    i2 b = idiv8(cellOf(d, pos));
+    if (inBlocks(d, b)) atomicOr(&d.blocks[blockIndex(d, b)], MARK_LIQUID);
+}
+
+__device__ v2 pairWay(uint32_t a, uint32_t b) {                          // :703-707
+    uint32_t h = pcg((a < b ? a : b) ^ pcg(a < b ? b : a));
+    float angle = (float)h * (float)(6.2831853 / 4294967296.0);
+    float s = a < b ? 1.0f : -1.0f;
+    return V2(cosf(angle) * s, sinf(angle) * s);
+}
+__device__ v2 apart(const Dev &d, uint32_t k, uint32_t h, v2 pos, uint32_t id) {   // :713-734
+    uint32_t start = CNT(d, h, G_START), n = CNT(d, h, G_SORT);
+    if (n < 2u) return pos;
+    int32_t px = 0, py = 0;
+    for (uint32_t m = start; m < start + n; m++) {
+        if (m == k) continue;
+        v2 dd = V2(pos.x - fbits(MOTION(d, d.sorted, m, 0)), pos.y - fbits(MOTION(d, d.sorted, m, 1)));
+        float l = wlength(dd);
+        if (l < APART) {
+            v2 way = l == 0.0f ? pairWay(id, STATE(d, d.sorted, m, 2)) : V2(dd.x / l, dd.y / l);
+            float s = 0.5f * (APART - l) * APART_FIX;
+            px += wi32(wround(s * way.x));
+            py += wi32(wround(s * way.y));
+        }
+    }
+    v2 c = V2(floorf(pos.x), floorf(pos.y));
+    v2 to = V2(pos.x + (float)px / APART_FIX, pos.y + (float)py / APART_FIX);
+    v2 r;
+    r.x = (to.x >= c.x + APART && to.x <= c.x + 1.0f - APART) ? to.x : pos.x;
+    r.y = (to.y >= c.y + APART && to.y <= c.y + 1.0f - APART) ? to.y : pos.y;
+    return r;
+}
+__device__ uint32_t rankById(const Dev &d, uint32_t h, uint32_t k, uint32_t id, uint32_t word, uint32_t drawn) {   // :668-678
+    uint32_t start = CNT(d, h, G_START), mine = wType(word) == drawn ? 1u : 0u, r = 0;
+    for (uint32_t m = start; m < start + CNT(d, h, G_SORT); m++) {
+        uint32_t ox = STATE(d, d.sorted, m, 0), oz = STATE(d, d.sorted, m, 2);
+        uint32_t theirs = wType(ox) == drawn ? 1u : 0u;
+        r += (theirs < mine || (theirs == mine && (oz < id || (oz == id && m < k)))) ? 1u : 0u;
+    }
+    return r;
+}
+__device__ uint32_t sharedLife(const Dev &d, uint32_t life, uint32_t was, uint32_t now, uint32_t id) {   // :4149-4161
+    if (now == was) return life;
+    float l = (float)life;
+    if (now < was) l = l * (float)now / (float)was;
+    else l = 255.0f - (255.0f - l) * (255.0f - (float)now) / (255.0f - (float)was);
+    Rng r;
+    seedRng(r, id, d.frame, d.seed ^ 0x5a17u);
+    uint32_t v = wu32(floorf(l + rnd(r)));
+    return v < 255u ? v : 255u;
+}
+
+struct PState { v2 pos, vel; uint32_t word; float temp; uint32_t id; float aff[4]; };
+
+__device__ __forceinline__ void storeParticle(const Dev &d, uint32_t m, const PState &q) {   // :534-538
+    *(uint4 *)&MOTION(d, d.parts, m, 0) = make_uint4(ubits(q.pos.x), ubits(q.pos.y), ubits(q.vel.x), ubits(q.vel.y));
+    *(uint4 *)&STATE(d, d.parts, m, 0) = make_uint4(q.word, ubits(q.temp), q.id, 0u);
+    *(uint4 *)&AFFINE(d, d.parts, m, 0) = make_uint4(ubits(q.aff[0]), ubits(q.aff[1]), ubits(q.aff[2]), ubits(q.aff[3]));
+}
+
+// Places in `parts` for a wavefront's particles, in lane order: one atomic
+// a wavefront (takePlaces, :743-755, takes one a workgroup).
+__device__ __forceinline__ uint32_t wave_take(const Dev &d, uint32_t n) {
+    uint32_t incl = wave_incl_scan(n);
+    uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+    uint32_t base = 0;
+    if (lane_id() == WAVE - 1 && total) base = atomicAdd(&d.parts[0], total);
+    base = __shfl(base, WAVE - 1, WAVE);
+    return base + incl - n;
+}
+
+// syncParticles (:767-831)
+__global__ void __launch_bounds__(256) k_syncParticles(Dev d) {
+    uint32_t cnt = ld(&d.sorted[0]);
+    uint32_t n = cnt < d.cap ? cnt : d.cap;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        uint32_t k = base + lane_id();
+        bool keep = false;
+        PState q;
+        if (k < n) {
+            uint4 mo = *(const uint4 *)&MOTION(d, d.sorted, k, 0);
+            uint4 st = *(const uint4 *)&STATE(d, d.sorted, k, 0);
+            uint4 af = *(const uint4 *)&AFFINE(d, d.sorted, k, 0);
+            q.pos = V2(fbits(mo.x), fbits(mo.y));
+            q.vel = V2(fbits(mo.z), fbits(mo.w));
+            q.word = st.x;
+            q.temp = fbits(st.y);
+            q.id = st.z;
+            q.aff[0] = fbits(af.x); q.aff[1] = fbits(af.y); q.aff[2] = fbits(af.z); q.aff[3] = fbits(af.w);
+            uint32_t h = cellIndex(d, cellOf(d, q.pos));
+            q.pos = apart(d, k, h, q.pos, q.id);
+            uint32_t dest = CNT(d, h, G_DEST);
+            if (dest == DEST_KEEP) {
+                keep = true;
+                markBlockOf(d, q.pos);
+            } else if ((dest & DEST_ASIDE) != 0u) {
+                uint32_t to = dest & ~DEST_ASIDE;
+                q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
+                keep = true;
+                markBlockOf(d, q.pos);
+            } else if ((dest & DEST_CELL) != 0u) {
+                uint32_t to = DEST_SPAN - (dest & DEST_SPAN);
+                uint4 v = *(const uint4 *)&d.cells[4ull * to];
+                uint32_t cn = v.w & LIQUID_COUNT_MAX;
+                if (cn == LIQUID_COUNT_MAX || CNT(d, h, G_SORT) <= cn || rankById(d, h, k, q.id, q.word, wType(v.x)) < cn) {
+                    if (to != h)
+                        q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
+                    q.temp = fbits(v.y);
+                    uint32_t drawn = CNT(d, h, G_TYPE), pt = wType(q.word);
+                    if (pt == E_WATER && pt == (drawn & 0xffu) && wType(v.x) == pt
+                        && (wAux(q.word) == ((drawn >> 8) & 0xffu) || wAux(q.word) == 0u)) {
+                        uint32_t life = sharedLife(d, wAux(q.word) == 0u ? 0u : wLife(q.word), (drawn >> 16) & 0xffu, wLife(v.x), q.id);
+                        uint32_t ww = wWithout(v.x, FL_DRAWN);
+                        uint32_t solute = wAux(ww) != 0u ? wAux(ww) : (drawn >> 8) & 0xffu;
+                        q.word = packWord(pt, life > 0u ? solute : 0u, life, wShade(ww), wFlags(ww));
+                    } else if ((wType(q.word) | (wAux(q.word) << 8)) == (drawn & 0xffffu)) {
+                        q.word = wWithout(v.x, FL_DRAWN);
+                    }
+                    keep = true;
+                }
+            }
+        }
+        uint32_t m = wave_take(d, keep ? 1u : 0u);
+        if (keep && m < d.cap) storeParticle(d, m, q);
+    }
+}
+
+__device__ __forceinline__ uint32_t seedId(const Dev &d, i2 c, uint32_t j) {   // :835-841
+    uint32_t h = (((uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x) * 0x9e3779b9u) ^ (d.frame * 0x85ebca6bu) ^ (j * 0xc2b2ae35u);
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    return h;
+}
+
+// syncSeed (:863-890), a workgroup per block of the world
+__global__ void __launch_bounds__(64) k_syncSeed(Dev d) {
+    uint32_t li = threadIdx.x;
+    i2 c = I2((int32_t)blockIdx.x * WIG_BLOCK + (int32_t)(li % 8u), (int32_t)blockIdx.y * WIG_BLOCK + (int32_t)(li / 8u));
+    uint32_t n = 0;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    Rng r;
+    r.s = 0;
+    uint32_t i = 0;
+    if (inWorld(d, c)) {
+        i = cellIndex(d, c);
+        v = *(const uint4 *)&d.cells[4ull * i];
+        if (kindOf(d, wType(v.x)) == KIND_LIQUID) {
+            uint32_t home = v.w >> 8, want = v.w & LIQUID_COUNT_MAX;
+            if (!wHas(v.x, FL_DRAWN)) n = LIQUID_PER_CELL;
+            else if (home >= d.C || CNT(d, home, G_DEST) != (DEST_CELL | (DEST_SPAN - i))) n = want > 1u ? want : 1u;
+            else if (want > CNT(d, home, G_SORT) && want < LIQUID_COUNT_MAX) n = want - CNT(d, home, G_SORT);
+            seedRng(r, i, d.frame, d.seed ^ 0x5eedu);
+        }
+    }
+    uint32_t m = wave_take(d, n);
+    for (uint32_t j = 0; j < n; j++) {
+        PState q;
+        if (n == 1u) {
+            q.pos = V2((float)c.x + 0.5f, (float)c.y + 0.5f);
+        } else {
+            float rx = rnd(r), ry = rnd(r);
+            v2 rr = V2(rx - 0.5f, ry - 0.5f);
+            if (n <= 4u) {
+                float qx = (j == 1u || j == 2u) ? 1.0f : 0.0f, qy = (j == 1u || j == 3u) ? 1.0f : 0.0f;
+                q.pos = V2((float)c.x + 0.25f + 0.5f * qx + 0.3f * rr.x, (float)c.y + 0.25f + 0.5f * qy + 0.3f * rr.y);
+            } else {
+                q.pos = V2((float)c.x + 0.5f + 0.9f * rr.x, (float)c.y + 0.5f + 0.9f * rr.y);
+            }
+        }
+        q.vel = unpack2x16(v.z);
+        q.word = wWithout(v.x, FL_DRAWN);
+        q.temp = fbits(v.y);
+        q.id = seedId(d, c, j);
+        q.aff[0] = q.aff[1] = q.aff[2] = q.aff[3] = 0.0f;
+        if (m + j < d.cap) storeParticle(d, m + j, q);
+    }
+}
+
+// compact (:896-921): the blocks worked on and those dropping out, listed in
+// block-index order by one workgroup (the reference's lists are in whatever
+// order its threads came).
+__global__ void __launch_bounds__(1024) k_compact(Dev d) {
+    __shared__ uint32_t sA[1024], sR[1024];
+    uint32_t t = threadIdx.x, T = blockDim.x;
+    uint32_t per = (d.B + T - 1u) / T, lo = t * per, hi = lo + per < d.B ? lo + per : d.B;
+    uint32_t na = 0, nr = 0;
+    for (uint32_t n = lo; n < hi; n++) {
+        i2 b = I2((int32_t)(n % d.bw), (int32_t)(n / d.bw));
+        uint32_t marks = 0;
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                i2 o = I2(b.x + x, b.y + y);
+                if (inBlocks(d, o)) marks |= d.blocks[blockIndex(d, o)];
+            }
+        bool nearl = (marks & MARK_LIQUID) != 0u, liquid = (marks & (MARK_LIQUID | MARK_REGION)) != 0u;
+        bool was = (d.blocks[STATE_SLOT(d, n)] & ACTIVE) != 0u;
+        d.scratch[n] = (liquid ? 1u : 0u) | (!liquid && was ? 2u : 0u) | (nearl ? 4u : 0u);
+        na += liquid ? 1u : 0u;
+        nr += (!liquid && was) ? 1u : 0u;
+    }
+    sA[t] = na;
+    sR[t] = nr;
+    __syncthreads();
+    for (uint32_t o = 1; o < T; o <<= 1) {
+        uint32_t a = t >= o ? sA[t - o] : 0u, r = t >= o ? sR[t - o] : 0u;
+        __syncthreads();
+        sA[t] += a;
+        sR[t] += r;
+        __syncthreads();
+    }
+    uint32_t ia = sA[t] - na, ir = sR[t] - nr;
+    for (uint32_t n = lo; n < hi; n++) {
+        uint32_t f = d.scratch[n];
+        d.blocks[STATE_SLOT(d, n)] = ((f & 1u) ? ACTIVE : 0u) | ((f & 4u) ? NEAR_LIQUID : 0u);
+        if (f & 1u) d.blocks[ACTIVE_SLOT(d, ia++)] = n;
+        if (f & 2u) d.blocks[RETIRED_SLOT(d, ir++)] = n;
+    }
+    if (t == T - 1u) {
+        d.blocks[COUNT_SLOT(d, COUNT_ACTIVE)] = sA[t];
+        d.blocks[COUNT_SLOT(d, COUNT_RETIRED)] = sR[t];
+    }
+}
+
+// retire (:924-951)
+__global__ void __launch_bounds__(64) k_retire(Dev d) {
+    for (uint32_t wid = blockIdx.x, nr = d.blocks[COUNT_SLOT(d, COUNT_RETIRED)]; wid < nr; wid += gridDim.x) {
+        uint32_t n = d.blocks[RETIRED_SLOT(d, wid)];
+        i2 c = blockCell(I2((int32_t)(n % d.bw), (int32_t)(n / d.bw)), threadIdx.x);
+        if (inWorld(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            *(float4 *)&FLD(d, 2u * i, 0) = make_float4(0.f, 0.f, 0.f, 0.f);
+            *(float4 *)&FLD(d, 2u * i + 1u, 0) = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (uint32_t g = 0; g < LIQUID_CELL_WORDS; g++) CNT(d, i, g) = 0u;
+            Face3 fs[4];
+            facesOf(d, c, fs);
+            for (int j = 0; j < 4; j++) {
+                if (fs[j].isU < 0) continue;
+                uint32_t f = faceIndex(d, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
+                *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(0.f, 0.f, 0.f, 0.f);
+                FLD(d, viscLastSlot(d, f), f % 4u) = 0.0f;
+                SUM(d, f, 0) = 0u;
+                SUM(d, f, 1) = 0u;
+            }
+        }
+        if (threadIdx.x == 0) d.blocks[n] = 0u;
+    }
+}
+
+// clear (:962-985)
+__global__ void __launch_bounds__(64) k_clear(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 b = activeBlock(d, wid);
+        if (threadIdx.x == 0) d.blocks[blockIndex(d, b)] = 0u;
+        i2 c = blockCell(b, threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        for (uint32_t g = 0; g < LIQUID_CELL_WORDS; g++) CNT(d, i, g) = 0u;
+        CNT(d, i, G_REGION) = i;
+        Face3 fs[4];
+        facesOf(d, c, fs);
+        for (int j = 0; j < 4; j++) {
+            if (fs[j].isU < 0) continue;
+            uint32_t f = faceIndex(d, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
+            SUM(d, f, 0) = 0u;
+            SUM(d, f, 1) = 0u;
+        }
+    }
+}
+
+// gather (:1000-1087): the splat as fixed-point integer atomics, whose sums
+// are the same in any order
+__device__ __forceinline__ void splatAxis(const Dev &d, v2 s, float vel, float gx, float gy, bool isU) {
+    float fx0 = floorf(s.x), fy0 = floorf(s.y);
+    i2 b = I2(wi32(fx0), wi32(fy0));
+    v2 f = V2(s.x - fx0, s.y - fy0);
+    for (int j = 0; j < 4; j++) {
+        int ox = j & 1, oy = j >> 1;
+        float wgt = (ox == 1 ? f.x : 1.0f - f.x) * (oy == 1 ? f.y : 1.0f - f.y);
+        i2 at = I2(b.x + ox, b.y + oy);
+        if (wgt <= 0.0f || !faceInWorld(d, at, isU)) continue;
+        uint32_t fi = faceIndex(d, at, isU);
+        float there = vel + (gx * ((float)at.x - s.x) + gy * ((float)at.y - s.y));
+        atomicAdd(&SUM(d, fi, 0), (uint32_t)wi32(wround(there * wgt * FIXF)));
+        atomicAdd(&SUM(d, fi, 1), (uint32_t)wi32(wround(wgt * FIXF)));
+    }
+}
+__device__ __forceinline__ void splatDensity(const Dev &d, v2 pos, float room) {
+    v2 s = V2(pos.x - 0.5f, pos.y - 0.5f);
+    float fx0 = floorf(s.x), fy0 = floorf(s.y);
+    i2 b = I2(wi32(fx0), wi32(fy0));
+    v2 f = V2(s.x - fx0, s.y - fy0);
+    for (int j = 0; j < 4; j++) {
+        int ox = j & 1, oy = j >> 1;
+        float wgt = (ox == 1 ? f.x : 1.0f - f.x) * (oy == 1 ? f.y : 1.0f - f.y);
+        i2 at = I2(b.x + ox, b.y + oy);
+        if (walls(d, at)) {
+            i2 x = I2(b.x + 1 - ox, at.y), y = I2(at.x, b.y + 1 - oy), xy = I2(b.x + 1 - ox, b.y + 1 - oy);
+            at = !walls(d, x) ? x : (!walls(d, y) ? y : (!walls(d, xy) ? xy : I2(-1, -1)));
+        }
+        if (inWorld(d, at) && wgt > 0.0f) atomicAdd(&CNT(d, cellIndex(d, at), G_DENSITY), wu32(wround(wgt * room * FIXF)));
+    }
+}
+__global__ void __launch_bounds__(256) k_gather(Dev d) {
+    uint32_t cnt = ld(&d.parts[0]);
+    uint32_t n = cnt < d.cap ? cnt : d.cap;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
+        uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
+        uint4 af = *(const uint4 *)&AFFINE(d, d.parts, k, 0);
+        v2 pos = V2(fbits(mo.x), fbits(mo.y));
+        float temp = fbits(st.y);
+        splatAxis(d, V2(pos.x - 0.0f, pos.y - 0.5f), fbits(mo.z), fbits(af.x), fbits(af.y), true);
+        splatAxis(d, V2(pos.x - 0.5f, pos.y - 0.0f), fbits(mo.w), fbits(af.z), fbits(af.w), false);
+        uint32_t home = cellIndex(d, cellOf(d, pos));
+        atomicAdd(&CNT(d, home, G_COUNT), 1u);
+        atomicAdd(&CNT(d, home, G_MASS), wu32(wround(liquidDensity(temp) * FIXF)));
+        splatDensity(d, pos, liquidVolume(temp));
+    }
+}
+
+// sealJoin, sealAnchor (:1503-1554)
+__device__ __forceinline__ bool mayBeSealed(const Dev &d, i2 c) {
+    if (countAt(d, c) != 0) return false;
+    uint32_t i = cellIndex(d, c);
+    return FLD(d, 2u * i, 2) > 0.0f || FLD(d, 2u * i + 1u, 2) != 0.0f;
+}
+__device__ __forceinline__ bool wasOpen(const Dev &d, i2 q) {
+    if (!inWorld(d, q) || countAt(d, q) < 0) return false;
+    return (wu32(FLD(d, 2u * cellIndex(d, q) + 1u, 3)) & stepBit(d, 1u)) == 0u;
+}
+__global__ void __launch_bounds__(64) k_sealJoin(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c) || !mayBeSealed(d, c)) continue;
+        for (int k = 0; k < 2; k++) {
+            i2 n = k == 0 ? I2(c.x + 1, c.y) : I2(c.x, c.y + 1);
+            if (mayBeSealed(d, n)) ufJoin(d, cellIndex(d, c), cellIndex(d, n));
+        }
+    }
+}
+__global__ void __launch_bounds__(64) k_sealAnchor(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c) || !mayBeSealed(d, c)) continue;
+        if (wasOpen(d, beside(c, 0)) || wasOpen(d, beside(c, 1)) || wasOpen(d, beside(c, 2)) || wasOpen(d, beside(c, 3)))
+            atomicOr(&CNT(d, ufFind(d, cellIndex(d, c)), G_REGION), REGION_OPEN);
+    }
+}
+
+// classify (:1566-1696)
+__device__ float columnWeight(const Dev &d, i2 p, float density) {
+    int over = 0;
+    i2 q = I2(p.x, p.y - 1);
+    while (over < COLUMN_MOST && countAt(d, q) > 0) {
+        over += 1;
+        q.y -= 1;
+    }
+    return density * d.earthG * ((float)over + 0.5f);
+}
+__global__ void __launch_bounds__(64) k_classify(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 bk = activeBlock(d, wid);
+        i2 p = blockCell(bk, threadIdx.x);
+        uint32_t bits = 0;
+        if (inWorld(d, p)) {
+            uint32_t i = cellIndex(d, p);
+            int32_t n = countAt(d, p);
+            float4 was = *(const float4 *)&FLD(d, 2u * i, 0);
+            float4 wasDrift = *(const float4 *)&FLD(d, 2u * i + 1u, 0);
+            float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            bool sealed = false;
+            if (n == 0 && mayBeSealed(d, p)) sealed = (ufLink(d, ufFind(d, i)) & REGION_OPEN) == 0u;
+            if (n < 0) {
+                c[2] = -1.0f;
+            } else if (n > 0) {
+                v2 li = liquidIn(d, i);
+                c[2] = li.x;
+                c[3] = li.y;
+            } else {
+                float fluid = 0.0f, nwalls = 0.0f;
+                v2 mean = V2(0.0f, 0.0f);
+                for (int k = 0; k < 4; k++) {
+                    i2 q = beside(p, k);
+                    int32_t m = countAt(d, q);
+                    if (m < 0) nwalls += 1.0f;
+                    else if (m > 0) {
+                        fluid += 1.0f;
+                        v2 li = liquidIn(d, cellIndex(d, q));
+                        mean.x += li.x;
+                        mean.y += li.y;
+                    }
+                }
+                if (fluid >= 2.0f && fluid + nwalls >= 3.0f) {
+                    c[2] = mean.x / fluid;
+                    c[3] = mean.y / fluid;
+                } else if (sealed) {
+                    c[2] = fluid > 0.0f ? mean.x / fluid : 1.0f;
+                    c[3] = fluid > 0.0f ? mean.y / fluid : 0.0f;
+                }
+            }
+            uint32_t mine = stepBit(d, 0u);
+            uint32_t liquid = c[2] > 0.0f ? mine : 0u;
+            float4 drift = make_float4(0.0f, 0.0f, sealed ? 1.0f : 0.0f, (float)((wu32(wasDrift.w) & ~(mine | fallingBit(d, 0u))) | liquid));
+            if (c[2] > 0.0f) {
+                if (was.z > 0.0f) {
+                    c[0] = was.x;
+                    drift.x = wasDrift.x * DRIFT_KEEP;
+                } else {
+                    c[0] = columnWeight(d, p, c[2]);
+                }
+                bits = c[3] > 0.0f ? (HAS_FLUID | HAS_VISCOUS) : HAS_FLUID;
+            }
+            *(float4 *)&FLD(d, 2u * i, 0) = make_float4(c[0], c[1], c[2], c[3]);
+            *(float4 *)&FLD(d, 2u * i + 1u, 0) = drift;
+        }
+        // the block's bits, OR-reduced over the wavefront
+        for (int h = 32; h > 0; h >>= 1) bits |= __shfl_xor(bits, h, WAVE);
+        if (threadIdx.x == 0) {
+            atomicOr(&d.blocks[STATE_SLOT(d, blockIndex(d, bk))], bits);
+            if (bits & HAS_VISCOUS) d.blocks[COUNT_SLOT(d, COUNT_VISCOUS)] = 1u;
+        }
+    }
+}
+
+// setFaces (:1936-1994), with unpress's part for faces no viscosity solves
+// (:3444-3446)
+__device__ __forceinline__ float obstacleVel(const Dev &d, i2 c, int axis) {
+    if (!inWorld(d, c)) return 0.0f;
+    uint4 v = *(const uint4 *)&d.cells[4ull * cellIndex(d, c)];
+    uint32_t k = kindOf(d, wType(v.x));
+    if (k == KIND_SOLID || k == KIND_GAS) return 0.0f;
+    v2 u = unpack2x16(v.z);
+    return (axis == 0 ? u.x : u.y) * POWDER_PUSH;
+}
+__device__ __forceinline__ float4 gridAt4(const Dev &d, i2 c) {
+    if (!inWorld(d, c)) return make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+    return *(const float4 *)&FLD(d, 2u * cellIndex(d, c), 0);
+}
+__device__ __forceinline__ float faceViscosity(float4 ga, float4 gb) {
+    if (ga.z < 0.0f || gb.z < 0.0f) return 0.0f;
+    float fa = ga.z > 0.0f ? 1.0f : 0.0f, fb = gb.z > 0.0f ? 1.0f : 0.0f;
+    return (fa * ga.w + fb * gb.w) / fmaxf(fa + fb, 1.0f);
+}
+__device__ void setFace(const Dev &d, i2 at, bool isU) {
+    uint32_t f = faceIndex(d, at, isU);
+    i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+    float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
+    int32_t weight = (int32_t)SUM(d, f, 1);
+    float u = 0.0f, held = 0.0f;
+    if (ga.z < 0.0f && gb.z < 0.0f) {
+        held = -1.0f;
+    } else if (ga.z < 0.0f || gb.z < 0.0f) {
+        u = obstacleVel(d, ga.z < 0.0f ? a : at, isU ? 0 : 1);
+        held = 1.0f;
+    } else if (weight > 0) {
+        u = (float)(int32_t)SUM(d, f, 0) / (float)weight;
+        held = 1.0f;
+    }
+    float now = u;
+    if (held > 0.0f) now += 0.0f;                        // the wind's drag: still air
+    if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) now += isU ? 0.0f : d.earthG;
+    *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(now, u, held, faceViscosity(ga, gb));
+    FLD(d, viscRhsSlot(d, f), f % 4u) = now;
+    if (FLD(d, viscLastSlot(d, f), f % 4u) != 0.0f) FLD(d, viscLastSlot(d, f), f % 4u) = 0.0f;
+}
+__global__ void __launch_bounds__(64) k_setFaces(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        Face3 fs[4];
+        facesOf(d, c, fs);
+        for (int j = 0; j < 4; j++)
+            if (fs[j].isU >= 0) setFace(d, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
+    }
+}
+
+// divergence (:1996-2025)
+__global__ void __launch_bounds__(64) k_divergence(Dev d) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL)] = 0u;
+        d.blocks[COUNT_SLOT(d, COUNT_CYCLES)] = 0u;
+    }
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        if (FLD(d, 2u * i, 2) <= 0.0f) continue;
+        float out = FLD(d, faceSlot(d, uFace(d, c.x + 1, c.y)), 0) - FLD(d, faceSlot(d, uFace(d, c.x, c.y)), 0)
+            + FLD(d, faceSlot(d, vFace(d, c.x, c.y + 1)), 0) - FLD(d, faceSlot(d, vFace(d, c.x, c.y)), 0);
+        FLD(d, 2u * i, 1) = out;
+        float n = densityAt(d, c);
+        float room = wclamp(1.0f, 0.1f, 1.9f);
+        FLD(d, 2u * i + 1u, 1) = FLD(d, 2u * i + 1u, 2) < 0.0f ? 0.0f : -SPREAD * (n - room);
+    }
+}
+
+// ============================================================ the solve
+__device__ __forceinline__ bool inLevel(const Dev &d, uint32_t l, i2 c) {
+    return c.x >= 0 && c.y >= 0 && c.x < (int32_t)d.lw[l] && c.y < (int32_t)d.lh[l];
+}
+__device__ __forceinline__ uint64_t levelSlot(const Dev &d, uint32_t l, i2 c) {
+    return d.lstart[l] + 2ull * ((uint64_t)c.y * d.lw[l] + (uint64_t)c.x);
+}
+__device__ __forceinline__ float rhoAt(const Dev &d, uint32_t l, i2 c) {     // :2251-2259
+    if (!inLevel(d, l, c)) return -1.0f;
+    if (l == 0) return FLD(d, 2u * cellIndex(d, c), 2);
+    return FLD(d, levelSlot(d, l, c) + 1u, 0);
+}
+__device__ __forceinline__ v2 potAt(const Dev &d, uint32_t l, i2 c) {        // :2262-2271
+    if (!inLevel(d, l, c)) return V2(0.0f, 0.0f);
+    if (l == 0) { uint32_t i = 2u * cellIndex(d, c); return V2(FLD(d, i, 0), FLD(d, i + 1u, 0)); }
+    uint64_t at = levelSlot(d, l, c);
+    return V2(FLD(d, at, 0), FLD(d, at, 1));
+}
+__device__ __forceinline__ v2 rhsAt(const Dev &d, uint32_t l, i2 c) {
+    if (l == 0) { uint32_t i = 2u * cellIndex(d, c); return V2(FLD(d, i, 1), FLD(d, i + 1u, 1)); }
+    uint64_t at = levelSlot(d, l, c);
+    return V2(FLD(d, at, 2), FLD(d, at, 3));
+}
+struct Stencil { v2 sum; float diag; };
+__device__ Stencil stencil(const Dev &d, uint32_t l, i2 c, float rho) {      // :2286-2302
+    Stencil s;
+    s.sum = V2(0.0f, 0.0f);
+    s.diag = 0.0f;
+    float dc = l == 0 ? densityAt(d, c) : 0.0f;
+    for (int k = 0; k < 4; k++) {
+        i2 n = beside(c, k);
+        float r = rhoAt(d, l, n);
+        if (r < 0.0f) continue;
+        float beta = l == 0 ? fineBeta(rho, r, dc, densityAt(d, n)) : faceBeta(rho, r);
+        v2 p = potAt(d, l, n);
+        s.sum.x += beta * p.x;
+        s.sum.y += beta * p.y;
+        s.diag += beta;
+    }
+    return s;
+}
+__device__ v2 residualAt(const Dev &d, uint32_t l, i2 c) {                  // :2306-2315
+    float rho = rhoAt(d, l, c);
+    if (rho <= 0.0f) return V2(0.0f, 0.0f);
+    Stencil s = stencil(d, l, c, rho);
+    float h2 = (float)(1u << (2u * l));
+    v2 pin = (l == 0 && pinnedAt(d, cellIndex(d, c))) ? V2(PINF, 0.0f) : V2(0.0f, 0.0f);
+    v2 pot = potAt(d, l, c), rhs = rhsAt(d, l, c);
+    return V2(rhs.x - (s.sum.x - (s.diag + pin.x) * pot.x) / h2, rhs.y - (s.sum.y - (s.diag + pin.y) * pot.y) / h2);
+}
+
+// relaxRims (:2096-2107)
+__global__ void __launch_bounds__(64) k_relaxRims(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        if (FLD(d, 2u * i, 2) > 0.0f) {
+            CNT(d, i, G_RIM) = ubits(FLD(d, 2u * i, 0));
+            CNT(d, i, G_RIM + 1u) = ubits(FLD(d, 2u * i + 1u, 0));
+        }
+    }
+}
+
+// relax / relaxShifted (:2080-2192): a 10 x 10 tile in LDS, its rim frozen
+// as relaxRims left it, sweep_pairs red-black pairs, one write back.
+__device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
+    if (wid < activeCount(d)) return activeBlock(d, wid);
+    int32_t wide = (int32_t)d.bw, high = (int32_t)(d.B / d.bw);
+    int32_t j = (int32_t)wid - (int32_t)d.B;
+    if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
+    return j <= high ? I2(wide, j) : I2(j - high - 1, high);
+}
+__global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift) {
+    if (converged(d, cycle)) return;
+    __shared__ float sPx[RIM * RIM], sPy[RIM * RIM], sRho[RIM * RIM], sShare[RIM * RIM];
+    uint32_t li = threadIdx.x;
+    uint32_t na = activeCount(d), edge = shift ? d.bw + d.bh + 1u : 0u;
+    // the tiles: the active list's blocks, then (shifted) the edge tiles,
+    // workgroup indices B .. B + edge of the reference's dispatch
+    for (uint32_t v = blockIdx.x; v < na + edge; v += gridDim.x) {
+        uint32_t wid = v < na ? v : d.B + (v - na);
+        i2 b = tileBlock(d, wid, shift);
+        i2 origin = I2(b.x * WIG_BLOCK - shift, b.y * WIG_BLOCK - shift);
+        if (b.x < 0 || origin.x >= d.W || origin.y >= d.H) continue;
+        bool live = false;
+        {
+            i2 b0 = idiv8(origin), b1 = idiv8(I2(origin.x + WIG_BLOCK - 1, origin.y + WIG_BLOCK - 1));
+            uint32_t bits = 0;
+            for (int32_t y = b0.y; y <= b1.y; y++)
+                for (int32_t x = b0.x; x <= b1.x; x++)
+                    if (inBlocks(d, I2(x, y))) bits |= d.blocks[STATE_SLOT(d, blockIndex(d, I2(x, y)))];
+            live = (bits & HAS_FLUID) != 0u;
+        }
+        if (!live) continue;
+        for (uint32_t k = li; k < RIM * RIM; k += 64u) {
+            i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
+            float px = 0.0f, py = 0.0f, r = -1.0f;
+            if (inWorld(d, c)) {
+                uint32_t ci = cellIndex(d, c);
+                px = FLD(d, 2u * ci, 0);
+                py = FLD(d, 2u * ci + 1u, 0);
+                r = FLD(d, 2u * ci, 2);
+                if (r > 0.0f) {
+                    px = fbits(CNT(d, ci, G_RIM));
+                    py = fbits(CNT(d, ci, G_RIM + 1u));
+                }
+            }
+            sPx[k] = px;
+            sPy[k] = py;
+            sRho[k] = r;
+            sShare[k] = densityAt(d, c);
+        }
+        __syncthreads();
+        i2 me = I2(origin.x + (int32_t)(li % 8u), origin.y + (int32_t)(li / 8u));
+        int t = (int)(li / 8u + 1u) * RIM + (int)(li % 8u) + 1;
+        float r0 = sRho[t];
+        float rhsx = 0.0f, rhsy = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+        if (r0 > 0.0f) {
+            uint32_t i = 2u * cellIndex(d, me);
+            rhsx = FLD(d, i, 1);
+            rhsy = FLD(d, i + 1u, 1);
+            float dd = sShare[t];
+            b0 = fineBeta(r0, sRho[t - 1], dd, sShare[t - 1]);
+            b1 = fineBeta(r0, sRho[t + 1], dd, sShare[t + 1]);
+            b2 = fineBeta(r0, sRho[t - RIM], dd, sShare[t - RIM]);
+            b3 = fineBeta(r0, sRho[t + RIM], dd, sShare[t + RIM]);
+        }
+        float diag = b0 + b1 + b2 + b3;
+        bool pinned = r0 > 0.0f && pinnedAt(d, cellIndex(d, me));
+        float dx = diag + (pinned ? PINF : 0.0f), dy = fmaxf(diag, 1e-30f);
+        bool solving = dx > 0.0f;
+        int color = (me.x + me.y) & 1;
+        for (int s = 0; s < 2 * (int)d.sweep_pairs; s++) {
+            if (solving && color == (s & 1)) {
+                float sx = b0 * sPx[t - 1] + b1 * sPx[t + 1] + b2 * sPx[t - RIM] + b3 * sPx[t + RIM];
+                float sy = b0 * sPy[t - 1] + b1 * sPy[t + 1] + b2 * sPy[t - RIM] + b3 * sPy[t + RIM];
+                float nx = wmix(sPx[t], (sx - rhsx) / dx, SMOOTH_OMEGA);
+                float ny = wmix(sPy[t], (sy - rhsy) / dy, SMOOTH_OMEGA);
+                sPx[t] = nx;
+                if (diag > 0.0f) sPy[t] = ny;
+            }
+            __syncthreads();
+        }
+        if (solving) {
+            uint32_t i = 2u * cellIndex(d, me);
+            FLD(d, i, 0) = sPx[t];
+            FLD(d, i + 1u, 0) = sPy[t];
+        }
+        __syncthreads();
+    }
+}
+
+// restrictLevel (:2342-2377)
+__global__ void __launch_bounds__(64) k_restrict(Dev d, uint32_t cycle, uint32_t l) {
+    if (converged(d, cycle) || activeCount(d) == 0u) return;
+    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
+    if (!inLevel(d, l, c)) return;
+    uint64_t at = levelSlot(d, l, c);
+    int32_t span = 1 << l;
+    if (span <= WIG_BLOCK) {
+        i2 b = I2(c.x * span / WIG_BLOCK, c.y * span / WIG_BLOCK);
+        if (inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u) {
+            *(float4 *)&FLD(d, at, 0) = make_float4(0.f, 0.f, 0.f, 0.f);
+            *(float4 *)&FLD(d, at + 1u, 0) = make_float4(0.f, 0.f, 0.f, 0.f);
+            return;
+        }
+    }
+    float fluid = 0.0f, sum = 0.0f;
+    bool air = false;
+    v2 r = V2(0.0f, 0.0f);
+    for (int j = 0; j < 4; j++) {
+        i2 k = I2(2 * c.x + (j & 1), 2 * c.y + (j >> 1));
+        float rho = rhoAt(d, l - 1u, k);
+        if (rho == 0.0f) air = true;
+        else if (rho > 0.0f) {
+            fluid += 1.0f;
+            sum += rho;
+            v2 q = residualAt(d, l - 1u, k);
+            r.x += q.x;
+            r.y += q.y;
+        }
+    }
+    float rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
+    *(float4 *)&FLD(d, at, 0) = make_float4(0.0f, 0.0f, rho > 0.0f ? r.x * 0.25f : 0.0f, rho > 0.0f ? r.y * 0.25f : 0.0f);
+    *(float4 *)&FLD(d, at + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
+}
+
+// smoothRed / smoothBlack (:2317-2387)
+__global__ void __launch_bounds__(64) k_smooth(Dev d, uint32_t cycle, uint32_t l, int color) {
+    if (converged(d, cycle) || activeCount(d) == 0u) return;
+    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
+    if (!inLevel(d, l, c) || ((c.x + c.y) & 1) != color) return;
+    uint64_t at = levelSlot(d, l, c);
+    float rho = FLD(d, at + 1u, 0);
+    if (rho <= 0.0f) return;
+    Stencil s = stencil(d, l, c, rho);
+    if (s.diag <= 0.0f) return;
+    float h2 = (float)(1u << (2u * l));
+    float4 e = *(const float4 *)&FLD(d, at, 0);
+    float sx = (s.sum.x - h2 * e.z) / s.diag, sy = (s.sum.y - h2 * e.w) / s.diag;
+    FLD(d, at, 0) = wmix(e.x, sx, 1.0f);
+    FLD(d, at, 1) = wmix(e.y, sy, 1.0f);
+}
+
+// solve (:2389-2456): the coarsest level in one workgroup's LDS
+__global__ void __launch_bounds__(256) k_coarsest(Dev d, uint32_t cycle, uint32_t l) {
+    if (converged(d, cycle)) return;
+    __shared__ float sPx[MG_COARSEST], sPy[MG_COARSEST], sRx[MG_COARSEST], sRy[MG_COARSEST], sRho[MG_COARSEST];
+    uint32_t li = threadIdx.x, sx = d.lw[l], sy = d.lh[l];
+    uint32_t n = sx * sy < (uint32_t)MG_COARSEST ? sx * sy : (uint32_t)MG_COARSEST;
+    for (uint32_t j = li; j < n; j += 256u) {
+        uint64_t at = levelSlot(d, l, I2((int32_t)(j % sx), (int32_t)(j / sx)));
+        float4 e = *(const float4 *)&FLD(d, at, 0);
+        sPx[j] = e.x; sPy[j] = e.y; sRx[j] = e.z; sRy[j] = e.w;
+        sRho[j] = FLD(d, at + 1u, 0);
+    }
+    __syncthreads();
+    float h2 = (float)(1u << (2u * l));
+    for (uint32_t s = 0; s < 2u * d.coarse_sweeps; s++) {
+        for (uint32_t j = li; j < n; j += 256u) {
+            i2 c = I2((int32_t)(j % sx), (int32_t)(j / sx));
+            float r0 = sRho[j];
+            if ((uint32_t)((c.x + c.y) & 1) != (s & 1u) || r0 <= 0.0f) continue;
+            float sumx = 0.0f, sumy = 0.0f, diag = 0.0f;
+            for (int k = 0; k < 4; k++) {
+                i2 o = beside(c, k);
+                if (o.x < 0 || o.y < 0 || o.x >= (int32_t)sx || o.y >= (int32_t)sy) continue;
+                uint32_t oj = (uint32_t)o.y * sx + (uint32_t)o.x;
+                float r = sRho[oj];
+                if (r < 0.0f) continue;
+                float beta = faceBeta(r0, r);
+                sumx += beta * sPx[oj];
+                sumy += beta * sPy[oj];
+                diag += beta;
+            }
+            if (diag > 0.0f) {
+                sPx[j] = wmix(sPx[j], (sumx - h2 * sRx[j]) / diag, OMEGA);
+                sPy[j] = wmix(sPy[j], (sumy - h2 * sRy[j]) / diag, OMEGA);
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t j = li; j < n; j += 256u) {
+        uint64_t at = levelSlot(d, l, I2((int32_t)(j % sx), (int32_t)(j / sx)));
+        *(float4 *)&FLD(d, at, 0) = make_float4(sPx[j], sPy[j], sRx[j], sRy[j]);
+    }
+}
+
+// prolong, prolongFine, prolonged (:2458-2512)
+__device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
+    i2 coarse = I2(c.x / 2, c.y / 2);
+    i2 s = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
+    v2 sum = V2(0.0f, 0.0f);
+    float weight = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        i2 off = k == 0 ? I2(0, 0) : (k == 1 ? I2(s.x, 0) : (k == 2 ? I2(0, s.y) : s));
+        float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
+        i2 at = I2(coarse.x + off.x, coarse.y + off.y);
+        float r = rhoAt(d, l + 1u, at);
+        if (r < 0.0f) continue;
+        weight += wk;
+        if (r > 0.0f) {
+            v2 p = potAt(d, l + 1u, at);
+            sum.x += wk * p.x;
+            sum.y += wk * p.y;
+        }
+    }
+    float dn = fmaxf(weight, 1e-6f);
+    return V2(sum.x / dn, sum.y / dn);
+}
+__global__ void __launch_bounds__(64) k_prolong(Dev d, uint32_t cycle, uint32_t l) {
+    if (converged(d, cycle) || activeCount(d) == 0u) return;
+    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
+    if (!inLevel(d, l, c)) return;
+    uint64_t at = levelSlot(d, l, c);
+    if (FLD(d, at + 1u, 0) > 0.0f) {
+        v2 e = prolonged(d, l, c);
+        FLD(d, at, 0) = FLD(d, at, 0) + e.x;
+        FLD(d, at, 1) = FLD(d, at, 1) + e.y;
+    }
+}
+__global__ void __launch_bounds__(64) k_prolongFine(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = 2u * cellIndex(d, c);
+        if (FLD(d, i, 2) > 0.0f) {
+            v2 e = prolonged(d, 0u, c);
+            FLD(d, i, 0) = FLD(d, i, 0) + e.x;
+            FLD(d, i + 1u, 0) = FLD(d, i + 1u, 0) + e.y;
+        }
+    }
+}
+
+// ------------------------------------------------ conjugate gradients
+__device__ __forceinline__ float applyA(const Dev &d, i2 c, float rho) {     // :2595-2613
+    float sum = 0.0f, diag = 0.0f, share = densityAt(d, c);
+    for (int k = 0; k < 4; k++) {
+        i2 n = beside(c, k);
+        float r = rhoAt(d, 0u, n);
+        if (r < 0.0f) continue;
+        float beta = fineBeta(rho, r, share, densityAt(d, n));
+        if (r > 0.0f) sum += beta * CGV(d, cellIndex(d, n), 2);
+        diag += beta;
+    }
+    diag += pinnedAt(d, cellIndex(d, c)) ? PINF : 0.0f;
+    return diag * CGV(d, cellIndex(d, c), 2) - sum;
+}
+
+__global__ void __launch_bounds__(64) k_cgStart(Dev d) {                     // :2665-2674
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SolveHead h = {};
+        *d.head = h;
+    }
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        *(float4 *)&CGV(d, i, 0) = make_float4(gridX(d, c), 0.0f, 0.0f, 0.0f);
+        CNT(d, i, G_REGION) = i;
+    }
+}
+__global__ void __launch_bounds__(64) k_cgJoin(Dev d) {                      // :2677-2689
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c) || gridZ(d, c) <= 0.0f) continue;
+        for (int k = 0; k < 2; k++) {
+            i2 n = k == 0 ? I2(c.x + 1, c.y) : I2(c.x, c.y + 1);
+            if (rhoAt(d, 0u, n) > 0.0f) ufJoin(d, cellIndex(d, c), cellIndex(d, n));
+        }
+    }
+}
+__global__ void __launch_bounds__(64) k_cgAnchor(Dev d) {                    // :2692-2705
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c) || gridZ(d, c) <= 0.0f) continue;
+        for (int k = 0; k < 4; k++) {
+            i2 n = beside(c, k);
+            if (inWorld(d, n) && rhoAt(d, 0u, n) == 0.0f) {
+                atomicOr(&CNT(d, ufFind(d, cellIndex(d, c)), G_REGION), REGION_OPEN);
+                break;
+            }
+        }
+    }
+}
+// cgResidual (:2708-2717), with the block partials of ||b||^2
+__global__ void __launch_bounds__(64) k_cgResidual(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 bk = activeBlock(d, wid);
+        i2 c = blockCell(bk, threadIdx.x);
+        double bb = 0.0;
+        if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
+            uint32_t i = cellIndex(d, c);
+            CGV(d, i, 1) = -residualAt(d, 0u, c).x;
+            double bi = -(double)FLD(d, 2u * i, 1);
+            bb = bi * bi;
+        }
+        bb = wave_tree_sum(bb);
+        if (threadIdx.x == 0) d.bpart[3ull * blockIndex(d, bk) + 2] = bb;
+    }
+}
+__global__ void __launch_bounds__(64) k_cgPrepare(Dev d) {                   // :2721-2732
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        FLD(d, 2u * i, 0) = 0.0f;
+        if (FLD(d, 2u * i, 2) > 0.0f) FLD(d, 2u * i, 1) = -CGV(d, i, 1);
+    }
+}
+
+// The active blocks' partials of sum `which`, added in block-index order by
+// one workgroup of 256 (:2645-2661).
+__device__ double sumBlocks(const Dev &d, uint32_t which, double *s) {
+    uint32_t li = threadIdx.x;
+    double acc = 0.0;
+    for (uint32_t k = li; k < d.B; k += 256u)
+        if (d.blocks[STATE_SLOT(d, k)] & ACTIVE) acc += d.bpart[3ull * k + which];
+    s[li] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (li < h) s[li] += s[li + h];
+        __syncthreads();
+    }
+    double r = s[0];
+    __syncthreads();
+    return r;
+}
+__global__ void __launch_bounds__(256) k_sumNorm(Dev d) {
+    __shared__ double s[256];
+    double bb = sumBlocks(d, 2, s);
+    if (threadIdx.x == 0) d.head->bb = bb;
+}
+
+// cgRho (:2736-2750)
+__global__ void __launch_bounds__(64) k_cgRho(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 bk = activeBlock(d, wid);
+        i2 c = blockCell(bk, threadIdx.x);
+        double v0 = 0.0, v1 = 0.0;
+        if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
+            uint32_t i = cellIndex(d, c);
+            double z = (double)FLD(d, 2u * i, 0);
+            v0 = (double)CGV(d, i, 1) * z;
+            v1 = (double)CGV(d, i, 3) * z;
+        }
+        v0 = wave_tree_sum(v0);
+        v1 = wave_tree_sum(v1);
+        if (threadIdx.x == 0) {
+            d.bpart[3ull * blockIndex(d, bk) + 0] = v0;
+            d.bpart[3ull * blockIndex(d, bk) + 1] = v1;
+        }
+    }
+}
+// cgBeta (:2752-2764): Notay's flexible beta, in FP64
+__global__ void __launch_bounds__(256) k_cgBeta(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    __shared__ double s[256];
+    double s0 = sumBlocks(d, 0, s), s1 = sumBlocks(d, 1, s);
+    if (threadIdx.x == 0) {
+        SolveHead *h = d.head;
+        double beta = (cycle == 0u || h->pq <= 0.0) ? 0.0 : -s1 / h->pq;
+        h->rz = s0;
+        h->beta = beta;
+        CGV(d, d.C + d.B, 0) = (float)s0;
+        CGV(d, d.C + d.B, 2) = (float)beta;
+    }
+}
+// cgDirection (:2767-2776)
+__global__ void __launch_bounds__(64) k_cgDirection(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    double beta = d.head->beta;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        float p = FLD(d, 2u * i, 2) > 0.0f ? (float)((double)FLD(d, 2u * i, 0) + beta * (double)CGV(d, i, 2)) : 0.0f;
+        CGV(d, i, 2) = p;
+    }
+}
+// cgApply (:2779-2795)
+__global__ void __launch_bounds__(64) k_cgApply(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 bk = activeBlock(d, wid);
+        i2 c = blockCell(bk, threadIdx.x);
+        double v = 0.0;
+        if (inWorld(d, c)) {
+            float rho = gridZ(d, c);
+            uint32_t i = cellIndex(d, c);
+            float q = rho > 0.0f ? applyA(d, c, rho) : 0.0f;
+            CGV(d, i, 3) = q;
+            v = (double)CGV(d, i, 2) * (double)q;
+        }
+        v = wave_tree_sum(v);
+        if (threadIdx.x == 0) d.bpart[3ull * blockIndex(d, bk) + 0] = v;
+    }
+}
+// cgAlpha (:2797-2814)
+__global__ void __launch_bounds__(256) k_cgAlpha(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    __shared__ double s[256];
+    double pq = sumBlocks(d, 0, s);
+    if (threadIdx.x == 0) {
+        SolveHead *h = d.head;
+        h->alpha = pq <= 0.0 ? 0.0 : h->rz / pq;
+        h->pq = pq;
+        CGV(d, d.C + d.B, 1) = (float)h->alpha;
+        CGV(d, d.C + d.B, 3) = (float)pq;
+    }
+}
+// cgUpdate (:2852-2880), with the block partials of ||r||^2
+__global__ void __launch_bounds__(64) k_cgUpdate(Dev d, uint32_t cycle) {
+    bool conv = converged(d, cycle);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + (cycle + 1u) % 3u)] = 0u;
+        if (!conv) atomicAdd(&d.blocks[COUNT_SLOT(d, COUNT_CYCLES)], 1u);
+    }
+    if (conv) return;
+    double alpha = d.head->alpha;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 bk = activeBlock(d, wid);
+        i2 c = blockCell(bk, threadIdx.x);
+        double rr = 0.0;
+        uint32_t most = 0;
+        if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
+            uint32_t i = cellIndex(d, c);
+            float4 v = *(const float4 *)&CGV(d, i, 0);
+            float rn = (float)((double)v.y - alpha * (double)v.w);
+            float xn = (float)((double)v.x + alpha * (double)v.z);
+            CGV(d, i, 0) = xn;
+            CGV(d, i, 1) = rn;
+            FLD(d, 2u * i, 0) = 0.0f;
+            FLD(d, 2u * i, 1) = -rn;
+            most = ubits(fabsf(rn));
+            rr = (double)rn * (double)rn;
+        }
+        rr = wave_tree_sum(rr);
+        most = wave_max_u32(most);
+        if (threadIdx.x == 0) {
+            d.bpart[3ull * blockIndex(d, bk) + 2] = rr;
+            atomicMax(&d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)], most);
+        }
+    }
+}
+// Whether the next iteration runs: max |r| under the tolerance (the
+// reference's converged(), :2223-2226), or ||r|| <= tol ||b||.
+__global__ void __launch_bounds__(256) k_cgCheck(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    __shared__ double s[256];
+    double rr = sumBlocks(d, 2, s);
+    if (threadIdx.x == 0) {
+        SolveHead *h = d.head;
+        h->rr = rr;
+        h->cycles += 1u;
+        bool conv;
+        if (d.tol_mode == 1u) {
+            double t = (double)d.tol;
+            conv = rr <= t * t * h->bb;
+        } else {
+            conv = fbits(d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)]) < d.tol;
+        }
+        __hip_atomic_store(&h->converged, conv ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+// cgFinish (:2884-2893)
+__global__ void __launch_bounds__(64) k_cgFinish(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        FLD(d, 2u * i, 0) = CGV(d, i, 0);
+        CNT(d, i, G_REGION) = 0u;
+    }
+}
+
+// ============================================================ after the solve
+// project (:2895-2942)
+__device__ void projectFace(const Dev &d, i2 at, bool isU) {
+    uint64_t f = faceSlot(d, faceIndex(d, at, isU));
+    i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+    float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
+    float beta = fineBeta(ga.z, gb.z, densityAt(d, a), densityAt(d, at));
+    if (beta > 0.0f) {
+        FLD(d, f, 0) = FLD(d, f, 0) - beta * (gb.x - ga.x);
+        FLD(d, f, 2) = PROJECTED;
+        FLD(d, f, 3) = wclamp(-beta * (driftAt(d, at) - driftAt(d, a)), -MAX_DRIFT, MAX_DRIFT);
+    }
+}
+__global__ void __launch_bounds__(64) k_project(Dev d) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicMax(&d.blocks[COUNT_SLOT(d, COUNT_MOST_CYCLES)], d.blocks[COUNT_SLOT(d, COUNT_CYCLES)]);
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        Face3 fs[4];
+        facesOf(d, c, fs);
+        for (int j = 0; j < 4; j++)
+            if (fs[j].isU >= 0) projectFace(d, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
+    }
+}
+// extrapolate, extrapolateFurther (:3220-3275).  A face written here holds z
+// under what its neighbours look for, so the pass reads the same in any
+// order.
+__global__ void __launch_bounds__(64) k_extrapolate(Dev d, int layer) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        Face3 fs[4];
+        facesOf(d, c, fs);
+        for (int j = 0; j < 4; j++) {
+            if (fs[j].isU < 0) continue;
+            bool isU = fs[j].isU == 1;
+            i2 at = I2(fs[j].x, fs[j].y);
+            uint64_t f = faceSlot(d, faceIndex(d, at, isU));
+            i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+            if (FLD(d, f, 2) != 1.0f || gridZ(d, a) != 0.0f || gridZ(d, at) != 0.0f) continue;
+            float reached = layer == 2 ? 1.5f : PROJECTED;
+            float sum = 0.0f, n = 0.0f;
+            for (int k = 0; k < 4; k++) {
+                i2 q = beside(at, k);
+                if (!faceInWorld(d, q, isU)) continue;
+                uint64_t g = faceSlot(d, faceIndex(d, q, isU));
+                float gz = FLD(d, g, 2);
+                if (gz >= reached) {
+                    sum += FLD(d, g, 0);
+                    n += 1.0f;
+                }
+            }
+            if (n > 0.0f) {
+                FLD(d, f, 0) = sum / n;
+                FLD(d, f, 2) = layer == 2 ? 1.25f : 1.5f;
+            }
+        }
+    }
+}
+
+// advect (:3468-3694)
+struct Sample { float now, before, weight, drift; v2 gradient; };
+__device__ Sample sampleAxis(const Dev &d, v2 s, bool isU) {
+    float fx0 = floorf(s.x), fy0 = floorf(s.y);
+    i2 b = I2(wi32(fx0), wi32(fy0));
+    v2 f = V2(s.x - fx0, s.y - fy0);
+    float ax = 0.0f, ay = 0.0f;
+    Sample r;
+    r.now = r.before = r.weight = r.drift = 0.0f;
+    int held = 0;
+    v2 g = V2(0.0f, 0.0f);
+    for (int j = 0; j < 4; j++) {
+        int ox = j & 1, oy = j >> 1;
+        float wgt = (ox == 1 ? f.x : 1.0f - f.x) * (oy == 1 ? f.y : 1.0f - f.y);
+        v2 dw = V2((ox == 1 ? 1.0f : -1.0f) * (oy == 1 ? f.y : 1.0f - f.y), (ox == 1 ? f.x : 1.0f - f.x) * (oy == 1 ? 1.0f : -1.0f));
+        i2 at = I2(b.x + ox, b.y + oy);
+        if (!faceInWorld(d, at, isU)) continue;
+        float4 fc = *(const float4 *)&FLD(d, faceSlot(d, faceIndex(d, at, isU)), 0);
+        r.drift += wgt * fc.w;
+        if (fc.z > 0.0f) {
+            ax += wgt * fc.x;
+            ay += wgt * fc.y;
+            r.weight += wgt;
+            g.x += dw.x * fc.x;
+            g.y += dw.y * fc.x;
+            held += fc.z == PROJECTED ? 1 : 0;
+        }
+    }
+    if (r.weight > 0.0f) {
+        r.now = ax / r.weight;
+        r.before = ay / r.weight;
+    }
+    r.gradient = held == 4 ? g : V2(0.0f, 0.0f);
+    return r;
+}
+__device__ __forceinline__ v2 clampSpeed(v2 v) {
+    float s = wlength(v);
+    return s > LIQUID_MAX_SPEED ? V2(v.x * (LIQUID_MAX_SPEED / s), v.y * (LIQUID_MAX_SPEED / s)) : v;
+}
+__device__ __forceinline__ bool escapesTo(const Dev &d, i2 q) { return inWorld(d, q) && !(gridZ(d, q) < 0.0f); }
+__device__ v2 escape(const Dev &d, v2 pos) {                                 // :3552-3594
+    if (!isObstaclePos(d, pos)) return pos;
+    i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
+    v2 best = pos;
+    float bestD = 1e9f;
+    for (int n = 0; n < 8; n++) {
+        int m = n + (n >= 4 ? 1 : 0);
+        i2 at = I2(c.x + m % 3 - 1, c.y + m / 3 - 1);
+        if (!escapesTo(d, at)) continue;
+        v2 inside = V2(wclamp(pos.x, (float)at.x + 0.02f, (float)at.x + 0.98f), wclamp(pos.y, (float)at.y + 0.02f, (float)at.y + 0.98f));
+        float dd = wlength(V2(pos.x - inside.x, pos.y - inside.y));
+        if (dd < bestD) { bestD = dd; best = inside; }
+    }
+    if (bestD < 1e9f) return best;
+    int32_t kt = (int32_t)stepIndex(I2(0, 1)) - 2;
+    i2 ways[4] = {I2(0, -1), I2(-1, 0), I2(1, 0), I2(0, 1)};
+    i2 way = turnStep(ways[0], kt);
+    int nearest = ESCAPE_REACH + 1;
+    for (int n = 0; n < 4; n++) {
+        i2 dir = turnStep(ways[n], kt);
+        for (int r = 2; r < nearest; r++) {
+            i2 q = I2(c.x + dir.x * r, c.y + dir.y * r);
+            if (!isObstaclePos(d, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) && escapesTo(d, q)) {
+                nearest = r;
+                way = dir;
+                break;
+            }
+        }
+    }
+    return inWorld(d, I2(c.x + way.x, c.y + way.y)) ? V2(pos.x + (float)way.x, pos.y + (float)way.y) : pos;
+}
+__global__ void __launch_bounds__(256) k_advect(Dev d) {
+    uint32_t cnt = ld(&d.parts[0]);
+    uint32_t n = cnt < d.cap ? cnt : d.cap;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
+        uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
+        v2 orig = V2(fbits(mo.x), fbits(mo.y));
+        v2 qvel = V2(fbits(mo.z), fbits(mo.w));
+        float temp = fbits(st.y);
+        v2 qpos = escape(d, orig);
+        Sample u = sampleAxis(d, V2(qpos.x - 0.0f, qpos.y - 0.5f), true);
+        Sample v = sampleAxis(d, V2(qpos.x - 0.5f, qpos.y - 0.0f), false);
+        v2 flow = V2(u.weight > 0.0f ? u.now : qvel.x, v.weight > 0.0f ? v.now : qvel.y);
+        v2 vel = clampSpeed(flow);
+        float4 aff = make_float4(AFFINE_KEEP * u.gradient.x, AFFINE_KEEP * u.gradient.y, AFFINE_KEEP * v.gradient.x, AFFINE_KEEP * v.gradient.y);
+        // slipOf (:3605-3614)
+        v2 slip = V2(0.0f, 0.0f);
+        {
+            uint32_t i = cellIndex(d, cellOf(d, qpos));
+            uint32_t nn = CNT(d, i, G_COUNT);
+            if (nn >= 2u) {
+                float mean = (float)CNT(d, i, G_MASS) / (FIXF * (float)nn);
+                float lighter = (mean - liquidDensity(temp)) / fmaxf(mean, 1e-3f);
+                slip = V2(-0.0f / WATER_WEIGHT * SLIP * lighter, -d.earthG / WATER_WEIGHT * SLIP * lighter);
+            }
+        }
+        v2 cs = clampSpeed(vel);
+        v2 travel = V2(cs.x + wclamp(u.drift + slip.x, -MAX_DRIFT, MAX_DRIFT), cs.y + wclamp(v.drift + slip.y, -MAX_DRIFT, MAX_DRIFT));
+        int32_t steps = wi32(ceilf(fmaxf(fabsf(travel.x), fabsf(travel.y)) * 2.0f));
+        steps = steps < 1 ? 1 : (steps > ADVECT_STEPS ? ADVECT_STEPS : steps);
+        v2 dd = V2(travel.x / (float)steps, travel.y / (float)steps);
+        v2 pos = qpos;
+        for (int32_t s = 0; s < steps; s++) {
+            v2 nx = V2(pos.x + dd.x, pos.y + 0.0f);
+            if (dd.x != 0.0f && isObstaclePos(d, nx)) {
+                if (vel.x * dd.x > 0.0f) vel.x = 0.0f;
+                dd.x = 0.0f;
+            } else {
+                pos = nx;
+            }
+            v2 ny = V2(pos.x + 0.0f, pos.y + dd.y);
+            if (dd.y != 0.0f && isObstaclePos(d, ny)) {
+                if (vel.y * dd.y > 0.0f) vel.y = 0.0f;
+                dd.y = 0.0f;
+            } else {
+                pos = ny;
+            }
+        }
+        if (!isActiveBlock(d, idiv8(cellOf(d, pos)))) {
+            pos = qpos;
+            if (!isActiveBlock(d, idiv8(cellOf(d, pos)))) pos = orig;
+        }
+        *(uint4 *)&MOTION(d, d.parts, k, 0) = make_uint4(ubits(pos.x), ubits(pos.y), ubits(vel.x), ubits(vel.y));
+        *(float4 *)&AFFINE(d, d.parts, k, 0) = aff;
+        uint32_t li = 2u * cellIndex(d, cellOf(d, qpos));
+        float lx = FLD(d, li, 0), lz = FLD(d, li, 2);
+        if (lz > 0.0f) {
+            bool falling = lx < 0.5f * lz * wlength(V2(0.0f, d.earthG));
+            atomicOr(&CNT(d, cellIndex(d, cellOf(d, pos)), G_CAME), falling ? CAME_FALLING : CAME_PRESSING);
+        }
+    }
+}
+
+// ================================================================ sort
+// sortCount (:3700-3708)
+__global__ void __launch_bounds__(256) k_sortCount(Dev d) {
+    uint32_t cnt = ld(&d.parts[0]);
+    uint32_t n = cnt < d.cap ? cnt : d.cap;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        v2 pos = V2(fbits(MOTION(d, d.parts, k, 0)), fbits(MOTION(d, d.parts, k, 1)));
+        STATE(d, d.parts, k, 3) = atomicAdd(&CNT(d, cellIndex(d, cellOf(d, pos)), G_SORT), 1u);
+    }
+}
+// scanBlocks (:3730-3753)
+__global__ void __launch_bounds__(64) k_scanBlocks(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        uint32_t n = inWorld(d, c) ? CNT(d, cellIndex(d, c), G_SORT) : 0u;
+        uint32_t incl = wave_incl_scan(n);
+        if (inWorld(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            CNT(d, i, G_START) = incl - n;
+            uint32_t came = CNT(d, i, G_CAME);
+            CNT(d, i, G_CAME) = 0u;
+            if (came == CAME_FALLING) FLD(d, 2u * i + 1u, 3) = (float)(wu32(FLD(d, 2u * i + 1u, 3)) | fallingBit(d, 0u));
+        }
+        if (threadIdx.x == 63) d.blocks[TOTAL_SLOT(d, wid)] = incl;
+    }
+}
+// scanTotals (:3757-3780): the blocks' runs' starts, in the active list's order
+__global__ void __launch_bounds__(256) k_scanTotals(Dev d) {
+    __shared__ uint32_t s[256];
+    uint32_t li = threadIdx.x, n = activeCount(d);
+    uint32_t per = (n + 255u) / 256u, lo = li * per < n ? li * per : n, hi = lo + per < n ? lo + per : n;
+    uint32_t sum = 0;
+    for (uint32_t j = lo; j < hi; j++) sum += d.blocks[TOTAL_SLOT(d, j)];
+    s[li] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256u; o <<= 1) {
+        uint32_t a = li >= o ? s[li - o] : 0u;
+        __syncthreads();
+        s[li] += a;
+        __syncthreads();
+    }
+    uint32_t run = s[li] - sum;
+    for (uint32_t j = lo; j < hi; j++) {
+        uint32_t t = d.blocks[TOTAL_SLOT(d, j)];
+        d.blocks[TOTAL_SLOT(d, j)] = run;
+        run += t;
+    }
+    if (li == 255u) {
+        d.sorted[0] = s[255];
+        d.blocks[COUNT_SLOT(d, COUNT_PARTICLES)] = s[255];
+        // the run totals (this library's): once a full step
+        uint32_t pc = d.parts[0];
+        d.totals[0] += 1ull;
+        d.totals[1] += (unsigned long long)d.blocks[COUNT_SLOT(d, COUNT_CYCLES)];
+        d.totals[2] += (unsigned long long)(pc < d.cap ? pc : d.cap);
+        d.totals[3] += 64ull * n;
+    }
+}
+// scanFinish (:3783-3790)
+__global__ void __launch_bounds__(64) k_scanFinish(Dev d) {
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (inWorld(d, c)) CNT(d, cellIndex(d, c), G_START) += d.blocks[TOTAL_SLOT(d, wid)];
+    }
+}
+// scatter (:3793-3808)
+__global__ void __launch_bounds__(256) k_scatter(Dev d) {
+    uint32_t cnt = ld(&d.parts[0]);
+    uint32_t n = cnt < d.cap ? cnt : d.cap;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
+        uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
+        uint4 af = *(const uint4 *)&AFFINE(d, d.parts, k, 0);
+        uint32_t i = cellIndex(d, cellOf(d, V2(fbits(mo.x), fbits(mo.y))));
+        uint32_t m = CNT(d, i, G_START) + st.w;
+        if (m < d.cap) {
+            *(uint4 *)&MOTION(d, d.sorted, m, 0) = mo;
+            *(uint4 *)&STATE(d, d.sorted, m, 0) = st;
+            *(uint4 *)&AFFINE(d, d.sorted, m, 0) = af;
+        }
+    }
+}
+
+// ================================================================ draw
+__device__ __forceinline__ bool vacated(const Dev &d, uint32_t j) {
+    return kindOf(d, wType(d.cells[4ull * j])) == KIND_LIQUID && CNT(d, j, G_SORT) == 0u;
+}
+__device__ uint32_t airTo(const Dev &d, i2 c, bool gas) {                    // :3898-3924
+    if (!gas && gridZ(d, c) > 0.0f) return WNONE;
+    int32_t kt = (int32_t)stepIndex(I2(0, 1)) - 2;
+    for (int n = 0; n < 8; n++) {
+        i2 o;
+        if (!gas) {
+            const int ax[8] = {0, -1, 1, 0, -1, 1, -1, 1}, ay[8] = {-1, 0, 0, 1, -1, -1, 1, 1};
+            o = I2(ax[n], ay[n]);
+        } else {
+            const int gx[8] = {0, -1, 1, -1, 1, -1, 1, 0}, gy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+            o = I2(gx[n], gy[n]);
+        }
+        o = turnStep(o, kt);
+        i2 at = I2(c.x + o.x, c.y + o.y);
+        if (!inWorld(d, at)) continue;
+        uint32_t j = cellIndex(d, at);
+        if (vacated(d, j) || (gas && wType(d.cells[4ull * j]) == E_AIR && CNT(d, j, G_SORT) == 0u)) return j;
+    }
+    return WNONE;
+}
+__global__ void __launch_bounds__(64) k_rasterClaims(Dev d) {               // :3928-3948
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c), to = WNONE;
+        uint32_t t = wType(d.cells[4ull * i]);
+        if (CNT(d, i, G_SORT) > 0u) {
+            if (t == E_AIR) to = airTo(d, c, false);
+            else if (kindOf(d, t) == KIND_GAS && gridZ(d, c) >= 0.0f) to = airTo(d, c, true);
+        }
+        CNT(d, i, G_AIR_TO) = to;
+        if (to != WNONE) atomicMax(&CNT(d, to, G_TAKEN), ~i);
+    }
+}
+__device__ __forceinline__ void setGasPair(const Dev &d, uint32_t i, uint32_t a, uint32_t b) {
+    uint64_t at = gasWord(d, i);
+    d.cells[at] = a;
+    d.cells[at + 1] = b;
+}
+__global__ void __launch_bounds__(64) k_raster(Dev d) {                      // :3983-4119
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c), n = CNT(d, i, G_SORT);
+        if (n == 0u) continue;
+        uint32_t start = CNT(d, i, G_START);
+        uint4 v = *(const uint4 *)&d.cells[4ull * i];
+        uint32_t t = wType(v.x);
+        bool drawn = false;
+        if (kindOf(d, t) == KIND_LIQUID) {
+            drawn = true;
+        } else if (t == E_AIR || (kindOf(d, t) == KIND_GAS && gridZ(d, c) >= 0.0f)) {
+            // moveOutOfTheWay (:3954-3973)
+            uint32_t to = CNT(d, i, G_AIR_TO);
+            if (to != WNONE && CNT(d, to, G_TAKEN) == ~i) {
+                if (wType(d.cells[4ull * to]) == E_AIR) setGasPair(d, to, 0u, 0u);
+                *(uint4 *)&d.cells[4ull * to] = v;
+                uint64_t gi = gasWord(d, i);
+                setGasPair(d, to, d.cells[gi], d.cells[gi + 1]);
+                setGasPair(d, i, 0u, 0u);
+                drawn = true;
+            } else if (t == E_AIR) {
+                setGasPair(d, i, 0u, 0u);
+                drawn = true;
+            }
+        }
+        if (!drawn) {
+            CNT(d, i, G_DEST) = DEST_KEEP;
+            continue;
+        }
+        uint32_t kinds[RASTER_KINDS] = {WNONE, WNONE, WNONE, WNONE}, counts[RASTER_KINDS] = {0, 0, 0, 0};
+        for (uint32_t k = start; k < start + n; k++) {
+            uint32_t pt = wType(STATE(d, d.sorted, k, 0)), at = RASTER_KINDS;
+            for (uint32_t m = 0; m < RASTER_KINDS; m++)
+                if (kinds[m] == pt) at = m;
+            if (at == RASTER_KINDS && pt < kinds[RASTER_KINDS - 1u]) {
+                at = RASTER_KINDS - 1u;
+                kinds[at] = pt;
+                counts[at] = 0u;
+                while (at > 0u && kinds[at - 1u] > kinds[at]) {
+                    uint32_t t0 = kinds[at - 1u], c0 = counts[at - 1u];
+                    kinds[at - 1u] = kinds[at];
+                    counts[at - 1u] = counts[at];
+                    kinds[at] = t0;
+                    counts[at] = c0;
+                    at--;
+                }
+            }
+            if (at < RASTER_KINDS) counts[at]++;
+        }
+        uint32_t most = kinds[0], votes = counts[0];
+        for (uint32_t m = 1; m < RASTER_KINDS; m++)
+            if (counts[m] > votes) { most = kinds[m]; votes = counts[m]; }
+        uint32_t word = 0u, lowest = WNONE, solute = 0u, soluteId = WNONE;
+        bool solutes = most == E_WATER;
+        for (uint32_t k = start; k < start + n; k++) {
+            uint32_t sx = STATE(d, d.sorted, k, 0), sz = STATE(d, d.sorted, k, 2);
+            if (wType(sx) == most && (lowest == WNONE || sz < lowest)) { word = sx; lowest = sz; }
+            if (solutes && wType(sx) == most && wAux(sx) != 0u && (soluteId == WNONE || sz < soluteId)) { solute = wAux(sx); soluteId = sz; }
+        }
+        if (solutes) word = packWord(wType(word), solute, wLife(word), wShade(word), wFlags(word));
+        float livesx = 0.0f, livesy = 0.0f;
+        u2 heat = {0, 0}, capOf = {0, 0}, velX = {0, 0}, velY = {0, 0};
+        for (uint32_t k = start; k < start + n; k++) {
+            uint4 st = *(const uint4 *)&STATE(d, d.sorted, k, 0);
+            uint32_t pt = wType(st.x);
+            if (pt == most && wAux(st.x) == wAux(word)) { livesx += (float)wLife(st.x); livesy += 1.0f; }
+            else if (pt == most && solutes && wAux(st.x) == 0u) { livesy += 1.0f; }
+            float hc = fmaxf(WATER_HEATCAP, 1e-6f);
+            heat = addExact(heat, hc * fbits(st.y));
+            capOf = addExact(capOf, hc);
+            velX = addExact(velX, fbits(MOTION(d, d.sorted, k, 2)));
+            velY = addExact(velY, fbits(MOTION(d, d.sorted, k, 3)));
+        }
+        Rng r;
+        seedRng(r, i, d.frame, d.seed ^ 0x7a57u);
+        uint32_t life = wu32(floorf(livesx / livesy + rnd(r)));
+        life = life < 255u ? life : 255u;
+        word = packWord(wType(word), wAux(word), life, wShade(word), wFlags(word));
+        float temp = exactValue(heat, EXACT_FINE) / exactValue(capOf, EXACT_FINE);
+        v2 vel = V2(exactValue(velX, EXACT_FINE) / (float)n, exactValue(velY, EXACT_FINE) / (float)n);
+        *(uint4 *)&d.cells[4ull * i] = make_uint4(wWith(word, FL_DRAWN), ubits(temp), pack2x16(vel), (i << 8) | (n < LIQUID_COUNT_MAX ? n : LIQUID_COUNT_MAX));
+        CNT(d, i, G_TYPE) = (wType(word) | (wAux(word) << 8)) | (life << 16);
+    }
+}
+__global__ void __launch_bounds__(64) k_unpaint(Dev d) {                     // :4163-4182
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        uint4 v = *(const uint4 *)&d.cells[4ull * i];
+        if (kindOf(d, wType(v.x)) == KIND_LIQUID && CNT(d, i, G_SORT) == 0u && CNT(d, i, G_TAKEN) == 0u) {
+            *(uint4 *)&d.cells[4ull * i] = make_uint4(E_AIR, ubits(effTemp(fbits(v.y))), 0u, 0u);
+            setGasPair(d, i, 0u, 0u);
+        }
+    }
+}

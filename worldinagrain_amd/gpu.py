@@ -1,0 +1,157 @@
+"""ctypes binding of libwig.so (include/wig.h): a world whose buffers live on
+the GPU and whose steps run the HIP kernels.  There is no CPU fallback: if
+the library is missing or no device is visible, this raises."""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwig.so")
+_lib = None
+
+
+class WigError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libwig.so (built by worldinagrain_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise WigError(f"libwig.so is not built ({LIB_PATH}); run `python -m worldinagrain_amd.build`")
+        L = ct.CDLL(LIB_PATH)
+        L.wig_create.restype = ct.c_void_p
+        L.wig_create.argtypes = [ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_uint32]
+        L.wig_destroy.argtypes = [ct.c_void_p]
+        L.wig_set_kinds.argtypes = [ct.c_void_p, ct.c_char_p]
+        L.wig_layout.argtypes = [ct.c_void_p, ct.POINTER(abi.WigLayoutInfo)]
+        L.wig_upload.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_size_t]
+        L.wig_download.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_size_t]
+        L.wig_device_ptr.argtypes = [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_void_p)]
+        L.wig_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.wig_fluid_step.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
+        L.wig_report.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint32)]
+        L.wig_solve_stats.argtypes = [ct.c_void_p, ct.POINTER(abi.WigSolveStats)]
+        L.wig_sync.argtypes = [ct.c_void_p]
+        L.wig_last_error.restype = ct.c_char_p
+        L.wig_device_ok.restype = ct.c_int
+        L.wig_run_totals.argtypes = [ct.c_void_p, ct.POINTER(abi.WigRunTotals)]
+        L.wig_time_kernel.argtypes = [ct.c_void_p, ct.c_int]
+        L.wig_kernel_time.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint32)]
+        L.wig_kernel_name.restype = ct.c_char_p
+        L.wig_kernel_name.argtypes = [ct.c_int]
+        L.wig_kernel_count.restype = ct.c_int
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status < 0:
+        raise WigError(f"wig error {status}: {lib().wig_last_error().decode()}")
+    return status
+
+
+_DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8}
+
+
+class GpuWorld:
+    """A world on the GPU (wig_create); the same interface as the tests' oracle world."""
+
+    def __init__(self, width, height, coarse_levels=0, kinds=None):
+        L = lib()
+        if not L.wig_device_ok():
+            raise WigError("no HIP device visible")
+        self._w = L.wig_create(width, height, coarse_levels, 0)
+        if not self._w:
+            raise WigError(f"wig_create failed: {L.wig_last_error().decode()}")
+        self.layout = abi.WigLayoutInfo()
+        _check(L.wig_layout(self._w, ct.byref(self.layout)))
+        if kinds is not None:
+            self.set_kinds(kinds)
+
+    def close(self):
+        if getattr(self, "_w", None):
+            lib().wig_destroy(self._w)
+            self._w = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_kinds(self, kinds):
+        k = np.ascontiguousarray(kinds, dtype=np.uint8)
+        assert k.shape == (256,)
+        _check(lib().wig_set_kinds(self._w, k.tobytes()))
+
+    def part_bytes(self, p):
+        return int(self.layout.part_bytes[p])
+
+    def upload(self, p, data):
+        buf = np.ascontiguousarray(data)
+        assert buf.nbytes == self.part_bytes(p), (p, buf.nbytes, self.part_bytes(p))
+        _check(lib().wig_upload(self._w, p, buf.ctypes.data, buf.nbytes))
+
+    def download(self, p):
+        dt = _DTYPES.get(p, np.uint32)
+        out = np.empty(self.part_bytes(p) // np.dtype(dt).itemsize, dtype=dt)
+        _check(lib().wig_download(self._w, p, out.ctypes.data, out.nbytes))
+        return out
+
+    part = download
+
+    def device_ptr(self, p):
+        ptr = ct.c_void_p()
+        _check(lib().wig_device_ptr(self._w, p, ct.byref(ptr)))
+        return ptr.value
+
+    def set_stream(self, stream_ptr):
+        _check(lib().wig_set_stream(self._w, ct.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def step(self, params, ctl):
+        return _check(lib().wig_fluid_step(self._w, ct.byref(params), ct.byref(ctl)))
+
+    def report(self):
+        stats = (ct.c_uint32 * abi.STATS)()
+        _check(lib().wig_report(self._w, stats))
+        return np.array(stats, dtype=np.uint32)
+
+    def solve_stats(self):
+        s = abi.WigSolveStats()
+        _check(lib().wig_solve_stats(self._w, ct.byref(s)))
+        return s
+
+    def sync(self):
+        _check(lib().wig_sync(self._w))
+
+    def run_totals(self):
+        t = abi.WigRunTotals()
+        _check(lib().wig_run_totals(self._w, ct.byref(t)))
+        return t
+
+    def time_kernel(self, name):
+        """Brackets every later launch of kernel `name` with HIP events (None stops)."""
+        if name is None:
+            _check(lib().wig_time_kernel(self._w, -1))
+            return
+        _check(lib().wig_time_kernel(self._w, kernel_id(name)))
+
+    def kernel_time(self):
+        """(ms summed, launches) of the timed kernel since time_kernel."""
+        ms, n = ct.c_double(), ct.c_uint32()
+        _check(lib().wig_kernel_time(self._w, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+
+def kernel_names():
+    L = lib()
+    return [L.wig_kernel_name(i).decode() for i in range(L.wig_kernel_count())]
+
+
+def kernel_id(name):
+    return kernel_names().index(name)
