@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--kernel", default=None, help="kernel for the roofline (default: the one taking most time)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (the HBM counters' calibration)")
     args = ap.parse_args()
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
@@ -101,6 +103,14 @@ def main():
     import torch
     from worldinagrain_amd import abi
     from worldinagrain_amd.gpu import GpuWorld, kernel_names
+
+    if args.copy_mib:
+        src = torch.empty(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     size = args.size
     cells, fluid_cells, kinds = build_world(size)

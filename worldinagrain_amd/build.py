@@ -21,6 +21,27 @@ FLAGS = [
 ]
 
 
+NAPI_SRC = os.path.join(HERE, "js", "wig_napi.c")
+NAPI_OUT = os.path.join(HERE, "js", "wig.node")
+
+
+def build_addon(verbose=False):
+    """The Node-API addon (wig.node) over libwig.so, if Node's headers are here."""
+    inc = "/usr/include/node"
+    if not os.path.exists(os.path.join(inc, "node_api.h")):
+        return None
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-I", inc, "-o", NAPI_OUT + ".tmp", NAPI_SRC,
+           "-L", HERE, "-lwig", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building wig.node failed")
+    os.replace(NAPI_OUT + ".tmp", NAPI_OUT)
+    return NAPI_OUT
+
+
 def build(verbose=False):
     cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
     if verbose:
@@ -30,6 +51,7 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libwig.so")
     os.replace(LIB + ".tmp", LIB)
+    build_addon(verbose)
     return LIB
 
 
