@@ -75,8 +75,15 @@ typedef struct WigStepCtl {
     uint32_t post_relax;     /* ... and after (POST_RELAX = 1) */
     uint32_t sweep_pairs;    /* red-black sweep pairs per relaxation dispatch (SWEEPS = 4) */
     uint32_t coarse_sweeps;  /* SOR sweep pairs on the coarsest level (COARSE_SWEEPS = 40) */
-    uint32_t reserved[4];
+    uint32_t first_chunk;    /* 0: encode fluid_cycles iterations, the reference's way.  Else this
+                                library's: encode this many, then wait for the solve's convergence flag
+                                and encode chunk more at a time until it converges or fluid_cycles run */
+    uint32_t chunk;
+    uint32_t reserved[2];
 } WigStepCtl;
+
+/* first_chunk: as many iterations as the world's last solve ran */
+#define WIG_CHUNK_AUTO 0xffffffffu
 
 #define WIG_MAX_CYCLES 256
 

@@ -152,6 +152,7 @@ typedef struct OWorld {
     /* captured at the solve's start for the FP64 oracle */
     float *cap_b;
     int capture;
+    uint32_t last_cycles;        /* iterations the last chunked solve ran */
 } OWorld;
 
 typedef struct { float x, y; } v2;
@@ -1386,13 +1387,16 @@ static void sumBlocksF(OWorld *w, float out[2]) {
     out[0] = s[0][0];
     out[1] = s[0][1];
 }
+/* FP64: the partials are kept by the block's place in the active list (in
+ * block-index order), and summed by 256 lanes, lane i taking places i,
+ * i + 256, ..., then halving. */
 static void sumBlocksD(OWorld *w, int which, double *out) {
     double s[256];
     double *bp = bpart(w);
+    uint32_t na = activeCount(w);
     for (int li = 0; li < 256; li++) {
         s[li] = 0.0;
-        for (uint32_t k = (uint32_t)li; k < w->B; k += 256u)
-            if (w->blocks[STATE_SLOT(k)] & ACTIVE) s[li] += bp[3ull * k + (uint32_t)which];
+        for (uint32_t j = (uint32_t)li; j < na; j += 256u) s[li] += bp[3ull * j + (uint32_t)which];
     }
     for (int h = 128; h > 0; h >>= 1)
         for (int li = 0; li < h; li++) s[li] += s[li + h];
@@ -1448,7 +1452,7 @@ static void k_cgResidual(OWorld *w) {
         }
         double out[3];
         sumBlockD(v, out);
-        bp[3ull * blockIndex(w, b) + 2] = out[2];
+        bp[3ull * wid + 2] = out[2];
     }
     double bb;
     sumBlocksD(w, 2, &bb);
@@ -1512,8 +1516,8 @@ static void k_cgRhoBeta(OWorld *w) {
             }
             double out[3];
             sumBlockD(v, out);
-            bp[3ull * blockIndex(w, b) + 0] = out[0];
-            bp[3ull * blockIndex(w, b) + 1] = out[1];
+            bp[3ull * wid + 0] = out[0];
+            bp[3ull * wid + 1] = out[1];
         }
         double s0, s1;
         sumBlocksD(w, 0, &s0);
@@ -1570,7 +1574,7 @@ static void k_cgApplyAlpha(OWorld *w) {
         } else {
             double out[3];
             sumBlockD(vd, out);
-            bp[3ull * blockIndex(w, b) + 0] = out[0];
+            bp[3ull * wid + 0] = out[0];
         }
     }
     if (w->dot_mode == 0) {
@@ -1626,7 +1630,7 @@ static void k_cgUpdate(OWorld *w) {
         }
         double out[3];
         sumBlockD(v, out);
-        bp[3ull * blockIndex(w, b) + 2] = out[2];
+        bp[3ull * wid + 2] = out[2];
     }
     uint32_t *slot = &w->blocks[COUNT_SLOT(COUNT_RESIDUAL + w->cycle % 3u)];
     if (most > *slot) *slot = most;
@@ -2098,7 +2102,18 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     k_cgResidual(w);
     k_cgPrepare(w);
     uint32_t deepest = w->L;
+    /* the C ABI's chunked encoding (WigStepCtl.first_chunk): the loop stops
+     * at a chunk's end once the solve has converged */
+    uint32_t first = ctl->first_chunk == WIG_CHUNK_AUTO ? (w->last_cycles > 1u ? w->last_cycles : 1u) : ctl->first_chunk;
+    uint32_t next_check = first ? first : WIG_MAX_CYCLES + 1u;
     for (uint32_t cycle = 0; cycle < ctl->fluid_cycles; cycle++) {
+        if (cycle == next_check) {
+            if (w->converged_flag) {
+                w->last_cycles = h->cycles;
+                break;
+            }
+            next_check = cycle + (ctl->chunk ? ctl->chunk : 1u);
+        }
         w->cycle = cycle;
         relaxPass(w, &shifted, ctl->pre_relax);
         for (uint32_t l = 1; l <= deepest; l++) {

@@ -37,7 +37,8 @@ class WigStepCtl(ct.Structure):
         ("full", ct.c_uint32), ("viscous", ct.c_uint32), ("surface", ct.c_uint32), ("bodies_run", ct.c_uint32),
         ("fluid_cycles", ct.c_uint32), ("shifted_first", ct.c_uint32), ("tol_mode", ct.c_uint32), ("tol", ct.c_float),
         ("pre_relax", ct.c_uint32), ("post_relax", ct.c_uint32), ("sweep_pairs", ct.c_uint32),
-        ("coarse_sweeps", ct.c_uint32), ("reserved", ct.c_uint32 * 4),
+        ("coarse_sweeps", ct.c_uint32), ("first_chunk", ct.c_uint32), ("chunk", ct.c_uint32),
+        ("reserved", ct.c_uint32 * 2),
     ]
 
 

@@ -15,6 +15,7 @@
 
 // the kernels (wig_kernels.hip)
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
+__global__ void k_setStep(StepParams *sp, uint32_t frame, uint32_t seed, float earthG);
 __global__ void k_sync(Dev d, int claims);
 __global__ void k_syncParticles(Dev d);
 __global__ void k_syncSeed(Dev d);
@@ -28,25 +29,21 @@ __global__ void k_classify(Dev d);
 __global__ void k_setFaces(Dev d);
 __global__ void k_divergence(Dev d);
 __global__ void k_relaxRims(Dev d, uint32_t cycle);
-__global__ void k_relax(Dev d, uint32_t cycle, int shift);
+__global__ void k_relax(Dev d, uint32_t cycle, int shift, int rimsOfDrift);
 __global__ void k_restrict(Dev d, uint32_t cycle, uint32_t l);
 __global__ void k_smooth(Dev d, uint32_t cycle, uint32_t l, int color);
-__global__ void k_coarsest(Dev d, uint32_t cycle, uint32_t l);
+__global__ void k_coarseLDS(Dev d, uint32_t cycle);
 __global__ void k_prolong(Dev d, uint32_t cycle, uint32_t l);
 __global__ void k_prolongFine(Dev d, uint32_t cycle);
 __global__ void k_cgStart(Dev d);
 __global__ void k_cgJoin(Dev d);
 __global__ void k_cgAnchor(Dev d);
 __global__ void k_cgResidual(Dev d);
-__global__ void k_sumNorm(Dev d);
 __global__ void k_cgPrepare(Dev d);
-__global__ void k_cgRho(Dev d, uint32_t cycle);
-__global__ void k_cgBeta(Dev d, uint32_t cycle);
+__global__ void k_cgRhoBeta(Dev d, uint32_t cycle);
 __global__ void k_cgDirection(Dev d, uint32_t cycle);
-__global__ void k_cgApply(Dev d, uint32_t cycle);
-__global__ void k_cgAlpha(Dev d, uint32_t cycle);
-__global__ void k_cgUpdate(Dev d, uint32_t cycle);
-__global__ void k_cgCheck(Dev d, uint32_t cycle);
+__global__ void k_cgApplyAlpha(Dev d, uint32_t cycle);
+__global__ void k_cgUpdateCheck(Dev d, uint32_t cycle);
 __global__ void k_cgFinish(Dev d);
 __global__ void k_project(Dev d);
 __global__ void k_extrapolate(Dev d, int layer);
@@ -79,10 +76,10 @@ static int fail(int code, const char *fmt, ...) {
     } while (0)
 
 // Kernel ids for wig_time_kernel, in launch order.
-#define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
+#define WIG_KERNELS(X) X(setStep) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
     X(sealJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgJoin) X(cgAnchor) \
-    X(cgResidual) X(sumNorm) X(cgPrepare) X(relaxRims) X(relax) X(restrict) X(smooth) X(coarsest) X(prolong) \
-    X(prolongFine) X(cgRho) X(cgBeta) X(cgDirection) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
+    X(cgResidual) X(cgPrepare) X(relaxRims) X(relax) X(restrict) X(smooth) X(coarseLDS) X(prolong) \
+    X(prolongFine) X(cgRhoBeta) X(cgDirection) X(cgApplyAlpha) X(cgUpdateCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
     X(scatter) X(rasterClaims) X(raster) X(unpaint)
 #define X_ENUM(n) KID_##n,
@@ -98,6 +95,11 @@ struct WigWorld {
     uint8_t *kind;               // 256 element kinds
     uint32_t *scratch;           // B words
     unsigned long long *totals;  // WigRunTotals on the device
+    StepParams *sp;              // the step's Params words
+    uint32_t *ticket;            // the fused reductions' counter
+    size_t lds_bytes;            // k_coarseLDS's dynamic LDS
+    uint32_t *flag_host;         // pinned: the solve's iterations and convergence flag, read between chunks
+    uint32_t last_cycles;        // iterations the last chunked solve ran
     hipStream_t own, stream;
     int device;
     int timed;                   // kernel id timed, -1 none
@@ -108,11 +110,15 @@ struct WigWorld {
 
 // A launch, bracketed by events when its kernel is the one being timed.
 template <typename... KArgs, typename... Args>
-static void launch(WigWorld *w, int id, void (*k)(KArgs...), dim3 grid, dim3 block, Args... args) {
+static void launch_lds(WigWorld *w, int id, void (*k)(KArgs...), dim3 grid, dim3 block, size_t lds, Args... args) {
     bool t = w->timed == id && w->ntimed < MAX_TIMED;
     if (t) hipEventRecord(w->ev[2 * w->ntimed], w->stream);
-    hipLaunchKernelGGL(k, grid, block, 0, w->stream, args...);
+    hipLaunchKernelGGL(k, grid, block, lds, w->stream, args...);
     if (t) hipEventRecord(w->ev[2 * w->ntimed + 1], w->stream), w->ntimed++;
+}
+template <typename... KArgs, typename... Args>
+static void launch(WigWorld *w, int id, void (*k)(KArgs...), dim3 grid, dim3 block, Args... args) {
+    launch_lds(w, id, k, grid, block, 0, args...);
 }
 
 static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
@@ -185,6 +191,11 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     WigWorld *w = new WigWorld();
     memset(&w->d, 0, sizeof w->d);
     size_world(w, width, height, coarse_levels);
+    if (w->d.lw[w->d.L] * w->d.lh[w->d.L] > (uint32_t)MG_COARSEST) {
+        fail(WIG_E_INVALID, "the coarsest level (%ux%u) holds more than %d cells", w->d.lw[w->d.L], w->d.lh[w->d.L], MG_COARSEST);
+        delete w;
+        return nullptr;
+    }
     if (hipGetDevice(&w->device) != hipSuccess || hipStreamCreate(&w->own) != hipSuccess) {
         fail(WIG_E_DEVICE, "no HIP device");
         delete w;
@@ -200,7 +211,9 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     }
     w->timed = -1;
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
-        || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess) {
+        || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess
+        || hipMalloc((void **)&w->sp, sizeof(StepParams)) != hipSuccess || hipMalloc((void **)&w->ticket, 64) != hipSuccess
+        || hipHostMalloc((void **)&w->flag_host, 64, hipHostMallocDefault) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
         wig_destroy(w);
         return nullptr;
@@ -209,8 +222,30 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         if (zero(w, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
     zero(w, w->kind, 256);
     zero(w, w->totals, sizeof(WigRunTotals));
+    zero(w, w->sp, sizeof(StepParams));
+    zero(w, w->ticket, 64);
     Dev &d = w->d;
     d.totals = w->totals;
+    d.sp = w->sp;
+    d.ticket = w->ticket;
+    // the coarse levels one workgroup's LDS holds (k_coarseLDS): from the
+    // first that fits with all below it, 20 bytes a cell
+    int lds_max = 65536;
+    hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, w->device);
+    d.lds_level = d.L;
+    for (uint32_t l = d.L; l >= 1; l--) {
+        size_t bytes = 0;
+        for (uint32_t k = l; k <= d.L; k++) bytes += 20ull * d.lw[k] * d.lh[k];
+        if (bytes > (size_t)lds_max) break;
+        d.lds_level = l;
+        w->lds_bytes = bytes;
+    }
+    if (w->lds_bytes == 0) w->lds_bytes = 20ull * d.lw[d.L] * d.lh[d.L];
+    if (w->lds_bytes > 65536 && hipFuncSetAttribute((const void *)k_coarseLDS, hipFuncAttributeMaxDynamicSharedMemorySize, (int)w->lds_bytes) != hipSuccess) {
+        fail(WIG_E_DEVICE, "k_coarseLDS cannot take %zu bytes of LDS", w->lds_bytes);
+        wig_destroy(w);
+        return nullptr;
+    }
     d.cells = (uint32_t *)w->part[WIG_PART_CELLS];
     d.sorted = (uint32_t *)w->part[WIG_PART_PARTICLES0];
     d.parts = (uint32_t *)w->part[WIG_PART_PARTICLES1];
@@ -238,6 +273,9 @@ int wig_destroy(WigWorld *w) {
     if (w->kind) hipFree(w->kind);
     if (w->scratch) hipFree(w->scratch);
     if (w->totals) hipFree(w->totals);
+    if (w->sp) hipFree(w->sp);
+    if (w->ticket) hipFree(w->ticket);
+    if (w->flag_host) hipHostFree(w->flag_host);
     if (w->ev_made)
         for (int i = 0; i < 2 * MAX_TIMED; i++) hipEventDestroy(w->ev[i]);
     if (w->own) hipStreamDestroy(w->own);
@@ -373,16 +411,14 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         return fail(WIG_E_INVALID, "viscosity, surface tension and bodies are not built (water only)");
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u)
         return fail(WIG_E_INVALID, "solver control out of range");
-    d.frame = P->frame;
-    d.seed = P->seed;
-    d.earthG = P->earthG;
     d.sweep_pairs = ctl->sweep_pairs;
     d.coarse_sweeps = ctl->coarse_sweeps;
     d.tol_mode = ctl->tol_mode;
     d.tol = ctl->tol;
     hipStream_t s = w->stream;
     dim3 world(d.bw, d.bh), b64(64), b256(256);
-    uint32_t bg = block_grid(d), pg = particle_grid(d);
+    uint32_t bg = block_grid(d), pg = particle_grid(d), bg4 = std::min<uint32_t>((d.B + 3u) / 4u, 1024u);
+    launch(w, KID_setStep, k_setStep, dim3(1), dim3(1), w->sp, P->frame, P->seed, P->earthG);
     if (!ctl->full) {
         launch(w, KID_sync, k_sync, world, b64, d, 0);
         launch(w, KID_compact, k_compact, dim3(1), dim3(1024), d);
@@ -405,27 +441,41 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     launch(w, KID_cgStart, k_cgStart, dim3(bg), b64, d);
     launch(w, KID_cgJoin, k_cgJoin, dim3(bg), b64, d);
     launch(w, KID_cgAnchor, k_cgAnchor, dim3(bg), b64, d);
-    launch(w, KID_cgResidual, k_cgResidual, dim3(bg), b64, d);
-    launch(w, KID_sumNorm, k_sumNorm, dim3(1), b256, d);
+    launch(w, KID_cgResidual, k_cgResidual, dim3(bg4), b256, d);
     launch(w, KID_cgPrepare, k_cgPrepare, dim3(bg), b64, d);
     bool shifted = ctl->shifted_first != 0;
-    uint32_t deepest = d.L;
+    // an iteration's first relaxations take their rims from the passes
+    // before them (cgPrepare or cgDirection, prolongFine); more of them
+    // snapshot their own (relaxRims)
+    bool fused = ctl->pre_relax >= 1 && ctl->post_relax >= 1;
+    uint32_t l0 = d.lds_level;
+    uint32_t first = ctl->first_chunk == WIG_CHUNK_AUTO ? std::max<uint32_t>(w->last_cycles, 1u) : ctl->first_chunk;
+    uint32_t next_check = first ? first : WIG_MAX_CYCLES + 1u;
     for (uint32_t cycle = 0; cycle < ctl->fluid_cycles; cycle++) {
+        if (cycle == next_check) {
+            // the chunk encoded so far has run: stop if the solve converged
+            HIPCHECK(hipMemcpyAsync(w->flag_host, &w->d.head->cycles, 8, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (w->flag_host[1]) {
+                w->last_cycles = w->flag_host[0];
+                break;
+            }
+            next_check = cycle + (ctl->chunk ? ctl->chunk : 1u);
+        }
         for (uint32_t k = 0; k < ctl->pre_relax; k++) {
-            launch(w, KID_relaxRims, k_relaxRims, dim3(bg), b64, d, cycle);
-            launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0);
+            bool own = !fused || k > 0;
+            if (own) launch(w, KID_relaxRims, k_relaxRims, dim3(bg), b64, d, cycle);
+            launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, own ? 0 : 1);
             shifted = !shifted;
         }
-        for (uint32_t l = 1; l <= deepest; l++) {
+        for (uint32_t l = 1; l < l0; l++) {
             dim3 lg(ceil_div(d.lw[l], 8u), ceil_div(d.lh[l], 8u));
             launch(w, KID_restrict, k_restrict, lg, b64, d, cycle, l);
-            if (l < deepest) {
-                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
-                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
-            }
+            launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
+            launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
         }
-        launch(w, KID_coarsest, k_coarsest, dim3(1), b256, d, cycle, deepest);
-        for (uint32_t l = deepest - 1; l >= 1; l--) {
+        launch_lds(w, KID_coarseLDS, k_coarseLDS, dim3(1), dim3(1024), w->lds_bytes, d, cycle);
+        for (uint32_t l = l0 - 1; l >= 1; l--) {
             dim3 lg(ceil_div(d.lw[l], 8u), ceil_div(d.lh[l], 8u));
             launch(w, KID_prolong, k_prolong, lg, b64, d, cycle, l);
             launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
@@ -433,17 +483,15 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         }
         launch(w, KID_prolongFine, k_prolongFine, dim3(bg), b64, d, cycle);
         for (uint32_t k = 0; k < ctl->post_relax; k++) {
-            launch(w, KID_relaxRims, k_relaxRims, dim3(bg), b64, d, cycle);
-            launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0);
+            bool own = !fused || k > 0;
+            if (own) launch(w, KID_relaxRims, k_relaxRims, dim3(bg), b64, d, cycle);
+            launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, 0);
             shifted = !shifted;
         }
-        launch(w, KID_cgRho, k_cgRho, dim3(bg), b64, d, cycle);
-        launch(w, KID_cgBeta, k_cgBeta, dim3(1), b256, d, cycle);
+        launch(w, KID_cgRhoBeta, k_cgRhoBeta, dim3(bg4), b256, d, cycle);
         launch(w, KID_cgDirection, k_cgDirection, dim3(bg), b64, d, cycle);
-        launch(w, KID_cgApply, k_cgApply, dim3(bg), b64, d, cycle);
-        launch(w, KID_cgAlpha, k_cgAlpha, dim3(1), b256, d, cycle);
-        launch(w, KID_cgUpdate, k_cgUpdate, dim3(bg), b64, d, cycle);
-        launch(w, KID_cgCheck, k_cgCheck, dim3(1), b256, d, cycle);
+        launch(w, KID_cgApplyAlpha, k_cgApplyAlpha, dim3(bg4), b256, d, cycle);
+        launch(w, KID_cgUpdateCheck, k_cgUpdateCheck, dim3(bg4), b256, d, cycle);
     }
     launch(w, KID_cgFinish, k_cgFinish, dim3(bg), b64, d);
     launch(w, KID_project, k_project, dim3(bg), b64, d);

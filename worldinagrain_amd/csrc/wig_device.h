@@ -96,6 +96,14 @@ struct SolveHead {
     uint32_t cycles, converged, pad[2];
 };
 
+// The step's Params words the liquid reads, in device memory so that a
+// captured step reads each replay's own (k_setStep writes them).
+struct StepParams {
+    uint32_t frame, seed;
+    float earthG;
+    uint32_t pad;
+};
+
 // Everything a kernel needs to know of the world and the step.
 struct Dev {
     int32_t W, H;
@@ -114,8 +122,9 @@ struct Dev {
     const uint8_t *kind;
     uint32_t *scratch;      // per-block flags and offsets (compact)
     unsigned long long *totals;  // steps, PCG iterations, particles, active cells (run totals)
-    uint32_t frame, seed;
-    float earthG;
+    StepParams *sp;
+    uint32_t *ticket;       // the last-workgroup counter of the fused reductions
+    uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
     uint32_t sweep_pairs, coarse_sweeps, tol_mode;
     float tol;
 };

@@ -46,8 +46,8 @@ __device__ __forceinline__ float fineBeta(float a, float b, float da, float db) 
 __device__ __forceinline__ bool isLiquidAt(const Dev &d, i2 c) {
     return inWorld(d, c) && kindOf(d, wType(d.cells[4ull * cellIndex(d, c)])) == KIND_LIQUID;
 }
-__device__ __forceinline__ uint32_t stepBit(const Dev &d, uint32_t back) { return 1u << ((d.frame + back) & 1u); }
-__device__ __forceinline__ uint32_t fallingBit(const Dev &d, uint32_t back) { return 4u << ((d.frame + back) & 1u); }
+__device__ __forceinline__ uint32_t stepBit(const Dev &d, uint32_t back) { return 1u << ((d.sp->frame + back) & 1u); }
+__device__ __forceinline__ uint32_t fallingBit(const Dev &d, uint32_t back) { return 4u << ((d.sp->frame + back) & 1u); }
 __device__ __forceinline__ bool fallingAt(const Dev &d, i2 q) {
     return inWorld(d, q) && (wu32(FLD(d, 2u * cellIndex(d, q) + 1u, 3)) & fallingBit(d, 1u)) != 0u;
 }
@@ -242,7 +242,7 @@ __device__ uint32_t sharedLife(const Dev &d, uint32_t life, uint32_t was, uint32
     if (now < was) l = l * (float)now / (float)was;
     else l = 255.0f - (255.0f - l) * (255.0f - (float)now) / (255.0f - (float)was);
     Rng r;
-    seedRng(r, id, d.frame, d.seed ^ 0x5a17u);
+    seedRng(r, id, d.sp->frame, d.sp->seed ^ 0x5a17u);
     uint32_t v = wu32(floorf(l + rnd(r)));
     return v < 255u ? v : 255u;
 }
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(256) k_syncParticles(Dev d) {
 }
 
 __device__ __forceinline__ uint32_t seedId(const Dev &d, i2 c, uint32_t j) {   // :835-841
-    uint32_t h = (((uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x) * 0x9e3779b9u) ^ (d.frame * 0x85ebca6bu) ^ (j * 0xc2b2ae35u);
+    uint32_t h = (((uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x) * 0x9e3779b9u) ^ (d.sp->frame * 0x85ebca6bu) ^ (j * 0xc2b2ae35u);
     h ^= h >> 16;
     h *= 0x7feb352du;
     h ^= h >> 15;
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(64) k_syncSeed(Dev d) {
             if (!wHas(v.x, FL_DRAWN)) n = LIQUID_PER_CELL;
             else if (home >= d.C || CNT(d, home, G_DEST) != (DEST_CELL | (DEST_SPAN - i))) n = want > 1u ? want : 1u;
             else if (want > CNT(d, home, G_SORT) && want < LIQUID_COUNT_MAX) n = want - CNT(d, home, G_SORT);
-            seedRng(r, i, d.frame, d.seed ^ 0x5eedu);
+            seedRng(r, i, d.sp->frame, d.sp->seed ^ 0x5eedu);
         }
     }
     uint32_t m = wave_take(d, n);
@@ -554,7 +554,7 @@ __device__ float columnWeight(const Dev &d, i2 p, float density) {
         over += 1;
         q.y -= 1;
     }
-    return density * d.earthG * ((float)over + 0.5f);
+    return density * d.sp->earthG * ((float)over + 0.5f);
 }
 __global__ void __launch_bounds__(64) k_classify(Dev d) {
     FOR_ACTIVE_BLOCKS(d, wid) {
@@ -657,7 +657,7 @@ __device__ void setFace(const Dev &d, i2 at, bool isU) {
     }
     float now = u;
     if (held > 0.0f) now += 0.0f;                        // the wind's drag: still air
-    if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) now += isU ? 0.0f : d.earthG;
+    if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) now += isU ? 0.0f : d.sp->earthG;
     *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(now, u, held, faceViscosity(ga, gb));
     FLD(d, viscRhsSlot(d, f), f % 4u) = now;
     if (FLD(d, viscLastSlot(d, f), f % 4u) != 0.0f) FLD(d, viscLastSlot(d, f), f % 4u) = 0.0f;
@@ -767,7 +767,7 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
     return j <= high ? I2(wide, j) : I2(j - high - 1, high);
 }
-__global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift) {
+__global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, int rimsOfDrift) {
     if (converged(d, cycle)) return;
     __shared__ float sPx[RIM * RIM], sPy[RIM * RIM], sRho[RIM * RIM], sShare[RIM * RIM];
     uint32_t li = threadIdx.x;
@@ -798,8 +798,11 @@ __global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift) 
                 py = FLD(d, 2u * ci + 1u, 0);
                 r = FLD(d, 2u * ci, 2);
                 if (r > 0.0f) {
-                    px = fbits(CNT(d, ci, G_RIM));
-                    py = fbits(CNT(d, ci, G_RIM + 1u));
+                    // the rim as relaxRims snapshot it; or, for an
+                    // iteration's first relaxation, the pressure's 0 and the
+                    // drift's snapshot after the last iteration
+                    px = rimsOfDrift ? 0.0f : fbits(CNT(d, ci, G_RIM));
+                    py = fbits(CNT(d, ci, rimsOfDrift ? G_AIR_TO : G_RIM + 1u));
                 }
             }
             sPx[k] = px;
@@ -899,50 +902,6 @@ __global__ void __launch_bounds__(64) k_smooth(Dev d, uint32_t cycle, uint32_t l
     FLD(d, at, 1) = wmix(e.y, sy, 1.0f);
 }
 
-// solve (:2389-2456): the coarsest level in one workgroup's LDS
-__global__ void __launch_bounds__(256) k_coarsest(Dev d, uint32_t cycle, uint32_t l) {
-    if (converged(d, cycle)) return;
-    __shared__ float sPx[MG_COARSEST], sPy[MG_COARSEST], sRx[MG_COARSEST], sRy[MG_COARSEST], sRho[MG_COARSEST];
-    uint32_t li = threadIdx.x, sx = d.lw[l], sy = d.lh[l];
-    uint32_t n = sx * sy < (uint32_t)MG_COARSEST ? sx * sy : (uint32_t)MG_COARSEST;
-    for (uint32_t j = li; j < n; j += 256u) {
-        uint64_t at = levelSlot(d, l, I2((int32_t)(j % sx), (int32_t)(j / sx)));
-        float4 e = *(const float4 *)&FLD(d, at, 0);
-        sPx[j] = e.x; sPy[j] = e.y; sRx[j] = e.z; sRy[j] = e.w;
-        sRho[j] = FLD(d, at + 1u, 0);
-    }
-    __syncthreads();
-    float h2 = (float)(1u << (2u * l));
-    for (uint32_t s = 0; s < 2u * d.coarse_sweeps; s++) {
-        for (uint32_t j = li; j < n; j += 256u) {
-            i2 c = I2((int32_t)(j % sx), (int32_t)(j / sx));
-            float r0 = sRho[j];
-            if ((uint32_t)((c.x + c.y) & 1) != (s & 1u) || r0 <= 0.0f) continue;
-            float sumx = 0.0f, sumy = 0.0f, diag = 0.0f;
-            for (int k = 0; k < 4; k++) {
-                i2 o = beside(c, k);
-                if (o.x < 0 || o.y < 0 || o.x >= (int32_t)sx || o.y >= (int32_t)sy) continue;
-                uint32_t oj = (uint32_t)o.y * sx + (uint32_t)o.x;
-                float r = sRho[oj];
-                if (r < 0.0f) continue;
-                float beta = faceBeta(r0, r);
-                sumx += beta * sPx[oj];
-                sumy += beta * sPy[oj];
-                diag += beta;
-            }
-            if (diag > 0.0f) {
-                sPx[j] = wmix(sPx[j], (sumx - h2 * sRx[j]) / diag, OMEGA);
-                sPy[j] = wmix(sPy[j], (sumy - h2 * sRy[j]) / diag, OMEGA);
-            }
-        }
-        __syncthreads();
-    }
-    for (uint32_t j = li; j < n; j += 256u) {
-        uint64_t at = levelSlot(d, l, I2((int32_t)(j % sx), (int32_t)(j / sx)));
-        *(float4 *)&FLD(d, at, 0) = make_float4(sPx[j], sPy[j], sRx[j], sRy[j]);
-    }
-}
-
 // prolong, prolongFine, prolonged (:2458-2512)
 __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
     i2 coarse = I2(c.x / 2, c.y / 2);
@@ -977,6 +936,7 @@ __global__ void __launch_bounds__(64) k_prolong(Dev d, uint32_t cycle, uint32_t 
     }
 }
 __global__ void __launch_bounds__(64) k_prolongFine(Dev d, uint32_t cycle) {
+    // and the post-relaxation's rims (relaxRims, :2096-2107), as it leaves them
     if (converged(d, cycle)) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
@@ -984,10 +944,196 @@ __global__ void __launch_bounds__(64) k_prolongFine(Dev d, uint32_t cycle) {
         uint32_t i = 2u * cellIndex(d, c);
         if (FLD(d, i, 2) > 0.0f) {
             v2 e = prolonged(d, 0u, c);
-            FLD(d, i, 0) = FLD(d, i, 0) + e.x;
-            FLD(d, i + 1u, 0) = FLD(d, i + 1u, 0) + e.y;
+            float px = FLD(d, i, 0) + e.x, py = FLD(d, i + 1u, 0) + e.y;
+            FLD(d, i, 0) = px;
+            FLD(d, i + 1u, 0) = py;
+            CNT(d, i / 2u, G_RIM) = ubits(px);
+            CNT(d, i / 2u, G_RIM + 1u) = ubits(py);
         }
     }
+}
+
+// The coarse levels from lds_level down, all in one workgroup's LDS:
+// restrictLevel into each (the first from the level above it in the field),
+// a red and a black sweep on each but the coarsest (smoothRed, smoothBlack),
+// the coarsest's SOR sweeps (solve), and back up, prolong and the two sweeps
+// again (fluid.wgsl:2317-2497), then the levels written back to the field.
+// The arithmetic is restrictLevel's, smoothAt's, solve's and prolong's, in
+// the same order; only where the values wait between passes differs.
+struct LdsLevel { float *px, *py, *rx, *ry, *rho; uint32_t w, h; };
+
+__device__ __forceinline__ float ldsRho(const LdsLevel &L, i2 c) {
+    if (c.x < 0 || c.y < 0 || c.x >= (int32_t)L.w || c.y >= (int32_t)L.h) return -1.0f;
+    return L.rho[(uint32_t)c.y * L.w + (uint32_t)c.x];
+}
+__device__ __forceinline__ Stencil ldsStencil(const LdsLevel &L, i2 c, float rho) {
+    Stencil s;
+    s.sum = V2(0.0f, 0.0f);
+    s.diag = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        i2 n = beside(c, k);
+        float r = ldsRho(L, n);
+        if (r < 0.0f) continue;
+        float beta = faceBeta(rho, r);
+        uint32_t j = (uint32_t)n.y * L.w + (uint32_t)n.x;
+        s.sum.x += beta * L.px[j];
+        s.sum.y += beta * L.py[j];
+        s.diag += beta;
+    }
+    return s;
+}
+__device__ __forceinline__ v2 ldsResidual(const LdsLevel &L, uint32_t l, i2 c) {
+    uint32_t j = (uint32_t)c.y * L.w + (uint32_t)c.x;
+    float rho = L.rho[j];
+    if (rho <= 0.0f) return V2(0.0f, 0.0f);
+    Stencil s = ldsStencil(L, c, rho);
+    float h2 = (float)(1u << (2u * l));
+    return V2(L.rx[j] - (s.sum.x - (s.diag + 0.0f) * L.px[j]) / h2, L.ry[j] - (s.sum.y - (s.diag + 0.0f) * L.py[j]) / h2);
+}
+__device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega, uint32_t limit) {
+    float h2 = (float)(1u << (2u * l));
+    uint32_t n = L.w * L.h < limit ? L.w * L.h : limit;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        i2 c = I2((int32_t)(j % L.w), (int32_t)(j / L.w));
+        if (((c.x + c.y) & 1) != color) continue;
+        float rho = L.rho[j];
+        if (rho <= 0.0f) continue;
+        Stencil s = ldsStencil(L, c, rho);
+        if (s.diag <= 0.0f) continue;
+        float sx = (s.sum.x - h2 * L.rx[j]) / s.diag, sy = (s.sum.y - h2 * L.ry[j]) / s.diag;
+        L.px[j] = wmix(L.px[j], sx, omega);
+        L.py[j] = wmix(L.py[j], sy, omega);
+    }
+    __syncthreads();
+}
+__global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    extern __shared__ float sm[];
+    const uint32_t l0 = activeCount(d) == 0u ? d.L : d.lds_level, L = d.L;
+    const bool idle = activeCount(d) == 0u;   // restrict and prolong pass over nothing; solve still sweeps
+    // level l's arrays in LDS, levels l0 .. L one after another
+    auto level = [&](uint32_t l) {
+        uint32_t off = 0;
+        for (uint32_t k = l0; k < l; k++) off += 5u * d.lw[k] * d.lh[k];
+        uint32_t n = d.lw[l] * d.lh[l];
+        LdsLevel T;
+        T.w = d.lw[l];
+        T.h = d.lh[l];
+        T.px = sm + off;
+        T.py = sm + off + n;
+        T.rx = sm + off + 2u * n;
+        T.ry = sm + off + 3u * n;
+        T.rho = sm + off + 4u * n;
+        return T;
+    };
+    if (idle) {
+        const LdsLevel T = level(L);
+        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
+            float4 e = *(const float4 *)&FLD(d, levelSlot(d, L, I2((int32_t)(j % T.w), (int32_t)(j / T.w))), 0);
+            T.px[j] = e.x; T.py[j] = e.y; T.rx[j] = e.z; T.ry[j] = e.w;
+            T.rho[j] = FLD(d, levelSlot(d, L, I2((int32_t)(j % T.w), (int32_t)(j / T.w))) + 1u, 0);
+        }
+        __syncthreads();
+    }
+    // down: restrictLevel (:2342-2377), then a red and a black sweep
+    for (uint32_t l = l0; l <= L && !idle; l++) {
+        const LdsLevel T = level(l);
+        int32_t span = 1 << l;
+        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
+            i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
+            float rho = 0.0f, rxv = 0.0f, ryv = 0.0f;
+            bool shortcut = false;
+            if (span <= WIG_BLOCK) {
+                i2 b = I2(c.x * span / WIG_BLOCK, c.y * span / WIG_BLOCK);
+                shortcut = inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u;
+            }
+            if (!shortcut) {
+                float fluid = 0.0f, sum = 0.0f;
+                bool air = false;
+                v2 r = V2(0.0f, 0.0f);
+                for (int q = 0; q < 4; q++) {
+                    i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
+                    float rk;
+                    if (l == l0) rk = rhoAt(d, l - 1u, k);
+                    else rk = ldsRho(level(l - 1u), k);
+                    if (rk == 0.0f) air = true;
+                    else if (rk > 0.0f) {
+                        fluid += 1.0f;
+                        sum += rk;
+                        v2 e = l == l0 ? residualAt(d, l - 1u, k) : ldsResidual(level(l - 1u), l - 1u, k);
+                        r.x += e.x;
+                        r.y += e.y;
+                    }
+                }
+                rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
+                rxv = rho > 0.0f ? r.x * 0.25f : 0.0f;
+                ryv = rho > 0.0f ? r.y * 0.25f : 0.0f;
+            }
+            T.px[j] = 0.0f;
+            T.py[j] = 0.0f;
+            T.rx[j] = rxv;
+            T.ry[j] = ryv;
+            T.rho[j] = rho;
+        }
+        __syncthreads();
+        if (l < L) {
+            ldsSmooth(T, l, 0, 1.0f, 0xffffffffu);
+            ldsSmooth(T, l, 1, 1.0f, 0xffffffffu);
+        }
+    }
+    // the coarsest (solve, :2407-2456): its first MG_COARSEST cells
+    {
+        const LdsLevel T = level(L);
+        for (uint32_t sw = 0; sw < 2u * d.coarse_sweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST);
+    }
+    // up: prolong (:2459-2497), then a red and a black sweep
+    for (int32_t l = (int32_t)L - 1; l >= (int32_t)l0 && !idle; l--) {
+        const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
+        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
+            if (T.rho[j] <= 0.0f) continue;
+            i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
+            i2 coarse = I2(c.x / 2, c.y / 2);
+            i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
+            v2 sum = V2(0.0f, 0.0f);
+            float weight = 0.0f;
+            for (int k = 0; k < 4; k++) {
+                i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
+                float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
+                i2 at = I2(coarse.x + o.x, coarse.y + o.y);
+                float r = ldsRho(U, at);
+                if (r < 0.0f) continue;
+                weight += wk;
+                if (r > 0.0f) {
+                    uint32_t ja = (uint32_t)at.y * U.w + (uint32_t)at.x;
+                    sum.x += wk * U.px[ja];
+                    sum.y += wk * U.py[ja];
+                }
+            }
+            float dn = fmaxf(weight, 1e-6f);
+            T.px[j] = T.px[j] + sum.x / dn;
+            T.py[j] = T.py[j] + sum.y / dn;
+        }
+        __syncthreads();
+        ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu);
+        ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu);
+    }
+    // the levels back into the field
+    for (uint32_t l = l0; l <= L; l++) {
+        const LdsLevel T = level(l);
+        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
+            uint64_t at = levelSlot(d, l, I2((int32_t)(j % T.w), (int32_t)(j / T.w)));
+            *(float4 *)&FLD(d, at, 0) = make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]);
+            if (!idle) *(float4 *)&FLD(d, at + 1u, 0) = make_float4(T.rho[j], 0.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
+// The step's Params words into device memory (k_setStep), before the step.
+__global__ void k_setStep(StepParams *sp, uint32_t frame, uint32_t seed, float earthG) {
+    sp->frame = frame;
+    sp->seed = seed;
+    sp->earthG = earthG;
+    sp->pad = 0u;
 }
 
 // ------------------------------------------------ conjugate gradients
@@ -1006,10 +1152,6 @@ __device__ __forceinline__ float applyA(const Dev &d, i2 c, float rho) {     // 
 }
 
 __global__ void __launch_bounds__(64) k_cgStart(Dev d) {                     // :2665-2674
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        SolveHead h = {};
-        *d.head = h;
-    }
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -1041,39 +1183,22 @@ __global__ void __launch_bounds__(64) k_cgAnchor(Dev d) {                    // 
         }
     }
 }
-// cgResidual (:2708-2717), with the block partials of ||b||^2
-__global__ void __launch_bounds__(64) k_cgResidual(Dev d) {
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 bk = activeBlock(d, wid);
-        i2 c = blockCell(bk, threadIdx.x);
-        double bb = 0.0;
-        if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
-            uint32_t i = cellIndex(d, c);
-            CGV(d, i, 1) = -residualAt(d, 0u, c).x;
-            double bi = -(double)FLD(d, 2u * i, 1);
-            bb = bi * bi;
-        }
-        bb = wave_tree_sum(bb);
-        if (threadIdx.x == 0) d.bpart[3ull * blockIndex(d, bk) + 2] = bb;
-    }
-}
-__global__ void __launch_bounds__(64) k_cgPrepare(Dev d) {                   // :2721-2732
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
-        if (!inWorld(d, c)) continue;
-        uint32_t i = cellIndex(d, c);
-        FLD(d, 2u * i, 0) = 0.0f;
-        if (FLD(d, 2u * i, 2) > 0.0f) FLD(d, 2u * i, 1) = -CGV(d, i, 1);
-    }
-}
+// ------------------------------------------------ FP64 sums, fused
+// A pass's block partials are kept by the block's place in the active list
+// (which is in block-index order) and summed by one workgroup of 256, lane i
+// taking places i, i + 256, ..., then halving: the same order every run.
+// The sum is taken by whichever workgroup of the pass finishes last (a
+// ticket), so the pass and its sum are one launch.
+#define WAVES 4   // wavefronts (blocks of cells) a workgroup of 256
 
-// The active blocks' partials of sum `which`, added in block-index order by
-// one workgroup of 256 (:2645-2661).
-__device__ double sumBlocks(const Dev &d, uint32_t which, double *s) {
-    uint32_t li = threadIdx.x;
+__device__ __forceinline__ double ld_f64(const double *p) {
+    unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)u);
+}
+__device__ double sumPlaces(const Dev &d, uint32_t which, double *s) {
+    uint32_t li = threadIdx.x, na = activeCount(d);
     double acc = 0.0;
-    for (uint32_t k = li; k < d.B; k += 256u)
-        if (d.blocks[STATE_SLOT(d, k)] & ACTIVE) acc += d.bpart[3ull * k + which];
+    for (uint32_t j = li; j < na; j += 256u) acc += ld_f64(&d.bpart[3ull * j + which]);
     s[li] = acc;
     __syncthreads();
     for (uint32_t h = 128; h > 0; h >>= 1) {
@@ -1084,18 +1209,69 @@ __device__ double sumBlocks(const Dev &d, uint32_t which, double *s) {
     __syncthreads();
     return r;
 }
-__global__ void __launch_bounds__(256) k_sumNorm(Dev d) {
-    __shared__ double s[256];
-    double bb = sumBlocks(d, 2, s);
-    if (threadIdx.x == 0) d.head->bb = bb;
+// After each wavefront wrote its partials: true in the workgroup that
+// finished last (which then takes the sums and resets the ticket).
+__device__ __forceinline__ bool lastGroup(const Dev &d) {
+    __shared__ uint32_t isLast;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) isLast = atomicAdd(d.ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+    __syncthreads();
+    if (!isLast) return false;
+    __threadfence();
+    return true;
 }
+#define FOR_ACTIVE_BLOCKS4(d, wid) \
+    for (uint32_t wid = blockIdx.x * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += gridDim.x * WAVES)
+#define LANE (threadIdx.x % 64u)
 
-// cgRho (:2736-2750)
-__global__ void __launch_bounds__(64) k_cgRho(Dev d, uint32_t cycle) {
-    if (converged(d, cycle)) return;
+// cgResidual (:2708-2717), and ||b||^2 for the relative stop
+__global__ void __launch_bounds__(256) k_cgResidual(Dev d) {
+    __shared__ double s[256];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SolveHead h = {};
+        *d.head = h;
+    }
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
+        double bb = 0.0;
+        if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
+            uint32_t i = cellIndex(d, c);
+            CGV(d, i, 1) = -residualAt(d, 0u, c).x;
+            double bi = -(double)FLD(d, 2u * i, 1);
+            bb = bi * bi;
+        }
+        bb = wave_tree_sum(bb);
+        if (LANE == 0) d.bpart[3ull * wid + 2] = bb;
+    }
+    if (!lastGroup(d)) return;
+    double bb = sumPlaces(d, 2, s);
+    if (threadIdx.x == 0) {
+        d.head->bb = bb;
+        *d.ticket = 0u;
+    }
+}
+// cgPrepare (:2721-2732).  The first iteration's first relaxation takes its
+// rims' drift from G_AIR_TO (free until rasterClaims writes it): the
+// pressure's rim is 0 there, as cgPrepare and cgUpdate leave it.
+__global__ void __launch_bounds__(64) k_cgPrepare(Dev d) {
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 bk = activeBlock(d, wid);
-        i2 c = blockCell(bk, threadIdx.x);
+        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        uint32_t i = cellIndex(d, c);
+        FLD(d, 2u * i, 0) = 0.0f;
+        if (FLD(d, 2u * i, 2) > 0.0f) {
+            FLD(d, 2u * i, 1) = -CGV(d, i, 1);
+            CNT(d, i, G_AIR_TO) = ubits(FLD(d, 2u * i + 1u, 0));
+        }
+    }
+}
+// cgRho + cgBeta (:2736-2764): r.z and z.Ap, and Notay's flexible beta in FP64
+__global__ void __launch_bounds__(256) k_cgRhoBeta(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    __shared__ double s[256];
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
         double v0 = 0.0, v1 = 0.0;
         if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
             uint32_t i = cellIndex(d, c);
@@ -1105,17 +1281,13 @@ __global__ void __launch_bounds__(64) k_cgRho(Dev d, uint32_t cycle) {
         }
         v0 = wave_tree_sum(v0);
         v1 = wave_tree_sum(v1);
-        if (threadIdx.x == 0) {
-            d.bpart[3ull * blockIndex(d, bk) + 0] = v0;
-            d.bpart[3ull * blockIndex(d, bk) + 1] = v1;
+        if (LANE == 0) {
+            d.bpart[3ull * wid + 0] = v0;
+            d.bpart[3ull * wid + 1] = v1;
         }
     }
-}
-// cgBeta (:2752-2764): Notay's flexible beta, in FP64
-__global__ void __launch_bounds__(256) k_cgBeta(Dev d, uint32_t cycle) {
-    if (converged(d, cycle)) return;
-    __shared__ double s[256];
-    double s0 = sumBlocks(d, 0, s), s1 = sumBlocks(d, 1, s);
+    if (!lastGroup(d)) return;
+    double s0 = sumPlaces(d, 0, s), s1 = sumPlaces(d, 1, s);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
         double beta = (cycle == 0u || h->pq <= 0.0) ? 0.0 : -s1 / h->pq;
@@ -1123,26 +1295,30 @@ __global__ void __launch_bounds__(256) k_cgBeta(Dev d, uint32_t cycle) {
         h->beta = beta;
         CGV(d, d.C + d.B, 0) = (float)s0;
         CGV(d, d.C + d.B, 2) = (float)beta;
+        *d.ticket = 0u;
     }
 }
-// cgDirection (:2767-2776)
+// cgDirection (:2767-2776); and the drift as the iteration's relaxations
+// left it, the next iteration's first relaxation's rims (G_AIR_TO)
 __global__ void __launch_bounds__(64) k_cgDirection(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
-    double beta = d.head->beta;
+    double beta = ld_f64(&d.head->beta);
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
-        float p = FLD(d, 2u * i, 2) > 0.0f ? (float)((double)FLD(d, 2u * i, 0) + beta * (double)CGV(d, i, 2)) : 0.0f;
+        bool fluid = FLD(d, 2u * i, 2) > 0.0f;
+        float p = fluid ? (float)((double)FLD(d, 2u * i, 0) + beta * (double)CGV(d, i, 2)) : 0.0f;
         CGV(d, i, 2) = p;
+        if (fluid) CNT(d, i, G_AIR_TO) = ubits(FLD(d, 2u * i + 1u, 0));
     }
 }
-// cgApply (:2779-2795)
-__global__ void __launch_bounds__(64) k_cgApply(Dev d, uint32_t cycle) {
+// cgApply + cgAlpha (:2779-2814)
+__global__ void __launch_bounds__(256) k_cgApplyAlpha(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 bk = activeBlock(d, wid);
-        i2 c = blockCell(bk, threadIdx.x);
+    __shared__ double s[256];
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
         double v = 0.0;
         if (inWorld(d, c)) {
             float rho = gridZ(d, c);
@@ -1152,34 +1328,34 @@ __global__ void __launch_bounds__(64) k_cgApply(Dev d, uint32_t cycle) {
             v = (double)CGV(d, i, 2) * (double)q;
         }
         v = wave_tree_sum(v);
-        if (threadIdx.x == 0) d.bpart[3ull * blockIndex(d, bk) + 0] = v;
+        if (LANE == 0) d.bpart[3ull * wid + 0] = v;
     }
-}
-// cgAlpha (:2797-2814)
-__global__ void __launch_bounds__(256) k_cgAlpha(Dev d, uint32_t cycle) {
-    if (converged(d, cycle)) return;
-    __shared__ double s[256];
-    double pq = sumBlocks(d, 0, s);
+    if (!lastGroup(d)) return;
+    double pq = sumPlaces(d, 0, s);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
         h->alpha = pq <= 0.0 ? 0.0 : h->rz / pq;
         h->pq = pq;
         CGV(d, d.C + d.B, 1) = (float)h->alpha;
         CGV(d, d.C + d.B, 3) = (float)pq;
+        *d.ticket = 0u;
     }
 }
-// cgUpdate (:2852-2880), with the block partials of ||r||^2
-__global__ void __launch_bounds__(64) k_cgUpdate(Dev d, uint32_t cycle) {
+// cgUpdate (:2852-2880), and whether the next
+// iteration runs: max |r| under the tolerance (the reference's converged(),
+// :2223-2226), or ||r|| <= tol ||b||
+__global__ void __launch_bounds__(256) k_cgUpdateCheck(Dev d, uint32_t cycle) {
     bool conv = converged(d, cycle);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + (cycle + 1u) % 3u)] = 0u;
         if (!conv) atomicAdd(&d.blocks[COUNT_SLOT(d, COUNT_CYCLES)], 1u);
     }
     if (conv) return;
-    double alpha = d.head->alpha;
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 bk = activeBlock(d, wid);
-        i2 c = blockCell(bk, threadIdx.x);
+    __shared__ double s[256];
+    double alpha = ld_f64(&d.head->alpha);
+    uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
         double rr = 0.0;
         uint32_t most = 0;
         if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
@@ -1196,30 +1372,26 @@ __global__ void __launch_bounds__(64) k_cgUpdate(Dev d, uint32_t cycle) {
         }
         rr = wave_tree_sum(rr);
         most = wave_max_u32(most);
-        if (threadIdx.x == 0) {
-            d.bpart[3ull * blockIndex(d, bk) + 2] = rr;
-            atomicMax(&d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)], most);
+        if (LANE == 0) {
+            d.bpart[3ull * wid + 2] = rr;
+            atomicMax(slot, most);
         }
     }
-}
-// Whether the next iteration runs: max |r| under the tolerance (the
-// reference's converged(), :2223-2226), or ||r|| <= tol ||b||.
-__global__ void __launch_bounds__(256) k_cgCheck(Dev d, uint32_t cycle) {
-    if (converged(d, cycle)) return;
-    __shared__ double s[256];
-    double rr = sumBlocks(d, 2, s);
+    if (!lastGroup(d)) return;
+    double rr = sumPlaces(d, 2, s);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
         h->rr = rr;
         h->cycles += 1u;
-        bool conv;
+        bool c2;
         if (d.tol_mode == 1u) {
             double t = (double)d.tol;
-            conv = rr <= t * t * h->bb;
+            c2 = rr <= t * t * h->bb;
         } else {
-            conv = fbits(d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)]) < d.tol;
+            c2 = fbits(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < d.tol;
         }
-        __hip_atomic_store(&h->converged, conv ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&h->converged, c2 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *d.ticket = 0u;
     }
 }
 // cgFinish (:2884-2893)
@@ -1388,7 +1560,7 @@ __global__ void __launch_bounds__(256) k_advect(Dev d) {
             if (nn >= 2u) {
                 float mean = (float)CNT(d, i, G_MASS) / (FIXF * (float)nn);
                 float lighter = (mean - liquidDensity(temp)) / fmaxf(mean, 1e-3f);
-                slip = V2(-0.0f / WATER_WEIGHT * SLIP * lighter, -d.earthG / WATER_WEIGHT * SLIP * lighter);
+                slip = V2(-0.0f / WATER_WEIGHT * SLIP * lighter, -d.sp->earthG / WATER_WEIGHT * SLIP * lighter);
             }
         }
         v2 cs = clampSpeed(vel);
@@ -1422,7 +1594,7 @@ __global__ void __launch_bounds__(256) k_advect(Dev d) {
         uint32_t li = 2u * cellIndex(d, cellOf(d, qpos));
         float lx = FLD(d, li, 0), lz = FLD(d, li, 2);
         if (lz > 0.0f) {
-            bool falling = lx < 0.5f * lz * wlength(V2(0.0f, d.earthG));
+            bool falling = lx < 0.5f * lz * wlength(V2(0.0f, d.sp->earthG));
             atomicOr(&CNT(d, cellIndex(d, cellOf(d, pos)), G_CAME), falling ? CAME_FALLING : CAME_PRESSING);
         }
     }
@@ -1630,7 +1802,7 @@ __global__ void __launch_bounds__(64) k_raster(Dev d) {                      // 
             velY = addExact(velY, fbits(MOTION(d, d.sorted, k, 3)));
         }
         Rng r;
-        seedRng(r, i, d.frame, d.seed ^ 0x7a57u);
+        seedRng(r, i, d.sp->frame, d.sp->seed ^ 0x7a57u);
         uint32_t life = wu32(floorf(livesx / livesy + rnd(r)));
         life = life < 255u ? life : 255u;
         word = packWord(wType(word), wAux(word), life, wShade(word), wFlags(word));

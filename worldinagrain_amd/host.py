@@ -32,6 +32,11 @@ class SolverSettings:
     post_relax: int = 1
     sweep_pairs: int = 4
     coarse_sweeps: int = 40
+    # this build's: iterations encoded before the first convergence check (0:
+    # all fluid_cycles, the reference's way; -1: as many as the last step
+    # ran), and then how many more at a time
+    first_chunk: int = 0
+    chunk: int = 2
 
     @staticmethod
     def reference():
@@ -41,7 +46,7 @@ class SolverSettings:
     def relative(tol, max_cycles=100, sweep_pairs=2):
         """A solve to ||r|| <= tol ||b|| (the benchmark configurations)."""
         return SolverSettings(tol_mode=abi.WIG_TOL_REL_L2, tol=tol, max_cycles=max_cycles, adaptive_cycles=False,
-                              sweep_pairs=sweep_pairs)
+                              sweep_pairs=sweep_pairs, first_chunk=-1)
 
 
 def mulberry32_next(state):
@@ -70,6 +75,7 @@ class Simulation:
         self.fluid_stats = {"active": 1, "viscous": 0}
         self.fluid_cycles = self.solver.max_cycles
         self.last_stats = None
+        self.last_cycles = 0
 
     # Simulation.seed / random (js/sim.js:1386-1397)
     def seed(self, n):
@@ -101,6 +107,11 @@ class Simulation:
         c.tol = s.tol
         c.pre_relax, c.post_relax = s.pre_relax, s.post_relax
         c.sweep_pairs, c.coarse_sweeps = s.sweep_pairs, s.coarse_sweeps
+        if s.first_chunk < 0:
+            c.first_chunk = 0xFFFFFFFF          # WIG_CHUNK_AUTO: the world's last solve's count
+            c.chunk = s.chunk
+        else:
+            c.first_chunk, c.chunk = s.first_chunk, s.chunk
         return c
 
     def step(self, read_report=True):
@@ -119,6 +130,7 @@ class Simulation:
         self.last_stats = stats
         self.fluid_stats = {"active": int(stats[abi.STAT_ACTIVE]) + int(stats[abi.STAT_PARTICLES]),
                             "viscous": int(stats[abi.STAT_VISCOUS])}
+        self.last_cycles = int(stats[abi.STAT_CYCLES])
         most = int(stats[abi.STAT_MOST_CYCLES])
         cap = self.solver.max_cycles
         self.fluid_cycles = cap if most >= self.fluid_cycles else min(most + FLUID_CYCLES_SPARE, cap)

@@ -18,6 +18,7 @@ const PART = { cells: 0, particles0: 1, particles1: 2, counters: 3, field: 4, bl
 const STAT = { ACTIVE: 0, RETIRED: 1, VISCOUS: 2, PARTICLES: 3, RESIDUAL: 4, CYCLES: 7, MOST_CYCLES: 8 };
 const TOL_ABS_MAX = 0;
 const TOL_REL_L2 = 1;
+const CHUNK_AUTO = 0xffffffff;
 const KIND = { SOLID: 1, POWDER: 2, LIQUID: 3, GAS: 4 };
 
 // elems[t].kind for the ids of js/elements.js's table: AIR 0, WALL 1,
@@ -54,6 +55,11 @@ class LiquidSimulation {
             postRelax: s.postRelax || 1,
             sweepPairs: s.sweepPairs || 4,
             coarseSweeps: s.coarseSweeps || 40,
+            // this build's chunked encoding: 0 encodes maxCycles iterations as
+            // the reference does; CHUNK_AUTO as many as the last solve ran,
+            // then `chunk` more at a time until it converges
+            firstChunk: s.firstChunk || 0,
+            chunk: s.chunk || 2,
         };
         this.shiftedFirst = false;
         this.fluidStats = { active: 1, viscous: 0 };
@@ -109,6 +115,8 @@ class LiquidSimulation {
         u[9] = this.solver.postRelax;
         u[10] = this.solver.sweepPairs;
         u[11] = this.solver.coarseSweeps;
+        u[12] = this.solver.firstChunk >>> 0;
+        u[13] = this.solver.chunk;
         return buf;
     }
 
@@ -151,4 +159,4 @@ class LiquidSimulation {
     }
 }
 
-module.exports = { LiquidSimulation, PART, STAT, KIND, TOL_ABS_MAX, TOL_REL_L2, GRAVITY, elementKinds, deviceOk: addon.deviceOk };
+module.exports = { LiquidSimulation, PART, STAT, KIND, TOL_ABS_MAX, TOL_REL_L2, CHUNK_AUTO, GRAVITY, elementKinds, deviceOk: addon.deviceOk };
