@@ -23,7 +23,8 @@ __global__ void k_compact(Dev d);
 __global__ void k_retire(Dev d);
 __global__ void k_clear(Dev d);
 __global__ void k_gather(Dev d);
-__global__ void k_sealJoin(Dev d);
+__global__ void k_localJoin(Dev d, int mode);
+__global__ void k_edgeJoin(Dev d, int mode);
 __global__ void k_sealAnchor(Dev d);
 __global__ void k_classify(Dev d);
 __global__ void k_setFaces(Dev d);
@@ -36,7 +37,6 @@ __global__ void k_coarseLDS(Dev d, uint32_t cycle);
 __global__ void k_prolong(Dev d, uint32_t cycle, uint32_t l);
 __global__ void k_prolongFine(Dev d, uint32_t cycle);
 __global__ void k_cgStart(Dev d);
-__global__ void k_cgJoin(Dev d);
 __global__ void k_cgAnchor(Dev d);
 __global__ void k_cgResidual(Dev d);
 __global__ void k_cgPrepare(Dev d);
@@ -77,7 +77,7 @@ static int fail(int code, const char *fmt, ...) {
 
 // Kernel ids for wig_time_kernel, in launch order.
 #define WIG_KERNELS(X) X(setStep) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
-    X(sealJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgJoin) X(cgAnchor) \
+    X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
     X(cgResidual) X(cgPrepare) X(relaxRims) X(relax) X(restrict) X(smooth) X(coarseLDS) X(prolong) \
     X(prolongFine) X(cgRhoBeta) X(cgDirection) X(cgApplyAlpha) X(cgUpdateCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
@@ -433,13 +433,15 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     launch(w, KID_retire, k_retire, dim3(bg), b64, d);
     launch(w, KID_clear, k_clear, dim3(bg), b64, d);
     launch(w, KID_gather, k_gather, dim3(pg), b256, d);
-    launch(w, KID_sealJoin, k_sealJoin, dim3(bg), b64, d);
+    launch(w, KID_localJoin, k_localJoin, dim3(bg), b64, d, 0);
+    launch(w, KID_edgeJoin, k_edgeJoin, dim3(bg), b64, d, 0);
     launch(w, KID_sealAnchor, k_sealAnchor, dim3(bg), b64, d);
     launch(w, KID_classify, k_classify, dim3(bg), b64, d);
     launch(w, KID_setFaces, k_setFaces, dim3(bg), b64, d);
     launch(w, KID_divergence, k_divergence, dim3(bg), b64, d);
     launch(w, KID_cgStart, k_cgStart, dim3(bg), b64, d);
-    launch(w, KID_cgJoin, k_cgJoin, dim3(bg), b64, d);
+    launch(w, KID_localJoin, k_localJoin, dim3(bg), b64, d, 1);
+    launch(w, KID_edgeJoin, k_edgeJoin, dim3(bg), b64, d, 1);
     launch(w, KID_cgAnchor, k_cgAnchor, dim3(bg), b64, d);
     launch(w, KID_cgResidual, k_cgResidual, dim3(bg4), b256, d);
     launch(w, KID_cgPrepare, k_cgPrepare, dim3(bg), b64, d);

@@ -266,60 +266,108 @@ __device__ __forceinline__ uint32_t wave_take(const Dev &d, uint32_t n) {
     return base + incl - n;
 }
 
-// syncParticles (:767-831)
+// syncParticles (:767-831).  A workgroup takes 1024 consecutive particles
+// of `sorted`, stages those it keeps in LDS, and puts them into `parts` in
+// their order with one atomic (takePlaces, :743-755, takes one a workgroup
+// of 256 in no set order).
+#define SP_CHUNK 1024u
+__device__ bool syncOne(const Dev &d, uint32_t k, PState &q) {
+    uint4 mo = *(const uint4 *)&MOTION(d, d.sorted, k, 0);
+    uint4 st = *(const uint4 *)&STATE(d, d.sorted, k, 0);
+    uint4 af = *(const uint4 *)&AFFINE(d, d.sorted, k, 0);
+    q.pos = V2(fbits(mo.x), fbits(mo.y));
+    q.vel = V2(fbits(mo.z), fbits(mo.w));
+    q.word = st.x;
+    q.temp = fbits(st.y);
+    q.id = st.z;
+    q.aff[0] = fbits(af.x); q.aff[1] = fbits(af.y); q.aff[2] = fbits(af.z); q.aff[3] = fbits(af.w);
+    uint32_t h = cellIndex(d, cellOf(d, q.pos));
+    q.pos = apart(d, k, h, q.pos, q.id);
+    uint32_t dest = CNT(d, h, G_DEST);
+    if (dest == DEST_KEEP) {
+        markBlockOf(d, q.pos);
+        return true;
+    }
+    if ((dest & DEST_ASIDE) != 0u) {
+        uint32_t to = dest & ~DEST_ASIDE;
+        q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
+        markBlockOf(d, q.pos);
+        return true;
+    }
+    if ((dest & DEST_CELL) == 0u) return false;
+    uint32_t to = DEST_SPAN - (dest & DEST_SPAN);
+    uint4 v = *(const uint4 *)&d.cells[4ull * to];
+    uint32_t cn = v.w & LIQUID_COUNT_MAX;
+    if (!(cn == LIQUID_COUNT_MAX || CNT(d, h, G_SORT) <= cn || rankById(d, h, k, q.id, q.word, wType(v.x)) < cn)) return false;
+    if (to != h)
+        q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
+    q.temp = fbits(v.y);
+    uint32_t drawn = CNT(d, h, G_TYPE), pt = wType(q.word);
+    if (pt == E_WATER && pt == (drawn & 0xffu) && wType(v.x) == pt
+        && (wAux(q.word) == ((drawn >> 8) & 0xffu) || wAux(q.word) == 0u)) {
+        uint32_t life = sharedLife(d, wAux(q.word) == 0u ? 0u : wLife(q.word), (drawn >> 16) & 0xffu, wLife(v.x), q.id);
+        uint32_t ww = wWithout(v.x, FL_DRAWN);
+        uint32_t solute = wAux(ww) != 0u ? wAux(ww) : (drawn >> 8) & 0xffu;
+        q.word = packWord(pt, life > 0u ? solute : 0u, life, wShade(ww), wFlags(ww));
+    } else if ((wType(q.word) | (wAux(q.word) << 8)) == (drawn & 0xffffu)) {
+        q.word = wWithout(v.x, FL_DRAWN);
+    }
+    return true;
+}
 __global__ void __launch_bounds__(256) k_syncParticles(Dev d) {
+    __shared__ uint4 sMo[SP_CHUNK], sSt[SP_CHUNK], sAf[SP_CHUNK];
+    __shared__ uint32_t sScan[256];
+    __shared__ uint32_t sBase;
     uint32_t cnt = ld(&d.sorted[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        uint32_t k = base + lane_id();
-        bool keep = false;
-        PState q;
-        if (k < n) {
-            uint4 mo = *(const uint4 *)&MOTION(d, d.sorted, k, 0);
-            uint4 st = *(const uint4 *)&STATE(d, d.sorted, k, 0);
-            uint4 af = *(const uint4 *)&AFFINE(d, d.sorted, k, 0);
-            q.pos = V2(fbits(mo.x), fbits(mo.y));
-            q.vel = V2(fbits(mo.z), fbits(mo.w));
-            q.word = st.x;
-            q.temp = fbits(st.y);
-            q.id = st.z;
-            q.aff[0] = fbits(af.x); q.aff[1] = fbits(af.y); q.aff[2] = fbits(af.z); q.aff[3] = fbits(af.w);
-            uint32_t h = cellIndex(d, cellOf(d, q.pos));
-            q.pos = apart(d, k, h, q.pos, q.id);
-            uint32_t dest = CNT(d, h, G_DEST);
-            if (dest == DEST_KEEP) {
-                keep = true;
-                markBlockOf(d, q.pos);
-            } else if ((dest & DEST_ASIDE) != 0u) {
-                uint32_t to = dest & ~DEST_ASIDE;
-                q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
-                keep = true;
-                markBlockOf(d, q.pos);
-            } else if ((dest & DEST_CELL) != 0u) {
-                uint32_t to = DEST_SPAN - (dest & DEST_SPAN);
-                uint4 v = *(const uint4 *)&d.cells[4ull * to];
-                uint32_t cn = v.w & LIQUID_COUNT_MAX;
-                if (cn == LIQUID_COUNT_MAX || CNT(d, h, G_SORT) <= cn || rankById(d, h, k, q.id, q.word, wType(v.x)) < cn) {
-                    if (to != h)
-                        q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
-                    q.temp = fbits(v.y);
-                    uint32_t drawn = CNT(d, h, G_TYPE), pt = wType(q.word);
-                    if (pt == E_WATER && pt == (drawn & 0xffu) && wType(v.x) == pt
-                        && (wAux(q.word) == ((drawn >> 8) & 0xffu) || wAux(q.word) == 0u)) {
-                        uint32_t life = sharedLife(d, wAux(q.word) == 0u ? 0u : wLife(q.word), (drawn >> 16) & 0xffu, wLife(v.x), q.id);
-                        uint32_t ww = wWithout(v.x, FL_DRAWN);
-                        uint32_t solute = wAux(ww) != 0u ? wAux(ww) : (drawn >> 8) & 0xffu;
-                        q.word = packWord(pt, life > 0u ? solute : 0u, life, wShade(ww), wFlags(ww));
-                    } else if ((wType(q.word) | (wAux(q.word) << 8)) == (drawn & 0xffffu)) {
-                        q.word = wWithout(v.x, FL_DRAWN);
-                    }
-                    keep = true;
-                }
+    uint32_t tid = threadIdx.x;
+    for (uint32_t base = blockIdx.x * SP_CHUNK; base < n; base += gridDim.x * SP_CHUNK) {
+        uint32_t keepBits = 0;
+        for (uint32_t r = 0; r < 4u; r++) {
+            uint32_t slot = r * 256u + tid, k = base + slot;
+            PState q;
+            bool keep = k < n && syncOne(d, k, q);
+            if (keep) {
+                sMo[slot] = make_uint4(ubits(q.pos.x), ubits(q.pos.y), ubits(q.vel.x), ubits(q.vel.y));
+                sSt[slot] = make_uint4(q.word, ubits(q.temp), q.id, 0u);
+                sAf[slot] = make_uint4(ubits(q.aff[0]), ubits(q.aff[1]), ubits(q.aff[2]), ubits(q.aff[3]));
             }
+            keepBits |= (keep ? 1u : 0u) << r;
         }
-        uint32_t m = wave_take(d, keep ? 1u : 0u);
-        if (keep && m < d.cap) storeParticle(d, m, q);
+        __syncthreads();
+        // the kept slots ranked in slot order: thread t owns slots 4t .. 4t + 3,
+        // whose keep bits thread (slot % 256) holds, round slot / 256
+        sScan[tid] = keepBits;
+        __syncthreads();
+        uint32_t bits4 = 0;
+        for (uint32_t j = 0; j < 4u; j++) {
+            uint32_t slot = 4u * tid + j;
+            bits4 |= ((sScan[slot % 256u] >> (slot / 256u)) & 1u) << j;
+        }
+        uint32_t mine = __builtin_popcount(bits4);
+        __syncthreads();
+        sScan[tid] = mine;
+        __syncthreads();
+        for (uint32_t o = 1; o < 256u; o <<= 1) {
+            uint32_t add = tid >= o ? sScan[tid - o] : 0u;
+            __syncthreads();
+            sScan[tid] += add;
+            __syncthreads();
+        }
+        if (tid == 255u) sBase = sScan[255] ? atomicAdd(&d.parts[0], sScan[255]) : 0u;
+        __syncthreads();
+        uint32_t m = sBase + sScan[tid] - mine;
+        for (uint32_t j = 0; j < 4u; j++) {
+            if (!((bits4 >> j) & 1u)) continue;
+            uint32_t slot = 4u * tid + j;
+            if (m < d.cap) {
+                *(uint4 *)&MOTION(d, d.parts, m, 0) = sMo[slot];
+                *(uint4 *)&STATE(d, d.parts, m, 0) = sSt[slot];
+                *(uint4 *)&AFFINE(d, d.parts, m, 0) = sAf[slot];
+            }
+            m++;
+        }
+        __syncthreads();
     }
 }
 
@@ -466,9 +514,29 @@ __global__ void __launch_bounds__(64) k_clear(Dev d) {
     }
 }
 
-// gather (:1000-1087): the splat as fixed-point integer atomics, whose sums
-// are the same in any order
-__device__ __forceinline__ void splatAxis(const Dev &d, v2 s, float vel, float gx, float gy, bool isU) {
+// gather (:1000-1087): the splat as fixed-point integers, whose sums are the
+// same in any order.  A workgroup takes 256 consecutive particles (which the
+// sort by cell keeps close together) and sums their splats into a tile in
+// LDS covering their cells and faces, then adds the tile's nonzero words
+// into the counters: a few global atomics a cell rather than 22 a particle
+// on the most contended words.  Particles too spread for a tile add
+// straight into the counters.
+#define GT_MAX 1024   // tile entries (cells, and faces of each axis)
+struct GatherTile {
+    int32_t ox, oy, tw, th;     // origin and size, in cells
+    uint32_t *um, *uw, *vm, *vw, *cnt, *mass, *dens;
+    __device__ __forceinline__ int at(i2 c) const {
+        int32_t x = c.x - ox, y = c.y - oy;
+        return (x >= 0 && y >= 0 && x < tw && y < th) ? y * tw + x : -1;
+    }
+};
+template <bool TILE>
+__device__ __forceinline__ void addWord(uint32_t *tile, int t, uint32_t *global, uint32_t v) {
+    if (TILE && t >= 0) atomicAdd(&tile[t], v);
+    else atomicAdd(global, v);
+}
+template <bool TILE>
+__device__ __forceinline__ void splatAxis(const Dev &d, const GatherTile &T, v2 s, float vel, float gx, float gy, bool isU) {
     float fx0 = floorf(s.x), fy0 = floorf(s.y);
     i2 b = I2(wi32(fx0), wi32(fy0));
     v2 f = V2(s.x - fx0, s.y - fy0);
@@ -479,11 +547,14 @@ __device__ __forceinline__ void splatAxis(const Dev &d, v2 s, float vel, float g
         if (wgt <= 0.0f || !faceInWorld(d, at, isU)) continue;
         uint32_t fi = faceIndex(d, at, isU);
         float there = vel + (gx * ((float)at.x - s.x) + gy * ((float)at.y - s.y));
-        atomicAdd(&SUM(d, fi, 0), (uint32_t)wi32(wround(there * wgt * FIXF)));
-        atomicAdd(&SUM(d, fi, 1), (uint32_t)wi32(wround(wgt * FIXF)));
+        uint32_t m = (uint32_t)wi32(wround(there * wgt * FIXF)), w = (uint32_t)wi32(wround(wgt * FIXF));
+        int t = TILE ? T.at(at) : -1;
+        addWord<TILE>(isU ? T.um : T.vm, t, &SUM(d, fi, 0), m);
+        addWord<TILE>(isU ? T.uw : T.vw, t, &SUM(d, fi, 1), w);
     }
 }
-__device__ __forceinline__ void splatDensity(const Dev &d, v2 pos, float room) {
+template <bool TILE>
+__device__ __forceinline__ void splatDensity(const Dev &d, const GatherTile &T, v2 pos, float room) {
     v2 s = V2(pos.x - 0.5f, pos.y - 0.5f);
     float fx0 = floorf(s.x), fy0 = floorf(s.y);
     i2 b = I2(wi32(fx0), wi32(fy0));
@@ -496,24 +567,99 @@ __device__ __forceinline__ void splatDensity(const Dev &d, v2 pos, float room) {
             i2 x = I2(b.x + 1 - ox, at.y), y = I2(at.x, b.y + 1 - oy), xy = I2(b.x + 1 - ox, b.y + 1 - oy);
             at = !walls(d, x) ? x : (!walls(d, y) ? y : (!walls(d, xy) ? xy : I2(-1, -1)));
         }
-        if (inWorld(d, at) && wgt > 0.0f) atomicAdd(&CNT(d, cellIndex(d, at), G_DENSITY), wu32(wround(wgt * room * FIXF)));
+        if (inWorld(d, at) && wgt > 0.0f)
+            addWord<TILE>(T.dens, TILE ? T.at(at) : -1, &CNT(d, cellIndex(d, at), G_DENSITY), wu32(wround(wgt * room * FIXF)));
     }
 }
+template <bool TILE>
+__device__ __forceinline__ void splatParticle(const Dev &d, const GatherTile &T, uint32_t k) {
+    uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
+    uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
+    uint4 af = *(const uint4 *)&AFFINE(d, d.parts, k, 0);
+    v2 pos = V2(fbits(mo.x), fbits(mo.y));
+    float temp = fbits(st.y);
+    splatAxis<TILE>(d, T, V2(pos.x - 0.0f, pos.y - 0.5f), fbits(mo.z), fbits(af.x), fbits(af.y), true);
+    splatAxis<TILE>(d, T, V2(pos.x - 0.5f, pos.y - 0.0f), fbits(mo.w), fbits(af.z), fbits(af.w), false);
+    i2 hc = cellOf(d, pos);
+    uint32_t home = cellIndex(d, hc);
+    int t = TILE ? T.at(hc) : -1;
+    addWord<TILE>(T.cnt, t, &CNT(d, home, G_COUNT), 1u);
+    addWord<TILE>(T.mass, t, &CNT(d, home, G_MASS), wu32(wround(liquidDensity(temp) * FIXF)));
+    splatDensity<TILE>(d, T, pos, liquidVolume(temp));
+}
 __global__ void __launch_bounds__(256) k_gather(Dev d) {
+    __shared__ uint32_t sTile[7 * GT_MAX];
+    __shared__ int32_t sBox[4];
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
-        uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
-        uint4 af = *(const uint4 *)&AFFINE(d, d.parts, k, 0);
-        v2 pos = V2(fbits(mo.x), fbits(mo.y));
-        float temp = fbits(st.y);
-        splatAxis(d, V2(pos.x - 0.0f, pos.y - 0.5f), fbits(mo.z), fbits(af.x), fbits(af.y), true);
-        splatAxis(d, V2(pos.x - 0.5f, pos.y - 0.0f), fbits(mo.w), fbits(af.z), fbits(af.w), false);
-        uint32_t home = cellIndex(d, cellOf(d, pos));
-        atomicAdd(&CNT(d, home, G_COUNT), 1u);
-        atomicAdd(&CNT(d, home, G_MASS), wu32(wround(liquidDensity(temp) * FIXF)));
-        splatDensity(d, pos, liquidVolume(temp));
+    uint32_t tid = threadIdx.x;
+    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+        uint32_t k = base + tid;
+        // the cells' box
+        int32_t cx = 0x7fffffff, cy = 0x7fffffff, mx = -1, my = -1;
+        if (k < n) {
+            i2 c = cellOf(d, V2(fbits(MOTION(d, d.parts, k, 0)), fbits(MOTION(d, d.parts, k, 1))));
+            cx = mx = c.x;
+            cy = my = c.y;
+        }
+        for (int h = 32; h > 0; h >>= 1) {
+            cx = min(cx, __shfl_xor(cx, h, WAVE));
+            cy = min(cy, __shfl_xor(cy, h, WAVE));
+            mx = max(mx, __shfl_xor(mx, h, WAVE));
+            my = max(my, __shfl_xor(my, h, WAVE));
+        }
+        if (tid == 0) { sBox[0] = 0x7fffffff; sBox[1] = 0x7fffffff; sBox[2] = -1; sBox[3] = -1; }
+        __syncthreads();
+        if (lane_id() == 0) {
+            atomicMin(&sBox[0], cx);
+            atomicMin(&sBox[1], cy);
+            atomicMax(&sBox[2], mx);
+            atomicMax(&sBox[3], my);
+        }
+        __syncthreads();
+        GatherTile T;
+        T.ox = sBox[0] - 1;
+        T.oy = sBox[1] - 1;
+        T.tw = sBox[2] - sBox[0] + 4;   // faces and cells a particle touches: its cell -1 .. +2
+        T.th = sBox[3] - sBox[1] + 4;
+        bool tiled = T.tw * T.th <= GT_MAX;
+        __syncthreads();
+        if (!tiled) {
+            if (k < n) splatParticle<false>(d, T, k);
+            continue;
+        }
+        int ne = T.tw * T.th;
+        T.um = sTile;
+        T.uw = sTile + ne;
+        T.vm = sTile + 2 * ne;
+        T.vw = sTile + 3 * ne;
+        T.cnt = sTile + 4 * ne;
+        T.mass = sTile + 5 * ne;
+        T.dens = sTile + 6 * ne;
+        for (int j = (int)tid; j < 7 * ne; j += 256) sTile[j] = 0u;
+        __syncthreads();
+        if (k < n) splatParticle<true>(d, T, k);
+        __syncthreads();
+        for (int j = (int)tid; j < ne; j += 256) {
+            i2 c = I2(T.ox + j % T.tw, T.oy + j / T.tw);
+            if (T.um[j] | T.uw[j]) {          // (a face outside the world was never added to)
+                uint32_t fi = uFace(d, c.x, c.y);
+                if (T.um[j]) atomicAdd(&SUM(d, fi, 0), T.um[j]);
+                if (T.uw[j]) atomicAdd(&SUM(d, fi, 1), T.uw[j]);
+            }
+            if (T.vm[j] | T.vw[j]) {
+                uint32_t fi = vFace(d, c.x, c.y);
+                if (T.vm[j]) atomicAdd(&SUM(d, fi, 0), T.vm[j]);
+                if (T.vw[j]) atomicAdd(&SUM(d, fi, 1), T.vw[j]);
+            }
+            if (T.cnt[j] | T.mass[j] | T.dens[j]) {
+                uint32_t ci = cellIndex(d, c);
+                if (T.cnt[j]) atomicAdd(&CNT(d, ci, G_COUNT), T.cnt[j]);
+                if (T.mass[j]) atomicAdd(&CNT(d, ci, G_MASS), T.mass[j]);
+                if (T.dens[j]) atomicAdd(&CNT(d, ci, G_DENSITY), T.dens[j]);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -527,22 +673,79 @@ __device__ __forceinline__ bool wasOpen(const Dev &d, i2 q) {
     if (!inWorld(d, q) || countAt(d, q) < 0) return false;
     return (wu32(FLD(d, 2u * cellIndex(d, q) + 1u, 3)) & stepBit(d, 1u)) == 0u;
 }
-__global__ void __launch_bounds__(64) k_sealJoin(Dev d) {
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
-        if (!inWorld(d, c) || !mayBeSealed(d, c)) continue;
-        for (int k = 0; k < 2; k++) {
-            i2 n = k == 0 ? I2(c.x + 1, c.y) : I2(c.x, c.y + 1);
-            if (mayBeSealed(d, n)) ufJoin(d, cellIndex(d, c), cellIndex(d, n));
-        }
+// The union-find's joins (label.wgsl's), of member cells with their
+// member neighbours to the right and below, in two passes: first each
+// block's own, in LDS, each member's link set to its component's lowest
+// cell in the block; then across the blocks' right and bottom edges, in the
+// counters' links.  The roots come out as the joins one at a time would
+// leave them: each component's lowest index.
+__device__ __forceinline__ uint32_t ldsFind(uint32_t *link, uint32_t i) {
+    uint32_t x = i;
+    for (;;) {
+        uint32_t up = link[x];
+        if (up >= x) return x;
+        x = up;
     }
 }
+__device__ void ldsJoin(uint32_t *link, uint32_t i, uint32_t j) {
+    uint32_t a = i, b = j;
+    for (;;) {
+        a = ldsFind(link, a);
+        b = ldsFind(link, b);
+        if (a == b) return;
+        uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        uint32_t old = atomicMin(&link[hi], lo);
+        if (old == hi) return;
+        a = old;
+        b = lo;
+    }
+}
+// member cells: 0 the gaps the liquid left (mayBeSealed), 1 the liquid
+__device__ __forceinline__ bool isMember(const Dev &d, i2 c, int mode) {
+    if (!inWorld(d, c)) return false;
+    return mode == 0 ? mayBeSealed(d, c) : FLD(d, 2u * cellIndex(d, c), 2) > 0.0f;
+}
+__global__ void __launch_bounds__(64) k_localJoin(Dev d, int mode) {
+    __shared__ uint32_t sLink[64];
+    __shared__ uint32_t sMember[64];
+    uint32_t li = threadIdx.x;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), li);
+        bool m = isMember(d, c, mode);
+        sLink[li] = li;
+        sMember[li] = m ? 1u : 0u;
+        __syncthreads();
+        if (m && li % 8u < 7u && sMember[li + 1u]) ldsJoin(sLink, li, li + 1u);
+        if (m && li / 8u < 7u && sMember[li + 8u]) ldsJoin(sLink, li, li + 8u);
+        __syncthreads();
+        if (m) {
+            uint32_t r = ldsFind(sLink, li);
+            i2 rc = blockCell(activeBlock(d, wid), r);   // within a block, index order is li order
+            CNT(d, cellIndex(d, c), G_REGION) = cellIndex(d, rc);
+        }
+        __syncthreads();
+    }
+}
+__global__ void __launch_bounds__(64) k_edgeJoin(Dev d, int mode) {
+    uint32_t li = threadIdx.x;
+    bool edge = li % 8u == 7u || li / 8u == 7u;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        if (!edge) continue;
+        i2 c = blockCell(activeBlock(d, wid), li);
+        if (!isMember(d, c, mode)) continue;
+        if (li % 8u == 7u && isMember(d, I2(c.x + 1, c.y), mode)) ufJoin(d, cellIndex(d, c), cellIndex(d, I2(c.x + 1, c.y)));
+        if (li / 8u == 7u && isMember(d, I2(c.x, c.y + 1), mode)) ufJoin(d, cellIndex(d, c), cellIndex(d, I2(c.x, c.y + 1)));
+    }
+}
+// sealAnchor: a gap meeting what was open air last step opens its region
 __global__ void __launch_bounds__(64) k_sealAnchor(Dev d) {
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c) || !mayBeSealed(d, c)) continue;
-        if (wasOpen(d, beside(c, 0)) || wasOpen(d, beside(c, 1)) || wasOpen(d, beside(c, 2)) || wasOpen(d, beside(c, 3)))
-            atomicOr(&CNT(d, ufFind(d, cellIndex(d, c)), G_REGION), REGION_OPEN);
+        if (wasOpen(d, beside(c, 0)) || wasOpen(d, beside(c, 1)) || wasOpen(d, beside(c, 2)) || wasOpen(d, beside(c, 3))) {
+            uint32_t r = ufFind(d, cellIndex(d, c));
+            if ((ufLink(d, r) & REGION_OPEN) == 0u) atomicOr(&CNT(d, r, G_REGION), REGION_OPEN);   // (one already set needs none)
+        }
     }
 }
 
@@ -1082,10 +1285,30 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
         }
     }
     // the coarsest (solve, :2407-2456): its first MG_COARSEST cells
-    {
+    // (one wavefront: its sweeps need no workgroup barrier, each lane
+    // seeing the others' LDS writes of the sweep before in program order)
+    if (threadIdx.x < 64u) {
         const LdsLevel T = level(L);
-        for (uint32_t sw = 0; sw < 2u * d.coarse_sweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST);
+        float h2 = (float)(1u << (2u * L));
+        uint32_t n = T.w * T.h < (uint32_t)MG_COARSEST ? T.w * T.h : (uint32_t)MG_COARSEST;
+        for (uint32_t sw = 0; sw < 2u * d.coarse_sweeps; sw++) {
+            int color = (int)(sw & 1u);
+            for (uint32_t j = threadIdx.x; j < n; j += 64u) {
+                i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
+                if (((c.x + c.y) & 1) != color) continue;
+                float rho = T.rho[j];
+                if (rho <= 0.0f) continue;
+                Stencil st = ldsStencil(T, c, rho);
+                if (st.diag <= 0.0f) continue;
+                float sx = (st.sum.x - h2 * T.rx[j]) / st.diag, sy = (st.sum.y - h2 * T.ry[j]) / st.diag;
+                T.px[j] = wmix(T.px[j], sx, OMEGA);
+                T.py[j] = wmix(T.py[j], sy, OMEGA);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
     }
+    __syncthreads();
     // up: prolong (:2459-2497), then a red and a black sweep
     for (int32_t l = (int32_t)L - 1; l >= (int32_t)l0 && !idle; l--) {
         const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
@@ -1160,16 +1383,6 @@ __global__ void __launch_bounds__(64) k_cgStart(Dev d) {                     // 
         CNT(d, i, G_REGION) = i;
     }
 }
-__global__ void __launch_bounds__(64) k_cgJoin(Dev d) {                      // :2677-2689
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
-        if (!inWorld(d, c) || gridZ(d, c) <= 0.0f) continue;
-        for (int k = 0; k < 2; k++) {
-            i2 n = k == 0 ? I2(c.x + 1, c.y) : I2(c.x, c.y + 1);
-            if (rhoAt(d, 0u, n) > 0.0f) ufJoin(d, cellIndex(d, c), cellIndex(d, n));
-        }
-    }
-}
 __global__ void __launch_bounds__(64) k_cgAnchor(Dev d) {                    // :2692-2705
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
@@ -1177,7 +1390,8 @@ __global__ void __launch_bounds__(64) k_cgAnchor(Dev d) {                    // 
         for (int k = 0; k < 4; k++) {
             i2 n = beside(c, k);
             if (inWorld(d, n) && rhoAt(d, 0u, n) == 0.0f) {
-                atomicOr(&CNT(d, ufFind(d, cellIndex(d, c)), G_REGION), REGION_OPEN);
+                uint32_t r = ufFind(d, cellIndex(d, c));
+                if ((ufLink(d, r) & REGION_OPEN) == 0u) atomicOr(&CNT(d, r, G_REGION), REGION_OPEN);
                 break;
             }
         }
@@ -1198,7 +1412,8 @@ __device__ __forceinline__ double ld_f64(const double *p) {
 __device__ double sumPlaces(const Dev &d, uint32_t which, double *s) {
     uint32_t li = threadIdx.x, na = activeCount(d);
     double acc = 0.0;
-    for (uint32_t j = li; j < na; j += 256u) acc += ld_f64(&d.bpart[3ull * j + which]);
+    // plain loads: lastGroup's fence has made the other workgroups' partials visible
+    for (uint32_t j = li; j < na; j += 256u) acc += d.bpart[3ull * j + which];
     s[li] = acc;
     __syncthreads();
     for (uint32_t h = 128; h > 0; h >>= 1) {
