@@ -79,7 +79,8 @@ typedef struct WigStepCtl {
                                 library's: encode this many, then wait for the solve's convergence flag
                                 and encode chunk more at a time until it converges or fluid_cycles run */
     uint32_t chunk;
-    uint32_t reserved[2];
+    uint32_t level_pairs;    /* red-black sweep pairs on each coarse level, down and up (0: 1, the reference's) */
+    uint32_t reserved;
 } WigStepCtl;
 
 /* first_chunk: as many iterations as the world's last solve ran */

@@ -2061,7 +2061,7 @@ static void relaxPass(OWorld *w, int *shifted, uint32_t times) {
 static int check_ctl(const OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     if (P->size[0] != (uint32_t)w->W || P->size[1] != (uint32_t)w->H) return WIG_E_INVALID;
     if (ctl->viscous || ctl->surface || ctl->bodies_run) return WIG_E_INVALID;
-    if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u) return WIG_E_INVALID;
+    if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->level_pairs > 64u) return WIG_E_INVALID;
     if (ctl->tol_mode > WIG_TOL_REL_L2) return WIG_E_INVALID;
     return WIG_OK;
 }
@@ -2116,18 +2116,22 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         }
         w->cycle = cycle;
         relaxPass(w, &shifted, ctl->pre_relax);
+        uint32_t pairs = ctl->level_pairs ? ctl->level_pairs : 1u;
         for (uint32_t l = 1; l <= deepest; l++) {
             k_restrict(w, l);
-            if (l < deepest) {
-                k_smooth(w, l, 0);
-                k_smooth(w, l, 1);
-            }
+            if (l < deepest)
+                for (uint32_t k = 0; k < pairs; k++) {
+                    k_smooth(w, l, 0);
+                    k_smooth(w, l, 1);
+                }
         }
         k_coarsest(w, deepest);
         for (uint32_t l = deepest - 1; l >= 1; l--) {
             k_prolong(w, l);
-            k_smooth(w, l, 0);
-            k_smooth(w, l, 1);
+            for (uint32_t k = 0; k < pairs; k++) {
+                k_smooth(w, l, 0);
+                k_smooth(w, l, 1);
+            }
         }
         k_prolongFine(w);
         relaxPass(w, &shifted, ctl->post_relax);

@@ -39,11 +39,15 @@ __global__ void k_prolongFine(Dev d, uint32_t cycle);
 __global__ void k_cgStart(Dev d);
 __global__ void k_cgAnchor(Dev d);
 __global__ void k_cgResidual(Dev d);
+__global__ void k_sumNorm(Dev d);
 __global__ void k_cgPrepare(Dev d);
-__global__ void k_cgRhoBeta(Dev d, uint32_t cycle);
+__global__ void k_cgRho(Dev d, uint32_t cycle);
+__global__ void k_cgBeta(Dev d, uint32_t cycle);
 __global__ void k_cgDirection(Dev d, uint32_t cycle);
-__global__ void k_cgApplyAlpha(Dev d, uint32_t cycle);
-__global__ void k_cgUpdateCheck(Dev d, uint32_t cycle);
+__global__ void k_cgApply(Dev d, uint32_t cycle);
+__global__ void k_cgAlpha(Dev d, uint32_t cycle);
+__global__ void k_cgUpdate(Dev d, uint32_t cycle);
+__global__ void k_cgCheck(Dev d, uint32_t cycle);
 __global__ void k_cgFinish(Dev d);
 __global__ void k_project(Dev d);
 __global__ void k_extrapolate(Dev d, int layer);
@@ -78,8 +82,8 @@ static int fail(int code, const char *fmt, ...) {
 // Kernel ids for wig_time_kernel, in launch order.
 #define WIG_KERNELS(X) X(setStep) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
     X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
-    X(cgResidual) X(cgPrepare) X(relaxRims) X(relax) X(restrict) X(smooth) X(coarseLDS) X(prolong) \
-    X(prolongFine) X(cgRhoBeta) X(cgDirection) X(cgApplyAlpha) X(cgUpdateCheck) \
+    X(cgResidual) X(sumNorm) X(cgPrepare) X(relaxRims) X(relax) X(restrict) X(smooth) X(coarseLDS) X(prolong) \
+    X(prolongFine) X(cgRho) X(cgBeta) X(cgDirection) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
     X(scatter) X(rasterClaims) X(raster) X(unpaint)
 #define X_ENUM(n) KID_##n,
@@ -96,7 +100,6 @@ struct WigWorld {
     uint32_t *scratch;           // B words
     unsigned long long *totals;  // WigRunTotals on the device
     StepParams *sp;              // the step's Params words
-    uint32_t *ticket;            // the fused reductions' counter
     size_t lds_bytes;            // k_coarseLDS's dynamic LDS
     uint32_t *flag_host;         // pinned: the solve's iterations and convergence flag, read between chunks
     uint32_t last_cycles;        // iterations the last chunked solve ran
@@ -212,7 +215,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     w->timed = -1;
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess
-        || hipMalloc((void **)&w->sp, sizeof(StepParams)) != hipSuccess || hipMalloc((void **)&w->ticket, 64) != hipSuccess
+        || hipMalloc((void **)&w->sp, sizeof(StepParams)) != hipSuccess
         || hipHostMalloc((void **)&w->flag_host, 64, hipHostMallocDefault) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
         wig_destroy(w);
@@ -223,11 +226,9 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     zero(w, w->kind, 256);
     zero(w, w->totals, sizeof(WigRunTotals));
     zero(w, w->sp, sizeof(StepParams));
-    zero(w, w->ticket, 64);
     Dev &d = w->d;
     d.totals = w->totals;
     d.sp = w->sp;
-    d.ticket = w->ticket;
     // the coarse levels one workgroup's LDS holds (k_coarseLDS): from the
     // first that fits with all below it, 20 bytes a cell
     int lds_max = 65536;
@@ -274,7 +275,6 @@ int wig_destroy(WigWorld *w) {
     if (w->scratch) hipFree(w->scratch);
     if (w->totals) hipFree(w->totals);
     if (w->sp) hipFree(w->sp);
-    if (w->ticket) hipFree(w->ticket);
     if (w->flag_host) hipHostFree(w->flag_host);
     if (w->ev_made)
         for (int i = 0; i < 2 * MAX_TIMED; i++) hipEventDestroy(w->ev[i]);
@@ -409,12 +409,14 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         return fail(WIG_E_INVALID, "params size %ux%u is not the world's %dx%d", P->size[0], P->size[1], d.W, d.H);
     if (ctl->viscous || ctl->surface || ctl->bodies_run)
         return fail(WIG_E_INVALID, "viscosity, surface tension and bodies are not built (water only)");
-    if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u)
+    if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u
+        || ctl->level_pairs > 64u)
         return fail(WIG_E_INVALID, "solver control out of range");
     d.sweep_pairs = ctl->sweep_pairs;
     d.coarse_sweeps = ctl->coarse_sweeps;
     d.tol_mode = ctl->tol_mode;
     d.tol = ctl->tol;
+    d.level_pairs = ctl->level_pairs ? ctl->level_pairs : 1u;
     hipStream_t s = w->stream;
     dim3 world(d.bw, d.bh), b64(64), b256(256);
     uint32_t bg = block_grid(d), pg = particle_grid(d), bg4 = std::min<uint32_t>((d.B + 3u) / 4u, 1024u);
@@ -444,6 +446,7 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     launch(w, KID_edgeJoin, k_edgeJoin, dim3(bg), b64, d, 1);
     launch(w, KID_cgAnchor, k_cgAnchor, dim3(bg), b64, d);
     launch(w, KID_cgResidual, k_cgResidual, dim3(bg4), b256, d);
+    launch(w, KID_sumNorm, k_sumNorm, dim3(1), b256, d);
     launch(w, KID_cgPrepare, k_cgPrepare, dim3(bg), b64, d);
     bool shifted = ctl->shifted_first != 0;
     // an iteration's first relaxations take their rims from the passes
@@ -470,18 +473,23 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
             launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, own ? 0 : 1);
             shifted = !shifted;
         }
-        for (uint32_t l = 1; l < l0; l++) {
+        for (uint32_t l = 1; l <= l0; l++) {
             dim3 lg(ceil_div(d.lw[l], 8u), ceil_div(d.lh[l], 8u));
             launch(w, KID_restrict, k_restrict, lg, b64, d, cycle, l);
-            launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
-            launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
+            if (l == l0) break;       // level l0's sweeps and the levels below: k_coarseLDS
+            for (uint32_t k = 0; k < d.level_pairs; k++) {
+                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
+                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
+            }
         }
         launch_lds(w, KID_coarseLDS, k_coarseLDS, dim3(1), dim3(1024), w->lds_bytes, d, cycle);
         for (uint32_t l = l0 - 1; l >= 1; l--) {
             dim3 lg(ceil_div(d.lw[l], 8u), ceil_div(d.lh[l], 8u));
             launch(w, KID_prolong, k_prolong, lg, b64, d, cycle, l);
-            launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
-            launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
+            for (uint32_t k = 0; k < d.level_pairs; k++) {
+                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
+                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
+            }
         }
         launch(w, KID_prolongFine, k_prolongFine, dim3(bg), b64, d, cycle);
         for (uint32_t k = 0; k < ctl->post_relax; k++) {
@@ -490,10 +498,13 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
             launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, 0);
             shifted = !shifted;
         }
-        launch(w, KID_cgRhoBeta, k_cgRhoBeta, dim3(bg4), b256, d, cycle);
+        launch(w, KID_cgRho, k_cgRho, dim3(bg4), b256, d, cycle);
+        launch(w, KID_cgBeta, k_cgBeta, dim3(1), b256, d, cycle);
         launch(w, KID_cgDirection, k_cgDirection, dim3(bg), b64, d, cycle);
-        launch(w, KID_cgApplyAlpha, k_cgApplyAlpha, dim3(bg4), b256, d, cycle);
-        launch(w, KID_cgUpdateCheck, k_cgUpdateCheck, dim3(bg4), b256, d, cycle);
+        launch(w, KID_cgApply, k_cgApply, dim3(bg4), b256, d, cycle);
+        launch(w, KID_cgAlpha, k_cgAlpha, dim3(1), b256, d, cycle);
+        launch(w, KID_cgUpdate, k_cgUpdate, dim3(bg4), b256, d, cycle);
+        launch(w, KID_cgCheck, k_cgCheck, dim3(1), b256, d, cycle);
     }
     launch(w, KID_cgFinish, k_cgFinish, dim3(bg), b64, d);
     launch(w, KID_project, k_project, dim3(bg), b64, d);

@@ -123,9 +123,8 @@ struct Dev {
     uint32_t *scratch;      // per-block flags and offsets (compact)
     unsigned long long *totals;  // steps, PCG iterations, particles, active cells (run totals)
     StepParams *sp;
-    uint32_t *ticket;       // the last-workgroup counter of the fused reductions
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
-    uint32_t sweep_pairs, coarse_sweeps, tol_mode;
+    uint32_t sweep_pairs, coarse_sweeps, tol_mode, level_pairs;
     float tol;
 };
 

@@ -1229,12 +1229,15 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
         T.rho = sm + off + 4u * n;
         return T;
     };
-    if (idle) {
-        const LdsLevel T = level(L);
+    {
+        // level l0 as k_restrict left it in the field (or, with no blocks
+        // worked on, the coarsest as it was)
+        const LdsLevel T = level(l0);
         for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
-            float4 e = *(const float4 *)&FLD(d, levelSlot(d, L, I2((int32_t)(j % T.w), (int32_t)(j / T.w))), 0);
+            uint64_t at = levelSlot(d, l0, I2((int32_t)(j % T.w), (int32_t)(j / T.w)));
+            float4 e = *(const float4 *)&FLD(d, at, 0);
             T.px[j] = e.x; T.py[j] = e.y; T.rx[j] = e.z; T.ry[j] = e.w;
-            T.rho[j] = FLD(d, levelSlot(d, L, I2((int32_t)(j % T.w), (int32_t)(j / T.w))) + 1u, 0);
+            T.rho[j] = FLD(d, at + 1u, 0);
         }
         __syncthreads();
     }
@@ -1242,7 +1245,7 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
     for (uint32_t l = l0; l <= L && !idle; l++) {
         const LdsLevel T = level(l);
         int32_t span = 1 << l;
-        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
+        for (uint32_t j = threadIdx.x; j < T.w * T.h && l > l0; j += blockDim.x) {
             i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
             float rho = 0.0f, rxv = 0.0f, ryv = 0.0f;
             bool shortcut = false;
@@ -1257,13 +1260,12 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
                 for (int q = 0; q < 4; q++) {
                     i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
                     float rk;
-                    if (l == l0) rk = rhoAt(d, l - 1u, k);
-                    else rk = ldsRho(level(l - 1u), k);
+                    rk = ldsRho(level(l - 1u), k);
                     if (rk == 0.0f) air = true;
                     else if (rk > 0.0f) {
                         fluid += 1.0f;
                         sum += rk;
-                        v2 e = l == l0 ? residualAt(d, l - 1u, k) : ldsResidual(level(l - 1u), l - 1u, k);
+                        v2 e = ldsResidual(level(l - 1u), l - 1u, k);
                         r.x += e.x;
                         r.y += e.y;
                     }
@@ -1279,36 +1281,17 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
             T.rho[j] = rho;
         }
         __syncthreads();
-        if (l < L) {
-            ldsSmooth(T, l, 0, 1.0f, 0xffffffffu);
-            ldsSmooth(T, l, 1, 1.0f, 0xffffffffu);
-        }
+        if (l < L)
+            for (uint32_t k = 0; k < d.level_pairs; k++) {
+                ldsSmooth(T, l, 0, 1.0f, 0xffffffffu);
+                ldsSmooth(T, l, 1, 1.0f, 0xffffffffu);
+            }
     }
     // the coarsest (solve, :2407-2456): its first MG_COARSEST cells
-    // (one wavefront: its sweeps need no workgroup barrier, each lane
-    // seeing the others' LDS writes of the sweep before in program order)
-    if (threadIdx.x < 64u) {
+    {
         const LdsLevel T = level(L);
-        float h2 = (float)(1u << (2u * L));
-        uint32_t n = T.w * T.h < (uint32_t)MG_COARSEST ? T.w * T.h : (uint32_t)MG_COARSEST;
-        for (uint32_t sw = 0; sw < 2u * d.coarse_sweeps; sw++) {
-            int color = (int)(sw & 1u);
-            for (uint32_t j = threadIdx.x; j < n; j += 64u) {
-                i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
-                if (((c.x + c.y) & 1) != color) continue;
-                float rho = T.rho[j];
-                if (rho <= 0.0f) continue;
-                Stencil st = ldsStencil(T, c, rho);
-                if (st.diag <= 0.0f) continue;
-                float sx = (st.sum.x - h2 * T.rx[j]) / st.diag, sy = (st.sum.y - h2 * T.ry[j]) / st.diag;
-                T.px[j] = wmix(T.px[j], sx, OMEGA);
-                T.py[j] = wmix(T.py[j], sy, OMEGA);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-        }
+        for (uint32_t sw = 0; sw < 2u * d.coarse_sweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST);
     }
-    __syncthreads();
     // up: prolong (:2459-2497), then a red and a black sweep
     for (int32_t l = (int32_t)L - 1; l >= (int32_t)l0 && !idle; l--) {
         const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
@@ -1337,8 +1320,10 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
             T.py[j] = T.py[j] + sum.y / dn;
         }
         __syncthreads();
-        ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu);
-        ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu);
+        for (uint32_t k = 0; k < d.level_pairs; k++) {
+            ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu);
+            ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu);
+        }
     }
     // the levels back into the field
     for (uint32_t l = l0; l <= L; l++) {
@@ -1397,12 +1382,12 @@ __global__ void __launch_bounds__(64) k_cgAnchor(Dev d) {                    // 
         }
     }
 }
-// ------------------------------------------------ FP64 sums, fused
+// ------------------------------------------------ FP64 sums
 // A pass's block partials are kept by the block's place in the active list
 // (which is in block-index order) and summed by one workgroup of 256, lane i
 // taking places i, i + 256, ..., then halving: the same order every run.
-// The sum is taken by whichever workgroup of the pass finishes last (a
-// ticket), so the pass and its sum are one launch.
+// (The sum is a launch of its own: a pass whose last workgroup took it would
+// need every workgroup to fence its partials out to all XCDs' caches.)
 #define WAVES 4   // wavefronts (blocks of cells) a workgroup of 256
 
 __device__ __forceinline__ double ld_f64(const double *p) {
@@ -1412,7 +1397,6 @@ __device__ __forceinline__ double ld_f64(const double *p) {
 __device__ double sumPlaces(const Dev &d, uint32_t which, double *s) {
     uint32_t li = threadIdx.x, na = activeCount(d);
     double acc = 0.0;
-    // plain loads: lastGroup's fence has made the other workgroups' partials visible
     for (uint32_t j = li; j < na; j += 256u) acc += d.bpart[3ull * j + which];
     s[li] = acc;
     __syncthreads();
@@ -1424,25 +1408,12 @@ __device__ double sumPlaces(const Dev &d, uint32_t which, double *s) {
     __syncthreads();
     return r;
 }
-// After each wavefront wrote its partials: true in the workgroup that
-// finished last (which then takes the sums and resets the ticket).
-__device__ __forceinline__ bool lastGroup(const Dev &d) {
-    __shared__ uint32_t isLast;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) isLast = atomicAdd(d.ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
-    __syncthreads();
-    if (!isLast) return false;
-    __threadfence();
-    return true;
-}
 #define FOR_ACTIVE_BLOCKS4(d, wid) \
     for (uint32_t wid = blockIdx.x * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += gridDim.x * WAVES)
 #define LANE (threadIdx.x % 64u)
 
 // cgResidual (:2708-2717), and ||b||^2 for the relative stop
 __global__ void __launch_bounds__(256) k_cgResidual(Dev d) {
-    __shared__ double s[256];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         SolveHead h = {};
         *d.head = h;
@@ -1459,12 +1430,11 @@ __global__ void __launch_bounds__(256) k_cgResidual(Dev d) {
         bb = wave_tree_sum(bb);
         if (LANE == 0) d.bpart[3ull * wid + 2] = bb;
     }
-    if (!lastGroup(d)) return;
+}
+__global__ void __launch_bounds__(256) k_sumNorm(Dev d) {
+    __shared__ double s[256];
     double bb = sumPlaces(d, 2, s);
-    if (threadIdx.x == 0) {
-        d.head->bb = bb;
-        *d.ticket = 0u;
-    }
+    if (threadIdx.x == 0) d.head->bb = bb;
 }
 // cgPrepare (:2721-2732).  The first iteration's first relaxation takes its
 // rims' drift from G_AIR_TO (free until rasterClaims writes it): the
@@ -1482,9 +1452,8 @@ __global__ void __launch_bounds__(64) k_cgPrepare(Dev d) {
     }
 }
 // cgRho + cgBeta (:2736-2764): r.z and z.Ap, and Notay's flexible beta in FP64
-__global__ void __launch_bounds__(256) k_cgRhoBeta(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgRho(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
-    __shared__ double s[256];
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double v0 = 0.0, v1 = 0.0;
@@ -1501,7 +1470,10 @@ __global__ void __launch_bounds__(256) k_cgRhoBeta(Dev d, uint32_t cycle) {
             d.bpart[3ull * wid + 1] = v1;
         }
     }
-    if (!lastGroup(d)) return;
+}
+__global__ void __launch_bounds__(256) k_cgBeta(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    __shared__ double s[256];
     double s0 = sumPlaces(d, 0, s), s1 = sumPlaces(d, 1, s);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
@@ -1510,7 +1482,6 @@ __global__ void __launch_bounds__(256) k_cgRhoBeta(Dev d, uint32_t cycle) {
         h->beta = beta;
         CGV(d, d.C + d.B, 0) = (float)s0;
         CGV(d, d.C + d.B, 2) = (float)beta;
-        *d.ticket = 0u;
     }
 }
 // cgDirection (:2767-2776); and the drift as the iteration's relaxations
@@ -1529,9 +1500,8 @@ __global__ void __launch_bounds__(64) k_cgDirection(Dev d, uint32_t cycle) {
     }
 }
 // cgApply + cgAlpha (:2779-2814)
-__global__ void __launch_bounds__(256) k_cgApplyAlpha(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgApply(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
-    __shared__ double s[256];
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double v = 0.0;
@@ -1545,7 +1515,10 @@ __global__ void __launch_bounds__(256) k_cgApplyAlpha(Dev d, uint32_t cycle) {
         v = wave_tree_sum(v);
         if (LANE == 0) d.bpart[3ull * wid + 0] = v;
     }
-    if (!lastGroup(d)) return;
+}
+__global__ void __launch_bounds__(256) k_cgAlpha(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    __shared__ double s[256];
     double pq = sumPlaces(d, 0, s);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
@@ -1553,20 +1526,18 @@ __global__ void __launch_bounds__(256) k_cgApplyAlpha(Dev d, uint32_t cycle) {
         h->pq = pq;
         CGV(d, d.C + d.B, 1) = (float)h->alpha;
         CGV(d, d.C + d.B, 3) = (float)pq;
-        *d.ticket = 0u;
     }
 }
 // cgUpdate (:2852-2880), and whether the next
 // iteration runs: max |r| under the tolerance (the reference's converged(),
 // :2223-2226), or ||r|| <= tol ||b||
-__global__ void __launch_bounds__(256) k_cgUpdateCheck(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgUpdate(Dev d, uint32_t cycle) {
     bool conv = converged(d, cycle);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + (cycle + 1u) % 3u)] = 0u;
         if (!conv) atomicAdd(&d.blocks[COUNT_SLOT(d, COUNT_CYCLES)], 1u);
     }
     if (conv) return;
-    __shared__ double s[256];
     double alpha = ld_f64(&d.head->alpha);
     uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
     FOR_ACTIVE_BLOCKS4(d, wid) {
@@ -1592,7 +1563,11 @@ __global__ void __launch_bounds__(256) k_cgUpdateCheck(Dev d, uint32_t cycle) {
             atomicMax(slot, most);
         }
     }
-    if (!lastGroup(d)) return;
+}
+__global__ void __launch_bounds__(256) k_cgCheck(Dev d, uint32_t cycle) {
+    if (converged(d, cycle)) return;
+    __shared__ double s[256];
+    uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
     double rr = sumPlaces(d, 2, s);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
@@ -1606,7 +1581,6 @@ __global__ void __launch_bounds__(256) k_cgUpdateCheck(Dev d, uint32_t cycle) {
             c2 = fbits(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < d.tol;
         }
         __hip_atomic_store(&h->converged, c2 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *d.ticket = 0u;
     }
 }
 // cgFinish (:2884-2893)
