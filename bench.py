@@ -87,7 +87,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--kernel", default=None, help="kernel for the roofline (default: the one taking most time)")
+    ap.add_argument("--kernel", default="solve",
+                    help="what the roofline prices: 'solve' (the MG-PCG solve, the byte model's unit of work) "
+                         "or a particle kernel (gather, advect, sortCount, scatter, syncParticles)")
+    ap.add_argument("--kernel-table", action="store_true", help="also time every kernel over one step each")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -123,24 +126,24 @@ def main():
         sim.step()
     torch.cuda.synchronize()
 
-    # which kernel takes the most time: each timed over one step, untimed
-    kname = args.kernel
+    # each kernel's time over one step (untimed steps), for the record
     per_kernel = {}
-    if kname is None:
+    if args.kernel_table:
         for name in kernel_names():
             world.time_kernel(name)
             sim.step(read_report=False)
             ms, n = world.kernel_time()
             per_kernel[name] = (ms, n)
         world.time_kernel(None)
-        kname = max(per_kernel, key=lambda k: per_kernel[k][0])
     torch.cuda.synchronize()
 
+    # the timed steps, the solve (cgStart .. cgFinish: the unit of work the
+    # byte model prices) bracketed by HIP events on the world's stream
     K = args.steps
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     t0 = world.run_totals()
-    world.time_kernel(kname)
+    world.time_kernel(args.kernel)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     for k in range(K):
@@ -160,19 +163,23 @@ def main():
     value = d[2] / (total_ms / 1e3)
     solves = K / (total_ms / 1e3)
     bytes_total = step_bytes(d)
+    ca_mean = d[3] / K
 
-    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": kname,
+    roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": args.kernel,
             "launches": klaunches, "mean_launch_ms": kms / max(klaunches, 1)}
-    if kname in KERNEL_BYTES_PER_PARTICLE and klaunches:
-        per_launch = KERNEL_BYTES_PER_PARTICLE[kname] * particles_per_step * (K / klaunches)
-        roof["algorithmic_bytes_per_launch"] = per_launch
-        roof["achieved"] = per_launch / (kms / klaunches / 1e3) / 1e9
+    if args.kernel == "solve":
+        # B_solve = C_a*16 + K*C_a*93 (FP32 V-cycle storage), per solve
+        per_launch = (d[3] * 16.0 + d[1] * ca_mean * 93.0) / K
+        roof["unit_of_work"] = "one MG-PCG solve (cgStart..cgFinish), B_solve = C_a*16 + K*C_a*93 bytes"
+        roof["C_a"] = ca_mean
+        roof["pcg_iterations"] = d[1] / K
+    elif args.kernel in KERNEL_BYTES_PER_PARTICLE:
+        per_launch = KERNEL_BYTES_PER_PARTICLE[args.kernel] * particles_per_step * (K / max(klaunches, 1))
     else:
-        # a solve or grid kernel: the step's byte model over the step time
-        roof["algorithmic_bytes_per_launch"] = None
-        roof["achieved"] = bytes_total / (total_ms / 1e3) / 1e9
-        roof["note"] = "no per-launch byte model for this kernel: the whole step's model over the step time"
-    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+        per_launch = None
+    roof["algorithmic_bytes_per_launch"] = per_launch
+    roof["achieved"] = per_launch / (kms / max(klaunches, 1) / 1e3) / 1e9 if per_launch and klaunches else None
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS if roof["achieved"] else None
     step_roof = bytes_total / (total_ms / 1e3) / 1e9
 
     out = {
@@ -190,6 +197,7 @@ def main():
         "wall_s": wall,
         "roofline": roof,
     }
+    out["solve_ms_mean"] = kms / max(klaunches, 1) if args.kernel == "solve" else None
     if per_kernel:
         out["kernel_ms_one_step"] = {k: round(v[0], 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:12]}
     if not args.no_cpu:

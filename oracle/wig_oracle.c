@@ -141,7 +141,7 @@ typedef struct OWorld {
     float *field;
     uint32_t *blocks;
     float *cg;
-    uint8_t *solve;              /* SolveHead, then 3 doubles a block */
+    uint8_t *solve;              /* SolveHead, then 4 doubles a block */
     uint8_t kind[256];
     int dot_mode;
     /* the step being run */
@@ -344,7 +344,7 @@ static void layout(OWorld *w, uint32_t width, uint32_t height, uint32_t coarse) 
     w->bytes[WIG_PART_FIELD] = 16ull * w->nfield;
     w->bytes[WIG_PART_BLOCKS] = 4ull * (5ull * w->B + FLUID_STATS);
     w->bytes[WIG_PART_FLUIDCG] = 16ull * (w->C + w->B + 1u);
-    w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 24ull * w->B;
+    w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * w->B;
 }
 
 /* ------------------------------------------------------------ address */
@@ -1388,17 +1388,17 @@ static void sumBlocksF(OWorld *w, float out[2]) {
     out[1] = s[0][1];
 }
 /* FP64: the partials are kept by the block's place in the active list (in
- * block-index order), and summed by 256 lanes, lane i taking places i,
- * i + 256, ..., then halving. */
+ * block-index order), and summed by 1024 lanes, lane i taking places i,
+ * i + 1024, ..., then halving. */
 static void sumBlocksD(OWorld *w, int which, double *out) {
-    double s[256];
+    double s[1024];
     double *bp = bpart(w);
     uint32_t na = activeCount(w);
-    for (int li = 0; li < 256; li++) {
+    for (int li = 0; li < 1024; li++) {
         s[li] = 0.0;
-        for (uint32_t j = (uint32_t)li; j < na; j += 256u) s[li] += bp[3ull * j + (uint32_t)which];
+        for (uint32_t j = (uint32_t)li; j < na; j += 1024u) s[li] += bp[4ull * j + (uint32_t)which];
     }
-    for (int h = 128; h > 0; h >>= 1)
+    for (int h = 512; h > 0; h >>= 1)
         for (int li = 0; li < h; li++) s[li] += s[li + h];
     *out = s[0];
 }
@@ -1452,7 +1452,7 @@ static void k_cgResidual(OWorld *w) {
         }
         double out[3];
         sumBlockD(v, out);
-        bp[3ull * wid + 2] = out[2];
+        bp[4ull * wid + 2] = out[2];
     }
     double bb;
     sumBlocksD(w, 2, &bb);
@@ -1516,8 +1516,8 @@ static void k_cgRhoBeta(OWorld *w) {
             }
             double out[3];
             sumBlockD(v, out);
-            bp[3ull * wid + 0] = out[0];
-            bp[3ull * wid + 1] = out[1];
+            bp[4ull * wid + 0] = out[0];
+            bp[4ull * wid + 1] = out[1];
         }
         double s0, s1;
         sumBlocksD(w, 0, &s0);
@@ -1574,7 +1574,7 @@ static void k_cgApplyAlpha(OWorld *w) {
         } else {
             double out[3];
             sumBlockD(vd, out);
-            bp[3ull * wid + 0] = out[0];
+            bp[4ull * wid + 0] = out[0];
         }
     }
     if (w->dot_mode == 0) {
@@ -1630,7 +1630,7 @@ static void k_cgUpdate(OWorld *w) {
         }
         double out[3];
         sumBlockD(v, out);
-        bp[3ull * wid + 2] = out[2];
+        bp[4ull * wid + 2] = out[2];
     }
     uint32_t *slot = &w->blocks[COUNT_SLOT(COUNT_RESIDUAL + w->cycle % 3u)];
     if (most > *slot) *slot = most;

@@ -1394,19 +1394,29 @@ __device__ __forceinline__ double ld_f64(const double *p) {
     unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return __longlong_as_double((long long)u);
 }
-__device__ double sumPlaces(const Dev &d, uint32_t which, double *s) {
+// (1024 lanes, lane i taking places i, i + 1024, ..., then halving; two sums
+// at once where a pass has two)
+#define SUM_LANES 1024u
+__device__ void sumPlaces2(const Dev &d, uint32_t w0, uint32_t w1, double *s0, double *s1, double *out0, double *out1) {
     uint32_t li = threadIdx.x, na = activeCount(d);
-    double acc = 0.0;
-    for (uint32_t j = li; j < na; j += 256u) acc += d.bpart[3ull * j + which];
-    s[li] = acc;
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t j = li; j < na; j += SUM_LANES) {
+        a0 += d.bpart[4ull * j + w0];
+        a1 += d.bpart[4ull * j + w1];
+    }
+    s0[li] = a0;
+    s1[li] = a1;
     __syncthreads();
-    for (uint32_t h = 128; h > 0; h >>= 1) {
-        if (li < h) s[li] += s[li + h];
+    for (uint32_t h = SUM_LANES / 2u; h > 0; h >>= 1) {
+        if (li < h) {
+            s0[li] += s0[li + h];
+            s1[li] += s1[li + h];
+        }
         __syncthreads();
     }
-    double r = s[0];
+    *out0 = s0[0];
+    *out1 = s1[0];
     __syncthreads();
-    return r;
 }
 #define FOR_ACTIVE_BLOCKS4(d, wid) \
     for (uint32_t wid = blockIdx.x * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += gridDim.x * WAVES)
@@ -1428,12 +1438,13 @@ __global__ void __launch_bounds__(256) k_cgResidual(Dev d) {
             bb = bi * bi;
         }
         bb = wave_tree_sum(bb);
-        if (LANE == 0) d.bpart[3ull * wid + 2] = bb;
+        if (LANE == 0) d.bpart[4ull * wid + 2] = bb;
     }
 }
-__global__ void __launch_bounds__(256) k_sumNorm(Dev d) {
-    __shared__ double s[256];
-    double bb = sumPlaces(d, 2, s);
+__global__ void __launch_bounds__(1024) k_sumNorm(Dev d) {
+    __shared__ double s0[SUM_LANES], s1[SUM_LANES];
+    double bb, unused;
+    sumPlaces2(d, 2, 2, s0, s1, &bb, &unused);
     if (threadIdx.x == 0) d.head->bb = bb;
 }
 // cgPrepare (:2721-2732).  The first iteration's first relaxation takes its
@@ -1466,15 +1477,16 @@ __global__ void __launch_bounds__(256) k_cgRho(Dev d, uint32_t cycle) {
         v0 = wave_tree_sum(v0);
         v1 = wave_tree_sum(v1);
         if (LANE == 0) {
-            d.bpart[3ull * wid + 0] = v0;
-            d.bpart[3ull * wid + 1] = v1;
+            d.bpart[4ull * wid + 0] = v0;
+            d.bpart[4ull * wid + 1] = v1;
         }
     }
 }
-__global__ void __launch_bounds__(256) k_cgBeta(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgBeta(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
-    __shared__ double s[256];
-    double s0 = sumPlaces(d, 0, s), s1 = sumPlaces(d, 1, s);
+    __shared__ double t0[SUM_LANES], t1[SUM_LANES];
+    double s0, s1;
+    sumPlaces2(d, 0, 1, t0, t1, &s0, &s1);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
         double beta = (cycle == 0u || h->pq <= 0.0) ? 0.0 : -s1 / h->pq;
@@ -1513,13 +1525,14 @@ __global__ void __launch_bounds__(256) k_cgApply(Dev d, uint32_t cycle) {
             v = (double)CGV(d, i, 2) * (double)q;
         }
         v = wave_tree_sum(v);
-        if (LANE == 0) d.bpart[3ull * wid + 0] = v;
+        if (LANE == 0) d.bpart[4ull * wid + 0] = v;
     }
 }
-__global__ void __launch_bounds__(256) k_cgAlpha(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgAlpha(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
-    __shared__ double s[256];
-    double pq = sumPlaces(d, 0, s);
+    __shared__ double t0[SUM_LANES], t1[SUM_LANES];
+    double pq, unused;
+    sumPlaces2(d, 0, 0, t0, t1, &pq, &unused);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
         h->alpha = pq <= 0.0 ? 0.0 : h->rz / pq;
@@ -1539,7 +1552,6 @@ __global__ void __launch_bounds__(256) k_cgUpdate(Dev d, uint32_t cycle) {
     }
     if (conv) return;
     double alpha = ld_f64(&d.head->alpha);
-    uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double rr = 0.0;
@@ -1559,17 +1571,32 @@ __global__ void __launch_bounds__(256) k_cgUpdate(Dev d, uint32_t cycle) {
         rr = wave_tree_sum(rr);
         most = wave_max_u32(most);
         if (LANE == 0) {
-            d.bpart[3ull * wid + 2] = rr;
-            atomicMax(slot, most);
+            d.bpart[4ull * wid + 2] = rr;
+            d.bpart[4ull * wid + 3] = (double)most;   // (the block's max |r|, as f32 bits; cgCheck takes the most)
         }
     }
 }
-__global__ void __launch_bounds__(256) k_cgCheck(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgCheck(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
-    __shared__ double s[256];
+    __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
-    double rr = sumPlaces(d, 2, s);
+    double rr, unused;
+    sumPlaces2(d, 2, 2, t0, t1, &rr, &unused);
+    // the most |r| left in any cell (non-negative floats order as their bits)
+    __shared__ uint32_t sMost[SUM_LANES];
+    uint32_t m = 0, na = activeCount(d);
+    for (uint32_t j = threadIdx.x; j < na; j += SUM_LANES) {
+        uint32_t b = (uint32_t)d.bpart[4ull * j + 3];
+        m = b > m ? b : m;
+    }
+    sMost[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t h = SUM_LANES / 2u; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sMost[threadIdx.x] = max(sMost[threadIdx.x], sMost[threadIdx.x + h]);
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
+        if (sMost[0] > *slot) *slot = sMost[0];
         SolveHead *h = d.head;
         h->rr = rr;
         h->cycles += 1u;
@@ -1578,7 +1605,7 @@ __global__ void __launch_bounds__(256) k_cgCheck(Dev d, uint32_t cycle) {
             double t = (double)d.tol;
             c2 = rr <= t * t * h->bb;
         } else {
-            c2 = fbits(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < d.tol;
+            c2 = fbits(*slot) < d.tol;
         }
         __hip_atomic_store(&h->converged, c2 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
