@@ -29,13 +29,12 @@ __global__ void k_sealAnchor(Dev d);
 __global__ void k_classify(Dev d);
 __global__ void k_setFaces(Dev d);
 __global__ void k_divergence(Dev d);
-__global__ void k_relaxRims(Dev d, uint32_t cycle);
-__global__ void k_relax(Dev d, uint32_t cycle, int shift, int rimsOfDrift);
-__global__ void k_restrict(Dev d, uint32_t cycle, uint32_t l);
-__global__ void k_smooth(Dev d, uint32_t cycle, uint32_t l, int color);
+__global__ void k_relax(Dev d, uint32_t cycle, int shift, int post);
+__global__ void k_restrict(Dev d, uint32_t cycle);
+__global__ void k_levelRho(Dev d);
+__global__ void k_levelDown(Dev d, uint32_t cycle, uint32_t l, uint32_t T);
 __global__ void k_coarseLDS(Dev d, uint32_t cycle);
-__global__ void k_prolong(Dev d, uint32_t cycle, uint32_t l);
-__global__ void k_prolongFine(Dev d, uint32_t cycle);
+__global__ void k_levelUp(Dev d, uint32_t cycle, uint32_t l, uint32_t T);
 __global__ void k_cgStart(Dev d);
 __global__ void k_cgAnchor(Dev d);
 __global__ void k_cgResidual(Dev d);
@@ -48,7 +47,7 @@ __global__ void k_cgApply(Dev d, uint32_t cycle);
 __global__ void k_cgAlpha(Dev d, uint32_t cycle);
 __global__ void k_cgUpdate(Dev d, uint32_t cycle);
 __global__ void k_cgCheck(Dev d, uint32_t cycle);
-__global__ void k_cgFinish(Dev d);
+__global__ void k_cgFinish(Dev d, int ran);
 __global__ void k_project(Dev d);
 __global__ void k_extrapolate(Dev d, int layer);
 __global__ void k_advect(Dev d);
@@ -82,8 +81,8 @@ static int fail(int code, const char *fmt, ...) {
 // Kernel ids for wig_time_kernel, in launch order.
 #define WIG_KERNELS(X) X(setStep) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
     X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
-    X(cgResidual) X(sumNorm) X(cgPrepare) X(relaxRims) X(relax) X(restrict) X(smooth) X(coarseLDS) X(prolong) \
-    X(prolongFine) X(cgRho) X(cgBeta) X(cgDirection) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
+    X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
+    X(cgRho) X(cgBeta) X(cgDirection) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
     X(scatter) X(rasterClaims) X(raster) X(unpaint)
 #define X_ENUM(n) KID_##n,
@@ -98,9 +97,11 @@ struct WigWorld {
     void *part[WIG_PART_COUNT];
     uint8_t *kind;               // 256 element kinds
     uint32_t *scratch;           // B words
+    void *fine;                  // the solve's packed fine level (Dev::coef ... r2)
     unsigned long long *totals;  // WigRunTotals on the device
     StepParams *sp;              // the step's Params words
     size_t lds_bytes;            // k_coarseLDS's dynamic LDS
+    uint32_t tile[9];            // k_levelDown/k_levelUp's tile side per level
     uint32_t *flag_host;         // pinned: the solve's iterations and convergence flag, read between chunks
     uint32_t last_cycles;        // iterations the last chunked solve ran
     hipStream_t own, stream;
@@ -125,6 +126,26 @@ static void launch(WigWorld *w, int id, void (*k)(KArgs...), dim3 grid, dim3 blo
 }
 
 static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
+
+// k_levelDown/k_levelUp: tiles of T x T level cells, each with a halo of
+// one cell a half-sweep (k_levelDown one more), in LDS 7 floats a cell, and
+// k_levelDown's residuals (2 a tile cell) or k_levelUp's coarse cells (3)
+#define MAX_LEVEL_PAIRS 8u
+static dim3 tileGrid(const Dev &d, uint32_t l, uint32_t T) { return dim3(ceil_div(d.lw[l], T), ceil_div(d.lh[l], T)); }
+static size_t downLds(const Dev &d, uint32_t T) {
+    size_t R = T + 2u * (2u * d.level_pairs + 1u);
+    return 4u * (7u * R * R + 2u * T * T);
+}
+static size_t upLds(const Dev &d, uint32_t T) {
+    size_t R = T + 4u * d.level_pairs, Rc = R / 2u + 3u;
+    return 4u * (7u * R * R + 3u * Rc * Rc);
+}
+// the world's tile for level l, narrowed to fit 64 KB of LDS
+static uint32_t levelTileSide(const WigWorld *w, const Dev &d, uint32_t l) {
+    uint32_t T = w->tile[l];
+    while (T > 8u && std::max(downLds(d, T), upLds(d, T)) > 65536u) T /= 2u;
+    return T;
+}
 
 // The world's sizes and parts (js/sim.js:741-786; fluid.wgsl:2228-2248).
 static void size_world(WigWorld *w, uint32_t width, uint32_t height, uint32_t coarse) {
@@ -214,6 +235,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     }
     w->timed = -1;
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
+        || hipMalloc(&w->fine, 41ull * w->d.C + 256u) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess
         || hipMalloc((void **)&w->sp, sizeof(StepParams)) != hipSuccess
         || hipHostMalloc((void **)&w->flag_host, 64, hipHostMallocDefault) != hipSuccess) {
@@ -224,24 +246,26 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     for (int p = 0; p < WIG_PART_COUNT; p++)
         if (zero(w, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
     zero(w, w->kind, 256);
+    zero(w, w->fine, 41ull * w->d.C + 256u);
     zero(w, w->totals, sizeof(WigRunTotals));
     zero(w, w->sp, sizeof(StepParams));
     Dev &d = w->d;
     d.totals = w->totals;
     d.sp = w->sp;
     // the coarse levels one workgroup's LDS holds (k_coarseLDS): from the
-    // first that fits with all below it, 20 bytes a cell
+    // first that fits with all below it
     int lds_max = 65536;
     hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, w->device);
     d.lds_level = d.L;
     for (uint32_t l = d.L; l >= 1; l--) {
         size_t bytes = 0;
-        for (uint32_t k = l; k <= d.L; k++) bytes += 20ull * d.lw[k] * d.lh[k];
+        for (uint32_t k = l; k <= d.L; k++) bytes += 4ull * LDS_LEVEL_FLOATS * d.lw[k] * d.lh[k];
         if (bytes > (size_t)lds_max) break;
         d.lds_level = l;
         w->lds_bytes = bytes;
     }
-    if (w->lds_bytes == 0) w->lds_bytes = 20ull * d.lw[d.L] * d.lh[d.L];
+    if (w->lds_bytes == 0) w->lds_bytes = 4ull * LDS_LEVEL_FLOATS * d.lw[d.L] * d.lh[d.L];
+    for (uint32_t l = 0; l < 9; l++) w->tile[l] = 16u;
     if (w->lds_bytes > 65536 && hipFuncSetAttribute((const void *)k_coarseLDS, hipFuncAttributeMaxDynamicSharedMemorySize, (int)w->lds_bytes) != hipSuccess) {
         fail(WIG_E_DEVICE, "k_coarseLDS cannot take %zu bytes of LDS", w->lds_bytes);
         wig_destroy(w);
@@ -258,6 +282,14 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     d.bpart = (double *)((uint8_t *)w->part[WIG_PART_SOLVE] + sizeof(SolveHead));
     d.kind = w->kind;
     d.scratch = w->scratch;
+    {
+        uint8_t *f = (uint8_t *)w->fine;
+        d.coef = (float4 *)f;
+        d.pa = (float2 *)(f + 16ull * d.C);
+        d.pb = (float2 *)(f + 24ull * d.C);
+        d.r2 = (float2 *)(f + 32ull * d.C);
+        d.cflag = f + 40ull * d.C;
+    }
     if (hipStreamSynchronize(w->stream) != hipSuccess) {
         fail(WIG_E_DEVICE, "stream failed at creation");
         wig_destroy(w);
@@ -273,6 +305,7 @@ int wig_destroy(WigWorld *w) {
         if (w->part[p]) hipFree(w->part[p]);
     if (w->kind) hipFree(w->kind);
     if (w->scratch) hipFree(w->scratch);
+    if (w->fine) hipFree(w->fine);
     if (w->totals) hipFree(w->totals);
     if (w->sp) hipFree(w->sp);
     if (w->flag_host) hipHostFree(w->flag_host);
@@ -410,8 +443,10 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     if (ctl->viscous || ctl->surface || ctl->bodies_run)
         return fail(WIG_E_INVALID, "viscosity, surface tension and bodies are not built (water only)");
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u
-        || ctl->level_pairs > 64u)
+        || ctl->level_pairs > MAX_LEVEL_PAIRS)
         return fail(WIG_E_INVALID, "solver control out of range");
+    if (ctl->pre_relax != 1u || ctl->post_relax != 1u)
+        return fail(WIG_E_INVALID, "one relaxation before and one after the coarse levels (PRE_RELAX = POST_RELAX = 1) is what is built");
     d.sweep_pairs = ctl->sweep_pairs;
     d.coarse_sweeps = ctl->coarse_sweeps;
     d.tol_mode = ctl->tol_mode;
@@ -451,11 +486,9 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     launch(w, KID_cgResidual, k_cgResidual, dim3(bg4), b256, d);
     launch(w, KID_sumNorm, k_sumNorm, dim3(1), dim3(1024), d);
     launch(w, KID_cgPrepare, k_cgPrepare, dim3(bg), b64, d);
+    dim3 lg1(ceil_div(d.lw[1], 8u), ceil_div(d.lh[1], 8u));
+    if (ctl->fluid_cycles > 0u) launch(w, KID_levelRho, k_levelRho, lg1, b64, d);
     bool shifted = ctl->shifted_first != 0;
-    // an iteration's first relaxations take their rims from the passes
-    // before them (cgPrepare or cgDirection, prolongFine); more of them
-    // snapshot their own (relaxRims)
-    bool fused = ctl->pre_relax >= 1 && ctl->post_relax >= 1;
     uint32_t l0 = d.lds_level;
     uint32_t first = ctl->first_chunk == WIG_CHUNK_AUTO ? std::max<uint32_t>(w->last_cycles, 1u) : ctl->first_chunk;
     uint32_t next_check = first ? first : WIG_MAX_CYCLES + 1u;
@@ -470,37 +503,20 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
             }
             next_check = cycle + (ctl->chunk ? ctl->chunk : 1u);
         }
-        for (uint32_t k = 0; k < ctl->pre_relax; k++) {
-            bool own = !fused || k > 0;
-            if (own) launch(w, KID_relaxRims, k_relaxRims, dim3(bg), b64, d, cycle);
-            launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, own ? 0 : 1);
-            shifted = !shifted;
-        }
-        for (uint32_t l = 1; l <= l0; l++) {
-            dim3 lg(ceil_div(d.lw[l], 8u), ceil_div(d.lh[l], 8u));
-            launch(w, KID_restrict, k_restrict, lg, b64, d, cycle, l);
-            if (l == l0) break;       // level l0's sweeps and the levels below: k_coarseLDS
-            for (uint32_t k = 0; k < d.level_pairs; k++) {
-                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
-                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
-            }
+        launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, 0);
+        shifted = !shifted;
+        launch(w, KID_restrict, k_restrict, lg1, b64, d, cycle);
+        for (uint32_t l = 1; l < l0; l++) {   // level l's sweeps, and restrictLevel into l + 1
+            uint32_t T = levelTileSide(w, d, l);
+            launch_lds(w, KID_levelDown, k_levelDown, tileGrid(d, l, T), dim3(256), downLds(d, T), d, cycle, l, T);
         }
         launch_lds(w, KID_coarseLDS, k_coarseLDS, dim3(1), dim3(1024), w->lds_bytes, d, cycle);
         for (uint32_t l = l0 - 1; l >= 1; l--) {
-            dim3 lg(ceil_div(d.lw[l], 8u), ceil_div(d.lh[l], 8u));
-            launch(w, KID_prolong, k_prolong, lg, b64, d, cycle, l);
-            for (uint32_t k = 0; k < d.level_pairs; k++) {
-                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 0);
-                launch(w, KID_smooth, k_smooth, lg, b64, d, cycle, l, 1);
-            }
+            uint32_t T = levelTileSide(w, d, l);
+            launch_lds(w, KID_levelUp, k_levelUp, tileGrid(d, l, T), dim3(256), upLds(d, T), d, cycle, l, T);
         }
-        launch(w, KID_prolongFine, k_prolongFine, dim3(bg), b64, d, cycle);
-        for (uint32_t k = 0; k < ctl->post_relax; k++) {
-            bool own = !fused || k > 0;
-            if (own) launch(w, KID_relaxRims, k_relaxRims, dim3(bg), b64, d, cycle);
-            launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, 0);
-            shifted = !shifted;
-        }
+        launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, 1);
+        shifted = !shifted;
         launch(w, KID_cgRho, k_cgRho, dim3(bg4), b256, d, cycle);
         launch(w, KID_cgBeta, k_cgBeta, dim3(1), dim3(1024), d, cycle);
         launch(w, KID_cgDirection, k_cgDirection, dim3(bg), b64, d, cycle);
@@ -509,7 +525,7 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         launch(w, KID_cgUpdate, k_cgUpdate, dim3(bg4), b256, d, cycle);
         launch(w, KID_cgCheck, k_cgCheck, dim3(1), dim3(1024), d, cycle);
     }
-    launch(w, KID_cgFinish, k_cgFinish, dim3(bg), b64, d);
+    launch(w, KID_cgFinish, k_cgFinish, dim3(bg), b64, d, ctl->fluid_cycles > 0u ? 1 : 0);
     if (timeSolve) hipEventRecord(w->ev[2 * w->ntimed + 1], s), w->ntimed++;
     launch(w, KID_project, k_project, dim3(bg), b64, d);
     launch(w, KID_extrapolate, k_extrapolate, dim3(bg), b64, d, 1);

@@ -12,6 +12,9 @@
 #define FLUID_STATS 9u
 #define MG_COARSEST 512
 #define MG_LEVELS 8
+#define CF_FLUID 1u
+#define CF_PINNED 2u
+#define LDS_LEVEL_FLOATS 7u   // k_coarseLDS: p, rhs (2 each), rho, two face betas a cell
 #define WNONE 0xffffffffu
 #define G_COUNT 0u
 #define G_TAKEN 1u
@@ -123,6 +126,16 @@ struct Dev {
     uint32_t *scratch;      // per-block flags and offsets (compact)
     unsigned long long *totals;  // steps, PCG iterations, particles, active cells (run totals)
     StepParams *sp;
+    // The solve's fine level, packed once a solve (cgPrepare) for the cells
+    // of the blocks worked on and the ring of cells beside them: each cell's
+    // four face betas in beside()'s order (fineBeta; the sign bit set where
+    // the neighbour is fluid), its fluid and pinned bits, the V-cycle's
+    // iterate (p, drift) in two buffers (a relaxation reads one and writes
+    // the other, so its tiles' rims stay as they were), and the rhs (-r, drift
+    // rhs).  The reference's words for these are written back at cgFinish.
+    float4 *coef;
+    uint8_t *cflag;
+    float2 *pa, *pb, *r2;
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
     uint32_t sweep_pairs, coarse_sweeps, tol_mode, level_pairs;
     float tol;

@@ -947,22 +947,31 @@ __device__ v2 residualAt(const Dev &d, uint32_t l, i2 c) {                  // :
     return V2(rhs.x - (s.sum.x - (s.diag + pin.x) * pot.x) / h2, rhs.y - (s.sum.y - (s.diag + pin.y) * pot.y) / h2);
 }
 
-// relaxRims (:2096-2107)
-__global__ void __launch_bounds__(64) k_relaxRims(Dev d, uint32_t cycle) {
-    if (converged(d, cycle)) return;
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
-        if (!inWorld(d, c)) continue;
-        uint32_t i = cellIndex(d, c);
-        if (FLD(d, 2u * i, 2) > 0.0f) {
-            CNT(d, i, G_RIM) = ubits(FLD(d, 2u * i, 0));
-            CNT(d, i, G_RIM + 1u) = ubits(FLD(d, 2u * i + 1u, 0));
-        }
+// The fine level's residual at cell i (residualAt, :2306-2315) from the
+// packed solve arrays: p as the pre-relaxation left it (pb), the betas
+// (0 where the stencil skips the neighbour), the pin, the rhs.
+__device__ __forceinline__ v2 fineResidual(const Dev &d, i2 c, uint32_t i, uint32_t f) {
+    float4 cf = d.coef[i];
+    const float b[4] = {fabsf(cf.x), fabsf(cf.y), fabsf(cf.z), fabsf(cf.w)};
+    float sx = 0.0f, sy = 0.0f, diag = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        if (b[k] == 0.0f) continue;
+        float2 q = d.pb[cellIndex(d, beside(c, k))];
+        sx += b[k] * q.x;
+        sy += b[k] * q.y;
+        diag += b[k];
     }
+    float2 p = d.pb[i], r = d.r2[i];
+    float pin = (f & CF_PINNED) ? PINF : 0.0f;
+    return V2(r.x - (sx - (diag + pin) * p.x), r.y - (sy - (diag + 0.0f) * p.y));
 }
 
 // relax / relaxShifted (:2080-2192): a 10 x 10 tile in LDS, its rim frozen
-// as relaxRims left it, sweep_pairs red-black pairs, one write back.
+// as relaxRims snapshot it, sweep_pairs red-black pairs, one write back.
+// The pre-relaxation reads pa and writes pb, the post-relaxation reads pb
+// plus prolongFine's correction (:2500-2512) and writes pa: the buffer read
+// is the rim snapshot, and every fluid cell of the tiles is written.
+__device__ v2 prolonged(const Dev &d, uint32_t l, i2 c);
 __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
     if (wid < activeCount(d)) return activeBlock(d, wid);
     int32_t wide = (int32_t)d.bw, high = (int32_t)(d.B / d.bw);
@@ -970,9 +979,11 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
     return j <= high ? I2(wide, j) : I2(j - high - 1, high);
 }
-__global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, int rimsOfDrift) {
+__global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, int post) {
     if (converged(d, cycle)) return;
-    __shared__ float sPx[RIM * RIM], sPy[RIM * RIM], sRho[RIM * RIM], sShare[RIM * RIM];
+    __shared__ float sPx[RIM * RIM], sPy[RIM * RIM];
+    const float2 *pin = post ? d.pb : d.pa;
+    float2 *pout = post ? d.pa : d.pb;
     uint32_t li = threadIdx.x;
     uint32_t na = activeCount(d), edge = shift ? d.bw + d.bh + 1u : 0u;
     // the tiles: the active list's blocks, then (shifted) the edge tiles,
@@ -994,42 +1005,42 @@ __global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, 
         if (!live) continue;
         for (uint32_t k = li; k < RIM * RIM; k += 64u) {
             i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
-            float px = 0.0f, py = 0.0f, r = -1.0f;
+            float2 v = make_float2(0.0f, 0.0f);
             if (inWorld(d, c)) {
                 uint32_t ci = cellIndex(d, c);
-                px = FLD(d, 2u * ci, 0);
-                py = FLD(d, 2u * ci + 1u, 0);
-                r = FLD(d, 2u * ci, 2);
-                if (r > 0.0f) {
-                    // the rim as relaxRims snapshot it; or, for an
-                    // iteration's first relaxation, the pressure's 0 and the
-                    // drift's snapshot after the last iteration
-                    px = rimsOfDrift ? 0.0f : fbits(CNT(d, ci, G_RIM));
-                    py = fbits(CNT(d, ci, rimsOfDrift ? G_AIR_TO : G_RIM + 1u));
+                v = pin[ci];
+                if (post && (d.cflag[ci] & CF_FLUID)) {
+                    v2 e = prolonged(d, 0u, c);
+                    v.x = v.x + e.x;
+                    v.y = v.y + e.y;
                 }
             }
-            sPx[k] = px;
-            sPy[k] = py;
-            sRho[k] = r;
-            sShare[k] = densityAt(d, c);
+            sPx[k] = v.x;
+            sPy[k] = v.y;
         }
         __syncthreads();
         i2 me = I2(origin.x + (int32_t)(li % 8u), origin.y + (int32_t)(li / 8u));
         int t = (int)(li / 8u + 1u) * RIM + (int)(li % 8u) + 1;
-        float r0 = sRho[t];
         float rhsx = 0.0f, rhsy = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
-        if (r0 > 0.0f) {
-            uint32_t i = 2u * cellIndex(d, me);
-            rhsx = FLD(d, i, 1);
-            rhsy = FLD(d, i + 1u, 1);
-            float dd = sShare[t];
-            b0 = fineBeta(r0, sRho[t - 1], dd, sShare[t - 1]);
-            b1 = fineBeta(r0, sRho[t + 1], dd, sShare[t + 1]);
-            b2 = fineBeta(r0, sRho[t - RIM], dd, sShare[t - RIM]);
-            b3 = fineBeta(r0, sRho[t + RIM], dd, sShare[t + RIM]);
+        bool fluid = false, pinned = false;
+        uint32_t mi = 0;
+        if (inWorld(d, me)) {
+            mi = cellIndex(d, me);
+            uint32_t f = d.cflag[mi];
+            fluid = (f & CF_FLUID) != 0u;
+            if (fluid) {
+                float4 cf = d.coef[mi];
+                float2 r = d.r2[mi];
+                b0 = fabsf(cf.x);
+                b1 = fabsf(cf.y);
+                b2 = fabsf(cf.z);
+                b3 = fabsf(cf.w);
+                rhsx = r.x;
+                rhsy = r.y;
+                pinned = (f & CF_PINNED) != 0u;
+            }
         }
         float diag = b0 + b1 + b2 + b3;
-        bool pinned = r0 > 0.0f && pinnedAt(d, cellIndex(d, me));
         float dx = diag + (pinned ? PINF : 0.0f), dy = fmaxf(diag, 1e-30f);
         bool solving = dx > 0.0f;
         int color = (me.x + me.y) & 1;
@@ -1044,68 +1055,58 @@ __global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, 
             }
             __syncthreads();
         }
-        if (solving) {
-            uint32_t i = 2u * cellIndex(d, me);
-            FLD(d, i, 0) = sPx[t];
-            FLD(d, i + 1u, 0) = sPy[t];
-        }
+        if (fluid) pout[mi] = make_float2(sPx[t], sPy[t]);
         __syncthreads();
     }
 }
 
-// restrictLevel (:2342-2377)
-__global__ void __launch_bounds__(64) k_restrict(Dev d, uint32_t cycle, uint32_t l) {
+// restrictLevel into level 1 (:2342-2377), each iteration: rhs from the
+// fluid children's residuals; rho as k_levelRho put it at the solve's start
+__global__ void __launch_bounds__(64) k_restrict(Dev d, uint32_t cycle) {
     if (converged(d, cycle) || activeCount(d) == 0u) return;
     i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
-    if (!inLevel(d, l, c)) return;
-    uint64_t at = levelSlot(d, l, c);
-    int32_t span = 1 << l;
-    if (span <= WIG_BLOCK) {
-        i2 b = I2(c.x * span / WIG_BLOCK, c.y * span / WIG_BLOCK);
-        if (inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u) {
-            *(float4 *)&FLD(d, at, 0) = make_float4(0.f, 0.f, 0.f, 0.f);
-            *(float4 *)&FLD(d, at + 1u, 0) = make_float4(0.f, 0.f, 0.f, 0.f);
-            return;
-        }
-    }
-    float fluid = 0.0f, sum = 0.0f;
-    bool air = false;
-    v2 r = V2(0.0f, 0.0f);
-    for (int j = 0; j < 4; j++) {
-        i2 k = I2(2 * c.x + (j & 1), 2 * c.y + (j >> 1));
-        float rho = rhoAt(d, l - 1u, k);
-        if (rho == 0.0f) air = true;
-        else if (rho > 0.0f) {
-            fluid += 1.0f;
-            sum += rho;
-            v2 q = residualAt(d, l - 1u, k);
-            r.x += q.x;
-            r.y += q.y;
-        }
-    }
-    float rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
-    *(float4 *)&FLD(d, at, 0) = make_float4(0.0f, 0.0f, rho > 0.0f ? r.x * 0.25f : 0.0f, rho > 0.0f ? r.y * 0.25f : 0.0f);
-    *(float4 *)&FLD(d, at + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
-}
-
-// smoothRed / smoothBlack (:2317-2387)
-__global__ void __launch_bounds__(64) k_smooth(Dev d, uint32_t cycle, uint32_t l, int color) {
-    if (converged(d, cycle) || activeCount(d) == 0u) return;
-    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
-    if (!inLevel(d, l, c) || ((c.x + c.y) & 1) != color) return;
-    uint64_t at = levelSlot(d, l, c);
+    if (!inLevel(d, 1u, c)) return;
+    uint64_t at = levelSlot(d, 1u, c);
     float rho = FLD(d, at + 1u, 0);
-    if (rho <= 0.0f) return;
-    Stencil s = stencil(d, l, c, rho);
-    if (s.diag <= 0.0f) return;
-    float h2 = (float)(1u << (2u * l));
-    float4 e = *(const float4 *)&FLD(d, at, 0);
-    float sx = (s.sum.x - h2 * e.z) / s.diag, sy = (s.sum.y - h2 * e.w) / s.diag;
-    FLD(d, at, 0) = wmix(e.x, sx, 1.0f);
-    FLD(d, at, 1) = wmix(e.y, sy, 1.0f);
+    float sx = 0.0f, sy = 0.0f;
+    if (rho > 0.0f) {
+        for (int q = 0; q < 4; q++) {
+            i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
+            if (!inWorld(d, k)) continue;
+            uint32_t ki = cellIndex(d, k), f = d.cflag[ki];
+            if (!(f & CF_FLUID)) continue;
+            v2 e = fineResidual(d, k, ki, f);
+            sx += e.x;
+            sy += e.y;
+        }
+    }
+    *(float4 *)&FLD(d, at, 0) = make_float4(0.0f, 0.0f, rho > 0.0f ? sx * 0.25f : 0.0f, rho > 0.0f ? sy * 0.25f : 0.0f);
+}
+// level 1's rho (restrictLevel's, :2342-2377), once a solve: it depends on
+// the fine rho alone
+__global__ void __launch_bounds__(64) k_levelRho(Dev d) {
+    if (activeCount(d) == 0u) return;
+    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
+    if (!inLevel(d, 1u, c)) return;
+    float rho = 0.0f;
+    i2 b = I2(c.x * 2 / WIG_BLOCK, c.y * 2 / WIG_BLOCK);
+    if (!(inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u)) {
+        float fluid = 0.0f, sum = 0.0f;
+        bool air = false;
+        for (int q = 0; q < 4; q++) {
+            float rk = rhoAt(d, 0u, I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1)));
+            if (rk == 0.0f) air = true;
+            else if (rk > 0.0f) {
+                fluid += 1.0f;
+                sum += rk;
+            }
+        }
+        rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
+    }
+    *(float4 *)&FLD(d, levelSlot(d, 1u, c) + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
 }
 
-// prolong, prolongFine, prolonged (:2458-2512)
+// prolonged (:2458-2497); prolong is in k_levelUp, prolongFine in k_relax
 __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
     i2 coarse = I2(c.x / 2, c.y / 2);
     i2 s = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
@@ -1127,85 +1128,294 @@ __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
     float dn = fmaxf(weight, 1e-6f);
     return V2(sum.x / dn, sum.y / dn);
 }
-__global__ void __launch_bounds__(64) k_prolong(Dev d, uint32_t cycle, uint32_t l) {
-    if (converged(d, cycle) || activeCount(d) == 0u) return;
-    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
-    if (!inLevel(d, l, c)) return;
-    uint64_t at = levelSlot(d, l, c);
-    if (FLD(d, at + 1u, 0) > 0.0f) {
-        v2 e = prolonged(d, l, c);
-        FLD(d, at, 0) = FLD(d, at, 0) + e.x;
-        FLD(d, at, 1) = FLD(d, at, 1) + e.y;
+// A level's red-black sweeps (smoothRed, smoothBlack: :2317-2387) over one
+// workgroup's T x T tile in LDS.  The tile carries a halo as wide as the
+// number of half-sweeps, so each half-sweep's values are right one cell
+// further in and the tile's own cells end exactly as the passes over the
+// whole level leave them.  The faces' betas (faceBeta, :369-374) are taken
+// once a tile: level rho does not change within a sweep.
+//
+// k_levelDown(l): level l as restrictLevel left it; the sweeps; then
+// restrictLevel into level l + 1 from the tile's cells (one more halo cell,
+// so the residuals at the tile's edge see right neighbours).
+// k_levelUp(l): prolong's correction (:2459-2497) from level l + 1, staged
+// in LDS; the sweeps.  Both write the tile's p back to the field.
+#define TILE_THREADS 256u
+struct LevelTile {
+    float *px, *py, *rx, *ry, *rho, *bR, *bD;
+    int32_t R, ox, oy;   // side with the halo; the halo's corner in level cells
+};
+__device__ __forceinline__ LevelTile levelTile(float *sm, uint32_t T, uint32_t H) {
+    LevelTile t;
+    t.R = (int32_t)(T + 2u * H);
+    uint32_t n = (uint32_t)(t.R * t.R);
+    t.px = sm;
+    t.py = sm + n;
+    t.rx = sm + 2u * n;
+    t.ry = sm + 3u * n;
+    t.rho = sm + 4u * n;
+    t.bR = sm + 5u * n;
+    t.bD = sm + 6u * n;
+    t.ox = (int32_t)(blockIdx.x * T) - (int32_t)H;
+    t.oy = (int32_t)(blockIdx.y * T) - (int32_t)H;
+    return t;
+}
+// the tile's cells, 16 x 16 threads stepping over its rows and columns
+#define FOR_TILE(t, x, y) \
+    for (int32_t y = (int32_t)(threadIdx.x / 16u); y < (t).R; y += 16) \
+        for (int32_t x = (int32_t)(threadIdx.x % 16u); x < (t).R; x += 16)
+// the betas of each cell's right and lower faces, 0 where the stencil skips
+// the neighbour (faceBeta is 0 exactly when rho > 0 and the other is < 0)
+__device__ __forceinline__ void tileBetas(const LevelTile &t) {
+    FOR_TILE(t, x, y) {
+        int32_t j = y * t.R + x;
+        float r = t.rho[j];
+        t.bR[j] = x + 1 < t.R ? faceBeta(r, t.rho[j + 1]) : 0.0f;
+        t.bD[j] = y + 1 < t.R ? faceBeta(r, t.rho[j + t.R]) : 0.0f;
     }
 }
-__global__ void __launch_bounds__(64) k_prolongFine(Dev d, uint32_t cycle) {
-    // and the post-relaxation's rims (relaxRims, :2096-2107), as it leaves them
-    if (converged(d, cycle)) return;
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
-        if (!inWorld(d, c)) continue;
-        uint32_t i = 2u * cellIndex(d, c);
-        if (FLD(d, i, 2) > 0.0f) {
-            v2 e = prolonged(d, 0u, c);
-            float px = FLD(d, i, 0) + e.x, py = FLD(d, i + 1u, 0) + e.y;
-            FLD(d, i, 0) = px;
-            FLD(d, i + 1u, 0) = py;
-            CNT(d, i / 2u, G_RIM) = ubits(px);
-            CNT(d, i / 2u, G_RIM + 1u) = ubits(py);
-        }
-    }
-}
-
-// The coarse levels from lds_level down, all in one workgroup's LDS:
-// restrictLevel into each (the first from the level above it in the field),
-// a red and a black sweep on each but the coarsest (smoothRed, smoothBlack),
-// the coarsest's SOR sweeps (solve), and back up, prolong and the two sweeps
-// again (fluid.wgsl:2317-2497), then the levels written back to the field.
-// The arithmetic is restrictLevel's, smoothAt's, solve's and prolong's, in
-// the same order; only where the values wait between passes differs.
-struct LdsLevel { float *px, *py, *rx, *ry, *rho; uint32_t w, h; };
-
-__device__ __forceinline__ float ldsRho(const LdsLevel &L, i2 c) {
-    if (c.x < 0 || c.y < 0 || c.x >= (int32_t)L.w || c.y >= (int32_t)L.h) return -1.0f;
-    return L.rho[(uint32_t)c.y * L.w + (uint32_t)c.x];
-}
-__device__ __forceinline__ Stencil ldsStencil(const LdsLevel &L, i2 c, float rho) {
-    Stencil s;
-    s.sum = V2(0.0f, 0.0f);
-    s.diag = 0.0f;
+// smoothAt's stencil sums at tile cell j (neighbours in beside()'s order)
+struct TileSum { float sx, sy, diag; };
+__device__ __forceinline__ TileSum tileStencil(const LevelTile &t, int32_t j) {
+    const float b[4] = {t.bR[j - 1], t.bR[j], t.bD[j - t.R], t.bD[j]};
+    const int32_t nb[4] = {j - 1, j + 1, j - t.R, j + t.R};
+    TileSum s = {0.0f, 0.0f, 0.0f};
     for (int k = 0; k < 4; k++) {
-        i2 n = beside(c, k);
-        float r = ldsRho(L, n);
-        if (r < 0.0f) continue;
-        float beta = faceBeta(rho, r);
-        uint32_t j = (uint32_t)n.y * L.w + (uint32_t)n.x;
-        s.sum.x += beta * L.px[j];
-        s.sum.y += beta * L.py[j];
-        s.diag += beta;
+        if (b[k] == 0.0f) continue;
+        s.sx += b[k] * t.px[nb[k]];
+        s.sy += b[k] * t.py[nb[k]];
+        s.diag += b[k];
     }
     return s;
 }
-__device__ __forceinline__ v2 ldsResidual(const LdsLevel &L, uint32_t l, i2 c) {
-    uint32_t j = (uint32_t)c.y * L.w + (uint32_t)c.x;
-    float rho = L.rho[j];
-    if (rho <= 0.0f) return V2(0.0f, 0.0f);
-    Stencil s = ldsStencil(L, c, rho);
+__device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
     float h2 = (float)(1u << (2u * l));
-    return V2(L.rx[j] - (s.sum.x - (s.diag + 0.0f) * L.px[j]) / h2, L.ry[j] - (s.sum.y - (s.diag + 0.0f) * L.py[j]) / h2);
+    const int32_t half = (t.R - 1) / 2;   // cells of one colour in a row's inner R - 2, at most
+    for (uint32_t s = 0; s < 2u * pairs; s++) {
+        int color = (int)(s & 1u);
+        for (int32_t y = 1 + (int32_t)(threadIdx.x / 16u); y < t.R - 1; y += 16) {
+            int32_t par = (color + t.ox + t.oy + y + 1) & 1;   // x = 1 + par + 2 i has (ox + x + oy + y) & 1 == color
+            for (int32_t i = (int32_t)(threadIdx.x % 16u); i < half; i += 16) {
+                int32_t x = 1 + par + 2 * i;
+                if (x >= t.R - 1) continue;
+                int32_t j = y * t.R + x;
+                float rho = t.rho[j];
+                if (rho <= 0.0f) continue;
+                TileSum st = tileStencil(t, j);
+                if (st.diag <= 0.0f) continue;
+                float nx = (st.sx - h2 * t.rx[j]) / st.diag, ny = (st.sy - h2 * t.ry[j]) / st.diag;
+                t.px[j] = wmix(t.px[j], nx, 1.0f);
+                t.py[j] = wmix(t.py[j], ny, 1.0f);
+            }
+        }
+        __syncthreads();
+    }
 }
+__global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Dev d, uint32_t cycle, uint32_t l, uint32_t T) {
+    if (converged(d, cycle) || activeCount(d) == 0u) return;
+    extern __shared__ float sm[];
+    const uint32_t H = 2u * d.level_pairs + 1u;
+    LevelTile t = levelTile(sm, T, H);
+    float *resx = sm + 7u * (uint32_t)(t.R * t.R), *resy = resx + T * T;
+    FOR_TILE(t, x, y) {
+        int32_t j = y * t.R + x;
+        i2 c = I2(t.ox + x, t.oy + y);
+        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float rho = -1.0f;
+        if (inLevel(d, l, c)) {
+            uint64_t at = levelSlot(d, l, c);
+            e = *(const float4 *)&FLD(d, at, 0);
+            rho = FLD(d, at + 1u, 0);
+        }
+        t.px[j] = e.x;
+        t.py[j] = e.y;
+        t.rx[j] = e.z;
+        t.ry[j] = e.w;
+        t.rho[j] = rho;
+    }
+    __syncthreads();
+    tileBetas(t);
+    __syncthreads();
+    tileSweeps(t, l, d.level_pairs);
+    // p back, and each tile cell's residual (residualAt, :2306-2315)
+    float h2 = (float)(1u << (2u * l));
+    for (uint32_t k = threadIdx.x; k < T * T; k += TILE_THREADS) {
+        int32_t x = (int32_t)(k % T + H), y = (int32_t)(k / T + H), j = y * t.R + x;
+        i2 c = I2(t.ox + x, t.oy + y);
+        float qx = 0.0f, qy = 0.0f;
+        if (t.rho[j] > 0.0f) {
+            TileSum st = tileStencil(t, j);
+            qx = t.rx[j] - (st.sx - (st.diag + 0.0f) * t.px[j]) / h2;
+            qy = t.ry[j] - (st.sy - (st.diag + 0.0f) * t.py[j]) / h2;
+        }
+        resx[k] = qx;
+        resy[k] = qy;
+        if (inLevel(d, l, c)) *(float2 *)&FLD(d, levelSlot(d, l, c), 0) = make_float2(t.px[j], t.py[j]);
+    }
+    __syncthreads();
+    // restrictLevel (:2342-2377) into the tile's parents
+    const uint32_t T2 = T / 2u, lc = l + 1u;
+    const int32_t span = 1 << lc;
+    for (uint32_t k = threadIdx.x; k < T2 * T2; k += TILE_THREADS) {
+        i2 c = I2((int32_t)(blockIdx.x * T2 + k % T2), (int32_t)(blockIdx.y * T2 + k / T2));
+        if (!inLevel(d, lc, c)) continue;
+        float rho = 0.0f, ox = 0.0f, oy = 0.0f;
+        bool shortcut = false;
+        if (span <= WIG_BLOCK) {
+            i2 b = I2(c.x * span / WIG_BLOCK, c.y * span / WIG_BLOCK);
+            shortcut = inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u;
+        }
+        if (!shortcut) {
+            float fluid = 0.0f, sum = 0.0f;
+            bool air = false;
+            float sx = 0.0f, sy = 0.0f;
+            for (int q = 0; q < 4; q++) {
+                uint32_t cx = 2u * (k % T2) + (uint32_t)(q & 1), cy = 2u * (k / T2) + (uint32_t)(q >> 1);
+                float rk = t.rho[(int32_t)(cy + H) * t.R + (int32_t)(cx + H)];
+                if (rk == 0.0f) air = true;
+                else if (rk > 0.0f) {
+                    fluid += 1.0f;
+                    sum += rk;
+                    sx += resx[cy * T + cx];
+                    sy += resy[cy * T + cx];
+                }
+            }
+            rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
+            ox = rho > 0.0f ? sx * 0.25f : 0.0f;
+            oy = rho > 0.0f ? sy * 0.25f : 0.0f;
+        }
+        uint64_t at = levelSlot(d, lc, c);
+        *(float4 *)&FLD(d, at, 0) = make_float4(0.0f, 0.0f, ox, oy);
+        *(float4 *)&FLD(d, at + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
+    }
+}
+__global__ void __launch_bounds__(TILE_THREADS) k_levelUp(Dev d, uint32_t cycle, uint32_t l, uint32_t T) {
+    if (converged(d, cycle) || activeCount(d) == 0u) return;
+    extern __shared__ float sm[];
+    const uint32_t H = 2u * d.level_pairs;
+    LevelTile t = levelTile(sm, T, H);
+    // level l + 1 under the tile, from one cell before its first to one after its last
+    const int32_t cx0 = (t.ox >> 1) - 1, cy0 = (t.oy >> 1) - 1, Rc = t.R / 2 + 3;
+    float *cpx = sm + 7u * (uint32_t)(t.R * t.R), *cpy = cpx + Rc * Rc, *crho = cpy + Rc * Rc;
+    for (int32_t k = (int32_t)threadIdx.x; k < Rc * Rc; k += (int32_t)TILE_THREADS) {
+        i2 c = I2(cx0 + k % Rc, cy0 + k / Rc);
+        float px = 0.0f, py = 0.0f, rho = -1.0f;
+        if (inLevel(d, l + 1u, c)) {
+            uint64_t at = levelSlot(d, l + 1u, c);
+            float2 e = *(const float2 *)&FLD(d, at, 0);
+            px = e.x;
+            py = e.y;
+            rho = FLD(d, at + 1u, 0);
+        }
+        cpx[k] = px;
+        cpy[k] = py;
+        crho[k] = rho;
+    }
+    FOR_TILE(t, x, y) {
+        int32_t j = y * t.R + x;
+        i2 c = I2(t.ox + x, t.oy + y);
+        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float rho = -1.0f;
+        if (inLevel(d, l, c)) {
+            uint64_t at = levelSlot(d, l, c);
+            e = *(const float4 *)&FLD(d, at, 0);
+            rho = FLD(d, at + 1u, 0);
+        }
+        t.px[j] = e.x;
+        t.py[j] = e.y;
+        t.rx[j] = e.z;
+        t.ry[j] = e.w;
+        t.rho[j] = rho;
+    }
+    __syncthreads();
+    // prolong (:2459-2497): prolonged() from the staged coarse cells
+    FOR_TILE(t, x, y) {
+        int32_t j = y * t.R + x;
+        i2 c = I2(t.ox + x, t.oy + y);
+        float r0 = t.rho[j];
+        if (!(r0 > 0.0f)) continue;     // (outside the level rho is -1)
+        i2 coarse = I2(c.x / 2, c.y / 2);
+        i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
+        float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+        for (int k = 0; k < 4; k++) {
+            i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
+            float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
+            int32_t q = (coarse.y + o.y - cy0) * Rc + (coarse.x + o.x - cx0);
+            float r = crho[q];
+            if (r < 0.0f) continue;
+            weight += wk;
+            if (r > 0.0f) {
+                sx += wk * cpx[q];
+                sy += wk * cpy[q];
+            }
+        }
+        float dn = fmaxf(weight, 1e-6f);
+        t.px[j] = t.px[j] + sx / dn;
+        t.py[j] = t.py[j] + sy / dn;
+    }
+    tileBetas(t);
+    __syncthreads();
+    tileSweeps(t, l, d.level_pairs);
+    for (uint32_t k = threadIdx.x; k < T * T; k += TILE_THREADS) {
+        int32_t x = (int32_t)(k % T + H), y = (int32_t)(k / T + H), j = y * t.R + x;
+        i2 c = I2(t.ox + x, t.oy + y);
+        if (!inLevel(d, l, c)) continue;
+        *(float2 *)&FLD(d, levelSlot(d, l, c), 0) = make_float2(t.px[j], t.py[j]);
+    }
+}
+// The coarse levels from lds_level down, all in one workgroup's LDS:
+// a red and a black sweep on each but the coarsest (smoothRed, smoothBlack)
+// and restrictLevel into the next from its residuals, the coarsest's SOR
+// sweeps (solve), and back up, prolong and the two sweeps again
+// (fluid.wgsl:2317-2497), then the levels written back to the field.  The
+// first level comes from the field as the pass above left it.  The
+// arithmetic is restrictLevel's, smoothAt's, solve's and prolong's, in the
+// same order; only where the values wait between passes differs.  A level's
+// face betas (faceBeta, :369-374) are taken once, when its rho is known.
+struct LdsLevel { float *px, *py, *rx, *ry, *rho, *bR, *bD; uint32_t w, h; };
+
+// each cell's right and lower face betas, 0 where the stencil skips
+__device__ void ldsBetas(const LdsLevel &L) {
+    for (uint32_t y = threadIdx.x / 32u; y < L.h; y += 32u)
+        for (uint32_t x = threadIdx.x % 32u; x < L.w; x += 32u) {
+            uint32_t j = y * L.w + x;
+            float r = L.rho[j];
+            L.bR[j] = x + 1u < L.w ? faceBeta(r, L.rho[j + 1u]) : 0.0f;
+            L.bD[j] = y + 1u < L.h ? faceBeta(r, L.rho[j + L.w]) : 0.0f;
+        }
+}
+struct LdsSum { float sx, sy, diag; };
+__device__ __forceinline__ LdsSum ldsStencil(const LdsLevel &L, uint32_t x, uint32_t y) {
+    uint32_t j = y * L.w + x;
+    const float b[4] = {x > 0u ? L.bR[j - 1u] : 0.0f, L.bR[j], y > 0u ? L.bD[j - L.w] : 0.0f, L.bD[j]};
+    const uint32_t nb[4] = {j - 1u, j + 1u, j - L.w, j + L.w};
+    LdsSum s = {0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 4; k++) {
+        if (b[k] == 0.0f) continue;
+        s.sx += b[k] * L.px[nb[k]];
+        s.sy += b[k] * L.py[nb[k]];
+        s.diag += b[k];
+    }
+    return s;
+}
+// one half-sweep: the cells of one colour among the first `limit`, 32 x 32
+// threads, a thread a cell of the colour
 __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega, uint32_t limit) {
     float h2 = (float)(1u << (2u * l));
-    uint32_t n = L.w * L.h < limit ? L.w * L.h : limit;
-    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
-        i2 c = I2((int32_t)(j % L.w), (int32_t)(j / L.w));
-        if (((c.x + c.y) & 1) != color) continue;
-        float rho = L.rho[j];
-        if (rho <= 0.0f) continue;
-        Stencil s = ldsStencil(L, c, rho);
-        if (s.diag <= 0.0f) continue;
-        float sx = (s.sum.x - h2 * L.rx[j]) / s.diag, sy = (s.sum.y - h2 * L.ry[j]) / s.diag;
-        L.px[j] = wmix(L.px[j], sx, omega);
-        L.py[j] = wmix(L.py[j], sy, omega);
+    uint32_t hw = (L.w + 1u) / 2u;
+    for (uint32_t y = threadIdx.x / 32u; y < L.h; y += 32u) {
+        uint32_t par = ((uint32_t)color + y) & 1u;
+        for (uint32_t i = threadIdx.x % 32u; i < hw; i += 32u) {
+            uint32_t x = 2u * i + par, j = y * L.w + x;
+            if (x >= L.w || j >= limit) continue;
+            float rho = L.rho[j];
+            if (rho <= 0.0f) continue;
+            LdsSum s = ldsStencil(L, x, y);
+            if (s.diag <= 0.0f) continue;
+            float sx = (s.sx - h2 * L.rx[j]) / s.diag, sy = (s.sy - h2 * L.ry[j]) / s.diag;
+            L.px[j] = wmix(L.px[j], sx, omega);
+            L.py[j] = wmix(L.py[j], sy, omega);
+        }
     }
     __syncthreads();
 }
@@ -1217,7 +1427,7 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
     // level l's arrays in LDS, levels l0 .. L one after another
     auto level = [&](uint32_t l) {
         uint32_t off = 0;
-        for (uint32_t k = l0; k < l; k++) off += 5u * d.lw[k] * d.lh[k];
+        for (uint32_t k = l0; k < l; k++) off += LDS_LEVEL_FLOATS * d.lw[k] * d.lh[k];
         uint32_t n = d.lw[l] * d.lh[l];
         LdsLevel T;
         T.w = d.lw[l];
@@ -1227,65 +1437,75 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
         T.rx = sm + off + 2u * n;
         T.ry = sm + off + 3u * n;
         T.rho = sm + off + 4u * n;
+        T.bR = sm + off + 5u * n;
+        T.bD = sm + off + 6u * n;
         return T;
     };
     {
-        // level l0 as k_restrict left it in the field (or, with no blocks
-        // worked on, the coarsest as it was)
+        // level l0 as the pass above left it in the field (or, with no
+        // blocks worked on, the coarsest as it was)
         const LdsLevel T = level(l0);
-        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
-            uint64_t at = levelSlot(d, l0, I2((int32_t)(j % T.w), (int32_t)(j / T.w)));
-            float4 e = *(const float4 *)&FLD(d, at, 0);
-            T.px[j] = e.x; T.py[j] = e.y; T.rx[j] = e.z; T.ry[j] = e.w;
-            T.rho[j] = FLD(d, at + 1u, 0);
-        }
+        for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
+            for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
+                uint32_t j = y * T.w + x;
+                uint64_t at = levelSlot(d, l0, I2((int32_t)x, (int32_t)y));
+                float4 e = *(const float4 *)&FLD(d, at, 0);
+                T.px[j] = e.x; T.py[j] = e.y; T.rx[j] = e.z; T.ry[j] = e.w;
+                T.rho[j] = FLD(d, at + 1u, 0);
+            }
+        __syncthreads();
+        ldsBetas(T);
         __syncthreads();
     }
-    // down: restrictLevel (:2342-2377), then a red and a black sweep
-    for (uint32_t l = l0; l <= L && !idle; l++) {
-        const LdsLevel T = level(l);
-        int32_t span = 1 << l;
-        for (uint32_t j = threadIdx.x; j < T.w * T.h && l > l0; j += blockDim.x) {
-            i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
-            float rho = 0.0f, rxv = 0.0f, ryv = 0.0f;
-            bool shortcut = false;
-            if (span <= WIG_BLOCK) {
-                i2 b = I2(c.x * span / WIG_BLOCK, c.y * span / WIG_BLOCK);
-                shortcut = inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u;
-            }
-            if (!shortcut) {
-                float fluid = 0.0f, sum = 0.0f;
-                bool air = false;
-                v2 r = V2(0.0f, 0.0f);
-                for (int q = 0; q < 4; q++) {
-                    i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
-                    float rk;
-                    rk = ldsRho(level(l - 1u), k);
-                    if (rk == 0.0f) air = true;
-                    else if (rk > 0.0f) {
-                        fluid += 1.0f;
-                        sum += rk;
-                        v2 e = ldsResidual(level(l - 1u), l - 1u, k);
-                        r.x += e.x;
-                        r.y += e.y;
-                    }
-                }
-                rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
-                rxv = rho > 0.0f ? r.x * 0.25f : 0.0f;
-                ryv = rho > 0.0f ? r.y * 0.25f : 0.0f;
-            }
-            T.px[j] = 0.0f;
-            T.py[j] = 0.0f;
-            T.rx[j] = rxv;
-            T.ry[j] = ryv;
-            T.rho[j] = rho;
+    // down: the sweeps, then restrictLevel (:2342-2377) into the next level
+    for (uint32_t l = l0; l < L && !idle; l++) {
+        const LdsLevel T = level(l), U = level(l + 1u);
+        for (uint32_t k = 0; k < d.level_pairs; k++) {
+            ldsSmooth(T, l, 0, 1.0f, 0xffffffffu);
+            ldsSmooth(T, l, 1, 1.0f, 0xffffffffu);
         }
-        __syncthreads();
-        if (l < L)
-            for (uint32_t k = 0; k < d.level_pairs; k++) {
-                ldsSmooth(T, l, 0, 1.0f, 0xffffffffu);
-                ldsSmooth(T, l, 1, 1.0f, 0xffffffffu);
+        // restrictLevel: each child's residual (residualAt, :2306-2315) summed
+        const float h2 = (float)(1u << (2u * l));
+        const int32_t span = 1 << (l + 1u);
+        for (uint32_t y = threadIdx.x / 32u; y < U.h; y += 32u)
+            for (uint32_t x = threadIdx.x % 32u; x < U.w; x += 32u) {
+                float rho = 0.0f, rxv = 0.0f, ryv = 0.0f;
+                bool shortcut = false;
+                if (span <= WIG_BLOCK) {
+                    i2 b = I2((int32_t)x * span / WIG_BLOCK, (int32_t)y * span / WIG_BLOCK);
+                    shortcut = inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u;
+                }
+                if (!shortcut) {
+                    float fluid = 0.0f, sum = 0.0f, sx = 0.0f, sy = 0.0f;
+                    bool air = false;
+                    for (int q = 0; q < 4; q++) {
+                        uint32_t cx = 2u * x + (uint32_t)(q & 1), cy = 2u * y + (uint32_t)(q >> 1);
+                        if (cx >= T.w || cy >= T.h) continue;     // outside the level: rho -1
+                        uint32_t j = cy * T.w + cx;
+                        float rk = T.rho[j];
+                        if (rk == 0.0f) air = true;
+                        else if (rk > 0.0f) {
+                            fluid += 1.0f;
+                            sum += rk;
+                            LdsSum st = ldsStencil(T, cx, cy);
+                            sx += T.rx[j] - (st.sx - (st.diag + 0.0f) * T.px[j]) / h2;
+                            sy += T.ry[j] - (st.sy - (st.diag + 0.0f) * T.py[j]) / h2;
+                        }
+                    }
+                    rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
+                    rxv = rho > 0.0f ? sx * 0.25f : 0.0f;
+                    ryv = rho > 0.0f ? sy * 0.25f : 0.0f;
+                }
+                uint32_t j = y * U.w + x;
+                U.px[j] = 0.0f;
+                U.py[j] = 0.0f;
+                U.rx[j] = rxv;
+                U.ry[j] = ryv;
+                U.rho[j] = rho;
             }
+        __syncthreads();
+        ldsBetas(U);
+        __syncthreads();
     }
     // the coarsest (solve, :2407-2456): its first MG_COARSEST cells
     {
@@ -1295,30 +1515,31 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
     // up: prolong (:2459-2497), then a red and a black sweep
     for (int32_t l = (int32_t)L - 1; l >= (int32_t)l0 && !idle; l--) {
         const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
-        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
-            if (T.rho[j] <= 0.0f) continue;
-            i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
-            i2 coarse = I2(c.x / 2, c.y / 2);
-            i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
-            v2 sum = V2(0.0f, 0.0f);
-            float weight = 0.0f;
-            for (int k = 0; k < 4; k++) {
-                i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
-                float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
-                i2 at = I2(coarse.x + o.x, coarse.y + o.y);
-                float r = ldsRho(U, at);
-                if (r < 0.0f) continue;
-                weight += wk;
-                if (r > 0.0f) {
+        for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
+            for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
+                uint32_t j = y * T.w + x;
+                if (T.rho[j] <= 0.0f) continue;
+                i2 coarse = I2((int32_t)x / 2, (int32_t)y / 2);
+                i2 sg = I2((x & 1u) == 1u ? 1 : -1, (y & 1u) == 1u ? 1 : -1);
+                float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+                for (int k = 0; k < 4; k++) {
+                    i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
+                    float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
+                    i2 at = I2(coarse.x + o.x, coarse.y + o.y);
+                    if (at.x < 0 || at.y < 0 || at.x >= (int32_t)U.w || at.y >= (int32_t)U.h) continue;   // rho -1
                     uint32_t ja = (uint32_t)at.y * U.w + (uint32_t)at.x;
-                    sum.x += wk * U.px[ja];
-                    sum.y += wk * U.py[ja];
+                    float r = U.rho[ja];
+                    if (r < 0.0f) continue;
+                    weight += wk;
+                    if (r > 0.0f) {
+                        sx += wk * U.px[ja];
+                        sy += wk * U.py[ja];
+                    }
                 }
+                float dn = fmaxf(weight, 1e-6f);
+                T.px[j] = T.px[j] + sx / dn;
+                T.py[j] = T.py[j] + sy / dn;
             }
-            float dn = fmaxf(weight, 1e-6f);
-            T.px[j] = T.px[j] + sum.x / dn;
-            T.py[j] = T.py[j] + sum.y / dn;
-        }
         __syncthreads();
         for (uint32_t k = 0; k < d.level_pairs; k++) {
             ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu);
@@ -1328,11 +1549,13 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
     // the levels back into the field
     for (uint32_t l = l0; l <= L; l++) {
         const LdsLevel T = level(l);
-        for (uint32_t j = threadIdx.x; j < T.w * T.h; j += blockDim.x) {
-            uint64_t at = levelSlot(d, l, I2((int32_t)(j % T.w), (int32_t)(j / T.w)));
-            *(float4 *)&FLD(d, at, 0) = make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]);
-            if (!idle) *(float4 *)&FLD(d, at + 1u, 0) = make_float4(T.rho[j], 0.0f, 0.0f, 0.0f);
-        }
+        for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
+            for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
+                uint32_t j = y * T.w + x;
+                uint64_t at = levelSlot(d, l, I2((int32_t)x, (int32_t)y));
+                *(float4 *)&FLD(d, at, 0) = make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]);
+                if (!idle) *(float4 *)&FLD(d, at + 1u, 0) = make_float4(T.rho[j], 0.0f, 0.0f, 0.0f);
+            }
     }
 }
 
@@ -1345,18 +1568,20 @@ __global__ void k_setStep(StepParams *sp, uint32_t frame, uint32_t seed, float e
 }
 
 // ------------------------------------------------ conjugate gradients
-__device__ __forceinline__ float applyA(const Dev &d, i2 c, float rho) {     // :2595-2613
-    float sum = 0.0f, diag = 0.0f, share = densityAt(d, c);
+// applyA (:2595-2613) from the packed betas: every face but an obstacle's
+// counts in the diagonal, only a fluid neighbour's p in the sum
+__device__ __forceinline__ float applyA(const Dev &d, i2 c, uint32_t i, uint32_t f) {
+    float4 cf = d.coef[i];
+    const float v[4] = {cf.x, cf.y, cf.z, cf.w};
+    float sum = 0.0f, diag = 0.0f;
     for (int k = 0; k < 4; k++) {
-        i2 n = beside(c, k);
-        float r = rhoAt(d, 0u, n);
-        if (r < 0.0f) continue;
-        float beta = fineBeta(rho, r, share, densityAt(d, n));
-        if (r > 0.0f) sum += beta * CGV(d, cellIndex(d, n), 2);
+        float beta = fabsf(v[k]);
+        if (beta == 0.0f) continue;
+        if (__float_as_uint(v[k]) >> 31) sum += beta * CGV(d, cellIndex(d, beside(c, k)), 2);
         diag += beta;
     }
-    diag += pinnedAt(d, cellIndex(d, c)) ? PINF : 0.0f;
-    return diag * CGV(d, cellIndex(d, c), 2) - sum;
+    diag += (f & CF_PINNED) ? PINF : 0.0f;
+    return diag * CGV(d, i, 2) - sum;
 }
 
 __global__ void __launch_bounds__(64) k_cgStart(Dev d) {                     // :2665-2674
@@ -1447,18 +1672,44 @@ __global__ void __launch_bounds__(1024) k_sumNorm(Dev d) {
     sumPlaces2(d, 2, 2, s0, s1, &bb, &unused);
     if (threadIdx.x == 0) d.head->bb = bb;
 }
-// cgPrepare (:2721-2732).  The first iteration's first relaxation takes its
-// rims' drift from G_AIR_TO (free until rasterClaims writes it): the
-// pressure's rim is 0 there, as cgPrepare and cgUpdate leave it.
+// cgPrepare (:2721-2732), and the solve's fine level packed (the arrays'
+// layout is with the Dev struct).  p is 0 and the drift as it stands, the
+// rhs -r and the drift rhs; the ring of cells beside the blocks worked on
+// holds its field values, as the relaxations' rims read them there.
 __global__ void __launch_bounds__(64) k_cgPrepare(Dev d) {
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 bk = activeBlock(d, wid), c = blockCell(bk, threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
-        FLD(d, 2u * i, 0) = 0.0f;
-        if (FLD(d, 2u * i, 2) > 0.0f) {
-            FLD(d, 2u * i, 1) = -CGV(d, i, 1);
-            CNT(d, i, G_AIR_TO) = ubits(FLD(d, 2u * i + 1u, 0));
+        float rho = FLD(d, 2u * i, 2);
+        float2 v = make_float2(0.0f, FLD(d, 2u * i + 1u, 0));
+        d.pa[i] = v;
+        d.pb[i] = v;
+        uint32_t f = 0;
+        float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float2 r = make_float2(0.0f, 0.0f);
+        if (rho > 0.0f) {
+            f = CF_FLUID | (pinnedAt(d, i) ? CF_PINNED : 0u);
+            float dc = densityAt(d, c), b[4];
+            for (int k = 0; k < 4; k++) {
+                i2 n = beside(c, k);
+                float rn = gridZ(d, n), beta = fineBeta(rho, rn, dc, densityAt(d, n));
+                b[k] = rn > 0.0f ? -beta : beta;
+            }
+            cf = make_float4(b[0], b[1], b[2], b[3]);
+            r = make_float2(-CGV(d, i, 1), FLD(d, 2u * i + 1u, 1));
+        }
+        d.coef[i] = cf;
+        d.cflag[i] = (uint8_t)f;
+        d.r2[i] = r;
+        for (int k = 0; k < 4; k++) {
+            i2 n = beside(c, k);
+            if (!inWorld(d, n) || (idiv8(n).x == bk.x && idiv8(n).y == bk.y) || isActiveBlock(d, idiv8(n))) continue;
+            uint32_t ni = cellIndex(d, n);
+            float2 q = make_float2(FLD(d, 2u * ni, 0), FLD(d, 2u * ni + 1u, 0));
+            d.pa[ni] = q;
+            d.pb[ni] = q;
+            d.cflag[ni] = FLD(d, 2u * ni, 2) > 0.0f ? (uint8_t)CF_FLUID : (uint8_t)0;
         }
     }
 }
@@ -1468,9 +1719,9 @@ __global__ void __launch_bounds__(256) k_cgRho(Dev d, uint32_t cycle) {
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double v0 = 0.0, v1 = 0.0;
-        if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
+        if (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID)) {
             uint32_t i = cellIndex(d, c);
-            double z = (double)FLD(d, 2u * i, 0);
+            double z = (double)d.pa[i].x;
             v0 = (double)CGV(d, i, 1) * z;
             v1 = (double)CGV(d, i, 3) * z;
         }
@@ -1496,8 +1747,7 @@ __global__ void __launch_bounds__(1024) k_cgBeta(Dev d, uint32_t cycle) {
         CGV(d, d.C + d.B, 2) = (float)beta;
     }
 }
-// cgDirection (:2767-2776); and the drift as the iteration's relaxations
-// left it, the next iteration's first relaxation's rims (G_AIR_TO)
+// cgDirection (:2767-2776): z is the post-relaxation's p
 __global__ void __launch_bounds__(64) k_cgDirection(Dev d, uint32_t cycle) {
     if (converged(d, cycle)) return;
     double beta = ld_f64(&d.head->beta);
@@ -1505,10 +1755,8 @@ __global__ void __launch_bounds__(64) k_cgDirection(Dev d, uint32_t cycle) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
-        bool fluid = FLD(d, 2u * i, 2) > 0.0f;
-        float p = fluid ? (float)((double)FLD(d, 2u * i, 0) + beta * (double)CGV(d, i, 2)) : 0.0f;
-        CGV(d, i, 2) = p;
-        if (fluid) CNT(d, i, G_AIR_TO) = ubits(FLD(d, 2u * i + 1u, 0));
+        bool fluid = (d.cflag[i] & CF_FLUID) != 0u;
+        CGV(d, i, 2) = fluid ? (float)((double)d.pa[i].x + beta * (double)CGV(d, i, 2)) : 0.0f;
     }
 }
 // cgApply + cgAlpha (:2779-2814)
@@ -1518,9 +1766,8 @@ __global__ void __launch_bounds__(256) k_cgApply(Dev d, uint32_t cycle) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double v = 0.0;
         if (inWorld(d, c)) {
-            float rho = gridZ(d, c);
-            uint32_t i = cellIndex(d, c);
-            float q = rho > 0.0f ? applyA(d, c, rho) : 0.0f;
+            uint32_t i = cellIndex(d, c), f = d.cflag[i];
+            float q = (f & CF_FLUID) ? applyA(d, c, i, f) : 0.0f;
             CGV(d, i, 3) = q;
             v = (double)CGV(d, i, 2) * (double)q;
         }
@@ -1556,15 +1803,15 @@ __global__ void __launch_bounds__(256) k_cgUpdate(Dev d, uint32_t cycle) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double rr = 0.0;
         uint32_t most = 0;
-        if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
+        if (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID)) {
             uint32_t i = cellIndex(d, c);
             float4 v = *(const float4 *)&CGV(d, i, 0);
             float rn = (float)((double)v.y - alpha * (double)v.w);
             float xn = (float)((double)v.x + alpha * (double)v.z);
             CGV(d, i, 0) = xn;
             CGV(d, i, 1) = rn;
-            FLD(d, 2u * i, 0) = 0.0f;
-            FLD(d, 2u * i, 1) = -rn;
+            d.pa[i].x = 0.0f;
+            d.r2[i].x = -rn;
             most = ubits(fabsf(rn));
             rr = (double)rn * (double)rn;
         }
@@ -1610,14 +1857,26 @@ __global__ void __launch_bounds__(1024) k_cgCheck(Dev d, uint32_t cycle) {
         __hip_atomic_store(&h->converged, c2 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
-// cgFinish (:2884-2893)
-__global__ void __launch_bounds__(64) k_cgFinish(Dev d) {
+// cgFinish (:2884-2893), and the words of the reference's fine level the
+// packed solve kept aside: the rhs (-r), the drift, and, when an iteration
+// ran, the rims the last post-relaxation read (prolongFine's p)
+__global__ void __launch_bounds__(64) k_cgFinish(Dev d, int ran) {
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
         FLD(d, 2u * i, 0) = CGV(d, i, 0);
         CNT(d, i, G_REGION) = 0u;
+        if (d.cflag[i] & CF_FLUID) {
+            FLD(d, 2u * i, 1) = d.r2[i].x;
+            FLD(d, 2u * i + 1u, 0) = d.pa[i].y;
+            if (ran) {
+                float2 p = d.pb[i];
+                v2 e = prolonged(d, 0u, c);
+                CNT(d, i, G_RIM) = ubits(p.x + e.x);
+                CNT(d, i, G_RIM + 1u) = ubits(p.y + e.y);
+            }
+        }
     }
 }
 
