@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--kernel-table", action="store_true", help="also time every kernel over one step each")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default coarse-level tile sides")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration)")
     args = ap.parse_args()
@@ -124,6 +125,12 @@ def main():
     sim = config_simulation(world, size, args.tol)
     for _ in range(max(args.warmup, 1)):
         sim.step()
+    # the coarse-level tile sides, by the autotuner on the world's own steps
+    # (they do not change results)
+    tiles = {}
+    if not args.no_tune:
+        from worldinagrain_amd.tune import tune_tiles
+        tiles = tune_tiles(world, sim)
     torch.cuda.synchronize()
 
     # each kernel's time over one step (untimed steps), for the record
@@ -190,7 +197,8 @@ def main():
                                f"40% of rows at 4 particles/cell, dt 1/120, MG-PCG {1 + world.layout.levels} levels "
                                f"to {args.tol:g} relative residual",
                    "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
-                   "tol": args.tol, "parallelism": "single world, one GPU"},
+                   "tol": args.tol, "parallelism": "single world, one GPU",
+                   "tiles": {str(k): v for k, v in tiles.items()}},
         "solves_per_s": solves, "pcg_iterations_per_solve": d[1] / K,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
         "step_roofline_frac": step_roof / HBM_PEAK_GBS, "step_achieved_GBs": step_roof,

@@ -183,6 +183,14 @@ int wig_kernel_time(WigWorld *w, double *ms, uint32_t *launches);
 const char *wig_kernel_name(int id);
 int wig_kernel_count(void);
 
+/* The tile side (8, 16 or 32 cells) of the coarse-level sweep kernels on
+ * coarse level `level`, for the levels swept by tiles rather than in one
+ * workgroup (1 .. the level where that begins; WIG_E_INVALID on others).
+ * Results do not depend on it, only time does: the autotuner
+ * (worldinagrain_amd/tune.py) sets it.  This library's own; the reference
+ * sweeps a whole level with 8 x 8 workgroups (fluid.wgsl:2317-2387). */
+int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side);
+
 /* Waits for everything enqueued on the world's stream. */
 int wig_sync(WigWorld *w);
 

@@ -60,6 +60,45 @@ def test_discs_config_mode():
     run_pair("discs", W, H, cells, 6, mode="config", check_every=3)
 
 
+def test_tiled_levels_odd_size():
+    """330 x 250: coarse levels 1 and 2 (165 x 125, 83 x 63) are swept in
+    tiles with partial tiles at their edges, 3 and 4 in one workgroup."""
+    W, H = 330, 250
+    cells, _ = scenes.discs_dam(W, H, discs=40, rmin=3, rmax=12)
+    wo, wg = run_pair("tiled levels", W, H, cells, 4, mode="config", check_every=2)
+    with pytest.raises(Exception):
+        wg.set_tile(3, 16)          # level 3 is in the one-workgroup kernel
+
+
+def test_tiled_levels_reference_settings():
+    W, H = 300, 260
+    run_pair("tiled levels, reference", W, H, scenes.dam_break(W, H, block=(140, 150)), 6, check_every=3)
+
+
+def test_tile_sides_do_not_change_results():
+    """The tiles' halos make their sweeps the whole-level passes': any tile
+    side gives the same bits; so does a clone of a world stepped alike."""
+    W, H = 330, 250
+    cells, _ = scenes.discs_dam(W, H, discs=40, rmin=3, rmax=12)
+    wa, wb = make_gpu(W, H, cells), make_gpu(W, H, cells)
+    sa, sb = config_sim(wa, W, H), config_sim(wb, W, H)
+    wb.set_tile(1, 8)
+    wb.set_tile(2, 32)
+    for s in range(3):
+        sa.step()
+        sb.step()
+    diffs = compare_worlds(wa, wb)
+    assert not diffs, "; ".join(diffs)
+    wc = wa.clone()
+    import copy
+    sc = copy.copy(sa)
+    sc.world = wc
+    sa.step()
+    sc.step()
+    diffs = compare_worlds(wa, wc)
+    assert not diffs, "; ".join(diffs)
+
+
 def test_still_tank_gpu():
     W, H = 32, 32
     wo, wg = run_pair("still tank", W, H, scenes.still_tank(W, H, 14), 30, check_every=10)

@@ -420,6 +420,14 @@ int wig_kernel_time(WigWorld *w, double *ms, uint32_t *launches) {
     return WIG_OK;
 }
 
+int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side) {
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    if (level < 1u || level >= w->d.lds_level) return fail(WIG_E_INVALID, "level %u is not swept by tiles (1 .. %u)", level, w->d.lds_level - 1u);
+    if (side != 8u && side != 16u && side != 32u) return fail(WIG_E_INVALID, "tile side %u: 8, 16 or 32", side);
+    w->tile[level] = side;
+    return WIG_OK;
+}
+
 const char *wig_kernel_name(int id) { return id >= 0 && id < KID_COUNT ? g_kernel_names[id] : (id == KID_COUNT ? "solve" : ""); }
 int wig_kernel_count(void) { return KID_COUNT + 1; }
 

@@ -45,6 +45,7 @@ def lib():
         L.wig_kernel_name.restype = ct.c_char_p
         L.wig_kernel_name.argtypes = [ct.c_int]
         L.wig_kernel_count.restype = ct.c_int
+        L.wig_set_tile.argtypes = [ct.c_void_p, ct.c_uint32, ct.c_uint32]
         _lib = L
     return _lib
 
@@ -70,6 +71,7 @@ class GpuWorld:
             raise WigError(f"wig_create failed: {L.wig_last_error().decode()}")
         self.layout = abi.WigLayoutInfo()
         _check(L.wig_layout(self._w, ct.byref(self.layout)))
+        self.kinds = None
         if kinds is not None:
             self.set_kinds(kinds)
 
@@ -88,6 +90,7 @@ class GpuWorld:
         k = np.ascontiguousarray(kinds, dtype=np.uint8)
         assert k.shape == (256,)
         _check(lib().wig_set_kinds(self._w, k.tobytes()))
+        self.kinds = k
 
     def part_bytes(self, p):
         return int(self.layout.part_bytes[p])
@@ -140,6 +143,17 @@ class GpuWorld:
             _check(lib().wig_time_kernel(self._w, -1))
             return
         _check(lib().wig_time_kernel(self._w, kernel_id(name)))
+
+    def set_tile(self, level, side):
+        """The coarse-level sweep kernels' tile side on `level` (wig_set_tile)."""
+        _check(lib().wig_set_tile(self._w, level, side))
+
+    def clone(self):
+        """A new world of the same size and kinds holding this one's parts."""
+        w = GpuWorld(self.layout.width, self.layout.height, coarse_levels=self.layout.levels, kinds=self.kinds)
+        for p in range(abi.PART_COUNT):
+            w.upload(p, self.download(p))
+        return w
 
     def kernel_time(self):
         """(ms summed, launches) of the timed kernel since time_kernel."""

@@ -1,0 +1,76 @@
+"""The autotuner: the coarse-level sweep kernels' tile sides and the
+smoother's sweep counts, chosen by timing them on the world they are for.
+
+Tile sides do not change results (each tile's halo makes its sweeps equal
+the whole-level passes), so they are tuned on the world's own steps.  Sweep
+counts change the iteration and its results, so each candidate runs on a
+copy of the world (its parts copied into a scratch world) from the same
+state, and is scored by the solve's time a step at the solver's own
+tolerance; the world itself is not stepped."""
+import copy
+import dataclasses
+
+from .gpu import WigError
+
+TILE_SIDES = (8, 16, 32)
+SWEEP_CANDIDATES = [(sp, lp, cs) for sp in (1, 2) for lp in (1, 2) for cs in (5, 10)]
+
+
+def _kernel_ms_per_iteration(world, sim, name, steps):
+    t0 = world.run_totals()
+    world.time_kernel(name)
+    for _ in range(steps):
+        sim.step()
+    ms, _ = world.kernel_time()
+    world.time_kernel(None)
+    cycles = world.run_totals().cycles - t0.cycles
+    return ms / max(cycles, 1)
+
+
+def tune_tiles(world, sim, sides=TILE_SIDES, steps=2):
+    """Each tiled coarse level's side in turn, by the tile kernels' time an
+    iteration over `steps` of the world's steps.  Returns {level: side}."""
+    chosen = {}
+    level = 1
+    while True:
+        try:
+            world.set_tile(level, sides[0])
+        except WigError:
+            break       # past the tiled levels
+        best = None
+        for side in sides:
+            world.set_tile(level, side)
+            ms = sum(_kernel_ms_per_iteration(world, sim, k, steps) for k in ("levelDown", "levelUp"))
+            if best is None or ms < best[0]:
+                best = (ms, side)
+        world.set_tile(level, best[1])
+        chosen[level] = best[1]
+        level += 1
+    return chosen
+
+
+def tune_sweeps(world, sim, candidates=SWEEP_CANDIDATES, steps=3):
+    """(sweep_pairs, level_pairs, coarse_sweeps) by the solve's time a step
+    on copies of the world.  Returns (best settings, [(ms a step, iterations
+    a step, candidate), ...]); `sim.solver` is left as it was."""
+    scores = []
+    for sp, lp, cs in candidates:
+        w = world.clone()
+        try:
+            s = copy.copy(sim)
+            s.world = w
+            s.fluid_stats = dict(sim.fluid_stats)
+            s.solver = dataclasses.replace(sim.solver, sweep_pairs=sp, level_pairs=lp, coarse_sweeps=cs)
+            s.step()            # its first step finds the iteration count the chunks start from
+            t0 = w.run_totals()
+            w.time_kernel("solve")
+            for _ in range(steps):
+                s.step()
+            ms, _ = w.kernel_time()
+            w.time_kernel(None)
+            its = (w.run_totals().cycles - t0.cycles) / steps
+            scores.append((ms / steps, its, (sp, lp, cs)))
+        finally:
+            w.close()
+    best = min(scores)[2]
+    return dataclasses.replace(sim.solver, sweep_pairs=best[0], level_pairs=best[1], coarse_sweeps=best[2]), scores
