@@ -194,6 +194,26 @@ int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side);
 /* Waits for everything enqueued on the world's stream. */
 int wig_sync(WigWorld *w);
 
+/* Batches: many worlds, of any sizes, stepped together, every kernel of the
+ * step one launch over all of them (this library's own; the reference steps
+ * one Simulation a frame).  The worlds stay the caller's and must outlive
+ * the batch; a world's results do not depend on the batch it is stepped in
+ * (each runs the solver iterations its own host would encode).  The batch
+ * runs on the first world's stream unless set otherwise; a world stepped
+ * both alone and in batches is the caller's to order. */
+typedef struct WigBatch WigBatch;
+WigBatch *wig_batch_create(WigWorld *const *worlds, uint32_t n);
+int wig_batch_destroy(WigBatch *b);
+int wig_batch_set_stream(WigBatch *b, void *stream);
+/* One liquid step of each world i, with params[i] and ctl[i] (as wig_fluid_step). */
+int wig_batch_step(WigBatch *b, const WigParams *params, const WigStepCtl *ctl);
+/* stats[i * WIG_STATS ...]: each world's report (as wig_report).  Waits. */
+int wig_batch_report(WigBatch *b, uint32_t *stats);
+int wig_batch_sync(WigBatch *b);
+/* Kernel timing over the batch's launches, as wig_time_kernel / wig_kernel_time. */
+int wig_batch_time_kernel(WigBatch *b, int id);
+int wig_batch_kernel_time(WigBatch *b, double *ms, uint32_t *launches);
+
 /* The last error on this thread, "" if none. */
 const char *wig_last_error(void);
 

@@ -1,0 +1,86 @@
+"""Worlds stepped together in one batch (wig_batch_step) against the same
+worlds stepped alone and against the oracle: each world's every buffer must
+be bit-identical whatever batch it runs in.  The batch mixes sizes (some
+with coarse levels swept in tiles), the reference's solver settings and the
+relative-residual ones, and an idle world."""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+from worldinagrain_amd import abi, scenes
+from worldinagrain_amd.host import Simulation, step_batch
+from tests.helpers import make_oracle, make_gpu, compare_worlds, config_sim
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def worlds_spec():
+    discs, _ = scenes.discs_dam(330, 250, discs=40, rmin=3, rmax=12)
+    small, _ = scenes.discs_dam(128, 96, discs=10, rmin=3, rmax=8)
+    return [
+        ("dam 64x48 reference", 64, 48, scenes.dam_break(64, 48, block=(30, 36)), "reference"),
+        ("odd 61x37 reference", 61, 37, scenes.dam_break(61, 37, block=(29, 30)), "reference"),
+        ("discs 330x250 relative", 330, 250, discs, "config"),
+        ("idle 32x32", 32, 32, scenes.empty_cells(32, 32), "reference"),
+        ("discs 128x96 relative", 128, 96, small, "config"),
+    ]
+
+
+def make_sim(world, W, H, mode):
+    return Simulation(world, W, H) if mode == "reference" else config_sim(world, W, H)
+
+
+def test_batch_matches_alone_and_oracle():
+    from worldinagrain_amd.gpu import GpuBatch
+    spec = worlds_spec()
+    batched, alone, oracles = [], [], []
+    for name, W, H, cells, mode in spec:
+        wb, wa, wo = make_gpu(W, H, cells), make_gpu(W, H, cells), make_oracle(W, H, cells)
+        batched.append((wb, make_sim(wb, W, H, mode)))
+        alone.append((wa, make_sim(wa, W, H, mode)))
+        oracles.append((wo, make_sim(wo, W, H, mode)))
+    batch = GpuBatch([w for w, _ in batched])
+    steps = 6
+    for s in range(steps):
+        say(f"[batch] step {s}")
+        step_batch(batch, [sim for _, sim in batched])
+        for (wa, sa), (wo, so) in zip(alone, oracles):
+            sa.step()
+            so.step()
+        for i, (name, *_rest) in enumerate(spec):
+            sb, sa, so = batched[i][1], alone[i][1], oracles[i][1]
+            assert np.array_equal(sb.last_stats, sa.last_stats), (name, s, sb.last_stats, sa.last_stats)
+            assert np.array_equal(sb.last_stats, so.last_stats), (name, s, sb.last_stats, so.last_stats)
+    for i, (name, *_rest) in enumerate(spec):
+        d = compare_worlds(batched[i][0], alone[i][0])
+        assert not d, f"{name} batch vs alone: " + "; ".join(d)
+        d = compare_worlds(batched[i][0], oracles[i][0])
+        assert not d, f"{name} batch vs oracle: " + "; ".join(d)
+    batch.close()
+
+
+def test_batch_of_one_and_regrouping():
+    """A world moves between batches of different company and still steps
+    as it would alone."""
+    from worldinagrain_amd.gpu import GpuBatch
+    W, H = 96, 80
+    cells, _ = scenes.discs_dam(W, H, discs=8, rmin=3, rmax=8)
+    wa, wb = make_gpu(W, H, cells), make_gpu(W, H, cells)
+    other = make_gpu(64, 48, scenes.dam_break(64, 48, block=(30, 36)))
+    sa, sb, so = config_sim(wa, W, H), config_sim(wb, W, H), Simulation(other, 64, 48)
+    b1 = GpuBatch([wb])
+    b2 = GpuBatch([other, wb])
+    for s in range(6):
+        sa.step()
+        if s % 2 == 0:
+            step_batch(b1, [sb])
+        else:
+            step_batch(b2, [so, sb])
+    d = compare_worlds(wa, wb)
+    assert not d, "; ".join(d)

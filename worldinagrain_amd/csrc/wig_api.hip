@@ -9,56 +9,58 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/wig.h"
 #include "wig_device.h"
 
 // the kernels (wig_kernels.hip)
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
-__global__ void k_setStep(StepParams *sp, uint32_t frame, uint32_t seed, float earthG);
-__global__ void k_sync(Dev d, int claims);
-__global__ void k_syncParticles(Dev d);
-__global__ void k_syncSeed(Dev d);
-__global__ void k_compact(Dev d);
-__global__ void k_retire(Dev d);
-__global__ void k_clear(Dev d);
-__global__ void k_gather(Dev d);
-__global__ void k_localJoin(Dev d, int mode);
-__global__ void k_edgeJoin(Dev d, int mode);
-__global__ void k_sealAnchor(Dev d);
-__global__ void k_classify(Dev d);
-__global__ void k_setFaces(Dev d);
-__global__ void k_divergence(Dev d);
-__global__ void k_relax(Dev d, uint32_t cycle, int shift, int post);
-__global__ void k_restrict(Dev d, uint32_t cycle);
-__global__ void k_levelRho(Dev d);
-__global__ void k_levelDown(Dev d, uint32_t cycle, uint32_t l, uint32_t T);
-__global__ void k_coarseLDS(Dev d, uint32_t cycle);
-__global__ void k_levelUp(Dev d, uint32_t cycle, uint32_t l, uint32_t T);
-__global__ void k_cgStart(Dev d);
-__global__ void k_cgAnchor(Dev d);
-__global__ void k_cgResidual(Dev d);
-__global__ void k_sumNorm(Dev d);
-__global__ void k_cgPrepare(Dev d);
-__global__ void k_cgRho(Dev d, uint32_t cycle);
-__global__ void k_cgBeta(Dev d, uint32_t cycle);
-__global__ void k_cgDirection(Dev d, uint32_t cycle);
-__global__ void k_cgApply(Dev d, uint32_t cycle);
-__global__ void k_cgAlpha(Dev d, uint32_t cycle);
-__global__ void k_cgUpdate(Dev d, uint32_t cycle);
-__global__ void k_cgCheck(Dev d, uint32_t cycle);
-__global__ void k_cgFinish(Dev d, int ran);
-__global__ void k_project(Dev d);
-__global__ void k_extrapolate(Dev d, int layer);
-__global__ void k_advect(Dev d);
-__global__ void k_sortCount(Dev d);
-__global__ void k_scanBlocks(Dev d);
-__global__ void k_scanTotals(Dev d);
-__global__ void k_scanFinish(Dev d);
-__global__ void k_scatter(Dev d);
-__global__ void k_rasterClaims(Dev d);
-__global__ void k_raster(Dev d);
-__global__ void k_unpaint(Dev d);
+__global__ void k_sync(WorldGrid g);
+__global__ void k_syncParticles(WorldGrid g);
+__global__ void k_syncSeed(WorldGrid g);
+__global__ void k_compact(WorldGrid g);
+__global__ void k_retire(WorldGrid g);
+__global__ void k_clear(WorldGrid g);
+__global__ void k_gather(WorldGrid g);
+__global__ void k_localJoin(WorldGrid g, int mode);
+__global__ void k_edgeJoin(WorldGrid g, int mode);
+__global__ void k_sealAnchor(WorldGrid g);
+__global__ void k_classify(WorldGrid g);
+__global__ void k_setFaces(WorldGrid g);
+__global__ void k_divergence(WorldGrid g);
+__global__ void k_relax(WorldGrid g, uint32_t cycle, int post);
+__global__ void k_restrict(WorldGrid g, uint32_t cycle);
+__global__ void k_levelRho(WorldGrid g);
+__global__ void k_levelDown(WorldGrid g, uint32_t cycle, uint32_t l);
+__global__ void k_coarseLDS(WorldGrid g, uint32_t cycle);
+__global__ void k_levelUp(WorldGrid g, uint32_t cycle, uint32_t l);
+__global__ void k_cgStart(WorldGrid g);
+__global__ void k_cgAnchor(WorldGrid g);
+__global__ void k_cgResidual(WorldGrid g);
+__global__ void k_sumNorm(WorldGrid g);
+__global__ void k_cgPrepare(WorldGrid g);
+__global__ void k_cgRho(WorldGrid g, uint32_t cycle);
+__global__ void k_cgBeta(WorldGrid g, uint32_t cycle);
+__global__ void k_cgDirection(WorldGrid g, uint32_t cycle);
+__global__ void k_cgApply(WorldGrid g, uint32_t cycle);
+__global__ void k_cgAlpha(WorldGrid g, uint32_t cycle);
+__global__ void k_cgUpdate(WorldGrid g, uint32_t cycle);
+__global__ void k_cgCheck(WorldGrid g, uint32_t cycle);
+__global__ void k_cgFinish(WorldGrid g);
+__global__ void k_project(WorldGrid g);
+__global__ void k_extrapolate(WorldGrid g, int layer);
+__global__ void k_advect(WorldGrid g);
+__global__ void k_sortCount(WorldGrid g);
+__global__ void k_scanBlocks(WorldGrid g);
+__global__ void k_scanTotals(WorldGrid g);
+__global__ void k_scanFinish(WorldGrid g);
+__global__ void k_scatter(WorldGrid g);
+__global__ void k_rasterClaims(WorldGrid g);
+__global__ void k_raster(WorldGrid g);
+__global__ void k_unpaint(WorldGrid g);
+__global__ void k_stops(WorldGrid g, uint32_t *out);
+__global__ void k_reports(WorldGrid g, uint32_t *out);
 
 static thread_local std::string g_err;
 
@@ -79,7 +81,7 @@ static int fail(int code, const char *fmt, ...) {
     } while (0)
 
 // Kernel ids for wig_time_kernel, in launch order.
-#define WIG_KERNELS(X) X(setStep) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
+#define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
     X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
     X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
     X(cgRho) X(cgBeta) X(cgDirection) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
@@ -91,39 +93,17 @@ enum { WIG_KERNELS(X_ENUM) KID_COUNT };
 static const char *g_kernel_names[] = {WIG_KERNELS(X_NAME)};
 #define MAX_TIMED 8192
 
-struct WigWorld {
-    Dev d;                       // sizes and device pointers
-    uint64_t bytes[WIG_PART_COUNT];
-    void *part[WIG_PART_COUNT];
-    uint8_t *kind;               // 256 element kinds
-    uint32_t *scratch;           // B words
-    void *fine;                  // the solve's packed fine level (Dev::coef ... r2)
-    unsigned long long *totals;  // WigRunTotals on the device
-    StepParams *sp;              // the step's Params words
-    size_t lds_bytes;            // k_coarseLDS's dynamic LDS
-    uint32_t tile[9];            // k_levelDown/k_levelUp's tile side per level
-    uint32_t *flag_host;         // pinned: the solve's iterations and convergence flag, read between chunks
-    uint32_t last_cycles;        // iterations the last chunked solve ran
-    hipStream_t own, stream;
-    int device;
-    int timed;                   // kernel id timed, -1 none
-    uint32_t ntimed;
-    hipEvent_t ev[2 * MAX_TIMED];
-    bool ev_made;
+// Launch shapes: how many workgroups a kernel takes for one world.
+enum {
+    SH_ONE,      // one (the single-workgroup passes)
+    SH_BG,       // 64-thread passes striding over the active blocks
+    SH_BG4,      // 256-thread passes, four blocks a workgroup
+    SH_PG,       // 256-thread passes over the particles
+    SH_WORLD,    // one workgroup a block, bw x bh
+    SH_LVL1,     // 8 x 8 cells of coarse level 1 a workgroup
+    SH_TILE,     // + l - 1: k_levelDown/k_levelUp's tiles on level l
+    SH_COUNT = SH_TILE + MG_LEVELS
 };
-
-// A launch, bracketed by events when its kernel is the one being timed.
-template <typename... KArgs, typename... Args>
-static void launch_lds(WigWorld *w, int id, void (*k)(KArgs...), dim3 grid, dim3 block, size_t lds, Args... args) {
-    bool t = w->timed == id && w->ntimed < MAX_TIMED;
-    if (t) hipEventRecord(w->ev[2 * w->ntimed], w->stream);
-    hipLaunchKernelGGL(k, grid, block, lds, w->stream, args...);
-    if (t) hipEventRecord(w->ev[2 * w->ntimed + 1], w->stream), w->ntimed++;
-}
-template <typename... KArgs, typename... Args>
-static void launch(WigWorld *w, int id, void (*k)(KArgs...), dim3 grid, dim3 block, Args... args) {
-    launch_lds(w, id, k, grid, block, 0, args...);
-}
 
 static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 
@@ -131,20 +111,101 @@ static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 // one cell a half-sweep (k_levelDown one more), in LDS 7 floats a cell, and
 // k_levelDown's residuals (2 a tile cell) or k_levelUp's coarse cells (3)
 #define MAX_LEVEL_PAIRS 8u
-static dim3 tileGrid(const Dev &d, uint32_t l, uint32_t T) { return dim3(ceil_div(d.lw[l], T), ceil_div(d.lh[l], T)); }
-static size_t downLds(const Dev &d, uint32_t T) {
-    size_t R = T + 2u * (2u * d.level_pairs + 1u);
+static size_t downLds(uint32_t pairs, uint32_t T) {
+    size_t R = T + 2u * (2u * pairs + 1u);
     return 4u * (7u * R * R + 2u * T * T);
 }
-static size_t upLds(const Dev &d, uint32_t T) {
-    size_t R = T + 4u * d.level_pairs, Rc = R / 2u + 3u;
+static size_t upLds(uint32_t pairs, uint32_t T) {
+    size_t R = T + 4u * pairs, Rc = R / 2u + 3u;
     return 4u * (7u * R * R + 3u * Rc * Rc);
 }
-// the world's tile for level l, narrowed to fit 64 KB of LDS
-static uint32_t levelTileSide(const WigWorld *w, const Dev &d, uint32_t l) {
-    uint32_t T = w->tile[l];
-    while (T > 8u && std::max(downLds(d, T), upLds(d, T)) > 65536u) T /= 2u;
+// a tile side for level_pairs sweep pairs, narrowed to fit 64 KB of LDS
+static uint32_t fitTile(uint32_t T, uint32_t pairs) {
+    while (T > 8u && std::max(downLds(pairs, T), upLds(pairs, T)) > 65536u) T /= 2u;
     return T;
+}
+
+// Grid of the passes over the blocks worked on: they stride over the active
+// list, so a few workgroups a CU cover any count.
+static uint32_t block_grid(const Dev &d) { return std::min<uint32_t>(d.B + d.bw + d.bh + 1u, 4096u); }
+static uint32_t particle_grid(const Dev &d) { return std::min<uint32_t>((d.cap + 255u) / 256u, 4096u); }
+
+static uint32_t shape_count(const Dev &d, int sh) {
+    switch (sh) {
+    case SH_ONE: return 1u;
+    case SH_BG: return block_grid(d);
+    case SH_BG4: return std::min<uint32_t>((d.B + 3u) / 4u, 1024u);
+    case SH_PG: return particle_grid(d);
+    case SH_WORLD: return d.bw * d.bh;
+    case SH_LVL1: return ceil_div(d.lw[1], 8u) * ceil_div(d.lh[1], 8u);
+    default: {
+        uint32_t l = (uint32_t)(sh - SH_TILE) + 1u;
+        if (l >= d.lds_level) return 0u;
+        return ceil_div(d.lw[l], d.tile[l]) * ceil_div(d.lh[l], d.tile[l]);
+    }
+    }
+}
+
+struct Batch;
+
+struct WigWorld {
+    Dev d;                       // sizes and device pointers (the step's words are filled per step)
+    uint64_t bytes[WIG_PART_COUNT];
+    void *part[WIG_PART_COUNT];
+    uint8_t *kind;               // 256 element kinds
+    uint32_t *scratch;           // B words
+    void *fine;                  // the solve's packed fine level (Dev::coef ... r2)
+    unsigned long long *totals;  // WigRunTotals on the device
+    uint32_t *wstate;            // WS_* words on the device
+    size_t lds_bytes;            // k_coarseLDS's dynamic LDS
+    uint32_t tile[MG_LEVELS + 1];   // tile side asked per level (wig_set_tile)
+    uint32_t last_hint;          // iterations its last solve ran, as the host saw (when to check first)
+    int device;
+    hipStream_t own;
+    Batch *solo;                 // the world alone, as wig_fluid_step steps it
+};
+
+// Worlds stepped together: the device table of their Dev, the launch
+// shapes' workgroup maps, and the stream and timing of their launches.
+struct Batch {
+    std::vector<WigWorld *> w;
+    std::vector<Dev> htab;
+    Dev *tab = nullptr;
+    uint32_t *maps = nullptr;    // per shape: map (a world index a workgroup), then first (n + 1)
+    size_t maps_words = 0;
+    uint64_t map_off[SH_COUNT] = {}, first_off[SH_COUNT] = {};
+    uint32_t total[SH_COUNT] = {};
+    std::vector<uint32_t> sig;   // the tile sides the maps were built for
+    uint32_t *stops = nullptr, *stops_host = nullptr;   // k_stops' words (device, pinned host)
+    uint32_t *reports = nullptr;
+    hipStream_t stream = nullptr;
+    int timed = -1;              // kernel id timed, -1 none
+    uint32_t ntimed = 0;
+    hipEvent_t ev[2 * MAX_TIMED];
+    bool ev_made = false;
+};
+
+static WorldGrid grid_of(const Batch &b, int sh) {
+    WorldGrid g;
+    g.tab = b.tab;
+    g.map = b.maps + b.map_off[sh];
+    g.first = b.maps + b.first_off[sh];
+    return g;
+}
+
+// A launch over the batch in shape `sh`, bracketed by events when its kernel
+// is the one being timed.
+template <typename... KArgs, typename... Args>
+static void launch_lds(Batch &b, int id, void (*k)(KArgs...), int sh, uint32_t threads, size_t lds, Args... args) {
+    if (b.total[sh] == 0u) return;
+    bool t = b.timed == id && b.ntimed < MAX_TIMED;
+    if (t) (void)hipEventRecord(b.ev[2 * b.ntimed], b.stream);
+    hipLaunchKernelGGL(k, dim3(b.total[sh]), dim3(threads), lds, b.stream, grid_of(b, sh), args...);
+    if (t) (void)hipEventRecord(b.ev[2 * b.ntimed + 1], b.stream), b.ntimed++;
+}
+template <typename... KArgs, typename... Args>
+static void launch(Batch &b, int id, void (*k)(KArgs...), int sh, uint32_t threads, Args... args) {
+    launch_lds(b, id, k, sh, threads, 0, args...);
 }
 
 // The world's sizes and parts (js/sim.js:741-786; fluid.wgsl:2228-2248).
@@ -183,6 +244,7 @@ static void size_world(WigWorld *w, uint32_t width, uint32_t height, uint32_t co
     w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * d.B;
 }
 
+
 extern "C" {
 
 const char *wig_last_error(void) { return g_err.c_str(); }
@@ -193,12 +255,69 @@ int wig_device_ok(void) {
     return 1;
 }
 
-static int zero(WigWorld *w, void *p, uint64_t bytes) {
+static int zero(hipStream_t s, void *p, uint64_t bytes) {
     uint64_t words = bytes / 4u;
     if (!words) return WIG_OK;
     uint32_t grid = (uint32_t)std::min<uint64_t>((words + 255u) / 256u, 65535u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, w->stream, (uint32_t *)p, words, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, s, (uint32_t *)p, words, 0u);
     HIPCHECK(hipGetLastError());
+    return WIG_OK;
+}
+
+static void batch_free(Batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->tab) (void)hipFree(b->tab);
+    if (b->maps) (void)hipFree(b->maps);
+    if (b->stops) (void)hipFree(b->stops);
+    if (b->stops_host) (void)hipHostFree(b->stops_host);
+    if (b->reports) (void)hipFree(b->reports);
+    if (b->ev_made)
+        for (int i = 0; i < 2 * MAX_TIMED; i++) (void)hipEventDestroy(b->ev[i]);
+    delete b;
+}
+
+static Batch *batch_new(WigWorld *const *worlds, uint32_t n, hipStream_t stream) {
+    Batch *b = new Batch();
+    b->w.assign(worlds, worlds + n);
+    b->htab.resize(n);
+    b->stream = stream;
+    if (hipMalloc((void **)&b->tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&b->stops, 8ull * n) != hipSuccess
+        || hipHostMalloc((void **)&b->stops_host, 8ull * n, hipHostMallocDefault) != hipSuccess
+        || hipMalloc((void **)&b->reports, 4ull * FLUID_STATS * n) != hipSuccess) {
+        fail(WIG_E_NOMEM, "batch allocation failed");
+        batch_free(b);
+        return nullptr;
+    }
+    return b;
+}
+
+// The shapes' workgroup maps for the worlds' sizes and tile sides.
+static int batch_maps(Batch &b) {
+    std::vector<uint32_t> sig;
+    for (WigWorld *w : b.w)
+        for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(w->d.tile[l]);
+    if (b.maps && sig == b.sig) return WIG_OK;
+    uint32_t n = (uint32_t)b.w.size();
+    std::vector<uint32_t> words;
+    for (int sh = 0; sh < SH_COUNT; sh++) {
+        std::vector<uint32_t> first(n + 1u, 0u);
+        for (uint32_t i = 0; i < n; i++) first[i + 1u] = first[i] + shape_count(b.w[i]->d, sh);
+        b.total[sh] = first[n];
+        b.map_off[sh] = words.size();
+        for (uint32_t i = 0; i < n; i++) words.insert(words.end(), first[i + 1u] - first[i], i);
+        b.first_off[sh] = words.size();
+        words.insert(words.end(), first.begin(), first.end());
+    }
+    if (words.size() > b.maps_words) {
+        if (b.maps) HIPCHECK(hipFree(b.maps));
+        b.maps = nullptr;
+        if (hipMalloc((void **)&b.maps, 4ull * words.size()) != hipSuccess) return fail(WIG_E_NOMEM, "batch maps: %zu words", words.size());
+        b.maps_words = words.size();
+    }
+    HIPCHECK(hipMemcpyAsync(b.maps, words.data(), 4ull * words.size(), hipMemcpyHostToDevice, b.stream));
+    HIPCHECK(hipStreamSynchronize(b.stream));
+    b.sig = sig;
     return WIG_OK;
 }
 
@@ -213,19 +332,12 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         return nullptr;
     }
     WigWorld *w = new WigWorld();
-    memset(&w->d, 0, sizeof w->d);
     size_world(w, width, height, coarse_levels);
-    if (w->d.lw[w->d.L] * w->d.lh[w->d.L] > (uint32_t)MG_COARSEST) {
-        fail(WIG_E_INVALID, "the coarsest level (%ux%u) holds more than %d cells", w->d.lw[w->d.L], w->d.lh[w->d.L], MG_COARSEST);
+    if (hipGetDevice(&w->device) != hipSuccess || hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking) != hipSuccess) {
+        fail(WIG_E_DEVICE, "no HIP device or stream");
         delete w;
         return nullptr;
     }
-    if (hipGetDevice(&w->device) != hipSuccess || hipStreamCreate(&w->own) != hipSuccess) {
-        fail(WIG_E_DEVICE, "no HIP device");
-        delete w;
-        return nullptr;
-    }
-    w->stream = w->own;
     for (int p = 0; p < WIG_PART_COUNT; p++) {
         if (hipMalloc(&w->part[p], w->bytes[p]) != hipSuccess) {
             fail(WIG_E_NOMEM, "hipMalloc of part %d (%llu bytes) failed", p, (unsigned long long)w->bytes[p]);
@@ -233,29 +345,27 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
             return nullptr;
         }
     }
-    w->timed = -1;
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
         || hipMalloc(&w->fine, 41ull * w->d.C + 256u) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess
-        || hipMalloc((void **)&w->sp, sizeof(StepParams)) != hipSuccess
-        || hipHostMalloc((void **)&w->flag_host, 64, hipHostMallocDefault) != hipSuccess) {
+        || hipMalloc((void **)&w->wstate, 4u * WS_WORDS) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
         wig_destroy(w);
         return nullptr;
     }
     for (int p = 0; p < WIG_PART_COUNT; p++)
-        if (zero(w, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
-    zero(w, w->kind, 256);
-    zero(w, w->fine, 41ull * w->d.C + 256u);
-    zero(w, w->totals, sizeof(WigRunTotals));
-    zero(w, w->sp, sizeof(StepParams));
+        if (zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
+    zero(w->own, w->kind, 256);
+    zero(w->own, w->fine, 41ull * w->d.C + 256u);
+    zero(w->own, w->totals, sizeof(WigRunTotals));
+    zero(w->own, w->wstate, 4u * WS_WORDS);
     Dev &d = w->d;
     d.totals = w->totals;
-    d.sp = w->sp;
+    d.wstate = w->wstate;
     // the coarse levels one workgroup's LDS holds (k_coarseLDS): from the
     // first that fits with all below it
     int lds_max = 65536;
-    hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, w->device);
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, w->device);
     d.lds_level = d.L;
     for (uint32_t l = d.L; l >= 1; l--) {
         size_t bytes = 0;
@@ -265,12 +375,14 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         w->lds_bytes = bytes;
     }
     if (w->lds_bytes == 0) w->lds_bytes = 4ull * LDS_LEVEL_FLOATS * d.lw[d.L] * d.lh[d.L];
-    for (uint32_t l = 0; l < 9; l++) w->tile[l] = 16u;
-    if (w->lds_bytes > 65536 && hipFuncSetAttribute((const void *)k_coarseLDS, hipFuncAttributeMaxDynamicSharedMemorySize, (int)w->lds_bytes) != hipSuccess) {
+    if (hipFuncSetAttribute((const void *)k_coarseLDS, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
+        || w->lds_bytes > (size_t)lds_max) {
         fail(WIG_E_DEVICE, "k_coarseLDS cannot take %zu bytes of LDS", w->lds_bytes);
         wig_destroy(w);
         return nullptr;
     }
+    for (uint32_t l = 0; l <= MG_LEVELS; l++) w->tile[l] = d.tile[l] = 16u;
+    d.level_pairs = 1u;
     d.cells = (uint32_t *)w->part[WIG_PART_CELLS];
     d.sorted = (uint32_t *)w->part[WIG_PART_PARTICLES0];
     d.parts = (uint32_t *)w->part[WIG_PART_PARTICLES1];
@@ -290,8 +402,10 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         d.r2 = (float2 *)(f + 32ull * d.C);
         d.cflag = f + 40ull * d.C;
     }
-    if (hipStreamSynchronize(w->stream) != hipSuccess) {
-        fail(WIG_E_DEVICE, "stream failed at creation");
+    WigWorld *self = w;
+    w->solo = batch_new(&self, 1u, w->own);
+    if (!w->solo || hipStreamSynchronize(w->own) != hipSuccess) {
+        if (w->solo) fail(WIG_E_DEVICE, "stream failed at creation");
         wig_destroy(w);
         return nullptr;
     }
@@ -300,26 +414,26 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
 
 int wig_destroy(WigWorld *w) {
     if (!w) return WIG_OK;
-    if (w->stream) hipStreamSynchronize(w->stream);
+    if (w->solo) batch_free(w->solo);
+    if (w->own) (void)hipStreamSynchronize(w->own);
     for (int p = 0; p < WIG_PART_COUNT; p++)
-        if (w->part[p]) hipFree(w->part[p]);
-    if (w->kind) hipFree(w->kind);
-    if (w->scratch) hipFree(w->scratch);
-    if (w->fine) hipFree(w->fine);
-    if (w->totals) hipFree(w->totals);
-    if (w->sp) hipFree(w->sp);
-    if (w->flag_host) hipHostFree(w->flag_host);
-    if (w->ev_made)
-        for (int i = 0; i < 2 * MAX_TIMED; i++) hipEventDestroy(w->ev[i]);
-    if (w->own) hipStreamDestroy(w->own);
+        if (w->part[p]) (void)hipFree(w->part[p]);
+    if (w->kind) (void)hipFree(w->kind);
+    if (w->scratch) (void)hipFree(w->scratch);
+    if (w->fine) (void)hipFree(w->fine);
+    if (w->totals) (void)hipFree(w->totals);
+    if (w->wstate) (void)hipFree(w->wstate);
+    if (w->own) (void)hipStreamDestroy(w->own);
     delete w;
     return WIG_OK;
 }
 
+static hipStream_t stream_of(const WigWorld *w) { return w->solo->stream; }
+
 int wig_set_kinds(WigWorld *w, const uint8_t kinds[256]) {
     if (!w || !kinds) return fail(WIG_E_INVALID, "null argument");
-    HIPCHECK(hipMemcpyAsync(w->kind, kinds, 256, hipMemcpyHostToDevice, w->stream));
-    HIPCHECK(hipStreamSynchronize(w->stream));
+    HIPCHECK(hipMemcpyAsync(w->kind, kinds, 256, hipMemcpyHostToDevice, stream_of(w)));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
 }
 
@@ -346,16 +460,16 @@ int wig_layout(const WigWorld *w, WigLayoutInfo *out) {
 int wig_upload(WigWorld *w, int part, const void *src, size_t bytes) {
     if (!w || !src || part < 0 || part >= WIG_PART_COUNT) return fail(WIG_E_INVALID, "bad part %d", part);
     if (bytes != w->bytes[part]) return fail(WIG_E_INVALID, "part %d is %llu bytes, not %zu", part, (unsigned long long)w->bytes[part], bytes);
-    HIPCHECK(hipMemcpyAsync(w->part[part], src, bytes, hipMemcpyHostToDevice, w->stream));
-    HIPCHECK(hipStreamSynchronize(w->stream));
+    HIPCHECK(hipMemcpyAsync(w->part[part], src, bytes, hipMemcpyHostToDevice, stream_of(w)));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
 }
 
 int wig_download(WigWorld *w, int part, void *dst, size_t bytes) {
     if (!w || !dst || part < 0 || part >= WIG_PART_COUNT) return fail(WIG_E_INVALID, "bad part %d", part);
     if (bytes != w->bytes[part]) return fail(WIG_E_INVALID, "part %d is %llu bytes, not %zu", part, (unsigned long long)w->bytes[part], bytes);
-    HIPCHECK(hipMemcpyAsync(dst, w->part[part], bytes, hipMemcpyDeviceToHost, w->stream));
-    HIPCHECK(hipStreamSynchronize(w->stream));
+    HIPCHECK(hipMemcpyAsync(dst, w->part[part], bytes, hipMemcpyDeviceToHost, stream_of(w)));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
 }
 
@@ -367,57 +481,74 @@ int wig_device_ptr(WigWorld *w, int part, void **ptr) {
 
 int wig_set_stream(WigWorld *w, void *stream) {
     if (!w) return fail(WIG_E_INVALID, "null world");
-    w->stream = stream ? (hipStream_t)stream : w->own;
+    w->solo->stream = stream ? (hipStream_t)stream : w->own;
     return WIG_OK;
 }
 
 int wig_sync(WigWorld *w) {
     if (!w) return fail(WIG_E_INVALID, "null world");
-    HIPCHECK(hipStreamSynchronize(w->stream));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
+    return WIG_OK;
+}
+
+static int batch_report(Batch &b, uint32_t *stats) {
+    uint32_t n = (uint32_t)b.w.size();
+    int st = batch_maps(b);
+    if (st < 0) return st;
+    for (uint32_t i = 0; i < n; i++) b.htab[i] = b.w[i]->d;
+    HIPCHECK(hipMemcpyAsync(b.tab, b.htab.data(), sizeof(Dev) * n, hipMemcpyHostToDevice, b.stream));
+    hipLaunchKernelGGL(k_reports, dim3(n), dim3(64), 0, b.stream, grid_of(b, SH_ONE), b.reports);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(stats, b.reports, 4ull * FLUID_STATS * n, hipMemcpyDeviceToHost, b.stream));
+    HIPCHECK(hipStreamSynchronize(b.stream));
     return WIG_OK;
 }
 
 int wig_report(WigWorld *w, uint32_t stats[WIG_STATS]) {
     if (!w || !stats) return fail(WIG_E_INVALID, "null argument");
-    uint32_t *src = (uint32_t *)w->part[WIG_PART_BLOCKS] + 4ull * w->d.B;
-    HIPCHECK(hipMemcpyAsync(stats, src, 4u * WIG_STATS, hipMemcpyDeviceToHost, w->stream));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, w->stream, src + WIG_STAT_MOST_CYCLES, (uint64_t)1, 0u);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(w->stream));
-    return WIG_OK;
+    return batch_report(*w->solo, stats);
 }
 
 int wig_run_totals(WigWorld *w, WigRunTotals *out) {
     if (!w || !out) return fail(WIG_E_INVALID, "null argument");
-    HIPCHECK(hipMemcpyAsync(out, w->totals, sizeof *out, hipMemcpyDeviceToHost, w->stream));
-    HIPCHECK(hipStreamSynchronize(w->stream));
+    HIPCHECK(hipMemcpyAsync(out, w->totals, sizeof *out, hipMemcpyDeviceToHost, stream_of(w)));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
+    return WIG_OK;
+}
+
+static int batch_time_kernel(Batch &b, int id) {
+    if (id > KID_COUNT) return fail(WIG_E_INVALID, "bad kernel id %d", id);
+    if (!b.ev_made) {
+        for (int i = 0; i < 2 * MAX_TIMED; i++) HIPCHECK(hipEventCreate(&b.ev[i]));
+        b.ev_made = true;
+    }
+    b.timed = id < 0 ? -1 : id;
+    b.ntimed = 0;
+    return WIG_OK;
+}
+static int batch_kernel_time(Batch &b, double *ms, uint32_t *launches) {
+    if (!ms || !launches) return fail(WIG_E_INVALID, "null argument");
+    HIPCHECK(hipStreamSynchronize(b.stream));
+    double total = 0.0;
+    for (uint32_t i = 0; i < b.ntimed; i++) {
+        float t = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&t, b.ev[2 * i], b.ev[2 * i + 1]));
+        total += t;
+    }
+    *ms = total;
+    *launches = b.ntimed;
+    b.ntimed = 0;
     return WIG_OK;
 }
 
 int wig_time_kernel(WigWorld *w, int id) {
-    if (!w || id > KID_COUNT) return fail(WIG_E_INVALID, "bad kernel id %d", id);
-    if (!w->ev_made) {
-        for (int i = 0; i < 2 * MAX_TIMED; i++) HIPCHECK(hipEventCreate(&w->ev[i]));
-        w->ev_made = true;
-    }
-    w->timed = id < 0 ? -1 : id;
-    w->ntimed = 0;
-    return WIG_OK;
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    return batch_time_kernel(*w->solo, id);
 }
 
 int wig_kernel_time(WigWorld *w, double *ms, uint32_t *launches) {
-    if (!w || !ms || !launches) return fail(WIG_E_INVALID, "null argument");
-    HIPCHECK(hipStreamSynchronize(w->stream));
-    double total = 0.0;
-    for (uint32_t i = 0; i < w->ntimed; i++) {
-        float t = 0.0f;
-        HIPCHECK(hipEventElapsedTime(&t, w->ev[2 * i], w->ev[2 * i + 1]));
-        total += t;
-    }
-    *ms = total;
-    *launches = w->ntimed;
-    w->ntimed = 0;
-    return WIG_OK;
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    return batch_kernel_time(*w->solo, ms, launches);
 }
 
 int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side) {
@@ -433,19 +564,14 @@ int wig_kernel_count(void) { return KID_COUNT + 1; }
 
 int wig_solve_stats(WigWorld *w, WigSolveStats *out) {
     if (!w || !out) return fail(WIG_E_INVALID, "null argument");
-    HIPCHECK(hipMemcpyAsync(out, w->part[WIG_PART_SOLVE], sizeof *out, hipMemcpyDeviceToHost, w->stream));
-    HIPCHECK(hipStreamSynchronize(w->stream));
+    HIPCHECK(hipMemcpyAsync(out, w->part[WIG_PART_SOLVE], sizeof *out, hipMemcpyDeviceToHost, stream_of(w)));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
 }
 
-// Grid of the passes over the blocks worked on: they stride over the active
-// list, so a few workgroups a CU cover any count.
-static uint32_t block_grid(const Dev &d) { return std::min<uint32_t>(d.B + d.bw + d.bh + 1u, 4096u); }
-static uint32_t particle_grid(const Dev &d) { return std::min<uint32_t>((d.cap + 255u) / 256u, 4096u); }
-
-int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
-    if (!w || !P || !ctl) return fail(WIG_E_INVALID, "null argument");
-    Dev d = w->d;
+// The step's words of world w's Dev, from its Params and controls.
+static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, Dev &d) {
+    d = w->d;
     if (P->size[0] != (uint32_t)d.W || P->size[1] != (uint32_t)d.H)
         return fail(WIG_E_INVALID, "params size %ux%u is not the world's %dx%d", P->size[0], P->size[1], d.W, d.H);
     if (ctl->viscous || ctl->surface || ctl->bodies_run)
@@ -455,100 +581,195 @@ int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         return fail(WIG_E_INVALID, "solver control out of range");
     if (ctl->pre_relax != 1u || ctl->post_relax != 1u)
         return fail(WIG_E_INVALID, "one relaxation before and one after the coarse levels (PRE_RELAX = POST_RELAX = 1) is what is built");
+    d.frame = P->frame;
+    d.seed = P->seed;
+    d.earthG = P->earthG;
+    d.full = ctl->full ? 1u : 0u;
+    d.fluid_cycles = ctl->fluid_cycles;
+    d.shifted_first = ctl->shifted_first ? 1u : 0u;
+    d.first_chunk = ctl->first_chunk;
+    d.chunk = ctl->chunk;
     d.sweep_pairs = ctl->sweep_pairs;
     d.coarse_sweeps = ctl->coarse_sweeps;
     d.tol_mode = ctl->tol_mode;
     d.tol = ctl->tol;
     d.level_pairs = ctl->level_pairs ? ctl->level_pairs : 1u;
-    hipStream_t s = w->stream;
-    dim3 world(d.bw, d.bh), b64(64), b256(256);
-    uint32_t bg = block_grid(d), pg = particle_grid(d), bg4 = std::min<uint32_t>((d.B + 3u) / 4u, 1024u);
-    launch(w, KID_setStep, k_setStep, dim3(1), dim3(1), w->sp, P->frame, P->seed, P->earthG);
-    if (!ctl->full) {
-        launch(w, KID_sync, k_sync, world, b64, d, 0);
-        launch(w, KID_compact, k_compact, dim3(1), dim3(1024), d);
-        launch(w, KID_retire, k_retire, dim3(bg), b64, d);
+    for (uint32_t l = 0; l <= MG_LEVELS; l++) d.tile[l] = fitTile(w->tile[l], d.level_pairs);
+    w->d.level_pairs = d.level_pairs;
+    for (uint32_t l = 0; l <= MG_LEVELS; l++) w->d.tile[l] = d.tile[l];
+    return WIG_OK;
+}
+
+// One liquid step of every world of the batch (encodeFluid, js/sim.js:2466-2612).
+static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
+    uint32_t n = (uint32_t)b.w.size();
+    bool any_full = false, timeSolve = b.timed == KID_COUNT && b.ntimed < MAX_TIMED;
+    uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
+    size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
+    for (uint32_t i = 0; i < n; i++) {
+        int st = step_words(b.w[i], &P[i], &ctl[i], b.htab[i]);
+        if (st < 0) return st;
+    }
+    int st = batch_maps(b);
+    if (st < 0) return st;
+    for (uint32_t i = 0; i < n; i++) {
+        const Dev &d = b.htab[i];
+        if (!d.full) continue;
+        any_full = true;
+        max_cycles = std::max(max_cycles, d.fluid_cycles);
+        if (d.first_chunk != 0u) {
+            uint32_t f = d.first_chunk == WIG_CHUNK_AUTO ? std::max(b.w[i]->last_hint, 1u) : d.first_chunk;
+            first_check = std::min(first_check, f);
+            chunk = std::min(chunk, d.chunk ? d.chunk : 1u);
+        }
+        coarse_lds = std::max(coarse_lds, b.w[i]->lds_bytes);
+        for (uint32_t l = 1; l < d.lds_level; l++) {
+            max_level = std::max(max_level, l);
+            down_lds[l] = std::max(down_lds[l], downLds(d.level_pairs, d.tile[l]));
+            up_lds[l] = std::max(up_lds[l], upLds(d.level_pairs, d.tile[l]));
+        }
+    }
+    HIPCHECK(hipMemcpyAsync(b.tab, b.htab.data(), sizeof(Dev) * n, hipMemcpyHostToDevice, b.stream));
+    hipStream_t s = b.stream;
+    launch(b, KID_sync, k_sync, SH_WORLD, 64);
+    launch(b, KID_syncParticles, k_syncParticles, SH_PG, 256);
+    launch(b, KID_syncSeed, k_syncSeed, SH_WORLD, 64);
+    launch(b, KID_compact, k_compact, SH_ONE, 1024);
+    launch(b, KID_retire, k_retire, SH_BG, 64);
+    if (!any_full) {
         HIPCHECK(hipGetLastError());
         return WIG_OK;
     }
-    launch(w, KID_sync, k_sync, world, b64, d, 1);
-    launch(w, KID_syncParticles, k_syncParticles, dim3(pg), b256, d);
-    launch(w, KID_syncSeed, k_syncSeed, world, b64, d);
-    launch(w, KID_compact, k_compact, dim3(1), dim3(1024), d);
-    launch(w, KID_retire, k_retire, dim3(bg), b64, d);
-    launch(w, KID_clear, k_clear, dim3(bg), b64, d);
-    launch(w, KID_gather, k_gather, dim3(pg), b256, d);
-    launch(w, KID_localJoin, k_localJoin, dim3(bg), b64, d, 0);
-    launch(w, KID_edgeJoin, k_edgeJoin, dim3(bg), b64, d, 0);
-    launch(w, KID_sealAnchor, k_sealAnchor, dim3(bg), b64, d);
-    launch(w, KID_classify, k_classify, dim3(bg), b64, d);
-    launch(w, KID_setFaces, k_setFaces, dim3(bg), b64, d);
-    launch(w, KID_divergence, k_divergence, dim3(bg), b64, d);
+    launch(b, KID_clear, k_clear, SH_BG, 64);
+    launch(b, KID_gather, k_gather, SH_PG, 256);
+    launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 0);
+    launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 0);
+    launch(b, KID_sealAnchor, k_sealAnchor, SH_BG, 64);
+    launch(b, KID_classify, k_classify, SH_BG, 64);
+    launch(b, KID_setFaces, k_setFaces, SH_BG, 64);
+    launch(b, KID_divergence, k_divergence, SH_BG, 64);
     // (the solve, cgStart .. cgFinish, timed as one: the pseudo-kernel "solve")
-    bool timeSolve = w->timed == KID_COUNT && w->ntimed < MAX_TIMED;
-    if (timeSolve) hipEventRecord(w->ev[2 * w->ntimed], s);
-    launch(w, KID_cgStart, k_cgStart, dim3(bg), b64, d);
-    launch(w, KID_localJoin, k_localJoin, dim3(bg), b64, d, 1);
-    launch(w, KID_edgeJoin, k_edgeJoin, dim3(bg), b64, d, 1);
-    launch(w, KID_cgAnchor, k_cgAnchor, dim3(bg), b64, d);
-    launch(w, KID_cgResidual, k_cgResidual, dim3(bg4), b256, d);
-    launch(w, KID_sumNorm, k_sumNorm, dim3(1), dim3(1024), d);
-    launch(w, KID_cgPrepare, k_cgPrepare, dim3(bg), b64, d);
-    dim3 lg1(ceil_div(d.lw[1], 8u), ceil_div(d.lh[1], 8u));
-    if (ctl->fluid_cycles > 0u) launch(w, KID_levelRho, k_levelRho, lg1, b64, d);
-    bool shifted = ctl->shifted_first != 0;
-    uint32_t l0 = d.lds_level;
-    uint32_t first = ctl->first_chunk == WIG_CHUNK_AUTO ? std::max<uint32_t>(w->last_cycles, 1u) : ctl->first_chunk;
-    uint32_t next_check = first ? first : WIG_MAX_CYCLES + 1u;
-    for (uint32_t cycle = 0; cycle < ctl->fluid_cycles; cycle++) {
+    if (timeSolve) HIPCHECK(hipEventRecord(b.ev[2 * b.ntimed], s));
+    launch(b, KID_cgStart, k_cgStart, SH_BG, 64);
+    launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 1);
+    launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 1);
+    launch(b, KID_cgAnchor, k_cgAnchor, SH_BG, 64);
+    launch(b, KID_cgResidual, k_cgResidual, SH_BG4, 256);
+    launch(b, KID_sumNorm, k_sumNorm, SH_ONE, 1024);
+    launch(b, KID_cgPrepare, k_cgPrepare, SH_BG, 64);
+    launch(b, KID_levelRho, k_levelRho, SH_LVL1, 64);
+    // iterations: each world runs the ones its own host would have encoded
+    // (encoded() in the kernels); the batch checks at its checkpoints
+    // whether every world has stopped
+    uint32_t next_check = first_check;
+    for (uint32_t cycle = 0; cycle < max_cycles; cycle++) {
         if (cycle == next_check) {
-            // the chunk encoded so far has run: stop if the solve converged
-            HIPCHECK(hipMemcpyAsync(w->flag_host, &w->d.head->cycles, 8, hipMemcpyDeviceToHost, s));
+            hipLaunchKernelGGL(k_stops, dim3(n), dim3(64), 0, s, grid_of(b, SH_ONE), b.stops);
+            HIPCHECK(hipMemcpyAsync(b.stops_host, b.stops, 8ull * n, hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
-            if (w->flag_host[1]) {
-                w->last_cycles = w->flag_host[0];
+            bool done = true;
+            for (uint32_t i = 0; i < n && done; i++) {
+                const Dev &d = b.htab[i];
+                done = !d.full || d.fluid_cycles <= cycle || b.stops_host[2 * i] <= cycle;
+            }
+            if (done) {
+                for (uint32_t i = 0; i < n; i++)
+                    if (b.htab[i].full && b.stops_host[2 * i] < b.htab[i].fluid_cycles) b.w[i]->last_hint = b.stops_host[2 * i + 1];
                 break;
             }
-            next_check = cycle + (ctl->chunk ? ctl->chunk : 1u);
+            next_check = cycle + chunk;
         }
-        launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, 0);
-        shifted = !shifted;
-        launch(w, KID_restrict, k_restrict, lg1, b64, d, cycle);
-        for (uint32_t l = 1; l < l0; l++) {   // level l's sweeps, and restrictLevel into l + 1
-            uint32_t T = levelTileSide(w, d, l);
-            launch_lds(w, KID_levelDown, k_levelDown, tileGrid(d, l, T), dim3(256), downLds(d, T), d, cycle, l, T);
-        }
-        launch_lds(w, KID_coarseLDS, k_coarseLDS, dim3(1), dim3(1024), w->lds_bytes, d, cycle);
-        for (uint32_t l = l0 - 1; l >= 1; l--) {
-            uint32_t T = levelTileSide(w, d, l);
-            launch_lds(w, KID_levelUp, k_levelUp, tileGrid(d, l, T), dim3(256), upLds(d, T), d, cycle, l, T);
-        }
-        launch(w, KID_relax, k_relax, dim3(bg), b64, d, cycle, shifted ? WIG_BLOCK / 2 : 0, 1);
-        shifted = !shifted;
-        launch(w, KID_cgRho, k_cgRho, dim3(bg4), b256, d, cycle);
-        launch(w, KID_cgBeta, k_cgBeta, dim3(1), dim3(1024), d, cycle);
-        launch(w, KID_cgDirection, k_cgDirection, dim3(bg), b64, d, cycle);
-        launch(w, KID_cgApply, k_cgApply, dim3(bg4), b256, d, cycle);
-        launch(w, KID_cgAlpha, k_cgAlpha, dim3(1), dim3(1024), d, cycle);
-        launch(w, KID_cgUpdate, k_cgUpdate, dim3(bg4), b256, d, cycle);
-        launch(w, KID_cgCheck, k_cgCheck, dim3(1), dim3(1024), d, cycle);
+        launch(b, KID_relax, k_relax, SH_BG, 64, cycle, 0);
+        launch(b, KID_restrict, k_restrict, SH_LVL1, 64, cycle);
+        for (uint32_t l = 1; l <= max_level; l++)   // level l's sweeps, and restrictLevel into l + 1
+            launch_lds(b, KID_levelDown, k_levelDown, SH_TILE + (int)l - 1, 256, down_lds[l], cycle, l);
+        launch_lds(b, KID_coarseLDS, k_coarseLDS, SH_ONE, 1024, coarse_lds, cycle);
+        for (uint32_t l = max_level; l >= 1; l--)
+            launch_lds(b, KID_levelUp, k_levelUp, SH_TILE + (int)l - 1, 256, up_lds[l], cycle, l);
+        launch(b, KID_relax, k_relax, SH_BG, 64, cycle, 1);
+        launch(b, KID_cgRho, k_cgRho, SH_BG4, 256, cycle);
+        launch(b, KID_cgBeta, k_cgBeta, SH_ONE, 1024, cycle);
+        launch(b, KID_cgDirection, k_cgDirection, SH_BG, 64, cycle);
+        launch(b, KID_cgApply, k_cgApply, SH_BG4, 256, cycle);
+        launch(b, KID_cgAlpha, k_cgAlpha, SH_ONE, 1024, cycle);
+        launch(b, KID_cgUpdate, k_cgUpdate, SH_BG4, 256, cycle);
+        launch(b, KID_cgCheck, k_cgCheck, SH_ONE, 1024, cycle);
     }
-    launch(w, KID_cgFinish, k_cgFinish, dim3(bg), b64, d, ctl->fluid_cycles > 0u ? 1 : 0);
-    if (timeSolve) hipEventRecord(w->ev[2 * w->ntimed + 1], s), w->ntimed++;
-    launch(w, KID_project, k_project, dim3(bg), b64, d);
-    launch(w, KID_extrapolate, k_extrapolate, dim3(bg), b64, d, 1);
-    launch(w, KID_extrapolate, k_extrapolate, dim3(bg), b64, d, 2);
-    launch(w, KID_advect, k_advect, dim3(pg), b256, d);
-    launch(w, KID_sortCount, k_sortCount, dim3(pg), b256, d);
-    launch(w, KID_scanBlocks, k_scanBlocks, dim3(bg), b64, d);
-    launch(w, KID_scanTotals, k_scanTotals, dim3(1), b256, d);
-    launch(w, KID_scanFinish, k_scanFinish, dim3(bg), b64, d);
-    launch(w, KID_scatter, k_scatter, dim3(pg), b256, d);
-    launch(w, KID_rasterClaims, k_rasterClaims, dim3(bg), b64, d);
-    launch(w, KID_raster, k_raster, dim3(bg), b64, d);
-    launch(w, KID_unpaint, k_unpaint, dim3(bg), b64, d);
+    launch(b, KID_cgFinish, k_cgFinish, SH_BG, 64);
+    if (timeSolve) {
+        HIPCHECK(hipEventRecord(b.ev[2 * b.ntimed + 1], s));
+        b.ntimed++;
+    }
+    launch(b, KID_project, k_project, SH_BG, 64);
+    launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 1);
+    launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 2);
+    launch(b, KID_advect, k_advect, SH_PG, 256);
+    launch(b, KID_sortCount, k_sortCount, SH_PG, 256);
+    launch(b, KID_scanBlocks, k_scanBlocks, SH_BG, 64);
+    launch(b, KID_scanTotals, k_scanTotals, SH_ONE, 256);
+    launch(b, KID_scanFinish, k_scanFinish, SH_BG, 64);
+    launch(b, KID_scatter, k_scatter, SH_PG, 256);
+    launch(b, KID_rasterClaims, k_rasterClaims, SH_BG, 64);
+    launch(b, KID_raster, k_raster, SH_BG, 64);
+    launch(b, KID_unpaint, k_unpaint, SH_BG, 64);
     HIPCHECK(hipGetLastError());
     return WIG_OK;
+}
+
+int wig_fluid_step(WigWorld *w, const WigParams *P, const WigStepCtl *ctl) {
+    if (!w || !P || !ctl) return fail(WIG_E_INVALID, "null argument");
+    return batch_step(*w->solo, P, ctl);
+}
+
+WigBatch *wig_batch_create(WigWorld *const *worlds, uint32_t n) {
+    if (!worlds || n == 0u) {
+        fail(WIG_E_INVALID, "a batch needs worlds");
+        return nullptr;
+    }
+    for (uint32_t i = 0; i < n; i++)
+        if (!worlds[i] || worlds[i]->device != worlds[0]->device) {
+            fail(WIG_E_INVALID, "world %u is null or on another device", i);
+            return nullptr;
+        }
+    return (WigBatch *)batch_new(worlds, n, worlds[0]->own);
+}
+
+int wig_batch_destroy(WigBatch *b) {
+    batch_free((Batch *)b);
+    return WIG_OK;
+}
+
+int wig_batch_set_stream(WigBatch *b, void *stream) {
+    if (!b) return fail(WIG_E_INVALID, "null batch");
+    Batch *B = (Batch *)b;
+    B->stream = stream ? (hipStream_t)stream : B->w[0]->own;
+    return WIG_OK;
+}
+
+int wig_batch_step(WigBatch *b, const WigParams *params, const WigStepCtl *ctl) {
+    if (!b || !params || !ctl) return fail(WIG_E_INVALID, "null argument");
+    return batch_step(*(Batch *)b, params, ctl);
+}
+
+int wig_batch_report(WigBatch *b, uint32_t *stats) {
+    if (!b || !stats) return fail(WIG_E_INVALID, "null argument");
+    return batch_report(*(Batch *)b, stats);
+}
+
+int wig_batch_sync(WigBatch *b) {
+    if (!b) return fail(WIG_E_INVALID, "null batch");
+    HIPCHECK(hipStreamSynchronize(((Batch *)b)->stream));
+    return WIG_OK;
+}
+
+int wig_batch_time_kernel(WigBatch *b, int id) {
+    if (!b) return fail(WIG_E_INVALID, "null batch");
+    return batch_time_kernel(*(Batch *)b, id);
+}
+
+int wig_batch_kernel_time(WigBatch *b, double *ms, uint32_t *launches) {
+    if (!b) return fail(WIG_E_INVALID, "null batch");
+    return batch_kernel_time(*(Batch *)b, ms, launches);
 }
 
 }  // extern "C"

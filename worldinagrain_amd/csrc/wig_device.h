@@ -99,13 +99,14 @@ struct SolveHead {
     uint32_t cycles, converged, pad[2];
 };
 
-// The step's Params words the liquid reads, in device memory so that a
-// captured step reads each replay's own (k_setStep writes them).
-struct StepParams {
-    uint32_t frame, seed;
-    float earthG;
-    uint32_t pad;
-};
+// A world's own device words that outlive a step: how many iterations its
+// last solve stopped after, and this solve's first convergence checkpoint
+// and the iteration it stops before (WIG_CHUNK_AUTO and chunk, include/wig.h)
+#define WS_LAST 0
+#define WS_FIRST 1
+#define WS_STOP 2
+#define WS_WORDS 16
+#define NO_STOP 0xffffffffu
 
 // Everything a kernel needs to know of the world and the step.
 struct Dev {
@@ -125,7 +126,7 @@ struct Dev {
     const uint8_t *kind;
     uint32_t *scratch;      // per-block flags and offsets (compact)
     unsigned long long *totals;  // steps, PCG iterations, particles, active cells (run totals)
-    StepParams *sp;
+    uint32_t *wstate;       // WS_* words
     // The solve's fine level, packed once a solve (cgPrepare) for the cells
     // of the blocks worked on and the ring of cells beside them: each cell's
     // four face betas in beside()'s order (fineBeta; the sign bit set where
@@ -137,9 +138,31 @@ struct Dev {
     uint8_t *cflag;
     float2 *pa, *pb, *r2;
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
+    uint32_t tile[MG_LEVELS + 1];   // k_levelDown/k_levelUp's tile side, per level below lds_level
+    // the step's: the Params words the liquid reads, and its controls
+    uint32_t frame, seed;
+    float earthG;
+    uint32_t full, fluid_cycles, shifted_first, first_chunk, chunk;
     uint32_t sweep_pairs, coarse_sweeps, tol_mode, level_pairs;
     float tol;
 };
+
+// A launch over a batch of worlds.  Its workgroups are split among the
+// worlds in contiguous runs, one run a world, laid out per launch shape:
+// `map` gives each workgroup's world and `first` each world's first
+// workgroup (and, at nworlds, the total).  A kernel takes its world's Dev
+// from `tab` and its workgroup's index and count within the world's run.
+struct WorldGrid {
+    const Dev *__restrict__ tab;
+    const uint32_t *__restrict__ map;
+    const uint32_t *__restrict__ first;
+};
+#define WORLD(g)                                                      \
+    const uint32_t wi_ = (g).map[blockIdx.x];                         \
+    const Dev &d = (g).tab[wi_];                                      \
+    const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
+    const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \
+    (void)nbid
 
 struct i2 { int32_t x, y; };
 struct v2 { float x, y; };

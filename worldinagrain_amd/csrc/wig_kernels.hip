@@ -20,6 +20,12 @@ __device__ __forceinline__ bool converged(const Dev &d, uint32_t cycle) {
 __device__ __forceinline__ uint32_t ld(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Whether the world's host would have encoded iteration `cycle`: within its
+// fluid_cycles, and before the convergence checkpoint its solve stops at (a
+// batch runs as many iterations as its slowest world; the others' skip).
+__device__ __forceinline__ bool encoded(const Dev &d, uint32_t cycle) {
+    return d.full && cycle < d.fluid_cycles && cycle < ld(&d.wstate[WS_STOP]);
+}
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // ------------------------------------------------------------ cell tests
@@ -46,8 +52,8 @@ __device__ __forceinline__ float fineBeta(float a, float b, float da, float db) 
 __device__ __forceinline__ bool isLiquidAt(const Dev &d, i2 c) {
     return inWorld(d, c) && kindOf(d, wType(d.cells[4ull * cellIndex(d, c)])) == KIND_LIQUID;
 }
-__device__ __forceinline__ uint32_t stepBit(const Dev &d, uint32_t back) { return 1u << ((d.sp->frame + back) & 1u); }
-__device__ __forceinline__ uint32_t fallingBit(const Dev &d, uint32_t back) { return 4u << ((d.sp->frame + back) & 1u); }
+__device__ __forceinline__ uint32_t stepBit(const Dev &d, uint32_t back) { return 1u << ((d.frame + back) & 1u); }
+__device__ __forceinline__ uint32_t fallingBit(const Dev &d, uint32_t back) { return 4u << ((d.frame + back) & 1u); }
 __device__ __forceinline__ bool fallingAt(const Dev &d, i2 q) {
     return inWorld(d, q) && (wu32(FLD(d, 2u * cellIndex(d, q) + 1u, 3)) & fallingBit(d, 1u)) != 0u;
 }
@@ -141,7 +147,7 @@ __device__ __forceinline__ void facesOf(const Dev &d, i2 c, Face3 fs[4]) {
 // the cell of thread li of the block at slot wid of the active list
 __device__ __forceinline__ i2 blockCell(i2 b, uint32_t li) { return I2(b.x * WIG_BLOCK + (int32_t)(li % 8u), b.y * WIG_BLOCK + (int32_t)(li / 8u)); }
 
-#define FOR_ACTIVE_BLOCKS(d, wid) for (uint32_t wid = blockIdx.x, n_ = activeCount(d); wid < n_; wid += gridDim.x)
+#define FOR_ACTIVE_BLOCKS(d, wid) for (uint32_t wid = bid, n_ = activeCount(d); wid < n_; wid += nbid)
 
 // -------------------------------------------------- wavefront reductions
 __device__ __forceinline__ double wave_tree_sum(double v) {
@@ -172,9 +178,12 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 }
 
 // sync / syncMarks (:608-659), a workgroup per block of the world
-__global__ void __launch_bounds__(64) k_sync(Dev d, int claims) {
+__global__ void __launch_bounds__(64) k_sync(WorldGrid g) {
+    WORLD(g);
+    const int claims = d.full ? 1 : 0;
+    const uint32_t gx_ = d.bw, bx = bid % gx_, by = bid / gx_;
     uint32_t li = threadIdx.x;
-    i2 c = I2((int32_t)blockIdx.x * WIG_BLOCK + (int32_t)(li % 8u), (int32_t)blockIdx.y * WIG_BLOCK + (int32_t)(li / 8u));
+    i2 c = I2((int32_t)bx * WIG_BLOCK + (int32_t)(li % 8u), (int32_t)by * WIG_BLOCK + (int32_t)(li / 8u));
     bool liquid = false;
     if (inWorld(d, c)) {
         uint32_t i = cellIndex(d, c);
@@ -191,7 +200,7 @@ __global__ void __launch_bounds__(64) k_sync(Dev d, int claims) {
             liquid = true;
         }
     }
-    if (__ballot(liquid) && li == 0) atomicOr(&d.blocks[blockIdx.y * d.bw + blockIdx.x], MARK_LIQUID);
+    if (__ballot(liquid) && li == 0) atomicOr(&d.blocks[by * d.bw + bx], MARK_LIQUID);
 }
 
 __device__ __forceinline__ void markBlockOf(const Dev &d, v2 pos) {
@@ -242,7 +251,7 @@ __device__ uint32_t sharedLife(const Dev &d, uint32_t life, uint32_t was, uint32
     if (now < was) l = l * (float)now / (float)was;
     else l = 255.0f - (255.0f - l) * (255.0f - (float)now) / (255.0f - (float)was);
     Rng r;
-    seedRng(r, id, d.sp->frame, d.sp->seed ^ 0x5a17u);
+    seedRng(r, id, d.frame, d.seed ^ 0x5a17u);
     uint32_t v = wu32(floorf(l + rnd(r)));
     return v < 255u ? v : 255u;
 }
@@ -314,14 +323,16 @@ __device__ bool syncOne(const Dev &d, uint32_t k, PState &q) {
     }
     return true;
 }
-__global__ void __launch_bounds__(256) k_syncParticles(Dev d) {
+__global__ void __launch_bounds__(256) k_syncParticles(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     __shared__ uint4 sMo[SP_CHUNK], sSt[SP_CHUNK], sAf[SP_CHUNK];
     __shared__ uint32_t sScan[256];
     __shared__ uint32_t sBase;
     uint32_t cnt = ld(&d.sorted[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
     uint32_t tid = threadIdx.x;
-    for (uint32_t base = blockIdx.x * SP_CHUNK; base < n; base += gridDim.x * SP_CHUNK) {
+    for (uint32_t base = bid * SP_CHUNK; base < n; base += nbid * SP_CHUNK) {
         uint32_t keepBits = 0;
         for (uint32_t r = 0; r < 4u; r++) {
             uint32_t slot = r * 256u + tid, k = base + slot;
@@ -372,7 +383,7 @@ __global__ void __launch_bounds__(256) k_syncParticles(Dev d) {
 }
 
 __device__ __forceinline__ uint32_t seedId(const Dev &d, i2 c, uint32_t j) {   // :835-841
-    uint32_t h = (((uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x) * 0x9e3779b9u) ^ (d.sp->frame * 0x85ebca6bu) ^ (j * 0xc2b2ae35u);
+    uint32_t h = (((uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x) * 0x9e3779b9u) ^ (d.frame * 0x85ebca6bu) ^ (j * 0xc2b2ae35u);
     h ^= h >> 16;
     h *= 0x7feb352du;
     h ^= h >> 15;
@@ -380,9 +391,12 @@ __device__ __forceinline__ uint32_t seedId(const Dev &d, i2 c, uint32_t j) {   /
 }
 
 // syncSeed (:863-890), a workgroup per block of the world
-__global__ void __launch_bounds__(64) k_syncSeed(Dev d) {
+__global__ void __launch_bounds__(64) k_syncSeed(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
+    const uint32_t gx_ = d.bw, bx = bid % gx_, by = bid / gx_;
     uint32_t li = threadIdx.x;
-    i2 c = I2((int32_t)blockIdx.x * WIG_BLOCK + (int32_t)(li % 8u), (int32_t)blockIdx.y * WIG_BLOCK + (int32_t)(li / 8u));
+    i2 c = I2((int32_t)bx * WIG_BLOCK + (int32_t)(li % 8u), (int32_t)by * WIG_BLOCK + (int32_t)(li / 8u));
     uint32_t n = 0;
     uint4 v = make_uint4(0, 0, 0, 0);
     Rng r;
@@ -396,7 +410,7 @@ __global__ void __launch_bounds__(64) k_syncSeed(Dev d) {
             if (!wHas(v.x, FL_DRAWN)) n = LIQUID_PER_CELL;
             else if (home >= d.C || CNT(d, home, G_DEST) != (DEST_CELL | (DEST_SPAN - i))) n = want > 1u ? want : 1u;
             else if (want > CNT(d, home, G_SORT) && want < LIQUID_COUNT_MAX) n = want - CNT(d, home, G_SORT);
-            seedRng(r, i, d.sp->frame, d.sp->seed ^ 0x5eedu);
+            seedRng(r, i, d.frame, d.seed ^ 0x5eedu);
         }
     }
     uint32_t m = wave_take(d, n);
@@ -426,7 +440,8 @@ __global__ void __launch_bounds__(64) k_syncSeed(Dev d) {
 // compact (:896-921): the blocks worked on and those dropping out, listed in
 // block-index order by one workgroup (the reference's lists are in whatever
 // order its threads came).
-__global__ void __launch_bounds__(1024) k_compact(Dev d) {
+__global__ void __launch_bounds__(1024) k_compact(WorldGrid g) {
+    WORLD(g);
     __shared__ uint32_t sA[1024], sR[1024];
     uint32_t t = threadIdx.x, T = blockDim.x;
     uint32_t per = (d.B + T - 1u) / T, lo = t * per, hi = lo + per < d.B ? lo + per : d.B;
@@ -469,8 +484,9 @@ __global__ void __launch_bounds__(1024) k_compact(Dev d) {
 }
 
 // retire (:924-951)
-__global__ void __launch_bounds__(64) k_retire(Dev d) {
-    for (uint32_t wid = blockIdx.x, nr = d.blocks[COUNT_SLOT(d, COUNT_RETIRED)]; wid < nr; wid += gridDim.x) {
+__global__ void __launch_bounds__(64) k_retire(WorldGrid g) {
+    WORLD(g);
+    for (uint32_t wid = bid, nr = d.blocks[COUNT_SLOT(d, COUNT_RETIRED)]; wid < nr; wid += nbid) {
         uint32_t n = d.blocks[RETIRED_SLOT(d, wid)];
         i2 c = blockCell(I2((int32_t)(n % d.bw), (int32_t)(n / d.bw)), threadIdx.x);
         if (inWorld(d, c)) {
@@ -494,7 +510,9 @@ __global__ void __launch_bounds__(64) k_retire(Dev d) {
 }
 
 // clear (:962-985)
-__global__ void __launch_bounds__(64) k_clear(Dev d) {
+__global__ void __launch_bounds__(64) k_clear(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 b = activeBlock(d, wid);
         if (threadIdx.x == 0) d.blocks[blockIndex(d, b)] = 0u;
@@ -587,13 +605,15 @@ __device__ __forceinline__ void splatParticle(const Dev &d, const GatherTile &T,
     addWord<TILE>(T.mass, t, &CNT(d, home, G_MASS), wu32(wround(liquidDensity(temp) * FIXF)));
     splatDensity<TILE>(d, T, pos, liquidVolume(temp));
 }
-__global__ void __launch_bounds__(256) k_gather(Dev d) {
+__global__ void __launch_bounds__(256) k_gather(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     __shared__ uint32_t sTile[7 * GT_MAX];
     __shared__ int32_t sBox[4];
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
     uint32_t tid = threadIdx.x;
-    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    for (uint32_t base = bid * 256u; base < n; base += nbid * 256u) {
         uint32_t k = base + tid;
         // the cells' box
         int32_t cx = 0x7fffffff, cy = 0x7fffffff, mx = -1, my = -1;
@@ -705,7 +725,9 @@ __device__ __forceinline__ bool isMember(const Dev &d, i2 c, int mode) {
     if (!inWorld(d, c)) return false;
     return mode == 0 ? mayBeSealed(d, c) : FLD(d, 2u * cellIndex(d, c), 2) > 0.0f;
 }
-__global__ void __launch_bounds__(64) k_localJoin(Dev d, int mode) {
+__global__ void __launch_bounds__(64) k_localJoin(WorldGrid g, int mode) {
+    WORLD(g);
+    if (!d.full) return;
     __shared__ uint32_t sLink[64];
     __shared__ uint32_t sMember[64];
     uint32_t li = threadIdx.x;
@@ -726,7 +748,9 @@ __global__ void __launch_bounds__(64) k_localJoin(Dev d, int mode) {
         __syncthreads();
     }
 }
-__global__ void __launch_bounds__(64) k_edgeJoin(Dev d, int mode) {
+__global__ void __launch_bounds__(64) k_edgeJoin(WorldGrid g, int mode) {
+    WORLD(g);
+    if (!d.full) return;
     uint32_t li = threadIdx.x;
     bool edge = li % 8u == 7u || li / 8u == 7u;
     FOR_ACTIVE_BLOCKS(d, wid) {
@@ -738,7 +762,9 @@ __global__ void __launch_bounds__(64) k_edgeJoin(Dev d, int mode) {
     }
 }
 // sealAnchor: a gap meeting what was open air last step opens its region
-__global__ void __launch_bounds__(64) k_sealAnchor(Dev d) {
+__global__ void __launch_bounds__(64) k_sealAnchor(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c) || !mayBeSealed(d, c)) continue;
@@ -757,9 +783,11 @@ __device__ float columnWeight(const Dev &d, i2 p, float density) {
         over += 1;
         q.y -= 1;
     }
-    return density * d.sp->earthG * ((float)over + 0.5f);
+    return density * d.earthG * ((float)over + 0.5f);
 }
-__global__ void __launch_bounds__(64) k_classify(Dev d) {
+__global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 bk = activeBlock(d, wid);
         i2 p = blockCell(bk, threadIdx.x);
@@ -860,12 +888,14 @@ __device__ void setFace(const Dev &d, i2 at, bool isU) {
     }
     float now = u;
     if (held > 0.0f) now += 0.0f;                        // the wind's drag: still air
-    if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) now += isU ? 0.0f : d.sp->earthG;
+    if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) now += isU ? 0.0f : d.earthG;
     *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(now, u, held, faceViscosity(ga, gb));
     FLD(d, viscRhsSlot(d, f), f % 4u) = now;
     if (FLD(d, viscLastSlot(d, f), f % 4u) != 0.0f) FLD(d, viscLastSlot(d, f), f % 4u) = 0.0f;
 }
-__global__ void __launch_bounds__(64) k_setFaces(Dev d) {
+__global__ void __launch_bounds__(64) k_setFaces(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -877,8 +907,10 @@ __global__ void __launch_bounds__(64) k_setFaces(Dev d) {
 }
 
 // divergence (:1996-2025)
-__global__ void __launch_bounds__(64) k_divergence(Dev d) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+__global__ void __launch_bounds__(64) k_divergence(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
+    if (bid == 0 && threadIdx.x == 0) {
         d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL)] = 0u;
         d.blocks[COUNT_SLOT(d, COUNT_CYCLES)] = 0u;
     }
@@ -979,7 +1011,10 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
     return j <= high ? I2(wide, j) : I2(j - high - 1, high);
 }
-__global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, int post) {
+__global__ void __launch_bounds__(64) k_relax(WorldGrid g, uint32_t cycle, int post) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
+    const int shift = ((d.shifted_first ^ (2u * cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
     if (converged(d, cycle)) return;
     __shared__ float sPx[RIM * RIM], sPy[RIM * RIM];
     const float2 *pin = post ? d.pb : d.pa;
@@ -988,7 +1023,7 @@ __global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, 
     uint32_t na = activeCount(d), edge = shift ? d.bw + d.bh + 1u : 0u;
     // the tiles: the active list's blocks, then (shifted) the edge tiles,
     // workgroup indices B .. B + edge of the reference's dispatch
-    for (uint32_t v = blockIdx.x; v < na + edge; v += gridDim.x) {
+    for (uint32_t v = bid; v < na + edge; v += nbid) {
         uint32_t wid = v < na ? v : d.B + (v - na);
         i2 b = tileBlock(d, wid, shift);
         i2 origin = I2(b.x * WIG_BLOCK - shift, b.y * WIG_BLOCK - shift);
@@ -1062,9 +1097,12 @@ __global__ void __launch_bounds__(64) k_relax(Dev d, uint32_t cycle, int shift, 
 
 // restrictLevel into level 1 (:2342-2377), each iteration: rhs from the
 // fluid children's residuals; rho as k_levelRho put it at the solve's start
-__global__ void __launch_bounds__(64) k_restrict(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(64) k_restrict(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
+    const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
     if (converged(d, cycle) || activeCount(d) == 0u) return;
-    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
+    i2 c = I2((int32_t)(bx * 8u + threadIdx.x % 8u), (int32_t)(by * 8u + threadIdx.x / 8u));
     if (!inLevel(d, 1u, c)) return;
     uint64_t at = levelSlot(d, 1u, c);
     float rho = FLD(d, at + 1u, 0);
@@ -1084,9 +1122,12 @@ __global__ void __launch_bounds__(64) k_restrict(Dev d, uint32_t cycle) {
 }
 // level 1's rho (restrictLevel's, :2342-2377), once a solve: it depends on
 // the fine rho alone
-__global__ void __launch_bounds__(64) k_levelRho(Dev d) {
+__global__ void __launch_bounds__(64) k_levelRho(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || d.fluid_cycles == 0u) return;
+    const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
     if (activeCount(d) == 0u) return;
-    i2 c = I2((int32_t)(blockIdx.x * 8u + threadIdx.x % 8u), (int32_t)(blockIdx.y * 8u + threadIdx.x / 8u));
+    i2 c = I2((int32_t)(bx * 8u + threadIdx.x % 8u), (int32_t)(by * 8u + threadIdx.x / 8u));
     if (!inLevel(d, 1u, c)) return;
     float rho = 0.0f;
     i2 b = I2(c.x * 2 / WIG_BLOCK, c.y * 2 / WIG_BLOCK);
@@ -1145,7 +1186,7 @@ struct LevelTile {
     float *px, *py, *rx, *ry, *rho, *bR, *bD;
     int32_t R, ox, oy;   // side with the halo; the halo's corner in level cells
 };
-__device__ __forceinline__ LevelTile levelTile(float *sm, uint32_t T, uint32_t H) {
+__device__ __forceinline__ LevelTile levelTile(float *sm, uint32_t T, uint32_t H, uint32_t bx, uint32_t by) {
     LevelTile t;
     t.R = (int32_t)(T + 2u * H);
     uint32_t n = (uint32_t)(t.R * t.R);
@@ -1156,8 +1197,8 @@ __device__ __forceinline__ LevelTile levelTile(float *sm, uint32_t T, uint32_t H
     t.rho = sm + 4u * n;
     t.bR = sm + 5u * n;
     t.bD = sm + 6u * n;
-    t.ox = (int32_t)(blockIdx.x * T) - (int32_t)H;
-    t.oy = (int32_t)(blockIdx.y * T) - (int32_t)H;
+    t.ox = (int32_t)(bx * T) - (int32_t)H;
+    t.oy = (int32_t)(by * T) - (int32_t)H;
     return t;
 }
 // the tile's cells, 16 x 16 threads stepping over its rows and columns
@@ -1211,11 +1252,15 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
         __syncthreads();
     }
 }
-__global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Dev d, uint32_t cycle, uint32_t l, uint32_t T) {
+__global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t cycle, uint32_t l) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
+    const uint32_t T = d.tile[l];
+    const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
     if (converged(d, cycle) || activeCount(d) == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs + 1u;
-    LevelTile t = levelTile(sm, T, H);
+    LevelTile t = levelTile(sm, T, H, bx, by);
     float *resx = sm + 7u * (uint32_t)(t.R * t.R), *resy = resx + T * T;
     FOR_TILE(t, x, y) {
         int32_t j = y * t.R + x;
@@ -1257,7 +1302,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Dev d, uint32_t cycl
     const uint32_t T2 = T / 2u, lc = l + 1u;
     const int32_t span = 1 << lc;
     for (uint32_t k = threadIdx.x; k < T2 * T2; k += TILE_THREADS) {
-        i2 c = I2((int32_t)(blockIdx.x * T2 + k % T2), (int32_t)(blockIdx.y * T2 + k / T2));
+        i2 c = I2((int32_t)(bx * T2 + k % T2), (int32_t)(by * T2 + k / T2));
         if (!inLevel(d, lc, c)) continue;
         float rho = 0.0f, ox = 0.0f, oy = 0.0f;
         bool shortcut = false;
@@ -1289,11 +1334,15 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Dev d, uint32_t cycl
         *(float4 *)&FLD(d, at + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
     }
 }
-__global__ void __launch_bounds__(TILE_THREADS) k_levelUp(Dev d, uint32_t cycle, uint32_t l, uint32_t T) {
+__global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t cycle, uint32_t l) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
+    const uint32_t T = d.tile[l];
+    const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
     if (converged(d, cycle) || activeCount(d) == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs;
-    LevelTile t = levelTile(sm, T, H);
+    LevelTile t = levelTile(sm, T, H, bx, by);
     // level l + 1 under the tile, from one cell before its first to one after its last
     const int32_t cx0 = (t.ox >> 1) - 1, cy0 = (t.oy >> 1) - 1, Rc = t.R / 2 + 3;
     float *cpx = sm + 7u * (uint32_t)(t.R * t.R), *cpy = cpx + Rc * Rc, *crho = cpy + Rc * Rc;
@@ -1419,7 +1468,9 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
     }
     __syncthreads();
 }
-__global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     extern __shared__ float sm[];
     const uint32_t l0 = activeCount(d) == 0u ? d.L : d.lds_level, L = d.L;
@@ -1560,12 +1611,6 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(Dev d, uint32_t cycle) {
 }
 
 // The step's Params words into device memory (k_setStep), before the step.
-__global__ void k_setStep(StepParams *sp, uint32_t frame, uint32_t seed, float earthG) {
-    sp->frame = frame;
-    sp->seed = seed;
-    sp->earthG = earthG;
-    sp->pad = 0u;
-}
 
 // ------------------------------------------------ conjugate gradients
 // applyA (:2595-2613) from the packed betas: every face but an obstacle's
@@ -1584,7 +1629,9 @@ __device__ __forceinline__ float applyA(const Dev &d, i2 c, uint32_t i, uint32_t
     return diag * CGV(d, i, 2) - sum;
 }
 
-__global__ void __launch_bounds__(64) k_cgStart(Dev d) {                     // :2665-2674
+__global__ void __launch_bounds__(64) k_cgStart(WorldGrid g) {                     // :2665-2674
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -1593,7 +1640,9 @@ __global__ void __launch_bounds__(64) k_cgStart(Dev d) {                     // 
         CNT(d, i, G_REGION) = i;
     }
 }
-__global__ void __launch_bounds__(64) k_cgAnchor(Dev d) {                    // :2692-2705
+__global__ void __launch_bounds__(64) k_cgAnchor(WorldGrid g) {                    // :2692-2705
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c) || gridZ(d, c) <= 0.0f) continue;
@@ -1644,14 +1693,21 @@ __device__ void sumPlaces2(const Dev &d, uint32_t w0, uint32_t w1, double *s0, d
     __syncthreads();
 }
 #define FOR_ACTIVE_BLOCKS4(d, wid) \
-    for (uint32_t wid = blockIdx.x * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += gridDim.x * WAVES)
+    for (uint32_t wid = bid * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += nbid * WAVES)
 #define LANE (threadIdx.x % 64u)
 
 // cgResidual (:2708-2717), and ||b||^2 for the relative stop
-__global__ void __launch_bounds__(256) k_cgResidual(Dev d) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+__global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
+    if (bid == 0 && threadIdx.x == 0) {
         SolveHead h = {};
         *d.head = h;
+        // the first convergence checkpoint: WIG_CHUNK_AUTO takes as many
+        // iterations as the last solve that stopped at one ran
+        uint32_t last = d.wstate[WS_LAST];
+        d.wstate[WS_FIRST] = d.first_chunk == 0xffffffffu ? (last > 1u ? last : 1u) : d.first_chunk;
+        d.wstate[WS_STOP] = NO_STOP;
     }
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
@@ -1666,7 +1722,9 @@ __global__ void __launch_bounds__(256) k_cgResidual(Dev d) {
         if (LANE == 0) d.bpart[4ull * wid + 2] = bb;
     }
 }
-__global__ void __launch_bounds__(1024) k_sumNorm(Dev d) {
+__global__ void __launch_bounds__(1024) k_sumNorm(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     __shared__ double s0[SUM_LANES], s1[SUM_LANES];
     double bb, unused;
     sumPlaces2(d, 2, 2, s0, s1, &bb, &unused);
@@ -1676,7 +1734,9 @@ __global__ void __launch_bounds__(1024) k_sumNorm(Dev d) {
 // layout is with the Dev struct).  p is 0 and the drift as it stands, the
 // rhs -r and the drift rhs; the ring of cells beside the blocks worked on
 // holds its field values, as the relaxations' rims read them there.
-__global__ void __launch_bounds__(64) k_cgPrepare(Dev d) {
+__global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 bk = activeBlock(d, wid), c = blockCell(bk, threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -1714,7 +1774,9 @@ __global__ void __launch_bounds__(64) k_cgPrepare(Dev d) {
     }
 }
 // cgRho + cgBeta (:2736-2764): r.z and z.Ap, and Notay's flexible beta in FP64
-__global__ void __launch_bounds__(256) k_cgRho(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgRho(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
@@ -1733,7 +1795,9 @@ __global__ void __launch_bounds__(256) k_cgRho(Dev d, uint32_t cycle) {
         }
     }
 }
-__global__ void __launch_bounds__(1024) k_cgBeta(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     double s0, s1;
@@ -1748,7 +1812,9 @@ __global__ void __launch_bounds__(1024) k_cgBeta(Dev d, uint32_t cycle) {
     }
 }
 // cgDirection (:2767-2776): z is the post-relaxation's p
-__global__ void __launch_bounds__(64) k_cgDirection(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(64) k_cgDirection(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     double beta = ld_f64(&d.head->beta);
     FOR_ACTIVE_BLOCKS(d, wid) {
@@ -1760,7 +1826,9 @@ __global__ void __launch_bounds__(64) k_cgDirection(Dev d, uint32_t cycle) {
     }
 }
 // cgApply + cgAlpha (:2779-2814)
-__global__ void __launch_bounds__(256) k_cgApply(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgApply(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
@@ -1775,7 +1843,9 @@ __global__ void __launch_bounds__(256) k_cgApply(Dev d, uint32_t cycle) {
         if (LANE == 0) d.bpart[4ull * wid + 0] = v;
     }
 }
-__global__ void __launch_bounds__(1024) k_cgAlpha(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     double pq, unused;
@@ -1791,9 +1861,11 @@ __global__ void __launch_bounds__(1024) k_cgAlpha(Dev d, uint32_t cycle) {
 // cgUpdate (:2852-2880), and whether the next
 // iteration runs: max |r| under the tolerance (the reference's converged(),
 // :2223-2226), or ||r|| <= tol ||b||
-__global__ void __launch_bounds__(256) k_cgUpdate(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     bool conv = converged(d, cycle);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + (cycle + 1u) % 3u)] = 0u;
         if (!conv) atomicAdd(&d.blocks[COUNT_SLOT(d, COUNT_CYCLES)], 1u);
     }
@@ -1823,7 +1895,9 @@ __global__ void __launch_bounds__(256) k_cgUpdate(Dev d, uint32_t cycle) {
         }
     }
 }
-__global__ void __launch_bounds__(1024) k_cgCheck(Dev d, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g, uint32_t cycle) {
+    WORLD(g);
+    if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
@@ -1855,12 +1929,27 @@ __global__ void __launch_bounds__(1024) k_cgCheck(Dev d, uint32_t cycle) {
             c2 = fbits(*slot) < d.tol;
         }
         __hip_atomic_store(&h->converged, c2 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // converged: the host's next checkpoint (first, first + chunk, ...)
+        // after this iteration stops the solve
+        uint32_t first = d.wstate[WS_FIRST], chunk = d.chunk ? d.chunk : 1u;
+        if (c2 && first != 0u) {
+            uint32_t next = cycle + 1u;
+            uint32_t stop = next <= first ? first : first + (next - first + chunk - 1u) / chunk * chunk;
+            __hip_atomic_store(&d.wstate[WS_STOP], stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 // cgFinish (:2884-2893), and the words of the reference's fine level the
 // packed solve kept aside: the rhs (-r), the drift, and, when an iteration
 // ran, the rims the last post-relaxation read (prolongFine's p)
-__global__ void __launch_bounds__(64) k_cgFinish(Dev d, int ran) {
+__global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
+    const int ran = d.fluid_cycles > 0u ? 1 : 0;
+    if (bid == 0 && threadIdx.x == 0) {
+        uint32_t stop = d.wstate[WS_STOP];
+        if (stop < d.fluid_cycles) d.wstate[WS_LAST] = d.head->cycles;
+    }
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -1893,8 +1982,10 @@ __device__ void projectFace(const Dev &d, i2 at, bool isU) {
         FLD(d, f, 3) = wclamp(-beta * (driftAt(d, at) - driftAt(d, a)), -MAX_DRIFT, MAX_DRIFT);
     }
 }
-__global__ void __launch_bounds__(64) k_project(Dev d) {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+__global__ void __launch_bounds__(64) k_project(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
+    if (bid == 0 && threadIdx.x == 0)
         atomicMax(&d.blocks[COUNT_SLOT(d, COUNT_MOST_CYCLES)], d.blocks[COUNT_SLOT(d, COUNT_CYCLES)]);
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
@@ -1908,7 +1999,9 @@ __global__ void __launch_bounds__(64) k_project(Dev d) {
 // extrapolate, extrapolateFurther (:3220-3275).  A face written here holds z
 // under what its neighbours look for, so the pass reads the same in any
 // order.
-__global__ void __launch_bounds__(64) k_extrapolate(Dev d, int layer) {
+__global__ void __launch_bounds__(64) k_extrapolate(WorldGrid g, int layer) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -2012,10 +2105,12 @@ __device__ v2 escape(const Dev &d, v2 pos) {                                 // 
     }
     return inWorld(d, I2(c.x + way.x, c.y + way.y)) ? V2(pos.x + (float)way.x, pos.y + (float)way.y) : pos;
 }
-__global__ void __launch_bounds__(256) k_advect(Dev d) {
+__global__ void __launch_bounds__(256) k_advect(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = bid * blockDim.x + threadIdx.x; k < n; k += nbid * blockDim.x) {
         uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
         uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
         v2 orig = V2(fbits(mo.x), fbits(mo.y));
@@ -2035,7 +2130,7 @@ __global__ void __launch_bounds__(256) k_advect(Dev d) {
             if (nn >= 2u) {
                 float mean = (float)CNT(d, i, G_MASS) / (FIXF * (float)nn);
                 float lighter = (mean - liquidDensity(temp)) / fmaxf(mean, 1e-3f);
-                slip = V2(-0.0f / WATER_WEIGHT * SLIP * lighter, -d.sp->earthG / WATER_WEIGHT * SLIP * lighter);
+                slip = V2(-0.0f / WATER_WEIGHT * SLIP * lighter, -d.earthG / WATER_WEIGHT * SLIP * lighter);
             }
         }
         v2 cs = clampSpeed(vel);
@@ -2069,7 +2164,7 @@ __global__ void __launch_bounds__(256) k_advect(Dev d) {
         uint32_t li = 2u * cellIndex(d, cellOf(d, qpos));
         float lx = FLD(d, li, 0), lz = FLD(d, li, 2);
         if (lz > 0.0f) {
-            bool falling = lx < 0.5f * lz * wlength(V2(0.0f, d.sp->earthG));
+            bool falling = lx < 0.5f * lz * wlength(V2(0.0f, d.earthG));
             atomicOr(&CNT(d, cellIndex(d, cellOf(d, pos)), G_CAME), falling ? CAME_FALLING : CAME_PRESSING);
         }
     }
@@ -2077,16 +2172,20 @@ __global__ void __launch_bounds__(256) k_advect(Dev d) {
 
 // ================================================================ sort
 // sortCount (:3700-3708)
-__global__ void __launch_bounds__(256) k_sortCount(Dev d) {
+__global__ void __launch_bounds__(256) k_sortCount(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = bid * blockDim.x + threadIdx.x; k < n; k += nbid * blockDim.x) {
         v2 pos = V2(fbits(MOTION(d, d.parts, k, 0)), fbits(MOTION(d, d.parts, k, 1)));
         STATE(d, d.parts, k, 3) = atomicAdd(&CNT(d, cellIndex(d, cellOf(d, pos)), G_SORT), 1u);
     }
 }
 // scanBlocks (:3730-3753)
-__global__ void __launch_bounds__(64) k_scanBlocks(Dev d) {
+__global__ void __launch_bounds__(64) k_scanBlocks(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         uint32_t n = inWorld(d, c) ? CNT(d, cellIndex(d, c), G_SORT) : 0u;
@@ -2102,7 +2201,9 @@ __global__ void __launch_bounds__(64) k_scanBlocks(Dev d) {
     }
 }
 // scanTotals (:3757-3780): the blocks' runs' starts, in the active list's order
-__global__ void __launch_bounds__(256) k_scanTotals(Dev d) {
+__global__ void __launch_bounds__(256) k_scanTotals(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     __shared__ uint32_t s[256];
     uint32_t li = threadIdx.x, n = activeCount(d);
     uint32_t per = (n + 255u) / 256u, lo = li * per < n ? li * per : n, hi = lo + per < n ? lo + per : n;
@@ -2134,17 +2235,21 @@ __global__ void __launch_bounds__(256) k_scanTotals(Dev d) {
     }
 }
 // scanFinish (:3783-3790)
-__global__ void __launch_bounds__(64) k_scanFinish(Dev d) {
+__global__ void __launch_bounds__(64) k_scanFinish(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (inWorld(d, c)) CNT(d, cellIndex(d, c), G_START) += d.blocks[TOTAL_SLOT(d, wid)];
     }
 }
 // scatter (:3793-3808)
-__global__ void __launch_bounds__(256) k_scatter(Dev d) {
+__global__ void __launch_bounds__(256) k_scatter(WorldGrid g) {
+    WORLD(g);
+    if (!d.full) return;
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = bid * blockDim.x + threadIdx.x; k < n; k += nbid * blockDim.x) {
         uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
         uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
         uint4 af = *(const uint4 *)&AFFINE(d, d.parts, k, 0);
@@ -2182,7 +2287,9 @@ __device__ uint32_t airTo(const Dev &d, i2 c, bool gas) {                    // 
     }
     return WNONE;
 }
-__global__ void __launch_bounds__(64) k_rasterClaims(Dev d) {               // :3928-3948
+__global__ void __launch_bounds__(64) k_rasterClaims(WorldGrid g) {               // :3928-3948
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -2201,7 +2308,9 @@ __device__ __forceinline__ void setGasPair(const Dev &d, uint32_t i, uint32_t a,
     d.cells[at] = a;
     d.cells[at + 1] = b;
 }
-__global__ void __launch_bounds__(64) k_raster(Dev d) {                      // :3983-4119
+__global__ void __launch_bounds__(64) k_raster(WorldGrid g) {                      // :3983-4119
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -2277,7 +2386,7 @@ __global__ void __launch_bounds__(64) k_raster(Dev d) {                      // 
             velY = addExact(velY, fbits(MOTION(d, d.sorted, k, 3)));
         }
         Rng r;
-        seedRng(r, i, d.sp->frame, d.sp->seed ^ 0x7a57u);
+        seedRng(r, i, d.frame, d.seed ^ 0x7a57u);
         uint32_t life = wu32(floorf(livesx / livesy + rnd(r)));
         life = life < 255u ? life : 255u;
         word = packWord(wType(word), wAux(word), life, wShade(word), wFlags(word));
@@ -2287,7 +2396,9 @@ __global__ void __launch_bounds__(64) k_raster(Dev d) {                      // 
         CNT(d, i, G_TYPE) = (wType(word) | (wAux(word) << 8)) | (life << 16);
     }
 }
-__global__ void __launch_bounds__(64) k_unpaint(Dev d) {                     // :4163-4182
+__global__ void __launch_bounds__(64) k_unpaint(WorldGrid g) {                     // :4163-4182
+    WORLD(g);
+    if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -2298,4 +2409,24 @@ __global__ void __launch_bounds__(64) k_unpaint(Dev d) {                     // 
             setGasPair(d, i, 0u, 0u);
         }
     }
+}
+
+// ============================================================ batches
+// A convergence checkpoint of a batch: each world's stop (the iteration its
+// solve stops before, NO_STOP while it runs) and the iterations run.
+__global__ void __launch_bounds__(64) k_stops(WorldGrid g, uint32_t *out) {
+    WORLD(g);
+    if (threadIdx.x == 0) {
+        out[2u * wi_] = ld(&d.wstate[WS_STOP]);
+        out[2u * wi_ + 1u] = ld(&d.head->cycles);
+    }
+}
+// A batch's reports (js/sim.js:1760-1780): each world's FLUID_STATS, then
+// its MOST_CYCLES cleared as a report on steps does.
+__global__ void __launch_bounds__(64) k_reports(WorldGrid g, uint32_t *out) {
+    WORLD(g);
+    uint32_t t = threadIdx.x;
+    if (t < (uint32_t)FLUID_STATS) out[wi_ * (uint32_t)FLUID_STATS + t] = d.blocks[COUNT_SLOT(d, t)];
+    __syncthreads();
+    if (t == 0) d.blocks[COUNT_SLOT(d, COUNT_MOST_CYCLES)] = 0u;
 }

@@ -46,6 +46,15 @@ def lib():
         L.wig_kernel_name.argtypes = [ct.c_int]
         L.wig_kernel_count.restype = ct.c_int
         L.wig_set_tile.argtypes = [ct.c_void_p, ct.c_uint32, ct.c_uint32]
+        L.wig_batch_create.restype = ct.c_void_p
+        L.wig_batch_create.argtypes = [ct.POINTER(ct.c_void_p), ct.c_uint32]
+        L.wig_batch_destroy.argtypes = [ct.c_void_p]
+        L.wig_batch_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.wig_batch_step.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
+        L.wig_batch_report.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint32)]
+        L.wig_batch_sync.argtypes = [ct.c_void_p]
+        L.wig_batch_time_kernel.argtypes = [ct.c_void_p, ct.c_int]
+        L.wig_batch_kernel_time.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint32)]
         _lib = L
     return _lib
 
@@ -159,6 +168,60 @@ class GpuWorld:
         """(ms summed, launches) of the timed kernel since time_kernel."""
         ms, n = ct.c_double(), ct.c_uint32()
         _check(lib().wig_kernel_time(self._w, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+
+class GpuBatch:
+    """Worlds stepped together, each kernel of a step one launch over all of
+    them (wig_batch_*).  The worlds stay the caller's."""
+
+    def __init__(self, worlds):
+        L = lib()
+        self.worlds = list(worlds)
+        arr = (ct.c_void_p * len(self.worlds))(*[w._w for w in self.worlds])
+        self._b = L.wig_batch_create(arr, len(self.worlds))
+        if not self._b:
+            raise WigError(f"wig_batch_create failed: {L.wig_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().wig_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.worlds)
+
+    def set_stream(self, stream_ptr):
+        _check(lib().wig_batch_set_stream(self._b, ct.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def step(self, params, ctls):
+        n = len(self.worlds)
+        assert len(params) == n and len(ctls) == n
+        P = (abi.WigParams * n)(*params)
+        C = (abi.WigStepCtl * n)(*ctls)
+        return _check(lib().wig_batch_step(self._b, P, C))
+
+    def report(self):
+        n = len(self.worlds)
+        out = (ct.c_uint32 * (abi.STATS * n))()
+        _check(lib().wig_batch_report(self._b, out))
+        return np.array(out, dtype=np.uint32).reshape(n, abi.STATS)
+
+    def sync(self):
+        _check(lib().wig_batch_sync(self._b))
+
+    def time_kernel(self, name):
+        _check(lib().wig_batch_time_kernel(self._b, -1 if name is None else kernel_id(name)))
+
+    def kernel_time(self):
+        ms, n = ct.c_double(), ct.c_uint32()
+        _check(lib().wig_batch_kernel_time(self._b, ct.byref(ms), ct.byref(n)))
         return ms.value, n.value
 
 

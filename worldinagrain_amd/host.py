@@ -140,3 +140,18 @@ class Simulation:
         most = int(stats[abi.STAT_MOST_CYCLES])
         cap = self.solver.max_cycles
         self.fluid_cycles = cap if most >= self.fluid_cycles else min(most + FLUID_CYCLES_SPARE, cap)
+
+
+def step_batch(batch, sims, read_report=True):
+    """One step of each simulation, all in one batch step (wig_batch_step):
+    each takes its own Params and controls as its step would, and its own
+    report after."""
+    params = [s.params() for s in sims]
+    ctls = [s.ctl() for s in sims]
+    status = batch.step(params, ctls)
+    for s in sims:
+        s.frame_no += 1
+    if read_report:
+        for s, row in zip(sims, batch.report()):
+            s.take_report(row)
+    return status
