@@ -82,6 +82,9 @@ def cpu_baseline(cells, kinds, size, tol, steps):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", choices=("single", "batch"), default="single",
+                    help="single: the 1024x1024 world (the metric's configuration); batch: many worlds in one batch")
+    ap.add_argument("--worlds", type=int, default=512, help="batch: how many worlds")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
@@ -107,6 +110,9 @@ def main():
     import torch
     from worldinagrain_amd import abi
     from worldinagrain_amd.gpu import GpuWorld, kernel_names
+
+    if args.workload == "batch":
+        return run_batch(args)
 
     if args.copy_mib:
         src = torch.empty(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -210,6 +216,123 @@ def main():
         out["kernel_ms_one_step"] = {k: round(v[0], 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:12]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, kinds, size, args.tol, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+def run_batch(args):
+    """Many worlds stepped together: sides drawn from {64, 128, 256, 512}
+    with probabilities {.4, .3, .2, .1}; each a dam block over 20-50% of its
+    cells and 0-15% solid discs (scenes.batch_world), log2(side) - 2 MG
+    levels, relative residual `tol`.  Tile sides and sweep counts per side
+    class come from the autotuner (on copies of a world of the class).  The
+    same worlds stepped one at a time (a sample of each class) give the
+    sequential rate beside it."""
+    import math
+    import numpy as np
+    import torch
+    from worldinagrain_amd import abi, scenes, elements as E
+    from worldinagrain_amd.gpu import GpuWorld, GpuBatch
+    from worldinagrain_amd.host import step_batch
+    from worldinagrain_amd.tune import tune_sweeps, tune_tiles_batch
+
+    kinds = E.kinds()
+    sides = scenes.batch_sides(args.worlds, seed=7)
+    worlds, sims = [], []
+    for i, side in enumerate(sides):
+        cells, _ = scenes.batch_world(side, seed=1000 + i)
+        w = GpuWorld(side, side, coarse_levels=int(math.log2(side)) - 3, kinds=kinds)
+        w.upload(abi.PART_CELLS, cells)
+        worlds.append(w)
+        sims.append(config_simulation(w, side, args.tol))
+    stream = torch.cuda.current_stream()
+    batch = GpuBatch(worlds)
+    batch.set_stream(stream.cuda_stream)
+    for w in worlds:
+        w.set_stream(stream.cuda_stream)
+    for _ in range(max(args.warmup, 1)):
+        step_batch(batch, sims)
+    # the autotuner: sweep counts per side class on copies of the class's
+    # first world; then tile sides per class in the batch itself
+    tuned = {}
+    if not args.no_tune:
+        import copy
+        for side in sorted(set(sides)):
+            i = sides.index(side)
+            clone = worlds[i].clone()
+            clone.set_stream(stream.cuda_stream)
+            sim = copy.copy(sims[i])
+            sim.world = clone
+            solver, scores = tune_sweeps(clone, sim)
+            clone.close()
+            tuned[side] = {"sweeps": [solver.sweep_pairs, solver.level_pairs, solver.coarse_sweeps]}
+            for j, sj in enumerate(sides):
+                if sj == side:
+                    sims[j].solver = solver
+        groups = {side: [i for i, s in enumerate(sides) if s == side] for side in sorted(set(sides))}
+        for side, tiles in tune_tiles_batch(batch, sims, groups).items():
+            tuned[side]["tiles"] = tiles
+    step_batch(batch, sims)
+    per_kernel = {}
+    if args.kernel_table:
+        from worldinagrain_amd.gpu import kernel_names
+        for name in kernel_names():
+            batch.time_kernel(name)
+            step_batch(batch, sims, read_report=False)
+            per_kernel[name] = batch.kernel_time()
+        batch.time_kernel(None)
+    torch.cuda.synchronize()
+    K = args.steps
+    t0 = [w.run_totals() for w in worlds]
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    lat = []
+    ev0.record(stream)
+    for _ in range(K):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        step_batch(batch, sims, read_report=False)
+        b.record(stream)
+        lat.append((a, b))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = ev0.elapsed_time(ev1)
+    t1 = [w.run_totals() for w in worlds]
+    parts = sum(b.particles - a.particles for a, b in zip(t0, t1))
+    cycles = sum(b.cycles - a.cycles for a, b in zip(t0, t1))
+    steps = sum(b.steps - a.steps for a, b in zip(t0, t1))
+    lat_ms = np.array([a.elapsed_time(b) for a, b in lat])
+    # the same worlds one at a time: a sample of each class, stepped alone
+    seq_ms_per_round = 0.0
+    seq = {}
+    for side in sorted(set(sides)):
+        idx = [i for i, s in enumerate(sides) if s == side][:4]
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in idx:
+            for _ in range(2):
+                sims[i].step(read_report=False)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        per = e0.elapsed_time(e1) / (2 * len(idx))
+        seq[side] = per
+        seq_ms_per_round += per * sides.count(side)
+    out = {
+        "metric": "solves/s and particle-steps/s of a batch of worlds", "workload": "batch",
+        "value": steps / (total_ms / 1e3), "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": total_ms / K, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "particle_steps_per_s": parts / (total_ms / 1e3), "pcg_iterations_per_solve": cycles / max(steps, 1),
+        "p50_step_ms": float(np.percentile(lat_ms, 50)), "p99_step_ms": float(np.percentile(lat_ms, 99)),
+        "config": {"workload": f"{len(sides)} worlds, sides {{64,128,256,512}} w.p. {{.4,.3,.2,.1}}, dam blocks over "
+                               f"20-50% of cells, 0-15% solid discs, log2(side)-2 levels, {args.tol:g} relative residual",
+                   "worlds": {str(s): sides.count(s) for s in sorted(set(sides))}, "cells": int(sum(s * s for s in sides)),
+                   "tuned": {str(k): v for k, v in tuned.items()}},
+        "sequential": {"ms_per_round": seq_ms_per_round, "solves_per_s": len(sides) / (seq_ms_per_round / 1e3),
+                       "ms_per_world_step": {str(k): v for k, v in seq.items()},
+                       "note": "the same worlds stepped one at a time (4 of each side, 2 steps each), scaled to the batch"},
+    }
+    out["speedup_vs_sequential"] = out["value"] / out["sequential"]["solves_per_s"]
+    if per_kernel:
+        out["kernel_ms_one_step"] = {k: [round(v[0], 4), v[1]] for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:16]}
     print(json.dumps(out), flush=True)
 
 

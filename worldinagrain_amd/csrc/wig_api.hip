@@ -42,7 +42,6 @@ __global__ void k_sumNorm(WorldGrid g);
 __global__ void k_cgPrepare(WorldGrid g);
 __global__ void k_cgRho(WorldGrid g, uint32_t cycle);
 __global__ void k_cgBeta(WorldGrid g, uint32_t cycle);
-__global__ void k_cgDirection(WorldGrid g, uint32_t cycle);
 __global__ void k_cgApply(WorldGrid g, uint32_t cycle);
 __global__ void k_cgAlpha(WorldGrid g, uint32_t cycle);
 __global__ void k_cgUpdate(WorldGrid g, uint32_t cycle);
@@ -84,7 +83,7 @@ static int fail(int code, const char *fmt, ...) {
 #define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
     X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
     X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
-    X(cgRho) X(cgBeta) X(cgDirection) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
+    X(cgRho) X(cgBeta) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
     X(scatter) X(rasterClaims) X(raster) X(unpaint)
 #define X_ENUM(n) KID_##n,
@@ -346,7 +345,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         }
     }
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
-        || hipMalloc(&w->fine, 41ull * w->d.C + 256u) != hipSuccess
+        || hipMalloc(&w->fine, 61ull * w->d.C + 256u) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess
         || hipMalloc((void **)&w->wstate, 4u * WS_WORDS) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
@@ -356,7 +355,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     for (int p = 0; p < WIG_PART_COUNT; p++)
         if (zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
     zero(w->own, w->kind, 256);
-    zero(w->own, w->fine, 41ull * w->d.C + 256u);
+    zero(w->own, w->fine, 61ull * w->d.C + 256u);
     zero(w->own, w->totals, sizeof(WigRunTotals));
     zero(w->own, w->wstate, 4u * WS_WORDS);
     Dev &d = w->d;
@@ -400,7 +399,12 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         d.pa = (float2 *)(f + 16ull * d.C);
         d.pb = (float2 *)(f + 24ull * d.C);
         d.r2 = (float2 *)(f + 32ull * d.C);
-        d.cflag = f + 40ull * d.C;
+        d.cx = (float *)(f + 40ull * d.C);
+        d.cr = (float *)(f + 44ull * d.C);
+        d.cp0 = (float *)(f + 48ull * d.C);
+        d.cp1 = (float *)(f + 52ull * d.C);
+        d.cq = (float *)(f + 56ull * d.C);
+        d.cflag = f + 60ull * d.C;
     }
     WigWorld *self = w;
     w->solo = batch_new(&self, 1u, w->own);
@@ -689,7 +693,6 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         launch(b, KID_relax, k_relax, SH_BG, 64, cycle, 1);
         launch(b, KID_cgRho, k_cgRho, SH_BG4, 256, cycle);
         launch(b, KID_cgBeta, k_cgBeta, SH_ONE, 1024, cycle);
-        launch(b, KID_cgDirection, k_cgDirection, SH_BG, 64, cycle);
         launch(b, KID_cgApply, k_cgApply, SH_BG4, 256, cycle);
         launch(b, KID_cgAlpha, k_cgAlpha, SH_ONE, 1024, cycle);
         launch(b, KID_cgUpdate, k_cgUpdate, SH_BG4, 256, cycle);

@@ -137,6 +137,9 @@ struct Dev {
     float4 *coef;
     uint8_t *cflag;
     float2 *pa, *pb, *r2;
+    // and the CG vectors (fluidCg's x, r, p, Ap), p in two buffers: an
+    // iteration's cgApply reads the last p and writes the new one
+    float *cx, *cr, *cq, *cp0, *cp1;
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
     uint32_t tile[MG_LEVELS + 1];   // k_levelDown/k_levelUp's tile side, per level below lds_level
     // the step's: the Params words the liquid reads, and its controls

@@ -1613,21 +1613,27 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g, uint32_t cycle)
 // The step's Params words into device memory (k_setStep), before the step.
 
 // ------------------------------------------------ conjugate gradients
+// cgDirection's p (:2767-2776) at cell i: z + beta p, 0 off the fluid
+__device__ __forceinline__ float newP(const Dev &d, uint32_t i, double beta, const float *pold) {
+    return (d.cflag[i] & CF_FLUID) ? (float)((double)d.pa[i].x + beta * (double)pold[i]) : 0.0f;
+}
 // applyA (:2595-2613) from the packed betas: every face but an obstacle's
-// counts in the diagonal, only a fluid neighbour's p in the sum
-__device__ __forceinline__ float applyA(const Dev &d, i2 c, uint32_t i, uint32_t f) {
-    float4 cf = d.coef[i];
+// counts in the diagonal, only a fluid neighbour's p in the sum (each
+// neighbour's new p taken as its own thread takes it)
+__device__ __forceinline__ float applyA(const Dev &d, i2 c, uint32_t f, float p, double beta, const float *pold) {
+    float4 cf = d.coef[cellIndex(d, c)];
     const float v[4] = {cf.x, cf.y, cf.z, cf.w};
     float sum = 0.0f, diag = 0.0f;
     for (int k = 0; k < 4; k++) {
-        float beta = fabsf(v[k]);
-        if (beta == 0.0f) continue;
-        if (__float_as_uint(v[k]) >> 31) sum += beta * CGV(d, cellIndex(d, beside(c, k)), 2);
-        diag += beta;
+        float b = fabsf(v[k]);
+        if (b == 0.0f) continue;
+        if (__float_as_uint(v[k]) >> 31) sum += b * newP(d, cellIndex(d, beside(c, k)), beta, pold);
+        diag += b;
     }
     diag += (f & CF_PINNED) ? PINF : 0.0f;
-    return diag * CGV(d, i, 2) - sum;
+    return diag * p - sum;
 }
+__device__ __forceinline__ float *pBuf(const Dev &d, uint32_t k) { return (k & 1u) ? d.cp1 : d.cp0; }
 
 __global__ void __launch_bounds__(64) k_cgStart(WorldGrid g) {                     // :2665-2674
     WORLD(g);
@@ -1636,7 +1642,10 @@ __global__ void __launch_bounds__(64) k_cgStart(WorldGrid g) {                  
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
-        *(float4 *)&CGV(d, i, 0) = make_float4(gridX(d, c), 0.0f, 0.0f, 0.0f);
+        d.cx[i] = gridX(d, c);
+        d.cr[i] = 0.0f;
+        d.cq[i] = 0.0f;
+        d.cp0[i] = 0.0f;
         CNT(d, i, G_REGION) = i;
     }
 }
@@ -1714,7 +1723,7 @@ __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
         double bb = 0.0;
         if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
             uint32_t i = cellIndex(d, c);
-            CGV(d, i, 1) = -residualAt(d, 0u, c).x;
+            d.cr[i] = -residualAt(d, 0u, c).x;
             double bi = -(double)FLD(d, 2u * i, 1);
             bb = bi * bi;
         }
@@ -1757,7 +1766,7 @@ __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) {
                 b[k] = rn > 0.0f ? -beta : beta;
             }
             cf = make_float4(b[0], b[1], b[2], b[3]);
-            r = make_float2(-CGV(d, i, 1), FLD(d, 2u * i + 1u, 1));
+            r = make_float2(-d.cr[i], FLD(d, 2u * i + 1u, 1));
         }
         d.coef[i] = cf;
         d.cflag[i] = (uint8_t)f;
@@ -1784,8 +1793,8 @@ __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g, uint32_t cycle) {
         if (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID)) {
             uint32_t i = cellIndex(d, c);
             double z = (double)d.pa[i].x;
-            v0 = (double)CGV(d, i, 1) * z;
-            v1 = (double)CGV(d, i, 3) * z;
+            v0 = (double)d.cr[i] * z;
+            v1 = (double)d.cq[i] * z;
         }
         v0 = wave_tree_sum(v0);
         v1 = wave_tree_sum(v1);
@@ -1811,33 +1820,25 @@ __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g, uint32_t cycle) {
         CGV(d, d.C + d.B, 2) = (float)beta;
     }
 }
-// cgDirection (:2767-2776): z is the post-relaxation's p
-__global__ void __launch_bounds__(64) k_cgDirection(WorldGrid g, uint32_t cycle) {
-    WORLD(g);
-    if (!encoded(d, cycle)) return;
-    if (converged(d, cycle)) return;
-    double beta = ld_f64(&d.head->beta);
-    FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
-        if (!inWorld(d, c)) continue;
-        uint32_t i = cellIndex(d, c);
-        bool fluid = (d.cflag[i] & CF_FLUID) != 0u;
-        CGV(d, i, 2) = fluid ? (float)((double)d.pa[i].x + beta * (double)CGV(d, i, 2)) : 0.0f;
-    }
-}
-// cgApply + cgAlpha (:2779-2814)
+// cgDirection + cgApply + cgAlpha (:2767-2814): the iteration's p (into
+// the other p buffer) and q = A p
 __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g, uint32_t cycle) {
     WORLD(g);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
+    double beta = ld_f64(&d.head->beta);
+    const float *pold = pBuf(d, cycle);
+    float *pnew = pBuf(d, cycle + 1u);
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double v = 0.0;
         if (inWorld(d, c)) {
             uint32_t i = cellIndex(d, c), f = d.cflag[i];
-            float q = (f & CF_FLUID) ? applyA(d, c, i, f) : 0.0f;
-            CGV(d, i, 3) = q;
-            v = (double)CGV(d, i, 2) * (double)q;
+            float p = newP(d, i, beta, pold);
+            float q = (f & CF_FLUID) ? applyA(d, c, f, p, beta, pold) : 0.0f;
+            pnew[i] = p;
+            d.cq[i] = q;
+            v = (double)p * (double)q;
         }
         v = wave_tree_sum(v);
         if (LANE == 0) d.bpart[4ull * wid + 0] = v;
@@ -1877,11 +1878,11 @@ __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g, uint32_t cycle) {
         uint32_t most = 0;
         if (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID)) {
             uint32_t i = cellIndex(d, c);
-            float4 v = *(const float4 *)&CGV(d, i, 0);
-            float rn = (float)((double)v.y - alpha * (double)v.w);
-            float xn = (float)((double)v.x + alpha * (double)v.z);
-            CGV(d, i, 0) = xn;
-            CGV(d, i, 1) = rn;
+            float x = d.cx[i], r = d.cr[i], p = pBuf(d, cycle + 1u)[i], q = d.cq[i];
+            float rn = (float)((double)r - alpha * (double)q);
+            float xn = (float)((double)x + alpha * (double)p);
+            d.cx[i] = xn;
+            d.cr[i] = rn;
             d.pa[i].x = 0.0f;
             d.r2[i].x = -rn;
             most = ubits(fabsf(rn));
@@ -1940,8 +1941,9 @@ __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g, uint32_t cycle) {
     }
 }
 // cgFinish (:2884-2893), and the words of the reference's fine level the
-// packed solve kept aside: the rhs (-r), the drift, and, when an iteration
-// ran, the rims the last post-relaxation read (prolongFine's p)
+// packed solve kept aside: fluidCg's vectors, the rhs (-r), the drift, and,
+// when an iteration ran, the rims the last post-relaxation read
+// (prolongFine's p)
 __global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
@@ -1954,7 +1956,10 @@ __global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
-        FLD(d, 2u * i, 0) = CGV(d, i, 0);
+        // fluidCg as the iterations left it: p is the last iteration run's
+        float x = d.cx[i];
+        *(float4 *)&CGV(d, i, 0) = make_float4(x, d.cr[i], pBuf(d, ld(&d.head->cycles))[i], d.cq[i]);
+        FLD(d, 2u * i, 0) = x;
         CNT(d, i, G_REGION) = 0u;
         if (d.cflag[i] & CF_FLUID) {
             FLD(d, 2u * i, 1) = d.r2[i].x;

@@ -116,3 +116,45 @@ def discs_dam(width=1024, height=1024, discs=200, rmin=4, rmax=24, solid_share=0
     vel = (g.normal(0.0, vel_sigma, size=(height, width)) * dt, g.normal(0.0, vel_sigma, size=(height, width)) * dt)
     paint(w, width, height, fluid, E.WATER, vel=vel)
     return w, int(fluid.sum())
+
+
+def batch_world(side, seed):
+    """A world of the batching workload: a side x side grid with solid
+    discs over a share U[0, 0.15] of its cells (radius U[2, side / 16 + 2]),
+    and a dam-break block of water at rest over a share U[0.2, 0.5] of them:
+    a rectangle standing on the floor against the left wall, its width a
+    random U[sqrt(share), 1] of the side (SplitMix64 `seed`).  Returns
+    (cells, water cells)."""
+    rng = SplitMix64(seed)
+    share = 0.2 + 0.3 * rng.uniform()
+    solid_share = 0.15 * rng.uniform()
+    solid = np.zeros((side, side), dtype=bool)
+    yy, xx = np.mgrid[0:side, 0:side]
+    rmax = side / 16.0 + 2.0
+    for _ in range(400):
+        if solid.mean() >= solid_share:
+            break
+        cx, cy = rng.uniform() * side, rng.uniform() * side
+        r = 2.0 + rng.uniform() * (rmax - 2.0)
+        solid |= (xx + 0.5 - cx) ** 2 + (yy + 0.5 - cy) ** 2 <= r * r
+    a = np.sqrt(share) + (1.0 - np.sqrt(share)) * rng.uniform()
+    bw = max(1, min(side, int(round(a * side))))
+    bh = max(1, min(side, int(round(share * side * side / bw))))
+    w = empty_cells(side, side)
+    paint(w, side, side, solid, E.WALL)
+    fluid = np.zeros((side, side), dtype=bool)
+    fluid[side - bh:, :bw] = True
+    fluid &= ~solid
+    paint(w, side, side, fluid, E.WATER)
+    return w, int(fluid.sum())
+
+
+def batch_sides(n, seed=7, sides=(64, 128, 256, 512), probs=(0.4, 0.3, 0.2, 0.1)):
+    """n world sides drawn with the given probabilities (SplitMix64 `seed`)."""
+    rng = SplitMix64(seed)
+    cum = np.cumsum(probs)
+    out = []
+    for _ in range(n):
+        u = rng.uniform()
+        out.append(sides[int(min(np.searchsorted(cum, u, side="right"), len(sides) - 1))])
+    return out

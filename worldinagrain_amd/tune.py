@@ -49,6 +49,51 @@ def tune_tiles(world, sim, sides=TILE_SIDES, steps=2):
     return chosen
 
 
+def tune_tiles_batch(batch, sims, groups, sides=TILE_SIDES, steps=1):
+    """Tile sides for worlds stepped in a batch, where the tile kernels
+    share the device with every other world's: for each group of worlds
+    (a list of indices into the batch, e.g. one side class) and each tiled
+    level, each side in turn for the whole group, scored by the batch's
+    tile kernels' mean launch time over `steps` batch steps.  Returns
+    {group key: {level: side}}."""
+    from .host import step_batch
+
+    def score():
+        ms = 0.0
+        for k in ("levelDown", "levelUp"):
+            batch.time_kernel(k)
+            for _ in range(steps):
+                step_batch(batch, sims, read_report=False)
+            t, n = batch.kernel_time()
+            ms += t / max(n, 1)
+        batch.time_kernel(None)
+        return ms
+
+    chosen = {}
+    for key, idx in groups.items():
+        chosen[key] = {}
+        worlds = [batch.worlds[i] for i in idx]
+        level = 1
+        while True:
+            try:
+                for w in worlds:
+                    w.set_tile(level, sides[0])
+            except WigError:
+                break
+            best = None
+            for side in sides:
+                for w in worlds:
+                    w.set_tile(level, side)
+                ms = score()
+                if best is None or ms < best[0]:
+                    best = (ms, side)
+            for w in worlds:
+                w.set_tile(level, best[1])
+            chosen[key][level] = best[1]
+            level += 1
+    return chosen
+
+
 def tune_sweeps(world, sim, candidates=SWEEP_CANDIDATES, steps=3):
     """(sweep_pairs, level_pairs, coarse_sweeps) by the solve's time a step
     on copies of the world.  Returns (best settings, [(ms a step, iterations
