@@ -210,6 +210,14 @@ int wig_batch_step(WigBatch *b, const WigParams *params, const WigStepCtl *ctl);
 /* stats[i * WIG_STATS ...]: each world's report (as wig_report).  Waits. */
 int wig_batch_report(WigBatch *b, uint32_t *stats);
 int wig_batch_sync(WigBatch *b);
+/* hipGraphs: with `on`, each phase of a step (the passes before the solve,
+ * its start, one iteration, its end, the rest) is captured once into a
+ * graph on its first run and replayed after, until the launch layout changes
+ * (world tiles, LDS sizes, the stream).  The stream must be one of the
+ * caller's or the world's own, not the null stream.  Off while a kernel is
+ * timed.  This library's own. */
+int wig_set_graphs(WigWorld *w, int on);
+int wig_batch_set_graphs(WigBatch *b, int on);
 /* Kernel timing over the batch's launches, as wig_time_kernel / wig_kernel_time. */
 int wig_batch_time_kernel(WigBatch *b, int id);
 int wig_batch_kernel_time(WigBatch *b, double *ms, uint32_t *launches);

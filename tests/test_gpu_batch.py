@@ -84,3 +84,37 @@ def test_batch_of_one_and_regrouping():
             step_batch(b2, [so, sb])
     d = compare_worlds(wa, wb)
     assert not d, "; ".join(d)
+
+
+def test_graph_replays_match_eager():
+    """Steps replayed as hipGraphs (a batch's and a lone world's) against the
+    same worlds stepped with plain launches, every step: each replay runs
+    after the host's own reads of the last step's reports."""
+    from worldinagrain_amd.gpu import GpuBatch
+    spec = worlds_spec()[1:4]
+    eager, graphed = [], []
+    for name, W, H, cells, mode in spec:
+        we, wg = make_gpu(W, H, cells), make_gpu(W, H, cells)
+        eager.append((name, we, make_sim(we, W, H, mode)))
+        graphed.append((name, wg, make_sim(wg, W, H, mode)))
+    batch = GpuBatch([w for _, w, _ in graphed])
+    batch.set_graphs(True)
+    W, H = 96, 80
+    cells, _ = scenes.discs_dam(W, H, discs=8, rmin=3, rmax=8)
+    lone_e, lone_g = make_gpu(W, H, cells), make_gpu(W, H, cells)
+    lone_g.set_graphs(True)
+    sle, slg = config_sim(lone_e, W, H), config_sim(lone_g, W, H)
+    for s in range(5):
+        say(f"[graphs] step {s}")
+        step_batch(batch, [sim for _, _, sim in graphed])
+        for _, _, sim in eager:
+            sim.step()
+        sle.step()
+        slg.step()
+        for (name, we, se), (_, wg, sg) in zip(eager, graphed):
+            assert np.array_equal(se.last_stats, sg.last_stats), (name, s)
+            d = compare_worlds(we, wg)
+            assert not d, f"{name} step {s}: " + "; ".join(d)
+        d = compare_worlds(lone_e, lone_g)
+        assert not d, f"lone world step {s}: " + "; ".join(d)
+    batch.close()

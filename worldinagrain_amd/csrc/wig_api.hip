@@ -29,23 +29,23 @@ __global__ void k_sealAnchor(WorldGrid g);
 __global__ void k_classify(WorldGrid g);
 __global__ void k_setFaces(WorldGrid g);
 __global__ void k_divergence(WorldGrid g);
-__global__ void k_relax(WorldGrid g, uint32_t cycle, int post);
-__global__ void k_restrict(WorldGrid g, uint32_t cycle);
+__global__ void k_relax(WorldGrid g, int post);
+__global__ void k_restrict(WorldGrid g);
 __global__ void k_levelRho(WorldGrid g);
-__global__ void k_levelDown(WorldGrid g, uint32_t cycle, uint32_t l);
-__global__ void k_coarseLDS(WorldGrid g, uint32_t cycle);
-__global__ void k_levelUp(WorldGrid g, uint32_t cycle, uint32_t l);
+__global__ void k_levelDown(WorldGrid g, uint32_t l);
+__global__ void k_coarseLDS(WorldGrid g);
+__global__ void k_levelUp(WorldGrid g, uint32_t l);
 __global__ void k_cgStart(WorldGrid g);
 __global__ void k_cgAnchor(WorldGrid g);
 __global__ void k_cgResidual(WorldGrid g);
 __global__ void k_sumNorm(WorldGrid g);
 __global__ void k_cgPrepare(WorldGrid g);
-__global__ void k_cgRho(WorldGrid g, uint32_t cycle);
-__global__ void k_cgBeta(WorldGrid g, uint32_t cycle);
-__global__ void k_cgApply(WorldGrid g, uint32_t cycle);
-__global__ void k_cgAlpha(WorldGrid g, uint32_t cycle);
-__global__ void k_cgUpdate(WorldGrid g, uint32_t cycle);
-__global__ void k_cgCheck(WorldGrid g, uint32_t cycle);
+__global__ void k_cgRho(WorldGrid g);
+__global__ void k_cgBeta(WorldGrid g);
+__global__ void k_cgApply(WorldGrid g);
+__global__ void k_cgAlpha(WorldGrid g);
+__global__ void k_cgUpdate(WorldGrid g);
+__global__ void k_cgCheck(WorldGrid g);
 __global__ void k_cgFinish(WorldGrid g);
 __global__ void k_project(WorldGrid g);
 __global__ void k_extrapolate(WorldGrid g, int layer);
@@ -59,6 +59,7 @@ __global__ void k_rasterClaims(WorldGrid g);
 __global__ void k_raster(WorldGrid g);
 __global__ void k_unpaint(WorldGrid g);
 __global__ void k_stops(WorldGrid g, uint32_t *out);
+__global__ void k_cycle(uint32_t *cyc, uint32_t set, uint32_t v);
 __global__ void k_reports(WorldGrid g, uint32_t *out);
 
 static thread_local std::string g_err;
@@ -177,7 +178,13 @@ struct Batch {
     std::vector<uint32_t> sig;   // the tile sides the maps were built for
     uint32_t *stops = nullptr, *stops_host = nullptr;   // k_stops' words (device, pinned host)
     uint32_t *reports = nullptr;
+    uint32_t *cyc = nullptr;     // the solve's iteration (device)
     hipStream_t stream = nullptr;
+    // hipGraphs of the step's phases (run_phase),
+    // captured for the launch layout `gsig` and replayed while it holds
+    bool graphs = false;
+    hipGraphExec_t gx[6] = {};
+    std::vector<uint64_t> gsig;
     int timed = -1;              // kernel id timed, -1 none
     uint32_t ntimed = 0;
     hipEvent_t ev[2 * MAX_TIMED];
@@ -189,6 +196,7 @@ static WorldGrid grid_of(const Batch &b, int sh) {
     g.tab = b.tab;
     g.map = b.maps + b.map_off[sh];
     g.first = b.maps + b.first_off[sh];
+    g.cyc = b.cyc;
     return g;
 }
 
@@ -271,6 +279,9 @@ static void batch_free(Batch *b) {
     if (b->stops) (void)hipFree(b->stops);
     if (b->stops_host) (void)hipHostFree(b->stops_host);
     if (b->reports) (void)hipFree(b->reports);
+    if (b->cyc) (void)hipFree(b->cyc);
+    for (hipGraphExec_t &x : b->gx)
+        if (x) (void)hipGraphExecDestroy(x);
     if (b->ev_made)
         for (int i = 0; i < 2 * MAX_TIMED; i++) (void)hipEventDestroy(b->ev[i]);
     delete b;
@@ -283,7 +294,8 @@ static Batch *batch_new(WigWorld *const *worlds, uint32_t n, hipStream_t stream)
     b->stream = stream;
     if (hipMalloc((void **)&b->tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&b->stops, 8ull * n) != hipSuccess
         || hipHostMalloc((void **)&b->stops_host, 8ull * n, hipHostMallocDefault) != hipSuccess
-        || hipMalloc((void **)&b->reports, 4ull * FLUID_STATS * n) != hipSuccess) {
+        || hipMalloc((void **)&b->reports, 4ull * FLUID_STATS * n) != hipSuccess
+        || hipMalloc((void **)&b->cyc, 256) != hipSuccess) {
         fail(WIG_E_NOMEM, "batch allocation failed");
         batch_free(b);
         return nullptr;
@@ -604,12 +616,114 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     return WIG_OK;
 }
 
+// What a step launches, for the batch's worlds and their controls.
+struct StepPlan {
+    bool any_full = false;
+    uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
+    size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
+};
+
+// The step's phases: the head every world runs (the idle path's passes,
+// and the full path's first), the full path up to the iterations, one
+// iteration, and the rest.
+static void enqueue_head(Batch &b) {
+    launch(b, KID_sync, k_sync, SH_WORLD, 64);
+    launch(b, KID_syncParticles, k_syncParticles, SH_PG, 256);
+    launch(b, KID_syncSeed, k_syncSeed, SH_WORLD, 64);
+    launch(b, KID_compact, k_compact, SH_ONE, 1024);
+    launch(b, KID_retire, k_retire, SH_BG, 64);
+}
+static void enqueue_pre(Batch &b) {
+    launch(b, KID_clear, k_clear, SH_BG, 64);
+    launch(b, KID_gather, k_gather, SH_PG, 256);
+    launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 0);
+    launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 0);
+    launch(b, KID_sealAnchor, k_sealAnchor, SH_BG, 64);
+    launch(b, KID_classify, k_classify, SH_BG, 64);
+    launch(b, KID_setFaces, k_setFaces, SH_BG, 64);
+    launch(b, KID_divergence, k_divergence, SH_BG, 64);
+}
+static void enqueue_solve_start(Batch &b) {
+    launch(b, KID_cgStart, k_cgStart, SH_BG, 64);
+    launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 1);
+    launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 1);
+    launch(b, KID_cgAnchor, k_cgAnchor, SH_BG, 64);
+    launch(b, KID_cgResidual, k_cgResidual, SH_BG4, 256);
+    launch(b, KID_sumNorm, k_sumNorm, SH_ONE, 1024);
+    launch(b, KID_cgPrepare, k_cgPrepare, SH_BG, 64);
+    launch(b, KID_levelRho, k_levelRho, SH_LVL1, 64);
+    hipLaunchKernelGGL(k_cycle, dim3(1), dim3(64), 0, b.stream, b.cyc, 1u, 0u);
+}
+static void enqueue_iteration(Batch &b, const StepPlan &pl) {
+    launch(b, KID_relax, k_relax, SH_BG, 64, 0);
+    launch(b, KID_restrict, k_restrict, SH_LVL1, 64);
+    for (uint32_t l = 1; l <= pl.max_level; l++)   // level l's sweeps, and restrictLevel into l + 1
+        launch_lds(b, KID_levelDown, k_levelDown, SH_TILE + (int)l - 1, 256, pl.down_lds[l], l);
+    launch_lds(b, KID_coarseLDS, k_coarseLDS, SH_ONE, 1024, pl.coarse_lds);
+    for (uint32_t l = pl.max_level; l >= 1; l--)
+        launch_lds(b, KID_levelUp, k_levelUp, SH_TILE + (int)l - 1, 256, pl.up_lds[l], l);
+    launch(b, KID_relax, k_relax, SH_BG, 64, 1);
+    launch(b, KID_cgRho, k_cgRho, SH_BG4, 256);
+    launch(b, KID_cgBeta, k_cgBeta, SH_ONE, 1024);
+    launch(b, KID_cgApply, k_cgApply, SH_BG4, 256);
+    launch(b, KID_cgAlpha, k_cgAlpha, SH_ONE, 1024);
+    launch(b, KID_cgUpdate, k_cgUpdate, SH_BG4, 256);
+    launch(b, KID_cgCheck, k_cgCheck, SH_ONE, 1024);
+    hipLaunchKernelGGL(k_cycle, dim3(1), dim3(64), 0, b.stream, b.cyc, 0u, 0u);
+}
+static void enqueue_post(Batch &b) {
+    launch(b, KID_project, k_project, SH_BG, 64);
+    launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 1);
+    launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 2);
+    launch(b, KID_advect, k_advect, SH_PG, 256);
+    launch(b, KID_sortCount, k_sortCount, SH_PG, 256);
+    launch(b, KID_scanBlocks, k_scanBlocks, SH_BG, 64);
+    launch(b, KID_scanTotals, k_scanTotals, SH_ONE, 256);
+    launch(b, KID_scanFinish, k_scanFinish, SH_BG, 64);
+    launch(b, KID_scatter, k_scatter, SH_PG, 256);
+    launch(b, KID_rasterClaims, k_rasterClaims, SH_BG, 64);
+    launch(b, KID_raster, k_raster, SH_BG, 64);
+    launch(b, KID_unpaint, k_unpaint, SH_BG, 64);
+}
+
+// Phase `ph` (0 head, 1 pre, 2 the solve's start, 3 an iteration, 4
+// cgFinish, 5 post) enqueued, or, with graphs on, its graph launched
+// (captured first when the launch layout changed).
+enum { PH_HEAD, PH_PRE, PH_START, PH_ITER, PH_FINISH, PH_POST };
+static int run_phase(Batch &b, int ph, const StepPlan &pl) {
+    auto enqueue = [&]() {
+        switch (ph) {
+        case PH_HEAD: enqueue_head(b); break;
+        case PH_PRE: enqueue_pre(b); break;
+        case PH_START: enqueue_solve_start(b); break;
+        case PH_ITER: enqueue_iteration(b, pl); break;
+        case PH_FINISH: launch(b, KID_cgFinish, k_cgFinish, SH_BG, 64); break;
+        default: enqueue_post(b); break;
+        }
+    };
+    if (!b.graphs || b.timed >= 0) {
+        enqueue();
+        HIPCHECK(hipGetLastError());
+        return WIG_OK;
+    }
+    if (!b.gx[ph]) {
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        HIPCHECK(hipStreamEndCapture(b.stream, &graph));
+        hipError_t e = hipGraphInstantiate(&b.gx[ph], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(WIG_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    HIPCHECK(hipGraphLaunch(b.gx[ph], b.stream));
+    return WIG_OK;
+}
+
 // One liquid step of every world of the batch (encodeFluid, js/sim.js:2466-2612).
 static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
     uint32_t n = (uint32_t)b.w.size();
-    bool any_full = false, timeSolve = b.timed == KID_COUNT && b.ntimed < MAX_TIMED;
-    uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
-    size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
+    bool timeSolve = b.timed == KID_COUNT && b.ntimed < MAX_TIMED;
+    StepPlan pl;
     for (uint32_t i = 0; i < n; i++) {
         int st = step_words(b.w[i], &P[i], &ctl[i], b.htab[i]);
         if (st < 0) return st;
@@ -619,54 +733,48 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
     for (uint32_t i = 0; i < n; i++) {
         const Dev &d = b.htab[i];
         if (!d.full) continue;
-        any_full = true;
-        max_cycles = std::max(max_cycles, d.fluid_cycles);
+        pl.any_full = true;
+        pl.max_cycles = std::max(pl.max_cycles, d.fluid_cycles);
         if (d.first_chunk != 0u) {
             uint32_t f = d.first_chunk == WIG_CHUNK_AUTO ? std::max(b.w[i]->last_hint, 1u) : d.first_chunk;
-            first_check = std::min(first_check, f);
-            chunk = std::min(chunk, d.chunk ? d.chunk : 1u);
+            pl.first_check = std::min(pl.first_check, f);
+            pl.chunk = std::min(pl.chunk, d.chunk ? d.chunk : 1u);
         }
-        coarse_lds = std::max(coarse_lds, b.w[i]->lds_bytes);
+        pl.coarse_lds = std::max(pl.coarse_lds, b.w[i]->lds_bytes);
         for (uint32_t l = 1; l < d.lds_level; l++) {
-            max_level = std::max(max_level, l);
-            down_lds[l] = std::max(down_lds[l], downLds(d.level_pairs, d.tile[l]));
-            up_lds[l] = std::max(up_lds[l], upLds(d.level_pairs, d.tile[l]));
+            pl.max_level = std::max(pl.max_level, l);
+            pl.down_lds[l] = std::max(pl.down_lds[l], downLds(d.level_pairs, d.tile[l]));
+            pl.up_lds[l] = std::max(pl.up_lds[l], upLds(d.level_pairs, d.tile[l]));
+        }
+    }
+    if (b.graphs) {
+        // the graphs hold the launch layout: maps, LDS sizes, levels, stream
+        std::vector<uint64_t> sig(b.sig.begin(), b.sig.end());
+        sig.push_back(pl.coarse_lds);
+        sig.push_back(pl.max_level);
+        for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(pl.down_lds[l]), sig.push_back(pl.up_lds[l]);
+        sig.push_back((uint64_t)(uintptr_t)b.stream);
+        if (sig != b.gsig) {
+            for (hipGraphExec_t &x : b.gx) {
+                if (x) HIPCHECK(hipGraphExecDestroy(x));
+                x = nullptr;
+            }
+            b.gsig = sig;
         }
     }
     HIPCHECK(hipMemcpyAsync(b.tab, b.htab.data(), sizeof(Dev) * n, hipMemcpyHostToDevice, b.stream));
     hipStream_t s = b.stream;
-    launch(b, KID_sync, k_sync, SH_WORLD, 64);
-    launch(b, KID_syncParticles, k_syncParticles, SH_PG, 256);
-    launch(b, KID_syncSeed, k_syncSeed, SH_WORLD, 64);
-    launch(b, KID_compact, k_compact, SH_ONE, 1024);
-    launch(b, KID_retire, k_retire, SH_BG, 64);
-    if (!any_full) {
-        HIPCHECK(hipGetLastError());
-        return WIG_OK;
-    }
-    launch(b, KID_clear, k_clear, SH_BG, 64);
-    launch(b, KID_gather, k_gather, SH_PG, 256);
-    launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 0);
-    launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 0);
-    launch(b, KID_sealAnchor, k_sealAnchor, SH_BG, 64);
-    launch(b, KID_classify, k_classify, SH_BG, 64);
-    launch(b, KID_setFaces, k_setFaces, SH_BG, 64);
-    launch(b, KID_divergence, k_divergence, SH_BG, 64);
+    if ((st = run_phase(b, PH_HEAD, pl)) < 0) return st;
+    if (!pl.any_full) return WIG_OK;
+    if ((st = run_phase(b, PH_PRE, pl)) < 0) return st;
     // (the solve, cgStart .. cgFinish, timed as one: the pseudo-kernel "solve")
     if (timeSolve) HIPCHECK(hipEventRecord(b.ev[2 * b.ntimed], s));
-    launch(b, KID_cgStart, k_cgStart, SH_BG, 64);
-    launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 1);
-    launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 1);
-    launch(b, KID_cgAnchor, k_cgAnchor, SH_BG, 64);
-    launch(b, KID_cgResidual, k_cgResidual, SH_BG4, 256);
-    launch(b, KID_sumNorm, k_sumNorm, SH_ONE, 1024);
-    launch(b, KID_cgPrepare, k_cgPrepare, SH_BG, 64);
-    launch(b, KID_levelRho, k_levelRho, SH_LVL1, 64);
+    if ((st = run_phase(b, PH_START, pl)) < 0) return st;
     // iterations: each world runs the ones its own host would have encoded
     // (encoded() in the kernels); the batch checks at its checkpoints
     // whether every world has stopped
-    uint32_t next_check = first_check;
-    for (uint32_t cycle = 0; cycle < max_cycles; cycle++) {
+    uint32_t next_check = pl.first_check;
+    for (uint32_t cycle = 0; cycle < pl.max_cycles; cycle++) {
         if (cycle == next_check) {
             hipLaunchKernelGGL(k_stops, dim3(n), dim3(64), 0, s, grid_of(b, SH_ONE), b.stops);
             HIPCHECK(hipMemcpyAsync(b.stops_host, b.stops, 8ull * n, hipMemcpyDeviceToHost, s));
@@ -681,41 +789,16 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
                     if (b.htab[i].full && b.stops_host[2 * i] < b.htab[i].fluid_cycles) b.w[i]->last_hint = b.stops_host[2 * i + 1];
                 break;
             }
-            next_check = cycle + chunk;
+            next_check = cycle + pl.chunk;
         }
-        launch(b, KID_relax, k_relax, SH_BG, 64, cycle, 0);
-        launch(b, KID_restrict, k_restrict, SH_LVL1, 64, cycle);
-        for (uint32_t l = 1; l <= max_level; l++)   // level l's sweeps, and restrictLevel into l + 1
-            launch_lds(b, KID_levelDown, k_levelDown, SH_TILE + (int)l - 1, 256, down_lds[l], cycle, l);
-        launch_lds(b, KID_coarseLDS, k_coarseLDS, SH_ONE, 1024, coarse_lds, cycle);
-        for (uint32_t l = max_level; l >= 1; l--)
-            launch_lds(b, KID_levelUp, k_levelUp, SH_TILE + (int)l - 1, 256, up_lds[l], cycle, l);
-        launch(b, KID_relax, k_relax, SH_BG, 64, cycle, 1);
-        launch(b, KID_cgRho, k_cgRho, SH_BG4, 256, cycle);
-        launch(b, KID_cgBeta, k_cgBeta, SH_ONE, 1024, cycle);
-        launch(b, KID_cgApply, k_cgApply, SH_BG4, 256, cycle);
-        launch(b, KID_cgAlpha, k_cgAlpha, SH_ONE, 1024, cycle);
-        launch(b, KID_cgUpdate, k_cgUpdate, SH_BG4, 256, cycle);
-        launch(b, KID_cgCheck, k_cgCheck, SH_ONE, 1024, cycle);
+        if ((st = run_phase(b, PH_ITER, pl)) < 0) return st;
     }
-    launch(b, KID_cgFinish, k_cgFinish, SH_BG, 64);
+    if ((st = run_phase(b, PH_FINISH, pl)) < 0) return st;
     if (timeSolve) {
         HIPCHECK(hipEventRecord(b.ev[2 * b.ntimed + 1], s));
         b.ntimed++;
     }
-    launch(b, KID_project, k_project, SH_BG, 64);
-    launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 1);
-    launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 2);
-    launch(b, KID_advect, k_advect, SH_PG, 256);
-    launch(b, KID_sortCount, k_sortCount, SH_PG, 256);
-    launch(b, KID_scanBlocks, k_scanBlocks, SH_BG, 64);
-    launch(b, KID_scanTotals, k_scanTotals, SH_ONE, 256);
-    launch(b, KID_scanFinish, k_scanFinish, SH_BG, 64);
-    launch(b, KID_scatter, k_scatter, SH_PG, 256);
-    launch(b, KID_rasterClaims, k_rasterClaims, SH_BG, 64);
-    launch(b, KID_raster, k_raster, SH_BG, 64);
-    launch(b, KID_unpaint, k_unpaint, SH_BG, 64);
-    HIPCHECK(hipGetLastError());
+    if ((st = run_phase(b, PH_POST, pl)) < 0) return st;
     return WIG_OK;
 }
 
@@ -762,6 +845,18 @@ int wig_batch_report(WigBatch *b, uint32_t *stats) {
 int wig_batch_sync(WigBatch *b) {
     if (!b) return fail(WIG_E_INVALID, "null batch");
     HIPCHECK(hipStreamSynchronize(((Batch *)b)->stream));
+    return WIG_OK;
+}
+
+int wig_set_graphs(WigWorld *w, int on) {
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    w->solo->graphs = on != 0;
+    return WIG_OK;
+}
+
+int wig_batch_set_graphs(WigBatch *b, int on) {
+    if (!b) return fail(WIG_E_INVALID, "null batch");
+    ((Batch *)b)->graphs = on != 0;
     return WIG_OK;
 }
 

@@ -159,6 +159,7 @@ struct WorldGrid {
     const Dev *__restrict__ tab;
     const uint32_t *__restrict__ map;
     const uint32_t *__restrict__ first;
+    const uint32_t *cyc;     // the solve's iteration, in device memory (so a captured iteration replays)
 };
 #define WORLD(g)                                                      \
     const uint32_t wi_ = (g).map[blockIdx.x];                         \

@@ -1011,8 +1011,9 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
     return j <= high ? I2(wide, j) : I2(j - high - 1, high);
 }
-__global__ void __launch_bounds__(64) k_relax(WorldGrid g, uint32_t cycle, int post) {
+__global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const int shift = ((d.shifted_first ^ (2u * cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
     if (converged(d, cycle)) return;
@@ -1097,8 +1098,9 @@ __global__ void __launch_bounds__(64) k_relax(WorldGrid g, uint32_t cycle, int p
 
 // restrictLevel into level 1 (:2342-2377), each iteration: rhs from the
 // fluid children's residuals; rho as k_levelRho put it at the solve's start
-__global__ void __launch_bounds__(64) k_restrict(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
     if (converged(d, cycle) || activeCount(d) == 0u) return;
@@ -1252,8 +1254,9 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
         __syncthreads();
     }
 }
-__global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t cycle, uint32_t l) {
+__global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t l) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const uint32_t T = d.tile[l];
     const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
@@ -1334,8 +1337,9 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_
         *(float4 *)&FLD(d, at + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
     }
 }
-__global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t cycle, uint32_t l) {
+__global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const uint32_t T = d.tile[l];
     const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
@@ -1468,8 +1472,9 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
     }
     __syncthreads();
 }
-__global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     extern __shared__ float sm[];
@@ -1783,8 +1788,9 @@ __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) {
     }
 }
 // cgRho + cgBeta (:2736-2764): r.z and z.Ap, and Notay's flexible beta in FP64
-__global__ void __launch_bounds__(256) k_cgRho(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     FOR_ACTIVE_BLOCKS4(d, wid) {
@@ -1804,8 +1810,9 @@ __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g, uint32_t cycle) {
         }
     }
 }
-__global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
@@ -1822,8 +1829,9 @@ __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g, uint32_t cycle) {
 }
 // cgDirection + cgApply + cgAlpha (:2767-2814): the iteration's p (into
 // the other p buffer) and q = A p
-__global__ void __launch_bounds__(256) k_cgApply(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     double beta = ld_f64(&d.head->beta);
@@ -1844,8 +1852,9 @@ __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g, uint32_t cycle) {
         if (LANE == 0) d.bpart[4ull * wid + 0] = v;
     }
 }
-__global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
@@ -1862,8 +1871,9 @@ __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g, uint32_t cycle) {
 // cgUpdate (:2852-2880), and whether the next
 // iteration runs: max |r| under the tolerance (the reference's converged(),
 // :2223-2226), or ||r|| <= tol ||b||
-__global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     bool conv = converged(d, cycle);
     if (bid == 0 && threadIdx.x == 0) {
@@ -1896,8 +1906,9 @@ __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g, uint32_t cycle) {
         }
     }
 }
-__global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g, uint32_t cycle) {
+__global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
     WORLD(g);
+    const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
@@ -2417,6 +2428,11 @@ __global__ void __launch_bounds__(64) k_unpaint(WorldGrid g) {                  
 }
 
 // ============================================================ batches
+// The solve's iteration counter: set at the solve's start, stepped after
+// each iteration (a one-thread launch, so captured iterations replay alike)
+__global__ void k_cycle(uint32_t *cyc, uint32_t set, uint32_t v) {
+    if (threadIdx.x == 0) *cyc = set ? v : *cyc + 1u;
+}
 // A convergence checkpoint of a batch: each world's stop (the iteration its
 // solve stops before, NO_STOP while it runs) and the iterations run.
 __global__ void __launch_bounds__(64) k_stops(WorldGrid g, uint32_t *out) {

@@ -54,6 +54,8 @@ def lib():
         L.wig_batch_report.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint32)]
         L.wig_batch_sync.argtypes = [ct.c_void_p]
         L.wig_batch_time_kernel.argtypes = [ct.c_void_p, ct.c_int]
+        L.wig_set_graphs.argtypes = [ct.c_void_p, ct.c_int]
+        L.wig_batch_set_graphs.argtypes = [ct.c_void_p, ct.c_int]
         L.wig_batch_kernel_time.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint32)]
         _lib = L
     return _lib
@@ -157,6 +159,10 @@ class GpuWorld:
         """The coarse-level sweep kernels' tile side on `level` (wig_set_tile)."""
         _check(lib().wig_set_tile(self._w, level, side))
 
+    def set_graphs(self, on):
+        """Steps replayed as hipGraphs (wig_set_graphs)."""
+        _check(lib().wig_set_graphs(self._w, 1 if on else 0))
+
     def clone(self):
         """A new world of the same size and kinds holding this one's parts."""
         w = GpuWorld(self.layout.width, self.layout.height, coarse_levels=self.layout.levels, kinds=self.kinds)
@@ -199,6 +205,9 @@ class GpuBatch:
 
     def set_stream(self, stream_ptr):
         _check(lib().wig_batch_set_stream(self._b, ct.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def set_graphs(self, on):
+        _check(lib().wig_batch_set_graphs(self._b, 1 if on else 0))
 
     def step(self, params, ctls):
         n = len(self.worlds)
