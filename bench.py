@@ -82,9 +82,15 @@ def cpu_baseline(cells, kinds, size, tol, steps):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=("single", "batch"), default="single",
-                    help="single: the 1024x1024 world (the metric's configuration); batch: many worlds in one batch")
+    ap.add_argument("--workload", choices=("single", "batch", "serve"), default="single",
+                    help="single: the 1024x1024 world (the metric's configuration); batch: many worlds in one batch; "
+                         "serve: the serving loop over many sessions")
     ap.add_argument("--worlds", type=int, default=512, help="batch: how many worlds")
+    ap.add_argument("--sessions", type=int, default=2000, help="serve: sessions")
+    ap.add_argument("--rate", type=float, default=20000.0, help="serve: Poisson arrivals a second")
+    ap.add_argument("--duration", type=float, default=0.25, help="serve: seconds of arrivals")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget for the worlds")
+    ap.add_argument("--round-cells", type=int, default=1 << 24, help="serve: most cells a round steps")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
@@ -97,6 +103,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default coarse-level tile sides")
+    ap.add_argument("--graphs", type=int, default=1, help="1: replay the step's phases as hipGraphs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration)")
     args = ap.parse_args()
@@ -113,6 +120,8 @@ def main():
 
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
 
     if args.copy_mib:
         src = torch.empty(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -126,8 +135,9 @@ def main():
     cells, fluid_cells, kinds = build_world(size)
     world = GpuWorld(size, size, kinds=kinds)
     world.upload(abi.PART_CELLS, cells)
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream() if args.graphs else torch.cuda.current_stream()
     world.set_stream(stream.cuda_stream)
+    world.set_graphs(bool(args.graphs))
     sim = config_simulation(world, size, args.tol)
     for _ in range(max(args.warmup, 1)):
         sim.step()
@@ -203,7 +213,7 @@ def main():
                                f"40% of rows at 4 particles/cell, dt 1/120, MG-PCG {1 + world.layout.levels} levels "
                                f"to {args.tol:g} relative residual",
                    "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
-                   "tol": args.tol, "parallelism": "single world, one GPU",
+                   "tol": args.tol, "parallelism": "single world, one GPU", "graphs": bool(args.graphs),
                    "tiles": {str(k): v for k, v in tiles.items()}},
         "solves_per_s": solves, "pcg_iterations_per_solve": d[1] / K,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
@@ -244,9 +254,10 @@ def run_batch(args):
         w.upload(abi.PART_CELLS, cells)
         worlds.append(w)
         sims.append(config_simulation(w, side, args.tol))
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream() if args.graphs else torch.cuda.current_stream()
     batch = GpuBatch(worlds)
     batch.set_stream(stream.cuda_stream)
+    batch.set_graphs(bool(args.graphs))
     for w in worlds:
         w.set_stream(stream.cuda_stream)
     for _ in range(max(args.warmup, 1)):
@@ -325,6 +336,7 @@ def run_batch(args):
         "config": {"workload": f"{len(sides)} worlds, sides {{64,128,256,512}} w.p. {{.4,.3,.2,.1}}, dam blocks over "
                                f"20-50% of cells, 0-15% solid discs, log2(side)-2 levels, {args.tol:g} relative residual",
                    "worlds": {str(s): sides.count(s) for s in sorted(set(sides))}, "cells": int(sum(s * s for s in sides)),
+                   "graphs": bool(args.graphs),
                    "tuned": {str(k): v for k, v in tuned.items()}},
         "sequential": {"ms_per_round": seq_ms_per_round, "solves_per_s": len(sides) / (seq_ms_per_round / 1e3),
                        "ms_per_world_step": {str(k): v for k, v in seq.items()},
@@ -333,6 +345,98 @@ def run_batch(args):
     out["speedup_vs_sequential"] = out["value"] / out["sequential"]["solves_per_s"]
     if per_kernel:
         out["kernel_ms_one_step"] = {k: [round(v[0], 4), v[1]] for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:16]}
+    print(json.dumps(out), flush=True)
+
+
+def run_serve(args):
+    """The serving loop: `sessions` sessions with sides log-uniform on
+    [64, 1024] rounded to a multiple of 32, each a dam block with solid discs
+    (scenes.batch_world; the reference seeds 4 particles a water cell);
+    requests of 1-8 steps of a uniformly chosen session, Poisson arrivals at
+    `rate` for `duration` seconds; continuous batching under an HBM budget of
+    `budget_gb` with LRU eviction of idle worlds to host memory
+    (worldinagrain_amd.serve).  Beside it, the plain single-world loop: the
+    served world-steps timed one world at a time on sample worlds of each
+    size."""
+    import math
+    import numpy as np
+    import torch
+    from worldinagrain_amd import abi, scenes, elements as E
+    from worldinagrain_amd.host import SolverSettings
+    from worldinagrain_amd.serve import Server, Session
+
+    kinds = E.kinds()
+    rng = np.random.default_rng(11)
+    sides = [int(min(1024, max(64, round(math.exp(rng.uniform(math.log(64), math.log(1024))) / 32) * 32)))
+             for _ in range(args.sessions)]
+    scene = {}
+    for side in sorted(set(sides)):
+        scene[side] = scenes.batch_world(side, seed=side)[0]
+    g = float(np.float32(9.81 * (1.0 / 120.0) ** 2))
+    kw = dict(dx=1.0, earth_g=g, dt=1.0 / 120.0)
+    tol = args.tol
+    sessions = [Session(i, s, s, scene[s], lambda: SolverSettings.relative(tol), kinds, sim_kw=kw) for i, s in enumerate(sides)]
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / args.rate)
+        if t > args.duration:
+            break
+        arrivals.append((t, int(rng.integers(args.sessions)), int(rng.integers(1, 9))))
+    stream = torch.cuda.Stream()
+    server = Server(sessions, int(args.budget_gb * (1 << 30)), max_round_cells=args.round_cells, stream=stream.cuda_stream)
+    last = [time.perf_counter()]
+
+    def progress(sv):
+        if time.perf_counter() - last[0] > 20.0:
+            last[0] = time.perf_counter()
+            print(f"[serve] round {sv.round_no}, {sv.evictions} evictions", file=sys.stderr, flush=True)
+    reqs = server.run(arrivals, on_round=progress)
+    lat = np.array([r.done_at - r.arrival for r in reqs]) * 1e3
+    world_steps = sum(s.steps_run for s in sessions)
+    elapsed = server.elapsed
+    rounds = np.array([r[1] for r in server.rounds])
+    round_ms = np.diff([0.0] + [r[0] for r in server.rounds]) * 1e3
+    # the plain single-world loop: the served steps one world at a time
+    from worldinagrain_amd.gpu import GpuWorld
+    from worldinagrain_amd.host import Simulation
+    probe = sorted(set([64, 128, 256, 512, 1024]) & set(sides) or set(sides))
+    solo_ms = {}
+    for side in probe:
+        w = GpuWorld(side, side, kinds=kinds)
+        w.set_stream(stream.cuda_stream)
+        w.upload(abi.PART_CELLS, scene[side])
+        sim = Simulation(w, side, side, solver=SolverSettings.relative(tol), **kw)
+        for _ in range(2):
+            sim.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            sim.step()
+        torch.cuda.synchronize()
+        solo_ms[side] = (time.perf_counter() - t0) / 3 * 1e3
+        w.close()
+    cells = np.array(sorted(solo_ms)) ** 2.0
+    ms = np.array([solo_ms[k] for k in sorted(solo_ms)])
+    seq_s = sum(s.steps_run * float(np.interp(s.width * s.height, cells, ms)) for s in sessions) / 1e3
+    out = {
+        "metric": "request latency p50/p99 and particle-steps/s, served", "workload": "serve",
+        "value": server.particle_steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "p50_request_ms": float(np.percentile(lat, 50)), "p99_request_ms": float(np.percentile(lat, 99)),
+        "requests": len(reqs), "world_steps": world_steps, "world_steps_per_s": world_steps / elapsed,
+        "elapsed_s": elapsed, "rounds": server.round_no, "worlds_per_round_mean": float(rounds.mean()),
+        "p50_round_ms": float(np.percentile(round_ms, 50)), "p99_round_ms": float(np.percentile(round_ms, 99)),
+        "evictions": server.evictions, "admissions": server.admissions,
+        "config": {"workload": f"{args.sessions} sessions, sides log-uniform on [64,1024] (multiples of 32), requests of "
+                               f"1-8 steps, Poisson arrivals at {args.rate:g}/s for {args.duration:g} s, HBM budget "
+                               f"{args.budget_gb:g} GB, LRU eviction to host, {args.tol:g} relative residual, FP32 V-cycle",
+                   "round_cells": args.round_cells},
+        "single_world_loop": {"solo_ms_per_step": {str(k): v for k, v in solo_ms.items()},
+                              "particle_steps_per_s": server.particle_steps / seq_s,
+                              "note": "the same world-steps run one world at a time, from per-size step times"},
+    }
+    out["speedup_vs_single_world_loop"] = out["value"] / out["single_world_loop"]["particle_steps_per_s"]
+    server.close()
     print(json.dumps(out), flush=True)
 
 

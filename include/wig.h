@@ -96,7 +96,7 @@ typedef struct WigStepCtl {
 #define WIG_PART_FIELD 4       /* vec4<f32>: 2 a cell, 1 a face, viscosity words, MG levels */
 #define WIG_PART_BLOCKS 5      /* per block marks, state, lists, FLUID_STATS, totals */
 #define WIG_PART_FLUIDCG 6     /* vec4<f32> per cell (x, r, p, Ap), per block, sums */
-#define WIG_PART_SOLVE 7       /* this library's FP64 solve state (no counterpart in the reference) */
+#define WIG_PART_SOLVE 7       /* this library's FP64 solve state and solver words (no counterpart in the reference) */
 #define WIG_PART_COUNT 8
 
 /* FLUID_STATS words (js/shared.js:271-272). */
@@ -204,6 +204,8 @@ int wig_sync(WigWorld *w);
 typedef struct WigBatch WigBatch;
 WigBatch *wig_batch_create(WigWorld *const *worlds, uint32_t n);
 int wig_batch_destroy(WigBatch *b);
+/* The batch's worlds replaced by these n (a server's next round of worlds). */
+int wig_batch_assign(WigBatch *b, WigWorld *const *worlds, uint32_t n);
 int wig_batch_set_stream(WigBatch *b, void *stream);
 /* One liquid step of each world i, with params[i] and ctl[i] (as wig_fluid_step). */
 int wig_batch_step(WigBatch *b, const WigParams *params, const WigStepCtl *ctl);

@@ -344,7 +344,7 @@ static void layout(OWorld *w, uint32_t width, uint32_t height, uint32_t coarse) 
     w->bytes[WIG_PART_FIELD] = 16ull * w->nfield;
     w->bytes[WIG_PART_BLOCKS] = 4ull * (5ull * w->B + FLUID_STATS);
     w->bytes[WIG_PART_FLUIDCG] = 16ull * (w->C + w->B + 1u);
-    w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * w->B;
+    w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * w->B + 64u;   /* (and the library's 16 solver words) */
 }
 
 /* ------------------------------------------------------------ address */

@@ -156,7 +156,7 @@ struct WigWorld {
     uint32_t *scratch;           // B words
     void *fine;                  // the solve's packed fine level (Dev::coef ... r2)
     unsigned long long *totals;  // WigRunTotals on the device
-    uint32_t *wstate;            // WS_* words on the device
+    uint32_t *wstate;            // WS_* words on the device (the end of the solve part)
     size_t lds_bytes;            // k_coarseLDS's dynamic LDS
     uint32_t tile[MG_LEVELS + 1];   // tile side asked per level (wig_set_tile)
     uint32_t last_hint;          // iterations its last solve ran, as the host saw (when to check first)
@@ -175,7 +175,7 @@ struct Batch {
     size_t maps_words = 0;
     uint64_t map_off[SH_COUNT] = {}, first_off[SH_COUNT] = {};
     uint32_t total[SH_COUNT] = {};
-    std::vector<uint32_t> sig;   // the tile sides the maps were built for
+    std::vector<uint64_t> sig;   // the worlds and tile sides the maps were built for
     uint32_t *stops = nullptr, *stops_host = nullptr;   // k_stops' words (device, pinned host)
     uint32_t *reports = nullptr;
     uint32_t *cyc = nullptr;     // the solve's iteration (device)
@@ -248,7 +248,8 @@ static void size_world(WigWorld *w, uint32_t width, uint32_t height, uint32_t co
     w->bytes[WIG_PART_FIELD] = 16ull * at;
     w->bytes[WIG_PART_BLOCKS] = 4ull * (5ull * d.B + FLUID_STATS);
     w->bytes[WIG_PART_FLUIDCG] = 16ull * (d.C + d.B + 1u);
-    w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * d.B;
+    // (then the world's WS_* words, so a snapshot of the part carries them)
+    w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * d.B + 4ull * WS_WORDS;
 }
 
 
@@ -305,9 +306,11 @@ static Batch *batch_new(WigWorld *const *worlds, uint32_t n, hipStream_t stream)
 
 // The shapes' workgroup maps for the worlds' sizes and tile sides.
 static int batch_maps(Batch &b) {
-    std::vector<uint32_t> sig;
-    for (WigWorld *w : b.w)
+    std::vector<uint64_t> sig;
+    for (WigWorld *w : b.w) {
+        sig.push_back((uint64_t)(uintptr_t)w);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(w->d.tile[l]);
+    }
     if (b.maps && sig == b.sig) return WIG_OK;
     uint32_t n = (uint32_t)b.w.size();
     std::vector<uint32_t> words;
@@ -358,8 +361,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     }
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
         || hipMalloc(&w->fine, 61ull * w->d.C + 256u) != hipSuccess
-        || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess
-        || hipMalloc((void **)&w->wstate, 4u * WS_WORDS) != hipSuccess) {
+        || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
         wig_destroy(w);
         return nullptr;
@@ -369,9 +371,9 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     zero(w->own, w->kind, 256);
     zero(w->own, w->fine, 61ull * w->d.C + 256u);
     zero(w->own, w->totals, sizeof(WigRunTotals));
-    zero(w->own, w->wstate, 4u * WS_WORDS);
     Dev &d = w->d;
     d.totals = w->totals;
+    w->wstate = (uint32_t *)((uint8_t *)w->part[WIG_PART_SOLVE] + sizeof(SolveHead) + 32ull * d.B);
     d.wstate = w->wstate;
     // the coarse levels one workgroup's LDS holds (k_coarseLDS): from the
     // first that fits with all below it
@@ -438,7 +440,6 @@ int wig_destroy(WigWorld *w) {
     if (w->scratch) (void)hipFree(w->scratch);
     if (w->fine) (void)hipFree(w->fine);
     if (w->totals) (void)hipFree(w->totals);
-    if (w->wstate) (void)hipFree(w->wstate);
     if (w->own) (void)hipStreamDestroy(w->own);
     delete w;
     return WIG_OK;
@@ -750,6 +751,8 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
     if (b.graphs) {
         // the graphs hold the launch layout: maps, LDS sizes, levels, stream
         std::vector<uint64_t> sig(b.sig.begin(), b.sig.end());
+        sig.push_back((uint64_t)(uintptr_t)b.maps);
+        sig.push_back((uint64_t)(uintptr_t)b.tab);
         sig.push_back(pl.coarse_lds);
         sig.push_back(pl.max_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(pl.down_lds[l]), sig.push_back(pl.up_lds[l]);
@@ -818,6 +821,34 @@ WigBatch *wig_batch_create(WigWorld *const *worlds, uint32_t n) {
             return nullptr;
         }
     return (WigBatch *)batch_new(worlds, n, worlds[0]->own);
+}
+
+int wig_batch_assign(WigBatch *bb, WigWorld *const *worlds, uint32_t n) {
+    Batch *b = (Batch *)bb;
+    if (!b || !worlds || n == 0u) return fail(WIG_E_INVALID, "a batch needs worlds");
+    for (uint32_t i = 0; i < n; i++)
+        if (!worlds[i] || worlds[i]->device != b->w[0]->device) return fail(WIG_E_INVALID, "world %u is null or on another device", i);
+    if (n > b->htab.size()) {
+        Dev *tab = nullptr;
+        uint32_t *stops = nullptr, *stops_host = nullptr, *reports = nullptr;
+        if (hipMalloc((void **)&tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&stops, 8ull * n) != hipSuccess
+            || hipHostMalloc((void **)&stops_host, 8ull * n, hipHostMallocDefault) != hipSuccess
+            || hipMalloc((void **)&reports, 4ull * FLUID_STATS * n) != hipSuccess)
+            return fail(WIG_E_NOMEM, "batch allocation failed");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        HIPCHECK(hipFree(b->tab));
+        HIPCHECK(hipFree(b->stops));
+        HIPCHECK(hipHostFree(b->stops_host));
+        HIPCHECK(hipFree(b->reports));
+        b->tab = tab;
+        b->stops = stops;
+        b->stops_host = stops_host;
+        b->reports = reports;
+    }
+    b->w.assign(worlds, worlds + n);
+    b->htab.resize(std::max<size_t>(b->htab.size(), n));
+    b->sig.clear();          // the maps are built again at the next step
+    return WIG_OK;
 }
 
 int wig_batch_destroy(WigBatch *b) {

@@ -49,6 +49,7 @@ def lib():
         L.wig_batch_create.restype = ct.c_void_p
         L.wig_batch_create.argtypes = [ct.POINTER(ct.c_void_p), ct.c_uint32]
         L.wig_batch_destroy.argtypes = [ct.c_void_p]
+        L.wig_batch_assign.argtypes = [ct.c_void_p, ct.POINTER(ct.c_void_p), ct.c_uint32]
         L.wig_batch_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
         L.wig_batch_step.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
         L.wig_batch_report.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint32)]
@@ -202,6 +203,12 @@ class GpuBatch:
 
     def __len__(self):
         return len(self.worlds)
+
+    def assign(self, worlds):
+        """The batch's worlds replaced (wig_batch_assign)."""
+        self.worlds = list(worlds)
+        arr = (ct.c_void_p * len(self.worlds))(*[w._w for w in self.worlds])
+        _check(lib().wig_batch_assign(self._b, arr, len(self.worlds)))
 
     def set_stream(self, stream_ptr):
         _check(lib().wig_batch_set_stream(self._b, ct.c_void_p(stream_ptr) if stream_ptr else None))

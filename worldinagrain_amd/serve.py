@@ -1,0 +1,196 @@
+"""The serving loop: step requests from many sessions, each session a world
+kept resident on the device between its requests, served by continuous
+batching.
+
+A request asks for `steps` consecutive liquid steps of one session's world.
+The server runs rounds.  A round takes the requests that have arrived, in
+arrival order, and makes their worlds resident: a world is admitted only if
+it fits the HBM budget, after idle worlds (no request waiting) are evicted
+to host memory, least recently used first; a request whose world cannot be
+admitted waits for a later round.  Then one step of every admitted
+request's world runs in a single batch step (wig_batch_step).  Requests
+that arrive while a round is on the device join the next round, so their
+steps overlap with the others'.  A request is done when its world has run
+its steps; its latency runs from its arrival to the end of that round.
+
+Eviction copies every part of a world (the snapshot parts, the solver's
+words included) to host memory and frees it; admission uploads them into a
+new world.  Each world's results are those of the same world stepped alone
+whatever it was batched with and whenever it was evicted: the batch step is
+exact per world and eviction is a copy."""
+import collections
+import time
+
+import numpy as np
+
+from . import abi
+from .gpu import GpuWorld, GpuBatch
+from .host import Simulation, step_batch
+
+
+def world_bytes(width, height, coarse_levels=0):
+    """HBM a world of this size takes (its parts and the solve's arrays)."""
+    c = width * height
+    b = ((width + 7) // 8) * ((height + 7) // 8)
+    cap = int(np.ceil(c * 4 * 1.25))
+    f = (width + 1) * height + width * (height + 1)
+    field = 16 * (2 * c + f + 2 * ((f + 3) // 4) + 2 * (c // 3 + 64))
+    parts = 24 * c + 2 * (16 + 48 * cap) + 4 * (11 * c + 2 * f) + field + 4 * (5 * b + 9) + 16 * (c + b + 1) + 32 * b + 64
+    return parts + 61 * c + 4 * b + 1024
+
+
+class Session:
+    """A sandbox session: its world's size, its initial cells, its solver
+    settings; its world on the device or its snapshot on the host; the
+    host-side simulation state (frame, seed, report) that goes with it."""
+
+    def __init__(self, sid, width, height, cells, solver_factory, kinds, coarse_levels=0, sim_kw=None):
+        self.sid = sid
+        self.width, self.height = width, height
+        self.cells = cells
+        self.kinds = kinds
+        self.coarse_levels = coarse_levels
+        self.solver_factory = solver_factory
+        self.sim_kw = sim_kw or {}
+        self.world = None
+        self.snapshot = None
+        self.sim = None
+        self.last_used = -1
+        self.bytes = world_bytes(width, height, coarse_levels)
+        self.steps_run = 0
+
+    def make_resident(self, stream=None):
+        w = GpuWorld(self.width, self.height, coarse_levels=self.coarse_levels, kinds=self.kinds)
+        if stream is not None:
+            w.set_stream(stream)
+        if self.snapshot is None:
+            w.upload(abi.PART_CELLS, self.cells)
+            self.sim = Simulation(w, self.width, self.height, solver=self.solver_factory(), **self.sim_kw)
+        else:
+            for p, data in self.snapshot.items():
+                w.upload(p, data)
+            self.snapshot = None
+            self.sim.world = w
+        self.world = w
+
+    def evict(self):
+        self.snapshot = {p: self.world.download(p) for p in range(abi.PART_COUNT)}
+        self.world.close()
+        self.world = None
+
+
+class Request:
+    __slots__ = ("sid", "steps", "arrival", "left", "done_at")
+
+    def __init__(self, sid, steps, arrival):
+        self.sid, self.steps, self.arrival = sid, steps, arrival
+        self.left = steps
+        self.done_at = None
+
+
+class Server:
+    """Continuous batching over sessions under an HBM budget (bytes)."""
+
+    def __init__(self, sessions, budget, max_round_cells=1 << 24, stream=None):
+        self.sessions = {s.sid: s for s in sessions}
+        self.budget = budget
+        self.max_round_cells = max_round_cells
+        self.stream = stream
+        self.resident_bytes = 0
+        self.batch = None
+        self.round_no = 0
+        self.evictions = 0
+        self.admissions = 0
+        self.particle_steps = 0
+        self.rounds = []
+
+    def _admit(self, s, busy):
+        """Makes session s resident within the budget, evicting idle worlds
+        (not in this round and with no request waiting) LRU first."""
+        if s.world is not None:
+            return True
+        if s.bytes > self.budget:
+            raise ValueError(f"session {s.sid}: a {s.width}x{s.height} world does not fit the budget")
+        if self.resident_bytes + s.bytes > self.budget:
+            idle = sorted((x for x in self.sessions.values() if x.world is not None and x.sid not in busy),
+                          key=lambda x: x.last_used)
+            for x in idle:
+                if self.resident_bytes + s.bytes <= self.budget:
+                    break
+                x.evict()
+                self.resident_bytes -= x.bytes
+                self.evictions += 1
+            if self.resident_bytes + s.bytes > self.budget:
+                return False
+        s.make_resident(self.stream)
+        self.resident_bytes += s.bytes
+        self.admissions += 1
+        return True
+
+    def run(self, arrivals, clock=time.perf_counter, on_round=None):
+        """Serves `arrivals` [(time offset in s, session id, steps)], sorted
+        by time, against the wall clock from now.  Returns the requests."""
+        reqs = [Request(sid, steps, t) for t, sid, steps in arrivals]
+        t0 = clock()
+        nxt = 0
+        queue = collections.deque()         # arrived, not done, in arrival order
+        while nxt < len(reqs) or queue:
+            now = clock() - t0
+            while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                queue.append(reqs[nxt])
+                nxt += 1
+            if not queue:
+                time.sleep(max(0.0, min(reqs[nxt].arrival - now, 0.002)))
+                continue
+            # this round: the queue's requests in order, one per session,
+            # while their worlds fit the round's cells and the budget
+            chosen, busy, cells = [], set(), 0
+            waiting = {r.sid for r in queue}
+            for r in queue:
+                if r.sid in busy:
+                    continue
+                s = self.sessions[r.sid]
+                c = s.width * s.height
+                if chosen and cells + c > self.max_round_cells:
+                    continue
+                if not self._admit(s, busy | (waiting if s.world is None else set())):
+                    continue
+                chosen.append(r)
+                busy.add(r.sid)
+                cells += c
+            if not chosen:
+                raise RuntimeError("no waiting request fits the HBM budget")
+            sess = [self.sessions[r.sid] for r in chosen]
+            worlds = [s.world for s in sess]
+            if self.batch is None:
+                self.batch = GpuBatch(worlds)
+                if self.stream is not None:
+                    self.batch.set_stream(self.stream)
+            else:
+                self.batch.assign(worlds)
+            step_batch(self.batch, [s.sim for s in sess])
+            end = clock() - t0
+            self.round_no += 1
+            for s in sess:
+                s.last_used = self.round_no
+                s.steps_run += 1
+                self.particle_steps += int(s.sim.last_stats[abi.STAT_PARTICLES])
+            self.rounds.append((end, len(chosen), cells))
+            for r in chosen:
+                r.left -= 1
+                if r.left == 0:
+                    r.done_at = end
+            queue = collections.deque(r for r in queue if r.left > 0)
+            if on_round:
+                on_round(self)
+        self.elapsed = clock() - t0
+        return reqs
+
+    def close(self):
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
+        for s in self.sessions.values():
+            if s.world is not None:
+                s.world.close()
+                s.world = None
