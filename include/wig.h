@@ -80,8 +80,16 @@ typedef struct WigStepCtl {
                                 and encode chunk more at a time until it converges or fluid_cycles run */
     uint32_t chunk;
     uint32_t level_pairs;    /* red-black sweep pairs on each coarse level, down and up (0: 1, the reference's) */
-    uint32_t reserved;
+    uint32_t vcycle_storage; /* WIG_STORE_*: what the V-cycle (the preconditioner) keeps its values in */
 } WigStepCtl;
+
+/* V-cycle storage: float in the reference's buffers (results bit-identical
+ * to the reference's arithmetic), or 16 bits a value (IEEE half or
+ * bfloat16, arithmetic in float) in buffers of this library's own; the
+ * outer PCG stays float with FP64 sums either way. */
+#define WIG_STORE_F32 0
+#define WIG_STORE_F16 1
+#define WIG_STORE_BF16 2
 
 /* first_chunk: as many iterations as the world's last solve ran */
 #define WIG_CHUNK_AUTO 0xffffffffu
@@ -116,7 +124,8 @@ typedef struct WigSolveStats {
     double bb;                    /* ||b||^2 of the step's right-hand side */
     uint32_t cycles;              /* iterations run */
     uint32_t converged;
-    uint32_t pad[2];
+    float vcycle_scale;           /* 16-bit V-cycle storage: the last power of two its rhs was scaled by */
+    uint32_t pad;
 } WigSolveStats;
 
 /* A world of width x height cells (the reference's Simulation buffers,

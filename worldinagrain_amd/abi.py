@@ -10,6 +10,7 @@ WIG_E_NOT_BUILT = -4
 
 WIG_TOL_ABS_MAX = 0
 WIG_TOL_REL_L2 = 1
+WIG_STORE_F32, WIG_STORE_F16, WIG_STORE_BF16 = 0, 1, 2
 WIG_MAX_CYCLES = 256
 
 PART_CELLS, PART_PARTICLES0, PART_PARTICLES1, PART_COUNTERS, PART_FIELD, PART_BLOCKS, PART_FLUIDCG, PART_SOLVE = range(8)
@@ -38,7 +39,7 @@ class WigStepCtl(ct.Structure):
         ("fluid_cycles", ct.c_uint32), ("shifted_first", ct.c_uint32), ("tol_mode", ct.c_uint32), ("tol", ct.c_float),
         ("pre_relax", ct.c_uint32), ("post_relax", ct.c_uint32), ("sweep_pairs", ct.c_uint32),
         ("coarse_sweeps", ct.c_uint32), ("first_chunk", ct.c_uint32), ("chunk", ct.c_uint32),
-        ("level_pairs", ct.c_uint32), ("reserved", ct.c_uint32),
+        ("level_pairs", ct.c_uint32), ("vcycle_storage", ct.c_uint32),
     ]
 
 
@@ -54,7 +55,7 @@ class WigSolveStats(ct.Structure):
     _fields_ = [
         ("rz", ct.c_double), ("alpha", ct.c_double), ("beta", ct.c_double), ("pq", ct.c_double),
         ("rr", ct.c_double), ("bb", ct.c_double), ("cycles", ct.c_uint32), ("converged", ct.c_uint32),
-        ("pad", ct.c_uint32 * 2),
+        ("vcycle_scale", ct.c_float), ("pad", ct.c_uint32),
     ]
 
 

@@ -29,24 +29,24 @@ __global__ void k_sealAnchor(WorldGrid g);
 __global__ void k_classify(WorldGrid g);
 __global__ void k_setFaces(WorldGrid g);
 __global__ void k_divergence(WorldGrid g);
-__global__ void k_relax(WorldGrid g, int post);
-__global__ void k_restrict(WorldGrid g);
-__global__ void k_levelRho(WorldGrid g);
-__global__ void k_levelDown(WorldGrid g, uint32_t l);
-__global__ void k_coarseLDS(WorldGrid g);
-__global__ void k_levelUp(WorldGrid g, uint32_t l);
+template <int S> __global__ void k_relax(WorldGrid g, int post);
+template <int S> __global__ void k_restrict(WorldGrid g);
+template <int S> __global__ void k_levelRho(WorldGrid g);
+template <int S> __global__ void k_levelDown(WorldGrid g, uint32_t l);
+template <int S> __global__ void k_coarseLDS(WorldGrid g);
+template <int S> __global__ void k_levelUp(WorldGrid g, uint32_t l);
 __global__ void k_cgStart(WorldGrid g);
 __global__ void k_cgAnchor(WorldGrid g);
 __global__ void k_cgResidual(WorldGrid g);
 __global__ void k_sumNorm(WorldGrid g);
-__global__ void k_cgPrepare(WorldGrid g);
-__global__ void k_cgRho(WorldGrid g);
+template <int S> __global__ void k_cgPrepare(WorldGrid g);
+template <int S> __global__ void k_cgRho(WorldGrid g);
 __global__ void k_cgBeta(WorldGrid g);
-__global__ void k_cgApply(WorldGrid g);
+template <int S> __global__ void k_cgApply(WorldGrid g);
 __global__ void k_cgAlpha(WorldGrid g);
-__global__ void k_cgUpdate(WorldGrid g);
+template <int S> __global__ void k_cgUpdate(WorldGrid g);
 __global__ void k_cgCheck(WorldGrid g);
-__global__ void k_cgFinish(WorldGrid g);
+template <int S> __global__ void k_cgFinish(WorldGrid g);
 __global__ void k_project(WorldGrid g);
 __global__ void k_extrapolate(WorldGrid g, int layer);
 __global__ void k_advect(WorldGrid g);
@@ -154,7 +154,9 @@ struct WigWorld {
     void *part[WIG_PART_COUNT];
     uint8_t *kind;               // 256 element kinds
     uint32_t *scratch;           // B words
-    void *fine;                  // the solve's packed fine level (Dev::coef ... r2)
+    void *fine;                  // the solve's packed fine level (Dev::coef ... cflag)
+    void *vlevels;               // the 16-bit V-cycle's coarse levels
+    uint64_t vlevel_bytes;
     unsigned long long *totals;  // WigRunTotals on the device
     uint32_t *wstate;            // WS_* words on the device (the end of the solve part)
     size_t lds_bytes;            // k_coarseLDS's dynamic LDS
@@ -347,6 +349,8 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     }
     WigWorld *w = new WigWorld();
     size_world(w, width, height, coarse_levels);
+    w->vlevel_bytes = 64u;
+    for (uint32_t l = 1; l <= w->d.L; l++) w->vlevel_bytes += 12ull * w->d.lw[l] * w->d.lh[l] + 64u;
     if (hipGetDevice(&w->device) != hipSuccess || hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking) != hipSuccess) {
         fail(WIG_E_DEVICE, "no HIP device or stream");
         delete w;
@@ -360,7 +364,8 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         }
     }
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
-        || hipMalloc(&w->fine, 61ull * w->d.C + 256u) != hipSuccess
+        || hipMalloc(&w->fine, 73ull * w->d.C + 256u) != hipSuccess
+        || hipMalloc(&w->vlevels, w->vlevel_bytes) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
         wig_destroy(w);
@@ -369,7 +374,8 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     for (int p = 0; p < WIG_PART_COUNT; p++)
         if (zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
     zero(w->own, w->kind, 256);
-    zero(w->own, w->fine, 61ull * w->d.C + 256u);
+    zero(w->own, w->fine, 73ull * w->d.C + 256u);
+    zero(w->own, w->vlevels, w->vlevel_bytes);
     zero(w->own, w->totals, sizeof(WigRunTotals));
     Dev &d = w->d;
     d.totals = w->totals;
@@ -388,7 +394,9 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         w->lds_bytes = bytes;
     }
     if (w->lds_bytes == 0) w->lds_bytes = 4ull * LDS_LEVEL_FLOATS * d.lw[d.L] * d.lh[d.L];
-    if (hipFuncSetAttribute((const void *)k_coarseLDS, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
+    if (hipFuncSetAttribute((const void *)k_coarseLDS<STORE_F32>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
+        || hipFuncSetAttribute((const void *)k_coarseLDS<STORE_F16>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
+        || hipFuncSetAttribute((const void *)k_coarseLDS<STORE_BF16>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
         || w->lds_bytes > (size_t)lds_max) {
         fail(WIG_E_DEVICE, "k_coarseLDS cannot take %zu bytes of LDS", w->lds_bytes);
         wig_destroy(w);
@@ -418,7 +426,18 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         d.cp0 = (float *)(f + 48ull * d.C);
         d.cp1 = (float *)(f + 52ull * d.C);
         d.cq = (float *)(f + 56ull * d.C);
-        d.cflag = f + 60ull * d.C;
+        d.ha = (uint32_t *)(f + 60ull * d.C);
+        d.hb = (uint32_t *)(f + 64ull * d.C);
+        d.hr2 = (uint32_t *)(f + 68ull * d.C);
+        d.cflag = f + 72ull * d.C;
+        // the 16-bit V-cycle's coarse levels: four halves and a float a cell
+        uint8_t *v = (uint8_t *)w->vlevels;
+        for (uint32_t l = 1; l <= d.L; l++) {
+            uint64_t n = (uint64_t)d.lw[l] * d.lh[l];
+            d.vlev[l] = (uint16_t *)v;
+            d.vrho[l] = (float *)(v + 8ull * n);
+            v += 12ull * n + 64u;
+        }
     }
     WigWorld *self = w;
     w->solo = batch_new(&self, 1u, w->own);
@@ -439,6 +458,7 @@ int wig_destroy(WigWorld *w) {
     if (w->kind) (void)hipFree(w->kind);
     if (w->scratch) (void)hipFree(w->scratch);
     if (w->fine) (void)hipFree(w->fine);
+    if (w->vlevels) (void)hipFree(w->vlevels);
     if (w->totals) (void)hipFree(w->totals);
     if (w->own) (void)hipStreamDestroy(w->own);
     delete w;
@@ -594,7 +614,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     if (ctl->viscous || ctl->surface || ctl->bodies_run)
         return fail(WIG_E_INVALID, "viscosity, surface tension and bodies are not built (water only)");
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u
-        || ctl->level_pairs > MAX_LEVEL_PAIRS)
+        || ctl->level_pairs > MAX_LEVEL_PAIRS || ctl->vcycle_storage > WIG_STORE_BF16)
         return fail(WIG_E_INVALID, "solver control out of range");
     if (ctl->pre_relax != 1u || ctl->post_relax != 1u)
         return fail(WIG_E_INVALID, "one relaxation before and one after the coarse levels (PRE_RELAX = POST_RELAX = 1) is what is built");
@@ -611,6 +631,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d.tol_mode = ctl->tol_mode;
     d.tol = ctl->tol;
     d.level_pairs = ctl->level_pairs ? ctl->level_pairs : 1u;
+    d.store = ctl->vcycle_storage;
     for (uint32_t l = 0; l <= MG_LEVELS; l++) d.tile[l] = fitTile(w->tile[l], d.level_pairs);
     w->d.level_pairs = d.level_pairs;
     for (uint32_t l = 0; l <= MG_LEVELS; l++) w->d.tile[l] = d.tile[l];
@@ -620,6 +641,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
 // What a step launches, for the batch's worlds and their controls.
 struct StepPlan {
     bool any_full = false;
+    uint32_t stores = 0;     // the V-cycle storages (1 << STORE_*) of the full worlds
     uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
     size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
 };
@@ -644,31 +666,52 @@ static void enqueue_pre(Batch &b) {
     launch(b, KID_setFaces, k_setFaces, SH_BG, 64);
     launch(b, KID_divergence, k_divergence, SH_BG, 64);
 }
-static void enqueue_solve_start(Batch &b) {
+extern "C++" {
+template <int S> static void enqueue_start_store(Batch &b) {
+    launch(b, KID_cgPrepare, k_cgPrepare<S>, SH_BG, 64);
+    launch(b, KID_levelRho, k_levelRho<S>, SH_LVL1, 64);
+}
+template <int S> static void enqueue_iteration_store(Batch &b, const StepPlan &pl, int part) {
+    if (part == 0) {
+        launch(b, KID_relax, k_relax<S>, SH_BG, 64, 0);
+        launch(b, KID_restrict, k_restrict<S>, SH_LVL1, 64);
+        for (uint32_t l = 1; l <= pl.max_level; l++)   // level l's sweeps, and restrictLevel into l + 1
+            launch_lds(b, KID_levelDown, k_levelDown<S>, SH_TILE + (int)l - 1, 256, pl.down_lds[l], l);
+        launch_lds(b, KID_coarseLDS, k_coarseLDS<S>, SH_ONE, 1024, pl.coarse_lds);
+        for (uint32_t l = pl.max_level; l >= 1; l--)
+            launch_lds(b, KID_levelUp, k_levelUp<S>, SH_TILE + (int)l - 1, 256, pl.up_lds[l], l);
+        launch(b, KID_relax, k_relax<S>, SH_BG, 64, 1);
+        launch(b, KID_cgRho, k_cgRho<S>, SH_BG4, 256);
+    } else if (part == 1) {
+        launch(b, KID_cgApply, k_cgApply<S>, SH_BG4, 256);
+    } else {
+        launch(b, KID_cgUpdate, k_cgUpdate<S>, SH_BG4, 256);
+    }
+}
+}  // extern "C++"
+static void enqueue_solve_start(Batch &b, const StepPlan &pl) {
     launch(b, KID_cgStart, k_cgStart, SH_BG, 64);
     launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 1);
     launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 1);
     launch(b, KID_cgAnchor, k_cgAnchor, SH_BG, 64);
     launch(b, KID_cgResidual, k_cgResidual, SH_BG4, 256);
     launch(b, KID_sumNorm, k_sumNorm, SH_ONE, 1024);
-    launch(b, KID_cgPrepare, k_cgPrepare, SH_BG, 64);
-    launch(b, KID_levelRho, k_levelRho, SH_LVL1, 64);
+    if (pl.stores & 1u) enqueue_start_store<STORE_F32>(b);
+    if (pl.stores & 2u) enqueue_start_store<STORE_F16>(b);
+    if (pl.stores & 4u) enqueue_start_store<STORE_BF16>(b);
     hipLaunchKernelGGL(k_cycle, dim3(1), dim3(64), 0, b.stream, b.cyc, 1u, 0u);
 }
 static void enqueue_iteration(Batch &b, const StepPlan &pl) {
-    launch(b, KID_relax, k_relax, SH_BG, 64, 0);
-    launch(b, KID_restrict, k_restrict, SH_LVL1, 64);
-    for (uint32_t l = 1; l <= pl.max_level; l++)   // level l's sweeps, and restrictLevel into l + 1
-        launch_lds(b, KID_levelDown, k_levelDown, SH_TILE + (int)l - 1, 256, pl.down_lds[l], l);
-    launch_lds(b, KID_coarseLDS, k_coarseLDS, SH_ONE, 1024, pl.coarse_lds);
-    for (uint32_t l = pl.max_level; l >= 1; l--)
-        launch_lds(b, KID_levelUp, k_levelUp, SH_TILE + (int)l - 1, 256, pl.up_lds[l], l);
-    launch(b, KID_relax, k_relax, SH_BG, 64, 1);
-    launch(b, KID_cgRho, k_cgRho, SH_BG4, 256);
+    auto stores = [&](int part) {
+        if (pl.stores & 1u) enqueue_iteration_store<STORE_F32>(b, pl, part);
+        if (pl.stores & 2u) enqueue_iteration_store<STORE_F16>(b, pl, part);
+        if (pl.stores & 4u) enqueue_iteration_store<STORE_BF16>(b, pl, part);
+    };
+    stores(0);
     launch(b, KID_cgBeta, k_cgBeta, SH_ONE, 1024);
-    launch(b, KID_cgApply, k_cgApply, SH_BG4, 256);
+    stores(1);
     launch(b, KID_cgAlpha, k_cgAlpha, SH_ONE, 1024);
-    launch(b, KID_cgUpdate, k_cgUpdate, SH_BG4, 256);
+    stores(2);
     launch(b, KID_cgCheck, k_cgCheck, SH_ONE, 1024);
     hipLaunchKernelGGL(k_cycle, dim3(1), dim3(64), 0, b.stream, b.cyc, 0u, 0u);
 }
@@ -696,9 +739,13 @@ static int run_phase(Batch &b, int ph, const StepPlan &pl) {
         switch (ph) {
         case PH_HEAD: enqueue_head(b); break;
         case PH_PRE: enqueue_pre(b); break;
-        case PH_START: enqueue_solve_start(b); break;
+        case PH_START: enqueue_solve_start(b, pl); break;
         case PH_ITER: enqueue_iteration(b, pl); break;
-        case PH_FINISH: launch(b, KID_cgFinish, k_cgFinish, SH_BG, 64); break;
+        case PH_FINISH:
+            if (pl.stores & 1u) launch(b, KID_cgFinish, k_cgFinish<STORE_F32>, SH_BG, 64);
+            if (pl.stores & 2u) launch(b, KID_cgFinish, k_cgFinish<STORE_F16>, SH_BG, 64);
+            if (pl.stores & 4u) launch(b, KID_cgFinish, k_cgFinish<STORE_BF16>, SH_BG, 64);
+            break;
         default: enqueue_post(b); break;
         }
     };
@@ -735,6 +782,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         const Dev &d = b.htab[i];
         if (!d.full) continue;
         pl.any_full = true;
+        pl.stores |= 1u << d.store;
         pl.max_cycles = std::max(pl.max_cycles, d.fluid_cycles);
         if (d.first_chunk != 0u) {
             uint32_t f = d.first_chunk == WIG_CHUNK_AUTO ? std::max(b.w[i]->last_hint, 1u) : d.first_chunk;
@@ -754,6 +802,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         sig.push_back((uint64_t)(uintptr_t)b.maps);
         sig.push_back((uint64_t)(uintptr_t)b.tab);
         sig.push_back(pl.coarse_lds);
+        sig.push_back(pl.stores);
         sig.push_back(pl.max_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(pl.down_lds[l]), sig.push_back(pl.up_lds[l]);
         sig.push_back((uint64_t)(uintptr_t)b.stream);

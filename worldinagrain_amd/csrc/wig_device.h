@@ -96,7 +96,9 @@
 // (two running sums of a pass and the squared residual).
 struct SolveHead {
     double rz, alpha, beta, pq, rr, bb;
-    uint32_t cycles, converged, pad[2];
+    uint32_t cycles, converged;
+    float vscale;       // 16-bit V-cycle storage: the power of two its pressure rhs is scaled by
+    uint32_t pad;
 };
 
 // A world's own device words that outlive a step: how many iterations its
@@ -140,6 +142,14 @@ struct Dev {
     // and the CG vectors (fluidCg's x, r, p, Ap), p in two buffers: an
     // iteration's cgApply reads the last p and writes the new one
     float *cx, *cr, *cq, *cp0, *cp1;
+    // The V-cycle's storage (WIG_STORE_*): with 16-bit storage its fine
+    // iterate and rhs (two halves a cell, as pa, pb, r2) and its coarse
+    // levels (p, rhs as four halves a cell, rho as a float) live here
+    // instead of the float arrays and the field's level records.
+    uint32_t store;
+    uint32_t *ha, *hb, *hr2;
+    uint16_t *vlev[MG_LEVELS + 1];
+    float *vrho[MG_LEVELS + 1];
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
     uint32_t tile[MG_LEVELS + 1];   // k_levelDown/k_levelUp's tile side, per level below lds_level
     // the step's: the Params words the liquid reads, and its controls
@@ -169,6 +179,26 @@ struct WorldGrid {
     (void)nbid
 
 struct i2 { int32_t x, y; };
+
+// 16-bit storage of V-cycle values: IEEE half (round to nearest even, the
+// hardware conversion) or bfloat16 (the float's top half, rounded to nearest
+// even); arithmetic stays float.
+#define STORE_F32 0
+#define STORE_F16 1
+#define STORE_BF16 2
+template <int S> __device__ __forceinline__ uint16_t enc16(float v) {
+    if (S == STORE_F16) {
+        _Float16 h = (_Float16)v;
+        return __builtin_bit_cast(uint16_t, h);
+    }
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // NaN stays NaN
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+template <int S> __device__ __forceinline__ float dec16(uint16_t v) {
+    if (S == STORE_F16) return (float)__builtin_bit_cast(_Float16, v);
+    return __uint_as_float((uint32_t)v << 16);
+}
 struct v2 { float x, y; };
 __device__ __forceinline__ i2 I2(int32_t x, int32_t y) { i2 r; r.x = x; r.y = y; return r; }
 __device__ __forceinline__ v2 V2(float x, float y) { v2 r; r.x = x; r.y = y; return r; }

@@ -23,6 +23,15 @@ __device__ __forceinline__ uint32_t ld(const uint32_t *p) {
 // Whether the world's host would have encoded iteration `cycle`: within its
 // fluid_cycles, and before the convergence checkpoint its solve stops at (a
 // batch runs as many iterations as its slowest world; the others' skip).
+// The 16-bit V-cycle's rhs scale: the power of two that takes a residual
+// whose largest |r| has these float bits into [0.5, 1) (the PCG's iterates
+// do not depend on the preconditioner's scale; 16 bits do not reach the
+// residuals of a converging solve unscaled).  1 for float storage.
+__device__ __forceinline__ float vcycleScale(const Dev &d, uint32_t most) {
+    uint32_t e = (most >> 23) & 0xffu;
+    if (d.store == 0u || most == 0u || e == 0xffu) return 1.0f;
+    return __uint_as_float((253u - e) << 23);
+}
 __device__ __forceinline__ bool encoded(const Dev &d, uint32_t cycle) {
     return d.full && cycle < d.fluid_cycles && cycle < ld(&d.wstate[WS_STOP]);
 }
@@ -951,6 +960,58 @@ __device__ __forceinline__ v2 rhsAt(const Dev &d, uint32_t l, i2 c) {
     uint64_t at = levelSlot(d, l, c);
     return V2(FLD(d, at, 2), FLD(d, at, 3));
 }
+// The V-cycle's storage by S (STORE_*): float in the reference's layouts
+// (pa, pb, r2; the field's level records), or two bytes a value in the
+// arrays beside them.  Level l >= 1's cell c holds p (px, py), the rhs
+// (rx, ry) and rho; the fine level's buffers (0 pa, 1 pb, 2 r2) two values.
+__device__ __forceinline__ uint64_t lvIndex(const Dev &d, uint32_t l, i2 c) { return (uint64_t)c.y * d.lw[l] + (uint64_t)c.x; }
+template <int S> __device__ __forceinline__ float4 lvLoad(const Dev &d, uint32_t l, i2 c) {
+    if (S == STORE_F32) return *(const float4 *)&FLD(d, levelSlot(d, l, c), 0);
+    uint2 u = *(const uint2 *)(d.vlev[l] + 4ull * lvIndex(d, l, c));
+    return make_float4(dec16<S>((uint16_t)u.x), dec16<S>((uint16_t)(u.x >> 16)), dec16<S>((uint16_t)u.y), dec16<S>((uint16_t)(u.y >> 16)));
+}
+template <int S> __device__ __forceinline__ float2 lvLoadP(const Dev &d, uint32_t l, i2 c) {
+    if (S == STORE_F32) return *(const float2 *)&FLD(d, levelSlot(d, l, c), 0);
+    uint32_t u = *(const uint32_t *)(d.vlev[l] + 4ull * lvIndex(d, l, c));
+    return make_float2(dec16<S>((uint16_t)u), dec16<S>((uint16_t)(u >> 16)));
+}
+template <int S> __device__ __forceinline__ float lvRho(const Dev &d, uint32_t l, i2 c) {
+    if (S == STORE_F32) return FLD(d, levelSlot(d, l, c) + 1u, 0);
+    return d.vrho[l][lvIndex(d, l, c)];
+}
+template <int S> __device__ __forceinline__ void lvStore(const Dev &d, uint32_t l, i2 c, float4 v) {
+    if (S == STORE_F32) {
+        *(float4 *)&FLD(d, levelSlot(d, l, c), 0) = v;
+        return;
+    }
+    uint2 u = make_uint2((uint32_t)enc16<S>(v.x) | ((uint32_t)enc16<S>(v.y) << 16), (uint32_t)enc16<S>(v.z) | ((uint32_t)enc16<S>(v.w) << 16));
+    *(uint2 *)(d.vlev[l] + 4ull * lvIndex(d, l, c)) = u;
+}
+template <int S> __device__ __forceinline__ void lvStoreP(const Dev &d, uint32_t l, i2 c, float2 p) {
+    if (S == STORE_F32) {
+        *(float2 *)&FLD(d, levelSlot(d, l, c), 0) = p;
+        return;
+    }
+    *(uint32_t *)(d.vlev[l] + 4ull * lvIndex(d, l, c)) = (uint32_t)enc16<S>(p.x) | ((uint32_t)enc16<S>(p.y) << 16);
+}
+template <int S> __device__ __forceinline__ void lvStoreRho(const Dev &d, uint32_t l, i2 c, float rho) {
+    if (S == STORE_F32) *(float4 *)&FLD(d, levelSlot(d, l, c) + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
+    else d.vrho[l][lvIndex(d, l, c)] = rho;
+}
+template <int S> __device__ __forceinline__ float2 fnLoad(const Dev &d, int buf, uint32_t i) {
+    if (S == STORE_F32) return (buf == 0 ? d.pa : buf == 1 ? d.pb : d.r2)[i];
+    uint32_t u = (buf == 0 ? d.ha : buf == 1 ? d.hb : d.hr2)[i];
+    return make_float2(dec16<S>((uint16_t)u), dec16<S>((uint16_t)(u >> 16)));
+}
+template <int S> __device__ __forceinline__ void fnStore(const Dev &d, int buf, uint32_t i, float2 v) {
+    if (S == STORE_F32) (buf == 0 ? d.pa : buf == 1 ? d.pb : d.r2)[i] = v;
+    else (buf == 0 ? d.ha : buf == 1 ? d.hb : d.hr2)[i] = (uint32_t)enc16<S>(v.x) | ((uint32_t)enc16<S>(v.y) << 16);
+}
+template <int S> __device__ __forceinline__ void fnStoreX(const Dev &d, int buf, uint32_t i, float x) {
+    if (S == STORE_F32) (buf == 0 ? d.pa : buf == 1 ? d.pb : d.r2)[i].x = x;
+    else ((uint16_t *)(buf == 0 ? d.ha : buf == 1 ? d.hb : d.hr2))[2ull * i] = enc16<S>(x);
+}
+
 struct Stencil { v2 sum; float diag; };
 __device__ Stencil stencil(const Dev &d, uint32_t l, i2 c, float rho) {      // :2286-2302
     Stencil s;
@@ -982,18 +1043,18 @@ __device__ v2 residualAt(const Dev &d, uint32_t l, i2 c) {                  // :
 // The fine level's residual at cell i (residualAt, :2306-2315) from the
 // packed solve arrays: p as the pre-relaxation left it (pb), the betas
 // (0 where the stencil skips the neighbour), the pin, the rhs.
-__device__ __forceinline__ v2 fineResidual(const Dev &d, i2 c, uint32_t i, uint32_t f) {
+template <int S> __device__ __forceinline__ v2 fineResidual(const Dev &d, i2 c, uint32_t i, uint32_t f) {
     float4 cf = d.coef[i];
     const float b[4] = {fabsf(cf.x), fabsf(cf.y), fabsf(cf.z), fabsf(cf.w)};
     float sx = 0.0f, sy = 0.0f, diag = 0.0f;
     for (int k = 0; k < 4; k++) {
         if (b[k] == 0.0f) continue;
-        float2 q = d.pb[cellIndex(d, beside(c, k))];
+        float2 q = fnLoad<S>(d, 1, cellIndex(d, beside(c, k)));
         sx += b[k] * q.x;
         sy += b[k] * q.y;
         diag += b[k];
     }
-    float2 p = d.pb[i], r = d.r2[i];
+    float2 p = fnLoad<S>(d, 1, i), r = fnLoad<S>(d, 2, i);
     float pin = (f & CF_PINNED) ? PINF : 0.0f;
     return V2(r.x - (sx - (diag + pin) * p.x), r.y - (sy - (diag + 0.0f) * p.y));
 }
@@ -1003,7 +1064,7 @@ __device__ __forceinline__ v2 fineResidual(const Dev &d, i2 c, uint32_t i, uint3
 // The pre-relaxation reads pa and writes pb, the post-relaxation reads pb
 // plus prolongFine's correction (:2500-2512) and writes pa: the buffer read
 // is the rim snapshot, and every fluid cell of the tiles is written.
-__device__ v2 prolonged(const Dev &d, uint32_t l, i2 c);
+template <int S> __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c);
 __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
     if (wid < activeCount(d)) return activeBlock(d, wid);
     int32_t wide = (int32_t)d.bw, high = (int32_t)(d.B / d.bw);
@@ -1011,15 +1072,14 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
     return j <= high ? I2(wide, j) : I2(j - high - 1, high);
 }
-__global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
+template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const int shift = ((d.shifted_first ^ (2u * cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
     if (converged(d, cycle)) return;
     __shared__ float sPx[RIM * RIM], sPy[RIM * RIM];
-    const float2 *pin = post ? d.pb : d.pa;
-    float2 *pout = post ? d.pa : d.pb;
     uint32_t li = threadIdx.x;
     uint32_t na = activeCount(d), edge = shift ? d.bw + d.bh + 1u : 0u;
     // the tiles: the active list's blocks, then (shifted) the edge tiles,
@@ -1044,9 +1104,9 @@ __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
             float2 v = make_float2(0.0f, 0.0f);
             if (inWorld(d, c)) {
                 uint32_t ci = cellIndex(d, c);
-                v = pin[ci];
+                v = fnLoad<S>(d, post ? 1 : 0, ci);
                 if (post && (d.cflag[ci] & CF_FLUID)) {
-                    v2 e = prolonged(d, 0u, c);
+                    v2 e = prolonged<S>(d, 0u, c);
                     v.x = v.x + e.x;
                     v.y = v.y + e.y;
                 }
@@ -1066,7 +1126,7 @@ __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
             fluid = (f & CF_FLUID) != 0u;
             if (fluid) {
                 float4 cf = d.coef[mi];
-                float2 r = d.r2[mi];
+                float2 r = fnLoad<S>(d, 2, mi);
                 b0 = fabsf(cf.x);
                 b1 = fabsf(cf.y);
                 b2 = fabsf(cf.z);
@@ -1091,23 +1151,23 @@ __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
             }
             __syncthreads();
         }
-        if (fluid) pout[mi] = make_float2(sPx[t], sPy[t]);
+        if (fluid) fnStore<S>(d, post ? 0 : 1, mi, make_float2(sPx[t], sPy[t]));
         __syncthreads();
     }
 }
 
 // restrictLevel into level 1 (:2342-2377), each iteration: rhs from the
 // fluid children's residuals; rho as k_levelRho put it at the solve's start
-__global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
     if (converged(d, cycle) || activeCount(d) == 0u) return;
     i2 c = I2((int32_t)(bx * 8u + threadIdx.x % 8u), (int32_t)(by * 8u + threadIdx.x / 8u));
     if (!inLevel(d, 1u, c)) return;
-    uint64_t at = levelSlot(d, 1u, c);
-    float rho = FLD(d, at + 1u, 0);
+    float rho = lvRho<S>(d, 1u, c);
     float sx = 0.0f, sy = 0.0f;
     if (rho > 0.0f) {
         for (int q = 0; q < 4; q++) {
@@ -1115,17 +1175,18 @@ __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
             if (!inWorld(d, k)) continue;
             uint32_t ki = cellIndex(d, k), f = d.cflag[ki];
             if (!(f & CF_FLUID)) continue;
-            v2 e = fineResidual(d, k, ki, f);
+            v2 e = fineResidual<S>(d, k, ki, f);
             sx += e.x;
             sy += e.y;
         }
     }
-    *(float4 *)&FLD(d, at, 0) = make_float4(0.0f, 0.0f, rho > 0.0f ? sx * 0.25f : 0.0f, rho > 0.0f ? sy * 0.25f : 0.0f);
+    lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, rho > 0.0f ? sx * 0.25f : 0.0f, rho > 0.0f ? sy * 0.25f : 0.0f));
 }
 // level 1's rho (restrictLevel's, :2342-2377), once a solve: it depends on
 // the fine rho alone
-__global__ void __launch_bounds__(64) k_levelRho(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(64) k_levelRho(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     if (!d.full || d.fluid_cycles == 0u) return;
     const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
     if (activeCount(d) == 0u) return;
@@ -1146,11 +1207,11 @@ __global__ void __launch_bounds__(64) k_levelRho(WorldGrid g) {
         }
         rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
     }
-    *(float4 *)&FLD(d, levelSlot(d, 1u, c) + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
+    lvStoreRho<S>(d, 1u, c, rho);
 }
 
 // prolonged (:2458-2497); prolong is in k_levelUp, prolongFine in k_relax
-__device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
+template <int S> __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
     i2 coarse = I2(c.x / 2, c.y / 2);
     i2 s = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
     v2 sum = V2(0.0f, 0.0f);
@@ -1159,11 +1220,11 @@ __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
         i2 off = k == 0 ? I2(0, 0) : (k == 1 ? I2(s.x, 0) : (k == 2 ? I2(0, s.y) : s));
         float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
         i2 at = I2(coarse.x + off.x, coarse.y + off.y);
-        float r = rhoAt(d, l + 1u, at);
+        float r = inLevel(d, l + 1u, at) ? lvRho<S>(d, l + 1u, at) : -1.0f;
         if (r < 0.0f) continue;
         weight += wk;
         if (r > 0.0f) {
-            v2 p = potAt(d, l + 1u, at);
+            float2 p = lvLoadP<S>(d, l + 1u, at);
             sum.x += wk * p.x;
             sum.y += wk * p.y;
         }
@@ -1254,8 +1315,9 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
         __syncthreads();
     }
 }
-__global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t l) {
+template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t l) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const uint32_t T = d.tile[l];
@@ -1271,9 +1333,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_
         float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float rho = -1.0f;
         if (inLevel(d, l, c)) {
-            uint64_t at = levelSlot(d, l, c);
-            e = *(const float4 *)&FLD(d, at, 0);
-            rho = FLD(d, at + 1u, 0);
+            e = lvLoad<S>(d, l, c);
+            rho = lvRho<S>(d, l, c);
         }
         t.px[j] = e.x;
         t.py[j] = e.y;
@@ -1298,7 +1359,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_
         }
         resx[k] = qx;
         resy[k] = qy;
-        if (inLevel(d, l, c)) *(float2 *)&FLD(d, levelSlot(d, l, c), 0) = make_float2(t.px[j], t.py[j]);
+        if (inLevel(d, l, c)) lvStoreP<S>(d, l, c, make_float2(t.px[j], t.py[j]));
     }
     __syncthreads();
     // restrictLevel (:2342-2377) into the tile's parents
@@ -1332,13 +1393,13 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_
             ox = rho > 0.0f ? sx * 0.25f : 0.0f;
             oy = rho > 0.0f ? sy * 0.25f : 0.0f;
         }
-        uint64_t at = levelSlot(d, lc, c);
-        *(float4 *)&FLD(d, at, 0) = make_float4(0.0f, 0.0f, ox, oy);
-        *(float4 *)&FLD(d, at + 1u, 0) = make_float4(rho, 0.0f, 0.0f, 0.0f);
+        lvStore<S>(d, lc, c, make_float4(0.0f, 0.0f, ox, oy));
+        lvStoreRho<S>(d, lc, c, rho);
     }
 }
-__global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
+template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     const uint32_t T = d.tile[l];
@@ -1354,11 +1415,10 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t 
         i2 c = I2(cx0 + k % Rc, cy0 + k / Rc);
         float px = 0.0f, py = 0.0f, rho = -1.0f;
         if (inLevel(d, l + 1u, c)) {
-            uint64_t at = levelSlot(d, l + 1u, c);
-            float2 e = *(const float2 *)&FLD(d, at, 0);
+            float2 e = lvLoadP<S>(d, l + 1u, c);
             px = e.x;
             py = e.y;
-            rho = FLD(d, at + 1u, 0);
+            rho = lvRho<S>(d, l + 1u, c);
         }
         cpx[k] = px;
         cpy[k] = py;
@@ -1370,9 +1430,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t 
         float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float rho = -1.0f;
         if (inLevel(d, l, c)) {
-            uint64_t at = levelSlot(d, l, c);
-            e = *(const float4 *)&FLD(d, at, 0);
-            rho = FLD(d, at + 1u, 0);
+            e = lvLoad<S>(d, l, c);
+            rho = lvRho<S>(d, l, c);
         }
         t.px[j] = e.x;
         t.py[j] = e.y;
@@ -1413,7 +1472,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t 
         int32_t x = (int32_t)(k % T + H), y = (int32_t)(k / T + H), j = y * t.R + x;
         i2 c = I2(t.ox + x, t.oy + y);
         if (!inLevel(d, l, c)) continue;
-        *(float2 *)&FLD(d, levelSlot(d, l, c), 0) = make_float2(t.px[j], t.py[j]);
+        lvStoreP<S>(d, l, c, make_float2(t.px[j], t.py[j]));
     }
 }
 // The coarse levels from lds_level down, all in one workgroup's LDS:
@@ -1472,8 +1531,9 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
     }
     __syncthreads();
 }
-__global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
@@ -1504,10 +1564,9 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
         for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
             for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
                 uint32_t j = y * T.w + x;
-                uint64_t at = levelSlot(d, l0, I2((int32_t)x, (int32_t)y));
-                float4 e = *(const float4 *)&FLD(d, at, 0);
+                float4 e = lvLoad<S>(d, l0, I2((int32_t)x, (int32_t)y));
                 T.px[j] = e.x; T.py[j] = e.y; T.rx[j] = e.z; T.ry[j] = e.w;
-                T.rho[j] = FLD(d, at + 1u, 0);
+                T.rho[j] = lvRho<S>(d, l0, I2((int32_t)x, (int32_t)y));
             }
         __syncthreads();
         ldsBetas(T);
@@ -1608,9 +1667,9 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
         for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
             for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
                 uint32_t j = y * T.w + x;
-                uint64_t at = levelSlot(d, l, I2((int32_t)x, (int32_t)y));
-                *(float4 *)&FLD(d, at, 0) = make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]);
-                if (!idle) *(float4 *)&FLD(d, at + 1u, 0) = make_float4(T.rho[j], 0.0f, 0.0f, 0.0f);
+                i2 c = I2((int32_t)x, (int32_t)y);
+                lvStore<S>(d, l, c, make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]));
+                if (!idle) lvStoreRho<S>(d, l, c, T.rho[j]);
             }
     }
 }
@@ -1619,20 +1678,20 @@ __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
 
 // ------------------------------------------------ conjugate gradients
 // cgDirection's p (:2767-2776) at cell i: z + beta p, 0 off the fluid
-__device__ __forceinline__ float newP(const Dev &d, uint32_t i, double beta, const float *pold) {
-    return (d.cflag[i] & CF_FLUID) ? (float)((double)d.pa[i].x + beta * (double)pold[i]) : 0.0f;
+template <int S> __device__ __forceinline__ float newP(const Dev &d, uint32_t i, double beta, const float *pold) {
+    return (d.cflag[i] & CF_FLUID) ? (float)((double)fnLoad<S>(d, 0, i).x + beta * (double)pold[i]) : 0.0f;
 }
 // applyA (:2595-2613) from the packed betas: every face but an obstacle's
 // counts in the diagonal, only a fluid neighbour's p in the sum (each
 // neighbour's new p taken as its own thread takes it)
-__device__ __forceinline__ float applyA(const Dev &d, i2 c, uint32_t f, float p, double beta, const float *pold) {
+template <int S> __device__ __forceinline__ float applyA(const Dev &d, i2 c, uint32_t f, float p, double beta, const float *pold) {
     float4 cf = d.coef[cellIndex(d, c)];
     const float v[4] = {cf.x, cf.y, cf.z, cf.w};
     float sum = 0.0f, diag = 0.0f;
     for (int k = 0; k < 4; k++) {
         float b = fabsf(v[k]);
         if (b == 0.0f) continue;
-        if (__float_as_uint(v[k]) >> 31) sum += b * newP(d, cellIndex(d, beside(c, k)), beta, pold);
+        if (__float_as_uint(v[k]) >> 31) sum += b * newP<S>(d, cellIndex(d, beside(c, k)), beta, pold);
         diag += b;
     }
     diag += (f & CF_PINNED) ? PINF : 0.0f;
@@ -1726,14 +1785,21 @@ __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double bb = 0.0;
+        uint32_t most = 0;
         if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
             uint32_t i = cellIndex(d, c);
-            d.cr[i] = -residualAt(d, 0u, c).x;
+            float r = -residualAt(d, 0u, c).x;
+            d.cr[i] = r;
+            most = ubits(fabsf(r));
             double bi = -(double)FLD(d, 2u * i, 1);
             bb = bi * bi;
         }
         bb = wave_tree_sum(bb);
-        if (LANE == 0) d.bpart[4ull * wid + 2] = bb;
+        most = wave_max_u32(most);
+        if (LANE == 0) {
+            d.bpart[4ull * wid + 2] = bb;
+            d.bpart[4ull * wid + 3] = (double)most;
+        }
     }
 }
 __global__ void __launch_bounds__(1024) k_sumNorm(WorldGrid g) {
@@ -1742,14 +1808,31 @@ __global__ void __launch_bounds__(1024) k_sumNorm(WorldGrid g) {
     __shared__ double s0[SUM_LANES], s1[SUM_LANES];
     double bb, unused;
     sumPlaces2(d, 2, 2, s0, s1, &bb, &unused);
-    if (threadIdx.x == 0) d.head->bb = bb;
+    // and the largest |r0|, for the 16-bit V-cycle's scale
+    __shared__ uint32_t sMost[SUM_LANES];
+    uint32_t m = 0, na = activeCount(d);
+    for (uint32_t j = threadIdx.x; j < na; j += SUM_LANES) {
+        uint32_t v = (uint32_t)d.bpart[4ull * j + 3];
+        m = v > m ? v : m;
+    }
+    sMost[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t h = SUM_LANES / 2u; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sMost[threadIdx.x] = max(sMost[threadIdx.x], sMost[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        d.head->bb = bb;
+        d.head->vscale = vcycleScale(d, sMost[0]);
+    }
 }
 // cgPrepare (:2721-2732), and the solve's fine level packed (the arrays'
 // layout is with the Dev struct).  p is 0 and the drift as it stands, the
 // rhs -r and the drift rhs; the ring of cells beside the blocks worked on
 // holds its field values, as the relaxations' rims read them there.
-__global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 bk = activeBlock(d, wid), c = blockCell(bk, threadIdx.x);
@@ -1757,8 +1840,8 @@ __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) {
         uint32_t i = cellIndex(d, c);
         float rho = FLD(d, 2u * i, 2);
         float2 v = make_float2(0.0f, FLD(d, 2u * i + 1u, 0));
-        d.pa[i] = v;
-        d.pb[i] = v;
+        fnStore<S>(d, 0, i, v);
+        fnStore<S>(d, 1, i, v);
         uint32_t f = 0;
         float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float2 r = make_float2(0.0f, 0.0f);
@@ -1771,25 +1854,26 @@ __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) {
                 b[k] = rn > 0.0f ? -beta : beta;
             }
             cf = make_float4(b[0], b[1], b[2], b[3]);
-            r = make_float2(-d.cr[i], FLD(d, 2u * i + 1u, 1));
+            r = make_float2(S == STORE_F32 ? -d.cr[i] : -d.cr[i] * d.head->vscale, FLD(d, 2u * i + 1u, 1));
         }
         d.coef[i] = cf;
         d.cflag[i] = (uint8_t)f;
-        d.r2[i] = r;
+        fnStore<S>(d, 2, i, r);
         for (int k = 0; k < 4; k++) {
             i2 n = beside(c, k);
             if (!inWorld(d, n) || (idiv8(n).x == bk.x && idiv8(n).y == bk.y) || isActiveBlock(d, idiv8(n))) continue;
             uint32_t ni = cellIndex(d, n);
             float2 q = make_float2(FLD(d, 2u * ni, 0), FLD(d, 2u * ni + 1u, 0));
-            d.pa[ni] = q;
-            d.pb[ni] = q;
+            fnStore<S>(d, 0, ni, q);
+            fnStore<S>(d, 1, ni, q);
             d.cflag[ni] = FLD(d, 2u * ni, 2) > 0.0f ? (uint8_t)CF_FLUID : (uint8_t)0;
         }
     }
 }
 // cgRho + cgBeta (:2736-2764): r.z and z.Ap, and Notay's flexible beta in FP64
-__global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
@@ -1798,7 +1882,7 @@ __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
         double v0 = 0.0, v1 = 0.0;
         if (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID)) {
             uint32_t i = cellIndex(d, c);
-            double z = (double)d.pa[i].x;
+            double z = (double)fnLoad<S>(d, 0, i).x;
             v0 = (double)d.cr[i] * z;
             v1 = (double)d.cq[i] * z;
         }
@@ -1829,8 +1913,9 @@ __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
 }
 // cgDirection + cgApply + cgAlpha (:2767-2814): the iteration's p (into
 // the other p buffer) and q = A p
-__global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
@@ -1842,8 +1927,8 @@ __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
         double v = 0.0;
         if (inWorld(d, c)) {
             uint32_t i = cellIndex(d, c), f = d.cflag[i];
-            float p = newP(d, i, beta, pold);
-            float q = (f & CF_FLUID) ? applyA(d, c, f, p, beta, pold) : 0.0f;
+            float p = newP<S>(d, i, beta, pold);
+            float q = (f & CF_FLUID) ? applyA<S>(d, c, f, p, beta, pold) : 0.0f;
             pnew[i] = p;
             d.cq[i] = q;
             v = (double)p * (double)q;
@@ -1871,8 +1956,9 @@ __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
 // cgUpdate (:2852-2880), and whether the next
 // iteration runs: max |r| under the tolerance (the reference's converged(),
 // :2223-2226), or ||r|| <= tol ||b||
-__global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     const uint32_t cycle = ld(g.cyc);
     if (!encoded(d, cycle)) return;
     bool conv = converged(d, cycle);
@@ -1882,6 +1968,7 @@ __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
     }
     if (conv) return;
     double alpha = ld_f64(&d.head->alpha);
+    const float vscale = d.head->vscale;
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double rr = 0.0;
@@ -1893,8 +1980,8 @@ __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
             float xn = (float)((double)x + alpha * (double)p);
             d.cx[i] = xn;
             d.cr[i] = rn;
-            d.pa[i].x = 0.0f;
-            d.r2[i].x = -rn;
+            fnStoreX<S>(d, 0, i, 0.0f);
+            fnStoreX<S>(d, 2, i, S == STORE_F32 ? -rn : -rn * vscale);
             most = ubits(fabsf(rn));
             rr = (double)rn * (double)rn;
         }
@@ -1930,6 +2017,7 @@ __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
     }
     if (threadIdx.x == 0) {
         if (sMost[0] > *slot) *slot = sMost[0];
+        d.head->vscale = vcycleScale(d, sMost[0]);
         SolveHead *h = d.head;
         h->rr = rr;
         h->cycles += 1u;
@@ -1955,8 +2043,9 @@ __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
 // packed solve kept aside: fluidCg's vectors, the rhs (-r), the drift, and,
 // when an iteration ran, the rims the last post-relaxation read
 // (prolongFine's p)
-__global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
+template <int S> __global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
     WORLD(g);
+    if (d.store != S) return;
     if (!d.full) return;
     const int ran = d.fluid_cycles > 0u ? 1 : 0;
     if (bid == 0 && threadIdx.x == 0) {
@@ -1973,11 +2062,11 @@ __global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
         FLD(d, 2u * i, 0) = x;
         CNT(d, i, G_REGION) = 0u;
         if (d.cflag[i] & CF_FLUID) {
-            FLD(d, 2u * i, 1) = d.r2[i].x;
-            FLD(d, 2u * i + 1u, 0) = d.pa[i].y;
+            FLD(d, 2u * i, 1) = -d.cr[i];
+            FLD(d, 2u * i + 1u, 0) = fnLoad<S>(d, 0, i).y;
             if (ran) {
-                float2 p = d.pb[i];
-                v2 e = prolonged(d, 0u, c);
+                float2 p = fnLoad<S>(d, 1, i);
+                v2 e = prolonged<S>(d, 0u, c);
                 CNT(d, i, G_RIM) = ubits(p.x + e.x);
                 CNT(d, i, G_RIM + 1u) = ubits(p.y + e.y);
             }
@@ -2451,3 +2540,20 @@ __global__ void __launch_bounds__(64) k_reports(WorldGrid g, uint32_t *out) {
     __syncthreads();
     if (t == 0) d.blocks[COUNT_SLOT(d, COUNT_MOST_CYCLES)] = 0u;
 }
+
+// the kernels of the V-cycle's storage, one of each STORE_*
+#define STORE_KERNELS(S) \
+    template __global__ void k_relax<S>(WorldGrid, int); \
+    template __global__ void k_restrict<S>(WorldGrid); \
+    template __global__ void k_levelRho<S>(WorldGrid); \
+    template __global__ void k_levelDown<S>(WorldGrid, uint32_t); \
+    template __global__ void k_levelUp<S>(WorldGrid, uint32_t); \
+    template __global__ void k_coarseLDS<S>(WorldGrid); \
+    template __global__ void k_cgPrepare<S>(WorldGrid); \
+    template __global__ void k_cgRho<S>(WorldGrid); \
+    template __global__ void k_cgApply<S>(WorldGrid); \
+    template __global__ void k_cgUpdate<S>(WorldGrid); \
+    template __global__ void k_cgFinish<S>(WorldGrid);
+STORE_KERNELS(0)
+STORE_KERNELS(1)
+STORE_KERNELS(2)

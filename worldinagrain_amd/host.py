@@ -38,20 +38,21 @@ class SolverSettings:
     first_chunk: int = 0
     chunk: int = 2
     level_pairs: int = 1     # red-black pairs a coarse level, down and up (the reference's 1)
+    storage: int = abi.WIG_STORE_F32   # the V-cycle's storage (WIG_STORE_*)
 
     @staticmethod
     def reference():
         return SolverSettings()
 
     @staticmethod
-    def relative(tol, max_cycles=100, sweep_pairs=2, level_pairs=2, coarse_sweeps=10):
+    def relative(tol, max_cycles=100, sweep_pairs=2, level_pairs=2, coarse_sweeps=10, storage=abi.WIG_STORE_F32):
         """A solve to ||r|| <= tol ||b|| (the benchmark configurations: 2
         red-black sweep pairs before and after the coarse levels, and on each
         of them; 10 SOR sweep pairs on the coarsest, which converge the
         iteration as fast as the reference's 40)."""
         return SolverSettings(tol_mode=abi.WIG_TOL_REL_L2, tol=tol, max_cycles=max_cycles, adaptive_cycles=False,
                               sweep_pairs=sweep_pairs, first_chunk=-1, level_pairs=level_pairs,
-                              coarse_sweeps=coarse_sweeps)
+                              coarse_sweeps=coarse_sweeps, storage=storage)
 
 
 def mulberry32_next(state):
@@ -113,6 +114,7 @@ class Simulation:
         c.pre_relax, c.post_relax = s.pre_relax, s.post_relax
         c.sweep_pairs, c.coarse_sweeps = s.sweep_pairs, s.coarse_sweeps
         c.level_pairs = s.level_pairs
+        c.vcycle_storage = s.storage
         if s.first_chunk < 0:
             c.first_chunk = 0xFFFFFFFF          # WIG_CHUNK_AUTO: the world's last solve's count
             c.chunk = s.chunk
