@@ -203,6 +203,25 @@ int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side);
 /* Waits for everything enqueued on the world's stream. */
 int wig_sync(WigWorld *w);
 
+/* The FP64 check of a pressure solve (this library's own; the reference has
+ * none).  With capture on, each solve keeps its right-hand side b.
+ * wig_cg64 then solves A x64 = b by plain CG in FP64 on the device, on the
+ * last solve's own matrix (its face betas and pins, as the solve applies
+ * them), from 0 to ||b - A x64|| <= tol ||b|| or maxit iterations, and
+ * measures the solve's float pressure x against it: err = ||x - x64|| /
+ * ||x64|| and relres = ||b - A x|| / ||b||, both in FP64 over the fluid
+ * cells.  Waits. */
+typedef struct WigCheck64 {
+    uint32_t iterations, pad;
+    double relres64;    /* of x64 */
+    double err;         /* of the solve's x against x64 */
+    double relres;      /* of the solve's x */
+} WigCheck64;
+int wig_capture(WigWorld *w, int on);
+int wig_cg64(WigWorld *w, double tol, uint32_t maxit, WigCheck64 *out);
+/* x64, one double a cell (0 off the fluid), after wig_cg64. */
+int wig_cg64_x(WigWorld *w, double *x);
+
 /* Batches: many worlds, of any sizes, stepped together, every kernel of the
  * step one launch over all of them (this library's own; the reference steps
  * one Simulation a frame).  The worlds stay the caller's and must outlive

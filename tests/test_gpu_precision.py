@@ -69,3 +69,32 @@ def test_batch_mixing_storages():
         assert not d, f"storage {st}: " + "; ".join(d)
     # and the 16-bit worlds did run a different preconditioner
     assert compare_worlds(inb[0][0], inb[1][0])
+
+
+def test_gpu_fp64_check_matches_oracle_cg64():
+    """wig_cg64 (FP64 CG on the device, on the solve's packed matrix and its
+    captured rhs) against the oracle's FP64 CG on the same step's matrix;
+    and the errors it reports against what the oracle's solution gives."""
+    W = H = 96
+    cells = scenes.dam_break(W, H, block=(48, 70))
+    wo, wg = make_oracle(W, H, cells), make_gpu(W, H, cells)
+    so, sg = config_sim(wo, W, H, tol=1e-6), config_sim(wg, W, H, tol=1e-6)
+    for _ in range(4):
+        so.step()
+        sg.step()
+    wo.capture(True)
+    wg.capture(True)
+    so.step()
+    sg.step()
+    x_cpu, _, _ = wo.cg64(wo.captured_b(), tol=1e-12)
+    chk, x_gpu = wg.cg64(tol=1e-12)
+    f = wg.download(abi.PART_FIELD)[:8 * W * H].reshape(-1, 8)
+    fl = f[:, 2] > 0
+    d = np.linalg.norm(x_gpu[fl] - x_cpu[fl]) / np.linalg.norm(x_cpu[fl])
+    err_cpu = np.linalg.norm(f[fl, 0] - x_cpu[fl]) / np.linalg.norm(x_cpu[fl])
+    say(f"  FP64 CG: {chk.iterations} iterations, relres {chk.relres64:.2e}; GPU vs CPU x64 {d:.2e}; "
+        f"solve err {chk.err:.3e} (vs CPU x64 {err_cpu:.3e}), solve relres {chk.relres:.2e}")
+    assert chk.relres64 <= 1e-11
+    assert d < 1e-9
+    assert abs(chk.err - err_cpu) <= 1e-3 * err_cpu + 1e-12
+    assert chk.relres < 1e-4

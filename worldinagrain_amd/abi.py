@@ -59,6 +59,11 @@ class WigSolveStats(ct.Structure):
     ]
 
 
+class WigCheck64(ct.Structure):
+    _fields_ = [("iterations", ct.c_uint32), ("pad", ct.c_uint32), ("relres64", ct.c_double), ("err", ct.c_double),
+                ("relres", ct.c_double)]
+
+
 assert ct.sizeof(WigParams) == 80
 assert ct.sizeof(WigSolveStats) == 64
 

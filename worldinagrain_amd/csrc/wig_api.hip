@@ -60,6 +60,14 @@ __global__ void k_raster(WorldGrid g);
 __global__ void k_unpaint(WorldGrid g);
 __global__ void k_stops(WorldGrid g, uint32_t *out);
 __global__ void k_cycle(uint32_t *cyc, uint32_t set, uint32_t v);
+__global__ void k64_start(WorldGrid g);
+__global__ void k64_sum(WorldGrid g, int which);
+__global__ void k64_apply(WorldGrid g);
+__global__ void k64_update(WorldGrid g);
+__global__ void k64_direction(WorldGrid g);
+__global__ void k64_check(WorldGrid g, int phase);
+__global__ void k64_residual(WorldGrid g, int of_x64);
+__global__ void k64_keep(WorldGrid g, int nv, int at);
 __global__ void k_reports(WorldGrid g, uint32_t *out);
 
 static thread_local std::string g_err;
@@ -156,6 +164,7 @@ struct WigWorld {
     uint32_t *scratch;           // B words
     void *fine;                  // the solve's packed fine level (Dev::coef ... cflag)
     void *vlevels;               // the 16-bit V-cycle's coarse levels
+    void *check64;               // wig_capture / wig_cg64's arrays (made on first use)
     uint64_t vlevel_bytes;
     unsigned long long *totals;  // WigRunTotals on the device
     uint32_t *wstate;            // WS_* words on the device (the end of the solve part)
@@ -459,6 +468,7 @@ int wig_destroy(WigWorld *w) {
     if (w->scratch) (void)hipFree(w->scratch);
     if (w->fine) (void)hipFree(w->fine);
     if (w->vlevels) (void)hipFree(w->vlevels);
+    if (w->check64) (void)hipFree(w->check64);
     if (w->totals) (void)hipFree(w->totals);
     if (w->own) (void)hipStreamDestroy(w->own);
     delete w;
@@ -937,6 +947,81 @@ int wig_set_graphs(WigWorld *w, int on) {
 int wig_batch_set_graphs(WigBatch *b, int on) {
     if (!b) return fail(WIG_E_INVALID, "null batch");
     ((Batch *)b)->graphs = on != 0;
+    return WIG_OK;
+}
+
+int wig_capture(WigWorld *w, int on) {
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    Dev &d = w->d;
+    if (on && !w->check64) {
+        uint64_t bytes = 4ull * d.C + 32ull * d.C + 32ull * d.B + 256u;
+        if (hipMalloc(&w->check64, bytes) != hipSuccess) return fail(WIG_E_NOMEM, "capture arrays: %llu bytes", (unsigned long long)bytes);
+        if (zero(stream_of(w), w->check64, bytes) != WIG_OK) return WIG_E_DEVICE;
+        uint8_t *c = (uint8_t *)w->check64;
+        d.head64 = (double *)c;
+        d.part64 = (double *)(c + 256u);
+        d.x64 = (double *)(c + 256u + 32ull * d.B);
+        d.r64 = d.x64 + d.C;
+        d.p64 = d.r64 + d.C;
+        d.q64 = d.p64 + d.C;
+        d.capb = (float *)(d.q64 + d.C);
+    }
+    if (!on) d.capb = nullptr;
+    else d.capb = (float *)(d.q64 + d.C);
+    return WIG_OK;
+}
+
+int wig_cg64(WigWorld *w, double tol, uint32_t maxit, WigCheck64 *out) {
+    if (!w || !out) return fail(WIG_E_INVALID, "null argument");
+    if (!w->d.capb) return fail(WIG_E_INVALID, "wig_cg64 needs a solve run with wig_capture on");
+    Batch &b = *w->solo;
+    int st = batch_maps(b);
+    if (st < 0) return st;
+    b.htab[0] = w->d;
+    hipStream_t s = b.stream;
+    HIPCHECK(hipMemcpyAsync(b.tab, b.htab.data(), sizeof(Dev), hipMemcpyHostToDevice, s));
+    WorldGrid g4 = grid_of(b, SH_BG4), g1 = grid_of(b, SH_ONE);
+    dim3 n4(b.total[SH_BG4]);
+    hipLaunchKernelGGL(k64_start, n4, dim3(256), 0, s, g4);
+    hipLaunchKernelGGL(k64_sum, dim3(1), dim3(1024), 0, s, g1, 0);
+    double h[16];
+    uint32_t it = 0;
+    for (; it < maxit; it++) {
+        if (it % 32u == 0u) {
+            HIPCHECK(hipMemcpyAsync(h, w->d.head64, 16, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (h[0] <= tol * tol * h[1]) break;
+        }
+        hipLaunchKernelGGL(k64_apply, n4, dim3(256), 0, s, g4);
+        hipLaunchKernelGGL(k64_sum, dim3(1), dim3(1024), 0, s, g1, 1);
+        hipLaunchKernelGGL(k64_update, n4, dim3(256), 0, s, g4);
+        hipLaunchKernelGGL(k64_sum, dim3(1), dim3(1024), 0, s, g1, 2);
+        hipLaunchKernelGGL(k64_direction, n4, dim3(256), 0, s, g4);
+    }
+    hipLaunchKernelGGL(k64_check, n4, dim3(256), 0, s, g4, 0);
+    hipLaunchKernelGGL(k64_check, n4, dim3(256), 0, s, g4, 1);
+    hipLaunchKernelGGL(k64_keep, dim3(1), dim3(1024), 0, s, g1, 2, 5);
+    hipLaunchKernelGGL(k64_residual, n4, dim3(256), 0, s, g4, 0);
+    hipLaunchKernelGGL(k64_keep, dim3(1), dim3(1024), 0, s, g1, 1, 7);
+    hipLaunchKernelGGL(k64_residual, n4, dim3(256), 0, s, g4, 1);
+    hipLaunchKernelGGL(k64_keep, dim3(1), dim3(1024), 0, s, g1, 1, 8);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(h, w->d.head64, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    out->iterations = it;
+    out->pad = 0;
+    double bb = h[1] > 0.0 ? h[1] : 1.0;
+    out->relres64 = std::sqrt(h[8] / bb);
+    out->err = h[6] > 0.0 ? std::sqrt(h[5] / h[6]) : 0.0;
+    out->relres = std::sqrt(h[7] / bb);
+    return WIG_OK;
+}
+
+int wig_cg64_x(WigWorld *w, double *x) {
+    if (!w || !x) return fail(WIG_E_INVALID, "null argument");
+    if (!w->check64) return fail(WIG_E_INVALID, "no FP64 solution (wig_cg64)");
+    HIPCHECK(hipMemcpyAsync(x, w->d.x64, 8ull * w->d.C, hipMemcpyDeviceToHost, stream_of(w)));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
 }
 

@@ -147,6 +147,10 @@ struct Dev {
     // levels (p, rhs as four halves a cell, rho as a float) live here
     // instead of the float arrays and the field's level records.
     uint32_t store;
+    // The FP64 check of a solve (wig_cg64): the solve's rhs as cgResidual
+    // saw it (when capturing), and FP64 CG vectors and per-block partials
+    float *capb;
+    double *x64, *r64, *p64, *q64, *part64, *head64;
     uint32_t *ha, *hb, *hr2;
     uint16_t *vlev[MG_LEVELS + 1];
     float *vrho[MG_LEVELS + 1];

@@ -1793,6 +1793,7 @@ __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
             most = ubits(fabsf(r));
             double bi = -(double)FLD(d, 2u * i, 1);
             bb = bi * bi;
+            if (d.capb) d.capb[i] = -FLD(d, 2u * i, 1);
         }
         bb = wave_tree_sum(bb);
         most = wave_max_u32(most);
@@ -2557,3 +2558,170 @@ __global__ void __launch_bounds__(64) k_reports(WorldGrid g, uint32_t *out) {
 STORE_KERNELS(0)
 STORE_KERNELS(1)
 STORE_KERNELS(2)
+
+// ============================================================ the FP64 check
+// Plain CG in FP64 on the matrix of the world's last solve (its packed
+// betas and pins, applied as cgApply applies them) for the rhs that solve
+// captured, from 0: the reference a solve's pressure is measured against.
+// Sums are per block (a wavefront's halving), then one workgroup's over the
+// blocks in list order.  head64: rr, bb, pq, alpha, beta, then the check's
+// four sums.
+__device__ __forceinline__ double apply64(const Dev &d, i2 c, uint32_t i, const double *p) {
+    float4 cf = d.coef[i];
+    const float v[4] = {cf.x, cf.y, cf.z, cf.w};
+    double sum = 0.0, diag = 0.0;
+    for (int k = 0; k < 4; k++) {
+        double b = (double)fabsf(v[k]);
+        if (b == 0.0) continue;
+        if (__float_as_uint(v[k]) >> 31) sum += b * p[cellIndex(d, beside(c, k))];
+        diag += b;
+    }
+    if (d.cflag[i] & CF_PINNED) diag += (double)PINF;
+    return diag * p[i] - sum;
+}
+#define FLUID_CELL(d, c) (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID))
+__device__ void sum64(const Dev &d, int nv, double *out) {
+    __shared__ double t[4][SUM_LANES];
+    uint32_t li = threadIdx.x, na = activeCount(d);
+    for (int k = 0; k < nv; k++) {
+        double a = 0.0;
+        for (uint32_t j = li; j < na; j += SUM_LANES) a += d.part64[4ull * j + k];
+        t[k][li] = a;
+    }
+    __syncthreads();
+    for (uint32_t h = SUM_LANES / 2u; h > 0; h >>= 1) {
+        if (li < h)
+            for (int k = 0; k < nv; k++) t[k][li] += t[k][li + h];
+        __syncthreads();
+    }
+    if (li == 0)
+        for (int k = 0; k < nv; k++) out[k] = t[k][0];
+}
+__global__ void __launch_bounds__(256) k64_start(WorldGrid g) {
+    WORLD(g);
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
+        double bb = 0.0;
+        if (inWorld(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            double b = FLUID_CELL(d, c) ? (double)d.capb[i] : 0.0;
+            d.x64[i] = 0.0;
+            d.r64[i] = b;
+            d.p64[i] = b;
+            bb = b * b;
+        }
+        bb = wave_tree_sum(bb);
+        if (LANE == 0) d.part64[4ull * wid] = bb;
+    }
+}
+// which 0: rr = bb (the start); 1: pq, alpha; 2: the new rr, beta
+__global__ void __launch_bounds__(1024) k64_sum(WorldGrid g, int which) {
+    WORLD(g);
+    double v[1];
+    sum64(d, 1, v);
+    if (threadIdx.x == 0) {
+        double *h = d.head64;
+        if (which == 0) {
+            h[0] = v[0];
+            h[1] = v[0];
+        } else if (which == 1) {
+            h[2] = v[0];
+            h[3] = v[0] > 0.0 ? h[0] / v[0] : 0.0;
+        } else {
+            h[4] = h[0] > 0.0 ? v[0] / h[0] : 0.0;
+            h[0] = v[0];
+        }
+    }
+}
+__global__ void __launch_bounds__(256) k64_apply(WorldGrid g) {
+    WORLD(g);
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
+        double pq = 0.0;
+        if (FLUID_CELL(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            double q = apply64(d, c, i, d.p64);
+            d.q64[i] = q;
+            pq = d.p64[i] * q;
+        }
+        pq = wave_tree_sum(pq);
+        if (LANE == 0) d.part64[4ull * wid] = pq;
+    }
+}
+__global__ void __launch_bounds__(256) k64_update(WorldGrid g) {
+    WORLD(g);
+    double alpha = d.head64[3];
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
+        double rr = 0.0;
+        if (FLUID_CELL(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            d.x64[i] += alpha * d.p64[i];
+            double r = d.r64[i] - alpha * d.q64[i];
+            d.r64[i] = r;
+            rr = r * r;
+        }
+        rr = wave_tree_sum(rr);
+        if (LANE == 0) d.part64[4ull * wid] = rr;
+    }
+}
+__global__ void __launch_bounds__(256) k64_direction(WorldGrid g) {
+    WORLD(g);
+    double beta = d.head64[4];
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
+        if (FLUID_CELL(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            d.p64[i] = d.r64[i] + beta * d.p64[i];
+        }
+    }
+}
+// the check: |x - x64|^2, |x64|^2 for the world's float x (the solve's),
+// then |b - A x|^2 and |b - A x64|^2 (p64 and q64 take x and x64 as float
+// x in double)
+__global__ void __launch_bounds__(256) k64_check(WorldGrid g, int phase) {
+    WORLD(g);
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
+        double a = 0.0, b2 = 0.0;
+        if (inWorld(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            if (phase == 0) {
+                d.p64[i] = FLUID_CELL(d, c) ? (double)d.cx[i] : 0.0;
+            } else if (FLUID_CELL(d, c)) {
+                double e = d.p64[i] - d.x64[i];
+                a = e * e;
+                b2 = d.x64[i] * d.x64[i];
+            }
+        }
+        if (phase == 0) continue;
+        a = wave_tree_sum(a);
+        b2 = wave_tree_sum(b2);
+        if (LANE == 0) {
+            d.part64[4ull * wid] = a;
+            d.part64[4ull * wid + 1] = b2;
+        }
+    }
+}
+__global__ void __launch_bounds__(256) k64_residual(WorldGrid g, int of_x64) {
+    WORLD(g);
+    const double *x = of_x64 ? d.x64 : d.p64;
+    FOR_ACTIVE_BLOCKS4(d, wid) {
+        i2 c = blockCell(activeBlock(d, wid), LANE);
+        double rr = 0.0;
+        if (FLUID_CELL(d, c)) {
+            uint32_t i = cellIndex(d, c);
+            double r = (double)d.capb[i] - apply64(d, c, i, x);
+            rr = r * r;
+        }
+        rr = wave_tree_sum(rr);
+        if (LANE == 0) d.part64[4ull * wid] = rr;
+    }
+}
+__global__ void __launch_bounds__(1024) k64_keep(WorldGrid g, int nv, int at) {
+    WORLD(g);
+    double v[4];
+    sum64(d, nv, v);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nv; k++) d.head64[at + k] = v[k];
+}

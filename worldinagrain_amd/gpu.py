@@ -56,6 +56,9 @@ def lib():
         L.wig_batch_sync.argtypes = [ct.c_void_p]
         L.wig_batch_time_kernel.argtypes = [ct.c_void_p, ct.c_int]
         L.wig_set_graphs.argtypes = [ct.c_void_p, ct.c_int]
+        L.wig_capture.argtypes = [ct.c_void_p, ct.c_int]
+        L.wig_cg64.argtypes = [ct.c_void_p, ct.c_double, ct.c_uint32, ct.POINTER(abi.WigCheck64)]
+        L.wig_cg64_x.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double)]
         L.wig_batch_set_graphs.argtypes = [ct.c_void_p, ct.c_int]
         L.wig_batch_kernel_time.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint32)]
         _lib = L
@@ -163,6 +166,19 @@ class GpuWorld:
     def set_graphs(self, on):
         """Steps replayed as hipGraphs (wig_set_graphs)."""
         _check(lib().wig_set_graphs(self._w, 1 if on else 0))
+
+    def capture(self, on=True):
+        """Each later solve keeps its right-hand side, for cg64 (wig_capture)."""
+        _check(lib().wig_capture(self._w, 1 if on else 0))
+
+    def cg64(self, tol=1e-12, maxit=200000):
+        """FP64 CG on the last solve's matrix and rhs, and the solve's pressure
+        measured against it (wig_cg64): (WigCheck64, x64 per cell)."""
+        out = abi.WigCheck64()
+        _check(lib().wig_cg64(self._w, tol, maxit, ct.byref(out)))
+        x = np.empty(self.layout.cells, dtype=np.float64)
+        _check(lib().wig_cg64_x(self._w, x.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return out, x
 
     def clone(self):
         """A new world of the same size and kinds holding this one's parts."""
