@@ -82,7 +82,7 @@ def cpu_baseline(cells, kinds, size, tol, steps):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=("single", "batch", "serve"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "serve", "precision"), default="single",
                     help="single: the 1024x1024 world (the metric's configuration); batch: many worlds in one batch; "
                          "serve: the serving loop over many sessions")
     ap.add_argument("--worlds", type=int, default=512, help="batch: how many worlds")
@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--duration", type=float, default=0.25, help="serve: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget for the worlds")
     ap.add_argument("--round-cells", type=int, default=1 << 24, help="serve: most cells a round steps")
+    ap.add_argument("--seeds", type=int, default=2, help="precision: worlds (seeds) per variant")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
@@ -122,6 +123,8 @@ def main():
         return run_batch(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "precision":
+        return run_precision(args)
 
     if args.copy_mib:
         src = torch.empty(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -345,6 +348,58 @@ def run_batch(args):
     out["speedup_vs_sequential"] = out["value"] / out["sequential"]["solves_per_s"]
     if per_kernel:
         out["kernel_ms_one_step"] = {k: [round(v[0], 4), v[1]] for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:16]}
+    print(json.dumps(out), flush=True)
+
+
+def run_precision(args):
+    """Isolated pressure solves (the first step's solve of a world) on a
+    `size` x `size` grid: per seed 70% water with velocities i.i.d. N(0, 1)
+    per component, 20% air in random rectangles (p = 0 there), 10% solid
+    discs, no gravity (scenes.precision_world), so the rhs is the
+    divergence of the random velocities (regions closed off from the air
+    are pinned, as the reference does).  For each tolerance (1e-6, 1e-8 on
+    ||r|| / ||b||) and V-cycle storage (float, half, bfloat16), a copy of the
+    same world solves it; its iterations, solve time, and its pressure's
+    error and true residual against FP64 CG on the same matrix (wig_cg64)."""
+    import numpy as np
+    import torch
+    from worldinagrain_amd import abi, scenes, elements as E
+    from worldinagrain_amd.gpu import GpuWorld
+    from worldinagrain_amd.host import Simulation, SolverSettings
+
+    kinds = E.kinds()
+    size = args.size if args.size != 1024 else 2048
+    names = {abi.WIG_STORE_F32: "fp32", abi.WIG_STORE_F16: "fp16", abi.WIG_STORE_BF16: "bf16"}
+    results = []
+    for seed in range(1, args.seeds + 1):
+        cells, nf = scenes.precision_world(size, seed)
+        base = GpuWorld(size, size, kinds=kinds)
+        base.upload(abi.PART_CELLS, cells)
+        for tol in (1e-6, 1e-8):
+            for st in (abi.WIG_STORE_F32, abi.WIG_STORE_F16, abi.WIG_STORE_BF16):
+                w = base.clone()
+                w.capture(True)
+                solver = SolverSettings.relative(tol, max_cycles=200, storage=st)
+                sim = Simulation(w, size, size, dx=1.0, earth_g=0.0, dt=1.0 / 120.0, solver=solver)
+                w.time_kernel("solve")
+                sim.step()
+                ms, _ = w.kernel_time()
+                w.time_kernel(None)
+                ss = w.solve_stats()
+                chk, _ = w.cg64(tol=1e-12)
+                results.append({"seed": seed, "tol": tol, "vcycle": names[st], "iterations": int(ss.cycles),
+                                "solve_ms": ms, "err_vs_fp64": chk.err, "true_relres": chk.relres,
+                                "recursive_relres": float(np.sqrt(ss.rr / ss.bb)) if ss.bb > 0 else 0.0,
+                                "fp64_iterations": int(chk.iterations), "fp64_relres": chk.relres64})
+                print(f"[precision] seed {seed} tol {tol:g} {names[st]}: {ss.cycles} iterations, {ms:.2f} ms, "
+                      f"err {chk.err:.2e}, true relres {chk.relres:.2e}", file=sys.stderr, flush=True)
+                w.close()
+        base.close()
+    out = {"metric": "iterations, solve time and error vs FP64 CG of isolated pressure solves", "workload": "precision",
+           "config": {"workload": f"{size}x{size}: 70% water with N(0,1) velocities, 20% air rectangles, 10% solid discs; "
+                                  f"MG-PCG, FP32 outer PCG with FP64 sums, V-cycle storage float/half/bfloat16",
+                      "seeds": args.seeds, "fluid_cells": nf},
+           "results": results}
     print(json.dumps(out), flush=True)
 
 

@@ -158,3 +158,36 @@ def batch_sides(n, seed=7, sides=(64, 128, 256, 512), probs=(0.4, 0.3, 0.2, 0.1)
         u = rng.uniform()
         out.append(sides[int(min(np.searchsorted(cum, u, side="right"), len(sides) - 1))])
     return out
+
+
+def precision_world(side, seed, fluid_share=0.70, air_share=0.20, solid_share=0.10, vel_sigma=1.0, dt=1.0 / 120.0):
+    """A world of the precision study: solid discs over `solid_share` of the
+    cells (radius U[2, side / 64 + 4]), then air in random axis-aligned
+    rectangles (sides U[side / 32, side / 8]) over `air_share`, the rest
+    water (about `fluid_share`) with each cell's velocity i.i.d.
+    N(0, vel_sigma^2) per component, cells per unit time times dt
+    (SplitMix64 and numpy `seed`).  Returns (cells, water cells)."""
+    rng = SplitMix64(seed)
+    solid = np.zeros((side, side), dtype=bool)
+    yy, xx = np.mgrid[0:side, 0:side]
+    rmax = side / 64.0 + 4.0
+    while solid.mean() < solid_share:
+        cx, cy = rng.uniform() * side, rng.uniform() * side
+        r = 2.0 + rng.uniform() * (rmax - 2.0)
+        x0, x1 = max(0, int(cx - r) - 1), min(side, int(cx + r) + 2)
+        y0, y1 = max(0, int(cy - r) - 1), min(side, int(cy + r) + 2)
+        solid[y0:y1, x0:x1] |= (xx[y0:y1, x0:x1] + 0.5 - cx) ** 2 + (yy[y0:y1, x0:x1] + 0.5 - cy) ** 2 <= r * r
+    air = np.zeros((side, side), dtype=bool)
+    lo, hi = max(1, side // 32), max(2, side // 8)
+    while (air & ~solid).mean() < air_share:
+        w = lo + int(rng.uniform() * (hi - lo))
+        h = lo + int(rng.uniform() * (hi - lo))
+        x0, y0 = int(rng.uniform() * (side - w)), int(rng.uniform() * (side - h))
+        air[y0:y0 + h, x0:x0 + w] = True
+    cells = empty_cells(side, side)
+    paint(cells, side, side, solid, E.WALL)
+    fluid = ~solid & ~air
+    g = np.random.default_rng(seed)
+    vel = (g.normal(0.0, vel_sigma, size=(side, side)) * dt, g.normal(0.0, vel_sigma, size=(side, side)) * dt)
+    paint(cells, side, side, fluid, E.WATER, vel=vel)
+    return cells, int(fluid.sum())
