@@ -60,14 +60,16 @@ def step_bytes(totals_delta):
     return parts * 224.0 + ca * (65.0 + 16.0) + cyc * ca_mean * 93.0
 
 
-def cpu_baseline(cells, kinds, size, tol, steps):
-    """The oracle on the host: the same world, its first (seeding) step
-    untimed, then `steps` steps timed."""
+def cpu_baseline(cells, kinds, size, tol, steps, solver=None):
+    """The oracle on the host: the same world with the same solver settings,
+    its first (seeding) step untimed, then `steps` steps timed."""
     from oracle.oracle import OracleWorld
     from worldinagrain_amd import abi
     w = OracleWorld(size, size, kinds=kinds, dot_mode=1)
     w.upload(abi.PART_CELLS, cells)
     sim = config_simulation(w, size, tol)
+    if solver is not None:
+        sim.solver = solver
     sim.step()
     n = int(sim.last_stats[abi.STAT_PARTICLES])
     t0 = time.perf_counter()
@@ -146,10 +148,17 @@ def main():
         sim.step()
     # the coarse-level tile sides, by the autotuner on the world's own steps
     # (they do not change results)
-    tiles = {}
+    tiles, sweeps = {}, None
     if not args.no_tune:
-        from worldinagrain_amd.tune import tune_tiles
+        from worldinagrain_amd.tune import tune_tiles, tune_sweeps
         tiles = tune_tiles(world, sim)
+        # and the sweep counts, on copies of the world (they change the
+        # iteration, so each is scored by its solve time at the tolerance)
+        solver, _ = tune_sweeps(world, sim)
+        sim.solver = solver
+        sweeps = [solver.sweep_pairs, solver.level_pairs, solver.coarse_sweeps]
+        for _ in range(2):
+            sim.step()
     torch.cuda.synchronize()
 
     # each kernel's time over one step (untimed steps), for the record
@@ -217,7 +226,8 @@ def main():
                                f"to {args.tol:g} relative residual",
                    "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
                    "tol": args.tol, "parallelism": "single world, one GPU", "graphs": bool(args.graphs),
-                   "tiles": {str(k): v for k, v in tiles.items()}},
+                   "tiles": {str(k): v for k, v in tiles.items()},
+                   "sweeps": sweeps},
         "solves_per_s": solves, "pcg_iterations_per_solve": d[1] / K,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
         "step_roofline_frac": step_roof / HBM_PEAK_GBS, "step_achieved_GBs": step_roof,
@@ -228,7 +238,7 @@ def main():
     if per_kernel:
         out["kernel_ms_one_step"] = {k: round(v[0], 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:12]}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cells, kinds, size, args.tol, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cells, kinds, size, args.tol, args.cpu_steps, solver=sim.solver)
     print(json.dumps(out), flush=True)
 
 

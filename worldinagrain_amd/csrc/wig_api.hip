@@ -59,7 +59,6 @@ __global__ void k_rasterClaims(WorldGrid g);
 __global__ void k_raster(WorldGrid g);
 __global__ void k_unpaint(WorldGrid g);
 __global__ void k_stops(WorldGrid g, uint32_t *out);
-__global__ void k_cycle(uint32_t *cyc, uint32_t set, uint32_t v);
 __global__ void k64_start(WorldGrid g);
 __global__ void k64_sum(WorldGrid g, int which);
 __global__ void k64_apply(WorldGrid g);
@@ -189,7 +188,6 @@ struct Batch {
     std::vector<uint64_t> sig;   // the worlds and tile sides the maps were built for
     uint32_t *stops = nullptr, *stops_host = nullptr;   // k_stops' words (device, pinned host)
     uint32_t *reports = nullptr;
-    uint32_t *cyc = nullptr;     // the solve's iteration (device)
     hipStream_t stream = nullptr;
     // hipGraphs of the step's phases (run_phase),
     // captured for the launch layout `gsig` and replayed while it holds
@@ -207,7 +205,6 @@ static WorldGrid grid_of(const Batch &b, int sh) {
     g.tab = b.tab;
     g.map = b.maps + b.map_off[sh];
     g.first = b.maps + b.first_off[sh];
-    g.cyc = b.cyc;
     return g;
 }
 
@@ -291,7 +288,6 @@ static void batch_free(Batch *b) {
     if (b->stops) (void)hipFree(b->stops);
     if (b->stops_host) (void)hipHostFree(b->stops_host);
     if (b->reports) (void)hipFree(b->reports);
-    if (b->cyc) (void)hipFree(b->cyc);
     for (hipGraphExec_t &x : b->gx)
         if (x) (void)hipGraphExecDestroy(x);
     if (b->ev_made)
@@ -306,8 +302,7 @@ static Batch *batch_new(WigWorld *const *worlds, uint32_t n, hipStream_t stream)
     b->stream = stream;
     if (hipMalloc((void **)&b->tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&b->stops, 8ull * n) != hipSuccess
         || hipHostMalloc((void **)&b->stops_host, 8ull * n, hipHostMallocDefault) != hipSuccess
-        || hipMalloc((void **)&b->reports, 4ull * FLUID_STATS * n) != hipSuccess
-        || hipMalloc((void **)&b->cyc, 256) != hipSuccess) {
+        || hipMalloc((void **)&b->reports, 4ull * FLUID_STATS * n) != hipSuccess) {
         fail(WIG_E_NOMEM, "batch allocation failed");
         batch_free(b);
         return nullptr;
@@ -709,7 +704,6 @@ static void enqueue_solve_start(Batch &b, const StepPlan &pl) {
     if (pl.stores & 1u) enqueue_start_store<STORE_F32>(b);
     if (pl.stores & 2u) enqueue_start_store<STORE_F16>(b);
     if (pl.stores & 4u) enqueue_start_store<STORE_BF16>(b);
-    hipLaunchKernelGGL(k_cycle, dim3(1), dim3(64), 0, b.stream, b.cyc, 1u, 0u);
 }
 static void enqueue_iteration(Batch &b, const StepPlan &pl) {
     auto stores = [&](int part) {
@@ -723,7 +717,6 @@ static void enqueue_iteration(Batch &b, const StepPlan &pl) {
     launch(b, KID_cgAlpha, k_cgAlpha, SH_ONE, 1024);
     stores(2);
     launch(b, KID_cgCheck, k_cgCheck, SH_ONE, 1024);
-    hipLaunchKernelGGL(k_cycle, dim3(1), dim3(64), 0, b.stream, b.cyc, 0u, 0u);
 }
 static void enqueue_post(Batch &b) {
     launch(b, KID_project, k_project, SH_BG, 64);

@@ -107,6 +107,7 @@ struct SolveHead {
 #define WS_LAST 0
 #define WS_FIRST 1
 #define WS_STOP 2
+#define WS_CYCLE 3      // the solve's iteration: set by cgResidual, stepped by cgCheck
 #define WS_WORDS 16
 #define NO_STOP 0xffffffffu
 
@@ -173,7 +174,6 @@ struct WorldGrid {
     const Dev *__restrict__ tab;
     const uint32_t *__restrict__ map;
     const uint32_t *__restrict__ first;
-    const uint32_t *cyc;     // the solve's iteration, in device memory (so a captured iteration replays)
 };
 #define WORLD(g)                                                      \
     const uint32_t wi_ = (g).map[blockIdx.x];                         \

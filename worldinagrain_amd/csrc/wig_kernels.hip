@@ -1075,7 +1075,7 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
 template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     const int shift = ((d.shifted_first ^ (2u * cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
     if (converged(d, cycle)) return;
@@ -1161,7 +1161,7 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
 template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
     if (converged(d, cycle) || activeCount(d) == 0u) return;
@@ -1318,7 +1318,7 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t l) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     const uint32_t T = d.tile[l];
     const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
@@ -1400,7 +1400,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     const uint32_t T = d.tile[l];
     const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
@@ -1534,7 +1534,7 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
 template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     extern __shared__ float sm[];
@@ -1781,6 +1781,7 @@ __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
         uint32_t last = d.wstate[WS_LAST];
         d.wstate[WS_FIRST] = d.first_chunk == 0xffffffffu ? (last > 1u ? last : 1u) : d.first_chunk;
         d.wstate[WS_STOP] = NO_STOP;
+        d.wstate[WS_CYCLE] = 0u;
     }
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
@@ -1875,7 +1876,7 @@ template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) 
 template <int S> __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     FOR_ACTIVE_BLOCKS4(d, wid) {
@@ -1897,7 +1898,7 @@ template <int S> __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
 }
 __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
     WORLD(g);
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
@@ -1917,7 +1918,7 @@ __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
 template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     double beta = ld_f64(&d.head->beta);
@@ -1940,7 +1941,7 @@ template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
 }
 __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
     WORLD(g);
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     if (converged(d, cycle)) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
@@ -1960,7 +1961,7 @@ __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
 template <int S> __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(g.cyc);
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
     if (!encoded(d, cycle)) return;
     bool conv = converged(d, cycle);
     if (bid == 0 && threadIdx.x == 0) {
@@ -1994,11 +1995,19 @@ template <int S> __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) 
         }
     }
 }
+// (and the iteration counter stepped, the iteration's last pass: the
+// counter lives in device memory so a captured iteration replays alike)
+__device__ __forceinline__ void stepCycle(const Dev &d, uint32_t cycle) {
+    if (threadIdx.x == 0) __hip_atomic_store(&d.wstate[WS_CYCLE], cycle + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
     WORLD(g);
-    const uint32_t cycle = ld(g.cyc);
-    if (!encoded(d, cycle)) return;
-    if (converged(d, cycle)) return;
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
+    if (!d.full) return;
+    if (!encoded(d, cycle) || converged(d, cycle)) {
+        stepCycle(d, cycle);
+        return;
+    }
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
     double rr, unused;
@@ -2039,6 +2048,7 @@ __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
             __hip_atomic_store(&d.wstate[WS_STOP], stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+    stepCycle(d, cycle);
 }
 // cgFinish (:2884-2893), and the words of the reference's fine level the
 // packed solve kept aside: fluidCg's vectors, the rhs (-r), the drift, and,
@@ -2518,11 +2528,6 @@ __global__ void __launch_bounds__(64) k_unpaint(WorldGrid g) {                  
 }
 
 // ============================================================ batches
-// The solve's iteration counter: set at the solve's start, stepped after
-// each iteration (a one-thread launch, so captured iterations replay alike)
-__global__ void k_cycle(uint32_t *cyc, uint32_t set, uint32_t v) {
-    if (threadIdx.x == 0) *cyc = set ? v : *cyc + 1u;
-}
 // A convergence checkpoint of a batch: each world's stop (the iteration its
 // solve stops before, NO_STOP while it runs) and the iterations run.
 __global__ void __launch_bounds__(64) k_stops(WorldGrid g, uint32_t *out) {
