@@ -103,7 +103,7 @@ def main():
                     help="what the roofline prices: 'solve' (the MG-PCG solve, the byte model's unit of work) "
                          "or a particle kernel (gather, advect, sortCount, scatter, syncParticles)")
     ap.add_argument("--kernel-table", action="store_true", help="also time every kernel over one step each")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed on the host (about 1.6 s each at 1024x1024)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default coarse-level tile sides")
     ap.add_argument("--graphs", type=int, default=1, help="1: replay the step's phases as hipGraphs")
