@@ -148,9 +148,10 @@ def main():
         sim.step()
     # the coarse-level tile sides, by the autotuner on the world's own steps
     # (they do not change results)
-    tiles, sweeps = {}, None
+    tiles, sweeps, lds_level = {}, None, None
     if not args.no_tune:
-        from worldinagrain_amd.tune import tune_tiles, tune_sweeps
+        from worldinagrain_amd.tune import tune_tiles, tune_sweeps, tune_lds_level
+        lds_level = tune_lds_level(world, sim)
         tiles = tune_tiles(world, sim)
         # and the sweep counts, on copies of the world (they change the
         # iteration, so each is scored by its solve time at the tolerance)
@@ -227,7 +228,7 @@ def main():
                    "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
                    "tol": args.tol, "parallelism": "single world, one GPU", "graphs": bool(args.graphs),
                    "tiles": {str(k): v for k, v in tiles.items()},
-                   "sweeps": sweeps},
+                   "sweeps": sweeps, "lds_level": lds_level},
         "solves_per_s": solves, "pcg_iterations_per_solve": d[1] / K,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
         "step_roofline_frac": step_roof / HBM_PEAK_GBS, "step_achieved_GBs": step_roof,
@@ -320,6 +321,24 @@ def run_batch(args):
     torch.cuda.synchronize()
     total_ms = ev0.elapsed_time(ev1)
     t1 = [w.run_totals() for w in worlds]
+    # the solves' roofline over the batch: SURVEY's B_solve summed over the
+    # worlds (each world's C_a and PCG iterations from its run totals) over
+    # the batch's solve time (HIP events around cgStart..cgFinish of a
+    # separate set of steps: event timing is off in the timed ones)
+    batch.time_kernel("solve")
+    t2 = [w.run_totals() for w in worlds]
+    for _ in range(max(2, K // 4)):
+        step_batch(batch, sims, read_report=False)
+    solve_ms, solve_n = batch.kernel_time()
+    batch.time_kernel(None)
+    t3 = [w.run_totals() for w in worlds]
+    bsolve = 0.0
+    for a, b in zip(t2, t3):
+        st = b.steps - a.steps
+        if st:
+            ca = (b.active_cells - a.active_cells) / st
+            bsolve += (b.active_cells - a.active_cells) * 16.0 + (b.cycles - a.cycles) * ca * 93.0
+    solve_achieved = bsolve / solve_n / (solve_ms / solve_n / 1e3) / 1e9 if solve_n else None
     parts = sum(b.particles - a.particles for a, b in zip(t0, t1))
     cycles = sum(b.cycles - a.cycles for a, b in zip(t0, t1))
     steps = sum(b.steps - a.steps for a, b in zip(t0, t1))
@@ -356,6 +375,13 @@ def run_batch(args):
                        "note": "the same worlds stepped one at a time (4 of each side, 2 steps each), scaled to the batch"},
     }
     out["speedup_vs_sequential"] = out["value"] / out["sequential"]["solves_per_s"]
+    out["roofline"] = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": "solve",
+                       "unit_of_work": "one batch step's MG-PCG solves (cgStart..cgFinish over all worlds), "
+                                       "sum of each world's C_a*16 + K*C_a*93 bytes",
+                       "launches": solve_n, "mean_launch_ms": solve_ms / max(solve_n, 1),
+                       "algorithmic_bytes_per_launch": bsolve / max(solve_n, 1),
+                       "achieved": solve_achieved,
+                       "frac": solve_achieved / HBM_PEAK_GBS if solve_achieved else None}
     if per_kernel:
         out["kernel_ms_one_step"] = {k: [round(v[0], 4), v[1]] for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:16]}
     print(json.dumps(out), flush=True)
@@ -459,8 +485,11 @@ def run_serve(args):
     lat = np.array([r.done_at - r.arrival for r in reqs]) * 1e3
     world_steps = sum(s.steps_run for s in sessions)
     elapsed = server.elapsed
-    rounds = np.array([r[1] for r in server.rounds])
-    round_ms = np.diff([0.0] + [r[0] for r in server.rounds]) * 1e3
+    rounds = np.array([r[2] for r in server.rounds])
+    round_ms = np.array([(r[1] - r[0]) * 1e3 for r in server.rounds])
+    # a step's latency for the worlds of 512 x 512 cells and more: the
+    # rounds that stepped one, from the round's start to its end
+    big_ms = np.array([(r[1] - r[0]) * 1e3 for r in server.rounds if r[4] >= 512 * 512] or [0.0])
     # the plain single-world loop: the served steps one world at a time
     from worldinagrain_amd.gpu import GpuWorld
     from worldinagrain_amd.host import Simulation
@@ -491,6 +520,7 @@ def run_serve(args):
         "requests": len(reqs), "world_steps": world_steps, "world_steps_per_s": world_steps / elapsed,
         "elapsed_s": elapsed, "rounds": server.round_no, "worlds_per_round_mean": float(rounds.mean()),
         "p50_round_ms": float(np.percentile(round_ms, 50)), "p99_round_ms": float(np.percentile(round_ms, 99)),
+        "p50_step_ms_512": float(np.percentile(big_ms, 50)), "p99_step_ms_512": float(np.percentile(big_ms, 99)),
         "evictions": server.evictions, "admissions": server.admissions,
         "config": {"workload": f"{args.sessions} sessions, sides log-uniform on [64,1024] (multiples of 32), requests of "
                                f"1-8 steps, Poisson arrivals at {args.rate:g}/s for {args.duration:g} s, HBM budget "

@@ -200,6 +200,13 @@ int wig_kernel_count(void);
  * sweeps a whole level with 8 x 8 workgroups (fluid.wgsl:2317-2387). */
 int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side);
 
+/* The coarse level from which one workgroup solves the rest of the V-cycle
+ * in LDS (the levels above it are swept in tiles): from the first level
+ * LDS can hold with all below it (the default) down to the coarsest.
+ * Results do not depend on it either; the autotuner sets it.  This
+ * library's own. */
+int wig_set_lds_level(WigWorld *w, uint32_t level);
+
 /* Waits for everything enqueued on the world's stream. */
 int wig_sync(WigWorld *w);
 

@@ -77,17 +77,23 @@ def test_tiled_levels_reference_settings():
 
 def test_tile_sides_do_not_change_results():
     """The tiles' halos make their sweeps the whole-level passes': any tile
-    side gives the same bits; so does a clone of a world stepped alike."""
+    side, and any level for the one-workgroup kernel to start at, gives the
+    same bits; so does a clone of a world stepped alike."""
     W, H = 330, 250
     cells, _ = scenes.discs_dam(W, H, discs=40, rmin=3, rmax=12)
-    wa, wb = make_gpu(W, H, cells), make_gpu(W, H, cells)
-    sa, sb = config_sim(wa, W, H), config_sim(wb, W, H)
+    wa, wb, wl = make_gpu(W, H, cells), make_gpu(W, H, cells), make_gpu(W, H, cells)
+    sa, sb, sl = config_sim(wa, W, H), config_sim(wb, W, H), config_sim(wl, W, H)
     wb.set_tile(1, 8)
     wb.set_tile(2, 32)
+    wl.set_lds_level(wl.layout.levels)       # every level but the coarsest in tiles
+    wl.set_tile(3, 8)
+    with pytest.raises(Exception):
+        wl.set_lds_level(1)                  # levels 1 .. 4 do not fit one workgroup's LDS
     for s in range(3):
         sa.step()
         sb.step()
-    diffs = compare_worlds(wa, wb)
+        sl.step()
+    diffs = compare_worlds(wa, wb) + compare_worlds(wa, wl)
     assert not diffs, "; ".join(diffs)
     wc = wa.clone()
     import copy

@@ -19,6 +19,7 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v);
 __global__ void k_sync(WorldGrid g);
 __global__ void k_syncParticles(WorldGrid g);
 __global__ void k_syncSeed(WorldGrid g);
+__global__ void k_compactMarks(WorldGrid g);
 __global__ void k_compact(WorldGrid g);
 __global__ void k_retire(WorldGrid g);
 __global__ void k_clear(WorldGrid g);
@@ -88,7 +89,7 @@ static int fail(int code, const char *fmt, ...) {
     } while (0)
 
 // Kernel ids for wig_time_kernel, in launch order.
-#define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compact) X(retire) X(clear) X(gather) \
+#define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compactMarks) X(compact) X(retire) X(clear) X(gather) \
     X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
     X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
     X(cgRho) X(cgBeta) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
@@ -108,6 +109,7 @@ enum {
     SH_PG,       // 256-thread passes over the particles
     SH_WORLD,    // one workgroup a block, bw x bh
     SH_LVL1,     // 8 x 8 cells of coarse level 1 a workgroup
+    SH_B256,     // 256 blocks a workgroup
     SH_TILE,     // + l - 1: k_levelDown/k_levelUp's tiles on level l
     SH_COUNT = SH_TILE + MG_LEVELS
 };
@@ -145,6 +147,7 @@ static uint32_t shape_count(const Dev &d, int sh) {
     case SH_PG: return particle_grid(d);
     case SH_WORLD: return d.bw * d.bh;
     case SH_LVL1: return ceil_div(d.lw[1], 8u) * ceil_div(d.lh[1], 8u);
+    case SH_B256: return ceil_div(d.B, 256u);
     default: {
         uint32_t l = (uint32_t)(sh - SH_TILE) + 1u;
         if (l >= d.lds_level) return 0u;
@@ -168,6 +171,7 @@ struct WigWorld {
     unsigned long long *totals;  // WigRunTotals on the device
     uint32_t *wstate;            // WS_* words on the device (the end of the solve part)
     size_t lds_bytes;            // k_coarseLDS's dynamic LDS
+    uint32_t lds_first;          // the first coarse level LDS can hold with all below it
     uint32_t tile[MG_LEVELS + 1];   // tile side asked per level (wig_set_tile)
     uint32_t last_hint;          // iterations its last solve ran, as the host saw (when to check first)
     int device;
@@ -315,6 +319,7 @@ static int batch_maps(Batch &b) {
     std::vector<uint64_t> sig;
     for (WigWorld *w : b.w) {
         sig.push_back((uint64_t)(uintptr_t)w);
+        sig.push_back(w->d.lds_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(w->d.tile[l]);
     }
     if (b.maps && sig == b.sig) return WIG_OK;
@@ -398,6 +403,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         w->lds_bytes = bytes;
     }
     if (w->lds_bytes == 0) w->lds_bytes = 4ull * LDS_LEVEL_FLOATS * d.lw[d.L] * d.lh[d.L];
+    w->lds_first = d.lds_level;
     if (hipFuncSetAttribute((const void *)k_coarseLDS<STORE_F32>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
         || hipFuncSetAttribute((const void *)k_coarseLDS<STORE_F16>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
         || hipFuncSetAttribute((const void *)k_coarseLDS<STORE_BF16>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess
@@ -601,6 +607,17 @@ int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side) {
     return WIG_OK;
 }
 
+int wig_set_lds_level(WigWorld *w, uint32_t level) {
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    Dev &d = w->d;
+    if (level < w->lds_first || level > d.L)
+        return fail(WIG_E_INVALID, "level %u: the one-workgroup kernel can start at levels %u .. %u", level, w->lds_first, d.L);
+    d.lds_level = level;
+    w->lds_bytes = 0;
+    for (uint32_t k = level; k <= d.L; k++) w->lds_bytes += 4ull * LDS_LEVEL_FLOATS * d.lw[k] * d.lh[k];
+    return WIG_OK;
+}
+
 const char *wig_kernel_name(int id) { return id >= 0 && id < KID_COUNT ? g_kernel_names[id] : (id == KID_COUNT ? "solve" : ""); }
 int wig_kernel_count(void) { return KID_COUNT + 1; }
 
@@ -658,6 +675,7 @@ static void enqueue_head(Batch &b) {
     launch(b, KID_sync, k_sync, SH_WORLD, 64);
     launch(b, KID_syncParticles, k_syncParticles, SH_PG, 256);
     launch(b, KID_syncSeed, k_syncSeed, SH_WORLD, 64);
+    launch(b, KID_compactMarks, k_compactMarks, SH_B256, 256);
     launch(b, KID_compact, k_compact, SH_ONE, 1024);
     launch(b, KID_retire, k_retire, SH_BG, 64);
 }

@@ -449,6 +449,24 @@ __global__ void __launch_bounds__(64) k_syncSeed(WorldGrid g) {
 // compact (:896-921): the blocks worked on and those dropping out, listed in
 // block-index order by one workgroup (the reference's lists are in whatever
 // order its threads came).
+// compact's per-block flags (:896-921): each block's 3 x 3 neighbourhood of
+// marks, one thread a block
+__global__ void __launch_bounds__(256) k_compactMarks(WorldGrid g) {
+    WORLD(g);
+    uint32_t n = bid * 256u + threadIdx.x;
+    if (n >= d.B) return;
+    i2 b = I2((int32_t)(n % d.bw), (int32_t)(n / d.bw));
+    uint32_t marks = 0;
+    for (int y = -1; y <= 1; y++)
+        for (int x = -1; x <= 1; x++) {
+            i2 o = I2(b.x + x, b.y + y);
+            if (inBlocks(d, o)) marks |= d.blocks[blockIndex(d, o)];
+        }
+    bool nearl = (marks & MARK_LIQUID) != 0u, liquid = (marks & (MARK_LIQUID | MARK_REGION)) != 0u;
+    bool was = (d.blocks[STATE_SLOT(d, n)] & ACTIVE) != 0u;
+    d.scratch[n] = (liquid ? 1u : 0u) | (!liquid && was ? 2u : 0u) | (nearl ? 4u : 0u);
+}
+// compact's lists, in block-index order: one workgroup scans the flags
 __global__ void __launch_bounds__(1024) k_compact(WorldGrid g) {
     WORLD(g);
     __shared__ uint32_t sA[1024], sR[1024];
@@ -456,18 +474,9 @@ __global__ void __launch_bounds__(1024) k_compact(WorldGrid g) {
     uint32_t per = (d.B + T - 1u) / T, lo = t * per, hi = lo + per < d.B ? lo + per : d.B;
     uint32_t na = 0, nr = 0;
     for (uint32_t n = lo; n < hi; n++) {
-        i2 b = I2((int32_t)(n % d.bw), (int32_t)(n / d.bw));
-        uint32_t marks = 0;
-        for (int y = -1; y <= 1; y++)
-            for (int x = -1; x <= 1; x++) {
-                i2 o = I2(b.x + x, b.y + y);
-                if (inBlocks(d, o)) marks |= d.blocks[blockIndex(d, o)];
-            }
-        bool nearl = (marks & MARK_LIQUID) != 0u, liquid = (marks & (MARK_LIQUID | MARK_REGION)) != 0u;
-        bool was = (d.blocks[STATE_SLOT(d, n)] & ACTIVE) != 0u;
-        d.scratch[n] = (liquid ? 1u : 0u) | (!liquid && was ? 2u : 0u) | (nearl ? 4u : 0u);
-        na += liquid ? 1u : 0u;
-        nr += (!liquid && was) ? 1u : 0u;
+        uint32_t f = d.scratch[n];
+        na += f & 1u;
+        nr += (f >> 1) & 1u;
     }
     sA[t] = na;
     sR[t] = nr;
@@ -1099,6 +1108,28 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
             live = (bits & HAS_FLUID) != 0u;
         }
         if (!live) continue;
+        // this thread's own cell first (its loads overlap the tile's)
+        i2 me = I2(origin.x + (int32_t)(li % 8u), origin.y + (int32_t)(li / 8u));
+        int t = (int)(li / 8u + 1u) * RIM + (int)(li % 8u) + 1;
+        float rhsx = 0.0f, rhsy = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+        bool fluid = false, pinned = false;
+        uint32_t mi = 0;
+        if (inWorld(d, me)) {
+            mi = cellIndex(d, me);
+            uint32_t f = d.cflag[mi];
+            float4 cf = d.coef[mi];
+            float2 r = fnLoad<S>(d, 2, mi);
+            fluid = (f & CF_FLUID) != 0u;
+            if (fluid) {
+                b0 = fabsf(cf.x);
+                b1 = fabsf(cf.y);
+                b2 = fabsf(cf.z);
+                b3 = fabsf(cf.w);
+                rhsx = r.x;
+                rhsy = r.y;
+                pinned = (f & CF_PINNED) != 0u;
+            }
+        }
         for (uint32_t k = li; k < RIM * RIM; k += 64u) {
             i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
             float2 v = make_float2(0.0f, 0.0f);
@@ -1115,27 +1146,6 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
             sPy[k] = v.y;
         }
         __syncthreads();
-        i2 me = I2(origin.x + (int32_t)(li % 8u), origin.y + (int32_t)(li / 8u));
-        int t = (int)(li / 8u + 1u) * RIM + (int)(li % 8u) + 1;
-        float rhsx = 0.0f, rhsy = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
-        bool fluid = false, pinned = false;
-        uint32_t mi = 0;
-        if (inWorld(d, me)) {
-            mi = cellIndex(d, me);
-            uint32_t f = d.cflag[mi];
-            fluid = (f & CF_FLUID) != 0u;
-            if (fluid) {
-                float4 cf = d.coef[mi];
-                float2 r = fnLoad<S>(d, 2, mi);
-                b0 = fabsf(cf.x);
-                b1 = fabsf(cf.y);
-                b2 = fabsf(cf.z);
-                b3 = fabsf(cf.w);
-                rhsx = r.x;
-                rhsy = r.y;
-                pinned = (f & CF_PINNED) != 0u;
-            }
-        }
         float diag = b0 + b1 + b2 + b3;
         float dx = diag + (pinned ? PINF : 0.0f), dy = fmaxf(diag, 1e-30f);
         bool solving = dx > 0.0f;
@@ -1158,6 +1168,21 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
 
 // restrictLevel into level 1 (:2342-2377), each iteration: rhs from the
 // fluid children's residuals; rho as k_levelRho put it at the solve's start
+// restrictLevel into level 1's cell c (rho > 0 there): the sums of its
+// fluid children's residuals, a quarter of each
+template <int S> __device__ __forceinline__ v2 restrictFine(const Dev &d, i2 c) {
+    float sx = 0.0f, sy = 0.0f;
+    for (int q = 0; q < 4; q++) {
+        i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
+        if (!inWorld(d, k)) continue;
+        uint32_t ki = cellIndex(d, k), f = d.cflag[ki];
+        if (!(f & CF_FLUID)) continue;
+        v2 e = fineResidual<S>(d, k, ki, f);
+        sx += e.x;
+        sy += e.y;
+    }
+    return V2(sx * 0.25f, sy * 0.25f);
+}
 template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
@@ -1168,19 +1193,8 @@ template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     i2 c = I2((int32_t)(bx * 8u + threadIdx.x % 8u), (int32_t)(by * 8u + threadIdx.x / 8u));
     if (!inLevel(d, 1u, c)) return;
     float rho = lvRho<S>(d, 1u, c);
-    float sx = 0.0f, sy = 0.0f;
-    if (rho > 0.0f) {
-        for (int q = 0; q < 4; q++) {
-            i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
-            if (!inWorld(d, k)) continue;
-            uint32_t ki = cellIndex(d, k), f = d.cflag[ki];
-            if (!(f & CF_FLUID)) continue;
-            v2 e = fineResidual<S>(d, k, ki, f);
-            sx += e.x;
-            sy += e.y;
-        }
-    }
-    lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, rho > 0.0f ? sx * 0.25f : 0.0f, rho > 0.0f ? sy * 0.25f : 0.0f));
+    v2 r = rho > 0.0f ? restrictFine<S>(d, c) : V2(0.0f, 0.0f);
+    lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, r.x, r.y));
 }
 // level 1's rho (restrictLevel's, :2342-2377), once a solve: it depends on
 // the fine rho alone
@@ -1333,8 +1347,8 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
         float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float rho = -1.0f;
         if (inLevel(d, l, c)) {
-            e = lvLoad<S>(d, l, c);
             rho = lvRho<S>(d, l, c);
+            e = lvLoad<S>(d, l, c);
         }
         t.px[j] = e.x;
         t.py[j] = e.y;

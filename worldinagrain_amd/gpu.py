@@ -46,6 +46,7 @@ def lib():
         L.wig_kernel_name.argtypes = [ct.c_int]
         L.wig_kernel_count.restype = ct.c_int
         L.wig_set_tile.argtypes = [ct.c_void_p, ct.c_uint32, ct.c_uint32]
+        L.wig_set_lds_level.argtypes = [ct.c_void_p, ct.c_uint32]
         L.wig_batch_create.restype = ct.c_void_p
         L.wig_batch_create.argtypes = [ct.POINTER(ct.c_void_p), ct.c_uint32]
         L.wig_batch_destroy.argtypes = [ct.c_void_p]
@@ -166,6 +167,10 @@ class GpuWorld:
     def set_graphs(self, on):
         """Steps replayed as hipGraphs (wig_set_graphs)."""
         _check(lib().wig_set_graphs(self._w, 1 if on else 0))
+
+    def set_lds_level(self, level):
+        """Where the one-workgroup coarse-level kernel starts (wig_set_lds_level)."""
+        _check(lib().wig_set_lds_level(self._w, level))
 
     def capture(self, on=True):
         """Each later solve keeps its right-hand side, for cg64 (wig_capture)."""

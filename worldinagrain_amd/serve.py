@@ -102,7 +102,7 @@ class Server:
         self.evictions = 0
         self.admissions = 0
         self.particle_steps = 0
-        self.rounds = []
+        self.rounds = []           # (start, end, worlds, cells, largest world's cells) a round
 
     def _admit(self, s, busy):
         """Makes session s resident within the budget, evicting idle worlds
@@ -142,6 +142,7 @@ class Server:
             if not queue:
                 time.sleep(max(0.0, min(reqs[nxt].arrival - now, 0.002)))
                 continue
+            start = clock() - t0
             # this round: the queue's requests in order, one per session,
             # while their worlds fit the round's cells and the budget
             chosen, busy, cells = [], set(), 0
@@ -175,7 +176,7 @@ class Server:
                 s.last_used = self.round_no
                 s.steps_run += 1
                 self.particle_steps += int(s.sim.last_stats[abi.STAT_PARTICLES])
-            self.rounds.append((end, len(chosen), cells))
+            self.rounds.append((start, end, len(chosen), cells, max(s.width * s.height for s in sess)))
             for r in chosen:
                 r.left -= 1
                 if r.left == 0:

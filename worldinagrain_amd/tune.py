@@ -13,7 +13,7 @@ import dataclasses
 from .gpu import WigError
 
 TILE_SIDES = (8, 16, 32)
-SWEEP_CANDIDATES = [(sp, lp, cs) for sp in (1, 2) for lp in (1, 2) for cs in (5, 10)]
+SWEEP_CANDIDATES = [(sp, lp, cs) for sp in (1, 2) for lp in (1, 2) for cs in (1, 2, 5)]
 
 
 def _kernel_ms_per_iteration(world, sim, name, steps):
@@ -47,6 +47,27 @@ def tune_tiles(world, sim, sides=TILE_SIDES, steps=2):
         chosen[level] = best[1]
         level += 1
     return chosen
+
+
+def tune_lds_level(world, sim, steps=2):
+    """Where the one-workgroup coarse-level kernel starts: each level from
+    the first LDS holds down to the coarsest, by the solve's time an
+    iteration over `steps` of the world's steps (results do not depend on
+    it).  Returns the level."""
+    best = None
+    level = None
+    for cand in range(1, world.layout.levels + 1):
+        try:
+            world.set_lds_level(cand)
+        except WigError:
+            continue
+        ms = _kernel_ms_per_iteration(world, sim, "solve", steps)
+        if best is None or ms < best[0]:
+            best = (ms, cand)
+    if best is not None:
+        world.set_lds_level(best[1])
+        level = best[1]
+    return level
 
 
 def tune_tiles_batch(batch, sims, groups, sides=TILE_SIDES, steps=1):
