@@ -197,6 +197,7 @@ struct Batch {
     // captured for the launch layout `gsig` and replayed while it holds
     bool graphs = false;
     hipGraphExec_t gx[6] = {};
+    Batch *tail = nullptr;       // the worlds still iterating, once few are (batch_step)
     std::vector<uint64_t> gsig;
     int timed = -1;              // kernel id timed, -1 none
     uint32_t ntimed = 0;
@@ -286,6 +287,7 @@ static int zero(hipStream_t s, void *p, uint64_t bytes) {
 
 static void batch_free(Batch *b) {
     if (!b) return;
+    if (b->tail) batch_free(b->tail);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->tab) (void)hipFree(b->tab);
     if (b->maps) (void)hipFree(b->maps);
@@ -788,6 +790,52 @@ static int run_phase(Batch &b, int ph, const StepPlan &pl) {
     return WIG_OK;
 }
 
+static int batch_assign(Batch *b, WigWorld *const *worlds, uint32_t n) {
+    if (!b || !worlds || n == 0u) return fail(WIG_E_INVALID, "a batch needs worlds");
+    for (uint32_t i = 0; i < n; i++)
+        if (!worlds[i] || worlds[i]->device != b->w[0]->device) return fail(WIG_E_INVALID, "world %u is null or on another device", i);
+    if (n > b->htab.size()) {
+        Dev *tab = nullptr;
+        uint32_t *stops = nullptr, *stops_host = nullptr, *reports = nullptr;
+        if (hipMalloc((void **)&tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&stops, 8ull * n) != hipSuccess
+            || hipHostMalloc((void **)&stops_host, 8ull * n, hipHostMallocDefault) != hipSuccess
+            || hipMalloc((void **)&reports, 4ull * FLUID_STATS * n) != hipSuccess)
+            return fail(WIG_E_NOMEM, "batch allocation failed");
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        HIPCHECK(hipFree(b->tab));
+        HIPCHECK(hipFree(b->stops));
+        HIPCHECK(hipHostFree(b->stops_host));
+        HIPCHECK(hipFree(b->reports));
+        b->tab = tab;
+        b->stops = stops;
+        b->stops_host = stops_host;
+        b->reports = reports;
+    }
+    b->w.assign(worlds, worlds + n);
+    b->htab.resize(std::max<size_t>(b->htab.size(), n));
+    b->sig.clear();          // the maps are built again at the next step
+    return WIG_OK;
+}
+
+// The plan of a batch's iterations: which storages, levels and LDS sizes.
+static void plan_iterations(const Batch &b, StepPlan &pl) {
+    pl.stores = 0;
+    pl.max_level = 0;
+    pl.coarse_lds = 0;
+    for (uint32_t l = 0; l <= MG_LEVELS; l++) pl.down_lds[l] = pl.up_lds[l] = 0;
+    for (uint32_t i = 0; i < (uint32_t)b.w.size(); i++) {
+        const Dev &d = b.htab[i];
+        if (!d.full) continue;
+        pl.stores |= 1u << d.store;
+        pl.coarse_lds = std::max(pl.coarse_lds, b.w[i]->lds_bytes);
+        for (uint32_t l = 1; l < d.lds_level; l++) {
+            pl.max_level = std::max(pl.max_level, l);
+            pl.down_lds[l] = std::max(pl.down_lds[l], downLds(d.level_pairs, d.tile[l]));
+            pl.up_lds[l] = std::max(pl.up_lds[l], upLds(d.level_pairs, d.tile[l]));
+        }
+    }
+}
+
 // One liquid step of every world of the batch (encodeFluid, js/sim.js:2466-2612).
 static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
     uint32_t n = (uint32_t)b.w.size();
@@ -846,25 +894,47 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
     // iterations: each world runs the ones its own host would have encoded
     // (encoded() in the kernels); the batch checks at its checkpoints
     // whether every world has stopped
+    // Once a checkpoint finds a quarter of the worlds or fewer still
+    // iterating, the rest of the iterations run on a batch of those alone
+    // (`cur`), so the finished ones' workgroups are not launched to exit.
     uint32_t next_check = pl.first_check;
+    Batch *cur = &b;
+    StepPlan cpl = pl;
     for (uint32_t cycle = 0; cycle < pl.max_cycles; cycle++) {
         if (cycle == next_check) {
-            hipLaunchKernelGGL(k_stops, dim3(n), dim3(64), 0, s, grid_of(b, SH_ONE), b.stops);
-            HIPCHECK(hipMemcpyAsync(b.stops_host, b.stops, 8ull * n, hipMemcpyDeviceToHost, s));
+            uint32_t m = (uint32_t)cur->w.size();
+            hipLaunchKernelGGL(k_stops, dim3(m), dim3(64), 0, s, grid_of(*cur, SH_ONE), cur->stops);
+            HIPCHECK(hipMemcpyAsync(cur->stops_host, cur->stops, 8ull * m, hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
-            bool done = true;
-            for (uint32_t i = 0; i < n && done; i++) {
-                const Dev &d = b.htab[i];
-                done = !d.full || d.fluid_cycles <= cycle || b.stops_host[2 * i] <= cycle;
+            std::vector<uint32_t> going;
+            for (uint32_t i = 0; i < m; i++) {
+                const Dev &d = cur->htab[i];
+                bool done = !d.full || d.fluid_cycles <= cycle || cur->stops_host[2 * i] <= cycle;
+                if (done && d.full && cur->stops_host[2 * i] < d.fluid_cycles) cur->w[i]->last_hint = cur->stops_host[2 * i + 1];
+                if (!done) going.push_back(i);
             }
-            if (done) {
-                for (uint32_t i = 0; i < n; i++)
-                    if (b.htab[i].full && b.stops_host[2 * i] < b.htab[i].fluid_cycles) b.w[i]->last_hint = b.stops_host[2 * i + 1];
-                break;
+            if (going.empty()) break;
+            if (cur == &b && b.timed < 0 && 4u * going.size() <= m && going.size() < m) {
+                std::vector<WigWorld *> ws;
+                for (uint32_t i : going) ws.push_back(b.w[i]);
+                if (!b.tail) {
+                    b.tail = batch_new(ws.data(), (uint32_t)ws.size(), s);
+                    if (!b.tail) return WIG_E_NOMEM;
+                } else if ((st = batch_assign(b.tail, ws.data(), (uint32_t)ws.size())) < 0) {
+                    return st;
+                }
+                Batch &t = *b.tail;
+                t.stream = s;
+                t.graphs = false;
+                for (uint32_t k = 0; k < (uint32_t)going.size(); k++) t.htab[k] = b.htab[going[k]];
+                if ((st = batch_maps(t)) < 0) return st;
+                HIPCHECK(hipMemcpyAsync(t.tab, t.htab.data(), sizeof(Dev) * going.size(), hipMemcpyHostToDevice, s));
+                plan_iterations(t, cpl);
+                cur = &t;
             }
             next_check = cycle + pl.chunk;
         }
-        if ((st = run_phase(b, PH_ITER, pl)) < 0) return st;
+        if ((st = run_phase(*cur, PH_ITER, cpl)) < 0) return st;
     }
     if ((st = run_phase(b, PH_FINISH, pl)) < 0) return st;
     if (timeSolve) {
@@ -894,31 +964,7 @@ WigBatch *wig_batch_create(WigWorld *const *worlds, uint32_t n) {
 }
 
 int wig_batch_assign(WigBatch *bb, WigWorld *const *worlds, uint32_t n) {
-    Batch *b = (Batch *)bb;
-    if (!b || !worlds || n == 0u) return fail(WIG_E_INVALID, "a batch needs worlds");
-    for (uint32_t i = 0; i < n; i++)
-        if (!worlds[i] || worlds[i]->device != b->w[0]->device) return fail(WIG_E_INVALID, "world %u is null or on another device", i);
-    if (n > b->htab.size()) {
-        Dev *tab = nullptr;
-        uint32_t *stops = nullptr, *stops_host = nullptr, *reports = nullptr;
-        if (hipMalloc((void **)&tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&stops, 8ull * n) != hipSuccess
-            || hipHostMalloc((void **)&stops_host, 8ull * n, hipHostMallocDefault) != hipSuccess
-            || hipMalloc((void **)&reports, 4ull * FLUID_STATS * n) != hipSuccess)
-            return fail(WIG_E_NOMEM, "batch allocation failed");
-        HIPCHECK(hipStreamSynchronize(b->stream));
-        HIPCHECK(hipFree(b->tab));
-        HIPCHECK(hipFree(b->stops));
-        HIPCHECK(hipHostFree(b->stops_host));
-        HIPCHECK(hipFree(b->reports));
-        b->tab = tab;
-        b->stops = stops;
-        b->stops_host = stops_host;
-        b->reports = reports;
-    }
-    b->w.assign(worlds, worlds + n);
-    b->htab.resize(std::max<size_t>(b->htab.size(), n));
-    b->sig.clear();          // the maps are built again at the next step
-    return WIG_OK;
+    return batch_assign((Batch *)bb, worlds, n);
 }
 
 int wig_batch_destroy(WigBatch *b) {
