@@ -296,13 +296,15 @@ def run_batch(args):
         groups = {side: [i for i, s in enumerate(sides) if s == side] for side in sorted(set(sides))}
         for side, tiles in tune_tiles_batch(batch, sims, groups).items():
             tuned[side]["tiles"] = tiles
-    step_batch(batch, sims)
+    from worldinagrain_amd.host import BatchHost
+    host = BatchHost(sims)          # the per-step host words for all worlds at once
+    host.step(batch)
     per_kernel = {}
     if args.kernel_table:
         from worldinagrain_amd.gpu import kernel_names
         for name in kernel_names():
             batch.time_kernel(name)
-            step_batch(batch, sims, read_report=False)
+            host.step(batch, read_report=False)
             per_kernel[name] = batch.kernel_time()
         batch.time_kernel(None)
     torch.cuda.synchronize()
@@ -314,7 +316,7 @@ def run_batch(args):
     for _ in range(K):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        step_batch(batch, sims, read_report=False)
+        host.step(batch, read_report=False)
         b.record(stream)
         lat.append((a, b))
     ev1.record(stream)
@@ -328,7 +330,7 @@ def run_batch(args):
     batch.time_kernel("solve")
     t2 = [w.run_totals() for w in worlds]
     for _ in range(max(2, K // 4)):
-        step_batch(batch, sims, read_report=False)
+        host.step(batch, read_report=False)
     solve_ms, solve_n = batch.kernel_time()
     batch.time_kernel(None)
     t3 = [w.run_totals() for w in worlds]

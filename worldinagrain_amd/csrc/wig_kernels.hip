@@ -1089,6 +1089,7 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
     const int shift = ((d.shifted_first ^ (2u * cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
     if (converged(d, cycle)) return;
     __shared__ float sPx[RIM * RIM], sPy[RIM * RIM];
+    __shared__ float cRho[64], cPx[64], cPy[64];
     uint32_t li = threadIdx.x;
     uint32_t na = activeCount(d), edge = shift ? d.bw + d.bh + 1u : 0u;
     // the tiles: the active list's blocks, then (shifted) the edge tiles,
@@ -1130,6 +1131,23 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
                 pinned = (f & CF_PINNED) != 0u;
             }
         }
+        // after the coarse levels: level 1 under the tile (8 x 8 cells from
+        // one before the rim's first), for prolongFine's correction
+        const int32_t cx0 = ((origin.x - 1) >> 1) - 1, cy0 = ((origin.y - 1) >> 1) - 1;
+        if (post) {
+            i2 cc = I2(cx0 + (int32_t)(li % 8u), cy0 + (int32_t)(li / 8u));
+            float r = -1.0f, px = 0.0f, py = 0.0f;
+            if (inLevel(d, 1u, cc)) {
+                r = lvRho<S>(d, 1u, cc);
+                float2 p = lvLoadP<S>(d, 1u, cc);
+                px = p.x;
+                py = p.y;
+            }
+            cRho[li] = r;
+            cPx[li] = px;
+            cPy[li] = py;
+            __syncthreads();
+        }
         for (uint32_t k = li; k < RIM * RIM; k += 64u) {
             i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
             float2 v = make_float2(0.0f, 0.0f);
@@ -1137,9 +1155,25 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
                 uint32_t ci = cellIndex(d, c);
                 v = fnLoad<S>(d, post ? 1 : 0, ci);
                 if (post && (d.cflag[ci] & CF_FLUID)) {
-                    v2 e = prolonged<S>(d, 0u, c);
-                    v.x = v.x + e.x;
-                    v.y = v.y + e.y;
+                    // prolonged() (:2458-2497) from the staged cells
+                    i2 coarse = I2(c.x / 2, c.y / 2);
+                    i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
+                    float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+                    for (int q = 0; q < 4; q++) {
+                        i2 o = q == 0 ? I2(0, 0) : (q == 1 ? I2(sg.x, 0) : (q == 2 ? I2(0, sg.y) : sg));
+                        float wk = q == 0 ? 9.0f : (q == 3 ? 1.0f : 3.0f);
+                        int32_t j = (coarse.y + o.y - cy0) * 8 + (coarse.x + o.x - cx0);
+                        float r = cRho[j];
+                        if (r < 0.0f) continue;
+                        weight += wk;
+                        if (r > 0.0f) {
+                            sx += wk * cPx[j];
+                            sy += wk * cPy[j];
+                        }
+                    }
+                    float dn = fmaxf(weight, 1e-6f);
+                    v.x = v.x + sx / dn;
+                    v.y = v.y + sy / dn;
                 }
             }
             sPx[k] = v.x;

@@ -244,6 +244,10 @@ class GpuBatch:
         C = (abi.WigStepCtl * n)(*ctls)
         return _check(lib().wig_batch_step(self._b, P, C))
 
+    def step_arrays(self, P, C):
+        """wig_batch_step over ready ctypes arrays of WigParams and WigStepCtl."""
+        return _check(lib().wig_batch_step(self._b, P, C))
+
     def report(self):
         n = len(self.worlds)
         out = (ct.c_uint32 * (abi.STATS * n))()

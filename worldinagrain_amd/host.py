@@ -157,3 +157,80 @@ def step_batch(batch, sims, read_report=True):
         for s, row in zip(sims, batch.report()):
             s.take_report(row)
     return status
+
+
+def mulberry32_many(state):
+    """mulberry32_next over an array of uint32 states: (new states, u32s)."""
+    st = (state.astype(np.uint64) + 0x6D2B79F5) & 0xFFFFFFFF
+    t = st.copy()
+    t = ((t ^ (t >> 15)) * (t | 1)) & 0xFFFFFFFF
+    t ^= (t + (((t ^ (t >> 7)) * (t | 61)) & 0xFFFFFFFF)) & 0xFFFFFFFF
+    return st.astype(np.uint32), ((t ^ (t >> 14)) & 0xFFFFFFFF).astype(np.uint32)
+
+
+class BatchHost:
+    """The host side of many Simulations stepped as one batch: each step's
+    Params and controls for all of them at once (numpy), the same words each
+    Simulation.step would write, and their reports taken in after.  The
+    simulations' own state (seed, frame, tiling flip, report) is kept up to
+    date, so they can be stepped alone again at any time."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        n = len(self.sims)
+        self.P = (abi.WigParams * n)()
+        self.C = (abi.WigStepCtl * n)()
+        for i, s in enumerate(self.sims):
+            p = self.P[i]
+            p.size[0], p.size[1] = s.width, s.height
+            p.dx, p.earthG, p.dt = s.dx, s.earth_g, s.dt
+            c = self.C[i]
+            sv = s.solver
+            c.tol_mode, c.tol = sv.tol_mode, sv.tol
+            c.pre_relax, c.post_relax = sv.pre_relax, sv.post_relax
+            c.sweep_pairs, c.coarse_sweeps = sv.sweep_pairs, sv.coarse_sweeps
+            c.level_pairs = sv.level_pairs
+            c.vcycle_storage = sv.storage
+            if sv.first_chunk < 0:
+                c.first_chunk, c.chunk = 0xFFFFFFFF, sv.chunk
+            else:
+                c.first_chunk, c.chunk = sv.first_chunk, sv.chunk
+        self.pw = np.frombuffer(self.P, dtype=np.uint32).reshape(n, -1)
+        self.cw = np.frombuffer(self.C, dtype=np.uint32).reshape(n, -1)
+        self._frame = abi.WigParams.frame.offset // 4
+        self._seed = abi.WigParams.seed.offset // 4
+        self._full = abi.WigStepCtl.full.offset // 4
+        self._cycles = abi.WigStepCtl.fluid_cycles.offset // 4
+        self._shift = abi.WigStepCtl.shifted_first.offset // 4
+        self.adaptive = np.array([s.solver.adaptive_cycles for s in self.sims])
+        self.cap = np.array([s.solver.max_cycles for s in self.sims], dtype=np.uint32)
+
+    def step(self, batch, read_report=True):
+        sims = self.sims
+        rng = np.array([s.rng_state for s in sims], dtype=np.uint32)
+        rng, seeds = mulberry32_many(rng)
+        shifted = np.array([not s.shifted_first for s in sims])
+        pw, cw = self.pw, self.cw
+        pw[:, self._frame] = np.array([s.frame_no & 0xFFFFFFFF for s in sims], dtype=np.uint32)
+        pw[:, self._seed] = seeds
+        cw[:, self._full] = np.array([1 if s.fluid_stats["active"] else 0 for s in sims], dtype=np.uint32)
+        cyc = np.array([s.fluid_cycles for s in sims], dtype=np.uint32)
+        cw[:, self._cycles] = np.where(self.adaptive, cyc, self.cap)
+        cw[:, self._shift] = shifted.astype(np.uint32)
+        status = batch.step_arrays(self.P, self.C)
+        for s, r, sh in zip(sims, rng.tolist(), shifted.tolist()):
+            s.rng_state = r
+            s.shifted_first = sh
+            s.frame_no += 1
+        if read_report:
+            rows = batch.report()
+            active = rows[:, abi.STAT_ACTIVE].astype(np.int64) + rows[:, abi.STAT_PARTICLES]
+            most = rows[:, abi.STAT_MOST_CYCLES]
+            cap = self.cap
+            new = np.where(most >= cyc, cap, np.minimum(most + FLUID_CYCLES_SPARE, cap))
+            for i, s in enumerate(sims):
+                s.last_stats = rows[i]
+                s.fluid_stats = {"active": int(active[i]), "viscous": int(rows[i, abi.STAT_VISCOUS])}
+                s.last_cycles = int(rows[i, abi.STAT_CYCLES])
+                s.fluid_cycles = int(new[i])
+        return status

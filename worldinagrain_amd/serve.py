@@ -25,7 +25,7 @@ import numpy as np
 
 from . import abi
 from .gpu import GpuWorld, GpuBatch
-from .host import Simulation, step_batch
+from .host import Simulation, BatchHost
 
 
 def world_bytes(width, height, coarse_levels=0):
@@ -169,7 +169,7 @@ class Server:
                     self.batch.set_stream(self.stream)
             else:
                 self.batch.assign(worlds)
-            step_batch(self.batch, [s.sim for s in sess])
+            BatchHost([s.sim for s in sess]).step(self.batch)
             end = clock() - t0
             self.round_no += 1
             for s in sess:
