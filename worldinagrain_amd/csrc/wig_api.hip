@@ -110,6 +110,7 @@ enum {
     SH_WORLD,    // one workgroup a block, bw x bh
     SH_LVL1,     // 8 x 8 cells of coarse level 1 a workgroup
     SH_B256,     // 256 blocks a workgroup
+    SH_BG16,     // 256-thread passes, sixteen blocks a workgroup (four a wavefront)
     SH_TILE,     // + l - 1: k_levelDown/k_levelUp's tiles on level l
     SH_COUNT = SH_TILE + MG_LEVELS
 };
@@ -148,6 +149,7 @@ static uint32_t shape_count(const Dev &d, int sh) {
     case SH_WORLD: return d.bw * d.bh;
     case SH_LVL1: return ceil_div(d.lw[1], 8u) * ceil_div(d.lh[1], 8u);
     case SH_B256: return ceil_div(d.B, 256u);
+    case SH_BG16: return std::min<uint32_t>((d.B + 15u) / 16u, 1024u);
     default: {
         uint32_t l = (uint32_t)(sh - SH_TILE) + 1u;
         if (l >= d.lds_level) return 0u;
@@ -328,8 +330,20 @@ static int batch_maps(Batch &b) {
     uint32_t n = (uint32_t)b.w.size();
     std::vector<uint32_t> words;
     for (int sh = 0; sh < SH_COUNT; sh++) {
+        std::vector<uint32_t> count(n);
+        uint64_t natural = 0;
+        for (uint32_t i = 0; i < n; i++) natural += count[i] = shape_count(b.w[i]->d, sh);
+        // The particle passes' and the CG's four-block passes' grids over a
+        // batch: each world's own grid, scaled down together past a few
+        // rounds of what the chip holds at once (workgroups that only find
+        // their world's range ended still cost a dispatch each).  The
+        // 64-thread passes keep theirs: more of them in flight hides more
+        // of their latency.
+        const uint64_t budget = 8192u;
+        if ((sh == SH_BG16 || sh == SH_PG) && natural > budget)
+            for (uint32_t i = 0; i < n; i++) count[i] = std::max<uint32_t>(1u, (uint32_t)(count[i] * budget / natural));
         std::vector<uint32_t> first(n + 1u, 0u);
-        for (uint32_t i = 0; i < n; i++) first[i + 1u] = first[i] + shape_count(b.w[i]->d, sh);
+        for (uint32_t i = 0; i < n; i++) first[i + 1u] = first[i] + count[i];
         b.total[sh] = first[n];
         b.map_off[sh] = words.size();
         for (uint32_t i = 0; i < n; i++) words.insert(words.end(), first[i + 1u] - first[i], i);
@@ -375,7 +389,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         }
     }
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
-        || hipMalloc(&w->fine, 73ull * w->d.C + 256u) != hipSuccess
+        || hipMalloc(&w->fine, 73ull * 64ull * w->d.B + 256u) != hipSuccess
         || hipMalloc(&w->vlevels, w->vlevel_bytes) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
@@ -385,7 +399,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     for (int p = 0; p < WIG_PART_COUNT; p++)
         if (zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
     zero(w->own, w->kind, 256);
-    zero(w->own, w->fine, 73ull * w->d.C + 256u);
+    zero(w->own, w->fine, 73ull * 64ull * w->d.B + 256u);
     zero(w->own, w->vlevels, w->vlevel_bytes);
     zero(w->own, w->totals, sizeof(WigRunTotals));
     Dev &d = w->d;
@@ -428,20 +442,22 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     d.kind = w->kind;
     d.scratch = w->scratch;
     {
+        // (block-major, 64 cells a block: packIndex)
         uint8_t *f = (uint8_t *)w->fine;
+        const uint64_t P = 64ull * d.B;
         d.coef = (float4 *)f;
-        d.pa = (float2 *)(f + 16ull * d.C);
-        d.pb = (float2 *)(f + 24ull * d.C);
-        d.r2 = (float2 *)(f + 32ull * d.C);
-        d.cx = (float *)(f + 40ull * d.C);
-        d.cr = (float *)(f + 44ull * d.C);
-        d.cp0 = (float *)(f + 48ull * d.C);
-        d.cp1 = (float *)(f + 52ull * d.C);
-        d.cq = (float *)(f + 56ull * d.C);
-        d.ha = (uint32_t *)(f + 60ull * d.C);
-        d.hb = (uint32_t *)(f + 64ull * d.C);
-        d.hr2 = (uint32_t *)(f + 68ull * d.C);
-        d.cflag = f + 72ull * d.C;
+        d.pa = (float2 *)(f + 16ull * P);
+        d.pb = (float2 *)(f + 24ull * P);
+        d.r2 = (float2 *)(f + 32ull * P);
+        d.cx = (float *)(f + 40ull * P);
+        d.cr = (float *)(f + 44ull * P);
+        d.cp0 = (float *)(f + 48ull * P);
+        d.cp1 = (float *)(f + 52ull * P);
+        d.cq = (float *)(f + 56ull * P);
+        d.ha = (uint32_t *)(f + 60ull * P);
+        d.hb = (uint32_t *)(f + 64ull * P);
+        d.hr2 = (uint32_t *)(f + 68ull * P);
+        d.cflag = f + 72ull * P;
         // the 16-bit V-cycle's coarse levels: four halves and a float a cell
         uint8_t *v = (uint8_t *)w->vlevels;
         for (uint32_t l = 1; l <= d.L; l++) {
@@ -706,11 +722,11 @@ template <int S> static void enqueue_iteration_store(Batch &b, const StepPlan &p
         for (uint32_t l = pl.max_level; l >= 1; l--)
             launch_lds(b, KID_levelUp, k_levelUp<S>, SH_TILE + (int)l - 1, 256, pl.up_lds[l], l);
         launch(b, KID_relax, k_relax<S>, SH_BG, 64, 1);
-        launch(b, KID_cgRho, k_cgRho<S>, SH_BG4, 256);
+        launch(b, KID_cgRho, k_cgRho<S>, SH_BG16, 256);
     } else if (part == 1) {
-        launch(b, KID_cgApply, k_cgApply<S>, SH_BG4, 256);
+        launch(b, KID_cgApply, k_cgApply<S>, SH_BG16, 256);
     } else {
-        launch(b, KID_cgUpdate, k_cgUpdate<S>, SH_BG4, 256);
+        launch(b, KID_cgUpdate, k_cgUpdate<S>, SH_BG16, 256);
     }
 }
 }  // extern "C++"
@@ -1011,20 +1027,21 @@ int wig_capture(WigWorld *w, int on) {
     if (!w) return fail(WIG_E_INVALID, "null world");
     Dev &d = w->d;
     if (on && !w->check64) {
-        uint64_t bytes = 4ull * d.C + 32ull * d.C + 32ull * d.B + 256u;
+        const uint64_t P = 64ull * d.B;   // block-major, as the packed fine level
+        uint64_t bytes = 4ull * P + 32ull * P + 32ull * d.B + 256u;
         if (hipMalloc(&w->check64, bytes) != hipSuccess) return fail(WIG_E_NOMEM, "capture arrays: %llu bytes", (unsigned long long)bytes);
         if (zero(stream_of(w), w->check64, bytes) != WIG_OK) return WIG_E_DEVICE;
         uint8_t *c = (uint8_t *)w->check64;
         d.head64 = (double *)c;
         d.part64 = (double *)(c + 256u);
         d.x64 = (double *)(c + 256u + 32ull * d.B);
-        d.r64 = d.x64 + d.C;
-        d.p64 = d.r64 + d.C;
-        d.q64 = d.p64 + d.C;
-        d.capb = (float *)(d.q64 + d.C);
+        d.r64 = d.x64 + P;
+        d.p64 = d.r64 + P;
+        d.q64 = d.p64 + P;
+        d.capb = (float *)(d.q64 + P);
     }
     if (!on) d.capb = nullptr;
-    else d.capb = (float *)(d.q64 + d.C);
+    else d.capb = (float *)(d.q64 + 64ull * d.B);
     return WIG_OK;
 }
 
@@ -1077,8 +1094,14 @@ int wig_cg64(WigWorld *w, double tol, uint32_t maxit, WigCheck64 *out) {
 int wig_cg64_x(WigWorld *w, double *x) {
     if (!w || !x) return fail(WIG_E_INVALID, "null argument");
     if (!w->check64) return fail(WIG_E_INVALID, "no FP64 solution (wig_cg64)");
-    HIPCHECK(hipMemcpyAsync(x, w->d.x64, 8ull * w->d.C, hipMemcpyDeviceToHost, stream_of(w)));
+    const Dev &d = w->d;
+    std::vector<double> blocked(64ull * d.B);
+    HIPCHECK(hipMemcpyAsync(blocked.data(), d.x64, 8ull * blocked.size(), hipMemcpyDeviceToHost, stream_of(w)));
     HIPCHECK(hipStreamSynchronize(stream_of(w)));
+    // block-major on the device, row-major for the caller
+    for (uint32_t y = 0; y < (uint32_t)d.H; y++)
+        for (uint32_t x0 = 0; x0 < (uint32_t)d.W; x0++)
+            x[(uint64_t)y * d.W + x0] = blocked[((uint64_t)(y >> 3) * d.bw + (x0 >> 3)) * 64u + ((y & 7u) << 3) + (x0 & 7u)];
     return WIG_OK;
 }
 

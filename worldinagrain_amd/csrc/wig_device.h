@@ -332,6 +332,12 @@ __device__ __forceinline__ float liquidVolume(float temp) { return 1.0f / expand
 __device__ __forceinline__ uint32_t kindOf(const Dev &d, uint32_t t) { return d.kind[t & 0xffu]; }
 __device__ __forceinline__ bool inWorld(const Dev &d, i2 c) { return c.x >= 0 && c.y >= 0 && c.x < d.W && c.y < d.H; }
 __device__ __forceinline__ uint32_t cellIndex(const Dev &d, i2 c) { return (uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x; }
+// The solve's own arrays (the packed fine level, the FP64 check's vectors)
+// are block-major: each 8 x 8 block's cells contiguous, row-major inside it,
+// so a wavefront working a block reads whole cache lines.  c in the world.
+__device__ __forceinline__ uint32_t packIndex(const Dev &d, i2 c) {
+    return (((uint32_t)c.y >> 3) * d.bw + ((uint32_t)c.x >> 3)) * 64u + (((uint32_t)c.y & 7u) << 3) + ((uint32_t)c.x & 7u);
+}
 __device__ __forceinline__ i2 cellOf(const Dev &d, v2 p) {
     int32_t x = wi32(floorf(p.x)), y = wi32(floorf(p.y));
     x = x < 0 ? 0 : (x > d.W - 1 ? d.W - 1 : x);

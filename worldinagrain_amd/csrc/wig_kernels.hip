@@ -14,15 +14,9 @@
 
 #define WAVE 64
 
-__device__ __forceinline__ bool converged(const Dev &d, uint32_t cycle) {
-    return cycle > 0u && __hip_atomic_load(&d.head->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-}
 __device__ __forceinline__ uint32_t ld(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Whether the world's host would have encoded iteration `cycle`: within its
-// fluid_cycles, and before the convergence checkpoint its solve stops at (a
-// batch runs as many iterations as its slowest world; the others' skip).
 // The 16-bit V-cycle's rhs scale: the power of two that takes a residual
 // whose largest |r| has these float bits into [0.5, 1) (the PCG's iterates
 // do not depend on the preconditioner's scale; 16 bits do not reach the
@@ -32,10 +26,25 @@ __device__ __forceinline__ float vcycleScale(const Dev &d, uint32_t most) {
     if (d.store == 0u || most == 0u || e == 0xffu) return 1.0f;
     return __uint_as_float((253u - e) << 23);
 }
-__device__ __forceinline__ bool encoded(const Dev &d, uint32_t cycle) {
-    return d.full && cycle < d.fluid_cycles && cycle < ld(&d.wstate[WS_STOP]);
-}
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+// An iteration pass's words, read together (one round trip to memory, not
+// one a test): the iteration's number; whether the world's host would have
+// encoded it (within its fluid_cycles, and before the convergence
+// checkpoint its solve stops at: a batch runs as many iterations as its
+// slowest world, the others' skip); whether the solve had converged before
+// it; and the active blocks' count.
+struct Iter { uint32_t cycle, active; bool encoded, converged; };
+__device__ __forceinline__ Iter iterWords(const Dev &d) {
+    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]), stop = ld(&d.wstate[WS_STOP]);
+    const uint32_t conv = __hip_atomic_load(&d.head->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t active = d.blocks[COUNT_SLOT(d, COUNT_ACTIVE)];
+    Iter it;
+    it.cycle = cycle;
+    it.active = active;
+    it.encoded = d.full && cycle < d.fluid_cycles && cycle < stop;
+    it.converged = cycle > 0u && conv != 0u;
+    return it;
+}
 
 // ------------------------------------------------------------ cell tests
 __device__ __forceinline__ float densityAt(const Dev &d, i2 c) {           // fluid.wgsl:322-327
@@ -1049,33 +1058,14 @@ __device__ v2 residualAt(const Dev &d, uint32_t l, i2 c) {                  // :
     return V2(rhs.x - (s.sum.x - (s.diag + pin.x) * pot.x) / h2, rhs.y - (s.sum.y - (s.diag + pin.y) * pot.y) / h2);
 }
 
-// The fine level's residual at cell i (residualAt, :2306-2315) from the
-// packed solve arrays: p as the pre-relaxation left it (pb), the betas
-// (0 where the stencil skips the neighbour), the pin, the rhs.
-template <int S> __device__ __forceinline__ v2 fineResidual(const Dev &d, i2 c, uint32_t i, uint32_t f) {
-    float4 cf = d.coef[i];
-    const float b[4] = {fabsf(cf.x), fabsf(cf.y), fabsf(cf.z), fabsf(cf.w)};
-    float sx = 0.0f, sy = 0.0f, diag = 0.0f;
-    for (int k = 0; k < 4; k++) {
-        if (b[k] == 0.0f) continue;
-        float2 q = fnLoad<S>(d, 1, cellIndex(d, beside(c, k)));
-        sx += b[k] * q.x;
-        sy += b[k] * q.y;
-        diag += b[k];
-    }
-    float2 p = fnLoad<S>(d, 1, i), r = fnLoad<S>(d, 2, i);
-    float pin = (f & CF_PINNED) ? PINF : 0.0f;
-    return V2(r.x - (sx - (diag + pin) * p.x), r.y - (sy - (diag + 0.0f) * p.y));
-}
-
 // relax / relaxShifted (:2080-2192): a 10 x 10 tile in LDS, its rim frozen
 // as relaxRims snapshot it, sweep_pairs red-black pairs, one write back.
 // The pre-relaxation reads pa and writes pb, the post-relaxation reads pb
 // plus prolongFine's correction (:2500-2512) and writes pa: the buffer read
 // is the rim snapshot, and every fluid cell of the tiles is written.
 template <int S> __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c);
-__device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
-    if (wid < activeCount(d)) return activeBlock(d, wid);
+__device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t na, uint32_t wid, int shift) {
+    if (wid < na) return activeBlock(d, wid);
     int32_t wide = (int32_t)d.bw, high = (int32_t)(d.B / d.bw);
     int32_t j = (int32_t)wid - (int32_t)d.B;
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
@@ -1084,97 +1074,107 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t wid, int shift) {
 template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
-    const int shift = ((d.shifted_first ^ (2u * cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
-    if (converged(d, cycle)) return;
+    const Iter it = iterWords(d);
+    if (!it.encoded || it.converged) return;
+    const int shift = ((d.shifted_first ^ (2u * it.cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
     __shared__ float sPx[RIM * RIM], sPy[RIM * RIM];
     __shared__ float cRho[64], cPx[64], cPy[64];
     uint32_t li = threadIdx.x;
-    uint32_t na = activeCount(d), edge = shift ? d.bw + d.bh + 1u : 0u;
+    uint32_t na = it.active, edge = shift ? d.bw + d.bh + 1u : 0u;
     // the tiles: the active list's blocks, then (shifted) the edge tiles,
     // workgroup indices B .. B + edge of the reference's dispatch
     for (uint32_t v = bid; v < na + edge; v += nbid) {
         uint32_t wid = v < na ? v : d.B + (v - na);
-        i2 b = tileBlock(d, wid, shift);
+        i2 b = tileBlock(d, na, wid, shift);
         i2 origin = I2(b.x * WIG_BLOCK - shift, b.y * WIG_BLOCK - shift);
         if (b.x < 0 || origin.x >= d.W || origin.y >= d.H) continue;
-        bool live = false;
+        // All of the tile's loads at once: the states of the blocks under
+        // it (a tile with no fluid is skipped), this thread's cell, level 1
+        // under the tile after the coarse levels, the rim buffer's cells.
+        uint32_t bits = 0;
         {
             i2 b0 = idiv8(origin), b1 = idiv8(I2(origin.x + WIG_BLOCK - 1, origin.y + WIG_BLOCK - 1));
-            uint32_t bits = 0;
             for (int32_t y = b0.y; y <= b1.y; y++)
                 for (int32_t x = b0.x; x <= b1.x; x++)
                     if (inBlocks(d, I2(x, y))) bits |= d.blocks[STATE_SLOT(d, blockIndex(d, I2(x, y)))];
-            live = (bits & HAS_FLUID) != 0u;
         }
-        if (!live) continue;
-        // this thread's own cell first (its loads overlap the tile's)
         i2 me = I2(origin.x + (int32_t)(li % 8u), origin.y + (int32_t)(li / 8u));
         int t = (int)(li / 8u + 1u) * RIM + (int)(li % 8u) + 1;
-        float rhsx = 0.0f, rhsy = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
-        bool fluid = false, pinned = false;
-        uint32_t mi = 0;
+        uint32_t mf = 0, mi = 0;
+        float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float2 r = make_float2(0.0f, 0.0f);
         if (inWorld(d, me)) {
-            mi = cellIndex(d, me);
-            uint32_t f = d.cflag[mi];
-            float4 cf = d.coef[mi];
-            float2 r = fnLoad<S>(d, 2, mi);
-            fluid = (f & CF_FLUID) != 0u;
-            if (fluid) {
-                b0 = fabsf(cf.x);
-                b1 = fabsf(cf.y);
-                b2 = fabsf(cf.z);
-                b3 = fabsf(cf.w);
-                rhsx = r.x;
-                rhsy = r.y;
-                pinned = (f & CF_PINNED) != 0u;
-            }
+            mi = packIndex(d, me);
+            mf = d.cflag[mi];
+            cf = d.coef[mi];
+            r = fnLoad<S>(d, 2, mi);
         }
-        // after the coarse levels: level 1 under the tile (8 x 8 cells from
-        // one before the rim's first), for prolongFine's correction
+        // (level 1 from 8 x 8 cells one before the rim's first, for
+        // prolongFine's correction)
         const int32_t cx0 = ((origin.x - 1) >> 1) - 1, cy0 = ((origin.y - 1) >> 1) - 1;
+        float lr = -1.0f;
+        float2 lp = make_float2(0.0f, 0.0f);
         if (post) {
             i2 cc = I2(cx0 + (int32_t)(li % 8u), cy0 + (int32_t)(li / 8u));
-            float r = -1.0f, px = 0.0f, py = 0.0f;
             if (inLevel(d, 1u, cc)) {
-                r = lvRho<S>(d, 1u, cc);
-                float2 p = lvLoadP<S>(d, 1u, cc);
-                px = p.x;
-                py = p.y;
+                lr = lvRho<S>(d, 1u, cc);
+                lp = lvLoadP<S>(d, 1u, cc);
             }
-            cRho[li] = r;
-            cPx[li] = px;
-            cPy[li] = py;
+        }
+        float2 rv[2] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
+        uint32_t rf[2] = {0u, 0u};
+        for (uint32_t q = 0; q < 2u; q++) {
+            uint32_t k = li + 64u * q;
+            i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
+            if (k < RIM * RIM && inWorld(d, c)) {
+                uint32_t ci = packIndex(d, c);
+                rv[q] = fnLoad<S>(d, post ? 1 : 0, ci);
+                if (post) rf[q] = d.cflag[ci];
+            }
+        }
+        __asm__ __volatile__("" ::: "memory");   // (the loads above issue before the test)
+        if ((bits & HAS_FLUID) == 0u) continue;
+        const bool fluid = (mf & CF_FLUID) != 0u, pinned = fluid && (mf & CF_PINNED) != 0u;
+        float rhsx = 0.0f, rhsy = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+        if (fluid) {
+            b0 = fabsf(cf.x);
+            b1 = fabsf(cf.y);
+            b2 = fabsf(cf.z);
+            b3 = fabsf(cf.w);
+            rhsx = r.x;
+            rhsy = r.y;
+        }
+        if (post) {
+            cRho[li] = lr;
+            cPx[li] = lp.x;
+            cPy[li] = lp.y;
             __syncthreads();
         }
-        for (uint32_t k = li; k < RIM * RIM; k += 64u) {
+        for (uint32_t q = 0; q < 2u; q++) {
+            uint32_t k = li + 64u * q;
+            if (k >= RIM * RIM) continue;
             i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
-            float2 v = make_float2(0.0f, 0.0f);
-            if (inWorld(d, c)) {
-                uint32_t ci = cellIndex(d, c);
-                v = fnLoad<S>(d, post ? 1 : 0, ci);
-                if (post && (d.cflag[ci] & CF_FLUID)) {
-                    // prolonged() (:2458-2497) from the staged cells
-                    i2 coarse = I2(c.x / 2, c.y / 2);
-                    i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
-                    float sx = 0.0f, sy = 0.0f, weight = 0.0f;
-                    for (int q = 0; q < 4; q++) {
-                        i2 o = q == 0 ? I2(0, 0) : (q == 1 ? I2(sg.x, 0) : (q == 2 ? I2(0, sg.y) : sg));
-                        float wk = q == 0 ? 9.0f : (q == 3 ? 1.0f : 3.0f);
-                        int32_t j = (coarse.y + o.y - cy0) * 8 + (coarse.x + o.x - cx0);
-                        float r = cRho[j];
-                        if (r < 0.0f) continue;
-                        weight += wk;
-                        if (r > 0.0f) {
-                            sx += wk * cPx[j];
-                            sy += wk * cPy[j];
-                        }
+            float2 v = rv[q];
+            if (post && (rf[q] & CF_FLUID)) {
+                // prolonged() (:2458-2497) from the staged cells
+                i2 coarse = I2(c.x / 2, c.y / 2);
+                i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
+                float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+                for (int o4 = 0; o4 < 4; o4++) {
+                    i2 o = o4 == 0 ? I2(0, 0) : (o4 == 1 ? I2(sg.x, 0) : (o4 == 2 ? I2(0, sg.y) : sg));
+                    float wk = o4 == 0 ? 9.0f : (o4 == 3 ? 1.0f : 3.0f);
+                    int32_t j = (coarse.y + o.y - cy0) * 8 + (coarse.x + o.x - cx0);
+                    float rc = cRho[j];
+                    if (rc < 0.0f) continue;
+                    weight += wk;
+                    if (rc > 0.0f) {
+                        sx += wk * cPx[j];
+                        sy += wk * cPy[j];
                     }
-                    float dn = fmaxf(weight, 1e-6f);
-                    v.x = v.x + sx / dn;
-                    v.y = v.y + sy / dn;
                 }
+                float dn = fmaxf(weight, 1e-6f);
+                v.x = v.x + sx / dn;
+                v.y = v.y + sy / dn;
             }
             sPx[k] = v.x;
             sPy[k] = v.y;
@@ -1184,8 +1184,8 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
         float dx = diag + (pinned ? PINF : 0.0f), dy = fmaxf(diag, 1e-30f);
         bool solving = dx > 0.0f;
         int color = (me.x + me.y) & 1;
-        for (int s = 0; s < 2 * (int)d.sweep_pairs; s++) {
-            if (solving && color == (s & 1)) {
+        for (int sw = 0; sw < 2 * (int)d.sweep_pairs; sw++) {
+            if (solving && color == (sw & 1)) {
                 float sx = b0 * sPx[t - 1] + b1 * sPx[t + 1] + b2 * sPx[t - RIM] + b3 * sPx[t + RIM];
                 float sy = b0 * sPy[t - 1] + b1 * sPy[t + 1] + b2 * sPy[t - RIM] + b3 * sPy[t + RIM];
                 float nx = wmix(sPx[t], (sx - rhsx) / dx, SMOOTH_OMEGA);
@@ -1201,34 +1201,87 @@ template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int 
 }
 
 // restrictLevel into level 1 (:2342-2377), each iteration: rhs from the
-// fluid children's residuals; rho as k_levelRho put it at the solve's start
-// restrictLevel into level 1's cell c (rho > 0 there): the sums of its
-// fluid children's residuals, a quarter of each
-template <int S> __device__ __forceinline__ v2 restrictFine(const Dev &d, i2 c) {
-    float sx = 0.0f, sy = 0.0f;
-    for (int q = 0; q < 4; q++) {
-        i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
-        if (!inWorld(d, k)) continue;
-        uint32_t ki = cellIndex(d, k), f = d.cflag[ki];
-        if (!(f & CF_FLUID)) continue;
-        v2 e = fineResidual<S>(d, k, ki, f);
-        sx += e.x;
-        sy += e.y;
-    }
-    return V2(sx * 0.25f, sy * 0.25f);
-}
+// fluid children's residuals, a quarter of their sum; rho as k_levelRho put
+// it at the solve's start.  The fine residual (residualAt, :2306-2315) from
+// the packed arrays: p as the pre-relaxation left it (pb), the betas (0
+// where the stencil skips the neighbour), the pin, the rhs.  A workgroup
+// takes 8 x 8 cells of level 1 over 16 x 16 fine cells, their pb and its
+// ring staged in LDS, every load made at once.
+#define RF 18
 template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
+    const Iter it = iterWords(d);
     const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
-    if (converged(d, cycle) || activeCount(d) == 0u) return;
-    i2 c = I2((int32_t)(bx * 8u + threadIdx.x % 8u), (int32_t)(by * 8u + threadIdx.x / 8u));
-    if (!inLevel(d, 1u, c)) return;
-    float rho = lvRho<S>(d, 1u, c);
-    v2 r = rho > 0.0f ? restrictFine<S>(d, c) : V2(0.0f, 0.0f);
-    lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, r.x, r.y));
+    if (!it.encoded || it.converged || it.active == 0u) return;
+    __shared__ float sPx[RF * RF], sPy[RF * RF];
+    const uint32_t li = threadIdx.x;
+    const int32_t fx0 = (int32_t)bx * 16 - 1, fy0 = (int32_t)by * 16 - 1;
+    // level 1's rho, and whether any of the four fine blocks under the tile
+    // is worked on: level 1's rho is 0 over the others (k_levelRho), and a
+    // cell of rho <= 0 restricts 0
+    const i2 c = I2((int32_t)(bx * 8u + li % 8u), (int32_t)(by * 8u + li / 8u));
+    const bool in = inLevel(d, 1u, c);
+    const float rho = in ? lvRho<S>(d, 1u, c) : 0.0f;
+    uint32_t st = 0;
+    for (int q = 0; q < 4; q++) {
+        i2 fb = I2(2 * (int32_t)bx + (q & 1), 2 * (int32_t)by + (q >> 1));
+        if (inBlocks(d, fb)) st |= d.blocks[STATE_SLOT(d, blockIndex(d, fb))];
+    }
+    if ((st & ACTIVE) == 0u) {
+        if (in) lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        return;
+    }
+    float2 pv[6];
+    for (uint32_t q = 0; q < 6u; q++) {
+        uint32_t k = li + 64u * q;
+        i2 fc = I2(fx0 + (int32_t)(k % RF), fy0 + (int32_t)(k / RF));
+        pv[q] = (k < RF * RF && inWorld(d, fc)) ? fnLoad<S>(d, 1, packIndex(d, fc)) : make_float2(0.0f, 0.0f);
+    }
+    uint32_t kf[4];
+    float4 kc[4];
+    float2 kr[4];
+    for (int q = 0; q < 4; q++) {
+        i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
+        kf[q] = 0u;
+        kc[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        kr[q] = make_float2(0.0f, 0.0f);
+        if (rho > 0.0f && inWorld(d, k)) {
+            uint32_t ki = packIndex(d, k);
+            kf[q] = d.cflag[ki];
+            kc[q] = d.coef[ki];
+            kr[q] = fnLoad<S>(d, 2, ki);
+        }
+    }
+    for (uint32_t q = 0; q < 6u; q++) {
+        uint32_t k = li + 64u * q;
+        if (k < RF * RF) {
+            sPx[k] = pv[q].x;
+            sPy[k] = pv[q].y;
+        }
+    }
+    __syncthreads();
+    if (!in) return;
+    // the fluid children's residuals
+    float sx = 0.0f, sy = 0.0f;
+    if (rho > 0.0f)
+        for (int q = 0; q < 4; q++) {
+            if (!(kf[q] & CF_FLUID)) continue;
+            const int32_t t = (2 * c.y + (q >> 1) - fy0) * RF + (2 * c.x + (q & 1) - fx0);
+            const int32_t at[4] = {t - 1, t + 1, t - RF, t + RF};
+            const float bk[4] = {fabsf(kc[q].x), fabsf(kc[q].y), fabsf(kc[q].z), fabsf(kc[q].w)};
+            float nx = 0.0f, ny = 0.0f, diag = 0.0f;
+            for (int k4 = 0; k4 < 4; k4++) {
+                if (bk[k4] == 0.0f) continue;
+                nx += bk[k4] * sPx[at[k4]];
+                ny += bk[k4] * sPy[at[k4]];
+                diag += bk[k4];
+            }
+            float pin = (kf[q] & CF_PINNED) ? PINF : 0.0f;
+            sx += kr[q].x - (nx - (diag + pin) * sPx[t]);
+            sy += kr[q].y - (ny - (diag + 0.0f) * sPy[t]);
+        }
+    lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, sx * 0.25f, sy * 0.25f));
 }
 // level 1's rho (restrictLevel's, :2342-2377), once a solve: it depends on
 // the fine rho alone
@@ -1366,11 +1419,10 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t l) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
+    const Iter it = iterWords(d);
     const uint32_t T = d.tile[l];
     const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
-    if (converged(d, cycle) || activeCount(d) == 0u) return;
+    if (!it.encoded || it.converged || it.active == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs + 1u;
     LevelTile t = levelTile(sm, T, H, bx, by);
@@ -1448,11 +1500,10 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
+    const Iter it = iterWords(d);
     const uint32_t T = d.tile[l];
     const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
-    if (converged(d, cycle) || activeCount(d) == 0u) return;
+    if (!it.encoded || it.converged || it.active == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs;
     LevelTile t = levelTile(sm, T, H, bx, by);
@@ -1582,12 +1633,11 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
 template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
-    if (converged(d, cycle)) return;
+    const Iter it = iterWords(d);
+    if (!it.encoded || it.converged) return;
     extern __shared__ float sm[];
-    const uint32_t l0 = activeCount(d) == 0u ? d.L : d.lds_level, L = d.L;
-    const bool idle = activeCount(d) == 0u;   // restrict and prolong pass over nothing; solve still sweeps
+    const uint32_t l0 = it.active == 0u ? d.L : d.lds_level, L = d.L;
+    const bool idle = it.active == 0u;   // restrict and prolong pass over nothing; solve still sweeps
     // level l's arrays in LDS, levels l0 .. L one after another
     auto level = [&](uint32_t l) {
         uint32_t off = 0;
@@ -1727,19 +1777,20 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
 // ------------------------------------------------ conjugate gradients
 // cgDirection's p (:2767-2776) at cell i: z + beta p, 0 off the fluid
 template <int S> __device__ __forceinline__ float newP(const Dev &d, uint32_t i, double beta, const float *pold) {
-    return (d.cflag[i] & CF_FLUID) ? (float)((double)fnLoad<S>(d, 0, i).x + beta * (double)pold[i]) : 0.0f;
+    const uint32_t f = d.cflag[i];
+    const float z = fnLoad<S>(d, 0, i).x, po = pold[i];   // (loaded with the flag, not after it)
+    return (f & CF_FLUID) ? (float)((double)z + beta * (double)po) : 0.0f;
 }
 // applyA (:2595-2613) from the packed betas: every face but an obstacle's
-// counts in the diagonal, only a fluid neighbour's p in the sum (each
-// neighbour's new p taken as its own thread takes it)
-template <int S> __device__ __forceinline__ float applyA(const Dev &d, i2 c, uint32_t f, float p, double beta, const float *pold) {
-    float4 cf = d.coef[cellIndex(d, c)];
+// counts in the diagonal, only a fluid neighbour's p in the sum (nb: the
+// neighbours' new p, beside()'s order)
+__device__ __forceinline__ float applyA(float4 cf, uint32_t f, float p, const float *nb) {
     const float v[4] = {cf.x, cf.y, cf.z, cf.w};
     float sum = 0.0f, diag = 0.0f;
     for (int k = 0; k < 4; k++) {
         float b = fabsf(v[k]);
         if (b == 0.0f) continue;
-        if (__float_as_uint(v[k]) >> 31) sum += b * newP<S>(d, cellIndex(d, beside(c, k)), beta, pold);
+        if (__float_as_uint(v[k]) >> 31) sum += b * nb[k];
         diag += b;
     }
     diag += (f & CF_PINNED) ? PINF : 0.0f;
@@ -1753,11 +1804,11 @@ __global__ void __launch_bounds__(64) k_cgStart(WorldGrid g) {                  
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
-        uint32_t i = cellIndex(d, c);
-        d.cx[i] = gridX(d, c);
-        d.cr[i] = 0.0f;
-        d.cq[i] = 0.0f;
-        d.cp0[i] = 0.0f;
+        uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
+        d.cx[pi] = gridX(d, c);
+        d.cr[pi] = 0.0f;
+        d.cq[pi] = 0.0f;
+        d.cp0[pi] = 0.0f;
         CNT(d, i, G_REGION) = i;
     }
 }
@@ -1816,6 +1867,29 @@ __device__ void sumPlaces2(const Dev &d, uint32_t w0, uint32_t w1, double *s0, d
 #define FOR_ACTIVE_BLOCKS4(d, wid) \
     for (uint32_t wid = bid * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += nbid * WAVES)
 #define LANE (threadIdx.x % 64u)
+// The iteration's passes over the fine level take NB places of the active
+// list a wavefront at once, so that NB blocks' loads are in flight together
+// (each block's sums are still its own wavefront halving, as above).
+#define NB 4u
+#define FOR_PLACES(it, w0) \
+    for (uint32_t w0 = (bid * WAVES + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64u)) * NB; w0 < (it).active; w0 += nbid * WAVES * NB)
+// this lane's cell in the blocks at places w0 .. w0 + NB - 1, as packed
+// indices (past the list's end block 0's: loads there are harmless, stores
+// are not made).  Every lane of a block has its slot; those outside the
+// world hold cflag 0.
+__device__ __forceinline__ void placeCells(const Dev &d, const Iter &it, uint32_t w0, uint32_t *i) {
+    for (uint32_t j = 0; j < NB; j++) {
+        uint32_t b = w0 + j < it.active ? d.blocks[ACTIVE_SLOT(d, w0 + j)] : 0u;
+        i[j] = b * 64u + LANE;
+    }
+}
+// the packed index of the cell beside packed cell i (beside()'s k) in the
+// next block over, for a lane on that edge of its block; i itself where
+// that is off the packed arrays (a cell outside the world, whose beta is 0)
+__device__ __forceinline__ uint32_t acrossEdge(const Dev &d, uint32_t i, int k) {
+    uint32_t n = k == 0 ? i - 57u : (k == 1 ? i + 57u : (k == 2 ? i - 64u * d.bw + 56u : i + 64u * d.bw - 56u));
+    return n < 64u * d.B ? n : i;
+}
 
 // cgResidual (:2708-2717), and ||b||^2 for the relative stop
 __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
@@ -1836,13 +1910,13 @@ __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
         double bb = 0.0;
         uint32_t most = 0;
         if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
-            uint32_t i = cellIndex(d, c);
+            uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
             float r = -residualAt(d, 0u, c).x;
-            d.cr[i] = r;
+            d.cr[pi] = r;
             most = ubits(fabsf(r));
             double bi = -(double)FLD(d, 2u * i, 1);
             bb = bi * bi;
-            if (d.capb) d.capb[i] = -FLD(d, 2u * i, 1);
+            if (d.capb) d.capb[pi] = -FLD(d, 2u * i, 1);
         }
         bb = wave_tree_sum(bb);
         most = wave_max_u32(most);
@@ -1887,11 +1961,11 @@ template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) 
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 bk = activeBlock(d, wid), c = blockCell(bk, threadIdx.x);
         if (!inWorld(d, c)) continue;
-        uint32_t i = cellIndex(d, c);
+        uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
         float rho = FLD(d, 2u * i, 2);
         float2 v = make_float2(0.0f, FLD(d, 2u * i + 1u, 0));
-        fnStore<S>(d, 0, i, v);
-        fnStore<S>(d, 1, i, v);
+        fnStore<S>(d, 0, pi, v);
+        fnStore<S>(d, 1, pi, v);
         uint32_t f = 0;
         float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float2 r = make_float2(0.0f, 0.0f);
@@ -1904,19 +1978,19 @@ template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) 
                 b[k] = rn > 0.0f ? -beta : beta;
             }
             cf = make_float4(b[0], b[1], b[2], b[3]);
-            r = make_float2(S == STORE_F32 ? -d.cr[i] : -d.cr[i] * d.head->vscale, FLD(d, 2u * i + 1u, 1));
+            r = make_float2(S == STORE_F32 ? -d.cr[pi] : -d.cr[pi] * d.head->vscale, FLD(d, 2u * i + 1u, 1));
         }
-        d.coef[i] = cf;
-        d.cflag[i] = (uint8_t)f;
-        fnStore<S>(d, 2, i, r);
+        d.coef[pi] = cf;
+        d.cflag[pi] = (uint8_t)f;
+        fnStore<S>(d, 2, pi, r);
         for (int k = 0; k < 4; k++) {
             i2 n = beside(c, k);
             if (!inWorld(d, n) || (idiv8(n).x == bk.x && idiv8(n).y == bk.y) || isActiveBlock(d, idiv8(n))) continue;
-            uint32_t ni = cellIndex(d, n);
+            uint32_t ni = cellIndex(d, n), pn = packIndex(d, n);
             float2 q = make_float2(FLD(d, 2u * ni, 0), FLD(d, 2u * ni + 1u, 0));
-            fnStore<S>(d, 0, ni, q);
-            fnStore<S>(d, 1, ni, q);
-            d.cflag[ni] = FLD(d, 2u * ni, 2) > 0.0f ? (uint8_t)CF_FLUID : (uint8_t)0;
+            fnStore<S>(d, 0, pn, q);
+            fnStore<S>(d, 1, pn, q);
+            d.cflag[pn] = FLD(d, 2u * ni, 2) > 0.0f ? (uint8_t)CF_FLUID : (uint8_t)0;
         }
     }
 }
@@ -1924,31 +1998,39 @@ template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) 
 template <int S> __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
-    if (converged(d, cycle)) return;
-    FOR_ACTIVE_BLOCKS4(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), LANE);
-        double v0 = 0.0, v1 = 0.0;
-        if (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID)) {
-            uint32_t i = cellIndex(d, c);
-            double z = (double)fnLoad<S>(d, 0, i).x;
-            v0 = (double)d.cr[i] * z;
-            v1 = (double)d.cq[i] * z;
+    const Iter it = iterWords(d);
+    if (!it.encoded || it.converged) return;
+    FOR_PLACES(it, w0) {
+        uint32_t i[NB];
+        placeCells(d, it, w0, i);
+        uint32_t f[NB];
+        float z[NB], r[NB], q[NB];
+        for (uint32_t j = 0; j < NB; j++) {
+            f[j] = d.cflag[i[j]];
+            z[j] = fnLoad<S>(d, 0, i[j]).x;
+            r[j] = d.cr[i[j]];
+            q[j] = d.cq[i[j]];
         }
-        v0 = wave_tree_sum(v0);
-        v1 = wave_tree_sum(v1);
-        if (LANE == 0) {
-            d.bpart[4ull * wid + 0] = v0;
-            d.bpart[4ull * wid + 1] = v1;
+        for (uint32_t j = 0; j < NB; j++) {
+            double v0 = 0.0, v1 = 0.0;
+            if (f[j] & CF_FLUID) {
+                v0 = (double)r[j] * (double)z[j];
+                v1 = (double)q[j] * (double)z[j];
+            }
+            v0 = wave_tree_sum(v0);
+            v1 = wave_tree_sum(v1);
+            if (LANE == 0 && w0 + j < it.active) {
+                d.bpart[4ull * (w0 + j) + 0] = v0;
+                d.bpart[4ull * (w0 + j) + 1] = v1;
+            }
         }
     }
 }
 __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
     WORLD(g);
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
-    if (converged(d, cycle)) return;
+    const Iter it = iterWords(d);
+    const uint32_t cycle = it.cycle;
+    if (!it.encoded || it.converged) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     double s0, s1;
     sumPlaces2(d, 0, 1, t0, t1, &s0, &s1);
@@ -1966,32 +2048,56 @@ __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
 template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
-    if (converged(d, cycle)) return;
-    double beta = ld_f64(&d.head->beta);
-    const float *pold = pBuf(d, cycle);
-    float *pnew = pBuf(d, cycle + 1u);
-    FOR_ACTIVE_BLOCKS4(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), LANE);
-        double v = 0.0;
-        if (inWorld(d, c)) {
-            uint32_t i = cellIndex(d, c), f = d.cflag[i];
-            float p = newP<S>(d, i, beta, pold);
-            float q = (f & CF_FLUID) ? applyA<S>(d, c, f, p, beta, pold) : 0.0f;
-            pnew[i] = p;
-            d.cq[i] = q;
-            v = (double)p * (double)q;
+    const Iter it = iterWords(d);
+    const double beta = ld_f64(&d.head->beta);
+    if (!it.encoded || it.converged) return;
+    const float *pold = pBuf(d, it.cycle);
+    float *pnew = pBuf(d, it.cycle + 1u);
+    const uint32_t x = LANE & 7u, y = LANE >> 3;
+    const bool edge[4] = {x == 0u, x == 7u, y == 0u, y == 7u};
+    const uint32_t from[4] = {(LANE + 63u) & 63u, (LANE + 1u) & 63u, (LANE + 56u) & 63u, (LANE + 8u) & 63u};
+    FOR_PLACES(it, w0) {
+        uint32_t i[NB];
+        placeCells(d, it, w0, i);
+        uint32_t f[NB];
+        float z[NB], po[NB], pe[NB][4];
+        float4 cf[NB];
+        for (uint32_t j = 0; j < NB; j++) {
+            f[j] = d.cflag[i[j]];
+            z[j] = fnLoad<S>(d, 0, i[j]).x;
+            po[j] = pold[i[j]];
+            cf[j] = d.coef[i[j]];
         }
-        v = wave_tree_sum(v);
-        if (LANE == 0) d.bpart[4ull * wid + 0] = v;
+        // the neighbours' p across the block's edges (newP there)
+        for (uint32_t j = 0; j < NB; j++)
+            for (int k = 0; k < 4; k++) {
+                pe[j][k] = 0.0f;
+                if (edge[k]) pe[j][k] = newP<S>(d, acrossEdge(d, i[j], k), beta, pold);
+            }
+        for (uint32_t j = 0; j < NB; j++) {
+            // cgDirection's p (newP) at this cell, the neighbours' inside the
+            // block from their lanes
+            float p = (f[j] & CF_FLUID) ? (float)((double)z[j] + beta * (double)po[j]) : 0.0f;
+            float nb[4];
+            for (int k = 0; k < 4; k++) {
+                float o = __shfl(p, (int)from[k], WAVE);
+                nb[k] = edge[k] ? pe[j][k] : o;
+            }
+            float q = (f[j] & CF_FLUID) ? applyA(cf[j], f[j], p, nb) : 0.0f;
+            double v = (double)p * (double)q;
+            v = wave_tree_sum(v);
+            if (w0 + j < it.active) {
+                pnew[i[j]] = p;
+                d.cq[i[j]] = q;
+                if (LANE == 0) d.bpart[4ull * (w0 + j) + 0] = v;
+            }
+        }
     }
 }
 __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
     WORLD(g);
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
-    if (converged(d, cycle)) return;
+    const Iter it = iterWords(d);
+    if (!it.encoded || it.converged) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     double pq, unused;
     sumPlaces2(d, 0, 0, t0, t1, &pq, &unused);
@@ -2009,37 +2115,49 @@ __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
 template <int S> __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
-    if (!encoded(d, cycle)) return;
-    bool conv = converged(d, cycle);
+    const Iter it = iterWords(d);
+    const double alpha = ld_f64(&d.head->alpha);
+    const float vscale = d.head->vscale;
+    if (!it.encoded) return;
+    const uint32_t cycle = it.cycle;
     if (bid == 0 && threadIdx.x == 0) {
         d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + (cycle + 1u) % 3u)] = 0u;
-        if (!conv) atomicAdd(&d.blocks[COUNT_SLOT(d, COUNT_CYCLES)], 1u);
+        if (!it.converged) atomicAdd(&d.blocks[COUNT_SLOT(d, COUNT_CYCLES)], 1u);
     }
-    if (conv) return;
-    double alpha = ld_f64(&d.head->alpha);
-    const float vscale = d.head->vscale;
-    FOR_ACTIVE_BLOCKS4(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), LANE);
-        double rr = 0.0;
-        uint32_t most = 0;
-        if (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID)) {
-            uint32_t i = cellIndex(d, c);
-            float x = d.cx[i], r = d.cr[i], p = pBuf(d, cycle + 1u)[i], q = d.cq[i];
-            float rn = (float)((double)r - alpha * (double)q);
-            float xn = (float)((double)x + alpha * (double)p);
-            d.cx[i] = xn;
-            d.cr[i] = rn;
-            fnStoreX<S>(d, 0, i, 0.0f);
-            fnStoreX<S>(d, 2, i, S == STORE_F32 ? -rn : -rn * vscale);
-            most = ubits(fabsf(rn));
-            rr = (double)rn * (double)rn;
+    if (it.converged) return;
+    const float *pn = pBuf(d, cycle + 1u);
+    FOR_PLACES(it, w0) {
+        uint32_t i[NB];
+        placeCells(d, it, w0, i);
+        uint32_t f[NB];
+        float x[NB], r[NB], p[NB], q[NB];
+        for (uint32_t j = 0; j < NB; j++) {
+            f[j] = d.cflag[i[j]];
+            x[j] = d.cx[i[j]];
+            r[j] = d.cr[i[j]];
+            p[j] = pn[i[j]];
+            q[j] = d.cq[i[j]];
         }
-        rr = wave_tree_sum(rr);
-        most = wave_max_u32(most);
-        if (LANE == 0) {
-            d.bpart[4ull * wid + 2] = rr;
-            d.bpart[4ull * wid + 3] = (double)most;   // (the block's max |r|, as f32 bits; cgCheck takes the most)
+        for (uint32_t j = 0; j < NB; j++) {
+            const bool mine = w0 + j < it.active;
+            double rr = 0.0;
+            uint32_t most = 0;
+            if (mine && (f[j] & CF_FLUID)) {
+                float rn = (float)((double)r[j] - alpha * (double)q[j]);
+                float xn = (float)((double)x[j] + alpha * (double)p[j]);
+                d.cx[i[j]] = xn;
+                d.cr[i[j]] = rn;
+                fnStoreX<S>(d, 0, i[j], 0.0f);
+                fnStoreX<S>(d, 2, i[j], S == STORE_F32 ? -rn : -rn * vscale);
+                most = ubits(fabsf(rn));
+                rr = (double)rn * (double)rn;
+            }
+            rr = wave_tree_sum(rr);
+            most = wave_max_u32(most);
+            if (mine && LANE == 0) {
+                d.bpart[4ull * (w0 + j) + 2] = rr;
+                d.bpart[4ull * (w0 + j) + 3] = (double)most;   // (the block's max |r|, as f32 bits; cgCheck takes the most)
+            }
         }
     }
 }
@@ -2050,9 +2168,10 @@ __device__ __forceinline__ void stepCycle(const Dev &d, uint32_t cycle) {
 }
 __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
     WORLD(g);
-    const uint32_t cycle = ld(&d.wstate[WS_CYCLE]);
+    const Iter it = iterWords(d);
+    const uint32_t cycle = it.cycle;
     if (!d.full) return;
-    if (!encoded(d, cycle) || converged(d, cycle)) {
+    if (!it.encoded || it.converged) {
         stepCycle(d, cycle);
         return;
     }
@@ -2114,17 +2233,17 @@ template <int S> __global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
     FOR_ACTIVE_BLOCKS(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
         if (!inWorld(d, c)) continue;
-        uint32_t i = cellIndex(d, c);
+        uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
         // fluidCg as the iterations left it: p is the last iteration run's
-        float x = d.cx[i];
-        *(float4 *)&CGV(d, i, 0) = make_float4(x, d.cr[i], pBuf(d, ld(&d.head->cycles))[i], d.cq[i]);
+        float x = d.cx[pi];
+        *(float4 *)&CGV(d, i, 0) = make_float4(x, d.cr[pi], pBuf(d, ld(&d.head->cycles))[pi], d.cq[pi]);
         FLD(d, 2u * i, 0) = x;
         CNT(d, i, G_REGION) = 0u;
-        if (d.cflag[i] & CF_FLUID) {
-            FLD(d, 2u * i, 1) = -d.cr[i];
-            FLD(d, 2u * i + 1u, 0) = fnLoad<S>(d, 0, i).y;
+        if (d.cflag[pi] & CF_FLUID) {
+            FLD(d, 2u * i, 1) = -d.cr[pi];
+            FLD(d, 2u * i + 1u, 0) = fnLoad<S>(d, 0, pi).y;
             if (ran) {
-                float2 p = fnLoad<S>(d, 1, i);
+                float2 p = fnLoad<S>(d, 1, pi);
                 v2 e = prolonged<S>(d, 0u, c);
                 CNT(d, i, G_RIM) = ubits(p.x + e.x);
                 CNT(d, i, G_RIM + 1u) = ubits(p.y + e.y);
@@ -2626,13 +2745,13 @@ __device__ __forceinline__ double apply64(const Dev &d, i2 c, uint32_t i, const 
     for (int k = 0; k < 4; k++) {
         double b = (double)fabsf(v[k]);
         if (b == 0.0) continue;
-        if (__float_as_uint(v[k]) >> 31) sum += b * p[cellIndex(d, beside(c, k))];
+        if (__float_as_uint(v[k]) >> 31) sum += b * p[packIndex(d, beside(c, k))];
         diag += b;
     }
     if (d.cflag[i] & CF_PINNED) diag += (double)PINF;
     return diag * p[i] - sum;
 }
-#define FLUID_CELL(d, c) (inWorld(d, c) && (d.cflag[cellIndex(d, c)] & CF_FLUID))
+#define FLUID_CELL(d, c) (inWorld(d, c) && (d.cflag[packIndex(d, c)] & CF_FLUID))
 __device__ void sum64(const Dev &d, int nv, double *out) {
     __shared__ double t[4][SUM_LANES];
     uint32_t li = threadIdx.x, na = activeCount(d);
@@ -2656,7 +2775,7 @@ __global__ void __launch_bounds__(256) k64_start(WorldGrid g) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double bb = 0.0;
         if (inWorld(d, c)) {
-            uint32_t i = cellIndex(d, c);
+            uint32_t i = packIndex(d, c);
             double b = FLUID_CELL(d, c) ? (double)d.capb[i] : 0.0;
             d.x64[i] = 0.0;
             d.r64[i] = b;
@@ -2692,7 +2811,7 @@ __global__ void __launch_bounds__(256) k64_apply(WorldGrid g) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double pq = 0.0;
         if (FLUID_CELL(d, c)) {
-            uint32_t i = cellIndex(d, c);
+            uint32_t i = packIndex(d, c);
             double q = apply64(d, c, i, d.p64);
             d.q64[i] = q;
             pq = d.p64[i] * q;
@@ -2708,7 +2827,7 @@ __global__ void __launch_bounds__(256) k64_update(WorldGrid g) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double rr = 0.0;
         if (FLUID_CELL(d, c)) {
-            uint32_t i = cellIndex(d, c);
+            uint32_t i = packIndex(d, c);
             d.x64[i] += alpha * d.p64[i];
             double r = d.r64[i] - alpha * d.q64[i];
             d.r64[i] = r;
@@ -2724,7 +2843,7 @@ __global__ void __launch_bounds__(256) k64_direction(WorldGrid g) {
     FOR_ACTIVE_BLOCKS4(d, wid) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         if (FLUID_CELL(d, c)) {
-            uint32_t i = cellIndex(d, c);
+            uint32_t i = packIndex(d, c);
             d.p64[i] = d.r64[i] + beta * d.p64[i];
         }
     }
@@ -2738,7 +2857,7 @@ __global__ void __launch_bounds__(256) k64_check(WorldGrid g, int phase) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double a = 0.0, b2 = 0.0;
         if (inWorld(d, c)) {
-            uint32_t i = cellIndex(d, c);
+            uint32_t i = packIndex(d, c);
             if (phase == 0) {
                 d.p64[i] = FLUID_CELL(d, c) ? (double)d.cx[i] : 0.0;
             } else if (FLUID_CELL(d, c)) {
@@ -2763,7 +2882,7 @@ __global__ void __launch_bounds__(256) k64_residual(WorldGrid g, int of_x64) {
         i2 c = blockCell(activeBlock(d, wid), LANE);
         double rr = 0.0;
         if (FLUID_CELL(d, c)) {
-            uint32_t i = cellIndex(d, c);
+            uint32_t i = packIndex(d, c);
             double r = (double)d.capb[i] - apply64(d, c, i, x);
             rr = r * r;
         }

@@ -36,7 +36,9 @@ def world_bytes(width, height, coarse_levels=0):
     f = (width + 1) * height + width * (height + 1)
     field = 16 * (2 * c + f + 2 * ((f + 3) // 4) + 2 * (c // 3 + 64))
     parts = 24 * c + 2 * (16 + 48 * cap) + 4 * (11 * c + 2 * f) + field + 4 * (5 * b + 9) + 16 * (c + b + 1) + 32 * b + 64
-    return parts + 61 * c + 4 * b + 1024
+    # the packed fine level (73 B a cell, block-major: 64 cells a block), the
+    # 16-bit V-cycle's levels (12 B a cell of each, about 4 B a fine cell)
+    return parts + 73 * 64 * b + 4 * c + 4 * b + 2048
 
 
 class Session:
