@@ -111,6 +111,7 @@ enum {
     SH_LVL1,     // 8 x 8 cells of coarse level 1 a workgroup
     SH_B256,     // 256 blocks a workgroup
     SH_BG16,     // 256-thread passes, sixteen blocks a workgroup (four a wavefront)
+    SH_RELAX,    // k_relax: eight tiles a 256-thread workgroup (two a wavefront)
     SH_TILE,     // + l - 1: k_levelDown/k_levelUp's tiles on level l
     SH_COUNT = SH_TILE + MG_LEVELS
 };
@@ -150,6 +151,7 @@ static uint32_t shape_count(const Dev &d, int sh) {
     case SH_LVL1: return ceil_div(d.lw[1], 8u) * ceil_div(d.lh[1], 8u);
     case SH_B256: return ceil_div(d.B, 256u);
     case SH_BG16: return std::min<uint32_t>((d.B + 15u) / 16u, 1024u);
+    case SH_RELAX: return std::min<uint32_t>((d.B + d.bw + d.bh + 8u) / 8u, 1024u);
     default: {
         uint32_t l = (uint32_t)(sh - SH_TILE) + 1u;
         if (l >= d.lds_level) return 0u;
@@ -714,14 +716,14 @@ template <int S> static void enqueue_start_store(Batch &b) {
 }
 template <int S> static void enqueue_iteration_store(Batch &b, const StepPlan &pl, int part) {
     if (part == 0) {
-        launch(b, KID_relax, k_relax<S>, SH_BG, 64, 0);
+        launch(b, KID_relax, k_relax<S>, SH_RELAX, 256, 0);
         launch(b, KID_restrict, k_restrict<S>, SH_LVL1, 64);
         for (uint32_t l = 1; l <= pl.max_level; l++)   // level l's sweeps, and restrictLevel into l + 1
             launch_lds(b, KID_levelDown, k_levelDown<S>, SH_TILE + (int)l - 1, 256, pl.down_lds[l], l);
         launch_lds(b, KID_coarseLDS, k_coarseLDS<S>, SH_ONE, 1024, pl.coarse_lds);
         for (uint32_t l = pl.max_level; l >= 1; l--)
             launch_lds(b, KID_levelUp, k_levelUp<S>, SH_TILE + (int)l - 1, 256, pl.up_lds[l], l);
-        launch(b, KID_relax, k_relax<S>, SH_BG, 64, 1);
+        launch(b, KID_relax, k_relax<S>, SH_RELAX, 256, 1);
         launch(b, KID_cgRho, k_cgRho<S>, SH_BG16, 256);
     } else if (part == 1) {
         launch(b, KID_cgApply, k_cgApply<S>, SH_BG16, 256);

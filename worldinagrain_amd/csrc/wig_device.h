@@ -175,7 +175,18 @@ struct WorldGrid {
     const uint32_t *__restrict__ map;
     const uint32_t *__restrict__ first;
 };
+// (d is the workgroup's own copy, read once at the kernel's start: the
+// kernels' stores could otherwise be taken to change it, and each use of a
+// field would read it again from memory)
 #define WORLD(g)                                                      \
+    const uint32_t wi_ = (g).map[blockIdx.x];                         \
+    const Dev d = (g).tab[wi_];                                       \
+    const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
+    const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \
+    (void)nbid
+// (the Dev in memory, for kernels that index its per-level arrays by a
+// level known only at run time: a copy would then live in scratch)
+#define WORLD_REF(g)                                                  \
     const uint32_t wi_ = (g).map[blockIdx.x];                         \
     const Dev &d = (g).tab[wi_];                                      \
     const uint32_t bid = blockIdx.x - (g).first[wi_];                  \

@@ -13,6 +13,7 @@
 #include "wig_device.h"
 
 #define WAVE 64
+#define WAVES 4   // wavefronts a workgroup of 256
 
 __device__ __forceinline__ uint32_t ld(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1071,131 +1072,177 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t na, uint32_t wid,
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
     return j <= high ? I2(wide, j) : I2(j - high - 1, high);
 }
-template <int S> __global__ void __launch_bounds__(64) k_relax(WorldGrid g, int post) {
+// (a cell clamped into a w x h grid: an address that is always there, for
+// loads made whatever the test, their values then chosen)
+__device__ __forceinline__ i2 clampTo(i2 c, int32_t w, int32_t h) {
+    return I2(c.x < 0 ? 0 : (c.x >= w ? w - 1 : c.x), c.y < 0 ? 0 : (c.y >= h ? h - 1 : c.y));
+}
+#define NT 2   // tiles a wavefront relaxes at once
+template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int post) {
     WORLD(g);
     if (d.store != S) return;
     const Iter it = iterWords(d);
     if (!it.encoded || it.converged) return;
     const int shift = ((d.shifted_first ^ (2u * it.cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
-    __shared__ float sPx[RIM * RIM], sPy[RIM * RIM];
-    __shared__ float cRho[64], cPx[64], cPy[64];
-    uint32_t li = threadIdx.x;
-    uint32_t na = it.active, edge = shift ? d.bw + d.bh + 1u : 0u;
+    // A workgroup takes 4 x NT tiles at a time, NT a wavefront, each in LDS
+    // of its own; the barriers are the workgroup's, so every wavefront
+    // passes each of them, tiles or none.  Every load of the NT tiles is
+    // made at once and unconditionally (clamped addresses, values chosen
+    // after), so that they are in flight together.
+    __shared__ float sPxA[WAVES * NT][RIM * RIM], sPyA[WAVES * NT][RIM * RIM];
+    __shared__ float cRhoA[WAVES * NT][64], cPxA[WAVES * NT][64], cPyA[WAVES * NT][64];
+    const uint32_t li = threadIdx.x % 64u, wave = threadIdx.x / 64u;
+    const uint32_t na = it.active, edge = shift ? d.bw + d.bh + 1u : 0u;
+    const int t = (int)(li / 8u + 1u) * RIM + (int)(li % 8u) + 1;
+    const int32_t lw1 = (int32_t)d.lw[1], lh1 = (int32_t)d.lh[1];
     // the tiles: the active list's blocks, then (shifted) the edge tiles,
     // workgroup indices B .. B + edge of the reference's dispatch
-    for (uint32_t v = bid; v < na + edge; v += nbid) {
-        uint32_t wid = v < na ? v : d.B + (v - na);
-        i2 b = tileBlock(d, na, wid, shift);
-        i2 origin = I2(b.x * WIG_BLOCK - shift, b.y * WIG_BLOCK - shift);
-        if (b.x < 0 || origin.x >= d.W || origin.y >= d.H) continue;
-        // All of the tile's loads at once: the states of the blocks under
-        // it (a tile with no fluid is skipped), this thread's cell, level 1
-        // under the tile after the coarse levels, the rim buffer's cells.
-        uint32_t bits = 0;
-        {
-            i2 b0 = idiv8(origin), b1 = idiv8(I2(origin.x + WIG_BLOCK - 1, origin.y + WIG_BLOCK - 1));
-            for (int32_t y = b0.y; y <= b1.y; y++)
-                for (int32_t x = b0.x; x <= b1.x; x++)
-                    if (inBlocks(d, I2(x, y))) bits |= d.blocks[STATE_SLOT(d, blockIndex(d, I2(x, y)))];
+    for (uint32_t base = bid * WAVES * NT; base < na + edge; base += nbid * WAVES * NT) {
+        bool live[NT];
+        i2 origin[NT];
+        uint32_t bits[NT], mf[NT], mi[NT], rf[NT][2];
+        float4 cf[NT];
+        float2 r[NT], lp[NT], rv[NT][2];
+        float lr[NT];
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const uint32_t v = base + wave * NT + (uint32_t)j;
+            const uint32_t wid = v < na ? v : d.B + (v - na);
+            i2 b = v < na + edge ? tileBlock(d, na, wid, shift) : I2(-1, -1);
+            origin[j] = I2(b.x * WIG_BLOCK - shift, b.y * WIG_BLOCK - shift);
+            live[j] = !(b.x < 0 || origin[j].x >= d.W || origin[j].y >= d.H);
         }
-        i2 me = I2(origin.x + (int32_t)(li % 8u), origin.y + (int32_t)(li / 8u));
-        int t = (int)(li / 8u + 1u) * RIM + (int)(li % 8u) + 1;
-        uint32_t mf = 0, mi = 0;
-        float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float2 r = make_float2(0.0f, 0.0f);
-        if (inWorld(d, me)) {
-            mi = packIndex(d, me);
-            mf = d.cflag[mi];
-            cf = d.coef[mi];
-            r = fnLoad<S>(d, 2, mi);
-        }
-        // (level 1 from 8 x 8 cells one before the rim's first, for
-        // prolongFine's correction)
-        const int32_t cx0 = ((origin.x - 1) >> 1) - 1, cy0 = ((origin.y - 1) >> 1) - 1;
-        float lr = -1.0f;
-        float2 lp = make_float2(0.0f, 0.0f);
-        if (post) {
-            i2 cc = I2(cx0 + (int32_t)(li % 8u), cy0 + (int32_t)(li / 8u));
-            if (inLevel(d, 1u, cc)) {
-                lr = lvRho<S>(d, 1u, cc);
-                lp = lvLoadP<S>(d, 1u, cc);
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const i2 o = origin[j];
+            // the states of the (up to 2 x 2) blocks under the tile
+            const i2 b0 = idiv8(o), b1 = idiv8(I2(o.x + WIG_BLOCK - 1, o.y + WIG_BLOCK - 1));
+            bits[j] = 0u;
+            for (int q = 0; q < 4; q++) {
+                const i2 bq = I2(q & 1 ? b1.x : b0.x, q & 2 ? b1.y : b0.y);
+                const bool use = live[j] && inBlocks(d, bq) && (q != 1 || b1.x != b0.x) && (q != 2 || b1.y != b0.y) && (q != 3 || (b1.x != b0.x && b1.y != b0.y));
+                const uint32_t sb = d.blocks[STATE_SLOT(d, inBlocks(d, bq) ? blockIndex(d, bq) : 0u)];
+                bits[j] |= use ? sb : 0u;
+            }
+            // this thread's cell
+            const i2 me = I2(o.x + (int32_t)(li % 8u), o.y + (int32_t)(li / 8u));
+            const bool meIn = live[j] && inWorld(d, me);
+            mi[j] = packIndex(d, clampTo(me, d.W, d.H));
+            const uint32_t f = d.cflag[mi[j]];
+            const float4 c4 = d.coef[mi[j]];
+            const float2 r2 = fnLoad<S>(d, 2, mi[j]);
+            mf[j] = meIn ? f : 0u;
+            cf[j] = c4;
+            r[j] = r2;
+            // level 1 from 8 x 8 cells one before the rim's first, for
+            // prolongFine's correction
+            const int32_t cx0 = ((o.x - 1) >> 1) - 1, cy0 = ((o.y - 1) >> 1) - 1;
+            lr[j] = -1.0f;
+            lp[j] = make_float2(0.0f, 0.0f);
+            if (post) {
+                const i2 cc = I2(cx0 + (int32_t)(li % 8u), cy0 + (int32_t)(li / 8u)), ck = clampTo(cc, lw1, lh1);
+                const float rr = lvRho<S>(d, 1u, ck);
+                const float2 pp = lvLoadP<S>(d, 1u, ck);
+                const bool ccIn = inLevel(d, 1u, cc);
+                lr[j] = ccIn ? rr : -1.0f;
+                lp[j] = ccIn ? pp : make_float2(0.0f, 0.0f);
+            }
+            // the rim buffer's cells (two a thread)
+            for (uint32_t q = 0; q < 2u; q++) {
+                const uint32_t k = li + 64u * q;
+                const i2 c = I2(o.x - 1 + (int32_t)(k % RIM), o.y - 1 + (int32_t)(k / RIM));
+                const bool cIn = live[j] && k < RIM * RIM && inWorld(d, c);
+                const uint32_t ci = packIndex(d, clampTo(c, d.W, d.H));
+                const float2 vv = fnLoad<S>(d, post ? 1 : 0, ci);
+                const uint32_t ff = d.cflag[ci];
+                rv[j][q] = cIn ? vv : make_float2(0.0f, 0.0f);
+                rf[j][q] = (cIn && post) ? ff : 0u;
             }
         }
-        float2 rv[2] = {make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)};
-        uint32_t rf[2] = {0u, 0u};
-        for (uint32_t q = 0; q < 2u; q++) {
-            uint32_t k = li + 64u * q;
-            i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
-            if (k < RIM * RIM && inWorld(d, c)) {
-                uint32_t ci = packIndex(d, c);
-                rv[q] = fnLoad<S>(d, post ? 1 : 0, ci);
-                if (post) rf[q] = d.cflag[ci];
-            }
-        }
-        __asm__ __volatile__("" ::: "memory");   // (the loads above issue before the test)
-        if ((bits & HAS_FLUID) == 0u) continue;
-        const bool fluid = (mf & CF_FLUID) != 0u, pinned = fluid && (mf & CF_PINNED) != 0u;
-        float rhsx = 0.0f, rhsy = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
-        if (fluid) {
-            b0 = fabsf(cf.x);
-            b1 = fabsf(cf.y);
-            b2 = fabsf(cf.z);
-            b3 = fabsf(cf.w);
-            rhsx = r.x;
-            rhsy = r.y;
-        }
         if (post) {
-            cRho[li] = lr;
-            cPx[li] = lp.x;
-            cPy[li] = lp.y;
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                const uint32_t s_ = wave * NT + (uint32_t)j;
+                cRhoA[s_][li] = lr[j];
+                cPxA[s_][li] = lp[j].x;
+                cPyA[s_][li] = lp[j].y;
+            }
             __syncthreads();
         }
-        for (uint32_t q = 0; q < 2u; q++) {
-            uint32_t k = li + 64u * q;
-            if (k >= RIM * RIM) continue;
-            i2 c = I2(origin.x - 1 + (int32_t)(k % RIM), origin.y - 1 + (int32_t)(k / RIM));
-            float2 v = rv[q];
-            if (post && (rf[q] & CF_FLUID)) {
-                // prolonged() (:2458-2497) from the staged cells
-                i2 coarse = I2(c.x / 2, c.y / 2);
-                i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
-                float sx = 0.0f, sy = 0.0f, weight = 0.0f;
-                for (int o4 = 0; o4 < 4; o4++) {
-                    i2 o = o4 == 0 ? I2(0, 0) : (o4 == 1 ? I2(sg.x, 0) : (o4 == 2 ? I2(0, sg.y) : sg));
-                    float wk = o4 == 0 ? 9.0f : (o4 == 3 ? 1.0f : 3.0f);
-                    int32_t j = (coarse.y + o.y - cy0) * 8 + (coarse.x + o.x - cx0);
-                    float rc = cRho[j];
-                    if (rc < 0.0f) continue;
-                    weight += wk;
-                    if (rc > 0.0f) {
-                        sx += wk * cPx[j];
-                        sy += wk * cPy[j];
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const uint32_t s_ = wave * NT + (uint32_t)j;
+            const i2 o = origin[j];
+            const int32_t cx0 = ((o.x - 1) >> 1) - 1, cy0 = ((o.y - 1) >> 1) - 1;
+            for (uint32_t q = 0; q < 2u; q++) {
+                uint32_t k = li + 64u * q;
+                if (k >= RIM * RIM) continue;
+                i2 c = I2(o.x - 1 + (int32_t)(k % RIM), o.y - 1 + (int32_t)(k / RIM));
+                float2 v2_ = rv[j][q];
+                if (post && (rf[j][q] & CF_FLUID)) {
+                    // prolonged() (:2458-2497) from the staged cells
+                    i2 coarse = I2(c.x / 2, c.y / 2);
+                    i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
+                    float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+                    for (int o4 = 0; o4 < 4; o4++) {
+                        i2 oo = o4 == 0 ? I2(0, 0) : (o4 == 1 ? I2(sg.x, 0) : (o4 == 2 ? I2(0, sg.y) : sg));
+                        float wk = o4 == 0 ? 9.0f : (o4 == 3 ? 1.0f : 3.0f);
+                        int32_t jj = (coarse.y + oo.y - cy0) * 8 + (coarse.x + oo.x - cx0);
+                        float rc = cRhoA[s_][jj];
+                        if (rc < 0.0f) continue;
+                        weight += wk;
+                        if (rc > 0.0f) {
+                            sx += wk * cPxA[s_][jj];
+                            sy += wk * cPyA[s_][jj];
+                        }
                     }
+                    float dn = fmaxf(weight, 1e-6f);
+                    v2_.x = v2_.x + sx / dn;
+                    v2_.y = v2_.y + sy / dn;
                 }
-                float dn = fmaxf(weight, 1e-6f);
-                v.x = v.x + sx / dn;
-                v.y = v.y + sy / dn;
+                sPxA[s_][k] = v2_.x;
+                sPyA[s_][k] = v2_.y;
             }
-            sPx[k] = v.x;
-            sPy[k] = v.y;
         }
         __syncthreads();
-        float diag = b0 + b1 + b2 + b3;
-        float dx = diag + (pinned ? PINF : 0.0f), dy = fmaxf(diag, 1e-30f);
-        bool solving = dx > 0.0f;
-        int color = (me.x + me.y) & 1;
+        float b0[NT], b1[NT], b2[NT], b3[NT], rhsx[NT], rhsy[NT], diag[NT], dx[NT], dy[NT];
+        bool fluid[NT], solving[NT];
+        int color[NT];
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            live[j] = live[j] && (bits[j] & HAS_FLUID) != 0u;
+            fluid[j] = live[j] && (mf[j] & CF_FLUID) != 0u;
+            const bool pinned = fluid[j] && (mf[j] & CF_PINNED) != 0u;
+            b0[j] = fluid[j] ? fabsf(cf[j].x) : 0.0f;
+            b1[j] = fluid[j] ? fabsf(cf[j].y) : 0.0f;
+            b2[j] = fluid[j] ? fabsf(cf[j].z) : 0.0f;
+            b3[j] = fluid[j] ? fabsf(cf[j].w) : 0.0f;
+            rhsx[j] = fluid[j] ? r[j].x : 0.0f;
+            rhsy[j] = fluid[j] ? r[j].y : 0.0f;
+            diag[j] = b0[j] + b1[j] + b2[j] + b3[j];
+            dx[j] = diag[j] + (pinned ? PINF : 0.0f);
+            dy[j] = fmaxf(diag[j], 1e-30f);
+            solving[j] = dx[j] > 0.0f;
+            color[j] = (origin[j].x + (int32_t)(li % 8u) + origin[j].y + (int32_t)(li / 8u)) & 1;
+        }
         for (int sw = 0; sw < 2 * (int)d.sweep_pairs; sw++) {
-            if (solving && color == (sw & 1)) {
-                float sx = b0 * sPx[t - 1] + b1 * sPx[t + 1] + b2 * sPx[t - RIM] + b3 * sPx[t + RIM];
-                float sy = b0 * sPy[t - 1] + b1 * sPy[t + 1] + b2 * sPy[t - RIM] + b3 * sPy[t + RIM];
-                float nx = wmix(sPx[t], (sx - rhsx) / dx, SMOOTH_OMEGA);
-                float ny = wmix(sPy[t], (sy - rhsy) / dy, SMOOTH_OMEGA);
-                sPx[t] = nx;
-                if (diag > 0.0f) sPy[t] = ny;
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                if (solving[j] && color[j] == (sw & 1)) {
+                    float *px = sPxA[wave * NT + (uint32_t)j], *py = sPyA[wave * NT + (uint32_t)j];
+                    float sx = b0[j] * px[t - 1] + b1[j] * px[t + 1] + b2[j] * px[t - RIM] + b3[j] * px[t + RIM];
+                    float sy = b0[j] * py[t - 1] + b1[j] * py[t + 1] + b2[j] * py[t - RIM] + b3[j] * py[t + RIM];
+                    float nx = wmix(px[t], (sx - rhsx[j]) / dx[j], SMOOTH_OMEGA);
+                    float ny = wmix(py[t], (sy - rhsy[j]) / dy[j], SMOOTH_OMEGA);
+                    px[t] = nx;
+                    if (diag[j] > 0.0f) py[t] = ny;
+                }
             }
             __syncthreads();
         }
-        if (fluid) fnStore<S>(d, post ? 0 : 1, mi, make_float2(sPx[t], sPy[t]));
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            if (fluid[j]) fnStore<S>(d, post ? 0 : 1, mi[j], make_float2(sPxA[wave * NT + (uint32_t)j][t], sPyA[wave * NT + (uint32_t)j][t]));
         __syncthreads();
     }
 }
@@ -1417,7 +1464,7 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
     }
 }
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t l) {
-    WORLD(g);
+    WORLD_REF(g);
     if (d.store != S) return;
     const Iter it = iterWords(d);
     const uint32_t T = d.tile[l];
@@ -1498,7 +1545,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     }
 }
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
-    WORLD(g);
+    WORLD_REF(g);
     if (d.store != S) return;
     const Iter it = iterWords(d);
     const uint32_t T = d.tile[l];
@@ -1631,7 +1678,7 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
     __syncthreads();
 }
 template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
-    WORLD(g);
+    WORLD_REF(g);
     if (d.store != S) return;
     const Iter it = iterWords(d);
     if (!it.encoded || it.converged) return;
@@ -1834,7 +1881,6 @@ __global__ void __launch_bounds__(64) k_cgAnchor(WorldGrid g) {                 
 // taking places i, i + 256, ..., then halving: the same order every run.
 // (The sum is a launch of its own: a pass whose last workgroup took it would
 // need every workgroup to fence its partials out to all XCDs' caches.)
-#define WAVES 4   // wavefronts (blocks of cells) a workgroup of 256
 
 __device__ __forceinline__ double ld_f64(const double *p) {
     unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
