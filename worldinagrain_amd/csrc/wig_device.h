@@ -184,8 +184,30 @@ struct WorldGrid {
     const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
     const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \
     (void)nbid
+// The level kernels' view: level l of the world as level 1 of the copy and
+// level l + 1 as its level 2 (a copy indexed by a level known only at run
+// time would live in scratch; constant indices keep it in registers), and
+// level l's tile side.
+#define WORLD_LEVEL(g, l)                                             \
+    const uint32_t wi_ = (g).map[blockIdx.x];                         \
+    const Dev &dg_ = (g).tab[wi_];                                    \
+    Dev d = dg_;                                                      \
+    d.lw[1] = dg_.lw[l];                                              \
+    d.lh[1] = dg_.lh[l];                                              \
+    d.lstart[1] = dg_.lstart[l];                                      \
+    d.vlev[1] = dg_.vlev[l];                                          \
+    d.vrho[1] = dg_.vrho[l];                                          \
+    d.lw[2] = dg_.lw[(l) + 1u];                                       \
+    d.lh[2] = dg_.lh[(l) + 1u];                                       \
+    d.lstart[2] = dg_.lstart[(l) + 1u];                               \
+    d.vlev[2] = dg_.vlev[(l) + 1u];                                   \
+    d.vrho[2] = dg_.vrho[(l) + 1u];                                   \
+    const uint32_t tile_ = dg_.tile[l];                               \
+    const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
+    const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \
+    (void)nbid
 // (the Dev in memory, for kernels that index its per-level arrays by a
-// level known only at run time: a copy would then live in scratch)
+// level known only at run time throughout)
 #define WORLD_REF(g)                                                  \
     const uint32_t wi_ = (g).map[blockIdx.x];                         \
     const Dev &d = (g).tab[wi_];                                      \

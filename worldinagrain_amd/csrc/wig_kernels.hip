@@ -1464,11 +1464,11 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
     }
 }
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(WorldGrid g, uint32_t l) {
-    WORLD_REF(g);
+    WORLD_LEVEL(g, l);
     if (d.store != S) return;
     const Iter it = iterWords(d);
-    const uint32_t T = d.tile[l];
-    const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
+    const uint32_t T = tile_;
+    const uint32_t gx_ = (d.lw[1] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
     if (!it.encoded || it.converged || it.active == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs + 1u;
@@ -1476,13 +1476,12 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     float *resx = sm + 7u * (uint32_t)(t.R * t.R), *resy = resx + T * T;
     FOR_TILE(t, x, y) {
         int32_t j = y * t.R + x;
-        i2 c = I2(t.ox + x, t.oy + y);
-        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float rho = -1.0f;
-        if (inLevel(d, l, c)) {
-            rho = lvRho<S>(d, l, c);
-            e = lvLoad<S>(d, l, c);
-        }
+        i2 c = I2(t.ox + x, t.oy + y), cc = clampTo(c, (int32_t)d.lw[1], (int32_t)d.lh[1]);
+        const bool in = inLevel(d, 1u, c);
+        const float rr = lvRho<S>(d, 1u, cc);
+        const float4 ee = lvLoad<S>(d, 1u, cc);
+        const float4 e = in ? ee : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float rho = in ? rr : -1.0f;
         t.px[j] = e.x;
         t.py[j] = e.y;
         t.rx[j] = e.z;
@@ -1506,7 +1505,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
         }
         resx[k] = qx;
         resy[k] = qy;
-        if (inLevel(d, l, c)) lvStoreP<S>(d, l, c, make_float2(t.px[j], t.py[j]));
+        if (inLevel(d, 1u, c)) lvStoreP<S>(d, 1u, c, make_float2(t.px[j], t.py[j]));
     }
     __syncthreads();
     // restrictLevel (:2342-2377) into the tile's parents
@@ -1514,7 +1513,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     const int32_t span = 1 << lc;
     for (uint32_t k = threadIdx.x; k < T2 * T2; k += TILE_THREADS) {
         i2 c = I2((int32_t)(bx * T2 + k % T2), (int32_t)(by * T2 + k / T2));
-        if (!inLevel(d, lc, c)) continue;
+        if (!inLevel(d, 2u, c)) continue;
         float rho = 0.0f, ox = 0.0f, oy = 0.0f;
         bool shortcut = false;
         if (span <= WIG_BLOCK) {
@@ -1540,16 +1539,16 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
             ox = rho > 0.0f ? sx * 0.25f : 0.0f;
             oy = rho > 0.0f ? sy * 0.25f : 0.0f;
         }
-        lvStore<S>(d, lc, c, make_float4(0.0f, 0.0f, ox, oy));
-        lvStoreRho<S>(d, lc, c, rho);
+        lvStore<S>(d, 2u, c, make_float4(0.0f, 0.0f, ox, oy));
+        lvStoreRho<S>(d, 2u, c, rho);
     }
 }
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
-    WORLD_REF(g);
+    WORLD_LEVEL(g, l);
     if (d.store != S) return;
     const Iter it = iterWords(d);
-    const uint32_t T = d.tile[l];
-    const uint32_t gx_ = (d.lw[l] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
+    const uint32_t T = tile_;
+    const uint32_t gx_ = (d.lw[1] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
     if (!it.encoded || it.converged || it.active == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs;
@@ -1558,27 +1557,22 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
     const int32_t cx0 = (t.ox >> 1) - 1, cy0 = (t.oy >> 1) - 1, Rc = t.R / 2 + 3;
     float *cpx = sm + 7u * (uint32_t)(t.R * t.R), *cpy = cpx + Rc * Rc, *crho = cpy + Rc * Rc;
     for (int32_t k = (int32_t)threadIdx.x; k < Rc * Rc; k += (int32_t)TILE_THREADS) {
-        i2 c = I2(cx0 + k % Rc, cy0 + k / Rc);
-        float px = 0.0f, py = 0.0f, rho = -1.0f;
-        if (inLevel(d, l + 1u, c)) {
-            float2 e = lvLoadP<S>(d, l + 1u, c);
-            px = e.x;
-            py = e.y;
-            rho = lvRho<S>(d, l + 1u, c);
-        }
-        cpx[k] = px;
-        cpy[k] = py;
-        crho[k] = rho;
+        i2 c = I2(cx0 + k % Rc, cy0 + k / Rc), cc = clampTo(c, (int32_t)d.lw[2], (int32_t)d.lh[2]);
+        const bool in = inLevel(d, 2u, c);
+        const float2 e = lvLoadP<S>(d, 2u, cc);
+        const float rr = lvRho<S>(d, 2u, cc);
+        cpx[k] = in ? e.x : 0.0f;
+        cpy[k] = in ? e.y : 0.0f;
+        crho[k] = in ? rr : -1.0f;
     }
     FOR_TILE(t, x, y) {
         int32_t j = y * t.R + x;
-        i2 c = I2(t.ox + x, t.oy + y);
-        float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float rho = -1.0f;
-        if (inLevel(d, l, c)) {
-            e = lvLoad<S>(d, l, c);
-            rho = lvRho<S>(d, l, c);
-        }
+        i2 c = I2(t.ox + x, t.oy + y), cc = clampTo(c, (int32_t)d.lw[1], (int32_t)d.lh[1]);
+        const bool in = inLevel(d, 1u, c);
+        const float4 ee = lvLoad<S>(d, 1u, cc);
+        const float rr = lvRho<S>(d, 1u, cc);
+        const float4 e = in ? ee : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float rho = in ? rr : -1.0f;
         t.px[j] = e.x;
         t.py[j] = e.y;
         t.rx[j] = e.z;
@@ -1617,8 +1611,8 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
     for (uint32_t k = threadIdx.x; k < T * T; k += TILE_THREADS) {
         int32_t x = (int32_t)(k % T + H), y = (int32_t)(k / T + H), j = y * t.R + x;
         i2 c = I2(t.ox + x, t.oy + y);
-        if (!inLevel(d, l, c)) continue;
-        lvStoreP<S>(d, l, c, make_float2(t.px[j], t.py[j]));
+        if (!inLevel(d, 1u, c)) continue;
+        lvStoreP<S>(d, 1u, c, make_float2(t.px[j], t.py[j]));
     }
 }
 // The coarse levels from lds_level down, all in one workgroup's LDS:
@@ -2114,12 +2108,11 @@ template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
             po[j] = pold[i[j]];
             cf[j] = d.coef[i[j]];
         }
-        // the neighbours' p across the block's edges (newP there)
+        // the neighbours' p across the block's edges (newP there; a lane
+        // off that edge loads its own cell's, unused, so that no load waits
+        // on a branch)
         for (uint32_t j = 0; j < NB; j++)
-            for (int k = 0; k < 4; k++) {
-                pe[j][k] = 0.0f;
-                if (edge[k]) pe[j][k] = newP<S>(d, acrossEdge(d, i[j], k), beta, pold);
-            }
+            for (int k = 0; k < 4; k++) pe[j][k] = newP<S>(d, edge[k] ? acrossEdge(d, i[j], k) : i[j], beta, pold);
         for (uint32_t j = 0; j < NB; j++) {
             // cgDirection's p (newP) at this cell, the neighbours' inside the
             // block from their lanes
