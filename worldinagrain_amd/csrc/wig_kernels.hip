@@ -1871,10 +1871,12 @@ __global__ void __launch_bounds__(64) k_cgAnchor(WorldGrid g) {                 
 }
 // ------------------------------------------------ FP64 sums
 // A pass's block partials are kept by the block's place in the active list
-// (which is in block-index order) and summed by one workgroup of 256, lane i
-// taking places i, i + 256, ..., then halving: the same order every run.
-// (The sum is a launch of its own: a pass whose last workgroup took it would
-// need every workgroup to fence its partials out to all XCDs' caches.)
+// (which is in block-index order) and summed by one workgroup of 1024, lane
+// i taking places i, i + 1024, ..., then halving: the same order every run.
+// (The sum is a launch of its own.  A pass whose last workgroup took it needs
+// every workgroup to fence its partials out of its XCD's L2, a write-back
+// each: on MI355X that made cgRho, cgApply and cgUpdate five to ten times
+// slower than the one-workgroup launch costs.)
 
 __device__ __forceinline__ double ld_f64(const double *p) {
     unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
