@@ -148,16 +148,19 @@ def main():
         sim.step()
     # the coarse-level tile sides, by the autotuner on the world's own steps
     # (they do not change results)
-    tiles, sweeps, lds_level = {}, None, None
+    tiles, sweeps, lds_level, coarse_wave = {}, None, None, None
     if not args.no_tune:
         from worldinagrain_amd.tune import tune_tiles, tune_sweeps, tune_lds_level
-        lds_level = tune_lds_level(world, sim)
+        lds_level, coarse_wave = tune_lds_level(world, sim)
         tiles = tune_tiles(world, sim)
         # and the sweep counts, on copies of the world (they change the
-        # iteration, so each is scored by its solve time at the tolerance)
+        # iteration, so each is scored by its solve time at the tolerance);
+        # then the speed settings again, for those counts
         solver, _ = tune_sweeps(world, sim)
         sim.solver = solver
         sweeps = [solver.sweep_pairs, solver.level_pairs, solver.coarse_sweeps]
+        lds_level, coarse_wave = tune_lds_level(world, sim)
+        tiles = tune_tiles(world, sim)
         for _ in range(2):
             sim.step()
     torch.cuda.synchronize()
@@ -228,7 +231,7 @@ def main():
                    "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
                    "tol": args.tol, "parallelism": "single world, one GPU", "graphs": bool(args.graphs),
                    "tiles": {str(k): v for k, v in tiles.items()},
-                   "sweeps": sweeps, "lds_level": lds_level},
+                   "sweeps": sweeps, "lds_level": lds_level, "coarse_wave_cells": coarse_wave},
         "solves_per_s": solves, "pcg_iterations_per_solve": d[1] / K,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
         "step_roofline_frac": step_roof / HBM_PEAK_GBS, "step_achieved_GBs": step_roof,
@@ -257,7 +260,7 @@ def run_batch(args):
     from worldinagrain_amd import abi, scenes, elements as E
     from worldinagrain_amd.gpu import GpuWorld, GpuBatch
     from worldinagrain_amd.host import step_batch
-    from worldinagrain_amd.tune import tune_sweeps, tune_tiles_batch
+    from worldinagrain_amd.tune import tune_sweeps, tune_tiles_batch, tune_coarse_wave_batch
 
     kinds = E.kinds()
     sides = scenes.batch_sides(args.worlds, seed=7)
@@ -296,6 +299,8 @@ def run_batch(args):
         groups = {side: [i for i, s in enumerate(sides) if s == side] for side in sorted(set(sides))}
         for side, tiles in tune_tiles_batch(batch, sims, groups).items():
             tuned[side]["tiles"] = tiles
+        for side, cells in tune_coarse_wave_batch(batch, sims, groups).items():
+            tuned[side]["coarse_wave_cells"] = cells
     from worldinagrain_amd.host import BatchHost
     host = BatchHost(sims)          # the per-step host words for all worlds at once
     host.step(batch)

@@ -207,6 +207,12 @@ int wig_set_tile(WigWorld *w, uint32_t level, uint32_t side);
  * library's own. */
 int wig_set_lds_level(WigWorld *w, uint32_t level);
 
+/* Within that workgroup, the coarse levels of at most `cells` cells below
+ * its first are worked by one wavefront, without the workgroup's barriers
+ * (0: none; 256 by default).  Results do not depend on it; the autotuner
+ * sets it.  This library's own. */
+int wig_set_coarse_wave(WigWorld *w, uint32_t cells);
+
 /* Waits for everything enqueued on the world's stream. */
 int wig_sync(WigWorld *w);
 

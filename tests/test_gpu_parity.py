@@ -77,14 +77,17 @@ def test_tiled_levels_reference_settings():
 
 def test_tile_sides_do_not_change_results():
     """The tiles' halos make their sweeps the whole-level passes': any tile
-    side, and any level for the one-workgroup kernel to start at, gives the
-    same bits; so does a clone of a world stepped alike."""
+    side, any level for the one-workgroup kernel to start at, and any split
+    of its levels between the workgroup and one wavefront gives the same
+    bits; so does a clone of a world stepped alike."""
     W, H = 330, 250
     cells, _ = scenes.discs_dam(W, H, discs=40, rmin=3, rmax=12)
     wa, wb, wl = make_gpu(W, H, cells), make_gpu(W, H, cells), make_gpu(W, H, cells)
     sa, sb, sl = config_sim(wa, W, H), config_sim(wb, W, H), config_sim(wl, W, H)
     wb.set_tile(1, 8)
     wb.set_tile(2, 32)
+    wb.set_coarse_wave(0)                    # every coarse level by the whole workgroup
+    wa.set_coarse_wave(1 << 20)              # every one below the first by one wavefront
     wl.set_lds_level(wl.layout.levels)       # every level but the coarsest in tiles
     wl.set_tile(3, 8)
     with pytest.raises(Exception):

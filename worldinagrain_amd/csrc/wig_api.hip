@@ -431,6 +431,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         return nullptr;
     }
     for (uint32_t l = 0; l <= MG_LEVELS; l++) w->tile[l] = d.tile[l] = 16u;
+    d.wave_cells = 256u;
     d.level_pairs = 1u;
     d.cells = (uint32_t *)w->part[WIG_PART_CELLS];
     d.sorted = (uint32_t *)w->part[WIG_PART_PARTICLES0];
@@ -635,6 +636,12 @@ int wig_set_lds_level(WigWorld *w, uint32_t level) {
     d.lds_level = level;
     w->lds_bytes = 0;
     for (uint32_t k = level; k <= d.L; k++) w->lds_bytes += 4ull * LDS_LEVEL_FLOATS * d.lw[k] * d.lh[k];
+    return WIG_OK;
+}
+
+int wig_set_coarse_wave(WigWorld *w, uint32_t cells) {
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    w->d.wave_cells = cells;
     return WIG_OK;
 }
 

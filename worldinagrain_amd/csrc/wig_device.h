@@ -156,6 +156,7 @@ struct Dev {
     uint16_t *vlev[MG_LEVELS + 1];
     float *vrho[MG_LEVELS + 1];
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
+    uint32_t wave_cells;    // and below it, levels of at most this many cells by its first wavefront
     uint32_t tile[MG_LEVELS + 1];   // k_levelDown/k_levelUp's tile side, per level below lds_level
     // the step's: the Params words the liquid reads, and its controls
     uint32_t frame, seed;

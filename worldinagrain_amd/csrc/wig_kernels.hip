@@ -1624,17 +1624,33 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
 // arithmetic is restrictLevel's, smoothAt's, solve's and prolong's, in the
 // same order; only where the values wait between passes differs.  A level's
 // face betas (faceBeta, :369-374) are taken once, when its rho is known.
+//
+// The first level is worked by the whole workgroup (its barriers), and so
+// are the next while they have more than the world's wave_cells cells; the
+// rest by its first wavefront alone, whose lanes pass each phase in step (a
+// wavefront's syncs, no barriers), while the other wavefronts wait at the
+// next barrier.  Every phase updates its
+// cells independently of one another, so how cells are dealt to threads does
+// not change a value.
 struct LdsLevel { float *px, *py, *rx, *ry, *rho, *bR, *bD; uint32_t w, h; };
-
+// a phase's end: the workgroup's barrier, or the first wavefront's sync
+__device__ __forceinline__ void phaseSync(bool wg) {
+    if (wg) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
 // each cell's right and lower face betas, 0 where the stencil skips
-__device__ void ldsBetas(const LdsLevel &L) {
-    for (uint32_t y = threadIdx.x / 32u; y < L.h; y += 32u)
-        for (uint32_t x = threadIdx.x % 32u; x < L.w; x += 32u) {
-            uint32_t j = y * L.w + x;
-            float r = L.rho[j];
-            L.bR[j] = x + 1u < L.w ? faceBeta(r, L.rho[j + 1u]) : 0.0f;
-            L.bD[j] = y + 1u < L.h ? faceBeta(r, L.rho[j + L.w]) : 0.0f;
-        }
+__device__ void ldsBetas(const LdsLevel &L, uint32_t tid, uint32_t nthr) {
+    for (uint32_t j = tid; j < L.w * L.h; j += nthr) {
+        uint32_t x = j % L.w, y = j / L.w;
+        float r = L.rho[j];
+        L.bR[j] = x + 1u < L.w ? faceBeta(r, L.rho[j + 1u]) : 0.0f;
+        L.bD[j] = y + 1u < L.h ? faceBeta(r, L.rho[j + L.w]) : 0.0f;
+    }
 }
 struct LdsSum { float sx, sy, diag; };
 __device__ __forceinline__ LdsSum ldsStencil(const LdsLevel &L, uint32_t x, uint32_t y) {
@@ -1650,26 +1666,91 @@ __device__ __forceinline__ LdsSum ldsStencil(const LdsLevel &L, uint32_t x, uint
     }
     return s;
 }
-// one half-sweep: the cells of one colour among the first `limit`, 32 x 32
-// threads, a thread a cell of the colour
-__device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega, uint32_t limit) {
+// one half-sweep: the cells of one colour among the first `limit`
+__device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega, uint32_t limit, uint32_t tid, uint32_t nthr, bool wg) {
     float h2 = (float)(1u << (2u * l));
     uint32_t hw = (L.w + 1u) / 2u;
-    for (uint32_t y = threadIdx.x / 32u; y < L.h; y += 32u) {
-        uint32_t par = ((uint32_t)color + y) & 1u;
-        for (uint32_t i = threadIdx.x % 32u; i < hw; i += 32u) {
-            uint32_t x = 2u * i + par, j = y * L.w + x;
-            if (x >= L.w || j >= limit) continue;
-            float rho = L.rho[j];
-            if (rho <= 0.0f) continue;
-            LdsSum s = ldsStencil(L, x, y);
-            if (s.diag <= 0.0f) continue;
-            float sx = (s.sx - h2 * L.rx[j]) / s.diag, sy = (s.sy - h2 * L.ry[j]) / s.diag;
-            L.px[j] = wmix(L.px[j], sx, omega);
-            L.py[j] = wmix(L.py[j], sy, omega);
-        }
+    for (uint32_t k = tid; k < hw * L.h; k += nthr) {
+        uint32_t y = k / hw, i = k % hw;
+        uint32_t x = 2u * i + (((uint32_t)color + y) & 1u), j = y * L.w + x;
+        if (x >= L.w || j >= limit) continue;
+        float rho = L.rho[j];
+        if (rho <= 0.0f) continue;
+        LdsSum s = ldsStencil(L, x, y);
+        if (s.diag <= 0.0f) continue;
+        float sx = (s.sx - h2 * L.rx[j]) / s.diag, sy = (s.sy - h2 * L.ry[j]) / s.diag;
+        L.px[j] = wmix(L.px[j], sx, omega);
+        L.py[j] = wmix(L.py[j], sy, omega);
     }
-    __syncthreads();
+    phaseSync(wg);
+}
+// restrictLevel (:2342-2377) from level l (T) into level l + 1 (U): each
+// child's residual (residualAt, :2306-2315) summed
+__device__ void ldsRestrict(const Dev &d, const LdsLevel &T, const LdsLevel &U, uint32_t l, uint32_t tid, uint32_t nthr) {
+    const float h2 = (float)(1u << (2u * l));
+    const int32_t span = 1 << (l + 1u);
+    for (uint32_t j = tid; j < U.w * U.h; j += nthr) {
+        uint32_t x = j % U.w, y = j / U.w;
+        float rho = 0.0f, rxv = 0.0f, ryv = 0.0f;
+        bool shortcut = false;
+        if (span <= WIG_BLOCK) {
+            i2 b = I2((int32_t)x * span / WIG_BLOCK, (int32_t)y * span / WIG_BLOCK);
+            shortcut = inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u;
+        }
+        if (!shortcut) {
+            float fluid = 0.0f, sum = 0.0f, sx = 0.0f, sy = 0.0f;
+            bool air = false;
+            for (int q = 0; q < 4; q++) {
+                uint32_t cx = 2u * x + (uint32_t)(q & 1), cy = 2u * y + (uint32_t)(q >> 1);
+                if (cx >= T.w || cy >= T.h) continue;     // outside the level: rho -1
+                uint32_t jc = cy * T.w + cx;
+                float rk = T.rho[jc];
+                if (rk == 0.0f) air = true;
+                else if (rk > 0.0f) {
+                    fluid += 1.0f;
+                    sum += rk;
+                    LdsSum st = ldsStencil(T, cx, cy);
+                    sx += T.rx[jc] - (st.sx - (st.diag + 0.0f) * T.px[jc]) / h2;
+                    sy += T.ry[jc] - (st.sy - (st.diag + 0.0f) * T.py[jc]) / h2;
+                }
+            }
+            rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
+            rxv = rho > 0.0f ? sx * 0.25f : 0.0f;
+            ryv = rho > 0.0f ? sy * 0.25f : 0.0f;
+        }
+        U.px[j] = 0.0f;
+        U.py[j] = 0.0f;
+        U.rx[j] = rxv;
+        U.ry[j] = ryv;
+        U.rho[j] = rho;
+    }
+}
+// prolong (:2459-2497) from level l + 1 (U) into level l (T)
+__device__ void ldsProlong(const LdsLevel &T, const LdsLevel &U, uint32_t tid, uint32_t nthr) {
+    for (uint32_t j = tid; j < T.w * T.h; j += nthr) {
+        uint32_t x = j % T.w, y = j / T.w;
+        if (T.rho[j] <= 0.0f) continue;
+        i2 coarse = I2((int32_t)x / 2, (int32_t)y / 2);
+        i2 sg = I2((x & 1u) == 1u ? 1 : -1, (y & 1u) == 1u ? 1 : -1);
+        float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+        for (int k = 0; k < 4; k++) {
+            i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
+            float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
+            i2 at = I2(coarse.x + o.x, coarse.y + o.y);
+            if (at.x < 0 || at.y < 0 || at.x >= (int32_t)U.w || at.y >= (int32_t)U.h) continue;   // rho -1
+            uint32_t ja = (uint32_t)at.y * U.w + (uint32_t)at.x;
+            float r = U.rho[ja];
+            if (r < 0.0f) continue;
+            weight += wk;
+            if (r > 0.0f) {
+                sx += wk * U.px[ja];
+                sy += wk * U.py[ja];
+            }
+        }
+        float dn = fmaxf(weight, 1e-6f);
+        T.px[j] = T.px[j] + sx / dn;
+        T.py[j] = T.py[j] + sy / dn;
+    }
 }
 template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
     WORLD_REF(g);
@@ -1679,14 +1760,17 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
     extern __shared__ float sm[];
     const uint32_t l0 = it.active == 0u ? d.L : d.lds_level, L = d.L;
     const bool idle = it.active == 0u;   // restrict and prolong pass over nothing; solve still sweeps
-    // level l's arrays in LDS, levels l0 .. L one after another
+    const uint32_t tid = threadIdx.x, W = (uint32_t)d.W, H = (uint32_t)d.H;
+    const uint32_t pairs = d.level_pairs, csweeps = d.coarse_sweeps;
+    // level l's arrays in LDS, levels l0 .. L one after another (the level
+    // sizes as the world's: side rounded up at each halving)
     auto level = [&](uint32_t l) {
         uint32_t off = 0;
-        for (uint32_t k = l0; k < l; k++) off += LDS_LEVEL_FLOATS * d.lw[k] * d.lh[k];
-        uint32_t n = d.lw[l] * d.lh[l];
+        for (uint32_t k = l0; k < l; k++) off += LDS_LEVEL_FLOATS * ((W + (1u << k) - 1u) >> k) * ((H + (1u << k) - 1u) >> k);
         LdsLevel T;
-        T.w = d.lw[l];
-        T.h = d.lh[l];
+        T.w = (W + (1u << l) - 1u) >> l;
+        T.h = (H + (1u << l) - 1u) >> l;
+        uint32_t n = T.w * T.h;
         T.px = sm + off;
         T.py = sm + off + n;
         T.rx = sm + off + 2u * n;
@@ -1700,116 +1784,88 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
         // level l0 as the pass above left it in the field (or, with no
         // blocks worked on, the coarsest as it was)
         const LdsLevel T = level(l0);
-        for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
-            for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
-                uint32_t j = y * T.w + x;
-                float4 e = lvLoad<S>(d, l0, I2((int32_t)x, (int32_t)y));
-                T.px[j] = e.x; T.py[j] = e.y; T.rx[j] = e.z; T.ry[j] = e.w;
-                T.rho[j] = lvRho<S>(d, l0, I2((int32_t)x, (int32_t)y));
-            }
-        __syncthreads();
-        ldsBetas(T);
-        __syncthreads();
-    }
-    // down: the sweeps, then restrictLevel (:2342-2377) into the next level
-    for (uint32_t l = l0; l < L && !idle; l++) {
-        const LdsLevel T = level(l), U = level(l + 1u);
-        for (uint32_t k = 0; k < d.level_pairs; k++) {
-            ldsSmooth(T, l, 0, 1.0f, 0xffffffffu);
-            ldsSmooth(T, l, 1, 1.0f, 0xffffffffu);
+        for (uint32_t j = tid; j < T.w * T.h; j += 1024u) {
+            const i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
+            const float4 e = lvLoad<S>(d, l0, c);
+            const float rho = lvRho<S>(d, l0, c);
+            T.px[j] = e.x;
+            T.py[j] = e.y;
+            T.rx[j] = e.z;
+            T.ry[j] = e.w;
+            T.rho[j] = rho;
         }
-        // restrictLevel: each child's residual (residualAt, :2306-2315) summed
-        const float h2 = (float)(1u << (2u * l));
-        const int32_t span = 1 << (l + 1u);
-        for (uint32_t y = threadIdx.x / 32u; y < U.h; y += 32u)
-            for (uint32_t x = threadIdx.x % 32u; x < U.w; x += 32u) {
-                float rho = 0.0f, rxv = 0.0f, ryv = 0.0f;
-                bool shortcut = false;
-                if (span <= WIG_BLOCK) {
-                    i2 b = I2((int32_t)x * span / WIG_BLOCK, (int32_t)y * span / WIG_BLOCK);
-                    shortcut = inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u;
-                }
-                if (!shortcut) {
-                    float fluid = 0.0f, sum = 0.0f, sx = 0.0f, sy = 0.0f;
-                    bool air = false;
-                    for (int q = 0; q < 4; q++) {
-                        uint32_t cx = 2u * x + (uint32_t)(q & 1), cy = 2u * y + (uint32_t)(q >> 1);
-                        if (cx >= T.w || cy >= T.h) continue;     // outside the level: rho -1
-                        uint32_t j = cy * T.w + cx;
-                        float rk = T.rho[j];
-                        if (rk == 0.0f) air = true;
-                        else if (rk > 0.0f) {
-                            fluid += 1.0f;
-                            sum += rk;
-                            LdsSum st = ldsStencil(T, cx, cy);
-                            sx += T.rx[j] - (st.sx - (st.diag + 0.0f) * T.px[j]) / h2;
-                            sy += T.ry[j] - (st.sy - (st.diag + 0.0f) * T.py[j]) / h2;
-                        }
-                    }
-                    rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
-                    rxv = rho > 0.0f ? sx * 0.25f : 0.0f;
-                    ryv = rho > 0.0f ? sy * 0.25f : 0.0f;
-                }
-                uint32_t j = y * U.w + x;
-                U.px[j] = 0.0f;
-                U.py[j] = 0.0f;
-                U.rx[j] = rxv;
-                U.ry[j] = ryv;
-                U.rho[j] = rho;
-            }
         __syncthreads();
-        ldsBetas(U);
+        ldsBetas(T, tid, 1024u);
         __syncthreads();
     }
-    // the coarsest (solve, :2407-2456): its first MG_COARSEST cells
-    {
+    // The levels from lw down are the first wavefront's: the first below
+    // l0 with at most wave_cells cells (L + 1: none).
+    uint32_t lw = L + 1u;
+    for (uint32_t l = L; l > l0 && ((W + (1u << l) - 1u) >> l) * ((H + (1u << l) - 1u) >> l) <= d.wave_cells; l--) lw = l;
+    // down: the sweeps, then restrictLevel into the next level
+    for (uint32_t l = l0; l < L && l < lw && !idle; l++) {
+        const LdsLevel T = level(l), U = level(l + 1u);
+        for (uint32_t k = 0; k < pairs; k++) {
+            ldsSmooth(T, l, 0, 1.0f, 0xffffffffu, tid, 1024u, true);
+            ldsSmooth(T, l, 1, 1.0f, 0xffffffffu, tid, 1024u, true);
+        }
+        ldsRestrict(d, T, U, l, tid, 1024u);
+        __syncthreads();
+        ldsBetas(U, tid, 1024u);
+        __syncthreads();
+    }
+    if (lw > L) {
+        // the coarsest (solve, :2407-2456), by the workgroup: its first
+        // MG_COARSEST cells
         const LdsLevel T = level(L);
-        for (uint32_t sw = 0; sw < 2u * d.coarse_sweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST);
-    }
-    // up: prolong (:2459-2497), then a red and a black sweep
-    for (int32_t l = (int32_t)L - 1; l >= (int32_t)l0 && !idle; l--) {
-        const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
-        for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
-            for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
-                uint32_t j = y * T.w + x;
-                if (T.rho[j] <= 0.0f) continue;
-                i2 coarse = I2((int32_t)x / 2, (int32_t)y / 2);
-                i2 sg = I2((x & 1u) == 1u ? 1 : -1, (y & 1u) == 1u ? 1 : -1);
-                float sx = 0.0f, sy = 0.0f, weight = 0.0f;
-                for (int k = 0; k < 4; k++) {
-                    i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
-                    float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
-                    i2 at = I2(coarse.x + o.x, coarse.y + o.y);
-                    if (at.x < 0 || at.y < 0 || at.x >= (int32_t)U.w || at.y >= (int32_t)U.h) continue;   // rho -1
-                    uint32_t ja = (uint32_t)at.y * U.w + (uint32_t)at.x;
-                    float r = U.rho[ja];
-                    if (r < 0.0f) continue;
-                    weight += wk;
-                    if (r > 0.0f) {
-                        sx += wk * U.px[ja];
-                        sy += wk * U.py[ja];
-                    }
-                }
-                float dn = fmaxf(weight, 1e-6f);
-                T.px[j] = T.px[j] + sx / dn;
-                T.py[j] = T.py[j] + sy / dn;
+        for (uint32_t sw = 0; sw < 2u * csweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST, tid, 1024u, true);
+    } else if (tid < 64u) {
+        // the first wavefront's levels: down, the coarsest, up to lw
+        for (uint32_t l = lw; l < L && !idle; l++) {
+            const LdsLevel T = level(l), U = level(l + 1u);
+            for (uint32_t k = 0; k < pairs; k++) {
+                ldsSmooth(T, l, 0, 1.0f, 0xffffffffu, tid, 64u, false);
+                ldsSmooth(T, l, 1, 1.0f, 0xffffffffu, tid, 64u, false);
             }
+            ldsRestrict(d, T, U, l, tid, 64u);
+            phaseSync(false);
+            ldsBetas(U, tid, 64u);
+            phaseSync(false);
+        }
+        {
+            const LdsLevel T = level(L);
+            for (uint32_t sw = 0; sw < 2u * csweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST, tid, 64u, false);
+        }
+        // up: prolong (:2459-2497), then a red and a black sweep
+        for (int32_t l = (int32_t)L - 1; l >= (int32_t)lw && !idle; l--) {
+            const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
+            ldsProlong(T, U, tid, 64u);
+            phaseSync(false);
+            for (uint32_t k = 0; k < pairs; k++) {
+                ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu, tid, 64u, false);
+                ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu, tid, 64u, false);
+            }
+        }
+    }
+    __syncthreads();
+    // up the workgroup's levels
+    for (int32_t l = (int32_t)(lw < L ? lw : L) - 1; l >= (int32_t)l0 && !idle; l--) {
+        const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
+        ldsProlong(T, U, tid, 1024u);
         __syncthreads();
-        for (uint32_t k = 0; k < d.level_pairs; k++) {
-            ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu);
-            ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu);
+        for (uint32_t k = 0; k < pairs; k++) {
+            ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu, tid, 1024u, true);
+            ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu, tid, 1024u, true);
         }
     }
     // the levels back into the field
     for (uint32_t l = l0; l <= L; l++) {
         const LdsLevel T = level(l);
-        for (uint32_t y = threadIdx.x / 32u; y < T.h; y += 32u)
-            for (uint32_t x = threadIdx.x % 32u; x < T.w; x += 32u) {
-                uint32_t j = y * T.w + x;
-                i2 c = I2((int32_t)x, (int32_t)y);
-                lvStore<S>(d, l, c, make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]));
-                if (!idle) lvStoreRho<S>(d, l, c, T.rho[j]);
-            }
+        for (uint32_t j = tid; j < T.w * T.h; j += 1024u) {
+            const i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
+            lvStore<S>(d, l, c, make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]));
+            if (!idle) lvStoreRho<S>(d, l, c, T.rho[j]);
+        }
     }
 }
 

@@ -47,6 +47,7 @@ def lib():
         L.wig_kernel_count.restype = ct.c_int
         L.wig_set_tile.argtypes = [ct.c_void_p, ct.c_uint32, ct.c_uint32]
         L.wig_set_lds_level.argtypes = [ct.c_void_p, ct.c_uint32]
+        L.wig_set_coarse_wave.argtypes = [ct.c_void_p, ct.c_uint32]
         L.wig_batch_create.restype = ct.c_void_p
         L.wig_batch_create.argtypes = [ct.POINTER(ct.c_void_p), ct.c_uint32]
         L.wig_batch_destroy.argtypes = [ct.c_void_p]
@@ -88,6 +89,7 @@ class GpuWorld:
         self.layout = abi.WigLayoutInfo()
         _check(L.wig_layout(self._w, ct.byref(self.layout)))
         self.kinds = None
+        self.knobs = {}         # speed settings made (tiles, coarse levels); a clone takes them
         if kinds is not None:
             self.set_kinds(kinds)
 
@@ -163,6 +165,7 @@ class GpuWorld:
     def set_tile(self, level, side):
         """The coarse-level sweep kernels' tile side on `level` (wig_set_tile)."""
         _check(lib().wig_set_tile(self._w, level, side))
+        self.knobs[("tile", level)] = side
 
     def set_graphs(self, on):
         """Steps replayed as hipGraphs (wig_set_graphs)."""
@@ -171,6 +174,13 @@ class GpuWorld:
     def set_lds_level(self, level):
         """Where the one-workgroup coarse-level kernel starts (wig_set_lds_level)."""
         _check(lib().wig_set_lds_level(self._w, level))
+        self.knobs[("lds_level",)] = level
+
+    def set_coarse_wave(self, cells):
+        """The most cells a coarse level below the one-workgroup kernel's
+        first may have to be worked by one wavefront (wig_set_coarse_wave)."""
+        _check(lib().wig_set_coarse_wave(self._w, cells))
+        self.knobs[("coarse_wave",)] = cells
 
     def capture(self, on=True):
         """Each later solve keeps its right-hand side, for cg64 (wig_capture)."""
@@ -186,10 +196,22 @@ class GpuWorld:
         return out, x
 
     def clone(self):
-        """A new world of the same size and kinds holding this one's parts."""
+        """A new world of the same size and kinds holding this one's parts,
+        with its speed settings."""
         w = GpuWorld(self.layout.width, self.layout.height, coarse_levels=self.layout.levels, kinds=self.kinds)
         for p in range(abi.PART_COUNT):
             w.upload(p, self.download(p))
+        lds = self.knobs.get(("lds_level",))
+        if lds is not None:
+            w.set_lds_level(lds)        # (before the tiles: it decides which levels are tiled)
+        for key, v in self.knobs.items():
+            if key[0] == "tile":
+                try:
+                    w.set_tile(key[1], v)
+                except WigError:
+                    pass        # (a level the one-workgroup kernel has since taken)
+            elif key[0] == "coarse_wave":
+                w.set_coarse_wave(v)
         return w
 
     def kernel_time(self):

@@ -49,25 +49,61 @@ def tune_tiles(world, sim, sides=TILE_SIDES, steps=2):
     return chosen
 
 
-def tune_lds_level(world, sim, steps=2):
-    """Where the one-workgroup coarse-level kernel starts: each level from
-    the first LDS holds down to the coarsest, by the solve's time an
-    iteration over `steps` of the world's steps (results do not depend on
-    it).  Returns the level."""
+WAVE_CELLS = (0, 256, 1024)
+
+
+def tune_lds_level(world, sim, steps=3, waves=WAVE_CELLS):
+    """Where the one-workgroup coarse-level kernel starts, and from which
+    level down its first wavefront works alone: each level from the first
+    LDS holds down to the coarsest with each of `waves` (wig_set_coarse_wave),
+    by the solve's time an iteration over `steps` of the world's steps
+    (results depend on neither).  Returns (level, wave cells)."""
     best = None
-    level = None
     for cand in range(1, world.layout.levels + 1):
         try:
             world.set_lds_level(cand)
         except WigError:
             continue
-        ms = _kernel_ms_per_iteration(world, sim, "solve", steps)
-        if best is None or ms < best[0]:
-            best = (ms, cand)
-    if best is not None:
-        world.set_lds_level(best[1])
-        level = best[1]
-    return level
+        for cells in waves:
+            world.set_coarse_wave(cells)
+            ms = _kernel_ms_per_iteration(world, sim, "solve", steps)
+            if best is None or ms < best[0]:
+                best = (ms, cand, cells)
+    if best is None:
+        return None, None
+    world.set_lds_level(best[1])
+    world.set_coarse_wave(best[2])
+    return best[1], best[2]
+
+
+def tune_coarse_wave_batch(batch, sims, groups, waves=WAVE_CELLS, steps=1):
+    """The one-wavefront coarse levels for worlds stepped in a batch: for
+    each group of worlds (indices into the batch), each of `waves` in turn
+    for the whole group, scored by the batch's coarse-level kernel's mean
+    launch time over `steps` batch steps.  Returns {group key: cells}."""
+    from .host import step_batch
+
+    def score():
+        batch.time_kernel("coarseLDS")
+        for _ in range(steps):
+            step_batch(batch, sims, read_report=False)
+        t, n = batch.kernel_time()
+        batch.time_kernel(None)
+        return t / max(n, 1)
+
+    chosen = {}
+    for key, idx in groups.items():
+        best = None
+        for cells in waves:
+            for i in idx:
+                batch.worlds[i].set_coarse_wave(cells)
+            ms = score()
+            if best is None or ms < best[0]:
+                best = (ms, cells)
+        for i in idx:
+            batch.worlds[i].set_coarse_wave(best[1])
+        chosen[key] = best[1]
+    return chosen
 
 
 def tune_tiles_batch(batch, sims, groups, sides=TILE_SIDES, steps=1):
