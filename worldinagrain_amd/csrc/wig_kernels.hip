@@ -295,11 +295,13 @@ __device__ __forceinline__ uint32_t wave_take(const Dev &d, uint32_t n) {
 }
 
 // syncParticles (:767-831).  A workgroup takes 1024 consecutive particles
-// of `sorted`, stages those it keeps in LDS, and puts them into `parts` in
-// their order with one atomic (takePlaces, :743-755, takes one a workgroup
-// of 256 in no set order).
+// of `sorted`, a wavefront 256 of them (four rounds of 64, held in
+// registers), and puts those it keeps into `parts` in their order with one
+// atomic (takePlaces, :743-755, takes one a workgroup of 256 in no set
+// order); the kept ones stay together in runs as long as the chunk, which
+// keeps the gather's tiles small.
 #define SP_CHUNK 1024u
-__device__ bool syncOne(const Dev &d, uint32_t k, PState &q) {
+__device__ __forceinline__ bool syncOne(const Dev &d, uint32_t k, PState &q) {
     uint4 mo = *(const uint4 *)&MOTION(d, d.sorted, k, 0);
     uint4 st = *(const uint4 *)&STATE(d, d.sorted, k, 0);
     uint4 af = *(const uint4 *)&AFFINE(d, d.sorted, k, 0);
@@ -345,57 +347,36 @@ __device__ bool syncOne(const Dev &d, uint32_t k, PState &q) {
 __global__ void __launch_bounds__(256) k_syncParticles(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
-    __shared__ uint4 sMo[SP_CHUNK], sSt[SP_CHUNK], sAf[SP_CHUNK];
-    __shared__ uint32_t sScan[256];
-    __shared__ uint32_t sBase;
+    __shared__ uint32_t sKept[WAVES + 1];
     uint32_t cnt = ld(&d.sorted[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64u;
+    const uint64_t below = (1ull << lane) - 1ull;
     for (uint32_t base = bid * SP_CHUNK; base < n; base += nbid * SP_CHUNK) {
-        uint32_t keepBits = 0;
+        PState q[4];
+        bool keep[4];
+        uint32_t before[4], total = 0;
+#pragma unroll
         for (uint32_t r = 0; r < 4u; r++) {
-            uint32_t slot = r * 256u + tid, k = base + slot;
-            PState q;
-            bool keep = k < n && syncOne(d, k, q);
-            if (keep) {
-                sMo[slot] = make_uint4(ubits(q.pos.x), ubits(q.pos.y), ubits(q.vel.x), ubits(q.vel.y));
-                sSt[slot] = make_uint4(q.word, ubits(q.temp), q.id, 0u);
-                sAf[slot] = make_uint4(ubits(q.aff[0]), ubits(q.aff[1]), ubits(q.aff[2]), ubits(q.aff[3]));
-            }
-            keepBits |= (keep ? 1u : 0u) << r;
+            const uint32_t k = base + 256u * wave + 64u * r + lane;
+            keep[r] = k < n && syncOne(d, k, q[r]);
+            const uint64_t kept = __ballot(keep[r]);
+            before[r] = total + (uint32_t)__popcll(kept & below);
+            total += (uint32_t)__popcll(kept);
+        }
+        if (lane == 0u) sKept[wave] = total;
+        __syncthreads();
+        if (threadIdx.x == 0u) {
+            uint32_t all = sKept[0] + sKept[1] + sKept[2] + sKept[3];
+            sKept[WAVES] = all ? atomicAdd(&d.parts[0], all) : 0u;
         }
         __syncthreads();
-        // the kept slots ranked in slot order: thread t owns slots 4t .. 4t + 3,
-        // whose keep bits thread (slot % 256) holds, round slot / 256
-        sScan[tid] = keepBits;
-        __syncthreads();
-        uint32_t bits4 = 0;
-        for (uint32_t j = 0; j < 4u; j++) {
-            uint32_t slot = 4u * tid + j;
-            bits4 |= ((sScan[slot % 256u] >> (slot / 256u)) & 1u) << j;
-        }
-        uint32_t mine = __builtin_popcount(bits4);
-        __syncthreads();
-        sScan[tid] = mine;
-        __syncthreads();
-        for (uint32_t o = 1; o < 256u; o <<= 1) {
-            uint32_t add = tid >= o ? sScan[tid - o] : 0u;
-            __syncthreads();
-            sScan[tid] += add;
-            __syncthreads();
-        }
-        if (tid == 255u) sBase = sScan[255] ? atomicAdd(&d.parts[0], sScan[255]) : 0u;
-        __syncthreads();
-        uint32_t m = sBase + sScan[tid] - mine;
-        for (uint32_t j = 0; j < 4u; j++) {
-            if (!((bits4 >> j) & 1u)) continue;
-            uint32_t slot = 4u * tid + j;
-            if (m < d.cap) {
-                *(uint4 *)&MOTION(d, d.parts, m, 0) = sMo[slot];
-                *(uint4 *)&STATE(d, d.parts, m, 0) = sSt[slot];
-                *(uint4 *)&AFFINE(d, d.parts, m, 0) = sAf[slot];
-            }
-            m++;
+        uint32_t at = sKept[WAVES];
+        for (uint32_t w = 0; w < wave; w++) at += sKept[w];
+#pragma unroll
+        for (uint32_t r = 0; r < 4u; r++) {
+            const uint32_t m = at + before[r];
+            if (keep[r] && m < d.cap) storeParticle(d, m, q[r]);
         }
         __syncthreads();
     }
@@ -2552,14 +2533,31 @@ __global__ void __launch_bounds__(256) k_advect(WorldGrid g) {
 
 // ================================================================ sort
 // sortCount (:3700-3708)
+// (each particle's place among its cell's: a run of lanes of one cell in a
+// wavefront takes its places with one atomic, the particles being mostly in
+// cell order already; the order within a cell is the atomics' either way)
 __global__ void __launch_bounds__(256) k_sortCount(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    for (uint32_t k = bid * blockDim.x + threadIdx.x; k < n; k += nbid * blockDim.x) {
-        v2 pos = V2(fbits(MOTION(d, d.parts, k, 0)), fbits(MOTION(d, d.parts, k, 1)));
-        STATE(d, d.parts, k, 3) = atomicAdd(&CNT(d, cellIndex(d, cellOf(d, pos)), G_SORT), 1u);
+    const uint32_t lane = lane_id();
+    const uint64_t upTo = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);   // lanes 0 .. this one
+    for (uint32_t base = bid * blockDim.x + (threadIdx.x & ~63u); base < n; base += nbid * blockDim.x) {   // (this wavefront's)
+        const uint32_t k = base + lane;
+        const bool on = k < n;
+        uint32_t cell = 0xffffffffu;
+        if (on) cell = cellIndex(d, cellOf(d, V2(fbits(MOTION(d, d.parts, k, 0)), fbits(MOTION(d, d.parts, k, 1)))));
+        const uint32_t before = __shfl_up(cell, 1, WAVE);
+        const bool head = lane == 0u || before != cell;
+        const uint64_t heads = __ballot(head);
+        const uint32_t start = 63u - (uint32_t)__clzll((long long)(heads & upTo));
+        const uint64_t after = heads & ~upTo;
+        const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+        uint32_t first = 0;
+        if (head && on) first = atomicAdd(&CNT(d, cell, G_SORT), next - lane);
+        first = __shfl(first, (int)start, WAVE);
+        if (on) STATE(d, d.parts, k, 3) = first + (lane - start);
     }
 }
 // scanBlocks (:3730-3753)
