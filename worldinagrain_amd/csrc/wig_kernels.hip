@@ -128,12 +128,21 @@ __device__ __forceinline__ bool walls(const Dev &d, i2 c) {               // :10
 
 // union-find over G_REGION words (label.wgsl:25-55)
 __device__ __forceinline__ uint32_t ufLink(const Dev &d, uint32_t at) { return ld(&CNT(d, at, G_REGION)); }
+// (with path halving: a cell's link is lowered to its grandparent on the
+// way up, an ancestor, so every find still ends at the same root; a root's
+// own word, where the region's open bit goes, is never written here)
 __device__ uint32_t ufFind(const Dev &d, uint32_t i) {
     uint32_t x = i;
     for (;;) {
         uint32_t up = ufLink(d, x);
         if (up >= x) break;
-        x = up;
+        uint32_t gp = ufLink(d, up);
+        if (gp >= up) {
+            x = up;
+            break;
+        }
+        atomicMin(&CNT(d, x, G_REGION), gp);
+        x = gp;
     }
     return x;
 }
@@ -757,17 +766,32 @@ __global__ void __launch_bounds__(64) k_localJoin(WorldGrid g, int mode) {
         __syncthreads();
     }
 }
+// (Each join starts from the two cells' links as the local joins left them,
+// their blocks' roots; a lane whose pair is its edge neighbour's, the lane
+// before it along the same block edge, leaves the join to that lane: the
+// same join, and a union done twice is done once.)
 __global__ void __launch_bounds__(64) k_edgeJoin(WorldGrid g, int mode) {
     WORLD(g);
     if (!d.full) return;
-    uint32_t li = threadIdx.x;
-    bool edge = li % 8u == 7u || li / 8u == 7u;
+    const uint32_t li = threadIdx.x;
+    const bool right = li % 8u == 7u, down = li / 8u == 7u;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        if (!edge) continue;
-        i2 c = blockCell(activeBlock(d, wid), li);
-        if (!isMember(d, c, mode)) continue;
-        if (li % 8u == 7u && isMember(d, I2(c.x + 1, c.y), mode)) ufJoin(d, cellIndex(d, c), cellIndex(d, I2(c.x + 1, c.y)));
-        if (li / 8u == 7u && isMember(d, I2(c.x, c.y + 1), mode)) ufJoin(d, cellIndex(d, c), cellIndex(d, I2(c.x, c.y + 1)));
+        const i2 c = blockCell(activeBlock(d, wid), li);
+        const bool mine = (right || down) && isMember(d, c, mode);
+        const bool jr = mine && right && isMember(d, I2(c.x + 1, c.y), mode);
+        const bool jd = mine && down && isMember(d, I2(c.x, c.y + 1), mode);
+        uint32_t a = 0, br = 0, bd = 0;
+        if (mine) a = ufLink(d, cellIndex(d, c));
+        if (jr) br = ufLink(d, cellIndex(d, I2(c.x + 1, c.y)));
+        if (jd) bd = ufLink(d, cellIndex(d, I2(c.x, c.y + 1)));
+        // the pair of the lane before along each edge: above on the right
+        // edge, to the left on the lower edge
+        const uint32_t ar = __shfl(a, (int)((li + 56u) & 63u), WAVE), brr = __shfl(br, (int)((li + 56u) & 63u), WAVE);
+        const bool jrr = __shfl(jr ? 1 : 0, (int)((li + 56u) & 63u), WAVE) != 0;
+        const uint32_t ad = __shfl(a, (int)((li + 63u) & 63u), WAVE), bdd = __shfl(bd, (int)((li + 63u) & 63u), WAVE);
+        const bool jdd = __shfl(jd ? 1 : 0, (int)((li + 63u) & 63u), WAVE) != 0;
+        if (jr && !(li >= 8u && jrr && ar == a && brr == br)) ufJoin(d, a, br);
+        if (jd && !(li % 8u > 0u && jdd && ad == a && bdd == bd)) ufJoin(d, a, bd);
     }
 }
 // sealAnchor: a gap meeting what was open air last step opens its region
