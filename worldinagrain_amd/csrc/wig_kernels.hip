@@ -623,6 +623,44 @@ __device__ __forceinline__ void splatParticle(const Dev &d, const GatherTile &T,
     addWord<TILE>(T.mass, t, &CNT(d, home, G_MASS), wu32(wround(liquidDensity(temp) * FIXF)));
     splatDensity<TILE>(d, T, pos, liquidVolume(temp));
 }
+// (the tile's words, added into the counters: one atomic a nonzero word;
+// threads t0, t0 + step, ... of the tile's entries)
+__device__ __forceinline__ void flushTile(const Dev &d, const GatherTile &T, int t0, int step) {
+    const int ne = T.tw * T.th;
+    for (int j = t0; j < ne; j += step) {
+        i2 c = I2(T.ox + j % T.tw, T.oy + j / T.tw);
+        if (T.um[j] | T.uw[j]) {          // (a face outside the world was never added to)
+            uint32_t fi = uFace(d, c.x, c.y);
+            if (T.um[j]) atomicAdd(&SUM(d, fi, 0), T.um[j]);
+            if (T.uw[j]) atomicAdd(&SUM(d, fi, 1), T.uw[j]);
+        }
+        if (T.vm[j] | T.vw[j]) {
+            uint32_t fi = vFace(d, c.x, c.y);
+            if (T.vm[j]) atomicAdd(&SUM(d, fi, 0), T.vm[j]);
+            if (T.vw[j]) atomicAdd(&SUM(d, fi, 1), T.vw[j]);
+        }
+        if (T.cnt[j] | T.mass[j] | T.dens[j]) {
+            uint32_t ci = cellIndex(d, c);
+            if (T.cnt[j]) atomicAdd(&CNT(d, ci, G_COUNT), T.cnt[j]);
+            if (T.mass[j]) atomicAdd(&CNT(d, ci, G_MASS), T.mass[j]);
+            if (T.dens[j]) atomicAdd(&CNT(d, ci, G_DENSITY), T.dens[j]);
+        }
+    }
+}
+__device__ __forceinline__ void tileWords(GatherTile &T, uint32_t *base) {
+    const int ne = T.tw * T.th;
+    T.um = base;
+    T.uw = base + ne;
+    T.vm = base + 2 * ne;
+    T.vw = base + 3 * ne;
+    T.cnt = base + 4 * ne;
+    T.mass = base + 5 * ne;
+    T.dens = base + 6 * ne;
+}
+// A workgroup takes 256 particles at a time.  Their cells' box (the faces
+// and cells they touch) in LDS when it fits; when it does not (the 256 span
+// two places, as runs of particles from different regions meet), each
+// wavefront's own box in a quarter of it; past that, the counters directly.
 __global__ void __launch_bounds__(256) k_gather(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
@@ -630,10 +668,10 @@ __global__ void __launch_bounds__(256) k_gather(WorldGrid g) {
     __shared__ int32_t sBox[4];
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, wave = tid / 64u;
     for (uint32_t base = bid * 256u; base < n; base += nbid * 256u) {
         uint32_t k = base + tid;
-        // the cells' box
+        // the cells' box, the wavefront's and the workgroup's
         int32_t cx = 0x7fffffff, cy = 0x7fffffff, mx = -1, my = -1;
         if (k < n) {
             i2 c = cellOf(d, V2(fbits(MOTION(d, d.parts, k, 0)), fbits(MOTION(d, d.parts, k, 1))));
@@ -660,43 +698,36 @@ __global__ void __launch_bounds__(256) k_gather(WorldGrid g) {
         T.oy = sBox[1] - 1;
         T.tw = sBox[2] - sBox[0] + 4;   // faces and cells a particle touches: its cell -1 .. +2
         T.th = sBox[3] - sBox[1] + 4;
-        bool tiled = T.tw * T.th <= GT_MAX;
+        const bool whole = T.tw * T.th <= GT_MAX;
         __syncthreads();
-        if (!tiled) {
-            if (k < n) splatParticle<false>(d, T, k);
+        if (whole) {
+            tileWords(T, sTile);
+            for (int j = (int)tid; j < 7 * T.tw * T.th; j += 256) sTile[j] = 0u;
+            __syncthreads();
+            if (k < n) splatParticle<true>(d, T, k);
+            __syncthreads();
+            flushTile(d, T, (int)tid, 256);
+            __syncthreads();
             continue;
         }
-        int ne = T.tw * T.th;
-        T.um = sTile;
-        T.uw = sTile + ne;
-        T.vm = sTile + 2 * ne;
-        T.vw = sTile + 3 * ne;
-        T.cnt = sTile + 4 * ne;
-        T.mass = sTile + 5 * ne;
-        T.dens = sTile + 6 * ne;
-        for (int j = (int)tid; j < 7 * ne; j += 256) sTile[j] = 0u;
+        // each wavefront's box (an empty wavefront's is empty)
+        GatherTile V;
+        V.ox = cx - 1;
+        V.oy = cy - 1;
+        V.tw = mx >= 0 ? mx - cx + 4 : 0;
+        V.th = mx >= 0 ? my - cy + 4 : 0;
+        const bool quarter = V.tw * V.th <= GT_MAX / WAVES;
+        uint32_t *mine = sTile + wave * (7u * GT_MAX / WAVES);
+        tileWords(V, mine);
+        if (quarter)
+            for (int j = (int)lane_id(); j < 7 * V.tw * V.th; j += 64) mine[j] = 0u;
         __syncthreads();
-        if (k < n) splatParticle<true>(d, T, k);
-        __syncthreads();
-        for (int j = (int)tid; j < ne; j += 256) {
-            i2 c = I2(T.ox + j % T.tw, T.oy + j / T.tw);
-            if (T.um[j] | T.uw[j]) {          // (a face outside the world was never added to)
-                uint32_t fi = uFace(d, c.x, c.y);
-                if (T.um[j]) atomicAdd(&SUM(d, fi, 0), T.um[j]);
-                if (T.uw[j]) atomicAdd(&SUM(d, fi, 1), T.uw[j]);
-            }
-            if (T.vm[j] | T.vw[j]) {
-                uint32_t fi = vFace(d, c.x, c.y);
-                if (T.vm[j]) atomicAdd(&SUM(d, fi, 0), T.vm[j]);
-                if (T.vw[j]) atomicAdd(&SUM(d, fi, 1), T.vw[j]);
-            }
-            if (T.cnt[j] | T.mass[j] | T.dens[j]) {
-                uint32_t ci = cellIndex(d, c);
-                if (T.cnt[j]) atomicAdd(&CNT(d, ci, G_COUNT), T.cnt[j]);
-                if (T.mass[j]) atomicAdd(&CNT(d, ci, G_MASS), T.mass[j]);
-                if (T.dens[j]) atomicAdd(&CNT(d, ci, G_DENSITY), T.dens[j]);
-            }
+        if (k < n) {
+            if (quarter) splatParticle<true>(d, V, k);
+            else splatParticle<false>(d, V, k);
         }
+        __syncthreads();
+        if (quarter) flushTile(d, V, (int)lane_id(), 64);
         __syncthreads();
     }
 }
@@ -2495,7 +2526,11 @@ __global__ void __launch_bounds__(256) k_advect(WorldGrid g) {
     if (!d.full) return;
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
-    for (uint32_t k = bid * blockDim.x + threadIdx.x; k < n; k += nbid * blockDim.x) {
+    const uint32_t lane = lane_id();
+    for (uint32_t base = bid * blockDim.x + (threadIdx.x & ~63u); base < n; base += nbid * blockDim.x) {   // (this wavefront's)
+        const uint32_t k = base + lane;
+        uint32_t came = 0xffffffffu, how = 0u;   // the cell arrived in and its came bit
+        if (k < n) {
         uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
         uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
         v2 orig = V2(fbits(mo.x), fbits(mo.y));
@@ -2550,8 +2585,22 @@ __global__ void __launch_bounds__(256) k_advect(WorldGrid g) {
         float lx = FLD(d, li, 0), lz = FLD(d, li, 2);
         if (lz > 0.0f) {
             bool falling = lx < 0.5f * lz * wlength(V2(0.0f, d.earthG));
-            atomicOr(&CNT(d, cellIndex(d, cellOf(d, pos)), G_CAME), falling ? CAME_FALLING : CAME_PRESSING);
+            came = cellIndex(d, cellOf(d, pos));
+            how = falling ? CAME_FALLING : CAME_PRESSING;
         }
+        }
+        // the came bits OR-ed over each run of lanes arriving in one cell,
+        // one atomic a run (OR is indifferent to order and repeats)
+        const uint32_t before = __shfl_up(came, 1, WAVE);
+        const bool head = lane == 0u || before != came;
+        const uint64_t heads = __ballot(head);
+        const uint64_t after = heads & (lane == 63u ? 0ull : ~((1ull << (lane + 1u)) - 1ull));
+        const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t o = __shfl_down(how, off, WAVE);
+            if (lane + off < next) how |= o;
+        }
+        if (head && came != 0xffffffffu && how) atomicOr(&CNT(d, came, G_CAME), how);
     }
 }
 
