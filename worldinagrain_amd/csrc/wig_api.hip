@@ -111,7 +111,7 @@ enum {
     SH_LVL1,     // 8 x 8 cells of coarse level 1 a workgroup
     SH_B256,     // 256 blocks a workgroup
     SH_BG16,     // 256-thread passes, sixteen blocks a workgroup (four a wavefront)
-    SH_RELAX,    // k_relax: eight tiles a 256-thread workgroup (two a wavefront)
+    SH_RELAX,    // k_relax: 4 x RELAX_NT tiles a 256-thread workgroup
     SH_TILE,     // + l - 1: k_levelDown/k_levelUp's tiles on level l
     SH_COUNT = SH_TILE + MG_LEVELS
 };
@@ -151,7 +151,7 @@ static uint32_t shape_count(const Dev &d, int sh) {
     case SH_LVL1: return ceil_div(d.lw[1], 8u) * ceil_div(d.lh[1], 8u);
     case SH_B256: return ceil_div(d.B, 256u);
     case SH_BG16: return std::min<uint32_t>((d.B + 15u) / 16u, 1024u);
-    case SH_RELAX: return std::min<uint32_t>((d.B + d.bw + d.bh + 8u) / 8u, 1024u);
+    case SH_RELAX: return std::min<uint32_t>((d.B + d.bw + d.bh + 4u * RELAX_NT) / (4u * RELAX_NT), 1024u);
     default: {
         uint32_t l = (uint32_t)(sh - SH_TILE) + 1u;
         if (l >= d.lds_level) return 0u;

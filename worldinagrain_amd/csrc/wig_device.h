@@ -53,6 +53,7 @@
 #define ADVECT_STEPS 17
 #define THETA_MIN 0.1f
 #define RIM 10
+#define RELAX_NT 1       // tiles a wavefront of k_relax works at once (four wavefronts a workgroup)
 #define PINF 1.0f
 #define REGION_OPEN 0x80000000u
 #define CAME_PRESSING 1u

@@ -1113,7 +1113,7 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t na, uint32_t wid,
 __device__ __forceinline__ i2 clampTo(i2 c, int32_t w, int32_t h) {
     return I2(c.x < 0 ? 0 : (c.x >= w ? w - 1 : c.x), c.y < 0 ? 0 : (c.y >= h ? h - 1 : c.y));
 }
-#define NT 2   // tiles a wavefront relaxes at once
+#define NT RELAX_NT
 template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int post) {
     WORLD(g);
     if (d.store != S) return;
