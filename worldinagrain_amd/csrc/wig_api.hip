@@ -377,7 +377,8 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     WigWorld *w = new WigWorld();
     size_world(w, width, height, coarse_levels);
     w->vlevel_bytes = 64u;
-    for (uint32_t l = 1; l <= w->d.L; l++) w->vlevel_bytes += 12ull * w->d.lw[l] * w->d.lh[l] + 64u;
+    for (uint32_t l = 1; l <= w->d.L; l++)   // (and each level's tile flags, for tiles of 8 or more)
+        w->vlevel_bytes += 12ull * w->d.lw[l] * w->d.lh[l] + 64u + ceil_div(w->d.lw[l], 8u) * ceil_div(w->d.lh[l], 8u) + 64u;
     if (hipGetDevice(&w->device) != hipSuccess || hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking) != hipSuccess) {
         fail(WIG_E_DEVICE, "no HIP device or stream");
         delete w;
@@ -468,6 +469,11 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
             d.vlev[l] = (uint16_t *)v;
             d.vrho[l] = (float *)(v + 8ull * n);
             v += 12ull * n + 64u;
+        }
+        // which tiles of each tiled level hold fluid (k_levelDown, each solve)
+        for (uint32_t l = 1; l <= d.L; l++) {
+            d.tflag[l] = v;
+            v += ceil_div(d.lw[l], 8u) * ceil_div(d.lh[l], 8u) + 64u;
         }
     }
     WigWorld *self = w;

@@ -156,6 +156,9 @@ struct Dev {
     uint32_t *ha, *hb, *hr2;
     uint16_t *vlev[MG_LEVELS + 1];
     float *vrho[MG_LEVELS + 1];
+    // a tiled level's tiles: whether any of a tile's own cells holds fluid,
+    // taken by k_levelDown in each solve's first iteration (tile by row)
+    uint8_t *tflag[MG_LEVELS + 1];
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
     uint32_t wave_cells;    // and below it, levels of at most this many cells by its first wavefront
     uint32_t tile[MG_LEVELS + 1];   // k_levelDown/k_levelUp's tile side, per level below lds_level
@@ -204,6 +207,7 @@ struct WorldGrid {
     d.lstart[2] = dg_.lstart[(l) + 1u];                               \
     d.vlev[2] = dg_.vlev[(l) + 1u];                                   \
     d.vrho[2] = dg_.vrho[(l) + 1u];                                   \
+    d.tflag[1] = dg_.tflag[l];                                        \
     const uint32_t tile_ = dg_.tile[l];                               \
     const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
     const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \

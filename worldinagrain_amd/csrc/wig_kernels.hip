@@ -1505,7 +1505,11 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     const Iter it = iterWords(d);
     const uint32_t T = tile_;
     const uint32_t gx_ = (d.lw[1] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
+    const uint8_t fluidTile = d.tflag[1][bid];
     if (!it.encoded || it.converged || it.active == 0u) return;
+    // a tile none of whose own cells holds fluid leaves every value as the
+    // solve's first iteration left it (only cells of rho > 0 change)
+    if (it.cycle > 0u && fluidTile == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs + 1u;
     LevelTile t = levelTile(sm, T, H, bx, by);
@@ -1530,6 +1534,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     tileSweeps(t, l, d.level_pairs);
     // p back, and each tile cell's residual (residualAt, :2306-2315)
     float h2 = (float)(1u << (2u * l));
+    int anyFluid = 0;
     for (uint32_t k = threadIdx.x; k < T * T; k += TILE_THREADS) {
         int32_t x = (int32_t)(k % T + H), y = (int32_t)(k / T + H), j = y * t.R + x;
         i2 c = I2(t.ox + x, t.oy + y);
@@ -1542,8 +1547,10 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
         resx[k] = qx;
         resy[k] = qy;
         if (inLevel(d, 1u, c)) lvStoreP<S>(d, 1u, c, make_float2(t.px[j], t.py[j]));
+        anyFluid |= t.rho[j] > 0.0f ? 1 : 0;
     }
-    __syncthreads();
+    anyFluid = __syncthreads_or(anyFluid);
+    if (it.cycle == 0u && threadIdx.x == 0) d.tflag[1][bid] = anyFluid ? 1u : 0u;
     // restrictLevel (:2342-2377) into the tile's parents
     const uint32_t T2 = T / 2u, lc = l + 1u;
     const int32_t span = 1 << lc;
@@ -1585,7 +1592,9 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
     const Iter it = iterWords(d);
     const uint32_t T = tile_;
     const uint32_t gx_ = (d.lw[1] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
+    const uint8_t fluidTile = d.tflag[1][bid];   // (k_levelDown's of this iteration's level l)
     if (!it.encoded || it.converged || it.active == 0u) return;
+    if (fluidTile == 0u) return;   // prolong and the sweeps change only cells of rho > 0
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs;
     LevelTile t = levelTile(sm, T, H, bx, by);
