@@ -1301,8 +1301,9 @@ template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     const uint32_t li = threadIdx.x;
     const int32_t fx0 = (int32_t)bx * 16 - 1, fy0 = (int32_t)by * 16 - 1;
     // level 1's rho, and whether any of the four fine blocks under the tile
-    // is worked on: level 1's rho is 0 over the others (k_levelRho), and a
-    // cell of rho <= 0 restricts 0
+    // holds fluid: level 1's rho is <= 0 over the others (0 over blocks not
+    // worked on, k_levelRho; from fine rho <= 0 elsewhere), and a cell of
+    // rho <= 0 restricts 0
     const i2 c = I2((int32_t)(bx * 8u + li % 8u), (int32_t)(by * 8u + li / 8u));
     const bool in = inLevel(d, 1u, c);
     const float rho = in ? lvRho<S>(d, 1u, c) : 0.0f;
@@ -1311,7 +1312,7 @@ template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
         i2 fb = I2(2 * (int32_t)bx + (q & 1), 2 * (int32_t)by + (q >> 1));
         if (inBlocks(d, fb)) st |= d.blocks[STATE_SLOT(d, blockIndex(d, fb))];
     }
-    if ((st & ACTIVE) == 0u) {
+    if ((st & HAS_FLUID) == 0u) {
         if (in) lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         return;
     }
