@@ -104,7 +104,9 @@ class Server:
         self.evictions = 0
         self.admissions = 0
         self.particle_steps = 0
-        self.rounds = []           # (start, end, worlds, cells, largest world's cells) a round
+        # a round: (start, end, worlds, cells, largest world's cells, cold: a
+        # world made resident for it or on its first step)
+        self.rounds = []
 
     def _admit(self, s, busy):
         """Makes session s resident within the budget, evicting idle worlds
@@ -148,6 +150,7 @@ class Server:
             # this round: the queue's requests in order, one per session,
             # while their worlds fit the round's cells and the budget
             chosen, busy, cells = [], set(), 0
+            admitted = self.admissions
             waiting = {r.sid for r in queue}
             for r in queue:
                 if r.sid in busy:
@@ -164,6 +167,7 @@ class Server:
             if not chosen:
                 raise RuntimeError("no waiting request fits the HBM budget")
             sess = [self.sessions[r.sid] for r in chosen]
+            cold = self.admissions != admitted or any(s.steps_run == 0 for s in sess)
             worlds = [s.world for s in sess]
             if self.batch is None:
                 self.batch = GpuBatch(worlds)
@@ -178,7 +182,7 @@ class Server:
                 s.last_used = self.round_no
                 s.steps_run += 1
                 self.particle_steps += int(s.sim.last_stats[abi.STAT_PARTICLES])
-            self.rounds.append((start, end, len(chosen), cells, max(s.width * s.height for s in sess)))
+            self.rounds.append((start, end, len(chosen), cells, max(s.width * s.height for s in sess), cold))
             for r in chosen:
                 r.left -= 1
                 if r.left == 0:
