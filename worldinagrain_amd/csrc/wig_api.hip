@@ -945,7 +945,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
                 if (!done) going.push_back(i);
             }
             if (going.empty()) break;
-            if (cur == &b && b.timed < 0 && 4u * going.size() <= m && going.size() < m) {
+            if (cur == &b && (b.timed < 0 || b.timed == KID_COUNT) && 4u * going.size() <= m && going.size() < m) {   // (the solve timed as a whole keeps the tail)
                 std::vector<WigWorld *> ws;
                 for (uint32_t i : going) ws.push_back(b.w[i]);
                 if (!b.tail) {

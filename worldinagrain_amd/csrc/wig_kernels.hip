@@ -1453,6 +1453,38 @@ __device__ __forceinline__ LevelTile levelTile(float *sm, uint32_t T, uint32_t H
 #define FOR_TILE(t, x, y) \
     for (int32_t y = (int32_t)(threadIdx.x / 16u); y < (t).R; y += 16) \
         for (int32_t x = (int32_t)(threadIdx.x % 16u); x < (t).R; x += 16)
+// The tile's cells of the kernel's level (its Dev's level 1) into LDS,
+// every load made at once: TILE_LOADS a thread a round, one round for the
+// tiles the LDS holds (a side with its halo of 48 or less); outside the
+// level p and rhs 0, rho -1.
+#define TILE_LOADS 9
+template <int S> __device__ __forceinline__ void loadLevelTile(const Dev &d, const LevelTile &t) {
+    const int32_t n = t.R * t.R, w = (int32_t)d.lw[1], h = (int32_t)d.lh[1], tid = (int32_t)threadIdx.x;
+    for (int32_t b0 = 0; b0 < n; b0 += TILE_LOADS * (int32_t)TILE_THREADS) {
+        float4 e[TILE_LOADS];
+        float r[TILE_LOADS];
+#pragma unroll
+        for (int q = 0; q < TILE_LOADS; q++) {
+            if (b0 + q * (int32_t)TILE_THREADS >= n) break;
+            const int32_t k = min(b0 + q * (int32_t)TILE_THREADS + tid, n - 1);
+            const i2 c = clampTo(I2(t.ox + k % t.R, t.oy + k / t.R), w, h);
+            e[q] = lvLoad<S>(d, 1u, c);
+            r[q] = lvRho<S>(d, 1u, c);
+        }
+#pragma unroll
+        for (int q = 0; q < TILE_LOADS; q++) {
+            if (b0 + q * (int32_t)TILE_THREADS >= n) break;
+            const int32_t k = b0 + q * (int32_t)TILE_THREADS + tid;
+            if (k >= n) continue;
+            const bool in = inLevel(d, 1u, I2(t.ox + k % t.R, t.oy + k / t.R));
+            t.px[k] = in ? e[q].x : 0.0f;
+            t.py[k] = in ? e[q].y : 0.0f;
+            t.rx[k] = in ? e[q].z : 0.0f;
+            t.ry[k] = in ? e[q].w : 0.0f;
+            t.rho[k] = in ? r[q] : -1.0f;
+        }
+    }
+}
 // the betas of each cell's right and lower faces, 0 where the stencil skips
 // the neighbour (faceBeta is 0 exactly when rho > 0 and the other is < 0)
 __device__ __forceinline__ void tileBetas(const LevelTile &t) {
@@ -1515,20 +1547,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     const uint32_t H = 2u * d.level_pairs + 1u;
     LevelTile t = levelTile(sm, T, H, bx, by);
     float *resx = sm + 7u * (uint32_t)(t.R * t.R), *resy = resx + T * T;
-    FOR_TILE(t, x, y) {
-        int32_t j = y * t.R + x;
-        i2 c = I2(t.ox + x, t.oy + y), cc = clampTo(c, (int32_t)d.lw[1], (int32_t)d.lh[1]);
-        const bool in = inLevel(d, 1u, c);
-        const float rr = lvRho<S>(d, 1u, cc);
-        const float4 ee = lvLoad<S>(d, 1u, cc);
-        const float4 e = in ? ee : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float rho = in ? rr : -1.0f;
-        t.px[j] = e.x;
-        t.py[j] = e.y;
-        t.rx[j] = e.z;
-        t.ry[j] = e.w;
-        t.rho[j] = rho;
-    }
+    loadLevelTile<S>(d, t);
     __syncthreads();
     tileBetas(t);
     __syncthreads();
@@ -1602,28 +1621,35 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
     // level l + 1 under the tile, from one cell before its first to one after its last
     const int32_t cx0 = (t.ox >> 1) - 1, cy0 = (t.oy >> 1) - 1, Rc = t.R / 2 + 3;
     float *cpx = sm + 7u * (uint32_t)(t.R * t.R), *cpy = cpx + Rc * Rc, *crho = cpy + Rc * Rc;
-    for (int32_t k = (int32_t)threadIdx.x; k < Rc * Rc; k += (int32_t)TILE_THREADS) {
-        i2 c = I2(cx0 + k % Rc, cy0 + k / Rc), cc = clampTo(c, (int32_t)d.lw[2], (int32_t)d.lh[2]);
+    // (its loads issued with the tile's: at most three a thread)
+    const int32_t nc = Rc * Rc, tid = (int32_t)threadIdx.x;
+    float2 ce[3];
+    float cr[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int32_t k = min(tid + q * (int32_t)TILE_THREADS, nc - 1);
+        const i2 c = clampTo(I2(cx0 + k % Rc, cy0 + k / Rc), (int32_t)d.lw[2], (int32_t)d.lh[2]);
+        ce[q] = lvLoadP<S>(d, 2u, c);
+        cr[q] = lvRho<S>(d, 2u, c);
+    }
+    loadLevelTile<S>(d, t);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int32_t k = tid + q * (int32_t)TILE_THREADS;
+        if (k >= nc) continue;
+        const bool in = inLevel(d, 2u, I2(cx0 + k % Rc, cy0 + k / Rc));
+        cpx[k] = in ? ce[q].x : 0.0f;
+        cpy[k] = in ? ce[q].y : 0.0f;
+        crho[k] = in ? cr[q] : -1.0f;
+    }
+    for (int32_t k = tid + 3 * (int32_t)TILE_THREADS; k < nc; k += (int32_t)TILE_THREADS) {   // (sides past 48)
+        const i2 c = I2(cx0 + k % Rc, cy0 + k / Rc), cc = clampTo(c, (int32_t)d.lw[2], (int32_t)d.lh[2]);
         const bool in = inLevel(d, 2u, c);
         const float2 e = lvLoadP<S>(d, 2u, cc);
         const float rr = lvRho<S>(d, 2u, cc);
         cpx[k] = in ? e.x : 0.0f;
         cpy[k] = in ? e.y : 0.0f;
         crho[k] = in ? rr : -1.0f;
-    }
-    FOR_TILE(t, x, y) {
-        int32_t j = y * t.R + x;
-        i2 c = I2(t.ox + x, t.oy + y), cc = clampTo(c, (int32_t)d.lw[1], (int32_t)d.lh[1]);
-        const bool in = inLevel(d, 1u, c);
-        const float4 ee = lvLoad<S>(d, 1u, cc);
-        const float rr = lvRho<S>(d, 1u, cc);
-        const float4 e = in ? ee : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float rho = in ? rr : -1.0f;
-        t.px[j] = e.x;
-        t.py[j] = e.y;
-        t.rx[j] = e.z;
-        t.ry[j] = e.w;
-        t.rho[j] = rho;
     }
     __syncthreads();
     // prolong (:2459-2497): prolonged() from the staged coarse cells
@@ -1829,16 +1855,38 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
     {
         // level l0 as the pass above left it in the field (or, with no
         // blocks worked on, the coarsest as it was)
+        // (every load made at once: six a thread hold what LDS can)
         const LdsLevel T = level(l0);
-        for (uint32_t j = tid; j < T.w * T.h; j += 1024u) {
+        const uint32_t n = T.w * T.h;
+        float4 e[6];
+        float r[6];
+#pragma unroll
+        for (uint32_t q = 0; q < 6u; q++) {
+            if (q * 1024u >= n) break;
+            const uint32_t j = min(tid + q * 1024u, n - 1u);
             const i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
-            const float4 e = lvLoad<S>(d, l0, c);
-            const float rho = lvRho<S>(d, l0, c);
-            T.px[j] = e.x;
-            T.py[j] = e.y;
-            T.rx[j] = e.z;
-            T.ry[j] = e.w;
-            T.rho[j] = rho;
+            e[q] = lvLoad<S>(d, l0, c);
+            r[q] = lvRho<S>(d, l0, c);
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 6u; q++) {
+            const uint32_t j = tid + q * 1024u;
+            if (q * 1024u >= n) break;
+            if (j >= n) continue;
+            T.px[j] = e[q].x;
+            T.py[j] = e[q].y;
+            T.rx[j] = e[q].z;
+            T.ry[j] = e[q].w;
+            T.rho[j] = r[q];
+        }
+        for (uint32_t j = tid + 6u * 1024u; j < n; j += 1024u) {   // (none while LDS holds 160 KB)
+            const i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
+            const float4 ee = lvLoad<S>(d, l0, c);
+            T.px[j] = ee.x;
+            T.py[j] = ee.y;
+            T.rx[j] = ee.z;
+            T.ry[j] = ee.w;
+            T.rho[j] = lvRho<S>(d, l0, c);
         }
         __syncthreads();
         ldsBetas(T, tid, 1024u);
@@ -2200,11 +2248,17 @@ template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
     const uint32_t x = LANE & 7u, y = LANE >> 3;
     const bool edge[4] = {x == 0u, x == 7u, y == 0u, y == 7u};
     const uint32_t from[4] = {(LANE + 63u) & 63u, (LANE + 1u) & 63u, (LANE + 56u) & 63u, (LANE + 8u) & 63u};
+    // the cells beside the block's four edges, one a lane of lanes 0 .. 31
+    // (left, right, upper, lower edge, 8 each), and the lane a cell on an
+    // edge finds its neighbour's p at
+    const uint32_t ek = (LANE / 8u) & 3u, em = LANE % 8u;
+    const uint32_t eat = ek == 0u ? em * 8u : (ek == 1u ? em * 8u + 7u : (ek == 2u ? em : 56u + em));
+    const uint32_t slot[4] = {y, 8u + y, 16u + x, 24u + x};
     FOR_PLACES(it, w0) {
         uint32_t i[NB];
         placeCells(d, it, w0, i);
         uint32_t f[NB];
-        float z[NB], po[NB], pe[NB][4];
+        float z[NB], po[NB], pe[NB];
         float4 cf[NB];
         for (uint32_t j = 0; j < NB; j++) {
             f[j] = d.cflag[i[j]];
@@ -2212,19 +2266,20 @@ template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
             po[j] = pold[i[j]];
             cf[j] = d.coef[i[j]];
         }
-        // the neighbours' p across the block's edges (newP there; a lane
-        // off that edge loads its own cell's, unused, so that no load waits
-        // on a branch)
-        for (uint32_t j = 0; j < NB; j++)
-            for (int k = 0; k < 4; k++) pe[j][k] = newP<S>(d, edge[k] ? acrossEdge(d, i[j], k) : i[j], beta, pold);
+        // newP beside the edges (lanes 32 .. 63 take their own cell's, unused)
+        for (uint32_t j = 0; j < NB; j++) {
+            const uint32_t b0 = i[j] - LANE;
+            pe[j] = newP<S>(d, LANE < 32u ? acrossEdge(d, b0 + eat, (int)ek) : i[j], beta, pold);
+        }
         for (uint32_t j = 0; j < NB; j++) {
             // cgDirection's p (newP) at this cell, the neighbours' inside the
-            // block from their lanes
+            // block from their lanes, across its edges from the edge lanes
             float p = (f[j] & CF_FLUID) ? (float)((double)z[j] + beta * (double)po[j]) : 0.0f;
             float nb[4];
             for (int k = 0; k < 4; k++) {
                 float o = __shfl(p, (int)from[k], WAVE);
-                nb[k] = edge[k] ? pe[j][k] : o;
+                float e = __shfl(pe[j], (int)slot[k], WAVE);
+                nb[k] = edge[k] ? e : o;
             }
             float q = (f[j] & CF_FLUID) ? applyA(cf[j], f[j], p, nb) : 0.0f;
             double v = (double)p * (double)q;

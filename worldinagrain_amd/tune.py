@@ -153,7 +153,7 @@ def tune_tiles_batch(batch, sims, groups, sides=TILE_SIDES, steps=1):
 
 def tune_sweeps(world, sim, candidates=SWEEP_CANDIDATES, steps=3):
     """(sweep_pairs, level_pairs, coarse_sweeps) by the solve's time a step
-    on copies of the world.  Returns (best settings, [(ms a step, iterations
+    on copies of the world (ties within 3 % to fewer iterations).  Returns (best settings, [(ms a step, iterations
     a step, candidate), ...]); `sim.solver` is left as it was."""
     scores = []
     for sp, lp, cs in candidates:
@@ -174,5 +174,9 @@ def tune_sweeps(world, sim, candidates=SWEEP_CANDIDATES, steps=3):
             scores.append((ms / steps, its, (sp, lp, cs)))
         finally:
             w.close()
-    best = min(scores)[2]
+    # the fewest iterations among those within 3 % of the fastest: a
+    # world alone is bound by its passes' latency, and the iteration count
+    # is what counts when it steps beside others
+    fastest = min(scores)[0]
+    best = min((its, ms, cand) for ms, its, cand in scores if ms <= 1.03 * fastest)[2]
     return dataclasses.replace(sim.solver, sweep_pairs=best[0], level_pairs=best[1], coarse_sweeps=best[2]), scores
