@@ -1715,10 +1715,31 @@ __device__ __forceinline__ void phaseSync(bool wg) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
+// A phase's cells of a w-wide level, j = tid, tid + nthr, ..., with their
+// column and row stepped along (one division a phase, not one a cell).
+struct LinStep { uint32_t x, y, dx, dy, w; };
+__device__ __forceinline__ LinStep linStart(uint32_t tid, uint32_t nthr, uint32_t w) {
+    LinStep s;
+    s.w = w;
+    s.x = tid % w;
+    s.y = tid / w;
+    s.dx = nthr % w;
+    s.dy = nthr / w;
+    return s;
+}
+__device__ __forceinline__ void linNext(LinStep &s) {
+    s.x += s.dx;
+    s.y += s.dy;
+    if (s.x >= s.w) {
+        s.x -= s.w;
+        s.y += 1u;
+    }
+}
 // each cell's right and lower face betas, 0 where the stencil skips
 __device__ void ldsBetas(const LdsLevel &L, uint32_t tid, uint32_t nthr) {
-    for (uint32_t j = tid; j < L.w * L.h; j += nthr) {
-        uint32_t x = j % L.w, y = j / L.w;
+    LinStep st = linStart(tid, nthr, L.w);
+    for (uint32_t j = tid; j < L.w * L.h; j += nthr, linNext(st)) {
+        const uint32_t x = st.x, y = st.y;
         float r = L.rho[j];
         L.bR[j] = x + 1u < L.w ? faceBeta(r, L.rho[j + 1u]) : 0.0f;
         L.bD[j] = y + 1u < L.h ? faceBeta(r, L.rho[j + L.w]) : 0.0f;
@@ -1742,8 +1763,9 @@ __device__ __forceinline__ LdsSum ldsStencil(const LdsLevel &L, uint32_t x, uint
 __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega, uint32_t limit, uint32_t tid, uint32_t nthr, bool wg) {
     float h2 = (float)(1u << (2u * l));
     uint32_t hw = (L.w + 1u) / 2u;
-    for (uint32_t k = tid; k < hw * L.h; k += nthr) {
-        uint32_t y = k / hw, i = k % hw;
+    LinStep st = linStart(tid, nthr, hw);
+    for (uint32_t k = tid; k < hw * L.h; k += nthr, linNext(st)) {
+        const uint32_t y = st.y, i = st.x;
         uint32_t x = 2u * i + (((uint32_t)color + y) & 1u), j = y * L.w + x;
         if (x >= L.w || j >= limit) continue;
         float rho = L.rho[j];
@@ -1761,8 +1783,9 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
 __device__ void ldsRestrict(const Dev &d, const LdsLevel &T, const LdsLevel &U, uint32_t l, uint32_t tid, uint32_t nthr) {
     const float h2 = (float)(1u << (2u * l));
     const int32_t span = 1 << (l + 1u);
-    for (uint32_t j = tid; j < U.w * U.h; j += nthr) {
-        uint32_t x = j % U.w, y = j / U.w;
+    LinStep st = linStart(tid, nthr, U.w);
+    for (uint32_t j = tid; j < U.w * U.h; j += nthr, linNext(st)) {
+        const uint32_t x = st.x, y = st.y;
         float rho = 0.0f, rxv = 0.0f, ryv = 0.0f;
         bool shortcut = false;
         if (span <= WIG_BLOCK) {
@@ -1799,8 +1822,9 @@ __device__ void ldsRestrict(const Dev &d, const LdsLevel &T, const LdsLevel &U, 
 }
 // prolong (:2459-2497) from level l + 1 (U) into level l (T)
 __device__ void ldsProlong(const LdsLevel &T, const LdsLevel &U, uint32_t tid, uint32_t nthr) {
-    for (uint32_t j = tid; j < T.w * T.h; j += nthr) {
-        uint32_t x = j % T.w, y = j / T.w;
+    LinStep st = linStart(tid, nthr, T.w);
+    for (uint32_t j = tid; j < T.w * T.h; j += nthr, linNext(st)) {
+        const uint32_t x = st.x, y = st.y;
         if (T.rho[j] <= 0.0f) continue;
         i2 coarse = I2((int32_t)x / 2, (int32_t)y / 2);
         i2 sg = I2((x & 1u) == 1u ? 1 : -1, (y & 1u) == 1u ? 1 : -1);
@@ -1955,8 +1979,9 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
     // the levels back into the field
     for (uint32_t l = l0; l <= L; l++) {
         const LdsLevel T = level(l);
-        for (uint32_t j = tid; j < T.w * T.h; j += 1024u) {
-            const i2 c = I2((int32_t)(j % T.w), (int32_t)(j / T.w));
+        LinStep st = linStart(tid, 1024u, T.w);
+        for (uint32_t j = tid; j < T.w * T.h; j += 1024u, linNext(st)) {
+            const i2 c = I2((int32_t)st.x, (int32_t)st.y);
             lvStore<S>(d, l, c, make_float4(T.px[j], T.py[j], T.rx[j], T.ry[j]));
             if (!idle) lvStoreRho<S>(d, l, c, T.rho[j]);
         }
