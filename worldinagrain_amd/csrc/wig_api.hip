@@ -207,6 +207,7 @@ struct Batch {
     uint32_t ntimed = 0;
     hipEvent_t ev[2 * MAX_TIMED];
     bool ev_made = false;
+    int device = 0;              // the device of the worlds it was made for
 };
 
 static WorldGrid grid_of(const Batch &b, int sh) {
@@ -214,6 +215,7 @@ static WorldGrid grid_of(const Batch &b, int sh) {
     g.tab = b.tab;
     g.map = b.maps + b.map_off[sh];
     g.first = b.maps + b.first_off[sh];
+    g.solo = b.w.size() == 1u ? 1u : 0u;
     return g;
 }
 
@@ -308,6 +310,7 @@ static void batch_free(Batch *b) {
 static Batch *batch_new(WigWorld *const *worlds, uint32_t n, hipStream_t stream) {
     Batch *b = new Batch();
     b->w.assign(worlds, worlds + n);
+    b->device = worlds[0]->device;
     b->htab.resize(n);
     b->stream = stream;
     if (hipMalloc((void **)&b->tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&b->stops, 8ull * n) != hipSuccess
@@ -325,6 +328,7 @@ static int batch_maps(Batch &b) {
     std::vector<uint64_t> sig;
     for (WigWorld *w : b.w) {
         sig.push_back((uint64_t)(uintptr_t)w);
+        sig.push_back(((uint64_t)(uint32_t)w->d.W << 32) | (uint32_t)w->d.H);   // (a new world may take a freed one's address)
         sig.push_back(w->d.lds_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(w->d.tile[l]);
     }
@@ -824,7 +828,7 @@ static int run_phase(Batch &b, int ph, const StepPlan &pl) {
 static int batch_assign(Batch *b, WigWorld *const *worlds, uint32_t n) {
     if (!b || !worlds || n == 0u) return fail(WIG_E_INVALID, "a batch needs worlds");
     for (uint32_t i = 0; i < n; i++)
-        if (!worlds[i] || worlds[i]->device != b->w[0]->device) return fail(WIG_E_INVALID, "world %u is null or on another device", i);
+        if (!worlds[i] || worlds[i]->device != b->device) return fail(WIG_E_INVALID, "world %u is null or on another device", i);   // (b's own worlds may be gone)
     if (n > b->htab.size()) {
         Dev *tab = nullptr;
         uint32_t *stops = nullptr, *stops_host = nullptr, *reports = nullptr;

@@ -179,22 +179,23 @@ struct WorldGrid {
     const Dev *__restrict__ tab;
     const uint32_t *__restrict__ map;
     const uint32_t *__restrict__ first;
+    uint32_t solo;      // one world: tab[0], the whole grid its run (no map to read first)
 };
 // (d is the workgroup's own copy, read once at the kernel's start: the
 // kernels' stores could otherwise be taken to change it, and each use of a
 // field would read it again from memory)
 #define WORLD(g)                                                      \
-    const uint32_t wi_ = (g).map[blockIdx.x];                         \
+    const uint32_t wi_ = (g).solo ? 0u : (g).map[blockIdx.x];         \
     const Dev d = (g).tab[wi_];                                       \
-    const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
-    const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \
+    const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
+    const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \
     (void)nbid
 // The level kernels' view: level l of the world as level 1 of the copy and
 // level l + 1 as its level 2 (a copy indexed by a level known only at run
 // time would live in scratch; constant indices keep it in registers), and
 // level l's tile side.
 #define WORLD_LEVEL(g, l)                                             \
-    const uint32_t wi_ = (g).map[blockIdx.x];                         \
+    const uint32_t wi_ = (g).solo ? 0u : (g).map[blockIdx.x];         \
     const Dev &dg_ = (g).tab[wi_];                                    \
     Dev d = dg_;                                                      \
     d.lw[1] = dg_.lw[l];                                              \
@@ -209,16 +210,16 @@ struct WorldGrid {
     d.vrho[2] = dg_.vrho[(l) + 1u];                                   \
     d.tflag[1] = dg_.tflag[l];                                        \
     const uint32_t tile_ = dg_.tile[l];                               \
-    const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
-    const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \
+    const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
+    const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \
     (void)nbid
 // (the Dev in memory, for kernels that index its per-level arrays by a
 // level known only at run time throughout)
 #define WORLD_REF(g)                                                  \
-    const uint32_t wi_ = (g).map[blockIdx.x];                         \
+    const uint32_t wi_ = (g).solo ? 0u : (g).map[blockIdx.x];         \
     const Dev &d = (g).tab[wi_];                                      \
-    const uint32_t bid = blockIdx.x - (g).first[wi_];                  \
-    const uint32_t nbid = (g).first[wi_ + 1u] - (g).first[wi_];       \
+    const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
+    const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \
     (void)nbid
 
 struct i2 { int32_t x, y; };

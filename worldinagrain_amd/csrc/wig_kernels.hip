@@ -175,7 +175,12 @@ __device__ __forceinline__ void facesOf(const Dev &d, i2 c, Face3 fs[4]) {
 // the cell of thread li of the block at slot wid of the active list
 __device__ __forceinline__ i2 blockCell(i2 b, uint32_t li) { return I2(b.x * WIG_BLOCK + (int32_t)(li % 8u), b.y * WIG_BLOCK + (int32_t)(li / 8u)); }
 
-#define FOR_ACTIVE_BLOCKS(d, wid) for (uint32_t wid = bid, n_ = activeCount(d); wid < n_; wid += nbid)
+// (the first block's list entry is read with the count, in one round trip;
+// the next as the loop goes: blk_, activeBlockAt)
+#define FOR_ACTIVE_BLOCKS(d, wid) \
+    for (uint32_t wid = bid, n_ = activeCount(d), blk_ = (d).blocks[ACTIVE_SLOT(d, bid < (d).B ? bid : 0u)]; wid < n_; \
+         wid += nbid, blk_ = wid < n_ ? (d).blocks[ACTIVE_SLOT(d, wid)] : 0u)
+__device__ __forceinline__ i2 activeBlockAt(const Dev &d, uint32_t b) { return I2((int32_t)(b % d.bw), (int32_t)(b / d.bw)); }
 
 // -------------------------------------------------- wavefront reductions
 __device__ __forceinline__ double wave_tree_sum(double v) {
@@ -532,7 +537,7 @@ __global__ void __launch_bounds__(64) k_clear(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 b = activeBlock(d, wid);
+        i2 b = activeBlockAt(d, blk_);
         if (threadIdx.x == 0) d.blocks[blockIndex(d, b)] = 0u;
         i2 c = blockCell(b, threadIdx.x);
         if (!inWorld(d, c)) continue;
@@ -781,7 +786,7 @@ __global__ void __launch_bounds__(64) k_localJoin(WorldGrid g, int mode) {
     __shared__ uint32_t sMember[64];
     uint32_t li = threadIdx.x;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), li);
+        i2 c = blockCell(activeBlockAt(d, blk_), li);
         bool m = isMember(d, c, mode);
         sLink[li] = li;
         sMember[li] = m ? 1u : 0u;
@@ -791,7 +796,7 @@ __global__ void __launch_bounds__(64) k_localJoin(WorldGrid g, int mode) {
         __syncthreads();
         if (m) {
             uint32_t r = ldsFind(sLink, li);
-            i2 rc = blockCell(activeBlock(d, wid), r);   // within a block, index order is li order
+            i2 rc = blockCell(activeBlockAt(d, blk_), r);   // within a block, index order is li order
             CNT(d, cellIndex(d, c), G_REGION) = cellIndex(d, rc);
         }
         __syncthreads();
@@ -807,7 +812,7 @@ __global__ void __launch_bounds__(64) k_edgeJoin(WorldGrid g, int mode) {
     const uint32_t li = threadIdx.x;
     const bool right = li % 8u == 7u, down = li / 8u == 7u;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        const i2 c = blockCell(activeBlock(d, wid), li);
+        const i2 c = blockCell(activeBlockAt(d, blk_), li);
         const bool mine = (right || down) && isMember(d, c, mode);
         const bool jr = mine && right && isMember(d, I2(c.x + 1, c.y), mode);
         const bool jd = mine && down && isMember(d, I2(c.x, c.y + 1), mode);
@@ -830,7 +835,7 @@ __global__ void __launch_bounds__(64) k_sealAnchor(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c) || !mayBeSealed(d, c)) continue;
         if (wasOpen(d, beside(c, 0)) || wasOpen(d, beside(c, 1)) || wasOpen(d, beside(c, 2)) || wasOpen(d, beside(c, 3))) {
             uint32_t r = ufFind(d, cellIndex(d, c));
@@ -853,7 +858,7 @@ __global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 bk = activeBlock(d, wid);
+        i2 bk = activeBlockAt(d, blk_);
         i2 p = blockCell(bk, threadIdx.x);
         uint32_t bits = 0;
         if (inWorld(d, p)) {
@@ -961,7 +966,7 @@ __global__ void __launch_bounds__(64) k_setFaces(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         Face3 fs[4];
         facesOf(d, c, fs);
@@ -979,7 +984,7 @@ __global__ void __launch_bounds__(64) k_divergence(WorldGrid g) {
         d.blocks[COUNT_SLOT(d, COUNT_CYCLES)] = 0u;
     }
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
         if (FLD(d, 2u * i, 2) <= 0.0f) continue;
@@ -2018,7 +2023,7 @@ __global__ void __launch_bounds__(64) k_cgStart(WorldGrid g) {                  
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
         d.cx[pi] = gridX(d, c);
@@ -2032,7 +2037,7 @@ __global__ void __launch_bounds__(64) k_cgAnchor(WorldGrid g) {                 
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c) || gridZ(d, c) <= 0.0f) continue;
         for (int k = 0; k < 4; k++) {
             i2 n = beside(c, k);
@@ -2176,7 +2181,7 @@ template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) 
     if (d.store != S) return;
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 bk = activeBlock(d, wid), c = blockCell(bk, threadIdx.x);
+        i2 bk = activeBlockAt(d, blk_), c = blockCell(bk, threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
         float rho = FLD(d, 2u * i, 2);
@@ -2454,7 +2459,7 @@ template <int S> __global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
         if (stop < d.fluid_cycles) d.wstate[WS_LAST] = d.head->cycles;
     }
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
         // fluidCg as the iterations left it: p is the last iteration run's
@@ -2494,7 +2499,7 @@ __global__ void __launch_bounds__(64) k_project(WorldGrid g) {
     if (bid == 0 && threadIdx.x == 0)
         atomicMax(&d.blocks[COUNT_SLOT(d, COUNT_MOST_CYCLES)], d.blocks[COUNT_SLOT(d, COUNT_CYCLES)]);
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         Face3 fs[4];
         facesOf(d, c, fs);
@@ -2509,7 +2514,7 @@ __global__ void __launch_bounds__(64) k_extrapolate(WorldGrid g, int layer) {
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         Face3 fs[4];
         facesOf(d, c, fs);
@@ -2728,7 +2733,7 @@ __global__ void __launch_bounds__(64) k_scanBlocks(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         uint32_t n = inWorld(d, c) ? CNT(d, cellIndex(d, c), G_SORT) : 0u;
         uint32_t incl = wave_incl_scan(n);
         if (inWorld(d, c)) {
@@ -2780,7 +2785,7 @@ __global__ void __launch_bounds__(64) k_scanFinish(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (inWorld(d, c)) CNT(d, cellIndex(d, c), G_START) += d.blocks[TOTAL_SLOT(d, wid)];
     }
 }
@@ -2832,7 +2837,7 @@ __global__ void __launch_bounds__(64) k_rasterClaims(WorldGrid g) {             
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c), to = WNONE;
         uint32_t t = wType(d.cells[4ull * i]);
@@ -2853,7 +2858,7 @@ __global__ void __launch_bounds__(64) k_raster(WorldGrid g) {                   
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c), n = CNT(d, i, G_SORT);
         if (n == 0u) continue;
@@ -2941,7 +2946,7 @@ __global__ void __launch_bounds__(64) k_unpaint(WorldGrid g) {                  
     WORLD(g);
     if (!d.full) return;
     FOR_ACTIVE_BLOCKS(d, wid) {
-        i2 c = blockCell(activeBlock(d, wid), threadIdx.x);
+        i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c);
         uint4 v = *(const uint4 *)&d.cells[4ull * i];
