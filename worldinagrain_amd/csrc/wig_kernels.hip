@@ -1558,7 +1558,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     __syncthreads();
     tileSweeps(t, l, d.level_pairs);
     // p back, and each tile cell's residual (residualAt, :2306-2315)
-    float h2 = (float)(1u << (2u * l));
+    const float ih2 = 1.0f / (float)(1u << (2u * l));   // (a power of two: x / h2 and x * ih2 are the same value)
     int anyFluid = 0;
     for (uint32_t k = threadIdx.x; k < T * T; k += TILE_THREADS) {
         int32_t x = (int32_t)(k % T + H), y = (int32_t)(k / T + H), j = y * t.R + x;
@@ -1566,8 +1566,8 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
         float qx = 0.0f, qy = 0.0f;
         if (t.rho[j] > 0.0f) {
             TileSum st = tileStencil(t, j);
-            qx = t.rx[j] - (st.sx - (st.diag + 0.0f) * t.px[j]) / h2;
-            qy = t.ry[j] - (st.sy - (st.diag + 0.0f) * t.py[j]) / h2;
+            qx = t.rx[j] - (st.sx - (st.diag + 0.0f) * t.px[j]) * ih2;
+            qy = t.ry[j] - (st.sy - (st.diag + 0.0f) * t.py[j]) * ih2;
         }
         resx[k] = qx;
         resy[k] = qy;
@@ -1786,7 +1786,7 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
 // restrictLevel (:2342-2377) from level l (T) into level l + 1 (U): each
 // child's residual (residualAt, :2306-2315) summed
 __device__ void ldsRestrict(const Dev &d, const LdsLevel &T, const LdsLevel &U, uint32_t l, uint32_t tid, uint32_t nthr) {
-    const float h2 = (float)(1u << (2u * l));
+    const float ih2 = 1.0f / (float)(1u << (2u * l));   // (a power of two: x / h2 and x * ih2 are the same value)
     const int32_t span = 1 << (l + 1u);
     LinStep st = linStart(tid, nthr, U.w);
     for (uint32_t j = tid; j < U.w * U.h; j += nthr, linNext(st)) {
@@ -1810,8 +1810,8 @@ __device__ void ldsRestrict(const Dev &d, const LdsLevel &T, const LdsLevel &U, 
                     fluid += 1.0f;
                     sum += rk;
                     LdsSum st = ldsStencil(T, cx, cy);
-                    sx += T.rx[jc] - (st.sx - (st.diag + 0.0f) * T.px[jc]) / h2;
-                    sy += T.ry[jc] - (st.sy - (st.diag + 0.0f) * T.py[jc]) / h2;
+                    sx += T.rx[jc] - (st.sx - (st.diag + 0.0f) * T.px[jc]) * ih2;
+                    sy += T.ry[jc] - (st.sy - (st.diag + 0.0f) * T.py[jc]) * ih2;
                 }
             }
             rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
