@@ -84,7 +84,7 @@ def cpu_baseline(cells, kinds, size, tol, steps, solver=None):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=("single", "batch", "serve", "precision"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "serve", "precision", "snapshot"), default="single",
                     help="single: the 1024x1024 world (the metric's configuration); batch: many worlds in one batch; "
                          "serve: the serving loop over many sessions")
     ap.add_argument("--worlds", type=int, default=512, help="batch: how many worlds")
@@ -128,6 +128,8 @@ def main():
         return run_serve(args)
     if args.workload == "precision":
         return run_precision(args)
+    if args.workload == "snapshot":
+        return run_snapshot(args)
 
     if args.copy_mib:
         src = torch.empty(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -445,6 +447,58 @@ def run_precision(args):
                       "seeds": args.seeds, "fluid_cells": nf},
            "results": results}
     print(json.dumps(out), flush=True)
+
+
+def run_snapshot(args):
+    """A world's snapshot (worldinagrain_amd.snapshot, the reference's
+    SANDSNAP container) taken and loaded back: the 1024x1024 discs world
+    after 3 steps.  Times, each the best of 3: take (every part copied to
+    pageable host memory, synchronous), the container packed, gzip at
+    levels 1 and 6, and back: gunzip, unpack, put (the parts uploaded).
+    The copies' rate is against the host link; the rest is host CPU work."""
+    import numpy as np
+    from worldinagrain_amd import abi, snapshot
+    from worldinagrain_amd.gpu import GpuWorld
+
+    size = args.size
+    cells, _, kinds = build_world(size)
+    w = GpuWorld(size, size, kinds=kinds)
+    w.upload(abi.PART_CELLS, cells)
+    sim = config_simulation(w, size, args.tol)
+    for _ in range(3):
+        sim.step()
+    w.sync()
+
+    def best(f, n=3):
+        t = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            r = f()
+            t.append(time.perf_counter() - t0)
+        return min(t), r
+
+    t_take, (header, parts) = best(lambda: snapshot.take(sim))
+    nbytes = sum(p["bytes"] for p in header["parts"])
+    t_pack, payload = best(lambda: snapshot.pack_payload(header, parts))
+    t_gz1, gz1 = best(lambda: snapshot.pack(header, parts, level=1), 1)
+    t_gz6, gz6 = best(lambda: snapshot.pack(header, parts, level=6), 1)
+    t_unpack, (h2, p2) = best(lambda: snapshot.unpack(gz1), 1)
+    w2 = GpuWorld(size, size, kinds=kinds)
+    sim2 = config_simulation(w2, size, args.tol)
+    t_put, _ = best(lambda: (snapshot.put(sim2, h2, p2), w2.sync()))
+    same = all(np.array_equal(np.asarray(w.download(p)).view(np.uint8), np.asarray(w2.download(p)).view(np.uint8))
+               for p in range(abi.PART_COUNT))
+    mb = nbytes / 1e6
+    out = {"metric": "snapshot save and load rates (MB of parts a second)", "workload": "snapshot", "n_gpus": 1,
+           "config": {"workload": f"{size}x{size} discs dam after 3 steps, {len(header['parts'])} parts, {mb:.1f} MB"},
+           "part_mb": mb, "take_s": t_take, "take_GBs": nbytes / t_take / 1e9,
+           "pack_s": t_pack, "gzip1_s": t_gz1, "gzip1_MBs": mb / t_gz1, "gzip1_ratio": len(gz1) / len(payload),
+           "gzip6_s": t_gz6, "gzip6_MBs": mb / t_gz6, "gzip6_ratio": len(gz6) / len(payload),
+           "gunzip_unpack_s": t_unpack, "put_s": t_put, "put_GBs": nbytes / t_put / 1e9,
+           "loaded_equal": bool(same)}
+    print(json.dumps(out), flush=True)
+    if not same:
+        sys.exit(1)
 
 
 def run_serve(args):

@@ -13,17 +13,20 @@ that arrive while a round is on the device join the next round, so their
 steps overlap with the others'.  A request is done when its world has run
 its steps; its latency runs from its arrival to the end of that round.
 
-Eviction copies every part of a world (the snapshot parts, the solver's
-words included) to host memory and frees it; admission uploads them into a
-new world.  Each world's results are those of the same world stepped alone
-whatever it was batched with and whenever it was evicted: the batch step is
-exact per world and eviction is a copy."""
+Eviction takes the world's snapshot (snapshot.take: every part by the
+reference's names, the solver's words included, and the simulation's
+state) into host memory, without the gzip container, and frees the world;
+admission loads it into a new world (snapshot.put).  Each world's results
+are those of the same world stepped alone whatever it was batched with and
+whenever it was evicted: the batch step is exact per world and eviction is
+a copy.  Session.save() gives an idle session's world as a snapshot file's
+bytes."""
 import collections
 import time
 
 import numpy as np
 
-from . import abi
+from . import abi, snapshot
 from .gpu import GpuWorld, GpuBatch
 from .host import Simulation, BatchHost
 
@@ -69,16 +72,21 @@ class Session:
             w.upload(abi.PART_CELLS, self.cells)
             self.sim = Simulation(w, self.width, self.height, solver=self.solver_factory(), **self.sim_kw)
         else:
-            for p, data in self.snapshot.items():
-                w.upload(p, data)
-            self.snapshot = None
             self.sim.world = w
+            snapshot.put(self.sim, *self.snapshot)
+            self.snapshot = None
         self.world = w
 
     def evict(self):
-        self.snapshot = {p: self.world.download(p) for p in range(abi.PART_COUNT)}
+        self.snapshot = snapshot.take(self.sim)
         self.world.close()
         self.world = None
+
+    def save(self, level=6):
+        """The session's world as a snapshot file's bytes (resident or not)."""
+        if self.world is not None:
+            return snapshot.save(self.sim, level=level)
+        return snapshot.pack(*self.snapshot, level=level)
 
 
 class Request:
