@@ -1106,8 +1106,9 @@ __device__ v2 residualAt(const Dev &d, uint32_t l, i2 c) {                  // :
 // plus prolongFine's correction (:2500-2512) and writes pa: the buffer read
 // is the rim snapshot, and every fluid cell of the tiles is written.
 template <int S> __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c);
-__device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t na, uint32_t wid, int shift) {
-    if (wid < na) return activeBlock(d, wid);
+// (entry: the active list's entry at wid, read ahead; clamped into the list)
+__device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t na, uint32_t wid, int shift, uint32_t entry) {
+    if (wid < na) return activeBlockAt(d, entry);
     int32_t wide = (int32_t)d.bw, high = (int32_t)(d.B / d.bw);
     int32_t j = (int32_t)wid - (int32_t)d.B;
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
@@ -1122,6 +1123,11 @@ __device__ __forceinline__ i2 clampTo(i2 c, int32_t w, int32_t h) {
 template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int post) {
     WORLD(g);
     if (d.store != S) return;
+    // the first tiles' list entries, read beside the iteration's words
+    const uint32_t first = bid * WAVES * NT + (threadIdx.x / 64u) * NT;
+    uint32_t entry[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) entry[j] = d.blocks[ACTIVE_SLOT(d, min(first + (uint32_t)j, d.B - 1u))];
     const Iter it = iterWords(d);
     if (!it.encoded || it.converged) return;
     const int shift = ((d.shifted_first ^ (2u * it.cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
@@ -1149,7 +1155,8 @@ template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int
         for (int j = 0; j < NT; j++) {
             const uint32_t v = base + wave * NT + (uint32_t)j;
             const uint32_t wid = v < na ? v : d.B + (v - na);
-            i2 b = v < na + edge ? tileBlock(d, na, wid, shift) : I2(-1, -1);
+            if (v != first + (uint32_t)j) entry[j] = d.blocks[ACTIVE_SLOT(d, min(v, d.B - 1u))];
+            i2 b = v < na + edge ? tileBlock(d, na, wid, shift, entry[j]) : I2(-1, -1);
             origin[j] = I2(b.x * WIG_BLOCK - shift, b.y * WIG_BLOCK - shift);
             live[j] = !(b.x < 0 || origin[j].x >= d.W || origin[j].y >= d.H);
         }
@@ -1301,7 +1308,6 @@ template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     if (d.store != S) return;
     const Iter it = iterWords(d);
     const uint32_t gx_ = (d.lw[1] + 7u) / 8u, bx = bid % gx_, by = bid / gx_;
-    if (!it.encoded || it.converged || it.active == 0u) return;
     __shared__ float sPx[RF * RF], sPy[RF * RF];
     const uint32_t li = threadIdx.x;
     const int32_t fx0 = (int32_t)bx * 16 - 1, fy0 = (int32_t)by * 16 - 1;
@@ -1311,37 +1317,53 @@ template <int S> __global__ void __launch_bounds__(64) k_restrict(WorldGrid g) {
     // rho <= 0 restricts 0
     const i2 c = I2((int32_t)(bx * 8u + li % 8u), (int32_t)(by * 8u + li / 8u));
     const bool in = inLevel(d, 1u, c);
-    const float rho = in ? lvRho<S>(d, 1u, c) : 0.0f;
+    // The loads at addresses always there, their values chosen after: the
+    // tile's rho and block states, then (a tile with fluid) its fine cells.
+    // One world makes the first beside the iteration's words (in a batch,
+    // worlds that have converged would make them for nothing).
+    float rho = 0.0f;
     uint32_t st = 0;
-    for (int q = 0; q < 4; q++) {
-        i2 fb = I2(2 * (int32_t)bx + (q & 1), 2 * (int32_t)by + (q >> 1));
-        if (inBlocks(d, fb)) st |= d.blocks[STATE_SLOT(d, blockIndex(d, fb))];
-    }
+    float2 pv[6];
+    uint32_t kf[4];
+    float4 kc[4];
+    float2 kr[4];
+    auto loadTile = [&]() {
+        const float rr = lvRho<S>(d, 1u, clampTo(c, (int32_t)d.lw[1], (int32_t)d.lh[1]));
+        for (int q = 0; q < 4; q++) {
+            i2 fb = I2(2 * (int32_t)bx + (q & 1), 2 * (int32_t)by + (q >> 1));
+            const uint32_t sb = d.blocks[STATE_SLOT(d, inBlocks(d, fb) ? blockIndex(d, fb) : 0u)];
+            st |= inBlocks(d, fb) ? sb : 0u;
+        }
+        rho = in ? rr : 0.0f;
+    };
+    auto loadCells = [&]() {
+        for (uint32_t q = 0; q < 6u; q++) {
+            uint32_t k = li + 64u * q;
+            i2 fc = I2(fx0 + (int32_t)(k % RF), fy0 + (int32_t)(k / RF));
+            const float2 v = fnLoad<S>(d, 1, packIndex(d, clampTo(fc, d.W, d.H)));
+            pv[q] = (k < RF * RF && inWorld(d, fc)) ? v : make_float2(0.0f, 0.0f);
+        }
+        for (int q = 0; q < 4; q++) {
+            i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
+            const uint32_t ki = packIndex(d, clampTo(k, d.W, d.H));
+            const uint32_t f = d.cflag[ki];
+            const float4 cc = d.coef[ki];
+            const float2 r = fnLoad<S>(d, 2, ki);
+            const bool use = rho > 0.0f && inWorld(d, k);
+            kf[q] = use ? f : 0u;
+            kc[q] = use ? cc : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            kr[q] = use ? r : make_float2(0.0f, 0.0f);
+        }
+    };
+    const bool early = g.solo != 0u;
+    if (early) loadTile();
+    if (!it.encoded || it.converged || it.active == 0u) return;
+    if (!early) loadTile();
     if ((st & HAS_FLUID) == 0u) {
         if (in) lvStore<S>(d, 1u, c, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
         return;
     }
-    float2 pv[6];
-    for (uint32_t q = 0; q < 6u; q++) {
-        uint32_t k = li + 64u * q;
-        i2 fc = I2(fx0 + (int32_t)(k % RF), fy0 + (int32_t)(k / RF));
-        pv[q] = (k < RF * RF && inWorld(d, fc)) ? fnLoad<S>(d, 1, packIndex(d, fc)) : make_float2(0.0f, 0.0f);
-    }
-    uint32_t kf[4];
-    float4 kc[4];
-    float2 kr[4];
-    for (int q = 0; q < 4; q++) {
-        i2 k = I2(2 * c.x + (q & 1), 2 * c.y + (q >> 1));
-        kf[q] = 0u;
-        kc[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        kr[q] = make_float2(0.0f, 0.0f);
-        if (rho > 0.0f && inWorld(d, k)) {
-            uint32_t ki = packIndex(d, k);
-            kf[q] = d.cflag[ki];
-            kc[q] = d.coef[ki];
-            kr[q] = fnLoad<S>(d, 2, ki);
-        }
-    }
+    loadCells();
     for (uint32_t q = 0; q < 6u; q++) {
         uint32_t k = li + 64u * q;
         if (k < RF * RF) {
@@ -1544,14 +1566,14 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     const uint32_t T = tile_;
     const uint32_t gx_ = (d.lw[1] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
     const uint8_t fluidTile = d.tflag[1][bid];
-    if (!it.encoded || it.converged || it.active == 0u) return;
-    // a tile none of whose own cells holds fluid leaves every value as the
-    // solve's first iteration left it (only cells of rho > 0 change)
-    if (it.cycle > 0u && fluidTile == 0u) return;
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs + 1u;
     LevelTile t = levelTile(sm, T, H, bx, by);
     float *resx = sm + 7u * (uint32_t)(t.R * t.R), *resy = resx + T * T;
+    if (!it.encoded || it.converged || it.active == 0u) return;
+    // a tile none of whose own cells holds fluid leaves every value as the
+    // solve's first iteration left it (only cells of rho > 0 change)
+    if (it.cycle > 0u && fluidTile == 0u) return;
     loadLevelTile<S>(d, t);
     __syncthreads();
     tileBetas(t);
@@ -1618,44 +1640,47 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
     const uint32_t T = tile_;
     const uint32_t gx_ = (d.lw[1] + T - 1u) / T, bx = bid % gx_, by = bid / gx_;
     const uint8_t fluidTile = d.tflag[1][bid];   // (k_levelDown's of this iteration's level l)
-    if (!it.encoded || it.converged || it.active == 0u) return;
-    if (fluidTile == 0u) return;   // prolong and the sweeps change only cells of rho > 0
     extern __shared__ float sm[];
     const uint32_t H = 2u * d.level_pairs;
     LevelTile t = levelTile(sm, T, H, bx, by);
     // level l + 1 under the tile, from one cell before its first to one after its last
     const int32_t cx0 = (t.ox >> 1) - 1, cy0 = (t.oy >> 1) - 1, Rc = t.R / 2 + 3;
     float *cpx = sm + 7u * (uint32_t)(t.R * t.R), *cpy = cpx + Rc * Rc, *crho = cpy + Rc * Rc;
-    // (its loads issued with the tile's: at most three a thread)
     const int32_t nc = Rc * Rc, tid = (int32_t)threadIdx.x;
-    float2 ce[3];
-    float cr[3];
+    auto load = [&]() {
+        // (the coarse cells' loads issued with the tile's: at most three a thread)
+        float2 ce[3];
+        float cr[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int32_t k = min(tid + q * (int32_t)TILE_THREADS, nc - 1);
-        const i2 c = clampTo(I2(cx0 + k % Rc, cy0 + k / Rc), (int32_t)d.lw[2], (int32_t)d.lh[2]);
-        ce[q] = lvLoadP<S>(d, 2u, c);
-        cr[q] = lvRho<S>(d, 2u, c);
-    }
-    loadLevelTile<S>(d, t);
+        for (int q = 0; q < 3; q++) {
+            const int32_t k = min(tid + q * (int32_t)TILE_THREADS, nc - 1);
+            const i2 c = clampTo(I2(cx0 + k % Rc, cy0 + k / Rc), (int32_t)d.lw[2], (int32_t)d.lh[2]);
+            ce[q] = lvLoadP<S>(d, 2u, c);
+            cr[q] = lvRho<S>(d, 2u, c);
+        }
+        loadLevelTile<S>(d, t);
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const int32_t k = tid + q * (int32_t)TILE_THREADS;
-        if (k >= nc) continue;
-        const bool in = inLevel(d, 2u, I2(cx0 + k % Rc, cy0 + k / Rc));
-        cpx[k] = in ? ce[q].x : 0.0f;
-        cpy[k] = in ? ce[q].y : 0.0f;
-        crho[k] = in ? cr[q] : -1.0f;
-    }
-    for (int32_t k = tid + 3 * (int32_t)TILE_THREADS; k < nc; k += (int32_t)TILE_THREADS) {   // (sides past 48)
-        const i2 c = I2(cx0 + k % Rc, cy0 + k / Rc), cc = clampTo(c, (int32_t)d.lw[2], (int32_t)d.lh[2]);
-        const bool in = inLevel(d, 2u, c);
-        const float2 e = lvLoadP<S>(d, 2u, cc);
-        const float rr = lvRho<S>(d, 2u, cc);
-        cpx[k] = in ? e.x : 0.0f;
-        cpy[k] = in ? e.y : 0.0f;
-        crho[k] = in ? rr : -1.0f;
-    }
+        for (int q = 0; q < 3; q++) {
+            const int32_t k = tid + q * (int32_t)TILE_THREADS;
+            if (k >= nc) continue;
+            const bool in = inLevel(d, 2u, I2(cx0 + k % Rc, cy0 + k / Rc));
+            cpx[k] = in ? ce[q].x : 0.0f;
+            cpy[k] = in ? ce[q].y : 0.0f;
+            crho[k] = in ? cr[q] : -1.0f;
+        }
+        for (int32_t k = tid + 3 * (int32_t)TILE_THREADS; k < nc; k += (int32_t)TILE_THREADS) {   // (sides past 48)
+            const i2 c = I2(cx0 + k % Rc, cy0 + k / Rc), cc = clampTo(c, (int32_t)d.lw[2], (int32_t)d.lh[2]);
+            const bool in = inLevel(d, 2u, c);
+            const float2 e = lvLoadP<S>(d, 2u, cc);
+            const float rr = lvRho<S>(d, 2u, cc);
+            cpx[k] = in ? e.x : 0.0f;
+            cpy[k] = in ? e.y : 0.0f;
+            crho[k] = in ? rr : -1.0f;
+        }
+    };
+    if (!it.encoded || it.converged || it.active == 0u) return;
+    if (fluidTile == 0u) return;   // prolong and the sweeps change only cells of rho > 0
+    load();
     __syncthreads();
     // prolong (:2459-2497): prolonged() from the staged coarse cells
     FOR_TILE(t, x, y) {
@@ -1857,10 +1882,9 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
     WORLD_REF(g);
     if (d.store != S) return;
     const Iter it = iterWords(d);
-    if (!it.encoded || it.converged) return;
     extern __shared__ float sm[];
-    const uint32_t l0 = it.active == 0u ? d.L : d.lds_level, L = d.L;
-    const bool idle = it.active == 0u;   // restrict and prolong pass over nothing; solve still sweeps
+    uint32_t l0 = d.lds_level;
+    const uint32_t L = d.L;
     const uint32_t tid = threadIdx.x, W = (uint32_t)d.W, H = (uint32_t)d.H;
     const uint32_t pairs = d.level_pairs, csweeps = d.coarse_sweeps;
     // level l's arrays in LDS, levels l0 .. L one after another (the level
@@ -1881,9 +1905,9 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
         T.bD = sm + off + 6u * n;
         return T;
     };
-    {
-        // level l0 as the pass above left it in the field (or, with no
-        // blocks worked on, the coarsest as it was)
+    // level l0 as the pass above left it in the field (or, with no blocks
+    // worked on, the coarsest as it was)
+    auto loadFirst = [&]() {
         // (every load made at once: six a thread hold what LDS can)
         const LdsLevel T = level(l0);
         const uint32_t n = T.w * T.h;
@@ -1917,10 +1941,14 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
             T.ry[j] = ee.w;
             T.rho[j] = lvRho<S>(d, l0, c);
         }
-        __syncthreads();
-        ldsBetas(T, tid, 1024u);
-        __syncthreads();
-    }
+    };
+    if (!it.encoded || it.converged) return;
+    const bool idle = it.active == 0u;   // restrict and prolong pass over nothing; solve still sweeps
+    if (idle) l0 = L;
+    loadFirst();
+    __syncthreads();
+    ldsBetas(level(l0), tid, 1024u);
+    __syncthreads();
     // The levels from lw down are the first wavefront's: the first below
     // l0 with at most wave_cells cells (L + 1: none).
     uint32_t lw = L + 1u;
@@ -2065,26 +2093,70 @@ __device__ __forceinline__ double ld_f64(const double *p) {
 // (1024 lanes, lane i taking places i, i + 1024, ..., then halving; two sums
 // at once where a pass has two)
 #define SUM_LANES 1024u
-__device__ void sumPlaces2(const Dev &d, uint32_t w0, uint32_t w1, double *s0, double *s1, double *out0, double *out1) {
-    uint32_t li = threadIdx.x, na = activeCount(d);
-    double a0 = 0.0, a1 = 0.0;
-    for (uint32_t j = li; j < na; j += SUM_LANES) {
-        a0 += d.bpart[4ull * j + w0];
-        a1 += d.bpart[4ull * j + w1];
+// Each lane's places, SUM_HELD of them (16384 places in all), held in
+// registers with their loads issued at once (those below the count: one
+// workgroup's loads are the launch's, and places past the count would
+// only slow it); places past those are read as the sum goes.
+#define SUM_HELD 16
+template <int F> struct PlaceHold { double v[F][SUM_HELD]; uint32_t na; };
+template <int F> __device__ __forceinline__ void holdPlaces(const Dev &d, uint32_t na, const uint32_t (&w)[F], PlaceHold<F> &h) {
+    h.na = na;
+#pragma unroll
+    for (int q = 0; q < SUM_HELD; q++) {
+        const uint32_t j = threadIdx.x + (uint32_t)q * SUM_LANES;
+#pragma unroll
+        for (int f = 0; f < F; f++) h.v[f][q] = j < na ? d.bpart[4ull * j + w[f]] : 0.0;
     }
+}
+// this lane's sum of word w (held as field f): places lane, lane +
+// SUM_LANES, ... below the count, in that order
+template <int F> __device__ __forceinline__ double laneSum(const Dev &d, const PlaceHold<F> &h, int f, uint32_t w) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < SUM_HELD; q++)
+        if (threadIdx.x + (uint32_t)q * SUM_LANES < h.na) a += h.v[f][q];
+    for (uint32_t j = threadIdx.x + SUM_HELD * SUM_LANES; j < h.na; j += SUM_LANES) a += d.bpart[4ull * j + w];
+    return a;
+}
+// The lanes' values halved, lane i taking lane i + h for h = 512 .. 1:
+// through LDS while h spans wavefronts, then within the first wavefront
+// (its shuffles take the same operands in the same order); the totals to
+// every thread.  m: a max over the lanes alongside, by the same halving.
+__device__ __forceinline__ void laneTree(double a0, double a1, uint32_t m, double *s0, double *s1, uint32_t *sm,
+                                         double *o0, double *o1, uint32_t *om) {
+    const uint32_t li = threadIdx.x;
     s0[li] = a0;
     s1[li] = a1;
+    if (sm) sm[li] = m;
     __syncthreads();
-    for (uint32_t h = SUM_LANES / 2u; h > 0; h >>= 1) {
+    for (uint32_t h = SUM_LANES / 2u; h >= 64u; h >>= 1) {
         if (li < h) {
             s0[li] += s0[li + h];
             s1[li] += s1[li + h];
+            if (sm) sm[li] = max(sm[li], sm[li + h]);
         }
         __syncthreads();
     }
-    *out0 = s0[0];
-    *out1 = s1[0];
+    if (li < 64u) {
+        double v0 = wave_tree_sum(s0[li]), v1 = wave_tree_sum(s1[li]);
+        uint32_t vm = sm ? wave_max_u32(sm[li]) : 0u;
+        if (li == 0u) {
+            s0[0] = v0;
+            s1[0] = v1;
+            if (sm) sm[0] = vm;
+        }
+    }
     __syncthreads();
+    *o0 = s0[0];
+    *o1 = s1[0];
+    if (sm) *om = sm[0];
+    __syncthreads();
+}
+__device__ void sumPlaces2(const Dev &d, uint32_t w0, uint32_t w1, double *s0, double *s1, double *out0, double *out1) {
+    PlaceHold<2> h;
+    const uint32_t w[2] = {w0, w1};
+    holdPlaces<2>(d, activeCount(d), w, h);
+    laneTree(laneSum<2>(d, h, 0, w0), laneSum<2>(d, h, 1, w1), 0u, s0, s1, nullptr, out0, out1, nullptr);
 }
 #define FOR_ACTIVE_BLOCKS4(d, wid) \
     for (uint32_t wid = bid * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += nbid * WAVES)
@@ -2093,17 +2165,19 @@ __device__ void sumPlaces2(const Dev &d, uint32_t w0, uint32_t w1, double *s0, d
 // list a wavefront at once, so that NB blocks' loads are in flight together
 // (each block's sums are still its own wavefront halving, as above).
 #define NB 4u
-#define FOR_PLACES(it, w0) \
-    for (uint32_t w0 = (bid * WAVES + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64u)) * NB; w0 < (it).active; w0 += nbid * WAVES * NB)
 // this lane's cell in the blocks at places w0 .. w0 + NB - 1, as packed
 // indices (past the list's end block 0's: loads there are harmless, stores
 // are not made).  Every lane of a block has its slot; those outside the
-// world hold cflag 0.
-__device__ __forceinline__ void placeCells(const Dev &d, const Iter &it, uint32_t w0, uint32_t *i) {
-    for (uint32_t j = 0; j < NB; j++) {
-        uint32_t b = w0 + j < it.active ? d.blocks[ACTIVE_SLOT(d, w0 + j)] : 0u;
-        i[j] = b * 64u + LANE;
-    }
+// world hold cflag 0.  A pass reads its first places' list entries beside
+// the iteration's words (the loads clamped into the list, made whatever the
+// count), and each next places' while it works on these.
+#define PLACES_FIRST ((bid * WAVES + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64u)) * NB)
+#define PLACES_STRIDE (nbid * WAVES * NB)
+__device__ __forceinline__ void placeEntries(const Dev &d, uint32_t w0, uint32_t *e) {
+    for (uint32_t j = 0; j < NB; j++) e[j] = d.blocks[ACTIVE_SLOT(d, min(w0 + j, d.B - 1u))];
+}
+__device__ __forceinline__ void placeCellsFrom(const Iter &it, uint32_t w0, const uint32_t *e, uint32_t *i) {
+    for (uint32_t j = 0; j < NB; j++) i[j] = (w0 + j < it.active ? e[j] : 0u) * 64u + LANE;
 }
 // the packed index of the cell beside packed cell i (beside()'s k) in the
 // next block over, for a lane on that edge of its block; i itself where
@@ -2220,11 +2294,15 @@ template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) 
 template <int S> __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
+    const uint32_t wf = PLACES_FIRST;
+    uint32_t e[NB];
+    placeEntries(d, wf, e);
     const Iter it = iterWords(d);
     if (!it.encoded || it.converged) return;
-    FOR_PLACES(it, w0) {
+    for (uint32_t w0 = wf; w0 < it.active; w0 += PLACES_STRIDE) {
         uint32_t i[NB];
-        placeCells(d, it, w0, i);
+        placeCellsFrom(it, w0, e, i);
+        placeEntries(d, w0 + PLACES_STRIDE, e);
         uint32_t f[NB];
         float z[NB], r[NB], q[NB];
         for (uint32_t j = 0; j < NB; j++) {
@@ -2251,11 +2329,14 @@ template <int S> __global__ void __launch_bounds__(256) k_cgRho(WorldGrid g) {
 __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
     WORLD(g);
     const Iter it = iterWords(d);
+    PlaceHold<2> ph;
+    const uint32_t w[2] = {0u, 1u};
+    holdPlaces<2>(d, it.active, w, ph);
     const uint32_t cycle = it.cycle;
     if (!it.encoded || it.converged) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     double s0, s1;
-    sumPlaces2(d, 0, 1, t0, t1, &s0, &s1);
+    laneTree(laneSum<2>(d, ph, 0, 0u), laneSum<2>(d, ph, 1, 1u), 0u, t0, t1, nullptr, &s0, &s1, nullptr);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
         double beta = (cycle == 0u || h->pq <= 0.0) ? 0.0 : -s1 / h->pq;
@@ -2270,6 +2351,9 @@ __global__ void __launch_bounds__(1024) k_cgBeta(WorldGrid g) {
 template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
+    const uint32_t wf = PLACES_FIRST;
+    uint32_t e[NB];
+    placeEntries(d, wf, e);
     const Iter it = iterWords(d);
     const double beta = ld_f64(&d.head->beta);
     if (!it.encoded || it.converged) return;
@@ -2284,9 +2368,10 @@ template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
     const uint32_t ek = (LANE / 8u) & 3u, em = LANE % 8u;
     const uint32_t eat = ek == 0u ? em * 8u : (ek == 1u ? em * 8u + 7u : (ek == 2u ? em : 56u + em));
     const uint32_t slot[4] = {y, 8u + y, 16u + x, 24u + x};
-    FOR_PLACES(it, w0) {
+    for (uint32_t w0 = wf; w0 < it.active; w0 += PLACES_STRIDE) {
         uint32_t i[NB];
-        placeCells(d, it, w0, i);
+        placeCellsFrom(it, w0, e, i);
+        placeEntries(d, w0 + PLACES_STRIDE, e);
         uint32_t f[NB];
         float z[NB], po[NB], pe[NB];
         float4 cf[NB];
@@ -2325,10 +2410,13 @@ template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
 __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
     WORLD(g);
     const Iter it = iterWords(d);
+    PlaceHold<1> ph;
+    const uint32_t w[1] = {0u};
+    holdPlaces<1>(d, it.active, w, ph);
     if (!it.encoded || it.converged) return;
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
     double pq, unused;
-    sumPlaces2(d, 0, 0, t0, t1, &pq, &unused);
+    laneTree(laneSum<1>(d, ph, 0, 0u), 0.0, 0u, t0, t1, nullptr, &pq, &unused, nullptr);
     if (threadIdx.x == 0) {
         SolveHead *h = d.head;
         h->alpha = pq <= 0.0 ? 0.0 : h->rz / pq;
@@ -2343,6 +2431,9 @@ __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
 template <int S> __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) {
     WORLD(g);
     if (d.store != S) return;
+    const uint32_t wf = PLACES_FIRST;
+    uint32_t e[NB];
+    placeEntries(d, wf, e);
     const Iter it = iterWords(d);
     const double alpha = ld_f64(&d.head->alpha);
     const float vscale = d.head->vscale;
@@ -2354,9 +2445,10 @@ template <int S> __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) 
     }
     if (it.converged) return;
     const float *pn = pBuf(d, cycle + 1u);
-    FOR_PLACES(it, w0) {
+    for (uint32_t w0 = wf; w0 < it.active; w0 += PLACES_STRIDE) {
         uint32_t i[NB];
-        placeCells(d, it, w0, i);
+        placeCellsFrom(it, w0, e, i);
+        placeEntries(d, w0 + PLACES_STRIDE, e);
         uint32_t f[NB];
         float x[NB], r[NB], p[NB], q[NB];
         for (uint32_t j = 0; j < NB; j++) {
@@ -2397,6 +2489,9 @@ __device__ __forceinline__ void stepCycle(const Dev &d, uint32_t cycle) {
 __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
     WORLD(g);
     const Iter it = iterWords(d);
+    PlaceHold<2> ph;
+    const uint32_t w[2] = {2u, 3u};
+    holdPlaces<2>(d, it.active, w, ph);
     const uint32_t cycle = it.cycle;
     if (!d.full) return;
     if (!it.encoded || it.converged) {
@@ -2404,25 +2499,26 @@ __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
         return;
     }
     __shared__ double t0[SUM_LANES], t1[SUM_LANES];
-    uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
-    double rr, unused;
-    sumPlaces2(d, 2, 2, t0, t1, &rr, &unused);
-    // the most |r| left in any cell (non-negative floats order as their bits)
     __shared__ uint32_t sMost[SUM_LANES];
-    uint32_t m = 0, na = activeCount(d);
-    for (uint32_t j = threadIdx.x; j < na; j += SUM_LANES) {
+    uint32_t *slot = &d.blocks[COUNT_SLOT(d, COUNT_RESIDUAL + cycle % 3u)];
+    // the sum of r.r, and the most |r| left in any cell (a block's as f32
+    // bits; non-negative floats order as their bits)
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < SUM_HELD; q++) {
+        const uint32_t b = (uint32_t)ph.v[1][q];
+        if (threadIdx.x + (uint32_t)q * SUM_LANES < ph.na) m = b > m ? b : m;
+    }
+    for (uint32_t j = threadIdx.x + SUM_HELD * SUM_LANES; j < ph.na; j += SUM_LANES) {
         uint32_t b = (uint32_t)d.bpart[4ull * j + 3];
         m = b > m ? b : m;
     }
-    sMost[threadIdx.x] = m;
-    __syncthreads();
-    for (uint32_t h = SUM_LANES / 2u; h > 0; h >>= 1) {
-        if (threadIdx.x < h) sMost[threadIdx.x] = max(sMost[threadIdx.x], sMost[threadIdx.x + h]);
-        __syncthreads();
-    }
+    double rr, unused;
+    uint32_t most;
+    laneTree(laneSum<2>(d, ph, 0, 2u), 0.0, m, t0, t1, sMost, &rr, &unused, &most);
     if (threadIdx.x == 0) {
-        if (sMost[0] > *slot) *slot = sMost[0];
-        d.head->vscale = vcycleScale(d, sMost[0]);
+        if (most > *slot) *slot = most;
+        d.head->vscale = vcycleScale(d, most);
         SolveHead *h = d.head;
         h->rr = rr;
         h->cycles += 1u;
