@@ -536,7 +536,9 @@ def run_serve(args):
             break
         arrivals.append((t, int(rng.integers(args.sessions)), int(rng.integers(1, 9))))
     stream = torch.cuda.Stream()
-    server = Server(sessions, int(args.budget_gb * (1 << 30)), max_round_cells=args.round_cells, stream=stream.cuda_stream)
+    copy = torch.cuda.Stream()      # the evictions' (a thread of their own)
+    server = Server(sessions, int(args.budget_gb * (1 << 30)), max_round_cells=args.round_cells, stream=stream.cuda_stream,
+                    copy_stream=copy.cuda_stream)
     last = [time.perf_counter()]
 
     def progress(sv):

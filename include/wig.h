@@ -237,11 +237,13 @@ int wig_cg64_x(WigWorld *w, double *x);
 
 /* Batches: many worlds, of any sizes, stepped together, every kernel of the
  * step one launch over all of them (this library's own; the reference steps
- * one Simulation a frame).  The worlds stay the caller's and must outlive
- * the batch; a world's results do not depend on the batch it is stepped in
+ * one Simulation a frame).  The worlds stay the caller's: each must outlive
+ * its use by the batch, until the batch is destroyed or assigned other
+ * worlds.  A world's results do not depend on the batch it is stepped in
  * (each runs the solver iterations its own host would encode).  The batch
- * runs on the first world's stream unless set otherwise; a world stepped
- * both alone and in batches is the caller's to order. */
+ * runs on a stream of its own unless set otherwise (wig_batch_set_stream;
+ * NULL: its own again); a world stepped both alone and in batches, or
+ * copied on its own stream, is the caller's to order. */
 typedef struct WigBatch WigBatch;
 WigBatch *wig_batch_create(WigWorld *const *worlds, uint32_t n);
 int wig_batch_destroy(WigBatch *b);

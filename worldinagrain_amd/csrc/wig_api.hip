@@ -197,6 +197,7 @@ struct Batch {
     uint32_t *stops = nullptr, *stops_host = nullptr;   // k_stops' words (device, pinned host)
     uint32_t *reports = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t own = nullptr;   // wig_batch_create's: the stream when none is set (not a world's, which may go first)
     // hipGraphs of the step's phases (run_phase),
     // captured for the launch layout `gsig` and replayed while it holds
     bool graphs = false;
@@ -304,6 +305,7 @@ static void batch_free(Batch *b) {
         if (x) (void)hipGraphExecDestroy(x);
     if (b->ev_made)
         for (int i = 0; i < 2 * MAX_TIMED; i++) (void)hipEventDestroy(b->ev[i]);
+    if (b->own) (void)hipStreamDestroy(b->own);
     delete b;
 }
 
@@ -995,7 +997,18 @@ WigBatch *wig_batch_create(WigWorld *const *worlds, uint32_t n) {
             fail(WIG_E_INVALID, "world %u is null or on another device", i);
             return nullptr;
         }
-    return (WigBatch *)batch_new(worlds, n, worlds[0]->own);
+    hipStream_t own = nullptr;
+    if (hipStreamCreate(&own) != hipSuccess) {
+        fail(WIG_E_DEVICE, "hipStreamCreate failed");
+        return nullptr;
+    }
+    Batch *b = batch_new(worlds, n, own);
+    if (!b) {
+        (void)hipStreamDestroy(own);
+        return nullptr;
+    }
+    b->own = own;
+    return (WigBatch *)b;
 }
 
 int wig_batch_assign(WigBatch *bb, WigWorld *const *worlds, uint32_t n) {
@@ -1010,7 +1023,7 @@ int wig_batch_destroy(WigBatch *b) {
 int wig_batch_set_stream(WigBatch *b, void *stream) {
     if (!b) return fail(WIG_E_INVALID, "null batch");
     Batch *B = (Batch *)b;
-    B->stream = stream ? (hipStream_t)stream : B->w[0]->own;
+    B->stream = stream ? (hipStream_t)stream : B->own;
     return WIG_OK;
 }
 

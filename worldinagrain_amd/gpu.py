@@ -76,6 +76,29 @@ def _check(status):
 _DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8}
 
 
+class HipStream:
+    """A HIP stream (default priority) of the HIP runtime libwig.so runs on,
+    for set_stream and the serving loop's copy stream.  (A process holds one
+    runtime if torch, which carries its own, is imported before libwig.so
+    loads: then torch's streams serve as well.  Loaded the other way round,
+    the two runtimes' streams do not mix, and these are the ones to use.)"""
+
+    def __init__(self):
+        lib()
+        self._hip = ct.CDLL("libamdhip64.so.7")         # the runtime libwig.so loaded
+        self._hip.hipStreamCreate.argtypes = [ct.POINTER(ct.c_void_p)]
+        self._hip.hipStreamDestroy.argtypes = [ct.c_void_p]
+        h = ct.c_void_p()
+        if self._hip.hipStreamCreate(ct.byref(h)) != 0:
+            raise WigError("hipStreamCreate failed")
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            self._hip.hipStreamDestroy(ct.c_void_p(self.handle))
+            self.handle = None
+
+
 class GpuWorld:
     """A world on the GPU (wig_create); the same interface as the tests' oracle world."""
 

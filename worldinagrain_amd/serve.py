@@ -6,8 +6,9 @@ A request asks for `steps` consecutive liquid steps of one session's world.
 The server runs rounds.  A round takes the requests that have arrived, in
 arrival order, and makes their worlds resident: a world is admitted only if
 it fits the HBM budget, after idle worlds (no request waiting) are evicted
-to host memory, least recently used first; a request whose world cannot be
-admitted waits for a later round.  Then one step of every admitted
+to host memory, least recently used first (in a thread of their own, on a
+copy stream, while the rounds go on: Evictor); a request whose world cannot
+be admitted waits for a later round.  Then one step of every admitted
 request's world runs in a single batch step (wig_batch_step).  Requests
 that arrive while a round is on the device join the next round, so their
 steps overlap with the others'.  A request is done when its world has run
@@ -22,6 +23,8 @@ whenever it was evicted: the batch step is exact per world and eviction is
 a copy.  Session.save() gives an idle session's world as a snapshot file's
 bytes."""
 import collections
+import queue
+import threading
 import time
 
 import numpy as np
@@ -63,6 +66,7 @@ class Session:
         self.last_used = -1
         self.bytes = world_bytes(width, height, coarse_levels)
         self.steps_run = 0
+        self.evicting = None    # an eviction in flight: set when it is done
 
     def make_resident(self, stream=None):
         w = GpuWorld(self.width, self.height, coarse_levels=self.coarse_levels, kinds=self.kinds)
@@ -89,6 +93,72 @@ class Session:
         return snapshot.pack(*self.snapshot, level=level)
 
 
+class Evictor:
+    """Evictions off the serving loop: a thread of its own takes each idle
+    world's snapshot on a copy stream of its own while the rounds run on the
+    server's, then frees the world.  The device holds the worlds in flight
+    beside the budget; past `limit` bytes of them the loop waits for the
+    thread.  (The worlds evicted are idle: their last round is done, and no
+    round uses them until their eviction is.)"""
+
+    def __init__(self, stream, limit):
+        self.stream, self.limit = stream, limit
+        self.inflight = 0
+        self.error = None
+        self.cv = threading.Condition()
+        self.q = queue.Queue()
+        self.thread = threading.Thread(target=self._run, name="evictor", daemon=True)
+        self.thread.start()
+
+    def submit(self, s):
+        with self.cv:
+            while self.inflight > 0 and self.inflight + s.bytes > self.limit:
+                self.cv.wait()
+            self._check()
+            self.inflight += s.bytes
+        s.evicting = threading.Event()
+        s.world.set_stream(self.stream)
+        self.q.put(s)
+
+    def wait(self, s):
+        """Waits for s's eviction, if one is in flight."""
+        if s.evicting is not None:
+            s.evicting.wait()
+            s.evicting = None
+            with self.cv:
+                self._check()
+
+    def drain(self):
+        """Waits for every eviction in flight."""
+        with self.cv:
+            while self.inflight > 0:
+                self.cv.wait()
+            self._check()
+
+    def _check(self):
+        if self.error is not None:
+            raise RuntimeError("an eviction failed") from self.error
+
+    def _run(self):
+        while True:
+            s = self.q.get()
+            if s is None:
+                return
+            try:
+                s.evict()
+            except BaseException as e:      # (raised on the loop's next submit or wait)
+                self.error = e
+            with self.cv:
+                self.inflight -= s.bytes
+                self.cv.notify_all()
+            s.evicting.set()
+
+    def close(self):
+        self.q.put(None)
+        self.thread.join()
+        self._check()
+
+
 class Request:
     __slots__ = ("sid", "steps", "arrival", "left", "done_at")
 
@@ -101,11 +171,15 @@ class Request:
 class Server:
     """Continuous batching over sessions under an HBM budget (bytes)."""
 
-    def __init__(self, sessions, budget, max_round_cells=1 << 24, stream=None):
+    def __init__(self, sessions, budget, max_round_cells=1 << 24, stream=None, copy_stream=None, evicting_limit=16 << 30):
+        """copy_stream: evictions run in a thread of their own on this stream
+        (Evictor), at most evicting_limit bytes of worlds in flight; None:
+        each in the loop, before its round."""
         self.sessions = {s.sid: s for s in sessions}
         self.budget = budget
         self.max_round_cells = max_round_cells
         self.stream = stream
+        self.evictor = Evictor(copy_stream, evicting_limit) if copy_stream is not None else None
         self.resident_bytes = 0
         self.batch = None
         self.round_no = 0
@@ -119,17 +193,23 @@ class Server:
     def _admit(self, s, busy):
         """Makes session s resident within the budget, evicting idle worlds
         (not in this round and with no request waiting) LRU first."""
+        if self.evictor is not None:
+            self.evictor.wait(s)
         if s.world is not None:
             return True
         if s.bytes > self.budget:
             raise ValueError(f"session {s.sid}: a {s.width}x{s.height} world does not fit the budget")
         if self.resident_bytes + s.bytes > self.budget:
-            idle = sorted((x for x in self.sessions.values() if x.world is not None and x.sid not in busy),
+            idle = sorted((x for x in self.sessions.values()
+                           if x.world is not None and x.evicting is None and x.sid not in busy),
                           key=lambda x: x.last_used)
             for x in idle:
                 if self.resident_bytes + s.bytes <= self.budget:
                     break
-                x.evict()
+                if self.evictor is not None:
+                    self.evictor.submit(x)
+                else:
+                    x.evict()
                 self.resident_bytes -= x.bytes
                 self.evictions += 1
             if self.resident_bytes + s.bytes > self.budget:
@@ -167,7 +247,7 @@ class Server:
                 c = s.width * s.height
                 if chosen and cells + c > self.max_round_cells:
                     continue
-                if not self._admit(s, busy | (waiting if s.world is None else set())):
+                if not self._admit(s, busy | (waiting if s.world is None or s.evicting is not None else set())):
                     continue
                 chosen.append(r)
                 busy.add(r.sid)
@@ -198,10 +278,15 @@ class Server:
             queue = collections.deque(r for r in queue if r.left > 0)
             if on_round:
                 on_round(self)
+        if self.evictor is not None:
+            self.evictor.drain()     # (the evictions in flight counted in the run's time)
         self.elapsed = clock() - t0
         return reqs
 
     def close(self):
+        if self.evictor is not None:
+            self.evictor.close()
+            self.evictor = None
         if self.batch is not None:
             self.batch.close()
             self.batch = None
