@@ -6,9 +6,10 @@ A request asks for `steps` consecutive liquid steps of one session's world.
 The server runs rounds.  A round takes the requests that have arrived, in
 arrival order, and makes their worlds resident: a world is admitted only if
 it fits the HBM budget, after idle worlds (no request waiting) are evicted
-to host memory, least recently used first (in a thread of their own, on a
-copy stream, while the rounds go on: Evictor); a request whose world cannot
-be admitted waits for a later round.  Then one step of every admitted
+to host memory, least recently used first; a request whose world cannot be
+admitted waits for a later round.  Given a copy stream, evictions and
+admissions run in threads of their own on it while the rounds go on
+(Evictor, Admitter), and a world joins the first round after it is ready.  Then one step of every admitted
 request's world runs in a single batch step (wig_batch_step).  Requests
 that arrive while a round is on the device join the next round, so their
 steps overlap with the others'.  A request is done when its world has run
@@ -67,6 +68,7 @@ class Session:
         self.bytes = world_bytes(width, height, coarse_levels)
         self.steps_run = 0
         self.evicting = None    # an eviction in flight: set when it is done
+        self.admitting = None   # an admission in flight: set when the world is ready
 
     def make_resident(self, stream=None):
         w = GpuWorld(self.width, self.height, coarse_levels=self.coarse_levels, kinds=self.kinds)
@@ -159,6 +161,62 @@ class Evictor:
         self._check()
 
 
+class Admitter:
+    """Admissions off the serving loop: a thread of its own makes each
+    session's world resident (created, and its cells or its snapshot
+    uploaded) on a copy stream of its own while the rounds run; the loop
+    takes the session into a round once it is ready.  (A session evicted and
+    asked for again is admitted once its eviction is done.)"""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.error = None
+        self.inflight = 0
+        self.cv = threading.Condition()
+        self.q = queue.Queue()
+        self.thread = threading.Thread(target=self._run, name="admitter", daemon=True)
+        self.thread.start()
+
+    def submit(self, s):
+        self._check()
+        s.admitting = threading.Event()
+        with self.cv:
+            self.inflight += 1
+        self.q.put(s)
+
+    def wait_any(self, timeout):
+        """Waits (at most timeout s) for an admission in flight to be done."""
+        with self.cv:
+            if self.inflight > 0:
+                self.cv.wait(timeout)
+        self._check()
+
+    def _check(self):
+        if self.error is not None:
+            raise RuntimeError("an admission failed") from self.error
+
+    def _run(self):
+        while True:
+            s = self.q.get()
+            if s is None:
+                return
+            try:
+                if s.evicting is not None:
+                    s.evicting.wait()
+                s.make_resident(self.stream)
+            except BaseException as e:      # (raised on the loop's next submit or wait)
+                self.error = e
+            s.admitting.set()
+            with self.cv:
+                self.inflight -= 1
+                self.cv.notify_all()
+
+    def close(self):
+        self.q.put(None)
+        self.thread.join()
+        self._check()
+
+
 class Request:
     __slots__ = ("sid", "steps", "arrival", "left", "done_at")
 
@@ -172,14 +230,16 @@ class Server:
     """Continuous batching over sessions under an HBM budget (bytes)."""
 
     def __init__(self, sessions, budget, max_round_cells=1 << 24, stream=None, copy_stream=None, evicting_limit=16 << 30):
-        """copy_stream: evictions run in a thread of their own on this stream
-        (Evictor), at most evicting_limit bytes of worlds in flight; None:
-        each in the loop, before its round."""
+        """copy_stream: evictions and admissions run in threads of their own
+        on this stream (Evictor, Admitter), at most evicting_limit bytes of
+        worlds in eviction at once; None: each in the loop, in the round
+        that needs it."""
         self.sessions = {s.sid: s for s in sessions}
         self.budget = budget
         self.max_round_cells = max_round_cells
         self.stream = stream
         self.evictor = Evictor(copy_stream, evicting_limit) if copy_stream is not None else None
+        self.admitter = Admitter(copy_stream) if copy_stream is not None else None
         self.resident_bytes = 0
         self.batch = None
         self.round_no = 0
@@ -190,18 +250,15 @@ class Server:
         # world made resident for it or on its first step)
         self.rounds = []
 
-    def _admit(self, s, busy):
-        """Makes session s resident within the budget, evicting idle worlds
-        (not in this round and with no request waiting) LRU first."""
-        if self.evictor is not None:
-            self.evictor.wait(s)
-        if s.world is not None:
-            return True
+    def _reserve(self, s, busy):
+        """Room in the budget for session s's world, evicting idle worlds (not
+        in this round, with no request waiting, not on their way in or out)
+        LRU first.  False: no room."""
         if s.bytes > self.budget:
             raise ValueError(f"session {s.sid}: a {s.width}x{s.height} world does not fit the budget")
         if self.resident_bytes + s.bytes > self.budget:
             idle = sorted((x for x in self.sessions.values()
-                           if x.world is not None and x.evicting is None and x.sid not in busy),
+                           if x.world is not None and x.evicting is None and x.admitting is None and x.sid not in busy),
                           key=lambda x: x.last_used)
             for x in idle:
                 if self.resident_bytes + s.bytes <= self.budget:
@@ -214,10 +271,36 @@ class Server:
                 self.evictions += 1
             if self.resident_bytes + s.bytes > self.budget:
                 return False
-        s.make_resident(self.stream)
         self.resident_bytes += s.bytes
         self.admissions += 1
         return True
+
+    def _admit(self, s, busy):
+        """Makes session s resident within the budget, in the loop."""
+        if self.evictor is not None:
+            self.evictor.wait(s)
+        if s.world is not None:
+            return True
+        if not self._reserve(s, busy):
+            return False
+        s.make_resident(self.stream)
+        return True
+
+    def _ready(self, s, busy):
+        """With the threads: whether s's world is resident and settled (its
+        eviction or admission done); if it is not and none is on its way,
+        its admission is started when the budget has room."""
+        if s.evicting is not None and s.evicting.is_set():
+            self.evictor.wait(s)
+        if s.admitting is not None and s.admitting.is_set():
+            s.admitting = None
+            self.admitter.wait_any(0.0)          # (an admission's error, raised)
+            s.world.set_stream(self.stream)
+        if s.world is not None and s.evicting is None and s.admitting is None:
+            return True
+        if s.admitting is None and self._reserve(s, busy):
+            self.admitter.submit(s)
+        return False
 
     def run(self, arrivals, clock=time.perf_counter, on_round=None):
         """Serves `arrivals` [(time offset in s, session id, steps)], sorted
@@ -247,15 +330,23 @@ class Server:
                 c = s.width * s.height
                 if chosen and cells + c > self.max_round_cells:
                     continue
-                if not self._admit(s, busy | (waiting if s.world is None or s.evicting is not None else set())):
+                protect = busy | (waiting if s.world is None or s.evicting is not None else set())
+                if self.admitter is not None:
+                    if not self._ready(s, protect):
+                        continue
+                elif not self._admit(s, protect):
                     continue
                 chosen.append(r)
                 busy.add(r.sid)
                 cells += c
             if not chosen:
+                if self.admitter is not None and self.admitter.inflight > 0:
+                    self.admitter.wait_any(0.002)    # (worlds on their way in)
+                    continue
                 raise RuntimeError("no waiting request fits the HBM budget")
             sess = [self.sessions[r.sid] for r in chosen]
-            cold = self.admissions != admitted or any(s.steps_run == 0 for s in sess)
+            # (with the threads, admissions are not the round's)
+            cold = (self.admitter is None and self.admissions != admitted) or any(s.steps_run == 0 for s in sess)
             worlds = [s.world for s in sess]
             if self.batch is None:
                 self.batch = GpuBatch(worlds)
@@ -284,6 +375,9 @@ class Server:
         return reqs
 
     def close(self):
+        if self.admitter is not None:
+            self.admitter.close()
+            self.admitter = None
         if self.evictor is not None:
             self.evictor.close()
             self.evictor = None
