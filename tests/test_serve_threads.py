@@ -1,7 +1,8 @@
-"""The serving loop's eviction thread (serve.Evictor), with stand-in worlds
-on the host: every eviction runs once, the bytes in flight stay within the
-limit (one world past it may run alone), waits return once a world's
-eviction is done, and an eviction's failure reaches the loop."""
+"""The serving loop's eviction and admission threads (serve.Evictor,
+serve.Admitter), with stand-in worlds on the host: every eviction runs
+once, the bytes in flight stay within the limit (one world past it may run
+alone), waits return once a world's eviction is done, an eviction's failure
+reaches the loop, and an admission waits for its session's eviction."""
 import threading
 import time
 
@@ -55,3 +56,31 @@ def test_eviction_failure_reaches_the_loop():
         ev.wait(bad)
     with pytest.raises(RuntimeError):
         ev.close()
+
+
+class _Admitted:
+    def __init__(self, sid, log):
+        self.sid, self.world, self.evicting, self.admitting, self.log = sid, None, None, None, log
+
+    def make_resident(self, stream):
+        self.log.append(("in", self.sid, stream))
+        self.world = _World()
+        self.world.set_stream(stream)
+
+
+def test_admission_waits_for_its_eviction():
+    log = []
+    ad = serve.Admitter("copy")
+    s = _Admitted(1, log)
+    s.evicting = threading.Event()          # an eviction of the same session still in flight
+    ad.submit(s)
+    time.sleep(0.02)
+    assert not s.admitting.is_set() and s.world is None
+    log.append(("out", 1, None))
+    s.evicting.set()
+    s.admitting.wait(5)
+    assert s.admitting.is_set() and s.world.stream == "copy"
+    assert [e[0] for e in log] == ["out", "in"]
+    ad.wait_any(0.0)
+    assert ad.inflight == 0
+    ad.close()
