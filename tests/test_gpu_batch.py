@@ -118,3 +118,27 @@ def test_graph_replays_match_eager():
         d = compare_worlds(lone_e, lone_g)
         assert not d, f"lone world step {s}: " + "; ".join(d)
     batch.close()
+
+
+def test_batch_outlives_its_first_world():
+    # a batch made without a stream runs on one of its own: its first world
+    # assigned away and destroyed, it still steps the others exactly
+    from worldinagrain_amd.gpu import GpuBatch
+    spec = worlds_spec()
+    say("[batch] first world destroyed")
+    a = make_gpu(spec[0][1], spec[0][2], spec[0][3])
+    b = make_gpu(spec[4][1], spec[4][2], spec[4][3])
+    sa, sb = make_sim(a, spec[0][1], spec[0][2], "reference"), make_sim(b, spec[4][1], spec[4][2], "config")
+    batch = GpuBatch([a, b])
+    step_batch(batch, [sa, sb])
+    batch.assign([b])
+    a.close()
+    for _ in range(2):
+        step_batch(batch, [sb])
+    alone = make_gpu(spec[4][1], spec[4][2], spec[4][3])
+    s2 = make_sim(alone, spec[4][1], spec[4][2], "config")
+    for _ in range(3):
+        s2.step()
+    d = compare_worlds(b, alone)
+    assert not d, "; ".join(d)
+    batch.close()
