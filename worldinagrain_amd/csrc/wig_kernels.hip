@@ -471,14 +471,27 @@ __global__ void __launch_bounds__(256) k_compactMarks(WorldGrid g) {
     bool was = (d.blocks[STATE_SLOT(d, n)] & ACTIVE) != 0u;
     d.scratch[n] = (liquid ? 1u : 0u) | (!liquid && was ? 2u : 0u) | (nearl ? 4u : 0u);
 }
+// A single-workgroup scan's thread takes a contiguous range of entries; the
+// first RANGE_HELD of them are loaded at once into registers (the loads
+// below the range's end, made together) and kept for the second pass;
+// entries past those are read one by one.
+#define RANGE_HELD 32
 // compact's lists, in block-index order: one workgroup scans the flags
 __global__ void __launch_bounds__(1024) k_compact(WorldGrid g) {
     WORLD(g);
     __shared__ uint32_t sA[1024], sR[1024];
     uint32_t t = threadIdx.x, T = blockDim.x;
-    uint32_t per = (d.B + T - 1u) / T, lo = t * per, hi = lo + per < d.B ? lo + per : d.B;
+    uint32_t per = (d.B + T - 1u) / T, lo = t * per < d.B ? t * per : d.B, hi = lo + per < d.B ? lo + per : d.B;
+    uint32_t held[RANGE_HELD];
+#pragma unroll
+    for (int q = 0; q < RANGE_HELD; q++) held[q] = lo + (uint32_t)q < hi ? d.scratch[lo + (uint32_t)q] : 0u;
     uint32_t na = 0, nr = 0;
-    for (uint32_t n = lo; n < hi; n++) {
+#pragma unroll
+    for (int q = 0; q < RANGE_HELD; q++) {
+        na += held[q] & 1u;
+        nr += (held[q] >> 1) & 1u;
+    }
+    for (uint32_t n = lo + RANGE_HELD; n < hi; n++) {
         uint32_t f = d.scratch[n];
         na += f & 1u;
         nr += (f >> 1) & 1u;
@@ -494,7 +507,15 @@ __global__ void __launch_bounds__(1024) k_compact(WorldGrid g) {
         __syncthreads();
     }
     uint32_t ia = sA[t] - na, ir = sR[t] - nr;
-    for (uint32_t n = lo; n < hi; n++) {
+#pragma unroll
+    for (int q = 0; q < RANGE_HELD; q++) {
+        const uint32_t n = lo + (uint32_t)q, f = held[q];
+        if (n >= hi) break;
+        d.blocks[STATE_SLOT(d, n)] = ((f & 1u) ? ACTIVE : 0u) | ((f & 4u) ? NEAR_LIQUID : 0u);
+        if (f & 1u) d.blocks[ACTIVE_SLOT(d, ia++)] = n;
+        if (f & 2u) d.blocks[RETIRED_SLOT(d, ir++)] = n;
+    }
+    for (uint32_t n = lo + RANGE_HELD; n < hi; n++) {
         uint32_t f = d.scratch[n];
         d.blocks[STATE_SLOT(d, n)] = ((f & 1u) ? ACTIVE : 0u) | ((f & 4u) ? NEAR_LIQUID : 0u);
         if (f & 1u) d.blocks[ACTIVE_SLOT(d, ia++)] = n;
@@ -2152,12 +2173,6 @@ __device__ __forceinline__ void laneTree(double a0, double a1, uint32_t m, doubl
     if (sm) *om = sm[0];
     __syncthreads();
 }
-__device__ void sumPlaces2(const Dev &d, uint32_t w0, uint32_t w1, double *s0, double *s1, double *out0, double *out1) {
-    PlaceHold<2> h;
-    const uint32_t w[2] = {w0, w1};
-    holdPlaces<2>(d, activeCount(d), w, h);
-    laneTree(laneSum<2>(d, h, 0, w0), laneSum<2>(d, h, 1, w1), 0u, s0, s1, nullptr, out0, out1, nullptr);
-}
 #define FOR_ACTIVE_BLOCKS4(d, wid) \
     for (uint32_t wid = bid * WAVES + threadIdx.x / 64u, n_ = activeCount(d); wid < n_; wid += nbid * WAVES)
 #define LANE (threadIdx.x % 64u)
@@ -2226,24 +2241,28 @@ __global__ void __launch_bounds__(1024) k_sumNorm(WorldGrid g) {
     WORLD(g);
     if (!d.full) return;
     __shared__ double s0[SUM_LANES], s1[SUM_LANES];
-    double bb, unused;
-    sumPlaces2(d, 2, 2, s0, s1, &bb, &unused);
-    // and the largest |r0|, for the 16-bit V-cycle's scale
     __shared__ uint32_t sMost[SUM_LANES];
-    uint32_t m = 0, na = activeCount(d);
-    for (uint32_t j = threadIdx.x; j < na; j += SUM_LANES) {
-        uint32_t v = (uint32_t)d.bpart[4ull * j + 3];
-        m = v > m ? v : m;
+    // ||b||^2, and the largest |r0| (a block's as f32 bits), for the 16-bit
+    // V-cycle's scale: one halving, as k_cgCheck's
+    PlaceHold<2> ph;
+    const uint32_t w[2] = {2u, 3u};
+    holdPlaces<2>(d, activeCount(d), w, ph);
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < SUM_HELD; q++) {
+        const uint32_t b = (uint32_t)ph.v[1][q];
+        if (threadIdx.x + (uint32_t)q * SUM_LANES < ph.na) m = b > m ? b : m;
     }
-    sMost[threadIdx.x] = m;
-    __syncthreads();
-    for (uint32_t h = SUM_LANES / 2u; h > 0; h >>= 1) {
-        if (threadIdx.x < h) sMost[threadIdx.x] = max(sMost[threadIdx.x], sMost[threadIdx.x + h]);
-        __syncthreads();
+    for (uint32_t j = threadIdx.x + SUM_HELD * SUM_LANES; j < ph.na; j += SUM_LANES) {
+        uint32_t b = (uint32_t)d.bpart[4ull * j + 3];
+        m = b > m ? b : m;
     }
+    double bb, unused;
+    uint32_t most;
+    laneTree(laneSum<2>(d, ph, 0, 2u), 0.0, m, s0, s1, sMost, &bb, &unused, &most);
     if (threadIdx.x == 0) {
         d.head->bb = bb;
-        d.head->vscale = vcycleScale(d, sMost[0]);
+        d.head->vscale = vcycleScale(d, most);
     }
 }
 // cgPrepare (:2721-2732), and the solve's fine level packed (the arrays'
@@ -2849,8 +2868,13 @@ __global__ void __launch_bounds__(256) k_scanTotals(WorldGrid g) {
     __shared__ uint32_t s[256];
     uint32_t li = threadIdx.x, n = activeCount(d);
     uint32_t per = (n + 255u) / 256u, lo = li * per < n ? li * per : n, hi = lo + per < n ? lo + per : n;
+    uint32_t held[RANGE_HELD];
+#pragma unroll
+    for (int q = 0; q < RANGE_HELD; q++) held[q] = lo + (uint32_t)q < hi ? d.blocks[TOTAL_SLOT(d, lo + (uint32_t)q)] : 0u;
     uint32_t sum = 0;
-    for (uint32_t j = lo; j < hi; j++) sum += d.blocks[TOTAL_SLOT(d, j)];
+#pragma unroll
+    for (int q = 0; q < RANGE_HELD; q++) sum += held[q];
+    for (uint32_t j = lo + RANGE_HELD; j < hi; j++) sum += d.blocks[TOTAL_SLOT(d, j)];
     s[li] = sum;
     __syncthreads();
     for (uint32_t o = 1; o < 256u; o <<= 1) {
@@ -2860,7 +2884,13 @@ __global__ void __launch_bounds__(256) k_scanTotals(WorldGrid g) {
         __syncthreads();
     }
     uint32_t run = s[li] - sum;
-    for (uint32_t j = lo; j < hi; j++) {
+#pragma unroll
+    for (int q = 0; q < RANGE_HELD; q++) {
+        if (lo + (uint32_t)q >= hi) break;
+        d.blocks[TOTAL_SLOT(d, lo + (uint32_t)q)] = run;
+        run += held[q];
+    }
+    for (uint32_t j = lo + RANGE_HELD; j < hi; j++) {
         uint32_t t = d.blocks[TOTAL_SLOT(d, j)];
         d.blocks[TOTAL_SLOT(d, j)] = run;
         run += t;
