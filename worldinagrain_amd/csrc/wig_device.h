@@ -189,7 +189,7 @@ struct WorldGrid {
     const Dev d = (g).tab[wi_];                                       \
     const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
     const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \
-    (void)nbid
+    (void)nbid; (void)bid
 // The level kernels' view: level l of the world as level 1 of the copy and
 // level l + 1 as its level 2 (a copy indexed by a level known only at run
 // time would live in scratch; constant indices keep it in registers), and
@@ -212,7 +212,7 @@ struct WorldGrid {
     const uint32_t tile_ = dg_.tile[l];                               \
     const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
     const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \
-    (void)nbid
+    (void)nbid; (void)bid
 // (the Dev in memory, for kernels that index its per-level arrays by a
 // level known only at run time throughout)
 #define WORLD_REF(g)                                                  \
@@ -220,7 +220,7 @@ struct WorldGrid {
     const Dev &d = (g).tab[wi_];                                      \
     const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
     const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \
-    (void)nbid
+    (void)nbid; (void)bid
 
 struct i2 { int32_t x, y; };
 
