@@ -116,6 +116,24 @@ def test_still_tank_gpu():
     assert np.all(np.abs(np.diff(col) - float(E.GRAVITY)) < 0.05 * float(E.GRAVITY))
 
 
+@pytest.mark.parametrize("W,H,block,mode", [
+    (8, 8, (5, 6), "reference"),        # one block, one level
+    (5, 3, (4, 3), "reference"),        # smaller than a block
+    (200, 9, (120, 8), "config"),       # a strip one block and a row high
+    (9, 130, (8, 100), "config"),       # a column
+])
+def test_small_and_thin_worlds(W, H, block, mode):
+    run_pair(f"small {W}x{H}", W, H, scenes.dam_break(W, H, block=block), 6, mode=mode, check_every=2)
+
+
+def test_full_size_discs_world():
+    # the benchmark's world at its full size (1024 x 1024, 200 discs, 1.47 M
+    # particles, MG-PCG to 1e-6): the oracle takes about 1.5 s a step
+    W = H = 1024
+    cells, _ = scenes.discs_dam(W, H)
+    run_pair("full size discs", W, H, cells, 2, mode="config", check_every=1)
+
+
 def test_idle_world():
     W = H = 32
     cells = scenes.empty_cells(W, H)
