@@ -43,21 +43,29 @@ def build_world(size, seed=2):
     return cells, fluid_cells, E.kinds()
 
 
-def config_simulation(world, size, tol):
+def config_simulation(world, size, tol, storage=0):
     import numpy as np
     from worldinagrain_amd.host import Simulation, SolverSettings
     g = float(np.float32(9.81 * (1.0 / 120.0) ** 2 / 1.0))
-    return Simulation(world, size, size, dx=1.0, earth_g=g, dt=1.0 / 120.0, solver=SolverSettings.relative(tol))
+    return Simulation(world, size, size, dx=1.0, earth_g=g, dt=1.0 / 120.0,
+                      solver=SolverSettings.relative(tol, storage=storage))
 
 
-def step_bytes(totals_delta):
+# The V-cycle's storage (--storage), and the byte model's bytes a cell a PCG
+# iteration for it: 9 + 24 + 16 + (4/3)(7.5 s_v + 3), s_v the bytes a stored
+# V-cycle value (4 float, 2 half or bfloat16)
+STORAGES = {"fp32": 0, "fp16": 1, "bf16": 2}
+B_ITER = {0: 93.0, 1: 73.0, 2: 73.0}
+
+
+def step_bytes(totals_delta, b_iter=93.0):
     """The per-step algorithmic byte model summed over the timed steps:
-    N*224 + C_a*65 + C_a*16 + K*C_a*93 (FP32 V-cycle storage)."""
+    N*224 + C_a*65 + C_a*16 + K*C_a*B_iter (93 FP32, 73 16-bit V-cycle storage)."""
     st, cyc, parts, ca = totals_delta
     if st == 0:
         return 0.0
     ca_mean = ca / st
-    return parts * 224.0 + ca * (65.0 + 16.0) + cyc * ca_mean * 93.0
+    return parts * 224.0 + ca * (65.0 + 16.0) + cyc * ca_mean * b_iter
 
 
 def cpu_baseline(cells, kinds, size, tol, steps, solver=None):
@@ -96,7 +104,9 @@ def main():
                     help="serve: most cells a round steps (4 M keeps warm rounds with 512x512 worlds under 16 ms at p99)")
     ap.add_argument("--seeds", type=int, default=2, help="precision: worlds (seeds) per variant")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (single: 100, batch: 50)")
+    ap.add_argument("--storage", choices=tuple(STORAGES), default=None,
+                    help="V-cycle storage (single: fp32, the plain configuration; batch and serve: fp16, mixed precision)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -111,6 +121,10 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 50 if args.workload == "batch" else 100
+    if args.storage is None:
+        args.storage = "fp16" if args.workload in ("batch", "serve") else "fp32"
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -146,7 +160,8 @@ def main():
     stream = torch.cuda.Stream() if args.graphs else torch.cuda.current_stream()
     world.set_stream(stream.cuda_stream)
     world.set_graphs(bool(args.graphs))
-    sim = config_simulation(world, size, args.tol)
+    storage = STORAGES[args.storage]
+    sim = config_simulation(world, size, args.tol, storage)
     for _ in range(max(args.warmup, 1)):
         sim.step()
     # the coarse-level tile sides, by the autotuner on the world's own steps
@@ -204,15 +219,16 @@ def main():
     particles_per_step = d[2] / K
     value = d[2] / (total_ms / 1e3)
     solves = K / (total_ms / 1e3)
-    bytes_total = step_bytes(d)
+    bytes_total = step_bytes(d, B_ITER[storage])
     ca_mean = d[3] / K
 
     roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": args.kernel,
             "launches": klaunches, "mean_launch_ms": kms / max(klaunches, 1)}
     if args.kernel == "solve":
-        # B_solve = C_a*16 + K*C_a*93 (FP32 V-cycle storage), per solve
-        per_launch = (d[3] * 16.0 + d[1] * ca_mean * 93.0) / K
-        roof["unit_of_work"] = "one MG-PCG solve (cgStart..cgFinish), B_solve = C_a*16 + K*C_a*93 bytes"
+        # B_solve = C_a*16 + K*C_a*B_iter, per solve
+        per_launch = (d[3] * 16.0 + d[1] * ca_mean * B_ITER[storage]) / K
+        roof["unit_of_work"] = (f"one MG-PCG solve (cgStart..cgFinish), B_solve = C_a*16 + K*C_a*{B_ITER[storage]:g} bytes "
+                                f"({args.storage} V-cycle storage)")
         roof["C_a"] = ca_mean
         roof["pcg_iterations"] = d[1] / K
     elif args.kernel in KERNEL_BYTES_PER_PARTICLE:
@@ -228,11 +244,13 @@ def main():
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
         "warmup": args.warmup, "ms_per_step": total_ms / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"plain FP32 at scale: {size}x{size} MAC grid, ~10% solid discs, water in the lowest "
+        "config": {"workload": f"{'plain FP32' if storage == 0 else args.storage + ' V-cycle storage'} at scale: "
+                               f"{size}x{size} MAC grid, ~10% solid discs, water in the lowest "
                                f"40% of rows at 4 particles/cell, dt 1/120, MG-PCG {1 + world.layout.levels} levels "
                                f"to {args.tol:g} relative residual",
                    "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
-                   "tol": args.tol, "parallelism": "single world, one GPU", "graphs": bool(args.graphs),
+                   "tol": args.tol, "vcycle_storage": args.storage, "parallelism": "single world, one GPU",
+                   "graphs": bool(args.graphs),
                    "tiles": {str(k): v for k, v in tiles.items()},
                    "sweeps": sweeps, "lds_level": lds_level, "coarse_wave_cells": coarse_wave},
         "solves_per_s": solves, "pcg_iterations_per_solve": d[1] / K,
@@ -253,7 +271,8 @@ def run_batch(args):
     """Many worlds stepped together: sides drawn from {64, 128, 256, 512}
     with probabilities {.4, .3, .2, .1}; each a dam block over 20-50% of its
     cells and 0-15% solid discs (scenes.batch_world), log2(side) - 2 MG
-    levels, relative residual `tol`.  Tile sides and sweep counts per side
+    levels, relative residual `tol`, the V-cycle in `storage` (16-bit by
+    default: the mixed-precision V-cycle).  Tile sides and sweep counts per side
     class come from the autotuner (on copies of a world of the class).  The
     same worlds stepped one at a time (a sample of each class) give the
     sequential rate beside it."""
@@ -273,7 +292,7 @@ def run_batch(args):
         w = GpuWorld(side, side, coarse_levels=int(math.log2(side)) - 3, kinds=kinds)
         w.upload(abi.PART_CELLS, cells)
         worlds.append(w)
-        sims.append(config_simulation(w, side, args.tol))
+        sims.append(config_simulation(w, side, args.tol, STORAGES[args.storage]))
     stream = torch.cuda.Stream() if args.graphs else torch.cuda.current_stream()
     batch = GpuBatch(worlds)
     batch.set_stream(stream.cuda_stream)
@@ -347,7 +366,7 @@ def run_batch(args):
         st = b.steps - a.steps
         if st:
             ca = (b.active_cells - a.active_cells) / st
-            bsolve += (b.active_cells - a.active_cells) * 16.0 + (b.cycles - a.cycles) * ca * 93.0
+            bsolve += (b.active_cells - a.active_cells) * 16.0 + (b.cycles - a.cycles) * ca * B_ITER[STORAGES[args.storage]]
     solve_achieved = bsolve / solve_n / (solve_ms / solve_n / 1e3) / 1e9 if solve_n else None
     parts = sum(b.particles - a.particles for a, b in zip(t0, t1))
     cycles = sum(b.cycles - a.cycles for a, b in zip(t0, t1))
@@ -378,7 +397,7 @@ def run_batch(args):
         "config": {"workload": f"{len(sides)} worlds, sides {{64,128,256,512}} w.p. {{.4,.3,.2,.1}}, dam blocks over "
                                f"20-50% of cells, 0-15% solid discs, log2(side)-2 levels, {args.tol:g} relative residual",
                    "worlds": {str(s): sides.count(s) for s in sorted(set(sides))}, "cells": int(sum(s * s for s in sides)),
-                   "graphs": bool(args.graphs),
+                   "vcycle_storage": args.storage, "graphs": bool(args.graphs),
                    "tuned": {str(k): v for k, v in tuned.items()}},
         "sequential": {"ms_per_round": seq_ms_per_round, "solves_per_s": len(sides) / (seq_ms_per_round / 1e3),
                        "ms_per_world_step": {str(k): v for k, v in seq.items()},
@@ -387,7 +406,8 @@ def run_batch(args):
     out["speedup_vs_sequential"] = out["value"] / out["sequential"]["solves_per_s"]
     out["roofline"] = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "kernel": "solve",
                        "unit_of_work": "one batch step's MG-PCG solves (cgStart..cgFinish over all worlds), "
-                                       "sum of each world's C_a*16 + K*C_a*93 bytes",
+                                       f"sum of each world's C_a*16 + K*C_a*{B_ITER[STORAGES[args.storage]]:g} bytes "
+                                       f"({args.storage} V-cycle storage)",
                        "launches": solve_n, "mean_launch_ms": solve_ms / max(solve_n, 1),
                        "algorithmic_bytes_per_launch": bsolve / max(solve_n, 1),
                        "achieved": solve_achieved,
@@ -506,7 +526,8 @@ def run_serve(args):
     [64, 1024] rounded to a multiple of 32, each a dam block with solid discs
     (scenes.batch_world; the reference seeds 4 particles a water cell);
     requests of 1-8 steps of a uniformly chosen session, Poisson arrivals at
-    `rate` for `duration` seconds; continuous batching under an HBM budget of
+    `rate` for `duration` seconds, the V-cycle in `storage` (16-bit by
+    default); continuous batching under an HBM budget of
     `budget_gb` with LRU eviction of idle worlds to host memory
     (worldinagrain_amd.serve).  Beside it, the plain single-world loop: the
     served world-steps timed one world at a time on sample worlds of each
@@ -528,7 +549,9 @@ def run_serve(args):
     g = float(np.float32(9.81 * (1.0 / 120.0) ** 2))
     kw = dict(dx=1.0, earth_g=g, dt=1.0 / 120.0)
     tol = args.tol
-    sessions = [Session(i, s, s, scene[s], lambda: SolverSettings.relative(tol), kinds, sim_kw=kw) for i, s in enumerate(sides)]
+    st = STORAGES[args.storage]
+    sessions = [Session(i, s, s, scene[s], lambda: SolverSettings.relative(tol, storage=st), kinds, sim_kw=kw)
+                for i, s in enumerate(sides)]
     t, arrivals = 0.0, []
     while True:
         t += rng.exponential(1.0 / args.rate)
@@ -592,7 +615,7 @@ def run_serve(args):
         "evictions": server.evictions, "admissions": server.admissions,
         "config": {"workload": f"{args.sessions} sessions, sides log-uniform on [64,1024] (multiples of 32), requests of "
                                f"1-8 steps, Poisson arrivals at {args.rate:g}/s for {args.duration:g} s, HBM budget "
-                               f"{args.budget_gb:g} GB, LRU eviction to host, {args.tol:g} relative residual, FP32 V-cycle",
+                               f"{args.budget_gb:g} GB, LRU eviction to host, {args.tol:g} relative residual, {args.storage} V-cycle storage",
                    "round_cells": args.round_cells},
         "single_world_loop": {"solo_ms_per_step": {str(k): v for k, v in solo_ms.items()},
                               "particle_steps_per_s": server.particle_steps / seq_s,
