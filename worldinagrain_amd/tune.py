@@ -13,7 +13,7 @@ import dataclasses
 from .gpu import WigError
 
 TILE_SIDES = (8, 16, 32)
-SWEEP_CANDIDATES = [(sp, lp, cs) for sp in (1, 2) for lp in (1, 2) for cs in (1, 2, 5)]
+SWEEP_CANDIDATES = [(sp, lp, cs) for sp in (1, 2, 3) for lp in (1, 2, 3) for cs in (1, 2, 5)]
 
 
 def _kernel_ms_per_iteration(world, sim, name, steps):
