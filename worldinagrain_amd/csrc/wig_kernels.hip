@@ -471,59 +471,65 @@ __global__ void __launch_bounds__(256) k_compactMarks(WorldGrid g) {
     bool was = (d.blocks[STATE_SLOT(d, n)] & ACTIVE) != 0u;
     d.scratch[n] = (liquid ? 1u : 0u) | (!liquid && was ? 2u : 0u) | (nearl ? 4u : 0u);
 }
-// A single-workgroup scan's thread takes a contiguous range of entries; the
-// first RANGE_HELD of them are loaded at once into registers (the loads
-// below the range's end, made together) and kept for the second pass;
-// entries past those are read one by one.
-#define RANGE_HELD 32
-// compact's lists, in block-index order: one workgroup scans the flags
+// compact's lists, in block-index order, by one workgroup of 1024: a chunk
+// of 16384 blocks at a time, each wave taking 1024 consecutive blocks in 16
+// rows of 64 (lane l of row q takes block 64q + l of its wave's), so that
+// the flags' loads, the block states' stores and the lists' stores are all
+// coalesced.  A block's place in a list is the blocks listed before its row
+// (earlier chunks, earlier waves, earlier rows) plus those before it in its
+// row (a ballot's bits below its lane).
+#define COMPACT_ROWS 16
+#define COMPACT_CHUNK (COMPACT_ROWS * 1024u)
 __global__ void __launch_bounds__(1024) k_compact(WorldGrid g) {
     WORLD(g);
-    __shared__ uint32_t sA[1024], sR[1024];
-    uint32_t t = threadIdx.x, T = blockDim.x;
-    uint32_t per = (d.B + T - 1u) / T, lo = t * per < d.B ? t * per : d.B, hi = lo + per < d.B ? lo + per : d.B;
-    uint32_t held[RANGE_HELD];
+    __shared__ uint32_t sA[16], sR[16];
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t baseA = 0, baseR = 0;
+    for (uint32_t c0 = 0; c0 < d.B; c0 += COMPACT_CHUNK) {
+        const uint32_t w0 = c0 + wave * (COMPACT_ROWS * 64u);
+        uint32_t f[COMPACT_ROWS];
 #pragma unroll
-    for (int q = 0; q < RANGE_HELD; q++) held[q] = lo + (uint32_t)q < hi ? d.scratch[lo + (uint32_t)q] : 0u;
-    uint32_t na = 0, nr = 0;
+        for (int q = 0; q < COMPACT_ROWS; q++) {
+            const uint32_t n = w0 + 64u * (uint32_t)q + lane;
+            f[q] = n < d.B ? d.scratch[n] : 0u;
+        }
+        uint32_t na = 0, nr = 0;
 #pragma unroll
-    for (int q = 0; q < RANGE_HELD; q++) {
-        na += held[q] & 1u;
-        nr += (held[q] >> 1) & 1u;
-    }
-    for (uint32_t n = lo + RANGE_HELD; n < hi; n++) {
-        uint32_t f = d.scratch[n];
-        na += f & 1u;
-        nr += (f >> 1) & 1u;
-    }
-    sA[t] = na;
-    sR[t] = nr;
-    __syncthreads();
-    for (uint32_t o = 1; o < T; o <<= 1) {
-        uint32_t a = t >= o ? sA[t - o] : 0u, r = t >= o ? sR[t - o] : 0u;
+        for (int q = 0; q < COMPACT_ROWS; q++) {
+            const uint32_t n = w0 + 64u * (uint32_t)q + lane;
+            if (n < d.B) d.blocks[STATE_SLOT(d, n)] = ((f[q] & 1u) ? ACTIVE : 0u) | ((f[q] & 4u) ? NEAR_LIQUID : 0u);
+            na += (uint32_t)__popcll(__ballot((f[q] & 1u) != 0u));
+            nr += (uint32_t)__popcll(__ballot((f[q] & 2u) != 0u));
+        }
+        if (lane == 0u) {
+            sA[wave] = na;
+            sR[wave] = nr;
+        }
         __syncthreads();
-        sA[t] += a;
-        sR[t] += r;
-        __syncthreads();
-    }
-    uint32_t ia = sA[t] - na, ir = sR[t] - nr;
+        uint32_t ia = baseA, ir = baseR;
+        for (uint32_t w = 0; w < wave; w++) {
+            ia += sA[w];
+            ir += sR[w];
+        }
+        for (uint32_t w = 0; w < 16u; w++) {
+            baseA += sA[w];
+            baseR += sR[w];
+        }
 #pragma unroll
-    for (int q = 0; q < RANGE_HELD; q++) {
-        const uint32_t n = lo + (uint32_t)q, f = held[q];
-        if (n >= hi) break;
-        d.blocks[STATE_SLOT(d, n)] = ((f & 1u) ? ACTIVE : 0u) | ((f & 4u) ? NEAR_LIQUID : 0u);
-        if (f & 1u) d.blocks[ACTIVE_SLOT(d, ia++)] = n;
-        if (f & 2u) d.blocks[RETIRED_SLOT(d, ir++)] = n;
+        for (int q = 0; q < COMPACT_ROWS; q++) {
+            const uint32_t n = w0 + 64u * (uint32_t)q + lane;
+            const uint64_t ma = __ballot((f[q] & 1u) != 0u), mr = __ballot((f[q] & 2u) != 0u);
+            if (f[q] & 1u) d.blocks[ACTIVE_SLOT(d, ia + (uint32_t)__popcll(ma & below))] = n;
+            if (f[q] & 2u) d.blocks[RETIRED_SLOT(d, ir + (uint32_t)__popcll(mr & below))] = n;
+            ia += (uint32_t)__popcll(ma);
+            ir += (uint32_t)__popcll(mr);
+        }
+        __syncthreads();   // every wave has read sA, sR before the next chunk's
     }
-    for (uint32_t n = lo + RANGE_HELD; n < hi; n++) {
-        uint32_t f = d.scratch[n];
-        d.blocks[STATE_SLOT(d, n)] = ((f & 1u) ? ACTIVE : 0u) | ((f & 4u) ? NEAR_LIQUID : 0u);
-        if (f & 1u) d.blocks[ACTIVE_SLOT(d, ia++)] = n;
-        if (f & 2u) d.blocks[RETIRED_SLOT(d, ir++)] = n;
-    }
-    if (t == T - 1u) {
-        d.blocks[COUNT_SLOT(d, COUNT_ACTIVE)] = sA[t];
-        d.blocks[COUNT_SLOT(d, COUNT_RETIRED)] = sR[t];
+    if (threadIdx.x == 0u) {
+        d.blocks[COUNT_SLOT(d, COUNT_ACTIVE)] = baseA;
+        d.blocks[COUNT_SLOT(d, COUNT_RETIRED)] = baseR;
     }
 }
 
@@ -2861,6 +2867,11 @@ __global__ void __launch_bounds__(64) k_scanBlocks(WorldGrid g) {
         if (threadIdx.x == 63) d.blocks[TOTAL_SLOT(d, wid)] = incl;
     }
 }
+// A single-workgroup scan's thread takes a contiguous range of entries; the
+// first RANGE_HELD of them are loaded at once into registers (the loads
+// below the range's end, made together) and kept for the second pass;
+// entries past those are read one by one.
+#define RANGE_HELD 32
 // scanTotals (:3757-3780): the blocks' runs' starts, in the active list's order
 __global__ void __launch_bounds__(256) k_scanTotals(WorldGrid g) {
     WORLD(g);
