@@ -267,6 +267,64 @@ int wig_batch_set_graphs(WigBatch *b, int on);
 int wig_batch_time_kernel(WigBatch *b, int id);
 int wig_batch_kernel_time(WigBatch *b, double *ms, uint32_t *launches);
 
+/* ------------------------------------------------------------------
+ * The air's projection: the MG-PCG that solves for the air's dynamic
+ * pressure each step before the liquid step (shaders/air.wgsl:869-1536),
+ * on the reference's buffers.  An air grid of width x height air cells
+ * (AIR_CELL 1: the world's cells) with its levels: halved, rounding up,
+ * until a level holds 256 cells or fewer, 9 levels at most
+ * (js/sim.js:712-719); a grid whose 9th level would hold more is refused.
+ * Replaces the projection's share of `new Simulation(...)`'s air
+ * allocation (js/sim.js:705-723,858-864). */
+typedef struct WigAir WigAir;
+/* Parts (the reference's buffers):
+ *   MG      airProjection: every level, two float4s a cell, level l at
+ *           level_base[l] float4s (air.wgsl:84-90,172-192);
+ *   CG      airCg: a float4 a level-0 cell (x, r, p, A p), then the sums,
+ *           four to a float4: r.z, alpha, beta, a spare, a partial per
+ *           8 x 8 group (air.wgsl:947-961);
+ *   STATUS  airFastest's 8 words (air.wgsl:96-104): [1..3] the
+ *           iterations' most |r| in turn, [4] the iterations run;
+ *   MEAN    a float per level-0 cell: meanExpansion of its zone
+ *           (air.wgsl:1006-1021), which cgStart adds to the divergence of
+ *           cells with gas; the zones are the caller's (none: zeros). */
+#define WIG_AIR_PART_MG 0
+#define WIG_AIR_PART_CG 1
+#define WIG_AIR_PART_STATUS 2
+#define WIG_AIR_PART_MEAN 3
+#define WIG_AIR_PART_COUNT 4
+typedef struct WigAirLayout {
+    uint32_t width, height, levels, groups;
+    uint32_t level_width[9], level_height[9], level_base[9];
+    uint64_t part_bytes[WIG_AIR_PART_COUNT];
+} WigAirLayout;
+WigAir *wig_air_create(uint32_t width, uint32_t height);
+int wig_air_destroy(WigAir *a);
+int wig_air_layout(const WigAir *a, WigAirLayout *out);
+/* Copies a part to or from the device, whole.  Synchronous. */
+int wig_air_upload(WigAir *a, int part, const void *src, size_t bytes);
+int wig_air_download(WigAir *a, int part, void *dst, size_t bytes);
+int wig_air_device_ptr(WigAir *a, int part, void **ptr);
+/* Runs later calls on this HIP stream (NULL: the grid's own). */
+int wig_air_set_stream(WigAir *a, void *stream);
+/* The side of the level passes' LDS tiles, 16 (default) or 32: the results
+ * are the same bits either way.  This library's own. */
+int wig_air_set_tile(WigAir *a, uint32_t side);
+/* The projection's solve, enqueued as encodeAir dispatches it
+ * (js/sim.js:2264-2289): gather's clearing of the solve's status slots
+ * (air.wgsl:444-449), cgStart, then `cycles` iterations (the host's
+ * airCycles, js/sim.js:36,1919), each a V-cycle, cgRho, cgBeta,
+ * cgDirection, cgApply, cgAlpha, cgUpdate; an iteration after the most |r|
+ * left falls under AIR_TOLERANCE (1e-4) changes nothing but the slots.
+ * Level 0 of MG (gather's and divergence's, air.wgsl:352-414,825-866) and
+ * CG's potentials (last step's) are the input. */
+int wig_air_project(WigAir *a, uint32_t cycles);
+int wig_air_sync(WigAir *a);
+/* With `on`, each wig_air_project is timed with HIP events on its stream
+ * (and waits for its end); wig_air_elapsed has the sum and the count. */
+int wig_air_time(WigAir *a, int on);
+int wig_air_elapsed(WigAir *a, double *ms, uint32_t *calls);
+
 /* The last error on this thread, "" if none. */
 const char *wig_last_error(void);
 

@@ -1,0 +1,679 @@
+// The air's projection on gfx950: the MG-PCG that solves for the air's
+// dynamic pressure (shaders/air.wgsl:869-1536), in the order
+// Simulation.encodeAir dispatches it (js/sim.js:2264-2289), on the
+// reference's buffers: airProjection (mg: every level, two float4s a
+// cell), airCg (cg: a float4 a level-0 cell, then the sums), airFastest
+// (status words).
+//
+// The V-cycle's level passes sweep tiles in LDS: a T x T square of a
+// level's cells with HALO = 5 more each side (two red-black pairs leave
+// each sweep one more ring in from the edge stale, and the restriction
+// reads one ring beyond the square), so that the square comes out as whole
+// sweeps of the level would leave it.  The sums are the reference's fixed
+// trees (a group of 64 level-0 cells, halving; 256 strided lanes, halving),
+// their last steps as wave shuffles, so every run gives the same bits.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/wig.h"
+
+#define AIR_LEVELS 9        // js/sim.js:28
+#define AIR_COARSEST 256    // js/shared.js:239
+#define ANCHOR 1e-6f        // air.wgsl:146
+#define COARSE_SWEEPS 20    // air.wgsl:153
+#define COARSE_OMEGA 1.7f   // air.wgsl:154
+#define COARSEN 1.8f        // air.wgsl:889
+#define CORRECT 1.5f        // air.wgsl:901
+#define AIR_TOLERANCE 1e-4f // air.wgsl:945
+#define AIR_SMOOTH 2        // air.wgsl:1214
+#define HALO (2 * AIR_SMOOTH + 1)
+
+struct AirDev {
+    float4 *mg;
+    float4 *cg;
+    uint32_t *status;
+    const float *mean;
+    uint32_t L, W, H, gx, groups;
+    uint32_t w[AIR_LEVELS], h[AIR_LEVELS];
+    uint32_t base[AIR_LEVELS];   // levelBase, in float4s
+};
+
+__device__ __forceinline__ uint32_t mgAt(const AirDev &a, uint32_t l, int32_t x, int32_t y) {
+    return a.base[l] + 2u * ((uint32_t)y * a.w[l] + (uint32_t)x);
+}
+__device__ __forceinline__ float *sums(const AirDev &a) { return (float *)(a.cg + (uint64_t)a.W * a.H); }
+__device__ __forceinline__ bool airConverged(const AirDev &a, uint32_t cycle) {   // :963-965
+    return __uint_as_float(a.status[1u + (cycle + 2u) % 3u]) < AIR_TOLERANCE;
+}
+__device__ __forceinline__ float anchorAt(const AirDev &a, uint32_t l, int32_t cx, int32_t cy) {   // :912-916
+    const int32_t lox = cx << l, loy = cy << l;
+    const int32_t hx = min(lox + (1 << l), (int32_t)a.W), hy = min(loy + (1 << l), (int32_t)a.H);
+    return ANCHOR * (float)((hx - lox) * (hy - loy)) / (l >= 1u ? COARSEN : 1.0f);
+}
+__device__ __forceinline__ uint32_t waveMaxU32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+// the 64-value tree (sumGroup, :1054-1069): lane i < s takes lane i + s's
+__device__ __forceinline__ float groupTree(float v) {
+    for (int s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
+    return v;
+}
+
+// gather's clearing of the solve's slots (:444-449)
+__global__ void k_airClear(AirDev a) {
+    if (threadIdx.x == 0) {
+        a.status[1] = 0u;
+        a.status[3] = 0u;
+        a.status[4] = 0u;
+    }
+}
+
+// applyA (:969-993) on cg's component `which`
+template <int WHICH>
+__device__ __forceinline__ float applyA(const AirDev &a, int32_t x, int32_t y, float4 m0, float4 m1) {
+    const uint32_t i = (uint32_t)y * a.W + (uint32_t)x;
+    const float *c = (const float *)a.cg;
+    float diag = m0.z + m0.w + m1.x + anchorAt(a, 0u, x, y);
+    float sum = 0.0f;
+    if ((uint32_t)x + 1u < a.W) sum += m0.z * c[4u * (i + 1u) + WHICH];
+    if ((uint32_t)y + 1u < a.H) sum += m0.w * c[4u * (i + a.W) + WHICH];
+    if (x > 0) {
+        const float w = a.mg[mgAt(a, 0u, x - 1, y)].z;
+        sum += w * c[4u * (i - 1u) + WHICH];
+        diag += w;
+    }
+    if (y > 0) {
+        const float w = a.mg[mgAt(a, 0u, x, y - 1)].w;
+        sum += w * c[4u * (i - a.W) + WHICH];
+        diag += w;
+    }
+    return diag * c[4u * i + WHICH] - sum;
+}
+
+// cgStart (:1027-1048)
+__global__ void __launch_bounds__(256) k_airStart(AirDev a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    uint32_t most = 0u;
+    if (i < a.W * a.H) {
+        const int32_t x = (int32_t)(i % a.W), y = (int32_t)(i / a.W);
+        const uint32_t at = mgAt(a, 0u, x, y);
+        float4 m0 = a.mg[at];
+        const float4 m1 = a.mg[at + 1u];
+        float r = 0.0f;
+        if (!(m1.y > 0.0f)) {
+            if (m1.w > 0.0f) m0.y += a.mean[i];
+            r = -m0.y - applyA<0>(a, x, y, m0, m1);
+        }
+        // (a pass writes no potential, so the neighbours' above are as they were)
+        const float px = a.cg[i].x;
+        a.cg[i] = make_float4(px, r, 0.0f, 0.0f);
+        a.mg[at].x = 0.0f;
+        a.mg[at].y = -r;
+        most = __float_as_uint(fabsf(r));
+    }
+    most = waveMaxU32(most);
+    if ((threadIdx.x & 63u) == 0u && most) atomicMax(&a.status[3], most);
+}
+
+// The V-cycle's level pass on level l (down: smoothDown, :1428-1459; up:
+// :1461-1474), a workgroup a T x T square of the level.
+template <int T>
+struct AirTile {
+    static constexpr int SPAN = T + 2 * HALO, CELLS = SPAN * SPAN;
+    float x[CELLS], rhs[CELLS], right[CELLS], down[CELLS], diag[CELLS];
+};
+
+template <int T, bool UP>
+__device__ __forceinline__ void loadTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t cx0, int32_t cy0) {   // :1255-1291
+    constexpr int SPAN = AirTile<T>::SPAN, CELLS = AirTile<T>::CELLS;
+    const int32_t dx = (int32_t)a.w[l], dy = (int32_t)a.h[l];
+    for (int k = (int)threadIdx.x; k < CELLS; k += 256) {
+        const int32_t x = cx0 + k % SPAN, y = cy0 + k / SPAN;
+        float vx = 0.0f, rhs = 0.0f, right = 0.0f, down = 0.0f, diag = -1.0f;
+        if (x >= 0 && y >= 0 && x < dx && y < dy) {
+            const uint32_t at = mgAt(a, l, x, y);
+            const float4 m0 = a.mg[at], m1 = a.mg[at + 1u];
+            rhs = m0.y;
+            right = m0.z;
+            down = m0.w;
+            if (UP) vx = l == 0u ? a.cg[(uint32_t)y * a.W + (uint32_t)x].w : m1.z;
+            if (!(m1.y > 0.0f)) {
+                if (UP) vx += CORRECT * a.mg[mgAt(a, l + 1u, x / 2, y / 2)].x;
+                diag = m0.z + m0.w + m1.x + anchorAt(a, l, x, y);
+            }
+        }
+        t.x[k] = vx;
+        t.rhs[k] = rhs;
+        t.right[k] = right;
+        t.down[k] = down;
+        t.diag[k] = diag;
+    }
+    __syncthreads();
+    // the left and upper faces' weights into the diagonal, from the tile
+    // (the outermost ring's are never read: it is not swept)
+    for (int k = (int)threadIdx.x; k < CELLS; k += 256) {
+        const int tx = k % SPAN, ty = k / SPAN;
+        if (tx == 0 || ty == 0 || !(t.diag[k] >= 0.0f)) continue;
+        const int32_t x = cx0 + tx, y = cy0 + ty;
+        float d = t.diag[k];
+        if (x > 0) d += t.right[k - 1];
+        if (y > 0) d += t.down[k - SPAN];
+        t.diag[k] = d;
+    }
+    __syncthreads();
+}
+
+// tileSum (:1293-1309)
+template <int T>
+__device__ __forceinline__ float tileSum(const AirTile<T> &t, int k, int32_t x, int32_t y, int32_t dx, int32_t dy) {
+    constexpr int SPAN = AirTile<T>::SPAN;
+    float sum = 0.0f;
+    if (x + 1 < dx) sum += t.right[k] * t.x[k + 1];
+    if (y + 1 < dy) sum += t.down[k] * t.x[k + SPAN];
+    if (x > 0) sum += t.right[k - 1] * t.x[k - 1];
+    if (y > 0) sum += t.down[k - SPAN] * t.x[k - SPAN];
+    return sum;
+}
+
+// sweepTile (:1313-1326): the cells of one colour but the outermost ring
+template <int T>
+__device__ __forceinline__ void sweepTile(AirTile<T> &t, int color, int32_t cx0, int32_t cy0, int32_t dx, int32_t dy) {
+    constexpr int SPAN = AirTile<T>::SPAN, INNER = SPAN - 2;
+    // the inner (SPAN - 2)^2 cells, half of them this colour: thread j takes
+    // the j-th of the colour in row order
+    for (int j = (int)threadIdx.x; j < (INNER * INNER + 1) / 2; j += 256) {
+        const int q = 2 * j, row = q / INNER;
+        int col = q % INNER;
+        const int32_t y = cy0 + 1 + row;
+        // the colour's cells of a row alternate; shift by one where the row's
+        // first cell is the other colour
+        if (((cx0 + 1 + y) & 1) != color) col += 1;
+        if (col >= INNER) continue;
+        const int32_t x = cx0 + 1 + col;
+        const int k = (row + 1) * SPAN + col + 1;
+        if (!(t.diag[k] >= 0.0f)) continue;
+        t.x[k] = (tileSum(t, k, x, y, dx, dy) - t.rhs[k]) / t.diag[k];
+    }
+    __syncthreads();
+}
+
+// restrictCell (:1349-1417): level l + 1's cell (cx, cy) from its four,
+// the fine cells' state from the tile
+template <int T>
+__device__ void restrictCell(const AirDev &a, const AirTile<T> &t, uint32_t l, int32_t cx0, int32_t cy0, int32_t cx, int32_t cy) {
+    constexpr int SPAN = AirTile<T>::SPAN;
+    const int32_t fdx = (int32_t)a.w[l], fdy = (int32_t)a.h[l];
+    auto kOf = [&](int32_t x, int32_t y) { return (y - cy0) * SPAN + (x - cx0); };
+    auto held = [&](int32_t x, int32_t y) { return t.diag[kOf(x, y)] < 0.0f; };
+    float rhs = 0.0f, edge = 0.0f;
+    uint32_t nfree = 0, nheld = 0;
+    for (int q = 0; q < 4; q++) {
+        const int32_t fx = 2 * cx + (q & 1), fy = 2 * cy + (q >> 1);
+        if (fx >= fdx || fy >= fdy) continue;
+        const int k = kOf(fx, fy);
+        if (t.diag[k] < 0.0f) {
+            nheld++;
+            continue;
+        }
+        nfree++;
+        edge += a.mg[mgAt(a, l, fx, fy) + 1u].x;
+        if (fx + 1 < fdx && held(fx + 1, fy)) edge += t.right[k];
+        if (fy + 1 < fdy && held(fx, fy + 1)) edge += t.down[k];
+        if (fx > 0 && held(fx - 1, fy)) edge += t.right[k - 1];
+        if (fy > 0 && held(fx, fy - 1)) edge += t.down[k - SPAN];
+        rhs += t.rhs[k] - (tileSum(t, k, fx, fy, fdx, fdy) - t.diag[k] * t.x[k]);
+    }
+    const int32_t right = min(2 * cx + 1, fdx - 1), bottom = min(2 * cy + 1, fdy - 1);
+    float wR = 0.0f, wD = 0.0f;
+    for (int r = 0; r < 2; r++) {
+        const int32_t y = 2 * cy + r;
+        if (y < fdy && !held(right, y) && (right + 1 >= fdx || !held(right + 1, y))) wR += t.right[kOf(right, y)];
+        const int32_t x = 2 * cx + r;
+        if (x < fdx && !held(x, bottom) && (bottom + 1 >= fdy || !held(x, bottom + 1))) wD += t.down[kOf(x, bottom)];
+    }
+    const bool fixed = nfree == 0u && nheld > 0u;
+    const uint32_t lc = l + 1u;
+    const float scale = lc == 1u ? 1.0f / COARSEN : 1.0f;
+    const uint32_t at = mgAt(a, lc, cx, cy);
+    a.mg[at] = make_float4(0.0f, fixed ? 0.0f : rhs, wR * scale, wD * scale);
+    a.mg[at + 1u] = make_float4(edge * scale, fixed ? 1.0f : 0.0f, 0.0f, 0.0f);
+}
+
+// down (restricts: the residual into level l + 1) or smoothTile (level 0
+// the coarsest: its correction into its potential)
+template <int T>
+__global__ void __launch_bounds__(256) k_airDown(AirDev a, uint32_t l, int restricts, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    __shared__ AirTile<T> t;
+    constexpr int SPAN = AirTile<T>::SPAN;
+    const int32_t ox = (int32_t)blockIdx.x * T, oy = (int32_t)blockIdx.y * T, cx0 = ox - HALO, cy0 = oy - HALO;
+    const int32_t dx = (int32_t)a.w[l], dy = (int32_t)a.h[l];
+    loadTile<T, false>(a, t, l, cx0, cy0);
+    for (int i = 0; i < AIR_SMOOTH; i++) {
+        sweepTile(t, 0, cx0, cy0, dx, dy);
+        sweepTile(t, 1, cx0, cy0, dx, dy);
+    }
+    // storeTile (:1328-1339)
+    for (int q = (int)threadIdx.x; q < T * T; q += 256) {
+        const int32_t x = ox + q % T, y = oy + q / T;
+        const int k = (q / T + HALO) * SPAN + q % T + HALO;
+        if (x >= dx || y >= dy || !(t.diag[k] >= 0.0f)) continue;
+        if (!restricts) a.mg[mgAt(a, l, x, y)].x = t.x[k];
+        else if (l == 0u) a.cg[(uint32_t)y * a.W + (uint32_t)x].w = t.x[k];
+        else a.mg[mgAt(a, l, x, y) + 1u].z = t.x[k];
+    }
+    if (restricts) {
+        const int32_t cw = (int32_t)a.w[l + 1u], ch = (int32_t)a.h[l + 1u];
+        for (int q = (int)threadIdx.x; q < (T / 2) * (T / 2); q += 256) {
+            const int32_t cx = ox / 2 + q % (T / 2), cy = oy / 2 + q / (T / 2);
+            if (cx < cw && cy < ch) restrictCell(a, t, l, cx0, cy0, cx, cy);
+        }
+    }
+}
+
+template <int T>
+__global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    __shared__ AirTile<T> t;
+    constexpr int SPAN = AirTile<T>::SPAN;
+    const int32_t ox = (int32_t)blockIdx.x * T, oy = (int32_t)blockIdx.y * T, cx0 = ox - HALO, cy0 = oy - HALO;
+    const int32_t dx = (int32_t)a.w[l], dy = (int32_t)a.h[l];
+    loadTile<T, true>(a, t, l, cx0, cy0);
+    for (int i = 0; i < AIR_SMOOTH; i++) {
+        sweepTile(t, 1, cx0, cy0, dx, dy);
+        sweepTile(t, 0, cx0, cy0, dx, dy);
+    }
+    for (int q = (int)threadIdx.x; q < T * T; q += 256) {
+        const int32_t x = ox + q % T, y = oy + q / T;
+        const int k = (q / T + HALO) * SPAN + q % T + HALO;
+        if (x < dx && y < dy && t.diag[k] >= 0.0f) a.mg[mgAt(a, l, x, y)].x = t.x[k];
+    }
+}
+
+// solveCoarsest (:1482-1535): symmetric SOR in LDS, a cell a thread
+__global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    __shared__ float4 A[AIR_COARSEST];
+    __shared__ float2 B[AIR_COARSEST];
+    const uint32_t l = a.L - 1u, dx = a.w[l], dy = a.h[l], n = min(dx * dy, (uint32_t)AIR_COARSEST);
+    const uint32_t j = threadIdx.x;
+    if (j < n) {
+        A[j] = a.mg[a.base[l] + 2u * j];
+        const float4 b = a.mg[a.base[l] + 2u * j + 1u];
+        B[j] = make_float2(b.x, b.y);
+    }
+    __syncthreads();
+    const int32_t cx = (int32_t)(j % dx), cy = (int32_t)(j / dx);
+    for (int sweep = 0; sweep < 2 * COARSE_SWEEPS; sweep++) {
+        const int color = sweep < COARSE_SWEEPS ? (sweep & 1) : ((2 * COARSE_SWEEPS - 1 - sweep) & 1);
+        if (j < n && ((cx + cy) & 1) == color && !(B[j].y > 0.0f)) {
+            const float4 m = A[j];
+            float sum = 0.0f;
+            float diag = m.z + m.w + B[j].x + anchorAt(a, l, cx, cy);
+            if ((uint32_t)cx + 1u < dx) sum += m.z * A[j + 1u].x;
+            if ((uint32_t)cy + 1u < dy) sum += m.w * A[j + dx].x;
+            if (cx > 0) {
+                const float4 o = A[j - 1u];
+                sum += o.z * o.x;
+                diag += o.z;
+            }
+            if (cy > 0) {
+                const float4 o = A[j - dx];
+                sum += o.w * o.x;
+                diag += o.w;
+            }
+            // mix(e1, e2, t) = e1 (1 - t) + e2 t
+            const float xn = (sum - m.y) / diag;
+            A[j].x = m.x * (1.0f - COARSE_OMEGA) + xn * COARSE_OMEGA;
+        }
+        __syncthreads();
+    }
+    if (j < n) a.mg[a.base[l] + 2u * j].x = A[j].x;
+}
+
+// cgRho (:1099-1112) and cgApply (:1141-1157): a wavefront a group of 8 x 8
+// level-0 cells, its partial by the group's tree into the sums
+__global__ void __launch_bounds__(256) k_airRho(AirDev a, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    const uint32_t g = blockIdx.x * 4u + threadIdx.x / 64u, lane = threadIdx.x & 63u;
+    if (g >= a.groups) return;
+    const uint32_t x = (g % a.gx) * 8u + lane % 8u, y = (g / a.gx) * 8u + lane / 8u;
+    float v = 0.0f;
+    if (x < a.W && y < a.H) v = a.cg[y * a.W + x].y * a.mg[mgAt(a, 0u, (int32_t)x, (int32_t)y)].x;
+    v = groupTree(v);
+    if (lane == 0u) sums(a)[4u + g] = v;
+}
+__global__ void __launch_bounds__(256) k_airApply(AirDev a, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    const uint32_t g = blockIdx.x * 4u + threadIdx.x / 64u, lane = threadIdx.x & 63u;
+    if (g >= a.groups) return;
+    const uint32_t x = (g % a.gx) * 8u + lane % 8u, y = (g / a.gx) * 8u + lane / 8u;
+    float v = 0.0f;
+    if (x < a.W && y < a.H) {
+        const uint32_t i = y * a.W + x, at = mgAt(a, 0u, (int32_t)x, (int32_t)y);
+        const float4 m0 = a.mg[at], m1 = a.mg[at + 1u];
+        const float q = m1.y > 0.0f ? 0.0f : applyA<2>(a, (int32_t)x, (int32_t)y, m0, m1);
+        a.cg[i].w = q;
+        v = a.cg[i].z * q;
+    }
+    v = groupTree(v);
+    if (lane == 0u) sums(a)[4u + g] = v;
+}
+
+// sumAll (:1081-1097): 256 lanes each its strided share, then halving
+__device__ float sumAll(const AirDev &a) {
+    __shared__ float wide[256];
+    const uint32_t li = threadIdx.x;
+    const float *s = sums(a) + 4;
+    float v = 0.0f;
+    for (uint32_t k = li; k < a.groups; k += 256u) v += s[k];
+    wide[li] = v;
+    __syncthreads();
+    if (li < 128u) wide[li] += wide[li + 128u];
+    __syncthreads();
+    if (li < 64u) wide[li] += wide[li + 64u];
+    __syncthreads();
+    float w = li < 64u ? wide[li] : 0.0f;
+    w = groupTree(w);
+    return w;   // lane 0's is the sum
+}
+// cgBeta (:1114-1126)
+__global__ void __launch_bounds__(256) k_airBeta(AirDev a, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    const float rho = sumAll(a);
+    if (threadIdx.x == 0u) {
+        float *s = sums(a);
+        const float before = s[0];
+        s[2] = (cycle == 0u || before == 0.0f) ? 0.0f : rho / before;
+        s[0] = rho;
+    }
+}
+// cgAlpha (:1159-1170)
+__global__ void __launch_bounds__(256) k_airAlpha(AirDev a, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    const float pq = sumAll(a);
+    if (threadIdx.x == 0u) {
+        float *s = sums(a);
+        s[1] = pq <= 0.0f ? 0.0f : s[0] / pq;
+    }
+}
+
+// cgDirection (:1128-1139)
+__global__ void __launch_bounds__(256) k_airDirection(AirDev a, uint32_t cycle) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.W * a.H || airConverged(a, cycle)) return;
+    const int32_t x = (int32_t)(i % a.W), y = (int32_t)(i / a.W);
+    const uint32_t at = mgAt(a, 0u, x, y);
+    const float beta = sums(a)[2];
+    const float p = a.mg[at + 1u].y > 0.0f ? 0.0f : a.mg[at].x + beta * a.cg[i].z;
+    a.cg[i].z = p;
+}
+
+// cgUpdate (:1172-1194)
+__global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const bool conv = airConverged(a, cycle);
+    if (i == 0u) {
+        a.status[1u + (cycle + 1u) % 3u] = 0u;
+        if (!conv) a.status[4] += 1u;
+    }
+    if (conv) return;
+    uint32_t most = 0u;
+    if (i < a.W * a.H) {
+        const int32_t x = (int32_t)(i % a.W), y = (int32_t)(i / a.W);
+        const uint32_t at = mgAt(a, 0u, x, y);
+        const float alpha = sums(a)[1];
+        const float4 v = a.cg[i];
+        const float r = v.y - alpha * v.w;
+        a.cg[i] = make_float4(v.x + alpha * v.z, r, v.z, v.w);
+        a.mg[at].x = 0.0f;
+        a.mg[at].y = -r;
+        most = __float_as_uint(fabsf(r));
+    }
+    most = waveMaxU32(most);
+    if ((threadIdx.x & 63u) == 0u && most) atomicMax(&a.status[1u + cycle % 3u], most);
+}
+
+// ------------------------------------------------------------ the C ABI
+
+int wig_fail_message(int code, const char *msg);
+
+struct WigAir {
+    AirDev d;
+    uint64_t mgBytes, cgBytes;
+    hipStream_t stream = nullptr, own = nullptr;
+    uint32_t tile = 16;
+    int timing = 0;
+    double ms = 0.0;
+    uint32_t calls = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+static int airFail(int code, const char *what, hipError_t e) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    return wig_fail_message(code, buf);
+}
+#define AIRCHECK(x)                                                 \
+    do {                                                            \
+        hipError_t e_ = (x);                                        \
+        if (e_ != hipSuccess) return airFail(WIG_E_DEVICE, #x, e_); \
+    } while (0)
+
+extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
+    if (width == 0 || height == 0 || width > 16384 || height > 16384) {
+        wig_fail_message(WIG_E_INVALID, "wig_air_create: sides must be 1..16384");
+        return nullptr;
+    }
+    WigAir *a = new WigAir();
+    AirDev &d = a->d;
+    d.W = width;
+    d.H = height;
+    // the levels (js/sim.js:712-719)
+    d.L = 1;
+    d.w[0] = width;
+    d.h[0] = height;
+    while (d.L < AIR_LEVELS && (uint64_t)d.w[d.L - 1] * d.h[d.L - 1] > AIR_COARSEST) {
+        d.w[d.L] = (d.w[d.L - 1] + 1u) / 2u;
+        d.h[d.L] = (d.h[d.L - 1] + 1u) / 2u;
+        d.L++;
+    }
+    if ((uint64_t)d.w[d.L - 1] * d.h[d.L - 1] > AIR_COARSEST) {
+        wig_fail_message(WIG_E_INVALID, "wig_air_create: the coarsest of 9 levels would hold more than 256 cells");
+        delete a;
+        return nullptr;
+    }
+    uint64_t cells = 0;
+    for (uint32_t l = 0; l < d.L; l++) {
+        d.base[l] = (uint32_t)(2u * cells);
+        cells += (uint64_t)d.w[l] * d.h[l];
+    }
+    for (uint32_t l = d.L; l < AIR_LEVELS; l++) d.w[l] = d.h[l] = d.base[l] = 0;
+    d.gx = (width + 7u) / 8u;
+    d.groups = d.gx * ((height + 7u) / 8u);
+    a->mgBytes = 32u * cells;
+    a->cgBytes = 16u * ((uint64_t)width * height + (4u + d.groups + 3u) / 4u);
+    void *mg = nullptr, *cg = nullptr, *st = nullptr, *mean = nullptr;
+    hipError_t e = hipMalloc(&mg, a->mgBytes);
+    if (e == hipSuccess) e = hipMalloc(&cg, a->cgBytes);
+    if (e == hipSuccess) e = hipMalloc(&st, 32);
+    if (e == hipSuccess) e = hipMalloc(&mean, 4ull * width * height);
+    if (e == hipSuccess) e = hipStreamCreate(&a->own);
+    if (e == hipSuccess) e = hipMemsetAsync(mg, 0, a->mgBytes, a->own);
+    if (e == hipSuccess) e = hipMemsetAsync(cg, 0, a->cgBytes, a->own);
+    if (e == hipSuccess) e = hipMemsetAsync(st, 0, 32, a->own);
+    if (e == hipSuccess) e = hipMemsetAsync(mean, 0, 4ull * width * height, a->own);
+    if (e == hipSuccess) e = hipStreamSynchronize(a->own);
+    if (e != hipSuccess) {
+        airFail(e == hipErrorOutOfMemory ? WIG_E_NOMEM : WIG_E_DEVICE, "wig_air_create", e);
+        if (mg) (void)hipFree(mg);
+        if (cg) (void)hipFree(cg);
+        if (st) (void)hipFree(st);
+        if (mean) (void)hipFree(mean);
+        if (a->own) (void)hipStreamDestroy(a->own);
+        delete a;
+        return nullptr;
+    }
+    d.mg = (float4 *)mg;
+    d.cg = (float4 *)cg;
+    d.status = (uint32_t *)st;
+    d.mean = (const float *)mean;
+    a->stream = a->own;
+    return a;
+}
+
+extern "C" int wig_air_destroy(WigAir *a) {
+    if (!a) return wig_fail_message(WIG_E_INVALID, "wig_air_destroy: NULL");
+    (void)hipStreamSynchronize(a->stream);
+    (void)hipFree(a->d.mg);
+    (void)hipFree(a->d.cg);
+    (void)hipFree(a->d.status);
+    (void)hipFree((void *)a->d.mean);
+    if (a->t0) (void)hipEventDestroy(a->t0);
+    if (a->t1) (void)hipEventDestroy(a->t1);
+    if (a->own) (void)hipStreamDestroy(a->own);
+    delete a;
+    return WIG_OK;
+}
+
+extern "C" int wig_air_layout(const WigAir *a, WigAirLayout *out) {
+    if (!a || !out) return wig_fail_message(WIG_E_INVALID, "wig_air_layout: NULL");
+    memset(out, 0, sizeof *out);
+    out->width = a->d.W;
+    out->height = a->d.H;
+    out->levels = a->d.L;
+    out->groups = a->d.groups;
+    for (uint32_t l = 0; l < a->d.L; l++) {
+        out->level_width[l] = a->d.w[l];
+        out->level_height[l] = a->d.h[l];
+        out->level_base[l] = a->d.base[l];
+    }
+    out->part_bytes[WIG_AIR_PART_MG] = a->mgBytes;
+    out->part_bytes[WIG_AIR_PART_CG] = a->cgBytes;
+    out->part_bytes[WIG_AIR_PART_STATUS] = 32;
+    out->part_bytes[WIG_AIR_PART_MEAN] = 4ull * a->d.W * a->d.H;
+    return WIG_OK;
+}
+
+static void *airPart(WigAir *a, int part, uint64_t *bytes) {
+    switch (part) {
+    case WIG_AIR_PART_MG: *bytes = a->mgBytes; return a->d.mg;
+    case WIG_AIR_PART_CG: *bytes = a->cgBytes; return a->d.cg;
+    case WIG_AIR_PART_STATUS: *bytes = 32; return a->d.status;
+    case WIG_AIR_PART_MEAN: *bytes = 4ull * a->d.W * a->d.H; return (void *)a->d.mean;
+    default: return nullptr;
+    }
+}
+
+extern "C" int wig_air_upload(WigAir *a, int part, const void *src, size_t bytes) {
+    if (!a || !src) return wig_fail_message(WIG_E_INVALID, "wig_air_upload: NULL");
+    uint64_t n = 0;
+    void *p = airPart(a, part, &n);
+    if (!p || bytes != n) return wig_fail_message(WIG_E_INVALID, "wig_air_upload: no such part, or not its size");
+    AIRCHECK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, a->stream));
+    AIRCHECK(hipStreamSynchronize(a->stream));
+    return WIG_OK;
+}
+
+extern "C" int wig_air_download(WigAir *a, int part, void *dst, size_t bytes) {
+    if (!a || !dst) return wig_fail_message(WIG_E_INVALID, "wig_air_download: NULL");
+    uint64_t n = 0;
+    void *p = airPart(a, part, &n);
+    if (!p || bytes != n) return wig_fail_message(WIG_E_INVALID, "wig_air_download: no such part, or not its size");
+    AIRCHECK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, a->stream));
+    AIRCHECK(hipStreamSynchronize(a->stream));
+    return WIG_OK;
+}
+
+extern "C" int wig_air_device_ptr(WigAir *a, int part, void **ptr) {
+    if (!a || !ptr) return wig_fail_message(WIG_E_INVALID, "wig_air_device_ptr: NULL");
+    uint64_t n = 0;
+    *ptr = airPart(a, part, &n);
+    return *ptr ? WIG_OK : wig_fail_message(WIG_E_INVALID, "wig_air_device_ptr: no such part");
+}
+
+extern "C" int wig_air_set_stream(WigAir *a, void *stream) {
+    if (!a) return wig_fail_message(WIG_E_INVALID, "wig_air_set_stream: NULL");
+    a->stream = stream ? (hipStream_t)stream : a->own;
+    return WIG_OK;
+}
+
+extern "C" int wig_air_set_tile(WigAir *a, uint32_t side) {
+    if (!a || (side != 16 && side != 32)) return wig_fail_message(WIG_E_INVALID, "wig_air_set_tile: side 16 or 32");
+    a->tile = side;
+    return WIG_OK;
+}
+
+template <int T>
+static void launchLevel(const WigAir *a, bool up, uint32_t l, int restricts, uint32_t cycle) {
+    const AirDev &d = a->d;
+    const dim3 grid((d.w[l] + T - 1) / T, (d.h[l] + T - 1) / T);
+    if (up) k_airUp<T><<<grid, 256, 0, a->stream>>>(d, l, cycle);
+    else k_airDown<T><<<grid, 256, 0, a->stream>>>(d, l, restricts, cycle);
+}
+
+extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
+    if (!a) return wig_fail_message(WIG_E_INVALID, "wig_air_project: NULL");
+    if (cycles > 4096) return wig_fail_message(WIG_E_INVALID, "wig_air_project: at most 4096 iterations");
+    const AirDev &d = a->d;
+    const uint32_t n = d.W * d.H, deepest = d.L - 1u;
+    const uint32_t cellBlocks = (n + 255u) / 256u, groupBlocks = (d.groups + 3u) / 4u;
+    if (a->timing) {
+        if (!a->t0) AIRCHECK(hipEventCreate(&a->t0));
+        if (!a->t1) AIRCHECK(hipEventCreate(&a->t1));
+        AIRCHECK(hipEventRecord(a->t0, a->stream));
+    }
+    auto level = [&](bool up, uint32_t l, int restricts, uint32_t k) {
+        if (a->tile == 32) launchLevel<32>(a, up, l, restricts, k);
+        else launchLevel<16>(a, up, l, restricts, k);
+    };
+    k_airClear<<<1, 64, 0, a->stream>>>(d);
+    k_airStart<<<cellBlocks, 256, 0, a->stream>>>(d);
+    for (uint32_t k = 0; k < cycles; k++) {
+        for (uint32_t l = 0; l < deepest; l++) level(false, l, 1, k);
+        if (deepest == 0) level(false, 0, 0, k);
+        k_airCoarsest<<<1, 256, 0, a->stream>>>(d, k);
+        for (int32_t l = (int32_t)deepest - 1; l >= 0; l--) level(true, (uint32_t)l, 0, k);
+        k_airRho<<<groupBlocks, 256, 0, a->stream>>>(d, k);
+        k_airBeta<<<1, 256, 0, a->stream>>>(d, k);
+        k_airDirection<<<cellBlocks, 256, 0, a->stream>>>(d, k);
+        k_airApply<<<groupBlocks, 256, 0, a->stream>>>(d, k);
+        k_airAlpha<<<1, 256, 0, a->stream>>>(d, k);
+        k_airUpdate<<<cellBlocks, 256, 0, a->stream>>>(d, k);
+    }
+    AIRCHECK(hipGetLastError());
+    if (a->timing) {
+        AIRCHECK(hipEventRecord(a->t1, a->stream));
+        AIRCHECK(hipEventSynchronize(a->t1));
+        float ms = 0.0f;
+        AIRCHECK(hipEventElapsedTime(&ms, a->t0, a->t1));
+        a->ms += ms;
+        a->calls++;
+    }
+    return WIG_OK;
+}
+
+extern "C" int wig_air_sync(WigAir *a) {
+    if (!a) return wig_fail_message(WIG_E_INVALID, "wig_air_sync: NULL");
+    AIRCHECK(hipStreamSynchronize(a->stream));
+    return WIG_OK;
+}
+
+extern "C" int wig_air_time(WigAir *a, int on) {
+    if (!a) return wig_fail_message(WIG_E_INVALID, "wig_air_time: NULL");
+    a->timing = on;
+    a->ms = 0.0;
+    a->calls = 0;
+    return WIG_OK;
+}
+
+extern "C" int wig_air_elapsed(WigAir *a, double *ms, uint32_t *calls) {
+    if (!a || !ms || !calls) return wig_fail_message(WIG_E_INVALID, "wig_air_elapsed: NULL");
+    *ms = a->ms;
+    *calls = a->calls;
+    return WIG_OK;
+}

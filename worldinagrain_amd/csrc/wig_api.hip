@@ -82,6 +82,8 @@ static int fail(int code, const char *fmt, ...) {
     g_err = buf;
     return code;
 }
+// (the air's projection's entry points, wig_air.hip, report through this)
+int wig_fail_message(int code, const char *msg) { return fail(code, "%s", msg); }
 #define HIPCHECK(x)                                                                         \
     do {                                                                                    \
         hipError_t e_ = (x);                                                                \

@@ -191,3 +191,68 @@ def precision_world(side, seed, fluid_share=0.70, air_share=0.20, solid_share=0.
     vel = (g.normal(0.0, vel_sigma, size=(side, side)) * dt, g.normal(0.0, vel_sigma, size=(side, side)) * dt)
     paint(cells, side, side, fluid, E.WATER, vel=vel)
     return cells, int(fluid.sum())
+
+
+def air_rooms(width, height, seed=1, walls=12, liquid_share=0.15, holes=2, rhs_sigma=0.05, warm_sigma=0.01, flow_seed=None):
+    """Level 0 of the air's projection as gather fills it (shaders/air.wgsl:
+    352-414, 825-866), for a world of walls and liquid in air.
+
+    Returns (mg0, x0): mg0 a (height, width, 8) float32 array of level 0's two
+    vec4s a cell, x0 the warm-start potential (cg's x).  Walls are random
+    axis-aligned rectangles (every one a hollow room: sealed pockets of air
+    inside, held to none only by ANCHOR); the lower `liquid_share` of rows
+    holds liquid (no gas: faces shut, its open share -1).  A face is open
+    (weight 1) where the cells across it both carry gas; the last column's
+    right faces, the last row's bottom faces and the first column's left
+    edges are open where their cell carries gas (the world's open edges).
+    `holes` black holes are held at a potential below none (fixed).  The
+    right-hand side is the net outflow of face velocities N(0, rhs_sigma) on
+    the open faces (drawn from `flow_seed` if given: the same rooms, another
+    step's flow); the warm start N(0, warm_sigma) in cells that carry gas."""
+    rng = np.random.default_rng(seed)
+    W, H = width, height
+    carries = np.ones((H, W), bool)
+    for _ in range(walls):
+        w, h = int(rng.integers(6, max(7, W // 3))), int(rng.integers(6, max(7, H // 3)))
+        x0, y0 = int(rng.integers(0, max(1, W - w))), int(rng.integers(0, max(1, H - h)))
+        carries[y0:y0 + h, x0:x0 + w] = False
+        carries[y0 + 1:y0 + h - 1, x0 + 1:x0 + w - 1] = True   # a room inside
+        if rng.random() < 0.5 and w > 4 and y0 + h // 2 < H:  # a door in some
+            carries[y0 + h // 2, x0] = True
+    liquid = np.zeros((H, W), bool)
+    liquid[int(H * (1.0 - liquid_share)):, :] = True
+    liquid &= carries
+    carries &= ~liquid
+    fixed = np.zeros((H, W), bool)
+    ys, xs = np.nonzero(carries)
+    if len(ys):
+        for j in rng.choice(len(ys), size=min(holes, len(ys)), replace=False):
+            fixed[ys[j], xs[j]] = True
+    right = np.zeros((H, W), bool)
+    right[:, :-1] = carries[:, :-1] & carries[:, 1:]
+    right[:, -1] = carries[:, -1]
+    down = np.zeros((H, W), bool)
+    down[:-1, :] = carries[:-1, :] & carries[1:, :]
+    down[-1, :] = carries[-1, :]
+    edge = np.zeros((H, W), bool)
+    edge[:, 0] = carries[:, 0]
+    share = np.where(carries, 1.0, np.where(liquid, -1.0, 0.0)).astype(np.float32)
+    mg0 = np.zeros((H, W, 8), np.float32)
+    # divergenceAt's net outflow (:825-838) of random face velocities: zero
+    # summed over a sealed room, as the flow through its faces cancels
+    flow = rng if flow_seed is None else np.random.default_rng(flow_seed)
+    u = flow.normal(0.0, rhs_sigma, (H, W)).astype(np.float32) * right
+    v = flow.normal(0.0, rhs_sigma, (H, W)).astype(np.float32) * down
+    out = u + v
+    out[:, 1:] -= u[:, :-1]
+    out[1:, :] -= v[:-1, :]
+    mg0[..., 1] = out
+    mg0[..., 2] = right
+    mg0[..., 3] = down
+    mg0[..., 4] = edge
+    mg0[..., 5] = fixed
+    mg0[..., 6] = np.where(carries, np.float32(0.0), np.uint32(0xffffffff).view(np.float32))   # piece bits
+    mg0[..., 7] = share
+    x0 = np.where(carries, rng.normal(0.0, warm_sigma, (H, W)), 0.0).astype(np.float32)
+    x0[fixed] = -0.5                                           # -sinkSpeed (gather, :406-408)
+    return mg0, x0
