@@ -92,7 +92,7 @@ def cpu_baseline(cells, kinds, size, tol, steps, solver=None):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=("single", "batch", "serve", "precision", "snapshot"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "serve", "precision", "snapshot", "air"), default="single",
                     help="single: the 1024x1024 world (the metric's configuration); batch: many worlds in one batch; "
                          "serve: the serving loop over many sessions")
     ap.add_argument("--worlds", type=int, default=512, help="batch: how many worlds")
@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--round-cells", type=int, default=1 << 22,
                     help="serve: most cells a round steps (4 M keeps warm rounds with 512x512 worlds under 16 ms at p99)")
     ap.add_argument("--seeds", type=int, default=2, help="precision: worlds (seeds) per variant")
+    ap.add_argument("--air-tile", type=int, default=32, help="air: the level passes' LDS tile side (16 or 32)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (single: 100, batch: 50)")
     ap.add_argument("--storage", choices=tuple(STORAGES), default=None,
@@ -122,7 +123,7 @@ def main():
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 50 if args.workload == "batch" else 100
+        args.steps = {"batch": 50, "air": 30}.get(args.workload, 100)
     if args.storage is None:
         args.storage = "fp16" if args.workload in ("batch", "serve") else "fp32"
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -144,6 +145,8 @@ def main():
         return run_precision(args)
     if args.workload == "snapshot":
         return run_snapshot(args)
+    if args.workload == "air":
+        return run_air(args)
 
     if args.copy_mib:
         src = torch.empty(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -519,6 +522,94 @@ def run_snapshot(args):
     print(json.dumps(out), flush=True)
     if not same:
         sys.exit(1)
+
+
+def air_bytes(dims):
+    """The air projection's algorithmic bytes: (per solve start, per
+    iteration).  cgStart reads level 0's two float4s, the potential and the
+    zones' mean and writes the CG's float4 and the potential's and rhs's
+    words: 64 B a cell.  An iteration: each level pass pair but the
+    coarsest's (down: 32 B read, the smoothed correction's 4 B and a quarter
+    of the coarse cell's 32 B written; up: 32 B, the correction's 4 B and a
+    quarter of the coarse potential's 4 B read, 4 B written) 85 B a cell of
+    its level; the CG passes 88 B a level-0 cell (cgRho 8, cgDirection 16,
+    cgApply 24, cgUpdate 40)."""
+    c0 = dims[0][0] * dims[0][1]
+    levels = sum(w * h for w, h in dims[:-1])
+    return 64.0 * c0, 85.0 * levels + 88.0 * c0
+
+
+def run_air(args):
+    """The air's projection (worldinagrain_amd.air; shaders/air.wgsl's
+    MG-PCG): a size x size air grid of 60 walled rooms (sealed air inside,
+    some with a door), the lower 15 % liquid, two black holes, the divergence
+    of N(0, 0.05) face velocities (scenes.air_rooms).  Each timed solve
+    starts from the same level 0 and warm potentials, restored by an upload
+    before it (outside the timed region), and runs up to AIR_CYCLES (64)
+    iterations, those after the most |r| falls under 1e-4 skipping; HIP
+    events around each solve on its stream."""
+    import numpy as np
+    from worldinagrain_amd import scenes
+    from worldinagrain_amd.air import GpuAir, PART_MG, PART_CG, PART_STATUS, AIR_CYCLES
+
+    size = args.size
+    mg0, x0 = scenes.air_rooms(size, size, seed=21, walls=60)
+    g = GpuAir(size, size)
+    g.set_tile(args.air_tile)
+    g.load(mg0, x0)
+    mg_init, cg_init = g.download(PART_MG), g.download(PART_CG)
+
+    def restore():
+        g.upload(PART_MG, mg_init)
+        g.upload(PART_CG, cg_init)
+
+    for _ in range(args.warmup):
+        restore()
+        g.project(AIR_CYCLES)
+        g.sync()
+    its = int(g.download(PART_STATUS)[4])
+    g.time(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        restore()
+        g.project(AIR_CYCLES)
+    wall = time.perf_counter() - t0
+    ms, n = g.elapsed()
+    its_last = int(g.download(PART_STATUS)[4])
+    r = g.download(PART_CG)[:4 * size * size].reshape(size, size, 4)[..., 1]
+    ms_solve = ms / n
+    start_b, iter_b = air_bytes(g.dims)
+    solve_b = start_b + its * iter_b
+    out = {
+        "metric": "air projection solves/s (MG-PCG, most |r| under 1e-4)", "workload": "air",
+        "value": 1000.0 / ms_solve, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_solve, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{size}x{size} air grid, 60 walled rooms, lower 15% liquid, 2 black holes, "
+                               f"divergence of N(0,0.05) face velocities, {len(g.dims)} levels, LDS tiles "
+                               f"{args.air_tile}", "iterations": its, "tile": args.air_tile},
+        "iterations": its, "ms_per_iteration": ms_solve / its if its else None,
+        "most_r": float(np.abs(r).max()), "wall_s_with_uploads": wall,
+        "roofline": {"bound": "hbm", "achieved": solve_b / (ms_solve * 1e-3) / 1e9, "peak": 8000.0,
+                     "unit": "GB/s", "frac": solve_b / (ms_solve * 1e-3) / 8e12, "traffic": None,
+                     "unit_of_work": "one solve: cgStart and its iterations",
+                     "bytes_per_solve": solve_b, "bytes_per_iteration": iter_b},
+    }
+    assert its_last == its and its < AIR_CYCLES, (its, its_last)
+    if not args.no_cpu:
+        from oracle.air import OracleAir
+        o = OracleAir(size, size)
+        o.load(mg0, x0)
+        t0 = time.perf_counter()
+        o.project(AIR_CYCLES)
+        dt = time.perf_counter() - t0
+        assert int(o.status[4]) == its
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"one solve of the same {size}x{size} grid ({its} iterations), "
+                                         f"single-threaded C restatement of air.wgsl's projection "
+                                         f"(oracle/air_oracle.c, -O2), {dt:.1f} s"}
+    g.close()
+    print(json.dumps(out), flush=True)
 
 
 def run_serve(args):

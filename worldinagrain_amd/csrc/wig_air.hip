@@ -57,6 +57,23 @@ __device__ __forceinline__ uint32_t waveMaxU32(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
     return v;
 }
+// The most |r| of a workgroup of 256 into a status slot: one atomic a
+// workgroup, and none where the slot already holds as much (a max is the
+// same in any order)
+__device__ __forceinline__ void slotMax(uint32_t *slot, uint32_t v) {
+    __shared__ uint32_t w4[4];
+    v = waveMaxU32(v);
+    if ((threadIdx.x & 63u) == 0u) w4[threadIdx.x / 64u] = v;
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        v = max(max(w4[0], w4[1]), max(w4[2], w4[3]));
+        if (v > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(slot, v);
+    }
+}
+// The level-0 passes over cells: a grid of at most CELL_GROUPS workgroups
+// striding over them (so that few workgroups, not every wavefront, take
+// the status slot's atomic)
+#define CELL_GROUPS 1024u
 // the 64-value tree (sumGroup, :1054-1069): lane i < s takes lane i + s's
 __device__ __forceinline__ float groupTree(float v) {
     for (int s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
@@ -96,9 +113,8 @@ __device__ __forceinline__ float applyA(const AirDev &a, int32_t x, int32_t y, f
 
 // cgStart (:1027-1048)
 __global__ void __launch_bounds__(256) k_airStart(AirDev a) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     uint32_t most = 0u;
-    if (i < a.W * a.H) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < a.W * a.H; i += gridDim.x * 256u) {
         const int32_t x = (int32_t)(i % a.W), y = (int32_t)(i / a.W);
         const uint32_t at = mgAt(a, 0u, x, y);
         float4 m0 = a.mg[at];
@@ -113,10 +129,9 @@ __global__ void __launch_bounds__(256) k_airStart(AirDev a) {
         a.cg[i] = make_float4(px, r, 0.0f, 0.0f);
         a.mg[at].x = 0.0f;
         a.mg[at].y = -r;
-        most = __float_as_uint(fabsf(r));
+        most = max(most, __float_as_uint(fabsf(r)));
     }
-    most = waveMaxU32(most);
-    if ((threadIdx.x & 63u) == 0u && most) atomicMax(&a.status[3], most);
+    slotMax(&a.status[3], most);
 }
 
 // The V-cycle's level pass on level l (down: smoothDown, :1428-1459; up:
@@ -369,8 +384,19 @@ __device__ float sumAll(const AirDev &a) {
     __shared__ float wide[256];
     const uint32_t li = threadIdx.x;
     const float *s = sums(a) + 4;
+    // each lane's share in order, its loads 16 at a time in flight
     float v = 0.0f;
-    for (uint32_t k = li; k < a.groups; k += 256u) v += s[k];
+    for (uint32_t k0 = li; k0 < a.groups; k0 += 16u * 256u) {
+        float h[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t k = k0 + 256u * (uint32_t)j;
+            h[j] = k < a.groups ? s[k] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (k0 + 256u * (uint32_t)j < a.groups) v += h[j];
+    }
     wide[li] = v;
     __syncthreads();
     if (li < 128u) wide[li] += wide[li + 128u];
@@ -415,15 +441,14 @@ __global__ void __launch_bounds__(256) k_airDirection(AirDev a, uint32_t cycle) 
 
 // cgUpdate (:1172-1194)
 __global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const bool conv = airConverged(a, cycle);
-    if (i == 0u) {
+    if (blockIdx.x == 0u && threadIdx.x == 0u) {
         a.status[1u + (cycle + 1u) % 3u] = 0u;
         if (!conv) a.status[4] += 1u;
     }
     if (conv) return;
     uint32_t most = 0u;
-    if (i < a.W * a.H) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < a.W * a.H; i += gridDim.x * 256u) {
         const int32_t x = (int32_t)(i % a.W), y = (int32_t)(i / a.W);
         const uint32_t at = mgAt(a, 0u, x, y);
         const float alpha = sums(a)[1];
@@ -432,10 +457,9 @@ __global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
         a.cg[i] = make_float4(v.x + alpha * v.z, r, v.z, v.w);
         a.mg[at].x = 0.0f;
         a.mg[at].y = -r;
-        most = __float_as_uint(fabsf(r));
+        most = max(most, __float_as_uint(fabsf(r)));
     }
-    most = waveMaxU32(most);
-    if ((threadIdx.x & 63u) == 0u && most) atomicMax(&a.status[1u + cycle % 3u], most);
+    slotMax(&a.status[1u + cycle % 3u], most);
 }
 
 // ------------------------------------------------------------ the C ABI
@@ -622,6 +646,7 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
     const AirDev &d = a->d;
     const uint32_t n = d.W * d.H, deepest = d.L - 1u;
     const uint32_t cellBlocks = (n + 255u) / 256u, groupBlocks = (d.groups + 3u) / 4u;
+    const uint32_t cellGrid = cellBlocks < CELL_GROUPS ? cellBlocks : CELL_GROUPS;
     if (a->timing) {
         if (!a->t0) AIRCHECK(hipEventCreate(&a->t0));
         if (!a->t1) AIRCHECK(hipEventCreate(&a->t1));
@@ -632,7 +657,7 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         else launchLevel<16>(a, up, l, restricts, k);
     };
     k_airClear<<<1, 64, 0, a->stream>>>(d);
-    k_airStart<<<cellBlocks, 256, 0, a->stream>>>(d);
+    k_airStart<<<cellGrid, 256, 0, a->stream>>>(d);
     for (uint32_t k = 0; k < cycles; k++) {
         for (uint32_t l = 0; l < deepest; l++) level(false, l, 1, k);
         if (deepest == 0) level(false, 0, 0, k);
@@ -643,7 +668,7 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         k_airDirection<<<cellBlocks, 256, 0, a->stream>>>(d, k);
         k_airApply<<<groupBlocks, 256, 0, a->stream>>>(d, k);
         k_airAlpha<<<1, 256, 0, a->stream>>>(d, k);
-        k_airUpdate<<<cellBlocks, 256, 0, a->stream>>>(d, k);
+        k_airUpdate<<<cellGrid, 256, 0, a->stream>>>(d, k);
     }
     AIRCHECK(hipGetLastError());
     if (a->timing) {
