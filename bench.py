@@ -103,7 +103,8 @@ def main():
     ap.add_argument("--round-cells", type=int, default=1 << 22,
                     help="serve: most cells a round steps (4 M keeps warm rounds with 512x512 worlds under 16 ms at p99)")
     ap.add_argument("--seeds", type=int, default=2, help="precision: worlds (seeds) per variant")
-    ap.add_argument("--air-tile", type=int, default=32, help="air: the level passes' LDS tile side (16 or 32)")
+    ap.add_argument("--air-tile", type=int, default=0,
+                    help="air: every level's LDS tile side (16 or 32); 0: 32 on level 0, 16 on the others")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (single: 100, batch: 50)")
     ap.add_argument("--storage", choices=tuple(STORAGES), default=None,
@@ -555,7 +556,8 @@ def run_air(args):
     size = args.size
     mg0, x0 = scenes.air_rooms(size, size, seed=21, walls=60)
     g = GpuAir(size, size)
-    g.set_tile(args.air_tile)
+    if args.air_tile:
+        g.set_tile(args.air_tile)
     g.load(mg0, x0)
     mg_init, cg_init = g.download(PART_MG), g.download(PART_CG)
 
@@ -587,7 +589,8 @@ def run_air(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{size}x{size} air grid, 60 walled rooms, lower 15% liquid, 2 black holes, "
                                f"divergence of N(0,0.05) face velocities, {len(g.dims)} levels, LDS tiles "
-                               f"{args.air_tile}", "iterations": its, "tile": args.air_tile},
+                               f"{args.air_tile or '32 on level 0, 16 on the others'}", "iterations": its,
+                   "tile": args.air_tile or "32/16"},
         "iterations": its, "ms_per_iteration": ms_solve / its if its else None,
         "most_r": float(np.abs(r).max()), "wall_s_with_uploads": wall,
         "roofline": {"bound": "hbm", "achieved": solve_b / (ms_solve * 1e-3) / 1e9, "peak": 8000.0,

@@ -307,9 +307,10 @@ int wig_air_download(WigAir *a, int part, void *dst, size_t bytes);
 int wig_air_device_ptr(WigAir *a, int part, void **ptr);
 /* Runs later calls on this HIP stream (NULL: the grid's own). */
 int wig_air_set_stream(WigAir *a, void *stream);
-/* The side of the level passes' LDS tiles, 16 (default) or 32: the results
- * are the same bits either way.  This library's own. */
-int wig_air_set_tile(WigAir *a, uint32_t side);
+/* The side of a level's LDS tiles in the V-cycle's passes, 16 or 32
+ * (default 32 on level 0, 16 on the others): the results are the same bits
+ * either way.  This library's own. */
+int wig_air_set_tile(WigAir *a, uint32_t level, uint32_t side);
 /* The projection's solve, enqueued as encodeAir dispatches it
  * (js/sim.js:2264-2289): gather's clearing of the solve's status slots
  * (air.wgsl:444-449), cgStart, then `cycles` iterations (the host's

@@ -39,15 +39,16 @@ def next_step(g, o, W, H, seed, step):
 
 
 @pytest.mark.parametrize("W,H,seed,tile", [
-    (96, 80, 3, 16), (96, 80, 3, 32), (200, 130, 5, 16), (200, 130, 5, 32), (37, 29, 4, 32),
-    (16, 12, 2, 16), (5, 190, 4, 16), (1, 1, 0, 16), (300, 7, 8, 32),
+    (96, 80, 3, 16), (96, 80, 3, 32), (96, 80, 3, 0), (200, 130, 5, 16), (200, 130, 5, 32), (200, 130, 5, 0),
+    (37, 29, 4, 32), (16, 12, 2, 16), (5, 190, 4, 16), (1, 1, 0, 16), (300, 7, 8, 32),
 ])
 def test_air_projection_matches_oracle(W, H, seed, tile):
     say(f"[air] {W}x{H} seed {seed} tile {tile}")
     mg0, x0 = scenes.air_rooms(W, H, seed=seed)
     g, o = GpuAir(W, H), OracleAir(W, H)
     assert g.dims == o.dims
-    g.set_tile(tile)
+    if tile:                      # (0: the default, 32 on level 0 and 16 below)
+        g.set_tile(tile)
     g.load(mg0, x0)
     o.load(mg0, x0)
     for step in range(3):
@@ -96,10 +97,9 @@ def test_air_host_policy_on_gpu():
 
 def test_air_full_size():
     W = H = 1024
-    say(f"[air] {W}x{H} tile 32")
+    say(f"[air] {W}x{H} default tiles")
     mg0, x0 = scenes.air_rooms(W, H, seed=21, walls=60)
     g, o = GpuAir(W, H), OracleAir(W, H)
-    g.set_tile(32)
     g.load(mg0, x0)
     o.load(mg0, x0)
     g.project(64)

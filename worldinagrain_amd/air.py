@@ -44,7 +44,7 @@ def lib():
         L.wig_air_download.argtypes = [P, ct.c_int, P, ct.c_size_t]
         L.wig_air_device_ptr.argtypes = [P, ct.c_int, ct.POINTER(P)]
         L.wig_air_set_stream.argtypes = [P, P]
-        L.wig_air_set_tile.argtypes = [P, ct.c_uint32]
+        L.wig_air_set_tile.argtypes = [P, ct.c_uint32, ct.c_uint32]
         L.wig_air_project.argtypes = [P, ct.c_uint32]
         L.wig_air_sync.argtypes = [P]
         L.wig_air_time.argtypes = [P, ct.c_int]
@@ -103,8 +103,10 @@ class GpuAir:
         self.upload(PART_STATUS, np.zeros(8, np.uint32))
         self.upload(PART_MEAN, np.zeros(n, np.float32) if mean is None else np.asarray(mean, np.float32).reshape(-1))
 
-    def set_tile(self, side):
-        _check(lib().wig_air_set_tile(self._a, side))
+    def set_tile(self, side, level=None):
+        """A level's LDS tile side (16 or 32); every level's if level is None."""
+        for l in range(self.levels) if level is None else [level]:
+            _check(lib().wig_air_set_tile(self._a, l, side))
 
     def set_stream(self, stream):
         _check(lib().wig_air_set_stream(self._a, stream))

@@ -74,6 +74,7 @@ __device__ __forceinline__ void slotMax(uint32_t *slot, uint32_t v) {
 // striding over them (so that few workgroups, not every wavefront, take
 // the status slot's atomic)
 #define CELL_GROUPS 1024u
+#define UPDATE_HELD 4
 // the 64-value tree (sumGroup, :1054-1069): lane i < s takes lane i + s's
 __device__ __forceinline__ float groupTree(float v) {
     for (int s = 32; s > 0; s >>= 1) v = v + __shfl_down(v, s, 64);
@@ -290,8 +291,10 @@ __global__ void __launch_bounds__(256) k_airDown(AirDev a, uint32_t l, int restr
     }
 }
 
+// (rho, on level 0: cgRho's partials of the square's 8 x 8 groups, r . z
+// with z as this pass leaves it, by the groups' tree, a wavefront a group)
 template <int T>
-__global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cycle) {
+__global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cycle, int rho) {
     if (airConverged(a, cycle)) return;
     __shared__ AirTile<T> t;
     constexpr int SPAN = AirTile<T>::SPAN;
@@ -306,6 +309,22 @@ __global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cy
         const int32_t x = ox + q % T, y = oy + q / T;
         const int k = (q / T + HALO) * SPAN + q % T + HALO;
         if (x < dx && y < dy && t.diag[k] >= 0.0f) a.mg[mgAt(a, l, x, y)].x = t.x[k];
+    }
+    if (!rho) return;
+    const uint32_t lane = threadIdx.x & 63u, gh = (a.H + 7u) / 8u;
+    for (uint32_t q = threadIdx.x / 64u; q < (uint32_t)(T / 8) * (T / 8); q += 4u) {
+        const uint32_t gx = (uint32_t)ox / 8u + q % (T / 8), gy = (uint32_t)oy / 8u + q / (T / 8);
+        if (gx >= a.gx || gy >= gh) continue;
+        const uint32_t x = gx * 8u + lane % 8u, y = gy * 8u + lane / 8u;
+        float v = 0.0f;
+        if (x < a.W && y < a.H) {
+            const int k = ((int32_t)y - cy0) * SPAN + ((int32_t)x - cx0);
+            // a held cell keeps the potential it had (none: cgUpdate's, cgStart's)
+            const float z = t.diag[k] >= 0.0f ? t.x[k] : a.mg[mgAt(a, 0u, (int32_t)x, (int32_t)y)].x;
+            v = a.cg[y * a.W + x].y * z;
+        }
+        v = groupTree(v);
+        if (lane == 0u) sums(a)[4u + gy * a.gx + gx] = v;
     }
 }
 
@@ -448,16 +467,25 @@ __global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
     }
     if (conv) return;
     uint32_t most = 0u;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < a.W * a.H; i += gridDim.x * 256u) {
-        const int32_t x = (int32_t)(i % a.W), y = (int32_t)(i / a.W);
-        const uint32_t at = mgAt(a, 0u, x, y);
-        const float alpha = sums(a)[1];
-        const float4 v = a.cg[i];
-        const float r = v.y - alpha * v.w;
-        a.cg[i] = make_float4(v.x + alpha * v.z, r, v.z, v.w);
-        a.mg[at].x = 0.0f;
-        a.mg[at].y = -r;
-        most = max(most, __float_as_uint(fabsf(r)));
+    const float alpha = sums(a)[1];
+    const uint32_t n = a.W * a.H, stride = gridDim.x * 256u;
+    // UPDATE_HELD cells a thread at a time, their loads made together
+    for (uint32_t i0 = blockIdx.x * 256u + threadIdx.x; i0 < n; i0 += UPDATE_HELD * stride) {
+        float4 v[UPDATE_HELD];
+#pragma unroll
+        for (int j = 0; j < UPDATE_HELD; j++) {
+            const uint32_t i = i0 + (uint32_t)j * stride;
+            v[j] = a.cg[i < n ? i : 0u];
+        }
+#pragma unroll
+        for (int j = 0; j < UPDATE_HELD; j++) {
+            const uint32_t i = i0 + (uint32_t)j * stride;
+            if (i >= n) continue;
+            const float r = v[j].y - alpha * v[j].w;
+            a.cg[i] = make_float4(v[j].x + alpha * v[j].z, r, v[j].z, v[j].w);
+            *(float2 *)&a.mg[mgAt(a, 0u, (int32_t)(i % a.W), (int32_t)(i / a.W))] = make_float2(0.0f, -r);
+            most = max(most, __float_as_uint(fabsf(r)));
+        }
     }
     slotMax(&a.status[1u + cycle % 3u], most);
 }
@@ -470,7 +498,7 @@ struct WigAir {
     AirDev d;
     uint64_t mgBytes, cgBytes;
     hipStream_t stream = nullptr, own = nullptr;
-    uint32_t tile = 16;
+    uint32_t tile[AIR_LEVELS] = {32, 16, 16, 16, 16, 16, 16, 16, 16};
     int timing = 0;
     double ms = 0.0;
     uint32_t calls = 0;
@@ -626,9 +654,10 @@ extern "C" int wig_air_set_stream(WigAir *a, void *stream) {
     return WIG_OK;
 }
 
-extern "C" int wig_air_set_tile(WigAir *a, uint32_t side) {
-    if (!a || (side != 16 && side != 32)) return wig_fail_message(WIG_E_INVALID, "wig_air_set_tile: side 16 or 32");
-    a->tile = side;
+extern "C" int wig_air_set_tile(WigAir *a, uint32_t level, uint32_t side) {
+    if (!a || (side != 16 && side != 32) || level >= a->d.L)
+        return wig_fail_message(WIG_E_INVALID, "wig_air_set_tile: a level of the grid, side 16 or 32");
+    a->tile[level] = side;
     return WIG_OK;
 }
 
@@ -636,7 +665,7 @@ template <int T>
 static void launchLevel(const WigAir *a, bool up, uint32_t l, int restricts, uint32_t cycle) {
     const AirDev &d = a->d;
     const dim3 grid((d.w[l] + T - 1) / T, (d.h[l] + T - 1) / T);
-    if (up) k_airUp<T><<<grid, 256, 0, a->stream>>>(d, l, cycle);
+    if (up) k_airUp<T><<<grid, 256, 0, a->stream>>>(d, l, cycle, l == 0u ? 1 : 0);
     else k_airDown<T><<<grid, 256, 0, a->stream>>>(d, l, restricts, cycle);
 }
 
@@ -653,7 +682,7 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         AIRCHECK(hipEventRecord(a->t0, a->stream));
     }
     auto level = [&](bool up, uint32_t l, int restricts, uint32_t k) {
-        if (a->tile == 32) launchLevel<32>(a, up, l, restricts, k);
+        if (a->tile[l] == 32) launchLevel<32>(a, up, l, restricts, k);
         else launchLevel<16>(a, up, l, restricts, k);
     };
     k_airClear<<<1, 64, 0, a->stream>>>(d);
@@ -663,7 +692,7 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         if (deepest == 0) level(false, 0, 0, k);
         k_airCoarsest<<<1, 256, 0, a->stream>>>(d, k);
         for (int32_t l = (int32_t)deepest - 1; l >= 0; l--) level(true, (uint32_t)l, 0, k);
-        k_airRho<<<groupBlocks, 256, 0, a->stream>>>(d, k);
+        if (deepest == 0) k_airRho<<<groupBlocks, 256, 0, a->stream>>>(d, k);   // (else in level 0's up)
         k_airBeta<<<1, 256, 0, a->stream>>>(d, k);
         k_airDirection<<<cellBlocks, 256, 0, a->stream>>>(d, k);
         k_airApply<<<groupBlocks, 256, 0, a->stream>>>(d, k);
