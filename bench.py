@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--seeds", type=int, default=2, help="precision: worlds (seeds) per variant")
     ap.add_argument("--air-tile", type=int, default=0,
                     help="air: every level's LDS tile side (16 or 32); 0: 32 on level 0, 16 on the others")
+    ap.add_argument("--air-coarse", type=int, default=0,
+                    help="air: 1, the levels that fit 64x64 in one workgroup's kernel; 0 (default), a launch each")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (single: 100, batch: 50)")
     ap.add_argument("--storage", choices=tuple(STORAGES), default=None,
@@ -558,6 +560,7 @@ def run_air(args):
     g = GpuAir(size, size)
     if args.air_tile:
         g.set_tile(args.air_tile)
+    g.set_coarse(args.air_coarse)
     g.load(mg0, x0)
     mg_init, cg_init = g.download(PART_MG), g.download(PART_CG)
 
@@ -590,7 +593,7 @@ def run_air(args):
         "config": {"workload": f"{size}x{size} air grid, 60 walled rooms, lower 15% liquid, 2 black holes, "
                                f"divergence of N(0,0.05) face velocities, {len(g.dims)} levels, LDS tiles "
                                f"{args.air_tile or '32 on level 0, 16 on the others'}", "iterations": its,
-                   "tile": args.air_tile or "32/16"},
+                   "tile": args.air_tile or "32/16", "coarse_fused": bool(args.air_coarse)},
         "iterations": its, "ms_per_iteration": ms_solve / its if its else None,
         "most_r": float(np.abs(r).max()), "wall_s_with_uploads": wall,
         "roofline": {"bound": "hbm", "achieved": solve_b / (ms_solve * 1e-3) / 1e9, "peak": 8000.0,

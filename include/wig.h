@@ -311,6 +311,10 @@ int wig_air_set_stream(WigAir *a, void *stream);
  * (default 32 on level 0, 16 on the others): the results are the same bits
  * either way.  This library's own. */
 int wig_air_set_tile(WigAir *a, uint32_t level, uint32_t side);
+/* With `on`, the levels from the first that fits 64 x 64 cells down to the
+ * coarsest and back run in one workgroup's kernel; off (the default), a
+ * launch each.  The same bits either way.  This library's own. */
+int wig_air_set_coarse(WigAir *a, int on);
 /* The projection's solve, enqueued as encodeAir dispatches it
  * (js/sim.js:2264-2289): gather's clearing of the solve's status slots
  * (air.wgsl:444-449), cgStart, then `cycles` iterations (the host's

@@ -45,6 +45,7 @@ def lib():
         L.wig_air_device_ptr.argtypes = [P, ct.c_int, ct.POINTER(P)]
         L.wig_air_set_stream.argtypes = [P, P]
         L.wig_air_set_tile.argtypes = [P, ct.c_uint32, ct.c_uint32]
+        L.wig_air_set_coarse.argtypes = [P, ct.c_int]
         L.wig_air_project.argtypes = [P, ct.c_uint32]
         L.wig_air_sync.argtypes = [P]
         L.wig_air_time.argtypes = [P, ct.c_int]
@@ -107,6 +108,10 @@ class GpuAir:
         """A level's LDS tile side (16 or 32); every level's if level is None."""
         for l in range(self.levels) if level is None else [level]:
             _check(lib().wig_air_set_tile(self._a, l, side))
+
+    def set_coarse(self, on):
+        """The coarse levels in one kernel, or a launch each (the default)."""
+        _check(lib().wig_air_set_coarse(self._a, 1 if on else 0))
 
     def set_stream(self, stream):
         _check(lib().wig_air_set_stream(self._a, stream))

@@ -35,6 +35,7 @@ struct AirDev {
     float4 *mg;
     float4 *cg;
     uint32_t *status;
+    uint32_t *most;     // a level-0 pass's most |r| bits, a workgroup each (this library's)
     const float *mean;
     uint32_t L, W, H, gx, groups;
     uint32_t w[AIR_LEVELS], h[AIR_LEVELS];
@@ -57,17 +58,29 @@ __device__ __forceinline__ uint32_t waveMaxU32(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
     return v;
 }
-// The most |r| of a workgroup of 256 into a status slot: one atomic a
-// workgroup, and none where the slot already holds as much (a max is the
-// same in any order)
-__device__ __forceinline__ void slotMax(uint32_t *slot, uint32_t v) {
+// The most |r| of a workgroup of 256, into its word of `most`; k_airSlot
+// then takes the most of those into the status slot (a max is the same in
+// any order, and no workgroup waits on another's atomic)
+__device__ __forceinline__ void groupMax(uint32_t *most, uint32_t v) {
     __shared__ uint32_t w4[4];
     v = waveMaxU32(v);
     if ((threadIdx.x & 63u) == 0u) w4[threadIdx.x / 64u] = v;
     __syncthreads();
+    if (threadIdx.x == 0u) most[blockIdx.x] = max(max(w4[0], w4[1]), max(w4[2], w4[3]));
+}
+// the status slot's atomicMax of every cell's |r| (:1047, :1193): the most
+// of the `groups` workgroups' words, into the slot as it stands
+__global__ void __launch_bounds__(1024) k_airSlot(AirDev a, uint32_t slot, uint32_t groups, uint32_t cycle, int always) {
+    if (!always && airConverged(a, cycle)) return;
+    __shared__ uint32_t w16[16];
+    uint32_t v = 0u;
+    for (uint32_t j = threadIdx.x; j < groups; j += 1024u) v = max(v, a.most[j]);
+    v = waveMaxU32(v);
+    if ((threadIdx.x & 63u) == 0u) w16[threadIdx.x / 64u] = v;
+    __syncthreads();
     if (threadIdx.x == 0u) {
-        v = max(max(w4[0], w4[1]), max(w4[2], w4[3]));
-        if (v > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(slot, v);
+        for (int w = 0; w < 16; w++) v = max(v, w16[w]);
+        a.status[slot] = max(a.status[slot], v);
     }
 }
 // The level-0 passes over cells: a grid of at most CELL_GROUPS workgroups
@@ -132,7 +145,7 @@ __global__ void __launch_bounds__(256) k_airStart(AirDev a) {
         a.mg[at].y = -r;
         most = max(most, __float_as_uint(fabsf(r)));
     }
-    slotMax(&a.status[3], most);
+    groupMax(a.most, most);
 }
 
 // The V-cycle's level pass on level l (down: smoothDown, :1428-1459; up:
@@ -143,23 +156,43 @@ struct AirTile {
     float x[CELLS], rhs[CELLS], right[CELLS], down[CELLS], diag[CELLS];
 };
 
-template <int T, bool UP>
+template <int T, int NT, bool UP>
 __device__ __forceinline__ void loadTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t cx0, int32_t cy0) {   // :1255-1291
-    constexpr int SPAN = AirTile<T>::SPAN, CELLS = AirTile<T>::CELLS;
+    constexpr int SPAN = AirTile<T>::SPAN, CELLS = AirTile<T>::CELLS, R = (CELLS + NT - 1) / NT;
     const int32_t dx = (int32_t)a.w[l], dy = (int32_t)a.h[l];
-    for (int k = (int)threadIdx.x; k < CELLS; k += 256) {
+    // a thread's R cells: every load made first (clamped addresses, values
+    // chosen after), so that they are in flight together
+    float4 m0[R], m1[R];
+    float pre[R], up[R];
+    bool in[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int k = (int)threadIdx.x + r * NT;
+        const int32_t x = cx0 + k % SPAN, y = cy0 + k / SPAN;
+        in[r] = k < CELLS && x >= 0 && y >= 0 && x < dx && y < dy;
+        const int32_t xc = in[r] ? x : 0, yc = in[r] ? y : 0;
+        const uint32_t at = mgAt(a, l, xc, yc);
+        m0[r] = a.mg[at];
+        m1[r] = a.mg[at + 1u];
+        if (UP) {
+            pre[r] = l == 0u ? a.cg[(uint32_t)yc * a.W + (uint32_t)xc].w : 0.0f;
+            up[r] = a.mg[mgAt(a, l + 1u, xc / 2, yc / 2)].x;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int k = (int)threadIdx.x + r * NT;
+        if (k >= CELLS) continue;
         const int32_t x = cx0 + k % SPAN, y = cy0 + k / SPAN;
         float vx = 0.0f, rhs = 0.0f, right = 0.0f, down = 0.0f, diag = -1.0f;
-        if (x >= 0 && y >= 0 && x < dx && y < dy) {
-            const uint32_t at = mgAt(a, l, x, y);
-            const float4 m0 = a.mg[at], m1 = a.mg[at + 1u];
-            rhs = m0.y;
-            right = m0.z;
-            down = m0.w;
-            if (UP) vx = l == 0u ? a.cg[(uint32_t)y * a.W + (uint32_t)x].w : m1.z;
-            if (!(m1.y > 0.0f)) {
-                if (UP) vx += CORRECT * a.mg[mgAt(a, l + 1u, x / 2, y / 2)].x;
-                diag = m0.z + m0.w + m1.x + anchorAt(a, l, x, y);
+        if (in[r]) {
+            rhs = m0[r].y;
+            right = m0[r].z;
+            down = m0[r].w;
+            if (UP) vx = l == 0u ? pre[r] : m1[r].z;
+            if (!(m1[r].y > 0.0f)) {
+                if (UP) vx += CORRECT * up[r];
+                diag = m0[r].z + m0[r].w + m1[r].x + anchorAt(a, l, x, y);
             }
         }
         t.x[k] = vx;
@@ -171,7 +204,7 @@ __device__ __forceinline__ void loadTile(const AirDev &a, AirTile<T> &t, uint32_
     __syncthreads();
     // the left and upper faces' weights into the diagonal, from the tile
     // (the outermost ring's are never read: it is not swept)
-    for (int k = (int)threadIdx.x; k < CELLS; k += 256) {
+    for (int k = (int)threadIdx.x; k < CELLS; k += NT) {
         const int tx = k % SPAN, ty = k / SPAN;
         if (tx == 0 || ty == 0 || !(t.diag[k] >= 0.0f)) continue;
         const int32_t x = cx0 + tx, y = cy0 + ty;
@@ -196,12 +229,12 @@ __device__ __forceinline__ float tileSum(const AirTile<T> &t, int k, int32_t x, 
 }
 
 // sweepTile (:1313-1326): the cells of one colour but the outermost ring
-template <int T>
+template <int T, int NT>
 __device__ __forceinline__ void sweepTile(AirTile<T> &t, int color, int32_t cx0, int32_t cy0, int32_t dx, int32_t dy) {
     constexpr int SPAN = AirTile<T>::SPAN, INNER = SPAN - 2;
     // the inner (SPAN - 2)^2 cells, half of them this colour: thread j takes
     // the j-th of the colour in row order
-    for (int j = (int)threadIdx.x; j < (INNER * INNER + 1) / 2; j += 256) {
+    for (int j = (int)threadIdx.x; j < (INNER * INNER + 1) / 2; j += NT) {
         const int q = 2 * j, row = q / INNER;
         int col = q % INNER;
         const int32_t y = cy0 + 1 + row;
@@ -260,21 +293,20 @@ __device__ void restrictCell(const AirDev &a, const AirTile<T> &t, uint32_t l, i
 }
 
 // down (restricts: the residual into level l + 1) or smoothTile (level 0
-// the coarsest: its correction into its potential)
-template <int T>
-__global__ void __launch_bounds__(256) k_airDown(AirDev a, uint32_t l, int restricts, uint32_t cycle) {
-    if (airConverged(a, cycle)) return;
-    __shared__ AirTile<T> t;
+// the coarsest: its correction into its potential), on the square at
+// (ox, oy), by NT threads
+template <int T, int NT>
+__device__ void downTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t ox, int32_t oy, int restricts) {
     constexpr int SPAN = AirTile<T>::SPAN;
-    const int32_t ox = (int32_t)blockIdx.x * T, oy = (int32_t)blockIdx.y * T, cx0 = ox - HALO, cy0 = oy - HALO;
+    const int32_t cx0 = ox - HALO, cy0 = oy - HALO;
     const int32_t dx = (int32_t)a.w[l], dy = (int32_t)a.h[l];
-    loadTile<T, false>(a, t, l, cx0, cy0);
+    loadTile<T, NT, false>(a, t, l, cx0, cy0);
     for (int i = 0; i < AIR_SMOOTH; i++) {
-        sweepTile(t, 0, cx0, cy0, dx, dy);
-        sweepTile(t, 1, cx0, cy0, dx, dy);
+        sweepTile<T, NT>(t, 0, cx0, cy0, dx, dy);
+        sweepTile<T, NT>(t, 1, cx0, cy0, dx, dy);
     }
     // storeTile (:1328-1339)
-    for (int q = (int)threadIdx.x; q < T * T; q += 256) {
+    for (int q = (int)threadIdx.x; q < T * T; q += NT) {
         const int32_t x = ox + q % T, y = oy + q / T;
         const int k = (q / T + HALO) * SPAN + q % T + HALO;
         if (x >= dx || y >= dy || !(t.diag[k] >= 0.0f)) continue;
@@ -284,35 +316,34 @@ __global__ void __launch_bounds__(256) k_airDown(AirDev a, uint32_t l, int restr
     }
     if (restricts) {
         const int32_t cw = (int32_t)a.w[l + 1u], ch = (int32_t)a.h[l + 1u];
-        for (int q = (int)threadIdx.x; q < (T / 2) * (T / 2); q += 256) {
+        for (int q = (int)threadIdx.x; q < (T / 2) * (T / 2); q += NT) {
             const int32_t cx = ox / 2 + q % (T / 2), cy = oy / 2 + q / (T / 2);
             if (cx < cw && cy < ch) restrictCell(a, t, l, cx0, cy0, cx, cy);
         }
     }
 }
 
-// (rho, on level 0: cgRho's partials of the square's 8 x 8 groups, r . z
-// with z as this pass leaves it, by the groups' tree, a wavefront a group)
-template <int T>
-__global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cycle, int rho) {
-    if (airConverged(a, cycle)) return;
-    __shared__ AirTile<T> t;
+// up, on the square at (ox, oy); with rho (level 0), cgRho's partials of
+// the square's 8 x 8 groups, r . z with z as this pass leaves it, by the
+// groups' tree, a wavefront a group
+template <int T, int NT>
+__device__ void upTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t ox, int32_t oy, int rho) {
     constexpr int SPAN = AirTile<T>::SPAN;
-    const int32_t ox = (int32_t)blockIdx.x * T, oy = (int32_t)blockIdx.y * T, cx0 = ox - HALO, cy0 = oy - HALO;
+    const int32_t cx0 = ox - HALO, cy0 = oy - HALO;
     const int32_t dx = (int32_t)a.w[l], dy = (int32_t)a.h[l];
-    loadTile<T, true>(a, t, l, cx0, cy0);
+    loadTile<T, NT, true>(a, t, l, cx0, cy0);
     for (int i = 0; i < AIR_SMOOTH; i++) {
-        sweepTile(t, 1, cx0, cy0, dx, dy);
-        sweepTile(t, 0, cx0, cy0, dx, dy);
+        sweepTile<T, NT>(t, 1, cx0, cy0, dx, dy);
+        sweepTile<T, NT>(t, 0, cx0, cy0, dx, dy);
     }
-    for (int q = (int)threadIdx.x; q < T * T; q += 256) {
+    for (int q = (int)threadIdx.x; q < T * T; q += NT) {
         const int32_t x = ox + q % T, y = oy + q / T;
         const int k = (q / T + HALO) * SPAN + q % T + HALO;
         if (x < dx && y < dy && t.diag[k] >= 0.0f) a.mg[mgAt(a, l, x, y)].x = t.x[k];
     }
     if (!rho) return;
     const uint32_t lane = threadIdx.x & 63u, gh = (a.H + 7u) / 8u;
-    for (uint32_t q = threadIdx.x / 64u; q < (uint32_t)(T / 8) * (T / 8); q += 4u) {
+    for (uint32_t q = threadIdx.x / 64u; q < (uint32_t)(T / 8) * (T / 8); q += NT / 64) {
         const uint32_t gx = (uint32_t)ox / 8u + q % (T / 8), gy = (uint32_t)oy / 8u + q / (T / 8);
         if (gx >= a.gx || gy >= gh) continue;
         const uint32_t x = gx * 8u + lane % 8u, y = gy * 8u + lane / 8u;
@@ -328,11 +359,23 @@ __global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cy
     }
 }
 
-// solveCoarsest (:1482-1535): symmetric SOR in LDS, a cell a thread
-__global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
+template <int T>
+__global__ void __launch_bounds__(256) k_airDown(AirDev a, uint32_t l, int restricts, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
-    __shared__ float4 A[AIR_COARSEST];
-    __shared__ float2 B[AIR_COARSEST];
+    __shared__ AirTile<T> t;
+    downTile<T, 256>(a, t, l, (int32_t)blockIdx.x * T, (int32_t)blockIdx.y * T, restricts);
+}
+
+template <int T>
+__global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cycle, int rho) {
+    if (airConverged(a, cycle)) return;
+    __shared__ AirTile<T> t;
+    upTile<T, 256>(a, t, l, (int32_t)blockIdx.x * T, (int32_t)blockIdx.y * T, rho);
+}
+
+// solveCoarsest (:1482-1535): symmetric SOR in LDS, a cell a thread (the
+// first 256 of the workgroup)
+__device__ void coarsestBody(const AirDev &a, float4 *A, float2 *B) {
     const uint32_t l = a.L - 1u, dx = a.w[l], dy = a.h[l], n = min(dx * dy, (uint32_t)AIR_COARSEST);
     const uint32_t j = threadIdx.x;
     if (j < n) {
@@ -367,6 +410,39 @@ __global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
         __syncthreads();
     }
     if (j < n) a.mg[a.base[l] + 2u * j].x = A[j].x;
+}
+__global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    __shared__ float4 A[AIR_COARSEST];
+    __shared__ float2 B[AIR_COARSEST];
+    coarsestBody(a, A, B);
+}
+
+// The levels from l0 (the first that fits a T x T square) to the coarsest
+// and back, by one workgroup: each level's down as one square, the
+// coarsest's SOR, each level's up; a level's results go to memory before
+// the next phase reads them, as between the separate passes.  LDS
+// (dynamic, COARSE_FUSED_BYTES): the tile, then the coarsest's cells.
+#define COARSE_FUSED_T 64
+#define COARSE_FUSED_NT 1024
+#define COARSE_FUSED_BYTES (sizeof(AirTile<COARSE_FUSED_T>) + AIR_COARSEST * (sizeof(float4) + sizeof(float2)))
+__global__ void __launch_bounds__(COARSE_FUSED_NT) k_airCoarse(AirDev a, uint32_t l0, uint32_t cycle) {
+    if (airConverged(a, cycle)) return;
+    extern __shared__ float4 smem[];
+    AirTile<COARSE_FUSED_T> &t = *(AirTile<COARSE_FUSED_T> *)smem;
+    float4 *A = (float4 *)((char *)smem + sizeof(AirTile<COARSE_FUSED_T>));
+    float2 *B = (float2 *)(A + AIR_COARSEST);
+    const uint32_t deepest = a.L - 1u;
+    for (uint32_t l = l0; l < deepest; l++) {
+        downTile<COARSE_FUSED_T, COARSE_FUSED_NT>(a, t, l, 0, 0, 1);
+        __syncthreads();
+    }
+    coarsestBody(a, A, B);
+    __syncthreads();
+    for (int32_t l = (int32_t)deepest - 1; l >= (int32_t)l0; l--) {
+        upTile<COARSE_FUSED_T, COARSE_FUSED_NT>(a, t, (uint32_t)l, 0, 0, l == 0 ? 1 : 0);
+        __syncthreads();
+    }
 }
 
 // cgRho (:1099-1112) and cgApply (:1141-1157): a wavefront a group of 8 x 8
@@ -487,7 +563,7 @@ __global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
             most = max(most, __float_as_uint(fabsf(r)));
         }
     }
-    slotMax(&a.status[1u + cycle % 3u], most);
+    groupMax(a.most, most);
 }
 
 // ------------------------------------------------------------ the C ABI
@@ -499,6 +575,7 @@ struct WigAir {
     uint64_t mgBytes, cgBytes;
     hipStream_t stream = nullptr, own = nullptr;
     uint32_t tile[AIR_LEVELS] = {32, 16, 16, 16, 16, 16, 16, 16, 16};
+    int coarse = 0;        // the levels that fit COARSE_FUSED_T in one kernel
     int timing = 0;
     double ms = 0.0;
     uint32_t calls = 0;
@@ -554,7 +631,11 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     if (e == hipSuccess) e = hipMalloc(&cg, a->cgBytes);
     if (e == hipSuccess) e = hipMalloc(&st, 32);
     if (e == hipSuccess) e = hipMalloc(&mean, 4ull * width * height);
+    void *most = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&most, 4ull * CELL_GROUPS);
     if (e == hipSuccess) e = hipStreamCreate(&a->own);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_airCoarse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)COARSE_FUSED_BYTES);
     if (e == hipSuccess) e = hipMemsetAsync(mg, 0, a->mgBytes, a->own);
     if (e == hipSuccess) e = hipMemsetAsync(cg, 0, a->cgBytes, a->own);
     if (e == hipSuccess) e = hipMemsetAsync(st, 0, 32, a->own);
@@ -566,6 +647,7 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
         if (cg) (void)hipFree(cg);
         if (st) (void)hipFree(st);
         if (mean) (void)hipFree(mean);
+        if (most) (void)hipFree(most);
         if (a->own) (void)hipStreamDestroy(a->own);
         delete a;
         return nullptr;
@@ -574,6 +656,7 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     d.cg = (float4 *)cg;
     d.status = (uint32_t *)st;
     d.mean = (const float *)mean;
+    d.most = (uint32_t *)most;
     a->stream = a->own;
     return a;
 }
@@ -585,6 +668,7 @@ extern "C" int wig_air_destroy(WigAir *a) {
     (void)hipFree(a->d.cg);
     (void)hipFree(a->d.status);
     (void)hipFree((void *)a->d.mean);
+    (void)hipFree(a->d.most);
     if (a->t0) (void)hipEventDestroy(a->t0);
     if (a->t1) (void)hipEventDestroy(a->t1);
     if (a->own) (void)hipStreamDestroy(a->own);
@@ -654,6 +738,12 @@ extern "C" int wig_air_set_stream(WigAir *a, void *stream) {
     return WIG_OK;
 }
 
+extern "C" int wig_air_set_coarse(WigAir *a, int on) {
+    if (!a) return wig_fail_message(WIG_E_INVALID, "wig_air_set_coarse: NULL");
+    a->coarse = on ? 1 : 0;
+    return WIG_OK;
+}
+
 extern "C" int wig_air_set_tile(WigAir *a, uint32_t level, uint32_t side) {
     if (!a || (side != 16 && side != 32) || level >= a->d.L)
         return wig_fail_message(WIG_E_INVALID, "wig_air_set_tile: a level of the grid, side 16 or 32");
@@ -685,19 +775,35 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         if (a->tile[l] == 32) launchLevel<32>(a, up, l, restricts, k);
         else launchLevel<16>(a, up, l, restricts, k);
     };
+    // the first level that fits one square, if the fused kernel takes it
+    uint32_t l0 = deepest;
+    if (a->coarse && deepest > 0)
+        for (uint32_t l = 0; l < deepest; l++)
+            if (d.w[l] <= COARSE_FUSED_T && d.h[l] <= COARSE_FUSED_T) {
+                l0 = l;
+                break;
+            }
     k_airClear<<<1, 64, 0, a->stream>>>(d);
     k_airStart<<<cellGrid, 256, 0, a->stream>>>(d);
+    k_airSlot<<<1, 1024, 0, a->stream>>>(d, 3u, cellGrid, 0u, 1);
     for (uint32_t k = 0; k < cycles; k++) {
-        for (uint32_t l = 0; l < deepest; l++) level(false, l, 1, k);
-        if (deepest == 0) level(false, 0, 0, k);
-        k_airCoarsest<<<1, 256, 0, a->stream>>>(d, k);
-        for (int32_t l = (int32_t)deepest - 1; l >= 0; l--) level(true, (uint32_t)l, 0, k);
+        if (l0 < deepest) {
+            for (uint32_t l = 0; l < l0; l++) level(false, l, 1, k);
+            k_airCoarse<<<1, COARSE_FUSED_NT, COARSE_FUSED_BYTES, a->stream>>>(d, l0, k);
+            for (int32_t l = (int32_t)l0 - 1; l >= 0; l--) level(true, (uint32_t)l, 0, k);
+        } else {
+            for (uint32_t l = 0; l < deepest; l++) level(false, l, 1, k);
+            if (deepest == 0) level(false, 0, 0, k);
+            k_airCoarsest<<<1, 256, 0, a->stream>>>(d, k);
+            for (int32_t l = (int32_t)deepest - 1; l >= 0; l--) level(true, (uint32_t)l, 0, k);
+        }
         if (deepest == 0) k_airRho<<<groupBlocks, 256, 0, a->stream>>>(d, k);   // (else in level 0's up)
         k_airBeta<<<1, 256, 0, a->stream>>>(d, k);
         k_airDirection<<<cellBlocks, 256, 0, a->stream>>>(d, k);
         k_airApply<<<groupBlocks, 256, 0, a->stream>>>(d, k);
         k_airAlpha<<<1, 256, 0, a->stream>>>(d, k);
         k_airUpdate<<<cellGrid, 256, 0, a->stream>>>(d, k);
+        k_airSlot<<<1, 1024, 0, a->stream>>>(d, 1u + k % 3u, cellGrid, k, 0);
     }
     AIRCHECK(hipGetLastError());
     if (a->timing) {
