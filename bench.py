@@ -148,8 +148,6 @@ def main():
         return run_precision(args)
     if args.workload == "snapshot":
         return run_snapshot(args)
-    if args.workload == "air":
-        return run_air(args)
 
     if args.copy_mib:
         src = torch.empty(args.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -158,6 +156,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload == "air":
+        return run_air(args)
 
     size = args.size
     cells, fluid_cells, kinds = build_world(size)
