@@ -128,3 +128,33 @@ def test_js_snapshot_continues_in_python_host(tmp_path):
     assert hb["state"]["frameNo"] == sim.frame_no
     d = compare_worlds(sim.world, end.world)
     assert not d, "; ".join(d)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(NODE is None, reason="no node")
+def test_js_air_projection_matches_oracle(tmp_path):
+    # the air's projection from the JavaScript host (worldinagrain_amd/js/
+    # air.js over the addon): three steps with its airCycles, every buffer
+    # bit-identical to the oracle's stepped with the same counts
+    from oracle.air import OracleAir
+    from worldinagrain_amd import scenes
+    from worldinagrain_amd.air import AirProjection
+    W, H, steps = 120, 88, 3
+    mg0, x0 = scenes.air_rooms(W, H, seed=13)
+    o = OracleAir(W, H)
+    o.load(mg0, x0)
+    for name, a in (("mg", o.mg), ("cg", o.cg), ("status", o.status), ("mean", o.mean)):
+        a.tofile(str(tmp_path / f"{name}.bin"))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "air_steps.js"), str(W), str(H), str(steps), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    p = AirProjection(o)
+    its = []
+    for s in range(steps):
+        if s > 0:
+            o.mg[:8 * W * H] = mg0.reshape(-1)
+        its.append(p.step())
+    assert json.load(open(tmp_path / "its.json")) == its
+    for name, a in (("mg", o.mg), ("cg", o.cg), ("status", o.status)):
+        got = np.fromfile(str(tmp_path / f"{name}.out"), dtype=a.dtype)
+        assert np.array_equal(got.view(np.uint32), a.view(np.uint32)), name

@@ -237,6 +237,118 @@ static napi_value js_device_ok(napi_env env, napi_callback_info info) {
     return r;
 }
 
+/* ---- the air's projection (wig_air_*) ---- */
+static void finalize_air(napi_env env, void *data, void *hint) {
+    (void)env;
+    (void)hint;
+    WigAir **slot = (WigAir **)data;
+    if (*slot) wig_air_destroy(*slot);
+    *slot = NULL;
+    free(slot);
+}
+
+static WigAir *air_of(napi_env env, napi_value v) {
+    void *data = NULL;
+    if (napi_get_value_external(env, v, &data) != napi_ok || !data || !*(WigAir **)data) {
+        napi_throw_type_error(env, NULL, "not a live air handle");
+        return NULL;
+    }
+    return *(WigAir **)data;
+}
+
+/* airCreate(width, height) -> handle */
+static napi_value js_air_create(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc < 2) { napi_throw_type_error(env, NULL, "airCreate(width, height)"); return NULL; }
+    WigAir *a = wig_air_create(arg_u32(env, argv[0]), arg_u32(env, argv[1]));
+    if (!a) return throw_wig(env, WIG_E_INVALID);
+    WigAir **slot = (WigAir **)malloc(sizeof *slot);
+    *slot = a;
+    napi_value h;
+    CALL(env, napi_create_external(env, slot, finalize_air, NULL, &h));
+    return h;
+}
+
+static napi_value js_air_destroy(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    void *data = NULL;
+    CALL(env, napi_get_value_external(env, argv[0], &data));
+    WigAir **slot = (WigAir **)data;
+    if (*slot) wig_air_destroy(*slot);
+    *slot = NULL;
+    return NULL;
+}
+
+/* airLayout(h) -> {width, height, levels, groups, levelWidth, levelHeight, partBytes} */
+static napi_value js_air_layout(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1], o, v, arr;
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    WigAir *a = air_of(env, argv[0]);
+    if (!a) return NULL;
+    WigAirLayout L;
+    int st = wig_air_layout(a, &L);
+    if (st < 0) return throw_wig(env, st);
+    CALL(env, napi_create_object(env, &o));
+#define FIELD(name, val) CALL(env, napi_create_uint32(env, (val), &v)); CALL(env, napi_set_named_property(env, o, name, v))
+    FIELD("width", L.width);
+    FIELD("height", L.height);
+    FIELD("levels", L.levels);
+    FIELD("groups", L.groups);
+#undef FIELD
+#define ARRAY(name, n, get) \
+    CALL(env, napi_create_array_with_length(env, (n), &arr)); \
+    for (uint32_t k = 0; k < (n); k++) { CALL(env, napi_create_double(env, (double)(get), &v)); CALL(env, napi_set_element(env, arr, k, v)); } \
+    CALL(env, napi_set_named_property(env, o, name, arr))
+    ARRAY("levelWidth", L.levels, L.level_width[k]);
+    ARRAY("levelHeight", L.levels, L.level_height[k]);
+    ARRAY("partBytes", WIG_AIR_PART_COUNT, L.part_bytes[k]);
+#undef ARRAY
+    return o;
+}
+
+/* airUpload(h, part, bytes) / airDownload(h, part, bytes) */
+static napi_value js_air_copy(napi_env env, napi_callback_info info, int up) {
+    size_t argc = 3;
+    napi_value argv[3];
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    WigAir *a = air_of(env, argv[0]);
+    void *data;
+    size_t len;
+    if (!a) return NULL;
+    if (!bytes_of(env, argv[2], &data, &len)) { napi_throw_type_error(env, NULL, "an ArrayBuffer or typed array"); return NULL; }
+    int part = (int)arg_u32(env, argv[1]);
+    int st = up ? wig_air_upload(a, part, data, len) : wig_air_download(a, part, data, len);
+    return st < 0 ? throw_wig(env, st) : NULL;
+}
+static napi_value js_air_upload(napi_env env, napi_callback_info info) { return js_air_copy(env, info, 1); }
+static napi_value js_air_download(napi_env env, napi_callback_info info) { return js_air_copy(env, info, 0); }
+
+/* airProject(h, cycles): enqueued */
+static napi_value js_air_project(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    WigAir *a = air_of(env, argv[0]);
+    if (!a) return NULL;
+    int st = wig_air_project(a, argc > 1 ? arg_u32(env, argv[1]) : 64u);
+    return st < 0 ? throw_wig(env, st) : NULL;
+}
+
+static napi_value js_air_sync(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1];
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    WigAir *a = air_of(env, argv[0]);
+    if (!a) return NULL;
+    int st = wig_air_sync(a);
+    return st < 0 ? throw_wig(env, st) : NULL;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor props[] = {
         {"create", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
@@ -250,6 +362,13 @@ static napi_value init(napi_env env, napi_value exports) {
         {"solveStats", NULL, js_solve_stats, NULL, NULL, NULL, napi_default, NULL},
         {"sync", NULL, js_sync, NULL, NULL, NULL, napi_default, NULL},
         {"deviceOk", NULL, js_device_ok, NULL, NULL, NULL, napi_default, NULL},
+        {"airCreate", NULL, js_air_create, NULL, NULL, NULL, napi_default, NULL},
+        {"airDestroy", NULL, js_air_destroy, NULL, NULL, NULL, napi_default, NULL},
+        {"airLayout", NULL, js_air_layout, NULL, NULL, NULL, napi_default, NULL},
+        {"airUpload", NULL, js_air_upload, NULL, NULL, NULL, napi_default, NULL},
+        {"airDownload", NULL, js_air_download, NULL, NULL, NULL, napi_default, NULL},
+        {"airProject", NULL, js_air_project, NULL, NULL, NULL, napi_default, NULL},
+        {"airSync", NULL, js_air_sync, NULL, NULL, NULL, napi_default, NULL},
     };
     if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return NULL;
     return exports;
