@@ -36,6 +36,8 @@ struct AirDev {
     float4 *cg;
     uint32_t *status;
     uint32_t *most;     // a level-0 pass's most |r| bits, a workgroup each (this library's)
+    float *pre0;        // level 0's smoothed correction, down to up (this library's; none where held)
+    float *pnew;        // cgDirection's p, cgApply to cgUpdate (this library's)
     const float *mean;
     uint32_t L, W, H, gx, groups;
     uint32_t w[AIR_LEVELS], h[AIR_LEVELS];
@@ -175,7 +177,7 @@ __device__ __forceinline__ void loadTile(const AirDev &a, AirTile<T> &t, uint32_
         m0[r] = a.mg[at];
         m1[r] = a.mg[at + 1u];
         if (UP) {
-            pre[r] = l == 0u ? a.cg[(uint32_t)yc * a.W + (uint32_t)xc].w : 0.0f;
+            pre[r] = l == 0u ? a.pre0[(uint32_t)yc * a.W + (uint32_t)xc] : 0.0f;
             up[r] = a.mg[mgAt(a, l + 1u, xc / 2, yc / 2)].x;
         }
     }
@@ -309,10 +311,13 @@ __device__ void downTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t ox,
     for (int q = (int)threadIdx.x; q < T * T; q += NT) {
         const int32_t x = ox + q % T, y = oy + q / T;
         const int k = (q / T + HALO) * SPAN + q % T + HALO;
-        if (x >= dx || y >= dy || !(t.diag[k] >= 0.0f)) continue;
+        if (x >= dx || y >= dy) continue;
+        // level 0's into an array of its own, held cells' none (the reference
+        // keeps it in cg's A p word, none there: cgStart's, cgApply's)
+        if (restricts && l == 0u) a.pre0[(uint32_t)y * a.W + (uint32_t)x] = t.x[k];
+        if (!(t.diag[k] >= 0.0f)) continue;
         if (!restricts) a.mg[mgAt(a, l, x, y)].x = t.x[k];
-        else if (l == 0u) a.cg[(uint32_t)y * a.W + (uint32_t)x].w = t.x[k];
-        else a.mg[mgAt(a, l, x, y) + 1u].z = t.x[k];
+        else if (l != 0u) a.mg[mgAt(a, l, x, y) + 1u].z = t.x[k];
     }
     if (restricts) {
         const int32_t cw = (int32_t)a.w[l + 1u], ch = (int32_t)a.h[l + 1u];
@@ -457,6 +462,14 @@ __global__ void __launch_bounds__(256) k_airRho(AirDev a, uint32_t cycle) {
     v = groupTree(v);
     if (lane == 0u) sums(a)[4u + g] = v;
 }
+// cgDirection (:1128-1139) and cgApply (:1141-1157) in one pass: each cell's
+// p = z + beta p (none where held) from its old p, its neighbours' likewise
+// (the same operands, so the same bits), A p as applyA has it; p into an
+// array of its own (the neighbours still read the old one), which cgUpdate
+// writes into cg's p
+__device__ __forceinline__ float newP(float4 m0, float4 m1, float pold, float beta) {
+    return m1.y > 0.0f ? 0.0f : m0.x + beta * pold;
+}
 __global__ void __launch_bounds__(256) k_airApply(AirDev a, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
     const uint32_t g = blockIdx.x * 4u + threadIdx.x / 64u, lane = threadIdx.x & 63u;
@@ -464,11 +477,39 @@ __global__ void __launch_bounds__(256) k_airApply(AirDev a, uint32_t cycle) {
     const uint32_t x = (g % a.gx) * 8u + lane % 8u, y = (g / a.gx) * 8u + lane / 8u;
     float v = 0.0f;
     if (x < a.W && y < a.H) {
-        const uint32_t i = y * a.W + x, at = mgAt(a, 0u, (int32_t)x, (int32_t)y);
+        const float beta = sums(a)[2];
+        const int32_t xi = (int32_t)x, yi = (int32_t)y;
+        const uint32_t i = y * a.W + x, at = mgAt(a, 0u, xi, yi);
         const float4 m0 = a.mg[at], m1 = a.mg[at + 1u];
-        const float q = m1.y > 0.0f ? 0.0f : applyA<2>(a, (int32_t)x, (int32_t)y, m0, m1);
+        const bool r_ = x + 1u < a.W, d_ = y + 1u < a.H, l_ = x > 0u, u_ = y > 0u;
+        // every load first (clamped to the cell itself where off the grid)
+        const uint32_t ir = r_ ? i + 1u : i, id = d_ ? i + a.W : i, il = l_ ? i - 1u : i, iu = u_ ? i - a.W : i;
+        const float4 rm0 = a.mg[mgAt(a, 0u, r_ ? xi + 1 : xi, yi)], rm1 = a.mg[mgAt(a, 0u, r_ ? xi + 1 : xi, yi) + 1u];
+        const float4 dm0 = a.mg[mgAt(a, 0u, xi, d_ ? yi + 1 : yi)], dm1 = a.mg[mgAt(a, 0u, xi, d_ ? yi + 1 : yi) + 1u];
+        const float4 lm0 = a.mg[mgAt(a, 0u, l_ ? xi - 1 : xi, yi)], lm1 = a.mg[mgAt(a, 0u, l_ ? xi - 1 : xi, yi) + 1u];
+        const float4 um0 = a.mg[mgAt(a, 0u, xi, u_ ? yi - 1 : yi)], um1 = a.mg[mgAt(a, 0u, xi, u_ ? yi - 1 : yi) + 1u];
+        const float pc = newP(m0, m1, a.cg[i].z, beta);
+        const float pr = newP(rm0, rm1, a.cg[ir].z, beta), pd = newP(dm0, dm1, a.cg[id].z, beta);
+        const float pl = newP(lm0, lm1, a.cg[il].z, beta), pu = newP(um0, um1, a.cg[iu].z, beta);
+        float q = 0.0f;
+        if (!(m1.y > 0.0f)) {
+            float diag = m0.z + m0.w + m1.x + anchorAt(a, 0u, xi, yi);
+            float sum = 0.0f;
+            if (r_) sum += m0.z * pr;
+            if (d_) sum += m0.w * pd;
+            if (l_) {
+                sum += lm0.z * pl;
+                diag += lm0.z;
+            }
+            if (u_) {
+                sum += um0.w * pu;
+                diag += um0.w;
+            }
+            q = diag * pc - sum;
+        }
+        a.pnew[i] = pc;
         a.cg[i].w = q;
-        v = a.cg[i].z * q;
+        v = pc * q;
     }
     v = groupTree(v);
     if (lane == 0u) sums(a)[4u + g] = v;
@@ -523,17 +564,6 @@ __global__ void __launch_bounds__(256) k_airAlpha(AirDev a, uint32_t cycle) {
     }
 }
 
-// cgDirection (:1128-1139)
-__global__ void __launch_bounds__(256) k_airDirection(AirDev a, uint32_t cycle) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.W * a.H || airConverged(a, cycle)) return;
-    const int32_t x = (int32_t)(i % a.W), y = (int32_t)(i / a.W);
-    const uint32_t at = mgAt(a, 0u, x, y);
-    const float beta = sums(a)[2];
-    const float p = a.mg[at + 1u].y > 0.0f ? 0.0f : a.mg[at].x + beta * a.cg[i].z;
-    a.cg[i].z = p;
-}
-
 // cgUpdate (:1172-1194)
 __global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
     const bool conv = airConverged(a, cycle);
@@ -557,8 +587,9 @@ __global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
         for (int j = 0; j < UPDATE_HELD; j++) {
             const uint32_t i = i0 + (uint32_t)j * stride;
             if (i >= n) continue;
+            const float p = a.pnew[i];   // (cgDirection's, from cgApply)
             const float r = v[j].y - alpha * v[j].w;
-            a.cg[i] = make_float4(v[j].x + alpha * v[j].z, r, v[j].z, v[j].w);
+            a.cg[i] = make_float4(v[j].x + alpha * p, r, p, v[j].w);
             *(float2 *)&a.mg[mgAt(a, 0u, (int32_t)(i % a.W), (int32_t)(i / a.W))] = make_float2(0.0f, -r);
             most = max(most, __float_as_uint(fabsf(r)));
         }
@@ -631,9 +662,11 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     if (e == hipSuccess) e = hipMalloc(&cg, a->cgBytes);
     if (e == hipSuccess) e = hipMalloc(&st, 32);
     if (e == hipSuccess) e = hipMalloc(&mean, 4ull * width * height);
-    void *most = nullptr;
+    void *most = nullptr, *work = nullptr;
     if (e == hipSuccess) e = hipMalloc(&most, 4ull * CELL_GROUPS);
+    if (e == hipSuccess) e = hipMalloc(&work, 8ull * width * height);
     if (e == hipSuccess) e = hipStreamCreate(&a->own);
+    if (e == hipSuccess) e = hipMemsetAsync(work, 0, 8ull * width * height, a->own);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void *)k_airCoarse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)COARSE_FUSED_BYTES);
     if (e == hipSuccess) e = hipMemsetAsync(mg, 0, a->mgBytes, a->own);
@@ -648,6 +681,7 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
         if (st) (void)hipFree(st);
         if (mean) (void)hipFree(mean);
         if (most) (void)hipFree(most);
+        if (work) (void)hipFree(work);
         if (a->own) (void)hipStreamDestroy(a->own);
         delete a;
         return nullptr;
@@ -657,6 +691,8 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     d.status = (uint32_t *)st;
     d.mean = (const float *)mean;
     d.most = (uint32_t *)most;
+    d.pre0 = (float *)work;
+    d.pnew = (float *)work + (uint64_t)width * height;
     a->stream = a->own;
     return a;
 }
@@ -669,6 +705,7 @@ extern "C" int wig_air_destroy(WigAir *a) {
     (void)hipFree(a->d.status);
     (void)hipFree((void *)a->d.mean);
     (void)hipFree(a->d.most);
+    (void)hipFree(a->d.pre0);
     if (a->t0) (void)hipEventDestroy(a->t0);
     if (a->t1) (void)hipEventDestroy(a->t1);
     if (a->own) (void)hipStreamDestroy(a->own);
@@ -799,7 +836,6 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         }
         if (deepest == 0) k_airRho<<<groupBlocks, 256, 0, a->stream>>>(d, k);   // (else in level 0's up)
         k_airBeta<<<1, 256, 0, a->stream>>>(d, k);
-        k_airDirection<<<cellBlocks, 256, 0, a->stream>>>(d, k);
         k_airApply<<<groupBlocks, 256, 0, a->stream>>>(d, k);
         k_airAlpha<<<1, 256, 0, a->stream>>>(d, k);
         k_airUpdate<<<cellGrid, 256, 0, a->stream>>>(d, k);
