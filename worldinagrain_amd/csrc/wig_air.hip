@@ -38,6 +38,13 @@ struct AirDev {
     uint32_t *most;     // a level-0 pass's most |r| bits, a workgroup each (this library's)
     float *pre0;        // level 0's smoothed correction, down to up (this library's; none where held)
     float *pnew;        // cgDirection's p, cgApply to cgUpdate (this library's)
+    // Level 0 during a solve (this library's): its record's words the solve
+    // does not change (right and bottom weights, edge, fixed), its rhs (-r)
+    // and the V-cycle's answer z.  The records' potential and rhs are
+    // written once, at the end (k_airFinish), but on a grid of one level,
+    // whose coarsest solve reads them every iteration.
+    float4 *w0;
+    float *rhs0, *z0;
     const float *mean;
     uint32_t L, W, H, gx, groups;
     uint32_t w[AIR_LEVELS], h[AIR_LEVELS];
@@ -145,6 +152,9 @@ __global__ void __launch_bounds__(256) k_airStart(AirDev a) {
         a.cg[i] = make_float4(px, r, 0.0f, 0.0f);
         a.mg[at].x = 0.0f;
         a.mg[at].y = -r;
+        a.w0[i] = make_float4(m0.z, m0.w, m1.x, m1.y);
+        a.rhs0[i] = -r;
+        a.z0[i] = 0.0f;
         most = max(most, __float_as_uint(fabsf(r)));
     }
     groupMax(a.most, most);
@@ -173,9 +183,16 @@ __device__ __forceinline__ void loadTile(const AirDev &a, AirTile<T> &t, uint32_
         const int32_t x = cx0 + k % SPAN, y = cy0 + k / SPAN;
         in[r] = k < CELLS && x >= 0 && y >= 0 && x < dx && y < dy;
         const int32_t xc = in[r] ? x : 0, yc = in[r] ? y : 0;
-        const uint32_t at = mgAt(a, l, xc, yc);
-        m0[r] = a.mg[at];
-        m1[r] = a.mg[at + 1u];
+        if (l == 0u) {
+            const uint32_t i = (uint32_t)yc * a.W + (uint32_t)xc;
+            const float4 w = a.w0[i];
+            m0[r] = make_float4(0.0f, a.rhs0[i], w.x, w.y);
+            m1[r] = make_float4(w.z, w.w, 0.0f, 0.0f);
+        } else {
+            const uint32_t at = mgAt(a, l, xc, yc);
+            m0[r] = a.mg[at];
+            m1[r] = a.mg[at + 1u];
+        }
         if (UP) {
             pre[r] = l == 0u ? a.pre0[(uint32_t)yc * a.W + (uint32_t)xc] : 0.0f;
             up[r] = a.mg[mgAt(a, l + 1u, xc / 2, yc / 2)].x;
@@ -344,7 +361,10 @@ __device__ void upTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t ox, i
     for (int q = (int)threadIdx.x; q < T * T; q += NT) {
         const int32_t x = ox + q % T, y = oy + q / T;
         const int k = (q / T + HALO) * SPAN + q % T + HALO;
-        if (x < dx && y < dy && t.diag[k] >= 0.0f) a.mg[mgAt(a, l, x, y)].x = t.x[k];
+        if (x < dx && y < dy && t.diag[k] >= 0.0f) {
+            if (l == 0u) a.z0[(uint32_t)y * a.W + (uint32_t)x] = t.x[k];
+            else a.mg[mgAt(a, l, x, y)].x = t.x[k];
+        }
     }
     if (!rho) return;
     const uint32_t lane = threadIdx.x & 63u, gh = (a.H + 7u) / 8u;
@@ -356,7 +376,7 @@ __device__ void upTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t ox, i
         if (x < a.W && y < a.H) {
             const int k = ((int32_t)y - cy0) * SPAN + ((int32_t)x - cx0);
             // a held cell keeps the potential it had (none: cgUpdate's, cgStart's)
-            const float z = t.diag[k] >= 0.0f ? t.x[k] : a.mg[mgAt(a, 0u, (int32_t)x, (int32_t)y)].x;
+            const float z = t.diag[k] >= 0.0f ? t.x[k] : a.z0[y * a.W + x];
             v = a.cg[y * a.W + x].y * z;
         }
         v = groupTree(v);
@@ -414,7 +434,10 @@ __device__ void coarsestBody(const AirDev &a, float4 *A, float2 *B) {
         }
         __syncthreads();
     }
-    if (j < n) a.mg[a.base[l] + 2u * j].x = A[j].x;
+    if (j < n) {
+        a.mg[a.base[l] + 2u * j].x = A[j].x;
+        if (l == 0u) a.z0[j] = A[j].x;
+    }
 }
 __global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
@@ -467,8 +490,8 @@ __global__ void __launch_bounds__(256) k_airRho(AirDev a, uint32_t cycle) {
 // (the same operands, so the same bits), A p as applyA has it; p into an
 // array of its own (the neighbours still read the old one), which cgUpdate
 // writes into cg's p
-__device__ __forceinline__ float newP(float4 m0, float4 m1, float pold, float beta) {
-    return m1.y > 0.0f ? 0.0f : m0.x + beta * pold;
+__device__ __forceinline__ float newP(float z, float4 w, float pold, float beta) {
+    return w.w > 0.0f ? 0.0f : z + beta * pold;
 }
 __global__ void __launch_bounds__(256) k_airApply(AirDev a, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
@@ -479,31 +502,28 @@ __global__ void __launch_bounds__(256) k_airApply(AirDev a, uint32_t cycle) {
     if (x < a.W && y < a.H) {
         const float beta = sums(a)[2];
         const int32_t xi = (int32_t)x, yi = (int32_t)y;
-        const uint32_t i = y * a.W + x, at = mgAt(a, 0u, xi, yi);
-        const float4 m0 = a.mg[at], m1 = a.mg[at + 1u];
+        const uint32_t i = y * a.W + x;
         const bool r_ = x + 1u < a.W, d_ = y + 1u < a.H, l_ = x > 0u, u_ = y > 0u;
-        // every load first (clamped to the cell itself where off the grid)
+        // every load first (clamped to the cell itself where off the grid);
+        // w: the packed level-0 words (right, bottom, edge, fixed)
         const uint32_t ir = r_ ? i + 1u : i, id = d_ ? i + a.W : i, il = l_ ? i - 1u : i, iu = u_ ? i - a.W : i;
-        const float4 rm0 = a.mg[mgAt(a, 0u, r_ ? xi + 1 : xi, yi)], rm1 = a.mg[mgAt(a, 0u, r_ ? xi + 1 : xi, yi) + 1u];
-        const float4 dm0 = a.mg[mgAt(a, 0u, xi, d_ ? yi + 1 : yi)], dm1 = a.mg[mgAt(a, 0u, xi, d_ ? yi + 1 : yi) + 1u];
-        const float4 lm0 = a.mg[mgAt(a, 0u, l_ ? xi - 1 : xi, yi)], lm1 = a.mg[mgAt(a, 0u, l_ ? xi - 1 : xi, yi) + 1u];
-        const float4 um0 = a.mg[mgAt(a, 0u, xi, u_ ? yi - 1 : yi)], um1 = a.mg[mgAt(a, 0u, xi, u_ ? yi - 1 : yi) + 1u];
-        const float pc = newP(m0, m1, a.cg[i].z, beta);
-        const float pr = newP(rm0, rm1, a.cg[ir].z, beta), pd = newP(dm0, dm1, a.cg[id].z, beta);
-        const float pl = newP(lm0, lm1, a.cg[il].z, beta), pu = newP(um0, um1, a.cg[iu].z, beta);
+        const float4 w = a.w0[i], wr = a.w0[ir], wd = a.w0[id], wl = a.w0[il], wu = a.w0[iu];
+        const float pc = newP(a.z0[i], w, a.cg[i].z, beta);
+        const float pr = newP(a.z0[ir], wr, a.cg[ir].z, beta), pd = newP(a.z0[id], wd, a.cg[id].z, beta);
+        const float pl = newP(a.z0[il], wl, a.cg[il].z, beta), pu = newP(a.z0[iu], wu, a.cg[iu].z, beta);
         float q = 0.0f;
-        if (!(m1.y > 0.0f)) {
-            float diag = m0.z + m0.w + m1.x + anchorAt(a, 0u, xi, yi);
+        if (!(w.w > 0.0f)) {
+            float diag = w.x + w.y + w.z + anchorAt(a, 0u, xi, yi);
             float sum = 0.0f;
-            if (r_) sum += m0.z * pr;
-            if (d_) sum += m0.w * pd;
+            if (r_) sum += w.x * pr;
+            if (d_) sum += w.y * pd;
             if (l_) {
-                sum += lm0.z * pl;
-                diag += lm0.z;
+                sum += wl.x * pl;
+                diag += wl.x;
             }
             if (u_) {
-                sum += um0.w * pu;
-                diag += um0.w;
+                sum += wu.y * pu;
+                diag += wu.y;
             }
             q = diag * pc - sum;
         }
@@ -590,11 +610,20 @@ __global__ void __launch_bounds__(256) k_airUpdate(AirDev a, uint32_t cycle) {
             const float p = a.pnew[i];   // (cgDirection's, from cgApply)
             const float r = v[j].y - alpha * v[j].w;
             a.cg[i] = make_float4(v[j].x + alpha * p, r, p, v[j].w);
-            *(float2 *)&a.mg[mgAt(a, 0u, (int32_t)(i % a.W), (int32_t)(i / a.W))] = make_float2(0.0f, -r);
+            a.rhs0[i] = -r;
+            if (a.L == 1u) *(float2 *)&a.mg[mgAt(a, 0u, (int32_t)(i % a.W), (int32_t)(i / a.W))] = make_float2(0.0f, -r);
             most = max(most, __float_as_uint(fabsf(r)));
         }
     }
     groupMax(a.most, most);
+}
+
+// The level-0 records' potential and rhs as the last cgUpdate (or cgStart)
+// left them: none, and -r
+__global__ void __launch_bounds__(256) k_airFinish(AirDev a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.W * a.H || a.L == 1u) return;
+    *(float2 *)&a.mg[mgAt(a, 0u, (int32_t)(i % a.W), (int32_t)(i / a.W))] = make_float2(0.0f, -a.cg[i].y);
 }
 
 // ------------------------------------------------------------ the C ABI
@@ -664,9 +693,9 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     if (e == hipSuccess) e = hipMalloc(&mean, 4ull * width * height);
     void *most = nullptr, *work = nullptr;
     if (e == hipSuccess) e = hipMalloc(&most, 4ull * CELL_GROUPS);
-    if (e == hipSuccess) e = hipMalloc(&work, 8ull * width * height);
+    if (e == hipSuccess) e = hipMalloc(&work, 32ull * width * height);
     if (e == hipSuccess) e = hipStreamCreate(&a->own);
-    if (e == hipSuccess) e = hipMemsetAsync(work, 0, 8ull * width * height, a->own);
+    if (e == hipSuccess) e = hipMemsetAsync(work, 0, 32ull * width * height, a->own);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void *)k_airCoarse, hipFuncAttributeMaxDynamicSharedMemorySize, (int)COARSE_FUSED_BYTES);
     if (e == hipSuccess) e = hipMemsetAsync(mg, 0, a->mgBytes, a->own);
@@ -691,8 +720,12 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     d.status = (uint32_t *)st;
     d.mean = (const float *)mean;
     d.most = (uint32_t *)most;
-    d.pre0 = (float *)work;
-    d.pnew = (float *)work + (uint64_t)width * height;
+    // work: w0 (16 B a cell), then pre0, pnew, rhs0, z0 (4 B each)
+    d.w0 = (float4 *)work;
+    d.pre0 = (float *)(d.w0 + (uint64_t)width * height);
+    d.pnew = d.pre0 + (uint64_t)width * height;
+    d.rhs0 = d.pnew + (uint64_t)width * height;
+    d.z0 = d.rhs0 + (uint64_t)width * height;
     a->stream = a->own;
     return a;
 }
@@ -705,7 +738,7 @@ extern "C" int wig_air_destroy(WigAir *a) {
     (void)hipFree(a->d.status);
     (void)hipFree((void *)a->d.mean);
     (void)hipFree(a->d.most);
-    (void)hipFree(a->d.pre0);
+    (void)hipFree(a->d.w0);
     if (a->t0) (void)hipEventDestroy(a->t0);
     if (a->t1) (void)hipEventDestroy(a->t1);
     if (a->own) (void)hipStreamDestroy(a->own);
@@ -841,6 +874,7 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         k_airUpdate<<<cellGrid, 256, 0, a->stream>>>(d, k);
         k_airSlot<<<1, 1024, 0, a->stream>>>(d, 1u + k % 3u, cellGrid, k, 0);
     }
+    k_airFinish<<<cellBlocks, 256, 0, a->stream>>>(d);
     AIRCHECK(hipGetLastError());
     if (a->timing) {
         AIRCHECK(hipEventRecord(a->t1, a->stream));
