@@ -384,18 +384,32 @@ __device__ void upTile(const AirDev &a, AirTile<T> &t, uint32_t l, int32_t ox, i
     }
 }
 
+// A level pass's square for this workgroup.  Workgroups go to the 8 XCDs in
+// turn (the b-th to XCD b % 8), each XCD with an L2 of its own; so the
+// squares are dealt out as 8 bands of consecutive squares, one an XCD, so
+// that neighbours, whose halos overlap, share an L2.  (The last n % 8, if
+// any, keep their own order: a one-to-one map of the squares.)
+template <int T>
+__device__ __forceinline__ int2 squareOrigin() {
+    const uint32_t n = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x, full = n / 8u * 8u;
+    const uint32_t q = b < full ? (b % 8u) * (n / 8u) + b / 8u : b;
+    return make_int2((int32_t)(q % gridDim.x) * T, (int32_t)(q / gridDim.x) * T);
+}
+
 template <int T>
 __global__ void __launch_bounds__(256) k_airDown(AirDev a, uint32_t l, int restricts, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
     __shared__ AirTile<T> t;
-    downTile<T, 256>(a, t, l, (int32_t)blockIdx.x * T, (int32_t)blockIdx.y * T, restricts);
+    const int2 o = squareOrigin<T>();
+    downTile<T, 256>(a, t, l, o.x, o.y, restricts);
 }
 
 template <int T>
 __global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cycle, int rho) {
     if (airConverged(a, cycle)) return;
     __shared__ AirTile<T> t;
-    upTile<T, 256>(a, t, l, (int32_t)blockIdx.x * T, (int32_t)blockIdx.y * T, rho);
+    const int2 o = squareOrigin<T>();
+    upTile<T, 256>(a, t, l, o.x, o.y, rho);
 }
 
 // solveCoarsest (:1482-1535): symmetric SOR in LDS, a cell a thread (the
