@@ -104,7 +104,9 @@ def main():
                     help="serve: most cells a round steps (4 M keeps warm rounds with 512x512 worlds under 16 ms at p99)")
     ap.add_argument("--seeds", type=int, default=2, help="precision: worlds (seeds) per variant")
     ap.add_argument("--air-tile", type=int, default=0,
-                    help="air: every level's LDS tile side (16 or 32); 0: 32 on level 0, 16 on the others")
+                    help="air: every level's LDS tile side (8, 16 or 32); 0: 32 on level 0, 16 on level 1, 8 below")
+    ap.add_argument("--air-tiles", default="",
+                    help="air: tile sides by level, comma-separated (as --air-tile, level by level)")
     ap.add_argument("--air-coarse", type=int, default=0,
                     help="air: 1, the levels that fit 64x64 in one workgroup's kernel; 0 (default), a launch each")
     ap.add_argument("--gpus", type=int, default=1)
@@ -560,6 +562,9 @@ def run_air(args):
     g = GpuAir(size, size)
     if args.air_tile:
         g.set_tile(args.air_tile)
+    for l, side in enumerate(int(v) for v in args.air_tiles.split(",") if v):
+        if l < g.levels:
+            g.set_tile(side, l)
     g.set_coarse(args.air_coarse)
     g.load(mg0, x0)
     mg_init, cg_init = g.download(PART_MG), g.download(PART_CG)
@@ -592,8 +597,8 @@ def run_air(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{size}x{size} air grid, 60 walled rooms, lower 15% liquid, 2 black holes, "
                                f"divergence of N(0,0.05) face velocities, {len(g.dims)} levels, LDS tiles "
-                               f"{args.air_tile or '32 on level 0, 16 on the others'}", "iterations": its,
-                   "tile": args.air_tile or "32/16", "coarse_fused": bool(args.air_coarse)},
+                               f"{args.air_tiles or args.air_tile or '32 on level 0, 16 on level 1, 8 below'}", "iterations": its,
+                   "tile": args.air_tiles or args.air_tile or "32/16/8", "coarse_fused": bool(args.air_coarse)},
         "iterations": its, "ms_per_iteration": ms_solve / its if its else None,
         "most_r": float(np.abs(r).max()), "wall_s_with_uploads": wall,
         "roofline": {"bound": "hbm", "achieved": solve_b / (ms_solve * 1e-3) / 1e9, "peak": 8000.0,

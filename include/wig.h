@@ -307,8 +307,8 @@ int wig_air_download(WigAir *a, int part, void *dst, size_t bytes);
 int wig_air_device_ptr(WigAir *a, int part, void **ptr);
 /* Runs later calls on this HIP stream (NULL: the grid's own). */
 int wig_air_set_stream(WigAir *a, void *stream);
-/* The side of a level's LDS tiles in the V-cycle's passes, 16 or 32
- * (default 32 on level 0, 16 on the others): the results are the same bits
+/* The side of a level's LDS tiles in the V-cycle's passes, 8, 16 or 32
+ * (default 32 on level 0, 16 on level 1, 8 below): the results are the same bits
  * either way.  This library's own. */
 int wig_air_set_tile(WigAir *a, uint32_t level, uint32_t side);
 /* With `on`, the levels from the first that fits 64 x 64 cells down to the

@@ -42,7 +42,7 @@ def next_step(g, o, W, H, seed, step):
     (96, 80, 3, 16, 1), (96, 80, 3, 32, 1), (96, 80, 3, 0, 1), (96, 80, 3, 16, 0),
     (200, 130, 5, 16, 1), (200, 130, 5, 32, 1), (200, 130, 5, 0, 1), (200, 130, 5, 0, 0),
     (48, 40, 6, 0, 1), (37, 29, 4, 32, 1), (16, 12, 2, 16, 1), (5, 190, 4, 16, 1), (5, 190, 4, 16, 0),
-    (1, 1, 0, 16, 1), (300, 7, 8, 32, 1), (130, 70, 9, 0, 1),
+    (1, 1, 0, 16, 1), (300, 7, 8, 32, 1), (130, 70, 9, 0, 1), (96, 80, 3, 8, 0), (200, 130, 5, 8, 0),
 ])
 def test_air_projection_matches_oracle(W, H, seed, tile, coarse):
     # coarse: the levels that fit 64 x 64 in one kernel (1, the default) or
@@ -52,7 +52,7 @@ def test_air_projection_matches_oracle(W, H, seed, tile, coarse):
     g, o = GpuAir(W, H), OracleAir(W, H)
     assert g.dims == o.dims
     g.set_coarse(coarse)
-    if tile:                      # (0: the default, 32 on level 0 and 16 below)
+    if tile:                      # (0: the default, 32 on level 0, 16 on level 1, 8 below)
         g.set_tile(tile)
     g.load(mg0, x0)
     o.load(mg0, x0)

@@ -105,7 +105,7 @@ class GpuAir:
         self.upload(PART_MEAN, np.zeros(n, np.float32) if mean is None else np.asarray(mean, np.float32).reshape(-1))
 
     def set_tile(self, side, level=None):
-        """A level's LDS tile side (16 or 32); every level's if level is None."""
+        """A level's LDS tile side (8, 16 or 32); every level's if level is None."""
         for l in range(self.levels) if level is None else [level]:
             _check(lib().wig_air_set_tile(self._a, l, side))
 

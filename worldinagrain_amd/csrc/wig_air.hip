@@ -648,7 +648,7 @@ struct WigAir {
     AirDev d;
     uint64_t mgBytes, cgBytes;
     hipStream_t stream = nullptr, own = nullptr;
-    uint32_t tile[AIR_LEVELS] = {32, 16, 16, 16, 16, 16, 16, 16, 16};
+    uint32_t tile[AIR_LEVELS] = {32, 16, 8, 8, 8, 8, 8, 8, 8};   // (measured at 1024 x 1024)
     int coarse = 0;        // the levels that fit COARSE_FUSED_T in one kernel
     int timing = 0;
     double ms = 0.0;
@@ -829,8 +829,8 @@ extern "C" int wig_air_set_coarse(WigAir *a, int on) {
 }
 
 extern "C" int wig_air_set_tile(WigAir *a, uint32_t level, uint32_t side) {
-    if (!a || (side != 16 && side != 32) || level >= a->d.L)
-        return wig_fail_message(WIG_E_INVALID, "wig_air_set_tile: a level of the grid, side 16 or 32");
+    if (!a || (side != 8 && side != 16 && side != 32) || level >= a->d.L)
+        return wig_fail_message(WIG_E_INVALID, "wig_air_set_tile: a level of the grid, side 8, 16 or 32");
     a->tile[level] = side;
     return WIG_OK;
 }
@@ -857,6 +857,7 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
     }
     auto level = [&](bool up, uint32_t l, int restricts, uint32_t k) {
         if (a->tile[l] == 32) launchLevel<32>(a, up, l, restricts, k);
+        else if (a->tile[l] == 8) launchLevel<8>(a, up, l, restricts, k);
         else launchLevel<16>(a, up, l, restricts, k);
     };
     // the first level that fits one square, if the fused kernel takes it
