@@ -298,6 +298,7 @@ typedef struct WigAirLayout {
     uint32_t level_width[9], level_height[9], level_base[9];
     uint64_t part_bytes[WIG_AIR_PART_COUNT];
 } WigAirLayout;
+/* Sides 1 .. 16384; NULL on error (wig_last_error has why). */
 WigAir *wig_air_create(uint32_t width, uint32_t height);
 int wig_air_destroy(WigAir *a);
 int wig_air_layout(const WigAir *a, WigAirLayout *out);
@@ -322,7 +323,8 @@ int wig_air_set_coarse(WigAir *a, int on);
  * cgDirection, cgApply, cgAlpha, cgUpdate; an iteration after the most |r|
  * left falls under AIR_TOLERANCE (1e-4) changes nothing but the slots.
  * Level 0 of MG (gather's and divergence's, air.wgsl:352-414,825-866) and
- * CG's potentials (last step's) are the input. */
+ * CG's potentials (last step's) are the input.  `cycles` at most 4096
+ * (WIG_E_INVALID above). */
 int wig_air_project(WigAir *a, uint32_t cycles);
 int wig_air_sync(WigAir *a);
 /* With `on`, each wig_air_project is timed with HIP events on its stream
