@@ -45,3 +45,19 @@ def test_errors_without_device_are_reported():
         pytest.skip("a device is visible")
     with pytest.raises(gpu.WigError):
         gpu.GpuWorld(16, 16)
+
+
+def test_air_errors_are_reported():
+    # the air grid's refusals: a side out of range, no device; each NULL
+    # with the reason in wig_last_error
+    from worldinagrain_amd import air, gpu
+    L = air.lib()
+    assert not L.wig_air_create(0, 16)
+    assert b"sides" in L.wig_last_error()
+    assert not L.wig_air_create(16, 20000)
+    assert L.wig_air_project(None, 1) < 0
+    if not L.wig_device_ok():
+        assert not L.wig_air_create(16, 16)
+        assert L.wig_last_error()
+        with pytest.raises(gpu.WigError):
+            air.GpuAir(16, 16)
