@@ -509,22 +509,32 @@ __device__ __forceinline__ float newP(float z, float4 w, float pold, float beta)
 }
 __global__ void __launch_bounds__(256) k_airApply(AirDev a, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
-    const uint32_t g = blockIdx.x * 4u + threadIdx.x / 64u, lane = threadIdx.x & 63u;
-    if (g >= a.groups) return;
+    // a wavefront's 8 x 8 group and the ring around it, 10 x 10 cells, in
+    // LDS: the packed words and the new p of each (none off the grid)
+    __shared__ float4 sw[4][100];
+    __shared__ float sp[4][100];
+    const uint32_t wave = threadIdx.x / 64u, g = blockIdx.x * 4u + wave, lane = threadIdx.x & 63u;
+    const int32_t gx0 = (int32_t)((g % a.gx) * 8u) - 1, gy0 = (int32_t)((g / a.gx) * 8u) - 1;
+    const float beta = sums(a)[2];
+    for (uint32_t k = lane; k < 100u; k += 64u) {
+        const int32_t cx = gx0 + (int32_t)(k % 10u), cy = gy0 + (int32_t)(k / 10u);
+        const bool in = g < a.groups && cx >= 0 && cy >= 0 && cx < (int32_t)a.W && cy < (int32_t)a.H;
+        const uint32_t ci = in ? (uint32_t)cy * a.W + (uint32_t)cx : 0u;
+        const float4 w = a.w0[ci];
+        const float z = a.z0[ci], po = a.cg[ci].z;
+        sw[wave][k] = w;
+        sp[wave][k] = in ? newP(z, w, po, beta) : 0.0f;
+    }
+    __syncthreads();
     const uint32_t x = (g % a.gx) * 8u + lane % 8u, y = (g / a.gx) * 8u + lane / 8u;
     float v = 0.0f;
-    if (x < a.W && y < a.H) {
-        const float beta = sums(a)[2];
+    if (g < a.groups && x < a.W && y < a.H) {
         const int32_t xi = (int32_t)x, yi = (int32_t)y;
-        const uint32_t i = y * a.W + x;
+        const uint32_t i = y * a.W + x, k = (lane / 8u + 1u) * 10u + lane % 8u + 1u;
         const bool r_ = x + 1u < a.W, d_ = y + 1u < a.H, l_ = x > 0u, u_ = y > 0u;
-        // every load first (clamped to the cell itself where off the grid);
-        // w: the packed level-0 words (right, bottom, edge, fixed)
-        const uint32_t ir = r_ ? i + 1u : i, id = d_ ? i + a.W : i, il = l_ ? i - 1u : i, iu = u_ ? i - a.W : i;
-        const float4 w = a.w0[i], wr = a.w0[ir], wd = a.w0[id], wl = a.w0[il], wu = a.w0[iu];
-        const float pc = newP(a.z0[i], w, a.cg[i].z, beta);
-        const float pr = newP(a.z0[ir], wr, a.cg[ir].z, beta), pd = newP(a.z0[id], wd, a.cg[id].z, beta);
-        const float pl = newP(a.z0[il], wl, a.cg[il].z, beta), pu = newP(a.z0[iu], wu, a.cg[iu].z, beta);
+        const float4 w = sw[wave][k], wl = sw[wave][k - 1u], wu = sw[wave][k - 10u];
+        const float pc = sp[wave][k], pr = sp[wave][k + 1u], pd = sp[wave][k + 10u];
+        const float pl = sp[wave][k - 1u], pu = sp[wave][k - 10u];
         float q = 0.0f;
         if (!(w.w > 0.0f)) {
             float diag = w.x + w.y + w.z + anchorAt(a, 0u, xi, yi);
@@ -546,7 +556,7 @@ __global__ void __launch_bounds__(256) k_airApply(AirDev a, uint32_t cycle) {
         v = pc * q;
     }
     v = groupTree(v);
-    if (lane == 0u) sums(a)[4u + g] = v;
+    if (lane == 0u && g < a.groups) sums(a)[4u + g] = v;
 }
 
 // sumAll (:1081-1097): 256 lanes each its strided share, then halving
