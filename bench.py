@@ -606,7 +606,10 @@ def run_air(args):
                      "unit_of_work": "one solve: cgStart and its iterations",
                      "bytes_per_solve": solve_b, "bytes_per_iteration": iter_b},
     }
-    assert its_last == its and its < AIR_CYCLES, (its, its_last)
+    assert its_last == its, (its, its_last)
+    # (a grid that needs more than AIR_CYCLES runs them all, as the
+    # reference's host gives no more; the line says so)
+    out["converged"] = bool(its < AIR_CYCLES)
     if not args.no_cpu:
         from oracle.air import OracleAir
         o = OracleAir(size, size)
