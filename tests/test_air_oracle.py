@@ -90,9 +90,12 @@ def started(W, H, seed, mean=None):
     return o, mg0, x0
 
 
-@pytest.mark.parametrize("W,H,seed", [(40, 36, 1), (37, 29, 4)])
-def test_coarse_operators_are_galerkin(W, H, seed):
-    o, _, _ = started(W, H, seed)
+@pytest.mark.parametrize("W,H,seed,fractional", [(40, 36, 1, False), (37, 29, 4, False), (41, 33, 8, True)])
+def test_coarse_operators_are_galerkin(W, H, seed, fractional):
+    mg0, x0 = scenes.air_rooms(W, H, seed=seed, fractional=fractional)
+    o = OracleAir(W, H)
+    o.load(mg0, x0)
+    o.start()
     # pin a cell so a coarse cell has a held one among its four
     o.level(0)[3, 4, 5] = 1.0
     o.vcycle()

@@ -117,3 +117,21 @@ def test_air_full_size():
     r = g.download(PART_CG)[:4 * W * H].reshape(H, W, 4)[..., 1]
     assert np.abs(r).max() < 1e-4
     g.close()
+
+
+@pytest.mark.parametrize("W,H,tile", [(96, 80, 0), (150, 110, 16)])
+def test_air_fractional_weights(W, H, tile):
+    # faces of any weight (gather's are 0 or 1): the same bits as the oracle
+    say(f"[air] {W}x{H} fractional weights, tile {tile}")
+    mg0, x0 = scenes.air_rooms(W, H, seed=17, fractional=True)
+    g, o = GpuAir(W, H), OracleAir(W, H)
+    if tile:
+        g.set_tile(tile)
+    g.load(mg0, x0)
+    o.load(mg0, x0)
+    g.project(64)
+    o.project(64)
+    g.sync()
+    same(g, o, "fractional")
+    assert int(o.status[4]) > 0
+    g.close()

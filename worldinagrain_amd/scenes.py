@@ -193,7 +193,8 @@ def precision_world(side, seed, fluid_share=0.70, air_share=0.20, solid_share=0.
     return cells, int(fluid.sum())
 
 
-def air_rooms(width, height, seed=1, walls=12, liquid_share=0.15, holes=2, rhs_sigma=0.05, warm_sigma=0.01, flow_seed=None):
+def air_rooms(width, height, seed=1, walls=12, liquid_share=0.15, holes=2, rhs_sigma=0.05, warm_sigma=0.01, flow_seed=None,
+              fractional=False):
     """Level 0 of the air's projection as gather fills it (shaders/air.wgsl:
     352-414, 825-866), for a world of walls and liquid in air.
 
@@ -208,7 +209,8 @@ def air_rooms(width, height, seed=1, walls=12, liquid_share=0.15, holes=2, rhs_s
     `holes` black holes are held at a potential below none (fixed).  The
     right-hand side is the net outflow of face velocities N(0, rhs_sigma) on
     the open faces (drawn from `flow_seed` if given: the same rooms, another
-    step's flow); the warm start N(0, warm_sigma) in cells that carry gas."""
+    step's flow); the warm start N(0, warm_sigma) in cells that carry gas.
+    With `fractional`, the open faces' weights are drawn from (0.25, 1]."""
     rng = np.random.default_rng(seed)
     W, H = width, height
     carries = np.ones((H, W), bool)
@@ -247,9 +249,17 @@ def air_rooms(width, height, seed=1, walls=12, liquid_share=0.15, holes=2, rhs_s
     out[:, 1:] -= u[:, :-1]
     out[1:, :] -= v[:-1, :]
     mg0[..., 1] = out
-    mg0[..., 2] = right
-    mg0[..., 3] = down
-    mg0[..., 4] = edge
+    if fractional:
+        # open faces of any weight in (0.25, 1] (gather's are 1: the solve's
+        # arithmetic is the same for any)
+        mg0[..., 2] = right * rng.uniform(0.25, 1.0, (H, W))
+        mg0[..., 3] = down * rng.uniform(0.25, 1.0, (H, W))
+        mg0[..., 4] = edge * rng.uniform(0.25, 1.0, (H, W))
+        mg0[..., 1] = out * 0.5
+    else:
+        mg0[..., 2] = right
+        mg0[..., 3] = down
+        mg0[..., 4] = edge
     mg0[..., 5] = fixed
     mg0[..., 6] = np.where(carries, np.float32(0.0), np.uint32(0xffffffff).view(np.float32))   # piece bits
     mg0[..., 7] = share
