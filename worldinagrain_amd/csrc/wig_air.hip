@@ -415,42 +415,47 @@ __global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cy
 // solveCoarsest (:1482-1535): symmetric SOR in LDS, a cell a thread (the
 // first 256 of the workgroup)
 __device__ void coarsestBody(const AirDev &a, float4 *A, float2 *B) {
+    (void)B;
     const uint32_t l = a.L - 1u, dx = a.w[l], dy = a.h[l], n = min(dx * dy, (uint32_t)AIR_COARSEST);
     const uint32_t j = threadIdx.x;
+    // only the potentials change from sweep to sweep: they go in LDS (A's
+    // room), the cell's weights, rhs and diagonal stay in registers, each
+    // the value the reference's sweep recomputes (the same operations in
+    // the same order)
+    float *X = (float *)A;
+    const int32_t cx = (int32_t)(j % dx), cy = (int32_t)(j / dx);
+    float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float lz = 0.0f, uw = 0.0f, diag = 1.0f;
+    bool free_ = false;
     if (j < n) {
-        A[j] = a.mg[a.base[l] + 2u * j];
+        m = a.mg[a.base[l] + 2u * j];
         const float4 b = a.mg[a.base[l] + 2u * j + 1u];
-        B[j] = make_float2(b.x, b.y);
+        X[j] = m.x;
+        free_ = !(b.y > 0.0f);
+        if (cx > 0) lz = a.mg[a.base[l] + 2u * (j - 1u)].z;
+        if (cy > 0) uw = a.mg[a.base[l] + 2u * (j - dx)].w;
+        diag = m.z + m.w + b.x + anchorAt(a, l, cx, cy);
+        if (cx > 0) diag += lz;
+        if (cy > 0) diag += uw;
     }
     __syncthreads();
-    const int32_t cx = (int32_t)(j % dx), cy = (int32_t)(j / dx);
     for (int sweep = 0; sweep < 2 * COARSE_SWEEPS; sweep++) {
         const int color = sweep < COARSE_SWEEPS ? (sweep & 1) : ((2 * COARSE_SWEEPS - 1 - sweep) & 1);
-        if (j < n && ((cx + cy) & 1) == color && !(B[j].y > 0.0f)) {
-            const float4 m = A[j];
+        if (j < n && ((cx + cy) & 1) == color && free_) {
             float sum = 0.0f;
-            float diag = m.z + m.w + B[j].x + anchorAt(a, l, cx, cy);
-            if ((uint32_t)cx + 1u < dx) sum += m.z * A[j + 1u].x;
-            if ((uint32_t)cy + 1u < dy) sum += m.w * A[j + dx].x;
-            if (cx > 0) {
-                const float4 o = A[j - 1u];
-                sum += o.z * o.x;
-                diag += o.z;
-            }
-            if (cy > 0) {
-                const float4 o = A[j - dx];
-                sum += o.w * o.x;
-                diag += o.w;
-            }
+            if ((uint32_t)cx + 1u < dx) sum += m.z * X[j + 1u];
+            if ((uint32_t)cy + 1u < dy) sum += m.w * X[j + dx];
+            if (cx > 0) sum += lz * X[j - 1u];
+            if (cy > 0) sum += uw * X[j - dx];
             // mix(e1, e2, t) = e1 (1 - t) + e2 t
             const float xn = (sum - m.y) / diag;
-            A[j].x = m.x * (1.0f - COARSE_OMEGA) + xn * COARSE_OMEGA;
+            X[j] = X[j] * (1.0f - COARSE_OMEGA) + xn * COARSE_OMEGA;
         }
         __syncthreads();
     }
     if (j < n) {
-        a.mg[a.base[l] + 2u * j].x = A[j].x;
-        if (l == 0u) a.z0[j] = A[j].x;
+        a.mg[a.base[l] + 2u * j].x = X[j];
+        if (l == 0u) a.z0[j] = X[j];
     }
 }
 __global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
