@@ -327,6 +327,21 @@ int wig_air_set_coarse(WigAir *a, int on);
  * (WIG_E_INVALID above). */
 int wig_air_project(WigAir *a, uint32_t cycles);
 int wig_air_sync(WigAir *a);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc): wig_air_project's projection, the same bits, on buffers the
+ * caller holds, laid out as wig_air_layout gives them for a grid of
+ * width x height (mg, cg, status, mean: the four parts), enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream).  It allocates nothing:
+ * its working arrays (4 KB + 32 B a cell) live in a static device array of
+ * the library's, one per device, held until the process ends, sized for
+ * WIG_AIR_STATIC_MAX_CELLS cells (WIG_E_INVALID above).  Every call on a
+ * device shares that device's array, so calls that overlap on one device
+ * (two streams, or two threads) race: order them.  The tile sides are the
+ * defaults and the coarse levels run a launch each. */
+#define WIG_AIR_STATIC_MAX_CELLS 262144u
+int wig_air_project_static(void *mg, void *cg, void *status, const void *mean, uint32_t width, uint32_t height,
+                           uint32_t cycles, void *stream);
 /* With `on`, each wig_air_project is timed with HIP events on its stream
  * (and waits for its end); wig_air_elapsed has the sum and the count. */
 int wig_air_time(WigAir *a, int on);

@@ -50,6 +50,7 @@ def lib():
         L.wig_air_sync.argtypes = [P]
         L.wig_air_time.argtypes = [P, ct.c_int]
         L.wig_air_elapsed.argtypes = [P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint32)]
+        L.wig_air_project_static.argtypes = [P, P, P, P, ct.c_uint32, ct.c_uint32, ct.c_uint32, P]
         _ready = True
     return L
 
@@ -119,6 +120,11 @@ class GpuAir:
     def project(self, cycles=AIR_CYCLES):
         _check(lib().wig_air_project(self._a, cycles))
 
+    def device_ptr(self, part):
+        p = ct.c_void_p()
+        _check(lib().wig_air_device_ptr(self._a, part, ct.byref(p)))
+        return p.value
+
     def sync(self):
         _check(lib().wig_air_sync(self._a))
 
@@ -129,6 +135,18 @@ class GpuAir:
         ms, n = ct.c_double(), ct.c_uint32()
         _check(lib().wig_air_elapsed(self._a, ct.byref(ms), ct.byref(n)))
         return ms.value, n.value
+
+
+# the most cells wig_air_project_static takes (WIG_AIR_STATIC_MAX_CELLS)
+STATIC_MAX_CELLS = 262144
+
+
+def project_static(mg, cg, status, mean, width, height, cycles=AIR_CYCLES, stream=None):
+    """wig_air_project_static: the projection on the caller's device buffers
+    (addresses of the four parts, laid out as a width x height grid's), with
+    the working arrays in the library's static device array.  Allocates
+    nothing; calls on one device must not overlap."""
+    _check(lib().wig_air_project_static(mg, cg, status, mean, width, height, cycles, stream))
 
 
 class AirProjection:

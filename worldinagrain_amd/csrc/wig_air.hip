@@ -16,7 +16,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/wig.h"
 
@@ -682,16 +684,13 @@ static int airFail(int code, const char *what, hipError_t e) {
         if (e_ != hipSuccess) return airFail(WIG_E_DEVICE, #x, e_); \
     } while (0)
 
-extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
-    if (width == 0 || height == 0 || width > 16384 || height > 16384) {
-        wig_fail_message(WIG_E_INVALID, "wig_air_create: sides must be 1..16384");
-        return nullptr;
-    }
-    WigAir *a = new WigAir();
-    AirDev &d = a->d;
+// The grid's levels (js/sim.js:712-719): halved, rounding up, until a level
+// holds AIR_COARSEST cells or fewer, AIR_LEVELS at most; their bases in
+// airProjection, and the 8 x 8 groups.  The cells of every level, or 0 if
+// the coarsest of AIR_LEVELS would hold more than AIR_COARSEST.
+static uint64_t airLevels(AirDev &d, uint32_t width, uint32_t height) {
     d.W = width;
     d.H = height;
-    // the levels (js/sim.js:712-719)
     d.L = 1;
     d.w[0] = width;
     d.h[0] = height;
@@ -700,11 +699,7 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
         d.h[d.L] = (d.h[d.L - 1] + 1u) / 2u;
         d.L++;
     }
-    if ((uint64_t)d.w[d.L - 1] * d.h[d.L - 1] > AIR_COARSEST) {
-        wig_fail_message(WIG_E_INVALID, "wig_air_create: the coarsest of 9 levels would hold more than 256 cells");
-        delete a;
-        return nullptr;
-    }
+    if ((uint64_t)d.w[d.L - 1] * d.h[d.L - 1] > AIR_COARSEST) return 0;
     uint64_t cells = 0;
     for (uint32_t l = 0; l < d.L; l++) {
         d.base[l] = (uint32_t)(2u * cells);
@@ -713,6 +708,32 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     for (uint32_t l = d.L; l < AIR_LEVELS; l++) d.w[l] = d.h[l] = d.base[l] = 0;
     d.gx = (width + 7u) / 8u;
     d.groups = d.gx * ((height + 7u) / 8u);
+    return cells;
+}
+// The working arrays after `most`: w0 (16 B a cell), then pre0, pnew, rhs0,
+// z0 (4 B each)
+static void airWork(AirDev &d, void *work) {
+    const uint64_t n = (uint64_t)d.W * d.H;
+    d.w0 = (float4 *)work;
+    d.pre0 = (float *)(d.w0 + n);
+    d.pnew = d.pre0 + n;
+    d.rhs0 = d.pnew + n;
+    d.z0 = d.rhs0 + n;
+}
+
+extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
+    if (width == 0 || height == 0 || width > 16384 || height > 16384) {
+        wig_fail_message(WIG_E_INVALID, "wig_air_create: sides must be 1..16384");
+        return nullptr;
+    }
+    WigAir *a = new WigAir();
+    AirDev &d = a->d;
+    const uint64_t cells = airLevels(d, width, height);
+    if (cells == 0) {
+        wig_fail_message(WIG_E_INVALID, "wig_air_create: the coarsest of 9 levels would hold more than 256 cells");
+        delete a;
+        return nullptr;
+    }
     a->mgBytes = 32u * cells;
     a->cgBytes = 16u * ((uint64_t)width * height + (4u + d.groups + 3u) / 4u);
     void *mg = nullptr, *cg = nullptr, *st = nullptr, *mean = nullptr;
@@ -749,12 +770,7 @@ extern "C" WigAir *wig_air_create(uint32_t width, uint32_t height) {
     d.status = (uint32_t *)st;
     d.mean = (const float *)mean;
     d.most = (uint32_t *)most;
-    // work: w0 (16 B a cell), then pre0, pnew, rhs0, z0 (4 B each)
-    d.w0 = (float4 *)work;
-    d.pre0 = (float *)(d.w0 + (uint64_t)width * height);
-    d.pnew = d.pre0 + (uint64_t)width * height;
-    d.rhs0 = d.pnew + (uint64_t)width * height;
-    d.z0 = d.rhs0 + (uint64_t)width * height;
+    airWork(d, work);
     a->stream = a->own;
     return a;
 }
@@ -915,6 +931,68 @@ extern "C" int wig_air_project(WigAir *a, uint32_t cycles) {
         a->calls++;
     }
     return WIG_OK;
+}
+
+// ------------------------------------------------------------ allocation-free projection
+//
+// wig_air_project_static: the projection of wig_air_project on buffers the
+// caller holds, for callers that cannot allocate (code under HIP graph
+// capture, serving loops that forbid hipMalloc).  Its working arrays (the
+// workgroups' maxima and level 0's packed words, 4 KB + 32 B a cell) live in
+// `scratch`, one array per device for the life of the process.  Host code
+// never passes `scratch` itself to a kernel: in host code the name stands
+// for a host-side shadow, not for the device's copy.  Each device's address
+// comes from hipGetSymbolAddress, taken once per device with that device
+// current, and only that address goes into kernel arguments.
+#define AIR_STATIC_MOST_BYTES (4ull * CELL_GROUPS)
+#define AIR_STATIC_BYTES (AIR_STATIC_MOST_BYTES + 32ull * WIG_AIR_STATIC_MAX_CELLS)
+static __device__ __align__(256) unsigned char scratch[AIR_STATIC_BYTES];
+
+#define AIR_STATIC_DEVICES 64
+static void *g_scratchAt[AIR_STATIC_DEVICES];
+static std::once_flag g_scratchOnce[AIR_STATIC_DEVICES];
+static hipError_t g_scratchErr[AIR_STATIC_DEVICES];
+
+// zeroes n 32-bit words (a fill kernel, not hipMemsetAsync, so that a
+// caller's graph capture records a kernel node)
+__global__ void __launch_bounds__(256) k_airZero(uint32_t *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) p[i] = 0u;
+}
+
+extern "C" int wig_air_project_static(void *mg, void *cg, void *status, const void *mean, uint32_t width,
+                                      uint32_t height, uint32_t cycles, void *stream) {
+    if (!mg || !cg || !status || !mean) return wig_fail_message(WIG_E_INVALID, "wig_air_project_static: NULL buffer");
+    if (width == 0 || height == 0 || (uint64_t)width * height > WIG_AIR_STATIC_MAX_CELLS)
+        return wig_fail_message(WIG_E_INVALID,
+                                "wig_air_project_static: the grid must hold 1 .. WIG_AIR_STATIC_MAX_CELLS (262144) cells");
+    if (cycles > 4096) return wig_fail_message(WIG_E_INVALID, "wig_air_project_static: at most 4096 iterations");
+    int dev = 0;
+    AIRCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= AIR_STATIC_DEVICES)
+        return wig_fail_message(WIG_E_INVALID, "wig_air_project_static: device ordinal past 63");
+    // the device's copy of the array, once per device (this device is current)
+    std::call_once(g_scratchOnce[dev], [dev] {
+        g_scratchErr[dev] = hipGetSymbolAddress(&g_scratchAt[dev], HIP_SYMBOL(scratch));
+    });
+    if (g_scratchErr[dev] != hipSuccess) return airFail(WIG_E_DEVICE, "wig_air_project_static: hipGetSymbolAddress", g_scratchErr[dev]);
+    unsigned char *at = (unsigned char *)g_scratchAt[dev];
+    WigAir a;   // a view on the caller's buffers and the array: nothing allocated
+    AirDev &d = a.d;
+    if (airLevels(d, width, height) == 0)
+        return wig_fail_message(WIG_E_INVALID, "wig_air_project_static: the coarsest of 9 levels would hold more than 256 cells");
+    d.mg = (float4 *)mg;
+    d.cg = (float4 *)cg;
+    d.status = (uint32_t *)status;
+    d.mean = (const float *)mean;
+    d.most = (uint32_t *)at;
+    airWork(d, at + AIR_STATIC_MOST_BYTES);
+    a.stream = (hipStream_t)stream;
+    // what the last call left: cleared (held cells' pre0 must read none)
+    const uint64_t words = (AIR_STATIC_MOST_BYTES + 32ull * width * height) / 4u;
+    const uint64_t blocks = std::min<uint64_t>((words + 255u) / 256u, 4096u);
+    k_airZero<<<(uint32_t)blocks, 256, 0, a.stream>>>((uint32_t *)at, words);
+    AIRCHECK(hipGetLastError());
+    return wig_air_project(&a, cycles);
 }
 
 extern "C" int wig_air_sync(WigAir *a) {
