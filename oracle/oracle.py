@@ -10,15 +10,27 @@ from worldinagrain_amd import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _PATH = os.path.join(_HERE, "libwig_oracle.so")
-_lib = None
+_libs = {}
+
+# one pass of the step (wor_run's WOR_PASS_* ids, wig_oracle.c)
+(PASS_SYNC, PASS_SYNC_PARTICLES, PASS_SYNC_SEED, PASS_COMPACT, PASS_RETIRE, PASS_CLEAR, PASS_GATHER, PASS_SEAL,
+ PASS_CLASSIFY, PASS_SET_FACES, PASS_UNPRESS, PASS_DIVERGENCE, PASS_RESTRICT, PASS_SMOOTH, PASS_COARSEST, PASS_PROLONG,
+ PASS_PROLONG_FINE, PASS_PROJECT, PASS_EXTRAPOLATE, PASS_ADVECT, PASS_SORT, PASS_RASTER_CLAIMS, PASS_RASTER,
+ PASS_UNPAINT, PASS_RELAX) = range(25)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_PATH):
-            raise RuntimeError(f"oracle not built: {_PATH} (run `make -C oracle`)")
-        L = ct.CDLL(_PATH)
+def mutant_path(k):
+    """The oracle built with planted mistake k (`make -C oracle mutants`)."""
+    return os.path.join(_HERE, "_mut", f"libwig_oracle_m{k}.so")
+
+
+def lib(path=None):
+    """The oracle's library (or, given a path, that build of it)."""
+    path = path or _PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        L = ct.CDLL(path)
         L.wor_create.restype = ct.c_void_p
         L.wor_create.argtypes = [ct.c_uint32, ct.c_uint32, ct.c_uint32]
         L.wor_destroy.argtypes = [ct.c_void_p]
@@ -50,8 +62,11 @@ def lib():
         L.wor_exact_value.argtypes = [ct.c_uint32, ct.c_uint32, ct.c_float]
         L.wor_fine_beta.restype = ct.c_float
         L.wor_fine_beta.argtypes = [ct.c_float] * 4
-        _lib = L
-    return _lib
+        L.wor_run.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
+        L.wor_level_start.restype = ct.c_uint64
+        L.wor_level_start.argtypes = [ct.c_void_p, ct.c_uint32]
+        _libs[path] = L
+    return _libs[path]
 
 
 _DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8}
@@ -60,8 +75,9 @@ _DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SO
 class OracleWorld:
     """A world in host memory, stepped by the CPU restatement."""
 
-    def __init__(self, width, height, coarse_levels=0, kinds=None, dot_mode=1):
-        L = lib()
+    def __init__(self, width, height, coarse_levels=0, kinds=None, dot_mode=1, L=None):
+        L = L or lib()
+        self.L = L
         self._w = L.wor_create(width, height, coarse_levels)
         if not self._w:
             raise ValueError("invalid world size")
@@ -73,7 +89,7 @@ class OracleWorld:
 
     def close(self):
         if self._w:
-            lib().wor_destroy(self._w)
+            self.L.wor_destroy(self._w)
             self._w = None
 
     def __del__(self):
@@ -85,12 +101,12 @@ class OracleWorld:
     def set_kinds(self, kinds):
         k = np.asarray(kinds, dtype=np.uint8)
         assert k.shape == (256,)
-        lib().wor_set_kinds(self._w, k.tobytes())
+        self.L.wor_set_kinds(self._w, k.tobytes())
 
     def part(self, p):
         """A numpy view (u32 words, f32 for field/fluidCg) of the part's memory."""
         n = ct.c_uint64()
-        ptr = lib().wor_part(self._w, p, ct.byref(n))
+        ptr = self.L.wor_part(self._w, p, ct.byref(n))
         dt = _DTYPES.get(p, np.uint32)
         count = n.value // np.dtype(dt).itemsize
         buf = (ct.c_char * n.value).from_address(ptr)
@@ -106,27 +122,36 @@ class OracleWorld:
         return self.part(p).copy()
 
     def step(self, params, ctl):
-        return lib().wor_fluid_step(self._w, ct.byref(params), ct.byref(ctl))
+        return self.L.wor_fluid_step(self._w, ct.byref(params), ct.byref(ctl))
 
     def capture(self, on=True):
-        lib().wor_capture(self._w, 1 if on else 0)
+        self.L.wor_capture(self._w, 1 if on else 0)
 
     def captured_b(self):
-        ptr = lib().wor_captured_b(self._w)
+        ptr = self.L.wor_captured_b(self._w)
         return np.ctypeslib.as_array(ptr, shape=(self.layout.cells,)).copy()
 
     def cg64(self, b, tol=1e-12, maxit=100000):
         b = np.ascontiguousarray(b, dtype=np.float32)
         x = np.zeros(self.layout.cells, dtype=np.float64)
         rel = ct.c_double()
-        it = lib().wor_cg64(self._w, b.ctypes.data_as(ct.POINTER(ct.c_float)),
+        it = self.L.wor_cg64(self._w, b.ctypes.data_as(ct.POINTER(ct.c_float)),
                             x.ctypes.data_as(ct.POINTER(ct.c_double)), tol, maxit, ct.byref(rel))
         return x, it, rel.value
 
     def relres(self, b, x):
         b = np.ascontiguousarray(b, dtype=np.float32)
         x = np.ascontiguousarray(x, dtype=np.float32)
-        return lib().wor_relres(self._w, b.ctypes.data_as(ct.POINTER(ct.c_float)), x.ctypes.data_as(ct.POINTER(ct.c_float)))
+        return self.L.wor_relres(self._w, b.ctypes.data_as(ct.POINTER(ct.c_float)), x.ctypes.data_as(ct.POINTER(ct.c_float)))
+
+    def run(self, pass_id, arg, params, ctl):
+        """One pass of the step on the buffers as they stand (wor_run)."""
+        st = self.L.wor_run(self._w, pass_id, arg, ct.byref(params), ct.byref(ctl))
+        assert st == 0, f"wor_run({pass_id}, {arg}): {st}"
+
+    def level_start(self, l):
+        """Where coarse level l starts in the field, in vec4s."""
+        return int(self.L.wor_level_start(self._w, l))
 
     def solve_stats(self):
         raw = self.part(abi.PART_SOLVE)[:64].tobytes()

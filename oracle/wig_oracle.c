@@ -34,6 +34,15 @@
 
 #include "../include/wig.h"
 
+/* Planted mistakes (test infrastructure): built with -DORACLE_MUTANT=k
+ * (oracle/Makefile's `mutants`), this restatement carries mistake k, one a
+ * build, which the pins in tests/ must catch (tests/test_oracle_mutants.py).
+ * 0, the default, plants none. */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#define MUTANT(k) (ORACLE_MUTANT == (k))
+
 /* ------------------------------------------------------------ constants */
 /* js/shared.js:270-289, fluid.wgsl:183-242,2062-2063,2221,2392,2579 */
 #define BLOCK 8
@@ -619,7 +628,7 @@ static v2 apart(OWorld *w, uint32_t k, uint32_t h, v2 pos, uint32_t id) {   /* :
         float l = wlength(d);
         if (l < APART) {
             v2 way = l == 0.0f ? pairWay(id, STATE(w->sorted, m, 2)) : V2(d.x / l, d.y / l);
-            float s = 0.5f * (APART - l) * APART_FIX;
+            float s = MUTANT(16) ? 0.0f : 0.5f * (APART - l) * APART_FIX;
             px += wi32(wround(s * way.x));
             py += wi32(wround(s * way.y));
         }
@@ -638,7 +647,8 @@ static uint32_t rankById(OWorld *w, uint32_t h, uint32_t k, uint32_t id, uint32_
     for (uint32_t m = start; m < start + CNT(h, G_SORT); m++) {
         uint32_t ox = STATE(w->sorted, m, 0), oz = STATE(w->sorted, m, 2);
         uint32_t theirs = wType(ox) == drawn ? 1u : 0u;
-        r += (theirs < mine || (theirs == mine && (oz < id || (oz == id && m < k)))) ? 1u : 0u;
+        if (MUTANT(17)) r += (theirs < mine || (theirs == mine && m < k)) ? 1u : 0u;
+        else r += (theirs < mine || (theirs == mine && (oz < id || (oz == id && m < k)))) ? 1u : 0u;
     }
     return r;
 }
@@ -753,8 +763,9 @@ static void k_compact(OWorld *w) {                           /* :896-921 */
     for (uint32_t n = 0; n < w->B; n++) {
         i2 b = I2((int32_t)(n % w->bw), (int32_t)(n / w->bw));
         uint32_t marks = 0;
-        for (int y = -1; y <= 1; y++)
-            for (int x = -1; x <= 1; x++) {
+        const int reach = MUTANT(20) ? 0 : 1;
+        for (int y = -reach; y <= reach; y++)
+            for (int x = -reach; x <= reach; x++) {
                 i2 o = I2(b.x + x, b.y + y);
                 if (inBlocks(w, o)) marks |= w->blocks[blockIndex(w, o)];
             }
@@ -921,7 +932,7 @@ static void k_classify(OWorld *w) {                          /* :1578-1696 */
                 float was[4], wasDrift[4];
                 for (int j = 0; j < 4; j++) { was[j] = FLD(2u * i, j); wasDrift[j] = FLD(2u * i + 1u, j); }
                 int sealed = 0;
-                if (n == 0 && mayBeSealed(w, p)) sealed = (ufLink(w, ufFind(w, i)) & REGION_OPEN) == 0u;
+                if (n == 0 && mayBeSealed(w, p) && !MUTANT(15)) sealed = (ufLink(w, ufFind(w, i)) & REGION_OPEN) == 0u;
                 if (n < 0) {
                     c[2] = -1.0f;
                 } else if (n > 0) {
@@ -942,7 +953,7 @@ static void k_classify(OWorld *w) {                          /* :1578-1696 */
                             mean.y += li.y;
                         }
                     }
-                    if (fluid >= 2.0f && fluid + nwalls >= 3.0f) {
+                    if (fluid >= (MUTANT(14) ? 1.0f : 2.0f) && fluid + nwalls >= 3.0f) {
                         c[2] = mean.x / fluid;
                         c[3] = mean.y / fluid;
                     } else if (sealed) {
@@ -1008,7 +1019,7 @@ static void setFace(OWorld *w, i2 at, int isU) {           /* :1936-1980 */
     }
     float now = u;
     if (held > 0.0f) now += 0.0f;                             /* the wind's drag: still air */
-    if (held > 0.0f && ga[2] >= 0.0f && gb[2] >= 0.0f) now += isU ? 0.0f : w->P->earthG;
+    if (held > 0.0f && ga[2] >= 0.0f && gb[2] >= 0.0f) now += (isU != MUTANT(21)) ? 0.0f : w->P->earthG;
     float start = now;
     FLD(faceSlot(w, f), 0) = start;
     FLD(faceSlot(w, f), 1) = u;
@@ -1049,7 +1060,7 @@ static void k_divergence(OWorld *w) {                        /* :1996-2025 */
         FLD(2u * i, 1) = out;
         float n = densityAt(w, c);
         float room = wclamp(1.0f, 0.1f, 1.9f);               /* posedRoom with no bodies */
-        FLD(2u * i + 1u, 1) = FLD(2u * i + 1u, 2) < 0.0f ? 0.0f : -SPREAD * (n - room);
+        FLD(2u * i + 1u, 1) = FLD(2u * i + 1u, 2) < 0.0f ? 0.0f : (MUTANT(24) ? SPREAD : -SPREAD) * (n - room);
     }
 }
 
@@ -1109,7 +1120,7 @@ static void smoothAt(OWorld *w, uint32_t l, i2 c, int color, float omega) {   /*
     if (rho <= 0.0f) return;
     Stencil s = stencil(w, l, c, rho);
     if (s.diag <= 0.0f) return;
-    float h2 = (float)(1u << (2u * l));
+    float h2 = MUTANT(5) ? 1.0f : (float)(1u << (2u * l));
     float ex = FLD(at, 0), ey = FLD(at, 1), ez = FLD(at, 2), ew = FLD(at, 3);
     float sx = (s.sum.x - h2 * ez) / s.diag, sy = (s.sum.y - h2 * ew) / s.diag;
     FLD(at, 0) = wmix(ex, sx, omega);
@@ -1139,7 +1150,7 @@ static void k_restrict(OWorld *w, uint32_t l) {              /* :2342-2377 */
             for (int j = 0; j < 4; j++) {
                 i2 k = I2(2 * c.x + (j & 1), 2 * c.y + (j >> 1));
                 float rho = rhoAt(w, l - 1u, k);
-                if (rho == 0.0f) air = 1;
+                if (rho == 0.0f) air = MUTANT(3) ? 0 : 1;
                 else if (rho > 0.0f) {
                     fluid += 1.0f;
                     sum += rho;
@@ -1151,8 +1162,9 @@ static void k_restrict(OWorld *w, uint32_t l) {              /* :2342-2377 */
             float rho = air ? 0.0f : (fluid > 0.0f ? sum / fluid : -1.0f);
             FLD(at, 0) = 0.0f;
             FLD(at, 1) = 0.0f;
-            FLD(at, 2) = rho > 0.0f ? r.x * 0.25f : 0.0f;
-            FLD(at, 3) = rho > 0.0f ? r.y * 0.25f : 0.0f;
+            const float quarter = MUTANT(4) ? 1.0f / fluid : 0.25f;
+            FLD(at, 2) = rho > 0.0f ? r.x * quarter : 0.0f;
+            FLD(at, 3) = rho > 0.0f ? r.y * quarter : 0.0f;
             FLD(at + 1u, 0) = rho;
             FLD(at + 1u, 1) = 0.0f;
             FLD(at + 1u, 2) = 0.0f;
@@ -1190,8 +1202,9 @@ static void k_coarsest(OWorld *w, uint32_t l) {              /* solve, :2407-245
                 diag += beta;
             }
             if (diag > 0.0f) {
-                pot[j][0] = wmix(pot[j][0], (sumx - h2 * rhs[j][0]) / diag, OMEGA);
-                pot[j][1] = wmix(pot[j][1], (sumy - h2 * rhs[j][1]) / diag, OMEGA);
+                const float omega = MUTANT(6) ? 1.0f : OMEGA;
+                pot[j][0] = wmix(pot[j][0], (sumx - h2 * rhs[j][0]) / diag, omega);
+                pot[j][1] = wmix(pot[j][1], (sumy - h2 * rhs[j][1]) / diag, omega);
             }
         }
     }
@@ -1205,7 +1218,7 @@ static v2 prolonged(const OWorld *w, uint32_t l, i2 c) {   /* :2478-2497 */
     i2 coarse = I2(c.x / 2, c.y / 2);
     i2 s = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
     const i2 offsets[4] = {{0, 0}, {s.x, 0}, {0, s.y}, {s.x, s.y}};
-    const float weights[4] = {9.0f, 3.0f, 3.0f, 1.0f};
+    const float weights[4] = {9.0f, MUTANT(2) ? 9.0f : 3.0f, MUTANT(2) ? 9.0f : 3.0f, MUTANT(2) ? 9.0f : 1.0f};
     v2 sum = V2(0.0f, 0.0f);
     float weight = 0.0f;
     for (int k = 0; k < 4; k++) {
@@ -1307,7 +1320,7 @@ static void k_relax(OWorld *w, int shift) {
                 rhsx[li] = FLD(i, 1);
                 rhsy[li] = FLD(i + 1u, 1);
                 float d = share[t];
-                beta[li][0] = fineBeta(r0, rho[t - 1], d, share[t - 1]);
+                beta[li][0] = MUTANT(19) ? faceBeta(r0, rho[t - 1]) : fineBeta(r0, rho[t - 1], d, share[t - 1]);
                 beta[li][1] = fineBeta(r0, rho[t + 1], d, share[t + 1]);
                 beta[li][2] = fineBeta(r0, rho[t - RIM], d, share[t - RIM]);
                 beta[li][3] = fineBeta(r0, rho[t + RIM], d, share[t + RIM]);
@@ -1669,7 +1682,7 @@ static void projectFace(OWorld *w, i2 at, int isU) {        /* :2897-2923 */
     gridAt4(w, at, gb);
     float beta = fineBeta(ga[2], gb[2], densityAt(w, a), densityAt(w, at));
     if (beta > 0.0f) {
-        FLD(f, 0) = FLD(f, 0) - beta * (gb[0] - ga[0]);
+        FLD(f, 0) = MUTANT(13) ? FLD(f, 0) + beta * (gb[0] - ga[0]) : FLD(f, 0) - beta * (gb[0] - ga[0]);
         FLD(f, 2) = PROJECTED;
         FLD(f, 3) = wclamp(-beta * (driftAt(w, at) - driftAt(w, a)), -MAX_DRIFT, MAX_DRIFT);
     }
@@ -1690,7 +1703,7 @@ static void extrapolateFace(OWorld *w, i2 at, int isU, int layer) {   /* :3226-3
     uint64_t f = faceSlot(w, faceIndex(w, at, isU));
     i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
     if (FLD(f, 2) != 1.0f || gridZ(w, a) != 0.0f || gridZ(w, at) != 0.0f) return;
-    float reached = layer == 2 ? 1.5f : PROJECTED;
+    float reached = layer == 2 && !MUTANT(12) ? 1.5f : PROJECTED;
     float sum = 0.0f, n = 0.0f;
     for (int d = 0; d < 4; d++) {
         i2 q = beside(at, d);
@@ -1752,14 +1765,15 @@ static Sample sampleAxis(OWorld *w, v2 s, int isU) {         /* :3480-3512 */
     return r;
 }
 static inline v2 clampSpeed(v2 v) {                           /* :437-440 */
+    const float most = MUTANT(7) ? 16.0f : LIQUID_MAX_SPEED;
     float s = wlength(v);
-    return s > LIQUID_MAX_SPEED ? V2(v.x * (LIQUID_MAX_SPEED / s), v.y * (LIQUID_MAX_SPEED / s)) : v;
+    return s > most ? V2(v.x * (most / s), v.y * (most / s)) : v;
 }
 static int escapesTo(OWorld *w, i2 q) {                      /* :3535-3541, no bodies */
     return inWorld(w, q) && !(gridZ(w, q) < 0.0f);
 }
 static v2 escape(OWorld *w, v2 pos) {                        /* :3552-3594 */
-    if (!isObstaclePos(w, pos)) return pos;
+    if (!isObstaclePos(w, pos) || MUTANT(22)) return pos;
     i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
     v2 best = pos;
     float bestD = 1e9f;
@@ -1809,7 +1823,8 @@ static void k_advect(OWorld *w) {                            /* :3616-3694 */
         Sample v = sampleAxis(w, V2(q.pos.x - 0.5f, q.pos.y - 0.0f), 0);
         v2 flow = V2(u.weight > 0.0f ? u.now : q.vel.x, v.weight > 0.0f ? v.now : q.vel.y);
         v2 vel = clampSpeed(flow);
-        float affine[4] = {AFFINE_KEEP * u.gradient.x, AFFINE_KEEP * u.gradient.y, AFFINE_KEEP * v.gradient.x, AFFINE_KEEP * v.gradient.y};
+        const float keep = MUTANT(1) ? 1.0f : AFFINE_KEEP;
+        float affine[4] = {keep * u.gradient.x, keep * u.gradient.y, keep * v.gradient.x, keep * v.gradient.y};
         v2 cs = clampSpeed(vel), slip = slipOf(w, &q);
         v2 travel = V2(cs.x + wclamp(u.drift + slip.x, -MAX_DRIFT, MAX_DRIFT), cs.y + wclamp(v.drift + slip.y, -MAX_DRIFT, MAX_DRIFT));
         int32_t steps = wi32(ceilf(fmaxf(fabsf(travel.x), fabsf(travel.y)) * 2.0f));
@@ -1819,7 +1834,7 @@ static void k_advect(OWorld *w) {                            /* :3616-3694 */
         for (int32_t s = 0; s < steps; s++) {
             v2 nx = V2(pos.x + d.x, pos.y + 0.0f);
             if (d.x != 0.0f && isObstaclePos(w, nx)) {
-                if (vel.x * d.x > 0.0f) vel.x = 0.0f;
+                if (vel.x * d.x > 0.0f && !MUTANT(8)) vel.x = 0.0f;
                 d.x = 0.0f;
             } else {
                 pos = nx;
@@ -1869,7 +1884,7 @@ static void k_scanBlocks(OWorld *w) {                        /* :3730-3753 */
             run += n;
             if (inWorld(w, c)) {
                 uint32_t i = cellIndex(w, c);
-                CNT(i, G_START) = run - n;
+                CNT(i, G_START) = MUTANT(9) ? run : run - n;
                 uint32_t came = CNT(i, G_CAME);
                 CNT(i, G_CAME) = 0u;
                 if (came == CAME_FALLING) FLD(2u * i + 1u, 3) = (float)(wu32(FLD(2u * i + 1u, 3)) | fallingBit(w, 0u));
@@ -1998,16 +2013,16 @@ static void k_raster(OWorld *w) {                            /* :3983-4119 */
         }
         uint32_t most = kinds[0], votes = counts[0];
         for (uint32_t m = 1; m < RASTER_KINDS; m++)
-            if (counts[m] > votes) { most = kinds[m]; votes = counts[m]; }
+            if (counts[m] > votes || (MUTANT(10) && counts[m] == votes)) { most = kinds[m]; votes = counts[m]; }
         uint32_t word = 0u, lowest = NONE, solute = 0u, soluteId = NONE;
         int solutes = holdsSolute(most);
         for (uint32_t k = start; k < start + n; k++) {
             uint32_t sx = STATE(w->sorted, k, 0), sz = STATE(w->sorted, k, 2);
-            if (wType(sx) == most && (lowest == NONE || sz < lowest)) { word = sx; lowest = sz; }
+            if (wType(sx) == most && (lowest == NONE || (MUTANT(11) ? sz > lowest : sz < lowest))) { word = sx; lowest = sz; }
             if (solutes && wType(sx) == most && wAux(sx) != 0u && (soluteId == NONE || sz < soluteId)) { solute = wAux(sx); soluteId = sz; }
         }
         if (solutes) word = packWord(wType(word), solute, wLife(word), wShade(word), wFlags(word));
-        float livesx = 0.0f, livesy = 0.0f;
+        float livesx = 0.0f, livesy = 0.0f, plainX = 0.0f;
         u2 heat = {0, 0}, capOf = {0, 0}, velX = {0, 0}, velY = {0, 0};
         for (uint32_t k = start; k < start + n; k++) {
             uint32_t sx = STATE(w->sorted, k, 0), pt = wType(sx);
@@ -2017,6 +2032,7 @@ static void k_raster(OWorld *w) {                            /* :3983-4119 */
             heat = addExact(heat, hc * fbits(STATE(w->sorted, k, 1)));
             capOf = addExact(capOf, hc);
             velX = addExact(velX, fbits(MOTION(w->sorted, k, 2)));
+            plainX += fbits(MOTION(w->sorted, k, 2));
             velY = addExact(velY, fbits(MOTION(w->sorted, k, 3)));
         }
         Rng r;
@@ -2025,7 +2041,7 @@ static void k_raster(OWorld *w) {                            /* :3983-4119 */
         life = life < 255u ? life : 255u;
         word = packWord(wType(word), wAux(word), life, wShade(word), wFlags(word));
         float temp = exactValue(heat, EXACT_FINE) / exactValue(capOf, EXACT_FINE);
-        v2 vel = V2(exactValue(velX, EXACT_FINE) / (float)n, exactValue(velY, EXACT_FINE) / (float)n);
+        v2 vel = V2((MUTANT(18) ? plainX : exactValue(velX, EXACT_FINE)) / (float)n, exactValue(velY, EXACT_FINE) / (float)n);
         w->cells[4ull * i + 0] = wWith(word, FL_DRAWN);
         w->cells[4ull * i + 1] = ubits(temp);
         w->cells[4ull * i + 2] = pack2x16(vel);
@@ -2038,7 +2054,7 @@ static void k_unpaint(OWorld *w) {                           /* :4163-4182 */
         if (!inWorld(w, c)) continue;
         uint32_t i = cellIndex(w, c);
         uint32_t vx = w->cells[4ull * i];
-        if (kindOf(w, wType(vx)) == KIND_LIQUID && CNT(i, G_SORT) == 0u && CNT(i, G_TAKEN) == 0u) {
+        if (kindOf(w, wType(vx)) == KIND_LIQUID && CNT(i, G_SORT) == 0u && (CNT(i, G_TAKEN) == 0u || MUTANT(23))) {
             float temp = effTemp(fbits(w->cells[4ull * i + 1]));
             w->cells[4ull * i + 0] = E_AIR;
             w->cells[4ull * i + 1] = ubits(temp);
@@ -2389,3 +2405,59 @@ float wor_f32(uint16_t h) { return f32_of(h); }
 void wor_exact_parts(float x, float scale, uint32_t *out) { u2 p = exactParts(x, scale); out[0] = p.lo; out[1] = p.hi; }
 float wor_exact_value(uint32_t lo, uint32_t hi, float scale) { u2 w = {lo, hi}; return exactValue(w, scale); }
 float wor_fine_beta(float a, float b, float da, float db) { return fineBeta(a, b, da, db); }
+
+/* --------------------------------------------- one pass (test hooks)
+ * The tests that pin a pass by itself build a state by hand and run that
+ * pass alone on it, with the step's params and controls (iteration 0 of the
+ * solve, not converged).  WOR_PASS_* ids; `arg` is the pass's level (restrict,
+ * smooth, coarsest, prolong), layer (extrapolate), or whether sync takes claims;
+ * smooth takes the colour in arg's bit 8.  The sort runs sortCount through
+ * scatter, the seal sealJoin then sealAnchor. */
+enum {
+    WOR_PASS_SYNC, WOR_PASS_SYNC_PARTICLES, WOR_PASS_SYNC_SEED, WOR_PASS_COMPACT, WOR_PASS_RETIRE, WOR_PASS_CLEAR,
+    WOR_PASS_GATHER, WOR_PASS_SEAL, WOR_PASS_CLASSIFY, WOR_PASS_SET_FACES, WOR_PASS_UNPRESS, WOR_PASS_DIVERGENCE,
+    WOR_PASS_RESTRICT, WOR_PASS_SMOOTH, WOR_PASS_COARSEST, WOR_PASS_PROLONG, WOR_PASS_PROLONG_FINE, WOR_PASS_PROJECT,
+    WOR_PASS_EXTRAPOLATE, WOR_PASS_ADVECT, WOR_PASS_SORT, WOR_PASS_RASTER_CLAIMS, WOR_PASS_RASTER, WOR_PASS_UNPAINT,
+    WOR_PASS_RELAX, WOR_PASSES
+};
+int wor_run(OWorld *w, int pass, int arg, const WigParams *P, const WigStepCtl *ctl) {
+    if (pass < 0 || pass >= WOR_PASSES) return WIG_E_INVALID;
+    w->P = P;
+    w->ctl = ctl;
+    w->cycle = 0;
+    w->converged_flag = 0;
+    const uint32_t l = (uint32_t)arg & 0xffu;
+    if ((pass == WOR_PASS_RESTRICT || pass == WOR_PASS_SMOOTH || pass == WOR_PASS_COARSEST || pass == WOR_PASS_PROLONG) &&
+        (l < 1u || l > w->L || (pass == WOR_PASS_RESTRICT && l < 1u) || (pass == WOR_PASS_PROLONG && l >= w->L)))
+        return WIG_E_INVALID;
+    switch (pass) {
+    case WOR_PASS_SYNC: k_sync(w, arg); break;
+    case WOR_PASS_SYNC_PARTICLES: k_syncParticles(w); break;
+    case WOR_PASS_SYNC_SEED: k_syncSeed(w); break;
+    case WOR_PASS_COMPACT: k_compact(w); break;
+    case WOR_PASS_RETIRE: k_retire(w); break;
+    case WOR_PASS_CLEAR: k_clear(w); break;
+    case WOR_PASS_GATHER: k_gather(w); break;
+    case WOR_PASS_SEAL: k_sealJoin(w); k_sealAnchor(w); break;
+    case WOR_PASS_CLASSIFY: k_classify(w); break;
+    case WOR_PASS_SET_FACES: k_setFaces(w); break;
+    case WOR_PASS_UNPRESS: k_unpress(w); break;
+    case WOR_PASS_DIVERGENCE: k_divergence(w); break;
+    case WOR_PASS_RESTRICT: k_restrict(w, l); break;
+    case WOR_PASS_SMOOTH: k_smooth(w, l, (arg >> 8) & 1); break;
+    case WOR_PASS_COARSEST: k_coarsest(w, l); break;
+    case WOR_PASS_PROLONG: k_prolong(w, l); break;
+    case WOR_PASS_PROLONG_FINE: k_prolongFine(w); break;
+    case WOR_PASS_PROJECT: k_project(w); break;
+    case WOR_PASS_EXTRAPOLATE: k_extrapolate(w, arg == 2 ? 2 : 1); break;
+    case WOR_PASS_ADVECT: k_advect(w); break;
+    case WOR_PASS_SORT: k_sortCount(w); k_scanBlocks(w); k_scanTotals(w); k_scanFinish(w); k_scatter(w); break;
+    case WOR_PASS_RASTER_CLAIMS: k_rasterClaims(w); break;
+    case WOR_PASS_RASTER: k_raster(w); break;
+    case WOR_PASS_UNPAINT: k_unpaint(w); break;
+    case WOR_PASS_RELAX: k_relax(w, arg ? BLOCK / 2 : 0); break;
+    }
+    return WIG_OK;
+}
+/* Where coarse level l (1 .. levels) starts in the field, in vec4s. */
+uint64_t wor_level_start(const OWorld *w, uint32_t l) { return l >= 1u && l <= w->L ? w->lstart[l] : 0u; }

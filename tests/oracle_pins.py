@@ -1,0 +1,898 @@
+"""Pins of the oracle's passes (oracle/wig_oracle.c), one pass at a time:
+each builds a state by hand, runs the pass alone (wor_run), and checks what
+the mathematics or the reference's own rule fixes without restating the
+pass -- known answers worked by hand, closed forms (an affine field
+prolonged exactly, a rigid rotation carried by APIC's C), invariants
+(Gauss-Seidel leaves the swept colour's equations solved, a projection
+leaves the liquid divergence-free, a sort is a permutation into contiguous
+runs) and brute force (the coarsest SOR against a dense solve).
+
+Each pin takes the oracle library it checks, so that tests/ can run it both
+on the oracle (it must pass) and on builds of the oracle with a planted
+mistake (`make -C oracle mutants`: each must fail at least one pin)."""
+import numpy as np
+
+from oracle.oracle import (OracleWorld, PASS_ADVECT, PASS_CLASSIFY, PASS_COARSEST, PASS_COMPACT, PASS_DIVERGENCE,
+                           PASS_EXTRAPOLATE, PASS_PROLONG, PASS_PROLONG_FINE, PASS_RASTER, PASS_RELAX, PASS_RESTRICT,
+                           PASS_SEAL, PASS_SET_FACES, PASS_SMOOTH, PASS_SORT, PASS_SYNC_PARTICLES, PASS_UNPAINT)
+from worldinagrain_amd import abi, scenes
+from worldinagrain_amd import elements as E
+from worldinagrain_amd.host import Simulation, SolverSettings
+
+FIX = 65536.0
+ACTIVE = 1
+G_COUNT, G_REGION, G_SORT, G_START, G_DEST, G_TYPE, G_DENSITY, G_MASS, G_RIM = 0, 1, 2, 3, 4, 5, 6, 7, 9
+G_TAKEN = 1
+HAS_FLUID, NEAR_LIQUID = 2, 8
+MARK_LIQUID, MARK_REGION = 1, 2
+DEST_KEEP, DEST_CELL, DEST_SPAN = 0xFFFFFFFF, 0x40000000, 0x3FFFFFFF
+FL_DRAWN = 4
+APART = 1.0 / 256.0
+
+
+def face_beta(a, b):
+    """faceBeta (fluid.wgsl:369-374), pinned by closed forms in test_oracle."""
+    if a < 0 or b < 0 or (a <= 0 and b <= 0):
+        return 0.0
+    return 2.0 / (a + b) if a > 0 and b > 0 else 1.0 / max(a, b)
+
+
+def word(t, shade=0, flags=0):
+    """A cell / particle word: type, aux 0, life 0, shade, flags (js/shared.js:79)."""
+    return (t & 0xFF) | ((shade & 0xF) << 24) | ((flags & 0xF) << 28)
+
+
+class Hand:
+    """A world in the oracle with numpy views on its buffers, to build a
+    state by hand and run one pass on it."""
+
+    def __init__(self, W, H, coarse=0, L=None, kinds=None):
+        self.w = OracleWorld(W, H, coarse_levels=coarse, kinds=E.kinds() if kinds is None else kinds, L=L)
+        lay = self.w.layout
+        self.W, self.H, self.C, self.F, self.B, self.cap = W, H, lay.cells, lay.faces, lay.blocks, lay.capacity
+        self.lw = list(lay.level_width)
+        self.lh = list(lay.level_height)
+        self.field = self.w.part(abi.PART_FIELD)
+        self.counters = self.w.part(abi.PART_COUNTERS)
+        self.blocks = self.w.part(abi.PART_BLOCKS)
+        self.cellwords = self.w.part(abi.PART_CELLS)
+        self.frame = 1
+        self.earth_g = 0.0
+
+    # views
+    def cell(self):
+        """(H, W, 8): p, div, rho, nu | drift, drift rhs, sealed code, step bits."""
+        return self.field[:8 * self.C].reshape(self.H, self.W, 8)
+
+    def cnt(self):
+        return self.counters[:11 * self.C].reshape(self.H, self.W, 11)
+
+    def words(self):
+        return self.cellwords[:4 * self.C].reshape(self.H, self.W, 4)
+
+    def ufaces(self):
+        """(H, W + 1, 4): u now, u before, held, drift; face (x, y) is the left side of cell (x, y)."""
+        a = 8 * self.C
+        return self.field[a:a + 4 * (self.W + 1) * self.H].reshape(self.H, self.W + 1, 4)
+
+    def vfaces(self):
+        """(H + 1, W, 4); face (x, y) is the top side of cell (x, y)."""
+        a = 8 * self.C + 4 * (self.W + 1) * self.H
+        return self.field[a:a + 4 * self.W * (self.H + 1)].reshape(self.H + 1, self.W, 4)
+
+    def level(self, l):
+        """(h_l, w_l, 8): correction (p, drift), rhs (p, drift) | rho, 0, 0, 0."""
+        s = self.w.level_start(l)
+        n = self.lw[l] * self.lh[l]
+        return self.field[4 * s:4 * (s + 2 * n)].reshape(self.lh[l], self.lw[l], 8)
+
+    def particles(self, part):
+        """(count view, motion f32 (cap, 4), state u32 (cap, 4), affine f32 (cap, 4))."""
+        p = self.w.part(part)
+        items = p[4:4 + 12 * self.cap].reshape(3, self.cap, 4)
+        return p, items[0].view(np.float32), items[1], items[2].view(np.float32)
+
+    def activate(self, blocks=None):
+        """Blocks worked on (all by default), listed in block order."""
+        B = self.B
+        which = list(range(B)) if blocks is None else sorted(blocks)
+        self.blocks[B:2 * B] = 0
+        for n, b in enumerate(which):
+            self.blocks[B + b] = ACTIVE
+            self.blocks[2 * B + n] = b
+        self.blocks[4 * B] = len(which)
+
+    def params(self):
+        p = abi.WigParams()
+        p.size[0], p.size[1] = self.W, self.H
+        p.frame = self.frame
+        p.seed = 0x1234
+        p.dx = 1.0
+        p.earthG = self.earth_g
+        p.dt = 1.0
+        return p
+
+    def ctl(self, coarse_sweeps=40):
+        c = abi.WigStepCtl()
+        c.full = 1
+        c.fluid_cycles = 1
+        c.tol_mode = abi.WIG_TOL_ABS_MAX
+        c.tol = 1e-2
+        c.pre_relax = c.post_relax = 1
+        c.sweep_pairs = 4
+        c.coarse_sweeps = coarse_sweeps
+        return c
+
+    def run(self, pass_id, arg=0, **ctl):
+        self.w.run(pass_id, arg, self.params(), self.ctl(**ctl))
+
+
+# ------------------------------------------------------------ V-cycle
+def pin_restrict(L):
+    """restrictLevel (fluid.wgsl:2336-2377), worked by hand: a coarse cell is
+    air if any of its four is air, else the mean density of its fluid ones,
+    else an obstacle; its rhs is a quarter of its fluid children's residuals;
+    over a block not worked on it is air; and the residual's stencil on a
+    coarse level carries 1/h^2 (h = 2^l)."""
+    h = Hand(16, 16, coarse=2, L=L)
+    h.activate([0, 1, 2])                 # block (1, 1) not worked on
+    c = h.cell()
+    cases = {   # coarse cell: children's rho and rhs in the order (0,0) (1,0) (0,1) (1,1); the coarse rho and rhs
+        (0, 0): ((1, 1, 1, 1), (1, 2, 3, 4), 1.0, 2.5),
+        (1, 0): ((1, 0, 1, 1), (1, 2, 3, 4), 0.0, 0.0),
+        (2, 0): ((2, -1, 1, -1), (4, 9, 8, 9), 1.5, 3.0),
+        (3, 0): ((-1, -1, -1, -1), (5, 5, 5, 5), -1.0, 0.0),
+        (0, 1): ((0.5, 1.5, 1, 1), (1, 1, 1, 1), 1.0, 1.0),
+        (1, 1): ((1, 1, 1, 1), (-2, 2, -2, 6), 1.0, 1.0),
+        (5, 5): ((1, 1, 1, 1), (1, 1, 1, 1), 0.0, 0.0),     # over block (1, 1)
+    }
+    for (X, Y), (rho, rhs, _, _) in cases.items():
+        for j in range(4):
+            x, y = 2 * X + (j & 1), 2 * Y + (j >> 1)
+            c[y, x, 2] = rho[j]
+            c[y, x, 1] = rhs[j]
+    h.run(PASS_RESTRICT, 1)
+    lv = h.level(1)
+    for (X, Y), (_, _, rho, rhs) in cases.items():
+        assert lv[Y, X, 4] == rho, f"coarse ({X},{Y}) rho {lv[Y, X, 4]}, want {rho}"
+        assert lv[Y, X, 2] == rhs, f"coarse ({X},{Y}) rhs {lv[Y, X, 2]}, want {rhs}"
+        assert lv[Y, X, 0] == 0.0 and lv[Y, X, 1] == 0.0
+
+    # level 1 -> 2 with p = x^2 on an all-fluid level 1 (8 x 8, rhs 0): the
+    # residual is -(sum_n p_n - 4 p) / 4, i.e. -2/4 inside, -1/4 at x = 0 and
+    # +13/4 at x = 7 (one neighbour each way missing)
+    h = Hand(16, 16, coarse=2, L=L)
+    h.activate()
+    l1 = h.level(1)
+    l1[..., 4] = 1.0
+    l1[..., 0] = (np.arange(8, dtype=np.float32) ** 2)[None, :]
+    h.run(PASS_RESTRICT, 2)
+    l2 = h.level(2)
+    want = np.array([-0.375, -0.5, -0.5, 1.375], np.float32)
+    assert np.array_equal(l2[..., 2], np.tile(want, (4, 1))), l2[..., 2]
+    assert np.all(l2[..., 4] == 1.0)
+
+
+def pin_smooth(L):
+    """smoothRed/Black (fluid.wgsl:2317-2387): Gauss-Seidel with omega 1
+    leaves every cell of the colour swept satisfying its own equation on
+    level l, rhs = (sum_n beta (p_n - p)) / h^2 with h = 2^l, and changes no
+    cell of the other colour."""
+    h = Hand(64, 64, coarse=2, L=L)
+    h.activate()
+    rng = np.random.default_rng(7)
+    for l in (1, 2):
+        lv = h.level(l)
+        hh, ww = lv.shape[:2]
+        kind = rng.random((hh, ww))
+        lv[..., 4] = np.where(kind < 0.1, -1.0, np.where(kind < 0.2, 0.0, rng.uniform(0.8, 1.25, (hh, ww))))
+        lv[..., 0] = rng.normal(0, 1, (hh, ww))
+        lv[..., 1] = rng.normal(0, 1, (hh, ww))
+        lv[..., 2] = rng.normal(0, 1, (hh, ww))
+        lv[..., 3] = rng.normal(0, 1, (hh, ww))
+        h2 = float(4 ** l)
+        for color in (0, 1):
+            before = lv.copy()
+            h.run(PASS_SMOOTH, l | (color << 8))
+            rho = lv[..., 4]
+            for y in range(hh):
+                for x in range(ww):
+                    mine = ((x + y) & 1) == color
+                    if not mine or rho[y, x] <= 0:
+                        assert np.array_equal(lv[y, x].view(np.uint32), before[y, x].view(np.uint32)), (l, x, y)
+                        continue
+                    for ch in (0, 1):
+                        s = diag = scale = 0.0
+                        for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                            xx, yy = x + dx, y + dy
+                            if not (0 <= xx < ww and 0 <= yy < hh):
+                                continue
+                            b = face_beta(float(rho[y, x]), float(rho[yy, xx]))
+                            s += b * float(lv[yy, xx, ch])
+                            diag += b
+                            scale += abs(b * float(lv[yy, xx, ch]))
+                        if diag <= 0:
+                            continue
+                        res = float(lv[y, x, 2 + ch]) - (s - diag * float(lv[y, x, ch])) / h2
+                        assert abs(res) <= 1e-5 * (1.0 + scale / h2 + abs(float(lv[y, x, 2 + ch]))), (l, x, y, ch, res)
+
+
+def _coarsest_exact(lv, h2):
+    """The coarsest level's equations solved densely in float64: for each
+    fluid cell, sum_n beta (e - e_n) = -h^2 rhs, air neighbours held at 0,
+    obstacles and cells past the edge left out."""
+    hh, ww = lv.shape[:2]
+    rho = lv[..., 4].astype(np.float64)
+    idx = -np.ones((hh, ww), int)
+    fluid = np.argwhere(rho > 0)
+    for k, (y, x) in enumerate(fluid):
+        idx[y, x] = k
+    n = len(fluid)
+    A = np.zeros((n, n))
+    for k, (y, x) in enumerate(fluid):
+        for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+            xx, yy = x + dx, y + dy
+            if not (0 <= xx < ww and 0 <= yy < hh) or rho[yy, xx] < 0:
+                continue
+            b = face_beta(rho[y, x], rho[yy, xx])
+            A[k, k] += b
+            if idx[yy, xx] >= 0:
+                A[k, idx[yy, xx]] -= b
+    out = []
+    for ch in (0, 1):
+        rhs = np.array([-h2 * float(lv[y, x, 2 + ch]) for y, x in fluid])
+        out.append(np.linalg.solve(A, rhs))
+    return fluid, out
+
+
+def pin_coarsest(L):
+    """solve (fluid.wgsl:2389-2456), by brute force: 40 SOR sweep pairs at
+    omega 1.8 on the coarsest level land on the dense solution of its
+    equations (4 x 4 to 32 x 16 cells, air along the top and in scattered
+    cells, isolated obstacles, random rhs in both channels).  Red-black
+    ordering is consistent, so above the optimal omega SOR contracts the
+    error by omega - 1 = 0.8 an iteration: 0.8^40 = 1.3e-4 after 40 pairs
+    (Young's theorem), which the bound 5e-4 leaves room for."""
+    rng = np.random.default_rng(11)
+    for (W, H, coarse, tol) in ((16, 16, 2, 5e-4), (32, 32, 2, 5e-4), (64, 64, 2, 5e-4), (128, 64, 2, 5e-4)):
+        h = Hand(W, H, coarse=coarse, L=L)
+        h.activate()
+        lv = h.level(coarse)
+        hh, ww = lv.shape[:2]
+        rho = np.ones((hh, ww), np.float32)
+        yy, xx = np.mgrid[0:hh, 0:ww]
+        rho[(xx % 4 == 2) & (yy % 4 == 2)] = -1.0
+        rho[0, :] = 0.0
+        rho[(rng.random((hh, ww)) < 0.05) & (rho > 0)] = 0.0
+        lv[..., 4] = rho
+        lv[..., 0:2] = 0.0
+        lv[..., 2] = rng.normal(0, 1, (hh, ww))
+        lv[..., 3] = rng.normal(0, 1, (hh, ww))
+        h.run(PASS_COARSEST, coarse)
+        fluid, exact = _coarsest_exact(lv, float(4 ** coarse))
+        for ch in (0, 1):
+            got = np.array([lv[y, x, ch] for y, x in fluid], np.float64)
+            err = np.linalg.norm(got - exact[ch]) / np.linalg.norm(exact[ch])
+            assert err < tol, f"{ww}x{hh} channel {ch}: SOR {err:.2e} from the dense solution"
+
+
+def pin_two_grid(L):
+    """The V-cycle's operators composed (fluid.wgsl:2317-2497): on the model
+    problem -- a 32 x 32 level of water under a row of air, the error
+    equation (rhs 0) from a random start -- one two-grid cycle (red-black
+    pairs, restrictLevel, the coarsest SOR, prolong, red-black pairs)
+    contracts the error by a factor that settles at 0.29 with one pair each
+    side and 0.23 with two, as cell-centred multigrid with these transfer
+    operators does; the pin allows 0.35 and 0.30."""
+    for pairs, bound in ((1, 0.35), (2, 0.30)):
+        h = Hand(64, 64, coarse=2, L=L)
+        h.activate()
+        l1 = h.level(1)
+        hh, ww = l1.shape[:2]
+        rho = np.ones((hh, ww), np.float32)
+        rho[0, :] = 0.0
+        l1[..., 4] = rho
+        l1[..., 0] = np.where(rho > 0, np.random.default_rng(5).normal(0, 1, (hh, ww)), 0.0)
+        norms = [np.linalg.norm(l1[..., 0])]
+        for _ in range(8):
+            for _ in range(pairs):
+                h.run(PASS_SMOOTH, 1)
+                h.run(PASS_SMOOTH, 1 | (1 << 8))
+            h.run(PASS_RESTRICT, 2)
+            h.run(PASS_COARSEST, 2)
+            h.run(PASS_PROLONG, 1)
+            for _ in range(pairs):
+                h.run(PASS_SMOOTH, 1)
+                h.run(PASS_SMOOTH, 1 | (1 << 8))
+            norms.append(np.linalg.norm(l1[..., 0]))
+        rates = [norms[i + 1] / norms[i] for i in range(len(norms) - 1)]
+        assert max(rates[-4:]) < bound, (pairs, rates)
+
+
+def _affine(X, Y, ch):
+    return (1.0 + 0.5 * X - 0.25 * Y) if ch == 0 else (2.0 - 0.125 * X + 0.375 * Y)
+
+
+def pin_prolong(L):
+    """prolong (fluid.wgsl:2458-2497): bilinear 9:3:3:1 interpolation
+    reproduces an affine correction exactly at every cell whose four coarse
+    neighbours are in the level; near obstacles and air, hand-worked cells:
+    obstacles drop out of the weights, air adds none."""
+    h = Hand(64, 64, coarse=2, L=L)
+    h.activate()
+    l1, l2 = h.level(1), h.level(2)
+    rng = np.random.default_rng(3)
+    Y2, X2 = np.mgrid[0:l2.shape[0], 0:l2.shape[1]].astype(np.float32)
+    l2[..., 4] = 1.0
+    for ch in (0, 1):
+        l2[..., ch] = _affine(X2, Y2, ch)
+    l1[..., 4] = 1.0
+    q = rng.normal(0, 1, l1.shape[:2] + (2,)).astype(np.float32)
+    l1[..., 0:2] = q
+    h.run(PASS_PROLONG, 1)
+    hh, ww = l1.shape[:2]
+    for y in range(1, hh - 1):
+        for x in range(1, ww - 1):
+            X, Y = (x - 0.5) / 2.0, (y - 0.5) / 2.0      # the fine cell's middle in coarse cells
+            for ch in (0, 1):
+                got = float(l1[y, x, ch]) - float(q[y, x, ch])
+                assert abs(got - _affine(X, Y, ch)) < 2e-5, (x, y, ch, got, _affine(X, Y, ch))
+
+    # hand cases about coarse cell (5, 5): (4, 5) air, (5, 4) an obstacle, the rest fluid at 2
+    h = Hand(64, 64, coarse=2, L=L)
+    h.activate()
+    l1, l2 = h.level(1), h.level(2)
+    l2[..., 4] = 1.0
+    l2[..., 0] = 2.0
+    l2[5, 4, 4], l2[5, 4, 0] = 0.0, 1000.0     # air: its correction is none, whatever it holds
+    l2[4, 5, 4] = -1.0
+    l1[..., 4] = 1.0
+    l1[11, 12, 4] = 0.0                         # a fine cell of air: left alone
+    l1[11, 12, 0] = 7.0
+    h.run(PASS_PROLONG, 1)
+    for (x, y), want in (((10, 10), 20.0 / 13.0), ((11, 10), 2.0), ((10, 11), 26.0 / 16.0), ((11, 11), 2.0)):
+        assert abs(float(l1[y, x, 0]) - want) < 1e-6, ((x, y), float(l1[y, x, 0]), want)
+    assert l1[11, 12, 0] == 7.0
+
+    # prolongFine: level 1 onto the fine pressure and drift
+    h = Hand(32, 32, coarse=1, L=L)
+    h.activate()
+    c, l1 = h.cell(), h.level(1)
+    Y1, X1 = np.mgrid[0:16, 0:16].astype(np.float32)
+    l1[..., 4] = 1.0
+    l1[..., 0] = _affine(X1, Y1, 0)
+    l1[..., 1] = _affine(X1, Y1, 1)
+    c[..., 2] = 1.0
+    h.run(PASS_PROLONG_FINE)
+    for y in range(1, 31):
+        for x in range(1, 31):
+            X, Y = (x - 0.5) / 2.0, (y - 0.5) / 2.0
+            assert abs(float(c[y, x, 0]) - _affine(X, Y, 0)) < 2e-5, (x, y)
+            assert abs(float(c[y, x, 4]) - _affine(X, Y, 1)) < 2e-5, (x, y)
+
+
+# ------------------------------------------------------------ G2P / advection
+def _advect_world(L, u, v, held=2.0):
+    """48 x 48 of water cells in the solve, every block worked on, no gravity,
+    face velocities u(x, y) and v(x, y) from the faces' positions."""
+    h = Hand(48, 48, L=L)
+    h.activate()
+    h.words()[..., 0] = word(E.WATER)
+    h.cell()[..., 2] = 1.0
+    uf, vf = h.ufaces(), h.vfaces()
+    yu, xu = np.mgrid[0:48, 0:49].astype(np.float64)
+    yv, xv = np.mgrid[0:49, 0:48].astype(np.float64)
+    uf[..., 0] = u(xu, yu + 0.5)        # a u face sits at (x, y + 1/2), a v face at (x + 1/2, y)
+    vf[..., 0] = v(xv + 0.5, yv)
+    uf[..., 2] = held
+    vf[..., 2] = held
+    return h
+
+
+def _advect(h, pos, vel=(0.0, 0.0)):
+    cnt, motion, state, affine = h.particles(abi.PART_PARTICLES1)
+    n = len(pos)
+    cnt[0] = n
+    motion[:n, 0:2] = np.asarray(pos, np.float32)
+    motion[:n, 2:4] = np.asarray(vel, np.float32)
+    state[:n, 0] = word(E.WATER)
+    state[:n, 1] = np.float32(20.0).view(np.uint32)
+    state[:n, 2] = np.arange(n) + 1
+    affine[:n] = 0.0
+    h.run(PASS_ADVECT)
+    return motion[:n, 0:2].copy(), motion[:n, 2:4].copy(), affine[:n].copy()
+
+
+def pin_advect(L):
+    """advect (fluid.wgsl:3468-3694) by closed forms: a uniform flow carries a
+    particle exactly and gives C = 0; a linear flow and a rigid rotation are
+    sampled exactly and give C = 0.95 grad u (AFFINE_KEEP), and C = 0 for an
+    axis whose four faces are not all PROJECTED; the speed is clamped to 8
+    cells a step; a wall stops a particle and zeroes that component."""
+    pos = [(20.3, 15.7), (10.1, 30.9), (33.6, 8.25)]
+    # uniform
+    h = _advect_world(L, lambda x, y: 0.3 + 0 * x, lambda x, y: -0.2 + 0 * x)
+    p, vel, aff = _advect(h, pos)
+    # one substep: moved by exactly the new velocity (the drift is none)
+    assert np.array_equal(p, np.asarray(pos, np.float32) + vel), (p, pos, vel)
+    assert np.allclose(vel, [[0.3, -0.2]] * 3, atol=1e-7, rtol=0)
+    assert np.allclose(aff, 0.0, atol=1e-7, rtol=0), aff
+    # linear: u = a x, v = b y
+    a, b = 0.01, -0.02
+    h = _advect_world(L, lambda x, y: a * x, lambda x, y: b * y)
+    p, vel, aff = _advect(h, pos)
+    P = np.asarray(pos)
+    assert np.allclose(vel[:, 0], a * P[:, 0], atol=2e-6, rtol=0) and np.allclose(vel[:, 1], b * P[:, 1], atol=2e-6, rtol=0)
+    assert np.allclose(aff, [[0.95 * a, 0.0, 0.0, 0.95 * b]] * 3, atol=2e-7, rtol=0), aff
+    # rigid rotation about (24, 24)
+    om = 0.01
+    h = _advect_world(L, lambda x, y: -om * (y - 24.0), lambda x, y: om * (x - 24.0))
+    p, vel, aff = _advect(h, pos)
+    assert np.allclose(vel[:, 0], -om * (P[:, 1] - 24.0), atol=2e-6, rtol=0)
+    assert np.allclose(vel[:, 1], om * (P[:, 0] - 24.0), atol=2e-6, rtol=0)
+    assert np.allclose(aff, [[0.0, -0.95 * om, 0.95 * om, 0.0]] * 3, atol=2e-7, rtol=0), aff
+    # one of the first particle's u faces held but not projected: no u gradient
+    h = _advect_world(L, lambda x, y: a * x, lambda x, y: b * y)
+    h.ufaces()[16, 21, 2] = 1.0            # its u sample (20.3, 15.2): faces x 20..21, y 15..16
+    p, vel, aff = _advect(h, pos[:1])
+    assert aff[0, 0] == 0.0 and aff[0, 1] == 0.0
+    assert abs(aff[0, 3] - 0.95 * b) < 2e-7
+    assert abs(vel[0, 0] - a * 20.3) < 2e-6
+    # the speed limit
+    h = _advect_world(L, lambda x, y: 20.0 + 0 * x, lambda x, y: 0 * x)
+    p, vel, aff = _advect(h, [(10.25, 20.5)])
+    assert abs(vel[0, 0] - 8.0) < 1e-6 and vel[0, 1] == 0.0, vel
+    assert abs(p[0, 0] - 18.25) < 1e-5, p
+    h = _advect_world(L, lambda x, y: 10.0 + 0 * x, lambda x, y: 10.0 + 0 * x)
+    p, vel, aff = _advect(h, [(10.25, 10.5)])
+    assert abs(np.hypot(vel[0, 0], vel[0, 1]) - 8.0) < 1e-5 and abs(vel[0, 0] - vel[0, 1]) < 1e-6, vel
+    # a wall: cells x = 30 solid
+    h = _advect_world(L, lambda x, y: 4.0 + 0 * x, lambda x, y: 0.1 + 0 * x)
+    h.cell()[:, 30, 2] = -1.0
+    h.words()[:, 30, 0] = word(E.WALL)
+    p, vel, aff = _advect(h, [(27.3, 20.5)])
+    assert vel[0, 0] == 0.0 and abs(vel[0, 1] - 0.1) < 1e-7, vel
+    assert 29.5 < p[0, 0] < 30.0 and abs(p[0, 1] - 20.6) < 1e-5, p
+
+
+# ------------------------------------------------------------ sort, raster
+def pin_sort(L):
+    """sortCount .. scatter (fluid.wgsl:3696-3808), by permutation
+    invariants after real steps of a dam break: `sorted` holds the same
+    particles as `parts`; along the scan's order (the active list's blocks,
+    each row-major) G_START is the exclusive prefix sum of the counts
+    G_SORT; and each cell's run of `sorted` lies in that cell."""
+    W, H = 40, 32
+    w = OracleWorld(W, H, kinds=E.kinds(), L=L)
+    w.upload(abi.PART_CELLS, scenes.dam_break(W, H, block=(16, 20)))
+    sim = Simulation(w, W, H)
+    from tests.helpers import particle_rows
+    for step in range(3):
+        sim.step()
+        n0, rows0 = particle_rows(w, abi.PART_PARTICLES0)
+        n1, rows1 = particle_rows(w, abi.PART_PARTICLES1)
+        assert n0 == n1 and np.array_equal(rows0, rows1), f"step {step}: sorted is not a permutation of parts"
+        B = w.layout.blocks
+        bw = w.layout.blocks_wide
+        blocks = w.part(abi.PART_BLOCKS)
+        cnt = w.part(abi.PART_COUNTERS)[:11 * W * H].reshape(W * H, 11)
+        run = 0
+        for n in range(int(blocks[4 * B])):
+            b = int(blocks[2 * B + n])
+            for li in range(64):
+                x, y = (b % bw) * 8 + li % 8, (b // bw) * 8 + li // 8
+                if x >= W or y >= H:
+                    continue
+                i = y * W + x
+                assert int(cnt[i, G_START]) == run, f"step {step}: cell {(x, y)} starts at {cnt[i, G_START]}, want {run}"
+                run += int(cnt[i, G_SORT])
+        assert run == n0
+        p = w.part(abi.PART_PARTICLES0)
+        cap = w.layout.capacity
+        mo = p[4:4 + 4 * cap].reshape(cap, 4)[:n0, :2].view(np.float32)
+        cells = np.floor(mo).astype(int)
+        idx = cells[:, 1] * W + cells[:, 0]
+        for i in np.unique(idx):
+            s, k = int(cnt[i, G_START]), int(cnt[i, G_SORT])
+            assert np.all(idx[s:s + k] == i), f"step {step}: cell {i}'s run holds other cells' particles"
+
+
+def pin_raster(L):
+    """raster (fluid.wgsl:3983-4119) on hand cases: a cell is drawn as the
+    majority of its particles' types (a tie to the lower type), its word is
+    its lowest-id particle's of that type, its mean velocity and
+    heat-capacity-weighted temperature are exact sums (1e4 + 1e-3 - 1e4
+    keeps the 1e-3), and w = home << 8 | n."""
+    kinds = E.kinds()
+    kinds[7] = kinds[9] = E.KIND_LIQUID
+    h = Hand(16, 16, L=L, kinds=kinds)
+    h.activate()
+    words, cnt = h.words(), h.cnt()
+    words[..., 0] = word(E.AIR)
+    cases = [   # cell, particle types, ids, shades, vel x, vel y, temperatures
+        ((3, 3), [9, 3, 9, 3, 9], [5, 6, 7, 8, 9], [1, 2, 3, 4, 5], [0.1] * 5, [0.0] * 5, [20.0] * 5),
+        ((5, 3), [9, 3, 9, 3], [20, 23, 22, 21], [1, 2, 3, 4], [0.0] * 4, [0.0] * 4, [20.0] * 4),
+        ((7, 3), [3, 3, 3], [50, 30, 40], [6, 7, 8], [1e4, 1e-3, -1e4], [0.5, 0.25, 0.75], [10.0, 20.0, 60.0]),
+    ]
+    cntv, motion, state, _ = h.particles(abi.PART_PARTICLES0)
+    k = 0
+    for (x, y), types, ids, shades, vx, vy, temps in cases:
+        words[y, x, 0] = word(E.WATER)
+        cnt[y, x, G_SORT] = len(types)
+        cnt[y, x, G_START] = k
+        for t, i, s, a, b, tt in zip(types, ids, shades, vx, vy, temps):
+            motion[k] = (x + 0.5, y + 0.5, a, b)
+            state[k] = (word(t, shade=s), np.float32(tt).view(np.uint32), i, 0)
+            k += 1
+    cntv[0] = k
+    h.run(PASS_RASTER)
+    for (x, y), want_type, want_shade in (((3, 3), 9, 1), ((5, 3), 3, 4), ((7, 3), 3, 7)):
+        w0 = int(words[y, x, 0])
+        assert w0 & 0xFF == want_type, f"cell {(x, y)} drawn as {w0 & 0xFF}, want {want_type}"
+        assert (w0 >> 24) & 0xF == want_shade, f"cell {(x, y)} takes shade {(w0 >> 24) & 0xF}, want {want_shade}"
+        assert (w0 >> 28) & FL_DRAWN
+        n = {3: 5, 5: 4, 7: 3}[x]
+        assert int(words[y, x, 3]) == ((y * 16 + x) << 8) | n
+    z = int(words[3, 7, 2])
+    vel = np.array([z & 0xFFFF, z >> 16], np.uint16).view(np.float16).astype(np.float64)
+    exact_x = float(np.float32(np.float32(1e-3) * np.float32(2.0 ** 32)).round()) / 2.0 ** 32 / 3.0
+    assert abs(vel[0] - exact_x) < 2e-3 * exact_x, (vel[0], exact_x)
+    assert vel[1] == 0.5
+    temp = float(words[3, 7, 1:2].view(np.float32)[0])
+    assert abs(temp - 30.0) < 1e-4, temp
+
+
+def pin_extrapolate(L):
+    """extrapolate, extrapolateFurther (fluid.wgsl:3220-3275) on hand cases:
+    a face between two air cells holding a splat takes the mean of its
+    same-axis neighbours that were projected (layer 1) or reached by layer 1
+    (layer 2), and a face beside a fluid cell is left alone."""
+    h = Hand(16, 16, L=L)
+    h.activate()
+    uf, vf = h.ufaces(), h.vfaces()
+    uf[5, 3, 0:3] = (1.0, 0, 2.0)
+    uf[4, 4, 0:3] = (3.0, 0, 2.0)
+    uf[5, 4, 0:3] = (9.0, 0, 1.0)
+    uf[5, 5, 0:3] = (9.0, 0, 1.0)
+    vf[8, 7, 0:3] = (-1.0, 0, 2.0)
+    vf[9, 8, 0:3] = (-3.0, 0, 2.0)
+    vf[9, 7, 0:3] = (5.0, 0, 1.0)
+    uf[5, 10, 0:3] = (9.0, 0, 1.0)          # beside the fluid cell (10, 5)
+    uf[5, 9, 0:3] = (4.0, 0, 2.0)
+    h.cell()[5, 10, 2] = 1.0
+    h.run(PASS_EXTRAPOLATE, 1)
+    assert uf[5, 4, 0] == 2.0 and uf[5, 4, 2] == 1.5, uf[5, 4]
+    assert uf[5, 5, 0] == 9.0 and uf[5, 5, 2] == 1.0, uf[5, 5]
+    assert vf[9, 7, 0] == -2.0 and vf[9, 7, 2] == 1.5, vf[9, 7]
+    assert uf[5, 10, 0] == 9.0 and uf[5, 10, 2] == 1.0
+    h.run(PASS_EXTRAPOLATE, 2)
+    assert uf[5, 5, 0] == 2.0 and uf[5, 5, 2] == 1.25, uf[5, 5]
+    assert uf[5, 4, 0] == 2.0 and uf[5, 4, 2] == 1.5
+    assert uf[5, 10, 0] == 9.0
+
+
+def pin_project(L):
+    """project (fluid.wgsl:2895-2942): after a step solved to 1e-8 relative,
+    the projected faces leave every fluid cell divergence-free to the
+    solve's residual: ||div u|| <= 1e-4 ||b|| (a dam break touching the air
+    everywhere, so no region is pinned)."""
+    W, H = 40, 40
+    w = OracleWorld(W, H, kinds=E.kinds(), L=L)
+    w.upload(abi.PART_CELLS, scenes.dam_break(W, H, block=(20, 30)))
+    from tests.helpers import config_sim
+    sim = config_sim(w, W, H, tol=1e-8)
+    for _ in range(3):
+        sim.step()
+    w.capture(True)
+    sim.step()
+    b = w.captured_b()
+    f = w.part(abi.PART_FIELD)
+    C = W * H
+    rho = f[:8 * C].reshape(H, W, 8)[..., 2]
+    u = f[8 * C:8 * C + 4 * (W + 1) * H].reshape(H, W + 1, 4)[..., 0].astype(np.float64)
+    v = f[8 * C + 4 * (W + 1) * H:8 * C + 4 * ((W + 1) * H + W * (H + 1))].reshape(H + 1, W, 4)[..., 0].astype(np.float64)
+    div = u[:, 1:] - u[:, :-1] + v[1:, :] - v[:-1, :]
+    fluid = rho > 0
+    assert fluid.sum() > 200
+    ratio = np.linalg.norm(div[fluid]) / np.linalg.norm(b)
+    assert ratio < 1e-4, ratio
+
+
+# ------------------------------------------------------------ classification, seals
+def pin_classify(L):
+    """classify (fluid.wgsl:1578-1696) and sealJoin/sealAnchor (:1503-1554)
+    on hand cases: an empty cell with two fluid neighbours and three of
+    fluid or wall is fluid at their mean density, with two fluid and no wall
+    or one fluid and two walls it is air; a gap the liquid left, touching
+    nothing that was air last step, is sealed (fluid, sealed code 1), and the
+    same gap beside a cell that was air last step is open (air)."""
+    def fluid_at(h, x, y, rho=1.0):
+        h.words()[y, x, 0] = word(E.WATER)
+        h.cnt()[y, x, G_COUNT] = 4
+        h.cnt()[y, x, G_MASS] = int(round(4 * FIX * rho))
+
+    h = Hand(16, 16, L=L)
+    h.activate()
+    h.words()[..., 0] = word(E.AIR)
+    for x, y, r in ((2, 10, 1.0), (4, 10, 0.5), (7, 10, 1.0), (9, 10, 1.0), (11, 10, 1.0)):
+        fluid_at(h, x, y, r)
+    for x, y in ((3, 9), (12, 9), (12, 11)):
+        h.words()[y, x, 0] = word(E.WALL)
+    h.cnt()[..., G_REGION] = np.arange(16 * 16, dtype=np.uint32).reshape(16, 16)
+    h.run(PASS_SEAL)
+    h.run(PASS_CLASSIFY)
+    c = h.cell()
+    assert c[10, 3, 2] == 0.75, c[10, 3, 2]          # 2 fluid + 1 wall
+    assert c[10, 8, 2] == 0.0, c[10, 8, 2]           # 2 fluid, no wall
+    assert c[10, 12, 2] == 0.0, c[10, 12, 2]         # 1 fluid, 2 walls
+
+    for was_air_above in (False, True):
+        h = Hand(16, 16, L=L)
+        h.activate()
+        h.frame = 3
+        last = 1 << ((h.frame + 1) & 1)                # stepBit(1): liquid last step
+        h.words()[..., 0] = word(E.AIR)
+        c = h.cell()
+        for y in range(2, 13):
+            for x in range(2, 13):
+                fluid_at(h, x, y)
+                c[y, x, 2] = 1.0
+                c[y, x, 7] = float(last)
+        for x in (5, 6, 7):                            # the gap: liquid last step, empty now
+            h.words()[6, x, 0] = word(E.AIR)
+            h.cnt()[6, x, G_COUNT] = 0
+            h.cnt()[6, x, G_MASS] = 0
+        if was_air_above:
+            c[5, 6, 7] = 0.0                           # (6, 5) was air last step
+            c[5, 6, 2] = 0.0
+        h.cnt()[..., G_REGION] = np.arange(16 * 16, dtype=np.uint32).reshape(16, 16)
+        h.run(PASS_SEAL)
+        h.run(PASS_CLASSIFY)
+        if was_air_above:
+            assert c[6, 6, 2] == 0.0 and c[6, 6, 6] == 0.0, c[6, 6]
+        else:
+            assert c[6, 6, 2] == 1.0 and c[6, 6, 6] == 1.0, c[6, 6]
+        assert c[6, 5, 2] == 1.0                       # an end of the gap: 3 fluid neighbours either way
+
+
+# ------------------------------------------------------------ syncParticles
+def pin_sync_particles(L, order_seed=0):
+    """syncParticles (fluid.wgsl:661-831): two particles at one point are
+    pushed apart to APART (1/256) about their middle (apart, :695-734);
+    where a drawn cell claims fewer particles than its home holds, it keeps
+    those of lowest id, whatever order the slots hold them in (rankById,
+    :668-678)."""
+    h = Hand(16, 16, L=L)
+    h.activate()
+    h.words()[..., 0] = word(E.AIR)
+    cnt = h.cnt()
+    cnt[..., G_DEST] = DEST_KEEP
+    cntv, motion, state, affine = h.particles(abi.PART_PARTICLES0)
+    # two at (5.5, 5.5)
+    cnt[5, 5, G_START], cnt[5, 5, G_SORT] = 0, 2
+    motion[0] = (5.5, 5.5, 0, 0)
+    motion[1] = (5.5, 5.5, 0, 0)
+    state[0] = (word(E.WATER), np.float32(20).view(np.uint32), 11, 0)
+    state[1] = (word(E.WATER), np.float32(20).view(np.uint32), 22, 0)
+    # six in cell (9, 9), drawn with 3
+    ids = np.array([40, 10, 30, 60, 20, 50])
+    order = np.random.default_rng(order_seed).permutation(6) if order_seed else np.arange(6)
+    home = 9 * 16 + 9
+    cnt[9, 9, G_START], cnt[9, 9, G_SORT] = 2, 6
+    cnt[9, 9, G_DEST] = DEST_CELL | (DEST_SPAN - home)
+    cnt[9, 9, G_TYPE] = E.WATER
+    wv = h.words()
+    wv[9, 9] = (word(E.WATER, flags=FL_DRAWN), np.float32(20).view(np.uint32), 0, (home << 8) | 3)
+    for j, o in enumerate(order):
+        motion[2 + j] = (9.1 + 0.13 * o, 9.2 + 0.11 * o, 0, 0)
+        state[2 + j] = (word(E.WATER), np.float32(20).view(np.uint32), ids[o], 0)
+    cntv[0] = 8
+    affine[:8] = 0.0
+    h.run(PASS_SYNC_PARTICLES)
+    pc, pm, ps, _ = h.particles(abi.PART_PARTICLES1)
+    n = int(pc[0])
+    assert n == 5, n
+    kept = sorted(int(i) for i in ps[:n, 2] if i not in (11, 22))
+    assert kept == [10, 20, 30], kept
+    two = [k for k in range(n) if ps[k, 2] in (11, 22)]
+    a, b = pm[two[0], :2].astype(np.float64), pm[two[1], :2].astype(np.float64)
+    assert abs(np.linalg.norm(a - b) - APART) < 1e-6, np.linalg.norm(a - b)
+    assert np.allclose((a + b) / 2, (5.5, 5.5), atol=2.0 ** -22)
+
+
+def pin_sync_particles_shuffled(L):
+    for seed in (1, 2, 3):
+        pin_sync_particles(L, order_seed=seed)
+
+
+# ------------------------------------------------------------ the fine level's passes
+def pin_relax(L):
+    """relax (fluid.wgsl:2047-2192) on a world of one block, where the tile's
+    rim lies outside the world: after its red-black pairs the colour swept
+    last satisfies the fine equations exactly -- fineBeta's ghost-fluid
+    weights toward air (pinned by closed forms in test_oracle), PIN on the
+    diagonal of a pinned root, the drift channel without it -- and cells not
+    in the solve are left alone."""
+    h = Hand(8, 8, L=L)
+    h.activate()
+    h.blocks[h.B] |= HAS_FLUID
+    rng = np.random.default_rng(21)
+    c, cnt = h.cell(), h.cnt()
+    kind = rng.random((8, 8))
+    rho = np.where(kind < 0.15, -1.0, np.where(kind < 0.3, 0.0, rng.uniform(0.8, 1.2, (8, 8)))).astype(np.float32)
+    rho[0, 0] = 1.0
+    share = rng.uniform(0.0, 1.5, (8, 8)).astype(np.float32)
+    c[..., 2] = rho
+    c[..., 0] = rng.normal(0, 1, (8, 8))
+    c[..., 4] = rng.normal(0, 1, (8, 8))
+    c[..., 1] = rng.normal(0, 1, (8, 8))
+    c[..., 5] = rng.normal(0, 1, (8, 8))
+    cnt[..., G_RIM] = c[..., 0].view(np.uint32)
+    cnt[..., G_RIM + 1] = c[..., 4].view(np.uint32)
+    cnt[..., G_DENSITY] = np.round(share * 4 * FIX).astype(np.uint32)
+    share = cnt[..., G_DENSITY].astype(np.float32) / np.float32(4 * FIX)
+    cnt[..., G_REGION] = 0                             # every cell linked to cell 0, the one pinned root
+    before = c.copy()
+    h.run(PASS_RELAX, 0)
+    fb = L.wor_fine_beta
+    for y in range(8):
+        for x in range(8):
+            if rho[y, x] <= 0:
+                assert np.array_equal(c[y, x].view(np.uint32), before[y, x].view(np.uint32)), (x, y)
+                continue
+            if (x + y) & 1 != 1:
+                continue
+            betas, diag = [], 0.0
+            for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                xx, yy = x + dx, y + dy
+                inside = 0 <= xx < 8 and 0 <= yy < 8
+                b = fb(float(rho[y, x]), float(rho[yy, xx]) if inside else -1.0, float(share[y, x]),
+                       float(share[yy, xx]) if inside else 0.0)
+                betas.append((b, (xx, yy) if inside else None))
+                diag += b
+            pin = 1.0 if (x, y) == (0, 0) else 0.0
+            for ch, rhs_at, with_pin in ((0, 1, True), (4, 5, False)):
+                s_ = sum(b * float(c[yy, xx, ch]) for b, q in betas if q for xx, yy in [q])
+                scale = 1.0 + sum(abs(b * float(c[yy, xx, ch])) for b, q in betas if q for xx, yy in [q])
+                res = s_ - (diag + (pin if with_pin else 0.0)) * float(c[y, x, ch]) - float(c[y, x, rhs_at])
+                if diag > 0:
+                    assert abs(res) <= 2e-6 * scale, (x, y, ch, res)
+
+
+def pin_compact(L):
+    """compact (fluid.wgsl:896-921) on a hand case: a block is worked on if it
+    or one of its eight neighbours is marked (liquid or a region), near the
+    liquid if a neighbour holds liquid; the list holds them in block order,
+    and a block worked on last step and not now is retired."""
+    h = Hand(48, 40, L=L)
+    bw, bh = 6, 5
+    h.blocks[1 * bw + 1] = MARK_LIQUID
+    h.blocks[3 * bw + 4] = MARK_REGION
+    h.blocks[h.B + 1 * bw + 1] = ACTIVE
+    h.blocks[h.B + 0 * bw + 5] = ACTIVE             # block (5, 0): active last step, nothing near now
+    h.run(PASS_COMPACT)
+    want_active = [y * bw + x for y in range(bh) for x in range(bw)
+                   if max(abs(x - 1), abs(y - 1)) <= 1 or max(abs(x - 4), abs(y - 3)) <= 1]
+    B = h.B
+    state = h.blocks[B:2 * B]
+    got_active = [b for b in range(B) if state[b] & ACTIVE]
+    assert got_active == want_active, got_active
+    near = [b for b in range(B) if state[b] & NEAR_LIQUID]
+    assert near == [y * bw + x for y in range(bh) for x in range(bw) if max(abs(x - 1), abs(y - 1)) <= 1]
+    assert int(h.blocks[4 * B]) == len(want_active)
+    assert list(h.blocks[2 * B:2 * B + len(want_active)]) == want_active
+    assert int(h.blocks[4 * B + 1]) == 1 and int(h.blocks[3 * B]) == 5
+
+
+def pin_set_faces(L):
+    """setFaces (fluid.wgsl:1934-1994) on hand cases: a face between cells in
+    the solve takes its splat's momentum over weight, plus earthG on v faces
+    (y down); a face against an obstacle takes the obstacle's velocity (0 for
+    a solid, 0.3 of a powder's), without gravity; one between two obstacles
+    is held at -1; a face with no weight and no obstacle is not held."""
+    h = Hand(16, 16, L=L)
+    h.activate()
+    h.earth_g = 0.25
+    c, words = h.cell(), h.words()
+    words[..., 0] = word(E.AIR)
+    c[2:10, 2:10, 2] = 1.0
+    words[2:10, 2:10, 0] = word(E.WATER)
+    for x, t in ((4, E.WALL), (5, E.WALL), (6, E.SAND)):
+        c[10, x, 2] = -1.0
+        words[10, x, 0] = word(t)
+    words[10, 6, 2] = int(np.array([0.5, -0.25], np.float16).view(np.uint16)[0]) | \
+        (int(np.array([0.5, -0.25], np.float16).view(np.uint16)[1]) << 16)
+    W, H = 16, 16
+    sums = h.counters[11 * h.C:11 * h.C + 2 * h.F].reshape(h.F, 2).view(np.int32)
+    uf = lambda x, y: y * (W + 1) + x
+    vf = lambda x, y: (W + 1) * H + y * W + x
+    sums[uf(5, 5)] = (98304, 131072)        # 1.5 / 2.0
+    sums[vf(5, 5)] = (-32768, 65536)        # -0.5 / 1.0
+    sums[uf(2, 3)] = (16384, 65536)         # fluid | air: 0.25
+    sums[vf(3, 2)] = (0, 65536)             # air over fluid: 0, then gravity
+    h.run(PASS_SET_FACES)
+    U, V = h.ufaces(), h.vfaces()
+    assert tuple(U[5, 5, :3]) == (0.75, 0.75, 1.0), U[5, 5]
+    assert tuple(V[5, 5, :3]) == (-0.25, -0.5, 1.0), V[5, 5]
+    assert tuple(U[3, 2, :3]) == (0.25, 0.25, 1.0), U[3, 2]
+    assert tuple(V[2, 3, :3]) == (0.25, 0.0, 1.0), V[2, 3]
+    assert tuple(V[2, 5, :3]) == (0.0, 0.0, 0.0), V[2, 5]            # air over fluid, no weight
+    assert tuple(V[10, 4, :3]) == (0.0, 0.0, 1.0), V[10, 4]          # fluid over a wall
+    assert V[10, 6, 0] == np.float32(np.float32(-0.25) * np.float32(0.3)) and V[10, 6, 2] == 1.0, V[10, 6]
+    assert tuple(U[10, 5, :3]) == (0.0, 0.0, -1.0), U[10, 5]         # wall | wall
+
+
+def pin_divergence(L):
+    """divergence (fluid.wgsl:1996-2045) on a hand case: a fluid cell's b is
+    u(x+1) - u(x) + v(y+1) - v(y); its drift rhs is -0.3 (share - 1) (0 for a
+    cell marked below zero); cells out of the solve keep theirs."""
+    h = Hand(16, 16, L=L)
+    h.activate()
+    c, cnt = h.cell(), h.cnt()
+    U, V = h.ufaces(), h.vfaces()
+    c[5, 5, 2] = c[5, 6, 2] = 1.0
+    c[5, 7, 1] = 9.0                        # air: left alone
+    U[5, 5, 0], U[5, 6, 0], V[5, 5, 0], V[6, 5, 0] = 0.5, 2.0, 0.25, -1.0
+    cnt[5, 5, G_DENSITY] = 6 * int(FIX)     # share 1.5
+    c[5, 6, 6] = -2.0                       # a body's cell
+    h.run(PASS_DIVERGENCE)
+    assert c[5, 5, 1] == 0.25, c[5, 5, 1]
+    assert c[5, 5, 5] == np.float32(-0.3) * np.float32(0.5), c[5, 5, 5]
+    assert c[5, 6, 5] == 0.0
+    assert c[5, 7, 1] == 9.0
+
+
+def pin_unpaint(L):
+    """unpaint (fluid.wgsl:4163-4182) on hand cases: a liquid cell its
+    particles left, not taken by displaced air, becomes air at its effective
+    temperature (100 for stored 150: effTempIn) with its gas cleared; one
+    with particles, or taken, stays."""
+    h = Hand(16, 16, L=L)
+    h.activate()
+    words, cnt = h.words(), h.cnt()
+    gas = h.cellwords[4 * h.C:4 * h.C + 2 * h.C].reshape(16, 16, 2)
+    for x, temp in ((3, 20.0), (5, 20.0), (7, 20.0), (9, 150.0)):
+        words[3, x] = (word(E.WATER, flags=FL_DRAWN), np.float32(temp).view(np.uint32), 77, (3 * 16 + x) << 8 | 4)
+        gas[3, x] = (123, 456)
+    cnt[3, 5, G_SORT] = 2
+    cnt[3, 7, G_TAKEN] = 99
+    h.run(PASS_UNPAINT)
+    for x, want in ((3, 20.0), (9, 100.0)):
+        assert tuple(words[3, x, [0, 2, 3]]) == (word(E.AIR), 0, 0), words[3, x]
+        assert words[3, x, 1:2].view(np.float32)[0] == want
+        assert tuple(gas[3, x]) == (0, 0)
+    for x in (5, 7):
+        assert words[3, x, 0] == word(E.WATER, flags=FL_DRAWN) and tuple(gas[3, x]) == (123, 456)
+
+
+def pin_escape(L):
+    """escape (fluid.wgsl:3543-3594): a particle inside an obstacle's cell
+    goes to the nearest point at least 0.02 inside a free neighbour."""
+    h = _advect_world(L, lambda x, y: 0 * x, lambda x, y: 0 * x)
+    h.cell()[20, 20, 2] = -1.0
+    h.words()[20, 20, 0] = word(E.WALL)
+    p, vel, aff = _advect(h, [(20.5, 20.05)])
+    assert p[0, 0] == np.float32(20.5) and p[0, 1] == np.float32(19.0) + np.float32(0.98), p
+
+
+PINS = {
+    "restrict": pin_restrict,
+    "smooth": pin_smooth,
+    "coarsest": pin_coarsest,
+    "prolong": pin_prolong,
+    "two_grid": pin_two_grid,
+    "advect": pin_advect,
+    "sort": pin_sort,
+    "raster": pin_raster,
+    "extrapolate": pin_extrapolate,
+    "project": pin_project,
+    "classify": pin_classify,
+    "sync_particles": pin_sync_particles,
+    "sync_particles_shuffled": pin_sync_particles_shuffled,
+    "relax": pin_relax,
+    "compact": pin_compact,
+    "set_faces": pin_set_faces,
+    "divergence": pin_divergence,
+    "unpaint": pin_unpaint,
+    "escape": pin_escape,
+}
