@@ -36,9 +36,14 @@ KERNEL_BYTES_PER_PARTICLE = {"gather": 48, "advect": 80, "sortCount": 8, "scatte
                              "syncParticles": 96}
 
 
-def build_world(size, seed=2):
+def build_world(size, seed=2, workload="single"):
     from worldinagrain_amd import scenes
     from worldinagrain_amd import elements as E
+    if workload == "dam":
+        # the small plain configuration: a 1-cell solid border, water on
+        # cells [1, size/2) x [1, size/2) at rest
+        cells = scenes.dam_break(size, size, block=(size // 2, size // 2), wall=1)
+        return cells, (size // 2 - 1) ** 2, E.kinds()
     cells, fluid_cells = scenes.discs_dam(size, size, seed=seed)
     return cells, fluid_cells, E.kinds()
 
@@ -68,32 +73,65 @@ def step_bytes(totals_delta, b_iter=93.0):
     return parts * 224.0 + ca * (65.0 + 16.0) + cyc * ca_mean * b_iter
 
 
-def cpu_baseline(cells, kinds, size, tol, steps, solver=None):
-    """The oracle on the host: the same world with the same solver settings,
-    its first (seeding) step untimed, then `steps` steps timed."""
-    from oracle.oracle import OracleWorld
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cells, kinds, size, tol, steps, solver=None, runs=3, one_thread_runs=3):
+    """The oracle (the CPU restatement of the reference's liquid step) on the
+    host, the same world with the same solver settings: its first (seeding)
+    step untimed, then `runs` runs of `steps` steps each, the median run
+    reported.  On all the threads OpenMP is given (OMP_NUM_THREADS, else the
+    host's cores; the OpenMP build, whose results are the serial build's),
+    and on one thread (the serial build)."""
+    import statistics
+    from oracle.oracle import OracleWorld, lib, OMP_PATH
     from worldinagrain_amd import abi
-    w = OracleWorld(size, size, kinds=kinds, dot_mode=1)
-    w.upload(abi.PART_CELLS, cells)
-    sim = config_simulation(w, size, tol)
-    if solver is not None:
-        sim.solver = solver
-    sim.step()
-    n = int(sim.last_stats[abi.STAT_PARTICLES])
-    t0 = time.perf_counter()
-    for _ in range(steps):
+
+    def measure(L, runs):
+        w = OracleWorld(size, size, kinds=kinds, dot_mode=1, L=L)
+        w.upload(abi.PART_CELLS, cells)
+        sim = config_simulation(w, size, tol)
+        if solver is not None:
+            sim.solver = solver
         sim.step()
-    dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of the same {size}x{size} world ({n} particles) after its seeding step, "
-                      f"single-threaded C restatement of fluid.wgsl (oracle/wig_oracle.c, -O2), "
-                      f"{dt:.1f} s"}
+        n = int(sim.last_stats[abi.STAT_PARTICLES])
+        rates, secs = [], []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                sim.step()
+            dt = time.perf_counter() - t0
+            rates.append(n * steps / dt)
+            secs.append(dt)
+        w.close()
+        return statistics.median(rates), n, secs
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    all_rate, n, all_secs = measure(lib(OMP_PATH), runs)
+    one_rate, _, one_secs = measure(lib(), one_thread_runs)
+    return {"value": all_rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "one_thread_value": one_rate, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "sample": f"{steps} steps of the same {size}x{size} world ({n} particles) after its seeding step, median of "
+                      f"{runs} runs: the C restatement of fluid.wgsl (oracle/wig_oracle.c) built -O3 with OpenMP on "
+                      f"{threads} threads (runs {', '.join(f'{t:.1f}' for t in all_secs)} s), and the serial -O2 build "
+                      f"on one thread ({len(one_secs)} run{'s' if len(one_secs) > 1 else ''}: "
+                      f"{', '.join(f'{t:.1f}' for t in one_secs)} s)"}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=("single", "batch", "serve", "precision", "snapshot", "air"), default="single",
-                    help="single: the 1024x1024 world (the metric's configuration); batch: many worlds in one batch; "
+    ap.add_argument("--workload", choices=("single", "dam", "batch", "serve", "precision", "snapshot", "air"),
+                    default="single",
+                    help="single: the 1024x1024 world (the metric's configuration); dam: the 256x256 dam break "
+                         "(5 levels, 2+2 red-black pairs, 1e-5, 200 steps); batch: many worlds in one batch; "
                          "serve: the serving loop over many sessions")
     ap.add_argument("--worlds", type=int, default=512, help="batch: how many worlds")
     ap.add_argument("--sessions", type=int, default=2000, help="serve: sessions")
@@ -114,13 +152,15 @@ def main():
     ap.add_argument("--storage", choices=tuple(STORAGES), default=None,
                     help="V-cycle storage (single: fp32, the plain configuration; batch and serve: fp16, mixed precision)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--size", type=int, default=None, help="grid side (single: 1024, dam: 256)")
+    ap.add_argument("--tol", type=float, default=None, help="relative residual (single: 1e-6, dam: 1e-5)")
     ap.add_argument("--kernel", default="solve",
                     help="what the roofline prices: 'solve' (the MG-PCG solve, the byte model's unit of work) "
                          "or a particle kernel (gather, advect, sortCount, scatter, syncParticles)")
     ap.add_argument("--kernel-table", action="store_true", help="also time every kernel over one step each")
-    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed on the host (about 1.6 s each at 1024x1024)")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="oracle steps in each of 3 timed runs on the host (single: 4, about 1.4 s a step on one "
+                         "thread at 1024x1024; dam: 200, the configuration's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default coarse-level tile sides")
     ap.add_argument("--graphs", type=int, default=1, help="1: replay the step's phases as hipGraphs")
@@ -128,7 +168,13 @@ def main():
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"batch": 50, "air": 30}.get(args.workload, 100)
+        args.steps = {"batch": 50, "air": 30, "dam": 200}.get(args.workload, 100)
+    if args.size is None:
+        args.size = 256 if args.workload == "dam" else 1024
+    if args.cpu_steps is None:
+        args.cpu_steps = 200 if args.workload == "dam" else 4
+    if args.tol is None:
+        args.tol = 1e-5 if args.workload == "dam" else 1e-6
     if args.storage is None:
         args.storage = "fp16" if args.workload in ("batch", "serve") else "fp32"
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -162,7 +208,7 @@ def main():
         return run_air(args)
 
     size = args.size
-    cells, fluid_cells, kinds = build_world(size)
+    cells, fluid_cells, kinds = build_world(size, workload=args.workload)
     world = GpuWorld(size, size, kinds=kinds)
     world.upload(abi.PART_CELLS, cells)
     stream = torch.cuda.Stream() if args.graphs else torch.cuda.current_stream()
@@ -181,12 +227,15 @@ def main():
         tiles = tune_tiles(world, sim)
         # and the sweep counts, on copies of the world (they change the
         # iteration, so each is scored by its solve time at the tolerance);
-        # then the speed settings again, for those counts
-        solver, _ = tune_sweeps(world, sim)
-        sim.solver = solver
+        # then the speed settings again, for those counts.  The dam keeps
+        # its configuration's 2 + 2 red-black pairs.
+        if args.workload != "dam":
+            solver, _ = tune_sweeps(world, sim)
+            sim.solver = solver
+            lds_level, coarse_wave = tune_lds_level(world, sim)
+            tiles = tune_tiles(world, sim)
+        solver = sim.solver
         sweeps = [solver.sweep_pairs, solver.level_pairs, solver.coarse_sweeps]
-        lds_level, coarse_wave = tune_lds_level(world, sim)
-        tiles = tune_tiles(world, sim)
         for _ in range(2):
             sim.step()
     torch.cuda.synchronize()
@@ -252,10 +301,14 @@ def main():
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
         "warmup": args.warmup, "ms_per_step": total_ms / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{'plain FP32' if storage == 0 else args.storage + ' V-cycle storage'} at scale: "
-                               f"{size}x{size} MAC grid, ~10% solid discs, water in the lowest "
-                               f"40% of rows at 4 particles/cell, dt 1/120, MG-PCG {1 + world.layout.levels} levels "
-                               f"to {args.tol:g} relative residual",
+        "config": {"workload": (f"plain small: {size}x{size} dam break, 1-cell solid border, water on cells "
+                                f"[1,{size // 2})^2 at 4 particles/cell, g 9.81, dt 1/120, MG-PCG "
+                                f"{1 + world.layout.levels} levels with 2+2 red-black pairs to {args.tol:g} relative "
+                                f"residual, {K} steps") if args.workload == "dam" else
+                               (f"{'plain FP32' if storage == 0 else args.storage + ' V-cycle storage'} at scale: "
+                                f"{size}x{size} MAC grid, ~10% solid discs, water in the lowest "
+                                f"40% of rows at 4 particles/cell, dt 1/120, MG-PCG {1 + world.layout.levels} levels "
+                                f"to {args.tol:g} relative residual"),
                    "grid": [size, size], "particles": particles_per_step, "levels": 1 + world.layout.levels,
                    "tol": args.tol, "vcycle_storage": args.storage, "parallelism": "single world, one GPU",
                    "graphs": bool(args.graphs),
@@ -271,7 +324,9 @@ def main():
     if per_kernel:
         out["kernel_ms_one_step"] = {k: round(v[0], 4) for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:12]}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cells, kinds, size, args.tol, args.cpu_steps, solver=sim.solver)
+        # (the dam's one-thread figure is one run of its 200 steps, about 20 s)
+        out["cpu_baseline"] = cpu_baseline(cells, kinds, size, args.tol, args.cpu_steps, solver=sim.solver,
+                                           one_thread_runs=1 if args.workload == "dam" else 3)
     print(json.dumps(out), flush=True)
 
 

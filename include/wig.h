@@ -81,6 +81,10 @@ typedef struct WigStepCtl {
     uint32_t chunk;
     uint32_t level_pairs;    /* red-black sweep pairs on each coarse level, down and up (0: 1, the reference's) */
     uint32_t vcycle_storage; /* WIG_STORE_*: what the V-cycle (the preconditioner) keeps its values in */
+    uint32_t residual_every; /* 0: the reference's recursive residual throughout.  k > 0 (this library's,
+                                at most WIG_MAX_CYCLES): after every k-th iteration the residual is
+                                recomputed as b - A x (FP64 accumulation) and replaces the recursive one,
+                                and the stop is judged only there, so a solve stops on its true residual */
 } WigStepCtl;
 
 /* V-cycle storage: float in the reference's buffers (results bit-identical

@@ -19,6 +19,9 @@ _libs = {}
  PASS_UNPAINT, PASS_RELAX) = range(25)
 
 
+OMP_PATH = os.path.join(_HERE, "libwig_oracle_omp.so")
+
+
 def mutant_path(k):
     """The oracle built with planted mistake k (`make -C oracle mutants`)."""
     return os.path.join(_HERE, "_mut", f"libwig_oracle_m{k}.so")

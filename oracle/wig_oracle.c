@@ -43,6 +43,32 @@
 #endif
 #define MUTANT(k) (ORACLE_MUTANT == (k))
 
+/* Built with -fopenmp (oracle/Makefile's `omp`, the CPU baseline's build),
+ * the loops the reference dispatches as independent invocations run on all
+ * cores; where two invocations meet in one word the reference uses an
+ * atomic, and so does this build (at_*), on integers, so every result is
+ * the serial build's (tests/test_oracle.py checks).  Union-find, the block
+ * lists and the fixed-order sums stay serial. */
+#ifdef _OPENMP
+#define PAR_FOR _Pragma("omp parallel for schedule(static)")
+#define PAR_FOR_MOST _Pragma("omp parallel for schedule(static) reduction(max: most)")
+static inline void at_add(uint32_t *p, uint32_t v) { __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
+static inline uint32_t at_take(uint32_t *p, uint32_t v) { return __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
+static inline void at_or(uint32_t *p, uint32_t v) { __atomic_fetch_or(p, v, __ATOMIC_RELAXED); }
+static inline void at_max(uint32_t *p, uint32_t v) {
+    uint32_t o = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (o < v && !__atomic_compare_exchange_n(p, &o, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
+#else
+#define PAR_FOR
+#define PAR_FOR_MOST
+static inline void at_add(uint32_t *p, uint32_t v) { *p += v; }
+static inline uint32_t at_take(uint32_t *p, uint32_t v) { uint32_t o = *p; *p += v; return o; }
+static inline void at_or(uint32_t *p, uint32_t v) { *p |= v; }
+static inline void at_max(uint32_t *p, uint32_t v) { if (*p < v) *p = v; }
+#endif
+
 /* ------------------------------------------------------------ constants */
 /* js/shared.js:270-289, fluid.wgsl:183-242,2062-2063,2221,2392,2579 */
 #define BLOCK 8
@@ -160,6 +186,8 @@ typedef struct OWorld {
     int converged_flag;
     /* captured at the solve's start for the FP64 oracle */
     float *cap_b;
+    /* the solve's b (= -divergence at fluid cells), for residual replacement */
+    float *bvec;
     int capture;
     uint32_t last_cycles;        /* iterations the last chunked solve ran */
 } OWorld;
@@ -579,10 +607,11 @@ static inline int isClaim(uint32_t d) { return d != DEST_KEEP && (d & DEST_ASIDE
 
 static void markBlockOf(OWorld *w, v2 pos) {                 /* :597-602 */
     i2 b = idiv8(cellOf(w, pos));
-    if (inBlocks(w, b)) w->blocks[blockIndex(w, b)] |= MARK_LIQUID;
+    if (inBlocks(w, b)) at_or(&w->blocks[blockIndex(w, b)], MARK_LIQUID);
 }
 
 static void k_sync(OWorld *w, int claims) {                  /* :608-659 */
+    PAR_FOR
     for (uint32_t by = 0; by < w->bh; by++)
         for (uint32_t bx = 0; bx < w->bw; bx++) {
             int liquidHere = 0;
@@ -602,7 +631,7 @@ static void k_sync(OWorld *w, int claims) {                  /* :608-659 */
                         uint32_t home = v[3] >> 8;
                         if (claims && wHas(v[0], FL_DRAWN) && home < w->C) {
                             uint32_t cl = claimOf(i);
-                            if (CNT(home, G_DEST) < cl) CNT(home, G_DEST) = cl;
+                            at_max(&CNT(home, G_DEST), cl);
                         }
                         liquidHere = 1;
                     }
@@ -666,6 +695,7 @@ static uint32_t sharedLife(const OWorld *w, uint32_t life, uint32_t was, uint32_
 
 static void k_syncParticles(OWorld *w) {                    /* :767-831 */
     uint32_t n = pslots(w, w->sorted);
+    PAR_FOR
     for (uint32_t k = 0; k < n; k++) {
         Particle q = loadP(w, w->sorted, k);
         int keep = 0;
@@ -704,7 +734,7 @@ static void k_syncParticles(OWorld *w) {                    /* :767-831 */
             }
         }
         if (keep) {
-            uint32_t m = w->parts[0]++;
+            uint32_t m = at_take(&w->parts[0], 1u);
             if (m < w->cap) storeP(w, m, &q);
         }
     }
@@ -728,6 +758,7 @@ static v2 seedPos(Rng *r, i2 c, uint32_t j, uint32_t n) {   /* :845-855 */
     return V2((float)c.x + 0.5f + 0.9f * q.x, (float)c.y + 0.5f + 0.9f * q.y);
 }
 static void k_syncSeed(OWorld *w) {                          /* :863-890 */
+    PAR_FOR
     for (uint32_t by = 0; by < w->bh; by++)
         for (uint32_t bx = 0; bx < w->bw; bx++)
             for (int ly = 0; ly < BLOCK; ly++)
@@ -743,8 +774,7 @@ static void k_syncSeed(OWorld *w) {                          /* :863-890 */
                     else if (home >= w->C || CNT(home, G_DEST) != claimOf(i)) n = want > 1u ? want : 1u;
                     else if (want > CNT(home, G_SORT) && want < LIQUID_COUNT_MAX) n = want - CNT(home, G_SORT);
                     seedRng(&r, i, w->P->frame, w->P->seed ^ 0x5eedu);
-                    uint32_t m = w->parts[0];
-                    w->parts[0] += n;
+                    uint32_t m = at_take(&w->parts[0], n);
                     for (uint32_t j = 0; j < n; j++) {
                         Particle q;
                         q.pos = seedPos(&r, c, j, n);
@@ -782,6 +812,7 @@ static void clearCell(OWorld *w, uint32_t i) {
 }
 
 static void k_retire(OWorld *w) {                            /* :924-951 */
+    PAR_FOR
     for (uint32_t wid = 0; wid < w->blocks[COUNT_SLOT(COUNT_RETIRED)]; wid++) {
         uint32_t n = w->blocks[RETIRED_SLOT(wid)];
         for (int ly = 0; ly < BLOCK; ly++)
@@ -807,6 +838,7 @@ static void k_retire(OWorld *w) {                            /* :924-951 */
 }
 
 static void k_clear(OWorld *w) {                             /* :962-985 */
+    PAR_FOR
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
         i2 b = activeBlock(w, wid);
         w->blocks[blockIndex(w, b)] = 0u;
@@ -841,8 +873,8 @@ static void splatAxis(OWorld *w, v2 s, float vel, float gx, float gy, int isU) {
         if (wgt <= 0.0f || !faceInWorld(w, at, isU)) continue;
         uint32_t fi = faceIndex(w, at, isU);
         float there = vel + (gx * ((float)at.x - s.x) + gy * ((float)at.y - s.y));
-        SUM(fi, 0) += (uint32_t)wi32(wround(there * wgt * FIX));
-        SUM(fi, 1) += (uint32_t)wi32(wround(wgt * FIX));
+        at_add(&SUM(fi, 0), (uint32_t)wi32(wround(there * wgt * FIX)));
+        at_add(&SUM(fi, 1), (uint32_t)wi32(wround(wgt * FIX)));
     }
 }
 static void splatDensity(OWorld *w, v2 pos, float room) {   /* :1069-1087 */
@@ -858,18 +890,19 @@ static void splatDensity(OWorld *w, v2 pos, float room) {   /* :1069-1087 */
             i2 x = I2(b.x + 1 - ox, at.y), y = I2(at.x, b.y + 1 - oy), xy = I2(b.x + 1 - ox, b.y + 1 - oy);
             at = !walls(w, x) ? x : (!walls(w, y) ? y : (!walls(w, xy) ? xy : I2(-1, -1)));
         }
-        if (inWorld(w, at) && wgt > 0.0f) CNT(cellIndex(w, at), G_DENSITY) += wu32(wround(wgt * room * FIX));
+        if (inWorld(w, at) && wgt > 0.0f) at_add(&CNT(cellIndex(w, at), G_DENSITY), wu32(wround(wgt * room * FIX)));
     }
 }
 static void k_gather(OWorld *w) {                            /* :1017-1039 */
     uint32_t n = pslots(w, w->parts);
+    PAR_FOR
     for (uint32_t k = 0; k < n; k++) {
         Particle q = loadP(w, w->parts, k);
         splatAxis(w, V2(q.pos.x - 0.0f, q.pos.y - 0.5f), q.vel.x, q.affine[0], q.affine[1], 1);
         splatAxis(w, V2(q.pos.x - 0.5f, q.pos.y - 0.0f), q.vel.y, q.affine[2], q.affine[3], 0);
         uint32_t home = cellIndex(w, cellOf(w, q.pos));
-        CNT(home, G_COUNT) += 1u;
-        CNT(home, G_MASS) += wu32(wround(liquidDensity(q.word, q.temp) * FIX));
+        at_add(&CNT(home, G_COUNT), 1u);
+        at_add(&CNT(home, G_MASS), wu32(wround(liquidDensity(q.word, q.temp) * FIX)));
         splatDensity(w, q.pos, liquidVolume(q.word, q.temp));
     }
 }
@@ -884,15 +917,16 @@ static int wasOpen(OWorld *w, i2 q) {                       /* :1503-1508 */
     if (!inWorld(w, q) || countAt(w, q) < 0) return 0;
     return (wu32(FLD(2u * cellIndex(w, q) + 1u, 3)) & stepBit(w, 1u)) == 0u;
 }
-#define FOR_ACTIVE_CELLS(w, c)                                                   \
+#define FOR_ACTIVE_CELLS_SERIAL(w, c)                                            \
     for (uint32_t wid_ = 0; wid_ < activeCount(w); wid_++)                       \
         for (int ly_ = 0; ly_ < BLOCK; ly_++)                                    \
             for (int lx_ = 0; lx_ < BLOCK; lx_++)                                \
                 for (i2 c = I2(activeBlock(w, wid_).x * BLOCK + lx_, activeBlock(w, wid_).y * BLOCK + ly_), \
                      *once_ = &c; once_; once_ = NULL)
+#define FOR_ACTIVE_CELLS(w, c) PAR_FOR FOR_ACTIVE_CELLS_SERIAL(w, c)
 
 static void k_sealJoin(OWorld *w) {                          /* :1531-1543 */
-    FOR_ACTIVE_CELLS(w, c) {
+    FOR_ACTIVE_CELLS_SERIAL(w, c) {
         if (!inWorld(w, c) || !mayBeSealed(w, c)) continue;
         for (int d = 0; d < 2; d++) {
             i2 n = d == 0 ? I2(c.x + 1, c.y) : I2(c.x, c.y + 1);
@@ -901,7 +935,7 @@ static void k_sealJoin(OWorld *w) {                          /* :1531-1543 */
     }
 }
 static void k_sealAnchor(OWorld *w) {                        /* :1545-1554 */
-    FOR_ACTIVE_CELLS(w, c) {
+    FOR_ACTIVE_CELLS_SERIAL(w, c) {
         if (!inWorld(w, c) || !mayBeSealed(w, c)) continue;
         if (wasOpen(w, beside(c, 0)) || wasOpen(w, beside(c, 1)) || wasOpen(w, beside(c, 2)) || wasOpen(w, beside(c, 3)))
             CNT(ufFind(w, cellIndex(w, c)), G_REGION) |= REGION_OPEN;
@@ -919,6 +953,7 @@ static float columnWeight(OWorld *w, i2 p, float density) {  /* :1566-1575 */
 }
 
 static void k_classify(OWorld *w) {                          /* :1578-1696 */
+    PAR_FOR
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
         i2 b = activeBlock(w, wid);
         uint32_t blockBits = 0;
@@ -977,7 +1012,7 @@ static void k_classify(OWorld *w) {                          /* :1578-1696 */
                 for (int j = 0; j < 4; j++) { FLD(2u * i, j) = c[j]; FLD(2u * i + 1u, j) = drift[j]; }
             }
         w->blocks[STATE_SLOT(blockIndex(w, b))] |= blockBits;
-        if (blockBits & HAS_VISCOUS) w->blocks[COUNT_SLOT(COUNT_VISCOUS)] = 1u;
+        if (blockBits & HAS_VISCOUS) at_or(&w->blocks[COUNT_SLOT(COUNT_VISCOUS)], 1u);
     }
 }
 
@@ -1127,14 +1162,16 @@ static void smoothAt(OWorld *w, uint32_t l, i2 c, int color, float omega) {   /*
     FLD(at, 1) = wmix(ey, sy, omega);
 }
 static void k_smooth(OWorld *w, uint32_t l, int color) {
+    PAR_FOR
     for (int32_t y = 0; y < (int32_t)w->lh[l]; y++)
         for (int32_t x = 0; x < (int32_t)w->lw[l]; x++) smoothAt(w, l, I2(x, y), color, 1.0f);
 }
 static void k_restrict(OWorld *w, uint32_t l) {              /* :2342-2377 */
+    if (activeCount(w) == 0u || converged(w)) return;
+    PAR_FOR
     for (int32_t y = 0; y < (int32_t)w->lh[l]; y++)
         for (int32_t x = 0; x < (int32_t)w->lw[l]; x++) {
             i2 c = I2(x, y);
-            if (activeCount(w) == 0u || converged(w)) return;
             uint64_t at = levelSlot(w, l, c);
             int32_t span = 1 << l;
             if (span <= BLOCK) {
@@ -1237,6 +1274,7 @@ static v2 prolonged(const OWorld *w, uint32_t l, i2 c) {   /* :2478-2497 */
 }
 static void k_prolong(OWorld *w, uint32_t l) {               /* :2459-2470 */
     if (activeCount(w) == 0u || converged(w)) return;
+    PAR_FOR
     for (int32_t y = 0; y < (int32_t)w->lh[l]; y++)
         for (int32_t x = 0; x < (int32_t)w->lw[l]; x++) {
             uint64_t at = levelSlot(w, l, I2(x, y));
@@ -1282,8 +1320,9 @@ static i2 tileBlock(const OWorld *w, uint32_t wid, int shift) {
 static void k_relax(OWorld *w, int shift) {
     if (converged(w)) return;
     uint32_t groups = shift ? w->B + w->bw + w->bh + 1u : w->B;
-    float pot[RIM * RIM][2], rho[RIM * RIM], share[RIM * RIM];
+    PAR_FOR
     for (uint32_t wid = 0; wid < groups; wid++) {
+        float pot[RIM * RIM][2], rho[RIM * RIM], share[RIM * RIM];
         i2 b = tileBlock(w, wid, shift);
         i2 origin = I2(b.x * BLOCK - shift, b.y * BLOCK - shift);
         if (b.x < 0 || origin.x >= w->W || origin.y >= w->H) continue;
@@ -1428,7 +1467,7 @@ static void k_cgStart(OWorld *w) {
     }
 }
 static void k_cgJoin(OWorld *w) {
-    FOR_ACTIVE_CELLS(w, c) {
+    FOR_ACTIVE_CELLS_SERIAL(w, c) {
         if (!inWorld(w, c) || gridZ(w, c) <= 0.0f) continue;
         for (int d = 0; d < 2; d++) {
             i2 n = d == 0 ? I2(c.x + 1, c.y) : I2(c.x, c.y + 1);
@@ -1437,7 +1476,7 @@ static void k_cgJoin(OWorld *w) {
     }
 }
 static void k_cgAnchor(OWorld *w) {
-    FOR_ACTIVE_CELLS(w, c) {
+    FOR_ACTIVE_CELLS_SERIAL(w, c) {
         if (!inWorld(w, c) || gridZ(w, c) <= 0.0f) continue;
         for (int d = 0; d < 4; d++) {
             i2 n = beside(c, d);
@@ -1450,6 +1489,7 @@ static void k_cgAnchor(OWorld *w) {
 }
 static void k_cgResidual(OWorld *w) {
     double *bp = bpart(w);
+    PAR_FOR
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
         i2 b = activeBlock(w, wid);
         double v[64][3];
@@ -1462,6 +1502,7 @@ static void k_cgResidual(OWorld *w) {
             double bi = -(double)FLD(2u * i, 1);
             v[li][2] = bi * bi;
             if (w->capture && w->cap_b) w->cap_b[i] = -FLD(2u * i, 1);
+            w->bvec[i] = -FLD(2u * i, 1);
         }
         double out[3];
         sumBlockD(v, out);
@@ -1484,6 +1525,7 @@ static void k_cgRhoBeta(OWorld *w) {
     if (converged(w)) return;
     float *sums = &CG(w->C + w->B, 0);
     if (w->dot_mode == 0) {
+        PAR_FOR
         for (uint32_t wid = 0; wid < activeCount(w); wid++) {
             i2 b = activeBlock(w, wid);
             float v[64][2];
@@ -1514,6 +1556,7 @@ static void k_cgRhoBeta(OWorld *w) {
         head(w)->beta = beta;
     } else {
         double *bp = bpart(w);
+        PAR_FOR
         for (uint32_t wid = 0; wid < activeCount(w); wid++) {
             i2 b = activeBlock(w, wid);
             double v[64][3];
@@ -1563,6 +1606,7 @@ static void k_cgApplyAlpha(OWorld *w) {
     if (converged(w)) return;
     float *sums = &CG(w->C + w->B, 0);
     double *bp = bpart(w);
+    PAR_FOR
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
         i2 b = activeBlock(w, wid);
         float vf[64][2];
@@ -1617,6 +1661,7 @@ static void k_cgUpdate(OWorld *w) {
     double alphad = head(w)->alpha;
     double *bp = bpart(w);
     uint32_t most = 0;
+    PAR_FOR_MOST
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
         i2 b = activeBlock(w, wid);
         double v[64][3];
@@ -1645,6 +1690,53 @@ static void k_cgUpdate(OWorld *w) {
         sumBlockD(v, out);
         bp[4ull * wid + 2] = out[2];
     }
+    /* residual replacement (WigStepCtl.residual_every, the C ABI's): after
+     * every k-th iteration r = b - A x from the updated x, the products
+     * summed in FP64 in beside()'s order, in place of the recursive r */
+    const uint32_t every = w->ctl->residual_every;
+    const int replace = every != 0u && (w->cycle + 1u) % every == 0u;
+    if (replace) {
+        most = 0;
+        PAR_FOR_MOST
+        for (uint32_t wid = 0; wid < activeCount(w); wid++) {
+            i2 b = activeBlock(w, wid);
+            double v[64][3];
+            for (int li = 0; li < 64; li++) {
+                i2 c = I2(b.x * BLOCK + li % BLOCK, b.y * BLOCK + li / BLOCK);
+                v[li][0] = v[li][1] = v[li][2] = 0.0;
+                if (!inWorld(w, c) || gridZ(w, c) <= 0.0f) continue;
+                uint32_t i = cellIndex(w, c);
+                float rho = gridZ(w, c), share = densityAt(w, c);
+                double sum = 0.0, diag = 0.0, mag = 0.0;
+                for (int d = 0; d < 4; d++) {
+                    i2 n = beside(c, d);
+                    float r = rhoAt(w, 0u, n);
+                    if (r < 0.0f) continue;
+                    float beta = fineBeta(rho, r, share, densityAt(w, n));
+                    if (beta == 0.0f) continue;
+                    if (r > 0.0f) {
+                        double xn = (double)CG(cellIndex(w, n), 0);
+                        sum += (double)beta * xn;
+                        mag += (double)beta * fabs(xn);
+                    }
+                    diag += (double)beta;
+                }
+                if (pinnedAt(w, i)) diag += (double)PIN;
+                float rn = (float)((double)w->bvec[i] - (diag * (double)CG(i, 0) - sum));
+                CG(i, 1) = rn;
+                FLD(2u * i, 1) = -rn;
+                uint32_t bits = ubits(fabsf(rn));
+                if (bits > most) most = bits;
+                v[li][2] = (double)rn * (double)rn;
+                double a = diag * fabs((double)CG(i, 0)) + mag;    /* (|A| |x|) at i */
+                v[li][1] = a * a;
+            }
+            double out[3];
+            sumBlockD(v, out);
+            bp[4ull * wid + 1] = out[1];
+            bp[4ull * wid + 2] = out[2];
+        }
+    }
     uint32_t *slot = &w->blocks[COUNT_SLOT(COUNT_RESIDUAL + w->cycle % 3u)];
     if (most > *slot) *slot = most;
     /* cgCheck: whether the next iteration is skipped */
@@ -1661,6 +1753,15 @@ static void k_cgUpdate(OWorld *w) {
         sumBlocksD(w, 2, &rr);
         h->rr = rr;
         w->converged_flag = fbits(*slot) < w->ctl->tol;
+    }
+    /* with residual replacement, only a replaced (true) residual stops the
+     * solve: under the tolerance, or within twice the rounding floor of an
+     * f32 iterate, ||b - A x|| <= 2^-25 ||(|A| |x|)|| (the floor measures
+     * about 2^-24 / 4 of it) */
+    if (every != 0u) {
+        double aa = 0.0;
+        if (replace) sumBlocksD(w, 1, &aa);
+        w->converged_flag = replace && (w->converged_flag || h->rr <= 8.8817841970012523e-16 * aa);
     }
     h->converged = (uint32_t)w->converged_flag;
 }
@@ -1815,6 +1916,7 @@ static v2 slipOf(OWorld *w, const Particle *q) {             /* :3605-3614 */
 static void k_advect(OWorld *w) {                            /* :3616-3694 */
     uint32_t n = pslots(w, w->parts);
     uint32_t *pb = w->parts;
+    PAR_FOR
     for (uint32_t k = 0; k < n; k++) {
         Particle q = loadP(w, pb, k);
         v2 orig = q.pos;
@@ -1860,7 +1962,7 @@ static void k_advect(OWorld *w) {                            /* :3616-3694 */
         float lx = FLD(li, 0), lz = FLD(li, 2);
         if (lz > 0.0f) {
             int falling = lx < 0.5f * lz * wlength(V2(0.0f, w->P->earthG));
-            CNT(cellIndex(w, cellOf(w, pos)), G_CAME) |= falling ? CAME_FALLING : CAME_PRESSING;
+            at_or(&CNT(cellIndex(w, cellOf(w, pos)), G_CAME), falling ? CAME_FALLING : CAME_PRESSING);
         }
     }
 }
@@ -1868,12 +1970,14 @@ static void k_advect(OWorld *w) {                            /* :3616-3694 */
 /* ------------------------------------------------------------------ sort */
 static void k_sortCount(OWorld *w) {                         /* :3700-3708 */
     uint32_t n = pslots(w, w->parts);
+    PAR_FOR
     for (uint32_t k = 0; k < n; k++) {
         v2 pos = V2(fbits(MOTION(w->parts, k, 0)), fbits(MOTION(w->parts, k, 1)));
-        STATE(w->parts, k, 3) = CNT(cellIndex(w, cellOf(w, pos)), G_SORT)++;
+        STATE(w->parts, k, 3) = at_take(&CNT(cellIndex(w, cellOf(w, pos)), G_SORT), 1u);
     }
 }
 static void k_scanBlocks(OWorld *w) {                        /* :3730-3753 */
+    PAR_FOR
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
         i2 b = activeBlock(w, wid);
         uint32_t run = 0;
@@ -1904,6 +2008,7 @@ static void k_scanTotals(OWorld *w) {                        /* :3757-3780 */
     w->blocks[COUNT_SLOT(COUNT_PARTICLES)] = run;
 }
 static void k_scanFinish(OWorld *w) {                        /* :3783-3790 */
+    PAR_FOR
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
         i2 b = activeBlock(w, wid);
         for (int li = 0; li < 64; li++) {
@@ -1914,6 +2019,7 @@ static void k_scanFinish(OWorld *w) {                        /* :3783-3790 */
 }
 static void k_scatter(OWorld *w) {                           /* :3793-3808 */
     uint32_t n = pslots(w, w->parts);
+    PAR_FOR
     for (uint32_t k = 0; k < n; k++) {
         v2 pos = V2(fbits(MOTION(w->parts, k, 0)), fbits(MOTION(w->parts, k, 1)));
         uint32_t i = cellIndex(w, cellOf(w, pos));
@@ -1956,7 +2062,7 @@ static void k_rasterClaims(OWorld *w) {                      /* :3928-3948 */
             else if (kindOf(w, t) == KIND_GAS && gridZ(w, c) >= 0.0f) to = airTo(w, c, 1);
         }
         CNT(i, G_AIR_TO) = to;
-        if (to != NONE && CNT(to, G_TAKEN) < ~i) CNT(to, G_TAKEN) = ~i;
+        if (to != NONE) at_max(&CNT(to, G_TAKEN), ~i);
     }
 }
 static int moveOutOfTheWay(OWorld *w, uint32_t i, const uint32_t v[4]) {   /* :3954-3973 */
@@ -2078,7 +2184,7 @@ static int check_ctl(const OWorld *w, const WigParams *P, const WigStepCtl *ctl)
     if (P->size[0] != (uint32_t)w->W || P->size[1] != (uint32_t)w->H) return WIG_E_INVALID;
     if (ctl->viscous || ctl->surface || ctl->bodies_run) return WIG_E_INVALID;
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->level_pairs > 64u) return WIG_E_INVALID;
-    if (ctl->tol_mode > WIG_TOL_REL_L2) return WIG_E_INVALID;
+    if (ctl->tol_mode > WIG_TOL_REL_L2 || ctl->residual_every > WIG_MAX_CYCLES) return WIG_E_INVALID;
     return WIG_OK;
 }
 
@@ -2187,14 +2293,15 @@ OWorld *wor_create(uint32_t width, uint32_t height, uint32_t coarse_levels) {
     w->blocks = (uint32_t *)calloc(1, w->bytes[WIG_PART_BLOCKS]);
     w->cg = (float *)calloc(1, w->bytes[WIG_PART_FLUIDCG]);
     w->solve = (uint8_t *)calloc(1, w->bytes[WIG_PART_SOLVE]);
+    w->bvec = (float *)calloc(w->C, sizeof(float));
     w->dot_mode = 1;
-    if (!w->cells || !w->sorted || !w->parts || !w->counters || !w->field || !w->blocks || !w->cg || !w->solve) return NULL;
+    if (!w->cells || !w->sorted || !w->parts || !w->counters || !w->field || !w->blocks || !w->cg || !w->solve || !w->bvec) return NULL;
     return w;
 }
 void wor_destroy(OWorld *w) {
     if (!w) return;
     free(w->cells); free(w->sorted); free(w->parts); free(w->counters);
-    free(w->field); free(w->blocks); free(w->cg); free(w->solve); free(w->cap_b);
+    free(w->field); free(w->blocks); free(w->cg); free(w->solve); free(w->cap_b); free(w->bvec);
     free(w);
 }
 void *wor_part(OWorld *w, int part, uint64_t *bytes) {

@@ -256,3 +256,65 @@ def test_fp64_cg_oracle_manufactured():
         b[y, x] = s + (xs[y, x] if (y, x) == tuple(first) else 0.0)
     x64, it, rel = w.cg64(b.astype(np.float32).ravel(), tol=1e-13)
     assert np.allclose(x64.reshape(H, W)[inner], xs[inner], atol=1e-5)
+
+
+@pytest.mark.parametrize("every", [1, 3])
+@pytest.mark.parametrize("scene", ["dam", "precision"])
+def test_residual_replacement_stops_on_the_true_residual(every, scene):
+    """With residual replacement every k iterations (WigStepCtl.residual_every)
+    the solve stops only on a recomputed residual, so its pressure's true
+    ||b - A x|| / ||b|| (FP64, from the f32 x) meets the tolerance, or, where
+    an f32 pressure cannot (the dam: even the FP64 CG solution rounded to
+    f32 leaves 8e-6), comes within twice that floor.  The iterations run
+    are a multiple of k."""
+    W = H = 64
+    tol = 1e-6
+    if scene == "dam":
+        w = make_oracle(W, H, scenes.dam_break(W, H, block=(40, 44)))
+        sim = config_sim(w, W, H, tol=tol)
+        warm = 3
+    else:
+        w = make_oracle(W, H, scenes.precision_world(W, 3)[0])
+        sim = config_sim(w, W, H, tol=tol)
+        sim.earth_g = 0.0
+        warm = 0
+    sim.solver = SolverSettings.relative(tol, residual_every=every)
+    for _ in range(warm):
+        sim.step()
+    w.capture(True)
+    sim.step()
+    b = w.captured_b()
+    x = w.part(abi.PART_FIELD)[:8 * W * H].reshape(-1, 8)[:, 0].copy()
+    s = w.solve_stats()
+    assert s.converged and s.cycles % every == 0, (s.cycles, every)
+    rel = w.relres(b, x)
+    x64, _, _ = w.cg64(b, tol=1e-13)
+    floor = w.relres(b, x64.astype(np.float32))
+    assert rel <= max(tol * 1.001, 2.2 * floor), (rel, floor)
+    if scene == "precision":
+        assert floor < 0.2 * tol and rel <= tol * 1.001
+    assert abs(np.sqrt(s.rr / s.bb) - rel) <= 1e-3 * rel     # the stop's residual is the true one
+
+
+def test_openmp_build_equals_serial():
+    """The CPU baseline's build (OpenMP over the reference's independent
+    invocations, atomics where they meet) steps every world to the serial
+    oracle's buffers: cells, counters, field, blocks and fluidCg bit for bit,
+    the particles as multisets (their slots' order is the atomics')."""
+    import os
+    from oracle.oracle import OracleWorld, lib, OMP_PATH
+    from tests.helpers import compare_worlds
+    omp = lib(OMP_PATH)
+    W, H = 96, 80
+    for cells in (scenes.dam_break(W, H, block=(48, 60)), scenes.discs_dam(W, H, discs=10, rmin=3, rmax=8)[0]):
+        ws = OracleWorld(W, H, kinds=E.kinds())
+        wp = OracleWorld(W, H, kinds=E.kinds(), L=omp)
+        sims = []
+        for w in (ws, wp):
+            w.upload(abi.PART_CELLS, cells)
+            sims.append(config_sim(w, W, H, tol=1e-6))
+        for step in range(12):
+            for s in sims:
+                s.step()
+            d = compare_worlds(ws, wp)
+            assert not d, f"step {step}: " + "; ".join(d)

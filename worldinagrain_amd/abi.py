@@ -39,7 +39,7 @@ class WigStepCtl(ct.Structure):
         ("fluid_cycles", ct.c_uint32), ("shifted_first", ct.c_uint32), ("tol_mode", ct.c_uint32), ("tol", ct.c_float),
         ("pre_relax", ct.c_uint32), ("post_relax", ct.c_uint32), ("sweep_pairs", ct.c_uint32),
         ("coarse_sweeps", ct.c_uint32), ("first_chunk", ct.c_uint32), ("chunk", ct.c_uint32),
-        ("level_pairs", ct.c_uint32), ("vcycle_storage", ct.c_uint32),
+        ("level_pairs", ct.c_uint32), ("vcycle_storage", ct.c_uint32), ("residual_every", ct.c_uint32),
     ]
 
 

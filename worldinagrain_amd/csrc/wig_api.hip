@@ -46,6 +46,7 @@ __global__ void k_cgBeta(WorldGrid g);
 template <int S> __global__ void k_cgApply(WorldGrid g);
 __global__ void k_cgAlpha(WorldGrid g);
 template <int S> __global__ void k_cgUpdate(WorldGrid g);
+template <int S> __global__ void k_cgTrue(WorldGrid g);
 __global__ void k_cgCheck(WorldGrid g);
 template <int S> __global__ void k_cgFinish(WorldGrid g);
 __global__ void k_project(WorldGrid g);
@@ -94,7 +95,7 @@ int wig_fail_message(int code, const char *msg) { return fail(code, "%s", msg); 
 #define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compactMarks) X(compact) X(retire) X(clear) X(gather) \
     X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
     X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
-    X(cgRho) X(cgBeta) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgCheck) \
+    X(cgRho) X(cgBeta) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgTrue) X(cgCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
     X(scatter) X(rasterClaims) X(raster) X(unpaint)
 #define X_ENUM(n) KID_##n,
@@ -386,7 +387,8 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     size_world(w, width, height, coarse_levels);
     w->vlevel_bytes = 64u;
     for (uint32_t l = 1; l <= w->d.L; l++)   // (and each level's tile flags, for tiles of 8 or more)
-        w->vlevel_bytes += 12ull * w->d.lw[l] * w->d.lh[l] + 64u + ceil_div(w->d.lw[l], 8u) * ceil_div(w->d.lh[l], 8u) + 64u;
+        w->vlevel_bytes += 12ull * w->d.lw[l] * w->d.lh[l] + 64u + ceil_div(w->d.lw[l], 8u) * ceil_div(w->d.lh[l], 8u) + 64u
+                           + 8ull * w->d.lw[l] * w->d.lh[l] + 64u;
     if (hipGetDevice(&w->device) != hipSuccess || hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking) != hipSuccess) {
         fail(WIG_E_DEVICE, "no HIP device or stream");
         delete w;
@@ -400,7 +402,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         }
     }
     if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
-        || hipMalloc(&w->fine, 73ull * 64ull * w->d.B + 256u) != hipSuccess
+        || hipMalloc(&w->fine, 77ull * 64ull * w->d.B + 256u) != hipSuccess
         || hipMalloc(&w->vlevels, w->vlevel_bytes) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess) {
         fail(WIG_E_NOMEM, "hipMalloc failed");
@@ -410,7 +412,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     for (int p = 0; p < WIG_PART_COUNT; p++)
         if (zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
     zero(w->own, w->kind, 256);
-    zero(w->own, w->fine, 73ull * 64ull * w->d.B + 256u);
+    zero(w->own, w->fine, 77ull * 64ull * w->d.B + 256u);
     zero(w->own, w->vlevels, w->vlevel_bytes);
     zero(w->own, w->totals, sizeof(WigRunTotals));
     Dev &d = w->d;
@@ -470,6 +472,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         d.hb = (uint32_t *)(f + 64ull * P);
         d.hr2 = (uint32_t *)(f + 68ull * P);
         d.cflag = f + 72ull * P;
+        d.cb = (float *)(f + 73ull * P);
         // the 16-bit V-cycle's coarse levels: four halves and a float a cell
         uint8_t *v = (uint8_t *)w->vlevels;
         for (uint32_t l = 1; l <= d.L; l++) {
@@ -482,6 +485,12 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         for (uint32_t l = 1; l <= d.L; l++) {
             d.tflag[l] = v;
             v += ceil_div(d.lw[l], 8u) * ceil_div(d.lh[l], 8u) + 64u;
+        }
+        // and each level's correction after the down pass (k_levelDown to k_levelUp)
+        v = (uint8_t *)(((uintptr_t)v + 63u) & ~(uintptr_t)63u);
+        for (uint32_t l = 1; l <= d.L; l++) {
+            d.pdown[l] = (uint2 *)v;
+            v += 8ull * d.lw[l] * d.lh[l] + 64u;
         }
     }
     WigWorld *self = w;
@@ -677,7 +686,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     if (ctl->viscous || ctl->surface || ctl->bodies_run)
         return fail(WIG_E_INVALID, "viscosity, surface tension and bodies are not built (water only)");
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u
-        || ctl->level_pairs > MAX_LEVEL_PAIRS || ctl->vcycle_storage > WIG_STORE_BF16)
+        || ctl->level_pairs > MAX_LEVEL_PAIRS || ctl->vcycle_storage > WIG_STORE_BF16 || ctl->residual_every > WIG_MAX_CYCLES)
         return fail(WIG_E_INVALID, "solver control out of range");
     if (ctl->pre_relax != 1u || ctl->post_relax != 1u)
         return fail(WIG_E_INVALID, "one relaxation before and one after the coarse levels (PRE_RELAX = POST_RELAX = 1) is what is built");
@@ -695,6 +704,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d.tol = ctl->tol;
     d.level_pairs = ctl->level_pairs ? ctl->level_pairs : 1u;
     d.store = ctl->vcycle_storage;
+    d.replace_every = ctl->residual_every;
     for (uint32_t l = 0; l <= MG_LEVELS; l++) d.tile[l] = fitTile(w->tile[l], d.level_pairs);
     w->d.level_pairs = d.level_pairs;
     for (uint32_t l = 0; l <= MG_LEVELS; l++) w->d.tile[l] = d.tile[l];
@@ -704,6 +714,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
 // What a step launches, for the batch's worlds and their controls.
 struct StepPlan {
     bool any_full = false;
+    bool replace = false;    // a full world replaces its residual (k_cgTrue in the iteration)
     uint32_t stores = 0;     // the V-cycle storages (1 << STORE_*) of the full worlds
     uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
     size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
@@ -750,6 +761,7 @@ template <int S> static void enqueue_iteration_store(Batch &b, const StepPlan &p
         launch(b, KID_cgApply, k_cgApply<S>, SH_BG16, 256);
     } else {
         launch(b, KID_cgUpdate, k_cgUpdate<S>, SH_BG16, 256);
+        if (pl.replace) launch(b, KID_cgTrue, k_cgTrue<S>, SH_BG16, 256);
     }
 }
 }  // extern "C++"
@@ -859,6 +871,7 @@ static int batch_assign(Batch *b, WigWorld *const *worlds, uint32_t n) {
 // The plan of a batch's iterations: which storages, levels and LDS sizes.
 static void plan_iterations(const Batch &b, StepPlan &pl) {
     pl.stores = 0;
+    pl.replace = false;
     pl.max_level = 0;
     pl.coarse_lds = 0;
     for (uint32_t l = 0; l <= MG_LEVELS; l++) pl.down_lds[l] = pl.up_lds[l] = 0;
@@ -866,6 +879,7 @@ static void plan_iterations(const Batch &b, StepPlan &pl) {
         const Dev &d = b.htab[i];
         if (!d.full) continue;
         pl.stores |= 1u << d.store;
+        pl.replace = pl.replace || d.replace_every != 0u;
         pl.coarse_lds = std::max(pl.coarse_lds, b.w[i]->lds_bytes);
         for (uint32_t l = 1; l < d.lds_level; l++) {
             pl.max_level = std::max(pl.max_level, l);
@@ -891,6 +905,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         if (!d.full) continue;
         pl.any_full = true;
         pl.stores |= 1u << d.store;
+        pl.replace = pl.replace || d.replace_every != 0u;
         pl.max_cycles = std::max(pl.max_cycles, d.fluid_cycles);
         if (d.first_chunk != 0u) {
             uint32_t f = d.first_chunk == WIG_CHUNK_AUTO ? std::max(b.w[i]->last_hint, 1u) : d.first_chunk;
@@ -911,6 +926,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         sig.push_back((uint64_t)(uintptr_t)b.tab);
         sig.push_back(pl.coarse_lds);
         sig.push_back(pl.stores);
+        sig.push_back(pl.replace);
         sig.push_back(pl.max_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(pl.down_lds[l]), sig.push_back(pl.up_lds[l]);
         sig.push_back((uint64_t)(uintptr_t)b.stream);

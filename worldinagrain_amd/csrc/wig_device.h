@@ -92,6 +92,7 @@
 #define KELVIN 273.15f
 #define VAPOUR_MAX 8.0f
 #define BOIL_K (100.0f + KELVIN)
+#define ATTAINABLE2 8.8817841970012523e-16   // (2^-25)^2: residual replacement's stop at twice an f32 iterate's rounding floor
 
 // The FP64 state of a solve: its header, then three doubles per block
 // (two running sums of a pass and the squared residual).
@@ -159,6 +160,12 @@ struct Dev {
     // a tiled level's tiles: whether any of a tile's own cells holds fluid,
     // taken by k_levelDown in each solve's first iteration (tile by row)
     uint8_t *tflag[MG_LEVELS + 1];
+    // a tiled level's correction as k_levelDown leaves it, for k_levelUp
+    // (8 B a cell: two floats, or two 16-bit values in the first word).
+    // The level's own records take k_levelUp's result; were the down pass's
+    // kept there too, a tile of either pass could read into its halo a value
+    // a neighbouring tile of the same launch had already overwritten.
+    uint2 *pdown[MG_LEVELS + 1];
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
     uint32_t wave_cells;    // and below it, levels of at most this many cells by its first wavefront
     uint32_t tile[MG_LEVELS + 1];   // k_levelDown/k_levelUp's tile side, per level below lds_level
@@ -168,6 +175,10 @@ struct Dev {
     uint32_t full, fluid_cycles, shifted_first, first_chunk, chunk;
     uint32_t sweep_pairs, coarse_sweeps, tol_mode, level_pairs;
     float tol;
+    // residual replacement (WigStepCtl.residual_every; 0 off) and the
+    // solve's b (packed), which it reads
+    uint32_t replace_every;
+    float *cb;
 };
 
 // A launch over a batch of worlds.  Its workgroups are split among the
@@ -209,6 +220,7 @@ struct WorldGrid {
     d.vlev[2] = dg_.vlev[(l) + 1u];                                   \
     d.vrho[2] = dg_.vrho[(l) + 1u];                                   \
     d.tflag[1] = dg_.tflag[l];                                        \
+    d.pdown[1] = dg_.pdown[l];                                        \
     const uint32_t tile_ = dg_.tile[l];                               \
     const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
     const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \

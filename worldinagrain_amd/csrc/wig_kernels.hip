@@ -1510,12 +1510,25 @@ __device__ __forceinline__ LevelTile levelTile(float *sm, uint32_t T, uint32_t H
 // The tile's cells of the kernel's level (its Dev's level 1) into LDS,
 // every load made at once: TILE_LOADS a thread a round, one round for the
 // tiles the LDS holds (a side with its halo of 48 or less); outside the
-// level p and rhs 0, rho -1.
+// level p and rhs 0, rho -1.  The correction p: with UP the down pass's
+// (pdown, which no tile of this launch writes); else 0, as restrictLevel
+// left every cell of the level (:2342-2377).
+template <int S> __device__ __forceinline__ float2 pdownLoad(const Dev &d, i2 c) {
+    const uint2 u = d.pdown[1][lvIndex(d, 1u, c)];
+    if (S == STORE_F32) return make_float2(__uint_as_float(u.x), __uint_as_float(u.y));
+    return make_float2(dec16<S>((uint16_t)u.x), dec16<S>((uint16_t)(u.x >> 16)));
+}
+template <int S> __device__ __forceinline__ void pdownStore(const Dev &d, i2 c, float2 p) {
+    uint2 &u = d.pdown[1][lvIndex(d, 1u, c)];
+    if (S == STORE_F32) u = make_uint2(__float_as_uint(p.x), __float_as_uint(p.y));
+    else u.x = (uint32_t)enc16<S>(p.x) | ((uint32_t)enc16<S>(p.y) << 16);
+}
 #define TILE_LOADS 9
-template <int S> __device__ __forceinline__ void loadLevelTile(const Dev &d, const LevelTile &t) {
+template <int S, bool UP> __device__ __forceinline__ void loadLevelTile(const Dev &d, const LevelTile &t) {
     const int32_t n = t.R * t.R, w = (int32_t)d.lw[1], h = (int32_t)d.lh[1], tid = (int32_t)threadIdx.x;
     for (int32_t b0 = 0; b0 < n; b0 += TILE_LOADS * (int32_t)TILE_THREADS) {
         float4 e[TILE_LOADS];
+        float2 p[TILE_LOADS];
         float r[TILE_LOADS];
 #pragma unroll
         for (int q = 0; q < TILE_LOADS; q++) {
@@ -1523,6 +1536,7 @@ template <int S> __device__ __forceinline__ void loadLevelTile(const Dev &d, con
             const int32_t k = min(b0 + q * (int32_t)TILE_THREADS + tid, n - 1);
             const i2 c = clampTo(I2(t.ox + k % t.R, t.oy + k / t.R), w, h);
             e[q] = lvLoad<S>(d, 1u, c);
+            p[q] = UP ? pdownLoad<S>(d, c) : make_float2(0.0f, 0.0f);
             r[q] = lvRho<S>(d, 1u, c);
         }
 #pragma unroll
@@ -1531,8 +1545,8 @@ template <int S> __device__ __forceinline__ void loadLevelTile(const Dev &d, con
             const int32_t k = b0 + q * (int32_t)TILE_THREADS + tid;
             if (k >= n) continue;
             const bool in = inLevel(d, 1u, I2(t.ox + k % t.R, t.oy + k / t.R));
-            t.px[k] = in ? e[q].x : 0.0f;
-            t.py[k] = in ? e[q].y : 0.0f;
+            t.px[k] = in ? p[q].x : 0.0f;
+            t.py[k] = in ? p[q].y : 0.0f;
             t.rx[k] = in ? e[q].z : 0.0f;
             t.ry[k] = in ? e[q].w : 0.0f;
             t.rho[k] = in ? r[q] : -1.0f;
@@ -1601,7 +1615,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     // a tile none of whose own cells holds fluid leaves every value as the
     // solve's first iteration left it (only cells of rho > 0 change)
     if (it.cycle > 0u && fluidTile == 0u) return;
-    loadLevelTile<S>(d, t);
+    loadLevelTile<S, false>(d, t);
     __syncthreads();
     tileBetas(t);
     __syncthreads();
@@ -1620,7 +1634,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
         }
         resx[k] = qx;
         resy[k] = qy;
-        if (inLevel(d, 1u, c)) lvStoreP<S>(d, 1u, c, make_float2(t.px[j], t.py[j]));
+        if (inLevel(d, 1u, c)) pdownStore<S>(d, c, make_float2(t.px[j], t.py[j]));
         anyFluid |= t.rho[j] > 0.0f ? 1 : 0;
     }
     anyFluid = __syncthreads_or(anyFluid);
@@ -1685,7 +1699,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
             ce[q] = lvLoadP<S>(d, 2u, c);
             cr[q] = lvRho<S>(d, 2u, c);
         }
-        loadLevelTile<S>(d, t);
+        loadLevelTile<S, true>(d, t);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             const int32_t k = tid + q * (int32_t)TILE_THREADS;
@@ -2234,6 +2248,7 @@ __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
             double bi = -(double)FLD(d, 2u * i, 1);
             bb = bi * bi;
             if (d.capb) d.capb[pi] = -FLD(d, 2u * i, 1);
+            if (d.replace_every) d.cb[pi] = -FLD(d, 2u * i, 1);
         }
         bb = wave_tree_sum(bb);
         most = wave_max_u32(most);
@@ -2506,6 +2521,76 @@ template <int S> __global__ void __launch_bounds__(256) k_cgUpdate(WorldGrid g) 
         }
     }
 }
+// Residual replacement (WigStepCtl.residual_every, this library's): after
+// every replace_every-th iteration, r = b - A x from the updated x, the
+// products summed in FP64 in beside()'s order, in place of the recursive
+// r; with it the V-cycle's next rhs, and the blocks' r.r and max |r| that
+// cgCheck judges the stop by.  Beside them each block's sum of
+// (|A| |x|)^2: an f32 iterate's rounding leaves ||b - A x|| near
+// 2^-24 ||(|A| |x|)|| / 4, the most a solve in f32 can attain.
+template <int S> __global__ void __launch_bounds__(256) k_cgTrue(WorldGrid g) {
+    WORLD(g);
+    if (d.store != S || d.replace_every == 0u) return;
+    const uint32_t wf = PLACES_FIRST;
+    uint32_t e[NB];
+    placeEntries(d, wf, e);
+    const Iter it = iterWords(d);
+    const float vscale = d.head->vscale;
+    if (!it.encoded || it.converged || (it.cycle + 1u) % d.replace_every != 0u) return;
+    const uint32_t x = LANE & 7u, y = LANE >> 3;
+    for (uint32_t w0 = wf; w0 < it.active; w0 += PLACES_STRIDE) {
+        uint32_t i[NB];
+        placeCellsFrom(it, w0, e, i);
+        placeEntries(d, w0 + PLACES_STRIDE, e);
+        uint32_t f[NB];
+        float4 cf[NB];
+        float xs[NB], bs[NB], xn[NB][4];
+        for (uint32_t j = 0; j < NB; j++) {
+            f[j] = d.cflag[i[j]];
+            cf[j] = d.coef[i[j]];
+            xs[j] = d.cx[i[j]];
+            bs[j] = d.cb[i[j]];
+            xn[j][0] = d.cx[x == 0u ? acrossEdge(d, i[j], 0) : i[j] - 1u];
+            xn[j][1] = d.cx[x == 7u ? acrossEdge(d, i[j], 1) : i[j] + 1u];
+            xn[j][2] = d.cx[y == 0u ? acrossEdge(d, i[j], 2) : i[j] - 8u];
+            xn[j][3] = d.cx[y == 7u ? acrossEdge(d, i[j], 3) : i[j] + 8u];
+        }
+        for (uint32_t j = 0; j < NB; j++) {
+            const bool mine = w0 + j < it.active;
+            double rr = 0.0, aa = 0.0;
+            uint32_t most = 0;
+            if (mine && (f[j] & CF_FLUID)) {
+                const float v[4] = {cf[j].x, cf[j].y, cf[j].z, cf[j].w};
+                double sum = 0.0, diag = 0.0, mag = 0.0;
+                for (int k = 0; k < 4; k++) {
+                    const float b = fabsf(v[k]);
+                    if (b == 0.0f) continue;
+                    if (__float_as_uint(v[k]) >> 31) {
+                        sum += (double)b * (double)xn[j][k];
+                        mag += (double)b * fabs((double)xn[j][k]);
+                    }
+                    diag += (double)b;
+                }
+                if (f[j] & CF_PINNED) diag += (double)PINF;
+                const float rn = (float)((double)bs[j] - (diag * (double)xs[j] - sum));
+                d.cr[i[j]] = rn;
+                fnStoreX<S>(d, 2, i[j], S == STORE_F32 ? -rn : -rn * vscale);
+                most = ubits(fabsf(rn));
+                rr = (double)rn * (double)rn;
+                const double a = diag * fabs((double)xs[j]) + mag;
+                aa = a * a;
+            }
+            rr = wave_tree_sum(rr);
+            aa = wave_tree_sum(aa);
+            most = wave_max_u32(most);
+            if (mine && LANE == 0) {
+                d.bpart[4ull * (w0 + j) + 1] = aa;
+                d.bpart[4ull * (w0 + j) + 2] = rr;
+                d.bpart[4ull * (w0 + j) + 3] = (double)most;
+            }
+        }
+    }
+}
 // (and the iteration counter stepped, the iteration's last pass: the
 // counter lives in device memory so a captured iteration replays alike)
 __device__ __forceinline__ void stepCycle(const Dev &d, uint32_t cycle) {
@@ -2538,9 +2623,15 @@ __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
         uint32_t b = (uint32_t)d.bpart[4ull * j + 3];
         m = b > m ? b : m;
     }
-    double rr, unused;
+    // (and, at a replaced residual, sum (|A| |x|)^2, cgTrue's: places in
+    // the order laneSum takes them)
+    const bool replaced = d.replace_every != 0u && (cycle + 1u) % d.replace_every == 0u;
+    double aLane = 0.0;
+    if (replaced)
+        for (uint32_t j = threadIdx.x; j < ph.na; j += SUM_LANES) aLane += d.bpart[4ull * j + 1];
+    double rr, aa;
     uint32_t most;
-    laneTree(laneSum<2>(d, ph, 0, 2u), 0.0, m, t0, t1, sMost, &rr, &unused, &most);
+    laneTree(laneSum<2>(d, ph, 0, 2u), aLane, m, t0, t1, sMost, &rr, &aa, &most);
     if (threadIdx.x == 0) {
         if (most > *slot) *slot = most;
         d.head->vscale = vcycleScale(d, most);
@@ -2554,6 +2645,10 @@ __global__ void __launch_bounds__(1024) k_cgCheck(WorldGrid g) {
         } else {
             c2 = fbits(*slot) < d.tol;
         }
+        // with residual replacement, only a replaced (true) residual stops the
+        // solve: under the tolerance, or within twice the rounding floor of
+        // an f32 iterate, (2^-24 / 2)^2 (|A| |x|)^2
+        if (d.replace_every) c2 = replaced && (c2 || rr <= ATTAINABLE2 * aa);
         __hip_atomic_store(&h->converged, c2 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // converged: the host's next checkpoint (first, first + chunk, ...)
         // after this iteration stops the solve
@@ -3126,6 +3221,7 @@ __global__ void __launch_bounds__(64) k_reports(WorldGrid g, uint32_t *out) {
     template __global__ void k_cgRho<S>(WorldGrid); \
     template __global__ void k_cgApply<S>(WorldGrid); \
     template __global__ void k_cgUpdate<S>(WorldGrid); \
+    template __global__ void k_cgTrue<S>(WorldGrid); \
     template __global__ void k_cgFinish<S>(WorldGrid);
 STORE_KERNELS(0)
 STORE_KERNELS(1)

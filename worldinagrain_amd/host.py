@@ -39,20 +39,24 @@ class SolverSettings:
     chunk: int = 2
     level_pairs: int = 1     # red-black pairs a coarse level, down and up (the reference's 1)
     storage: int = abi.WIG_STORE_F32   # the V-cycle's storage (WIG_STORE_*)
+    # this build's: every k-th iteration the residual recomputed as b - A x
+    # and the stop judged only there (0: the recursive residual throughout)
+    residual_every: int = 0
 
     @staticmethod
     def reference():
         return SolverSettings()
 
     @staticmethod
-    def relative(tol, max_cycles=100, sweep_pairs=2, level_pairs=2, coarse_sweeps=10, storage=abi.WIG_STORE_F32):
+    def relative(tol, max_cycles=100, sweep_pairs=2, level_pairs=2, coarse_sweeps=10, storage=abi.WIG_STORE_F32,
+                 residual_every=0):
         """A solve to ||r|| <= tol ||b|| (the benchmark configurations: 2
         red-black sweep pairs before and after the coarse levels, and on each
         of them; 10 SOR sweep pairs on the coarsest, which converge the
         iteration as fast as the reference's 40)."""
         return SolverSettings(tol_mode=abi.WIG_TOL_REL_L2, tol=tol, max_cycles=max_cycles, adaptive_cycles=False,
                               sweep_pairs=sweep_pairs, first_chunk=-1, level_pairs=level_pairs,
-                              coarse_sweeps=coarse_sweeps, storage=storage)
+                              coarse_sweeps=coarse_sweeps, storage=storage, residual_every=residual_every)
 
 
 def mulberry32_next(state):
@@ -115,6 +119,7 @@ class Simulation:
         c.sweep_pairs, c.coarse_sweeps = s.sweep_pairs, s.coarse_sweeps
         c.level_pairs = s.level_pairs
         c.vcycle_storage = s.storage
+        c.residual_every = s.residual_every
         if s.first_chunk < 0:
             c.first_chunk = 0xFFFFFFFF          # WIG_CHUNK_AUTO: the world's last solve's count
             c.chunk = s.chunk
@@ -191,6 +196,7 @@ class BatchHost:
             c.sweep_pairs, c.coarse_sweeps = sv.sweep_pairs, sv.coarse_sweeps
             c.level_pairs = sv.level_pairs
             c.vcycle_storage = sv.storage
+            c.residual_every = sv.residual_every
             if sv.first_chunk < 0:
                 c.first_chunk, c.chunk = 0xFFFFFFFF, sv.chunk
             else:

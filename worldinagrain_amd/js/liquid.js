@@ -103,7 +103,7 @@ class LiquidSimulation {
     // relaxation's starting tiling (flipped each step), the solve's settings.
     stepControl() {
         this.shiftedFirst = !this.shiftedFirst;
-        const buf = new ArrayBuffer(64);
+        const buf = new ArrayBuffer(68);
         const u = new Uint32Array(buf);
         const f = new Float32Array(buf);
         u[0] = this.fluidStats.active ? 1 : 0;
@@ -117,6 +117,7 @@ class LiquidSimulation {
         u[11] = this.solver.coarseSweeps;
         u[12] = this.solver.firstChunk >>> 0;
         u[13] = this.solver.chunk;
+        u[16] = this.solver.residualEvery || 0;
         return buf;
     }
 
