@@ -165,7 +165,9 @@ int wig_download(WigWorld *w, int part, void *dst, size_t bytes);
  * world (bytes of it as wig_layout gives them). */
 int wig_device_ptr(WigWorld *w, int part, void **ptr);
 
-/* Runs later calls on this HIP stream (a hipStream_t; NULL: the world's own). */
+/* Runs later calls on this HIP stream (a hipStream_t; NULL: the world's own).
+ * They are ordered after the work already enqueued on the stream the world
+ * ran on (the new stream waits for an event recorded on the old). */
 int wig_set_stream(WigWorld *w, void *stream);
 
 /* One liquid step, enqueued (Simulation.encodeFluid, js/sim.js:2466-2612). */
@@ -310,7 +312,8 @@ int wig_air_layout(const WigAir *a, WigAirLayout *out);
 int wig_air_upload(WigAir *a, int part, const void *src, size_t bytes);
 int wig_air_download(WigAir *a, int part, void *dst, size_t bytes);
 int wig_air_device_ptr(WigAir *a, int part, void **ptr);
-/* Runs later calls on this HIP stream (NULL: the grid's own). */
+/* Runs later calls on this HIP stream (NULL: the grid's own), ordered after
+ * the work already enqueued on the old one, as wig_set_stream. */
 int wig_air_set_stream(WigAir *a, void *stream);
 /* The side of a level's LDS tiles in the V-cycle's passes, 8, 16 or 32
  * (default 32 on level 0, 16 on level 1, 8 below): the results are the same bits

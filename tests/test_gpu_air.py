@@ -45,7 +45,7 @@ def next_step(g, o, W, H, seed, step):
     (1, 1, 0, 16, 1), (300, 7, 8, 32, 1), (130, 70, 9, 0, 1), (96, 80, 3, 8, 0), (200, 130, 5, 8, 0),
 ])
 def test_air_projection_matches_oracle(W, H, seed, tile, coarse):
-    # coarse: the levels that fit 64 x 64 in one kernel (1, the default) or
+    # coarse: the levels that fit 64 x 64 in one kernel (1, opt-in) or
     # a launch each (0)
     say(f"[air] {W}x{H} seed {seed} tile {tile} coarse {coarse}")
     mg0, x0 = scenes.air_rooms(W, H, seed=seed)

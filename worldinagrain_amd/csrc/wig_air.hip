@@ -416,15 +416,13 @@ __global__ void __launch_bounds__(256) k_airUp(AirDev a, uint32_t l, uint32_t cy
 
 // solveCoarsest (:1482-1535): symmetric SOR in LDS, a cell a thread (the
 // first 256 of the workgroup)
-__device__ void coarsestBody(const AirDev &a, float4 *A, float2 *B) {
-    (void)B;
+__device__ void coarsestBody(const AirDev &a, float *X) {
     const uint32_t l = a.L - 1u, dx = a.w[l], dy = a.h[l], n = min(dx * dy, (uint32_t)AIR_COARSEST);
     const uint32_t j = threadIdx.x;
-    // only the potentials change from sweep to sweep: they go in LDS (A's
-    // room), the cell's weights, rhs and diagonal stay in registers, each
-    // the value the reference's sweep recomputes (the same operations in
-    // the same order)
-    float *X = (float *)A;
+    // only the potentials change from sweep to sweep: they go in LDS (X,
+    // a float a cell), the cell's weights, rhs and diagonal stay in
+    // registers, each the value the reference's sweep recomputes (the same
+    // operations in the same order)
     const int32_t cx = (int32_t)(j % dx), cy = (int32_t)(j / dx);
     float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float lz = 0.0f, uw = 0.0f, diag = 1.0f;
@@ -462,9 +460,8 @@ __device__ void coarsestBody(const AirDev &a, float4 *A, float2 *B) {
 }
 __global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
-    __shared__ float4 A[AIR_COARSEST];
-    __shared__ float2 B[AIR_COARSEST];
-    coarsestBody(a, A, B);
+    __shared__ float X[AIR_COARSEST];
+    coarsestBody(a, X);
 }
 
 // The levels from l0 (the first that fits a T x T square) to the coarsest
@@ -474,19 +471,18 @@ __global__ void __launch_bounds__(256) k_airCoarsest(AirDev a, uint32_t cycle) {
 // (dynamic, COARSE_FUSED_BYTES): the tile, then the coarsest's cells.
 #define COARSE_FUSED_T 64
 #define COARSE_FUSED_NT 1024
-#define COARSE_FUSED_BYTES (sizeof(AirTile<COARSE_FUSED_T>) + AIR_COARSEST * (sizeof(float4) + sizeof(float2)))
+#define COARSE_FUSED_BYTES (sizeof(AirTile<COARSE_FUSED_T>) + AIR_COARSEST * sizeof(float))
 __global__ void __launch_bounds__(COARSE_FUSED_NT) k_airCoarse(AirDev a, uint32_t l0, uint32_t cycle) {
     if (airConverged(a, cycle)) return;
     extern __shared__ float4 smem[];
     AirTile<COARSE_FUSED_T> &t = *(AirTile<COARSE_FUSED_T> *)smem;
-    float4 *A = (float4 *)((char *)smem + sizeof(AirTile<COARSE_FUSED_T>));
-    float2 *B = (float2 *)(A + AIR_COARSEST);
+    float *X = (float *)((char *)smem + sizeof(AirTile<COARSE_FUSED_T>));
     const uint32_t deepest = a.L - 1u;
     for (uint32_t l = l0; l < deepest; l++) {
         downTile<COARSE_FUSED_T, COARSE_FUSED_NT>(a, t, l, 0, 0, 1);
         __syncthreads();
     }
-    coarsestBody(a, A, B);
+    coarsestBody(a, X);
     __syncthreads();
     for (int32_t l = (int32_t)deepest - 1; l >= (int32_t)l0; l--) {
         upTile<COARSE_FUSED_T, COARSE_FUSED_NT>(a, t, (uint32_t)l, 0, 0, l == 0 ? 1 : 0);
@@ -660,6 +656,7 @@ __global__ void __launch_bounds__(256) k_airFinish(AirDev a) {
 // ------------------------------------------------------------ the C ABI
 
 int wig_fail_message(int code, const char *msg);
+extern "C" int wig_order_streams(hipStream_t from, hipStream_t to);   // (wig_api.hip)
 
 struct WigAir {
     AirDev d;
@@ -849,7 +846,10 @@ extern "C" int wig_air_device_ptr(WigAir *a, int part, void **ptr) {
 
 extern "C" int wig_air_set_stream(WigAir *a, void *stream) {
     if (!a) return wig_fail_message(WIG_E_INVALID, "wig_air_set_stream: NULL");
-    a->stream = stream ? (hipStream_t)stream : a->own;
+    hipStream_t next = stream ? (hipStream_t)stream : a->own;
+    int st = wig_order_streams(a->stream, next);
+    if (st < 0) return st;
+    a->stream = next;
     return WIG_OK;
 }
 

@@ -571,9 +571,25 @@ int wig_device_ptr(WigWorld *w, int part, void **ptr) {
     return WIG_OK;
 }
 
+// Work enqueued on `to` from now on waits for what `from` holds so far (an
+// event recorded on `from`); nothing when they are the same stream.
+int wig_order_streams(hipStream_t from, hipStream_t to) {
+    if (from == to) return WIG_OK;
+    hipEvent_t ev = nullptr;
+    HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, from);
+    if (e == hipSuccess) e = hipStreamWaitEvent(to, ev, 0);
+    (void)hipEventDestroy(ev);   // (released once the wait is done)
+    if (e != hipSuccess) return fail(WIG_E_DEVICE, "ordering the new stream after the old: %s", hipGetErrorString(e));
+    return WIG_OK;
+}
+
 int wig_set_stream(WigWorld *w, void *stream) {
     if (!w) return fail(WIG_E_INVALID, "null world");
-    w->solo->stream = stream ? (hipStream_t)stream : w->own;
+    hipStream_t next = stream ? (hipStream_t)stream : w->own;
+    int st = wig_order_streams(w->solo->stream, next);
+    if (st < 0) return st;
+    w->solo->stream = next;
     return WIG_OK;
 }
 
@@ -850,8 +866,14 @@ static int batch_assign(Batch *b, WigWorld *const *worlds, uint32_t n) {
         uint32_t *stops = nullptr, *stops_host = nullptr, *reports = nullptr;
         if (hipMalloc((void **)&tab, sizeof(Dev) * n) != hipSuccess || hipMalloc((void **)&stops, 8ull * n) != hipSuccess
             || hipHostMalloc((void **)&stops_host, 8ull * n, hipHostMallocDefault) != hipSuccess
-            || hipMalloc((void **)&reports, 4ull * FLUID_STATS * n) != hipSuccess)
+            || hipMalloc((void **)&reports, 4ull * FLUID_STATS * n) != hipSuccess) {
+            // (what was made before the failure, freed; the batch keeps its own)
+            if (tab) (void)hipFree(tab);
+            if (stops) (void)hipFree(stops);
+            if (stops_host) (void)hipHostFree(stops_host);
+            if (reports) (void)hipFree(reports);
             return fail(WIG_E_NOMEM, "batch allocation failed");
+        }
         HIPCHECK(hipStreamSynchronize(b->stream));
         HIPCHECK(hipFree(b->tab));
         HIPCHECK(hipFree(b->stops));
@@ -1041,7 +1063,10 @@ int wig_batch_destroy(WigBatch *b) {
 int wig_batch_set_stream(WigBatch *b, void *stream) {
     if (!b) return fail(WIG_E_INVALID, "null batch");
     Batch *B = (Batch *)b;
-    B->stream = stream ? (hipStream_t)stream : B->own;
+    hipStream_t next = stream ? (hipStream_t)stream : B->own;
+    int st = wig_order_streams(B->stream, next);
+    if (st < 0) return st;
+    B->stream = next;
     return WIG_OK;
 }
 
