@@ -85,6 +85,17 @@ typedef struct WigStepCtl {
                                 at most WIG_MAX_CYCLES): after every k-th iteration the residual is
                                 recomputed as b - A x (FP64 accumulation) and replaces the recursive one,
                                 and the stop is judged only there, so a solve stops on its true residual */
+    uint32_t air;            /* 1: the air couples in, as the reference's bindings air[0] and airSource
+                                (fluid.wgsl:116-181): WIG_PART_AIR's pressure pushes surface particles
+                                (gather, :1027-1032), its wind drags the surface faces (setFaces,
+                                :1874-1932), and what the step gives the air goes into
+                                WIG_PART_AIR_SOURCE (the drag's pushes; the gas the liquid displaces,
+                                raster's pushGasOut, :3855-3861).  0: the air at rest at uniform
+                                pressure, nothing added to WIG_PART_AIR_SOURCE (displaced gas is let go) */
+    float surface_drag;      /* with air: the wind's drag coefficient along the surface,
+                                (KARMAN / ln(max(dx / 2 / 1e-4, 2)))^2 (fluid.wgsl:1884-1895); the host
+                                works it out from dx (worldinagrain_amd/host.py) so every backend takes
+                                the same float */
 } WigStepCtl;
 
 /* V-cycle storage: float in the reference's buffers (results bit-identical
@@ -109,7 +120,15 @@ typedef struct WigStepCtl {
 #define WIG_PART_BLOCKS 5      /* per block marks, state, lists, FLUID_STATS, totals */
 #define WIG_PART_FLUIDCG 6     /* vec4<f32> per cell (x, r, p, Ap), per block, sums */
 #define WIG_PART_SOLVE 7       /* this library's FP64 solve state and solver words (no counterpart in the reference) */
-#define WIG_PART_COUNT 8
+#define WIG_PART_AIR 8         /* air[0]: a vec4<f32> an air cell (AIR_CELL 1: a world cell): gauge pressure
+                                  (P_ATM 12 an atmosphere), the velocity on its right face and on its bottom
+                                  face (cells a step), temperature or AIR_BLOCKED (-1e4: no gas) or AIR_SOLID
+                                  (-2e4) (common.wgsl:280-288; air.wgsl:1742-1767); read */
+#define WIG_PART_AIR_SOURCE 9  /* airSource: 16 u32 an air cell, exact sums of two words (low, high) in
+                                  units of 2^-32: the velocity added to its right and bottom faces, then
+                                  gas amount, vapour, mist, oxygen lost, heat (airsource.wgsl:10-33);
+                                  added to by the step, taken in and cleared by the air's own passes */
+#define WIG_PART_COUNT 10
 
 /* FLUID_STATS words (js/shared.js:271-272). */
 #define WIG_STAT_ACTIVE 0

@@ -156,6 +156,24 @@ static inline void at_max(uint32_t *p, uint32_t v) { if (*p < v) *p = v; }
 #define KELVIN 273.15f
 #define VAPOUR_MAX 8.0f
 #define BOIL_K (100.0f + KELVIN)
+/* the air (js/shared.js:58,137-175,238,289; common.wgsl:287-288;
+ * js/elements.js:424,449-457,912-914: the packed table's f32 values) */
+#define P_ATM 12.0f
+#define AIR_BLOCKED (-10000.0f)
+#define AIR_SOLID (-20000.0f)
+#define WATER_RHO 1000.0f
+#define AIR_RHO ((float)(1000.0 * 0.001204))
+#define AIR_HEATCAP ((float)(1.2e-3 * (1000.0 / 8.0 / 0.598)))
+#define VAPOUR_HEAT_CAP 2.0f
+#define WATER_VAPOUR 64.0f
+#define SOURCE_WORDS 16u
+#define SOURCE_RIGHT 0u
+#define SOURCE_DOWN 1u
+#define SOURCE_AMOUNT 2u
+#define SOURCE_VAPOUR 3u
+#define SOURCE_MIST 4u
+#define SOURCE_LOST 5u
+#define SOURCE_HEAT 6u
 
 /* ----------------------------------------------------------- the world */
 typedef struct {
@@ -177,6 +195,8 @@ typedef struct OWorld {
     uint32_t *blocks;
     float *cg;
     uint8_t *solve;              /* SolveHead, then 4 doubles a block */
+    float *air;                  /* air[0]: a vec4 an air cell */
+    uint32_t *source;            /* airSource: SOURCE_WORDS an air cell */
     uint8_t kind[256];
     int dot_mode;
     /* the step being run */
@@ -382,6 +402,8 @@ static void layout(OWorld *w, uint32_t width, uint32_t height, uint32_t coarse) 
     w->bytes[WIG_PART_BLOCKS] = 4ull * (5ull * w->B + FLUID_STATS);
     w->bytes[WIG_PART_FLUIDCG] = 16ull * (w->C + w->B + 1u);
     w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * w->B + 64u;   /* (and the library's 16 solver words) */
+    w->bytes[WIG_PART_AIR] = 16ull * w->C;
+    w->bytes[WIG_PART_AIR_SOURCE] = 4ull * SOURCE_WORDS * w->C;
 }
 
 /* ------------------------------------------------------------ address */
@@ -537,6 +559,93 @@ static inline int walls(const OWorld *w, i2 c) {            /* :1056-1062 */
     if (!inWorld(w, c)) return 1;
     uint32_t k = kindOf(w, wType(w->cells[4ull * cellIndex(w, c)]));
     return k == KIND_SOLID || k == KIND_POWDER;
+}
+
+/* ------------------------------------------------------------- the air
+ * air[0] and airSource, as the liquid reads and adds to them (AIR_CELL 1:
+ * an air cell is a world cell, airDims the world's size). */
+static inline const float *airAt(const OWorld *w, i2 i) { return &w->air[4ull * ((uint64_t)i.y * (uint32_t)w->W + (uint32_t)i.x)]; }
+static inline int inAir(const OWorld *w, i2 i) { return i.x >= 0 && i.y >= 0 && i.x < w->W && i.y < w->H; }
+static v2 fluidPressureAt(const OWorld *w, i2 i) {          /* airsample.wgsl:83-89 */
+    if (!inAir(w, i)) return V2(0.0f, 0.0f);
+    const float *a = airAt(w, i);
+    return V2(a[0], a[3] <= AIR_SOLID ? 0.0f : 1.0f);
+}
+static float pressureSlope(const OWorld *w, i2 c, i2 d) {   /* airsample.wgsl:97-111 */
+    float own = airAt(w, c)[0];
+    v2 lo = fluidPressureAt(w, I2(c.x - d.x, c.y - d.y)), hi = fluidPressureAt(w, I2(c.x + d.x, c.y + d.y));
+    if (lo.y > 0.0f && hi.y > 0.0f) return 0.5f * (hi.x - lo.x);
+    if (hi.y > 0.0f) return hi.x - own;
+    if (lo.y > 0.0f) return own - lo.x;
+    return 0.0f;
+}
+static v2 airGrad(const OWorld *w, v2 pos) {                /* sampleAir's grad, airsample.wgsl:171-180 */
+    v2 q = V2(pos.x / 1.0f, pos.y / 1.0f);
+    i2 c = I2(wi32(floorf(q.x)), wi32(floorf(q.y)));
+    c.x = c.x < 0 ? 0 : (c.x > w->W - 1 ? w->W - 1 : c.x);
+    c.y = c.y < 0 ? 0 : (c.y > w->H - 1 ? w->H - 1 : c.y);
+    return V2(pressureSlope(w, c, I2(1, 0)) / 1.0f, pressureSlope(w, c, I2(0, 1)) / 1.0f);
+}
+static v2 pressurePush(const OWorld *w, v2 grad, float rho) {   /* airsample.wgsl:11-14 */
+    float s = w->P->dt / w->P->dx, m = fmaxf(rho, 1e-3f);
+    if (MUTANT(25)) return V2(grad.x * (101325.0f / P_ATM) * s * s / m, grad.y * (101325.0f / P_ATM) * s * s / m);
+    return V2(-grad.x * (101325.0f / P_ATM) * s * s / m, -grad.y * (101325.0f / P_ATM) * s * s / m);
+}
+static inline int gasCellAt(const OWorld *w, i2 i) { return inAir(w, i) && airAt(w, i)[3] > AIR_BLOCKED; }   /* airsample.wgsl:130-132 */
+static inline float faceU(const OWorld *w, i2 i) { return inAir(w, i) ? airAt(w, i)[1] : 0.0f; }           /* :115-127 */
+static inline float faceV(const OWorld *w, i2 i) { return inAir(w, i) ? airAt(w, i)[2] : 0.0f; }
+static void addSource(OWorld *w, uint32_t i, uint32_t k, float x) {   /* airsource.wgsl:27-35 */
+    if (x == 0.0f) return;
+    uint64_t at = SOURCE_WORDS * (uint64_t)i + 2u * k;
+    u2 p = exactParts(x, EXACT_FINE);
+    uint32_t old = at_take(&w->source[at], p.lo);
+    at_add(&w->source[at + 1u], p.hi + (old + p.lo < old ? 1u : 0u));
+}
+static void pushFace(OWorld *w, i2 i, v2 dv) {               /* airsource.wgsl:37-41,59-63 */
+    if (!inAir(w, i)) return;
+    uint32_t j = (uint32_t)i.y * (uint32_t)w->W + (uint32_t)i.x;
+    addSource(w, j, SOURCE_RIGHT, dv.x);
+    addSource(w, j, SOURCE_DOWN, dv.y);
+}
+static void pushGasOut(OWorld *w, uint32_t i, float temp) {  /* fluid.wgsl:3855-3861; airsource.wgsl:65-72 */
+    uint64_t at = gasWord(w, i);
+    v2 m = unpack2x16(w->cells[at]), r = unpack2x16(w->cells[at + 1]);
+    float vapour = m.x, mist = m.y, lost = r.x, amount = r.y;
+    addSource(w, i, SOURCE_AMOUNT, amount);
+    if (!MUTANT(27)) addSource(w, i, SOURCE_VAPOUR, vapour);
+    addSource(w, i, SOURCE_MIST, mist);
+    addSource(w, i, SOURCE_LOST, lost * amount);
+    /* gasHeatCap (common.wgsl:473-475) times the temperature */
+    addSource(w, i, SOURCE_HEAT, (AIR_HEATCAP * amount + (VAPOUR_HEAT_CAP * vapour + WATER_HEATCAP * mist) / WATER_VAPOUR) * temp);
+    setGasPair(w, i, 0u, 0u);
+}
+static v2 pairDrag(float u, float wv, float ml, float ma, float c) {   /* fluid.wgsl:1869-1878 */
+    if (ml <= 0.0f || ma <= 0.0f || c <= 0.0f) return V2(0.0f, 0.0f);
+    float mu = ml / ma;
+    float k = c * fabsf(u - wv);
+    float change = (u - wv) * (1.0f - 1.0f / (1.0f + k * (1.0f + 1.0f / mu)));
+    if (MUTANT(26)) return V2(-change / (1.0f + mu), change / (1.0f + mu));
+    return V2(-change / (1.0f + mu), change * mu / (1.0f + mu));
+}
+static float airMassAt(const OWorld *w, i2 c) {             /* fluid.wgsl:1862-1865 */
+    return AIR_RHO / 1000.0f * gasAmountAt(w, cellIndex(w, c));
+}
+static float surfaceDrag(OWorld *w, i2 a, i2 b, int isU, const float ga[4], const float gb[4], float u) {   /* :1914-1932 */
+    i2 across = isU ? I2(0, 1) : I2(1, 0);
+    float du = 0.0f;
+    if (ga[2] > 0.0f && gb[2] > 0.0f) {
+        float k = w->ctl->surface_drag;
+        for (int s = -1; s <= 1; s += 2) {
+            i2 oa = I2(a.x + across.x * s, a.y + across.y * s), ob = I2(b.x + across.x * s, b.y + across.y * s);
+            if (gasCellAt(w, oa) && gasCellAt(w, ob)) {
+                float wv = isU ? faceU(w, oa) : faceV(w, oa);
+                v2 drag = pairDrag(u + du, wv, 0.5f * (ga[2] + gb[2]), 0.5f * (airMassAt(w, oa) + airMassAt(w, ob)), k);
+                du += drag.x;
+                pushFace(w, oa, isU ? V2(drag.y, 0.0f) : V2(0.0f, drag.y));
+            }
+        }
+    }
+    return du;
 }
 
 /* union-find (label.wgsl:25-55) over G_REGION words */
@@ -898,6 +1007,23 @@ static void k_gather(OWorld *w) {                            /* :1017-1039 */
     PAR_FOR
     for (uint32_t k = 0; k < n; k++) {
         Particle q = loadP(w, w->parts, k);
+        /* the air's pressure on a particle beside gas (:1024-1032): its
+         * velocity pushed, and kept */
+        if (w->ctl->air) {
+            i2 c = I2(wi32(floorf(q.pos.x)), wi32(floorf(q.pos.y)));
+            int gas = 0;
+            for (int d = 0; d < 4; d++) {
+                i2 n = beside(c, d);
+                gas |= inWorld(w, n) && kindOf(w, wType(w->cells[4ull * cellIndex(w, n)])) == KIND_GAS;
+            }
+            if (gas) {
+                v2 push = pressurePush(w, airGrad(w, q.pos), WATER_RHO);
+                q.vel.x += push.x;
+                q.vel.y += push.y;
+                MOTION(w->parts, k, 2) = ubits(q.vel.x);
+                MOTION(w->parts, k, 3) = ubits(q.vel.y);
+            }
+        }
         splatAxis(w, V2(q.pos.x - 0.0f, q.pos.y - 0.5f), q.vel.x, q.affine[0], q.affine[1], 1);
         splatAxis(w, V2(q.pos.x - 0.5f, q.pos.y - 0.0f), q.vel.y, q.affine[2], q.affine[3], 0);
         uint32_t home = cellIndex(w, cellOf(w, q.pos));
@@ -1053,7 +1179,7 @@ static void setFace(OWorld *w, i2 at, int isU) {           /* :1936-1980 */
         held = 1.0f;
     }
     float now = u;
-    if (held > 0.0f) now += 0.0f;                             /* the wind's drag: still air */
+    if (held > 0.0f && w->ctl->air) now += surfaceDrag(w, a, at, isU, ga, gb, u);   /* the wind's drag */
     if (held > 0.0f && ga[2] >= 0.0f && gb[2] >= 0.0f) now += (isU != MUTANT(21)) ? 0.0f : w->P->earthG;
     float start = now;
     FLD(faceSlot(w, f), 0) = start;
@@ -2068,7 +2194,12 @@ static void k_rasterClaims(OWorld *w) {                      /* :3928-3948 */
 static int moveOutOfTheWay(OWorld *w, uint32_t i, const uint32_t v[4]) {   /* :3954-3973 */
     uint32_t to = CNT(i, G_AIR_TO);
     if (to != NONE && CNT(to, G_TAKEN) == ~i) {
-        if (wType(w->cells[4ull * to]) == E_AIR) setGasPair(w, to, 0u, 0u);   /* pushGasOut */
+        /* pushGasOut at the air's temperature as stored (AIR has no phase
+         * points: js/elements.js:424-426), or, without the air, let go */
+        if (wType(w->cells[4ull * to]) == E_AIR) {
+            if (w->ctl->air) pushGasOut(w, to, fbits(w->cells[4ull * to + 1]));
+            else setGasPair(w, to, 0u, 0u);
+        }
         for (int j = 0; j < 4; j++) w->cells[4ull * to + j] = v[j];
         uint64_t gi = gasWord(w, i);
         setGasPair(w, to, w->cells[gi], w->cells[gi + 1]);
@@ -2076,7 +2207,8 @@ static int moveOutOfTheWay(OWorld *w, uint32_t i, const uint32_t v[4]) {   /* :3
         return 1;
     }
     if (wType(v[0]) == E_AIR) {
-        setGasPair(w, i, 0u, 0u);
+        if (w->ctl->air) pushGasOut(w, i, fbits(v[1]));
+        else setGasPair(w, i, 0u, 0u);
         return 1;
     }
     return 0;
@@ -2294,20 +2426,26 @@ OWorld *wor_create(uint32_t width, uint32_t height, uint32_t coarse_levels) {
     w->cg = (float *)calloc(1, w->bytes[WIG_PART_FLUIDCG]);
     w->solve = (uint8_t *)calloc(1, w->bytes[WIG_PART_SOLVE]);
     w->bvec = (float *)calloc(w->C, sizeof(float));
+    w->air = (float *)calloc(1, w->bytes[WIG_PART_AIR]);
+    w->source = (uint32_t *)calloc(1, w->bytes[WIG_PART_AIR_SOURCE]);
     w->dot_mode = 1;
-    if (!w->cells || !w->sorted || !w->parts || !w->counters || !w->field || !w->blocks || !w->cg || !w->solve || !w->bvec) return NULL;
+    if (!w->cells || !w->sorted || !w->parts || !w->counters || !w->field || !w->blocks || !w->cg || !w->solve || !w->bvec
+        || !w->air || !w->source)
+        return NULL;
     return w;
 }
 void wor_destroy(OWorld *w) {
     if (!w) return;
     free(w->cells); free(w->sorted); free(w->parts); free(w->counters);
     free(w->field); free(w->blocks); free(w->cg); free(w->solve); free(w->cap_b); free(w->bvec);
+    free(w->air); free(w->source);
     free(w);
 }
 void *wor_part(OWorld *w, int part, uint64_t *bytes) {
     if (part < 0 || part >= WIG_PART_COUNT) return NULL;
     if (bytes) *bytes = w->bytes[part];
-    void *p[WIG_PART_COUNT] = {w->cells, w->sorted, w->parts, w->counters, w->field, w->blocks, w->cg, w->solve};
+    void *p[WIG_PART_COUNT] = {w->cells, w->sorted, w->parts, w->counters, w->field, w->blocks, w->cg, w->solve,
+                               w->air, w->source};
     return p[part];
 }
 void wor_set_kinds(OWorld *w, const uint8_t kinds[256]) { memcpy(w->kind, kinds, 256); }

@@ -107,13 +107,29 @@ class Hand:
         p.size[0], p.size[1] = self.W, self.H
         p.frame = self.frame
         p.seed = 0x1234
-        p.dx = 1.0
+        p.dx = getattr(self, "dx", 1.0)
         p.earthG = self.earth_g
-        p.dt = 1.0
+        p.dt = getattr(self, "dt", 1.0)
         return p
 
-    def ctl(self, coarse_sweeps=40):
+    def air(self):
+        """(H, W, 4): air[0], pressure, right and bottom faces' velocity, temperature or AIR_BLOCKED/AIR_SOLID."""
+        return self.w.part(abi.PART_AIR).reshape(self.H, self.W, 4)
+
+    def source(self):
+        """(H, W, 8): airSource's exact sums a cell, as float64 (units of 2^-32 summed)."""
+        s = self.w.part(abi.PART_AIR_SOURCE).reshape(self.H, self.W, abi.SOURCE_WORDS).astype(np.uint64)
+        v = (s[..., 1::2] << np.uint64(32)) | s[..., 0::2]
+        return v.view(np.int64).astype(np.float64) / 2.0 ** 32
+
+    def gas(self):
+        """(H, W, 2): the gas plane's two words a cell (vapour, mist | lost, amount as f16 pairs)."""
+        return self.cellwords[4 * self.C:4 * self.C + 2 * self.C].reshape(self.H, self.W, 2)
+
+    def ctl(self, coarse_sweeps=40, air=0, surface_drag=0.0):
         c = abi.WigStepCtl()
+        c.air = air
+        c.surface_drag = surface_drag
         c.full = 1
         c.fluid_cycles = 1
         c.tol_mode = abi.WIG_TOL_ABS_MAX
@@ -875,6 +891,134 @@ def pin_escape(L):
     assert p[0, 0] == np.float32(20.5) and p[0, 1] == np.float32(19.0) + np.float32(0.98), p
 
 
+# ------------------------------------------------------------ the air
+def _f16pair(a, b):
+    return int(np.array([a, b], np.float16).view(np.uint32)[0])
+
+
+def pin_air_push(L):
+    """gather's push of the air's pressure on a particle beside gas
+    (fluid.wgsl:1024-1032; airsample.wgsl:11-14,83-111,171-180): with the
+    pressure p = g x in the gas, a surface particle's velocity gains
+    -g 101325/P_ATM (dt/dx)^2 / 1000 along x (and keeps it); with a solid
+    air cell on one side the slope is one-sided (hi - own), and with solids
+    both sides none; a particle with no gas beside it is not pushed."""
+    h = Hand(32, 32, L=L)
+    h.activate()
+    h.dt, h.dx = 1.0 / 60.0, 0.01
+    words, air = h.words(), h.air()
+    words[..., 0] = word(E.AIR)
+    words[10:20, 5:20, 0] = word(E.WATER)
+    gslope = np.float32(0.75)
+    xs = np.arange(32, dtype=np.float32)
+    air[..., 0] = gslope * xs[None, :]
+    air[..., 3] = 20.0
+    air[10:20, 5:20, 3] = abi.AIR_BLOCKED
+    air[12, 21, 3] = abi.AIR_SOLID          # right of the particle at (20, 12): one-sided
+    air[15, 19:22, 3] = abi.AIR_SOLID       # (20, 15): solids both sides
+    cnt, motion, state, affine = h.particles(abi.PART_PARTICLES1)
+    pos = [(19.5, 10.5), (20.5, 12.5), (20.5, 15.5), (12.5, 15.5)]
+    n = len(pos)
+    cnt[0] = n
+    motion[:n, :2] = pos
+    motion[:n, 2:] = (0.25, -0.125)
+    state[:n, 0] = word(E.WATER)
+    state[:n, 1] = np.float32(20.0).view(np.uint32)
+    state[:n, 2] = np.arange(n) + 1
+    affine[:n] = 0.0
+    h.run(abi_pass_gather(), air=1)
+    s = np.float32(np.float32(h.dt) / np.float32(h.dx))
+    k = float(np.float32(101325.0 / 12.0) * s * s / np.float32(1000.0))
+    dv = motion[:n, 2:].astype(np.float64) - np.array([0.25, -0.125])
+    # (19, 10): water beside air; slope 0.75 both sides
+    assert abs(dv[0, 0] + 0.75 * k) <= 1e-6 * 0.75 * k and abs(dv[0, 1]) <= 1e-9, dv[0]
+    # (20, 12): air beside it, a solid to its right: own - left = 0.75
+    assert abs(dv[1, 0] + 0.75 * k) <= 1e-6 * 0.75 * k, dv[1]
+    # (20, 15): solids both sides of its air cell: no x slope
+    assert dv[2, 0] == 0.0, dv[2]
+    # (12, 15): deep in the water, no gas beside: not pushed
+    assert dv[3, 0] == 0.0 and dv[3, 1] == 0.0, dv[3]
+
+
+def abi_pass_gather():
+    from oracle.oracle import PASS_GATHER
+    return PASS_GATHER
+
+
+def pin_surface_drag(L):
+    """setFaces' wind drag along the surface (fluid.wgsl:1874-1932): the
+    face between two water cells under gas gives up what the gas's face
+    takes, mass for mass (the liquid's mass times its change plus the air's
+    times its own is none; the air's into airSource's right-face sum), and
+    neither passes the other; past a strong coefficient they move together
+    at their mass-weighted mean; with no gas beside, no drag."""
+    for c, strong in ((0.011, False), (1e6, True)):
+        h = Hand(16, 16, L=L)
+        h.activate()
+        words, air, cell = h.words(), h.air(), h.cell()
+        words[..., 0] = word(E.AIR)
+        words[8:12, 3:12, 0] = word(E.WATER)
+        cell[8:12, 3:12, 2] = 1.0
+        air[..., 3] = 20.0
+        air[..., 1] = 0.4                       # the wind on every right face
+        air[8:12, 3:12, 3] = abi.AIR_BLOCKED
+        gas = h.gas()
+        gas[..., 1] = _f16pair(0.0, 1.0)        # a cell of air: amount 1
+        W = 16
+        sums = h.counters[11 * h.C:11 * h.C + 2 * h.F].reshape(h.F, 2).view(np.int32)
+        sums[8 * (W + 1) + 6] = (int(0.1 * 65536), 65536)     # u face (6, 8), water both sides, gas above
+        sums[10 * (W + 1) + 6] = (int(0.1 * 65536), 65536)    # u face (6, 10): water above and below
+        h.run(PASS_SET_FACES, air=1, surface_drag=c)
+        U = h.ufaces()
+        u0 = int(0.1 * 65536) / 65536.0                       # the face's mean from its sums
+        du = float(U[8, 6, 0]) - u0
+        dw = float(h.source()[7, 5, abi.SOURCE_RIGHT])        # the gas face above (5..6, 8)
+        ml, ma = 1.0, float(np.float32(np.float32(1000 * 0.001204) / np.float32(1000.0)))
+        assert du > 0.0 > dw, (du, dw)                        # the faster wind drags the water on
+        assert abs(ml * du + ma * dw) <= float(np.spacing(np.float32(u0))), (du, dw)   # the f32 face rounds du
+        assert u0 + du <= 0.4 + dw + 1e-6
+        if strong:
+            mean = (ml * u0 + ma * 0.4) / (ml + ma)
+            assert abs(u0 + du - mean) < 1e-6 and abs(0.4 + dw - mean) < 1e-3, (u0 + du, 0.4 + dw, mean)
+        else:
+            assert du < 0.5 * ma * (0.4 - u0), du             # a weak coefficient moves it part way
+        assert U[10, 6, 0] == np.float32(u0)                  # no gas beside: no drag
+        assert h.source()[9, 5, abi.SOURCE_RIGHT] == 0.0
+
+
+def pin_push_gas_out(L):
+    """raster's displaced gas (fluid.wgsl:3855-3861,3954-3973; airsource.wgsl:
+    27-72): a cell of air the liquid is drawn into gives its gas, whole, to
+    its airSource slot -- amount, vapour, mist, the oxygen lost times the
+    amount, and its heat, gasHeatCap times its temperature, as exact sums --
+    and keeps none."""
+    h = Hand(16, 16, L=L)
+    h.activate()
+    words, cnt = h.words(), h.cnt()
+    words[..., 0] = word(E.AIR)
+    words[..., 1] = np.float32(30.0).view(np.uint32)
+    gas = h.gas()
+    gas[6, 6] = (_f16pair(0.5, 0.25), _f16pair(0.125, 1.0))   # vapour, mist | lost, amount
+    cntv, motion, state, _ = h.particles(abi.PART_PARTICLES0)
+    cnt[6, 6, G_SORT], cnt[6, 6, G_START] = 2, 0
+    cnt[..., 8] = 0xFFFFFFFF                                   # G_AIR_TO: no claim
+    for k in range(2):
+        motion[k] = (6.5, 6.5, 0.0, 0.0)
+        state[k] = (word(E.WATER), np.float32(20.0).view(np.uint32), 10 + k, 0)
+    cntv[0] = 2
+    h.run(PASS_RASTER, air=1)
+    src = h.source()[6, 6]
+    heat = float(np.float32(np.float32(np.float32(1.2e-3 * (1000.0 / 8.0 / 0.598)) * np.float32(1.0)
+                                       + (np.float32(2.0) * np.float32(0.5) + np.float32(4.18) * np.float32(0.25)) / np.float32(64.0))
+                            * np.float32(30.0)))
+    want = {abi.SOURCE_AMOUNT: 1.0, abi.SOURCE_VAPOUR: 0.5, abi.SOURCE_MIST: 0.25, abi.SOURCE_LOST: 0.125}
+    for k, v in want.items():
+        assert src[k] == v, (k, src[k], v)
+    assert abs(src[abi.SOURCE_HEAT] - heat) <= 2.0 ** -32, (src[abi.SOURCE_HEAT], heat)
+    assert tuple(gas[6, 6]) == (0, 0)
+    assert words[6, 6, 0] & 0xFF == E.WATER
+
+
 PINS = {
     "restrict": pin_restrict,
     "smooth": pin_smooth,
@@ -895,4 +1039,7 @@ PINS = {
     "divergence": pin_divergence,
     "unpaint": pin_unpaint,
     "escape": pin_escape,
+    "air_push": pin_air_push,
+    "surface_drag": pin_surface_drag,
+    "push_gas_out": pin_push_gas_out,
 }

@@ -38,6 +38,9 @@ CAUGHT_BY = {
     22: "escape",                  # a buried particle stays buried
     23: "unpaint",                 # a cell taken by displaced air unpainted anyway
     24: "divergence",              # the drift rhs's sign
+    25: "air_push",                # the air's pressure pushes up its gradient
+    26: "surface_drag",            # the drag's push on the air without the mass ratio
+    27: "push_gas_out",            # displaced gas's vapour lost
 }
 
 

@@ -14,8 +14,15 @@ WIG_STORE_F32, WIG_STORE_F16, WIG_STORE_BF16 = 0, 1, 2
 WIG_MAX_CYCLES = 256
 
 PART_CELLS, PART_PARTICLES0, PART_PARTICLES1, PART_COUNTERS, PART_FIELD, PART_BLOCKS, PART_FLUIDCG, PART_SOLVE = range(8)
-PART_COUNT = 8
-PART_NAMES = ("cells", "particles0", "particles1", "counters", "field", "blocks", "fluidCg", "solve")
+PART_AIR, PART_AIR_SOURCE = 8, 9
+PART_COUNT = 10
+LIQUID_PARTS = 8          # the liquid's own parts (a snapshot's); the air's two are the caller's air step's
+PART_NAMES = ("cells", "particles0", "particles1", "counters", "field", "blocks", "fluidCg", "solve", "air0",
+              "airSource")
+# airSource's words an air cell (airsource.wgsl:18-25), each an exact sum of two words
+SOURCE_WORDS = 16
+SOURCE_RIGHT, SOURCE_DOWN, SOURCE_AMOUNT, SOURCE_VAPOUR, SOURCE_MIST, SOURCE_LOST, SOURCE_HEAT = range(7)
+AIR_BLOCKED, AIR_SOLID = -10000.0, -20000.0
 
 STAT_ACTIVE, STAT_RETIRED, STAT_VISCOUS, STAT_PARTICLES, STAT_RESIDUAL = 0, 1, 2, 3, 4
 STAT_CYCLES, STAT_MOST_CYCLES = 7, 8
@@ -40,6 +47,7 @@ class WigStepCtl(ct.Structure):
         ("pre_relax", ct.c_uint32), ("post_relax", ct.c_uint32), ("sweep_pairs", ct.c_uint32),
         ("coarse_sweeps", ct.c_uint32), ("first_chunk", ct.c_uint32), ("chunk", ct.c_uint32),
         ("level_pairs", ct.c_uint32), ("vcycle_storage", ct.c_uint32), ("residual_every", ct.c_uint32),
+        ("air", ct.c_uint32), ("surface_drag", ct.c_float),
     ]
 
 
@@ -47,7 +55,7 @@ class WigLayoutInfo(ct.Structure):
     _fields_ = [
         ("width", ct.c_uint32), ("height", ct.c_uint32), ("cells", ct.c_uint32), ("faces", ct.c_uint32),
         ("blocks", ct.c_uint32), ("blocks_wide", ct.c_uint32), ("capacity", ct.c_uint32), ("levels", ct.c_uint32),
-        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 8),
+        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 10),
     ]
 
 

@@ -273,6 +273,8 @@ static void size_world(WigWorld *w, uint32_t width, uint32_t height, uint32_t co
     w->bytes[WIG_PART_FLUIDCG] = 16ull * (d.C + d.B + 1u);
     // (then the world's WS_* words, so a snapshot of the part carries them)
     w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * d.B + 4ull * WS_WORDS;
+    w->bytes[WIG_PART_AIR] = 16ull * d.C;
+    w->bytes[WIG_PART_AIR_SOURCE] = 4ull * SOURCE_WORDS * d.C;
 }
 
 
@@ -453,6 +455,8 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     d.cg = (float *)w->part[WIG_PART_FLUIDCG];
     d.head = (SolveHead *)w->part[WIG_PART_SOLVE];
     d.bpart = (double *)((uint8_t *)w->part[WIG_PART_SOLVE] + sizeof(SolveHead));
+    d.air = (const float4 *)w->part[WIG_PART_AIR];
+    d.source = (uint32_t *)w->part[WIG_PART_AIR_SOURCE];
     d.kind = w->kind;
     d.scratch = w->scratch;
     {
@@ -721,6 +725,10 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d.level_pairs = ctl->level_pairs ? ctl->level_pairs : 1u;
     d.store = ctl->vcycle_storage;
     d.replace_every = ctl->residual_every;
+    d.air_on = ctl->air ? 1u : 0u;
+    d.surface_drag = ctl->surface_drag;
+    d.dt = P->dt;
+    d.dx = P->dx;
     for (uint32_t l = 0; l <= MG_LEVELS; l++) d.tile[l] = fitTile(w->tile[l], d.level_pairs);
     w->d.level_pairs = d.level_pairs;
     for (uint32_t l = 0; l <= MG_LEVELS; l++) w->d.tile[l] = d.tile[l];

@@ -93,6 +93,24 @@
 #define VAPOUR_MAX 8.0f
 #define BOIL_K (100.0f + KELVIN)
 #define ATTAINABLE2 8.8817841970012523e-16   // (2^-25)^2: residual replacement's stop at twice an f32 iterate's rounding floor
+// the air (js/shared.js:58,137-175,238,289; common.wgsl:287-288;
+// js/elements.js:424,449-457,912-914: the packed table's f32 values)
+#define P_ATM 12.0f
+#define AIR_BLOCKED (-10000.0f)
+#define AIR_SOLID (-20000.0f)
+#define WATER_RHO 1000.0f
+#define AIR_RHO ((float)(1000.0 * 0.001204))
+#define AIR_HEATCAP ((float)(1.2e-3 * (1000.0 / 8.0 / 0.598)))
+#define VAPOUR_HEAT_CAP 2.0f
+#define WATER_VAPOUR 64.0f
+#define SOURCE_WORDS 16u
+#define SOURCE_RIGHT 0u
+#define SOURCE_DOWN 1u
+#define SOURCE_AMOUNT 2u
+#define SOURCE_VAPOUR 3u
+#define SOURCE_MIST 4u
+#define SOURCE_LOST 5u
+#define SOURCE_HEAT 6u
 
 // The FP64 state of a solve: its header, then three doubles per block
 // (two running sums of a pass and the squared residual).
@@ -179,6 +197,12 @@ struct Dev {
     // solve's b (packed), which it reads
     uint32_t replace_every;
     float *cb;
+    // the air (WigStepCtl.air): air[0] and airSource, the wind's drag
+    // coefficient, and the step's dt and dx (the pressure's push)
+    const float4 *air;
+    uint32_t *source;
+    uint32_t air_on;
+    float surface_drag, dt, dx;
 };
 
 // A launch over a batch of worlds.  Its workgroups are split among the

@@ -76,11 +76,81 @@ __device__ __forceinline__ uint32_t fallingBit(const Dev &d, uint32_t back) { re
 __device__ __forceinline__ bool fallingAt(const Dev &d, i2 q) {
     return inWorld(d, q) && (wu32(FLD(d, 2u * cellIndex(d, q) + 1u, 3)) & fallingBit(d, 1u)) != 0u;
 }
+// ------------------------------------------------------------ the air
+// air[0] and airSource as the liquid reads and adds to them (AIR_CELL 1: an
+// air cell is a world cell)
+__device__ __forceinline__ bool inAir(const Dev &d, i2 i) { return i.x >= 0 && i.y >= 0 && i.x < d.W && i.y < d.H; }
+__device__ __forceinline__ float4 airAt(const Dev &d, i2 i) { return d.air[(uint64_t)i.y * (uint32_t)d.W + (uint32_t)i.x]; }
+__device__ __forceinline__ v2 fluidPressureAt(const Dev &d, i2 i) {            // airsample.wgsl:83-89
+    if (!inAir(d, i)) return V2(0.0f, 0.0f);
+    const float4 a = airAt(d, i);
+    return V2(a.x, a.w <= AIR_SOLID ? 0.0f : 1.0f);
+}
+__device__ float pressureSlope(const Dev &d, i2 c, i2 s) {                     // airsample.wgsl:97-111
+    const float own = airAt(d, c).x;
+    const v2 lo = fluidPressureAt(d, I2(c.x - s.x, c.y - s.y)), hi = fluidPressureAt(d, I2(c.x + s.x, c.y + s.y));
+    if (lo.y > 0.0f && hi.y > 0.0f) return 0.5f * (hi.x - lo.x);
+    if (hi.y > 0.0f) return hi.x - own;
+    if (lo.y > 0.0f) return own - lo.x;
+    return 0.0f;
+}
+__device__ v2 airGrad(const Dev &d, v2 pos) {                                  // sampleAir's grad, :171-180
+    const v2 q = V2(pos.x / 1.0f, pos.y / 1.0f);
+    int32_t cx = wi32(floorf(q.x)), cy = wi32(floorf(q.y));
+    cx = cx < 0 ? 0 : (cx > d.W - 1 ? d.W - 1 : cx);
+    cy = cy < 0 ? 0 : (cy > d.H - 1 ? d.H - 1 : cy);
+    return V2(pressureSlope(d, I2(cx, cy), I2(1, 0)) / 1.0f, pressureSlope(d, I2(cx, cy), I2(0, 1)) / 1.0f);
+}
+__device__ __forceinline__ v2 pressurePush(const Dev &d, v2 grad, float rho) {  // airsample.wgsl:11-14
+    const float s = d.dt / d.dx, m = fmaxf(rho, 1e-3f);
+    return V2(-grad.x * (101325.0f / P_ATM) * s * s / m, -grad.y * (101325.0f / P_ATM) * s * s / m);
+}
+__device__ __forceinline__ bool gasCellAt(const Dev &d, i2 i) { return inAir(d, i) && airAt(d, i).w > AIR_BLOCKED; }
+__device__ __forceinline__ float faceU(const Dev &d, i2 i) { return inAir(d, i) ? airAt(d, i).y : 0.0f; }
+__device__ __forceinline__ float faceV(const Dev &d, i2 i) { return inAir(d, i) ? airAt(d, i).z : 0.0f; }
+__device__ void addSource(const Dev &d, uint32_t i, uint32_t k, float x) {     // airsource.wgsl:27-35
+    if (x == 0.0f) return;
+    const uint64_t at = SOURCE_WORDS * (uint64_t)i + 2u * k;
+    const u2 p = exactParts(x, EXACT_FINE);
+    const uint32_t old = atomicAdd(&d.source[at], p.lo);
+    atomicAdd(&d.source[at + 1u], p.hi + (old + p.lo < old ? 1u : 0u));
+}
+__device__ void pushFace(const Dev &d, i2 i, v2 dv) {                          // airsource.wgsl:37-41,59-63
+    if (!inAir(d, i)) return;
+    const uint32_t j = (uint32_t)i.y * (uint32_t)d.W + (uint32_t)i.x;
+    addSource(d, j, SOURCE_RIGHT, dv.x);
+    addSource(d, j, SOURCE_DOWN, dv.y);
+}
+__device__ __forceinline__ v2 pairDrag(float u, float wv, float ml, float ma, float c) {   // fluid.wgsl:1869-1878
+    if (ml <= 0.0f || ma <= 0.0f || c <= 0.0f) return V2(0.0f, 0.0f);
+    const float mu = ml / ma;
+    const float k = c * fabsf(u - wv);
+    const float change = (u - wv) * (1.0f - 1.0f / (1.0f + k * (1.0f + 1.0f / mu)));
+    return V2(-change / (1.0f + mu), change * mu / (1.0f + mu));
+}
 __device__ __forceinline__ float gasAmountAt(const Dev &d, uint32_t i) {
     uint64_t at = gasWord(d, i);
     v2 m = unpack2x16(d.cells[at]);
     v2 r = unpack2x16(d.cells[at + 1]);
     return r.y + m.x * (AMBIENT_T + KELVIN) / (BOIL_K * VAPOUR_MAX);
+}
+__device__ float airMassAt(const Dev &d, i2 c) { return AIR_RHO / 1000.0f * gasAmountAt(d, cellIndex(d, c)); }   // :1862-1865
+// the wind's drag along the surface at face (a | b) (surfaceDrag, :1914-1932)
+__device__ float surfaceDrag(const Dev &d, i2 a, i2 b, bool isU, float4 ga, float4 gb, float u) {
+    const i2 across = isU ? I2(0, 1) : I2(1, 0);
+    float du = 0.0f;
+    if (ga.z > 0.0f && gb.z > 0.0f) {
+        for (int s = -1; s <= 1; s += 2) {
+            const i2 oa = I2(a.x + across.x * s, a.y + across.y * s), ob = I2(b.x + across.x * s, b.y + across.y * s);
+            if (gasCellAt(d, oa) && gasCellAt(d, ob)) {
+                const float wv = isU ? faceU(d, oa) : faceV(d, oa);
+                const v2 drag = pairDrag(u + du, wv, 0.5f * (ga.z + gb.z), 0.5f * (airMassAt(d, oa) + airMassAt(d, ob)), d.surface_drag);
+                du += drag.x;
+                pushFace(d, oa, isU ? V2(drag.y, 0.0f) : V2(0.0f, drag.y));
+            }
+        }
+    }
+    return du;
 }
 __device__ bool isObstacleCell(const Dev &d, i2 c, uint32_t vx) {       // :1123-1147
     uint32_t t = wType(vx), k = kindOf(d, t);
@@ -645,6 +715,23 @@ __device__ __forceinline__ void splatParticle(const Dev &d, const GatherTile &T,
     uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
     uint4 af = *(const uint4 *)&AFFINE(d, d.parts, k, 0);
     v2 pos = V2(fbits(mo.x), fbits(mo.y));
+    // the air's pressure on a particle beside gas (:1024-1032): its
+    // velocity pushed, and kept
+    if (d.air_on) {
+        const i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
+        bool gas = false;
+        for (int q = 0; q < 4; q++) {
+            const i2 n = beside(c, q);
+            gas = gas || (inWorld(d, n) && kindOf(d, wType(d.cells[4ull * cellIndex(d, n)])) == KIND_GAS);
+        }
+        if (gas) {
+            const v2 push = pressurePush(d, airGrad(d, pos), WATER_RHO);
+            mo.z = ubits(fbits(mo.z) + push.x);
+            mo.w = ubits(fbits(mo.w) + push.y);
+            MOTION(d, d.parts, k, 2) = mo.z;
+            MOTION(d, d.parts, k, 3) = mo.w;
+        }
+    }
     float temp = fbits(st.y);
     splatAxis<TILE>(d, T, V2(pos.x - 0.0f, pos.y - 0.5f), fbits(mo.z), fbits(af.x), fbits(af.y), true);
     splatAxis<TILE>(d, T, V2(pos.x - 0.5f, pos.y - 0.0f), fbits(mo.w), fbits(af.z), fbits(af.w), false);
@@ -983,7 +1070,7 @@ __device__ void setFace(const Dev &d, i2 at, bool isU) {
         held = 1.0f;
     }
     float now = u;
-    if (held > 0.0f) now += 0.0f;                        // the wind's drag: still air
+    if (held > 0.0f && d.air_on) now += surfaceDrag(d, a, at, isU, ga, gb, u);   // the wind's drag
     if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) now += isU ? 0.0f : d.earthG;
     *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(now, u, held, faceViscosity(ga, gb));
     FLD(d, viscRhsSlot(d, f), f % 4u) = now;
@@ -3086,6 +3173,20 @@ __device__ __forceinline__ void setGasPair(const Dev &d, uint32_t i, uint32_t a,
     d.cells[at] = a;
     d.cells[at + 1] = b;
 }
+// pushGasOut (:3855-3861; airsource.wgsl:65-72): cell i's gas into its
+// airSource slot, with gasHeatCap (common.wgsl:473-475) times temp, and none
+// left in the cell
+__device__ void pushGasOut(const Dev &d, uint32_t i, float temp) {
+    const uint64_t at = gasWord(d, i);
+    const v2 m = unpack2x16(d.cells[at]), r = unpack2x16(d.cells[at + 1]);
+    const float vapour = m.x, mist = m.y, lost = r.x, amount = r.y;
+    addSource(d, i, SOURCE_AMOUNT, amount);
+    addSource(d, i, SOURCE_VAPOUR, vapour);
+    addSource(d, i, SOURCE_MIST, mist);
+    addSource(d, i, SOURCE_LOST, lost * amount);
+    addSource(d, i, SOURCE_HEAT, (AIR_HEATCAP * amount + (VAPOUR_HEAT_CAP * vapour + WATER_HEATCAP * mist) / WATER_VAPOUR) * temp);
+    setGasPair(d, i, 0u, 0u);
+}
 __global__ void __launch_bounds__(64) k_raster(WorldGrid g) {                      // :3983-4119
     WORLD(g);
     if (!d.full) return;
@@ -3104,14 +3205,20 @@ __global__ void __launch_bounds__(64) k_raster(WorldGrid g) {                   
             // moveOutOfTheWay (:3954-3973)
             uint32_t to = CNT(d, i, G_AIR_TO);
             if (to != WNONE && CNT(d, to, G_TAKEN) == ~i) {
-                if (wType(d.cells[4ull * to]) == E_AIR) setGasPair(d, to, 0u, 0u);
+                // pushGasOut at the air's temperature as stored (AIR has no
+                // phase points: js/elements.js:424-426), or, without the air, let go
+                if (wType(d.cells[4ull * to]) == E_AIR) {
+                    if (d.air_on) pushGasOut(d, to, fbits(d.cells[4ull * to + 1]));
+                    else setGasPair(d, to, 0u, 0u);
+                }
                 *(uint4 *)&d.cells[4ull * to] = v;
                 uint64_t gi = gasWord(d, i);
                 setGasPair(d, to, d.cells[gi], d.cells[gi + 1]);
                 setGasPair(d, i, 0u, 0u);
                 drawn = true;
             } else if (t == E_AIR) {
-                setGasPair(d, i, 0u, 0u);
+                if (d.air_on) pushGasOut(d, i, fbits(v.y));
+                else setGasPair(d, i, 0u, 0u);
                 drawn = true;
             }
         }

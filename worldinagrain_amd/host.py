@@ -59,6 +59,20 @@ class SolverSettings:
                               coarse_sweeps=coarse_sweeps, storage=storage, residual_every=residual_every)
 
 
+KARMAN = np.float32(0.41)               # js/shared.js:289
+SURFACE_ROUGHNESS = np.float32(1e-4)    # fluid.wgsl:1890
+
+
+def surface_drag_coefficient(dx):
+    """The wind's drag coefficient along a liquid surface, (KARMAN /
+    ln(max(dx / 2 / z0, 2)))^2 in f32 (fluid.wgsl:1884-1895), worked out on
+    the host once a step so every backend takes the same float (the log in
+    double, rounded to f32, as the JavaScript host does)."""
+    x = max(np.float32(np.float32(0.5) * np.float32(dx)) / SURFACE_ROUGHNESS, np.float32(2.0))
+    r = KARMAN / np.float32(np.log(np.float64(np.float32(x))))
+    return float(np.float32(r * r))
+
+
 def mulberry32_next(state):
     """(new state, u32): js/sim.js:1390-1397 (Tommy Ettinger's mulberry32)."""
     state = (state + 0x6D2B79F5) & 0xFFFFFFFF
@@ -86,6 +100,9 @@ class Simulation:
         self.fluid_cycles = self.solver.max_cycles
         self.last_stats = None
         self.last_cycles = 0
+        # the air coupled in (WigStepCtl.air): the world's air part read, its
+        # airSource part added to; off, the air is at rest
+        self.air = False
 
     # Simulation.seed / random (js/sim.js:1386-1397)
     def seed(self, n):
@@ -120,6 +137,8 @@ class Simulation:
         c.level_pairs = s.level_pairs
         c.vcycle_storage = s.storage
         c.residual_every = s.residual_every
+        c.air = 1 if self.air else 0
+        c.surface_drag = surface_drag_coefficient(self.dx)
         if s.first_chunk < 0:
             c.first_chunk = 0xFFFFFFFF          # WIG_CHUNK_AUTO: the world's last solve's count
             c.chunk = s.chunk
@@ -197,6 +216,8 @@ class BatchHost:
             c.level_pairs = sv.level_pairs
             c.vcycle_storage = sv.storage
             c.residual_every = sv.residual_every
+            c.air = 1 if s.air else 0
+            c.surface_drag = surface_drag_coefficient(s.dx)
             if sv.first_chunk < 0:
                 c.first_chunk, c.chunk = 0xFFFFFFFF, sv.chunk
             else:

@@ -14,12 +14,22 @@ const FLUID_CYCLES_SPARE = 2;   // js/sim.js:70
 const LIQUID_TOLERANCE = 1e-2;  // fluid.wgsl:2221
 const GRAVITY = 9.81 / 3600 / 0.01;   // js/elements.js:30, water's weight
 
-const PART = { cells: 0, particles0: 1, particles1: 2, counters: 3, field: 4, blocks: 5, fluidCg: 6, solve: 7 };
+const PART = { cells: 0, particles0: 1, particles1: 2, counters: 3, field: 4, blocks: 5, fluidCg: 6, solve: 7, air0: 8,
+    airSource: 9 };
 const STAT = { ACTIVE: 0, RETIRED: 1, VISCOUS: 2, PARTICLES: 3, RESIDUAL: 4, CYCLES: 7, MOST_CYCLES: 8 };
 const TOL_ABS_MAX = 0;
 const TOL_REL_L2 = 1;
 const CHUNK_AUTO = 0xffffffff;
 const KIND = { SOLID: 1, POWDER: 2, LIQUID: 3, GAS: 4 };
+
+// The wind's drag coefficient along a liquid surface, (KARMAN / ln(max(dx /
+// 2 / z0, 2)))^2 in f32 (fluid.wgsl:1884-1895; KARMAN js/shared.js:289), as
+// the Python host works it out, so both hand the library the same float.
+function surfaceDragCoefficient(dx) {
+    const f = Math.fround;
+    const r = f(f(0.41) / f(Math.log(Math.max(f(f(f(0.5) * f(dx)) / f(1e-4)), 2))));
+    return f(r * r);
+}
 
 // elems[t].kind for the ids of js/elements.js's table: AIR 0, WALL 1,
 // SAND 2, WATER 3, STONE 4, GRAVEL 5.
@@ -45,6 +55,7 @@ class LiquidSimulation {
         this.earthG = options.earthG === undefined ? GRAVITY : options.earthG;
         this.dt = options.dt === undefined ? 1 / 60 : options.dt;
         this.frameNo = options.frame === undefined ? 1 : options.frame;
+        this.air = !!options.air;      // the air coupled in (air0 read, airSource added to)
         const s = options.solver || {};
         this.solver = {
             tolMode: s.tolMode === undefined ? TOL_ABS_MAX : s.tolMode,
@@ -103,7 +114,7 @@ class LiquidSimulation {
     // relaxation's starting tiling (flipped each step), the solve's settings.
     stepControl() {
         this.shiftedFirst = !this.shiftedFirst;
-        const buf = new ArrayBuffer(68);
+        const buf = new ArrayBuffer(76);
         const u = new Uint32Array(buf);
         const f = new Float32Array(buf);
         u[0] = this.fluidStats.active ? 1 : 0;
@@ -118,6 +129,8 @@ class LiquidSimulation {
         u[12] = this.solver.firstChunk >>> 0;
         u[13] = this.solver.chunk;
         u[16] = this.solver.residualEvery || 0;
+        u[17] = this.air ? 1 : 0;
+        f[18] = this.air ? surfaceDragCoefficient(this.metresPerCell) : 0;
         return buf;
     }
 

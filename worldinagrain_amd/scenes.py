@@ -266,3 +266,26 @@ def air_rooms(width, height, seed=1, walls=12, liquid_share=0.15, holes=2, rhs_s
     x0 = np.where(carries, rng.normal(0.0, warm_sigma, (H, W)), 0.0).astype(np.float32)
     x0[fixed] = -0.5                                           # -sinkSpeed (gather, :406-408)
     return mg0, x0
+
+
+def air_over(width, height, words, seed=1, wind=0.05, temp=20.0, lapse=0.002):
+    """An air[0] buffer (H x W x 4 f32: pressure, the right and bottom
+    faces' velocities, temperature) over the world `words`, as the air's own
+    step would hand the liquid step one: a pressure rising down the world
+    by `lapse` a row about 1 atm plus noise, winds N(0, wind) on every face,
+    temperatures about `temp`; liquid cells flagged AIR_BLOCKED and solid
+    cells AIR_SOLID (airsample.wgsl:130-132)."""
+    from . import abi
+    rng = np.random.default_rng(seed)
+    C = width * height
+    kinds = E.kinds()
+    elem = (words[:4 * C].reshape(height, width, 4)[..., 0] & 0xFF).astype(np.int64)
+    kind = np.asarray(kinds, dtype=np.int64)[elem]
+    air = np.empty((height, width, 4), dtype=np.float32)
+    rows = np.arange(height, dtype=np.float32)[:, None]
+    air[..., 0] = 12.0 + lapse * rows + rng.normal(0.0, lapse, (height, width))
+    air[..., 1:3] = rng.normal(0.0, wind, (height, width, 2))
+    air[..., 3] = temp + rng.normal(0.0, 1.0, (height, width))
+    air[kind == E.KIND_LIQUID, 3] = abi.AIR_BLOCKED
+    air[(kind == E.KIND_SOLID) | (kind == E.KIND_POWDER), 3] = abi.AIR_SOLID
+    return air

@@ -169,7 +169,7 @@ def take(sim, elements=None, extra=None):
     reference snapshot's water, gas and bodies)."""
     w = sim.world
     parts, items = {}, []
-    for p in range(abi.PART_COUNT):
+    for p in range(abi.LIQUID_PARTS):
         name = _part_name(p)
         data = np.ascontiguousarray(w.download(p))
         parts[name] = data
@@ -201,7 +201,7 @@ def put(sim, header, parts, elements=None):
         raise SnapshotError("saved with a different set of elements")
     cur = int(header["state"].get("cur", 0))
     load, misfits = [], []
-    for p in range(abi.PART_COUNT):
+    for p in range(abi.LIQUID_PARTS):
         name, size = _part_name(p, cur), int(w.layout.part_bytes[p])
         data = parts.get(name)
         if data is None and _part_name(p) in ADDED:
