@@ -1571,6 +1571,72 @@ template <int S> __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
 // k_levelUp(l): prolong's correction (:2459-2497) from level l + 1, staged
 // in LDS; the sweeps.  Both write the tile's p back to the field.
 #define TILE_THREADS 256u
+// A thread's cells of one colour, at most SWEEP_K of them, held in registers
+// across a level's half-sweeps with what no sweep changes: the four
+// neighbours' betas in beside()'s order (0 where the stencil skips one),
+// their sum (the diagonal) and h^2 times the rhs.  Each half-sweep then
+// issues all its LDS loads at once and sums without branches.  A skipped
+// neighbour's term is selected away, so the sums take the same operations
+// in the same order as smoothAt's loop and the values are the same, bit for
+// bit; its load reads the cell itself, an address always in the level.
+#define SWEEP_K 2
+struct SweepSet {
+    int32_t j[SWEEP_K];   // the cell (-1: none)
+    float b[SWEEP_K][4], diag[SWEEP_K], hx[SWEEP_K], hy[SWEEP_K];
+};
+__device__ __forceinline__ void sweepNone(SweepSet &s, int m) {
+    s.j[m] = -1;
+    s.b[m][0] = s.b[m][1] = s.b[m][2] = s.b[m][3] = 0.0f;
+    s.diag[m] = 1.0f;
+    s.hx[m] = s.hy[m] = 0.0f;
+}
+// cell j with betas b0..b3 (rho > 0 there): kept if the diagonal is positive
+__device__ __forceinline__ void sweepCell(SweepSet &s, int m, int32_t j, float b0, float b1, float b2, float b3, float h2, float rx, float ry) {
+    float diag = 0.0f;
+    diag = b0 != 0.0f ? diag + b0 : diag;
+    diag = b1 != 0.0f ? diag + b1 : diag;
+    diag = b2 != 0.0f ? diag + b2 : diag;
+    diag = b3 != 0.0f ? diag + b3 : diag;
+    if (diag <= 0.0f) return;
+    s.j[m] = j;
+    s.b[m][0] = b0;
+    s.b[m][1] = b1;
+    s.b[m][2] = b2;
+    s.b[m][3] = b3;
+    s.diag[m] = diag;
+    s.hx[m] = h2 * rx;
+    s.hy[m] = h2 * ry;
+}
+// one half-sweep of a set over the w-wide arrays px, py
+__device__ __forceinline__ void sweepHalf(const SweepSet &s, float *px, float *py, int32_t w, float omega) {
+    float ox[SWEEP_K], oy[SWEEP_K], nx[SWEEP_K][4], ny[SWEEP_K][4];
+#pragma unroll
+    for (int m = 0; m < SWEEP_K; m++) {
+        const int32_t j = s.j[m] < 0 ? 0 : s.j[m];
+        const int32_t nb[4] = {j - 1, j + 1, j - w, j + w};
+        ox[m] = px[j];
+        oy[m] = py[j];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int32_t q = s.b[m][k] != 0.0f ? nb[k] : j;
+            nx[m][k] = px[q];
+            ny[m][k] = py[q];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < SWEEP_K; m++) {
+        if (s.j[m] < 0) continue;
+        float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float b = s.b[m][k];
+            sx = b != 0.0f ? sx + b * nx[m][k] : sx;
+            sy = b != 0.0f ? sy + b * ny[m][k] : sy;
+        }
+        px[s.j[m]] = wmix(ox[m], (sx - s.hx[m]) / s.diag[m], omega);
+        py[s.j[m]] = wmix(oy[m], (sy - s.hy[m]) / s.diag[m], omega);
+    }
+}
 struct LevelTile {
     float *px, *py, *rx, *ry, *rho, *bR, *bD;
     int32_t R, ox, oy;   // side with the halo; the halo's corner in level cells
@@ -1667,6 +1733,34 @@ __device__ __forceinline__ TileSum tileStencil(const LevelTile &t, int32_t j) {
 __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
     float h2 = (float)(1u << (2u * l));
     const int32_t half = (t.R - 1) / 2;   // cells of one colour in a row's inner R - 2, at most
+    if (t.R - 2 <= 16 * SWEEP_K) {
+        // a thread's cells: rows 1 + tid / 16 + 16 m, the tid % 16-th of
+        // the row's cells of the colour
+        // (two sets by name, not an array: an array indexed at run time
+        // would sit in scratch memory)
+        SweepSet red, black;
+        const int32_t col = (int32_t)(threadIdx.x % 16u);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int m = 0; m < SWEEP_K; m++) {
+                SweepSet &set = c == 0 ? red : black;
+                sweepNone(set, m);
+                const int32_t y = 1 + (int32_t)(threadIdx.x / 16u) + 16 * m;
+                const int32_t x = 1 + ((c + t.ox + t.oy + y + 1) & 1) + 2 * col;
+                if (y >= t.R - 1 || col >= half || x >= t.R - 1) continue;
+                const int32_t j = y * t.R + x;
+                if (t.rho[j] <= 0.0f) continue;
+                sweepCell(set, m, j, t.bR[j - 1], t.bR[j], t.bD[j - t.R], t.bD[j], h2, t.rx[j], t.ry[j]);
+            }
+        for (uint32_t s = 0; s < pairs; s++) {
+            sweepHalf(red, t.px, t.py, t.R, 1.0f);
+            __syncthreads();
+            sweepHalf(black, t.px, t.py, t.R, 1.0f);
+            __syncthreads();
+        }
+        return;
+    }
     for (uint32_t s = 0; s < 2u * pairs; s++) {
         int color = (int)(s & 1u);
         for (int32_t y = 1 + (int32_t)(threadIdx.x / 16u); y < t.R - 1; y += 16) {
@@ -1936,6 +2030,40 @@ __device__ void ldsSmooth(const LdsLevel &L, uint32_t l, int color, float omega,
     }
     phaseSync(wg);
 }
+// `pairs` red-black pairs (a red then a black half-sweep) over the cells of
+// a level below its first `limit`, by nthr threads: with register sets where
+// a colour's cells number at most SWEEP_K a thread, else ldsSmooth's loop
+__device__ void ldsSweeps(const LdsLevel &L, uint32_t l, uint32_t pairs, float omega, uint32_t limit, uint32_t tid, uint32_t nthr, bool wg) {
+    const uint32_t hw = (L.w + 1u) / 2u;
+    if (hw * L.h > SWEEP_K * nthr) {
+        for (uint32_t k = 0; k < pairs; k++) {
+            ldsSmooth(L, l, 0, omega, limit, tid, nthr, wg);
+            ldsSmooth(L, l, 1, omega, limit, tid, nthr, wg);
+        }
+        return;
+    }
+    const float h2 = (float)(1u << (2u * l));
+    SweepSet red, black;   // (by name: see tileSweeps)
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int m = 0; m < SWEEP_K; m++) {
+            SweepSet &set = c == 0 ? red : black;
+            sweepNone(set, m);
+            const uint32_t k = tid + (uint32_t)m * nthr;
+            if (k >= hw * L.h) continue;
+            const uint32_t y = k / hw, x = 2u * (k - y * hw) + (((uint32_t)c + y) & 1u), j = y * L.w + x;
+            if (x >= L.w || j >= limit || L.rho[j] <= 0.0f) continue;
+            sweepCell(set, m, (int32_t)j, x > 0u ? L.bR[j - 1u] : 0.0f, L.bR[j], y > 0u ? L.bD[j - L.w] : 0.0f, L.bD[j],
+                      h2, L.rx[j], L.ry[j]);
+        }
+    for (uint32_t s = 0; s < pairs; s++) {
+        sweepHalf(red, L.px, L.py, (int32_t)L.w, omega);
+        phaseSync(wg);
+        sweepHalf(black, L.px, L.py, (int32_t)L.w, omega);
+        phaseSync(wg);
+    }
+}
 // restrictLevel (:2342-2377) from level l (T) into level l + 1 (U): each
 // child's residual (residualAt, :2306-2315) summed
 __device__ void ldsRestrict(const Dev &d, const LdsLevel &T, const LdsLevel &U, uint32_t l, uint32_t tid, uint32_t nthr) {
@@ -2084,10 +2212,7 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
     // down: the sweeps, then restrictLevel into the next level
     for (uint32_t l = l0; l < L && l < lw && !idle; l++) {
         const LdsLevel T = level(l), U = level(l + 1u);
-        for (uint32_t k = 0; k < pairs; k++) {
-            ldsSmooth(T, l, 0, 1.0f, 0xffffffffu, tid, 1024u, true);
-            ldsSmooth(T, l, 1, 1.0f, 0xffffffffu, tid, 1024u, true);
-        }
+        ldsSweeps(T, l, pairs, 1.0f, 0xffffffffu, tid, 1024u, true);
         ldsRestrict(d, T, U, l, tid, 1024u);
         __syncthreads();
         ldsBetas(U, tid, 1024u);
@@ -2097,15 +2222,12 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
         // the coarsest (solve, :2407-2456), by the workgroup: its first
         // MG_COARSEST cells
         const LdsLevel T = level(L);
-        for (uint32_t sw = 0; sw < 2u * csweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST, tid, 1024u, true);
+        ldsSweeps(T, L, csweeps, OMEGA, (uint32_t)MG_COARSEST, tid, 1024u, true);
     } else if (tid < 64u) {
         // the first wavefront's levels: down, the coarsest, up to lw
         for (uint32_t l = lw; l < L && !idle; l++) {
             const LdsLevel T = level(l), U = level(l + 1u);
-            for (uint32_t k = 0; k < pairs; k++) {
-                ldsSmooth(T, l, 0, 1.0f, 0xffffffffu, tid, 64u, false);
-                ldsSmooth(T, l, 1, 1.0f, 0xffffffffu, tid, 64u, false);
-            }
+            ldsSweeps(T, l, pairs, 1.0f, 0xffffffffu, tid, 64u, false);
             ldsRestrict(d, T, U, l, tid, 64u);
             phaseSync(false);
             ldsBetas(U, tid, 64u);
@@ -2113,17 +2235,14 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
         }
         {
             const LdsLevel T = level(L);
-            for (uint32_t sw = 0; sw < 2u * csweeps; sw++) ldsSmooth(T, L, (int)(sw & 1u), OMEGA, (uint32_t)MG_COARSEST, tid, 64u, false);
+            ldsSweeps(T, L, csweeps, OMEGA, (uint32_t)MG_COARSEST, tid, 64u, false);
         }
         // up: prolong (:2459-2497), then a red and a black sweep
         for (int32_t l = (int32_t)L - 1; l >= (int32_t)lw && !idle; l--) {
             const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
             ldsProlong(T, U, tid, 64u);
             phaseSync(false);
-            for (uint32_t k = 0; k < pairs; k++) {
-                ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu, tid, 64u, false);
-                ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu, tid, 64u, false);
-            }
+            ldsSweeps(T, (uint32_t)l, pairs, 1.0f, 0xffffffffu, tid, 64u, false);
         }
     }
     __syncthreads();
@@ -2132,10 +2251,7 @@ template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g
         const LdsLevel T = level((uint32_t)l), U = level((uint32_t)l + 1u);
         ldsProlong(T, U, tid, 1024u);
         __syncthreads();
-        for (uint32_t k = 0; k < pairs; k++) {
-            ldsSmooth(T, (uint32_t)l, 0, 1.0f, 0xffffffffu, tid, 1024u, true);
-            ldsSmooth(T, (uint32_t)l, 1, 1.0f, 0xffffffffu, tid, 1024u, true);
-        }
+        ldsSweeps(T, (uint32_t)l, pairs, 1.0f, 0xffffffffu, tid, 1024u, true);
     }
     // the levels back into the field
     for (uint32_t l = l0; l <= L; l++) {
