@@ -392,6 +392,27 @@ __device__ __forceinline__ float liquidDensity(float temp) { return 100.0f / 100
 __device__ __forceinline__ float liquidVolume(float temp) { return 1.0f / expandedAt(effTemp(temp)); }
 
 // ---------------------------------------------------------------- addressing
+// Memory by address space.  The Dev struct's pointers are generic, and so
+// are pointers into LDS once stored in a struct; an access through a generic
+// pointer compiles to a FLAT instruction, which the LDS serves far slower
+// than its own instructions and whose waits tie the LDS and scalar counters
+// to the vector memory one.  These name the space: lds*() for pointers into
+// __shared__ arrays, glb*() for device memory.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) const uint8_t lds_cu8;
+typedef __attribute__((address_space(1))) uint32_t glb_u32;
+__device__ __forceinline__ void ldsAdd(uint32_t *p, uint32_t v) {
+    __hip_atomic_fetch_add((lds_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t ldsGet(const uint32_t *p) { return *(const lds_u32 *)p; }
+__device__ __forceinline__ uint8_t ldsGetU8(const uint8_t *p) { return *(lds_cu8 *)p; }
+__device__ __forceinline__ void glbAdd(uint32_t *p, uint32_t v) {
+    __hip_atomic_fetch_add((glb_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a device-memory object through a generic pointer, as device memory
+template <typename T> __device__ __forceinline__ __attribute__((address_space(1))) T &G(T &x) {
+    return *(__attribute__((address_space(1))) T *)&x;
+}
 #define CNT(d, i, g) ((d).counters[LIQUID_CELL_WORDS * (uint64_t)(i) + (g)])
 #define SUM(d, f, k) ((d).counters[LIQUID_CELL_WORDS * (uint64_t)(d).C + 2ull * (f) + (k)])
 #define FLD(d, v, c) ((d).field[4ull * (uint64_t)(v) + (c)])
@@ -407,6 +428,11 @@ __device__ __forceinline__ float liquidVolume(float temp) { return 1.0f / expand
 
 __device__ __forceinline__ uint32_t kindOf(const Dev &d, uint32_t t) { return d.kind[t & 0xffu]; }
 __device__ __forceinline__ bool inWorld(const Dev &d, i2 c) { return c.x >= 0 && c.y >= 0 && c.x < d.W && c.y < d.H; }
+// (a cell clamped into a w x h grid: an address that is always there, for
+// loads made whatever the test, their values then chosen)
+__device__ __forceinline__ i2 clampTo(i2 c, int32_t w, int32_t h) {
+    return I2(c.x < 0 ? 0 : (c.x >= w ? w - 1 : c.x), c.y < 0 ? 0 : (c.y >= h ? h - 1 : c.y));
+}
 __device__ __forceinline__ uint32_t cellIndex(const Dev &d, i2 c) { return (uint32_t)c.y * (uint32_t)d.W + (uint32_t)c.x; }
 // The solve's own arrays (the packed fine level, the FP64 check's vectors)
 // are block-major: each 8 x 8 block's cells contiguous, row-major inside it,

@@ -670,8 +670,8 @@ struct GatherTile {
 };
 template <bool TILE>
 __device__ __forceinline__ void addWord(uint32_t *tile, int t, uint32_t *global, uint32_t v) {
-    if (TILE && t >= 0) atomicAdd(&tile[t], v);
-    else atomicAdd(global, v);
+    if (TILE && t >= 0) ldsAdd(&tile[t], v);
+    else glbAdd(global, v);
 }
 template <bool TILE>
 __device__ __forceinline__ void splatAxis(const Dev &d, const GatherTile &T, v2 s, float vel, float gx, float gy, bool isU) {
@@ -691,29 +691,52 @@ __device__ __forceinline__ void splatAxis(const Dev &d, const GatherTile &T, v2 
         addWord<TILE>(isU ? T.uw : T.vw, t, &SUM(d, fi, 1), w);
     }
 }
+// splatDensity's four cells, b .. b + 1 from its corner b = floor(pos - 0.5),
+// and their first words (0 off the world), loaded when the particle's are
+struct DensityCells { i2 b; uint32_t w[4]; };
+__device__ __forceinline__ DensityCells densityCells(const Dev &d, v2 pos) {
+    DensityCells q;
+    q.b = I2(wi32(floorf(pos.x - 0.5f)), wi32(floorf(pos.y - 0.5f)));
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const i2 c = I2(q.b.x + (j & 1), q.b.y + (j >> 1));
+        const uint32_t w = d.cells[4ull * cellIndex(d, clampTo(c, d.W, d.H))];
+        q.w[j] = inWorld(d, c) ? w : 0u;
+    }
+    return q;
+}
 template <bool TILE>
-__device__ __forceinline__ void splatDensity(const Dev &d, const GatherTile &T, v2 pos, float room) {
+__device__ __forceinline__ void splatDensity(const Dev &d, const GatherTile &T, v2 pos, float room, const DensityCells &q,
+                                             const uint8_t *kinds) {
     v2 s = V2(pos.x - 0.5f, pos.y - 0.5f);
     float fx0 = floorf(s.x), fy0 = floorf(s.y);
     i2 b = I2(wi32(fx0), wi32(fy0));
     v2 f = V2(s.x - fx0, s.y - fy0);
+    // walls() (:1056-1062) of the four cells: off the world, solid or powder
+    bool wall[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t k = ldsGetU8(&kinds[wType(q.w[j]) & 0xffu]);
+        wall[j] = !inWorld(d, I2(b.x + (j & 1), b.y + (j >> 1))) || k == KIND_SOLID || k == KIND_POWDER;
+    }
     for (int j = 0; j < 4; j++) {
         int ox = j & 1, oy = j >> 1;
         float wgt = (ox == 1 ? f.x : 1.0f - f.x) * (oy == 1 ? f.y : 1.0f - f.y);
         i2 at = I2(b.x + ox, b.y + oy);
-        if (walls(d, at)) {
+        if (wall[j]) {
+            // the corner across in x, in y, then the diagonal
             i2 x = I2(b.x + 1 - ox, at.y), y = I2(at.x, b.y + 1 - oy), xy = I2(b.x + 1 - ox, b.y + 1 - oy);
-            at = !walls(d, x) ? x : (!walls(d, y) ? y : (!walls(d, xy) ? xy : I2(-1, -1)));
+            at = !wall[j ^ 1] ? x : (!wall[j ^ 2] ? y : (!wall[j ^ 3] ? xy : I2(-1, -1)));
         }
         if (inWorld(d, at) && wgt > 0.0f)
             addWord<TILE>(T.dens, TILE ? T.at(at) : -1, &CNT(d, cellIndex(d, at), G_DENSITY), wu32(wround(wgt * room * FIXF)));
     }
 }
+// (particle k's motion, state and affine words and its density cells,
+// loaded by the caller; kinds: the element kinds staged in LDS)
 template <bool TILE>
-__device__ __forceinline__ void splatParticle(const Dev &d, const GatherTile &T, uint32_t k) {
-    uint4 mo = *(const uint4 *)&MOTION(d, d.parts, k, 0);
-    uint4 st = *(const uint4 *)&STATE(d, d.parts, k, 0);
-    uint4 af = *(const uint4 *)&AFFINE(d, d.parts, k, 0);
+__device__ __forceinline__ void splatParticle(const Dev &d, const GatherTile &T, uint32_t k, uint4 mo, const uint4 st, const uint4 af,
+                                              const DensityCells &q, const uint8_t *kinds) {
     v2 pos = V2(fbits(mo.x), fbits(mo.y));
     // the air's pressure on a particle beside gas (:1024-1032): its
     // velocity pushed, and kept
@@ -740,7 +763,7 @@ __device__ __forceinline__ void splatParticle(const Dev &d, const GatherTile &T,
     int t = TILE ? T.at(hc) : -1;
     addWord<TILE>(T.cnt, t, &CNT(d, home, G_COUNT), 1u);
     addWord<TILE>(T.mass, t, &CNT(d, home, G_MASS), wu32(wround(liquidDensity(temp) * FIXF)));
-    splatDensity<TILE>(d, T, pos, liquidVolume(temp));
+    splatDensity<TILE>(d, T, pos, liquidVolume(temp), q, kinds);
 }
 // (the tile's words, added into the counters: one atomic a nonzero word;
 // threads t0, t0 + step, ... of the tile's entries)
@@ -748,21 +771,23 @@ __device__ __forceinline__ void flushTile(const Dev &d, const GatherTile &T, int
     const int ne = T.tw * T.th;
     for (int j = t0; j < ne; j += step) {
         i2 c = I2(T.ox + j % T.tw, T.oy + j / T.tw);
-        if (T.um[j] | T.uw[j]) {          // (a face outside the world was never added to)
+        const uint32_t um = ldsGet(&T.um[j]), uw = ldsGet(&T.uw[j]), vm = ldsGet(&T.vm[j]), vw = ldsGet(&T.vw[j]);
+        const uint32_t cn = ldsGet(&T.cnt[j]), ms = ldsGet(&T.mass[j]), dn = ldsGet(&T.dens[j]);
+        if (um | uw) {          // (a face outside the world was never added to)
             uint32_t fi = uFace(d, c.x, c.y);
-            if (T.um[j]) atomicAdd(&SUM(d, fi, 0), T.um[j]);
-            if (T.uw[j]) atomicAdd(&SUM(d, fi, 1), T.uw[j]);
+            if (um) glbAdd(&SUM(d, fi, 0), um);
+            if (uw) glbAdd(&SUM(d, fi, 1), uw);
         }
-        if (T.vm[j] | T.vw[j]) {
+        if (vm | vw) {
             uint32_t fi = vFace(d, c.x, c.y);
-            if (T.vm[j]) atomicAdd(&SUM(d, fi, 0), T.vm[j]);
-            if (T.vw[j]) atomicAdd(&SUM(d, fi, 1), T.vw[j]);
+            if (vm) glbAdd(&SUM(d, fi, 0), vm);
+            if (vw) glbAdd(&SUM(d, fi, 1), vw);
         }
-        if (T.cnt[j] | T.mass[j] | T.dens[j]) {
+        if (cn | ms | dn) {
             uint32_t ci = cellIndex(d, c);
-            if (T.cnt[j]) atomicAdd(&CNT(d, ci, G_COUNT), T.cnt[j]);
-            if (T.mass[j]) atomicAdd(&CNT(d, ci, G_MASS), T.mass[j]);
-            if (T.dens[j]) atomicAdd(&CNT(d, ci, G_DENSITY), T.dens[j]);
+            if (cn) glbAdd(&CNT(d, ci, G_COUNT), cn);
+            if (ms) glbAdd(&CNT(d, ci, G_MASS), ms);
+            if (dn) glbAdd(&CNT(d, ci, G_DENSITY), dn);
         }
     }
 }
@@ -787,13 +812,22 @@ __global__ void __launch_bounds__(256) k_gather(WorldGrid g) {
     __shared__ int32_t sBox[4];
     uint32_t cnt = ld(&d.parts[0]);
     uint32_t n = cnt < d.cap ? cnt : d.cap;
+    __shared__ uint8_t sKind[256];
     const uint32_t tid = threadIdx.x, wave = tid / 64u;
+    sKind[tid] = (uint8_t)d.kind[tid];   // (the first box's barriers pass before a read)
     for (uint32_t base = bid * 256u; base < n; base += nbid * 256u) {
         uint32_t k = base + tid;
+        // the particle's words and its density cells' words, all loaded at
+        // once, before the box is known
+        const uint32_t kk = k < n ? k : n - 1u;
+        const uint4 mo = *(const uint4 *)&MOTION(d, d.parts, kk, 0);
+        const uint4 st = *(const uint4 *)&STATE(d, d.parts, kk, 0);
+        const uint4 af = *(const uint4 *)&AFFINE(d, d.parts, kk, 0);
+        const DensityCells q = densityCells(d, V2(fbits(mo.x), fbits(mo.y)));
         // the cells' box, the wavefront's and the workgroup's
         int32_t cx = 0x7fffffff, cy = 0x7fffffff, mx = -1, my = -1;
         if (k < n) {
-            i2 c = cellOf(d, V2(fbits(MOTION(d, d.parts, k, 0)), fbits(MOTION(d, d.parts, k, 1))));
+            i2 c = cellOf(d, V2(fbits(mo.x), fbits(mo.y)));
             cx = mx = c.x;
             cy = my = c.y;
         }
@@ -823,7 +857,7 @@ __global__ void __launch_bounds__(256) k_gather(WorldGrid g) {
             tileWords(T, sTile);
             for (int j = (int)tid; j < 7 * T.tw * T.th; j += 256) sTile[j] = 0u;
             __syncthreads();
-            if (k < n) splatParticle<true>(d, T, k);
+            if (k < n) splatParticle<true>(d, T, k, mo, st, af, q, sKind);
             __syncthreads();
             flushTile(d, T, (int)tid, 256);
             __syncthreads();
@@ -842,8 +876,8 @@ __global__ void __launch_bounds__(256) k_gather(WorldGrid g) {
             for (int j = (int)lane_id(); j < 7 * V.tw * V.th; j += 64) mine[j] = 0u;
         __syncthreads();
         if (k < n) {
-            if (quarter) splatParticle<true>(d, V, k);
-            else splatParticle<false>(d, V, k);
+            if (quarter) splatParticle<true>(d, V, k, mo, st, af, q, sKind);
+            else splatParticle<false>(d, V, k, mo, st, af, q, sKind);
         }
         __syncthreads();
         if (quarter) flushTile(d, V, (int)lane_id(), 64);
@@ -1228,10 +1262,23 @@ __device__ __forceinline__ i2 tileBlock(const Dev &d, uint32_t na, uint32_t wid,
     if (shift == 0 || j < 0 || j > high + wide) return I2(-1, -1);
     return j <= high ? I2(wide, j) : I2(j - high - 1, high);
 }
-// (a cell clamped into a w x h grid: an address that is always there, for
-// loads made whatever the test, their values then chosen)
-__device__ __forceinline__ i2 clampTo(i2 c, int32_t w, int32_t h) {
-    return I2(c.x < 0 ? 0 : (c.x >= w ? w - 1 : c.x), c.y < 0 ? 0 : (c.y >= h ? h - 1 : c.y));
+// prolonged's weighted sums (:2458-2497) over its four coarse cells in its
+// order (the cell under, the one across in x, in y, the diagonal; weights
+// 9, 3, 3, 1), from their rho and p loaded beforehand: a cell of rho < 0
+// is skipped, one of rho 0 adds its weight only.  Selects in place of the
+// loop's branches: the same operations in the same order.
+struct ProlongSum { float sx, sy, weight; };
+__device__ __forceinline__ ProlongSum prolongSum(const float r[4], const float px[4], const float py[4]) {
+    ProlongSum s = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
+        const bool on = !(r[k] < 0.0f), add = r[k] > 0.0f;
+        s.weight = on ? s.weight + wk : s.weight;
+        s.sx = add ? s.sx + wk * px[k] : s.sx;
+        s.sy = add ? s.sy + wk * py[k] : s.sy;
+    }
+    return s;
 }
 #define NT RELAX_NT
 template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int post) {
@@ -1345,22 +1392,19 @@ template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int
                     // prolonged() (:2458-2497) from the staged cells
                     i2 coarse = I2(c.x / 2, c.y / 2);
                     i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
-                    float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+                    float rc[4], cx[4], cy[4];
+#pragma unroll
                     for (int o4 = 0; o4 < 4; o4++) {
                         i2 oo = o4 == 0 ? I2(0, 0) : (o4 == 1 ? I2(sg.x, 0) : (o4 == 2 ? I2(0, sg.y) : sg));
-                        float wk = o4 == 0 ? 9.0f : (o4 == 3 ? 1.0f : 3.0f);
                         int32_t jj = (coarse.y + oo.y - cy0) * 8 + (coarse.x + oo.x - cx0);
-                        float rc = cRhoA[s_][jj];
-                        if (rc < 0.0f) continue;
-                        weight += wk;
-                        if (rc > 0.0f) {
-                            sx += wk * cPxA[s_][jj];
-                            sy += wk * cPyA[s_][jj];
-                        }
+                        rc[o4] = cRhoA[s_][jj];
+                        cx[o4] = cPxA[s_][jj];
+                        cy[o4] = cPyA[s_][jj];
                     }
-                    float dn = fmaxf(weight, 1e-6f);
-                    v2_.x = v2_.x + sx / dn;
-                    v2_.y = v2_.y + sy / dn;
+                    const ProlongSum ps = prolongSum(rc, cx, cy);
+                    float dn = fmaxf(ps.weight, 1e-6f);
+                    v2_.x = v2_.x + ps.sx / dn;
+                    v2_.y = v2_.y + ps.sy / dn;
                 }
                 sPxA[s_][k] = v2_.x;
                 sPyA[s_][k] = v2_.y;
@@ -1716,17 +1760,26 @@ __device__ __forceinline__ void tileBetas(const LevelTile &t) {
         t.bD[j] = y + 1 < t.R ? faceBeta(r, t.rho[j + t.R]) : 0.0f;
     }
 }
-// smoothAt's stencil sums at tile cell j (neighbours in beside()'s order)
+// smoothAt's stencil sums at tile cell j, not on the tile's rim
+// (neighbours in beside()'s order; every load made at once, a skipped
+// neighbour's term selected away)
 struct TileSum { float sx, sy, diag; };
 __device__ __forceinline__ TileSum tileStencil(const LevelTile &t, int32_t j) {
     const float b[4] = {t.bR[j - 1], t.bR[j], t.bD[j - t.R], t.bD[j]};
     const int32_t nb[4] = {j - 1, j + 1, j - t.R, j + t.R};
-    TileSum s = {0.0f, 0.0f, 0.0f};
+    float px[4], py[4];
+#pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (b[k] == 0.0f) continue;
-        s.sx += b[k] * t.px[nb[k]];
-        s.sy += b[k] * t.py[nb[k]];
-        s.diag += b[k];
+        px[k] = t.px[nb[k]];
+        py[k] = t.py[nb[k]];
+    }
+    TileSum s = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool use = b[k] != 0.0f;
+        s.sx = use ? s.sx + b[k] * px[k] : s.sx;
+        s.sy = use ? s.sy + b[k] * py[k] : s.sy;
+        s.diag = use ? s.diag + b[k] : s.diag;
     }
     return s;
 }
@@ -1912,22 +1965,19 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
         if (!(r0 > 0.0f)) continue;     // (outside the level rho is -1)
         i2 coarse = I2(c.x / 2, c.y / 2);
         i2 sg = I2((c.x & 1) == 1 ? 1 : -1, (c.y & 1) == 1 ? 1 : -1);
-        float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+        float rc[4], cx[4], cy[4];
+#pragma unroll
         for (int k = 0; k < 4; k++) {
             i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
-            float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
             int32_t q = (coarse.y + o.y - cy0) * Rc + (coarse.x + o.x - cx0);
-            float r = crho[q];
-            if (r < 0.0f) continue;
-            weight += wk;
-            if (r > 0.0f) {
-                sx += wk * cpx[q];
-                sy += wk * cpy[q];
-            }
+            rc[k] = crho[q];
+            cx[k] = cpx[q];
+            cy[k] = cpy[q];
         }
-        float dn = fmaxf(weight, 1e-6f);
-        t.px[j] = t.px[j] + sx / dn;
-        t.py[j] = t.py[j] + sy / dn;
+        const ProlongSum ps = prolongSum(rc, cx, cy);
+        float dn = fmaxf(ps.weight, 1e-6f);
+        t.px[j] = t.px[j] + ps.sx / dn;
+        t.py[j] = t.py[j] + ps.sy / dn;
     }
     tileBetas(t);
     __syncthreads();
@@ -1998,16 +2048,24 @@ __device__ void ldsBetas(const LdsLevel &L, uint32_t tid, uint32_t nthr) {
     }
 }
 struct LdsSum { float sx, sy, diag; };
+// (as tileStencil; a neighbour off the level is read at the cell itself)
 __device__ __forceinline__ LdsSum ldsStencil(const LdsLevel &L, uint32_t x, uint32_t y) {
     uint32_t j = y * L.w + x;
     const float b[4] = {x > 0u ? L.bR[j - 1u] : 0.0f, L.bR[j], y > 0u ? L.bD[j - L.w] : 0.0f, L.bD[j]};
-    const uint32_t nb[4] = {j - 1u, j + 1u, j - L.w, j + L.w};
-    LdsSum s = {0.0f, 0.0f, 0.0f};
+    const uint32_t nb[4] = {x > 0u ? j - 1u : j, x + 1u < L.w ? j + 1u : j, y > 0u ? j - L.w : j, y + 1u < L.h ? j + L.w : j};
+    float px[4], py[4];
+#pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (b[k] == 0.0f) continue;
-        s.sx += b[k] * L.px[nb[k]];
-        s.sy += b[k] * L.py[nb[k]];
-        s.diag += b[k];
+        px[k] = L.px[nb[k]];
+        py[k] = L.py[nb[k]];
+    }
+    LdsSum s = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool use = b[k] != 0.0f;
+        s.sx = use ? s.sx + b[k] * px[k] : s.sx;
+        s.sy = use ? s.sy + b[k] * py[k] : s.sy;
+        s.diag = use ? s.diag + b[k] : s.diag;
     }
     return s;
 }
@@ -2114,24 +2172,22 @@ __device__ void ldsProlong(const LdsLevel &T, const LdsLevel &U, uint32_t tid, u
         if (T.rho[j] <= 0.0f) continue;
         i2 coarse = I2((int32_t)x / 2, (int32_t)y / 2);
         i2 sg = I2((x & 1u) == 1u ? 1 : -1, (y & 1u) == 1u ? 1 : -1);
-        float sx = 0.0f, sy = 0.0f, weight = 0.0f;
+        float rc[4], cx[4], cy[4];
+#pragma unroll
         for (int k = 0; k < 4; k++) {
             i2 o = k == 0 ? I2(0, 0) : (k == 1 ? I2(sg.x, 0) : (k == 2 ? I2(0, sg.y) : sg));
-            float wk = k == 0 ? 9.0f : (k == 3 ? 1.0f : 3.0f);
             i2 at = I2(coarse.x + o.x, coarse.y + o.y);
-            if (at.x < 0 || at.y < 0 || at.x >= (int32_t)U.w || at.y >= (int32_t)U.h) continue;   // rho -1
-            uint32_t ja = (uint32_t)at.y * U.w + (uint32_t)at.x;
-            float r = U.rho[ja];
-            if (r < 0.0f) continue;
-            weight += wk;
-            if (r > 0.0f) {
-                sx += wk * U.px[ja];
-                sy += wk * U.py[ja];
-            }
+            const bool in = !(at.x < 0 || at.y < 0 || at.x >= (int32_t)U.w || at.y >= (int32_t)U.h);   // else rho -1
+            const uint32_t ja = in ? (uint32_t)at.y * U.w + (uint32_t)at.x : 0u;
+            const float r = U.rho[ja];
+            rc[k] = in ? r : -1.0f;
+            cx[k] = U.px[ja];
+            cy[k] = U.py[ja];
         }
-        float dn = fmaxf(weight, 1e-6f);
-        T.px[j] = T.px[j] + sx / dn;
-        T.py[j] = T.py[j] + sy / dn;
+        const ProlongSum ps = prolongSum(rc, cx, cy);
+        float dn = fmaxf(ps.weight, 1e-6f);
+        T.px[j] = T.px[j] + ps.sx / dn;
+        T.py[j] = T.py[j] + ps.sy / dn;
     }
 }
 template <int S> __global__ void __launch_bounds__(1024) k_coarseLDS(WorldGrid g) {
