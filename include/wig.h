@@ -96,6 +96,13 @@ typedef struct WigStepCtl {
                                 (KARMAN / ln(max(dx / 2 / 1e-4, 2)))^2 (fluid.wgsl:1884-1895); the host
                                 works it out from dx (worldinagrain_amd/host.py) so every backend takes
                                 the same float */
+    uint32_t occupants;      /* 1: people's bodies, WIG_PART_OCCUPANTS, are obstacles that move
+                                (fluid.wgsl:175-178): a particle's or the density splat's wall
+                                (:424-431, :1056-1062), an obstacle cell to classify (:1123-1125), the
+                                faces beside one move at the person's velocity (obstacleVel,
+                                :1820-1827), no gas is claimed into one (:3919, :3937), and raster leaves
+                                its particles undrawn and its liquid turned air (:3995-4004).  0: no
+                                one (the part is not read) */
 } WigStepCtl;
 
 /* V-cycle storage: float in the reference's buffers (results bit-identical
@@ -128,7 +135,11 @@ typedef struct WigStepCtl {
                                   units of 2^-32: the velocity added to its right and bottom faces, then
                                   gas amount, vapour, mist, oxygen lost, heat (airsource.wgsl:10-33);
                                   added to by the step, taken in and cleared by the air's own passes */
-#define WIG_PART_COUNT 10
+#define WIG_PART_OCCUPANTS 10  /* occupants: a u32 a cell, common.wgsl's occupantWord (:238-254): 0 nobody,
+                                  else the person's slot + 1 in the low byte and their velocity times 128
+                                  (cells a step) in two 12-bit two's-complement fields, x at bit 8 and y at
+                                  bit 20; read */
+#define WIG_PART_COUNT 11
 
 /* FLUID_STATS words (js/shared.js:271-272). */
 #define WIG_STAT_ACTIVE 0

@@ -197,6 +197,7 @@ typedef struct OWorld {
     uint8_t *solve;              /* SolveHead, then 4 doubles a block */
     float *air;                  /* air[0]: a vec4 an air cell */
     uint32_t *source;            /* airSource: SOURCE_WORDS an air cell */
+    uint32_t *occ;               /* occupants: occupantWord a cell */
     uint8_t kind[256];
     int dot_mode;
     /* the step being run */
@@ -404,6 +405,7 @@ static void layout(OWorld *w, uint32_t width, uint32_t height, uint32_t coarse) 
     w->bytes[WIG_PART_SOLVE] = sizeof(SolveHead) + 32ull * w->B + 64u;   /* (and the library's 16 solver words) */
     w->bytes[WIG_PART_AIR] = 16ull * w->C;
     w->bytes[WIG_PART_AIR_SOURCE] = 4ull * SOURCE_WORDS * w->C;
+    w->bytes[WIG_PART_OCCUPANTS] = 4ull * w->C;
 }
 
 /* ------------------------------------------------------------ address */
@@ -518,7 +520,19 @@ static inline uint32_t fallingBit(const OWorld *w, uint32_t back) { return 4u <<
 static inline int fallingAt(const OWorld *w, i2 q) {         /* :1120-1122 */
     return inWorld(w, q) && (wu32(FLD(2u * cellIndex(w, q) + 1u, 3)) & fallingBit(w, 1u)) != 0u;
 }
+/* people's bodies (common.wgsl:238-254; fluid.wgsl:175-178): cell i's
+ * occupantWord, 0 for nobody or with the occupants off */
+#define OCCUPANT_SCALE 128.0f
+static inline uint32_t occupantAt(const OWorld *w, uint32_t i) { return w->ctl->occupants ? w->occ[i] : 0u; }
+static inline int isOccupant(uint32_t o) { return (o & 0xffu) != 0u; }
+static inline v2 occupantVel(uint32_t o) {
+    /* the 12-bit fields sign-extended by shifting them to the top first */
+    int32_t x = (int32_t)(o << 12u) >> 20, y = (int32_t)o >> 20;
+    if (MUTANT(28)) y = (int32_t)(o << 12u) >> 20;
+    return V2((float)x / OCCUPANT_SCALE, (float)y / OCCUPANT_SCALE);
+}
 static int isObstacleCell(const OWorld *w, i2 c, uint32_t vx) {   /* :1123-1147 */
+    if (inWorld(w, c) && isOccupant(occupantAt(w, cellIndex(w, c)))) return 1;
     uint32_t t = wType(vx), k = kindOf(w, t);
     if (k == KIND_GAS) {
         int wet = 0;
@@ -551,12 +565,12 @@ static v2 liquidIn(const OWorld *w, uint32_t i) {           /* :1165-1179 */
 }
 static int isObstaclePos(const OWorld *w, v2 pos) {          /* :424-431 */
     i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
-    if (gridZ(w, c) < 0.0f) return 1;
+    if (gridZ(w, c) < 0.0f || isOccupant(occupantAt(w, cellIndex(w, c)))) return 1;
     uint32_t k = kindOf(w, wType(w->cells[4ull * cellIndex(w, c)]));
     return k == KIND_SOLID || k == KIND_POWDER;
 }
 static inline int walls(const OWorld *w, i2 c) {            /* :1056-1062 */
-    if (!inWorld(w, c)) return 1;
+    if (!inWorld(w, c) || isOccupant(occupantAt(w, cellIndex(w, c)))) return 1;
     uint32_t k = kindOf(w, wType(w->cells[4ull * cellIndex(w, c)]));
     return k == KIND_SOLID || k == KIND_POWDER;
 }
@@ -1145,6 +1159,11 @@ static void k_classify(OWorld *w) {                          /* :1578-1696 */
 /* ----------------------------------------------------------------- faces */
 static float obstacleVel(OWorld *w, i2 c, int axis) {       /* :1820-1834 */
     if (!inWorld(w, c)) return 0.0f;
+    uint32_t held = occupantAt(w, cellIndex(w, c));
+    if (isOccupant(held)) {
+        v2 u = occupantVel(held);
+        return axis == 0 ? u.x : u.y;
+    }
     const uint32_t *v = &w->cells[4ull * cellIndex(w, c)];
     uint32_t k = kindOf(w, wType(v[0]));
     if (k == KIND_SOLID || k == KIND_GAS) return 0.0f;
@@ -2174,7 +2193,7 @@ static uint32_t airTo(OWorld *w, i2 c, int gas) {            /* :3898-3924 */
         i2 at = I2(c.x + o.x, c.y + o.y);
         if (!inWorld(w, at)) continue;
         uint32_t j = cellIndex(w, at);
-        if (vacated(w, j) || (gas && wType(w->cells[4ull * j]) == E_AIR && CNT(j, G_SORT) == 0u)) return j;
+        if (vacated(w, j) || (gas && wType(w->cells[4ull * j]) == E_AIR && CNT(j, G_SORT) == 0u && !isOccupant(occupantAt(w, j)))) return j;
     }
     return NONE;
 }
@@ -2183,7 +2202,7 @@ static void k_rasterClaims(OWorld *w) {                      /* :3928-3948 */
         if (!inWorld(w, c)) continue;
         uint32_t i = cellIndex(w, c), to = NONE;
         uint32_t t = wType(w->cells[4ull * i]);
-        if (CNT(i, G_SORT) > 0u) {
+        if (CNT(i, G_SORT) > 0u && !isOccupant(occupantAt(w, i))) {
             if (t == E_AIR) to = airTo(w, c, 0);
             else if (kindOf(w, t) == KIND_GAS && gridZ(w, c) >= 0.0f) to = airTo(w, c, 1);
         }
@@ -2222,6 +2241,20 @@ static void k_raster(OWorld *w) {                            /* :3983-4119 */
         uint32_t v[4];
         for (int j = 0; j < 4; j++) v[j] = w->cells[4ull * i + j];
         uint32_t t = wType(v[0]);
+        /* a person's body (:3995-4004): not drawn, its particles left to find
+         * their way out, and liquid drawn there before the person came is air
+         * (at the liquid's wordTempOf, water's effective temperature) */
+        if (isOccupant(occupantAt(w, i))) {
+            if (kindOf(w, t) == KIND_LIQUID && !MUTANT(29)) {
+                w->cells[4ull * i] = E_AIR;
+                w->cells[4ull * i + 1] = ubits(effTemp(fbits(v[1])));
+                w->cells[4ull * i + 2] = 0u;
+                w->cells[4ull * i + 3] = 0u;
+                setGasPair(w, i, 0u, 0u);
+            }
+            CNT(i, G_DEST) = DEST_KEEP;
+            continue;
+        }
         int drawn = 0;
         if (kindOf(w, t) == KIND_LIQUID) drawn = 1;
         else if (t == E_AIR || (kindOf(w, t) == KIND_GAS && gridZ(w, c) >= 0.0f)) drawn = moveOutOfTheWay(w, i, v);
@@ -2428,6 +2461,7 @@ OWorld *wor_create(uint32_t width, uint32_t height, uint32_t coarse_levels) {
     w->bvec = (float *)calloc(w->C, sizeof(float));
     w->air = (float *)calloc(1, w->bytes[WIG_PART_AIR]);
     w->source = (uint32_t *)calloc(1, w->bytes[WIG_PART_AIR_SOURCE]);
+    w->occ = (uint32_t *)calloc(1, w->bytes[WIG_PART_OCCUPANTS]);
     w->dot_mode = 1;
     if (!w->cells || !w->sorted || !w->parts || !w->counters || !w->field || !w->blocks || !w->cg || !w->solve || !w->bvec
         || !w->air || !w->source)
@@ -2438,14 +2472,14 @@ void wor_destroy(OWorld *w) {
     if (!w) return;
     free(w->cells); free(w->sorted); free(w->parts); free(w->counters);
     free(w->field); free(w->blocks); free(w->cg); free(w->solve); free(w->cap_b); free(w->bvec);
-    free(w->air); free(w->source);
+    free(w->air); free(w->source); free(w->occ);
     free(w);
 }
 void *wor_part(OWorld *w, int part, uint64_t *bytes) {
     if (part < 0 || part >= WIG_PART_COUNT) return NULL;
     if (bytes) *bytes = w->bytes[part];
     void *p[WIG_PART_COUNT] = {w->cells, w->sorted, w->parts, w->counters, w->field, w->blocks, w->cg, w->solve,
-                               w->air, w->source};
+                               w->air, w->source, w->occ};
     return p[part];
 }
 void wor_set_kinds(OWorld *w, const uint8_t kinds[256]) { memcpy(w->kind, kinds, 256); }

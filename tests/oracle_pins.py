@@ -14,7 +14,8 @@ import numpy as np
 
 from oracle.oracle import (OracleWorld, PASS_ADVECT, PASS_CLASSIFY, PASS_COARSEST, PASS_COMPACT, PASS_DIVERGENCE,
                            PASS_EXTRAPOLATE, PASS_PROLONG, PASS_PROLONG_FINE, PASS_RASTER, PASS_RELAX, PASS_RESTRICT,
-                           PASS_SEAL, PASS_SET_FACES, PASS_SMOOTH, PASS_SORT, PASS_SYNC_PARTICLES, PASS_UNPAINT)
+                           PASS_RASTER_CLAIMS, PASS_SEAL, PASS_SET_FACES, PASS_SMOOTH, PASS_SORT, PASS_SYNC_PARTICLES,
+                           PASS_UNPAINT)
 from worldinagrain_amd import abi, scenes
 from worldinagrain_amd import elements as E
 from worldinagrain_amd.host import Simulation, SolverSettings
@@ -26,6 +27,7 @@ G_TAKEN = 1
 HAS_FLUID, NEAR_LIQUID = 2, 8
 MARK_LIQUID, MARK_REGION = 1, 2
 DEST_KEEP, DEST_CELL, DEST_SPAN = 0xFFFFFFFF, 0x40000000, 0x3FFFFFFF
+NONE_WORD = 0xFFFFFFFF
 FL_DRAWN = 4
 APART = 1.0 / 256.0
 
@@ -126,9 +128,14 @@ class Hand:
         """(H, W, 2): the gas plane's two words a cell (vapour, mist | lost, amount as f16 pairs)."""
         return self.cellwords[4 * self.C:4 * self.C + 2 * self.C].reshape(self.H, self.W, 2)
 
-    def ctl(self, coarse_sweeps=40, air=0, surface_drag=0.0):
+    def occupants(self):
+        """(H, W): the occupants part's occupantWord a cell."""
+        return self.w.part(abi.PART_OCCUPANTS).reshape(self.H, self.W)
+
+    def ctl(self, coarse_sweeps=40, air=0, surface_drag=0.0, occupants=0):
         c = abi.WigStepCtl()
         c.air = air
+        c.occupants = occupants
         c.surface_drag = surface_drag
         c.full = 1
         c.fluid_cycles = 1
@@ -1019,6 +1026,84 @@ def pin_push_gas_out(L):
     assert words[6, 6, 0] & 0xFF == E.WATER
 
 
+# ------------------------------------------------------------ people
+def pin_occupants(L):
+    """People's bodies (fluid.wgsl:175-178; common.wgsl:238-254): the
+    occupant word round-trips a velocity of either sign at 1/128 of a cell;
+    a face between liquid and someone's cell takes their velocity
+    (obstacleVel, :1820-1827), and nothing with the occupants off; raster
+    leaves an occupied cell's particles undrawn and turns the liquid there
+    into air at water's effective temperature (:3995-4004); no gas is claimed
+    out of an occupied cell (:3937) nor into one (:3919)."""
+    for occ_on in (1, 0):
+        h = Hand(16, 16, L=L)
+        h.activate()
+        words, cell, occ = h.words(), h.cell(), h.occupants()
+        words[..., 0] = word(E.AIR)
+        words[..., 1] = np.float32(20.0).view(np.uint32)
+        cell[..., 2] = 0.0
+        for (x, y), (vx, vy) in {(9, 7): (0.25, -0.5), (3, 3): (-1.5, 2.0)}.items():
+            occ[y, x] = abi.occupant_word(2, vx, vy)
+            cell[y, x, 2] = -1.0                     # classified an obstacle
+            for (wx, wy) in ((x - 1, y), (x, y - 1)):
+                words[wy, wx, 0] = word(E.WATER)
+                cell[wy, wx, 2] = 1.0
+        h.run(PASS_SET_FACES, occupants=occ_on)
+        U, V = h.ufaces(), h.vfaces()
+        for (x, y), (vx, vy) in {(9, 7): (0.25, -0.5), (3, 3): (-1.5, 2.0)}.items():
+            want = (vx, vy) if occ_on else (0.0, 0.0)
+            assert (U[y, x, 0], U[y, x, 2]) == (np.float32(want[0]), 1.0), (occ_on, x, y, U[y, x])
+            assert (V[y, x, 0], V[y, x, 2]) == (np.float32(want[1]), 1.0), (occ_on, x, y, V[y, x])
+    # raster and its claims
+    kinds = E.kinds().copy()
+    STEAM = 6
+    kinds[STEAM] = E.KIND_GAS
+    for occ_on in (1, 0):
+        h = Hand(16, 16, L=L, kinds=kinds)
+        h.activate()
+        words, cell, cnt, occ = h.words(), h.cell(), h.cnt(), h.occupants()
+        words[..., 0] = word(E.AIR)
+        words[..., 1] = np.float32(20.0).view(np.uint32)
+        cnt[..., 8] = 0
+        cnt[..., G_DEST] = 0
+        # (5, 5): water with someone in it, its particles there
+        words[5, 5] = (word(E.WATER), np.float32(150.0).view(np.uint32), 0x3C003C00, 7)
+        occ[5, 5] = abi.occupant_word(0, 0.0, 0.0)
+        cntv, motion, state, _ = h.particles(abi.PART_PARTICLES0)
+        cnt[5, 5, G_SORT], cnt[5, 5, G_START] = 2, 0
+        for k in range(2):
+            motion[k] = (5.5, 5.5, 0.0, 0.0)
+            state[k] = (word(E.WATER), np.float32(150.0).view(np.uint32), 10 + k, 0)
+        cntv[0] = 2
+        # (7, 11): air with someone in it and particles come in; water
+        # vacated above it (a claim's target without the person)
+        occ[11, 7] = abi.occupant_word(1, 0.0, 0.0)
+        cnt[11, 7, G_SORT] = 1
+        words[10, 7, 0] = word(E.WATER)
+        # (12, 3): steam liquid comes into, every air cell round it occupied
+        words[3, 12, 0] = word(STEAM)
+        cnt[3, 12, G_SORT] = 1
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                if dx or dy:
+                    occ[3 + dy, 12 + dx] = abi.occupant_word(3, 0.0, 0.0)
+        h.run(PASS_RASTER_CLAIMS, occupants=occ_on)
+        if occ_on:
+            assert cnt[11, 7, 8] == NONE_WORD and cnt[3, 12, 8] == NONE_WORD, (cnt[11, 7, 8], cnt[3, 12, 8])
+        else:
+            assert cnt[11, 7, 8] == 10 * 16 + 7, cnt[11, 7, 8]
+            assert cnt[3, 12, 8] != NONE_WORD
+        cnt[11, 7, G_SORT] = 0
+        cnt[3, 12, G_SORT] = 0
+        cnt[..., 8] = NONE_WORD
+        h.run(PASS_RASTER, occupants=occ_on)
+        if occ_on:
+            assert tuple(words[5, 5]) == (E.AIR, np.float32(100.0).view(np.uint32), 0, 0), words[5, 5]
+            assert tuple(h.gas()[5, 5]) == (0, 0) and cnt[5, 5, G_DEST] == DEST_KEEP
+        else:
+            assert words[5, 5, 0] & 0xFF == E.WATER and cnt[5, 5, G_DEST] != DEST_KEEP
+
+
 PINS = {
     "restrict": pin_restrict,
     "smooth": pin_smooth,
@@ -1042,4 +1127,5 @@ PINS = {
     "air_push": pin_air_push,
     "surface_drag": pin_surface_drag,
     "push_gas_out": pin_push_gas_out,
+    "occupants": pin_occupants,
 }

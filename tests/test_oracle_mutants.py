@@ -41,6 +41,8 @@ CAUGHT_BY = {
     25: "air_push",                # the air's pressure pushes up its gradient
     26: "surface_drag",            # the drag's push on the air without the mass ratio
     27: "push_gas_out",            # displaced gas's vapour lost
+    28: "occupants",               # the person's y velocity decoded from the x field
+    29: "occupants",               # liquid left drawn under a person
 }
 
 

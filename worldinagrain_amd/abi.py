@@ -14,11 +14,14 @@ WIG_STORE_F32, WIG_STORE_F16, WIG_STORE_BF16 = 0, 1, 2
 WIG_MAX_CYCLES = 256
 
 PART_CELLS, PART_PARTICLES0, PART_PARTICLES1, PART_COUNTERS, PART_FIELD, PART_BLOCKS, PART_FLUIDCG, PART_SOLVE = range(8)
-PART_AIR, PART_AIR_SOURCE = 8, 9
-PART_COUNT = 10
-LIQUID_PARTS = 8          # the liquid's own parts (a snapshot's); the air's two are the caller's air step's
+PART_AIR, PART_AIR_SOURCE, PART_OCCUPANTS = 8, 9, 10
+PART_COUNT = 11
+LIQUID_PARTS = 8          # the liquid's own parts (a snapshot's); the rest are the caller's (the air's, the people's)
 PART_NAMES = ("cells", "particles0", "particles1", "counters", "field", "blocks", "fluidCg", "solve", "air0",
-              "airSource")
+              "airSource", "occupants")
+# occupantWord (common.wgsl:238-254): slot + 1 in the low byte, velocity x 128
+# in two 12-bit fields (x at bit 8, y at bit 20)
+OCCUPANT_SCALE = 128.0
 # airSource's words an air cell (airsource.wgsl:18-25), each an exact sum of two words
 SOURCE_WORDS = 16
 SOURCE_RIGHT, SOURCE_DOWN, SOURCE_AMOUNT, SOURCE_VAPOUR, SOURCE_MIST, SOURCE_LOST, SOURCE_HEAT = range(7)
@@ -47,7 +50,7 @@ class WigStepCtl(ct.Structure):
         ("pre_relax", ct.c_uint32), ("post_relax", ct.c_uint32), ("sweep_pairs", ct.c_uint32),
         ("coarse_sweeps", ct.c_uint32), ("first_chunk", ct.c_uint32), ("chunk", ct.c_uint32),
         ("level_pairs", ct.c_uint32), ("vcycle_storage", ct.c_uint32), ("residual_every", ct.c_uint32),
-        ("air", ct.c_uint32), ("surface_drag", ct.c_float),
+        ("air", ct.c_uint32), ("surface_drag", ct.c_float), ("occupants", ct.c_uint32),
     ]
 
 
@@ -55,7 +58,7 @@ class WigLayoutInfo(ct.Structure):
     _fields_ = [
         ("width", ct.c_uint32), ("height", ct.c_uint32), ("cells", ct.c_uint32), ("faces", ct.c_uint32),
         ("blocks", ct.c_uint32), ("blocks_wide", ct.c_uint32), ("capacity", ct.c_uint32), ("levels", ct.c_uint32),
-        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 10),
+        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 11),
     ]
 
 
@@ -79,3 +82,11 @@ assert ct.sizeof(WigSolveStats) == 64
 class WigRunTotals(ct.Structure):
     _fields_ = [("steps", ct.c_uint64), ("cycles", ct.c_uint64), ("particles", ct.c_uint64),
                 ("active_cells", ct.c_uint64)]
+
+
+def occupant_word(slot, vx, vy):
+    """common.wgsl's occupantWord (:240-243): a person's slot and velocity
+    (cells a step) as the occupants part's word."""
+    import numpy as np
+    q = np.clip(np.rint(np.array([vx, vy], dtype=np.float32) * np.float32(OCCUPANT_SCALE)), -2047.0, 2047.0).astype(np.int32)
+    return int((slot + 1) | ((int(q[0]) & 0xFFF) << 8) | ((int(q[1]) & 0xFFF) << 20))

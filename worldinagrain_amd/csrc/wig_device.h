@@ -200,8 +200,9 @@ struct Dev {
     // the air (WigStepCtl.air): air[0] and airSource, the wind's drag
     // coefficient, and the step's dt and dx (the pressure's push)
     const float4 *air;
+    const uint32_t *occ;         // occupants (WIG_PART_OCCUPANTS), read when occ_on
     uint32_t *source;
-    uint32_t air_on;
+    uint32_t air_on, occ_on;
     float surface_drag, dt, dx;
 };
 

@@ -152,7 +152,18 @@ __device__ float surfaceDrag(const Dev &d, i2 a, i2 b, bool isU, float4 ga, floa
     }
     return du;
 }
+// people's bodies (common.wgsl:238-254): cell i's occupant word, 0 for
+// nobody or with occupants off
+#define OCCUPANT_SCALE 128.0f
+__device__ __forceinline__ uint32_t occupantAt(const Dev &d, uint32_t i) { return d.occ_on ? d.occ[i] : 0u; }
+__device__ __forceinline__ bool isOccupant(uint32_t w) { return (w & 0xffu) != 0u; }
+__device__ __forceinline__ v2 occupantVel(uint32_t w) {
+    // (the 12-bit fields sign-extended by shifting them to the top first)
+    const int32_t x = (int32_t)(w << 12u) >> 20, y = (int32_t)w >> 20;
+    return V2((float)x / OCCUPANT_SCALE, (float)y / OCCUPANT_SCALE);
+}
 __device__ bool isObstacleCell(const Dev &d, i2 c, uint32_t vx) {       // :1123-1147
+    if (inWorld(d, c) && isOccupant(occupantAt(d, cellIndex(d, c)))) return true;
     uint32_t t = wType(vx), k = kindOf(d, t);
     if (k == KIND_GAS) {
         int wet = 0;
@@ -186,12 +197,13 @@ __device__ v2 liquidIn(const Dev &d, uint32_t i) {                         // :1
 __device__ __forceinline__ bool isObstaclePos(const Dev &d, v2 pos) {     // :424-431
     i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
     if (gridZ(d, c) < 0.0f) return true;
-    // (outside the world gridZ is -1: the cell word below is never read there)
+    // (outside the world gridZ is -1: the words below are never read there)
+    if (isOccupant(occupantAt(d, cellIndex(d, c)))) return true;
     uint32_t k = kindOf(d, wType(d.cells[4ull * cellIndex(d, c)]));
     return k == KIND_SOLID || k == KIND_POWDER;
 }
 __device__ __forceinline__ bool walls(const Dev &d, i2 c) {               // :1056-1062
-    if (!inWorld(d, c)) return true;
+    if (!inWorld(d, c) || isOccupant(occupantAt(d, cellIndex(d, c)))) return true;
     uint32_t k = kindOf(d, wType(d.cells[4ull * cellIndex(d, c)]));
     return k == KIND_SOLID || k == KIND_POWDER;
 }
@@ -693,15 +705,17 @@ __device__ __forceinline__ void splatAxis(const Dev &d, const GatherTile &T, v2 
 }
 // splatDensity's four cells, b .. b + 1 from its corner b = floor(pos - 0.5),
 // and their first words (0 off the world), loaded when the particle's are
-struct DensityCells { i2 b; uint32_t w[4]; };
+struct DensityCells { i2 b; uint32_t w[4], occ[4]; };
 __device__ __forceinline__ DensityCells densityCells(const Dev &d, v2 pos) {
     DensityCells q;
     q.b = I2(wi32(floorf(pos.x - 0.5f)), wi32(floorf(pos.y - 0.5f)));
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const i2 c = I2(q.b.x + (j & 1), q.b.y + (j >> 1));
-        const uint32_t w = d.cells[4ull * cellIndex(d, clampTo(c, d.W, d.H))];
+        const uint32_t ci = cellIndex(d, clampTo(c, d.W, d.H));
+        const uint32_t w = d.cells[4ull * ci], o = occupantAt(d, ci);
         q.w[j] = inWorld(d, c) ? w : 0u;
+        q.occ[j] = inWorld(d, c) ? o : 0u;
     }
     return q;
 }
@@ -712,12 +726,13 @@ __device__ __forceinline__ void splatDensity(const Dev &d, const GatherTile &T, 
     float fx0 = floorf(s.x), fy0 = floorf(s.y);
     i2 b = I2(wi32(fx0), wi32(fy0));
     v2 f = V2(s.x - fx0, s.y - fy0);
-    // walls() (:1056-1062) of the four cells: off the world, solid or powder
+    // walls() (:1056-1062) of the four cells: off the world, someone's
+    // body, solid or powder
     bool wall[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const uint32_t k = ldsGetU8(&kinds[wType(q.w[j]) & 0xffu]);
-        wall[j] = !inWorld(d, I2(b.x + (j & 1), b.y + (j >> 1))) || k == KIND_SOLID || k == KIND_POWDER;
+        wall[j] = !inWorld(d, I2(b.x + (j & 1), b.y + (j >> 1))) || isOccupant(q.occ[j]) || k == KIND_SOLID || k == KIND_POWDER;
     }
     for (int j = 0; j < 4; j++) {
         int ox = j & 1, oy = j >> 1;
@@ -1073,6 +1088,11 @@ __global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
 // (:3444-3446)
 __device__ __forceinline__ float obstacleVel(const Dev &d, i2 c, int axis) {
     if (!inWorld(d, c)) return 0.0f;
+    const uint32_t held = occupantAt(d, cellIndex(d, c));
+    if (isOccupant(held)) {
+        const v2 u = occupantVel(held);
+        return axis == 0 ? u.x : u.y;
+    }
     uint4 v = *(const uint4 *)&d.cells[4ull * cellIndex(d, c)];
     uint32_t k = kindOf(d, wType(v.x));
     if (k == KIND_SOLID || k == KIND_GAS) return 0.0f;
@@ -3320,7 +3340,7 @@ __device__ uint32_t airTo(const Dev &d, i2 c, bool gas) {                    // 
         i2 at = I2(c.x + o.x, c.y + o.y);
         if (!inWorld(d, at)) continue;
         uint32_t j = cellIndex(d, at);
-        if (vacated(d, j) || (gas && wType(d.cells[4ull * j]) == E_AIR && CNT(d, j, G_SORT) == 0u)) return j;
+        if (vacated(d, j) || (gas && wType(d.cells[4ull * j]) == E_AIR && CNT(d, j, G_SORT) == 0u && !isOccupant(occupantAt(d, j)))) return j;
     }
     return WNONE;
 }
@@ -3332,7 +3352,7 @@ __global__ void __launch_bounds__(64) k_rasterClaims(WorldGrid g) {             
         if (!inWorld(d, c)) continue;
         uint32_t i = cellIndex(d, c), to = WNONE;
         uint32_t t = wType(d.cells[4ull * i]);
-        if (CNT(d, i, G_SORT) > 0u) {
+        if (CNT(d, i, G_SORT) > 0u && !isOccupant(occupantAt(d, i))) {
             if (t == E_AIR) to = airTo(d, c, false);
             else if (kindOf(d, t) == KIND_GAS && gridZ(d, c) >= 0.0f) to = airTo(d, c, true);
         }
@@ -3370,6 +3390,16 @@ __global__ void __launch_bounds__(64) k_raster(WorldGrid g) {                   
         uint32_t start = CNT(d, i, G_START);
         uint4 v = *(const uint4 *)&d.cells[4ull * i];
         uint32_t t = wType(v.x);
+        // a person's body: not drawn, its particles left to find their way
+        // out, and liquid drawn there before the person came turned air
+        if (isOccupant(occupantAt(d, i))) {
+            if (kindOf(d, t) == KIND_LIQUID) {
+                *(uint4 *)&d.cells[4ull * i] = make_uint4(E_AIR, ubits(effTemp(fbits(v.y))), 0u, 0u);
+                setGasPair(d, i, 0u, 0u);
+            }
+            CNT(d, i, G_DEST) = DEST_KEEP;
+            continue;
+        }
         bool drawn = false;
         if (kindOf(d, t) == KIND_LIQUID) {
             drawn = true;

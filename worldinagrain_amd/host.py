@@ -103,6 +103,9 @@ class Simulation:
         # the air coupled in (WigStepCtl.air): the world's air part read, its
         # airSource part added to; off, the air is at rest
         self.air = False
+        # people's bodies (WigStepCtl.occupants): the world's occupants part
+        # read as moving obstacles; off, nobody is there
+        self.occupants = False
 
     # Simulation.seed / random (js/sim.js:1386-1397)
     def seed(self, n):
@@ -139,6 +142,7 @@ class Simulation:
         c.residual_every = s.residual_every
         c.air = 1 if self.air else 0
         c.surface_drag = surface_drag_coefficient(self.dx)
+        c.occupants = 1 if self.occupants else 0
         if s.first_chunk < 0:
             c.first_chunk = 0xFFFFFFFF          # WIG_CHUNK_AUTO: the world's last solve's count
             c.chunk = s.chunk
@@ -218,6 +222,7 @@ class BatchHost:
             c.residual_every = sv.residual_every
             c.air = 1 if s.air else 0
             c.surface_drag = surface_drag_coefficient(s.dx)
+            c.occupants = 1 if s.occupants else 0
             if sv.first_chunk < 0:
                 c.first_chunk, c.chunk = 0xFFFFFFFF, sv.chunk
             else:

@@ -15,7 +15,7 @@ const LIQUID_TOLERANCE = 1e-2;  // fluid.wgsl:2221
 const GRAVITY = 9.81 / 3600 / 0.01;   // js/elements.js:30, water's weight
 
 const PART = { cells: 0, particles0: 1, particles1: 2, counters: 3, field: 4, blocks: 5, fluidCg: 6, solve: 7, air0: 8,
-    airSource: 9 };
+    airSource: 9, occupants: 10 };
 const STAT = { ACTIVE: 0, RETIRED: 1, VISCOUS: 2, PARTICLES: 3, RESIDUAL: 4, CYCLES: 7, MOST_CYCLES: 8 };
 const TOL_ABS_MAX = 0;
 const TOL_REL_L2 = 1;
@@ -56,6 +56,7 @@ class LiquidSimulation {
         this.dt = options.dt === undefined ? 1 / 60 : options.dt;
         this.frameNo = options.frame === undefined ? 1 : options.frame;
         this.air = !!options.air;      // the air coupled in (air0 read, airSource added to)
+        this.occupants = !!options.occupants;   // people's bodies (the occupants part) as moving obstacles
         const s = options.solver || {};
         this.solver = {
             tolMode: s.tolMode === undefined ? TOL_ABS_MAX : s.tolMode,
@@ -114,7 +115,7 @@ class LiquidSimulation {
     // relaxation's starting tiling (flipped each step), the solve's settings.
     stepControl() {
         this.shiftedFirst = !this.shiftedFirst;
-        const buf = new ArrayBuffer(76);
+        const buf = new ArrayBuffer(80);
         const u = new Uint32Array(buf);
         const f = new Float32Array(buf);
         u[0] = this.fluidStats.active ? 1 : 0;
@@ -131,6 +132,7 @@ class LiquidSimulation {
         u[16] = this.solver.residualEvery || 0;
         u[17] = this.air ? 1 : 0;
         f[18] = this.air ? surfaceDragCoefficient(this.metresPerCell) : 0;
+        u[19] = this.occupants ? 1 : 0;
         return buf;
     }
 
