@@ -26,6 +26,7 @@ extern "C" {
 
 #define WIG_OK 0
 #define WIG_WARN_OVERFLOW 1       /* more particles than capacity: those past it lost */
+#define WIG_NOT_READY 2           /* wig_report_take without waiting: that report has not landed yet */
 #define WIG_E_INVALID (-1)        /* bad argument, size or unsupported control */
 #define WIG_E_NOMEM (-2)          /* device allocation failed */
 #define WIG_E_DEVICE (-3)         /* a HIP call failed (device lost, launch error) */
@@ -207,6 +208,17 @@ int wig_fluid_step(WigWorld *w, const WigParams *params, const WigStepCtl *ctl);
  * MOST_CYCLES cleared as a report on steps does.  Waits for the stream. */
 int wig_report(WigWorld *w, uint32_t stats[WIG_STATS]);
 
+/* The same report asked for without waiting, as the reference maps its
+ * report buffer asynchronously and takes it in when it lands
+ * (Simulation.readReport's mapAsync, js/sim.js:1882-1901): the words are
+ * copied behind the work already enqueued into one of two pinned host
+ * slots, and *ticket names them.  wig_report_take copies them out: with
+ * `wait` once they have landed, else at once or WIG_NOT_READY.  The two
+ * latest tickets are valid (a third reuses the first's slot); older ones,
+ * and a batch's tickets from before wig_batch_assign, are WIG_E_INVALID. */
+int wig_report_async(WigWorld *w, uint64_t *ticket);
+int wig_report_take(WigWorld *w, uint64_t ticket, uint32_t stats[WIG_STATS], int wait);
+
 /* The last solve's FP64 scalars.  Waits for the stream. */
 int wig_solve_stats(WigWorld *w, WigSolveStats *out);
 
@@ -290,6 +302,9 @@ int wig_batch_set_stream(WigBatch *b, void *stream);
 int wig_batch_step(WigBatch *b, const WigParams *params, const WigStepCtl *ctl);
 /* stats[i * WIG_STATS ...]: each world's report (as wig_report).  Waits. */
 int wig_batch_report(WigBatch *b, uint32_t *stats);
+/* The batch's reports without waiting, as wig_report_async / wig_report_take. */
+int wig_batch_report_async(WigBatch *b, uint64_t *ticket);
+int wig_batch_report_take(WigBatch *b, uint64_t ticket, uint32_t *stats, int wait);
 int wig_batch_sync(WigBatch *b);
 /* hipGraphs: with `on`, each phase of a step (the passes before the solve,
  * its start, one iteration, its end, the rest) is captured once into a

@@ -47,6 +47,35 @@ def test_js_host_matches_python_host(tmp_path):
         assert hashlib.sha256(w.download(part).tobytes()).hexdigest() == js["digest"][name], name
 
 
+@pytest.mark.gpu
+@pytest.mark.skipif(NODE is None, reason="no node")
+def test_js_late_reports_match_python_host(tmp_path):
+    # each report taken a step late (the reference's mapAsync), adaptive
+    # cycles following the late reports on both hosts
+    from tests.helpers import make_gpu
+    from worldinagrain_amd.host import Simulation
+    W, H, steps = 48, 40, 8
+    cells = scenes.dam_break(W, H, block=(24, 28))
+    path = tmp_path / "cells.bin"
+    cells.tofile(path)
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "dam_steps.js"), str(W), str(H), str(steps), str(path),
+                        "late"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    w = make_gpu(W, H, cells)
+    sim = Simulation(w, W, H)
+    got = []
+    for s in range(steps):
+        sim.step(read_report="late")
+        if s > 0:
+            got.append(list(map(int, sim.last_stats)))
+    sim.take_report(w.report_take(sim.pending_report, wait=True))
+    got.append(list(map(int, sim.last_stats)))
+    assert got == js["stats"]
+    for name, part in (("cells", abi.PART_CELLS), ("counters", abi.PART_COUNTERS), ("field", abi.PART_FIELD)):
+        assert hashlib.sha256(w.download(part).tobytes()).hexdigest() == js["digest"][name], name
+
+
 SNAP_JS = os.path.join(ROOT, "tests", "js", "snapshot_check.js")
 
 

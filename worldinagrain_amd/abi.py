@@ -3,6 +3,7 @@ import ctypes as ct
 
 WIG_OK = 0
 WIG_WARN_OVERFLOW = 1
+WIG_NOT_READY = 2
 WIG_E_INVALID = -1
 WIG_E_NOMEM = -2
 WIG_E_DEVICE = -3

@@ -199,6 +199,15 @@ struct Batch {
     std::vector<uint64_t> sig;   // the worlds and tile sides the maps were built for
     uint32_t *stops = nullptr, *stops_host = nullptr;   // k_stops' words (device, pinned host)
     uint32_t *reports = nullptr;
+    // the reports asked for without waiting: two pinned slots of rep_cap
+    // worlds' words, the events that mark each landed, the worlds in each,
+    // the next ticket and the first still valid (earlier ones belong to the
+    // slots' last size or to the worlds before an assign)
+    uint32_t *rep_host = nullptr;
+    uint32_t rep_cap = 0, rep_n[2] = {};
+    hipEvent_t rep_ev[2] = {};
+    bool rep_made = false;
+    uint64_t rep_next = 0, rep_floor = 0;
     hipStream_t stream = nullptr;
     hipStream_t own = nullptr;   // wig_batch_create's: the stream when none is set (not a world's, which may go first)
     // hipGraphs of the step's phases (run_phase),
@@ -307,6 +316,9 @@ static void batch_free(Batch *b) {
     if (b->stops) (void)hipFree(b->stops);
     if (b->stops_host) (void)hipHostFree(b->stops_host);
     if (b->reports) (void)hipFree(b->reports);
+    if (b->rep_host) (void)hipHostFree(b->rep_host);
+    if (b->rep_made)
+        for (hipEvent_t &e : b->rep_ev) (void)hipEventDestroy(e);
     for (hipGraphExec_t &x : b->gx)
         if (x) (void)hipGraphExecDestroy(x);
     if (b->ev_made)
@@ -605,7 +617,8 @@ int wig_sync(WigWorld *w) {
     return WIG_OK;
 }
 
-static int batch_report(Batch &b, uint32_t *stats) {
+// the worlds' reports gathered into b.reports, behind the enqueued work
+static int enqueue_reports(Batch &b) {
     uint32_t n = (uint32_t)b.w.size();
     int st = batch_maps(b);
     if (st < 0) return st;
@@ -613,9 +626,74 @@ static int batch_report(Batch &b, uint32_t *stats) {
     HIPCHECK(hipMemcpyAsync(b.tab, b.htab.data(), sizeof(Dev) * n, hipMemcpyHostToDevice, b.stream));
     hipLaunchKernelGGL(k_reports, dim3(n), dim3(64), 0, b.stream, grid_of(b, SH_ONE), b.reports);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(stats, b.reports, 4ull * FLUID_STATS * n, hipMemcpyDeviceToHost, b.stream));
+    return WIG_OK;
+}
+
+static int batch_report(Batch &b, uint32_t *stats) {
+    int st = enqueue_reports(b);
+    if (st < 0) return st;
+    HIPCHECK(hipMemcpyAsync(stats, b.reports, 4ull * FLUID_STATS * b.w.size(), hipMemcpyDeviceToHost, b.stream));
     HIPCHECK(hipStreamSynchronize(b.stream));
     return WIG_OK;
+}
+
+static int batch_report_async(Batch &b, uint64_t *ticket) {
+    if (!ticket) return fail(WIG_E_INVALID, "null ticket");
+    const uint32_t n = (uint32_t)b.w.size();
+    if (!b.rep_made) {
+        for (hipEvent_t &e : b.rep_ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b.rep_made = true;
+    }
+    if (n > b.rep_cap) {
+        // (the slots may still be filling: both landed before they go; an
+        // event never recorded has nothing to wait for)
+        for (hipEvent_t &e : b.rep_ev) HIPCHECK(hipEventSynchronize(e));
+        uint32_t *h = nullptr;
+        if (hipHostMalloc((void **)&h, 2ull * 4ull * FLUID_STATS * n, hipHostMallocDefault) != hipSuccess)
+            return fail(WIG_E_NOMEM, "report slots: pinned allocation failed");
+        if (b.rep_host) HIPCHECK(hipHostFree(b.rep_host));
+        b.rep_host = h;
+        b.rep_cap = n;
+        b.rep_floor = b.rep_next;
+    }
+    int st = enqueue_reports(b);
+    if (st < 0) return st;
+    const uint32_t slot = (uint32_t)(b.rep_next & 1u);
+    HIPCHECK(hipMemcpyAsync(b.rep_host + (uint64_t)slot * FLUID_STATS * b.rep_cap, b.reports, 4ull * FLUID_STATS * n,
+                            hipMemcpyDeviceToHost, b.stream));
+    HIPCHECK(hipEventRecord(b.rep_ev[slot], b.stream));
+    b.rep_n[slot] = n;
+    *ticket = b.rep_next++;
+    return WIG_OK;
+}
+
+static int batch_report_take(Batch &b, uint64_t ticket, uint32_t *stats, int wait) {
+    if (!stats) return fail(WIG_E_INVALID, "null stats");
+    if (!b.rep_made || ticket < b.rep_floor || ticket >= b.rep_next || ticket + 2u < b.rep_next)
+        return fail(WIG_E_INVALID, "report %llu is not one of the last two asked for", (unsigned long long)ticket);
+    const uint32_t slot = (uint32_t)(ticket & 1u);
+    if (wait) {
+        HIPCHECK(hipEventSynchronize(b.rep_ev[slot]));
+    } else {
+        const hipError_t e = hipEventQuery(b.rep_ev[slot]);
+        if (e == hipErrorNotReady) {
+            (void)hipGetLastError();   // (not an error: kept from the next launch's check)
+            return WIG_NOT_READY;
+        }
+        HIPCHECK(e);
+    }
+    memcpy(stats, b.rep_host + (uint64_t)slot * FLUID_STATS * b.rep_cap, 4ull * FLUID_STATS * b.rep_n[slot]);
+    return WIG_OK;
+}
+
+int wig_report_async(WigWorld *w, uint64_t *ticket) {
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    return batch_report_async(*w->solo, ticket);
+}
+
+int wig_report_take(WigWorld *w, uint64_t ticket, uint32_t stats[WIG_STATS], int wait) {
+    if (!w) return fail(WIG_E_INVALID, "null world");
+    return batch_report_take(*w->solo, ticket, stats, wait);
 }
 
 int wig_report(WigWorld *w, uint32_t stats[WIG_STATS]) {
@@ -897,6 +975,7 @@ static int batch_assign(Batch *b, WigWorld *const *worlds, uint32_t n) {
     }
     b->w.assign(worlds, worlds + n);
     b->htab.resize(std::max<size_t>(b->htab.size(), n));
+    b->rep_floor = b->rep_next;   // (the last reports were of the worlds before)
     b->sig.clear();          // the maps are built again at the next step
     return WIG_OK;
 }
@@ -1089,6 +1168,16 @@ int wig_batch_step(WigBatch *b, const WigParams *params, const WigStepCtl *ctl) 
 int wig_batch_report(WigBatch *b, uint32_t *stats) {
     if (!b || !stats) return fail(WIG_E_INVALID, "null argument");
     return batch_report(*(Batch *)b, stats);
+}
+
+int wig_batch_report_async(WigBatch *b, uint64_t *ticket) {
+    if (!b) return fail(WIG_E_INVALID, "null batch");
+    return batch_report_async(*(Batch *)b, ticket);
+}
+
+int wig_batch_report_take(WigBatch *b, uint64_t ticket, uint32_t *stats, int wait) {
+    if (!b) return fail(WIG_E_INVALID, "null batch");
+    return batch_report_take(*(Batch *)b, ticket, stats, wait);
 }
 
 int wig_batch_sync(WigBatch *b) {

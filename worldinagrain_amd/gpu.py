@@ -35,6 +35,8 @@ def lib():
         L.wig_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
         L.wig_fluid_step.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
         L.wig_report.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint32)]
+        L.wig_report_async.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint64)]
+        L.wig_report_take.argtypes = [ct.c_void_p, ct.c_uint64, ct.POINTER(ct.c_uint32), ct.c_int]
         L.wig_solve_stats.argtypes = [ct.c_void_p, ct.POINTER(abi.WigSolveStats)]
         L.wig_sync.argtypes = [ct.c_void_p]
         L.wig_last_error.restype = ct.c_char_p
@@ -55,6 +57,8 @@ def lib():
         L.wig_batch_set_stream.argtypes = [ct.c_void_p, ct.c_void_p]
         L.wig_batch_step.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
         L.wig_batch_report.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint32)]
+        L.wig_batch_report_async.argtypes = [ct.c_void_p, ct.POINTER(ct.c_uint64)]
+        L.wig_batch_report_take.argtypes = [ct.c_void_p, ct.c_uint64, ct.POINTER(ct.c_uint32), ct.c_int]
         L.wig_batch_sync.argtypes = [ct.c_void_p]
         L.wig_batch_time_kernel.argtypes = [ct.c_void_p, ct.c_int]
         L.wig_set_graphs.argtypes = [ct.c_void_p, ct.c_int]
@@ -163,6 +167,21 @@ class GpuWorld:
     def report(self):
         stats = (ct.c_uint32 * abi.STATS)()
         _check(lib().wig_report(self._w, stats))
+        return np.array(stats, dtype=np.uint32)
+
+    def report_async(self):
+        """The report asked for without waiting (wig_report_async): its ticket."""
+        t = ct.c_uint64()
+        _check(lib().wig_report_async(self._w, ct.byref(t)))
+        return t.value
+
+    def report_take(self, ticket, wait=True):
+        """The report of `ticket` (wig_report_take); None if it has not landed and not `wait`."""
+        stats = (ct.c_uint32 * abi.STATS)()
+        st = lib().wig_report_take(self._w, ct.c_uint64(ticket), stats, 1 if wait else 0)
+        if st == abi.WIG_NOT_READY:
+            return None
+        _check(st)
         return np.array(stats, dtype=np.uint32)
 
     def solve_stats(self):
@@ -297,6 +316,20 @@ class GpuBatch:
         n = len(self.worlds)
         out = (ct.c_uint32 * (abi.STATS * n))()
         _check(lib().wig_batch_report(self._b, out))
+        return np.array(out, dtype=np.uint32).reshape(n, abi.STATS)
+
+    def report_async(self):
+        t = ct.c_uint64()
+        _check(lib().wig_batch_report_async(self._b, ct.byref(t)))
+        return t.value
+
+    def report_take(self, ticket, wait=True):
+        n = len(self.worlds)
+        out = (ct.c_uint32 * (abi.STATS * n))()
+        st = lib().wig_batch_report_take(self._b, ct.c_uint64(ticket), out, 1 if wait else 0)
+        if st == abi.WIG_NOT_READY:
+            return None
+        _check(st)
         return np.array(out, dtype=np.uint32).reshape(n, abi.STATS)
 
     def sync(self):

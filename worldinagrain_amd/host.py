@@ -151,12 +151,20 @@ class Simulation:
         return c
 
     def step(self, read_report=True):
-        """One step's liquid (encodeFluid), then its report taken in."""
+        """One step's liquid (encodeFluid), then its report taken in.
+        read_report "late": the report asked for without waiting and the
+        last step's taken in instead, once landed, as the reference takes a
+        report in when its mapAsync resolves (js/sim.js:1882-1901): the
+        host never waits for the step it has just enqueued."""
         status = self.world.step(self.params(), self.ctl())
         if status < 0:
             raise RuntimeError(f"fluid step failed ({status})")
         self.frame_no += 1
-        if read_report:
+        if read_report == "late":
+            last, self.pending_report = getattr(self, "pending_report", None), self.world.report_async()
+            if last is not None:
+                self.take_report(self.world.report_take(last, wait=True))
+        elif read_report:
             self.take_report(self.world.report())
         return status
 

@@ -143,8 +143,20 @@ class LiquidSimulation {
         return status;
     }
 
+    // The report asked for now and taken in when it lands (readReport's
+    // mapAsync, js/sim.js:1882-1901): the last step's, so the host never
+    // waits for the step it has just enqueued.
+    readReportLate() {
+        const last = this.pendingReport;
+        this.pendingReport = addon.reportAsync(this.world);
+        return last === undefined ? null : this.takeReport(addon.reportTake(this.world, last, true));
+    }
+
     readReport() {
-        const u = addon.report(this.world);
+        return this.takeReport(addon.report(this.world));
+    }
+
+    takeReport(u) {
         this.lastStats = u;
         this.fluidStats = { active: u[STAT.ACTIVE] + u[STAT.PARTICLES], viscous: u[STAT.VISCOUS] };
         const most = u[STAT.MOST_CYCLES];

@@ -196,6 +196,45 @@ static napi_value js_report(napi_env env, napi_callback_info info) {
     return ta;
 }
 
+/* reportAsync(h) -> ticket: the report asked for without waiting (wig_report_async) */
+static napi_value js_report_async(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value argv[1], r;
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    WigWorld *w = world_of(env, argv[0]);
+    if (!w) return NULL;
+    uint64_t ticket = 0;
+    int st = wig_report_async(w, &ticket);
+    if (st < 0) return throw_wig(env, st);
+    CALL(env, napi_create_double(env, (double)ticket, &r));
+    return r;
+}
+
+/* reportTake(h, ticket, wait) -> Uint32Array(9), or null if it has not landed
+ * and wait is false (wig_report_take) */
+static napi_value js_report_take(napi_env env, napi_callback_info info) {
+    size_t argc = 3;
+    napi_value argv[3], ab, ta;
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    WigWorld *w = world_of(env, argv[0]);
+    if (!w) return NULL;
+    double ticket = 0.0;
+    bool wait = true;
+    CALL(env, napi_get_value_double(env, argv[1], &ticket));
+    if (argc > 2) CALL(env, napi_get_value_bool(env, argv[2], &wait));
+    void *data;
+    CALL(env, napi_create_arraybuffer(env, 4 * WIG_STATS, &data, &ab));
+    int st = wig_report_take(w, (uint64_t)ticket, (uint32_t *)data, wait ? 1 : 0);
+    if (st == WIG_NOT_READY) {
+        napi_value nul;
+        CALL(env, napi_get_null(env, &nul));
+        return nul;
+    }
+    if (st < 0) return throw_wig(env, st);
+    CALL(env, napi_create_typedarray(env, napi_uint32_array, WIG_STATS, ab, 0, &ta));
+    return ta;
+}
+
 /* solveStats(h) -> Float64Array(6) [rz, alpha, beta, pq, rr, bb] plus .cycles */
 static napi_value js_solve_stats(napi_env env, napi_callback_info info) {
     size_t argc = 1;
@@ -359,6 +398,8 @@ static napi_value init(napi_env env, napi_value exports) {
         {"download", NULL, js_download, NULL, NULL, NULL, napi_default, NULL},
         {"fluidStep", NULL, js_fluid_step, NULL, NULL, NULL, napi_default, NULL},
         {"report", NULL, js_report, NULL, NULL, NULL, napi_default, NULL},
+        {"reportAsync", NULL, js_report_async, NULL, NULL, NULL, napi_default, NULL},
+        {"reportTake", NULL, js_report_take, NULL, NULL, NULL, napi_default, NULL},
         {"solveStats", NULL, js_solve_stats, NULL, NULL, NULL, napi_default, NULL},
         {"sync", NULL, js_sync, NULL, NULL, NULL, napi_default, NULL},
         {"deviceOk", NULL, js_device_ok, NULL, NULL, NULL, napi_default, NULL},
