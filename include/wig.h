@@ -104,6 +104,12 @@ typedef struct WigStepCtl {
                                 :1820-1827), no gas is claimed into one (:3919, :3937), and raster leaves
                                 its particles undrawn and its liquid turned air (:3995-4004).  0: no
                                 one (the part is not read) */
+    uint32_t gravity;        /* 1: the gravity pass's field, WIG_PART_GRAVITY, gives the pull and down
+                                (pull.wgsl:13-44, read through fluid.wgsl's binding 10): the faces'
+                                pull (setFaces, :1968-1972), the slip and the falling test (advect,
+                                :3613,3686), and down for the bubble test, the escape and where
+                                displaced gas goes (:1138, :3577, :3822-3825, :3912).  0: Earth's pull
+                                alone, (0, earthG), down (0, 1) (the part is not read) */
 } WigStepCtl;
 
 /* V-cycle storage: float in the reference's buffers (results bit-identical
@@ -140,7 +146,12 @@ typedef struct WigStepCtl {
                                   else the person's slot + 1 in the low byte and their velocity times 128
                                   (cells a step) in two 12-bit two's-complement fields, x at bit 8 and y at
                                   bit 20; read */
-#define WIG_PART_COUNT 11
+#define WIG_PART_GRAVITY 11    /* the gravity pass's field (a texture_2d<f32> in the reference): a vec4<f32>
+                                  a light cell of 2 x 2 cells (GRAVITY_BLOCK), ceil(W/2) x ceil(H/2) of them,
+                                  row-major: the masses' pull (x, y; cells a step squared), their potential
+                                  (z), and down (w: the step index, 0..7, plus 8 where the pull is too weak
+                                  to settle anything) (pull.wgsl:1-44); read */
+#define WIG_PART_COUNT 12
 
 /* FLUID_STATS words (js/shared.js:271-272). */
 #define WIG_STAT_ACTIVE 0

@@ -72,7 +72,8 @@ def lib(path=None):
     return _libs[path]
 
 
-_DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8, abi.PART_AIR: np.float32}
+_DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8, abi.PART_AIR: np.float32,
+           abi.PART_GRAVITY: np.float32}
 
 
 class OracleWorld:

@@ -10,7 +10,8 @@
  *
  * Scope (the inputs the reference's other phases supply, fixed here):
  *   - no free or held bodies, no people (occupants all zero);
- *   - gravity uniform: pullAt = (0, earthG), down = (0, 1) (pull.wgsl:32-44);
+ *   - gravity: with ctl->gravity the gravity pass's field (pull.wgsl:13-44),
+ *     else Earth's pull alone, pullAt = (0, earthG), down = (0, 1);
  *   - the air grid at rest at uniform pressure: the pressure push on surface
  *     particles is zero and the wind's surface drag is left out
  *     (airsample.wgsl:11-14; fluid.wgsl:1914-1932);
@@ -198,6 +199,7 @@ typedef struct OWorld {
     float *air;                  /* air[0]: a vec4 an air cell */
     uint32_t *source;            /* airSource: SOURCE_WORDS an air cell */
     uint32_t *occ;               /* occupants: occupantWord a cell */
+    float *grav;                 /* the gravity pass's field: a vec4 a light cell */
     uint8_t kind[256];
     int dot_mode;
     /* the step being run */
@@ -406,6 +408,7 @@ static void layout(OWorld *w, uint32_t width, uint32_t height, uint32_t coarse) 
     w->bytes[WIG_PART_AIR] = 16ull * w->C;
     w->bytes[WIG_PART_AIR_SOURCE] = 4ull * SOURCE_WORDS * w->C;
     w->bytes[WIG_PART_OCCUPANTS] = 4ull * w->C;
+    w->bytes[WIG_PART_GRAVITY] = 16ull * (uint64_t)((w->W + 1) / 2) * (uint64_t)((w->H + 1) / 2);
 }
 
 /* ------------------------------------------------------------ address */
@@ -531,6 +534,47 @@ static inline v2 occupantVel(uint32_t o) {
     if (MUTANT(28)) y = (int32_t)(o << 12u) >> 20;
     return V2((float)x / OCCUPANT_SCALE, (float)y / OCCUPANT_SCALE);
 }
+/* The pull and down (pull.wgsl:13-44) from the gravity pass's field, a vec4
+ * a light cell of GRAVITY_BLOCK x GRAVITY_BLOCK cells: the masses' pull
+ * (xy), their potential (z), down (w).  Off, Earth's pull alone and down
+ * (0, 1): the field's texel for that, no masses' pull and down step 2. */
+#define GRAVITY_BLOCK 2
+static void gravityTexel(const OWorld *w, i2 c, float t[4]) {
+    if (!w->ctl->gravity) { t[0] = t[1] = t[2] = 0.0f; t[3] = 2.0f; return; }
+    int32_t gw = (w->W + 1) / 2, gh = (w->H + 1) / 2;
+    int32_t x = c.x < 0 ? 0 : (c.x > gw - 1 ? gw - 1 : c.x), y = c.y < 0 ? 0 : (c.y > gh - 1 ? gh - 1 : c.y);
+    memcpy(t, &w->grav[4ull * ((uint64_t)y * (uint64_t)gw + (uint64_t)x)], 16);
+}
+static v2 fieldPull(const OWorld *w, v2 pos) {                /* fieldAt's xy (pull.wgsl:19-30) */
+    float qx = pos.x / (float)GRAVITY_BLOCK - 0.5f, qy = pos.y / (float)GRAVITY_BLOCK - 0.5f;
+    float fx0 = floorf(qx), fy0 = floorf(qy);
+    i2 i = I2(wi32(fx0), wi32(fy0));
+    float fx = qx - fx0, fy = qy - fy0;
+    float a[4], b[4], c[4], e[4];
+    gravityTexel(w, i, a);
+    gravityTexel(w, I2(i.x + 1, i.y), b);
+    gravityTexel(w, I2(i.x, i.y + 1), c);
+    gravityTexel(w, I2(i.x + 1, i.y + 1), e);
+    if (MUTANT(30)) fx = 1.0f - fx;
+    v2 top = V2(wmix(a[0], b[0], fx), wmix(a[1], b[1], fx)), bottom = V2(wmix(c[0], e[0], fx), wmix(c[1], e[1], fx));
+    return V2(wmix(top.x, bottom.x, fy), wmix(top.y, bottom.y, fy));
+}
+static v2 pullAt(const OWorld *w, v2 pos) {                   /* pull.wgsl:32-34 */
+    if (!w->ctl->gravity) return V2(0.0f, w->P->earthG);
+    v2 f = fieldPull(w, pos);
+    return V2(0.0f + f.x, w->P->earthG + f.y);
+}
+static i2 downAt(const OWorld *w, i2 p) {                     /* pull.wgsl:36-44, p in the world */
+    if (!w->ctl->gravity) return I2(0, 1);
+    float t[4];
+    gravityTexel(w, I2(p.x / GRAVITY_BLOCK, p.y / GRAVITY_BLOCK), t);
+    return stepAt(wu32(t[3]) & 7u);
+}
+static int32_t downTurns(const OWorld *w, i2 c) {             /* fluid.wgsl:3822-3825 */
+    i2 at = I2(c.x < 0 ? 0 : (c.x > w->W - 1 ? w->W - 1 : c.x), c.y < 0 ? 0 : (c.y > w->H - 1 ? w->H - 1 : c.y));
+    if (MUTANT(31)) return 0;
+    return (int32_t)stepIndex(downAt(w, at)) - 2;
+}
 static int isObstacleCell(const OWorld *w, i2 c, uint32_t vx) {   /* :1123-1147 */
     if (inWorld(w, c) && isOccupant(occupantAt(w, cellIndex(w, c)))) return 1;
     uint32_t t = wType(vx), k = kindOf(w, t);
@@ -539,7 +583,8 @@ static int isObstacleCell(const OWorld *w, i2 c, uint32_t vx) {   /* :1123-1147 
         for (int d = 0; d < 4; d++) wet += isLiquidAt(w, beside(c, d)) ? 1 : 0;
         if (wet < 2) return 0;
         if (t == E_AIR) {
-            i2 up = I2(c.x, c.y - 1), up2 = I2(c.x, c.y - 2);    /* c - downAt(c), down (0, 1) */
+            i2 dn = downAt(w, c);
+            i2 up = I2(c.x - dn.x, c.y - dn.y), up2 = I2(c.x - 2 * dn.x, c.y - 2 * dn.y);   /* c - downAt(c) */
             if (!isLiquidAt(w, up) || !isLiquidAt(w, up2) || fallingAt(w, up)) return 0;
             return gasAmountAt(w, cellIndex(w, c)) >= BUBBLE_GAS;
         }
@@ -1199,7 +1244,10 @@ static void setFace(OWorld *w, i2 at, int isU) {           /* :1936-1980 */
     }
     float now = u;
     if (held > 0.0f && w->ctl->air) now += surfaceDrag(w, a, at, isU, ga, gb, u);   /* the wind's drag */
-    if (held > 0.0f && ga[2] >= 0.0f && gb[2] >= 0.0f) now += (isU != MUTANT(21)) ? 0.0f : w->P->earthG;
+    if (held > 0.0f && ga[2] >= 0.0f && gb[2] >= 0.0f) {   /* the pull as it is at the face (:1968-1972) */
+        v2 pull = pullAt(w, isU ? V2((float)at.x, (float)at.y + 0.5f) : V2((float)at.x + 0.5f, (float)at.y));
+        now += (isU != MUTANT(21)) ? pull.x : pull.y;
+    }
     float start = now;
     FLD(faceSlot(w, f), 0) = start;
     FLD(faceSlot(w, f), 1) = u;
@@ -2033,7 +2081,7 @@ static v2 escape(OWorld *w, v2 pos) {                        /* :3552-3594 */
     }
     if (bestD < 1e9f) return best;
     static const i2 ways[4] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
-    int32_t k = (int32_t)stepIndex(I2(0, 1)) - 2;            /* downTurns, down (0, 1) */
+    int32_t k = downTurns(w, c);
     i2 way = turnStep(ways[0], k);
     int nearest = ESCAPE_REACH + 1;
     for (int n = 0; n < 4; n++) {
@@ -2055,7 +2103,7 @@ static v2 slipOf(OWorld *w, const Particle *q) {             /* :3605-3614 */
     if (n < 2u) return V2(0.0f, 0.0f);
     float mean = (float)CNT(i, G_MASS) / (FIX * (float)n);
     float lighter = (mean - liquidDensity(q->word, q->temp)) / fmaxf(mean, 1e-3f);
-    v2 pull = V2(0.0f, w->P->earthG);
+    v2 pull = pullAt(w, q->pos);
     return V2(-pull.x / WATER_WEIGHT * SLIP * lighter, -pull.y / WATER_WEIGHT * SLIP * lighter);
 }
 static void k_advect(OWorld *w) {                            /* :3616-3694 */
@@ -2106,7 +2154,7 @@ static void k_advect(OWorld *w) {                            /* :3616-3694 */
         uint32_t li = 2u * cellIndex(w, cellOf(w, q.pos));
         float lx = FLD(li, 0), lz = FLD(li, 2);
         if (lz > 0.0f) {
-            int falling = lx < 0.5f * lz * wlength(V2(0.0f, w->P->earthG));
+            int falling = lx < 0.5f * lz * wlength(pullAt(w, q.pos));
             at_or(&CNT(cellIndex(w, cellOf(w, pos)), G_CAME), falling ? CAME_FALLING : CAME_PRESSING);
         }
     }
@@ -2187,7 +2235,7 @@ static uint32_t airTo(OWorld *w, i2 c, int gas) {            /* :3898-3924 */
     if (!gas && gridZ(w, c) > 0.0f) return NONE;
     static const i2 airOrder[8] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}, {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
     static const i2 gasOrder[8] = {{0, -1}, {-1, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {1, 1}, {0, 1}};
-    int32_t k = (int32_t)stepIndex(I2(0, 1)) - 2;
+    int32_t k = downTurns(w, c);
     for (int n = 0; n < 8; n++) {
         i2 o = turnStep(gas ? gasOrder[n] : airOrder[n], k);
         i2 at = I2(c.x + o.x, c.y + o.y);
@@ -2462,6 +2510,7 @@ OWorld *wor_create(uint32_t width, uint32_t height, uint32_t coarse_levels) {
     w->air = (float *)calloc(1, w->bytes[WIG_PART_AIR]);
     w->source = (uint32_t *)calloc(1, w->bytes[WIG_PART_AIR_SOURCE]);
     w->occ = (uint32_t *)calloc(1, w->bytes[WIG_PART_OCCUPANTS]);
+    w->grav = (float *)calloc(1, w->bytes[WIG_PART_GRAVITY]);
     w->dot_mode = 1;
     if (!w->cells || !w->sorted || !w->parts || !w->counters || !w->field || !w->blocks || !w->cg || !w->solve || !w->bvec
         || !w->air || !w->source)
@@ -2472,14 +2521,14 @@ void wor_destroy(OWorld *w) {
     if (!w) return;
     free(w->cells); free(w->sorted); free(w->parts); free(w->counters);
     free(w->field); free(w->blocks); free(w->cg); free(w->solve); free(w->cap_b); free(w->bvec);
-    free(w->air); free(w->source); free(w->occ);
+    free(w->air); free(w->source); free(w->occ); free(w->grav);
     free(w);
 }
 void *wor_part(OWorld *w, int part, uint64_t *bytes) {
     if (part < 0 || part >= WIG_PART_COUNT) return NULL;
     if (bytes) *bytes = w->bytes[part];
     void *p[WIG_PART_COUNT] = {w->cells, w->sorted, w->parts, w->counters, w->field, w->blocks, w->cg, w->solve,
-                               w->air, w->source, w->occ};
+                               w->air, w->source, w->occ, w->grav};
     return p[part];
 }
 void wor_set_kinds(OWorld *w, const uint8_t kinds[256]) { memcpy(w->kind, kinds, 256); }

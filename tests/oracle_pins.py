@@ -128,12 +128,17 @@ class Hand:
         """(H, W, 2): the gas plane's two words a cell (vapour, mist | lost, amount as f16 pairs)."""
         return self.cellwords[4 * self.C:4 * self.C + 2 * self.C].reshape(self.H, self.W, 2)
 
+    def gravity(self):
+        """(ceil(H/2), ceil(W/2), 4): the gravity pass's field, a light cell of 2 x 2 cells."""
+        return self.w.part(abi.PART_GRAVITY).reshape((self.H + 1) // 2, (self.W + 1) // 2, 4)
+
     def occupants(self):
         """(H, W): the occupants part's occupantWord a cell."""
         return self.w.part(abi.PART_OCCUPANTS).reshape(self.H, self.W)
 
-    def ctl(self, coarse_sweeps=40, air=0, surface_drag=0.0, occupants=0):
+    def ctl(self, coarse_sweeps=40, air=0, surface_drag=0.0, occupants=0, gravity=0):
         c = abi.WigStepCtl()
+        c.gravity = gravity
         c.air = air
         c.occupants = occupants
         c.surface_drag = surface_drag
@@ -1104,6 +1109,54 @@ def pin_occupants(L):
             assert words[5, 5, 0] & 0xFF == E.WATER and cnt[5, 5, G_DEST] != DEST_KEEP
 
 
+# ------------------------------------------------------------ gravity
+def pin_gravity(L):
+    """The gravity pass's field (pull.wgsl:13-44): the faces' pull is
+    Earth's plus the masses' field, bilinear between the light cells'
+    middles, so a field linear in the light cells comes out linear in the
+    faces' positions (0.125 (x/2 - 1/2) on a u face at x, inside the clamp),
+    and off it is Earth's alone (setFaces, fluid.wgsl:1968-1972); and down
+    turns where displaced gas goes: with down +x, "up first" is -x
+    (rasterClaims' airTo, :3822-3825, :3898-3924)."""
+    for grav in (1, 0):
+        h = Hand(16, 16, L=L)
+        h.activate()
+        h.earth_g = 0.5
+        words, cell = h.words(), h.cell()
+        words[..., 0] = word(E.WATER)
+        cell[..., 2] = 1.0
+        G = h.gravity()
+        gy, gx = np.mgrid[0:G.shape[0], 0:G.shape[1]]
+        G[..., 0] = 0.125 * gx
+        G[..., 1] = -0.0625 * gy
+        G[..., 3] = 2.0
+        W = 16
+        sums = h.counters[11 * h.C:11 * h.C + 2 * h.F].reshape(h.F, 2).view(np.int32)
+        sums[:, 1] = 65536                      # every face weighed, no momentum
+        h.run(PASS_SET_FACES, gravity=grav)
+        U, V = h.ufaces(), h.vfaces()
+        for y in (3, 8):
+            for x in (3, 6, 11):
+                want_u = 0.125 * (x / 2 - 0.5) if grav else 0.0
+                want_v = 0.5 + (-0.0625 * (y / 2 - 0.5) if grav else 0.0)
+                assert abs(float(U[y, x, 0]) - want_u) <= 1e-6, (grav, x, y, U[y, x, 0], want_u)
+                assert abs(float(V[y, x, 0]) - want_v) <= 1e-6, (grav, x, y, V[y, x, 0], want_v)
+    for grav in (1, 0):
+        h = Hand(16, 16, L=L)
+        h.activate()
+        words, cell, cnt = h.words(), h.cell(), h.cnt()
+        words[..., 0] = word(E.AIR)
+        cnt[..., 8] = 0
+        G = h.gravity()
+        G[..., 3] = 0.0                          # down: step 0, +x
+        cnt[5, 5, G_SORT] = 1                    # liquid comes into the air at (5, 5)
+        words[4, 5, 0] = word(E.WATER)           # vacated above it ...
+        words[5, 4, 0] = word(E.WATER)           # ... and to its left, up from a +x down
+        h.run(PASS_RASTER_CLAIMS, gravity=grav)
+        want = 5 * W + 4 if grav else 4 * W + 5
+        assert cnt[5, 5, 8] == want, (grav, cnt[5, 5, 8], want)
+
+
 PINS = {
     "restrict": pin_restrict,
     "smooth": pin_smooth,
@@ -1128,4 +1181,5 @@ PINS = {
     "surface_drag": pin_surface_drag,
     "push_gas_out": pin_push_gas_out,
     "occupants": pin_occupants,
+    "gravity": pin_gravity,
 }

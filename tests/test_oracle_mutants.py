@@ -43,6 +43,8 @@ CAUGHT_BY = {
     27: "push_gas_out",            # displaced gas's vapour lost
     28: "occupants",               # the person's y velocity decoded from the x field
     29: "occupants",               # liquid left drawn under a person
+    30: "gravity",                 # the field's bilinear weights mirrored in x
+    31: "gravity",                 # down taken as Earth's whatever the field
 }
 
 

@@ -15,11 +15,15 @@ WIG_STORE_F32, WIG_STORE_F16, WIG_STORE_BF16 = 0, 1, 2
 WIG_MAX_CYCLES = 256
 
 PART_CELLS, PART_PARTICLES0, PART_PARTICLES1, PART_COUNTERS, PART_FIELD, PART_BLOCKS, PART_FLUIDCG, PART_SOLVE = range(8)
-PART_AIR, PART_AIR_SOURCE, PART_OCCUPANTS = 8, 9, 10
-PART_COUNT = 11
+PART_AIR, PART_AIR_SOURCE, PART_OCCUPANTS, PART_GRAVITY = 8, 9, 10, 11
+PART_COUNT = 12
 LIQUID_PARTS = 8          # the liquid's own parts (a snapshot's); the rest are the caller's (the air's, the people's)
 PART_NAMES = ("cells", "particles0", "particles1", "counters", "field", "blocks", "fluidCg", "solve", "air0",
-              "airSource", "occupants")
+              "airSource", "occupants", "gravity")
+# the gravity pass's field: a vec4 (pull x, y, potential, down) a light cell
+# of GRAVITY_BLOCK x GRAVITY_BLOCK cells (pull.wgsl:1-44)
+GRAVITY_BLOCK = 2
+DOWN_WEIGHTLESS = 8
 # occupantWord (common.wgsl:238-254): slot + 1 in the low byte, velocity x 128
 # in two 12-bit fields (x at bit 8, y at bit 20)
 OCCUPANT_SCALE = 128.0
@@ -52,6 +56,7 @@ class WigStepCtl(ct.Structure):
         ("coarse_sweeps", ct.c_uint32), ("first_chunk", ct.c_uint32), ("chunk", ct.c_uint32),
         ("level_pairs", ct.c_uint32), ("vcycle_storage", ct.c_uint32), ("residual_every", ct.c_uint32),
         ("air", ct.c_uint32), ("surface_drag", ct.c_float), ("occupants", ct.c_uint32),
+        ("gravity", ct.c_uint32),
     ]
 
 
@@ -59,7 +64,7 @@ class WigLayoutInfo(ct.Structure):
     _fields_ = [
         ("width", ct.c_uint32), ("height", ct.c_uint32), ("cells", ct.c_uint32), ("faces", ct.c_uint32),
         ("blocks", ct.c_uint32), ("blocks_wide", ct.c_uint32), ("capacity", ct.c_uint32), ("levels", ct.c_uint32),
-        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 11),
+        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 12),
     ]
 
 

@@ -285,6 +285,7 @@ static void size_world(WigWorld *w, uint32_t width, uint32_t height, uint32_t co
     w->bytes[WIG_PART_AIR] = 16ull * d.C;
     w->bytes[WIG_PART_AIR_SOURCE] = 4ull * SOURCE_WORDS * d.C;
     w->bytes[WIG_PART_OCCUPANTS] = 4ull * d.C;
+    w->bytes[WIG_PART_GRAVITY] = 16ull * ((d.W + 1u) / 2u) * ((d.H + 1u) / 2u);
 }
 
 
@@ -471,6 +472,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     d.air = (const float4 *)w->part[WIG_PART_AIR];
     d.source = (uint32_t *)w->part[WIG_PART_AIR_SOURCE];
     d.occ = (const uint32_t *)w->part[WIG_PART_OCCUPANTS];
+    d.grav = (const float4 *)w->part[WIG_PART_GRAVITY];
     d.kind = w->kind;
     d.scratch = w->scratch;
     {
@@ -807,6 +809,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d.replace_every = ctl->residual_every;
     d.air_on = ctl->air ? 1u : 0u;
     d.occ_on = ctl->occupants ? 1u : 0u;
+    d.grav_on = ctl->gravity ? 1u : 0u;
     d.surface_drag = ctl->surface_drag;
     d.dt = P->dt;
     d.dx = P->dx;

@@ -201,8 +201,9 @@ struct Dev {
     // coefficient, and the step's dt and dx (the pressure's push)
     const float4 *air;
     const uint32_t *occ;         // occupants (WIG_PART_OCCUPANTS), read when occ_on
+    const float4 *grav;          // the gravity pass's field (WIG_PART_GRAVITY), read when grav_on
     uint32_t *source;
-    uint32_t air_on, occ_on;
+    uint32_t air_on, occ_on, grav_on;
     float surface_drag, dt, dx;
 };
 

@@ -162,6 +162,41 @@ __device__ __forceinline__ v2 occupantVel(uint32_t w) {
     const int32_t x = (int32_t)(w << 12u) >> 20, y = (int32_t)w >> 20;
     return V2((float)x / OCCUPANT_SCALE, (float)y / OCCUPANT_SCALE);
 }
+// The pull and down (pull.wgsl:13-44) from the gravity pass's field, a
+// vec4 a light cell of GRAVITY_BLOCK x GRAVITY_BLOCK cells: the masses' pull
+// (xy), their potential (z), down (w).  Off, Earth's pull alone and down
+// (0, 1), as the field's texel for that (no masses' pull, down step 2).
+#define GRAVITY_BLOCK 2
+__device__ __forceinline__ float4 gravityTexel(const Dev &d, i2 c) {
+    if (!d.grav_on) return make_float4(0.0f, 0.0f, 0.0f, 2.0f);
+    const int32_t gw = (d.W + 1) / 2, gh = (d.H + 1) / 2;
+    const i2 k = clampTo(c, gw, gh);
+    return d.grav[(uint32_t)k.y * (uint32_t)gw + (uint32_t)k.x];
+}
+// fieldAt's pull (xy): bilinear between the light cells' middles
+__device__ __forceinline__ v2 fieldPull(const Dev &d, v2 pos) {
+    const float qx = pos.x / (float)GRAVITY_BLOCK - 0.5f, qy = pos.y / (float)GRAVITY_BLOCK - 0.5f;
+    const float fx0 = floorf(qx), fy0 = floorf(qy);
+    const i2 i = I2(wi32(fx0), wi32(fy0));
+    const float fx = qx - fx0, fy = qy - fy0;
+    const float4 a = gravityTexel(d, i), b = gravityTexel(d, I2(i.x + 1, i.y));
+    const float4 c = gravityTexel(d, I2(i.x, i.y + 1)), e = gravityTexel(d, I2(i.x + 1, i.y + 1));
+    const v2 top = V2(wmix(a.x, b.x, fx), wmix(a.y, b.y, fx)), bottom = V2(wmix(c.x, e.x, fx), wmix(c.y, e.y, fx));
+    return V2(wmix(top.x, bottom.x, fy), wmix(top.y, bottom.y, fy));
+}
+__device__ __forceinline__ v2 pullAt(const Dev &d, v2 pos) {
+    if (!d.grav_on) return V2(0.0f, d.earthG);
+    const v2 f = fieldPull(d, pos);
+    return V2(0.0f + f.x, d.earthG + f.y);
+}
+__device__ __forceinline__ i2 downAt(const Dev &d, i2 p) {      // (p in the world)
+    if (!d.grav_on) return I2(0, 1);
+    return stepAt(wu32(gravityTexel(d, I2(p.x / GRAVITY_BLOCK, p.y / GRAVITY_BLOCK)).w) & 7u);
+}
+// eighths of a turn from the Earth's down to the down at c (:3822-3825)
+__device__ __forceinline__ int32_t downTurns(const Dev &d, i2 c) {
+    return (int32_t)stepIndex(downAt(d, clampTo(c, d.W, d.H))) - 2;
+}
 __device__ bool isObstacleCell(const Dev &d, i2 c, uint32_t vx) {       // :1123-1147
     if (inWorld(d, c) && isOccupant(occupantAt(d, cellIndex(d, c)))) return true;
     uint32_t t = wType(vx), k = kindOf(d, t);
@@ -170,7 +205,8 @@ __device__ bool isObstacleCell(const Dev &d, i2 c, uint32_t vx) {       // :1123
         for (int j = 0; j < 4; j++) wet += isLiquidAt(d, beside(c, j)) ? 1 : 0;
         if (wet < 2) return false;
         if (t == E_AIR) {
-            i2 up = I2(c.x, c.y - 1), up2 = I2(c.x, c.y - 2);
+            const i2 dn = downAt(d, c);
+            i2 up = I2(c.x - dn.x, c.y - dn.y), up2 = I2(c.x - 2 * dn.x, c.y - 2 * dn.y);
             if (!isLiquidAt(d, up) || !isLiquidAt(d, up2) || fallingAt(d, up)) return false;
             return gasAmountAt(d, cellIndex(d, c)) >= BUBBLE_GAS;
         }
@@ -1125,7 +1161,12 @@ __device__ void setFace(const Dev &d, i2 at, bool isU) {
     }
     float now = u;
     if (held > 0.0f && d.air_on) now += surfaceDrag(d, a, at, isU, ga, gb, u);   // the wind's drag
-    if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) now += isU ? 0.0f : d.earthG;
+    // (liquids all fall alike, under the Earth's pull and the masses', along
+    // both axes as it is at the face, :1968-1972)
+    if (held > 0.0f && ga.z >= 0.0f && gb.z >= 0.0f) {
+        const v2 pull = pullAt(d, isU ? V2((float)at.x, (float)at.y + 0.5f) : V2((float)at.x + 0.5f, (float)at.y));
+        now += isU ? pull.x : pull.y;
+    }
     *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(now, u, held, faceViscosity(ga, gb));
     FLD(d, viscRhsSlot(d, f), f % 4u) = now;
     if (FLD(d, viscLastSlot(d, f), f % 4u) != 0.0f) FLD(d, viscLastSlot(d, f), f % 4u) = 0.0f;
@@ -1300,6 +1341,12 @@ __device__ __forceinline__ ProlongSum prolongSum(const float r[4], const float p
     }
     return s;
 }
+// a wavefront's own LDS phase ended: its lanes' LDS writes seen by its lanes
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 #define NT RELAX_NT
 template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int post) {
     WORLD(g);
@@ -1313,10 +1360,10 @@ template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int
     if (!it.encoded || it.converged) return;
     const int shift = ((d.shifted_first ^ (2u * it.cycle + (uint32_t)post)) & 1u) ? WIG_BLOCK / 2 : 0;
     // A workgroup takes 4 x NT tiles at a time, NT a wavefront, each in LDS
-    // of its own; the barriers are the workgroup's, so every wavefront
-    // passes each of them, tiles or none.  Every load of the NT tiles is
-    // made at once and unconditionally (clamped addresses, values chosen
-    // after), so that they are in flight together.
+    // of its own: no wavefront reads another's, so each phase ends on the
+    // wavefront's own sync, and the four go at their own pace.  Every load
+    // of the NT tiles is made at once and unconditionally (clamped
+    // addresses, values chosen after), so that they are in flight together.
     __shared__ float sPxA[WAVES * NT][RIM * RIM], sPyA[WAVES * NT][RIM * RIM];
     __shared__ float cRhoA[WAVES * NT][64], cPxA[WAVES * NT][64], cPyA[WAVES * NT][64];
     const uint32_t li = threadIdx.x % 64u, wave = threadIdx.x / 64u;
@@ -1396,7 +1443,7 @@ template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int
                 cPxA[s_][li] = lp[j].x;
                 cPyA[s_][li] = lp[j].y;
             }
-            __syncthreads();
+            waveSync();
         }
 #pragma unroll
         for (int j = 0; j < NT; j++) {
@@ -1430,7 +1477,7 @@ template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int
                 sPyA[s_][k] = v2_.y;
             }
         }
-        __syncthreads();
+        waveSync();
         float b0[NT], b1[NT], b2[NT], b3[NT], rhsx[NT], rhsy[NT], diag[NT], dx[NT], dy[NT];
         bool fluid[NT], solving[NT];
         int color[NT];
@@ -1464,12 +1511,12 @@ template <int S> __global__ void __launch_bounds__(256) k_relax(WorldGrid g, int
                     if (diag[j] > 0.0f) py[t] = ny;
                 }
             }
-            __syncthreads();
+            waveSync();
         }
 #pragma unroll
         for (int j = 0; j < NT; j++)
             if (fluid[j]) fnStore<S>(d, post ? 0 : 1, mi[j], make_float2(sPxA[wave * NT + (uint32_t)j][t], sPyA[wave * NT + (uint32_t)j][t]));
-        __syncthreads();
+        waveSync();
     }
 }
 
@@ -2029,13 +2076,8 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
 struct LdsLevel { float *px, *py, *rx, *ry, *rho, *bR, *bD; uint32_t w, h; };
 // a phase's end: the workgroup's barrier, or the first wavefront's sync
 __device__ __forceinline__ void phaseSync(bool wg) {
-    if (wg) {
-        __syncthreads();
-    } else {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    if (wg) __syncthreads();
+    else waveSync();
 }
 // A phase's cells of a w-wide level, j = tid, tid + nthr, ..., with their
 // column and row stepped along (one division a phase, not one a cell).
@@ -3094,7 +3136,7 @@ __device__ v2 escape(const Dev &d, v2 pos) {                                 // 
         if (dd < bestD) { bestD = dd; best = inside; }
     }
     if (bestD < 1e9f) return best;
-    int32_t kt = (int32_t)stepIndex(I2(0, 1)) - 2;
+    int32_t kt = downTurns(d, c);
     i2 ways[4] = {I2(0, -1), I2(-1, 0), I2(1, 0), I2(0, 1)};
     i2 way = turnStep(ways[0], kt);
     int nearest = ESCAPE_REACH + 1;
@@ -3140,7 +3182,8 @@ __global__ void __launch_bounds__(256) k_advect(WorldGrid g) {
             if (nn >= 2u) {
                 float mean = (float)CNT(d, i, G_MASS) / (FIXF * (float)nn);
                 float lighter = (mean - liquidDensity(temp)) / fmaxf(mean, 1e-3f);
-                slip = V2(-0.0f / WATER_WEIGHT * SLIP * lighter, -d.earthG / WATER_WEIGHT * SLIP * lighter);
+                const v2 pull = pullAt(d, qpos);
+                slip = V2(-pull.x / WATER_WEIGHT * SLIP * lighter, -pull.y / WATER_WEIGHT * SLIP * lighter);
             }
         }
         v2 cs = clampSpeed(vel);
@@ -3174,7 +3217,7 @@ __global__ void __launch_bounds__(256) k_advect(WorldGrid g) {
         uint32_t li = 2u * cellIndex(d, cellOf(d, qpos));
         float lx = FLD(d, li, 0), lz = FLD(d, li, 2);
         if (lz > 0.0f) {
-            bool falling = lx < 0.5f * lz * wlength(V2(0.0f, d.earthG));
+            bool falling = lx < 0.5f * lz * wlength(pullAt(d, qpos));
             came = cellIndex(d, cellOf(d, pos));
             how = falling ? CAME_FALLING : CAME_PRESSING;
         }
@@ -3326,7 +3369,7 @@ __device__ __forceinline__ bool vacated(const Dev &d, uint32_t j) {
 }
 __device__ uint32_t airTo(const Dev &d, i2 c, bool gas) {                    // :3898-3924
     if (!gas && gridZ(d, c) > 0.0f) return WNONE;
-    int32_t kt = (int32_t)stepIndex(I2(0, 1)) - 2;
+    int32_t kt = downTurns(d, c);
     for (int n = 0; n < 8; n++) {
         i2 o;
         if (!gas) {

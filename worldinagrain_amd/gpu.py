@@ -77,7 +77,8 @@ def _check(status):
     return status
 
 
-_DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8, abi.PART_AIR: np.float32}
+_DTYPES = {abi.PART_FIELD: np.float32, abi.PART_FLUIDCG: np.float32, abi.PART_SOLVE: np.uint8, abi.PART_AIR: np.float32,
+           abi.PART_GRAVITY: np.float32}
 
 
 class HipStream:

@@ -106,6 +106,9 @@ class Simulation:
         # people's bodies (WigStepCtl.occupants): the world's occupants part
         # read as moving obstacles; off, nobody is there
         self.occupants = False
+        # the gravity pass's field (WigStepCtl.gravity): the world's gravity
+        # part read for the pull and down; off, Earth's pull alone
+        self.gravity = False
 
     # Simulation.seed / random (js/sim.js:1386-1397)
     def seed(self, n):
@@ -143,6 +146,7 @@ class Simulation:
         c.air = 1 if self.air else 0
         c.surface_drag = surface_drag_coefficient(self.dx)
         c.occupants = 1 if self.occupants else 0
+        c.gravity = 1 if self.gravity else 0
         if s.first_chunk < 0:
             c.first_chunk = 0xFFFFFFFF          # WIG_CHUNK_AUTO: the world's last solve's count
             c.chunk = s.chunk
@@ -231,6 +235,7 @@ class BatchHost:
             c.air = 1 if s.air else 0
             c.surface_drag = surface_drag_coefficient(s.dx)
             c.occupants = 1 if s.occupants else 0
+            c.gravity = 1 if s.gravity else 0
             if sv.first_chunk < 0:
                 c.first_chunk, c.chunk = 0xFFFFFFFF, sv.chunk
             else:

@@ -15,7 +15,7 @@ const LIQUID_TOLERANCE = 1e-2;  // fluid.wgsl:2221
 const GRAVITY = 9.81 / 3600 / 0.01;   // js/elements.js:30, water's weight
 
 const PART = { cells: 0, particles0: 1, particles1: 2, counters: 3, field: 4, blocks: 5, fluidCg: 6, solve: 7, air0: 8,
-    airSource: 9, occupants: 10 };
+    airSource: 9, occupants: 10, gravity: 11 };
 const STAT = { ACTIVE: 0, RETIRED: 1, VISCOUS: 2, PARTICLES: 3, RESIDUAL: 4, CYCLES: 7, MOST_CYCLES: 8 };
 const TOL_ABS_MAX = 0;
 const TOL_REL_L2 = 1;
@@ -57,6 +57,7 @@ class LiquidSimulation {
         this.frameNo = options.frame === undefined ? 1 : options.frame;
         this.air = !!options.air;      // the air coupled in (air0 read, airSource added to)
         this.occupants = !!options.occupants;   // people's bodies (the occupants part) as moving obstacles
+        this.gravity = !!options.gravity;       // the gravity pass's field (the gravity part): pull and down
         const s = options.solver || {};
         this.solver = {
             tolMode: s.tolMode === undefined ? TOL_ABS_MAX : s.tolMode,
@@ -115,7 +116,7 @@ class LiquidSimulation {
     // relaxation's starting tiling (flipped each step), the solve's settings.
     stepControl() {
         this.shiftedFirst = !this.shiftedFirst;
-        const buf = new ArrayBuffer(80);
+        const buf = new ArrayBuffer(84);
         const u = new Uint32Array(buf);
         const f = new Float32Array(buf);
         u[0] = this.fluidStats.active ? 1 : 0;
@@ -133,6 +134,7 @@ class LiquidSimulation {
         u[17] = this.air ? 1 : 0;
         f[18] = this.air ? surfaceDragCoefficient(this.metresPerCell) : 0;
         u[19] = this.occupants ? 1 : 0;
+        u[20] = this.gravity ? 1 : 0;
         return buf;
     }
 

@@ -158,7 +158,7 @@ static napi_value js_copy(napi_env env, napi_callback_info info, int up) {
 static napi_value js_upload(napi_env env, napi_callback_info info) { return js_copy(env, info, 1); }
 static napi_value js_download(napi_env env, napi_callback_info info) { return js_copy(env, info, 0); }
 
-/* fluidStep(h, params: ArrayBuffer(80), ctl: ArrayBuffer(80)) -> status */
+/* fluidStep(h, params: ArrayBuffer(80), ctl: ArrayBuffer(84)) -> status */
 static napi_value js_fluid_step(napi_env env, napi_callback_info info) {
     size_t argc = 3;
     napi_value argv[3], r;
