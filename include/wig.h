@@ -190,6 +190,18 @@ int wig_destroy(WigWorld *w);
  * (the `elems` binding, fluid.wgsl:117). */
 int wig_set_kinds(WigWorld *w, const uint8_t kinds[256]);
 
+/* The elements' columns the liquid step reads beside their kind
+ * (common.wgsl's Elem; js/elements.js's table): a liquid's viscosity
+ * (cells^2 a step) at viscosity_t (degrees C) and its temperature scale
+ * viscosity_e (degrees C; <= 0: the same at every temperature), as
+ * viscosityAt (common.wgsl:1486-1493) takes them; the surface tension
+ * factor (surface, fluid.wgsl:1779); a solid's wetting share (:1781).  All
+ * zero until set: no viscosity, no tension. */
+typedef struct {
+    float viscosity, viscosity_e, viscosity_t, tension, wetting;
+} WigElementProps;
+int wig_set_element_props(WigWorld *w, const WigElementProps props[256]);
+
 /* The world's sizes: cells, faces, blocks, particle capacity, coarse levels. */
 typedef struct WigLayoutInfo {
     uint32_t width, height, cells, faces, blocks, blocks_wide, capacity, levels;

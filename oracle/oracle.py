@@ -16,7 +16,7 @@ _libs = {}
 (PASS_SYNC, PASS_SYNC_PARTICLES, PASS_SYNC_SEED, PASS_COMPACT, PASS_RETIRE, PASS_CLEAR, PASS_GATHER, PASS_SEAL,
  PASS_CLASSIFY, PASS_SET_FACES, PASS_UNPRESS, PASS_DIVERGENCE, PASS_RESTRICT, PASS_SMOOTH, PASS_COARSEST, PASS_PROLONG,
  PASS_PROLONG_FINE, PASS_PROJECT, PASS_EXTRAPOLATE, PASS_ADVECT, PASS_SORT, PASS_RASTER_CLAIMS, PASS_RASTER,
- PASS_UNPAINT, PASS_RELAX) = range(25)
+ PASS_UNPAINT, PASS_RELAX, PASS_VISCOSITY) = range(26)
 
 
 OMP_PATH = os.path.join(_HERE, "libwig_oracle_omp.so")
@@ -40,6 +40,7 @@ def lib(path=None):
         L.wor_part.restype = ct.c_void_p
         L.wor_part.argtypes = [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_uint64)]
         L.wor_set_kinds.argtypes = [ct.c_void_p, ct.c_char_p]
+        L.wor_set_element_props.argtypes = [ct.c_void_p, ct.c_void_p]
         L.wor_set_dot_mode.argtypes = [ct.c_void_p, ct.c_int]
         L.wor_layout.argtypes = [ct.c_void_p, ct.POINTER(abi.WigLayoutInfo)]
         L.wor_fluid_step.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
@@ -106,6 +107,12 @@ class OracleWorld:
         k = np.asarray(kinds, dtype=np.uint8)
         assert k.shape == (256,)
         self.L.wor_set_kinds(self._w, k.tobytes())
+
+    def set_element_props(self, props):
+        """(256, 5) f32: viscosity, viscosity_e, viscosity_t, tension, wetting."""
+        p = np.ascontiguousarray(props, dtype=np.float32)
+        assert p.shape == (256, 5)
+        self.L.wor_set_element_props(self._w, p.ctypes.data)
 
     def part(self, p):
         """A numpy view (u32 words, f32 for field/fluidCg) of the part's memory."""

@@ -201,6 +201,7 @@ typedef struct OWorld {
     uint32_t *occ;               /* occupants: occupantWord a cell */
     float *grav;                 /* the gravity pass's field: a vec4 a light cell */
     uint8_t kind[256];
+    WigElementProps props[256];  /* the elements' liquid columns */
     int dot_mode;
     /* the step being run */
     const WigParams *P;
@@ -598,11 +599,33 @@ static int32_t countAt(const OWorld *w, i2 c) {              /* :1150-1159 */
     if (isObstacleCell(w, c, w->cells[4ull * i])) return -1;
     return (int32_t)CNT(i, G_COUNT);
 }
+/* e^x in float from basic operations only (round to nearest, no fused
+ * multiply-add), the same steps on the GPU, so both give the same bits:
+ * x = k ln2 + r with k = rint(x / ln2) and ln2 split in two (Cody and
+ * Waite), e^r by its Taylor series to r^7 on |r| <= ln2 / 2 (relative
+ * error under 2^-23 there), times 2^k.  viscosityAt's exp. */
+static float wexpf(float x) {
+    if (!(x < 88.5f)) return x != x ? x : INFINITY;
+    if (x < -103.0f) return 0.0f;
+    float k = rintf(x * 1.44269504f);
+    float r = (x - k * 0.693145752f) - k * 1.42860677e-6f;
+    float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f + r * (0.00833333377f
+              + r * (0.00138888892f + r * 0.000198412701f))))));
+    return ldexpf(p, (int)k);
+}
+/* viscosityAt (common.wgsl:1486-1493): a liquid's viscosity at temp */
+#define VISCOSITY_MOST 1e4f
+static float viscosityAt(const WigElementProps *e, float temp) {
+    if (e->viscosity_e <= 0.0f) return e->viscosity;
+    return fminf(e->viscosity * wexpf((e->viscosity_t - temp) / e->viscosity_e), VISCOSITY_MOST);
+}
 static v2 liquidIn(const OWorld *w, uint32_t i) {           /* :1165-1179 */
     uint32_t vx = w->cells[4ull * i];
     float temp = fbits(w->cells[4ull * i + 1]);
     uint32_t k = kindOf(w, wType(vx));
-    float viscosity = 0.0f;                                  /* water's, at any temperature */
+    /* at the temperature it acts at: water's effTempIn (every liquid has
+     * water's phase points here) */
+    float viscosity = k == KIND_LIQUID ? viscosityAt(&w->props[wType(vx)], effTemp(temp)) : 0.0f;
     uint32_t n = CNT(i, G_COUNT);
     if (n > 0u) return V2((float)CNT(i, G_MASS) / (FIX * (float)n), viscosity);
     if (k != KIND_LIQUID) return V2(1.0f, 0.0f);
@@ -1224,6 +1247,15 @@ static void gridAt4(const OWorld *w, i2 c, float g[4]) {
     if (!inWorld(w, c)) { g[0] = 0.0f; g[1] = 0.0f; g[2] = -1.0f; g[3] = 0.0f; return; }
     for (int j = 0; j < 4; j++) g[j] = FLD(2u * cellIndex(w, c), j);
 }
+/* wasLiquid (:3315-3317): cell c liquid last step too */
+static inline int wasLiquid(const OWorld *w, i2 c) {
+    return (wu32(FLD(2u * cellIndex(w, c) + 1u, 3)) & stepBit(w, 1u)) != 0u;
+}
+/* lastPush (:3305-3310): last step's pressure push across the face */
+static float lastPush(const OWorld *w, i2 a, i2 at, const float ga[4], const float gb[4]) {
+    if ((ga[2] > 0.0f && !wasLiquid(w, a)) || (gb[2] > 0.0f && !wasLiquid(w, at))) return 0.0f;
+    return fineBeta(ga[2], gb[2], densityAt(w, a), densityAt(w, at)) * (gb[0] - ga[0]);
+}
 static void setFace(OWorld *w, i2 at, int isU) {           /* :1936-1980 */
     uint32_t f = faceIndex(w, at, isU);
     i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
@@ -1249,6 +1281,10 @@ static void setFace(OWorld *w, i2 at, int isU) {           /* :1936-1980 */
         now += (isU != MUTANT(21)) ? pull.x : pull.y;
     }
     float start = now;
+    if (faceViscosity(ga, gb) > 0.0f) {                      /* isViscousFace: last step's push off (:1975-1978) */
+        now -= lastPush(w, a, at, ga, gb);
+        start = now + FLD(viscLastSlot(w, f), f % 4u);
+    }
     FLD(faceSlot(w, f), 0) = start;
     FLD(faceSlot(w, f), 1) = u;
     FLD(faceSlot(w, f), 2) = held;
@@ -1264,18 +1300,82 @@ static void k_setFaces(OWorld *w) {                          /* :1982-1994 */
             if (fs[j].isU >= 0) setFace(w, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
     }
 }
-static void k_unpress(OWorld *w) {                           /* :3405-3456, no viscous face */
-    FOR_ACTIVE_CELLS(w, c) {
-        if (!inWorld(w, c)) continue;
-        Face3 fs[4];
-        facesOf(w, c, fs);
-        for (int j = 0; j < 4; j++) {
-            if (fs[j].isU < 0) continue;
-            uint32_t f = faceIndex(w, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
-            if (FLD(viscLastSlot(w, f), f % 4u) != 0.0f) FLD(viscLastSlot(w, f), f % 4u) = 0.0f;
+/* viscousFace (:3361-3403): a red-black Gauss-Seidel step of implicit
+ * viscosity on the face at `at` if it is of `color` (no bodies: heavy 1) */
+#define VISC_OMEGA 1.6f
+#define VISCOSITY_ITERATIONS 8                               /* js/sim.js:71 */
+static void viscousFace(OWorld *w, i2 at, int isU, int color) {
+    if (((at.x + at.y) & 1) != color) return;
+    i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+    float ga[4], gb[4];
+    gridAt4(w, a, ga);
+    gridAt4(w, at, gb);
+    if (ga[2] < 0.0f || gb[2] < 0.0f) return;
+    uint32_t f = faceIndex(w, at, isU);
+    uint64_t fs = faceSlot(w, f);
+    float nu = FLD(fs, 3);
+    if (nu <= 0.0f) return;
+    float heavy = 1.0f;
+    float through[2] = {ga[2] > 0.0f ? ga[3] : 0.0f, gb[2] > 0.0f ? gb[3] : 0.0f};
+    float sum = 0.0f, n = 0.0f;
+    for (int d = 0; d < 4; d++) {
+        i2 o = beside(at, d);
+        int across = isU == (d >= 2);
+        float other[4] = {0.0f, 0.0f, -1.0f, 0.0f};
+        if (faceInWorld(w, o, isU)) {
+            uint64_t os = faceSlot(w, faceIndex(w, o, isU));
+            for (int j = 0; j < 4; j++) other[j] = FLD(os, j);
+        }
+        float k = (d & 1) ? through[1] : through[0];
+        if (across) k = other[2] < 0.0f ? nu : 2.0f * nu * other[3] / fmaxf(nu + other[3], 1e-12f);
+        if (other[2] < 0.0f) n += k * ((across && !MUTANT(33)) ? 2.0f : 1.0f);
+        else if (other[2] > 0.0f) {
+            sum += k * other[0];
+            n += k;
+        }
+    }
+    float rhs = FLD(2ull * w->C + w->F + f / 4u, f % 4u);   /* viscRhs */
+    FLD(fs, 0) = wmix(FLD(fs, 0), (rhs + heavy * sum) / (1.0f + heavy * n), MUTANT(32) ? 1.0f : VISC_OMEGA);
+}
+/* viscosity (:3407-3447): over the blocks worked on with viscous liquid in
+ * or beside them, each cell's faces: a sweep of one colour (0, 1) or
+ * unpress (2): what the sweeps did kept for next step (viscLast) and last
+ * step's push back on; on the faces they don't solve, viscLast cleared */
+static void k_viscosity(OWorld *w, int color) {
+    PAR_FOR
+    for (uint32_t wid = 0; wid < activeCount(w); wid++) {
+        i2 b = activeBlock(w, wid);
+        if ((statesOf(w, I2(b.x - 1, b.y - 1), I2(b.x + 1, b.y + 1)) & HAS_VISCOUS) == 0u) continue;
+        for (int li = 0; li < BLOCK * BLOCK; li++) {
+            i2 c = I2(b.x * BLOCK + li % BLOCK, b.y * BLOCK + li / BLOCK);
+            if (!inWorld(w, c)) continue;
+            Face3 fs[4];
+            facesOf(w, c, fs);
+            for (int j = 0; j < 4; j++) {
+                if (fs[j].isU < 0) continue;
+                i2 at = I2(fs[j].x, fs[j].y);
+                int isU = fs[j].isU == 1;
+                if (color < 2) {
+                    viscousFace(w, at, isU, color);
+                    continue;
+                }
+                i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+                float ga[4], gb[4];
+                gridAt4(w, a, ga);
+                gridAt4(w, at, gb);
+                uint32_t f = faceIndex(w, at, isU);
+                if (faceViscosity(ga, gb) > 0.0f) {
+                    uint64_t fs0 = faceSlot(w, f);
+                    FLD(viscLastSlot(w, f), f % 4u) = FLD(fs0, 0) - FLD(2ull * w->C + w->F + f / 4u, f % 4u);
+                    FLD(fs0, 0) += lastPush(w, a, at, ga, gb);
+                } else if (FLD(viscLastSlot(w, f), f % 4u) != 0.0f) {
+                    FLD(viscLastSlot(w, f), f % 4u) = 0.0f;
+                }
+            }
         }
     }
 }
+static void k_unpress(OWorld *w) { k_viscosity(w, 2); }   /* :3449-3454 */
 static void k_divergence(OWorld *w) {                        /* :1996-2025 */
     w->blocks[COUNT_SLOT(COUNT_RESIDUAL)] = 0u;
     w->blocks[COUNT_SLOT(COUNT_CYCLES)] = 0u;
@@ -2395,7 +2495,7 @@ static void relaxPass(OWorld *w, int *shifted, uint32_t times) {
 
 static int check_ctl(const OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     if (P->size[0] != (uint32_t)w->W || P->size[1] != (uint32_t)w->H) return WIG_E_INVALID;
-    if (ctl->viscous || ctl->surface || ctl->bodies_run) return WIG_E_INVALID;
+    if (ctl->surface || ctl->bodies_run) return WIG_E_INVALID;
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->level_pairs > 64u) return WIG_E_INVALID;
     if (ctl->tol_mode > WIG_TOL_REL_L2 || ctl->residual_every > WIG_MAX_CYCLES) return WIG_E_INVALID;
     return WIG_OK;
@@ -2426,6 +2526,11 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     k_sealAnchor(w);
     k_classify(w);
     k_setFaces(w);
+    /* viscosity's sweeps before the projection (js/sim.js:2505-2514) */
+    for (int i = 0; ctl->viscous && i < VISCOSITY_ITERATIONS; i++) {
+        k_viscosity(w, 0);
+        k_viscosity(w, 1);
+    }
     k_unpress(w);
     k_divergence(w);
     int shifted = (int)ctl->shifted_first;
@@ -2532,6 +2637,7 @@ void *wor_part(OWorld *w, int part, uint64_t *bytes) {
     return p[part];
 }
 void wor_set_kinds(OWorld *w, const uint8_t kinds[256]) { memcpy(w->kind, kinds, 256); }
+void wor_set_element_props(OWorld *w, const WigElementProps props[256]) { memcpy(w->props, props, sizeof w->props); }
 void wor_set_dot_mode(OWorld *w, int mode) { w->dot_mode = mode ? 1 : 0; }
 int wor_layout(const OWorld *w, WigLayoutInfo *out) {
     memset(out, 0, sizeof *out);
@@ -2746,7 +2852,7 @@ enum {
     WOR_PASS_GATHER, WOR_PASS_SEAL, WOR_PASS_CLASSIFY, WOR_PASS_SET_FACES, WOR_PASS_UNPRESS, WOR_PASS_DIVERGENCE,
     WOR_PASS_RESTRICT, WOR_PASS_SMOOTH, WOR_PASS_COARSEST, WOR_PASS_PROLONG, WOR_PASS_PROLONG_FINE, WOR_PASS_PROJECT,
     WOR_PASS_EXTRAPOLATE, WOR_PASS_ADVECT, WOR_PASS_SORT, WOR_PASS_RASTER_CLAIMS, WOR_PASS_RASTER, WOR_PASS_UNPAINT,
-    WOR_PASS_RELAX, WOR_PASSES
+    WOR_PASS_RELAX, WOR_PASS_VISCOSITY, WOR_PASSES
 };
 int wor_run(OWorld *w, int pass, int arg, const WigParams *P, const WigStepCtl *ctl) {
     if (pass < 0 || pass >= WOR_PASSES) return WIG_E_INVALID;
@@ -2784,6 +2890,7 @@ int wor_run(OWorld *w, int pass, int arg, const WigParams *P, const WigStepCtl *
     case WOR_PASS_RASTER: k_raster(w); break;
     case WOR_PASS_UNPAINT: k_unpaint(w); break;
     case WOR_PASS_RELAX: k_relax(w, arg ? BLOCK / 2 : 0); break;
+    case WOR_PASS_VISCOSITY: k_viscosity(w, arg & 1); break;
     }
     return WIG_OK;
 }

@@ -15,7 +15,7 @@ import numpy as np
 from oracle.oracle import (OracleWorld, PASS_ADVECT, PASS_CLASSIFY, PASS_COARSEST, PASS_COMPACT, PASS_DIVERGENCE,
                            PASS_EXTRAPOLATE, PASS_PROLONG, PASS_PROLONG_FINE, PASS_RASTER, PASS_RELAX, PASS_RESTRICT,
                            PASS_RASTER_CLAIMS, PASS_SEAL, PASS_SET_FACES, PASS_SMOOTH, PASS_SORT, PASS_SYNC_PARTICLES,
-                           PASS_UNPAINT)
+                           PASS_UNPAINT, PASS_UNPRESS, PASS_VISCOSITY)
 from worldinagrain_amd import abi, scenes
 from worldinagrain_amd import elements as E
 from worldinagrain_amd.host import Simulation, SolverSettings
@@ -1109,6 +1109,78 @@ def pin_occupants(L):
             assert words[5, 5, 0] & 0xFF == E.WATER and cnt[5, 5, G_DEST] != DEST_KEEP
 
 
+# ------------------------------------------------------------ viscosity
+VISCOUS, HEATED = 6, 7          # test liquids: constant viscosity, and viscosity by temperature
+
+
+def viscous_elements():
+    """The kinds and liquid columns of two test liquids beside the table's."""
+    kinds = E.kinds().copy()
+    kinds[VISCOUS] = kinds[HEATED] = E.KIND_LIQUID
+    props = np.zeros((256, 5), dtype=np.float32)
+    props[VISCOUS] = (0.5, 0.0, 0.0, 0.0, 0.0)
+    props[HEATED] = (0.02, 25.0, 100.0, 0.0, 0.0)
+    return kinds, props
+
+
+def pin_viscosity(L):
+    """Viscosity (fluid.wgsl:3277-3466; common.wgsl:1486-1493).  A liquid's
+    viscosity at temperature T is v e^((T0 - T)/E) to 2 ulp of the exact
+    value (its exp this build's own), at most 1e4, and the same at every
+    temperature with E <= 0, at water's effective temperature (140 -> 100).
+    One red sweep on a hand face: along the flow the cells' viscosity,
+    across it the faces' harmonic mean, an obstacle face across counted
+    twice, mixed by 1.6: (1 + 0.45) / (1 + 2.5) * 1.6; a black face left
+    alone.  unpress keeps what the sweeps did (x - rhs) and puts last
+    step's push back on: faceBeta(1, 1) (0.3 - 0.1)."""
+    kinds, props = viscous_elements()
+    # viscosityAt, through classify's cell viscosity after a step
+    for T, want in ((20.0, 0.02 * np.exp(80 / 25)), (60.0, 0.02 * np.exp(40 / 25)), (100.0, 0.02),
+                    (140.0, 0.02), (-250.0, 1e4)):
+        from worldinagrain_amd.host import Simulation
+        w = OracleWorld(24, 24, kinds=kinds, L=L)
+        w.set_element_props(props)
+        cells = scenes.dam_break(24, 24, block=(12, 12))
+        words = cells[:4 * 24 * 24].reshape(24, 24, 4)
+        liquid = (words[..., 0] & 0xFF) == E.WATER
+        words[liquid, 0] = (words[liquid, 0] & np.uint32(0xFFFFFF00)) | np.uint32(HEATED)
+        words[liquid, 1] = np.float32(T).view(np.uint32)
+        w.upload(abi.PART_CELLS, cells)
+        Simulation(w, 24, 24).step()
+        nu = w.part(abi.PART_FIELD)[:8 * 24 * 24].reshape(24, 24, 8)[18, 5, 3]
+        assert abs(float(nu) - want) <= 2 * np.spacing(np.float32(want)), (T, nu, want)
+        w.close()
+    # one red sweep, then unpress, on hand faces
+    h = Hand(16, 16, L=L, kinds=kinds)
+    h.activate()
+    h.blocks[h.B:2 * h.B] |= 4                               # HAS_VISCOUS
+    cell, U = h.cell(), h.ufaces()
+    cell[..., 2] = 1.0                                       # liquid
+    cell[..., 3] = 0.5                                       # its viscosity
+    cell[6, 5, 0], cell[6, 6, 0] = 0.1, 0.3                  # pressures either side of u face (6, 6)
+    cell[..., 7] = 1.0                                       # liquid last step too (stepBit(1) at frame 1)
+    U[..., 2] = 1.0
+    U[..., 3] = 0.5
+    U[6, 6, 0] = 0.0
+    U[6, 5, 0], U[6, 7, 0], U[7, 6, 0] = 0.2, 0.4, 0.3       # along (x -+ 1), below
+    U[5, 6, 2] = -1.0                                        # above: an obstacle's face
+    W = 16
+    F = h.F
+    rhs = h.field[4 * (2 * h.C + F):4 * (2 * h.C + F) + F].copy()
+    rhs[:] = 0.0
+    rhs[6 * (W + 1) + 6] = 1.0
+    h.field[4 * (2 * h.C + F):4 * (2 * h.C + F) + F] = rhs
+    h.run(PASS_VISCOSITY, 0)
+    want = np.float32(1.45) / np.float32(3.5) * np.float32(1.6)
+    assert abs(float(U[6, 6, 0]) - float(want)) <= 4e-7, (U[6, 6, 0], want)
+    assert U[6, 5, 0] == np.float32(0.2) and U[6, 7, 0] == np.float32(0.4)    # black faces left alone
+    x = float(U[6, 6, 0])
+    h.run(PASS_UNPRESS)
+    last = h.field[4 * (2 * h.C + F + (F + 3) // 4):][:F]
+    assert abs(float(U[6, 6, 0]) - (x + 0.2)) <= 4e-7, (U[6, 6, 0], x + 0.2)
+    assert abs(float(last[6 * (W + 1) + 6]) - (x - 1.0)) <= 4e-7, last[6 * (W + 1) + 6]
+
+
 # ------------------------------------------------------------ gravity
 def pin_gravity(L):
     """The gravity pass's field (pull.wgsl:13-44): the faces' pull is
@@ -1182,4 +1254,5 @@ PINS = {
     "push_gas_out": pin_push_gas_out,
     "occupants": pin_occupants,
     "gravity": pin_gravity,
+    "viscosity": pin_viscosity,
 }

@@ -45,6 +45,8 @@ CAUGHT_BY = {
     29: "occupants",               # liquid left drawn under a person
     30: "gravity",                 # the field's bilinear weights mirrored in x
     31: "gravity",                 # down taken as Earth's whatever the field
+    32: "viscosity",               # the sweeps unrelaxed (omega 1)
+    33: "viscosity",               # an obstacle's face across the flow counted once
 }
 
 

@@ -29,6 +29,7 @@ __global__ void k_edgeJoin(WorldGrid g, int mode);
 __global__ void k_sealAnchor(WorldGrid g);
 __global__ void k_classify(WorldGrid g);
 __global__ void k_setFaces(WorldGrid g);
+__global__ void k_viscosity(WorldGrid g, int color);
 __global__ void k_divergence(WorldGrid g);
 template <int S> __global__ void k_relax(WorldGrid g, int post);
 template <int S> __global__ void k_restrict(WorldGrid g);
@@ -93,7 +94,7 @@ int wig_fail_message(int code, const char *msg) { return fail(code, "%s", msg); 
 
 // Kernel ids for wig_time_kernel, in launch order.
 #define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compactMarks) X(compact) X(retire) X(clear) X(gather) \
-    X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(divergence) X(cgStart) X(cgAnchor) \
+    X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(viscosity) X(divergence) X(cgStart) X(cgAnchor) \
     X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
     X(cgRho) X(cgBeta) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgTrue) X(cgCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
@@ -417,7 +418,8 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
             return nullptr;
         }
     }
-    if (hipMalloc((void **)&w->kind, 256) != hipSuccess || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
+    if (hipMalloc((void **)&w->kind, 256 + 256 * sizeof(WigElementProps)) != hipSuccess
+        || hipMalloc((void **)&w->scratch, 4ull * w->d.B) != hipSuccess
         || hipMalloc(&w->fine, 77ull * 64ull * w->d.B + 256u) != hipSuccess
         || hipMalloc(&w->vlevels, w->vlevel_bytes) != hipSuccess
         || hipMalloc((void **)&w->totals, sizeof(WigRunTotals)) != hipSuccess) {
@@ -427,7 +429,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     }
     for (int p = 0; p < WIG_PART_COUNT; p++)
         if (zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
-    zero(w->own, w->kind, 256);
+    zero(w->own, w->kind, 256 + 256 * sizeof(WigElementProps));
     zero(w->own, w->fine, 77ull * 64ull * w->d.B + 256u);
     zero(w->own, w->vlevels, w->vlevel_bytes);
     zero(w->own, w->totals, sizeof(WigRunTotals));
@@ -474,6 +476,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     d.occ = (const uint32_t *)w->part[WIG_PART_OCCUPANTS];
     d.grav = (const float4 *)w->part[WIG_PART_GRAVITY];
     d.kind = w->kind;
+    d.eprops = (const WigElementProps *)(w->kind + 256);   // (the kinds' 256 bytes, then the columns)
     d.scratch = w->scratch;
     {
         // (block-major, 64 cells a block: packIndex)
@@ -545,6 +548,13 @@ static hipStream_t stream_of(const WigWorld *w) { return w->solo->stream; }
 int wig_set_kinds(WigWorld *w, const uint8_t kinds[256]) {
     if (!w || !kinds) return fail(WIG_E_INVALID, "null argument");
     HIPCHECK(hipMemcpyAsync(w->kind, kinds, 256, hipMemcpyHostToDevice, stream_of(w)));
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));
+    return WIG_OK;
+}
+
+int wig_set_element_props(WigWorld *w, const WigElementProps props[256]) {
+    if (!w || !props) return fail(WIG_E_INVALID, "null argument");
+    HIPCHECK(hipMemcpyAsync(w->kind + 256, props, 256 * sizeof(WigElementProps), hipMemcpyHostToDevice, stream_of(w)));
     HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
 }
@@ -785,8 +795,8 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d = w->d;
     if (P->size[0] != (uint32_t)d.W || P->size[1] != (uint32_t)d.H)
         return fail(WIG_E_INVALID, "params size %ux%u is not the world's %dx%d", P->size[0], P->size[1], d.W, d.H);
-    if (ctl->viscous || ctl->surface || ctl->bodies_run)
-        return fail(WIG_E_INVALID, "viscosity, surface tension and bodies are not built (water only)");
+    if (ctl->surface || ctl->bodies_run)
+        return fail(WIG_E_INVALID, "surface tension and bodies are not built");
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u
         || ctl->level_pairs > MAX_LEVEL_PAIRS || ctl->vcycle_storage > WIG_STORE_BF16 || ctl->residual_every > WIG_MAX_CYCLES)
         return fail(WIG_E_INVALID, "solver control out of range");
@@ -810,6 +820,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d.air_on = ctl->air ? 1u : 0u;
     d.occ_on = ctl->occupants ? 1u : 0u;
     d.grav_on = ctl->gravity ? 1u : 0u;
+    d.visc_on = ctl->viscous ? 1u : 0u;
     d.surface_drag = ctl->surface_drag;
     d.dt = P->dt;
     d.dx = P->dx;
@@ -823,6 +834,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
 struct StepPlan {
     bool any_full = false;
     bool replace = false;    // a full world replaces its residual (k_cgTrue in the iteration)
+    bool viscous = false;    // a full world runs viscosity's sweeps
     uint32_t stores = 0;     // the V-cycle storages (1 << STORE_*) of the full worlds
     uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
     size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
@@ -839,7 +851,8 @@ static void enqueue_head(Batch &b) {
     launch(b, KID_compact, k_compact, SH_ONE, 1024);
     launch(b, KID_retire, k_retire, SH_BG, 64);
 }
-static void enqueue_pre(Batch &b) {
+#define VISCOSITY_ITERATIONS 8   // js/sim.js:71
+static void enqueue_pre(Batch &b, const StepPlan &pl) {
     launch(b, KID_clear, k_clear, SH_BG, 64);
     launch(b, KID_gather, k_gather, SH_PG, 256);
     launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 0);
@@ -847,6 +860,13 @@ static void enqueue_pre(Batch &b) {
     launch(b, KID_sealAnchor, k_sealAnchor, SH_BG, 64);
     launch(b, KID_classify, k_classify, SH_BG, 64);
     launch(b, KID_setFaces, k_setFaces, SH_BG, 64);
+    // viscosity's sweeps before the projection; unpress whether they ran or
+    // not (js/sim.js:2505-2514)
+    for (int i = 0; pl.viscous && i < VISCOSITY_ITERATIONS; i++) {
+        launch(b, KID_viscosity, k_viscosity, SH_BG, 64, 0);
+        launch(b, KID_viscosity, k_viscosity, SH_BG, 64, 1);
+    }
+    launch(b, KID_viscosity, k_viscosity, SH_BG, 64, 2);
     launch(b, KID_divergence, k_divergence, SH_BG, 64);
 }
 extern "C++" {
@@ -920,7 +940,7 @@ static int run_phase(Batch &b, int ph, const StepPlan &pl) {
     auto enqueue = [&]() {
         switch (ph) {
         case PH_HEAD: enqueue_head(b); break;
-        case PH_PRE: enqueue_pre(b); break;
+        case PH_PRE: enqueue_pre(b, pl); break;
         case PH_START: enqueue_solve_start(b, pl); break;
         case PH_ITER: enqueue_iteration(b, pl); break;
         case PH_FINISH:
@@ -1021,6 +1041,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         pl.any_full = true;
         pl.stores |= 1u << d.store;
         pl.replace = pl.replace || d.replace_every != 0u;
+        pl.viscous = pl.viscous || d.visc_on != 0u;
         pl.max_cycles = std::max(pl.max_cycles, d.fluid_cycles);
         if (d.first_chunk != 0u) {
             uint32_t f = d.first_chunk == WIG_CHUNK_AUTO ? std::max(b.w[i]->last_hint, 1u) : d.first_chunk;
@@ -1042,6 +1063,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         sig.push_back(pl.coarse_lds);
         sig.push_back(pl.stores);
         sig.push_back(pl.replace);
+        sig.push_back(pl.viscous);
         sig.push_back(pl.max_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(pl.down_lds[l]), sig.push_back(pl.up_lds[l]);
         sig.push_back((uint64_t)(uintptr_t)b.stream);

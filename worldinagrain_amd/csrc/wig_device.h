@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/wig.h"
+
 #define WIG_BLOCK 8
 #define LIQUID_CELL_WORDS 11u
 #define LIQUID_PER_CELL 4u
@@ -147,6 +149,7 @@ struct Dev {
     SolveHead *head;
     double *bpart;
     const uint8_t *kind;
+    const WigElementProps *eprops;   // the elements' liquid columns (wig_set_element_props)
     uint32_t *scratch;      // per-block flags and offsets (compact)
     unsigned long long *totals;  // steps, PCG iterations, particles, active cells (run totals)
     uint32_t *wstate;       // WS_* words
@@ -203,7 +206,7 @@ struct Dev {
     const uint32_t *occ;         // occupants (WIG_PART_OCCUPANTS), read when occ_on
     const float4 *grav;          // the gravity pass's field (WIG_PART_GRAVITY), read when grav_on
     uint32_t *source;
-    uint32_t air_on, occ_on, grav_on;
+    uint32_t air_on, occ_on, grav_on, visc_on;
     float surface_drag, dt, dx;
 };
 

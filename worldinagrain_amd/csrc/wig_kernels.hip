@@ -220,11 +220,32 @@ __device__ int32_t countAt(const Dev &d, i2 c) {                           // :1
     if (isObstacleCell(d, c, d.cells[4ull * i])) return -1;
     return (int32_t)CNT(d, i, G_COUNT);
 }
+// e^x from basic float operations (round to nearest, no fused
+// multiply-add): x = k ln2 + r, k = rint(x / ln2), ln2 split in two (Cody
+// and Waite); e^r by its Taylor series to r^7 on |r| <= ln2 / 2; times 2^k
+__device__ __forceinline__ float wexpf(float x) {
+    if (!(x < 88.5f)) return x != x ? x : __int_as_float(0x7f800000);
+    if (x < -103.0f) return 0.0f;
+    const float k = rintf(x * 1.44269504f);
+    const float r = (x - k * 0.693145752f) - k * 1.42860677e-6f;
+    const float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f + r * (0.00833333377f
+                    + r * (0.00138888892f + r * 0.000198412701f))))));
+    return ldexpf(p, (int)k);
+}
+// viscosityAt (common.wgsl:1486-1493): a liquid's viscosity at temp
+#define VISCOSITY_MOST 1e4f
+__device__ __forceinline__ float viscosityAt(const WigElementProps &e, float temp) {
+    if (e.viscosity_e <= 0.0f) return e.viscosity;
+    return fminf(e.viscosity * wexpf((e.viscosity_t - temp) / e.viscosity_e), VISCOSITY_MOST);
+}
 __device__ v2 liquidIn(const Dev &d, uint32_t i) {                         // :1165-1179
     uint32_t vx = d.cells[4ull * i];
     float temp = fbits(d.cells[4ull * i + 1]);
     uint32_t k = kindOf(d, wType(vx));
-    float viscosity = 0.0f;
+    // (at the temperature it acts at: water's effTempIn, every liquid having
+    // water's phase points here)
+    const WigElementProps e = d.eprops[wType(vx)];
+    float viscosity = k == KIND_LIQUID ? viscosityAt(e, effTemp(temp)) : 0.0f;
     uint32_t n = CNT(d, i, G_COUNT);
     if (n > 0u) return V2((float)CNT(d, i, G_MASS) / (FIXF * (float)n), viscosity);
     if (k != KIND_LIQUID) return V2(1.0f, 0.0f);
@@ -1120,8 +1141,7 @@ __global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
     }
 }
 
-// setFaces (:1936-1994), with unpress's part for faces no viscosity solves
-// (:3444-3446)
+// setFaces (:1936-1994)
 __device__ __forceinline__ float obstacleVel(const Dev &d, i2 c, int axis) {
     if (!inWorld(d, c)) return 0.0f;
     const uint32_t held = occupantAt(d, cellIndex(d, c));
@@ -1143,6 +1163,15 @@ __device__ __forceinline__ float faceViscosity(float4 ga, float4 gb) {
     if (ga.z < 0.0f || gb.z < 0.0f) return 0.0f;
     float fa = ga.z > 0.0f ? 1.0f : 0.0f, fb = gb.z > 0.0f ? 1.0f : 0.0f;
     return (fa * ga.w + fb * gb.w) / fmaxf(fa + fb, 1.0f);
+}
+// wasLiquid (:3315-3317) and lastPush (:3305-3310): last step's pressure
+// push across a face
+__device__ __forceinline__ bool wasLiquid(const Dev &d, i2 c) {
+    return (wu32(FLD(d, 2u * cellIndex(d, c) + 1u, 3)) & stepBit(d, 1u)) != 0u;
+}
+__device__ __forceinline__ float lastPush(const Dev &d, i2 a, i2 at, float4 ga, float4 gb) {
+    if ((ga.z > 0.0f && !wasLiquid(d, a)) || (gb.z > 0.0f && !wasLiquid(d, at))) return 0.0f;
+    return fineBeta(ga.z, gb.z, densityAt(d, a), densityAt(d, at)) * (gb.x - ga.x);
 }
 __device__ void setFace(const Dev &d, i2 at, bool isU) {
     uint32_t f = faceIndex(d, at, isU);
@@ -1167,9 +1196,14 @@ __device__ void setFace(const Dev &d, i2 at, bool isU) {
         const v2 pull = pullAt(d, isU ? V2((float)at.x, (float)at.y + 0.5f) : V2((float)at.x + 0.5f, (float)at.y));
         now += isU ? pull.x : pull.y;
     }
-    *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(now, u, held, faceViscosity(ga, gb));
+    float start = now;
+    const float nu = faceViscosity(ga, gb);
+    if (nu > 0.0f) {            // isViscousFace: last step's push off, the sweeps' last change on (:1975-1978)
+        now -= lastPush(d, a, at, ga, gb);
+        start = now + FLD(d, viscLastSlot(d, f), f % 4u);
+    }
+    *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(start, u, held, nu);
     FLD(d, viscRhsSlot(d, f), f % 4u) = now;
-    if (FLD(d, viscLastSlot(d, f), f % 4u) != 0.0f) FLD(d, viscLastSlot(d, f), f % 4u) = 0.0f;
 }
 __global__ void __launch_bounds__(64) k_setFaces(WorldGrid g) {
     WORLD(g);
@@ -1181,6 +1215,86 @@ __global__ void __launch_bounds__(64) k_setFaces(WorldGrid g) {
         facesOf(d, c, fs);
         for (int j = 0; j < 4; j++)
             if (fs[j].isU >= 0) setFace(d, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
+    }
+}
+
+// viscousFace (:3361-3403): a red-black Gauss-Seidel step of implicit
+// viscosity on the face at `at` if it is of `color` (no bodies: heavy 1)
+#define VISC_OMEGA 1.6f
+__device__ void viscousFace(const Dev &d, i2 at, bool isU, int color) {
+    if (((at.x + at.y) & 1) != color) return;
+    const i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+    const float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
+    if (ga.z < 0.0f || gb.z < 0.0f) return;
+    const uint32_t f = faceIndex(d, at, isU);
+    const uint64_t fs = faceSlot(d, f);
+    const float nu = FLD(d, fs, 3);
+    if (nu <= 0.0f) return;
+    const float heavy = 1.0f;
+    const float through[2] = {ga.z > 0.0f ? ga.w : 0.0f, gb.z > 0.0f ? gb.w : 0.0f};
+    // the four faces beside along the same axis, loaded at once
+    float4 other[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const i2 o = beside(at, k);
+        const bool in = faceInWorld(d, o, isU);
+        const float4 v = *(const float4 *)&FLD(d, faceSlot(d, faceIndex(d, in ? o : at, isU)), 0);
+        other[k] = in ? v : make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+    }
+    float sum = 0.0f, n = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool across = isU == (k >= 2);
+        float kk = (k & 1) ? through[1] : through[0];
+        if (across) kk = other[k].z < 0.0f ? nu : 2.0f * nu * other[k].w / fmaxf(nu + other[k].w, 1e-12f);
+        if (other[k].z < 0.0f) {
+            n += kk * (across ? 2.0f : 1.0f);
+        } else if (other[k].z > 0.0f) {
+            sum += kk * other[k].x;
+            n += kk;
+        }
+    }
+    const float rhs = FLD(d, viscRhsSlot(d, f), f % 4u);
+    FLD(d, fs, 0) = wmix(FLD(d, fs, 0), (rhs + heavy * sum) / (1.0f + heavy * n), VISC_OMEGA);
+}
+// viscosityRed / viscosityBlack (color 0, 1) and unpress (2) (:3405-3466):
+// over the blocks worked on with viscous liquid in or beside them, each
+// cell's faces.  unpress: on the faces the sweeps solve, what they did kept
+// for the next step (viscLast) and last step's push back on; on the rest,
+// viscLast cleared.
+__global__ void __launch_bounds__(64) k_viscosity(WorldGrid g, int color) {
+    WORLD(g);
+    if (!d.full || (color < 2 && !d.visc_on)) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 b = activeBlockAt(d, blk_);
+        uint32_t near = 0u;
+        for (int32_t y = b.y - 1; y <= b.y + 1; y++)
+            for (int32_t x = b.x - 1; x <= b.x + 1; x++)
+                if (inBlocks(d, I2(x, y))) near |= d.blocks[STATE_SLOT(d, blockIndex(d, I2(x, y)))];
+        if ((near & HAS_VISCOUS) == 0u) continue;
+        const i2 c = blockCell(b, threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        Face3 fs[4];
+        facesOf(d, c, fs);
+        for (int j = 0; j < 4; j++) {
+            if (fs[j].isU < 0) continue;
+            const i2 at = I2(fs[j].x, fs[j].y);
+            const bool isU = fs[j].isU == 1;
+            if (color < 2) {
+                viscousFace(d, at, isU, color);
+                continue;
+            }
+            const i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+            const float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
+            const uint32_t f = faceIndex(d, at, isU);
+            if (faceViscosity(ga, gb) > 0.0f) {
+                const uint64_t fs0 = faceSlot(d, f);
+                FLD(d, viscLastSlot(d, f), f % 4u) = FLD(d, fs0, 0) - FLD(d, viscRhsSlot(d, f), f % 4u);
+                FLD(d, fs0, 0) += lastPush(d, a, at, ga, gb);
+            } else if (FLD(d, viscLastSlot(d, f), f % 4u) != 0.0f) {
+                FLD(d, viscLastSlot(d, f), f % 4u) = 0.0f;
+            }
+        }
     }
 }
 

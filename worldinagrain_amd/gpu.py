@@ -28,6 +28,7 @@ def lib():
         L.wig_create.argtypes = [ct.c_uint32, ct.c_uint32, ct.c_uint32, ct.c_uint32]
         L.wig_destroy.argtypes = [ct.c_void_p]
         L.wig_set_kinds.argtypes = [ct.c_void_p, ct.c_char_p]
+        L.wig_set_element_props.argtypes = [ct.c_void_p, ct.c_void_p]
         L.wig_layout.argtypes = [ct.c_void_p, ct.POINTER(abi.WigLayoutInfo)]
         L.wig_upload.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_size_t]
         L.wig_download.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_size_t]
@@ -137,6 +138,13 @@ class GpuWorld:
         assert k.shape == (256,)
         _check(lib().wig_set_kinds(self._w, k.tobytes()))
         self.kinds = k
+
+    def set_element_props(self, props):
+        """The elements' liquid columns (wig_set_element_props): (256, 5) f32,
+        viscosity, viscosity_e, viscosity_t, tension, wetting."""
+        p = np.ascontiguousarray(props, dtype=np.float32)
+        assert p.shape == (256, 5)
+        _check(lib().wig_set_element_props(self._w, p.ctypes.data))
 
     def part_bytes(self, p):
         return int(self.layout.part_bytes[p])

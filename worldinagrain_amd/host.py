@@ -134,6 +134,7 @@ class Simulation:
         self.shifted_first = not self.shifted_first       # js/sim.js:2530
         c = abi.WigStepCtl()
         c.full = 1 if self.fluid_stats["active"] else 0
+        c.viscous = 1 if self.fluid_stats["viscous"] else 0      # viscosity's sweeps (js/sim.js:2509)
         c.fluid_cycles = self.fluid_cycles if s.adaptive_cycles else s.max_cycles
         c.shifted_first = 1 if self.shifted_first else 0
         c.tol_mode = s.tol_mode
@@ -245,6 +246,7 @@ class BatchHost:
         self._frame = abi.WigParams.frame.offset // 4
         self._seed = abi.WigParams.seed.offset // 4
         self._full = abi.WigStepCtl.full.offset // 4
+        self._visc = abi.WigStepCtl.viscous.offset // 4
         self._cycles = abi.WigStepCtl.fluid_cycles.offset // 4
         self._shift = abi.WigStepCtl.shifted_first.offset // 4
         self.adaptive = np.array([s.solver.adaptive_cycles for s in self.sims])
@@ -259,6 +261,7 @@ class BatchHost:
         pw[:, self._frame] = np.array([s.frame_no & 0xFFFFFFFF for s in sims], dtype=np.uint32)
         pw[:, self._seed] = seeds
         cw[:, self._full] = np.array([1 if s.fluid_stats["active"] else 0 for s in sims], dtype=np.uint32)
+        cw[:, self._visc] = np.array([1 if s.fluid_stats["viscous"] else 0 for s in sims], dtype=np.uint32)
         cyc = np.array([s.fluid_cycles for s in sims], dtype=np.uint32)
         cw[:, self._cycles] = np.where(self.adaptive, cyc, self.cap)
         cw[:, self._shift] = shifted.astype(np.uint32)

@@ -120,6 +120,7 @@ class LiquidSimulation {
         const u = new Uint32Array(buf);
         const f = new Float32Array(buf);
         u[0] = this.fluidStats.active ? 1 : 0;
+        u[1] = this.fluidStats.viscous ? 1 : 0;      // viscosity's sweeps (js/sim.js:2509)
         u[4] = this.solver.adaptive ? this.fluidCycles : this.solver.maxCycles;
         u[5] = this.shiftedFirst ? 1 : 0;
         u[6] = this.solver.tolMode;
