@@ -16,7 +16,7 @@ _libs = {}
 (PASS_SYNC, PASS_SYNC_PARTICLES, PASS_SYNC_SEED, PASS_COMPACT, PASS_RETIRE, PASS_CLEAR, PASS_GATHER, PASS_SEAL,
  PASS_CLASSIFY, PASS_SET_FACES, PASS_UNPRESS, PASS_DIVERGENCE, PASS_RESTRICT, PASS_SMOOTH, PASS_COARSEST, PASS_PROLONG,
  PASS_PROLONG_FINE, PASS_PROJECT, PASS_EXTRAPOLATE, PASS_ADVECT, PASS_SORT, PASS_RASTER_CLAIMS, PASS_RASTER,
- PASS_UNPAINT, PASS_RELAX, PASS_VISCOSITY) = range(26)
+ PASS_UNPAINT, PASS_RELAX, PASS_VISCOSITY, PASS_SURFACE) = range(27)
 
 
 OMP_PATH = os.path.join(_HERE, "libwig_oracle_omp.so")

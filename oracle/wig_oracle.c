@@ -1160,6 +1160,57 @@ static float columnWeight(OWorld *w, i2 p, float density) {  /* :1566-1575 */
     return density * w->P->earthG * ((float)over + 0.5f);
 }
 
+/* surface tension (:1698-1806): waterTension, the liquid's outline, and the
+ * surface pass, σκ on the open cells beside the liquid */
+#define CAPILLARY_LENGTH 0.0027f                             /* js/shared.js:285-286 */
+#define SURFACE_MIN_LENGTH 0.3f
+#define MAX_CURVATURE 1.0f
+#define CURVATURE_GAIN 2.0f
+#define CONTACT 2.0f
+#define SURFACE_DAMP 1.0f
+static float waterTension(const OWorld *w) {                 /* :1736-1739 */
+    float l = CAPILLARY_LENGTH / w->P->dx;
+    return l >= SURFACE_MIN_LENGTH ? WATER_WEIGHT * l * l : 0.0f;
+}
+static float outline(const OWorld *w, i2 c) {                 /* :1744-1754 */
+    if (!inWorld(w, c)) return 0.5f;
+    uint32_t i = cellIndex(w, c);
+    if (isObstacleCell(w, c, w->cells[4ull * i])) return 0.5f;
+    return CNT(i, G_COUNT) > 0u ? 1.0f : 0.0f;
+}
+static void k_surface(OWorld *w) {                           /* :1760-1807 */
+    FOR_ACTIVE_CELLS(w, c) {
+        if (!inWorld(w, c)) continue;
+        uint32_t i = cellIndex(w, c);
+        if (FLD(2u * i, 2) != 0.0f) continue;
+        float tension = 0.0f, contact = 0.0f;
+        for (int d = 0; d < 4; d++) {
+            i2 q = beside(c, d);
+            if (!inWorld(w, q)) continue;
+            uint32_t j = cellIndex(w, q);
+            float z = FLD(2u * j, 2);
+            if (z > 0.0f) tension = fmaxf(tension, w->props[wType(w->cells[4ull * j])].tension);
+            else if (z < 0.0f) contact += 2.0f * w->props[wType(w->cells[4ull * j])].wetting - 1.0f;
+        }
+        if (tension <= 0.0f) continue;
+        float gx = 0.0f, gy = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f;
+        for (int y = -2; y <= 2; y++)
+            for (int x = -2; x <= 2; x++) {
+                float kx = (float)x, ky = (float)y;
+                float wt = wexpf(-0.5f * (kx * kx + ky * ky)) * outline(w, I2(c.x + x, c.y + y));
+                gx += kx * wt;
+                gy += ky * wt;
+                hx += (kx * kx - 1.0f) * wt;
+                hy += (ky * ky - 1.0f) * wt;
+                hz += (kx * ky) * wt;
+            }
+        float g2 = gx * gx + gy * gy, kappa = 0.0f;
+        if (g2 >= 1e-6f) kappa = -(hx * gy * gy - 2.0f * gx * gy * hz + hy * gx * gx) / (g2 * sqrtf(g2));
+        if (MUTANT(34)) kappa = -kappa;
+        float curved = wclamp(kappa * CURVATURE_GAIN, -MAX_CURVATURE, MAX_CURVATURE);
+        FLD(2u * i, 0) = waterTension(w) * tension * (curved - CONTACT * contact);
+    }
+}
 static void k_classify(OWorld *w) {                          /* :1578-1696 */
     PAR_FOR
     for (uint32_t wid = 0; wid < activeCount(w); wid++) {
@@ -1182,6 +1233,12 @@ static void k_classify(OWorld *w) {                          /* :1578-1696 */
                     v2 li = liquidIn(w, i);
                     c[2] = li.x;
                     c[3] = li.y;
+                    /* liquid at the surface, where surface tension acts, takes on
+                     * its viscosity (SURFACE_DAMP, :1626-1633) */
+                    float sigma = waterTension(w);
+                    if (sigma > 0.0f && (countAt(w, beside(p, 0)) == 0 || countAt(w, beside(p, 1)) == 0
+                                         || countAt(w, beside(p, 2)) == 0 || countAt(w, beside(p, 3)) == 0))
+                        c[3] = fmaxf(c[3], sigma * w->props[wType(w->cells[4ull * i])].tension * SURFACE_DAMP);
                 } else {
                     float fluid = 0.0f, nwalls = 0.0f;
                     v2 mean = V2(0.0f, 0.0f);
@@ -2495,7 +2552,7 @@ static void relaxPass(OWorld *w, int *shifted, uint32_t times) {
 
 static int check_ctl(const OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     if (P->size[0] != (uint32_t)w->W || P->size[1] != (uint32_t)w->H) return WIG_E_INVALID;
-    if (ctl->surface || ctl->bodies_run) return WIG_E_INVALID;
+    if (ctl->bodies_run || (ctl->surface && ctl->residual_every)) return WIG_E_INVALID;
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->level_pairs > 64u) return WIG_E_INVALID;
     if (ctl->tol_mode > WIG_TOL_REL_L2 || ctl->residual_every > WIG_MAX_CYCLES) return WIG_E_INVALID;
     return WIG_OK;
@@ -2525,6 +2582,7 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     k_sealJoin(w);
     k_sealAnchor(w);
     k_classify(w);
+    if (ctl->surface) k_surface(w);
     k_setFaces(w);
     /* viscosity's sweeps before the projection (js/sim.js:2505-2514) */
     for (int i = 0; ctl->viscous && i < VISCOSITY_ITERATIONS; i++) {
@@ -2852,7 +2910,7 @@ enum {
     WOR_PASS_GATHER, WOR_PASS_SEAL, WOR_PASS_CLASSIFY, WOR_PASS_SET_FACES, WOR_PASS_UNPRESS, WOR_PASS_DIVERGENCE,
     WOR_PASS_RESTRICT, WOR_PASS_SMOOTH, WOR_PASS_COARSEST, WOR_PASS_PROLONG, WOR_PASS_PROLONG_FINE, WOR_PASS_PROJECT,
     WOR_PASS_EXTRAPOLATE, WOR_PASS_ADVECT, WOR_PASS_SORT, WOR_PASS_RASTER_CLAIMS, WOR_PASS_RASTER, WOR_PASS_UNPAINT,
-    WOR_PASS_RELAX, WOR_PASS_VISCOSITY, WOR_PASSES
+    WOR_PASS_RELAX, WOR_PASS_VISCOSITY, WOR_PASS_SURFACE, WOR_PASSES
 };
 int wor_run(OWorld *w, int pass, int arg, const WigParams *P, const WigStepCtl *ctl) {
     if (pass < 0 || pass >= WOR_PASSES) return WIG_E_INVALID;
@@ -2891,6 +2949,7 @@ int wor_run(OWorld *w, int pass, int arg, const WigParams *P, const WigStepCtl *
     case WOR_PASS_UNPAINT: k_unpaint(w); break;
     case WOR_PASS_RELAX: k_relax(w, arg ? BLOCK / 2 : 0); break;
     case WOR_PASS_VISCOSITY: k_viscosity(w, arg & 1); break;
+    case WOR_PASS_SURFACE: k_surface(w); break;
     }
     return WIG_OK;
 }

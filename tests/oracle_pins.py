@@ -15,7 +15,7 @@ import numpy as np
 from oracle.oracle import (OracleWorld, PASS_ADVECT, PASS_CLASSIFY, PASS_COARSEST, PASS_COMPACT, PASS_DIVERGENCE,
                            PASS_EXTRAPOLATE, PASS_PROLONG, PASS_PROLONG_FINE, PASS_RASTER, PASS_RELAX, PASS_RESTRICT,
                            PASS_RASTER_CLAIMS, PASS_SEAL, PASS_SET_FACES, PASS_SMOOTH, PASS_SORT, PASS_SYNC_PARTICLES,
-                           PASS_UNPAINT, PASS_UNPRESS, PASS_VISCOSITY)
+                           PASS_UNPAINT, PASS_UNPRESS, PASS_VISCOSITY, PASS_SURFACE)
 from worldinagrain_amd import abi, scenes
 from worldinagrain_amd import elements as E
 from worldinagrain_amd.host import Simulation, SolverSettings
@@ -1181,6 +1181,79 @@ def pin_viscosity(L):
     assert abs(float(last[6 * (W + 1) + 6]) - (x - 1.0)) <= 4e-7, last[6 * (W + 1) + 6]
 
 
+# ------------------------------------------------------------ surface tension
+def pin_surface(L):
+    """Surface tension (fluid.wgsl:1698-1807): with millimetre cells the
+    open cells beside the liquid hold σκ, σ = ρ g l² (l the capillary length
+    in cells); κ from the liquid's smoothed outline is positive outside a
+    drop and negative inside a bubble, and larger for a smaller drop; a solid
+    that the liquid wets fully (cos θ = 1) beside an open cell lowers its
+    pressure by CONTACT σ against one at cos θ = 0; at centimetre cells
+    (l < 0.3) nothing is written; and classify gives the liquid at the
+    surface a viscosity of σ times its tension (SURFACE_DAMP)."""
+    W = 32
+    kinds = E.kinds()
+    yy, xx = np.mgrid[0:W, 0:W]
+
+    def run(liquid, wall=None, wetting=0.5, dx=0.001):
+        props = np.zeros((256, 5), dtype=np.float32)
+        props[E.WATER, 3] = 1.0
+        props[E.WALL, 4] = wetting
+        h = Hand(W, W, L=L, kinds=kinds)
+        h.w.set_element_props(props)
+        h.activate()
+        h.dx = dx
+        words, cell, cnt = h.words(), h.cell(), h.cnt()
+        words[..., 0] = word(E.AIR)
+        cell[..., 2] = 0.0
+        cell[..., 0] = 0.0
+        words[liquid, 0] = word(E.WATER)
+        cnt[liquid, G_COUNT] = 4
+        cell[liquid, 2] = 1.0
+        if wall is not None:
+            words[wall, 0] = word(E.WALL)
+            cell[wall, 2] = -1.0
+        h.run(PASS_SURFACE)
+        return cell[..., 0].copy()
+
+    l = np.float32(0.0027) / np.float32(0.001)
+    sigma = float(np.float32(np.float32(9.81 / 3600.0 / 0.01) * l) * l)
+    r2 = (xx - 15.5) ** 2 + (yy - 15.5) ** 2
+    ring = lambda r: (r2 > r * r) & (r2 <= (r + 1.2) ** 2)
+    small, big = run(r2 <= 36), run(r2 <= 100)
+    assert small[ring(6)].mean() > 0.0 and big[ring(10)].mean() > 0.0
+    assert small[ring(6)].mean() > 1.1 * big[ring(10)].mean(), (small[ring(6)].mean(), big[ring(10)].mean())
+    inside = (r2 > 36) & (xx > 2) & (yy > 2) & (xx < 29) & (yy < 29)
+    hole = run(inside)
+    near = (r2 <= 36) & (r2 > 16)
+    assert hole[near & (hole != 0.0)].mean() < 0.0
+    # contact: a pool in a walled box, the open cell beside the wall just over it
+    wall = (xx == 3) | (xx == 28) | (yy == 28)
+    pool = (xx > 3) & (xx < 28) & (yy > 20) & (yy < 28)
+    wet, neutral = run(pool, wall, wetting=1.0), run(pool, wall, wetting=0.5)
+    d = float(wet[20, 4]) - float(neutral[20, 4])
+    assert abs(d + 2.0 * sigma) <= 1e-5 * sigma, (d, -2.0 * sigma)
+    assert not np.any(run(r2 <= 36, dx=0.01))                 # centimetre cells: none
+    # classify's damping of the surface
+    h = Hand(W, W, L=L, kinds=kinds)
+    props = np.zeros((256, 5), dtype=np.float32)
+    props[E.WATER, 3] = 1.0
+    h.w.set_element_props(props)
+    h.activate()
+    h.dx = 0.001
+    words, cnt = h.words(), h.cnt()
+    words[..., 0] = word(E.AIR)
+    blob = r2 <= 36
+    words[blob, 0] = word(E.WATER)
+    words[blob, 1] = np.float32(20.0).view(np.uint32)
+    cnt[blob, G_COUNT] = 4
+    cnt[blob, G_MASS] = 4 * 65536
+    h.run(PASS_CLASSIFY)
+    nu = h.cell()[..., 3]
+    assert abs(float(nu[15, 10]) - sigma) <= 1e-6 * sigma, (nu[15, 10], sigma)     # at the surface
+    assert nu[15, 15] == 0.0                                                          # inside
+
+
 # ------------------------------------------------------------ gravity
 def pin_gravity(L):
     """The gravity pass's field (pull.wgsl:13-44): the faces' pull is
@@ -1255,4 +1328,5 @@ PINS = {
     "occupants": pin_occupants,
     "gravity": pin_gravity,
     "viscosity": pin_viscosity,
+    "surface": pin_surface,
 }

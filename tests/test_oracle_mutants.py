@@ -47,6 +47,7 @@ CAUGHT_BY = {
     31: "gravity",                 # down taken as Earth's whatever the field
     32: "viscosity",               # the sweeps unrelaxed (omega 1)
     33: "viscosity",               # an obstacle's face across the flow counted once
+    34: "surface",                 # the curvature's sign flipped
 }
 
 

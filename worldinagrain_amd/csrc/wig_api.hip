@@ -30,6 +30,7 @@ __global__ void k_sealAnchor(WorldGrid g);
 __global__ void k_classify(WorldGrid g);
 __global__ void k_setFaces(WorldGrid g);
 __global__ void k_viscosity(WorldGrid g, int color);
+__global__ void k_surface(WorldGrid g);
 __global__ void k_divergence(WorldGrid g);
 template <int S> __global__ void k_relax(WorldGrid g, int post);
 template <int S> __global__ void k_restrict(WorldGrid g);
@@ -94,7 +95,7 @@ int wig_fail_message(int code, const char *msg) { return fail(code, "%s", msg); 
 
 // Kernel ids for wig_time_kernel, in launch order.
 #define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compactMarks) X(compact) X(retire) X(clear) X(gather) \
-    X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(setFaces) X(viscosity) X(divergence) X(cgStart) X(cgAnchor) \
+    X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(surface) X(setFaces) X(viscosity) X(divergence) X(cgStart) X(cgAnchor) \
     X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
     X(cgRho) X(cgBeta) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgTrue) X(cgCheck) \
     X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
@@ -795,8 +796,9 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d = w->d;
     if (P->size[0] != (uint32_t)d.W || P->size[1] != (uint32_t)d.H)
         return fail(WIG_E_INVALID, "params size %ux%u is not the world's %dx%d", P->size[0], P->size[1], d.W, d.H);
-    if (ctl->surface || ctl->bodies_run)
-        return fail(WIG_E_INVALID, "surface tension and bodies are not built");
+    if (ctl->bodies_run) return fail(WIG_E_INVALID, "bodies are not built");
+    if (ctl->surface && ctl->residual_every)
+        return fail(WIG_E_INVALID, "residual replacement takes the open cells' pressure as 0: not with surface tension");
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u
         || ctl->level_pairs > MAX_LEVEL_PAIRS || ctl->vcycle_storage > WIG_STORE_BF16 || ctl->residual_every > WIG_MAX_CYCLES)
         return fail(WIG_E_INVALID, "solver control out of range");
@@ -821,6 +823,7 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d.occ_on = ctl->occupants ? 1u : 0u;
     d.grav_on = ctl->gravity ? 1u : 0u;
     d.visc_on = ctl->viscous ? 1u : 0u;
+    d.surf_on = ctl->surface ? 1u : 0u;
     d.surface_drag = ctl->surface_drag;
     d.dt = P->dt;
     d.dx = P->dx;
@@ -835,6 +838,7 @@ struct StepPlan {
     bool any_full = false;
     bool replace = false;    // a full world replaces its residual (k_cgTrue in the iteration)
     bool viscous = false;    // a full world runs viscosity's sweeps
+    bool surface = false;    // a full world runs the surface pass
     uint32_t stores = 0;     // the V-cycle storages (1 << STORE_*) of the full worlds
     uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
     size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
@@ -859,6 +863,7 @@ static void enqueue_pre(Batch &b, const StepPlan &pl) {
     launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 0);
     launch(b, KID_sealAnchor, k_sealAnchor, SH_BG, 64);
     launch(b, KID_classify, k_classify, SH_BG, 64);
+    if (pl.surface) launch(b, KID_surface, k_surface, SH_BG, 64);
     launch(b, KID_setFaces, k_setFaces, SH_BG, 64);
     // viscosity's sweeps before the projection; unpress whether they ran or
     // not (js/sim.js:2505-2514)
@@ -1042,6 +1047,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         pl.stores |= 1u << d.store;
         pl.replace = pl.replace || d.replace_every != 0u;
         pl.viscous = pl.viscous || d.visc_on != 0u;
+        pl.surface = pl.surface || d.surf_on != 0u;
         pl.max_cycles = std::max(pl.max_cycles, d.fluid_cycles);
         if (d.first_chunk != 0u) {
             uint32_t f = d.first_chunk == WIG_CHUNK_AUTO ? std::max(b.w[i]->last_hint, 1u) : d.first_chunk;
@@ -1064,6 +1070,7 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         sig.push_back(pl.stores);
         sig.push_back(pl.replace);
         sig.push_back(pl.viscous);
+        sig.push_back(pl.surface);
         sig.push_back(pl.max_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(pl.down_lds[l]), sig.push_back(pl.up_lds[l]);
         sig.push_back((uint64_t)(uintptr_t)b.stream);

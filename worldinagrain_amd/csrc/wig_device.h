@@ -206,7 +206,7 @@ struct Dev {
     const uint32_t *occ;         // occupants (WIG_PART_OCCUPANTS), read when occ_on
     const float4 *grav;          // the gravity pass's field (WIG_PART_GRAVITY), read when grav_on
     uint32_t *source;
-    uint32_t air_on, occ_on, grav_on, visc_on;
+    uint32_t air_on, occ_on, grav_on, visc_on, surf_on;
     float surface_drag, dt, dx;
 };
 

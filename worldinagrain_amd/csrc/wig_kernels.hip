@@ -238,6 +238,18 @@ __device__ __forceinline__ float viscosityAt(const WigElementProps &e, float tem
     if (e.viscosity_e <= 0.0f) return e.viscosity;
     return fminf(e.viscosity * wexpf((e.viscosity_t - temp) / e.viscosity_e), VISCOSITY_MOST);
 }
+// surface tension's constants (fluid.wgsl:1715-1732; js/shared.js:285-286)
+// and waterTension (:1736-1739): water's σ in the solve's units, ρ g l²
+#define CAPILLARY_LENGTH 0.0027f
+#define SURFACE_MIN_LENGTH 0.3f
+#define MAX_CURVATURE 1.0f
+#define CURVATURE_GAIN 2.0f
+#define CONTACT 2.0f
+#define SURFACE_DAMP 1.0f
+__device__ __forceinline__ float waterTension(const Dev &d) {
+    const float l = CAPILLARY_LENGTH / d.dx;
+    return l >= SURFACE_MIN_LENGTH ? WATER_WEIGHT * l * l : 0.0f;
+}
 __device__ v2 liquidIn(const Dev &d, uint32_t i) {                         // :1165-1179
     uint32_t vx = d.cells[4ull * i];
     float temp = fbits(d.cells[4ull * i + 1]);
@@ -1095,6 +1107,12 @@ __global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
                 v2 li = liquidIn(d, i);
                 c[2] = li.x;
                 c[3] = li.y;
+                // liquid at the surface, where surface tension acts, takes on
+                // its viscosity (SURFACE_DAMP, :1626-1633)
+                const float sigma = waterTension(d);
+                if (sigma > 0.0f && (countAt(d, beside(p, 0)) == 0 || countAt(d, beside(p, 1)) == 0
+                                     || countAt(d, beside(p, 2)) == 0 || countAt(d, beside(p, 3)) == 0))
+                    c[3] = fmaxf(c[3], sigma * d.eprops[wType(d.cells[4ull * i])].tension * SURFACE_DAMP);
             } else {
                 float fluid = 0.0f, nwalls = 0.0f;
                 v2 mean = V2(0.0f, 0.0f);
@@ -1138,6 +1156,51 @@ __global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
             atomicOr(&d.blocks[STATE_SLOT(d, blockIndex(d, bk))], bits);
             if (bits & HAS_VISCOUS) d.blocks[COUNT_SLOT(d, COUNT_VISCOUS)] = 1u;
         }
+    }
+}
+
+// surface (:1760-1807): on each open cell beside the liquid, σκ, κ from the
+// liquid's outline smoothed over 5 x 5 cells, and the contact lines' pull
+__device__ __forceinline__ float outline(const Dev &d, i2 c) {            // :1744-1754
+    if (!inWorld(d, c)) return 0.5f;
+    const uint32_t i = cellIndex(d, c);
+    if (isObstacleCell(d, c, d.cells[4ull * i])) return 0.5f;
+    return CNT(d, i, G_COUNT) > 0u ? 1.0f : 0.0f;
+}
+__global__ void __launch_bounds__(64) k_surface(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.surf_on) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        const uint32_t i = cellIndex(d, c);
+        if (FLD(d, 2u * i, 2) != 0.0f) continue;
+        float tension = 0.0f, contact = 0.0f;
+        for (int k = 0; k < 4; k++) {
+            const i2 q = beside(c, k);
+            if (!inWorld(d, q)) continue;
+            const uint32_t j = cellIndex(d, q);
+            const float z = FLD(d, 2u * j, 2);
+            if (z > 0.0f) tension = fmaxf(tension, d.eprops[wType(d.cells[4ull * j])].tension);
+            else if (z < 0.0f) contact += 2.0f * d.eprops[wType(d.cells[4ull * j])].wetting - 1.0f;
+        }
+        if (tension <= 0.0f) continue;
+        float gx = 0.0f, gy = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f;
+        for (int y = -2; y <= 2; y++)
+            for (int x = -2; x <= 2; x++) {
+                const float kx = (float)x, ky = (float)y;
+                const float wt = wexpf(-0.5f * (kx * kx + ky * ky)) * outline(d, I2(c.x + x, c.y + y));
+                gx += kx * wt;
+                gy += ky * wt;
+                hx += (kx * kx - 1.0f) * wt;
+                hy += (ky * ky - 1.0f) * wt;
+                hz += (kx * ky) * wt;
+            }
+        const float g2 = gx * gx + gy * gy;
+        float kappa = 0.0f;
+        if (g2 >= 1e-6f) kappa = -(hx * gy * gy - 2.0f * gx * gy * hz + hy * gx * gx) / (g2 * sqrtf(g2));
+        const float curved = wclamp(kappa * CURVATURE_GAIN, -MAX_CURVATURE, MAX_CURVATURE);
+        FLD(d, 2u * i, 0) = waterTension(d) * tension * (curved - CONTACT * contact);
     }
 }
 

@@ -63,6 +63,16 @@ KARMAN = np.float32(0.41)               # js/shared.js:289
 SURFACE_ROUGHNESS = np.float32(1e-4)    # fluid.wgsl:1890
 
 
+CAPILLARY_LENGTH = 0.0027        # js/shared.js:285-286, water's, metres
+SURFACE_MIN_LENGTH = 0.3
+
+
+def surface_tension(dx):
+    """Whether a cell of dx metres is small enough for surface tension to
+    show (Simulation.surfaceTension, js/sim.js:2449-2454)."""
+    return CAPILLARY_LENGTH / dx >= SURFACE_MIN_LENGTH
+
+
 def surface_drag_coefficient(dx):
     """The wind's drag coefficient along a liquid surface, (KARMAN /
     ln(max(dx / 2 / z0, 2)))^2 in f32 (fluid.wgsl:1884-1895), worked out on
@@ -134,7 +144,9 @@ class Simulation:
         self.shifted_first = not self.shifted_first       # js/sim.js:2530
         c = abi.WigStepCtl()
         c.full = 1 if self.fluid_stats["active"] else 0
-        c.viscous = 1 if self.fluid_stats["viscous"] else 0      # viscosity's sweeps (js/sim.js:2509)
+        # the surface pass, and viscosity's sweeps with it (js/sim.js:2496-2509)
+        c.surface = 1 if surface_tension(self.dx) else 0
+        c.viscous = 1 if self.fluid_stats["viscous"] or c.surface else 0
         c.fluid_cycles = self.fluid_cycles if s.adaptive_cycles else s.max_cycles
         c.shifted_first = 1 if self.shifted_first else 0
         c.tol_mode = s.tol_mode
@@ -247,6 +259,8 @@ class BatchHost:
         self._seed = abi.WigParams.seed.offset // 4
         self._full = abi.WigStepCtl.full.offset // 4
         self._visc = abi.WigStepCtl.viscous.offset // 4
+        self._surf = np.array([1 if surface_tension(s.dx) else 0 for s in self.sims], dtype=np.uint32)
+        self.cw[:, abi.WigStepCtl.surface.offset // 4] = self._surf
         self._cycles = abi.WigStepCtl.fluid_cycles.offset // 4
         self._shift = abi.WigStepCtl.shifted_first.offset // 4
         self.adaptive = np.array([s.solver.adaptive_cycles for s in self.sims])
@@ -261,7 +275,7 @@ class BatchHost:
         pw[:, self._frame] = np.array([s.frame_no & 0xFFFFFFFF for s in sims], dtype=np.uint32)
         pw[:, self._seed] = seeds
         cw[:, self._full] = np.array([1 if s.fluid_stats["active"] else 0 for s in sims], dtype=np.uint32)
-        cw[:, self._visc] = np.array([1 if s.fluid_stats["viscous"] else 0 for s in sims], dtype=np.uint32)
+        cw[:, self._visc] = np.array([1 if s.fluid_stats["viscous"] else 0 for s in sims], dtype=np.uint32) | self._surf
         cyc = np.array([s.fluid_cycles for s in sims], dtype=np.uint32)
         cw[:, self._cycles] = np.where(self.adaptive, cyc, self.cap)
         cw[:, self._shift] = shifted.astype(np.uint32)

@@ -13,6 +13,8 @@ const FLUID_CYCLES = 16;        // js/sim.js:63
 const FLUID_CYCLES_SPARE = 2;   // js/sim.js:70
 const LIQUID_TOLERANCE = 1e-2;  // fluid.wgsl:2221
 const GRAVITY = 9.81 / 3600 / 0.01;   // js/elements.js:30, water's weight
+const CAPILLARY_LENGTH = 0.0027;       // js/shared.js:285-286
+const SURFACE_MIN_LENGTH = 0.3;
 
 const PART = { cells: 0, particles0: 1, particles1: 2, counters: 3, field: 4, blocks: 5, fluidCg: 6, solve: 7, air0: 8,
     airSource: 9, occupants: 10, gravity: 11 };
@@ -112,6 +114,12 @@ class LiquidSimulation {
         return buf;
     }
 
+    // Whether a cell is small enough for surface tension to show
+    // (Simulation.surfaceTension, js/sim.js:2449-2454).
+    surfaceTension() {
+        return CAPILLARY_LENGTH / this.metresPerCell >= SURFACE_MIN_LENGTH;
+    }
+
     // What encodeFluid decides: idle or full, the iterations to encode, the
     // relaxation's starting tiling (flipped each step), the solve's settings.
     stepControl() {
@@ -120,7 +128,9 @@ class LiquidSimulation {
         const u = new Uint32Array(buf);
         const f = new Float32Array(buf);
         u[0] = this.fluidStats.active ? 1 : 0;
-        u[1] = this.fluidStats.viscous ? 1 : 0;      // viscosity's sweeps (js/sim.js:2509)
+        // the surface pass, and viscosity's sweeps with it (js/sim.js:2496-2509)
+        u[2] = this.surfaceTension() ? 1 : 0;
+        u[1] = this.fluidStats.viscous || u[2] ? 1 : 0;
         u[4] = this.solver.adaptive ? this.fluidCycles : this.solver.maxCycles;
         u[5] = this.shiftedFirst ? 1 : 0;
         u[6] = this.solver.tolMode;
