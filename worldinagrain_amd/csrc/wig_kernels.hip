@@ -1871,7 +1871,16 @@ template <int S> __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
 struct SweepSet {
     int32_t j[SWEEP_K];   // the cell (-1: none)
     float b[SWEEP_K][4], diag[SWEEP_K], hx[SWEEP_K], hy[SWEEP_K];
+    bool any[SWEEP_K];    // whether any lane of the wavefront has a cell m (the same in every lane)
 };
+// Each slot's wavefront vote, once the sets are made: a slot no lane of the
+// wavefront uses is passed over whole, its LDS loads included (on the small
+// levels most wavefronts have no cell, and their loads had taken the LDS's
+// cycles all the same).
+__device__ __forceinline__ void sweepVote(SweepSet &s) {
+#pragma unroll
+    for (int m = 0; m < SWEEP_K; m++) s.any[m] = __any(s.j[m] >= 0) != 0;
+}
 __device__ __forceinline__ void sweepNone(SweepSet &s, int m) {
     s.j[m] = -1;
     s.b[m][0] = s.b[m][1] = s.b[m][2] = s.b[m][3] = 0.0f;
@@ -1900,6 +1909,7 @@ __device__ __forceinline__ void sweepHalf(const SweepSet &s, float *px, float *p
     float ox[SWEEP_K], oy[SWEEP_K], nx[SWEEP_K][4], ny[SWEEP_K][4];
 #pragma unroll
     for (int m = 0; m < SWEEP_K; m++) {
+        if (!s.any[m]) continue;
         const int32_t j = s.j[m] < 0 ? 0 : s.j[m];
         const int32_t nb[4] = {j - 1, j + 1, j - w, j + w};
         ox[m] = px[j];
@@ -1913,7 +1923,7 @@ __device__ __forceinline__ void sweepHalf(const SweepSet &s, float *px, float *p
     }
 #pragma unroll
     for (int m = 0; m < SWEEP_K; m++) {
-        if (s.j[m] < 0) continue;
+        if (!s.any[m] || s.j[m] < 0) continue;
         float sx = 0.0f, sy = 0.0f;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -2050,6 +2060,8 @@ __device__ void tileSweeps(const LevelTile &t, uint32_t l, uint32_t pairs) {
                 if (t.rho[j] <= 0.0f) continue;
                 sweepCell(set, m, j, t.bR[j - 1], t.bR[j], t.bD[j - t.R], t.bD[j], h2, t.rx[j], t.ry[j]);
             }
+        sweepVote(red);
+        sweepVote(black);
         for (uint32_t s = 0; s < pairs; s++) {
             sweepHalf(red, t.px, t.py, t.R, 1.0f);
             __syncthreads();
@@ -2354,6 +2366,8 @@ __device__ void ldsSweeps(const LdsLevel &L, uint32_t l, uint32_t pairs, float o
             sweepCell(set, m, (int32_t)j, x > 0u ? L.bR[j - 1u] : 0.0f, L.bR[j], y > 0u ? L.bD[j - L.w] : 0.0f, L.bD[j],
                       h2, L.rx[j], L.ry[j]);
         }
+    sweepVote(red);
+    sweepVote(black);
     for (uint32_t s = 0; s < pairs; s++) {
         sweepHalf(red, L.px, L.py, (int32_t)L.w, omega);
         phaseSync(wg);
