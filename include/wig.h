@@ -65,9 +65,11 @@ typedef struct WigParams {
  * fluid.wgsl:2063,2392). */
 typedef struct WigStepCtl {
     uint32_t full;           /* fluidStats.active || bodiesRun(); 0 takes the idle path (:2478-2485) */
-    uint32_t viscous;        /* viscosity sweeps: must be 0 (water has none; see DESIGN) */
-    uint32_t surface;        /* surface tension pass: must be 0 (off at cm cells) */
-    uint32_t bodies_run;     /* free bodies: must be 0 */
+    uint32_t viscous;        /* fluidStats.viscous || surfaceTension(): the viscosity's sweeps run
+                                (:3277-3466; js/sim.js:2505-2520) */
+    uint32_t surface;        /* surfaceTension(): the surface pass runs (:1698-1807; js/sim.js:2449-2497) */
+    uint32_t bodies_run;     /* bodiesRun(): what the liquid did to the bodies is handed to them (rigidify,
+                                bodyPush: :2944-3218, js/sim.js:2594-2599); needs `bodies` */
     uint32_t fluid_cycles;   /* PCG iterations encoded (sim.js fluidCycles, <= WIG_MAX_CYCLES) */
     uint32_t shifted_first;  /* tiling of the step's first relaxation (sim.js shiftedFirst after its toggle) */
     uint32_t tol_mode;       /* WIG_TOL_* */
@@ -110,6 +112,14 @@ typedef struct WigStepCtl {
                                 :3613,3686), and down for the bubble test, the escape and where
                                 displaced gas goes (:1138, :3577, :3822-3825, :3912).  0: Earth's pull
                                 alone, (0, earthG), down (0, 1) (the part is not read) */
+    uint32_t bodies;         /* 1: free and held bodies, as the reference's bindings 12, 14 and 16
+                                (fluid.wgsl:1181-1475): WIG_PART_MOTION's table and WIG_PART_SWEPT's
+                                words are read, a loose structural solid (its element's `breaks`) of an
+                                alive and weighed body is the body's cell to classify, its faces move
+                                with the body, the solve takes its give (A + J^T M^-1 J: cgResidual,
+                                cgApply), the projection moves its faces with the push found, and
+                                WIG_PART_PUSHES holds the step's push sums.  0: no body anywhere (the
+                                parts are not read).  Not with residual_every (WIG_E_INVALID) */
 } WigStepCtl;
 
 /* V-cycle storage: float in the reference's buffers (results bit-identical
@@ -151,7 +161,25 @@ typedef struct WigStepCtl {
                                   row-major: the masses' pull (x, y; cells a step squared), their potential
                                   (z), and down (w: the step index, 0..7, plus 8 where the pull is too weak
                                   to settle anything) (pull.wgsl:1-44); read */
-#define WIG_PART_COUNT 12
+#define WIG_PART_MOTION 12     /* bodyMotion (binding 12): the `motion` buffer's words up to the end of its
+                                  liquid slots, WIG_MOTION_WORDS u32 (bodies.wgsl's layout: MAX_BODIES
+                                  words, then the bodies' table, MAX_BODIES slots of 116 words (js/shared.js's
+                                  BODY_WORD: pose, velocity, spin, mass, inertia, cells, flags, subcell,
+                                  cargo, mobility), then the contacts, then 4 words a body of what the
+                                  liquid did to it); the table read, the liquid slots added to
+                                  (fluid.wgsl:1211-1228) */
+#define WIG_PART_SWEPT 13      /* the bodies buffer's swept words (binding 14 from common.wgsl's
+                                  bodySweptSlot(size, 0, 0) on): two u32 a cell, for even and odd steps, the
+                                  body that last moved over it (common.wgsl:355-370); read */
+#define WIG_PART_PUSHES 14     /* bodyPushes (binding 16): WIG_BODY_PUSH_WORDS u32 a body, six sets of exact
+                                  sums of a force and a torque (fluid.wgsl:1304-1340); the step's own scratch,
+                                  emptied as it starts */
+#define WIG_PART_COUNT 15
+
+/* The bodies (js/shared.js:299-333). */
+#define WIG_MAX_BODIES 1024u
+#define WIG_BODY_PUSH_WORDS 36u
+#define WIG_MOTION_WORDS (WIG_MAX_BODIES * (1u + 116u) + 16384u * 12u + WIG_MAX_BODIES * 4u)
 
 /* FLUID_STATS words (js/shared.js:271-272). */
 #define WIG_STAT_ACTIVE 0
@@ -195,10 +223,13 @@ int wig_set_kinds(WigWorld *w, const uint8_t kinds[256]);
  * (cells^2 a step) at viscosity_t (degrees C) and its temperature scale
  * viscosity_e (degrees C; <= 0: the same at every temperature), as
  * viscosityAt (common.wgsl:1486-1493) takes them; the surface tension
- * factor (surface, fluid.wgsl:1779); a solid's wetting share (:1781).  All
- * zero until set: no viscosity, no tension. */
+ * factor (surface, fluid.wgsl:1779); a solid's wetting share (:1781); and
+ * whether a structural solid can come loose as a free body's.  All zero
+ * until set: no viscosity, no tension, no loose solid. */
 typedef struct {
     float viscosity, viscosity_e, viscosity_t, tension, wetting;
+    float breaks;   /* 1: elems[t].breakTo != NO_ELEM, a structural solid that can come loose
+                       in a free body (fluid.wgsl:1233-1236); 0: not */
 } WigElementProps;
 int wig_set_element_props(WigWorld *w, const WigElementProps props[256]);
 

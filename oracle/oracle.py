@@ -67,6 +67,8 @@ def lib(path=None):
         L.wor_fine_beta.restype = ct.c_float
         L.wor_fine_beta.argtypes = [ct.c_float] * 4
         L.wor_run.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl)]
+        L.wor_body_probe.argtypes = [ct.c_void_p, ct.POINTER(abi.WigParams), ct.POINTER(abi.WigStepCtl), ct.c_int,
+                                     ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.wor_level_start.restype = ct.c_uint64
         L.wor_level_start.argtypes = [ct.c_void_p, ct.c_uint32]
         _libs[path] = L
@@ -109,9 +111,9 @@ class OracleWorld:
         self.L.wor_set_kinds(self._w, k.tobytes())
 
     def set_element_props(self, props):
-        """(256, 5) f32: viscosity, viscosity_e, viscosity_t, tension, wetting."""
-        p = np.ascontiguousarray(props, dtype=np.float32)
-        assert p.shape == (256, 5)
+        """(256, 6) f32: viscosity, viscosity_e, viscosity_t, tension, wetting,
+        breaks (fewer columns padded with 0)."""
+        p = np.ascontiguousarray(abi.element_props(props))
         self.L.wor_set_element_props(self._w, p.ctypes.data)
 
     def part(self, p):
@@ -159,6 +161,18 @@ class OracleWorld:
         """One pass of the step on the buffers as they stand (wor_run)."""
         st = self.L.wor_run(self._w, pass_id, arg, ct.byref(params), ct.byref(ctl))
         assert st == 0, f"wor_run({pass_id}, {arg}): {st}"
+
+    def body_probe(self, op, ints=(), floats=(), params=None, ctl=None):
+        """One of the bodies' helpers (wor_body_probe's op): (floats out, u32 out)."""
+        ia = (ct.c_int32 * 8)(*[int(v) for v in ints])
+        fa = (ct.c_float * 8)(*[float(v) for v in floats])
+        fo = (ct.c_float * 4)()
+        uo = (ct.c_uint32 * 4)()
+        params = params if params is not None else abi.WigParams()
+        ctl = ctl if ctl is not None else abi.WigStepCtl()
+        st = self.L.wor_body_probe(self._w, ct.byref(params), ct.byref(ctl), op, ia, fa, fo, uo)
+        assert st == 0, f"wor_body_probe({op}): {st}"
+        return list(fo), list(uo)
 
     def level_start(self, l):
         """Where coarse level l starts in the field, in vec4s."""

@@ -8,15 +8,14 @@
  * reference's order, each kernel a loop over its dispatch domain.  Citations
  * are to shaders/fluid.wgsl unless another file is named.
  *
- * Scope (the inputs the reference's other phases supply, fixed here):
- *   - no free or held bodies, no people (occupants all zero);
- *   - gravity: with ctl->gravity the gravity pass's field (pull.wgsl:13-44),
- *     else Earth's pull alone, pullAt = (0, earthG), down = (0, 1);
- *   - the air grid at rest at uniform pressure: the pressure push on surface
- *     particles is zero and the wind's surface drag is left out
- *     (airsample.wgsl:11-14; fluid.wgsl:1914-1932);
- *   - liquid arithmetic with water's constants (js/elements.js:449-457),
- *     viscosity 0, fresh (no solute), surface tension off (dx >= 9 mm).
+ * The inputs the reference's other phases supply come in as parts, each
+ * read only when its WigStepCtl switch is on (else what the reference
+ * computes with none): the air (air[0], airSource), people's bodies
+ * (occupants), the gravity pass's field, and the free and held bodies (the
+ * bodies' table in `motion`, the swept words of the bodies buffer, and the
+ * step's push sums).  Liquid arithmetic takes water's constants
+ * (js/elements.js:449-457), fresh (no solute); viscosity and surface
+ * tension by the element columns of wig_set_element_props.
  * Everything order-dependent in the reference is made order-free there
  * (fixed-point sums, exact sums, claims by atomicMax); where the reference's
  * order is arbitrary (the compaction of `parts`, the active list, ranks in a
@@ -175,6 +174,40 @@ static inline void at_max(uint32_t *p, uint32_t v) { if (*p < v) *p = v; }
 #define SOURCE_MIST 4u
 #define SOURCE_LOST 5u
 #define SOURCE_HEAT 6u
+/* the bodies (js/shared.js:294-333; common.wgsl:353,377-387; fluid.wgsl:1304-1318,2963) */
+#define MAX_BODIES WIG_MAX_BODIES
+#define BODY_TABLE_WORDS 116u
+#define BODY_CONTACTS 16384u
+#define BODY_CONTACT_WORDS 12u
+#define BODY_LIQUID_WORDS 4u
+#define BODY_PUSH_WORDS 36u
+#define BW_POS 0u
+#define BW_VEL 3u
+#define BW_SPIN 5u
+#define BW_MASS 6u
+#define BW_INERTIA 7u
+#define BW_CELLS 13u
+#define BW_FLAGS 18u
+#define BW_SUBCELL 25u
+#define BW_CARGO 105u
+#define BW_CARGO_INERTIA 106u
+#define BW_MOBILITY 108u
+#define BW_MOBILITY_AT 114u
+#define BF_ALIVE 1u
+#define BF_NEW 2u
+#define BF_HELD 32u
+#define BF_WET 64u
+#define FL_LOOSE 8u
+#define RIGID_FIX 16777216.0f
+#define EXACT_COARSE 16777216.0f
+#define LIQUID_WET 0x80000000u
+#define CLAIM_PERIOD 0xfffffu
+#define SEEP_REACH 128
+#define PUSH_P0 0u
+#define PUSH_X 2u
+#define PUSH_F 3u
+#define PUSH_SURFACE 4u
+#define PUSH_DRAG 5u
 
 /* ----------------------------------------------------------- the world */
 typedef struct {
@@ -200,6 +233,9 @@ typedef struct OWorld {
     uint32_t *source;            /* airSource: SOURCE_WORDS an air cell */
     uint32_t *occ;               /* occupants: occupantWord a cell */
     float *grav;                 /* the gravity pass's field: a vec4 a light cell */
+    uint32_t *motion;            /* the bodies' table and liquid slots (WIG_PART_MOTION) */
+    uint32_t *swept;             /* the bodies buffer's swept words, two a cell (WIG_PART_SWEPT) */
+    uint32_t *pushes;            /* the step's push sums (WIG_PART_PUSHES) */
     uint8_t kind[256];
     WigElementProps props[256];  /* the elements' liquid columns */
     int dot_mode;
@@ -410,6 +446,9 @@ static void layout(OWorld *w, uint32_t width, uint32_t height, uint32_t coarse) 
     w->bytes[WIG_PART_AIR_SOURCE] = 4ull * SOURCE_WORDS * w->C;
     w->bytes[WIG_PART_OCCUPANTS] = 4ull * w->C;
     w->bytes[WIG_PART_GRAVITY] = 16ull * (uint64_t)((w->W + 1) / 2) * (uint64_t)((w->H + 1) / 2);
+    w->bytes[WIG_PART_MOTION] = 4ull * WIG_MOTION_WORDS;
+    w->bytes[WIG_PART_SWEPT] = 8ull * w->C;
+    w->bytes[WIG_PART_PUSHES] = 4ull * WIG_MAX_BODIES * WIG_BODY_PUSH_WORDS;
 }
 
 /* ------------------------------------------------------------ address */
@@ -598,6 +637,233 @@ static int32_t countAt(const OWorld *w, i2 c) {              /* :1150-1159 */
     uint32_t i = cellIndex(w, c);
     if (isObstacleCell(w, c, w->cells[4ull * i])) return -1;
     return (int32_t)CNT(i, G_COUNT);
+}
+static void gridAt4(const OWorld *w, i2 c, float g[4]) {
+    if (!inWorld(w, c)) { g[0] = 0.0f; g[1] = 0.0f; g[2] = -1.0f; g[3] = 0.0f; return; }
+    for (int j = 0; j < 4; j++) g[j] = FLD(2u * cellIndex(w, c), j);
+}
+/* ------------------------------------------------------------ bodies
+ * Free and held bodies in the liquid (fluid.wgsl:1181-1475): the bodies'
+ * table in `motion` (bodies.wgsl's, js/shared.js:294-333's BODY_WORD
+ * words), the swept words of the bodies buffer (common.wgsl:355-370), and
+ * the step's push sums (`pushes`).  With ctl->bodies off none is read and
+ * no cell is a body's. */
+typedef struct { float x, y, z; } v3;
+static inline v3 V3(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline int bodiesOn(const OWorld *w) { return w->ctl->bodies != 0u; }
+static inline uint32_t bodyWord(const OWorld *w, uint32_t b, uint32_t k) {   /* :1211-1213 */
+    return w->motion[MAX_BODIES + b * BODY_TABLE_WORDS + k];
+}
+static inline float bodyF(const OWorld *w, uint32_t b, uint32_t k) { return fbits(bodyWord(w, b, k)); }
+static inline v2 bodyV(const OWorld *w, uint32_t b, uint32_t k) { return V2(bodyF(w, b, k), bodyF(w, b, k + 1u)); }
+static inline uint32_t liquidSlot(uint32_t b, uint32_t wd) {                    /* :1226-1228 */
+    return MAX_BODIES * (1u + BODY_TABLE_WORDS) + BODY_CONTACTS * BODY_CONTACT_WORDS + b * BODY_LIQUID_WORDS + wd;
+}
+/* alive and weighed: not found this step, its mass set (:1244-1246) */
+static inline int bodyWeighed(const OWorld *w, uint32_t b) {
+    uint32_t flags = bodyWord(w, b, BW_FLAGS);
+    return !((flags & BF_ALIVE) == 0u || (flags & BF_NEW) != 0u || bodyF(w, b, BW_MASS) <= 0.0f);
+}
+static uint32_t rigidBodyOf(const OWorld *w, const uint32_t *v) {            /* :1233-1248 */
+    if (!bodiesOn(w)) return NONE;
+    uint32_t t = wType(v[0]);
+    if (!wHas(v[0], FL_LOOSE) || kindOf(w, t) != KIND_SOLID || w->props[t].breaks == 0.0f) return NONE;
+    float id = unpack2x16(v[3]).x;
+    if (id < 0.0f || id >= (float)MAX_BODIES) return NONE;
+    uint32_t b = wu32(id);
+    return bodyWeighed(w, b) ? b : NONE;
+}
+static inline uint32_t claimStep(uint32_t frame) { return frame % CLAIM_PERIOD + 1u; }   /* common.wgsl:384-386 */
+/* The body whose cup (seam 0) or seam (1) cell i was stamped as `back`
+ * steps ago (common.wgsl:364-366's two words), alive and weighed when
+ * `weighed` (cupOf :1252-1266, seamOf :1272-1286; cupBefore :3520-3527
+ * asks only for a slot). */
+static uint32_t sweptBody(const OWorld *w, uint32_t i, uint32_t back, uint32_t seam, int weighed) {
+    if (!bodiesOn(w)) return NONE;
+    uint32_t frame = w->P->frame - back;
+    uint32_t key = w->swept[2ull * i + (frame & 1u)];
+    if ((key >> 12) != claimStep(frame) || (key & 1u) != seam) return NONE;
+    uint32_t b = 2047u - ((key >> 1) & 0x7ffu);
+    if (b >= MAX_BODIES) return NONE;
+    if (weighed && !bodyWeighed(w, b)) return NONE;
+    return b;
+}
+static inline uint32_t cupOf(const OWorld *w, uint32_t i) { return sweptBody(w, i, 1u, 0u, 1); }
+static inline uint32_t seamOf(const OWorld *w, uint32_t i) { return sweptBody(w, i, 1u, 1u, 1); }
+static inline uint32_t cupBefore(const OWorld *w, uint32_t i) { return sweptBody(w, i, 2u, 0u, 0); }
+static inline float rigidMass(const OWorld *w, uint32_t b) { return bodyF(w, b, BW_MASS) + bodyF(w, b, BW_CARGO); }   /* :1290-1296 */
+static inline float rigidInertia(const OWorld *w, uint32_t b) { return bodyF(w, b, BW_INERTIA) + bodyF(w, b, BW_CARGO_INERTIA); }
+static inline float bodyDensity(const OWorld *w, uint32_t b) {                  /* :1300-1302 */
+    return rigidMass(w, b) / fmaxf((float)bodyWord(w, b, BW_CELLS), 1.0f);
+}
+static inline uint32_t pushSlot(uint32_t b, uint32_t which, uint32_t k) { return b * BODY_PUSH_WORDS + 6u * which + 2u * k; }
+static void addPush(OWorld *w, uint32_t b, uint32_t which, v3 v) {             /* :1324-1331 */
+    float c[3] = {v.x, v.y, v.z};
+    for (uint32_t k = 0; k < 3u; k++) {
+        u2 e = exactParts(c[k], EXACT_COARSE);
+        uint32_t at = pushSlot(b, which, k);
+        uint32_t old = at_take(&w->pushes[at], e.lo);
+        at_add(&w->pushes[at + 1u], e.hi + (old + e.lo < old ? 1u : 0u));
+    }
+}
+static v3 pushOf(const OWorld *w, uint32_t b, uint32_t which) {               /* :1333-1340 */
+    float c[3];
+    for (uint32_t k = 0; k < 3u; k++) {
+        uint32_t at = pushSlot(b, which, k);
+        u2 e = {w->pushes[at], w->pushes[at + 1u]};
+        c[k] = exactValue(e, EXACT_COARSE);
+    }
+    return V3(c[0], c[1], c[2]);
+}
+static v3 giveWay(const OWorld *w, uint32_t b, v3 g) {                        /* :1349-1356 */
+    if (bodyWord(w, b, BW_MOBILITY_AT) != w->P->frame - 1u)
+        return V3(g.x / rigidMass(w, b), g.y / rigidMass(w, b), MUTANT(35) ? g.z / rigidMass(w, b) : g.z / rigidInertia(w, b));
+    float m[6];
+    for (uint32_t k = 0; k < 6u; k++) m[k] = bodyF(w, b, BW_MOBILITY + k);
+    return V3(m[0] * g.x + m[1] * g.y + m[2] * g.z, m[1] * g.x + m[3] * g.y + m[4] * g.z, m[2] * g.x + m[4] * g.y + m[5] * g.z);
+}
+static inline v3 pushChange(const OWorld *w, uint32_t b, uint32_t which) { return giveWay(w, b, pushOf(w, b, which)); }
+static inline v3 pushChangeFree(const OWorld *w, uint32_t b, uint32_t which) {   /* :1367-1370 */
+    v3 g = pushOf(w, b, which);
+    return V3(g.x / rigidMass(w, b), g.y / rigidMass(w, b), g.z / rigidInertia(w, b));
+}
+/* the face from cell c to body b's cell q: (n, r x n) (:1374-1378) */
+static v3 bodyFaceJ(const OWorld *w, i2 c, i2 q, uint32_t b) {
+    v2 n = V2((float)(q.x - c.x), (float)(q.y - c.y));
+    v2 pos = bodyV(w, b, BW_POS);
+    v2 r = V2((float)c.x + 0.5f + 0.5f * n.x - pos.x, (float)c.y + 0.5f + 0.5f * n.y - pos.y);
+    return V3(n.x, n.y, MUTANT(36) ? r.y * n.x - r.x * n.y : r.x * n.y - r.y * n.x);
+}
+/* the body whose cell c is in the solve, as classify found it (:1422-1429) */
+static uint32_t rigidAt(const OWorld *w, i2 c) {
+    if (!bodiesOn(w) || !inWorld(w, c)) return NONE;
+    float z = FLD(2u * cellIndex(w, c) + 1u, 2);
+    uint32_t b = wu32(fmaxf(-z, 1.0f)) - 1u;
+    return z < 0.0f && b < MAX_BODIES ? b : NONE;
+}
+static uint32_t heldAt(const OWorld *w, i2 c) {                               /* :1450-1457 */
+    if (!bodiesOn(w) || !inWorld(w, c)) return NONE;
+    float z = FLD(2u * cellIndex(w, c) + 1u, 2);
+    uint32_t b = wu32(fmaxf(-z, 1.0f)) - 1u;
+    return z < 0.0f && b >= MAX_BODIES ? b - MAX_BODIES : NONE;
+}
+static inline int isHeld(const OWorld *w, uint32_t b) { return (bodyWord(w, b, BW_FLAGS) & BF_HELD) != 0u; }
+static inline v2 rigidVel(const OWorld *w, uint32_t b, v2 x) {                  /* :1460-1464 */
+    v2 pos = bodyV(w, b, BW_POS), vel = bodyV(w, b, BW_VEL);
+    v2 r = V2(x.x - pos.x, x.y - pos.y);
+    float spin = bodyF(w, b, BW_SPIN);
+    return V2(vel.x + -spin * r.y, vel.y + spin * r.x);
+}
+static inline int shutAt(const OWorld *w, i2 c) { return gridZ(w, c) < 0.0f || rigidAt(w, c) != NONE; }   /* :1468-1470 */
+static inline v2 faceMiddle(i2 at, int isU) { return isU ? V2((float)at.x, (float)at.y + 0.5f) : V2((float)at.x + 0.5f, (float)at.y); }
+static inline float bodyReach(const OWorld *w, uint32_t b, i2 n) {             /* :413-415 */
+    v2 s = unpack2x16(bodyWord(w, b, BW_SUBCELL));
+    return wclamp(0.5f + (s.x * (float)n.x + s.y * (float)n.y), 0.0f, 1.0f);
+}
+static void addPressurePush(OWorld *w, i2 c, float p, uint32_t which) {       /* :1381-1389 */
+    for (int d = 0; d < 4; d++) {
+        i2 q = beside(c, d);
+        uint32_t b = rigidAt(w, q);
+        if (b != NONE) {
+            v3 j = bodyFaceJ(w, c, q, b);
+            addPush(w, b, which, V3(j.x * p, j.y * p, j.z * p));
+        }
+    }
+}
+static float couplingAt(const OWorld *w, i2 c, uint32_t which) {             /* :1393-1403 */
+    float sum = 0.0f;
+    for (int d = 0; d < 4; d++) {
+        i2 q = beside(c, d);
+        uint32_t b = rigidAt(w, q);
+        if (b != NONE) {
+            v3 j = bodyFaceJ(w, c, q, b), ch = pushChange(w, b, which);
+            sum += j.x * ch.x + j.y * ch.y + j.z * ch.z;
+        }
+    }
+    return sum;
+}
+/* body b's velocity at x going into the solve (:1410-1419) */
+static v2 bodyMoving(const OWorld *w, uint32_t b, v2 x, int drag) {
+    v2 pos = bodyV(w, b, BW_POS);
+    v2 pull = pullAt(w, pos);
+    float mass = rigidMass(w, b);
+    v3 s = pushChange(w, b, PUSH_SURFACE), g = giveWay(w, b, V3(mass * pull.x, mass * pull.y, 0.0f));
+    v3 ch = V3(s.x + g.x, s.y + g.y, s.z + g.z);
+    if (drag) {
+        v3 dr = pushChange(w, b, PUSH_DRAG);
+        ch = V3(ch.x + dr.x, ch.y + dr.y, ch.z + dr.z);
+    }
+    v2 r = V2(x.x - pos.x, x.y - pos.y);
+    v2 rv = rigidVel(w, b, x), px = pullAt(w, x);
+    return V2(rv.x + px.x - pull.x + ch.x + -ch.z * r.y, rv.y + px.y - pull.y + ch.y + ch.z * r.x);
+}
+/* the pull's potential at pos, Earth's and the masses' (:2954-2956) */
+static float potentialAt(const OWorld *w, v2 pos) {
+    float z = 0.0f;
+    if (w->ctl->gravity) {
+        float qx = pos.x / (float)GRAVITY_BLOCK - 0.5f, qy = pos.y / (float)GRAVITY_BLOCK - 0.5f;
+        float fx0 = floorf(qx), fy0 = floorf(qy);
+        i2 i = I2(wi32(fx0), wi32(fy0));
+        float fx = qx - fx0, fy = qy - fy0;
+        float a[4], b[4], c[4], e[4];
+        gravityTexel(w, i, a);
+        gravityTexel(w, I2(i.x + 1, i.y), b);
+        gravityTexel(w, I2(i.x, i.y + 1), c);
+        gravityTexel(w, I2(i.x + 1, i.y + 1), e);
+        z = wmix(wmix(a[2], b[2], fx), wmix(c[2], e[2], fx), fy);
+    }
+    return w->P->earthG * ((float)w->P->size[1] - pos.y) + z;
+}
+/* the liquid's pressure seeping in at x on body h's face from c toward n
+ * (:2976-3001) */
+static float seepAt(const OWorld *w, i2 c, i2 n, uint32_t h, int held, v2 x) {
+    i2 ways[2] = {I2(-1, 0), I2(1, 0)};
+    int count = 2;
+    if (n.x != 0) {
+        ways[0] = I2(-n.x, -n.y);
+        count = 1;
+    }
+    int nearest = SEEP_REACH + 1;
+    float p = 0.0f;
+    for (int k2 = 0; k2 < count; k2++) {
+        for (int k = 1; k < nearest; k++) {
+            i2 q = I2(c.x + ways[k2].x * k, c.y + ways[k2].y * k);
+            if ((held ? heldAt(w, q) : rigidAt(w, q)) == h) continue;
+            float g[4];
+            gridAt4(w, q, g);
+            if (g[2] > 0.0f) {
+                nearest = k;
+                p = g[0] + g[2] * (potentialAt(w, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) - potentialAt(w, x));
+            }
+            break;
+        }
+    }
+    return p;
+}
+/* what the liquid's pressure on held body h's cell c does to it (:3010-3035) */
+static void heldPush(const OWorld *w, i2 c, uint32_t h, int32_t out[3]) {
+    v2 pos = bodyV(w, h, BW_POS);
+    v2 f = V2(0.0f, 0.0f);
+    float torque = 0.0f;
+    for (int d = 0; d < 4; d++) {
+        i2 q = beside(c, d);
+        i2 n = I2(q.x - c.x, q.y - c.y);
+        if (heldAt(w, q) == h) continue;
+        float g[4];
+        gridAt4(w, q, g);
+        v2 x = V2((float)c.x + 0.5f + 0.5f * (float)n.x, (float)c.y + 0.5f + 0.5f * (float)n.y);
+        float p = g[0];
+        if (g[2] > 0.0f) p += g[2] * (potentialAt(w, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) - potentialAt(w, x));
+        else if (g[2] < 0.0f) p = seepAt(w, c, n, h, 1, x);
+        v2 push = V2(-p * (float)n.x, -p * (float)n.y);
+        f = V2(f.x + push.x, f.y + push.y);
+        v2 r = V2(x.x - pos.x, x.y - pos.y);
+        torque += r.x * push.y - r.y * push.x;
+    }
+    float m = rigidMass(w, h), in = rigidInertia(w, h);
+    out[0] = wi32(wround(f.x / m * RIGID_FIX));
+    out[1] = wi32(wround(f.y / m * RIGID_FIX));
+    out[2] = wi32(wround(torque / in * RIGID_FIX));
 }
 /* e^x in float from basic operations only (round to nearest, no fused
  * multiply-add), the same steps on the GPU, so both give the same bits:
@@ -801,11 +1067,16 @@ static void markBlockOf(OWorld *w, v2 pos) {                 /* :597-602 */
     if (inBlocks(w, b)) at_or(&w->blocks[blockIndex(w, b)], MARK_LIQUID);
 }
 
+/* a cell of a held body that was wet in the last solve (:583-595) */
+static inline int wetRegionAt(const OWorld *w, const uint32_t *v) {
+    uint32_t b = rigidBodyOf(w, v);
+    return b != NONE && (bodyWord(w, b, BW_FLAGS) & (BF_WET | BF_HELD)) == (BF_WET | BF_HELD);
+}
 static void k_sync(OWorld *w, int claims) {                  /* :608-659 */
     PAR_FOR
     for (uint32_t by = 0; by < w->bh; by++)
         for (uint32_t bx = 0; bx < w->bw; bx++) {
-            int liquidHere = 0;
+            int liquidHere = 0, regionHere = 0;
             for (int ly = 0; ly < BLOCK; ly++)
                 for (int lx = 0; lx < BLOCK; lx++) {
                     i2 c = I2((int32_t)bx * BLOCK + lx, (int32_t)by * BLOCK + ly);
@@ -825,9 +1096,12 @@ static void k_sync(OWorld *w, int claims) {                  /* :608-659 */
                             at_max(&CNT(home, G_DEST), cl);
                         }
                         liquidHere = 1;
+                    } else if (wetRegionAt(w, v)) {
+                        regionHere = 1;
                     }
                 }
             if (liquidHere) w->blocks[by * w->bw + bx] |= MARK_LIQUID;
+            if (regionHere) w->blocks[by * w->bw + bx] |= MARK_REGION;
         }
 }
 
@@ -897,9 +1171,10 @@ static void k_syncParticles(OWorld *w) {                    /* :767-831 */
             keep = 1;
             markBlockOf(w, q.pos);
         } else if ((d & DEST_ASIDE) != 0u) {
-            uint32_t to = d & ~DEST_ASIDE;                        /* no cups: cupOf is NONE */
-            q.pos = V2((float)(to % (uint32_t)w->W) + (q.pos.x - floorf(q.pos.x)),
-                       (float)(to / (uint32_t)w->W) + (q.pos.y - floorf(q.pos.y)));
+            uint32_t to = d & ~DEST_ASIDE;                        /* not into another body's cup (:785-794) */
+            if (cupOf(w, to) == NONE)
+                q.pos = V2((float)(to % (uint32_t)w->W) + (q.pos.x - floorf(q.pos.x)),
+                           (float)(to / (uint32_t)w->W) + (q.pos.y - floorf(q.pos.y)));
             keep = 1;
             markBlockOf(w, q.pos);
         } else if (isClaim(d)) {
@@ -1227,8 +1502,26 @@ static void k_classify(OWorld *w) {                          /* :1578-1696 */
                 for (int j = 0; j < 4; j++) { was[j] = FLD(2u * i, j); wasDrift[j] = FLD(2u * i + 1u, j); }
                 int sealed = 0;
                 if (n == 0 && mayBeSealed(w, p) && !MUTANT(15)) sealed = (ufLink(w, ufFind(w, i)) & REGION_OPEN) == 0u;
-                if (n < 0) {
+                uint32_t rigid = NONE, held = NONE;
+                uint32_t seam = n < 0 ? NONE : seamOf(w, i);
+                if (seam != NONE) {
+                    /* a gap the stamp opened in a body's wall: the body's (:1603-1612) */
                     c[2] = -1.0f;
+                    rigid = seam;
+                    if (isHeld(w, seam)) {
+                        held = seam;
+                        rigid = NONE;
+                    }
+                    sealed = 0;
+                } else if (n < 0) {
+                    c[2] = -1.0f;
+                    rigid = rigidBodyOf(w, &w->cells[4ull * i]);
+                    /* powder in its cup is its cargo, moving with it (:1616-1619) */
+                    if (rigid == NONE && kindOf(w, wType(w->cells[4ull * i])) == KIND_POWDER) rigid = cupOf(w, i);
+                    if (rigid != NONE && isHeld(w, rigid)) {
+                        held = rigid;
+                        rigid = NONE;
+                    }
                 } else if (n > 0) {
                     v2 li = liquidIn(w, i);
                     c[2] = li.x;
@@ -1262,9 +1555,11 @@ static void k_classify(OWorld *w) {                          /* :1578-1696 */
                     }
                 }
                 uint32_t mine = stepBit(w, 0u);
-                uint32_t liquid = c[2] > 0.0f ? mine : 0u;
+                uint32_t liquid = (c[2] > 0.0f || rigid != NONE || held != NONE) ? mine : 0u;
                 float drift[4] = {0.0f, 0.0f, sealed ? 1.0f : 0.0f,
                                   (float)((wu32(wasDrift[3]) & ~(mine | fallingBit(w, 0u))) | liquid)};
+                if (rigid != NONE) drift[2] = -1.0f - (float)rigid;
+                else if (held != NONE) drift[2] = -1.0f - (float)(held + MAX_BODIES);
                 if (c[2] > 0.0f) {
                     if (was[2] > 0.0f) {
                         c[0] = was[0];
@@ -1300,10 +1595,6 @@ static float faceViscosity(const float ga[4], const float gb[4]) {   /* :3279-32
     float fa = ga[2] > 0.0f ? 1.0f : 0.0f, fb = gb[2] > 0.0f ? 1.0f : 0.0f;
     return (fa * ga[3] + fb * gb[3]) / fmaxf(fa + fb, 1.0f);
 }
-static void gridAt4(const OWorld *w, i2 c, float g[4]) {
-    if (!inWorld(w, c)) { g[0] = 0.0f; g[1] = 0.0f; g[2] = -1.0f; g[3] = 0.0f; return; }
-    for (int j = 0; j < 4; j++) g[j] = FLD(2u * cellIndex(w, c), j);
-}
 /* wasLiquid (:3315-3317): cell c liquid last step too */
 static inline int wasLiquid(const OWorld *w, i2 c) {
     return (wu32(FLD(2u * cellIndex(w, c) + 1u, 3)) & stepBit(w, 1u)) != 0u;
@@ -1313,6 +1604,31 @@ static float lastPush(const OWorld *w, i2 a, i2 at, const float ga[4], const flo
     if ((ga[2] > 0.0f && !wasLiquid(w, a)) || (gb[2] > 0.0f && !wasLiquid(w, at))) return 0.0f;
     return fineBeta(ga[2], gb[2], densityAt(w, a), densityAt(w, at)) * (gb[0] - ga[0]);
 }
+/* a face of a free body's cell: the body's velocity there going into the
+ * solve, the mean between two bodies' cells (:1841-1858) */
+static void setBodyFace(OWorld *w, uint32_t f, i2 at, int isU, const float ga[4], const float gb[4], uint32_t ra, uint32_t rb,
+                        int drag) {
+    int axis = isU ? 0 : 1;
+    v2 x = faceMiddle(at, isU);
+    float now, u;
+    if (ra != NONE && rb != NONE) {
+        v2 ma = bodyMoving(w, ra, x, drag), mb = bodyMoving(w, rb, x, drag), va = rigidVel(w, ra, x), vb = rigidVel(w, rb, x);
+        now = 0.5f * ((axis ? ma.y : ma.x) + (axis ? mb.y : mb.x));
+        u = 0.5f * ((axis ? va.y : va.x) + (axis ? vb.y : vb.x));
+    } else {
+        uint32_t b = ra != NONE ? ra : rb;
+        v2 m = bodyMoving(w, b, x, drag), v = rigidVel(w, b, x);
+        now = axis ? m.y : m.x;
+        u = axis ? v.y : v.x;
+    }
+    const float *other = ra != NONE ? gb : ga;
+    float nu = (ra == NONE || rb == NONE) && other[2] > 0.0f ? other[3] : 0.0f;
+    FLD(faceSlot(w, f), 0) = now;
+    FLD(faceSlot(w, f), 1) = u;
+    FLD(faceSlot(w, f), 2) = 1.0f;
+    FLD(faceSlot(w, f), 3) = nu;
+    FLD(2ull * w->C + w->F + f / 4u, f % 4u) = now;          /* setViscRhs */
+}
 static void setFace(OWorld *w, i2 at, int isU) {           /* :1936-1980 */
     uint32_t f = faceIndex(w, at, isU);
     i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
@@ -1321,6 +1637,11 @@ static void setFace(OWorld *w, i2 at, int isU) {           /* :1936-1980 */
     gridAt4(w, a, ga);
     gridAt4(w, at, gb);
     int32_t weight = (int32_t)SUM(f, 1);
+    uint32_t ra = rigidAt(w, a), rb = rigidAt(w, at);
+    if (ra != NONE || rb != NONE) {
+        setBodyFace(w, f, at, isU, ga, gb, ra, rb, 0);
+        return;
+    }
     float u = 0.0f, held = 0.0f;
     if (ga[2] < 0.0f && gb[2] < 0.0f) {
         held = -1.0f;
@@ -1357,8 +1678,9 @@ static void k_setFaces(OWorld *w) {                          /* :1982-1994 */
             if (fs[j].isU >= 0) setFace(w, I2(fs[j].x, fs[j].y), fs[j].isU == 1);
     }
 }
-/* viscousFace (:3361-3403): a red-black Gauss-Seidel step of implicit
- * viscosity on the face at `at` if it is of `color` (no bodies: heavy 1) */
+/* viscousFace (:3347-3403): a red-black Gauss-Seidel step of implicit
+ * viscosity on the face at `at` if it is of `color`; a body's face with
+ * liquid beyond it too, the body's mass on its side (heavy) */
 #define VISC_OMEGA 1.6f
 #define VISCOSITY_ITERATIONS 8                               /* js/sim.js:71 */
 static void viscousFace(OWorld *w, i2 at, int isU, int color) {
@@ -1367,12 +1689,18 @@ static void viscousFace(OWorld *w, i2 at, int isU, int color) {
     float ga[4], gb[4];
     gridAt4(w, a, ga);
     gridAt4(w, at, gb);
-    if (ga[2] < 0.0f || gb[2] < 0.0f) return;
+    uint32_t ra = rigidAt(w, a), rb = rigidAt(w, at);
+    int surface = (ra != NONE) != (rb != NONE) && (ra != NONE ? gb[2] : ga[2]) > 0.0f;
+    if (!surface && (ga[2] < 0.0f || gb[2] < 0.0f || ra != NONE || rb != NONE)) return;
     uint32_t f = faceIndex(w, at, isU);
     uint64_t fs = faceSlot(w, f);
     float nu = FLD(fs, 3);
     if (nu <= 0.0f) return;
     float heavy = 1.0f;
+    if (surface) {
+        float liquid = ra != NONE ? gb[2] : ga[2];
+        heavy = 2.0f * liquid / (liquid + bodyDensity(w, ra != NONE ? ra : rb));
+    }
     float through[2] = {ga[2] > 0.0f ? ga[3] : 0.0f, gb[2] > 0.0f ? gb[3] : 0.0f};
     float sum = 0.0f, n = 0.0f;
     for (int d = 0; d < 4; d++) {
@@ -1421,7 +1749,7 @@ static void k_viscosity(OWorld *w, int color) {
                 gridAt4(w, a, ga);
                 gridAt4(w, at, gb);
                 uint32_t f = faceIndex(w, at, isU);
-                if (faceViscosity(ga, gb) > 0.0f) {
+                if (rigidAt(w, a) == NONE && rigidAt(w, at) == NONE && faceViscosity(ga, gb) > 0.0f) {
                     uint64_t fs0 = faceSlot(w, f);
                     FLD(viscLastSlot(w, f), f % 4u) = FLD(fs0, 0) - FLD(2ull * w->C + w->F + f / 4u, f % 4u);
                     FLD(fs0, 0) += lastPush(w, a, at, ga, gb);
@@ -1444,7 +1772,13 @@ static void k_divergence(OWorld *w) {                        /* :1996-2025 */
             + FLD(faceSlot(w, vFace(w, c.x, c.y + 1)), 0) - FLD(faceSlot(w, vFace(w, c.x, c.y)), 0);
         FLD(2u * i, 1) = out;
         float n = densityAt(w, c);
-        float room = wclamp(1.0f, 0.1f, 1.9f);               /* posedRoom with no bodies */
+        float room = 1.0f;                                   /* posedRoom (:2035-2045) */
+        for (int d = 0; d < 4; d++) {
+            i2 q = beside(c, d);
+            uint32_t b = rigidAt(w, q);
+            if (b != NONE) room += 0.5f - bodyReach(w, b, I2(c.x - q.x, c.y - q.y));
+        }
+        room = wclamp(room, 0.1f, 1.9f);
         FLD(2u * i + 1u, 1) = FLD(2u * i + 1u, 2) < 0.0f ? 0.0f : (MUTANT(24) ? SPREAD : -SPREAD) * (n - room);
     }
 }
@@ -1848,7 +2182,7 @@ static void k_cgResidual(OWorld *w) {
             v[li][0] = v[li][1] = v[li][2] = 0.0;
             if (!inWorld(w, c) || gridZ(w, c) <= 0.0f) continue;
             uint32_t i = cellIndex(w, c);
-            CG(i, 1) = -residualAt(w, 0u, c).x;
+            CG(i, 1) = -residualAt(w, 0u, c).x - couplingAt(w, c, PUSH_X);   /* (:2714-2716) */
             double bi = -(double)FLD(2u * i, 1);
             v[li][2] = bi * bi;
             if (w->capture && w->cap_b) w->cap_b[i] = -FLD(2u * i, 1);
@@ -1968,7 +2302,7 @@ static void k_cgApplyAlpha(OWorld *w) {
             if (!inWorld(w, c)) continue;
             float rho = gridZ(w, c);
             uint32_t i = cellIndex(w, c);
-            float q = rho > 0.0f ? applyA(w, c, rho) : 0.0f;
+            float q = rho > 0.0f ? applyA(w, c, rho) + couplingAt(w, c, PUSH_P0 + w->cycle % 2u) : 0.0f;   /* (:2790) */
             CG(i, 3) = q;
             vf[li][0] = CG(i, 2) * q;
             vd[li][0] = (double)CG(i, 2) * (double)q;
@@ -2001,6 +2335,10 @@ static void k_cgApplyAlpha(OWorld *w) {
         sums[1] = (float)h->alpha;
         sums[3] = (float)pq;
     }
+    /* the bodies' sums of this iteration's direction, done with, emptied
+     * for the iteration after next (:2808-2813) */
+    if (bodiesOn(w))
+        for (uint32_t k = 0; k < MAX_BODIES * 6u; k++) w->pushes[pushSlot(k / 6u, PUSH_P0 + w->cycle % 2u, 0u) + k % 6u] = 0u;
 }
 /* cgUpdate (:2852-2880) and the convergence test the next iteration reads */
 static void k_cgUpdate(OWorld *w) {
@@ -2132,10 +2470,22 @@ static void projectFace(OWorld *w, i2 at, int isU) {        /* :2897-2923 */
     gridAt4(w, a, ga);
     gridAt4(w, at, gb);
     float beta = fineBeta(ga[2], gb[2], densityAt(w, a), densityAt(w, at));
+    uint32_t ra = rigidAt(w, a), rb = rigidAt(w, at);
     if (beta > 0.0f) {
         FLD(f, 0) = MUTANT(13) ? FLD(f, 0) + beta * (gb[0] - ga[0]) : FLD(f, 0) - beta * (gb[0] - ga[0]);
         FLD(f, 2) = PROJECTED;
-        FLD(f, 3) = wclamp(-beta * (driftAt(w, at) - driftAt(w, a)), -MAX_DRIFT, MAX_DRIFT);
+        /* no drift into or out of a body's cells (:2907-2909) */
+        FLD(f, 3) = ra != NONE || rb != NONE ? 0.0f : wclamp(-beta * (driftAt(w, at) - driftAt(w, a)), -MAX_DRIFT, MAX_DRIFT);
+    }
+    /* a face between the liquid and a free body moves as the body does with
+     * the pressure's push the solve found (:2911-2921) */
+    if ((ra == NONE) != (rb == NONE) && (ra != NONE ? gb[2] : ga[2]) > 0.0f) {
+        uint32_t b = ra != NONE ? ra : rb;
+        v3 ch = pushChange(w, b, PUSH_F);
+        v2 m = faceMiddle(at, isU), pos = bodyV(w, b, BW_POS);
+        v2 r = V2(m.x - pos.x, m.y - pos.y);
+        FLD(f, 0) += isU ? ch.x - ch.z * r.y : ch.y + ch.z * r.x;
+        FLD(f, 2) = PROJECTED;
     }
 }
 static void k_project(OWorld *w) {
@@ -2220,18 +2570,21 @@ static inline v2 clampSpeed(v2 v) {                           /* :437-440 */
     float s = wlength(v);
     return s > most ? V2(v.x * (most / s), v.y * (most / s)) : v;
 }
-static int escapesTo(OWorld *w, i2 q) {                      /* :3535-3541, no bodies */
-    return inWorld(w, q) && !(gridZ(w, q) < 0.0f);
+static int escapesTo(OWorld *w, i2 q, uint32_t side) {       /* :3535-3541 */
+    if (!inWorld(w, q) || shutAt(w, q)) return 0;
+    uint32_t cup = cupOf(w, cellIndex(w, q));
+    return cup == NONE || cup == side;
 }
 static v2 escape(OWorld *w, v2 pos) {                        /* :3552-3594 */
     if (!isObstaclePos(w, pos) || MUTANT(22)) return pos;
     i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
+    uint32_t side = inWorld(w, c) ? cupBefore(w, cellIndex(w, c)) : NONE;
     v2 best = pos;
     float bestD = 1e9f;
     for (int n = 0; n < 8; n++) {
         int m = n + (n >= 4 ? 1 : 0);
         i2 at = I2(c.x + m % 3 - 1, c.y + m / 3 - 1);
-        if (!escapesTo(w, at)) continue;
+        if (!escapesTo(w, at, side)) continue;
         v2 inside = V2(wclamp(pos.x, (float)at.x + 0.02f, (float)at.x + 0.98f), wclamp(pos.y, (float)at.y + 0.02f, (float)at.y + 0.98f));
         float d = wlength(V2(pos.x - inside.x, pos.y - inside.y));
         if (d < bestD) { bestD = d; best = inside; }
@@ -2245,7 +2598,7 @@ static v2 escape(OWorld *w, v2 pos) {                        /* :3552-3594 */
         i2 dir = turnStep(ways[n], k);
         for (int r = 2; r < nearest; r++) {
             i2 q = I2(c.x + dir.x * r, c.y + dir.y * r);
-            if (!isObstaclePos(w, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) && escapesTo(w, q)) {
+            if (!isObstaclePos(w, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) && escapesTo(w, q, side)) {
                 nearest = r;
                 way = dir;
                 break;
@@ -2542,6 +2895,155 @@ static void k_unpaint(OWorld *w) {                           /* :4163-4182 */
 }
 
 /* ------------------------------------------------------------- the step */
+/* ------------------------------------------------------- the bodies' passes */
+/* bodyStart (:3065-3070): the step's push sums emptied */
+static void k_bodyStart(OWorld *w) {
+    if (bodiesOn(w)) memset(w->pushes, 0, 4ull * MAX_BODIES * BODY_PUSH_WORDS);
+}
+/* seepLoads (:3132-3143): the liquid's pressure on free body b's cell c
+ * where it rests against what isn't liquid or air */
+static void seepLoads(OWorld *w, i2 c, uint32_t b) {
+    for (int d = 0; d < 4; d++) {
+        i2 q = beside(c, d);
+        if (!inWorld(w, q) || rigidAt(w, q) == b || gridZ(w, q) >= 0.0f) continue;
+        i2 n = I2(q.x - c.x, q.y - c.y);
+        v2 x = V2((float)c.x + 0.5f + 0.5f * (float)n.x, (float)c.y + 0.5f + 0.5f * (float)n.y);
+        v3 j = bodyFaceJ(w, q, c, b);
+        float s = seepAt(w, c, n, b, 0, x);
+        addPush(w, b, PUSH_SURFACE, V3(j.x * s, j.y * s, j.z * s));
+    }
+}
+/* bodyLoads (:3097-3121): the liquid's pressure where each free body's
+ * pose puts its surface, past what the cell's middle gives */
+static void k_bodyLoads(OWorld *w) {
+    if (!bodiesOn(w)) return;
+    FOR_ACTIVE_CELLS(w, c) {
+        if (!inWorld(w, c)) continue;
+        uint32_t own = rigidAt(w, c);
+        if (own != NONE) {
+            seepLoads(w, c, own);
+            continue;
+        }
+        float g[4];
+        gridAt4(w, c, g);
+        if (g[2] <= 0.0f) continue;
+        v2 mid = V2((float)c.x + 0.5f, (float)c.y + 0.5f);
+        for (int d = 0; d < 4; d++) {
+            i2 q = beside(c, d);
+            uint32_t b = rigidAt(w, q);
+            if (b == NONE) continue;
+            i2 n = I2(q.x - c.x, q.y - c.y);
+            float reach = 1.0f - bodyReach(w, b, I2(-n.x, -n.y));
+            v2 surface = V2(mid.x + (float)n.x * reach, mid.y + (float)n.y * reach);
+            v3 j = bodyFaceJ(w, c, q, b);
+            float dp = potentialAt(w, mid) - potentialAt(w, surface);
+            addPush(w, b, PUSH_SURFACE, V3(j.x * g[2] * dp, j.y * g[2] * dp, j.z * g[2] * dp));
+        }
+    }
+}
+/* bodyDrag (:3150-3173): the liquid's drag on each free body as the
+ * viscosity's sweeps left its faces */
+static void k_bodyDrag(OWorld *w) {
+    if (!bodiesOn(w)) return;
+    FOR_ACTIVE_CELLS(w, c) {
+        uint32_t b = rigidAt(w, c);
+        if (!inWorld(w, c) || b == NONE) continue;
+        v2 pos = bodyV(w, b, BW_POS);
+        float rho = bodyDensity(w, b);
+        v3 push = V3(0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 4; k++) {
+            int isU = k < 2;
+            int s = (k & 1) == 1 ? 1 : -1;
+            i2 n = isU ? I2(s, 0) : I2(0, s);
+            if (gridZ(w, I2(c.x + n.x, c.y + n.y)) <= 0.0f) continue;
+            i2 at = isU ? I2(c.x + (k & 1), c.y) : I2(c.x, c.y + (k & 1));
+            v2 x = faceMiddle(at, isU);
+            int axis = isU ? 0 : 1;
+            v2 mv = bodyMoving(w, b, x, 0);
+            float du = bodyReach(w, b, n) * rho * (FLD(faceSlot(w, faceIndex(w, at, isU)), 0) - (axis == 0 ? mv.x : mv.y));
+            v2 r = V2(x.x - pos.x, x.y - pos.y);
+            if (axis == 0) push.x += du;
+            else push.y += du;
+            push.z += isU ? -r.y * du : r.x * du;
+        }
+        if (push.x != 0.0f || push.y != 0.0f || push.z != 0.0f) addPush(w, b, PUSH_DRAG, push);
+    }
+}
+/* bodyFaces (:3180-3201): the bodies' faces, as the sweeps left them,
+ * moving with the bodies again, the drag and all */
+static void k_bodyFaces(OWorld *w) {
+    if (!bodiesOn(w)) return;
+    FOR_ACTIVE_CELLS(w, c) {
+        if (!inWorld(w, c)) continue;
+        Face3 fs[4];
+        facesOf(w, c, fs);
+        for (int j = 0; j < 4; j++) {
+            if (fs[j].isU < 0) continue;
+            i2 at = I2(fs[j].x, fs[j].y);
+            int isU = fs[j].isU == 1;
+            i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+            uint32_t ra = rigidAt(w, a), rb = rigidAt(w, at);
+            if (ra != NONE || rb != NONE) {
+                float ga[4], gb[4];
+                gridAt4(w, a, ga);
+                gridAt4(w, at, gb);
+                setBodyFace(w, faceIndex(w, at, isU), at, isU, ga, gb, ra, rb, 1);
+            }
+        }
+    }
+}
+/* cgBodyX, cgBodyP, cgBodyF (:2819-2845): the pressure's push on each free
+ * body, of the pressure the solve starts from, of an iteration's direction
+ * (into the iteration's set by its parity), of the pressure it ends with */
+static void k_cgBodyX(OWorld *w) {
+    if (!bodiesOn(w)) return;
+    FOR_ACTIVE_CELLS(w, c) {
+        if (inWorld(w, c) && gridZ(w, c) > 0.0f) addPressurePush(w, c, CG(cellIndex(w, c), 0), PUSH_X);
+    }
+}
+static void k_cgBodyP(OWorld *w) {
+    if (!bodiesOn(w) || converged(w)) return;
+    FOR_ACTIVE_CELLS(w, c) {
+        if (inWorld(w, c) && gridZ(w, c) > 0.0f) addPressurePush(w, c, CG(cellIndex(w, c), 2), PUSH_P0 + w->cycle % 2u);
+    }
+}
+static void k_cgBodyF(OWorld *w) {
+    if (!bodiesOn(w)) return;
+    FOR_ACTIVE_CELLS(w, c) {
+        if (inWorld(w, c) && gridZ(w, c) > 0.0f) addPressurePush(w, c, FLD(2u * cellIndex(w, c), 0), PUSH_F);
+    }
+}
+/* rigidify (:3037-3063): what the liquid's pressure did to each held body,
+ * and whether it is wet; summed into its liquid slot's words */
+static void k_rigidify(OWorld *w) {
+    if (!bodiesOn(w)) return;
+    PAR_FOR
+    for (uint32_t wid = 0; wid < activeCount(w); wid++) {
+        i2 b = activeBlock(w, wid);
+        for (int li = 0; li < BLOCK * BLOCK; li++) {
+            i2 c = I2(b.x * BLOCK + li % BLOCK, b.y * BLOCK + li / BLOCK);
+            uint32_t h = heldAt(w, c);
+            if (h == NONE) continue;
+            int32_t d[3];
+            heldPush(w, c, h, d);
+            if ((w->blocks[STATE_SLOT(blockIndex(w, idiv8(c)))] & NEAR_LIQUID) != 0u) at_or(&w->motion[liquidSlot(h, 3u)], LIQUID_WET);
+            for (uint32_t k = 0; k < 3u; k++) at_add(&w->motion[liquidSlot(h, k)], (uint32_t)d[k]);
+        }
+    }
+}
+/* bodyPush (:3206-3218): what the liquid did to each free body this step */
+static void k_bodyPush(OWorld *w) {
+    if (!bodiesOn(w)) return;
+    for (uint32_t b = 0; b < MAX_BODIES; b++) {
+        v3 s = pushChangeFree(w, b, PUSH_SURFACE), dr = pushChangeFree(w, b, PUSH_DRAG), f = pushChangeFree(w, b, PUSH_F);
+        v3 ch = V3(s.x + dr.x + f.x, s.y + dr.y + f.y, s.z + dr.z + f.z);
+        if (ch.x == 0.0f && ch.y == 0.0f && ch.z == 0.0f) continue;
+        w->motion[liquidSlot(b, 0u)] += (uint32_t)wi32(wround(ch.x * RIGID_FIX));
+        w->motion[liquidSlot(b, 1u)] += (uint32_t)wi32(wround(ch.y * RIGID_FIX));
+        w->motion[liquidSlot(b, 2u)] += (uint32_t)wi32(wround(ch.z * RIGID_FIX));
+    }
+}
+
 static void relaxPass(OWorld *w, int *shifted, uint32_t times) {
     for (uint32_t k = 0; k < times; k++) {
         k_relaxRims(w);
@@ -2552,7 +3054,7 @@ static void relaxPass(OWorld *w, int *shifted, uint32_t times) {
 
 static int check_ctl(const OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     if (P->size[0] != (uint32_t)w->W || P->size[1] != (uint32_t)w->H) return WIG_E_INVALID;
-    if (ctl->bodies_run || (ctl->surface && ctl->residual_every)) return WIG_E_INVALID;
+    if ((ctl->surface || ctl->bodies) && ctl->residual_every) return WIG_E_INVALID;
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->level_pairs > 64u) return WIG_E_INVALID;
     if (ctl->tol_mode > WIG_TOL_REL_L2 || ctl->residual_every > WIG_MAX_CYCLES) return WIG_E_INVALID;
     return WIG_OK;
@@ -2583,6 +3085,8 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     k_sealAnchor(w);
     k_classify(w);
     if (ctl->surface) k_surface(w);
+    k_bodyStart(w);
+    k_bodyLoads(w);
     k_setFaces(w);
     /* viscosity's sweeps before the projection (js/sim.js:2505-2514) */
     for (int i = 0; ctl->viscous && i < VISCOSITY_ITERATIONS; i++) {
@@ -2590,6 +3094,12 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         k_viscosity(w, 1);
     }
     k_unpress(w);
+    /* the liquid's drag on the bodies, as the sweeps left their faces, and
+     * their faces moving with them again (js/sim.js:2515-2520) */
+    if (ctl->viscous) {
+        k_bodyDrag(w);
+        k_bodyFaces(w);
+    }
     k_divergence(w);
     int shifted = (int)ctl->shifted_first;
     SolveHead *h = head(w);
@@ -2597,6 +3107,7 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
     k_cgStart(w);
     k_cgJoin(w);
     k_cgAnchor(w);
+    k_cgBodyX(w);
     k_cgResidual(w);
     k_cgPrepare(w);
     uint32_t deepest = w->L;
@@ -2635,12 +3146,19 @@ int wor_fluid_step(OWorld *w, const WigParams *P, const WigStepCtl *ctl) {
         relaxPass(w, &shifted, ctl->post_relax);
         k_cgRhoBeta(w);
         k_cgDirection(w);
+        k_cgBodyP(w);
         k_cgApplyAlpha(w);
         k_cgUpdate(w);
     }
     w->cycle = 0;
     k_cgFinish(w);
+    k_cgBodyF(w);
     k_project(w);
+    /* what the liquid did to the bodies in it (js/sim.js:2594-2599) */
+    if (ctl->bodies_run) {
+        k_rigidify(w);
+        k_bodyPush(w);
+    }
     k_extrapolate(w, 1);
     k_extrapolate(w, 2);
     k_advect(w);
@@ -2674,9 +3192,12 @@ OWorld *wor_create(uint32_t width, uint32_t height, uint32_t coarse_levels) {
     w->source = (uint32_t *)calloc(1, w->bytes[WIG_PART_AIR_SOURCE]);
     w->occ = (uint32_t *)calloc(1, w->bytes[WIG_PART_OCCUPANTS]);
     w->grav = (float *)calloc(1, w->bytes[WIG_PART_GRAVITY]);
+    w->motion = (uint32_t *)calloc(1, w->bytes[WIG_PART_MOTION]);
+    w->swept = (uint32_t *)calloc(1, w->bytes[WIG_PART_SWEPT]);
+    w->pushes = (uint32_t *)calloc(1, w->bytes[WIG_PART_PUSHES]);
     w->dot_mode = 1;
     if (!w->cells || !w->sorted || !w->parts || !w->counters || !w->field || !w->blocks || !w->cg || !w->solve || !w->bvec
-        || !w->air || !w->source)
+        || !w->air || !w->source || !w->occ || !w->grav || !w->motion || !w->swept || !w->pushes)
         return NULL;
     return w;
 }
@@ -2685,13 +3206,14 @@ void wor_destroy(OWorld *w) {
     free(w->cells); free(w->sorted); free(w->parts); free(w->counters);
     free(w->field); free(w->blocks); free(w->cg); free(w->solve); free(w->cap_b); free(w->bvec);
     free(w->air); free(w->source); free(w->occ); free(w->grav);
+    free(w->motion); free(w->swept); free(w->pushes);
     free(w);
 }
 void *wor_part(OWorld *w, int part, uint64_t *bytes) {
     if (part < 0 || part >= WIG_PART_COUNT) return NULL;
     if (bytes) *bytes = w->bytes[part];
     void *p[WIG_PART_COUNT] = {w->cells, w->sorted, w->parts, w->counters, w->field, w->blocks, w->cg, w->solve,
-                               w->air, w->source, w->occ, w->grav};
+                               w->air, w->source, w->occ, w->grav, w->motion, w->swept, w->pushes};
     return p[part];
 }
 void wor_set_kinds(OWorld *w, const uint8_t kinds[256]) { memcpy(w->kind, kinds, 256); }
@@ -2955,3 +3477,41 @@ int wor_run(OWorld *w, int pass, int arg, const WigParams *P, const WigStepCtl *
 }
 /* Where coarse level l (1 .. levels) starts in the field, in vec4s. */
 uint64_t wor_level_start(const OWorld *w, uint32_t l) { return l >= 1u && l <= w->L ? w->lstart[l] : 0u; }
+
+/* The bodies' helpers one at a time, for the pins in tests/ (the step's
+ * Params and controls as a pass sees them):
+ *   0 bodyFaceJ(c (i0, i1), q (i2, i3), body i4)          -> f[0..2]
+ *   1 couplingAt(c (i0, i1), set i2)                      -> f[0]
+ *   2 addPressurePush(c (i0, i1), p f0, set i2)
+ *   3 rigidAt(c (i0, i1)), 4 heldAt(c (i0, i1))           -> u[0]
+ *   5 the swept body of cell i0, i1 steps back, seam i2, weighed i3 -> u[0]
+ *   6 giveWay(body i0, g (f0, f1, f2))                    -> f[0..2]
+ *   7 bodyMoving(body i0, x (f0, f1), drag i1)            -> f[0..1]
+ *   8 potentialAt((f0, f1))                               -> f[0]
+ *   9 heldPush(c (i0, i1), body i2)                       -> u[0..2] (i32)
+ *  10 rigidBodyOf(cell i0's word)                         -> u[0]
+ *  11 seepAt(c (i0, i1), n (i2, i3), body i4, held i5, x (f0, f1)) -> f[0]
+ *  12 pushOf(body i0, set i1)                             -> f[0..2] */
+int wor_body_probe(OWorld *w, const WigParams *P, const WigStepCtl *ctl, int op, const int32_t *ia, const float *fa,
+                   float *fo, uint32_t *uo) {
+    w->P = P;
+    w->ctl = ctl;
+    i2 c = I2(ia[0], ia[1]);
+    switch (op) {
+    case 0: { v3 j = bodyFaceJ(w, c, I2(ia[2], ia[3]), (uint32_t)ia[4]); fo[0] = j.x; fo[1] = j.y; fo[2] = j.z; break; }
+    case 1: fo[0] = couplingAt(w, c, (uint32_t)ia[2]); break;
+    case 2: addPressurePush(w, c, fa[0], (uint32_t)ia[2]); break;
+    case 3: uo[0] = rigidAt(w, c); break;
+    case 4: uo[0] = heldAt(w, c); break;
+    case 5: uo[0] = sweptBody(w, (uint32_t)ia[0], (uint32_t)ia[1], (uint32_t)ia[2], ia[3]); break;
+    case 6: { v3 g = giveWay(w, (uint32_t)ia[0], V3(fa[0], fa[1], fa[2])); fo[0] = g.x; fo[1] = g.y; fo[2] = g.z; break; }
+    case 7: { v2 m = bodyMoving(w, (uint32_t)ia[0], V2(fa[0], fa[1]), ia[1]); fo[0] = m.x; fo[1] = m.y; break; }
+    case 8: fo[0] = potentialAt(w, V2(fa[0], fa[1])); break;
+    case 9: { int32_t o[3]; heldPush(w, c, (uint32_t)ia[2], o); for (int k = 0; k < 3; k++) uo[k] = (uint32_t)o[k]; break; }
+    case 10: uo[0] = rigidBodyOf(w, &w->cells[4ull * (uint32_t)ia[0]]); break;
+    case 11: fo[0] = seepAt(w, c, I2(ia[2], ia[3]), (uint32_t)ia[4], ia[5], V2(fa[0], fa[1])); break;
+    case 12: { v3 g = pushOf(w, (uint32_t)ia[0], (uint32_t)ia[1]); fo[0] = g.x; fo[1] = g.y; fo[2] = g.z; break; }
+    default: return WIG_E_INVALID;
+    }
+    return WIG_OK;
+}

@@ -1302,6 +1302,252 @@ def pin_gravity(L):
         assert cnt[5, 5, 8] == want, (grav, cnt[5, 5, 8], want)
 
 
+# ------------------------------------------------------------ bodies
+BODY_PROPS = np.zeros((256, 6), dtype=np.float32)
+BODY_PROPS[E.STONE, 5] = 1.0          # stone comes loose as a free body's (breaks)
+
+
+def _body_world(L, W, H, bodies, cells=None, cups=(), seams=(), frame=1):
+    """An oracle world with these scenes.Body's painted in and their table,
+    cups and seams loaded; (world, params, ctl) for the probes."""
+    words = scenes.empty_cells(W, H) if cells is None else cells
+    motion, swept = scenes.body_parts(W, H, words, bodies, cups=cups, seams=seams, frame=frame)
+    w = OracleWorld(W, H, kinds=E.kinds(), L=L)
+    w.set_element_props(BODY_PROPS)
+    w.upload(abi.PART_CELLS, words)
+    w.upload(abi.PART_MOTION, motion)
+    w.upload(abi.PART_SWEPT, swept)
+    P = abi.WigParams()
+    P.size[0], P.size[1] = W, H
+    P.frame = frame
+    P.earthG = 0.25
+    c = abi.WigStepCtl()
+    c.bodies = 1
+    return w, P, c
+
+
+def _field_cells(w, W, H):
+    return w.part(abi.PART_FIELD)[:8 * W * H].reshape(H, W, 8)
+
+
+def pin_body_face(L):
+    """bodyFaceJ (fluid.wgsl:1374-1378) by hand: the face from a liquid cell
+    to a body's cell pushes along the normal into the body, and turns it by
+    r x n, r from the body's centre to the face's middle."""
+    m = np.zeros((16, 16), dtype=bool)
+    m[5, 5] = True
+    w, P, c = _body_world(L, 16, 16, [scenes.Body(2, m, pos=(5.5, 6.0))])
+    # from the left: n (1, 0), face middle (5.0, 5.5), r (-0.5, -0.5): r x n = 0.5
+    f, _ = w.body_probe(0, (4, 5, 5, 5, 2), params=P, ctl=c)
+    assert f[:3] == [1.0, 0.0, 0.5], f
+    # from the right: n (-1, 0), face middle (6.0, 5.5), r (0.5, -0.5): r x n = -0.5
+    f, _ = w.body_probe(0, (6, 5, 5, 5, 2), params=P, ctl=c)
+    assert f[:3] == [-1.0, 0.0, -0.5], f
+    # from above: n (0, 1), face middle (5.5, 5.0), r (0, -1): r x n = 0
+    f, _ = w.body_probe(0, (5, 4, 5, 5, 2), params=P, ctl=c)
+    assert f[:3] == [0.0, 1.0, 0.0], f
+    # from below, the centre off to the side: n (0, -1), middle (5.5, 6.0), r (0, 0)
+    f, _ = w.body_probe(0, (5, 6, 5, 5, 2), params=P, ctl=c)
+    assert f[:3] == [0.0, -1.0, 0.0], f
+
+
+def pin_give_way(L):
+    """giveWay (fluid.wgsl:1349-1356): a push over the body's mass and moment
+    of inertia, its cargo's added, unless its contacts' mobility was worked
+    out last step: then the symmetric 3 x 3 of MOBILITY times the push."""
+    m = np.zeros((16, 16), dtype=bool)
+    m[4:6, 4:8] = True
+    mob = (0.5, 0.25, 0.125, 2.0, 0.0625, 4.0)
+    bodies = [scenes.Body(1, m, mass=3.0, inertia=5.0, cargo=1.0, cargo_inertia=3.0),
+              scenes.Body(9, m, mass=2.0, inertia=7.0, mobility=mob, mobility_at=6)]
+    w, P, c = _body_world(L, 16, 16, bodies, frame=7)
+    f, _ = w.body_probe(6, (1,), (2.0, -6.0, 4.0), params=P, ctl=c)
+    assert f[:3] == [0.5, -1.5, 0.5], f          # (2, -6) / (3 + 1), 4 / (5 + 3)
+    f, _ = w.body_probe(6, (9,), (2.0, -6.0, 4.0), params=P, ctl=c)
+    M = np.array([[mob[0], mob[1], mob[2]], [mob[1], mob[3], mob[4]], [mob[2], mob[4], mob[5]]])
+    assert np.allclose(f[:3], M @ np.array([2.0, -6.0, 4.0]), rtol=1e-6), f
+    # a step later the mobility is stale: mass and inertia again
+    P.frame = 8
+    f, _ = w.body_probe(6, (9,), (2.0, -6.0, 4.0), params=P, ctl=c)
+    assert f[:3] == [1.0, -3.0, 4.0 / 7.0] or np.allclose(f[:3], [1.0, -3.0, 4.0 / 7.0], rtol=1e-7), f
+
+
+def _mark_codes(w, W, H, bodies):
+    """drift.z as classify leaves it for the bodies' cells (free)."""
+    cf = _field_cells(w, W, H)
+    for b in bodies:
+        cf[b.mask, 6] = -1.0 - b.slot
+
+
+def pin_coupling(L):
+    """The solve's coupling, J^T M^-1 J (fluid.wgsl:1381-1403): a liquid
+    cell's pressure pushed onto a one-cell body from one face comes back as
+    p (n.n / m + (r x n)^2 / I); over many cells and bodies the operator is
+    symmetric, <q, K p> = <p, K q>, and positive semidefinite."""
+    W = H = 16
+    m = np.zeros((H, W), dtype=bool)
+    m[5, 5] = True
+    b = scenes.Body(2, m, pos=(5.5, 6.0), mass=4.0, inertia=2.0)
+    w, P, c = _body_world(L, W, H, [b])
+    _mark_codes(w, W, H, [b])
+    w.body_probe(2, (4, 5, 0), (3.0,), params=P, ctl=c)     # p = 3 at (4, 5) into set 0
+    f, _ = w.body_probe(12, (2, 0), params=P, ctl=c)
+    assert f[:3] == [3.0, 0.0, 1.5], f                        # J p = (1, 0, 0.5) * 3
+    f, _ = w.body_probe(1, (4, 5, 0), params=P, ctl=c)
+    assert abs(f[0] - 3.0 * (1.0 / 4.0 + 0.25 / 2.0)) < 1e-6, f
+    # symmetry and positivity over two bodies, one with a mobility
+    W = H = 24
+    m1 = np.zeros((H, W), dtype=bool)
+    m1[4:8, 4:9] = True
+    m2 = np.zeros((H, W), dtype=bool)
+    m2[12:15, 10:13] = True
+    mob = (0.3, 0.02, 0.01, 0.25, -0.015, 0.05)
+    bodies = [scenes.Body(4, m1, density=1.5, pos=(6.3, 6.2)), scenes.Body(11, m2, density=0.7, mobility=mob, mobility_at=0)]
+    rng = np.random.default_rng(3)
+    cells = [(x, y) for y in range(H) for x in range(W) if not (m1[y, x] or m2[y, x])]
+    def apply(vec, wset):
+        w, P, c = _body_world(L, W, H, bodies)
+        _mark_codes(w, W, H, bodies)
+        for (x, y), v in zip(cells, vec):
+            w.body_probe(2, (x, y, wset), (float(v),), params=P, ctl=c)
+        out = []
+        for (x, y) in cells:
+            f, _ = w.body_probe(1, (x, y, wset), params=P, ctl=c)
+            out.append(f[0])
+        return np.array(out, dtype=np.float64)
+    p = rng.normal(size=len(cells)).astype(np.float32)
+    q = rng.normal(size=len(cells)).astype(np.float32)
+    Kp, Kq = apply(p, 1), apply(q, 1)
+    a, b2 = float(q @ Kp), float(p @ Kq)
+    assert abs(a - b2) <= 1e-4 * max(abs(a), 1.0), (a, b2)
+    assert float(p @ Kp) > 0.0
+
+
+def pin_swept(L):
+    """cupOf, seamOf, cupBefore (fluid.wgsl:1252-1286,3520-3527;
+    common.wgsl:355-387): the swept word of the step before keys the body by
+    2047 less its slot, bit 0 a seam; a word of another step, a seam asked
+    as a cup, a body not alive, or a slot past MAX_BODIES is none."""
+    W = H = 8
+    m = np.zeros((H, W), dtype=bool)
+    m[0, 0] = True
+    dead = np.zeros((H, W), dtype=bool)
+    dead[0, 7] = True
+    bodies = [scenes.Body(5, m), scenes.Body(6, dead, alive=False)]
+    frame = 10
+    w, P, c = _body_world(L, W, H, bodies, frame=frame)
+    sw = w.part(abi.PART_SWEPT)
+    NONE = 0xFFFFFFFF
+    last, before = (frame - 1) & 1, (frame - 2) & 1
+    sw[2 * 9 + last] = abi.swept_key(frame - 1, 5)                 # cell 9: body 5's cup last step
+    sw[2 * 10 + last] = abi.swept_key(frame - 1, 5, seam=True)     # cell 10: its seam
+    sw[2 * 11 + last] = abi.swept_key(frame - 2, 5)                # cell 11: a word of the wrong step
+    sw[2 * 12 + last] = abi.swept_key(frame - 1, 6)                # cell 12: a dead body's cup
+    sw[2 * 13 + before] = abi.swept_key(frame - 2, 6)              # cell 13: the dead body's cup two steps back
+    sw[2 * 14 + last] = abi.swept_key(frame - 1, 1500)             # cell 14: a slot past MAX_BODIES
+    probe = lambda i, back, seam, weighed: w.body_probe(5, (i, back, seam, weighed), params=P, ctl=c)[1][0]
+    assert probe(9, 1, 0, 1) == 5 and probe(9, 1, 1, 1) == NONE
+    assert probe(10, 1, 1, 1) == 5 and probe(10, 1, 0, 1) == NONE
+    assert probe(11, 1, 0, 1) == NONE
+    assert probe(12, 1, 0, 1) == NONE and probe(12, 1, 0, 0) == 6
+    assert probe(13, 2, 0, 0) == 6 and probe(13, 1, 0, 1) == NONE
+    assert probe(14, 1, 0, 0) == NONE
+
+
+def pin_body_classify(L):
+    """classify's bodies (fluid.wgsl:1600-1622,1668-1675): a loose stone cell
+    of an alive, weighed body is that body's in the solve (drift.z
+    -1 - slot), a held body's -1 - (slot + MAX_BODIES); powder in a body's
+    cup is the cup's body; an empty seam cell is the body's wall; a body not
+    yet weighed, or stone not loose, is a plain wall; a body's cells count as
+    liquid this step."""
+    W, H = 24, 16
+    def rect(x0, y0, x1, y1):
+        m = np.zeros((H, W), dtype=bool)
+        m[y0:y1, x0:x1] = True
+        return m
+    cells = scenes.empty_cells(W, H)
+    scenes.paint(cells, W, H, scenes.border(W, H), E.WALL)
+    free, held, new = rect(2, 2, 5, 5), rect(8, 11, 12, 15), rect(14, 2, 16, 4)
+    cup, seam = rect(18, 4, 21, 6), rect(17, 8, 18, 9)
+    bodies = [scenes.Body(3, free), scenes.Body(9, held, held=True), scenes.Body(4, new, mass=0.0)]
+    w, P, c = _body_world(L, W, H, bodies, cells=cells, cups=[(3, cup)], seams=[(9, seam)])
+    scenes.paint(cells, W, H, cup, E.SAND)
+    w.upload(abi.PART_CELLS, cells)
+    h = Hand.__new__(Hand)
+    h.w, h.W, h.H, h.C = w, W, H, W * H
+    lay = w.layout
+    h.B, h.F, h.cap = lay.blocks, lay.faces, lay.capacity
+    h.field, h.counters, h.blocks = w.part(abi.PART_FIELD), w.part(abi.PART_COUNTERS), w.part(abi.PART_BLOCKS)
+    h.cellwords = w.part(abi.PART_CELLS)
+    h.frame, h.earth_g = 1, 0.0
+    h.activate()
+    ctl = h.ctl()
+    ctl.bodies = 1
+    w.run(PASS_CLASSIFY, 0, h.params(), ctl)
+    cf = _field_cells(w, W, H)
+    z, bits = cf[..., 6], cf[..., 7].astype(np.uint32)
+    assert np.all(z[free] == -1.0 - 3) and np.all(cf[..., 2][free] == -1.0)
+    assert np.all(z[held] == -1.0 - (9 + abi.MAX_BODIES))
+    assert np.all(z[new] == 0.0)                       # not weighed: a wall like any
+    assert np.all(z[cup] == -1.0 - 3)                  # the cup's powder, its cargo
+    assert np.all(z[seam] == -1.0 - (9 + abi.MAX_BODIES)) and np.all(cf[..., 2][seam] == -1.0)
+    mine = 1 << (h.frame & 1)
+    assert np.all(bits[free] & mine) and np.all(bits[held] & mine) and not np.any(bits[new] & mine)
+    # and rigidAt / heldAt read them back
+    assert w.body_probe(3, (3, 3), params=P, ctl=c)[1][0] == 3
+    assert w.body_probe(4, (9, 12), params=P, ctl=c)[1][0] == 9
+    assert w.body_probe(3, (9, 12), params=P, ctl=c)[1][0] == 0xFFFFFFFF
+
+
+def _buoyancy(L, density, held=False, W=48, H=48, depth=34, size=6, steps=1):
+    """A square body of `density` at rest in a still tank (held: on its
+    floor), stepped with the bodies on: the liquid's change of its velocity
+    (bodyPush, rigidify: fluid.wgsl:3037-3063,3206-3218), in units of g."""
+    cells = scenes.still_tank(W, H, depth)
+    m = np.zeros((H, W), dtype=bool)
+    if held:
+        m[H - 1 - size:H - 1, W // 2 - size // 2:W // 2 + size // 2] = True
+    else:
+        y0 = H - 1 - depth + (depth - size) // 2
+        m[y0:y0 + size, W // 2 - size // 2:W // 2 + size // 2] = True
+    b = scenes.Body(5, m, density=density, held=held, wet=held)
+    motion, swept = scenes.body_parts(W, H, cells, [b])
+    w = OracleWorld(W, H, kinds=E.kinds(), L=L)
+    w.set_element_props(BODY_PROPS)
+    w.upload(abi.PART_CELLS, cells)
+    w.upload(abi.PART_MOTION, motion)
+    w.upload(abi.PART_SWEPT, swept)
+    sim = Simulation(w, W, H, solver=SolverSettings.relative(1e-8, max_cycles=200))
+    sim.bodies = sim.bodies_run = True
+    slot = slice(abi.liquid_slot(5, 0), abi.liquid_slot(5, 3))
+    out = []
+    for _ in range(steps):
+        before = w.download(abi.PART_MOTION).view(np.int32)[slot].copy()
+        sim.step()
+        out.append((w.download(abi.PART_MOTION).view(np.int32)[slot] - before) / abi.RIGID_FIX / sim.earth_g)
+    return out
+
+
+def pin_buoyancy(L):
+    """The liquid's push on a body at rest (Archimedes; a square's added mass
+    in 2-D potential flow): a held block on the floor is lifted by exactly
+    its displaced water's weight, -rho_w / rho_b of g; a free body as dense
+    as water is held up by exactly g; a heavier or lighter square starts to
+    sink or rise at a = g (rho_b - rho_w) / (rho_b + C_a rho_w), C_a = 1.19
+    for a square (4.754 rho a^2 of half-side a: Brennen 1982, table 2.1), so
+    the liquid's push is a - g; no sideways push nor turn."""
+    dv = _buoyancy(L, 2.5, held=True)[0]
+    assert abs(dv[1] + 1.0 / 2.5) < 1e-3 and abs(dv[0]) < 1e-3 and abs(dv[2]) < 1e-3, dv
+    dv = _buoyancy(L, 1.0)[0]
+    assert abs(dv[1] + 1.0) < 2e-3 and abs(dv[0]) < 1e-4 and abs(dv[2]) < 1e-4, dv
+    for rho in (2.0, 0.5):
+        dv = _buoyancy(L, rho)[0]
+        want = (rho - 1.0) / (rho + 1.19) - 1.0
+        assert abs(dv[1] - want) < 0.03 * abs(want), (rho, dv, want)
+        assert abs(dv[0]) < 1e-3 and abs(dv[2]) < 1e-3, dv
+
+
 PINS = {
     "restrict": pin_restrict,
     "smooth": pin_smooth,
@@ -1329,4 +1575,10 @@ PINS = {
     "gravity": pin_gravity,
     "viscosity": pin_viscosity,
     "surface": pin_surface,
+    "body_face": pin_body_face,
+    "give_way": pin_give_way,
+    "coupling": pin_coupling,
+    "swept": pin_swept,
+    "body_classify": pin_body_classify,
+    "buoyancy": pin_buoyancy,
 }

@@ -48,6 +48,8 @@ CAUGHT_BY = {
     32: "viscosity",               # the sweeps unrelaxed (omega 1)
     33: "viscosity",               # an obstacle's face across the flow counted once
     34: "surface",                 # the curvature's sign flipped
+    35: "give_way",                # a push's turn over the body's mass, not its moment of inertia
+    36: "body_face",               # the face's arm r x n taken n x r
 }
 
 

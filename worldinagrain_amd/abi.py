@@ -16,10 +16,61 @@ WIG_MAX_CYCLES = 256
 
 PART_CELLS, PART_PARTICLES0, PART_PARTICLES1, PART_COUNTERS, PART_FIELD, PART_BLOCKS, PART_FLUIDCG, PART_SOLVE = range(8)
 PART_AIR, PART_AIR_SOURCE, PART_OCCUPANTS, PART_GRAVITY = 8, 9, 10, 11
-PART_COUNT = 12
-LIQUID_PARTS = 8          # the liquid's own parts (a snapshot's); the rest are the caller's (the air's, the people's)
+PART_MOTION, PART_SWEPT, PART_PUSHES = 12, 13, 14
+PART_COUNT = 15
+LIQUID_PARTS = 8          # the liquid's own parts (a snapshot's); the rest are the caller's (the air's, the people's, the bodies')
 PART_NAMES = ("cells", "particles0", "particles1", "counters", "field", "blocks", "fluidCg", "solve", "air0",
-              "airSource", "occupants", "gravity")
+              "airSource", "occupants", "gravity", "bodyMotion", "bodySwept", "bodyPushes")
+# the bodies (js/shared.js:299-333; common.wgsl:353-387): the table's slots
+# and words (BODY_WORD), their flags (BODY_FLAG), the liquid's slots after the
+# contacts, the push sets
+MAX_BODIES = 1024
+BODY_TABLE_WORDS = 116
+BODY_CONTACTS, BODY_CONTACT_WORDS, BODY_LIQUID_WORDS = 16384, 12, 4
+BODY_PUSH_WORDS = 36
+MOTION_WORDS = MAX_BODIES * (1 + BODY_TABLE_WORDS) + BODY_CONTACTS * BODY_CONTACT_WORDS + MAX_BODIES * BODY_LIQUID_WORDS
+BW_POS, BW_VEL, BW_SPIN, BW_MASS, BW_INERTIA, BW_CELLS, BW_FLAGS, BW_FRAME = 0, 3, 5, 6, 7, 13, 18, 19
+BW_SUBCELL, BW_CARGO, BW_CARGO_INERTIA, BW_MOBILITY, BW_MOBILITY_AT = 25, 105, 106, 108, 114
+BF_ALIVE, BF_NEW, BF_HELD, BF_WET = 1, 2, 32, 64
+FL_LOOSE = 8
+RIGID_FIX = 16777216.0
+EXACT_COARSE = 16777216.0
+LIQUID_WET = 0x80000000
+CLAIM_PERIOD = 0xFFFFF
+PUSH_P0, PUSH_X, PUSH_F, PUSH_SURFACE, PUSH_DRAG = 0, 2, 3, 4, 5
+# an element's columns (WigElementProps): viscosity, viscosity_e, viscosity_t,
+# tension, wetting, breaks
+PROP_COLUMNS = 6
+
+
+def liquid_slot(b, w):
+    """Word w of body b's liquid slot in the motion part (fluid.wgsl:1226-1228)."""
+    return MAX_BODIES * (1 + BODY_TABLE_WORDS) + BODY_CONTACTS * BODY_CONTACT_WORDS + b * BODY_LIQUID_WORDS + w
+
+
+def body_word(b, k):
+    """Word k of body b's slot in the motion part's table (fluid.wgsl:1211-1213)."""
+    return MAX_BODIES + b * BODY_TABLE_WORDS + k
+
+
+def claim_step(frame):
+    """common.wgsl:384-386."""
+    return (frame % CLAIM_PERIOD) + 1
+
+
+def swept_key(frame, b, seam=False):
+    """A swept word for body b stamped at `frame` (common.wgsl:368-370; bit 0 a seam)."""
+    return ((claim_step(frame) << 12) | ((2047 - b) << 1) | (1 if seam else 0)) & 0xFFFFFFFF
+
+
+def element_props(props):
+    """props as the C ABI takes them: (256, PROP_COLUMNS) f32, short rows padded with 0."""
+    import numpy as np
+    p = np.asarray(props, dtype=np.float32)
+    assert p.ndim == 2 and p.shape[0] == 256 and p.shape[1] <= PROP_COLUMNS
+    out = np.zeros((256, PROP_COLUMNS), dtype=np.float32)
+    out[:, :p.shape[1]] = p
+    return out
 # the gravity pass's field: a vec4 (pull x, y, potential, down) a light cell
 # of GRAVITY_BLOCK x GRAVITY_BLOCK cells (pull.wgsl:1-44)
 GRAVITY_BLOCK = 2
@@ -56,7 +107,7 @@ class WigStepCtl(ct.Structure):
         ("coarse_sweeps", ct.c_uint32), ("first_chunk", ct.c_uint32), ("chunk", ct.c_uint32),
         ("level_pairs", ct.c_uint32), ("vcycle_storage", ct.c_uint32), ("residual_every", ct.c_uint32),
         ("air", ct.c_uint32), ("surface_drag", ct.c_float), ("occupants", ct.c_uint32),
-        ("gravity", ct.c_uint32),
+        ("gravity", ct.c_uint32), ("bodies", ct.c_uint32),
     ]
 
 
@@ -64,7 +115,7 @@ class WigLayoutInfo(ct.Structure):
     _fields_ = [
         ("width", ct.c_uint32), ("height", ct.c_uint32), ("cells", ct.c_uint32), ("faces", ct.c_uint32),
         ("blocks", ct.c_uint32), ("blocks_wide", ct.c_uint32), ("capacity", ct.c_uint32), ("levels", ct.c_uint32),
-        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * 12),
+        ("level_width", ct.c_uint32 * 9), ("level_height", ct.c_uint32 * 9), ("part_bytes", ct.c_uint64 * PART_COUNT),
     ]
 
 

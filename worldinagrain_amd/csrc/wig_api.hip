@@ -51,6 +51,15 @@ template <int S> __global__ void k_cgUpdate(WorldGrid g);
 template <int S> __global__ void k_cgTrue(WorldGrid g);
 __global__ void k_cgCheck(WorldGrid g);
 template <int S> __global__ void k_cgFinish(WorldGrid g);
+__global__ void k_bodyStart(WorldGrid g);
+__global__ void k_bodyLoads(WorldGrid g);
+__global__ void k_bodyDrag(WorldGrid g);
+__global__ void k_bodyFaces(WorldGrid g);
+__global__ void k_cgBodyX(WorldGrid g);
+template <int S> __global__ void k_cgBodyP(WorldGrid g);
+__global__ void k_cgBodyF(WorldGrid g);
+__global__ void k_rigidify(WorldGrid g);
+__global__ void k_bodyPush(WorldGrid g);
 __global__ void k_project(WorldGrid g);
 __global__ void k_extrapolate(WorldGrid g, int layer);
 __global__ void k_advect(WorldGrid g);
@@ -95,11 +104,12 @@ int wig_fail_message(int code, const char *msg) { return fail(code, "%s", msg); 
 
 // Kernel ids for wig_time_kernel, in launch order.
 #define WIG_KERNELS(X) X(sync) X(syncParticles) X(syncSeed) X(compactMarks) X(compact) X(retire) X(clear) X(gather) \
-    X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(surface) X(setFaces) X(viscosity) X(divergence) X(cgStart) X(cgAnchor) \
+    X(localJoin) X(edgeJoin) X(sealAnchor) X(classify) X(surface) X(bodyStart) X(bodyLoads) X(setFaces) X(viscosity) \
+    X(bodyDrag) X(bodyFaces) X(divergence) X(cgStart) X(cgAnchor) X(cgBodyX) \
     X(cgResidual) X(sumNorm) X(cgPrepare) X(levelRho) X(relax) X(restrict) X(levelDown) X(coarseLDS) X(levelUp) \
-    X(cgRho) X(cgBeta) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgTrue) X(cgCheck) \
-    X(cgFinish) X(project) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) X(scanTotals) X(scanFinish) \
-    X(scatter) X(rasterClaims) X(raster) X(unpaint)
+    X(cgRho) X(cgBeta) X(cgBodyP) X(cgApply) X(cgAlpha) X(cgUpdate) X(cgTrue) X(cgCheck) \
+    X(cgFinish) X(cgBodyF) X(project) X(rigidify) X(bodyPush) X(extrapolate) X(advect) X(sortCount) X(scanBlocks) \
+    X(scanTotals) X(scanFinish) X(scatter) X(rasterClaims) X(raster) X(unpaint)
 #define X_ENUM(n) KID_##n,
 enum { WIG_KERNELS(X_ENUM) KID_COUNT };
 #define X_NAME(n) #n,
@@ -288,6 +298,9 @@ static void size_world(WigWorld *w, uint32_t width, uint32_t height, uint32_t co
     w->bytes[WIG_PART_AIR_SOURCE] = 4ull * SOURCE_WORDS * d.C;
     w->bytes[WIG_PART_OCCUPANTS] = 4ull * d.C;
     w->bytes[WIG_PART_GRAVITY] = 16ull * ((d.W + 1u) / 2u) * ((d.H + 1u) / 2u);
+    w->bytes[WIG_PART_MOTION] = 4ull * WIG_MOTION_WORDS;
+    w->bytes[WIG_PART_SWEPT] = 8ull * d.C;
+    w->bytes[WIG_PART_PUSHES] = 4ull * WIG_MAX_BODIES * WIG_BODY_PUSH_WORDS;
 }
 
 
@@ -476,6 +489,9 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     d.source = (uint32_t *)w->part[WIG_PART_AIR_SOURCE];
     d.occ = (const uint32_t *)w->part[WIG_PART_OCCUPANTS];
     d.grav = (const float4 *)w->part[WIG_PART_GRAVITY];
+    d.motion = (uint32_t *)w->part[WIG_PART_MOTION];
+    d.swept = (const uint32_t *)w->part[WIG_PART_SWEPT];
+    d.pushes = (uint32_t *)w->part[WIG_PART_PUSHES];
     d.kind = w->kind;
     d.eprops = (const WigElementProps *)(w->kind + 256);   // (the kinds' 256 bytes, then the columns)
     d.scratch = w->scratch;
@@ -796,9 +812,11 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d = w->d;
     if (P->size[0] != (uint32_t)d.W || P->size[1] != (uint32_t)d.H)
         return fail(WIG_E_INVALID, "params size %ux%u is not the world's %dx%d", P->size[0], P->size[1], d.W, d.H);
-    if (ctl->bodies_run) return fail(WIG_E_INVALID, "bodies are not built");
     if (ctl->surface && ctl->residual_every)
         return fail(WIG_E_INVALID, "residual replacement takes the open cells' pressure as 0: not with surface tension");
+    if (ctl->bodies && ctl->residual_every)
+        return fail(WIG_E_INVALID, "residual replacement leaves out the bodies' coupling: not with bodies");
+    if (ctl->bodies_run && !ctl->bodies) return fail(WIG_E_INVALID, "bodies_run needs the bodies' parts (ctl.bodies)");
     if (ctl->fluid_cycles > WIG_MAX_CYCLES || ctl->sweep_pairs > 64u || ctl->coarse_sweeps > 4096u || ctl->tol_mode > 1u
         || ctl->level_pairs > MAX_LEVEL_PAIRS || ctl->vcycle_storage > WIG_STORE_BF16 || ctl->residual_every > WIG_MAX_CYCLES)
         return fail(WIG_E_INVALID, "solver control out of range");
@@ -824,6 +842,8 @@ static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, De
     d.grav_on = ctl->gravity ? 1u : 0u;
     d.visc_on = ctl->viscous ? 1u : 0u;
     d.surf_on = ctl->surface ? 1u : 0u;
+    d.body_on = ctl->bodies ? 1u : 0u;
+    d.bodies_run = ctl->bodies_run ? 1u : 0u;
     d.surface_drag = ctl->surface_drag;
     d.dt = P->dt;
     d.dx = P->dx;
@@ -839,6 +859,8 @@ struct StepPlan {
     bool replace = false;    // a full world replaces its residual (k_cgTrue in the iteration)
     bool viscous = false;    // a full world runs viscosity's sweeps
     bool surface = false;    // a full world runs the surface pass
+    bool bodies = false;     // a full world has bodies (the bodies' passes)
+    bool bodies_run = false; // ... and hands them what the liquid did (rigidify, bodyPush)
     uint32_t stores = 0;     // the V-cycle storages (1 << STORE_*) of the full worlds
     uint32_t max_cycles = 0, first_check = NO_STOP, chunk = NO_STOP, max_level = 0;
     size_t coarse_lds = 0, down_lds[MG_LEVELS + 1] = {}, up_lds[MG_LEVELS + 1] = {};
@@ -864,6 +886,10 @@ static void enqueue_pre(Batch &b, const StepPlan &pl) {
     launch(b, KID_sealAnchor, k_sealAnchor, SH_BG, 64);
     launch(b, KID_classify, k_classify, SH_BG, 64);
     if (pl.surface) launch(b, KID_surface, k_surface, SH_BG, 64);
+    if (pl.bodies) {
+        launch(b, KID_bodyStart, k_bodyStart, SH_BG, 64);
+        launch(b, KID_bodyLoads, k_bodyLoads, SH_BG, 64);
+    }
     launch(b, KID_setFaces, k_setFaces, SH_BG, 64);
     // viscosity's sweeps before the projection; unpress whether they ran or
     // not (js/sim.js:2505-2514)
@@ -872,6 +898,12 @@ static void enqueue_pre(Batch &b, const StepPlan &pl) {
         launch(b, KID_viscosity, k_viscosity, SH_BG, 64, 1);
     }
     launch(b, KID_viscosity, k_viscosity, SH_BG, 64, 2);
+    // the liquid's drag on the bodies, and their faces moving with them again
+    // (js/sim.js:2515-2520)
+    if (pl.bodies && pl.viscous) {
+        launch(b, KID_bodyDrag, k_bodyDrag, SH_BG, 64);
+        launch(b, KID_bodyFaces, k_bodyFaces, SH_BG, 64);
+    }
     launch(b, KID_divergence, k_divergence, SH_BG, 64);
 }
 extern "C++" {
@@ -891,6 +923,7 @@ template <int S> static void enqueue_iteration_store(Batch &b, const StepPlan &p
         launch(b, KID_relax, k_relax<S>, SH_RELAX, 256, 1);
         launch(b, KID_cgRho, k_cgRho<S>, SH_BG16, 256);
     } else if (part == 1) {
+        if (pl.bodies) launch(b, KID_cgBodyP, k_cgBodyP<S>, SH_BG, 64);
         launch(b, KID_cgApply, k_cgApply<S>, SH_BG16, 256);
     } else {
         launch(b, KID_cgUpdate, k_cgUpdate<S>, SH_BG16, 256);
@@ -903,6 +936,7 @@ static void enqueue_solve_start(Batch &b, const StepPlan &pl) {
     launch(b, KID_localJoin, k_localJoin, SH_BG, 64, 1);
     launch(b, KID_edgeJoin, k_edgeJoin, SH_BG, 64, 1);
     launch(b, KID_cgAnchor, k_cgAnchor, SH_BG, 64);
+    if (pl.bodies) launch(b, KID_cgBodyX, k_cgBodyX, SH_BG, 64);
     launch(b, KID_cgResidual, k_cgResidual, SH_BG4, 256);
     launch(b, KID_sumNorm, k_sumNorm, SH_ONE, 1024);
     if (pl.stores & 1u) enqueue_start_store<STORE_F32>(b);
@@ -922,8 +956,14 @@ static void enqueue_iteration(Batch &b, const StepPlan &pl) {
     stores(2);
     launch(b, KID_cgCheck, k_cgCheck, SH_ONE, 1024);
 }
-static void enqueue_post(Batch &b) {
+static void enqueue_post(Batch &b, const StepPlan &pl) {
+    if (pl.bodies) launch(b, KID_cgBodyF, k_cgBodyF, SH_BG, 64);
     launch(b, KID_project, k_project, SH_BG, 64);
+    // what the liquid did to the bodies in it (js/sim.js:2594-2599)
+    if (pl.bodies_run) {
+        launch(b, KID_rigidify, k_rigidify, SH_BG, 64);
+        launch(b, KID_bodyPush, k_bodyPush, SH_ONE, 1024);
+    }
     launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 1);
     launch(b, KID_extrapolate, k_extrapolate, SH_BG, 64, 2);
     launch(b, KID_advect, k_advect, SH_PG, 256);
@@ -953,7 +993,7 @@ static int run_phase(Batch &b, int ph, const StepPlan &pl) {
             if (pl.stores & 2u) launch(b, KID_cgFinish, k_cgFinish<STORE_F16>, SH_BG, 64);
             if (pl.stores & 4u) launch(b, KID_cgFinish, k_cgFinish<STORE_BF16>, SH_BG, 64);
             break;
-        default: enqueue_post(b); break;
+        default: enqueue_post(b, pl); break;
         }
     };
     if (!b.graphs || b.timed >= 0) {
@@ -1048,6 +1088,8 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         pl.replace = pl.replace || d.replace_every != 0u;
         pl.viscous = pl.viscous || d.visc_on != 0u;
         pl.surface = pl.surface || d.surf_on != 0u;
+        pl.bodies = pl.bodies || d.body_on != 0u;
+        pl.bodies_run = pl.bodies_run || d.bodies_run != 0u;
         pl.max_cycles = std::max(pl.max_cycles, d.fluid_cycles);
         if (d.first_chunk != 0u) {
             uint32_t f = d.first_chunk == WIG_CHUNK_AUTO ? std::max(b.w[i]->last_hint, 1u) : d.first_chunk;
@@ -1071,6 +1113,8 @@ static int batch_step(Batch &b, const WigParams *P, const WigStepCtl *ctl) {
         sig.push_back(pl.replace);
         sig.push_back(pl.viscous);
         sig.push_back(pl.surface);
+        sig.push_back(pl.bodies);
+        sig.push_back(pl.bodies_run);
         sig.push_back(pl.max_level);
         for (uint32_t l = 0; l <= MG_LEVELS; l++) sig.push_back(pl.down_lds[l]), sig.push_back(pl.up_lds[l]);
         sig.push_back((uint64_t)(uintptr_t)b.stream);

@@ -16,6 +16,7 @@
 #define MG_LEVELS 8
 #define CF_FLUID 1u
 #define CF_PINNED 2u
+#define CF_BODY 4u      // a fluid cell beside a free body's (the solve couples it: couplingAt)
 #define LDS_LEVEL_FLOATS 7u   // k_coarseLDS: p, rhs (2 each), rho, two face betas a cell
 #define WNONE 0xffffffffu
 #define G_COUNT 0u
@@ -113,6 +114,40 @@
 #define SOURCE_MIST 4u
 #define SOURCE_LOST 5u
 #define SOURCE_HEAT 6u
+// the bodies (js/shared.js:294-333; common.wgsl:353,377-387; fluid.wgsl:1304-1318,2963)
+#define MAX_BODIES WIG_MAX_BODIES
+#define BODY_TABLE_WORDS 116u
+#define BODY_CONTACTS 16384u
+#define BODY_CONTACT_WORDS 12u
+#define BODY_LIQUID_WORDS 4u
+#define BODY_PUSH_WORDS WIG_BODY_PUSH_WORDS
+#define BW_POS 0u
+#define BW_VEL 3u
+#define BW_SPIN 5u
+#define BW_MASS 6u
+#define BW_INERTIA 7u
+#define BW_CELLS 13u
+#define BW_FLAGS 18u
+#define BW_SUBCELL 25u
+#define BW_CARGO 105u
+#define BW_CARGO_INERTIA 106u
+#define BW_MOBILITY 108u
+#define BW_MOBILITY_AT 114u
+#define BF_ALIVE 1u
+#define BF_NEW 2u
+#define BF_HELD 32u
+#define BF_WET 64u
+#define FL_LOOSE 8u
+#define RIGID_FIX 16777216.0f
+#define EXACT_COARSE 16777216.0f
+#define LIQUID_WET 0x80000000u
+#define CLAIM_PERIOD 0xfffffu
+#define SEEP_REACH 128
+#define PUSH_P0 0u
+#define PUSH_X 2u
+#define PUSH_F 3u
+#define PUSH_SURFACE 4u
+#define PUSH_DRAG 5u
 
 // The FP64 state of a solve: its header, then three doubles per block
 // (two running sums of a pass and the squared residual).
@@ -207,6 +242,12 @@ struct Dev {
     const float4 *grav;          // the gravity pass's field (WIG_PART_GRAVITY), read when grav_on
     uint32_t *source;
     uint32_t air_on, occ_on, grav_on, visc_on, surf_on;
+    // the bodies (WigStepCtl.bodies): the bodies' table and liquid slots, the
+    // swept words, the step's push sums; read when body_on
+    uint32_t *motion;
+    const uint32_t *swept;
+    uint32_t *pushes;
+    uint32_t body_on, bodies_run;
     float surface_drag, dt, dx;
 };
 

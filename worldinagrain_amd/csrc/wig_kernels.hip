@@ -220,6 +220,236 @@ __device__ int32_t countAt(const Dev &d, i2 c) {                           // :1
     if (isObstacleCell(d, c, d.cells[4ull * i])) return -1;
     return (int32_t)CNT(d, i, G_COUNT);
 }
+__device__ __forceinline__ float4 gridAt4(const Dev &d, i2 c) {
+    if (!inWorld(d, c)) return make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+    return *(const float4 *)&FLD(d, 2u * cellIndex(d, c), 0);
+}
+// ------------------------------------------------------------ bodies
+// Free and held bodies in the liquid (fluid.wgsl:1181-1475): the bodies'
+// table in `motion` (js/shared.js:294-333's BODY_WORD words), the bodies
+// buffer's swept words (common.wgsl:355-370), and the step's push sums.
+// With the bodies off (d.body_on 0) none is read and no cell is a body's.
+struct v3 { float x, y, z; };
+__device__ __forceinline__ v3 V3(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ uint32_t bodyWord(const Dev &d, uint32_t b, uint32_t k) {    // :1211-1213
+    return d.motion[MAX_BODIES + b * BODY_TABLE_WORDS + k];
+}
+__device__ __forceinline__ float bodyF(const Dev &d, uint32_t b, uint32_t k) { return __uint_as_float(bodyWord(d, b, k)); }
+__device__ __forceinline__ v2 bodyV(const Dev &d, uint32_t b, uint32_t k) { return V2(bodyF(d, b, k), bodyF(d, b, k + 1u)); }
+__device__ __forceinline__ uint32_t liquidSlot(uint32_t b, uint32_t w) {                  // :1226-1228
+    return MAX_BODIES * (1u + BODY_TABLE_WORDS) + BODY_CONTACTS * BODY_CONTACT_WORDS + b * BODY_LIQUID_WORDS + w;
+}
+// alive and weighed (:1244-1246)
+__device__ __forceinline__ bool bodyWeighed(const Dev &d, uint32_t b) {
+    const uint32_t flags = bodyWord(d, b, BW_FLAGS);
+    return !((flags & BF_ALIVE) == 0u || (flags & BF_NEW) != 0u || bodyF(d, b, BW_MASS) <= 0.0f);
+}
+__device__ uint32_t rigidBodyOf(const Dev &d, uint4 v) {                                 // :1233-1248
+    if (!d.body_on) return WNONE;
+    const uint32_t t = wType(v.x);
+    if (!wHas(v.x, FL_LOOSE) || kindOf(d, t) != KIND_SOLID || d.eprops[t].breaks == 0.0f) return WNONE;
+    const float id = unpack2x16(v.w).x;
+    if (id < 0.0f || id >= (float)MAX_BODIES) return WNONE;
+    const uint32_t b = wu32(id);
+    return bodyWeighed(d, b) ? b : WNONE;
+}
+__device__ __forceinline__ uint32_t claimStep(uint32_t frame) { return frame % CLAIM_PERIOD + 1u; }   // common.wgsl:384-386
+// the body whose cup (seam 0) or seam (1) cell i was stamped as `back` steps
+// ago, alive and weighed when `weighed` (cupOf :1252-1266, seamOf
+// :1272-1286; cupBefore :3520-3527 asks only for a slot)
+__device__ uint32_t sweptBody(const Dev &d, uint32_t i, uint32_t back, uint32_t seam, bool weighed) {
+    if (!d.body_on) return WNONE;
+    const uint32_t frame = d.frame - back;
+    const uint32_t key = d.swept[2ull * i + (frame & 1u)];
+    if ((key >> 12) != claimStep(frame) || (key & 1u) != seam) return WNONE;
+    const uint32_t b = 2047u - ((key >> 1) & 0x7ffu);
+    if (b >= MAX_BODIES) return WNONE;
+    if (weighed && !bodyWeighed(d, b)) return WNONE;
+    return b;
+}
+__device__ __forceinline__ uint32_t cupOf(const Dev &d, uint32_t i) { return sweptBody(d, i, 1u, 0u, true); }
+__device__ __forceinline__ uint32_t seamOf(const Dev &d, uint32_t i) { return sweptBody(d, i, 1u, 1u, true); }
+__device__ __forceinline__ uint32_t cupBefore(const Dev &d, uint32_t i) { return sweptBody(d, i, 2u, 0u, false); }
+__device__ __forceinline__ float rigidMass(const Dev &d, uint32_t b) { return bodyF(d, b, BW_MASS) + bodyF(d, b, BW_CARGO); }
+__device__ __forceinline__ float rigidInertia(const Dev &d, uint32_t b) { return bodyF(d, b, BW_INERTIA) + bodyF(d, b, BW_CARGO_INERTIA); }
+__device__ __forceinline__ float bodyDensity(const Dev &d, uint32_t b) {                 // :1300-1302
+    return rigidMass(d, b) / fmaxf((float)bodyWord(d, b, BW_CELLS), 1.0f);
+}
+__device__ __forceinline__ uint32_t pushSlot(uint32_t b, uint32_t which, uint32_t k) { return b * BODY_PUSH_WORDS + 6u * which + 2u * k; }
+__device__ void addPush(const Dev &d, uint32_t b, uint32_t which, v3 v) {                 // :1324-1331
+    const float c[3] = {v.x, v.y, v.z};
+#pragma unroll
+    for (uint32_t k = 0; k < 3u; k++) {
+        const u2 e = exactParts(c[k], EXACT_COARSE);
+        const uint32_t at = pushSlot(b, which, k);
+        const uint32_t old = atomicAdd(&d.pushes[at], e.lo);
+        atomicAdd(&d.pushes[at + 1u], e.hi + (old + e.lo < old ? 1u : 0u));
+    }
+}
+__device__ v3 pushOf(const Dev &d, uint32_t b, uint32_t which) {                           // :1333-1340
+    float c[3];
+#pragma unroll
+    for (uint32_t k = 0; k < 3u; k++) {
+        const uint32_t at = pushSlot(b, which, k);
+        u2 e;
+        e.lo = d.pushes[at];
+        e.hi = d.pushes[at + 1u];
+        c[k] = exactValue(e, EXACT_COARSE);
+    }
+    return V3(c[0], c[1], c[2]);
+}
+__device__ v3 giveWay(const Dev &d, uint32_t b, v3 g) {                                     // :1349-1356
+    if (bodyWord(d, b, BW_MOBILITY_AT) != d.frame - 1u)
+        return V3(g.x / rigidMass(d, b), g.y / rigidMass(d, b), g.z / rigidInertia(d, b));
+    float m[6];
+#pragma unroll
+    for (uint32_t k = 0; k < 6u; k++) m[k] = bodyF(d, b, BW_MOBILITY + k);
+    return V3(m[0] * g.x + m[1] * g.y + m[2] * g.z, m[1] * g.x + m[3] * g.y + m[4] * g.z, m[2] * g.x + m[4] * g.y + m[5] * g.z);
+}
+__device__ __forceinline__ v3 pushChange(const Dev &d, uint32_t b, uint32_t which) { return giveWay(d, b, pushOf(d, b, which)); }
+__device__ __forceinline__ v3 pushChangeFree(const Dev &d, uint32_t b, uint32_t which) {   // :1367-1370
+    const v3 g = pushOf(d, b, which);
+    return V3(g.x / rigidMass(d, b), g.y / rigidMass(d, b), g.z / rigidInertia(d, b));
+}
+// the face from cell c to body b's cell q: (n, r x n) (:1374-1378)
+__device__ v3 bodyFaceJ(const Dev &d, i2 c, i2 q, uint32_t b) {
+    const v2 n = V2((float)(q.x - c.x), (float)(q.y - c.y));
+    const v2 pos = bodyV(d, b, BW_POS);
+    const v2 r = V2((float)c.x + 0.5f + 0.5f * n.x - pos.x, (float)c.y + 0.5f + 0.5f * n.y - pos.y);
+    return V3(n.x, n.y, r.x * n.y - r.y * n.x);
+}
+// the body whose cell c is in the solve, as classify found it (:1422-1429),
+// and the held body's (:1450-1457)
+__device__ __forceinline__ uint32_t rigidAt(const Dev &d, i2 c) {
+    if (!d.body_on || !inWorld(d, c)) return WNONE;
+    const float z = FLD(d, 2u * cellIndex(d, c) + 1u, 2);
+    const uint32_t b = wu32(fmaxf(-z, 1.0f)) - 1u;
+    return z < 0.0f && b < MAX_BODIES ? b : WNONE;
+}
+__device__ __forceinline__ uint32_t heldAt(const Dev &d, i2 c) {
+    if (!d.body_on || !inWorld(d, c)) return WNONE;
+    const float z = FLD(d, 2u * cellIndex(d, c) + 1u, 2);
+    const uint32_t b = wu32(fmaxf(-z, 1.0f)) - 1u;
+    return z < 0.0f && b >= MAX_BODIES ? b - MAX_BODIES : WNONE;
+}
+__device__ __forceinline__ bool isHeld(const Dev &d, uint32_t b) { return (bodyWord(d, b, BW_FLAGS) & BF_HELD) != 0u; }
+__device__ __forceinline__ v2 rigidVel(const Dev &d, uint32_t b, v2 x) {                   // :1460-1464
+    const v2 pos = bodyV(d, b, BW_POS), vel = bodyV(d, b, BW_VEL);
+    const v2 r = V2(x.x - pos.x, x.y - pos.y);
+    const float spin = bodyF(d, b, BW_SPIN);
+    return V2(vel.x + -spin * r.y, vel.y + spin * r.x);
+}
+__device__ __forceinline__ bool shutAt(const Dev &d, i2 c) { return gridZ(d, c) < 0.0f || rigidAt(d, c) != WNONE; }   // :1468-1470
+__device__ __forceinline__ v2 faceMiddle(i2 at, bool isU) { return isU ? V2((float)at.x, (float)at.y + 0.5f) : V2((float)at.x + 0.5f, (float)at.y); }
+__device__ __forceinline__ float bodyReach(const Dev &d, uint32_t b, i2 n) {                // :413-415
+    const v2 s = unpack2x16(bodyWord(d, b, BW_SUBCELL));
+    return wclamp(0.5f + (s.x * (float)n.x + s.y * (float)n.y), 0.0f, 1.0f);
+}
+__device__ void addPressurePush(const Dev &d, i2 c, float p, uint32_t which) {            // :1381-1389
+    for (int k = 0; k < 4; k++) {
+        const i2 q = beside(c, k);
+        const uint32_t b = rigidAt(d, q);
+        if (b != WNONE) {
+            const v3 j = bodyFaceJ(d, c, q, b);
+            addPush(d, b, which, V3(j.x * p, j.y * p, j.z * p));
+        }
+    }
+}
+__device__ float couplingAt(const Dev &d, i2 c, uint32_t which) {                          // :1393-1403
+    float sum = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        const i2 q = beside(c, k);
+        const uint32_t b = rigidAt(d, q);
+        if (b != WNONE) {
+            const v3 j = bodyFaceJ(d, c, q, b), ch = pushChange(d, b, which);
+            sum += j.x * ch.x + j.y * ch.y + j.z * ch.z;
+        }
+    }
+    return sum;
+}
+// body b's velocity at x going into the solve (:1410-1419)
+__device__ v2 bodyMoving(const Dev &d, uint32_t b, v2 x, bool drag) {
+    const v2 pos = bodyV(d, b, BW_POS);
+    const v2 pull = pullAt(d, pos);
+    const float mass = rigidMass(d, b);
+    const v3 s = pushChange(d, b, PUSH_SURFACE), g = giveWay(d, b, V3(mass * pull.x, mass * pull.y, 0.0f));
+    v3 ch = V3(s.x + g.x, s.y + g.y, s.z + g.z);
+    if (drag) {
+        const v3 dr = pushChange(d, b, PUSH_DRAG);
+        ch = V3(ch.x + dr.x, ch.y + dr.y, ch.z + dr.z);
+    }
+    const v2 r = V2(x.x - pos.x, x.y - pos.y);
+    const v2 rv = rigidVel(d, b, x), px = pullAt(d, x);
+    return V2(rv.x + px.x - pull.x + ch.x + -ch.z * r.y, rv.y + px.y - pull.y + ch.y + ch.z * r.x);
+}
+// the pull's potential at pos, Earth's and the masses' (:2954-2956)
+__device__ float potentialAt(const Dev &d, v2 pos) {
+    float z = 0.0f;
+    if (d.grav_on) {
+        const float qx = pos.x / (float)GRAVITY_BLOCK - 0.5f, qy = pos.y / (float)GRAVITY_BLOCK - 0.5f;
+        const float fx0 = floorf(qx), fy0 = floorf(qy);
+        const i2 i = I2(wi32(fx0), wi32(fy0));
+        const float fx = qx - fx0, fy = qy - fy0;
+        const float4 a = gravityTexel(d, i), b = gravityTexel(d, I2(i.x + 1, i.y));
+        const float4 c = gravityTexel(d, I2(i.x, i.y + 1)), e = gravityTexel(d, I2(i.x + 1, i.y + 1));
+        z = wmix(wmix(a.z, b.z, fx), wmix(c.z, e.z, fx), fy);
+    }
+    return d.earthG * ((float)d.H - pos.y) + z;
+}
+// the liquid's pressure seeping in at x on body h's face from c toward n
+// (:2976-3001)
+__device__ float seepAt(const Dev &d, i2 c, i2 n, uint32_t h, bool held, v2 x) {
+    i2 ways[2] = {I2(-1, 0), I2(1, 0)};
+    int count = 2;
+    if (n.x != 0) {
+        ways[0] = I2(-n.x, -n.y);
+        count = 1;
+    }
+    int nearest = SEEP_REACH + 1;
+    float p = 0.0f;
+    for (int k2 = 0; k2 < count; k2++) {
+        for (int k = 1; k < nearest; k++) {
+            const i2 q = I2(c.x + ways[k2].x * k, c.y + ways[k2].y * k);
+            if ((held ? heldAt(d, q) : rigidAt(d, q)) == h) continue;
+            const float4 g = gridAt4(d, q);
+            if (g.z > 0.0f) {
+                nearest = k;
+                p = g.x + g.z * (potentialAt(d, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) - potentialAt(d, x));
+            }
+            break;
+        }
+    }
+    return p;
+}
+// what the liquid's pressure on held body h's cell c does to it (:3010-3035)
+__device__ void heldPush(const Dev &d, i2 c, uint32_t h, int32_t out[3]) {
+    const v2 pos = bodyV(d, h, BW_POS);
+    v2 f = V2(0.0f, 0.0f);
+    float torque = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        const i2 q = beside(c, k);
+        const i2 n = I2(q.x - c.x, q.y - c.y);
+        if (heldAt(d, q) == h) continue;
+        const float4 g = gridAt4(d, q);
+        const v2 x = V2((float)c.x + 0.5f + 0.5f * (float)n.x, (float)c.y + 0.5f + 0.5f * (float)n.y);
+        float p = g.x;
+        if (g.z > 0.0f) p += g.z * (potentialAt(d, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) - potentialAt(d, x));
+        else if (g.z < 0.0f) p = seepAt(d, c, n, h, true, x);
+        const v2 push = V2(-p * (float)n.x, -p * (float)n.y);
+        f = V2(f.x + push.x, f.y + push.y);
+        const v2 r = V2(x.x - pos.x, x.y - pos.y);
+        torque += r.x * push.y - r.y * push.x;
+    }
+    const float m = rigidMass(d, h), in = rigidInertia(d, h);
+    out[0] = wi32(wround(f.x / m * RIGID_FIX));
+    out[1] = wi32(wround(f.y / m * RIGID_FIX));
+    out[2] = wi32(wround(torque / in * RIGID_FIX));
+}
+// a cell of a held body that was wet in the last solve (:583-595)
+__device__ __forceinline__ bool wetRegionAt(const Dev &d, uint4 v) {
+    const uint32_t b = rigidBodyOf(d, v);
+    return b != WNONE && (bodyWord(d, b, BW_FLAGS) & (BF_WET | BF_HELD)) == (BF_WET | BF_HELD);
+}
 // e^x from basic float operations (round to nearest, no fused
 // multiply-add): x = k ln2 + r, k = rint(x / ln2), ln2 split in two (Cody
 // and Waite); e^r by its Taylor series to r^7 on |r| <= ln2 / 2; times 2^k
@@ -368,7 +598,7 @@ __global__ void __launch_bounds__(64) k_sync(WorldGrid g) {
     const uint32_t gx_ = d.bw, bx = bid % gx_, by = bid / gx_;
     uint32_t li = threadIdx.x;
     i2 c = I2((int32_t)bx * WIG_BLOCK + (int32_t)(li % 8u), (int32_t)by * WIG_BLOCK + (int32_t)(li / 8u));
-    bool liquid = false;
+    bool liquid = false, region = false;
     if (inWorld(d, c)) {
         uint32_t i = cellIndex(d, c);
         if (i == 0u) {
@@ -382,9 +612,12 @@ __global__ void __launch_bounds__(64) k_sync(WorldGrid g) {
             uint32_t home = v.w >> 8;
             if (claims && wHas(v.x, FL_DRAWN) && home < d.C) atomicMax(&CNT(d, home, G_DEST), DEST_CELL | (DEST_SPAN - i));
             liquid = true;
+        } else if (d.body_on) {
+            region = wetRegionAt(d, v);
         }
     }
     if (__ballot(liquid) && li == 0) atomicOr(&d.blocks[by * d.bw + bx], MARK_LIQUID);
+    if (__ballot(region) && li == 0) atomicOr(&d.blocks[by * d.bw + bx], MARK_REGION);
 }
 
 __device__ __forceinline__ void markBlockOf(const Dev &d, v2 pos) {
@@ -485,7 +718,9 @@ __device__ __forceinline__ bool syncOne(const Dev &d, uint32_t k, PState &q) {
     }
     if ((dest & DEST_ASIDE) != 0u) {
         uint32_t to = dest & ~DEST_ASIDE;
-        q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
+        // not into another body's cup (:785-794)
+        if (cupOf(d, to) == WNONE)
+            q.pos = V2((float)(to % (uint32_t)d.W) + (q.pos.x - floorf(q.pos.x)), (float)(to / (uint32_t)d.W) + (q.pos.y - floorf(q.pos.y)));
         markBlockOf(d, q.pos);
         return true;
     }
@@ -1101,8 +1336,29 @@ __global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
             float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             bool sealed = false;
             if (n == 0 && mayBeSealed(d, p)) sealed = (ufLink(d, ufFind(d, i)) & REGION_OPEN) == 0u;
-            if (n < 0) {
+            uint32_t rigid = WNONE, held = WNONE;
+            const uint32_t seam = n < 0 ? WNONE : seamOf(d, i);
+            if (seam != WNONE) {
+                // a gap the stamp opened in a body's wall: the body's (:1603-1612)
                 c[2] = -1.0f;
+                rigid = seam;
+                if (isHeld(d, seam)) {
+                    held = seam;
+                    rigid = WNONE;
+                }
+                sealed = false;
+            } else if (n < 0) {
+                c[2] = -1.0f;
+                if (d.body_on) {
+                    const uint4 v = *(const uint4 *)&d.cells[4ull * i];
+                    rigid = rigidBodyOf(d, v);
+                    // powder in its cup is its cargo, moving with it (:1616-1619)
+                    if (rigid == WNONE && kindOf(d, wType(v.x)) == KIND_POWDER) rigid = cupOf(d, i);
+                    if (rigid != WNONE && isHeld(d, rigid)) {
+                        held = rigid;
+                        rigid = WNONE;
+                    }
+                }
             } else if (n > 0) {
                 v2 li = liquidIn(d, i);
                 c[2] = li.x;
@@ -1136,8 +1392,10 @@ __global__ void __launch_bounds__(64) k_classify(WorldGrid g) {
                 }
             }
             uint32_t mine = stepBit(d, 0u);
-            uint32_t liquid = c[2] > 0.0f ? mine : 0u;
+            uint32_t liquid = (c[2] > 0.0f || rigid != WNONE || held != WNONE) ? mine : 0u;
             float4 drift = make_float4(0.0f, 0.0f, sealed ? 1.0f : 0.0f, (float)((wu32(wasDrift.w) & ~(mine | fallingBit(d, 0u))) | liquid));
+            if (rigid != WNONE) drift.z = -1.0f - (float)rigid;
+            else if (held != WNONE) drift.z = -1.0f - (float)(held + MAX_BODIES);
             if (c[2] > 0.0f) {
                 if (was.z > 0.0f) {
                     c[0] = was.x;
@@ -1218,10 +1476,6 @@ __device__ __forceinline__ float obstacleVel(const Dev &d, i2 c, int axis) {
     v2 u = unpack2x16(v.z);
     return (axis == 0 ? u.x : u.y) * POWDER_PUSH;
 }
-__device__ __forceinline__ float4 gridAt4(const Dev &d, i2 c) {
-    if (!inWorld(d, c)) return make_float4(0.0f, 0.0f, -1.0f, 0.0f);
-    return *(const float4 *)&FLD(d, 2u * cellIndex(d, c), 0);
-}
 __device__ __forceinline__ float faceViscosity(float4 ga, float4 gb) {
     if (ga.z < 0.0f || gb.z < 0.0f) return 0.0f;
     float fa = ga.z > 0.0f ? 1.0f : 0.0f, fb = gb.z > 0.0f ? 1.0f : 0.0f;
@@ -1236,11 +1490,39 @@ __device__ __forceinline__ float lastPush(const Dev &d, i2 a, i2 at, float4 ga, 
     if ((ga.z > 0.0f && !wasLiquid(d, a)) || (gb.z > 0.0f && !wasLiquid(d, at))) return 0.0f;
     return fineBeta(ga.z, gb.z, densityAt(d, a), densityAt(d, at)) * (gb.x - ga.x);
 }
+// a face of a free body's cell: the body's velocity there going into the
+// solve, the mean between two bodies' cells (:1841-1858)
+__device__ void setBodyFace(const Dev &d, uint32_t f, i2 at, bool isU, float4 ga, float4 gb, uint32_t ra, uint32_t rb, bool drag) {
+    const int axis = isU ? 0 : 1;
+    const v2 x = faceMiddle(at, isU);
+    float now, u;
+    if (ra != WNONE && rb != WNONE) {
+        const v2 ma = bodyMoving(d, ra, x, drag), mb = bodyMoving(d, rb, x, drag), va = rigidVel(d, ra, x), vb = rigidVel(d, rb, x);
+        now = 0.5f * ((axis ? ma.y : ma.x) + (axis ? mb.y : mb.x));
+        u = 0.5f * ((axis ? va.y : va.x) + (axis ? vb.y : vb.x));
+    } else {
+        const uint32_t b = ra != WNONE ? ra : rb;
+        const v2 m = bodyMoving(d, b, x, drag), v = rigidVel(d, b, x);
+        now = axis ? m.y : m.x;
+        u = axis ? v.y : v.x;
+    }
+    const float4 other = ra != WNONE ? gb : ga;
+    const float nu = (ra == WNONE || rb == WNONE) && other.z > 0.0f ? other.w : 0.0f;
+    *(float4 *)&FLD(d, faceSlot(d, f), 0) = make_float4(now, u, 1.0f, nu);
+    FLD(d, viscRhsSlot(d, f), f % 4u) = now;
+}
 __device__ void setFace(const Dev &d, i2 at, bool isU) {
     uint32_t f = faceIndex(d, at, isU);
     i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
     float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
     int32_t weight = (int32_t)SUM(d, f, 1);
+    if (d.body_on) {
+        const uint32_t ra = rigidAt(d, a), rb = rigidAt(d, at);
+        if (ra != WNONE || rb != WNONE) {
+            setBodyFace(d, f, at, isU, ga, gb, ra, rb, false);
+            return;
+        }
+    }
     float u = 0.0f, held = 0.0f;
     if (ga.z < 0.0f && gb.z < 0.0f) {
         held = -1.0f;
@@ -1288,12 +1570,20 @@ __device__ void viscousFace(const Dev &d, i2 at, bool isU, int color) {
     if (((at.x + at.y) & 1) != color) return;
     const i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
     const float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
-    if (ga.z < 0.0f || gb.z < 0.0f) return;
+    // a body's face with liquid beyond it is solved too, the body's mass on
+    // its side (:3361-3377)
+    const uint32_t ra = rigidAt(d, a), rb = rigidAt(d, at);
+    const bool surface = (ra != WNONE) != (rb != WNONE) && (ra != WNONE ? gb.z : ga.z) > 0.0f;
+    if (!surface && (ga.z < 0.0f || gb.z < 0.0f || ra != WNONE || rb != WNONE)) return;
     const uint32_t f = faceIndex(d, at, isU);
     const uint64_t fs = faceSlot(d, f);
     const float nu = FLD(d, fs, 3);
     if (nu <= 0.0f) return;
-    const float heavy = 1.0f;
+    float heavy = 1.0f;
+    if (surface) {
+        const float liquid = ra != WNONE ? gb.z : ga.z;
+        heavy = 2.0f * liquid / (liquid + bodyDensity(d, ra != WNONE ? ra : rb));
+    }
     const float through[2] = {ga.z > 0.0f ? ga.w : 0.0f, gb.z > 0.0f ? gb.w : 0.0f};
     // the four faces beside along the same axis, loaded at once
     float4 other[4];
@@ -1350,7 +1640,7 @@ __global__ void __launch_bounds__(64) k_viscosity(WorldGrid g, int color) {
             const i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
             const float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
             const uint32_t f = faceIndex(d, at, isU);
-            if (faceViscosity(ga, gb) > 0.0f) {
+            if (rigidAt(d, a) == WNONE && rigidAt(d, at) == WNONE && faceViscosity(ga, gb) > 0.0f) {
                 const uint64_t fs0 = faceSlot(d, f);
                 FLD(d, viscLastSlot(d, f), f % 4u) = FLD(d, fs0, 0) - FLD(d, viscRhsSlot(d, f), f % 4u);
                 FLD(d, fs0, 0) += lastPush(d, a, at, ga, gb);
@@ -1378,7 +1668,14 @@ __global__ void __launch_bounds__(64) k_divergence(WorldGrid g) {
             + FLD(d, faceSlot(d, vFace(d, c.x, c.y + 1)), 0) - FLD(d, faceSlot(d, vFace(d, c.x, c.y)), 0);
         FLD(d, 2u * i, 1) = out;
         float n = densityAt(d, c);
-        float room = wclamp(1.0f, 0.1f, 1.9f);
+        float room = 1.0f;                                   // posedRoom (:2035-2045)
+        if (d.body_on)
+            for (int k = 0; k < 4; k++) {
+                const i2 q = beside(c, k);
+                const uint32_t b = rigidAt(d, q);
+                if (b != WNONE) room += 0.5f - bodyReach(d, b, I2(c.x - q.x, c.y - q.y));
+            }
+        room = wclamp(room, 0.1f, 1.9f);
         FLD(d, 2u * i + 1u, 1) = FLD(d, 2u * i + 1u, 2) < 0.0f ? 0.0f : -SPREAD * (n - room);
     }
 }
@@ -2729,6 +3026,11 @@ __device__ __forceinline__ void placeCellsFrom(const Iter &it, uint32_t w0, cons
 // the packed index of the cell beside packed cell i (beside()'s k) in the
 // next block over, for a lane on that edge of its block; i itself where
 // that is off the packed arrays (a cell outside the world, whose beta is 0)
+// the cell of packed index i
+__device__ __forceinline__ i2 packedCell(const Dev &d, uint32_t i) {
+    const uint32_t b = i / 64u, l = i % 64u;
+    return I2((int32_t)((b % d.bw) * 8u + (l & 7u)), (int32_t)((b / d.bw) * 8u + (l >> 3)));
+}
 __device__ __forceinline__ uint32_t acrossEdge(const Dev &d, uint32_t i, int k) {
     uint32_t n = k == 0 ? i - 57u : (k == 1 ? i + 57u : (k == 2 ? i - 64u * d.bw + 56u : i + 64u * d.bw - 56u));
     return n < 64u * d.B ? n : i;
@@ -2755,6 +3057,7 @@ __global__ void __launch_bounds__(256) k_cgResidual(WorldGrid g) {
         if (inWorld(d, c) && gridZ(d, c) > 0.0f) {
             uint32_t i = cellIndex(d, c), pi = packIndex(d, c);
             float r = -residualAt(d, 0u, c).x;
+            if (d.body_on) r = r - couplingAt(d, c, PUSH_X);   // (:2714-2716)
             d.cr[pi] = r;
             most = ubits(fabsf(r));
             double bi = -(double)FLD(d, 2u * i, 1);
@@ -2819,6 +3122,9 @@ template <int S> __global__ void __launch_bounds__(64) k_cgPrepare(WorldGrid g) 
         float2 r = make_float2(0.0f, 0.0f);
         if (rho > 0.0f) {
             f = CF_FLUID | (pinnedAt(d, i) ? CF_PINNED : 0u);
+            if (d.body_on)
+                for (int k = 0; k < 4; k++)
+                    if (rigidAt(d, beside(c, k)) != WNONE) f |= CF_BODY;
             float dc = densityAt(d, c), b[4];
             for (int k = 0; k < 4; k++) {
                 i2 n = beside(c, k);
@@ -2949,6 +3255,9 @@ template <int S> __global__ void __launch_bounds__(256) k_cgApply(WorldGrid g) {
                 nb[k] = edge[k] ? e : o;
             }
             float q = (f[j] & CF_FLUID) ? applyA(cf[j], f[j], p, nb) : 0.0f;
+            // the bodies' give, J^T M^-1 J p (:2790)
+            if (d.body_on && (f[j] & CF_FLUID))
+                q = q + ((f[j] & CF_BODY) ? couplingAt(d, packedCell(d, i[j]), PUSH_P0 + it.cycle % 2u) : 0.0f);
             double v = (double)p * (double)q;
             v = wave_tree_sum(v);
             if (w0 + j < it.active) {
@@ -2976,6 +3285,11 @@ __global__ void __launch_bounds__(1024) k_cgAlpha(WorldGrid g) {
         CGV(d, d.C + d.B, 1) = (float)h->alpha;
         CGV(d, d.C + d.B, 3) = (float)pq;
     }
+    // the bodies' sums of this iteration's direction, done with, emptied for
+    // the iteration after next (:2808-2813)
+    if (d.body_on)
+        for (uint32_t k = threadIdx.x; k < MAX_BODIES * 6u; k += blockDim.x)
+            d.pushes[pushSlot(k / 6u, PUSH_P0 + it.cycle % 2u, 0u) + k % 6u] = 0u;
 }
 // cgUpdate (:2852-2880), and whether the next
 // iteration runs: max |r| under the tolerance (the reference's converged(),
@@ -3208,6 +3522,169 @@ template <int S> __global__ void __launch_bounds__(64) k_cgFinish(WorldGrid g) {
     }
 }
 
+
+// ============================================================ the bodies' passes
+// bodyStart (:3065-3070): the step's push sums emptied
+__global__ void __launch_bounds__(64) k_bodyStart(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on) return;
+    for (uint32_t k = bid * 64u + threadIdx.x; k < MAX_BODIES * BODY_PUSH_WORDS; k += nbid * 64u) d.pushes[k] = 0u;
+}
+// seepLoads (:3132-3143): the liquid's pressure on free body b's cell c where
+// it rests against what isn't liquid or air
+__device__ void seepLoads(const Dev &d, i2 c, uint32_t b) {
+    for (int k = 0; k < 4; k++) {
+        const i2 q = beside(c, k);
+        if (!inWorld(d, q) || rigidAt(d, q) == b || gridZ(d, q) >= 0.0f) continue;
+        const i2 n = I2(q.x - c.x, q.y - c.y);
+        const v2 x = V2((float)c.x + 0.5f + 0.5f * (float)n.x, (float)c.y + 0.5f + 0.5f * (float)n.y);
+        const v3 j = bodyFaceJ(d, q, c, b);
+        const float s = seepAt(d, c, n, b, false, x);
+        addPush(d, b, PUSH_SURFACE, V3(j.x * s, j.y * s, j.z * s));
+    }
+}
+// bodyLoads (:3097-3121): the liquid's pressure where each free body's pose
+// puts its surface, past what the cell's middle gives
+__global__ void __launch_bounds__(64) k_bodyLoads(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        const uint32_t own = rigidAt(d, c);
+        if (own != WNONE) {
+            seepLoads(d, c, own);
+            continue;
+        }
+        const float4 g4 = gridAt4(d, c);
+        if (g4.z <= 0.0f) continue;
+        const v2 mid = V2((float)c.x + 0.5f, (float)c.y + 0.5f);
+        for (int k = 0; k < 4; k++) {
+            const i2 q = beside(c, k);
+            const uint32_t b = rigidAt(d, q);
+            if (b == WNONE) continue;
+            const i2 n = I2(q.x - c.x, q.y - c.y);
+            const float reach = 1.0f - bodyReach(d, b, I2(-n.x, -n.y));
+            const v2 surface = V2(mid.x + (float)n.x * reach, mid.y + (float)n.y * reach);
+            const v3 j = bodyFaceJ(d, c, q, b);
+            const float dp = potentialAt(d, mid) - potentialAt(d, surface);
+            addPush(d, b, PUSH_SURFACE, V3(j.x * g4.z * dp, j.y * g4.z * dp, j.z * g4.z * dp));
+        }
+    }
+}
+// bodyDrag (:3150-3173): the liquid's drag on each free body as the
+// viscosity's sweeps left its faces
+__global__ void __launch_bounds__(64) k_bodyDrag(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on || !d.visc_on) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
+        const uint32_t b = rigidAt(d, c);
+        if (!inWorld(d, c) || b == WNONE) continue;
+        const v2 pos = bodyV(d, b, BW_POS);
+        const float rho = bodyDensity(d, b);
+        v3 push = V3(0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 4; k++) {
+            const bool isU = k < 2;
+            const int s = (k & 1) == 1 ? 1 : -1;
+            const i2 n = isU ? I2(s, 0) : I2(0, s);
+            if (gridZ(d, I2(c.x + n.x, c.y + n.y)) <= 0.0f) continue;
+            const i2 at = isU ? I2(c.x + (k & 1), c.y) : I2(c.x, c.y + (k & 1));
+            const v2 x = faceMiddle(at, isU);
+            const v2 mv = bodyMoving(d, b, x, false);
+            const float du = bodyReach(d, b, n) * rho * (FLD(d, faceSlot(d, faceIndex(d, at, isU)), 0) - (isU ? mv.x : mv.y));
+            const v2 r = V2(x.x - pos.x, x.y - pos.y);
+            if (isU) push.x += du;
+            else push.y += du;
+            push.z += isU ? -r.y * du : r.x * du;
+        }
+        if (push.x != 0.0f || push.y != 0.0f || push.z != 0.0f) addPush(d, b, PUSH_DRAG, push);
+    }
+}
+// bodyFaces (:3180-3201): the bodies' faces, as the sweeps left them, moving
+// with the bodies again, the drag and all
+__global__ void __launch_bounds__(64) k_bodyFaces(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on || !d.visc_on) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        Face3 fs[4];
+        facesOf(d, c, fs);
+        for (int j = 0; j < 4; j++) {
+            if (fs[j].isU < 0) continue;
+            const i2 at = I2(fs[j].x, fs[j].y);
+            const bool isU = fs[j].isU == 1;
+            const i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
+            const uint32_t ra = rigidAt(d, a), rb = rigidAt(d, at);
+            if (ra != WNONE || rb != WNONE) setBodyFace(d, faceIndex(d, at, isU), at, isU, gridAt4(d, a), gridAt4(d, at), ra, rb, true);
+        }
+    }
+}
+// cgBodyX, cgBodyP, cgBodyF (:2819-2845): the pressure's push on each free
+// body: of the pressure the solve starts from, of an iteration's direction
+// (into the iteration's set by its parity: cgDirection's p, newP, worked
+// out again here as cgApply works it out), of the pressure it ends with
+__global__ void __launch_bounds__(64) k_cgBodyX(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
+        if (inWorld(d, c) && gridZ(d, c) > 0.0f) addPressurePush(d, c, gridX(d, c), PUSH_X);
+    }
+}
+template <int S> __global__ void __launch_bounds__(64) k_cgBodyP(WorldGrid g) {
+    WORLD(g);
+    if (d.store != S || !d.body_on) return;
+    const Iter it = iterWords(d);
+    const double beta = ld_f64(&d.head->beta);
+    if (!it.encoded || it.converged) return;
+    const float *pold = pBuf(d, it.cycle);
+    for (uint32_t wid = bid; wid < it.active; wid += nbid) {
+        const i2 c = blockCell(activeBlockAt(d, d.blocks[ACTIVE_SLOT(d, wid)]), threadIdx.x);
+        if (!inWorld(d, c)) continue;
+        const uint32_t pi = packIndex(d, c);
+        if ((d.cflag[pi] & CF_BODY) == 0u) continue;
+        addPressurePush(d, c, newP<S>(d, pi, beta, pold), PUSH_P0 + it.cycle % 2u);
+    }
+}
+__global__ void __launch_bounds__(64) k_cgBodyF(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
+        if (inWorld(d, c) && gridZ(d, c) > 0.0f) addPressurePush(d, c, gridX(d, c), PUSH_F);
+    }
+}
+// rigidify (:3037-3063): what the liquid's pressure did to each held body,
+// and whether it is wet, into its liquid slot's words
+__global__ void __launch_bounds__(64) k_rigidify(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on || !d.bodies_run) return;
+    FOR_ACTIVE_BLOCKS(d, wid) {
+        const i2 c = blockCell(activeBlockAt(d, blk_), threadIdx.x);
+        const uint32_t h = heldAt(d, c);
+        if (h == WNONE) continue;
+        int32_t dd[3];
+        heldPush(d, c, h, dd);
+        if ((d.blocks[STATE_SLOT(d, blockIndex(d, idiv8(c)))] & NEAR_LIQUID) != 0u) atomicOr(&d.motion[liquidSlot(h, 3u)], LIQUID_WET);
+        for (uint32_t k = 0; k < 3u; k++) atomicAdd(&d.motion[liquidSlot(h, k)], (uint32_t)dd[k]);
+    }
+}
+// bodyPush (:3206-3218): what the liquid did to each free body this step
+__global__ void __launch_bounds__(1024) k_bodyPush(WorldGrid g) {
+    WORLD(g);
+    if (!d.full || !d.body_on || !d.bodies_run) return;
+    for (uint32_t b = threadIdx.x; b < MAX_BODIES; b += blockDim.x) {
+        const v3 s = pushChangeFree(d, b, PUSH_SURFACE), dr = pushChangeFree(d, b, PUSH_DRAG), f = pushChangeFree(d, b, PUSH_F);
+        const v3 ch = V3(s.x + dr.x + f.x, s.y + dr.y + f.y, s.z + dr.z + f.z);
+        if (ch.x == 0.0f && ch.y == 0.0f && ch.z == 0.0f) continue;
+        d.motion[liquidSlot(b, 0u)] += (uint32_t)wi32(wround(ch.x * RIGID_FIX));
+        d.motion[liquidSlot(b, 1u)] += (uint32_t)wi32(wround(ch.y * RIGID_FIX));
+        d.motion[liquidSlot(b, 2u)] += (uint32_t)wi32(wround(ch.z * RIGID_FIX));
+    }
+}
+
 // ============================================================ after the solve
 // project (:2895-2942)
 __device__ void projectFace(const Dev &d, i2 at, bool isU) {
@@ -3215,10 +3692,22 @@ __device__ void projectFace(const Dev &d, i2 at, bool isU) {
     i2 a = isU ? I2(at.x - 1, at.y) : I2(at.x, at.y - 1);
     float4 ga = gridAt4(d, a), gb = gridAt4(d, at);
     float beta = fineBeta(ga.z, gb.z, densityAt(d, a), densityAt(d, at));
+    const uint32_t ra = rigidAt(d, a), rb = rigidAt(d, at);
     if (beta > 0.0f) {
         FLD(d, f, 0) = FLD(d, f, 0) - beta * (gb.x - ga.x);
         FLD(d, f, 2) = PROJECTED;
-        FLD(d, f, 3) = wclamp(-beta * (driftAt(d, at) - driftAt(d, a)), -MAX_DRIFT, MAX_DRIFT);
+        // no drift into or out of a body's cells (:2907-2909)
+        FLD(d, f, 3) = ra != WNONE || rb != WNONE ? 0.0f : wclamp(-beta * (driftAt(d, at) - driftAt(d, a)), -MAX_DRIFT, MAX_DRIFT);
+    }
+    // a face between the liquid and a free body moves as the body does with
+    // the pressure's push the solve found (:2911-2921)
+    if ((ra == WNONE) != (rb == WNONE) && (ra != WNONE ? gb.z : ga.z) > 0.0f) {
+        const uint32_t b = ra != WNONE ? ra : rb;
+        const v3 ch = pushChange(d, b, PUSH_F);
+        const v2 m = faceMiddle(at, isU), pos = bodyV(d, b, BW_POS);
+        const v2 r = V2(m.x - pos.x, m.y - pos.y);
+        FLD(d, f, 0) += isU ? ch.x - ch.z * r.y : ch.y + ch.z * r.x;
+        FLD(d, f, 2) = PROJECTED;
     }
 }
 __global__ void __launch_bounds__(64) k_project(WorldGrid g) {
@@ -3312,16 +3801,21 @@ __device__ __forceinline__ v2 clampSpeed(v2 v) {
     float s = wlength(v);
     return s > LIQUID_MAX_SPEED ? V2(v.x * (LIQUID_MAX_SPEED / s), v.y * (LIQUID_MAX_SPEED / s)) : v;
 }
-__device__ __forceinline__ bool escapesTo(const Dev &d, i2 q) { return inWorld(d, q) && !(gridZ(d, q) < 0.0f); }
+__device__ __forceinline__ bool escapesTo(const Dev &d, i2 q, uint32_t side) {   // :3535-3541
+    if (!inWorld(d, q) || shutAt(d, q)) return false;
+    const uint32_t cup = cupOf(d, cellIndex(d, q));
+    return cup == WNONE || cup == side;
+}
 __device__ v2 escape(const Dev &d, v2 pos) {                                 // :3552-3594
     if (!isObstaclePos(d, pos)) return pos;
     i2 c = I2(wi32(floorf(pos.x)), wi32(floorf(pos.y)));
+    const uint32_t side = inWorld(d, c) ? cupBefore(d, cellIndex(d, c)) : WNONE;
     v2 best = pos;
     float bestD = 1e9f;
     for (int n = 0; n < 8; n++) {
         int m = n + (n >= 4 ? 1 : 0);
         i2 at = I2(c.x + m % 3 - 1, c.y + m / 3 - 1);
-        if (!escapesTo(d, at)) continue;
+        if (!escapesTo(d, at, side)) continue;
         v2 inside = V2(wclamp(pos.x, (float)at.x + 0.02f, (float)at.x + 0.98f), wclamp(pos.y, (float)at.y + 0.02f, (float)at.y + 0.98f));
         float dd = wlength(V2(pos.x - inside.x, pos.y - inside.y));
         if (dd < bestD) { bestD = dd; best = inside; }
@@ -3335,7 +3829,7 @@ __device__ v2 escape(const Dev &d, v2 pos) {                                 // 
         i2 dir = turnStep(ways[n], kt);
         for (int r = 2; r < nearest; r++) {
             i2 q = I2(c.x + dir.x * r, c.y + dir.y * r);
-            if (!isObstaclePos(d, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) && escapesTo(d, q)) {
+            if (!isObstaclePos(d, V2((float)q.x + 0.5f, (float)q.y + 0.5f)) && escapesTo(d, q, side)) {
                 nearest = r;
                 way = dir;
                 break;
@@ -3765,7 +4259,8 @@ __global__ void __launch_bounds__(64) k_reports(WorldGrid g, uint32_t *out) {
     template __global__ void k_cgApply<S>(WorldGrid); \
     template __global__ void k_cgUpdate<S>(WorldGrid); \
     template __global__ void k_cgTrue<S>(WorldGrid); \
-    template __global__ void k_cgFinish<S>(WorldGrid);
+    template __global__ void k_cgFinish<S>(WorldGrid); \
+    template __global__ void k_cgBodyP<S>(WorldGrid);
 STORE_KERNELS(0)
 STORE_KERNELS(1)
 STORE_KERNELS(2)

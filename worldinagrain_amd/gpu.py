@@ -140,10 +140,10 @@ class GpuWorld:
         self.kinds = k
 
     def set_element_props(self, props):
-        """The elements' liquid columns (wig_set_element_props): (256, 5) f32,
-        viscosity, viscosity_e, viscosity_t, tension, wetting."""
-        p = np.ascontiguousarray(props, dtype=np.float32)
-        assert p.shape == (256, 5)
+        """The elements' columns (wig_set_element_props): (256, 6) f32,
+        viscosity, viscosity_e, viscosity_t, tension, wetting, breaks (fewer
+        columns padded with 0)."""
+        p = np.ascontiguousarray(abi.element_props(props))
         _check(lib().wig_set_element_props(self._w, p.ctypes.data))
 
     def part_bytes(self, p):

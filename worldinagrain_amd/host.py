@@ -119,6 +119,14 @@ class Simulation:
         # the gravity pass's field (WigStepCtl.gravity): the world's gravity
         # part read for the pull and down; off, Earth's pull alone
         self.gravity = False
+        # free and held bodies (WigStepCtl.bodies): the world's bodyMotion and
+        # bodySwept parts read, bodyPushes its scratch; off, no body anywhere.
+        # bodies_run is the reference's bodiesRun() (something loose as of the
+        # last report, or that report too old: js/sim.js:2375-2383), which the
+        # caller's body pass knows: it makes the step full and hands what the
+        # liquid did to the bodies (rigidify, bodyPush)
+        self.bodies = False
+        self.bodies_run = False
 
     # Simulation.seed / random (js/sim.js:1386-1397)
     def seed(self, n):
@@ -143,7 +151,10 @@ class Simulation:
         s = self.solver
         self.shifted_first = not self.shifted_first       # js/sim.js:2530
         c = abi.WigStepCtl()
-        c.full = 1 if self.fluid_stats["active"] else 0
+        run = bool(self.bodies and self.bodies_run)
+        c.full = 1 if self.fluid_stats["active"] or run else 0   # js/sim.js:2479
+        c.bodies_run = 1 if run else 0
+        c.bodies = 1 if self.bodies else 0
         # the surface pass, and viscosity's sweeps with it (js/sim.js:2496-2509)
         c.surface = 1 if surface_tension(self.dx) else 0
         c.viscous = 1 if self.fluid_stats["viscous"] or c.surface else 0
@@ -249,6 +260,8 @@ class BatchHost:
             c.surface_drag = surface_drag_coefficient(s.dx)
             c.occupants = 1 if s.occupants else 0
             c.gravity = 1 if s.gravity else 0
+            c.bodies = 1 if s.bodies else 0
+            c.bodies_run = 1 if s.bodies and s.bodies_run else 0
             if sv.first_chunk < 0:
                 c.first_chunk, c.chunk = 0xFFFFFFFF, sv.chunk
             else:
@@ -274,7 +287,8 @@ class BatchHost:
         pw, cw = self.pw, self.cw
         pw[:, self._frame] = np.array([s.frame_no & 0xFFFFFFFF for s in sims], dtype=np.uint32)
         pw[:, self._seed] = seeds
-        cw[:, self._full] = np.array([1 if s.fluid_stats["active"] else 0 for s in sims], dtype=np.uint32)
+        cw[:, self._full] = np.array([1 if s.fluid_stats["active"] or (s.bodies and s.bodies_run) else 0 for s in sims],
+                                     dtype=np.uint32)
         cw[:, self._visc] = np.array([1 if s.fluid_stats["viscous"] else 0 for s in sims], dtype=np.uint32) | self._surf
         cyc = np.array([s.fluid_cycles for s in sims], dtype=np.uint32)
         cw[:, self._cycles] = np.where(self.adaptive, cyc, self.cap)

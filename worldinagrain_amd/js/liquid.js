@@ -17,7 +17,7 @@ const CAPILLARY_LENGTH = 0.0027;       // js/shared.js:285-286
 const SURFACE_MIN_LENGTH = 0.3;
 
 const PART = { cells: 0, particles0: 1, particles1: 2, counters: 3, field: 4, blocks: 5, fluidCg: 6, solve: 7, air0: 8,
-    airSource: 9, occupants: 10, gravity: 11 };
+    airSource: 9, occupants: 10, gravity: 11, bodyMotion: 12, bodySwept: 13, bodyPushes: 14 };
 const STAT = { ACTIVE: 0, RETIRED: 1, VISCOUS: 2, PARTICLES: 3, RESIDUAL: 4, CYCLES: 7, MOST_CYCLES: 8 };
 const TOL_ABS_MAX = 0;
 const TOL_REL_L2 = 1;
@@ -60,6 +60,8 @@ class LiquidSimulation {
         this.air = !!options.air;      // the air coupled in (air0 read, airSource added to)
         this.occupants = !!options.occupants;   // people's bodies (the occupants part) as moving obstacles
         this.gravity = !!options.gravity;       // the gravity pass's field (the gravity part): pull and down
+        this.bodies = !!options.bodies;         // free and held bodies (bodyMotion, bodySwept; bodyPushes scratch)
+        this.bodiesRun = false;                 // bodiesRun(): the caller's body pass sets it each step
         const s = options.solver || {};
         this.solver = {
             tolMode: s.tolMode === undefined ? TOL_ABS_MAX : s.tolMode,
@@ -124,10 +126,14 @@ class LiquidSimulation {
     // relaxation's starting tiling (flipped each step), the solve's settings.
     stepControl() {
         this.shiftedFirst = !this.shiftedFirst;
-        const buf = new ArrayBuffer(84);
+        const buf = new ArrayBuffer(88);
         const u = new Uint32Array(buf);
         const f = new Float32Array(buf);
-        u[0] = this.fluidStats.active ? 1 : 0;
+        // bodiesRun() makes the step full and hands the bodies what the
+        // liquid did to them (js/sim.js:2479, 2596-2599)
+        const run = this.bodies && this.bodiesRun;
+        u[0] = this.fluidStats.active || run ? 1 : 0;
+        u[3] = run ? 1 : 0;
         // the surface pass, and viscosity's sweeps with it (js/sim.js:2496-2509)
         u[2] = this.surfaceTension() ? 1 : 0;
         u[1] = this.fluidStats.viscous || u[2] ? 1 : 0;
@@ -146,6 +152,7 @@ class LiquidSimulation {
         f[18] = this.air ? surfaceDragCoefficient(this.metresPerCell) : 0;
         u[19] = this.occupants ? 1 : 0;
         u[20] = this.gravity ? 1 : 0;
+        u[21] = this.bodies ? 1 : 0;
         return buf;
     }
 

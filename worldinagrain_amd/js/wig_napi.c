@@ -158,7 +158,7 @@ static napi_value js_copy(napi_env env, napi_callback_info info, int up) {
 static napi_value js_upload(napi_env env, napi_callback_info info) { return js_copy(env, info, 1); }
 static napi_value js_download(napi_env env, napi_callback_info info) { return js_copy(env, info, 0); }
 
-/* fluidStep(h, params: ArrayBuffer(80), ctl: ArrayBuffer(84)) -> status */
+/* fluidStep(h, params: ArrayBuffer(80), ctl: ArrayBuffer(88)) -> status */
 static napi_value js_fluid_step(napi_env env, napi_callback_info info) {
     size_t argc = 3;
     napi_value argv[3], r;
@@ -168,7 +168,7 @@ static napi_value js_fluid_step(napi_env env, napi_callback_info info) {
     size_t pl, cl;
     if (!w) return NULL;
     if (!bytes_of(env, argv[1], &p, &pl) || pl != sizeof(WigParams) || !bytes_of(env, argv[2], &c, &cl) || cl != sizeof(WigStepCtl)) {
-        napi_throw_type_error(env, NULL, "fluidStep(h, params (80 bytes), ctl (64 bytes))");
+        napi_throw_type_error(env, NULL, "fluidStep(h, params (80 bytes), ctl (88 bytes))");
         return NULL;
     }
     WigParams P;

@@ -289,3 +289,112 @@ def air_over(width, height, words, seed=1, wind=0.05, temp=20.0, lapse=0.002):
     air[kind == E.KIND_LIQUID, 3] = abi.AIR_BLOCKED
     air[(kind == E.KIND_SOLID) | (kind == E.KIND_POWDER), 3] = abi.AIR_SOLID
     return air
+
+
+class Body:
+    """A body as the bodies' pass leaves it in its table (js/shared.js:316-326's
+    BODY_WORD words): its stamped cells (a y-down boolean mask), its pose's
+    centre `pos` (cells), velocity, spin, mass and moment of inertia (cells of
+    water; None: `density` times its cells, and a rectangle's about its
+    centre), whether it is held, its pose's subcell offset (SUBCELL), and a
+    mobility (six words of a symmetric 3 x 3) taken when `mobility_at` is
+    the step before the one run (MOBILITY_AT)."""
+
+    def __init__(self, slot, mask, density=1.0, pos=None, vel=(0.0, 0.0), spin=0.0, mass=None, inertia=None, held=False,
+                 wet=False, subcell=(0.0, 0.0), mobility=None, mobility_at=0xFFFFFFFF, cargo=0.0, cargo_inertia=0.0,
+                 alive=True):
+        self.slot, self.mask = slot, np.asarray(mask, dtype=bool)
+        ys, xs = np.nonzero(self.mask)
+        n = len(xs)
+        self.cells = n
+        if pos is None:
+            pos = (float(xs.mean()) + 0.5, float(ys.mean()) + 0.5)
+        self.pos = pos
+        self.vel, self.spin = vel, spin
+        self.mass = density * n if mass is None else mass
+        if inertia is None:
+            inertia = float(self.mass / max(n, 1) * (((xs + 0.5 - pos[0]) ** 2 + (ys + 0.5 - pos[1]) ** 2).sum() + n / 6.0))
+        self.inertia = inertia
+        self.held, self.wet, self.alive = held, wet, alive
+        self.subcell = subcell
+        self.mobility, self.mobility_at = mobility, mobility_at
+        self.cargo, self.cargo_inertia = cargo, cargo_inertia
+
+
+def _f16pair(a, b):
+    return int(np.float16(a).view(np.uint16)) | (int(np.float16(b).view(np.uint16)) << 16)
+
+
+def body_parts(width, height, words, bodies, elem=E.STONE, cups=(), seams=(), frame=1):
+    """The parts of a world with free and held bodies: each body's cells
+    painted `elem` marked loose (FL_LOOSE) with its slot in frac.x (common.wgsl:
+    156-162), its table slot in the motion part, and the swept words of the
+    step before `frame` for the cells of `cups` and `seams` ((body slot, mask)
+    pairs; common.wgsl:355-370).  Returns (motion, swept) as u32 arrays."""
+    from . import abi
+    C = width * height
+    cells = words[:4 * C].reshape(height, width, 4)
+    motion = np.zeros(abi.MOTION_WORDS, dtype=np.uint32)
+    mf = motion.view(np.float32)
+    for b in bodies:
+        paint(words, width, height, b.mask, elem)
+        cells[b.mask, 0] |= np.uint32(abi.FL_LOOSE << 28)
+        cells[b.mask, 3] = _f16pair(float(b.slot), 0.0)
+        base = abi.body_word(b.slot, 0)
+        mf[base + abi.BW_POS], mf[base + abi.BW_POS + 1] = b.pos
+        mf[base + abi.BW_VEL], mf[base + abi.BW_VEL + 1] = b.vel
+        mf[base + abi.BW_SPIN] = b.spin
+        mf[base + abi.BW_MASS] = b.mass
+        mf[base + abi.BW_INERTIA] = b.inertia
+        motion[base + abi.BW_CELLS] = b.cells
+        motion[base + abi.BW_FLAGS] = ((abi.BF_ALIVE if b.alive else 0) | (abi.BF_HELD if b.held else 0)
+                                       | (abi.BF_WET if b.wet else 0))
+        motion[base + abi.BW_SUBCELL] = _f16pair(*b.subcell)
+        mf[base + abi.BW_CARGO] = b.cargo
+        mf[base + abi.BW_CARGO_INERTIA] = b.cargo_inertia
+        if b.mobility is not None:
+            mf[base + abi.BW_MOBILITY:base + abi.BW_MOBILITY + 6] = b.mobility
+        motion[base + abi.BW_MOBILITY_AT] = b.mobility_at & 0xFFFFFFFF
+    swept = np.zeros(2 * C, dtype=np.uint32)
+    back = (frame - 1) & 0xFFFFFFFF
+    for seam, marks in ((False, cups), (True, seams)):
+        for slot, mask in marks:
+            idx = np.flatnonzero(np.asarray(mask, dtype=bool).ravel())
+            swept[2 * idx + (back & 1)] = abi.swept_key(back, slot, seam)
+    return motion, swept
+
+
+def bodies_tank(width=64, height=48, depth=28, wall=1, frame=1):
+    """A closed tank of water `depth` rows deep with bodies in it: a light
+    floating plank, a heavy box sinking and turning (its mobility given for
+    the first step, its pose a quarter cell off its stamp), a held block on
+    the floor, wet, and a cup (a U of a free body) with powder in it and a
+    seam in its wall.  Returns (cells, motion, swept, kinds, props)."""
+    w = still_tank(width, height, depth, wall)
+    surface = height - wall - depth
+    def rect(x0, y0, x1, y1):
+        m = np.zeros((height, width), dtype=bool)
+        m[y0:y1, x0:x1] = True
+        return m
+    plank = rect(6, surface - 1, 18, surface + 2)
+    box = rect(24, surface + 8, 29, surface + 13)
+    block = rect(36, height - wall - 4, 44, height - wall)
+    cup = rect(48, surface + 4, 58, surface + 12) & ~rect(50, surface + 4, 56, surface + 10)
+    inside = rect(50, surface + 4, 56, surface + 10)
+    seam = rect(49, surface + 6, 50, surface + 7)
+    cup_wall = cup & ~seam
+    bodies = [
+        Body(3, plank, density=0.5, vel=(0.01, 0.0)),
+        Body(7, box, density=2.0, vel=(0.0, 0.05), spin=0.01, subcell=(0.25, -0.125),
+             mobility=(0.02, 0.001, 0.0005, 0.03, 0.0002, 0.004), mobility_at=frame - 1),
+        Body(12, block, density=2.5, held=True, wet=True),
+        Body(20, cup_wall, density=0.8, cargo=4.0, cargo_inertia=10.0),
+    ]
+    motion, swept = body_parts(width, height, w, bodies, cups=[(20, inside)], seams=[(20, seam)], frame=frame)
+    # powder settled in the cup's floor: its cargo (classify, :1616-1619)
+    sand = rect(50, surface + 9, 56, surface + 10)
+    paint(w, width, height, sand, E.SAND)
+    kinds = E.kinds()
+    props = np.zeros((256, 6), dtype=np.float32)
+    props[E.STONE, 5] = 1.0
+    return w, motion, swept, kinds, props
