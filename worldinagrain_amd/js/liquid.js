@@ -51,6 +51,9 @@ class LiquidSimulation {
         this.height = height;
         this.world = addon.create(width, height, options.coarseLevels || 0);
         addon.setKinds(this.world, options.kinds || elementKinds());
+        // the elements' columns (viscosity, its temperature law, tension,
+        // wetting, breaks): 256 rows of 6 floats, zeros unless given
+        if (options.elementProps) addon.setElementProps(this.world, options.elementProps);
         this.layout = addon.layout(this.world);
         this.rngState = (options.seed === undefined ? 0x5eed : options.seed) >>> 0;
         this.metresPerCell = options.dx === undefined ? 0.01 : options.dx;

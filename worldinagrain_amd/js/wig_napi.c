@@ -141,6 +141,24 @@ static napi_value js_set_kinds(napi_env env, napi_callback_info info) {
     return st < 0 ? throw_wig(env, st) : NULL;
 }
 
+/* setElementProps(h, Float32Array(256 * 6)): viscosity, viscosityE,
+ * viscosityT, tension, wetting, breaks a row (WigElementProps) */
+static napi_value js_set_element_props(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    WigWorld *w = world_of(env, argv[0]);
+    void *data;
+    size_t len;
+    if (!w) return NULL;
+    if (!bytes_of(env, argv[1], &data, &len) || len != 256 * sizeof(WigElementProps)) {
+        napi_throw_type_error(env, NULL, "element props: 256 rows of 6 floats");
+        return NULL;
+    }
+    int st = wig_set_element_props(w, (const WigElementProps *)data);
+    return st < 0 ? throw_wig(env, st) : NULL;
+}
+
 /* upload(h, part, bytes) / download(h, part, bytes) */
 static napi_value js_copy(napi_env env, napi_callback_info info, int up) {
     size_t argc = 3;
@@ -394,6 +412,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"destroy", NULL, js_destroy, NULL, NULL, NULL, napi_default, NULL},
         {"layout", NULL, js_layout, NULL, NULL, NULL, napi_default, NULL},
         {"setKinds", NULL, js_set_kinds, NULL, NULL, NULL, napi_default, NULL},
+        {"setElementProps", NULL, js_set_element_props, NULL, NULL, NULL, napi_default, NULL},
         {"upload", NULL, js_upload, NULL, NULL, NULL, napi_default, NULL},
         {"download", NULL, js_download, NULL, NULL, NULL, napi_default, NULL},
         {"fluidStep", NULL, js_fluid_step, NULL, NULL, NULL, napi_default, NULL},
