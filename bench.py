@@ -163,6 +163,10 @@ def main():
                          "thread at 1024x1024; dam: 200, the configuration's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default coarse-level tile sides")
+    ap.add_argument("--settings", default="",
+                    help="the autotuner's choices as a printed line gives them, in place of tuning: JSON with "
+                         "'tiles', 'sweeps', 'lds_level' and 'coarse_wave_cells' (a profile's run takes the "
+                         "bench line's own)")
     ap.add_argument("--graphs", type=int, default=1, help="1: replay the step's phases as hipGraphs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (the HBM counters' calibration)")
@@ -221,7 +225,24 @@ def main():
     # the coarse-level tile sides, by the autotuner on the world's own steps
     # (they do not change results)
     tiles, sweeps, lds_level, coarse_wave = {}, None, None, None
-    if not args.no_tune:
+    if args.settings:
+        import dataclasses
+        pin = json.loads(args.settings)
+        sweeps = pin.get("sweeps")
+        if sweeps:
+            sim.solver = dataclasses.replace(sim.solver, sweep_pairs=sweeps[0], level_pairs=sweeps[1],
+                                             coarse_sweeps=sweeps[2])
+        lds_level, coarse_wave = pin.get("lds_level"), pin.get("coarse_wave_cells")
+        if lds_level is not None:
+            world.set_lds_level(int(lds_level))
+        if coarse_wave is not None:
+            world.set_coarse_wave(int(coarse_wave))
+        tiles = {int(k): int(v) for k, v in pin.get("tiles", {}).items()}
+        for level, side in tiles.items():
+            world.set_tile(level, side)
+        for _ in range(2):
+            sim.step()
+    elif not args.no_tune:
         from worldinagrain_amd.tune import tune_tiles, tune_sweeps, tune_lds_level
         lds_level, coarse_wave = tune_lds_level(world, sim)
         tiles = tune_tiles(world, sim)
