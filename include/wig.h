@@ -242,7 +242,11 @@ typedef struct WigLayoutInfo {
 int wig_layout(const WigWorld *w, WigLayoutInfo *out);
 
 /* Copies a part to or from the device (js/sim.js:1429-1493's snapshot
- * parts; queue.writeBuffer / mapAsync).  Synchronous. */
+ * parts; queue.writeBuffer / mapAsync).  Synchronous.  The parts of the
+ * step's optional inputs (air, airSource, occupants, gravity and the
+ * bodies') take device memory only once uploaded, asked for by
+ * wig_device_ptr, or stepped with their WigStepCtl switch on; a download
+ * before that reads zeros. */
 int wig_upload(WigWorld *w, int part, const void *src, size_t bytes);
 int wig_download(WigWorld *w, int part, void *dst, size_t bytes);
 

@@ -62,15 +62,16 @@ class _Admitted:
     def __init__(self, sid, log):
         self.sid, self.world, self.evicting, self.admitting, self.log = sid, None, None, None, log
 
-    def make_resident(self, stream):
+    def make_resident(self, stream=None):
         self.log.append(("in", self.sid, stream))
         self.world = _World()
-        self.world.set_stream(stream)
+        if stream is not None:
+            self.world.set_stream(stream)
 
 
 def test_admission_waits_for_its_eviction():
     log = []
-    ad = serve.Admitter("copy")
+    ad = serve.Admitter()
     s = _Admitted(1, log)
     s.evicting = threading.Event()          # an eviction of the same session still in flight
     ad.submit(s)
@@ -79,7 +80,8 @@ def test_admission_waits_for_its_eviction():
     log.append(("out", 1, None))
     s.evicting.set()
     s.admitting.wait(5)
-    assert s.admitting.is_set() and s.world.stream == "copy"
+    # (made on its own stream: no stream set, none shared with other copies)
+    assert s.admitting.is_set() and s.world.stream is None
     assert [e[0] for e in log] == ["out", "in"]
     ad.wait_any(0.0)
     assert ad.inflight == 0

@@ -322,6 +322,15 @@ static int zero(hipStream_t s, void *p, uint64_t bytes) {
     HIPCHECK(hipGetLastError());
     return WIG_OK;
 }
+// The parts of the step's optional inputs (the air's, the people's, the
+// gravity pass's, the bodies') are allocated when a caller first uploads one
+// or steps with its switch on: most worlds use none, and airSource alone is
+// 64 B a cell, which every world a server admits would otherwise allocate
+// and clear.  Until then a download reads zeros (what a fresh part holds).
+static bool lazyPart(int p) {
+    return p == WIG_PART_AIR || p == WIG_PART_AIR_SOURCE || p == WIG_PART_OCCUPANTS || p == WIG_PART_GRAVITY
+           || p == WIG_PART_MOTION || p == WIG_PART_SWEPT || p == WIG_PART_PUSHES;
+}
 
 static void batch_free(Batch *b) {
     if (!b) return;
@@ -426,6 +435,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         return nullptr;
     }
     for (int p = 0; p < WIG_PART_COUNT; p++) {
+        if (lazyPart(p)) continue;   // (allocated when first uploaded or stepped with: ensure_part)
         if (hipMalloc(&w->part[p], w->bytes[p]) != hipSuccess) {
             fail(WIG_E_NOMEM, "hipMalloc of part %d (%llu bytes) failed", p, (unsigned long long)w->bytes[p]);
             wig_destroy(w);
@@ -442,7 +452,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         return nullptr;
     }
     for (int p = 0; p < WIG_PART_COUNT; p++)
-        if (zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
+        if (w->part[p] && zero(w->own, w->part[p], w->bytes[p]) != WIG_OK) { wig_destroy(w); return nullptr; }
     zero(w->own, w->kind, 256 + 256 * sizeof(WigElementProps));
     zero(w->own, w->fine, 77ull * 64ull * w->d.B + 256u);
     zero(w->own, w->vlevels, w->vlevel_bytes);
@@ -596,9 +606,30 @@ int wig_layout(const WigWorld *w, WigLayoutInfo *out) {
     return WIG_OK;
 }
 
+// (a lazy part's memory, zeroed, made now if it is not yet)
+static int ensure_part(WigWorld *w, int p) {
+    if (w->part[p]) return WIG_OK;
+    void *mem = nullptr;
+    if (hipMalloc(&mem, w->bytes[p]) != hipSuccess) return fail(WIG_E_NOMEM, "hipMalloc of part %d failed", p);
+    w->part[p] = mem;
+    int st = zero(stream_of(w), mem, w->bytes[p]);
+    if (st < 0) return st;
+    HIPCHECK(hipStreamSynchronize(stream_of(w)));   // (cleared before any stream, a batch's too, uses it)
+    if (p == WIG_PART_AIR) w->d.air = (const float4 *)mem;
+    if (p == WIG_PART_AIR_SOURCE) w->d.source = (uint32_t *)mem;
+    if (p == WIG_PART_OCCUPANTS) w->d.occ = (const uint32_t *)mem;
+    if (p == WIG_PART_GRAVITY) w->d.grav = (const float4 *)mem;
+    if (p == WIG_PART_MOTION) w->d.motion = (uint32_t *)mem;
+    if (p == WIG_PART_SWEPT) w->d.swept = (const uint32_t *)mem;
+    if (p == WIG_PART_PUSHES) w->d.pushes = (uint32_t *)mem;
+    return WIG_OK;
+}
+
 int wig_upload(WigWorld *w, int part, const void *src, size_t bytes) {
     if (!w || !src || part < 0 || part >= WIG_PART_COUNT) return fail(WIG_E_INVALID, "bad part %d", part);
     if (bytes != w->bytes[part]) return fail(WIG_E_INVALID, "part %d is %llu bytes, not %zu", part, (unsigned long long)w->bytes[part], bytes);
+    int st = ensure_part(w, part);
+    if (st < 0) return st;
     HIPCHECK(hipMemcpyAsync(w->part[part], src, bytes, hipMemcpyHostToDevice, stream_of(w)));
     HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
@@ -607,6 +638,10 @@ int wig_upload(WigWorld *w, int part, const void *src, size_t bytes) {
 int wig_download(WigWorld *w, int part, void *dst, size_t bytes) {
     if (!w || !dst || part < 0 || part >= WIG_PART_COUNT) return fail(WIG_E_INVALID, "bad part %d", part);
     if (bytes != w->bytes[part]) return fail(WIG_E_INVALID, "part %d is %llu bytes, not %zu", part, (unsigned long long)w->bytes[part], bytes);
+    if (!w->part[part]) {            // (a lazy part not made yet: zeros)
+        memset(dst, 0, bytes);
+        return WIG_OK;
+    }
     HIPCHECK(hipMemcpyAsync(dst, w->part[part], bytes, hipMemcpyDeviceToHost, stream_of(w)));
     HIPCHECK(hipStreamSynchronize(stream_of(w)));
     return WIG_OK;
@@ -614,19 +649,35 @@ int wig_download(WigWorld *w, int part, void *dst, size_t bytes) {
 
 int wig_device_ptr(WigWorld *w, int part, void **ptr) {
     if (!w || !ptr || part < 0 || part >= WIG_PART_COUNT) return fail(WIG_E_INVALID, "bad part %d", part);
+    int st = ensure_part(w, part);
+    if (st < 0) return st;
     *ptr = w->part[part];
     return WIG_OK;
 }
 
 // Work enqueued on `to` from now on waits for what `from` holds so far (an
-// event recorded on `from`); nothing when they are the same stream.
+// event recorded on `from`); nothing when they are the same stream or
+// `from` has finished all its work (hipStreamQuery: the common case, a
+// serving loop's streams drained by its reports, where an event's record
+// and wait had cost its rounds up to 15 ms at p99).  The event is the
+// calling thread's own, made once (per device) and recorded again each
+// time: a wait takes the record made before it, so a later record does not
+// move it.
 int wig_order_streams(hipStream_t from, hipStream_t to) {
     if (from == to) return WIG_OK;
-    hipEvent_t ev = nullptr;
-    HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    const hipError_t idle = hipStreamQuery(from);
+    if (idle == hipSuccess) return WIG_OK;
+    if (idle == hipErrorNotReady) (void)hipGetLastError();   // (not an error: kept from the next call's check)
+    static thread_local hipEvent_t ev = nullptr;
+    static thread_local int ev_device = -1;
+    int device = 0;
+    HIPCHECK(hipGetDevice(&device));
+    if (!ev || ev_device != device) {
+        HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));   // (one a thread and device, kept)
+        ev_device = device;
+    }
     hipError_t e = hipEventRecord(ev, from);
     if (e == hipSuccess) e = hipStreamWaitEvent(to, ev, 0);
-    (void)hipEventDestroy(ev);   // (released once the wait is done)
     if (e != hipSuccess) return fail(WIG_E_DEVICE, "ordering the new stream after the old: %s", hipGetErrorString(e));
     return WIG_OK;
 }
@@ -809,6 +860,15 @@ int wig_solve_stats(WigWorld *w, WigSolveStats *out) {
 
 // The step's words of world w's Dev, from its Params and controls.
 static int step_words(WigWorld *w, const WigParams *P, const WigStepCtl *ctl, Dev &d) {
+    const bool need[WIG_PART_COUNT] = {false, false, false, false, false, false, false, false,
+                                        ctl->air != 0u, ctl->air != 0u, ctl->occupants != 0u, ctl->gravity != 0u,
+                                        ctl->bodies != 0u, ctl->bodies != 0u, ctl->bodies != 0u};
+    static_assert(WIG_PART_COUNT == 15, "the parts a step's switches need");
+    for (int p = 0; p < WIG_PART_COUNT; p++)
+        if (need[p]) {
+            int st = ensure_part(w, p);
+            if (st < 0) return st;
+        }
     d = w->d;
     if (P->size[0] != (uint32_t)d.W || P->size[1] != (uint32_t)d.H)
         return fail(WIG_E_INVALID, "params size %ux%u is not the world's %dx%d", P->size[0], P->size[1], d.W, d.H);

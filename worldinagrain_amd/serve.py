@@ -164,12 +164,11 @@ class Evictor:
 class Admitter:
     """Admissions off the serving loop: a thread of its own makes each
     session's world resident (created, and its cells or its snapshot
-    uploaded) on a copy stream of its own while the rounds run; the loop
+    uploaded) on the world's own stream while the rounds run; the loop
     takes the session into a round once it is ready.  (A session evicted and
     asked for again is admitted once its eviction is done.)"""
 
-    def __init__(self, stream):
-        self.stream = stream
+    def __init__(self):
         self.error = None
         self.inflight = 0
         self.cv = threading.Condition()
@@ -203,7 +202,7 @@ class Admitter:
             try:
                 if s.evicting is not None:
                     s.evicting.wait()
-                s.make_resident(self.stream)
+                s.make_resident()
             except BaseException as e:      # (raised on the loop's next submit or wait)
                 self.error = e
             s.admitting.set()
@@ -230,16 +229,19 @@ class Server:
     """Continuous batching over sessions under an HBM budget (bytes)."""
 
     def __init__(self, sessions, budget, max_round_cells=1 << 24, stream=None, copy_stream=None, evicting_limit=16 << 30):
-        """copy_stream: evictions and admissions run in threads of their own
-        on this stream (Evictor, Admitter), at most evicting_limit bytes of
-        worlds in eviction at once; None: each in the loop, in the round
-        that needs it."""
+        """copy_stream: evictions run in a thread of their own on this stream
+        (Evictor), at most evicting_limit bytes of worlds in eviction at
+        once, and admissions in another (Admitter), each world made and
+        loaded on its own stream; None: each in the loop, in the round that
+        needs it.  (A world joining a round orders the round's stream after
+        the work the world's stream holds: on a stream shared with other
+        copies, the round waited for those too.)"""
         self.sessions = {s.sid: s for s in sessions}
         self.budget = budget
         self.max_round_cells = max_round_cells
         self.stream = stream
         self.evictor = Evictor(copy_stream, evicting_limit) if copy_stream is not None else None
-        self.admitter = Admitter(copy_stream) if copy_stream is not None else None
+        self.admitter = Admitter() if copy_stream is not None else None
         self.resident_bytes = 0
         self.batch = None
         self.round_no = 0
@@ -388,3 +390,4 @@ class Server:
             if s.world is not None:
                 s.world.close()
                 s.world = None
+
