@@ -162,6 +162,8 @@ def main():
                     help="oracle steps in each of 3 timed runs on the host (single: 4, about 1.4 s a step on one "
                          "thread at 1024x1024; dam: 200, the configuration's)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="single: leave out the batched and served forms' short runs beside the line (configs 4 and 5)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default coarse-level tile sides")
     ap.add_argument("--settings", default="",
                     help="the autotuner's choices as a printed line gives them, in place of tuning: JSON with "
@@ -193,9 +195,11 @@ def main():
     from worldinagrain_amd.gpu import GpuWorld, kernel_names
 
     if args.workload == "batch":
-        return run_batch(args)
+        print(json.dumps(run_batch(args)), flush=True)
+        return
     if args.workload == "serve":
-        return run_serve(args)
+        print(json.dumps(run_serve(args)), flush=True)
+        return
     if args.workload == "precision":
         return run_precision(args)
     if args.workload == "snapshot":
@@ -348,7 +352,42 @@ def main():
         # (the dam's one-thread figure is one run of its 200 steps, about 20 s)
         out["cpu_baseline"] = cpu_baseline(cells, kinds, size, args.tol, args.cpu_steps, solver=sim.solver,
                                            one_thread_runs=1 if args.workload == "dam" else 3)
+    if args.workload == "single" and not args.no_forms:
+        del sim
+        world.close()
+        torch.cuda.synchronize()
+        out["forms"] = run_forms(args)
     print(json.dumps(out), flush=True)
+
+
+def run_forms(args):
+    """The batched and the served forms (configs 4 and 5: the 512-world
+    batch, and the serving loop over 2000 sessions) run after the
+    single-world line in the same process, each with its workload's
+    defaults, and summed up beside it; a form that fails is reported, not
+    raised."""
+    import copy
+    forms = {}
+    for wl in ("batch", "serve"):
+        a = copy.copy(args)
+        a.workload, a.storage, a.tol, a.steps, a.warmup, a.no_tune = wl, "fp16", 1e-6, 50, 5, False
+        a.kernel, a.kernel_table, a.settings = "solve", False, ""
+        try:
+            r = run_batch(a) if wl == "batch" else run_serve(a)
+        except Exception as e:       # (the single-world line stands on its own)
+            forms[wl] = {"error": repr(e)}
+            continue
+        if wl == "batch":
+            forms[wl] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "p50_step_ms", "p99_step_ms",
+                                           "particle_steps_per_s", "pcg_iterations_per_solve", "speedup_vs_sequential")}
+            forms[wl]["roofline"] = {k: r["roofline"][k] for k in ("kernel", "achieved", "peak", "frac", "unit")}
+            forms[wl]["config"] = {k: r["config"][k] for k in ("workload", "worlds", "cells", "vcycle_storage", "graphs")}
+        else:
+            forms[wl] = {k: r[k] for k in ("metric", "value", "unit", "p50_request_ms", "p99_request_ms", "p50_step_ms_512",
+                                           "p99_step_ms_512", "p50_step_ms_512_warm", "p99_step_ms_512_warm", "rounds",
+                                           "evictions", "admissions", "elapsed_s", "speedup_vs_single_world_loop")}
+            forms[wl]["config"] = r["config"]
+    return forms
 
 
 def run_batch(args):
@@ -498,7 +537,10 @@ def run_batch(args):
                        "frac": solve_achieved / HBM_PEAK_GBS if solve_achieved else None}
     if per_kernel:
         out["kernel_ms_one_step"] = {k: [round(v[0], 4), v[1]] for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:16]}
-    print(json.dumps(out), flush=True)
+    batch.close()
+    for w in worlds:
+        w.close()
+    return out
 
 
 def run_precision(args):
@@ -804,7 +846,7 @@ def run_serve(args):
     }
     out["speedup_vs_single_world_loop"] = out["value"] / out["single_world_loop"]["particle_steps_per_s"]
     server.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
