@@ -428,7 +428,7 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
     w->vlevel_bytes = 64u;
     for (uint32_t l = 1; l <= w->d.L; l++)   // (and each level's tile flags, for tiles of 8 or more)
         w->vlevel_bytes += 12ull * w->d.lw[l] * w->d.lh[l] + 64u + ceil_div(w->d.lw[l], 8u) * ceil_div(w->d.lh[l], 8u) + 64u
-                           + 8ull * w->d.lw[l] * w->d.lh[l] + 64u;
+                           + 8ull * w->d.lw[l] * w->d.lh[l] + 64u + 8ull * w->d.lw[l] * w->d.lh[l] + 64u;
     if (hipGetDevice(&w->device) != hipSuccess || hipStreamCreateWithFlags(&w->own, hipStreamNonBlocking) != hipSuccess) {
         fail(WIG_E_DEVICE, "no HIP device or stream");
         delete w;
@@ -540,6 +540,11 @@ WigWorld *wig_create(uint32_t width, uint32_t height, uint32_t coarse_levels, ui
         v = (uint8_t *)(((uintptr_t)v + 63u) & ~(uintptr_t)63u);
         for (uint32_t l = 1; l <= d.L; l++) {
             d.pdown[l] = (uint2 *)v;
+            v += 8ull * d.lw[l] * d.lh[l] + 64u;
+        }
+        // and each level's face betas (k_levelDown's first iteration, each solve)
+        for (uint32_t l = 1; l <= d.L; l++) {
+            d.vbeta[l] = (float2 *)v;
             v += 8ull * d.lw[l] * d.lh[l] + 64u;
         }
     }

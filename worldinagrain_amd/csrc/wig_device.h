@@ -222,6 +222,10 @@ struct Dev {
     // kept there too, a tile of either pass could read into its halo a value
     // a neighbouring tile of the same launch had already overwritten.
     uint2 *pdown[MG_LEVELS + 1];
+    // a tiled level's face betas (faceBeta, of its right and lower faces; 0
+    // where the stencil skips), taken by k_levelDown in each solve's first
+    // iteration: level rho does not change within a solve
+    float2 *vbeta[MG_LEVELS + 1];
     uint32_t lds_level;     // the first coarse level solved in one workgroup's LDS
     uint32_t wave_cells;    // and below it, levels of at most this many cells by its first wavefront
     uint32_t tile[MG_LEVELS + 1];   // k_levelDown/k_levelUp's tile side, per level below lds_level
@@ -291,6 +295,7 @@ struct WorldGrid {
     d.vrho[2] = dg_.vrho[(l) + 1u];                                   \
     d.tflag[1] = dg_.tflag[l];                                        \
     d.pdown[1] = dg_.pdown[l];                                        \
+    d.vbeta[1] = dg_.vbeta[l];                                        \
     const uint32_t tile_ = dg_.tile[l];                               \
     const uint32_t bid = (g).solo ? blockIdx.x : blockIdx.x - (g).first[wi_];          \
     const uint32_t nbid = (g).solo ? gridDim.x : (g).first[wi_ + 1u] - (g).first[wi_]; \

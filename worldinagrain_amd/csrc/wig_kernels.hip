@@ -2156,6 +2156,7 @@ template <int S> __device__ v2 prolonged(const Dev &d, uint32_t l, i2 c) {
 // k_levelUp(l): prolong's correction (:2459-2497) from level l + 1, staged
 // in LDS; the sweeps.  Both write the tile's p back to the field.
 #define TILE_THREADS 256u
+static_assert(16u * 16u <= TILE_THREADS, "k_levelDown: a thread restricts at most one coarse cell (tile sides of 32 or less)");
 // A thread's cells of one colour, at most SWEEP_K of them, held in registers
 // across a level's half-sweeps with what no sweep changes: the four
 // neighbours' betas in beside()'s order (0 where the stencil skips one),
@@ -2272,11 +2273,13 @@ template <int S> __device__ __forceinline__ void pdownStore(const Dev &d, i2 c, 
     else u.x = (uint32_t)enc16<S>(p.x) | ((uint32_t)enc16<S>(p.y) << 16);
 }
 #define TILE_LOADS 9
-template <int S, bool UP> __device__ __forceinline__ void loadLevelTile(const Dev &d, const LevelTile &t) {
+// BETAS: the cells' face betas too, as the solve's first iteration left them
+// in the level's beta array (0 outside the level, as tileBetas leaves them)
+template <int S, bool UP, bool BETAS> __device__ __forceinline__ void loadLevelTile(const Dev &d, const LevelTile &t) {
     const int32_t n = t.R * t.R, w = (int32_t)d.lw[1], h = (int32_t)d.lh[1], tid = (int32_t)threadIdx.x;
     for (int32_t b0 = 0; b0 < n; b0 += TILE_LOADS * (int32_t)TILE_THREADS) {
         float4 e[TILE_LOADS];
-        float2 p[TILE_LOADS];
+        float2 p[TILE_LOADS], bt[TILE_LOADS];
         float r[TILE_LOADS];
 #pragma unroll
         for (int q = 0; q < TILE_LOADS; q++) {
@@ -2286,6 +2289,7 @@ template <int S, bool UP> __device__ __forceinline__ void loadLevelTile(const De
             e[q] = lvLoad<S>(d, 1u, c);
             p[q] = UP ? pdownLoad<S>(d, c) : make_float2(0.0f, 0.0f);
             r[q] = lvRho<S>(d, 1u, c);
+            bt[q] = BETAS ? d.vbeta[1][lvIndex(d, 1u, c)] : make_float2(0.0f, 0.0f);
         }
 #pragma unroll
         for (int q = 0; q < TILE_LOADS; q++) {
@@ -2298,6 +2302,10 @@ template <int S, bool UP> __device__ __forceinline__ void loadLevelTile(const De
             t.rx[k] = in ? e[q].z : 0.0f;
             t.ry[k] = in ? e[q].w : 0.0f;
             t.rho[k] = in ? r[q] : -1.0f;
+            if (BETAS) {
+                t.bR[k] = in ? bt[q].x : 0.0f;
+                t.bD[k] = in ? bt[q].y : 0.0f;
+            }
         }
     }
 }
@@ -2309,6 +2317,18 @@ __device__ __forceinline__ void tileBetas(const LevelTile &t) {
         float r = t.rho[j];
         t.bR[j] = x + 1 < t.R ? faceBeta(r, t.rho[j + 1]) : 0.0f;
         t.bD[j] = y + 1 < t.R ? faceBeta(r, t.rho[j + t.R]) : 0.0f;
+    }
+}
+// The tile's own cells' betas into the level's beta array, for the solve's
+// later iterations and k_levelUp.  (The rim's last column and row, whose
+// betas tileBetas leaves 0, are halo: no sweep, residual or restriction reads
+// a beta there, so a tile loading the array's true values there computes
+// the same.)
+__device__ __forceinline__ void storeTileBetas(const Dev &d, const LevelTile &t, uint32_t T, uint32_t H) {
+    for (uint32_t k = threadIdx.x; k < T * T; k += TILE_THREADS) {
+        const int32_t x = (int32_t)(k % T + H), y = (int32_t)(k / T + H), j = y * t.R + x;
+        const i2 c = I2(t.ox + x, t.oy + y);
+        if (inLevel(d, 1u, c)) d.vbeta[1][lvIndex(d, 1u, c)] = make_float2(t.bR[j], t.bD[j]);
     }
 }
 // smoothAt's stencil sums at tile cell j, not on the tile's rim
@@ -2398,14 +2418,31 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
     const uint32_t H = 2u * d.level_pairs + 1u;
     LevelTile t = levelTile(sm, T, H, bx, by);
     float *resx = sm + 7u * (uint32_t)(t.R * t.R), *resy = resx + T * T;
+    // The block states restrictLevel's shortcut reads (below), loaded
+    // before the tile: a thread's coarse cell is at most one (T <= 32).
+    const uint32_t T2 = T / 2u, lc = l + 1u;
+    const int32_t span = 1 << lc;
+    bool idleBlock = false;
+    if (span <= WIG_BLOCK && threadIdx.x < T2 * T2) {
+        const i2 c = I2((int32_t)(bx * T2 + threadIdx.x % T2), (int32_t)(by * T2 + threadIdx.x / T2));
+        const i2 b = I2(c.x * span / WIG_BLOCK, c.y * span / WIG_BLOCK);
+        const uint32_t sb = d.blocks[STATE_SLOT(d, inBlocks(d, b) ? blockIndex(d, b) : 0u)];
+        idleBlock = inBlocks(d, b) && (sb & ACTIVE) == 0u;
+    }
     if (!it.encoded || it.converged || it.active == 0u) return;
     // a tile none of whose own cells holds fluid leaves every value as the
     // solve's first iteration left it (only cells of rho > 0 change)
     if (it.cycle > 0u && fluidTile == 0u) return;
-    loadLevelTile<S, false>(d, t);
+    // The betas: taken in the solve's first iteration, loaded after it.
+    const bool first = it.cycle == 0u;
+    if (first) loadLevelTile<S, false, false>(d, t);
+    else loadLevelTile<S, false, true>(d, t);
     __syncthreads();
-    tileBetas(t);
-    __syncthreads();
+    if (first) {
+        tileBetas(t);
+        __syncthreads();
+        storeTileBetas(d, t, T, H);
+    }
     tileSweeps(t, l, d.level_pairs);
     // p back, and each tile cell's residual (residualAt, :2306-2315)
     const float ih2 = 1.0f / (float)(1u << (2u * l));   // (a power of two: x / h2 and x * ih2 are the same value)
@@ -2425,19 +2462,14 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
         anyFluid |= t.rho[j] > 0.0f ? 1 : 0;
     }
     anyFluid = __syncthreads_or(anyFluid);
-    if (it.cycle == 0u && threadIdx.x == 0) d.tflag[1][bid] = anyFluid ? 1u : 0u;
-    // restrictLevel (:2342-2377) into the tile's parents
-    const uint32_t T2 = T / 2u, lc = l + 1u;
-    const int32_t span = 1 << lc;
+    if (first && threadIdx.x == 0) d.tflag[1][bid] = anyFluid ? 1u : 0u;
+    // restrictLevel (:2342-2377) into the tile's parents (their rho, the same
+    // each iteration of a solve, stored in its first)
     for (uint32_t k = threadIdx.x; k < T2 * T2; k += TILE_THREADS) {
         i2 c = I2((int32_t)(bx * T2 + k % T2), (int32_t)(by * T2 + k / T2));
         if (!inLevel(d, 2u, c)) continue;
         float rho = 0.0f, ox = 0.0f, oy = 0.0f;
-        bool shortcut = false;
-        if (span <= WIG_BLOCK) {
-            i2 b = I2(c.x * span / WIG_BLOCK, c.y * span / WIG_BLOCK);
-            shortcut = inBlocks(d, b) && (d.blocks[STATE_SLOT(d, blockIndex(d, b))] & ACTIVE) == 0u;
-        }
+        const bool shortcut = span <= WIG_BLOCK && idleBlock;   // (k is the thread's)
         if (!shortcut) {
             float fluid = 0.0f, sum = 0.0f;
             bool air = false;
@@ -2458,7 +2490,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelDown(Wor
             oy = rho > 0.0f ? sy * 0.25f : 0.0f;
         }
         lvStore<S>(d, 2u, c, make_float4(0.0f, 0.0f, ox, oy));
-        lvStoreRho<S>(d, 2u, c, rho);
+        if (first) lvStoreRho<S>(d, 2u, c, rho);
     }
 }
 template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(WorldGrid g, uint32_t l) {
@@ -2486,7 +2518,7 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
             ce[q] = lvLoadP<S>(d, 2u, c);
             cr[q] = lvRho<S>(d, 2u, c);
         }
-        loadLevelTile<S, true>(d, t);
+        loadLevelTile<S, true, true>(d, t);   // (the betas k_levelDown left)
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             const int32_t k = tid + q * (int32_t)TILE_THREADS;
@@ -2532,7 +2564,6 @@ template <int S> __global__ void __launch_bounds__(TILE_THREADS) k_levelUp(World
         t.px[j] = t.px[j] + ps.sx / dn;
         t.py[j] = t.py[j] + ps.sy / dn;
     }
-    tileBetas(t);
     __syncthreads();
     tileSweeps(t, l, d.level_pairs);
     for (uint32_t k = threadIdx.x; k < T * T; k += TILE_THREADS) {
